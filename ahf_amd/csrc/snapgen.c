@@ -1,0 +1,442 @@
+/*
+ * snapgen.c -- seeded synthetic snapshots for the deposit + unbinding path.
+ *
+ * Writes the project's snapshot container (oracle/snapfile.h) holding
+ * particles in AHF's internal units (positions in box units, velocities in
+ * units where v * v_fac = peculiar km/s, v_fac = boxsize/t_unit/a with
+ * t_unit = 1/H0, src/startrun.c:547 and src/libahf/ahf_halos.c:201) and the
+ * halo list that sort_halo_particles()/rem_unbound() receive.
+ *
+ * Workloads (all sizes scale with `side`, the cube root of the particle
+ * count; the defaults are the full sizes):
+ *   config 0: 128^3 particles, 50 Mpc/h, 40 NFW halos, M log-uniform in
+ *             [1e11,1e13] Msun/h, c ~ U(5,15), Maxwellian velocities at the
+ *             virial dispersion, 10% of each halo's particles at 2 v_esc;
+ *             uniform background with sigma_v = 50 km/s.  LgridDomain 64.
+ *   config 1: 256^3, 100 Mpc/h, 2000 hosts with dN/dM ~ M^-1.9 on
+ *             [1e10,1e14]; hosts above 1e12 carry 5-20 NFW subhaloes holding
+ *             10% of the host mass; 10% interlopers.  LgridDomain 128.
+ *   config 2: 100000 isolated NFW haloes with N ~ N^-2 on [20, 2e6] particles,
+ *             c ~ U(4,20), 15% of particles above escape speed and 5% within
+ *             1e-4 (relative) of VescTune * v_esc.  No background, no grid.
+ *   config 3: 512^3, 300 Mpc/h, 70/30 background/haloes, 20000 hosts from the
+ *             config-1 mass function.  LgridDomain 256.
+ * All use Omega_m = 0.3, Omega_L = 0.7, a = 1, VescTune 1.5, NminPerHalo 20,
+ * NperDomCell = NperRefCell = 5, LgridMax = LgridDomain * 2^8.
+ *
+ * Randomness is a counter-based splitmix64 stream per (seed, object), so the
+ * file is byte-identical for a given (config, seed, side) whatever the
+ * thread count.  Particles are finally ordered along a 21-bit Morton curve
+ * (AHF orders them along a Hilbert curve, src/main.c:341-356; any fixed
+ * order is a valid input to the path and the checkers replay the same one).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../../oracle/snapfile.h"
+
+#define RHOC0 2.7755397e11   /* h^2 Msun / Mpc^3, src/param.h:38 */
+#define GRAV 4.3006485e-9    /* Mpc km^2 / (Msun s^2), src/param.h:39 */
+#define PI_D 3.14159265358979323846
+
+/* ------------------------------------------------------------ random ---- */
+
+static inline uint64_t mix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+typedef struct { uint64_t key, ctr; } rng_t;
+
+static inline rng_t rng_make(uint64_t seed, uint64_t stream)
+{
+  rng_t r = {mix64(seed * 0x632BE59BD9B4E019ull + mix64(stream)), 0};
+  return r;
+}
+static inline uint64_t rng_u64(rng_t *r) { return mix64(r->key + 0x9E3779B97F4A7C15ull * ++r->ctr); }
+static inline double rng_uniform(rng_t *r) { return (double)(rng_u64(r) >> 11) * (1.0 / 9007199254740992.0); }
+static inline double rng_gauss(rng_t *r)
+{
+  double u1 = rng_uniform(r), u2 = rng_uniform(r);
+  if (u1 < 1e-300) u1 = 1e-300;
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * PI_D * u2);
+}
+static inline void rng_dir(rng_t *r, double v[3])
+{
+  double ct = 2.0 * rng_uniform(r) - 1.0, ph = 2.0 * PI_D * rng_uniform(r);
+  double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+  v[0] = st * cos(ph); v[1] = st * sin(ph); v[2] = ct;
+}
+
+/* --------------------------------------------------------------- NFW ---- */
+
+typedef struct {
+  double M;          /* Msun/h */
+  double c;
+  double cen[3];     /* Mpc/h */
+  double vbulk[3];   /* km/s */
+  int64_t n;         /* particles of this halo (incl. interlopers) */
+  int64_t first;     /* first particle slot */
+  double frac_unb;   /* fraction boosted above escape */
+  double frac_edge;  /* fraction placed at the escape threshold */
+  int host;          /* index of host halo, -1 for hosts */
+} halo_spec;
+
+static inline double nfw_mu(double x) { return log1p(x) - x / (1.0 + x); }
+
+static double r_vir_of(double M) { return cbrt(3.0 * M / (4.0 * PI_D * 200.0 * RHOC0)); }
+
+/* radius (Mpc/h) with M(<r)/M(<Rvir) = u */
+static double nfw_radius(double u, double c, double rs)
+{
+  double target = u * nfw_mu(c), lo = 0.0, hi = c;
+  for (int it = 0; it < 60; it++) {
+    double mid = 0.5 * (lo + hi);
+    if (nfw_mu(mid) < target) lo = mid; else hi = mid;
+  }
+  return 0.5 * (lo + hi) * rs;
+}
+
+/* escape speed (km/s) of an untruncated NFW halo at radius r */
+static double nfw_vesc(double r, double M, double c, double rs)
+{
+  double x = fmax(r / rs, 1e-6);
+  double phi = GRAV * M / nfw_mu(c) * log1p(x) / (x * rs);
+  return sqrt(2.0 * phi);
+}
+
+/* -------------------------------------------------------------- build ---- */
+
+typedef struct {
+  int config;
+  int64_t npart;
+  double box, pmass, vtune;
+  int L_dom, L_max;
+  double Nth_dom, Nth_ref;
+  int isolated;
+  double sigma_bg;
+} gen_params;
+
+static int64_t n_of_mass(double M, double mp) { return (int64_t)floor(M / mp + 0.5); }
+
+/* draw from p(M) ~ M^-alpha on [m1, m2] */
+static double powerlaw(rng_t *r, double alpha, double m1, double m2)
+{
+  double a = 1.0 - alpha, u = rng_uniform(r);
+  return pow(pow(m1, a) + u * (pow(m2, a) - pow(m1, a)), 1.0 / a);
+}
+
+static void place_halo(halo_spec *h, const gen_params *g, rng_t *r)
+{
+  for (int d = 0; d < 3; d++) {
+    h->cen[d] = (g->isolated ? 0.1 + 0.8 * rng_uniform(r) : rng_uniform(r)) * g->box;
+    h->vbulk[d] = 150.0 * rng_gauss(r);
+  }
+}
+
+/* hosts (and for config 1 their subhaloes); returns count */
+static int64_t make_halos(const gen_params *g, uint64_t seed, int64_t nhost, int subs,
+                          double mlo, double mhi, int logflat, halo_spec **out)
+{
+  int64_t cap = nhost * 22 + 16, nh = 0;
+  halo_spec *hs = (halo_spec *)calloc((size_t)cap, sizeof(halo_spec));
+  rng_t r = rng_make(seed, 1);
+  for (int64_t i = 0; i < nhost; i++) {
+    halo_spec *h = &hs[nh++];
+    h->M = logflat ? exp(log(mlo) + rng_uniform(&r) * (log(mhi) - log(mlo)))
+                   : powerlaw(&r, 1.9, mlo, mhi);
+    h->c = 5.0 + 10.0 * rng_uniform(&r);
+    h->frac_unb = 0.10;
+    h->frac_edge = 0.0;
+    h->host = -1;
+    place_halo(h, g, &r);
+    if (subs && h->M > 1e12) {
+      int ns = 5 + (int)(rng_uniform(&r) * 16.0);
+      if (ns > 20) ns = 20;
+      double w[20], wsum = 0.0;
+      for (int k = 0; k < ns; k++) { w[k] = 0.2 + rng_uniform(&r); wsum += w[k]; }
+      double Mhost = h->M, rv = r_vir_of(Mhost), rs = rv / h->c;
+      double sig = sqrt(GRAV * Mhost / (2.0 * rv));
+      int64_t hi = nh - 1;
+      hs[hi].M = 0.9 * Mhost;
+      for (int k = 0; k < ns; k++) {
+        halo_spec *s = &hs[nh++];
+        s->M = 0.1 * Mhost * w[k] / wsum;
+        s->c = 5.0 + 10.0 * rng_uniform(&r);
+        s->frac_unb = 0.10;
+        s->host = (int)hi;
+        double rr = nfw_radius(0.1 + 0.8 * rng_uniform(&r), hs[hi].c, rs), dir[3];
+        rng_dir(&r, dir);
+        for (int d = 0; d < 3; d++) {
+          s->cen[d] = hs[hi].cen[d] + rr * dir[d];
+          s->vbulk[d] = hs[hi].vbulk[d] + sig * rng_gauss(&r);
+        }
+      }
+    }
+  }
+  *out = hs;
+  return nh;
+}
+
+static int cmp_u64pair(const void *a, const void *b)
+{
+  const uint64_t *x = (const uint64_t *)a, *y = (const uint64_t *)b;
+  if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+  return x[1] < y[1] ? -1 : (x[1] > y[1]);
+}
+
+static uint64_t spread21(uint64_t v)
+{
+  v &= 0x1FFFFF;
+  v = (v | v << 32) & 0x1F00000000FFFFull;
+  v = (v | v << 16) & 0x1F0000FF0000FFull;
+  v = (v | v << 8) & 0x100F00F00F00F00Full;
+  v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+
+static inline float to_unit(double x_mpc, double box, int periodic)
+{
+  double u = x_mpc / box;
+  if (periodic) u -= floor(u);
+  float f = (float)u;
+  if (f >= 1.0f) f = 0.0f;
+  if (f < 0.0f) f = 0.0f;
+  return f;
+}
+
+static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo_spec *hs,
+                      int64_t nh, int64_t n_bg)
+{
+  const double a = 1.0, v_fac = g->box / 0.01 / a;
+  int64_t nhalo_part = 0;
+  for (int64_t i = 0; i < nh; i++) { hs[i].first = nhalo_part; nhalo_part += hs[i].n; }
+  int64_t N = nhalo_part + n_bg;
+  float *pos = (float *)malloc((size_t)N * 3 * sizeof(float));
+  float *mom = (float *)malloc((size_t)N * 3 * sizeof(float));
+  if (!pos || !mom) return -1;
+
+  /* haloes */
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t i = 0; i < nh; i++) {
+    halo_spec *h = &hs[i];
+    rng_t r = rng_make(seed, 1000 + (uint64_t)i);
+    double rv = r_vir_of(h->M), rs = rv / h->c, sig = sqrt(GRAV * h->M / (2.0 * rv));
+    int64_t n_unb = (int64_t)floor(h->frac_unb * (double)h->n + 0.5);
+    int64_t n_edge = (int64_t)floor(h->frac_edge * (double)h->n + 0.5);
+    for (int64_t j = 0; j < h->n; j++) {
+      double rr = nfw_radius(rng_uniform(&r), h->c, rs), dir[3], v[3];
+      rng_dir(&r, dir);
+      if (j < n_unb || j < n_unb + n_edge) {
+        double vesc = nfw_vesc(rr, h->M, h->c, rs);
+        double speed = (j < n_unb) ? 2.0 * vesc : g->vtune * vesc * (1.0 + 1e-4 * (2.0 * rng_uniform(&r) - 1.0));
+        double vd[3];
+        rng_dir(&r, vd);
+        for (int d = 0; d < 3; d++) v[d] = speed * vd[d];
+      } else {
+        for (int d = 0; d < 3; d++) v[d] = sig * rng_gauss(&r);
+      }
+      int64_t p = h->first + j;
+      for (int d = 0; d < 3; d++) {
+        pos[3 * p + d] = to_unit(h->cen[d] + rr * dir[d], g->box, 1);
+        mom[3 * p + d] = (float)((h->vbulk[d] + v[d]) / v_fac);
+      }
+    }
+  }
+  /* uniform background in blocks of 65536 */
+  int64_t nblk = (n_bg + 65535) / 65536;
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nblk; b++) {
+    rng_t r = rng_make(seed, 0x100000000ull + (uint64_t)b);
+    int64_t j1 = (b + 1) * 65536 < n_bg ? (b + 1) * 65536 : n_bg;
+    for (int64_t j = b * 65536; j < j1; j++) {
+      int64_t p = nhalo_part + j;
+      for (int d = 0; d < 3; d++) {
+        pos[3 * p + d] = to_unit(rng_uniform(&r) * g->box, g->box, 1);
+        mom[3 * p + d] = (float)(g->sigma_bg * rng_gauss(&r) / v_fac);
+      }
+    }
+  }
+
+  /* Morton order */
+  uint64_t *kv = (uint64_t *)malloc((size_t)N * 2 * sizeof(uint64_t));
+  if (!kv) return -1;
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < N; p++) {
+    uint64_t ix = (uint64_t)(pos[3 * p + 0] * 2097152.0f);
+    uint64_t iy = (uint64_t)(pos[3 * p + 1] * 2097152.0f);
+    uint64_t iz = (uint64_t)(pos[3 * p + 2] * 2097152.0f);
+    kv[2 * p] = spread21(ix) | spread21(iy) << 1 | spread21(iz) << 2;
+    kv[2 * p + 1] = (uint64_t)p;
+  }
+  qsort(kv, (size_t)N, 2 * sizeof(uint64_t), cmp_u64pair);
+  uint64_t *newidx = (uint64_t *)malloc((size_t)N * sizeof(uint64_t));
+  float *pos2 = (float *)malloc((size_t)N * 3 * sizeof(float));
+  float *mom2 = (float *)malloc((size_t)N * 3 * sizeof(float));
+  uint64_t *id = (uint64_t *)malloc((size_t)N * sizeof(uint64_t));
+  if (!newidx || !pos2 || !mom2 || !id) return -1;
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < N; q++) {
+    uint64_t p = kv[2 * q + 1];
+    newidx[p] = (uint64_t)q;
+    memcpy(pos2 + 3 * q, pos + 3 * p, 3 * sizeof(float));
+    memcpy(mom2 + 3 * q, mom + 3 * p, 3 * sizeof(float));
+    id[q] = p;
+  }
+  free(kv); free(pos); free(mom);
+
+  /* halo lists: a host's list holds its own particles and its subhaloes' */
+  int64_t n_ip = 0;
+  for (int64_t i = 0; i < nh; i++) n_ip += hs[i].n + (hs[i].host >= 0 ? hs[i].n : 0);
+  int64_t *hostn = (int64_t *)calloc((size_t)nh, sizeof(int64_t));
+  for (int64_t i = 0; i < nh; i++) hostn[i] = hs[i].n;
+  for (int64_t i = 0; i < nh; i++)
+    if (hs[i].host >= 0) hostn[hs[i].host] += hs[i].n;
+  n_ip = 0;
+  for (int64_t i = 0; i < nh; i++) n_ip += hostn[i];
+  uint64_t *ipart = (uint64_t *)malloc((size_t)(n_ip ? n_ip : 1) * sizeof(uint64_t));
+  uint64_t *hnp = (uint64_t *)malloc((size_t)(nh ? nh : 1) * sizeof(uint64_t));
+  double *cen = (double *)malloc((size_t)(nh ? nh : 1) * 3 * sizeof(double));
+  int64_t *fill = (int64_t *)calloc((size_t)(nh ? nh : 1), sizeof(int64_t));
+  int64_t off = 0;
+  for (int64_t i = 0; i < nh; i++) { fill[i] = off; off += hostn[i]; }
+  for (int64_t i = 0; i < nh; i++) {
+    for (int64_t j = 0; j < hs[i].n; j++) ipart[fill[i]++] = newidx[hs[i].first + j];
+    if (hs[i].host >= 0) {
+      int64_t hh = hs[i].host;
+      for (int64_t j = 0; j < hs[i].n; j++) ipart[fill[hh]++] = newidx[hs[i].first + j];
+    }
+  }
+  for (int64_t i = 0; i < nh; i++) {
+    hnp[i] = (uint64_t)hostn[i];
+    for (int d = 0; d < 3; d++) {
+      double u = hs[i].cen[d] / g->box;
+      if (!g->isolated) u -= floor(u);
+      cen[3 * i + d] = u;
+    }
+  }
+
+  snap_header hd;
+  memset(&hd, 0, sizeof(hd));
+  memcpy(hd.magic, SNAP_MAGIC, 8);
+  hd.npart = (uint64_t)N;
+  hd.boxsize = g->box;
+  hd.omega0 = 0.3;
+  hd.lambda0 = 0.7;
+  hd.a = a;
+  hd.pmass = g->pmass;
+  hd.L_dom = g->L_dom;
+  hd.L_max = g->L_max;
+  hd.Nth_dom = g->Nth_dom;
+  hd.Nth_ref = g->Nth_ref;
+  hd.minpart = 20;
+  hd.vtune = g->vtune;
+  hd.r_fac = g->box * a;
+  hd.v_fac = v_fac;
+  hd.phi_fac = GRAV * g->pmass / (g->box * a);
+  hd.hubble = 100.0 * sqrt(0.7 * (1.0 - 1.0 / (a * a)) + 0.3 * (1.0 / (a * a * a) - 1.0 / (a * a)) + 1.0 / (a * a));
+  hd.nhalos = (uint64_t)nh;
+  hd.n_ipart = (uint64_t)n_ip;
+  hd.seed = seed;
+  hd.config = g->config;
+
+  FILE *f = fopen(path, "wb");
+  int bad = !f;
+  if (!bad) {
+    bad |= fwrite(&hd, sizeof(hd), 1, f) != 1;
+    bad |= fwrite(pos2, sizeof(float), (size_t)N * 3, f) != (size_t)N * 3;
+    bad |= fwrite(mom2, sizeof(float), (size_t)N * 3, f) != (size_t)N * 3;
+    bad |= fwrite(id, sizeof(uint64_t), (size_t)N, f) != (size_t)N;
+    bad |= fwrite(cen, sizeof(double), (size_t)nh * 3, f) != (size_t)nh * 3;
+    bad |= fwrite(hnp, sizeof(uint64_t), (size_t)nh, f) != (size_t)nh;
+    bad |= fwrite(ipart, sizeof(uint64_t), (size_t)n_ip, f) != (size_t)n_ip;
+    bad |= fclose(f) != 0;
+  }
+  free(newidx); free(pos2); free(mom2); free(id); free(hostn);
+  free(ipart); free(hnp); free(cen); free(fill);
+  return bad ? -2 : 0;
+}
+
+/*
+ * ahf_snapgen_write: generate workload `config` at `side`^3 particles
+ * (side <= 0: the full size) with `seed`, scaling the halo count by
+ * (side/side_full)^3; writes `path`.  Returns 0 on success.
+ */
+int ahf_snapgen_write(int config, int side, uint64_t seed, const char *path)
+{
+  gen_params g;
+  memset(&g, 0, sizeof(g));
+  g.config = config;
+  g.vtune = 1.5;
+  g.Nth_dom = 5.0;
+  g.Nth_ref = 5.0;
+  g.sigma_bg = 50.0;
+  int full_side = config == 0 ? 128 : config == 1 ? 256 : config == 3 ? 512 : 0;
+  int dom_full = config == 0 ? 64 : config == 1 ? 128 : config == 3 ? 256 : 0;
+  halo_spec *hs = NULL;
+  int64_t nh = 0, n_bg = 0;
+  if (config == 2) {
+    g.isolated = 1;
+    g.box = 100.0;
+    g.pmass = 1e8;
+    g.L_dom = 64;
+    g.L_max = 64 << 8;
+    int64_t nhalo = side > 0 ? (int64_t)side : 100000;
+    hs = (halo_spec *)calloc((size_t)nhalo, sizeof(halo_spec));
+    rng_t r = rng_make(seed, 2);
+    for (int64_t i = 0; i < nhalo; i++) {
+      halo_spec *h = &hs[i];
+      double u = rng_uniform(&r);
+      /* N ~ N^-2 on [20, 2e6] */
+      double n = 1.0 / (1.0 / 20.0 - u * (1.0 / 20.0 - 1.0 / 2e6));
+      h->n = (int64_t)floor(n);
+      h->M = (double)h->n * g.pmass;
+      h->c = 4.0 + 16.0 * rng_uniform(&r);
+      h->frac_unb = 0.15;
+      h->frac_edge = 0.05;
+      h->host = -1;
+      place_halo(h, &g, &r);
+    }
+    nh = nhalo;
+    n_bg = 0;
+  } else if (config == 0 || config == 1 || config == 3) {
+    if (side <= 0) side = full_side;
+    double scale = (double)side / (double)full_side;
+    g.npart = (int64_t)side * side * side;
+    g.box = (config == 0 ? 50.0 : config == 1 ? 100.0 : 300.0) * scale;
+    g.pmass = 0.3 * RHOC0 * g.box * g.box * g.box / (double)g.npart;
+    g.L_dom = (int)(dom_full * scale);
+    if (g.L_dom < 4) g.L_dom = 4;
+    g.L_max = g.L_dom << 8;
+    int64_t nhost = config == 0 ? 40 : config == 1 ? 2000 : 20000;
+    nhost = (int64_t)ceil((double)nhost * scale * scale * scale);
+    if (config == 0) nh = make_halos(&g, seed, nhost, 0, 1e11, 1e13, 1, &hs);
+    else nh = make_halos(&g, seed, nhost, config == 1, 1e10, 1e14, 0, &hs);
+    int64_t tot = 0;
+    for (int64_t i = 0; i < nh; i++) { hs[i].n = n_of_mass(hs[i].M, g.pmass); tot += hs[i].n; }
+    int64_t budget = config == 3 ? (int64_t)(0.3 * (double)g.npart) : g.npart;
+    if (config == 3 && tot > 0) {
+      /* rescale halo masses so haloes hold 30% of the particles */
+      double f = (double)budget / (double)tot;
+      tot = 0;
+      for (int64_t i = 0; i < nh; i++) { hs[i].M *= f; hs[i].n = n_of_mass(hs[i].M, g.pmass); tot += hs[i].n; }
+    }
+    while (tot > g.npart && nh > 0) { tot -= hs[nh - 1].n; nh--; }
+    n_bg = g.npart - tot;
+  } else {
+    return -10;
+  }
+  int rc = write_snap(path, &g, seed, hs, nh, n_bg);
+  free(hs);
+  return rc;
+}

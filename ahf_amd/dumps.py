@@ -1,0 +1,218 @@
+"""Readers for the grid-hierarchy and halo dumps (AHFGRID1 / AHFHALO1) and the
+snapshot container (AHFSNAP1).
+
+The same dump layout is written by the reference harness
+(oracle/ref_driver.c), the CPU restatement (oracle/ahf_oracle.c) and this
+package's GPU path, so any two of them can be compared field by field.
+"""
+from __future__ import annotations
+
+import dataclasses
+import struct
+
+import numpy as np
+
+SNAP_HEADER = struct.Struct("<8sQddddd ii dd ii ddddd QQQ ii")
+
+
+@dataclasses.dataclass
+class Snapshot:
+    npart: int
+    boxsize: float
+    omega0: float
+    lambda0: float
+    a: float
+    pmass: float
+    L_dom: int
+    L_max: int
+    Nth_dom: float
+    Nth_ref: float
+    minpart: int
+    vtune: float
+    r_fac: float
+    v_fac: float
+    phi_fac: float
+    hubble: float
+    nhalos: int
+    n_ipart: int
+    seed: int
+    config: int
+    pos: np.ndarray          # (npart, 3) float32, box units
+    mom: np.ndarray          # (npart, 3) float32
+    ids: np.ndarray          # (npart,) uint64
+    centre: np.ndarray       # (nhalos, 3) float64
+    halo_npart: np.ndarray   # (nhalos,) uint64
+    ipart: np.ndarray        # (n_ipart,) uint64
+
+    @property
+    def halo_offsets(self) -> np.ndarray:
+        off = np.zeros(self.nhalos + 1, dtype=np.int64)
+        np.cumsum(self.halo_npart.astype(np.int64), out=off[1:])
+        return off
+
+
+def read_snapshot(path: str) -> Snapshot:
+    with open(path, "rb") as f:
+        raw = f.read(SNAP_HEADER.size)
+        v = SNAP_HEADER.unpack(raw)
+        if v[0] != b"AHFSNAP1":
+            raise ValueError(f"{path}: not an AHFSNAP1 file")
+        (npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, _p0, vt, rf, vf, pf, hb,
+         nh, ni, seed, cfg, _p1) = v[1:]
+        pos = np.fromfile(f, dtype=np.float32, count=3 * npart).reshape(npart, 3)
+        mom = np.fromfile(f, dtype=np.float32, count=3 * npart).reshape(npart, 3)
+        ids = np.fromfile(f, dtype=np.uint64, count=npart)
+        cen = np.fromfile(f, dtype=np.float64, count=3 * nh).reshape(nh, 3)
+        hnp = np.fromfile(f, dtype=np.uint64, count=nh)
+        ip = np.fromfile(f, dtype=np.uint64, count=ni)
+    return Snapshot(npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, vt, rf, vf, pf, hb,
+                    nh, ni, seed, cfg, pos, mom, ids, cen, hnp, ip)
+
+
+def write_snapshot(path: str, pos, mom=None, ids=None, *, boxsize=100.0, pmass=1e10,
+                   L_dom=16, L_max=None, Nth_dom=5.0, Nth_ref=5.0, minpart=20, vtune=1.5,
+                   r_fac=None, v_fac=None, phi_fac=None, hubble=100.0, centres=None,
+                   halo_lists=None, seed=0, config=-1, omega0=0.3, lambda0=0.7, a=1.0):
+    """Write a hand-made snapshot (tests).  pos in box units."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+    n = pos.shape[0]
+    mom = np.zeros((n, 3), np.float32) if mom is None else np.ascontiguousarray(mom, np.float32)
+    ids = np.arange(n, dtype=np.uint64) if ids is None else np.ascontiguousarray(ids, np.uint64)
+    if L_max is None:
+        L_max = L_dom << 8
+    r_fac = boxsize * a if r_fac is None else r_fac
+    v_fac = boxsize / 0.01 / a if v_fac is None else v_fac
+    phi_fac = 4.3006485e-9 * pmass / (boxsize * a) if phi_fac is None else phi_fac
+    centres = np.zeros((0, 3)) if centres is None else np.asarray(centres, np.float64).reshape(-1, 3)
+    halo_lists = [] if halo_lists is None else halo_lists
+    hnp = np.array([len(h) for h in halo_lists], dtype=np.uint64)
+    ip = (np.concatenate([np.asarray(h, np.uint64) for h in halo_lists])
+          if halo_lists else np.zeros(0, np.uint64))
+    hdr = SNAP_HEADER.pack(b"AHFSNAP1", n, boxsize, omega0, lambda0, a, pmass, L_dom, L_max,
+                           Nth_dom, Nth_ref, minpart, 0, vtune, r_fac, v_fac, phi_fac, hubble,
+                           len(halo_lists), ip.size, seed, config, 0)
+    with open(path, "wb") as f:
+        f.write(hdr)
+        pos.tofile(f)
+        mom.tofile(f)
+        ids.tofile(f)
+        centres.astype(np.float64).tofile(f)
+        hnp.tofile(f)
+        ip.tofile(f)
+
+
+@dataclasses.dataclass
+class GridLevel:
+    l1dim: int
+    size_no_nodes: int
+    size_no_part: int
+    critdens: float
+    masstopartdens: float
+    nquads: np.ndarray      # (n, 4) int32: z, y, x, length (traversal order)
+    pquads: np.ndarray      # (n, 2) int32: z, length
+    cquads: np.ndarray      # (n, 3) int32: z, y, length
+    dens: np.ndarray        # (nodes,) float32, traversal order
+    counts: np.ndarray      # (nodes,) uint32 list lengths
+    lists: np.ndarray       # (sum counts,) uint64 particle indices in list order
+
+    def node_xyz(self) -> np.ndarray:
+        """(nodes, 3) int64 coordinates x, y, z in traversal order."""
+        nq = self.nquads.astype(np.int64)
+        lens = nq[:, 3]
+        tot = int(lens.sum())
+        rep = np.repeat(np.arange(nq.shape[0]), lens)
+        start = np.zeros(nq.shape[0], np.int64)
+        np.cumsum(lens[:-1], out=start[1:]) if nq.shape[0] > 1 else None
+        off = np.arange(tot, dtype=np.int64) - start[rep]
+        out = np.empty((tot, 3), np.int64)
+        out[:, 0] = nq[rep, 2] + off
+        out[:, 1] = nq[rep, 1]
+        out[:, 2] = nq[rep, 0]
+        return out
+
+
+def read_grids(path: str) -> list[GridLevel]:
+    levels = []
+    with open(path, "rb") as f:
+        if f.read(8) != b"AHFGRID1":
+            raise ValueError(f"{path}: not an AHFGRID1 dump")
+        ng, _dom = struct.unpack("<ii", f.read(8))
+        for _ in range(ng):
+            l1dim, nn, npt = struct.unpack("<QQQ", f.read(24))
+            crit, m2pd = struct.unpack("<dd", f.read(16))
+            n_nq, n_nodes, n_listed, n_pq, n_cq = struct.unpack("<QQQQQ", f.read(40))
+            nquads = np.fromfile(f, np.int32, 4 * n_nq).reshape(n_nq, 4)
+            pquads = np.fromfile(f, np.int32, 2 * n_pq).reshape(n_pq, 2)
+            cquads = np.fromfile(f, np.int32, 3 * n_cq).reshape(n_cq, 3)
+            dens = np.fromfile(f, np.float32, n_nodes)
+            counts = np.fromfile(f, np.uint32, n_nodes)
+            lists = np.fromfile(f, np.uint64, n_listed)
+            levels.append(GridLevel(l1dim, nn, npt, crit, m2pd, nquads, pquads, cquads,
+                                    dens, counts, lists))
+    return levels
+
+
+@dataclasses.dataclass
+class HaloResult:
+    npart: np.ndarray     # (nhalos,) uint64
+    M_vir: np.ndarray
+    R_vir: np.ndarray
+    Phi0: np.ndarray
+    ipart: list           # per-halo uint64 arrays (bound members, in order)
+
+
+def read_halos(path: str) -> HaloResult:
+    with open(path, "rb") as f:
+        if f.read(8) != b"AHFHALO1":
+            raise ValueError(f"{path}: not an AHFHALO1 dump")
+        (nh,) = struct.unpack("<Q", f.read(8))
+        npart = np.zeros(nh, np.uint64)
+        mv = np.zeros(nh)
+        rv = np.zeros(nh)
+        p0 = np.zeros(nh)
+        lists = []
+        for i in range(nh):
+            (n,) = struct.unpack("<Q", f.read(8))
+            vals = struct.unpack("<dddd", f.read(32))
+            npart[i] = n
+            mv[i], rv[i], p0[i] = vals[0], vals[1], vals[2]
+            lists.append(np.fromfile(f, np.uint64, n))
+    return HaloResult(npart, mv, rv, p0, lists)
+
+
+def compare_grids(a: list[GridLevel], b: list[GridLevel], *, dens_exact=True, rtol=0.0):
+    """Return a list of human-readable differences (empty when identical)."""
+    diffs = []
+    if len(a) != len(b):
+        diffs.append(f"number of grids {len(a)} != {len(b)}")
+    for li, (x, y) in enumerate(zip(a, b)):
+        tag = f"level {li} (l1dim {x.l1dim})"
+        for fld in ("l1dim", "size_no_nodes", "size_no_part"):
+            if getattr(x, fld) != getattr(y, fld):
+                diffs.append(f"{tag}: {fld} {getattr(x, fld)} != {getattr(y, fld)}")
+        for fld in ("critdens", "masstopartdens"):
+            if getattr(x, fld) != getattr(y, fld):
+                diffs.append(f"{tag}: {fld} {getattr(x, fld)!r} != {getattr(y, fld)!r}")
+        for fld in ("nquads", "pquads", "cquads"):
+            u, v = getattr(x, fld), getattr(y, fld)
+            if u.shape != v.shape or not np.array_equal(u, v):
+                diffs.append(f"{tag}: {fld} differ ({u.shape} vs {v.shape})")
+        if x.dens.shape != y.dens.shape:
+            diffs.append(f"{tag}: node count {x.dens.shape[0]} != {y.dens.shape[0]}")
+            continue
+        if dens_exact:
+            bad = np.flatnonzero(x.dens.view(np.uint32) != y.dens.view(np.uint32))
+            if bad.size:
+                diffs.append(f"{tag}: {bad.size} dens values differ bitwise (first at node {bad[0]}: "
+                             f"{x.dens[bad[0]]!r} vs {y.dens[bad[0]]!r})")
+        else:
+            d = np.abs(x.dens.astype(np.float64) - y.dens.astype(np.float64))
+            scale = np.abs(y.dens.astype(np.float64)) + 1.0
+            worst = float(np.max(d / scale)) if d.size else 0.0
+            if worst > rtol:
+                diffs.append(f"{tag}: dens relative error {worst:.3e} > {rtol:.1e}")
+        if not np.array_equal(x.counts, y.counts):
+            diffs.append(f"{tag}: per-node particle counts differ")
+        elif not np.array_equal(x.lists, y.lists):
+            diffs.append(f"{tag}: per-node particle lists differ in content or order")
+    return diffs

@@ -1,0 +1,59 @@
+/*
+ * ahf_oracle.h -- CPU restatement of AHF's particle->grid and unbinding hot
+ * path.  TEST INFRASTRUCTURE ONLY: tests/, the smoke check and the CPU leg of
+ * the benchmark may call it, as the checker; the product path never does.
+ *
+ * Every function restates the reference function named in its comment,
+ * with the same floating-point operation order (FP32 density accumulated as
+ * float += double, FP64 everywhere in the unbinding except the FP32 |v|^2 of
+ * the bulk-velocity seed), compiled with -ffp-contract=off.
+ */
+#ifndef AHF_ORACLE_H
+#define AHF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* AMR hierarchy on a snapshot; writes the AHFGRID1 dump (same layout as
+ * oracle/ref_driver.c).  Returns number of grids, <0 on error. */
+int oracle_amr_dump(const char *snap_path, const char *out_path);
+
+/* sort_halo_particles + rem_unbound for every halo of the snapshot; writes the
+ * AHFHALO1 dump.  stable_ties != 0 orders equal distances by list position
+ * instead of reproducing indexx's quicksort permutation.  threads <= 0: all. */
+int oracle_unbind_dump(const char *snap_path, const char *out_path, int threads, int stable_ties);
+
+/* lower-level pieces, used by the known-answer tests */
+
+/* TSC weights of one coordinate (src/libamr_serial/density.c:356-360) */
+void oracle_tsc_weights(double pn_sep, double w[3]);
+
+/* indexx permutation (1-based, src/libutility/general.c:1093-1153) */
+void oracle_indexx(uint64_t n, const double *arr1, uint64_t *indx1);
+
+/* rem_unbound on one halo given by member indices (modified in place) */
+typedef struct {
+  int32_t minpart;
+  double vtune, r_fac, v_fac, phi_fac, hubble;
+} oracle_unbind_params;
+
+typedef struct {
+  uint64_t npart;     /* bound members after unbinding */
+  double M_vir, R_vir, Phi0;
+  int32_t niter;
+  int32_t rvir_set;   /* 0: no particle was ever bound (R_vir undefined in AHF) */
+} oracle_unbind_result;
+
+void oracle_sort_halo(const float *pos, uint64_t n, uint64_t *ipart, const double centre[3],
+                      int32_t minpart, int stable_ties);
+void oracle_rem_unbound(const float *pos, const float *mom, uint64_t *ipart, uint64_t n,
+                        const double centre[3], const oracle_unbind_params *p,
+                        oracle_unbind_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

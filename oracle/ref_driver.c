@@ -1,0 +1,382 @@
+/*
+ * ref_driver.c -- test harness that runs AHF's own hot-path functions on a
+ * snapshot file and dumps what they produce.  Test infrastructure only.
+ *
+ * Built two ways by oracle/Makefile:
+ *   - oracle/_ref/ahf_ref: compiled against the reference headers and linked
+ *     with the reference's own objects, compiled from the sources where they
+ *     lie under the reference tree (no copies, no stand-ins);
+ *   - _build/ahf_dropin_drv (tests only): the same harness linked against
+ *     this project's drop-in library, to show the drop-in is call-compatible.
+ *
+ * It follows the call order of src/main.c:614-648 (gen_domgrids, ll,
+ * zero_dens + assign_npart, gen_AMRhierarchy) and of the per-halo part of
+ * src/libahf/ahf_halos.c:504-510 + ahf_halos_sfc.c:140-149
+ * (sort_halo_particles, rem_unbound; the gather and rem_outsideRvir steps
+ * are replaced by the halo lists stored in the snapshot).
+ *
+ * Usage:
+ *   ahf_ref amr    <snap> <grids.out> [--reps R]   dump the AMR hierarchy
+ *   ahf_ref unbind <snap> <halos.out> [--threads T] [--reps R]
+ *   ahf_ref time   <snap> [--threads T] [--max-halo-part N]
+ *   ahf_ref layout                                  struct sizes/offsets (JSON)
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifdef AHF_REF_BUILD
+#include "common.h"
+#include "param.h"
+#include "tdef.h"
+#include "libamr_serial/amr_serial.h"
+#include "libahf/ahf_halos.h"
+#else
+#include "ahf_dropin.h"
+#endif
+
+#include "snapfile.h"
+
+static double now_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void die(const char *msg)
+{
+  fprintf(stderr, "ahf_ref: %s\n", msg);
+  exit(2);
+}
+
+/* the parameter derivation of src/startrun.c:499-547 for what the path reads */
+static void setup_globals(const snap_data *s)
+{
+  memset(&simu, 0, sizeof(simu));
+  simu.NGRID_DOM = s->h.L_dom;
+  simu.NGRID_MIN = s->h.L_dom;
+  simu.NGRID_MAX = s->h.L_max;
+  simu.Nth_dom = s->h.Nth_dom;
+  simu.Nth_ref = s->h.Nth_ref;
+  simu.AHF_MINPART = s->h.minpart;
+  simu.AHF_VTUNE = s->h.vtune;
+  simu.no_part = s->h.npart;
+  simu.no_vpart = (double)s->h.npart;
+  simu.np_limit = 0;
+  simu.mean_dens = 1.0;
+  simu.boxsize = s->h.boxsize;
+  simu.omega0 = s->h.omega0;
+  simu.lambda0 = s->h.lambda0;
+  simu.pmass = s->h.pmass;
+  simu.t_unit = 0.01;
+  memset(&global, 0, sizeof(global));
+  global.a = s->h.a;
+  global.fst_cycle = 1;
+  r_fac = s->h.r_fac;
+  v_fac = s->h.v_fac;
+  phi_fac = s->h.phi_fac;
+  Hubble = s->h.hubble;
+}
+
+static partptr make_particles(const snap_data *s)
+{
+  uint64_t n = s->h.npart;
+  partptr p = (partptr)calloc(n ? n : 1, sizeof(part));
+  if (!p) die("out of memory (particles)");
+  for (uint64_t i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      p[i].pos[d] = s->pos[3 * i + d];
+      p[i].mom[d] = s->mom[3 * i + d];
+    }
+    p[i].id = s->id[i];
+  }
+  return p;
+}
+
+/* ---------------------------------------------------------------- AMR ---- */
+
+typedef struct {
+  FILE *f;
+  partptr fst;
+} dumper;
+
+static void put(dumper *d, const void *p, size_t sz, size_t n)
+{
+  if (n && fwrite(p, sz, n, d->f) != n) die("write failed");
+}
+
+static void dump_grid(dumper *d, gridls *g)
+{
+  /* pass 1: count */
+  uint64_t n_nq = 0, n_nodes = 0, n_listed = 0, n_pq = 0, n_cq = 0;
+  for (pqptr pq = g->pquad; pq; pq = pq->next) {
+    n_pq++;
+    for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++)
+      for (cqptr cq = cq0; cq; cq = cq->next) {
+        n_cq++;
+        for (nqptr nq0 = cq->loc; nq0 < cq->loc + cq->length; nq0++)
+          for (nqptr nq = nq0; nq; nq = nq->next) {
+            n_nq++;
+            for (nptr nd = nq->loc; nd < nq->loc + nq->length; nd++) {
+              n_nodes++;
+              for (partptr pp = nd->ll; pp; pp = pp->ll) n_listed++;
+            }
+          }
+      }
+  }
+  uint64_t hdr[3] = {(uint64_t)g->l1dim, (uint64_t)g->size.no_nodes, (uint64_t)g->size.no_part};
+  double dh[2] = {g->critdens, g->masstopartdens};
+  uint64_t cnt[5] = {n_nq, n_nodes, n_listed, n_pq, n_cq};
+  put(d, hdr, 8, 3);
+  put(d, dh, 8, 2);
+  put(d, cnt, 8, 5);
+  /* nquads (z, y, x, length) in traversal order */
+  for (pqptr pq = g->pquad; pq; pq = pq->next) {
+    long z = pq->z;
+    for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++, z++)
+      for (cqptr cq = cq0; cq; cq = cq->next) {
+        long y = cq->y;
+        for (nqptr nq0 = cq->loc; nq0 < cq->loc + cq->length; nq0++, y++)
+          for (nqptr nq = nq0; nq; nq = nq->next) {
+            int32_t r[4] = {(int32_t)z, (int32_t)y, (int32_t)nq->x, (int32_t)nq->length};
+            put(d, r, 4, 4);
+          }
+      }
+  }
+  /* pquads (z, length) and cquads (z, y, length) */
+  for (pqptr pq = g->pquad; pq; pq = pq->next) {
+    int32_t r[2] = {(int32_t)pq->z, (int32_t)pq->length};
+    put(d, r, 4, 2);
+  }
+  for (pqptr pq = g->pquad; pq; pq = pq->next) {
+    long z = pq->z;
+    for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++, z++)
+      for (cqptr cq = cq0; cq; cq = cq->next) {
+        int32_t r[3] = {(int32_t)z, (int32_t)cq->y, (int32_t)cq->length};
+        put(d, r, 4, 3);
+      }
+  }
+  /* node dens (raw float bits), list lengths, list contents */
+#define NODE_LOOP(BODY)                                                     \
+  for (pqptr pq = g->pquad; pq; pq = pq->next)                              \
+    for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++)            \
+      for (cqptr cq = cq0; cq; cq = cq->next)                               \
+        for (nqptr nq0 = cq->loc; nq0 < cq->loc + cq->length; nq0++)        \
+          for (nqptr nq = nq0; nq; nq = nq->next)                           \
+            for (nptr nd = nq->loc; nd < nq->loc + nq->length; nd++) { BODY }
+  NODE_LOOP(float v = nd->dens; put(d, &v, 4, 1);)
+  NODE_LOOP(uint32_t c = 0; for (partptr pp = nd->ll; pp; pp = pp->ll) c++; put(d, &c, 4, 1);)
+  NODE_LOOP(for (partptr pp = nd->ll; pp; pp = pp->ll) {
+              uint64_t idx = (uint64_t)(pp - d->fst);
+              put(d, &idx, 8, 1);
+            })
+#undef NODE_LOOP
+}
+
+static double run_amr(const snap_data *s, partptr fst, gridls **grid_list_out, int *no_grids_out)
+{
+  int no_grids;
+  double t0 = now_s();
+  gridls *grid_list = gen_domgrids(&no_grids);
+  ll(s->h.npart, fst, global.dom_grid);
+  global.fst_part = fst;
+  global.no_part = s->h.npart;
+  zero_dens(global.dom_grid);
+  assign_npart(global.dom_grid);
+  global.fst_cycle = 1;
+  gen_AMRhierarchy(&grid_list, &no_grids);
+  double t1 = now_s();
+  *grid_list_out = grid_list;
+  *no_grids_out = no_grids;
+  return t1 - t0;
+}
+
+static void cmd_amr(const snap_data *s, const char *out)
+{
+  setup_globals(s);
+  partptr fst = make_particles(s);
+  gridls *grid_list;
+  int no_grids;
+  double t = run_amr(s, fst, &grid_list, &no_grids);
+  dumper d = {fopen(out, "wb"), fst};
+  if (!d.f) die("cannot open output");
+  put(&d, "AHFGRID1", 1, 8);
+  int32_t ng[2] = {no_grids, global.domgrid_no};
+  put(&d, ng, 4, 2);
+  for (int g = 0; g < no_grids; g++) dump_grid(&d, grid_list + g);
+  fclose(d.f);
+  fprintf(stdout, "{\"amr_seconds\": %.6f, \"no_grids\": %d}\n", t, no_grids);
+}
+
+/* ------------------------------------------------------------- unbind ---- */
+
+static HALO *make_halos(const snap_data *s)
+{
+  uint64_t nh = s->h.nhalos;
+  HALO *h = (HALO *)calloc(nh ? nh : 1, sizeof(HALO));
+  if (!h) die("out of memory (halos)");
+  uint64_t off = 0;
+  for (uint64_t i = 0; i < nh; i++) {
+    uint64_t n = s->halo_npart[i];
+    h[i].npart = n;
+    h[i].ipart = (long unsigned *)malloc((n ? n : 1) * sizeof(long unsigned));
+    for (uint64_t j = 0; j < n; j++) h[i].ipart[j] = s->ipart[off + j];
+    off += n;
+    h[i].pos.x = s->centre[3 * i + 0];
+    h[i].pos.y = s->centre[3 * i + 1];
+    h[i].pos.z = s->centre[3 * i + 2];
+  }
+  return h;
+}
+
+static double run_unbind(HALO *h, uint64_t nh, int threads)
+{
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  double t0 = now_s();
+  long i;
+  /* one halo per thread, dynamic schedule: src/libahf/ahf_halos.c:504-510 */
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++) {
+    sort_halo_particles(h + i);
+    rem_unbound(h + i);
+  }
+  return now_s() - t0;
+}
+
+static void cmd_unbind(const snap_data *s, const char *out, int threads)
+{
+  setup_globals(s);
+  partptr fst = make_particles(s);
+  global.fst_part = fst;
+  global.no_part = s->h.npart;
+  HALO *h = make_halos(s);
+  double t = run_unbind(h, s->h.nhalos, threads);
+  FILE *f = fopen(out, "wb");
+  if (!f) die("cannot open output");
+  fwrite("AHFHALO1", 1, 8, f);
+  uint64_t nh = s->h.nhalos;
+  fwrite(&nh, 8, 1, f);
+  for (uint64_t i = 0; i < nh; i++) {
+    uint64_t n = h[i].npart;
+    double v[4] = {h[i].M_vir, h[i].R_vir, h[i].Phi0, 0.0};
+    fwrite(&n, 8, 1, f);
+    fwrite(v, 8, 4, f);
+    for (uint64_t j = 0; j < n; j++) {
+      uint64_t q = h[i].ipart[j];
+      fwrite(&q, 8, 1, f);
+    }
+  }
+  fclose(f);
+  fprintf(stdout, "{\"unbind_seconds\": %.6f, \"threads\": %d}\n", t, threads);
+}
+
+/* -------------------------------------------------------------- time ---- */
+
+static void cmd_time(const snap_data *s, int threads, int reps)
+{
+  setup_globals(s);
+  double best_amr = 1e30, best_unb = 1e30, best_unb1 = 1e30;
+  int no_grids = 0;
+  for (int r = 0; r < reps; r++) {
+    setup_globals(s);
+    partptr fst = make_particles(s);
+    gridls *gl;
+    double t = run_amr(s, fst, &gl, &no_grids);
+    if (t < best_amr) best_amr = t;
+    free(fst); /* grids are leaked on purpose: freeing them is not timed */
+  }
+  partptr fst = make_particles(s);
+  global.fst_part = fst;
+  global.no_part = s->h.npart;
+  for (int r = 0; r < reps; r++) {
+    HALO *h = make_halos(s);
+    double t = run_unbind(h, s->h.nhalos, threads);
+    if (t < best_unb) best_unb = t;
+    HALO *h1 = make_halos(s);
+    double t1 = run_unbind(h1, s->h.nhalos, 1);
+    if (t1 < best_unb1) best_unb1 = t1;
+  }
+  fprintf(stdout,
+          "{\"amr_seconds\": %.6f, \"unbind_seconds\": %.6f, \"unbind_seconds_1thread\": %.6f, "
+          "\"threads\": %d, \"no_grids\": %d}\n",
+          best_amr, best_unb, best_unb1, threads, no_grids);
+}
+
+/* ------------------------------------------------------------ layout ---- */
+
+#define OFF(T, F) printf("\"%s.%s\": %zu, ", #T, #F, offsetof(T, F))
+static void cmd_layout(void)
+{
+  printf("{");
+  printf("\"sizeof.part\": %zu, \"sizeof.node\": %zu, \"sizeof.nquad\": %zu, ", sizeof(part),
+         sizeof(node), sizeof(nquad));
+  printf("\"sizeof.cquad\": %zu, \"sizeof.pquad\": %zu, \"sizeof.gridls\": %zu, ", sizeof(cquad),
+         sizeof(pquad), sizeof(gridls));
+  printf("\"sizeof.HALO\": %zu, \"sizeof.HALOPROFILE\": %zu, ", sizeof(HALO), sizeof(HALOPROFILE));
+  printf("\"sizeof.param_simu\": %zu, \"sizeof.info_global\": %zu, ", sizeof(param_simu),
+         sizeof(info_global));
+  OFF(part, pos); OFF(part, mom); OFF(part, sfckey); OFF(part, id);
+  OFF(node, dens); OFF(node, ll); OFF(node, force);
+  OFF(nquad, loc); OFF(nquad, x); OFF(nquad, length); OFF(nquad, next);
+  OFF(cquad, y); OFF(cquad, next); OFF(pquad, z); OFF(pquad, next);
+  OFF(gridls, pquad); OFF(gridls, no_pquad); OFF(gridls, pquad_array); OFF(gridls, l1dim);
+  OFF(gridls, critdens); OFF(gridls, masstodens); OFF(gridls, masstopartdens);
+  OFF(gridls, multistep); OFF(gridls, size); OFF(gridls, time); OFF(gridls, next);
+  OFF(HALO, npart); OFF(HALO, ipart); OFF(HALO, pos); OFF(HALO, M_vir); OFF(HALO, R_vir);
+  OFF(HALO, Phi0); OFF(HALO, prof); OFF(HALO, hostHalo); OFF(HALO, gatherRad);
+  OFF(HALO, numNodes);
+  OFF(info_global, dom_grid); OFF(info_global, domgrid_no); OFF(info_global, fst_part);
+  OFF(info_global, no_part); OFF(info_global, a); OFF(info_global, fin_l1dim);
+  OFF(info_global, fst_cycle); OFF(info_global, ioflag);
+  OFF(param_simu, NGRID_DOM); OFF(param_simu, NGRID_MAX); OFF(param_simu, Nth_dom);
+  OFF(param_simu, Nth_ref); OFF(param_simu, AHF_VTUNE); OFF(param_simu, AHF_MINPART);
+  OFF(param_simu, timeline); OFF(param_simu, mean_dens); OFF(param_simu, boxsize);
+  OFF(param_simu, pmass); OFF(param_simu, no_vpart); OFF(param_simu, no_part);
+  OFF(param_simu, NGRID_MIN); OFF(param_simu, np_limit);
+  printf("\"end\": 0}\n");
+}
+
+int main(int argc, char **argv)
+{
+  if (argc >= 2 && !strcmp(argv[1], "layout")) {
+    cmd_layout();
+    return 0;
+  }
+  if (argc < 3) die("usage: ahf_ref amr|unbind|time|layout <snap> [out] [options]");
+  int threads = 0, reps = 1;
+  for (int i = 3; i < argc; i++) {
+    if (!strcmp(argv[i], "--threads") && i + 1 < argc) threads = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--reps") && i + 1 < argc) reps = atoi(argv[++i]);
+  }
+  snap_data s;
+  int rc = snap_read(argv[2], &s);
+  if (rc) {
+    fprintf(stderr, "ahf_ref: cannot read snapshot %s (%d)\n", argv[2], rc);
+    return 2;
+  }
+  if (!strcmp(argv[1], "amr")) {
+    if (argc < 4) die("amr needs an output path");
+    cmd_amr(&s, argv[3]);
+  } else if (!strcmp(argv[1], "unbind")) {
+    if (argc < 4) die("unbind needs an output path");
+    cmd_unbind(&s, argv[3], threads);
+  } else if (!strcmp(argv[1], "time")) {
+    cmd_time(&s, threads, reps < 1 ? 1 : reps);
+  } else {
+    die("unknown command");
+  }
+  snap_free(&s);
+  return 0;
+}
