@@ -1,0 +1,327 @@
+/*
+ * amr_emu.c -- test-only host replay of the device AMR pipeline.
+ *
+ * Runs the same per-thread functions as the HIP kernels
+ * (ahf_amd/csrc/amr_core.h), one loop iteration per GPU thread and in the
+ * same kernel order as ahf_amd/csrc/amr_gpu.hip, then writes the AHFGRID1
+ * dump.  It exists so the level representation and every per-thread rule can
+ * be checked against the CPU oracle without a GPU; it is never linked into
+ * the library.
+ *
+ *   amr_emu <snapshot> <grids.out>
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../ahf_amd/csrc/amr_core.h"
+#include "../../ahf_amd/csrc/level_host.h"
+#include "../../oracle/snapfile.h"
+
+typedef struct {
+  level_view v;
+  uint64_t *oct_key;
+  uint8_t *oct_link;
+  int32_t *nbr;
+  uint64_t *row_key;
+  int32_t *row_oct0, *row_plane, *plane_z, *plane_row0;
+  int64_t *oct_parent;
+  uint32_t *pmask;
+  float *dens;
+  uint8_t *flags;
+  int64_t *node_p0;
+  uint32_t *plist;
+  uint8_t *moved;
+  int32_t *node2oct;
+  double m2d, m2pd, crit;
+  int rolled_back_child; /* next level was built and rolled back */
+  int64_t fine_n_oct;    /* octets of the rolled-back level */
+  int64_t *fine_p0;      /* its CSR (kept for the appended lists) */
+  uint32_t *fine_list;
+} lev_t;
+
+static void *xm(size_t n)
+{
+  void *p = calloc(n ? n : 1, 1);
+  if (!p) { fprintf(stderr, "emu: out of memory\n"); exit(3); }
+  return p;
+}
+
+static void set_view(lev_t *l, int64_t L, int64_t n_oct, int64_t n_row, int64_t n_plane)
+{
+  l->v.L = L; l->v.n_oct = n_oct; l->v.n_row = n_row; l->v.n_plane = n_plane;
+  l->v.oct_key = l->oct_key; l->v.oct_link = l->oct_link; l->v.nbr = l->nbr;
+  l->v.row_key = l->row_key; l->v.row_oct0 = l->row_oct0; l->v.row_plane = l->row_plane;
+  l->v.plane_z = l->plane_z; l->v.plane_row0 = l->plane_row0;
+}
+
+static void deposit(lev_t *l, const snap_data *s, const float *px, const float *py, const float *pz, int descending)
+{
+  int64_t nn = 8 * l->v.n_oct;
+  l->pmask = (uint32_t *)xm((size_t)nn * 4);
+  l->dens = (float *)xm((size_t)nn * 4);
+  for (int64_t n = 0; n < nn; n++) l->pmask[n] = ahf_presence_mask(&l->v, n / 8, (int)(n % 8));
+  for (int64_t n = 0; n < nn; n++)
+    l->dens[n] = ahf_gather_dens(&l->v, l->pmask, l->node_p0, l->plist, descending, px, py, pz, l->m2pd,
+                                 n / 8, (int)(n % 8));
+  (void)s;
+}
+
+int main(int argc, char **argv)
+{
+  if (argc < 3) { fprintf(stderr, "usage: amr_emu snap out\n"); return 2; }
+  snap_data s;
+  if (snap_read(argv[1], &s)) { fprintf(stderr, "emu: bad snapshot\n"); return 2; }
+  const int64_t N = (int64_t)s.h.npart, L0 = s.h.L_dom, half = L0 / 2;
+  float *px = (float *)xm((size_t)N * 4), *py = (float *)xm((size_t)N * 4), *pz = (float *)xm((size_t)N * 4);
+  for (int64_t p = 0; p < N; p++) { px[p] = s.pos[3 * p]; py[p] = s.pos[3 * p + 1]; pz[p] = s.pos[3 * p + 2]; }
+
+  lev_t lev[64];
+  memset(lev, 0, sizeof(lev));
+  int nlev = 1;
+  /* ---- domain level as octets of a virtual L/2 grid ---- */
+  lev_t *d = &lev[0];
+  int64_t no = half * half * half;
+  d->oct_key = (uint64_t *)xm((size_t)no * 8);
+  d->oct_link = (uint8_t *)xm((size_t)no);
+  d->nbr = (int32_t *)xm((size_t)no * 27 * 4);
+  d->row_key = (uint64_t *)xm((size_t)half * half * 8);
+  d->row_oct0 = (int32_t *)xm((size_t)(half * half + 1) * 4);
+  d->row_plane = (int32_t *)xm((size_t)half * half * 4);
+  d->plane_z = (int32_t *)xm((size_t)half * 4);
+  d->plane_row0 = (int32_t *)xm((size_t)(half + 1) * 4);
+  for (int64_t o = 0; o < no; o++) {
+    int64_t ox = o % half, oy = (o / half) % half, oz = o / (half * half);
+    d->oct_key[o] = ahf_pack3((uint64_t)ox, (uint64_t)oy, (uint64_t)oz);
+    d->oct_link[o] = ox < half - 1;
+    for (int ez = -1; ez <= 1; ez++)
+      for (int ey = -1; ey <= 1; ey++)
+        for (int ex = -1; ex <= 1; ex++) {
+          int64_t nx = (ox + ex + half) % half, ny = (oy + ey + half) % half, nz = (oz + ez + half) % half;
+          d->nbr[27 * o + ahf_e27(ex, ey, ez)] = (int32_t)((nz * half + ny) * half + nx);
+        }
+  }
+  for (int64_t r = 0; r < half * half; r++) {
+    d->row_key[r] = (uint64_t)(r % half) | ((uint64_t)(r / half) << 21);
+    d->row_oct0[r] = (int32_t)(r * half);
+    d->row_plane[r] = (int32_t)(r / half);
+  }
+  d->row_oct0[half * half] = (int32_t)no;
+  for (int64_t p = 0; p <= half; p++) {
+    if (p < half) d->plane_z[p] = (int32_t)p;
+    d->plane_row0[p] = (int32_t)(p * half);
+  }
+  set_view(d, L0, no, half * half, half);
+  double dl = (double)L0;
+  d->m2d = (dl * dl * dl) / (double)N;
+  d->m2pd = (dl * dl * dl) / (double)N;
+  d->crit = s.h.Nth_dom * d->m2d;
+  /* ll(): floor binning, stable bucket in array order */
+  {
+    int64_t nn = 8 * no;
+    int64_t *key = (int64_t *)xm((size_t)N * 8);
+    d->node_p0 = (int64_t *)xm((size_t)(nn + 1) * 8);
+    for (int64_t p = 0; p < N; p++) {
+      int64_t c[3];
+      for (int k = 0; k < 3; k++) {
+        uint64_t n = (uint64_t)(dl * (double)s.pos[3 * p + k]);
+        if (n > (uint64_t)(L0 - 1)) n = 0;
+        c[k] = (int64_t)n;
+      }
+      int64_t o = ((c[2] >> 1) * half + (c[1] >> 1)) * half + (c[0] >> 1);
+      key[p] = 8 * o + (c[2] & 1) * 4 + (c[1] & 1) * 2 + (c[0] & 1);
+      d->node_p0[key[p] + 1]++;
+    }
+    for (int64_t n = 0; n < nn; n++) d->node_p0[n + 1] += d->node_p0[n];
+    int64_t *fill = (int64_t *)xm((size_t)nn * 8);
+    d->plist = (uint32_t *)xm((size_t)N * 4);
+    for (int64_t p = 0; p < N; p++) d->plist[d->node_p0[key[p]] + fill[key[p]]++] = (uint32_t)p;
+    free(fill);
+    free(key);
+  }
+  deposit(d, &s, px, py, pz, 1);
+
+  int anomaly = 0;
+  for (;;) {
+    lev_t *c = &lev[nlev - 1];
+    const int64_t L = c->v.L, nn = 8 * c->v.n_oct;
+    c->moved = (uint8_t *)xm((size_t)c->node_p0[nn]);
+    if (L == s.h.L_max) break;
+    /* flags (test_node) */
+    c->flags = (uint8_t *)xm((size_t)nn);
+    for (int64_t n = 0; n < nn; n++)
+      c->flags[n] = ahf_node_flags(&c->v, c->pmask, c->dens, c->crit - 1.0, n / 8, (int)(n % 8));
+    /* refine rows (tested planes/rows only) */
+    int64_t nrank = 4 * c->v.n_row;
+    int64_t *cnt = (int64_t *)xm((size_t)(nrank + 1) * 8);
+    for (int64_t r = 0; r < c->v.n_row; r++)
+      for (int cz = 0; cz < 2; cz++)
+        for (int cy = 0; cy < 2; cy++) {
+          if (!ahf_plane_tested(&c->v, c->v.row_plane[r], cz)) continue;
+          if (!ahf_row_tested(&c->v, r, cy)) continue;
+          cnt[ahf_row_rank(&c->v, r, cy, cz)] = ahf_refine_row(&c->v, c->flags, r, cy, cz, &anomaly);
+        }
+    int64_t tot = 0;
+    for (int64_t k = 0; k < nrank; k++) tot += cnt[k];
+    if (tot == 0) { free(cnt); break; }
+    /* compaction: rows in rank order */
+    lev_t *f = &lev[nlev];
+    memset(f, 0, sizeof(*f));
+    f->oct_key = (uint64_t *)xm((size_t)tot * 8);
+    f->oct_link = (uint8_t *)xm((size_t)tot);
+    f->oct_parent = (int64_t *)xm((size_t)tot * 8);
+    f->row_key = (uint64_t *)xm((size_t)nrank * 8);
+    f->row_oct0 = (int32_t *)xm((size_t)(nrank + 1) * 4);
+    f->row_plane = (int32_t *)xm((size_t)nrank * 4);
+    f->plane_z = (int32_t *)xm((size_t)(2 * c->v.n_plane) * 4);
+    f->plane_row0 = (int32_t *)xm((size_t)(2 * c->v.n_plane + 1) * 4);
+    c->node2oct = (int32_t *)xm((size_t)nn * 4);
+    for (int64_t n = 0; n < nn; n++) c->node2oct[n] = -1;
+    int64_t no2 = 0, nr2 = 0, np2 = 0;
+    for (int64_t p = 0; p < c->v.n_plane; p++)
+      for (int cz = 0; cz < 2; cz++) {
+        int64_t Z = 2 * c->v.plane_z[p] + cz;
+        int plane_open = 0;
+        for (int64_t r = c->v.plane_row0[p]; r < c->v.plane_row0[p + 1]; r++)
+          for (int cy = 0; cy < 2; cy++) {
+            if (!cnt[ahf_row_rank(&c->v, r, cy, cz)]) continue;
+            int64_t Y = 2 * ahf_row_y(c->v.row_key[r]) + cy;
+            if (!plane_open) { f->plane_z[np2] = (int32_t)Z; f->plane_row0[np2] = (int32_t)nr2; np2++; plane_open = 1; }
+            f->row_key[nr2] = (uint64_t)Y | ((uint64_t)Z << 21);
+            f->row_oct0[nr2] = (int32_t)no2;
+            f->row_plane[nr2] = (int32_t)(np2 - 1);
+            nr2++;
+            for (int64_t o = c->v.row_oct0[r]; o < c->v.row_oct0[r + 1]; o++)
+              for (int cx = 0; cx < 2; cx++) {
+                int64_t n = 8 * o + cz * 4 + cy * 2 + cx;
+                if (!(c->flags[n] & NF_REFINED)) continue;
+                int64_t X = 2 * ahf_kx(c->v.oct_key[o]) + cx;
+                f->oct_key[no2] = ahf_pack3((uint64_t)X, (uint64_t)Y, (uint64_t)Z);
+                f->oct_link[no2] = (c->flags[n] & NF_LINK) ? 1 : 0;
+                f->oct_parent[no2] = n;
+                c->node2oct[n] = (int32_t)no2;
+                no2++;
+              }
+          }
+      }
+    f->row_oct0[nr2] = (int32_t)no2;
+    f->plane_row0[np2] = (int32_t)nr2;
+    free(cnt);
+    /* neighbour table */
+    f->nbr = (int32_t *)xm((size_t)no2 * 27 * 4);
+    for (int64_t o2 = 0; o2 < no2; o2++) {
+      int64_t pn = f->oct_parent[o2];
+      for (int ez = -1; ez <= 1; ez++)
+        for (int ey = -1; ey <= 1; ey++)
+          for (int ex = -1; ex <= 1; ex++) {
+            int64_t m = ahf_nbr_node(c->v.nbr, pn / 8, (int)(pn % 8), ex, ey, ez);
+            f->nbr[27 * o2 + ahf_e27(ex, ey, ez)] = m < 0 ? -1 : c->node2oct[m];
+          }
+    }
+    set_view(f, 2 * L, no2, nr2, np2);
+    f->m2d = c->m2d * 8.0;
+    f->m2pd = c->m2pd * 8.0;
+    f->crit = s.h.Nth_ref * f->m2d;
+    /* presence masks of the fine level (needed by relink) */
+    int64_t nnf = 8 * no2;
+    f->pmask = (uint32_t *)xm((size_t)nnf * 4);
+    for (int64_t n = 0; n < nnf; n++) f->pmask[n] = ahf_presence_mask(&f->v, n / 8, (int)(n % 8));
+    /* relink: counts then stable fill, mothers in node order */
+    f->node_p0 = (int64_t *)xm((size_t)(nnf + 1) * 8);
+    int8_t *child = (int8_t *)xm((size_t)c->node_p0[nn]);
+    for (int64_t m = 0; m < nn; m++) {
+      int32_t o2 = c->node2oct[m];
+      if (o2 < 0) continue;
+      uint8_t imask = 0;
+      for (int k = 0; k < 8; k++)
+        if (f->pmask[8 * (int64_t)o2 + k] == AHF_FULL27) imask |= (uint8_t)(1u << k);
+      uint64_t mk = c->v.oct_key[m / 8];
+      int cm = (int)(m % 8);
+      int64_t MX = 2 * ahf_kx(mk) + (cm & 1), MY = 2 * ahf_ky(mk) + ((cm >> 1) & 1), MZ = 2 * ahf_kz(mk) + (cm >> 2);
+      for (int64_t q = c->node_p0[m]; q < c->node_p0[m + 1]; q++) {
+        uint32_t p = c->plist[q];
+        int ch = ahf_relink_child(px[p], py[p], pz[p], MX, MY, MZ, (double)(2 * L), imask);
+        child[q] = (int8_t)ch;
+        if (ch >= 0) { f->node_p0[8 * (int64_t)o2 + ch + 1]++; c->moved[q] = 1; }
+      }
+    }
+    for (int64_t n = 0; n < nnf; n++) f->node_p0[n + 1] += f->node_p0[n];
+    f->plist = (uint32_t *)xm((size_t)f->node_p0[nnf] * 4);
+    int64_t *fill = (int64_t *)xm((size_t)nnf * 8);
+    for (int64_t m = 0; m < nn; m++) {
+      int32_t o2 = c->node2oct[m];
+      if (o2 < 0) continue;
+      for (int64_t q = c->node_p0[m]; q < c->node_p0[m + 1]; q++) {
+        if (child[q] < 0) continue;
+        int64_t fn = 8 * (int64_t)o2 + child[q];
+        f->plist[f->node_p0[fn] + fill[fn]++] = c->plist[q];
+      }
+    }
+    free(fill);
+    free(child);
+    /* deposit on the fine level: list order flips each level */
+    f->dens = (float *)xm((size_t)nnf * 4);
+    int fine_desc = (nlev % 2) == 0;
+    for (int64_t n = 0; n < nnf; n++)
+      f->dens[n] = ahf_gather_dens(&f->v, f->pmask, f->node_p0, f->plist, fine_desc, px, py, pz, f->m2pd,
+                                   n / 8, (int)(n % 8));
+    if (nnf < AHF_MIN_NNODES) {
+      /* relink_back: keep the fine lists to append them to the mothers */
+      c->rolled_back_child = 1;
+      c->fine_n_oct = no2;
+      c->fine_p0 = f->node_p0;
+      c->fine_list = f->plist;
+      break;
+    }
+    nlev++;
+  }
+
+  /* final per-node lists and dump */
+  host_level *hl = (host_level *)xm(sizeof(host_level) * (size_t)nlev);
+  for (int i = 0; i < nlev; i++) {
+    lev_t *c = &lev[i];
+    int64_t nn = 8 * c->v.n_oct;
+    host_level *h = &hl[i];
+    h->L = c->v.L;
+    h->n_oct = c->v.n_oct;
+    h->critdens = c->crit;
+    h->masstopartdens = c->m2pd;
+    h->size_no_nodes = (uint64_t)nn;
+    h->size_no_part = (uint64_t)c->node_p0[nn];
+    h->oct_key = c->oct_key;
+    h->oct_link = c->oct_link;
+    h->dens = c->dens;
+    h->list0 = (int64_t *)xm((size_t)(nn + 1) * 8);
+    h->list = (uint32_t *)xm((size_t)c->node_p0[nn] * 4 + 4);
+    int desc = (i % 2) == 0, fine_desc = ((i + 1) % 2) == 0;
+    int64_t w = 0;
+    for (int64_t n = 0; n < nn; n++) {
+      h->list0[n] = w;
+      int64_t p0 = c->node_p0[n], p1 = c->node_p0[n + 1];
+      for (int64_t k = 0; k < p1 - p0; k++) {
+        int64_t q = desc ? p1 - 1 - k : p0 + k;
+        if (!c->moved || !c->moved[q] || c->rolled_back_child) h->list[w++] = c->plist[q];
+      }
+      if (c->rolled_back_child && c->node2oct && c->node2oct[n] >= 0) {
+        /* moved particles were appended back child by child: undo the
+         * removal above by dropping them first, then append in fine order */
+        w = h->list0[n];
+        for (int64_t k = 0; k < p1 - p0; k++) {
+          int64_t q = desc ? p1 - 1 - k : p0 + k;
+          if (!c->moved[q]) h->list[w++] = c->plist[q];
+        }
+        for (int ch = 0; ch < 8; ch++) {
+          int64_t fn = 8 * (int64_t)c->node2oct[n] + ch;
+          int64_t f0 = c->fine_p0[fn], f1 = c->fine_p0[fn + 1];
+          for (int64_t k = 0; k < f1 - f0; k++) h->list[w++] = c->fine_list[fine_desc ? f1 - 1 - k : f0 + k];
+        }
+      }
+    }
+    h->list0[nn] = w;
+  }
+  int rc = hierarchy_dump(argv[2], hl, nlev);
+  printf("{\"levels\": %d, \"anomaly\": %d}\n", nlev, anomaly);
+  return rc ? 1 : (anomaly ? 3 : 0);
+}
