@@ -1,0 +1,135 @@
+"""ahf_amd -- MI355X (gfx950) implementation of AHF's particle->grid deposit,
+AMR refinement and halo unbinding hot path.
+
+The compute lives in ``lib/libahf_amd.so`` (hand-written HIP kernels behind
+the C ABI of ``include/ahf_gpu.h``).  This module is a thin ctypes host
+mirror of that ABI; it never computes anything itself and raises
+``NativeLibraryMissing`` when the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+MAX_LEVELS = 40
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+class AhfGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ahfgpu error {code}: {msg}")
+        self.code = code
+
+
+class AmrParams(ctypes.Structure):
+    _fields_ = [("L_dom", ctypes.c_int32), ("L_max", ctypes.c_int32),
+                ("Nth_dom", ctypes.c_double), ("Nth_ref", ctypes.c_double)]
+
+
+class AmrInfo(ctypes.Structure):
+    _fields_ = [("n_levels", ctypes.c_int32), ("rolled_back", ctypes.c_int32),
+                ("n_nodes", ctypes.c_int64 * MAX_LEVELS), ("n_part", ctypes.c_int64 * MAX_LEVELS),
+                ("deposits", ctypes.c_int64)]
+
+
+class UnbindParams(ctypes.Structure):
+    _fields_ = [("minpart", ctypes.c_int32), ("pad", ctypes.c_int32), ("vtune", ctypes.c_double),
+                ("r_fac", ctypes.c_double), ("v_fac", ctypes.c_double), ("phi_fac", ctypes.c_double),
+                ("hubble", ctypes.c_double)]
+
+
+class UnbindInfo(ctypes.Structure):
+    _fields_ = [("n_halos", ctypes.c_int64), ("n_member_in", ctypes.c_int64),
+                ("n_member_out", ctypes.c_int64), ("particle_iterations", ctypes.c_int64),
+                ("n_exact_replays", ctypes.c_int64), ("max_iterations", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib_path(name="libahf_amd.so"):
+    return os.path.join(LIBDIR, name)
+
+
+def load():
+    """Load the HIP library (raises NativeLibraryMissing if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise NativeLibraryMissing(f"{path} is not built; run __graft_entry__.build() or "
+                                   f"python ahf_amd/build.py")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    lib.ahfgpu_create.argtypes = [ctypes.POINTER(P), ctypes.c_int]
+    lib.ahfgpu_destroy.argtypes = [P]
+    lib.ahfgpu_destroy.restype = None
+    lib.ahfgpu_last_error.argtypes = [P]
+    lib.ahfgpu_last_error.restype = ctypes.c_char_p
+    lib.ahfgpu_set_particles.argtypes = [P, P, P, ctypes.c_uint64]
+    lib.ahfgpu_amr_build.argtypes = [P, ctypes.POINTER(AmrParams), ctypes.POINTER(AmrInfo)]
+    lib.ahfgpu_amr_dump.argtypes = [P, ctypes.c_char_p]
+    lib.ahfgpu_amr_level_sizes.argtypes = [P, ctypes.c_int] + [P] * 5
+    lib.ahfgpu_amr_level_copy.argtypes = [P, ctypes.c_int] + [P] * 5
+    lib.ahfgpu_last_timing.argtypes = [P, P, P, P, P]
+    if hasattr(lib, "ahfgpu_unbind"):
+        lib.ahfgpu_unbind.argtypes = [P, P, P, P, ctypes.c_uint64, ctypes.POINTER(UnbindParams),
+                                      ctypes.c_int, P, P, ctypes.POINTER(UnbindInfo)]
+    _lib = lib
+    return lib
+
+
+class Context:
+    """One device context of the HIP library (resident particles + results)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.ahfgpu_create(ctypes.byref(h), device)
+        if rc:
+            raise AhfGpuError(rc, f"cannot create a context on device {device}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ahfgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise AhfGpuError(rc, self.lib.ahfgpu_last_error(self.h).decode(errors="replace"))
+
+    def set_particles(self, pos, mom=None):
+        import numpy as np
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        momp = None
+        if mom is not None:
+            mom = np.ascontiguousarray(mom, dtype=np.float32)
+            momp = mom.ctypes.data
+        self._check(self.lib.ahfgpu_set_particles(self.h, pos.ctypes.data, momp, pos.shape[0]))
+
+    def amr_build(self, L_dom, L_max, Nth_dom, Nth_ref):
+        p = AmrParams(L_dom, L_max, Nth_dom, Nth_ref)
+        info = AmrInfo()
+        self._check(self.lib.ahfgpu_amr_build(self.h, ctypes.byref(p), ctypes.byref(info)))
+        return info
+
+    def amr_dump(self, path):
+        self._check(self.lib.ahfgpu_amr_dump(self.h, path.encode()))
+
+    def last_timing(self):
+        a, u, g = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self.lib.ahfgpu_last_timing(self.h, ctypes.byref(a), ctypes.byref(u), ctypes.byref(g),
+                                                ctypes.byref(n)))
+        return {"amr_ms": a.value, "unbind_ms": u.value, "gather_ms": g.value, "gather_launches": n.value}

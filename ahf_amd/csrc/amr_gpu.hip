@@ -1,0 +1,660 @@
+/*
+ * amr_gpu.hip -- device pipeline of AHF's AMR hierarchy on gfx950.
+ *
+ * One call builds what src/main.c:614-648 builds on the CPU (gen_domgrids,
+ * ll, zero_dens + assign_npart, the recursive gen_AMRhierarchy with
+ * refine_grid / relink / relink_back), level by level, with everything
+ * resident in HBM.  Per level the kernels are:
+ *
+ *   k_pmask        presence mask of every node (get_TSCnodes rules)
+ *   k_gather       exact-order TSC density of every node (assign_npart)
+ *   k_flags        interior + refinement criterion (test_node)
+ *   k_refine_rows  ref_pquad/ref_cquad/ref_nquad block tests and run state
+ *                  machine, one thread per level row
+ *   scans +
+ *   k_compact      octets of the next level, sorted (z,y,x)
+ *   k_nbr_next     27-octet neighbour table of the next level
+ *   k_relink_*     particles to the first interior child containing them
+ *
+ * The per-thread rules live in amr_core.h.  The domain level is binned by a
+ * stable LSD radix sort (devprims.h) so every node's bucket is in particle
+ * order, which fixes AHF's list order and hence its FP32 summation order.
+ */
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/ahf_gpu.h"
+#include "amr_core.h"
+#include "devprims.h"
+#include "gpu_ctx.h"
+#include "level_host.h"
+
+using namespace ahf;
+
+/* ================================================================ kernels */
+
+__global__ void k_domain_octets(int64_t half, uint64_t *__restrict__ key, uint8_t *__restrict__ link,
+                                int32_t *__restrict__ nbr)
+{
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t no = half * half * half;
+  if (o >= no) return;
+  const int64_t ox = o % half, oy = (o / half) % half, oz = o / (half * half);
+  key[o] = ahf_pack3((uint64_t)ox, (uint64_t)oy, (uint64_t)oz);
+  link[o] = ox < half - 1 ? 1 : 0;
+  for (int ez = -1; ez <= 1; ez++)
+    for (int ey = -1; ey <= 1; ey++)
+      for (int ex = -1; ex <= 1; ex++) {
+        const int64_t nx = (ox + ex + half) % half, ny = (oy + ey + half) % half, nz = (oz + ez + half) % half;
+        nbr[27 * o + ahf_e27(ex, ey, ez)] = (int32_t)((nz * half + ny) * half + nx);
+      }
+}
+
+__global__ void k_domain_rows(int64_t half, uint64_t *__restrict__ row_key, int32_t *__restrict__ row_oct0,
+                              int32_t *__restrict__ row_plane, int32_t *__restrict__ plane_z,
+                              int32_t *__restrict__ plane_row0)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r <= half * half) row_oct0[r] = (int32_t)(r * half);
+  if (r < half * half) {
+    row_key[r] = (uint64_t)(r % half) | ((uint64_t)(r / half) << AHF_KEY_BITS);
+    row_plane[r] = (int32_t)(r / half);
+  }
+  if (r <= half) plane_row0[r] = (int32_t)(r * half);
+  if (r < half) plane_z[r] = (int32_t)r;
+}
+
+/* ll(): cell = (unsigned long)(L*pos), cells beyond L-1 go to 0
+ * (src/libamr_serial/lltools.c:59-65); key = octet-major node index */
+__global__ void k_bin_domain(const float *__restrict__ px, const float *__restrict__ py,
+                             const float *__restrict__ pz, int64_t n, int64_t L, uint32_t *__restrict__ key,
+                             uint32_t *__restrict__ val)
+{
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double dl = (double)L;
+  const float pp[3] = {px[p], py[p], pz[p]};
+  int64_t c[3];
+  for (int d = 0; d < 3; d++) {
+    uint64_t v = (uint64_t)(dl * (double)pp[d]);
+    if (v > (uint64_t)(L - 1)) v = 0;
+    c[d] = (int64_t)v;
+  }
+  const int64_t half = L / 2;
+  const int64_t o = ((c[2] >> 1) * half + (c[1] >> 1)) * half + (c[0] >> 1);
+  key[p] = (uint32_t)(8 * o + (c[2] & 1) * 4 + (c[1] & 1) * 2 + (c[0] & 1));
+  val[p] = (uint32_t)p;
+}
+
+__global__ void k_iota(uint32_t *__restrict__ v, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (uint32_t)i;
+}
+
+/* CSR start of every node from the sorted keys: first index with key >= m */
+__global__ void k_csr_lower_bound(const uint32_t *__restrict__ skey, int64_t n, int64_t nn,
+                                  int64_t *__restrict__ p0)
+{
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m > nn) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)skey[mid] < m) lo = mid + 1; else hi = mid;
+  }
+  p0[m] = lo;
+}
+
+__global__ void k_pmask(level_view v, uint32_t *__restrict__ pmask)
+{
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= 8 * v.n_oct) return;
+  pmask[n] = ahf_presence_mask(&v, n >> 3, (int)(n & 7));
+}
+
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather(level_view v, const uint32_t *__restrict__ pmask,
+                                                      const int64_t *__restrict__ p0,
+                                                      const uint32_t *__restrict__ plist, int descending,
+                                                      const float *__restrict__ px, const float *__restrict__ py,
+                                                      const float *__restrict__ pz, double m2pd,
+                                                      float *__restrict__ dens)
+{
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= 8 * v.n_oct) return;
+  dens[n] = ahf_gather_dens(&v, pmask, p0, plist, descending, px, py, pz, m2pd, n >> 3, (int)(n & 7));
+}
+
+__global__ void k_flags(level_view v, const uint32_t *__restrict__ pmask, const float *__restrict__ dens,
+                        double thr, uint8_t *__restrict__ flags)
+{
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= 8 * v.n_oct) return;
+  flags[n] = ahf_node_flags(&v, pmask, dens, thr, n >> 3, (int)(n & 7));
+}
+
+/* one thread per level row (octet row r, cy, cz) */
+__global__ void k_refine_rows(level_view v, uint8_t *__restrict__ flags, int64_t *__restrict__ cnt,
+                              int32_t *__restrict__ anomaly)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * v.n_row) return;
+  const int64_t r = t >> 2;
+  const int cy = (int)(t & 1), cz = (int)((t >> 1) & 1);
+  int64_t c = 0;
+  if (ahf_plane_tested(&v, v.row_plane[r], cz) && ahf_row_tested(&v, r, cy)) {
+    int an = 0;
+    c = ahf_refine_row(&v, flags, r, cy, cz, &an);
+    if (an) atomicOr(anomaly, 1);
+  }
+  cnt[ahf_row_rank(&v, r, cy, cz)] = c;
+}
+
+/* has-flag of every level plane (octet plane p, cz) */
+__global__ void k_plane_has(level_view v, const int64_t *__restrict__ cnt, int32_t *__restrict__ has)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * v.n_plane) return;
+  const int64_t p = t >> 1;
+  const int cz = (int)(t & 1);
+  const int64_t q0 = v.plane_row0[p], nr = v.plane_row0[p + 1] - q0;
+  const int64_t r0 = 4 * q0 + cz * 2 * nr;
+  int32_t h = 0;
+  for (int64_t k = 0; k < 2 * nr && !h; k++) h = cnt[r0 + k] > 0;
+  has[t] = h;
+}
+
+__global__ void k_nonzero(const int64_t *__restrict__ cnt, int64_t n, int32_t *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = cnt[i] > 0;
+}
+
+/* writes the next level's octets, octet rows and node2oct of this level */
+__global__ void k_compact(level_view v, const uint8_t *__restrict__ flags, const int64_t *__restrict__ cnt,
+                          const int64_t *__restrict__ oct_off, const int32_t *__restrict__ row_off,
+                          const int32_t *__restrict__ plane_off, uint64_t *__restrict__ key2,
+                          uint8_t *__restrict__ link2, int64_t *__restrict__ parent2, uint64_t *__restrict__ rkey2,
+                          int32_t *__restrict__ roct2, int32_t *__restrict__ rplane2, int32_t *__restrict__ node2oct)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * v.n_row) return;
+  const int64_t r = t >> 2;
+  const int cy = (int)(t & 1), cz = (int)((t >> 1) & 1);
+  const int64_t rank = ahf_row_rank(&v, r, cy, cz);
+  if (cnt[rank] == 0) return;
+  const int64_t p = v.row_plane[r];
+  const int64_t Y = 2 * ahf_row_y(v.row_key[r]) + cy, Z = 2 * (int64_t)v.plane_z[p] + cz;
+  const int64_t nr = row_off[rank];
+  int64_t w = oct_off[rank];
+  rkey2[nr] = (uint64_t)Y | ((uint64_t)Z << AHF_KEY_BITS);
+  roct2[nr] = (int32_t)w;
+  rplane2[nr] = plane_off[2 * p + cz];
+  for (int64_t o = v.row_oct0[r]; o < v.row_oct0[r + 1]; o++)
+    for (int cx = 0; cx < 2; cx++) {
+      const int64_t nd = 8 * o + cz * 4 + cy * 2 + cx;
+      const uint8_t f = flags[nd];
+      if (!(f & NF_REFINED)) continue;
+      const int64_t X = 2 * ahf_kx(v.oct_key[o]) + cx;
+      key2[w] = ahf_pack3((uint64_t)X, (uint64_t)Y, (uint64_t)Z);
+      link2[w] = (f & NF_LINK) ? 1 : 0;
+      parent2[w] = nd;
+      node2oct[nd] = (int32_t)w;
+      w++;
+    }
+}
+
+__global__ void k_plane_fill(level_view v, const int32_t *__restrict__ has, const int32_t *__restrict__ plane_off,
+                             const int32_t *__restrict__ row_off, int32_t *__restrict__ pz2,
+                             int32_t *__restrict__ prow2)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * v.n_plane || !has[t]) return;
+  const int64_t p = t >> 1;
+  const int cz = (int)(t & 1);
+  const int64_t q0 = v.plane_row0[p], nr = v.plane_row0[p + 1] - q0;
+  const int32_t np = plane_off[t];
+  pz2[np] = 2 * v.plane_z[p] + cz;
+  prow2[np] = row_off[4 * q0 + cz * 2 * nr];
+}
+
+__global__ void k_set2(int32_t *__restrict__ a, int64_t ia, int32_t va, int32_t *__restrict__ b, int64_t ib,
+                       int32_t vb)
+{
+  a[ia] = va;
+  b[ib] = vb;
+}
+
+__global__ void k_nbr_next(const int32_t *__restrict__ nbr, const int64_t *__restrict__ parent2,
+                           const int32_t *__restrict__ node2oct, int64_t n2, int32_t *__restrict__ nbr2)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 27 * n2) return;
+  const int64_t o2 = t / 27;
+  const int e = (int)(t % 27);
+  const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
+  const int64_t pn = parent2[o2];
+  const int64_t m = ahf_nbr_node(nbr, pn >> 3, (int)(pn & 7), ex, ey, ez);
+  nbr2[t] = m < 0 ? -1 : node2oct[m];
+}
+
+/* relink, pass 1: per refined mother node, the child of each particle and
+ * the per-child counts (each fine node is written by its mother only) */
+__global__ void k_relink_count(int64_t nn, const uint64_t *__restrict__ key, const int64_t *__restrict__ p0,
+                               const uint32_t *__restrict__ plist, const int32_t *__restrict__ node2oct,
+                               const uint32_t *__restrict__ pmask2, const float *__restrict__ px,
+                               const float *__restrict__ py, const float *__restrict__ pz, double Lf,
+                               int8_t *__restrict__ child, int64_t *__restrict__ cnt2)
+{
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nn) return;
+  const int32_t o2 = node2oct[m];
+  if (o2 < 0) return;
+  uint8_t imask = 0;
+  for (int k = 0; k < 8; k++)
+    if (pmask2[8 * (int64_t)o2 + k] == AHF_FULL27) imask |= (uint8_t)(1u << k);
+  const uint64_t mk = key[m >> 3];
+  const int cm = (int)(m & 7);
+  const int64_t MX = 2 * ahf_kx(mk) + (cm & 1), MY = 2 * ahf_ky(mk) + ((cm >> 1) & 1),
+                MZ = 2 * ahf_kz(mk) + (cm >> 2);
+  int64_t c8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t q = p0[m]; q < p0[m + 1]; q++) {
+    const uint32_t p = plist[q];
+    const int ch = imask ? ahf_relink_child(px[p], py[p], pz[p], MX, MY, MZ, Lf, imask) : -1;
+    child[q] = (int8_t)ch;
+    if (ch >= 0) c8[ch]++;
+  }
+  for (int k = 0; k < 8; k++) cnt2[8 * (int64_t)o2 + k] = c8[k];
+}
+
+/* relink, pass 2: stable fill of the fine buckets (ascending particle index) */
+__global__ void k_relink_fill(int64_t nn, const int64_t *__restrict__ p0, const uint32_t *__restrict__ plist,
+                              const int32_t *__restrict__ node2oct, const int8_t *__restrict__ child,
+                              const int64_t *__restrict__ p02, uint32_t *__restrict__ plist2)
+{
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nn) return;
+  const int32_t o2 = node2oct[m];
+  if (o2 < 0) return;
+  int64_t w[8];
+  for (int k = 0; k < 8; k++) w[k] = p02[8 * (int64_t)o2 + k];
+  for (int64_t q = p0[m]; q < p0[m + 1]; q++) {
+    const int ch = child[q];
+    if (ch >= 0) plist2[w[ch]++] = plist[q];
+  }
+}
+
+/* ============================================================ host side */
+
+static inline unsigned nblk(int64_t n)
+{
+  return (unsigned)((n + AHF_BLOCK - 1) / AHF_BLOCK);
+}
+
+#define LAUNCH(kern, n, ...)                                                                 \
+  do {                                                                                       \
+    if ((n) > 0) hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(AHF_BLOCK), 0, c->st, __VA_ARGS__); \
+  } while (0)
+
+static level_view make_view(const DevLevel &l)
+{
+  level_view v;
+  v.L = l.L;
+  v.n_oct = l.n_oct;
+  v.oct_key = l.oct_key;
+  v.oct_link = l.oct_link;
+  v.nbr = l.nbr;
+  v.n_row = l.n_row;
+  v.row_key = l.row_key;
+  v.row_oct0 = l.row_oct0;
+  v.row_plane = l.row_plane;
+  v.n_plane = l.n_plane;
+  v.plane_z = l.plane_z;
+  v.plane_row0 = l.plane_row0;
+  return v;
+}
+
+static int gather_level(ahfgpu_ctx *c, DevLevel &l, int level_index)
+{
+  const int64_t nn = 8 * l.n_oct;
+  level_view v = make_view(l);
+  l.pmask = c->arena.get<uint32_t>(nn);
+  l.dens = c->arena.get<float>(nn);
+  if (!l.pmask || !l.dens) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", level_index);
+  LAUNCH(k_pmask, nn, v, l.pmask);
+  hipEvent_t e0 = c->timer_begin();
+  LAUNCH(k_gather, nn, v, l.pmask, l.node_p0, l.plist, (level_index % 2) == 0, c->px, c->py, c->pz, l.m2pd,
+         l.dens);
+  c->timer_end(e0, 12.0 * (double)l.n_part + 13.0 * (double)nn);
+  return 0;
+}
+
+static int build_domain(ahfgpu_ctx *c, DevLevel &d)
+{
+  const int64_t L = c->params.L_dom, half = L / 2, no = half * half * half, N = c->N;
+  d.L = L;
+  d.n_oct = no;
+  d.n_row = half * half;
+  d.n_plane = half;
+  Arena &A = c->arena;
+  d.oct_key = A.get<uint64_t>(no);
+  d.oct_link = A.get<uint8_t>(no);
+  d.nbr = A.get<int32_t>(27 * no);
+  d.row_key = A.get<uint64_t>(d.n_row);
+  d.row_oct0 = A.get<int32_t>(d.n_row + 1);
+  d.row_plane = A.get<int32_t>(d.n_row);
+  d.plane_z = A.get<int32_t>(d.n_plane);
+  d.plane_row0 = A.get<int32_t>(d.n_plane + 1);
+  d.node_p0 = A.get<int64_t>(8 * no + 1);
+  uint32_t *k0 = A.get<uint32_t>(N), *v0 = A.get<uint32_t>(N), *k1 = A.get<uint32_t>(N), *v1 = A.get<uint32_t>(N);
+  uint32_t *rtmp = A.get<uint32_t>(radix_tmp_u32<uint32_t>(N));
+  if (!d.oct_key || !d.oct_link || !d.nbr || !d.row_key || !d.row_oct0 || !d.row_plane || !d.plane_z ||
+      !d.plane_row0 || !d.node_p0 || !k0 || !v0 || !k1 || !v1 || !rtmp)
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (domain)");
+  LAUNCH(k_domain_octets, no, half, d.oct_key, d.oct_link, d.nbr);
+  LAUNCH(k_domain_rows, half * half + 1, half, d.row_key, d.row_oct0, d.row_plane, d.plane_z, d.plane_row0);
+  LAUNCH(k_bin_domain, N, c->px, c->py, c->pz, N, L, k0, v0);
+  int bits = 1;
+  while ((1ll << bits) < 8 * no) bits++;
+  const int where = radix_sort<uint32_t, uint32_t>(k0, v0, k1, v1, N, bits, rtmp, c->st);
+  uint32_t *skey = where ? k1 : k0;
+  d.plist = where ? v1 : v0;
+  LAUNCH(k_csr_lower_bound, 8 * no + 1, skey, N, 8 * no, d.node_p0);
+  d.n_part = N;
+  const double dl = (double)L;
+  d.m2d = (dl * dl * dl) / (double)N;
+  d.m2pd = (dl * dl * dl) / (double)N;
+  d.crit = c->params.Nth_dom * d.m2d;
+  return gather_level(c, d, 0);
+}
+
+/* refine level `ci` into a new level; returns 1 if a level was added,
+ * 0 when nothing refines or the new level is rolled back, <0 on error */
+static int refine_level(ahfgpu_ctx *c, int ci)
+{
+  DevLevel &l = c->lev[ci];
+  Arena &A = c->arena;
+  const int64_t nn = 8 * l.n_oct, nrank = 4 * l.n_row, nplr = 2 * l.n_plane;
+  level_view v = make_view(l);
+  l.flags = A.get<uint8_t>(nn);
+  int64_t *cnt = A.get<int64_t>(nrank + 1);
+  int64_t *oct_off = A.get<int64_t>(nrank + 1);
+  int32_t *rhas = A.get<int32_t>(nrank + 1);
+  int32_t *row_off = A.get<int32_t>(nrank + 1);
+  int32_t *phas = A.get<int32_t>(nplr + 1);
+  int32_t *plane_off = A.get<int32_t>(nplr + 1);
+  int64_t *stmp64 = A.get<int64_t>(scan_tmp_elems(nrank));
+  int32_t *stmp32 = A.get<int32_t>(scan_tmp_elems(nrank > nplr ? nrank : nplr));
+  if (!l.flags || !cnt || !oct_off || !rhas || !row_off || !phas || !plane_off || !stmp64 || !stmp32)
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (flags, level %d)", ci);
+  LAUNCH(k_flags, nn, v, l.pmask, l.dens, l.crit - 1.0, l.flags);
+  hipMemsetAsync(c->d_anomaly, 0, sizeof(int32_t), c->st);
+  LAUNCH(k_refine_rows, nrank, v, l.flags, cnt, c->d_anomaly);
+  exclusive_scan<int64_t, int64_t>(cnt, oct_off, nrank, stmp64, c->st);
+  LAUNCH(k_nonzero, nrank, cnt, nrank, rhas);
+  exclusive_scan<int32_t, int32_t>(rhas, row_off, nrank, stmp32, c->st);
+  LAUNCH(k_plane_has, nplr, v, cnt, phas);
+  exclusive_scan<int32_t, int32_t>(phas, plane_off, nplr, stmp32, c->st);
+  /* sizes of the new level */
+  struct {
+    int64_t n_oct;
+    int32_t n_row, n_plane, anomaly, pad;
+  } sz;
+  hipMemcpyAsync(&sz.n_oct, oct_off + nrank, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
+  hipMemcpyAsync(&sz.n_row, row_off + nrank, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
+  hipMemcpyAsync(&sz.n_plane, plane_off + nplr, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
+  hipMemcpyAsync(&sz.anomaly, c->d_anomaly, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
+  if (hipStreamSynchronize(c->st) != hipSuccess) return c->fail(AHFGPU_EHIP, "refine sizes: %s", hipGetErrorString(hipGetLastError()));
+  if (sz.anomaly)
+    return c->fail(AHFGPU_EUNSUP, "level %d: AHF would create an empty x-run here (undefined behaviour)", ci);
+  if (sz.n_oct == 0) return 0;
+  if (ci + 1 >= AHFGPU_MAX_LEVELS) return c->fail(AHFGPU_EINVAL, "too many levels");
+  DevLevel f;
+  f.L = 2 * l.L;
+  f.n_oct = sz.n_oct;
+  f.n_row = sz.n_row;
+  f.n_plane = sz.n_plane;
+  const int64_t n2 = f.n_oct, nn2 = 8 * n2;
+  f.oct_key = A.get<uint64_t>(n2);
+  f.oct_link = A.get<uint8_t>(n2);
+  f.oct_parent = A.get<int64_t>(n2);
+  f.nbr = A.get<int32_t>(27 * n2);
+  f.row_key = A.get<uint64_t>(f.n_row);
+  f.row_oct0 = A.get<int32_t>(f.n_row + 1);
+  f.row_plane = A.get<int32_t>(f.n_row);
+  f.plane_z = A.get<int32_t>(f.n_plane);
+  f.plane_row0 = A.get<int32_t>(f.n_plane + 1);
+  l.node2oct = A.get<int32_t>(nn);
+  l.child = A.get<int8_t>(l.n_part);
+  int64_t *cnt2 = A.get<int64_t>(nn2 + 1);
+  f.node_p0 = A.get<int64_t>(nn2 + 1);
+  int64_t *stmp2 = A.get<int64_t>(scan_tmp_elems(nn2));
+  f.pmask = A.get<uint32_t>(nn2);
+  if (!f.oct_key || !f.oct_link || !f.oct_parent || !f.nbr || !f.row_key || !f.row_oct0 || !f.row_plane ||
+      !f.plane_z || !f.plane_row0 || !l.node2oct || !l.child || !cnt2 || !f.node_p0 || !stmp2 || !f.pmask)
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", ci + 1);
+  hipMemsetAsync(l.node2oct, 0xFF, (size_t)nn * sizeof(int32_t), c->st);
+  hipMemsetAsync(l.child, 0xFF, (size_t)l.n_part, c->st);
+  hipMemsetAsync(cnt2, 0, (size_t)(nn2 + 1) * sizeof(int64_t), c->st);
+  LAUNCH(k_compact, nrank, v, l.flags, cnt, oct_off, row_off, plane_off, f.oct_key, f.oct_link, f.oct_parent,
+         f.row_key, f.row_oct0, f.row_plane, l.node2oct);
+  LAUNCH(k_plane_fill, nplr, v, phas, plane_off, row_off, f.plane_z, f.plane_row0);
+  hipLaunchKernelGGL(k_set2, dim3(1), dim3(1), 0, c->st, f.row_oct0, f.n_row, (int32_t)f.n_oct, f.plane_row0,
+                     f.n_plane, (int32_t)f.n_row);
+  LAUNCH(k_nbr_next, 27 * n2, l.nbr, f.oct_parent, l.node2oct, n2, f.nbr);
+  level_view fv = make_view(f);
+  LAUNCH(k_pmask, nn2, fv, f.pmask);
+  LAUNCH(k_relink_count, nn, nn, l.oct_key, l.node_p0, l.plist, l.node2oct, f.pmask, c->px, c->py, c->pz,
+         (double)f.L, l.child, cnt2);
+  exclusive_scan<int64_t, int64_t>(cnt2, f.node_p0, nn2, stmp2, c->st);
+  hipMemcpyAsync(&f.n_part, f.node_p0 + nn2, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
+  if (hipStreamSynchronize(c->st) != hipSuccess) return c->fail(AHFGPU_EHIP, "relink sizes: %s", hipGetErrorString(hipGetLastError()));
+  f.plist = A.get<uint32_t>(f.n_part > 0 ? f.n_part : 1);
+  if (!f.plist) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (relink, level %d)", ci + 1);
+  LAUNCH(k_relink_fill, nn, nn, l.node_p0, l.plist, l.node2oct, l.child, f.node_p0, f.plist);
+  f.m2d = l.m2d * AHF_CRITMULTI_D;
+  f.m2pd = l.m2pd * AHF_CRITMULTI_D;
+  f.crit = c->params.Nth_ref * f.m2d;
+  /* density of the new level: gather_level allocates pmask again, so reuse */
+  f.dens = A.get<float>(nn2);
+  if (!f.dens) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (dens, level %d)", ci + 1);
+  {
+    hipEvent_t e0 = c->timer_begin();
+    LAUNCH(k_gather, nn2, fv, f.pmask, f.node_p0, f.plist, ((ci + 1) % 2) == 0, c->px, c->py, c->pz, f.m2pd,
+           f.dens);
+    c->timer_end(e0, 12.0 * (double)f.n_part + 13.0 * (double)nn2);
+  }
+  if (nn2 < AHF_MIN_NNODES) {
+    /* relink_back (src/libamr_serial/generate_grids.c:231-238): the fine
+     * lists go back to their mothers; keep them for the final lists */
+    c->rb = f;
+    c->rolled_back = 1;
+    return 0;
+  }
+  c->lev.push_back(f);
+  return 1;
+}
+
+/* ============================================================== C ABI */
+
+extern "C" int ahfgpu_amr_build(ahfgpu_ctx *c, const ahfgpu_amr_params *p, ahfgpu_amr_info *info)
+{
+  if (!c || !p) return AHFGPU_EINVAL;
+  if (c->N <= 0) return c->fail(AHFGPU_EINVAL, "no particles resident (call ahfgpu_set_particles first)");
+  if (p->L_dom < 4 || (p->L_dom & (p->L_dom - 1)) || p->L_max < p->L_dom || (p->L_max & (p->L_max - 1)))
+    return c->fail(AHFGPU_EINVAL, "L_dom and L_max must be powers of two with 4 <= L_dom <= L_max");
+  if (p->L_max > (1 << 22)) return c->fail(AHFGPU_EINVAL, "L_max above 2^22 is not supported");
+  if (c->N >= (int64_t)UINT32_MAX) return c->fail(AHFGPU_EINVAL, "at most 2^32-1 particles");
+  if ((int64_t)p->L_dom * p->L_dom * p->L_dom >= ((int64_t)1 << 31))
+    return c->fail(AHFGPU_EINVAL, "domain grid too large");
+  c->params = *p;
+  c->arena.reset();
+  c->lev.clear();
+  c->rolled_back = 0;
+  c->rb = DevLevel();
+  c->timers_reset();
+  hipEventRecord(c->ev_a0, c->st);
+  c->lev.push_back(DevLevel());
+  int rc = build_domain(c, c->lev[0]);
+  if (rc) return rc;
+  for (;;) {
+    const int ci = (int)c->lev.size() - 1;
+    if (c->lev[ci].L == p->L_max) break;
+    rc = refine_level(c, ci);
+    if (rc < 0) return rc;
+    if (rc == 0) break;
+  }
+  hipEventRecord(c->ev_a1, c->st);
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "amr build: %s", hipGetErrorString(hipGetLastError()));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1);
+  c->amr_ms = ms;
+  c->timers_collect();
+  c->built = 1;
+  if (info) {
+    memset(info, 0, sizeof(*info));
+    info->n_levels = (int32_t)c->lev.size();
+    info->rolled_back = c->rolled_back;
+    for (size_t i = 0; i < c->lev.size() && i < AHFGPU_MAX_LEVELS; i++) {
+      info->n_nodes[i] = 8 * c->lev[i].n_oct;
+      info->n_part[i] = c->lev[i].n_part;
+      info->deposits += c->lev[i].n_part;
+    }
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------- read back levels */
+
+template <typename T>
+static std::vector<T> d2h(const T *d, int64_t n, hipStream_t st)
+{
+  std::vector<T> h((size_t)(n > 0 ? n : 0));
+  if (n > 0) hipMemcpyAsync(h.data(), d, (size_t)n * sizeof(T), hipMemcpyDeviceToHost, st);
+  return h;
+}
+
+/* final per-node lists of level `lev` in AHF's list order, on the host */
+static int final_lists(ahfgpu_ctx *c, int lev, std::vector<int64_t> &list0, std::vector<uint32_t> &list)
+{
+  const DevLevel &l = c->lev[lev];
+  const int64_t nn = 8 * l.n_oct;
+  const bool last = lev == (int)c->lev.size() - 1;
+  std::vector<int64_t> p0 = d2h(l.node_p0, nn + 1, c->st);
+  std::vector<uint32_t> pl = d2h(l.plist, l.n_part, c->st);
+  std::vector<int8_t> ch;
+  std::vector<int32_t> n2o;
+  std::vector<int64_t> fp0;
+  std::vector<uint32_t> fpl;
+  const bool moved_info = l.child != nullptr;
+  const bool rb = last && c->rolled_back;
+  if (moved_info) ch = d2h(l.child, l.n_part, c->st);
+  if (rb) {
+    n2o = d2h(l.node2oct, nn, c->st);
+    fp0 = d2h(c->rb.node_p0, 8 * c->rb.n_oct + 1, c->st);
+    fpl = d2h(c->rb.plist, c->rb.n_part, c->st);
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess) return c->fail(AHFGPU_EHIP, "read back lists");
+  list0.assign((size_t)nn + 1, 0);
+  list.resize((size_t)l.n_part);
+  const bool desc = (lev % 2) == 0, fdesc = ((lev + 1) % 2) == 0;
+  int64_t w = 0;
+  for (int64_t n = 0; n < nn; n++) {
+    list0[(size_t)n] = w;
+    const int64_t a = p0[(size_t)n], b = p0[(size_t)n + 1];
+    for (int64_t k = 0; k < b - a; k++) {
+      const int64_t q = desc ? b - 1 - k : a + k;
+      if (!moved_info || ch[(size_t)q] < 0) list[(size_t)w++] = pl[(size_t)q];
+    }
+    if (rb && n2o[(size_t)n] >= 0) {
+      for (int k = 0; k < 8; k++) {
+        const int64_t fn = 8 * (int64_t)n2o[(size_t)n] + k;
+        const int64_t f0 = fp0[(size_t)fn], f1 = fp0[(size_t)fn + 1];
+        for (int64_t j = 0; j < f1 - f0; j++) list[(size_t)w++] = fpl[(size_t)(fdesc ? f1 - 1 - j : f0 + j)];
+      }
+    }
+  }
+  list0[(size_t)nn] = w;
+  return 0;
+}
+
+extern "C" int ahfgpu_amr_level_sizes(ahfgpu_ctx *c, int lev, int64_t *L, int64_t *n_oct, int64_t *n_listed,
+                                      double *critdens, double *m2pd)
+{
+  if (!c || !c->built || lev < 0 || lev >= (int)c->lev.size()) return AHFGPU_EINVAL;
+  const DevLevel &l = c->lev[lev];
+  if (L) *L = l.L;
+  if (n_oct) *n_oct = l.n_oct;
+  if (n_listed) {
+    /* particles that stay on this level (plus any rolled back into it) */
+    int64_t n = l.n_part;
+    if (l.child && !(lev == (int)c->lev.size() - 1 && c->rolled_back)) {
+      int64_t moved = lev + 1 < (int)c->lev.size() ? c->lev[lev + 1].n_part : 0;
+      n -= moved;
+    }
+    *n_listed = n;
+  }
+  if (critdens) *critdens = l.crit;
+  if (m2pd) *m2pd = l.m2pd;
+  return 0;
+}
+
+extern "C" int ahfgpu_amr_level_copy(ahfgpu_ctx *c, int lev, uint64_t *oct_key, uint8_t *oct_link, float *dens,
+                                     int64_t *list0, uint32_t *list)
+{
+  if (!c || !c->built || lev < 0 || lev >= (int)c->lev.size()) return AHFGPU_EINVAL;
+  const DevLevel &l = c->lev[lev];
+  if (oct_key) hipMemcpyAsync(oct_key, l.oct_key, (size_t)l.n_oct * 8, hipMemcpyDeviceToHost, c->st);
+  if (oct_link) hipMemcpyAsync(oct_link, l.oct_link, (size_t)l.n_oct, hipMemcpyDeviceToHost, c->st);
+  if (dens) hipMemcpyAsync(dens, l.dens, (size_t)l.n_oct * 8 * 4, hipMemcpyDeviceToHost, c->st);
+  if (hipStreamSynchronize(c->st) != hipSuccess) return c->fail(AHFGPU_EHIP, "level copy");
+  if (list0 || list) {
+    std::vector<int64_t> l0;
+    std::vector<uint32_t> li;
+    int rc = final_lists(c, lev, l0, li);
+    if (rc) return rc;
+    if (list0) memcpy(list0, l0.data(), l0.size() * sizeof(int64_t));
+    if (list) memcpy(list, li.data(), (size_t)l0.back() * sizeof(uint32_t));
+  }
+  return 0;
+}
+
+extern "C" int ahfgpu_amr_dump(ahfgpu_ctx *c, const char *path)
+{
+  if (!c || !c->built || !path) return AHFGPU_EINVAL;
+  const int nl = (int)c->lev.size();
+  std::vector<host_level> hl((size_t)nl);
+  std::vector<std::vector<uint64_t>> keys((size_t)nl);
+  std::vector<std::vector<uint8_t>> links((size_t)nl);
+  std::vector<std::vector<float>> dens((size_t)nl);
+  std::vector<std::vector<int64_t>> l0((size_t)nl);
+  std::vector<std::vector<uint32_t>> li((size_t)nl);
+  for (int i = 0; i < nl; i++) {
+    const DevLevel &l = c->lev[i];
+    keys[(size_t)i] = d2h(l.oct_key, l.n_oct, c->st);
+    links[(size_t)i] = d2h(l.oct_link, l.n_oct, c->st);
+    dens[(size_t)i] = d2h(l.dens, 8 * l.n_oct, c->st);
+    int rc = final_lists(c, i, l0[(size_t)i], li[(size_t)i]);
+    if (rc) return rc;
+    host_level &h = hl[(size_t)i];
+    memset(&h, 0, sizeof(h));
+    h.L = l.L;
+    h.n_oct = l.n_oct;
+    h.critdens = l.crit;
+    h.masstopartdens = l.m2pd;
+    h.size_no_nodes = (uint64_t)(8 * l.n_oct);
+    h.size_no_part = (uint64_t)l.n_part;
+    h.oct_key = keys[(size_t)i].data();
+    h.oct_link = links[(size_t)i].data();
+    h.dens = dens[(size_t)i].data();
+    h.list0 = l0[(size_t)i].data();
+    h.list = li[(size_t)i].data();
+  }
+  return hierarchy_dump(path, hl.data(), nl) ? c->fail(AHFGPU_EIO, "cannot write %s", path) : 0;
+}

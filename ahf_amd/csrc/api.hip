@@ -1,0 +1,129 @@
+/*
+ * api.hip -- context management and particle upload of the C ABI
+ * (include/ahf_gpu.h).
+ */
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include "../../include/ahf_gpu.h"
+#include "devprims.h"
+#include "gpu_ctx.h"
+
+__global__ void k_deinterleave(const float *__restrict__ xyz, int64_t n, float *__restrict__ x,
+                               float *__restrict__ y, float *__restrict__ z)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = xyz[3 * i];
+  y[i] = xyz[3 * i + 1];
+  z[i] = xyz[3 * i + 2];
+}
+
+extern "C" int ahfgpu_create(ahfgpu_ctx **out, int device)
+{
+  if (!out) return AHFGPU_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return AHFGPU_EHIP;
+  }
+  if (device < 0 || device >= ndev) return AHFGPU_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return AHFGPU_EHIP;
+  ahfgpu_ctx *c = new ahfgpu_ctx();
+  c->device = device;
+  if (hipStreamCreate(&c->st) != hipSuccess || hipMalloc((void **)&c->d_anomaly, 64) != hipSuccess ||
+      hipEventCreate(&c->ev_a0) != hipSuccess || hipEventCreate(&c->ev_a1) != hipSuccess ||
+      hipEventCreate(&c->ev_u0) != hipSuccess || hipEventCreate(&c->ev_u1) != hipSuccess) {
+    delete c;
+    return AHFGPU_EHIP;
+  }
+  *out = c;
+  return 0;
+}
+
+static void free_particles(ahfgpu_ctx *c)
+{
+  float **ps[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz};
+  for (auto p : ps) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  c->cap = 0;
+}
+
+extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
+{
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->st);
+  c->arena.release();
+  c->uarena.release();
+  free_particles(c);
+  if (c->d_anomaly) (void)hipFree(c->d_anomaly);
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  hipEvent_t evs[] = {c->ev_a0, c->ev_a1, c->ev_u0, c->ev_u1};
+  for (auto e : evs)
+    if (e) (void)hipEventDestroy(e);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+extern "C" const char *ahfgpu_last_error(const ahfgpu_ctx *c)
+{
+  return c ? c->err : "no context";
+}
+
+extern "C" int ahfgpu_set_particles(ahfgpu_ctx *c, const float *pos, const float *mom, uint64_t n)
+{
+  if (!c || !pos || n == 0) return AHFGPU_EINVAL;
+  if (n >= (uint64_t)UINT32_MAX) return c->fail(AHFGPU_EINVAL, "at most 2^32-1 particles");
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  if ((int64_t)n > c->cap) {
+    free_particles(c);
+    float **ps[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz};
+    for (auto p : ps)
+      if (hipMalloc((void **)p, n * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        free_particles(c);
+        return c->fail(AHFGPU_ENOMEM, "particle arrays (%llu)", (unsigned long long)n);
+      }
+    c->cap = (int64_t)n;
+  }
+  float *stage = nullptr;
+  if (hipMalloc((void **)&stage, n * 3 * sizeof(float)) != hipSuccess) {
+    (void)hipGetLastError();
+    return c->fail(AHFGPU_ENOMEM, "staging buffer");
+  }
+  const unsigned nb = (unsigned)((n + AHF_BLOCK - 1) / AHF_BLOCK);
+  hipMemcpyAsync(stage, pos, n * 3 * sizeof(float), hipMemcpyHostToDevice, c->st);
+  hipLaunchKernelGGL(k_deinterleave, dim3(nb), dim3(AHF_BLOCK), 0, c->st, stage, (int64_t)n, c->px, c->py, c->pz);
+  if (mom) {
+    hipStreamSynchronize(c->st);
+    hipMemcpyAsync(stage, mom, n * 3 * sizeof(float), hipMemcpyHostToDevice, c->st);
+    hipLaunchKernelGGL(k_deinterleave, dim3(nb), dim3(AHF_BLOCK), 0, c->st, stage, (int64_t)n, c->vx, c->vy,
+                       c->vz);
+  } else {
+    hipMemsetAsync(c->vx, 0, n * sizeof(float), c->st);
+    hipMemsetAsync(c->vy, 0, n * sizeof(float), c->st);
+    hipMemsetAsync(c->vz, 0, n * sizeof(float), c->st);
+  }
+  hipError_t e = hipStreamSynchronize(c->st);
+  (void)hipFree(stage);
+  if (e != hipSuccess) return c->fail(AHFGPU_EHIP, "particle upload: %s", hipGetErrorString(e));
+  c->N = (int64_t)n;
+  c->built = 0;
+  return 0;
+}
+
+extern "C" int ahfgpu_last_timing(ahfgpu_ctx *c, double *amr_ms, double *unbind_ms, double *gather_ms,
+                                  int64_t *gather_launches)
+{
+  if (!c) return AHFGPU_EINVAL;
+  if (amr_ms) *amr_ms = c->amr_ms;
+  if (unbind_ms) *unbind_ms = c->unbind_ms;
+  if (gather_ms) *gather_ms = c->gather_ms;
+  if (gather_launches) *gather_launches = c->gather_launches;
+  return 0;
+}

@@ -1,0 +1,120 @@
+/*
+ * ahf_gpu.h -- C ABI of the MI355X (gfx950) implementation of AHF's
+ * particle->grid and unbinding hot path.
+ *
+ * Two layers share this library:
+ *
+ *  1. The device-resident API below (ahfgpu_*): particles are uploaded once
+ *     and stay in HBM; the whole AMR hierarchy (TSC deposit, refinement
+ *     flagging, fine-grid construction, relinking on every level) and the
+ *     unbinding of every halo run on the GPU.  Results are read back on
+ *     demand.  This replaces, as one call each:
+ *       gen_domgrids + ll + zero_dens/assign_npart + gen_AMRhierarchy
+ *         (src/main.c:614-648, src/libamr_serial/generate_grids.c:24-432,
+ *          lltools.c:32-92, density.c:238-492, refine_grid.c:113-985,
+ *          relink.c:31-423)
+ *       the per-halo sort_halo_particles + rem_unbound loop
+ *         (src/libahf/ahf_halos.c:504-510, 3292-3607, 5786-5894).
+ *
+ *  2. Drop-in versions of AHF's own entry points with AHF's prototypes and
+ *     data structures (ahf_dropin.h), implemented on top of layer 1.
+ *
+ * All functions return 0 on success and a negative AHFGPU_E* code on
+ * failure; ahfgpu_last_error() returns a message for the last failure of a
+ * context.  No function exits the process.  A context is bound to the
+ * device it was created on and must not be used by two threads at once.
+ */
+#ifndef AHF_GPU_H
+#define AHF_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AHFGPU_OK 0
+#define AHFGPU_EINVAL (-1)   /* bad argument or call order               */
+#define AHFGPU_EHIP (-2)     /* HIP runtime error                         */
+#define AHFGPU_ENOMEM (-3)   /* device or host allocation failed          */
+#define AHFGPU_EUNSUP (-4)   /* input reaches behaviour AHF leaves undefined */
+#define AHFGPU_EIO (-5)      /* file could not be read or written          */
+
+#define AHFGPU_MAX_LEVELS 40
+
+typedef struct ahfgpu_ctx ahfgpu_ctx;
+
+/* AMR parameters: AHF.input's LgridDomain, LgridMax, NperDomCell,
+ * NperRefCell (src/startrun.c:499-517) */
+typedef struct {
+  int32_t L_dom;
+  int32_t L_max;
+  double Nth_dom;
+  double Nth_ref;
+} ahfgpu_amr_params;
+
+typedef struct {
+  int32_t n_levels;                    /* grids kept (domain = level 0)        */
+  int32_t rolled_back;                 /* last attempted level had <125 nodes  */
+  int64_t n_nodes[AHFGPU_MAX_LEVELS];  /* nodes per level                      */
+  int64_t n_part[AHFGPU_MAX_LEVELS];   /* particles deposited per level        */
+  int64_t deposits;                    /* sum of n_part over levels            */
+} ahfgpu_amr_info;
+
+/* unbinding parameters: NminPerHalo, VescTune and the conversion factors of
+ * src/libahf/ahf_halos.c:199-213 */
+typedef struct {
+  int32_t minpart;
+  int32_t pad;
+  double vtune;
+  double r_fac;
+  double v_fac;
+  double phi_fac;
+  double hubble;
+} ahfgpu_unbind_params;
+
+typedef struct {
+  int64_t n_halos;
+  int64_t n_member_in;                 /* members before unbinding             */
+  int64_t n_member_out;                /* bound members after                  */
+  int64_t particle_iterations;         /* sum over halos and iterations        */
+  int64_t n_exact_replays;             /* halo iterations re-run sequentially  */
+  int32_t max_iterations;
+  int32_t pad;
+} ahfgpu_unbind_info;
+
+int ahfgpu_create(ahfgpu_ctx **ctx, int device);
+void ahfgpu_destroy(ahfgpu_ctx *ctx);
+const char *ahfgpu_last_error(const ahfgpu_ctx *ctx);
+
+/* copy n particles (positions in box units and AHF velocities, both as
+ * interleaved xyz FP32) from host memory to the device */
+int ahfgpu_set_particles(ahfgpu_ctx *ctx, const float *pos_xyz, const float *mom_xyz, uint64_t n);
+
+/* build the AMR hierarchy of the resident particles (device only; the last
+ * hierarchy stays resident until the next build) */
+int ahfgpu_amr_build(ahfgpu_ctx *ctx, const ahfgpu_amr_params *p, ahfgpu_amr_info *info);
+
+/* write the last hierarchy in the AHFGRID1 dump layout (tests, tools) */
+int ahfgpu_amr_dump(ahfgpu_ctx *ctx, const char *path);
+
+/* read back level `lev` of the last hierarchy.  Octets are the 2x2x2 node
+ * blocks of the level at resolution L/2 sorted by (z,y,x): key = ox |
+ * oy<<21 | oz<<42, link bit 0 = the x-run continues into the next octet.
+ * Nodes are numbered 8*octet + (cz*4+cy*2+cx).  list0/list are the final
+ * per-node particle lists in AHF's list order.  Pass NULL for unwanted
+ * arrays; sizes: n_oct, n_oct, 8*n_oct, 8*n_oct+1, list0[8*n_oct]. */
+int ahfgpu_amr_level_sizes(ahfgpu_ctx *ctx, int lev, int64_t *L, int64_t *n_oct, int64_t *n_listed,
+                           double *critdens, double *masstopartdens);
+int ahfgpu_amr_level_copy(ahfgpu_ctx *ctx, int lev, uint64_t *oct_key, uint8_t *oct_link, float *dens,
+                          int64_t *list0, uint32_t *list);
+
+/* device milliseconds of the last build / unbind (HIP events on the
+ * library's stream), and of the dominant kernel of the last build */
+int ahfgpu_last_timing(ahfgpu_ctx *ctx, double *amr_ms, double *unbind_ms, double *gather_ms,
+                       int64_t *gather_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,43 @@
+"""The per-thread rules of the HIP kernels (ahf_amd/csrc/amr_core.h),
+replayed on the host by tests/emu/amr_emu.c, against the CPU oracle."""
+import os
+import subprocess
+
+import pytest
+
+from ahf_amd import dumps
+
+from helpers import fuzz_snapshot
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("emu") / "amr_emu")
+    subprocess.run(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-Wno-unused-function", "-o", exe,
+                    os.path.join(HERE, "emu", "amr_emu.c"), "-lm"], check=True)
+    return exe
+
+
+def _cmp(oracle, emu, snap, tmp_path):
+    o, e = str(tmp_path / "o.grid"), str(tmp_path / "e.grid")
+    assert oracle.oracle_amr_dump(snap.encode(), o.encode()) > 0
+    r = subprocess.run([emu, snap, e], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    diffs = dumps.compare_grids(dumps.read_grids(e), dumps.read_grids(o))
+    assert not diffs, diffs[:5]
+
+
+@pytest.mark.parametrize("seed", range(200, 260))
+def test_emu_fuzz(oracle, emu, tmp_path, seed):
+    snap = str(tmp_path / "f.snap")
+    fuzz_snapshot(snap, seed)
+    _cmp(oracle, emu, snap, tmp_path)
+
+
+@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 21), (1, 64, 1001), (0, 32, 1000)])
+def test_emu_configs(oracle, emu, snapgen, tmp_path, cfg, side, seed):
+    snap = str(tmp_path / "g.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    _cmp(oracle, emu, snap, tmp_path)
