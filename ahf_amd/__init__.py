@@ -79,9 +79,12 @@ def load():
     lib.ahfgpu_amr_level_sizes.argtypes = [P, ctypes.c_int] + [P] * 5
     lib.ahfgpu_amr_level_copy.argtypes = [P, ctypes.c_int] + [P] * 5
     lib.ahfgpu_last_timing.argtypes = [P, P, P, P, P]
-    if hasattr(lib, "ahfgpu_unbind"):
-        lib.ahfgpu_unbind.argtypes = [P, P, P, P, ctypes.c_uint64, ctypes.POINTER(UnbindParams),
-                                      ctypes.c_int, P, P, ctypes.POINTER(UnbindInfo)]
+    lib.ahfgpu_set_halos.argtypes = [P, P, P, P, ctypes.c_uint64]
+    lib.ahfgpu_unbind.argtypes = [P, ctypes.POINTER(UnbindParams), ctypes.c_int, ctypes.c_double,
+                                  ctypes.POINTER(UnbindInfo)]
+    lib.ahfgpu_unbind_results.argtypes = [P, P, P, P]
+    lib.ahfgpu_release_halos.argtypes = [P]
+    lib.ahfgpu_release_halos.restype = None
     _lib = lib
     return lib
 
@@ -133,3 +136,29 @@ class Context:
         self._check(self.lib.ahfgpu_last_timing(self.h, ctypes.byref(a), ctypes.byref(u), ctypes.byref(g),
                                                 ctypes.byref(n)))
         return {"amr_ms": a.value, "unbind_ms": u.value, "gather_ms": g.value, "gather_launches": n.value}
+
+    def set_halos(self, centres, halo_npart, ipart):
+        import numpy as np
+        self._cen = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
+        self._hnp = np.ascontiguousarray(halo_npart, dtype=np.uint64)
+        ip = np.ascontiguousarray(ipart, dtype=np.uint64)
+        self._check(self.lib.ahfgpu_set_halos(self.h, self._cen.ctypes.data, self._hnp.ctypes.data,
+                                              ip.ctypes.data, self._hnp.shape[0]))
+
+    def unbind(self, minpart, vtune, r_fac, v_fac, phi_fac, hubble, mode=0, band=1e-4):
+        p = UnbindParams(minpart, 0, vtune, r_fac, v_fac, phi_fac, hubble)
+        info = UnbindInfo()
+        self._check(self.lib.ahfgpu_unbind(self.h, ctypes.byref(p), mode, band, ctypes.byref(info)))
+        return info
+
+    def unbind_results(self):
+        """(npart, props[nh,5] = M_vir, R_vir, Phi0, iterations, R_vir set, list of member arrays)"""
+        import numpy as np
+        nh = self._hnp.shape[0]
+        npart = np.zeros(nh, np.uint64)
+        props = np.zeros((nh, 5), np.float64)
+        ip = np.zeros(int(self._hnp.sum()) + 1, np.uint64)
+        self._check(self.lib.ahfgpu_unbind_results(self.h, npart.ctypes.data, props.ctypes.data, ip.ctypes.data))
+        off = np.concatenate([[0], np.cumsum(self._hnp.astype(np.int64))])
+        lists = [ip[off[h]:off[h] + int(npart[h])].copy() for h in range(nh)]
+        return npart, props, lists

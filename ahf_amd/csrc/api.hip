@@ -58,6 +58,7 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
+  ahfgpu_release_halos(c);
   c->arena.release();
   c->uarena.release();
   free_particles(c);

@@ -109,6 +109,7 @@ struct ahfgpu_ctx {
   int rolled_back = 0;
   int built = 0;
   int32_t *d_anomaly = nullptr;
+  void *halo_set = nullptr; /* resident halo lists (unbind_gpu.hip) */
   hipEvent_t ev_a0 = nullptr, ev_a1 = nullptr, ev_u0 = nullptr, ev_u1 = nullptr;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_next = 0;

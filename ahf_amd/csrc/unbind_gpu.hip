@@ -1,0 +1,711 @@
+/*
+ * unbind_gpu.hip -- AHF's radial sort + iterative unbinding of all haloes on
+ * gfx950 (src/libahf/ahf_halos.c:5786-5894 sort_halo_particles and
+ * :3292-3607 rem_unbound, driven per halo as in :504-510).
+ *
+ * 1. Sort: every member gets the key dist^2 (signed periodic wrap, FP64);
+ *    a stable LSD radix sort on the key followed by a stable sort on the
+ *    halo index orders each halo's members by distance, equal distances in
+ *    input order.  Haloes below NminPerHalo keep their order (AHF does not
+ *    sort them).
+ * 2. Unbinding: a persistent kernel, one workgroup per halo, haloes taken
+ *    largest first from an atomic queue.  Per iteration:
+ *      - dist_j, the mid-point terms of the integral of M(<r)/r^2 and their
+ *        FP64 prefix sum Phi_j; Phi0 = Phi_{n-1} + n/dist_{n-1} (AHF's two
+ *        passes add the same terms in the same order);
+ *      - the escape test of member j uses the bulk velocity of the seed and
+ *        of every bound member before j: a sequential recurrence.  It is
+ *        solved by sweeps that assume a bound set, prefix-sum the velocities
+ *        of that set and re-test; the sweeps stop at the fixed point, which
+ *        is the sequential answer;
+ *      - a member whose vel^2 lies within `band` (relative) of
+ *        VescTune^2 v_esc^2 is decided by an exact sequential replay of the
+ *        iteration with AHF's own operation order (rounding of the parallel
+ *        sums can only matter there);
+ *      - stable compaction of the bound members.
+ */
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/ahf_gpu.h"
+#include "devprims.h"
+#include "gpu_ctx.h"
+
+using namespace ahf;
+
+#define MZERO 5e-16 /* MACHINE_ZERO, src/param.h:125 */
+
+struct UnbindConst {
+  double v2_tune, v_fac, r_fac, phi_fac, hubble, band;
+  int32_t minpart, no_vbulk;
+};
+
+__device__ inline double wrap_s(double d)
+{
+  if (d > 0.5) d -= 1.0;
+  if (d < -0.5) d += 1.0;
+  return d;
+}
+
+/* ----------------------------------------------------------- sort keys */
+
+__device__ inline int64_t halo_of(const int64_t *off, int64_t nh, int64_t e)
+{
+  int64_t lo = 0, hi = nh - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_dist_keys(const int64_t *__restrict__ off, int64_t nh, int64_t ne, const uint32_t *__restrict__ ip,
+                            const double *__restrict__ cen, const float *__restrict__ px,
+                            const float *__restrict__ py, const float *__restrict__ pz, int32_t minpart,
+                            uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int64_t h = halo_of(off, nh, e);
+  val[e] = (uint32_t)e;
+  if (off[h + 1] - off[h] < minpart) {
+    key[e] = 0; /* not sorted by AHF: keep input order */
+    return;
+  }
+  const uint32_t p = ip[e];
+  const double dX = wrap_s((double)px[p] - cen[3 * h]);
+  const double dY = wrap_s((double)py[p] - cen[3 * h + 1]);
+  const double dZ = wrap_s((double)pz[p] - cen[3 * h + 2]);
+  const double d2 = dX * dX + dY * dY + dZ * dZ;
+  key[e] = (uint64_t)__double_as_longlong(d2); /* non-negative: bit order = value order */
+}
+
+__global__ void k_halo_keys(const int64_t *__restrict__ off, int64_t nh, int64_t ne, const uint32_t *__restrict__ v,
+                            uint32_t *__restrict__ hk)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ne) hk[i] = (uint32_t)halo_of(off, nh, (int64_t)v[i]);
+}
+
+__global__ void k_gather_members(const uint32_t *__restrict__ v, const uint32_t *__restrict__ ip, int64_t ne,
+                                  uint32_t *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ne) out[i] = ip[v[i]];
+}
+
+/* ------------------------------------------------------ block helpers */
+
+struct D4 {
+  double x, y, z, m;
+};
+
+__device__ inline D4 shfl_up4(D4 a, int d)
+{
+  D4 r;
+  r.x = __shfl_up(a.x, d, 64);
+  r.y = __shfl_up(a.y, d, 64);
+  r.z = __shfl_up(a.z, d, 64);
+  r.m = __shfl_up(a.m, d, 64);
+  return r;
+}
+
+/* exclusive block scan of D4; total in *tot (all threads) */
+__device__ inline D4 block_excl_scan4(D4 v, D4 *tot, D4 *sh /* [AHF_WAVES] */)
+{
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  D4 inc = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    D4 u = shfl_up4(inc, d);
+    if (lane >= d) {
+      inc.x += u.x; inc.y += u.y; inc.z += u.z; inc.m += u.m;
+    }
+  }
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  D4 base = {0, 0, 0, 0}, t = {0, 0, 0, 0};
+  for (int k = 0; k < AHF_WAVES; k++) {
+    const D4 s = sh[k];
+    if (k < w) { base.x += s.x; base.y += s.y; base.z += s.z; base.m += s.m; }
+    t.x += s.x; t.y += s.y; t.z += s.z; t.m += s.m;
+  }
+  __syncthreads();
+  *tot = t;
+  D4 ex;
+  ex.x = base.x + inc.x - v.x;
+  ex.y = base.y + inc.y - v.y;
+  ex.z = base.z + inc.z - v.z;
+  ex.m = base.m + inc.m - v.m;
+  return ex;
+}
+
+/* exact replay helper: AHF's indexx on n <= 64 values (1-based) */
+__device__ void dev_indexx(int n, const double *arr1, int *indx1)
+{
+  int stack[24];
+  int top = 0, lo = 1, hi = n;
+  for (int j = 1; j <= n; j++) indx1[j] = j;
+  if (n == 0) return;
+  for (;;) {
+    if (hi - lo < 7) {
+      for (int j = lo + 1; j <= hi; j++) {
+        const int it = indx1[j];
+        const double a = arr1[it];
+        int i = j - 1;
+        for (; i >= 1; i--) {
+          if (arr1[indx1[i]] <= a) break;
+          indx1[i + 1] = indx1[i];
+        }
+        indx1[i + 1] = it;
+      }
+      if (top == 0) break;
+      hi = stack[top--];
+      lo = stack[top--];
+    } else {
+      const int k = (lo + hi) >> 1;
+      int t;
+#define SW(a, b) { t = (a); (a) = (b); (b) = t; }
+      SW(indx1[k], indx1[lo + 1]);
+      if (arr1[indx1[lo + 1]] > arr1[indx1[hi]]) SW(indx1[lo + 1], indx1[hi]);
+      if (arr1[indx1[lo]] > arr1[indx1[hi]]) SW(indx1[lo], indx1[hi]);
+      if (arr1[indx1[lo + 1]] > arr1[indx1[lo]]) SW(indx1[lo + 1], indx1[lo]);
+      int i = lo + 1, j = hi;
+      const int it = indx1[lo];
+      const double a = arr1[it];
+      for (;;) {
+        do i++; while (arr1[indx1[i]] < a);
+        do j--; while (arr1[indx1[j]] > a);
+        if (j < i) break;
+        SW(indx1[i], indx1[j]);
+      }
+#undef SW
+      indx1[lo] = indx1[j];
+      indx1[j] = it;
+      top += 2;
+      if (hi - i + 1 >= j - lo) {
+        stack[top] = hi;
+        stack[top - 1] = i;
+        hi = j - 1;
+      } else {
+        stack[top] = j - 1;
+        stack[top - 1] = lo;
+        lo = i;
+      }
+    }
+  }
+}
+
+/* per-halo output record */
+struct HaloOut {
+  int64_t npart;
+  double M_vir, R_vir, Phi0;
+  int32_t niter, replays;
+  int32_t in_b; /* final list is in buffer b (0 = a) */
+  int32_t rvir_set;
+  int64_t particle_iterations;
+};
+
+/*
+ * The unbinding of one halo by one workgroup.  a/b: two member buffers at
+ * the halo's offset (sorted list in a); dist/phi: FP64 scratch; st: status.
+ */
+__device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint32_t *b, double *dist,
+                            double *phi, uint8_t *st, const double cen[3], const float *__restrict__ px,
+                            const float *__restrict__ py, const float *__restrict__ pz,
+                            const float *__restrict__ vx, const float *__restrict__ vy,
+                            const float *__restrict__ vz, HaloOut *out)
+{
+  __shared__ D4 sh4[AHF_WAVES];
+  __shared__ double shd[AHF_WAVES];
+  __shared__ int sflag[4];
+  __shared__ double sPhi0;
+  __shared__ int64_t s_seed;
+  __shared__ unsigned long long s_last; /* 1 + index of the last bound member, 0 = none */
+  __shared__ double s_rvir;
+  const int t = threadIdx.x;
+  int64_t n = n0;
+  uint32_t *cur = a, *nxt = b;
+  int niter = 0, replays = 0, in_b = 0, rvir_set = 0;
+  double R_vir = 0.0, Phi0 = 0.0, M_vir = (double)n;
+  int64_t pit = 0;
+  if (n < K.minpart) {
+    if (t == 0) {
+      out->npart = n; out->M_vir = 0.0; out->R_vir = 0.0; out->Phi0 = 0.0;
+      out->niter = 0; out->replays = 0; out->in_b = 0; out->rvir_set = 0; out->particle_iterations = 0;
+    }
+    return;
+  }
+  int64_t nremove = 4;
+  while (nremove > 3) {
+    niter++;
+    pit += n;
+    /* pass 1: distances */
+    for (int64_t j = t; j < n; j += AHF_BLOCK) {
+      const uint32_t p = cur[j];
+      const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
+                   dZ = wrap_s((double)pz[p] - cen[2]);
+      dist[j] = sqrt(dX * dX + dY * dY + dZ * dZ);
+    }
+    __syncthreads();
+    /* pass 2: mid-point terms and their prefix sum Phi_j */
+    double carry = 0.0;
+    for (int64_t base = 0; base < n; base += AHF_BLOCK) {
+      const int64_t j = base + t;
+      double term = 0.0;
+      if (j < n) {
+        const double dj = dist[j];
+        if (dj > MZERO) {
+          const double I_now = (double)(j + 1) / (dj * dj);
+          double I_prev = 0.0, d_prev = 0.0;
+          if (j > 0) {
+            d_prev = dist[j - 1];
+            if (d_prev > MZERO) I_prev = (double)j / (d_prev * d_prev);
+          }
+          const double I_mid = (I_now + I_prev) / 2.;
+          term = I_mid * (dj - d_prev);
+        }
+      }
+      double tot;
+      const double ex = block_excl_scan<double>(term, &tot);
+      if (j < n) phi[j] = carry + ex + term;
+      carry += tot;
+    }
+    __syncthreads();
+    if (t == 0) sPhi0 = phi[n - 1] + (double)n / dist[n - 1];
+    /* seed of the bulk velocity (:3441-3470) */
+    if (t == 0) {
+      if (niter == 1) {
+        double m2[65];
+        int ix[65];
+        const int nv = K.no_vbulk;
+        for (int j = 0; j < nv; j++) {
+          const uint32_t p = cur[j];
+          const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+          m2[j + 1] = (double)v2;
+        }
+        dev_indexx(nv, m2, ix);
+        s_seed = ix[nv / 2] - 1;
+      } else {
+        s_seed = 0;
+      }
+    }
+    for (int64_t j = t; j < n; j += AHF_BLOCK) st[j] = 1;
+    __syncthreads();
+    Phi0 = sPhi0;
+    const uint32_t ps = cur[s_seed];
+    const double sx = (double)vx[ps], sy = (double)vy[ps], sz = (double)vz[ps];
+    /* sweeps to the fixed point of the sequential bound-set recurrence */
+    int uncertain = 0;
+    for (int sweep = 0;; sweep++) {
+      if (t == 0) { sflag[0] = 0; sflag[1] = 0; }
+      __syncthreads();
+      D4 cv = {sx, sy, sz, 1.0};
+      for (int64_t base = 0; base < n; base += AHF_BLOCK) {
+        const int64_t j = base + t;
+        const int bj = j < n ? st[j] : 0;
+        uint32_t p = 0;
+        double fx = 0, fy = 0, fz = 0;
+        if (j < n) {
+          p = cur[j];
+          fx = (double)vx[p]; fy = (double)vy[p]; fz = (double)vz[p];
+        }
+        D4 c = {bj ? fx : 0.0, bj ? fy : 0.0, bj ? fz : 0.0, bj ? 1.0 : 0.0};
+        D4 tot;
+        D4 ex = block_excl_scan4(c, &tot, sh4);
+        int nb = bj;
+        if (j < n) {
+          const double VX = cv.x + ex.x, VY = cv.y + ex.y, VZ = cv.z + ex.z, M = cv.m + ex.m;
+          const double dj = dist[j];
+          double v_esc2;
+          if (dj > MZERO) v_esc2 = (2 * fabs(phi[j] - Phi0) * K.phi_fac);
+          else v_esc2 = 1e30;
+          const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
+                       dZ = wrap_s((double)pz[p] - cen[2]);
+          const double dVX = (fx - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
+          const double dVY = (fy - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
+          const double dVZ = (fz - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
+          const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
+          const double lim = K.v2_tune * v_esc2;
+          nb = !(vel2 > lim);
+          if (v_esc2 < 1e30 && fabs(vel2 - lim) <= K.band * lim) sflag[1] = 1;
+          if (nb != bj) { st[j] = (uint8_t)nb; sflag[0] = 1; }
+        }
+        /* later chunks see the updated statuses of this chunk */
+        D4 nv = {nb && j < n ? fx : 0.0, nb && j < n ? fy : 0.0, nb && j < n ? fz : 0.0, nb && j < n ? 1.0 : 0.0};
+        D4 tot2;
+        (void)block_excl_scan4(nv, &tot2, sh4);
+        cv.x += tot2.x; cv.y += tot2.y; cv.z += tot2.z; cv.m += tot2.m;
+      }
+      __syncthreads();
+      const int changed = sflag[0];
+      uncertain = sflag[1];
+      __syncthreads();
+      if (!changed) break;
+    }
+    /* exact replay of this iteration where rounding could decide (thread 0) */
+    if (uncertain) {
+      replays++;
+      if (t == 0) {
+        double I_now = 0, I_prev = 0, d_prev = 0, M_r = 0, P0 = 0, dd = 0;
+        for (int64_t j = 0; j < n; j++) {
+          M_r += 1.0;
+          dd = dist[j];
+          if (dd > MZERO) {
+            I_now = M_r / (dd * dd);
+            const double I_mid = (I_now + I_prev) / 2.;
+            P0 += I_mid * (dd - d_prev);
+          }
+          d_prev = dd;
+          I_prev = I_now;
+        }
+        P0 += M_r / dd;
+        sPhi0 = P0;
+        double VX = sx, VY = sy, VZ = sz, M = 1.0, Phi = 0;
+        I_now = 0; I_prev = 0; d_prev = 0; M_r = 0;
+        for (int64_t j = 0; j < n; j++) {
+          const uint32_t p = cur[j];
+          M_r += 1.0;
+          const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
+                       dZ = wrap_s((double)pz[p] - cen[2]);
+          dd = sqrt(dX * dX + dY * dY + dZ * dZ);
+          double v_esc2;
+          if (dd > MZERO) {
+            I_now = M_r / (dd * dd);
+            const double I_mid = (I_now + I_prev) / 2.;
+            Phi += I_mid * (dd - d_prev);
+            v_esc2 = (2 * fabs(Phi - P0) * K.phi_fac);
+          } else {
+            v_esc2 = 1e30;
+          }
+          const double fx = (double)vx[p], fy = (double)vy[p], fz = (double)vz[p];
+          const double dVX = (fx - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
+          const double dVY = (fy - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
+          const double dVZ = (fz - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
+          const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
+          if (vel2 > K.v2_tune * v_esc2) {
+            st[j] = 0;
+          } else {
+            st[j] = 1;
+            M += 1.0;
+            VX += 1.0 * vx[p];
+            VY += 1.0 * vy[p];
+            VZ += 1.0 * vz[p];
+          }
+          I_prev = I_now;
+          d_prev = dd;
+        }
+      }
+      __syncthreads();
+      Phi0 = sPhi0;
+    }
+    /* stable compaction into the other buffer; R_vir = distance of the last
+     * bound member */
+    if (t == 0) s_last = 0ull;
+    __syncthreads();
+    int64_t w0 = 0;
+    for (int64_t base = 0; base < n; base += AHF_BLOCK) {
+      const int64_t j = base + t;
+      const int bj = j < n ? st[j] : 0;
+      int64_t tot;
+      const int64_t ex = block_excl_scan<int64_t>((int64_t)bj, &tot);
+      if (bj) {
+        nxt[w0 + ex] = cur[j];
+        atomicMax(&s_last, (unsigned long long)(j + 1));
+      }
+      w0 += tot;
+    }
+    __syncthreads();
+    if (t == 0 && s_last > 0) s_rvir = dist[s_last - 1];
+    __syncthreads();
+    if (s_last > 0) {
+      R_vir = s_rvir;
+      rvir_set = 1;
+    }
+    nremove = n - w0;
+    n = w0;
+    M_vir = (double)w0;
+    uint32_t *tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+    in_b ^= 1;
+    __syncthreads();
+    if ((double)n < (double)K.minpart) break;
+  }
+  if (t == 0) {
+    out->npart = n;
+    out->M_vir = M_vir;
+    out->R_vir = R_vir;
+    out->Phi0 = Phi0;
+    out->niter = niter;
+    out->replays = replays;
+    out->in_b = in_b;
+    out->rvir_set = rvir_set;
+    out->particle_iterations = pit;
+  }
+}
+
+/* persistent form: workgroups take haloes from the queue (largest first) */
+__global__ void __launch_bounds__(AHF_BLOCK) k_unbind_batched(UnbindConst K, const int64_t *__restrict__ off,
+                                                              const int32_t *__restrict__ order, int64_t nh,
+                                                              int32_t *__restrict__ next, uint32_t *a, uint32_t *b,
+                                                              double *dist, double *phi, uint8_t *st,
+                                                              const double *__restrict__ cen,
+                                                              const float *__restrict__ px, const float *__restrict__ py,
+                                                              const float *__restrict__ pz, const float *__restrict__ vx,
+                                                              const float *__restrict__ vy, const float *__restrict__ vz,
+                                                              HaloOut *__restrict__ out)
+{
+  __shared__ int64_t sh_h;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const int32_t q = atomicAdd(next, 1);
+      sh_h = q < nh ? order[q] : -1;
+    }
+    __syncthreads();
+    const int64_t h = sh_h;
+    __syncthreads();
+    if (h < 0) return;
+    const int64_t o = off[h], n = off[h + 1] - o;
+    const double c3[3] = {cen[3 * h], cen[3 * h + 1], cen[3 * h + 2]};
+    unbind_halo(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz, out + h);
+    __syncthreads();
+  }
+}
+
+/* plain form: one halo per launch */
+__global__ void __launch_bounds__(AHF_BLOCK) k_unbind_one(UnbindConst K, int64_t h, const int64_t *__restrict__ off,
+                                                          uint32_t *a, uint32_t *b, double *dist, double *phi,
+                                                          uint8_t *st, const double *__restrict__ cen,
+                                                          const float *__restrict__ px, const float *__restrict__ py,
+                                                          const float *__restrict__ pz, const float *__restrict__ vx,
+                                                          const float *__restrict__ vy, const float *__restrict__ vz,
+                                                          HaloOut *__restrict__ out)
+{
+  const int64_t o = off[h], n = off[h + 1] - o;
+  const double c3[3] = {cen[3 * h], cen[3 * h + 1], cen[3 * h + 2]};
+  unbind_halo(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz, out + h);
+}
+
+__global__ void k_u64_to_u32(const uint64_t *__restrict__ in, int64_t n, uint32_t *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)in[i];
+}
+
+/* ============================================================ host side */
+
+struct HaloSet {
+  int64_t nh = 0, ne = 0;
+  int64_t *off = nullptr;     /* nh+1 */
+  uint32_t *ip_in = nullptr;  /* ne: members as given */
+  double *cen = nullptr;      /* nh*3 */
+  int32_t *order = nullptr;   /* haloes largest first */
+  std::vector<int64_t> h_off;
+  HaloOut *out = nullptr;
+  uint32_t *buf_a = nullptr, *buf_b = nullptr;
+  int done = 0;
+};
+
+static inline unsigned nb_of(int64_t n)
+{
+  return (unsigned)((n + AHF_BLOCK - 1) / AHF_BLOCK);
+}
+
+static void free_halos(HaloSet *s)
+{
+  if (!s) return;
+  void *ps[] = {s->off, s->ip_in, s->cen, s->order};
+  for (auto p : ps)
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+extern "C" int ahfgpu_set_halos(ahfgpu_ctx *c, const double *centres, const uint64_t *npart, const uint64_t *ipart,
+                                uint64_t nh)
+{
+  if (!c || (nh && (!centres || !npart))) return AHFGPU_EINVAL;
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  free_halos((HaloSet *)c->halo_set);
+  c->halo_set = nullptr;
+  HaloSet *s = new HaloSet();
+  s->nh = (int64_t)nh;
+  s->h_off.assign(nh + 1, 0);
+  for (uint64_t h = 0; h < nh; h++) s->h_off[h + 1] = s->h_off[h] + (int64_t)npart[h];
+  s->ne = s->h_off[nh];
+  if (s->ne >= (int64_t)UINT32_MAX) {
+    delete s;
+    return c->fail(AHFGPU_EINVAL, "at most 2^32-1 halo members in total");
+  }
+  for (int64_t e = 0; e < s->ne; e++)
+    if (ipart[e] >= (uint64_t)c->N) {
+      delete s;
+      return c->fail(AHFGPU_EINVAL, "member %lld refers to particle %llu beyond the resident set", (long long)e,
+                     (unsigned long long)ipart[e]);
+    }
+  std::vector<int32_t> order(nh);
+  for (uint64_t h = 0; h < nh; h++) order[h] = (int32_t)h;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return npart[x] > npart[y]; });
+  bool ok = hipMalloc((void **)&s->off, (nh + 1) * sizeof(int64_t)) == hipSuccess &&
+            hipMalloc((void **)&s->ip_in, (s->ne + 1) * sizeof(uint32_t)) == hipSuccess &&
+            hipMalloc((void **)&s->cen, (3 * nh + 1) * sizeof(double)) == hipSuccess &&
+            hipMalloc((void **)&s->order, (nh + 1) * sizeof(int32_t)) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    free_halos(s);
+    return c->fail(AHFGPU_ENOMEM, "halo arrays");
+  }
+  hipMemcpyAsync(s->off, s->h_off.data(), (nh + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->st);
+  if (nh) {
+    hipMemcpyAsync(s->cen, centres, 3 * nh * sizeof(double), hipMemcpyHostToDevice, c->st);
+    hipMemcpyAsync(s->order, order.data(), nh * sizeof(int32_t), hipMemcpyHostToDevice, c->st);
+  }
+  if (s->ne) {
+    uint64_t *stage = nullptr;
+    if (hipMalloc((void **)&stage, s->ne * sizeof(uint64_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      free_halos(s);
+      return c->fail(AHFGPU_ENOMEM, "halo staging");
+    }
+    hipMemcpyAsync(stage, ipart, s->ne * sizeof(uint64_t), hipMemcpyHostToDevice, c->st);
+    hipLaunchKernelGGL(k_u64_to_u32, dim3(nb_of(s->ne)), dim3(AHF_BLOCK), 0, c->st, stage, s->ne, s->ip_in);
+    hipStreamSynchronize(c->st);
+    (void)hipFree(stage);
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess) {
+    free_halos(s);
+    return c->fail(AHFGPU_EHIP, "halo upload");
+  }
+  c->halo_set = s;
+  return 0;
+}
+
+extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int mode, double band,
+                             ahfgpu_unbind_info *info)
+{
+  if (!c || !p) return AHFGPU_EINVAL;
+  HaloSet *s = (HaloSet *)c->halo_set;
+  if (!s) return c->fail(AHFGPU_EINVAL, "no halos resident (call ahfgpu_set_halos first)");
+  if (p->minpart < 2 || p->minpart / 2 > 64) return c->fail(AHFGPU_EINVAL, "NminPerHalo must be in [2,129]");
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  Arena &A = c->uarena;
+  A.reset();
+  const int64_t ne = s->ne, nh = s->nh;
+  UnbindConst K;
+  K.v2_tune = p->vtune * p->vtune;
+  K.v_fac = p->v_fac;
+  K.r_fac = p->r_fac;
+  K.phi_fac = p->phi_fac;
+  K.hubble = p->hubble;
+  K.band = band;
+  K.minpart = p->minpart;
+  K.no_vbulk = p->minpart / 2;
+  uint64_t *k0 = A.get<uint64_t>(ne), *k1 = A.get<uint64_t>(ne);
+  uint32_t *v0 = A.get<uint32_t>(ne), *v1 = A.get<uint32_t>(ne);
+  uint32_t *hk0 = A.get<uint32_t>(ne), *hk1 = A.get<uint32_t>(ne);
+  uint32_t *rtmp = A.get<uint32_t>(radix_tmp_u32<uint64_t>(ne));
+  s->buf_a = A.get<uint32_t>(ne);
+  s->buf_b = A.get<uint32_t>(ne);
+  double *dist = A.get<double>(ne), *phi = A.get<double>(ne);
+  uint8_t *st = A.get<uint8_t>(ne);
+  s->out = A.get<HaloOut>(nh);
+  int32_t *next = A.get<int32_t>(1);
+  if (!k0 || !k1 || !v0 || !v1 || !hk0 || !hk1 || !rtmp || !s->buf_a || !s->buf_b || !dist || !phi || !st ||
+      !s->out || !next)
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (unbinding)");
+  hipEventRecord(c->ev_u0, c->st);
+  if (ne > 0) {
+    hipLaunchKernelGGL(k_dist_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, ne, s->ip_in, s->cen,
+                       c->px, c->py, c->pz, p->minpart, k0, v0);
+    int w = radix_sort<uint64_t, uint32_t>(k0, v0, k1, v1, ne, 64, (uint32_t *)rtmp, c->st);
+    uint32_t *vs = w ? v1 : v0;
+    uint32_t *vo = w ? v0 : v1;
+    hipLaunchKernelGGL(k_halo_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, ne, vs, hk0);
+    int hb = 1;
+    while ((1ll << hb) < nh) hb++;
+    int w2 = radix_sort<uint32_t, uint32_t>(hk0, vs, hk1, vo, ne, hb, (uint32_t *)rtmp, c->st);
+    uint32_t *vsorted = w2 ? vo : vs;
+    hipLaunchKernelGGL(k_gather_members, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, vsorted, s->ip_in, ne,
+                       s->buf_a);
+  }
+  if (mode == 0) {
+    hipMemsetAsync(next, 0, sizeof(int32_t), c->st);
+    int dev_cu = 256;
+    hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+    const int64_t grid = std::min<int64_t>(nh > 0 ? nh : 1, (int64_t)dev_cu * 8);
+    hipLaunchKernelGGL(k_unbind_batched, dim3((unsigned)grid), dim3(AHF_BLOCK), 0, c->st, K, s->off, s->order, nh,
+                       next, s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
+                       s->out);
+  } else {
+    /* plain form: one halo per launch, results synchronised per halo as a
+     * per-call rem_unbound() would need them */
+    for (int64_t h = 0; h < nh; h++) {
+      hipLaunchKernelGGL(k_unbind_one, dim3(1), dim3(AHF_BLOCK), 0, c->st, K, h, s->off, s->buf_a, s->buf_b, dist,
+                         phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz, s->out);
+      hipStreamSynchronize(c->st);
+    }
+  }
+  hipEventRecord(c->ev_u1, c->st);
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "unbinding: %s", hipGetErrorString(hipGetLastError()));
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev_u0, c->ev_u1);
+  c->unbind_ms = ms;
+  s->done = 1;
+  if (info) {
+    std::vector<HaloOut> ho((size_t)nh);
+    if (nh) hipMemcpy(ho.data(), s->out, (size_t)nh * sizeof(HaloOut), hipMemcpyDeviceToHost);
+    memset(info, 0, sizeof(*info));
+    info->n_halos = nh;
+    info->n_member_in = ne;
+    for (auto &o : ho) {
+      info->n_member_out += o.npart;
+      info->particle_iterations += o.particle_iterations;
+      info->n_exact_replays += o.replays;
+      if (o.niter > info->max_iterations) info->max_iterations = o.niter;
+    }
+  }
+  return 0;
+}
+
+extern "C" int ahfgpu_unbind_results(ahfgpu_ctx *c, uint64_t *npart, double *props, uint64_t *ipart)
+{
+  if (!c) return AHFGPU_EINVAL;
+  HaloSet *s = (HaloSet *)c->halo_set;
+  if (!s || !s->done) return c->fail(AHFGPU_EINVAL, "no unbinding result");
+  const int64_t nh = s->nh, ne = s->ne;
+  std::vector<HaloOut> ho((size_t)nh);
+  std::vector<uint32_t> a((size_t)ne), b((size_t)ne);
+  if (nh) hipMemcpy(ho.data(), s->out, (size_t)nh * sizeof(HaloOut), hipMemcpyDeviceToHost);
+  if (ne) {
+    hipMemcpy(a.data(), s->buf_a, (size_t)ne * 4, hipMemcpyDeviceToHost);
+    hipMemcpy(b.data(), s->buf_b, (size_t)ne * 4, hipMemcpyDeviceToHost);
+  }
+  for (int64_t h = 0; h < nh; h++) {
+    const HaloOut &o = ho[(size_t)h];
+    if (npart) npart[h] = (uint64_t)o.npart;
+    if (props) {
+      props[5 * h + 0] = o.M_vir;
+      props[5 * h + 1] = o.R_vir;
+      props[5 * h + 2] = o.Phi0;
+      props[5 * h + 3] = (double)o.niter;
+      props[5 * h + 4] = (double)o.rvir_set;
+    }
+    if (ipart) {
+      const std::vector<uint32_t> &src = o.in_b ? b : a;
+      const int64_t base = s->h_off[(size_t)h];
+      for (int64_t j = 0; j < o.npart; j++) ipart[base + j] = src[(size_t)(base + j)];
+    }
+  }
+  return 0;
+}
+
+extern "C" void ahfgpu_release_halos(ahfgpu_ctx *c)
+{
+  if (!c) return;
+  free_halos((HaloSet *)c->halo_set);
+  c->halo_set = nullptr;
+}

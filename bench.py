@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X deposit + unbinding path on one GPU.
+
+One step = one pass of the hot path over the resident snapshot:
+  the full AMR hierarchy (binning, TSC deposit, refinement flagging,
+  fine-grid construction and relinking on every level) followed by the
+  radial sort and unbinding of every halo (batched persistent form).
+Inputs (particles, halo member lists) are resident in HBM before timing.
+
+Default workload: 256^3 = 16,777,216 particles in a 100 Mpc/h box with 2000
+NFW hosts (dN/dM ~ M^-1.9 on [1e10,1e14] Msun/h, hosts above 1e12 carrying
+5-20 subhaloes with 10% of their mass, 10% interlopers), LgridDomain 128,
+LgridMax 32768, NperDomCell = NperRefCell = 5, VescTune 1.5, NminPerHalo 20.
+
+Prints one JSON line (rank 0).  Run `python bench.py --help` for options.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particles/s (TSC deposit + unbinding) at 256^3; batched vs plain GPU vs CPU"
+HBM_PEAK_GBS = 8000.0
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ahf_ref")
+
+
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def snapshot_path(cfg, side, seed):
+    d = os.environ.get("AHF_BENCH_DIR", "/tmp")
+    return os.path.join(d, f"ahf_bench_cfg{cfg}_{side}_{seed}.snap")
+
+
+def make_snapshot(cfg, side, seed):
+    import ctypes
+    path = snapshot_path(cfg, side, seed)
+    if os.path.exists(path):
+        return path
+    lib_path = os.path.join(ROOT, "ahf_amd", "lib", "libahf_snapgen.so")
+    if not os.path.exists(lib_path):
+        from ahf_amd import build
+        build.build_snapgen()
+    lib = ctypes.CDLL(lib_path)
+    lib.ahf_snapgen_write.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_char_p]
+    tmp = path + f".{os.getpid()}.tmp"
+    t0 = time.time()
+    rc = lib.ahf_snapgen_write(cfg, side, seed, tmp.encode())
+    if rc:
+        raise RuntimeError(f"snapshot generation failed ({rc})")
+    os.replace(tmp, path)
+    log(f"generated {path} in {time.time() - t0:.1f}s")
+    return path
+
+
+def cpu_baseline(snap, threads):
+    """The reference's own compiled functions (oracle/_ref/ahf_ref) on the
+    same snapshot: AMR build single-threaded as in the reference's default
+    build, unbinding with OpenMP over haloes."""
+    if not os.path.exists(REF_BIN):
+        return None
+    r = subprocess.run([REF_BIN, "time", snap, "--threads", str(threads), "--reps", "1"],
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log(f"reference timing failed: {r.stderr[-500:]}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1, help="0, 1 (default) or 3: box workloads")
+    ap.add_argument("--side", type=int, default=0, help="particles per dimension (0: full size)")
+    ap.add_argument("--seed", type=int, default=1001)
+    ap.add_argument("--plain-steps", type=int, default=2, help="steps of the one-halo-per-launch form")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU timing")
+    ap.add_argument("--band", type=float, default=1e-4)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              f"the path is sharded by halo and by slab as described in DESIGN.md, "
+              f"but multi-GPU runs are not part of it.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import ahf_amd
+    from ahf_amd import dumps
+
+    snap = make_snapshot(args.config, args.side, args.seed)
+    s = dumps.read_snapshot(snap)
+    log(f"snapshot: {s.npart} particles, {s.nhalos} haloes, {s.n_ipart} members, L_dom {s.L_dom}")
+
+    ctx = ahf_amd.Context(0)
+    ctx.set_particles(s.pos, s.mom)
+    ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+    uargs = (s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble)
+
+    def step(mode=0):
+        info = ctx.amr_build(s.L_dom, s.L_max, s.Nth_dom, s.Nth_ref)
+        uinfo = ctx.unbind(*uargs, mode=mode, band=args.band)
+        return info, uinfo
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup} done")
+
+    amr_ms, unb_ms, g_ms, g_n, g_bytes = [], [], [], [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        info, uinfo = step()
+        t = ctx.last_timing()
+        amr_ms.append(t["amr_ms"])
+        unb_ms.append(t["unbind_ms"])
+        g_ms.append(t["gather_ms"])
+        g_n.append(t["gather_launches"])
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = s.npart * args.steps / wall
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step; amr {np.mean(amr_ms):.3f} ms, "
+        f"unbind {np.mean(unb_ms):.3f} ms (device events)")
+
+    # plain form: one halo per launch, synchronised per halo
+    plain = None
+    if args.plain_steps > 0:
+        tp0 = time.perf_counter()
+        for i in range(args.plain_steps):
+            step(mode=1)
+        tp1 = time.perf_counter()
+        plain = s.npart * args.plain_steps / (tp1 - tp0)
+        log(f"plain form: {1e3 * (tp1 - tp0) / args.plain_steps:.3f} ms/step")
+
+    deposits = int(info.deposits)
+    gather_avg_ms = float(np.sum(g_ms) / np.sum(g_n))
+    # algorithmic bytes of one deposit launch: 12 B (FP32 position) per
+    # deposited particle + 13 B per node (8 B density accumulator, 4 B count,
+    # 1 B flag), averaged over the launches of one build
+    levels = [(int(info.n_part[i]), int(info.n_nodes[i])) for i in range(info.n_levels)]
+    alg_bytes = sum(12 * p + 13 * n for p, n in levels) / max(1, len(levels))
+    achieved = alg_bytes / (gather_avg_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, ahf_amd/csrc/snapgen.c)",
+        "config": {
+            "workload": (f"{s.npart} particles ({round(s.npart ** (1 / 3))}^3), {s.boxsize:g} Mpc/h box, "
+                         f"{s.nhalos} NFW haloes incl. subhaloes, LgridDomain {s.L_dom}, LgridMax {s.L_max}, "
+                         f"NperDomCell {s.Nth_dom:g}, NperRefCell {s.Nth_ref:g}, VescTune {s.vtune:g}, "
+                         f"NminPerHalo {s.minpart}"),
+            "levels": int(info.n_levels),
+            "deposits_per_step": deposits,
+            "halo_members": int(s.n_ipart),
+            "seed": args.seed,
+        },
+        "amr_ms": float(np.mean(amr_ms)),
+        "unbind_ms": float(np.mean(unb_ms)),
+        "deposits_per_s": deposits / (np.mean(amr_ms) * 1e-3),
+        "halos_per_s": s.nhalos / (np.mean(unb_ms) * 1e-3),
+        "particle_iterations_per_s": uinfo.particle_iterations / (np.mean(unb_ms) * 1e-3),
+        "exact_replays": int(uinfo.n_exact_replays),
+        "plain_value": plain,
+        "batched_over_plain": (value / plain) if plain else None,
+        "roofline": {
+            "kernel": "k_gather (exact-order TSC deposit, one launch per level)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": gather_avg_ms,
+            "launches_per_step": int(np.mean(g_n)),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = os.cpu_count() or 1
+        threads = min(threads, 16)
+        rb = cpu_baseline(snap, threads)
+        if rb:
+            cpu_s = rb["amr_seconds"] + rb["unbind_seconds"]
+            out["cpu_baseline"] = {
+                "value": s.npart / cpu_s,
+                "unit": "particles/s",
+                "cores": threads,
+                "kind": "reference",
+                "sample": (f"whole workload once: AMR build {rb['amr_seconds']:.2f}s on 1 thread "
+                           f"(reference default: deposit is serial), unbinding {rb['unbind_seconds']:.3f}s "
+                           f"with OpenMP over haloes on {threads} threads; host {cpu_model()}"),
+            }
+            out["vs_cpu"] = value / out["cpu_baseline"]["value"]
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -109,6 +109,27 @@ int ahfgpu_amr_level_sizes(ahfgpu_ctx *ctx, int lev, int64_t *L, int64_t *n_oct,
 int ahfgpu_amr_level_copy(ahfgpu_ctx *ctx, int lev, uint64_t *oct_key, uint8_t *oct_link, float *dens,
                           int64_t *list0, uint32_t *list);
 
+/* copy nh haloes to the device: centres (box units, nh x 3), member counts
+ * and member particle indices (CSR, indices into the resident particles) --
+ * the HALO.pos / npart / ipart that sort_halo_particles() receives */
+int ahfgpu_set_halos(ahfgpu_ctx *ctx, const double *centres, const uint64_t *npart, const uint64_t *ipart,
+                     uint64_t nh);
+
+/* radial sort + unbinding of every resident halo.  mode 0: all haloes in one
+ * persistent launch (size-ordered work queue); mode 1: one launch per halo,
+ * synchronised per halo (the per-call form).  band: members whose
+ * vel^2 lies within band*VescTune^2*v_esc^2 of the threshold are decided by
+ * an exact sequential replay (1e-4 recommended).  The resident input lists
+ * are not modified, so repeated calls redo the same work. */
+int ahfgpu_unbind(ahfgpu_ctx *ctx, const ahfgpu_unbind_params *p, int mode, double band,
+                  ahfgpu_unbind_info *info);
+
+/* results of the last ahfgpu_unbind: per halo npart and 5 numbers (M_vir,
+ * R_vir, Phi0, iterations, R_vir-was-set), and the bound members of halo h
+ * at ipart[offset_h .. offset_h + npart_h) (offsets of the input CSR) */
+int ahfgpu_unbind_results(ahfgpu_ctx *ctx, uint64_t *npart, double *props, uint64_t *ipart);
+void ahfgpu_release_halos(ahfgpu_ctx *ctx);
+
 /* device milliseconds of the last build / unbind (HIP events on the
  * library's stream), and of the dominant kernel of the last build */
 int ahfgpu_last_timing(ahfgpu_ctx *ctx, double *amr_ms, double *unbind_ms, double *gather_ms,
