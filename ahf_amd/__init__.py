@@ -85,6 +85,8 @@ def load():
     lib.ahfgpu_unbind_results.argtypes = [P, P, P, P]
     lib.ahfgpu_release_halos.argtypes = [P]
     lib.ahfgpu_release_halos.restype = None
+    lib.ahfgpu_timer_count.argtypes = []
+    lib.ahfgpu_timer_get.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), P, P, P]
     _lib = lib
     return lib
 
@@ -162,3 +164,15 @@ class Context:
         off = np.concatenate([[0], np.cumsum(self._hnp.astype(np.int64))])
         lists = [ip[off[h]:off[h] + int(npart[h])].copy() for h in range(nh)]
         return npart, props, lists
+
+    def timers(self):
+        """{stage: (ms, launches, algorithmic bytes)} of the last build + unbinding"""
+        out = {}
+        for i in range(self.lib.ahfgpu_timer_count()):
+            name = ctypes.c_char_p()
+            ms, b = ctypes.c_double(), ctypes.c_double()
+            n = ctypes.c_int64()
+            self._check(self.lib.ahfgpu_timer_get(self.h, i, ctypes.byref(name), ctypes.byref(ms),
+                                                  ctypes.byref(n), ctypes.byref(b)))
+            out[name.value.decode()] = (ms.value, n.value, b.value)
+        return out

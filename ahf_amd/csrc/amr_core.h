@@ -224,6 +224,83 @@ AHF_HD float ahf_gather_dens(const level_view *lv, const uint32_t *pmask, const 
   return dens;
 }
 
+/* coordinates of node (o,c) of a level */
+AHF_HD void ahf_node_xyz(const level_view *lv, int64_t o, int c, int64_t *X, int64_t *Y, int64_t *Z)
+{
+  const uint64_t key = lv->oct_key[o];
+  *X = 2 * ahf_kx(key) + (c & 1);
+  *Y = 2 * ahf_ky(key) + ((c >> 1) & 1);
+  *Z = 2 * ahf_kz(key) + (c >> 2);
+}
+
+/* separations of a particle from the centre of the node it is linked to,
+ * in grid units, one per axis (src/libamr_serial/density.c:342-355) */
+AHF_HD void ahf_particle_sep(const level_view *lv, int64_t node, float x, float y, float z, double sep[3])
+{
+  const double L = (double)lv->L;
+  int64_t X, Y, Z;
+  ahf_node_xyz(lv, node >> 3, (int)(node & 7), &X, &Y, &Z);
+  sep[0] = ahf_pn_sep(x, ahf_centre(X, L), L);
+  sep[1] = ahf_pn_sep(y, ahf_centre(Y, L), L);
+  sep[2] = ahf_pn_sep(z, ahf_centre(Z, L), L);
+}
+
+/* one contribution mass2dens*wz*wy*wx for target-source offset (dx,dy,dz),
+ * multiplied in AHF's order (src/libamr_serial/density.c:393-400) */
+AHF_HD double ahf_contrib(double m, double sx, double sy, double sz, int dx, int dy, int dz)
+{
+  return m * ahf_tsc_w(sz, dz) * ahf_tsc_w(sy, dy) * ahf_tsc_w(sx, dx);
+}
+
+/*
+ * Exact-order density of one target node from precomputed separations
+ * (sx/sy/sz hold, for every bucket entry, the separation of that particle
+ * from its own node).  Same sequence of FP32 roundings as ahf_gather_dens.
+ */
+AHF_HD float ahf_gather_dens_sep(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0,
+                                 const double *sx, const double *sy, const double *sz, int descending,
+                                 double mass2dens, int64_t o, int c)
+{
+  int64_t X, Y, Z;
+  ahf_node_xyz(lv, o, c, &X, &Y, &Z);
+  int oz[3], oy[3], ox[3];
+  ahf_axis_order(Z, lv->L, oz);
+  ahf_axis_order(Y, lv->L, oy);
+  ahf_axis_order(X, lv->L, ox);
+  float dens = -1.0f;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      for (int e = 0; e < 3; e++) {
+        const int ez = oz[a], ey = oy[b], ex = ox[e];
+        const int64_t s = ahf_nbr_node(lv->nbr, o, c, ex, ey, ez);
+        if (s < 0) continue;
+        if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
+        const int64_t p0 = node_p0[s], p1 = node_p0[s + 1];
+        for (int64_t k = 0; k < p1 - p0; k++) {
+          const int64_t q = descending ? p1 - 1 - k : p0 + k;
+          dens = (float)((double)dens + ahf_contrib(mass2dens, sx[q], sy[q], sz[q], -ex, -ey, -ez));
+        }
+      }
+  return dens;
+}
+
+/* number of contributions a target node receives (sum of the bucket sizes of
+ * the sources that see it) */
+AHF_HD int64_t ahf_gather_count(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0, int64_t o,
+                                int c)
+{
+  int64_t k = 0;
+  for (int ez = -1; ez <= 1; ez++)
+    for (int ey = -1; ey <= 1; ey++)
+      for (int ex = -1; ex <= 1; ex++) {
+        const int64_t s = ahf_nbr_node(lv->nbr, o, c, ex, ey, ez);
+        if (s < 0) continue;
+        if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
+        k += node_p0[s + 1] - node_p0[s];
+      }
+  return k;
+}
+
 /* test_node, src/libamr_serial/refine_grid.c:113-138: returns NF_* bits */
 AHF_HD uint8_t ahf_node_flags(const level_view *lv, const uint32_t *pmask, const float *dens,
                               double thr, int64_t o, int c)

@@ -7,14 +7,17 @@
  * resident in HBM.  Per level the kernels are:
  *
  *   k_pmask        presence mask of every node (get_TSCnodes rules)
- *   k_gather       exact-order TSC density of every node (assign_npart)
+ *   k_prep         particle-node separations of every bucket entry
+ *   k_gather_light exact-order TSC density, one lane per node (assign_npart)
+ *   k_gather_heavy the same, one wavefront per node with many contributions
  *   k_flags        interior + refinement criterion (test_node)
  *   k_refine_rows  ref_pquad/ref_cquad/ref_nquad block tests and run state
  *                  machine, one thread per level row
  *   scans +
  *   k_compact      octets of the next level, sorted (z,y,x)
  *   k_nbr_next     27-octet neighbour table of the next level
- *   k_relink_*     particles to the first interior child containing them
+ *   k_relink_child particles to the first interior child containing them,
+ *                  then a stable radix sort builds the fine buckets
  *
  * The per-thread rules live in amr_core.h.  The domain level is binned by a
  * stable LSD radix sort (devprims.h) so every node's bucket is in particle
@@ -118,16 +121,100 @@ __global__ void k_pmask(level_view v, uint32_t *__restrict__ pmask)
   pmask[n] = ahf_presence_mask(&v, n >> 3, (int)(n & 7));
 }
 
-__global__ void __launch_bounds__(AHF_BLOCK) k_gather(level_view v, const uint32_t *__restrict__ pmask,
-                                                      const int64_t *__restrict__ p0,
-                                                      const uint32_t *__restrict__ plist, int descending,
-                                                      const float *__restrict__ px, const float *__restrict__ py,
-                                                      const float *__restrict__ pz, double m2pd,
-                                                      float *__restrict__ dens)
+/* separations of every bucket entry from its node (entries [0, *n)) */
+__global__ void k_prep(level_view v, const int64_t *__restrict__ n_ptr, int64_t n_max,
+                       const uint32_t *__restrict__ pnode, const uint32_t *__restrict__ plist,
+                       const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
+                       double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_max || q >= *n_ptr) return;
+  const uint32_t p = plist[q];
+  double sep[3];
+  ahf_particle_sep(&v, (int64_t)pnode[q], px[p], py[p], pz[p], sep);
+  sx[q] = sep[0];
+  sy[q] = sep[1];
+  sz[q] = sep[2];
+}
+
+#define AHF_GATHER_KMAX 384
+
+/* exact-order density, one lane per target node; targets with more than
+ * AHF_GATHER_KMAX contributions are left to k_gather_heavy */
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const uint32_t *__restrict__ pmask,
+                                                            const int64_t *__restrict__ p0,
+                                                            const double *__restrict__ sx,
+                                                            const double *__restrict__ sy,
+                                                            const double *__restrict__ sz, int descending, double m2pd,
+                                                            float *__restrict__ dens, uint32_t *__restrict__ heavy,
+                                                            uint32_t *__restrict__ n_heavy)
 {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= 8 * v.n_oct) return;
-  dens[n] = ahf_gather_dens(&v, pmask, p0, plist, descending, px, py, pz, m2pd, n >> 3, (int)(n & 7));
+  const int64_t o = n >> 3;
+  const int c = (int)(n & 7);
+  if (ahf_gather_count(&v, pmask, p0, o, c) > AHF_GATHER_KMAX) {
+    heavy[atomicAdd(n_heavy, 1u)] = (uint32_t)n;
+    return;
+  }
+  dens[n] = ahf_gather_dens_sep(&v, pmask, p0, sx, sy, sz, descending, m2pd, o, c);
+}
+
+__device__ inline double readlane_d(double x, int l)
+{
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+
+/* exact-order density of the heavy targets, one wavefront per target: the
+ * lanes compute 64 contributions at a time, the wave then folds them into
+ * the FP32 node value in AHF's order */
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const uint32_t *__restrict__ pmask,
+                                                            const int64_t *__restrict__ p0,
+                                                            const double *__restrict__ sx,
+                                                            const double *__restrict__ sy,
+                                                            const double *__restrict__ sz, int descending, double m2pd,
+                                                            float *__restrict__ dens,
+                                                            const uint32_t *__restrict__ heavy,
+                                                            const uint32_t *__restrict__ n_heavy)
+{
+  const int lane = threadIdx.x & 63;
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t nh = (int64_t)*n_heavy;
+  for (int64_t i = gw; i < nh; i += nw) {
+    const int64_t n = heavy[i];
+    const int64_t o = n >> 3;
+    const int c = (int)(n & 7);
+    int64_t X, Y, Z;
+    ahf_node_xyz(&v, o, c, &X, &Y, &Z);
+    int oz[3], oy[3], ox[3];
+    ahf_axis_order(Z, v.L, oz);
+    ahf_axis_order(Y, v.L, oy);
+    ahf_axis_order(X, v.L, ox);
+    float d = -1.0f;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        for (int e = 0; e < 3; e++) {
+          const int ez = oz[a], ey = oy[b], ex = ox[e];
+          const int64_t s = ahf_nbr_node(v.nbr, o, c, ex, ey, ez);
+          if (s < 0) continue;
+          if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
+          const int64_t q0 = p0[s], cnt = p0[s + 1] - q0;
+          for (int64_t base = 0; base < cnt; base += 64) {
+            const int64_t k = base + lane;
+            double cv = 0.0;
+            if (k < cnt) {
+              const int64_t q = descending ? q0 + cnt - 1 - k : q0 + k;
+              cv = ahf_contrib(m2pd, sx[q], sy[q], sz[q], -ex, -ey, -ez);
+            }
+            const int mm = (int)(cnt - base < 64 ? cnt - base : 64);
+            for (int l = 0; l < mm; l++) d = (float)((double)d + readlane_d(cv, l));
+          }
+        }
+    if (lane == 0) dens[n] = d;
+  }
 }
 
 __global__ void k_flags(level_view v, const uint32_t *__restrict__ pmask, const float *__restrict__ dens,
@@ -243,50 +330,61 @@ __global__ void k_nbr_next(const int32_t *__restrict__ nbr, const int64_t *__res
   nbr2[t] = m < 0 ? -1 : node2oct[m];
 }
 
-/* relink, pass 1: per refined mother node, the child of each particle and
- * the per-child counts (each fine node is written by its mother only) */
-__global__ void k_relink_count(int64_t nn, const uint64_t *__restrict__ key, const int64_t *__restrict__ p0,
+/* relink (src/libamr_serial/relink.c:31-288): per bucket entry, the fine
+ * child its particle moves to (first interior child, in fine traversal
+ * order, whose closed cell holds it); the key for the stable sort that
+ * builds the fine buckets is the fine node, or all-ones if it stays */
+__global__ void k_relink_child(int64_t n_ent, const uint32_t *__restrict__ pnode,
                                const uint32_t *__restrict__ plist, const int32_t *__restrict__ node2oct,
-                               const uint32_t *__restrict__ pmask2, const float *__restrict__ px,
-                               const float *__restrict__ py, const float *__restrict__ pz, double Lf,
-                               int8_t *__restrict__ child, int64_t *__restrict__ cnt2)
+                               const uint64_t *__restrict__ key, const uint32_t *__restrict__ pmask2,
+                               const float *__restrict__ px, const float *__restrict__ py,
+                               const float *__restrict__ pz, double Lf, int8_t *__restrict__ child,
+                               uint32_t *__restrict__ kout, uint32_t *__restrict__ vout)
 {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nn) return;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_ent) return;
+  const uint32_t m = pnode[q], p = plist[q];
+  vout[q] = p;
   const int32_t o2 = node2oct[m];
-  if (o2 < 0) return;
-  uint8_t imask = 0;
-  for (int k = 0; k < 8; k++)
-    if (pmask2[8 * (int64_t)o2 + k] == AHF_FULL27) imask |= (uint8_t)(1u << k);
-  const uint64_t mk = key[m >> 3];
-  const int cm = (int)(m & 7);
-  const int64_t MX = 2 * ahf_kx(mk) + (cm & 1), MY = 2 * ahf_ky(mk) + ((cm >> 1) & 1),
-                MZ = 2 * ahf_kz(mk) + (cm >> 2);
-  int64_t c8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t q = p0[m]; q < p0[m + 1]; q++) {
-    const uint32_t p = plist[q];
-    const int ch = imask ? ahf_relink_child(px[p], py[p], pz[p], MX, MY, MZ, Lf, imask) : -1;
-    child[q] = (int8_t)ch;
-    if (ch >= 0) c8[ch]++;
+  int ch = -1;
+  if (o2 >= 0) {
+    uint8_t imask = 0;
+    for (int k = 0; k < 8; k++)
+      if (pmask2[8 * (int64_t)o2 + k] == AHF_FULL27) imask |= (uint8_t)(1u << k);
+    if (imask) {
+      const uint64_t mk = key[m >> 3];
+      const int cm = (int)(m & 7);
+      const int64_t MX = 2 * ahf_kx(mk) + (cm & 1), MY = 2 * ahf_ky(mk) + ((cm >> 1) & 1),
+                    MZ = 2 * ahf_kz(mk) + (cm >> 2);
+      ch = ahf_relink_child(px[p], py[p], pz[p], MX, MY, MZ, Lf, imask);
+    }
   }
-  for (int k = 0; k < 8; k++) cnt2[8 * (int64_t)o2 + k] = c8[k];
+  child[q] = (int8_t)ch;
+  kout[q] = ch >= 0 ? (uint32_t)(8 * (int64_t)o2 + ch) : 0xFFFFFFFFu;
 }
 
-/* relink, pass 2: stable fill of the fine buckets (ascending particle index) */
-__global__ void k_relink_fill(int64_t nn, const int64_t *__restrict__ p0, const uint32_t *__restrict__ plist,
-                              const int32_t *__restrict__ node2oct, const int8_t *__restrict__ child,
-                              const int64_t *__restrict__ p02, uint32_t *__restrict__ plist2)
+/* neighbour table of a standalone level: binary search of the periodic
+ * neighbour key in the sorted octet keys */
+__global__ void k_nbr_search(const uint64_t *__restrict__ key, int64_t n_oct, int64_t half,
+                             int32_t *__restrict__ nbr)
 {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nn) return;
-  const int32_t o2 = node2oct[m];
-  if (o2 < 0) return;
-  int64_t w[8];
-  for (int k = 0; k < 8; k++) w[k] = p02[8 * (int64_t)o2 + k];
-  for (int64_t q = p0[m]; q < p0[m + 1]; q++) {
-    const int ch = child[q];
-    if (ch >= 0) plist2[w[ch]++] = plist[q];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 27 * n_oct) return;
+  const int64_t o = t / 27;
+  const int e = (int)(t % 27);
+  const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
+  const uint64_t k = key[o];
+  const uint64_t want = ahf_pack3((uint64_t)((ahf_kx(k) + ex + half) % half),
+                                  (uint64_t)((ahf_ky(k) + ey + half) % half),
+                                  (uint64_t)((ahf_kz(k) + ez + half) % half));
+  int64_t lo = 0, hi = n_oct - 1, found = -1;
+  while (lo <= hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const uint64_t km = key[mid];
+    if (km == want) { found = mid; break; }
+    if (km < want) lo = mid + 1; else hi = mid - 1;
   }
+  nbr[t] = (int32_t)found;
 }
 
 /* ============================================================ host side */
@@ -319,18 +417,40 @@ static level_view make_view(const DevLevel &l)
   return v;
 }
 
-static int gather_level(ahfgpu_ctx *c, DevLevel &l, int level_index)
+/* TSC deposit of one level in AHF's exact order: separations per bucket
+ * entry, then one lane per light target and one wavefront per heavy one.
+ * n_max bounds the number of bucket entries (the exact count is on the
+ * device, node_p0[nodes]). */
+static int deposit_level(ahfgpu_ctx *c, DevLevel &l, int level_index, int64_t n_max)
 {
   const int64_t nn = 8 * l.n_oct;
   level_view v = make_view(l);
-  l.pmask = c->arena.get<uint32_t>(nn);
-  l.dens = c->arena.get<float>(nn);
-  if (!l.pmask || !l.dens) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", level_index);
-  LAUNCH(k_pmask, nn, v, l.pmask);
+  Arena &A = c->arena;
+  if (!l.pmask) {
+    l.pmask = A.get<uint32_t>(nn);
+    if (!l.pmask) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (pmask, level %d)", level_index);
+    LAUNCH(k_pmask, nn, v, l.pmask);
+  }
+  l.dens = A.get<float>(nn);
+  l.sx = A.get<double>(n_max);
+  l.sy = A.get<double>(n_max);
+  l.sz = A.get<double>(n_max);
+  uint32_t *heavy = A.get<uint32_t>(nn);
+  if (!l.dens || !l.sx || !l.sy || !l.sz || !heavy)
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit, level %d)", level_index);
+  uint32_t *n_heavy = (uint32_t *)(c->d_anomaly + 1);
+  hipMemsetAsync(n_heavy, 0, sizeof(uint32_t), c->st);
+  const int desc = (level_index % 2) == 0;
   hipEvent_t e0 = c->timer_begin();
-  LAUNCH(k_gather, nn, v, l.pmask, l.node_p0, l.plist, (level_index % 2) == 0, c->px, c->py, c->pz, l.m2pd,
-         l.dens);
-  c->timer_end(e0, 12.0 * (double)l.n_part + 13.0 * (double)nn);
+  LAUNCH(k_prep, n_max, v, l.node_p0 + nn, n_max, l.pnode, l.plist, c->px, c->py, c->pz, l.sx, l.sy, l.sz);
+  c->timer_end(e0, TIMER_PREP, 16.0 * (double)n_max + 24.0 * (double)n_max);
+  hipEvent_t e1 = c->timer_begin();
+  LAUNCH(k_gather_light, nn, v, l.pmask, l.node_p0, l.sx, l.sy, l.sz, desc, l.m2pd, l.dens, heavy, n_heavy);
+  c->timer_end(e1, TIMER_GATHER, 24.0 * (double)n_max + 13.0 * (double)nn);
+  hipEvent_t e2 = c->timer_begin();
+  hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.sx, l.sy,
+                     l.sz, desc, l.m2pd, l.dens, heavy, n_heavy);
+  c->timer_end(e2, TIMER_HEAVY, 0.0);
   return 0;
 }
 
@@ -358,19 +478,21 @@ static int build_domain(ahfgpu_ctx *c, DevLevel &d)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (domain)");
   LAUNCH(k_domain_octets, no, half, d.oct_key, d.oct_link, d.nbr);
   LAUNCH(k_domain_rows, half * half + 1, half, d.row_key, d.row_oct0, d.row_plane, d.plane_z, d.plane_row0);
+  hipEvent_t e0 = c->timer_begin();
   LAUNCH(k_bin_domain, N, c->px, c->py, c->pz, N, L, k0, v0);
   int bits = 1;
   while ((1ll << bits) < 8 * no) bits++;
   const int where = radix_sort<uint32_t, uint32_t>(k0, v0, k1, v1, N, bits, rtmp, c->st);
-  uint32_t *skey = where ? k1 : k0;
+  d.pnode = where ? k1 : k0;
   d.plist = where ? v1 : v0;
-  LAUNCH(k_csr_lower_bound, 8 * no + 1, skey, N, 8 * no, d.node_p0);
+  LAUNCH(k_csr_lower_bound, 8 * no + 1, d.pnode, N, 8 * no, d.node_p0);
+  c->timer_end(e0, TIMER_BIN, 28.0 * (double)N);
   d.n_part = N;
   const double dl = (double)L;
   d.m2d = (dl * dl * dl) / (double)N;
   d.m2pd = (dl * dl * dl) / (double)N;
   d.crit = c->params.Nth_dom * d.m2d;
-  return gather_level(c, d, 0);
+  return deposit_level(c, d, 0, N);
 }
 
 /* refine level `ci` into a new level; returns 1 if a level was added,
@@ -392,6 +514,7 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   int32_t *stmp32 = A.get<int32_t>(scan_tmp_elems(nrank > nplr ? nrank : nplr));
   if (!l.flags || !cnt || !oct_off || !rhas || !row_off || !phas || !plane_off || !stmp64 || !stmp32)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (flags, level %d)", ci);
+  hipEvent_t e0 = c->timer_begin();
   LAUNCH(k_flags, nn, v, l.pmask, l.dens, l.crit - 1.0, l.flags);
   hipMemsetAsync(c->d_anomaly, 0, sizeof(int32_t), c->st);
   LAUNCH(k_refine_rows, nrank, v, l.flags, cnt, c->d_anomaly);
@@ -400,26 +523,31 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   exclusive_scan<int32_t, int32_t>(rhas, row_off, nrank, stmp32, c->st);
   LAUNCH(k_plane_has, nplr, v, cnt, phas);
   exclusive_scan<int32_t, int32_t>(phas, plane_off, nplr, stmp32, c->st);
-  /* sizes of the new level */
+  c->timer_end(e0, TIMER_REFINE, 4.0 * (double)nn);
+  /* sizes of the new level, and the particle count of this level */
   struct {
-    int64_t n_oct;
+    int64_t n_oct, n_part;
     int32_t n_row, n_plane, anomaly, pad;
   } sz;
   hipMemcpyAsync(&sz.n_oct, oct_off + nrank, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
+  hipMemcpyAsync(&sz.n_part, l.node_p0 + nn, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
   hipMemcpyAsync(&sz.n_row, row_off + nrank, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
   hipMemcpyAsync(&sz.n_plane, plane_off + nplr, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
   hipMemcpyAsync(&sz.anomaly, c->d_anomaly, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
-  if (hipStreamSynchronize(c->st) != hipSuccess) return c->fail(AHFGPU_EHIP, "refine sizes: %s", hipGetErrorString(hipGetLastError()));
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "refine sizes: %s", hipGetErrorString(hipGetLastError()));
+  l.n_part = sz.n_part;
   if (sz.anomaly)
     return c->fail(AHFGPU_EUNSUP, "level %d: AHF would create an empty x-run here (undefined behaviour)", ci);
   if (sz.n_oct == 0) return 0;
   if (ci + 1 >= AHFGPU_MAX_LEVELS) return c->fail(AHFGPU_EINVAL, "too many levels");
+  if (8 * sz.n_oct >= (int64_t)UINT32_MAX) return c->fail(AHFGPU_EINVAL, "level %d: too many nodes", ci + 1);
   DevLevel f;
   f.L = 2 * l.L;
   f.n_oct = sz.n_oct;
   f.n_row = sz.n_row;
   f.n_plane = sz.n_plane;
-  const int64_t n2 = f.n_oct, nn2 = 8 * n2;
+  const int64_t n2 = f.n_oct, nn2 = 8 * n2, nent = l.n_part;
   f.oct_key = A.get<uint64_t>(n2);
   f.oct_link = A.get<uint8_t>(n2);
   f.oct_parent = A.get<int64_t>(n2);
@@ -429,18 +557,19 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   f.row_plane = A.get<int32_t>(f.n_row);
   f.plane_z = A.get<int32_t>(f.n_plane);
   f.plane_row0 = A.get<int32_t>(f.n_plane + 1);
-  l.node2oct = A.get<int32_t>(nn);
-  l.child = A.get<int8_t>(l.n_part);
-  int64_t *cnt2 = A.get<int64_t>(nn2 + 1);
-  f.node_p0 = A.get<int64_t>(nn2 + 1);
-  int64_t *stmp2 = A.get<int64_t>(scan_tmp_elems(nn2));
   f.pmask = A.get<uint32_t>(nn2);
+  f.node_p0 = A.get<int64_t>(nn2 + 1);
+  l.node2oct = A.get<int32_t>(nn);
+  l.child = A.get<int8_t>(nent);
+  uint32_t *k0 = A.get<uint32_t>(nent), *v0 = A.get<uint32_t>(nent), *k1 = A.get<uint32_t>(nent),
+           *v1 = A.get<uint32_t>(nent);
+  uint32_t *rtmp = A.get<uint32_t>(radix_tmp_u32<uint32_t>(nent));
   if (!f.oct_key || !f.oct_link || !f.oct_parent || !f.nbr || !f.row_key || !f.row_oct0 || !f.row_plane ||
-      !f.plane_z || !f.plane_row0 || !l.node2oct || !l.child || !cnt2 || !f.node_p0 || !stmp2 || !f.pmask)
+      !f.plane_z || !f.plane_row0 || !f.pmask || !f.node_p0 || !l.node2oct || !l.child || !k0 || !v0 || !k1 ||
+      !v1 || !rtmp)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", ci + 1);
+  hipEvent_t e1 = c->timer_begin();
   hipMemsetAsync(l.node2oct, 0xFF, (size_t)nn * sizeof(int32_t), c->st);
-  hipMemsetAsync(l.child, 0xFF, (size_t)l.n_part, c->st);
-  hipMemsetAsync(cnt2, 0, (size_t)(nn2 + 1) * sizeof(int64_t), c->st);
   LAUNCH(k_compact, nrank, v, l.flags, cnt, oct_off, row_off, plane_off, f.oct_key, f.oct_link, f.oct_parent,
          f.row_key, f.row_oct0, f.row_plane, l.node2oct);
   LAUNCH(k_plane_fill, nplr, v, phas, plane_off, row_off, f.plane_z, f.plane_row0);
@@ -449,26 +578,24 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   LAUNCH(k_nbr_next, 27 * n2, l.nbr, f.oct_parent, l.node2oct, n2, f.nbr);
   level_view fv = make_view(f);
   LAUNCH(k_pmask, nn2, fv, f.pmask);
-  LAUNCH(k_relink_count, nn, nn, l.oct_key, l.node_p0, l.plist, l.node2oct, f.pmask, c->px, c->py, c->pz,
-         (double)f.L, l.child, cnt2);
-  exclusive_scan<int64_t, int64_t>(cnt2, f.node_p0, nn2, stmp2, c->st);
-  hipMemcpyAsync(&f.n_part, f.node_p0 + nn2, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
-  if (hipStreamSynchronize(c->st) != hipSuccess) return c->fail(AHFGPU_EHIP, "relink sizes: %s", hipGetErrorString(hipGetLastError()));
-  f.plist = A.get<uint32_t>(f.n_part > 0 ? f.n_part : 1);
-  if (!f.plist) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (relink, level %d)", ci + 1);
-  LAUNCH(k_relink_fill, nn, nn, l.node_p0, l.plist, l.node2oct, l.child, f.node_p0, f.plist);
+  c->timer_end(e1, TIMER_BUILD, 150.0 * (double)n2);
+  /* relink: child of every entry, then a stable sort by fine node */
+  hipEvent_t e2 = c->timer_begin();
+  LAUNCH(k_relink_child, nent, nent, l.pnode, l.plist, l.node2oct, l.oct_key, f.pmask, c->px, c->py, c->pz,
+         (double)f.L, l.child, k0, v0);
+  int bits = 1;
+  while ((1ll << bits) < nn2 + 1) bits++;
+  const int w = radix_sort<uint32_t, uint32_t>(k0, v0, k1, v1, nent, bits, rtmp, c->st);
+  f.pnode = w ? k1 : k0;
+  f.plist = w ? v1 : v0;
+  LAUNCH(k_csr_lower_bound, nn2 + 1, f.pnode, nent, nn2, f.node_p0);
+  c->timer_end(e2, TIMER_RELINK, 28.0 * (double)nent);
+  f.n_part = -1; /* on the device until the next synchronisation */
   f.m2d = l.m2d * AHF_CRITMULTI_D;
   f.m2pd = l.m2pd * AHF_CRITMULTI_D;
   f.crit = c->params.Nth_ref * f.m2d;
-  /* density of the new level: gather_level allocates pmask again, so reuse */
-  f.dens = A.get<float>(nn2);
-  if (!f.dens) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (dens, level %d)", ci + 1);
-  {
-    hipEvent_t e0 = c->timer_begin();
-    LAUNCH(k_gather, nn2, fv, f.pmask, f.node_p0, f.plist, ((ci + 1) % 2) == 0, c->px, c->py, c->pz, f.m2pd,
-           f.dens);
-    c->timer_end(e0, 12.0 * (double)f.n_part + 13.0 * (double)nn2);
-  }
+  int rc = deposit_level(c, f, ci + 1, nent);
+  if (rc) return rc;
   if (nn2 < AHF_MIN_NNODES) {
     /* relink_back (src/libamr_serial/generate_grids.c:231-238): the fine
      * lists go back to their mothers; keep them for the final lists */
@@ -497,7 +624,9 @@ extern "C" int ahfgpu_amr_build(ahfgpu_ctx *c, const ahfgpu_amr_params *p, ahfgp
   c->lev.clear();
   c->rolled_back = 0;
   c->rb = DevLevel();
-  c->timers_reset();
+  c->timers_reset(TIMER_BIN, TIMER_RELINK);
+  c->timed.clear();
+  c->ev_next = 0;
   hipEventRecord(c->ev_a0, c->st);
   c->lev.push_back(DevLevel());
   int rc = build_domain(c, c->lev[0]);
@@ -512,6 +641,11 @@ extern "C" int ahfgpu_amr_build(ahfgpu_ctx *c, const ahfgpu_amr_params *p, ahfgp
   hipEventRecord(c->ev_a1, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "amr build: %s", hipGetErrorString(hipGetLastError()));
+  for (auto *lv : {&c->lev.back(), &c->rb}) {
+    if (lv->n_part < 0 && lv->node_p0) {
+      hipMemcpy(&lv->n_part, lv->node_p0 + 8 * lv->n_oct, sizeof(int64_t), hipMemcpyDeviceToHost);
+    }
+  }
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->ev_a0, c->ev_a1);
   c->amr_ms = ms;
@@ -657,4 +791,99 @@ extern "C" int ahfgpu_amr_dump(ahfgpu_ctx *c, const char *path)
     h.list = li[(size_t)i].data();
   }
   return hierarchy_dump(path, hl.data(), nl) ? c->fail(AHFGPU_EIO, "cannot write %s", path) : 0;
+}
+
+/* ------------------------------------------------ standalone deposit */
+
+extern "C" int ahfgpu_deposit_level(ahfgpu_ctx *c, int64_t L, const uint64_t *oct_key, const uint8_t *oct_link,
+                                    int64_t n_oct, const int64_t *list0, const uint32_t *list, double m2pd,
+                                    float *dens_out)
+{
+  if (!c || !oct_key || !oct_link || !list0 || n_oct <= 0 || L < 4 || (L & 1)) return AHFGPU_EINVAL;
+  if (c->N <= 0) return c->fail(AHFGPU_EINVAL, "no particles resident");
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  const int64_t nn = 8 * n_oct, ne = list0[nn], half = L / 2;
+  for (int64_t i = 1; i < n_oct; i++)
+    if (oct_key[i] <= oct_key[i - 1]) return c->fail(AHFGPU_EINVAL, "octet keys must be strictly increasing");
+  for (int64_t q = 0; q < ne; q++)
+    if ((int64_t)list[q] >= c->N) return c->fail(AHFGPU_EINVAL, "list entry beyond the resident particles");
+  /* rows and planes of the octets */
+  std::vector<uint64_t> rkey;
+  std::vector<int32_t> roct0, rplane, pz, prow0;
+  for (int64_t i = 0; i < n_oct; i++) {
+    const uint64_t k = oct_key[i];
+    const bool new_row = i == 0 || ahf_ky(k) != ahf_ky(oct_key[i - 1]) || ahf_kz(k) != ahf_kz(oct_key[i - 1]);
+    if (!new_row) continue;
+    if (i == 0 || ahf_kz(k) != ahf_kz(oct_key[i - 1])) {
+      pz.push_back((int32_t)ahf_kz(k));
+      prow0.push_back((int32_t)rkey.size());
+    }
+    rkey.push_back((uint64_t)ahf_ky(k) | ((uint64_t)ahf_kz(k) << AHF_KEY_BITS));
+    roct0.push_back((int32_t)i);
+    rplane.push_back((int32_t)pz.size() - 1);
+  }
+  roct0.push_back((int32_t)n_oct);
+  prow0.push_back((int32_t)rkey.size());
+  std::vector<uint32_t> pnode((size_t)(ne > 0 ? ne : 1));
+  for (int64_t n = 0; n < nn; n++)
+    for (int64_t q = list0[n]; q < list0[n + 1]; q++) pnode[(size_t)q] = (uint32_t)n;
+  c->arena.reset();
+  c->lev.clear();
+  c->built = 0;
+  Arena &A = c->arena;
+  DevLevel l;
+  l.L = L;
+  l.n_oct = n_oct;
+  l.n_row = (int64_t)rkey.size();
+  l.n_plane = (int64_t)pz.size();
+  l.oct_key = A.get<uint64_t>(n_oct);
+  l.oct_link = A.get<uint8_t>(n_oct);
+  l.nbr = A.get<int32_t>(27 * n_oct);
+  l.row_key = A.get<uint64_t>(l.n_row);
+  l.row_oct0 = A.get<int32_t>(l.n_row + 1);
+  l.row_plane = A.get<int32_t>(l.n_row);
+  l.plane_z = A.get<int32_t>(l.n_plane);
+  l.plane_row0 = A.get<int32_t>(l.n_plane + 1);
+  l.node_p0 = A.get<int64_t>(nn + 1);
+  l.plist = A.get<uint32_t>(ne);
+  l.pnode = A.get<uint32_t>(ne);
+  if (!l.oct_key || !l.oct_link || !l.nbr || !l.row_key || !l.row_oct0 || !l.row_plane || !l.plane_z ||
+      !l.plane_row0 || !l.node_p0 || !l.plist || !l.pnode)
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit)");
+  hipMemcpyAsync(l.oct_key, oct_key, (size_t)n_oct * 8, hipMemcpyHostToDevice, c->st);
+  hipMemcpyAsync(l.oct_link, oct_link, (size_t)n_oct, hipMemcpyHostToDevice, c->st);
+  hipMemcpyAsync(l.row_key, rkey.data(), rkey.size() * 8, hipMemcpyHostToDevice, c->st);
+  hipMemcpyAsync(l.row_oct0, roct0.data(), roct0.size() * 4, hipMemcpyHostToDevice, c->st);
+  hipMemcpyAsync(l.row_plane, rplane.data(), rplane.size() * 4, hipMemcpyHostToDevice, c->st);
+  hipMemcpyAsync(l.plane_z, pz.data(), pz.size() * 4, hipMemcpyHostToDevice, c->st);
+  hipMemcpyAsync(l.plane_row0, prow0.data(), prow0.size() * 4, hipMemcpyHostToDevice, c->st);
+  hipMemcpyAsync(l.node_p0, list0, (size_t)(nn + 1) * 8, hipMemcpyHostToDevice, c->st);
+  if (ne > 0) {
+    hipMemcpyAsync(l.plist, list, (size_t)ne * 4, hipMemcpyHostToDevice, c->st);
+    hipMemcpyAsync(l.pnode, pnode.data(), (size_t)ne * 4, hipMemcpyHostToDevice, c->st);
+  }
+  LAUNCH(k_nbr_search, 27 * n_oct, l.oct_key, n_oct, half, l.nbr);
+  l.m2pd = m2pd;
+  /* buckets hold the lists in list order: walk them forwards */
+  const int64_t nn_ = nn;
+  level_view v = make_view(l);
+  l.pmask = A.get<uint32_t>(nn);
+  l.dens = A.get<float>(nn);
+  l.sx = A.get<double>(ne);
+  l.sy = A.get<double>(ne);
+  l.sz = A.get<double>(ne);
+  uint32_t *heavy = A.get<uint32_t>(nn);
+  if (!l.pmask || !l.dens || !l.sx || !l.sy || !l.sz || !heavy)
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit)");
+  uint32_t *n_heavy = (uint32_t *)(c->d_anomaly + 1);
+  hipMemsetAsync(n_heavy, 0, sizeof(uint32_t), c->st);
+  LAUNCH(k_pmask, nn, v, l.pmask);
+  LAUNCH(k_prep, ne, v, l.node_p0 + nn_, ne, l.pnode, l.plist, c->px, c->py, c->pz, l.sx, l.sy, l.sz);
+  LAUNCH(k_gather_light, nn, v, l.pmask, l.node_p0, l.sx, l.sy, l.sz, 0, l.m2pd, l.dens, heavy, n_heavy);
+  hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.sx, l.sy,
+                     l.sz, 0, l.m2pd, l.dens, heavy, n_heavy);
+  if (dens_out) hipMemcpyAsync(dens_out, l.dens, (size_t)nn * 4, hipMemcpyDeviceToHost, c->st);
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "deposit: %s", hipGetErrorString(hipGetLastError()));
+  return 0;
 }

@@ -128,3 +128,19 @@ extern "C" int ahfgpu_last_timing(ahfgpu_ctx *c, double *amr_ms, double *unbind_
   if (gather_launches) *gather_launches = c->gather_launches;
   return 0;
 }
+
+extern "C" int ahfgpu_timer_count(void)
+{
+  return TIMER_COUNT;
+}
+
+extern "C" int ahfgpu_timer_get(ahfgpu_ctx *c, int i, const char **name, double *ms, int64_t *launches,
+                                double *alg_bytes)
+{
+  if (!c || i < 0 || i >= TIMER_COUNT) return AHFGPU_EINVAL;
+  if (name) *name = k_timer_names[i];
+  if (ms) *ms = c->t_ms[i];
+  if (launches) *launches = c->t_n[i];
+  if (alg_bytes) *alg_bytes = c->t_bytes[i];
+  return 0;
+}

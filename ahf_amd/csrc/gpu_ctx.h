@@ -83,13 +83,32 @@ struct DevLevel {
   uint8_t *flags = nullptr;
   int64_t *node_p0 = nullptr;
   uint32_t *plist = nullptr;
+  uint32_t *pnode = nullptr;   /* per bucket entry: its node */
+  double *sx = nullptr, *sy = nullptr, *sz = nullptr; /* per bucket entry: particle-node separation */
   int8_t *child = nullptr;     /* per bucket entry: fine child it moved to, or -1 */
   int32_t *node2oct = nullptr; /* per node: octet of the next level, or -1 */
   double m2d = 0, m2pd = 0, crit = 0;
 };
 
+/* timed stages of one build / unbinding (HIP events on the library stream) */
+enum {
+  TIMER_BIN = 0,     /* domain binning: key + radix sort + CSR           */
+  TIMER_PREP,        /* k_prep                                          */
+  TIMER_GATHER,      /* k_gather_light                                  */
+  TIMER_HEAVY,       /* k_gather_heavy                                  */
+  TIMER_REFINE,      /* k_flags + k_refine_rows + scans                 */
+  TIMER_BUILD,       /* k_compact + k_nbr_next + k_pmask of a new level */
+  TIMER_RELINK,      /* k_relink_child + radix sort + CSR               */
+  TIMER_USORT,       /* unbinding: radial sort                          */
+  TIMER_UNBIND,      /* unbinding: iterations                           */
+  TIMER_COUNT
+};
+static const char *const k_timer_names[TIMER_COUNT] = {"bin", "prep", "gather_light", "gather_heavy", "refine",
+                                                       "build_level", "relink", "unbind_sort", "unbind"};
+
 struct TimedLaunch {
   hipEvent_t a, b;
+  int cat;
   double bytes;
 };
 
@@ -116,6 +135,8 @@ struct ahfgpu_ctx {
   std::vector<TimedLaunch> timed;
   double amr_ms = 0, unbind_ms = 0, gather_ms = 0, gather_bytes = 0;
   int64_t gather_launches = 0;
+  double t_ms[TIMER_COUNT] = {0}, t_bytes[TIMER_COUNT] = {0};
+  int64_t t_n[TIMER_COUNT] = {0};
 
   int fail(int code, const char *fmt, ...)
   {
@@ -125,10 +146,13 @@ struct ahfgpu_ctx {
     va_end(ap);
     return code;
   }
-  void timers_reset()
+  void timers_reset(int first_cat, int last_cat)
   {
-    ev_next = 0;
-    timed.clear();
+    for (int k = first_cat; k <= last_cat; k++) {
+      t_ms[k] = 0;
+      t_bytes[k] = 0;
+      t_n[k] = 0;
+    }
   }
   hipEvent_t take_event()
   {
@@ -145,23 +169,27 @@ struct ahfgpu_ctx {
     hipEventRecord(e, st);
     return e;
   }
-  void timer_end(hipEvent_t a, double bytes)
+  void timer_end(hipEvent_t a, int cat, double bytes)
   {
     hipEvent_t b = take_event();
     hipEventRecord(b, st);
-    timed.push_back(TimedLaunch{a, b, bytes});
+    timed.push_back(TimedLaunch{a, b, cat, bytes});
   }
+  /* after a synchronisation: fold the pending events into the totals */
   void timers_collect()
   {
-    gather_ms = 0;
-    gather_bytes = 0;
-    gather_launches = (int64_t)timed.size();
     for (auto &t : timed) {
       float ms = 0.f;
       hipEventElapsedTime(&ms, t.a, t.b);
-      gather_ms += ms;
-      gather_bytes += t.bytes;
+      t_ms[t.cat] += ms;
+      t_bytes[t.cat] += t.bytes;
+      t_n[t.cat] += 1;
     }
+    timed.clear();
+    ev_next = 0;
+    gather_ms = t_ms[TIMER_GATHER];
+    gather_bytes = t_bytes[TIMER_GATHER];
+    gather_launches = t_n[TIMER_GATHER];
   }
 };
 

@@ -616,7 +616,11 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   if (!k0 || !k1 || !v0 || !v1 || !hk0 || !hk1 || !rtmp || !s->buf_a || !s->buf_b || !dist || !phi || !st ||
       !s->out || !next)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (unbinding)");
+  c->timers_reset(TIMER_USORT, TIMER_UNBIND);
+  c->timed.clear();
+  c->ev_next = 0;
   hipEventRecord(c->ev_u0, c->st);
+  hipEvent_t es = c->timer_begin();
   if (ne > 0) {
     hipLaunchKernelGGL(k_dist_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, ne, s->ip_in, s->cen,
                        c->px, c->py, c->pz, p->minpart, k0, v0);
@@ -631,6 +635,8 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     hipLaunchKernelGGL(k_gather_members, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, vsorted, s->ip_in, ne,
                        s->buf_a);
   }
+  c->timer_end(es, TIMER_USORT, 2.0 * 12.0 * (double)ne + 12.0 * (double)ne);
+  hipEvent_t eu = c->timer_begin();
   if (mode == 0) {
     hipMemsetAsync(next, 0, sizeof(int32_t), c->st);
     int dev_cu = 256;
@@ -648,9 +654,11 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
       hipStreamSynchronize(c->st);
     }
   }
+  c->timer_end(eu, TIMER_UNBIND, 0.0);
   hipEventRecord(c->ev_u1, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "unbinding: %s", hipGetErrorString(hipGetLastError()));
+  c->timers_collect();
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->ev_u0, c->ev_u1);
   c->unbind_ms = ms;

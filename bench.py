@@ -130,15 +130,17 @@ def main():
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
 
-    amr_ms, unb_ms, g_ms, g_n, g_bytes = [], [], [], [], []
+    amr_ms, unb_ms, stages = [], [], {}
     t0 = time.perf_counter()
     for i in range(args.steps):
         info, uinfo = step()
         t = ctx.last_timing()
         amr_ms.append(t["amr_ms"])
         unb_ms.append(t["unbind_ms"])
-        g_ms.append(t["gather_ms"])
-        g_n.append(t["gather_launches"])
+        for k, (ms, n, b) in ctx.timers().items():
+            acc = stages.setdefault(k, [0.0, 0, 0.0])
+            acc[0] += ms
+            acc[1] += n
     t1 = time.perf_counter()
     wall = t1 - t0
     ms_per_step = 1e3 * wall / args.steps
@@ -157,13 +159,18 @@ def main():
         log(f"plain form: {1e3 * (tp1 - tp0) / args.plain_steps:.3f} ms/step")
 
     deposits = int(info.deposits)
-    gather_avg_ms = float(np.sum(g_ms) / np.sum(g_n))
-    # algorithmic bytes of one deposit launch: 12 B (FP32 position) per
-    # deposited particle + 13 B per node (8 B density accumulator, 4 B count,
-    # 1 B flag), averaged over the launches of one build
+    stage_ms = {k: v[0] / args.steps for k, v in stages.items()}
+    stage_n = {k: v[1] / args.steps for k, v in stages.items()}
+    log("stages (ms/step): " + ", ".join(f"{k} {v:.3f} ({stage_n[k]:.0f})" for k, v in stage_ms.items()))
+    # roofline of the dominant stage.  Algorithmic bytes of the deposit per
+    # level: 12 B (FP32 position) per deposited particle + 13 B per node
+    # (8 B density accumulator, 4 B count, 1 B flag), summed over the levels
+    # of one build; the gather launches one light kernel per level.
+    dom = max(stage_ms, key=stage_ms.get)
     levels = [(int(info.n_part[i]), int(info.n_nodes[i])) for i in range(info.n_levels)]
-    alg_bytes = sum(12 * p + 13 * n for p, n in levels) / max(1, len(levels))
-    achieved = alg_bytes / (gather_avg_ms * 1e-3) / 1e9
+    dep_bytes = sum(12 * p + 13 * n for p, n in levels)
+    gather_avg_ms = stage_ms["gather_light"] / max(1.0, stage_n["gather_light"])
+    achieved = (dep_bytes / max(1.0, stage_n["gather_light"])) / (gather_avg_ms * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -196,8 +203,10 @@ def main():
         "exact_replays": int(uinfo.n_exact_replays),
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,
+        "stage_ms": stage_ms,
+        "dominant_stage": dom,
         "roofline": {
-            "kernel": "k_gather (exact-order TSC deposit, one launch per level)",
+            "kernel": "k_gather_light (exact-order TSC deposit, one launch per level)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -205,7 +214,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "avg_launch_ms": gather_avg_ms,
-            "launches_per_step": int(np.mean(g_n)),
+            "launches_per_step": stage_n["gather_light"],
         },
         "cpu_baseline": None,
     }

@@ -1,0 +1,78 @@
+/*
+ * ahf_dropin.h -- AHF's own entry points for the deposit / refinement /
+ * unbinding path, with AHF's prototypes, structs and globals, implemented on
+ * the MI355X library (ahf_gpu.h).  Link this library in place of the
+ * reference objects that define them; every function keeps the reference's
+ * argument meaning, ownership and return convention.
+ *
+ * Globals read, as the reference functions read them (they are defined by
+ * AHF itself, src/common.c:23-34 and src/libahf/ahf_halos.c:163; this library
+ * carries weak definitions only so that it also links on its own):
+ *   simu   NGRID_DOM, NGRID_MAX, Nth_dom, Nth_ref, AHF_MINPART, AHF_VTUNE,
+ *          mean_dens, no_part, no_vpart
+ *   global fst_part, no_part, dom_grid, domgrid_no, fst_cycle, fin_l1dim
+ *   r_fac, v_fac, phi_fac, Hubble
+ *
+ * Errors: like the reference, the grid functions terminate the process with
+ * a message on stderr when they cannot continue (device missing, allocation
+ * failure); the message names the GPU failure.
+ */
+#ifndef AHF_DROPIN_H
+#define AHF_DROPIN_H
+
+#include "ahf_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+extern param_simu simu;
+extern info_global global;
+extern double r_fac, v_fac, phi_fac, Hubble;
+
+/* src/libamr_serial/generate_grids.h:7 -- the domain grid, with AHF's quads */
+gridls *gen_domgrids(int *no_grids);
+
+/* src/libamr_serial/lltools.h -- link every particle to its domain node
+ * (head insertion, src/libamr_serial/lltools.c:32-92) */
+void ll(long unsigned npart, partptr fst_part, gridls *cur_grid);
+
+/* src/libamr_serial/density.h:7 -- dens = -mean_dens on every node */
+void zero_dens(gridls *cur_grid);
+
+/* src/libamr_serial/density.h:18 -- TSC number density of the particles
+ * linked to the grid, in AHF's accumulation order, on the GPU; FALSE if the
+ * grid has fewer than 125 nodes */
+boolean assign_npart(gridls *cur_grid);
+
+/* src/libamr_serial/generate_grids.h:9 -- the whole refinement recursion on
+ * the GPU (gen_AMRhierarchy + gen_refgrid + refine_grid + relink +
+ * relink_back + assign_npart on every level, generate_grids.c:126-432);
+ * returns the refined grids in *grid_list with AHF's quads, densities and
+ * particle lists.  Precondition as in src/main.c:623-648: every particle of
+ * global.fst_part is linked to the domain grid (ll() was called). */
+boolean gen_AMRhierarchy(gridls **grid_list, int *no_grids);
+
+/* src/libahf/ahf_halos.h:30 -- members ordered by distance from halo->pos */
+void sort_halo_particles(HALO *halo);
+
+/* src/libahf/ahf_halos.h:21 -- iterative removal of unbound members;
+ * halo->ipart is replaced by a new malloc'd array, as in the reference.
+ * Safe to call from several threads (calls are serialised on the GPU). */
+void rem_unbound(HALO *halo);
+
+/* No counterpart in the reference: sort_halo_particles + rem_unbound for
+ * all nhalos haloes in one batched GPU pass, replacing the per-halo calls
+ * of the OpenMP loop of src/libahf/ahf_halos.c:504-510.  Returns 0, or a
+ * negative AHFGPU_E* code. */
+int ahf_unbind_halos(HALO *halos, long unsigned nhalos);
+
+/* No counterpart in the reference: choose the device (default 0, first use)
+ * and release the library's device state. */
+int ahf_gpu_init(int device);
+void ahf_gpu_finalize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

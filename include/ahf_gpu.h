@@ -109,6 +109,16 @@ int ahfgpu_amr_level_sizes(ahfgpu_ctx *ctx, int lev, int64_t *L, int64_t *n_oct,
 int ahfgpu_amr_level_copy(ahfgpu_ctx *ctx, int lev, uint64_t *oct_key, uint8_t *oct_link, float *dens,
                           int64_t *list0, uint32_t *list);
 
+/* TSC deposit (assign_npart) on one standalone level given as octets (as
+ * above) and per-node particle lists in list order (CSR over octet-major
+ * nodes, list0 has 8*n_oct+1 entries, indices into the resident
+ * particles); dens_out receives 8*n_oct FP32 densities (octet-major),
+ * accumulated in AHF's order.  Replaces one assign_npart() call
+ * (src/libamr_serial/density.c:238-438) on any grid. */
+int ahfgpu_deposit_level(ahfgpu_ctx *ctx, int64_t L, const uint64_t *oct_key, const uint8_t *oct_link,
+                         int64_t n_oct, const int64_t *list0, const uint32_t *list, double masstopartdens,
+                         float *dens_out);
+
 /* copy nh haloes to the device: centres (box units, nh x 3), member counts
  * and member particle indices (CSR, indices into the resident particles) --
  * the HALO.pos / npart / ipart that sort_halo_particles() receives */
@@ -134,6 +144,13 @@ void ahfgpu_release_halos(ahfgpu_ctx *ctx);
  * library's stream), and of the dominant kernel of the last build */
 int ahfgpu_last_timing(ahfgpu_ctx *ctx, double *amr_ms, double *unbind_ms, double *gather_ms,
                        int64_t *gather_launches);
+
+/* per-stage device timing of the last build and unbinding: number of stages,
+ * and for stage i its name, total milliseconds, launches and algorithmic
+ * bytes (HIP events on the library's stream) */
+int ahfgpu_timer_count(void);
+int ahfgpu_timer_get(ahfgpu_ctx *ctx, int i, const char **name, double *ms, int64_t *launches,
+                     double *alg_bytes);
 
 #ifdef __cplusplus
 }
