@@ -49,11 +49,11 @@ def build_gpu(verbose=False):
     if os.path.exists(dropin):
         obj = os.path.join(LIB, "dropin.o")
         _run(["gcc", "-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
-              "-I", os.path.join(ROOT, "include"), "-c", dropin, "-o", obj])
+              "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c", dropin, "-o", obj])
         objs.append(obj)
     out = os.path.join(LIB, "libahf_amd.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-lm"])
     os.replace(tmp, out)
     for o in objs:
         os.remove(o)

@@ -68,6 +68,11 @@ typedef struct {
   const int32_t *plane_row0;/* [n_plane+1] first row of each plane         */
 } level_view;
 
+/* bucket entry record: FP32 position of the entry's particle (w unused) */
+typedef struct __attribute__((aligned(16))) {
+  float x, y, z, w;
+} ahf_q4;
+
 AHF_HD uint64_t ahf_pack3(uint64_t x, uint64_t y, uint64_t z)
 {
   return x | (y << AHF_KEY_BITS) | (z << (2 * AHF_KEY_BITS));
@@ -253,14 +258,15 @@ AHF_HD double ahf_contrib(double m, double sx, double sy, double sz, int dx, int
 }
 
 /*
- * Exact-order density of one target node from precomputed separations
- * (sx/sy/sz hold, for every bucket entry, the separation of that particle
- * from its own node).  Same sequence of FP32 roundings as ahf_gather_dens.
+ * Exact-order density of one target node from the particle positions stored
+ * per bucket entry (qp[q] = FP32 position of the particle of entry q).  Same
+ * sequence of roundings as ahf_gather_dens: the separation of a particle from
+ * its own node's centre is recomputed at every visit.
  */
-AHF_HD float ahf_gather_dens_sep(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0,
-                                 const double *sx, const double *sy, const double *sz, int descending,
-                                 double mass2dens, int64_t o, int c)
+AHF_HD float ahf_gather_dens_q(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0,
+                               const ahf_q4 *qp, int descending, double mass2dens, int64_t o, int c)
 {
+  const double L = (double)lv->L;
   int64_t X, Y, Z;
   ahf_node_xyz(lv, o, c, &X, &Y, &Z);
   int oz[3], oy[3], ox[3];
@@ -276,12 +282,45 @@ AHF_HD float ahf_gather_dens_sep(const level_view *lv, const uint32_t *pmask, co
         if (s < 0) continue;
         if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
         const int64_t p0 = node_p0[s], p1 = node_p0[s + 1];
+        if (p0 == p1) continue;
+        const double cx = ahf_centre((X + ex + lv->L) % lv->L, L);
+        const double cy = ahf_centre((Y + ey + lv->L) % lv->L, L);
+        const double cz = ahf_centre((Z + ez + lv->L) % lv->L, L);
         for (int64_t k = 0; k < p1 - p0; k++) {
-          const int64_t q = descending ? p1 - 1 - k : p0 + k;
-          dens = (float)((double)dens + ahf_contrib(mass2dens, sx[q], sy[q], sz[q], -ex, -ey, -ez));
+          const ahf_q4 e4 = qp[descending ? p1 - 1 - k : p0 + k];
+          const double w = ahf_contrib(mass2dens, ahf_pn_sep(e4.x, cx, L), ahf_pn_sep(e4.y, cy, L),
+                                       ahf_pn_sep(e4.z, cz, L), -ex, -ey, -ez);
+          dens = (float)((double)dens + w);
         }
       }
   return dens;
+}
+
+/* Morton index of an octet coordinate triple (21 bits per axis) */
+AHF_HD uint64_t ahf_spread3(uint64_t v)
+{
+  v &= 0x1FFFFFull;
+  v = (v | v << 32) & 0x1F00000000FFFFull;
+  v = (v | v << 16) & 0x1F0000FF0000FFull;
+  v = (v | v << 8) & 0x100F00F00F00F00Full;
+  v = (v | v << 4) & 0x10C30C30C30C30C3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+AHF_HD uint64_t ahf_compact3(uint64_t v)
+{
+  v &= 0x1249249249249249ull;
+  v = (v ^ (v >> 2)) & 0x10C30C30C30C30C3ull;
+  v = (v ^ (v >> 4)) & 0x100F00F00F00F00Full;
+  v = (v ^ (v >> 8)) & 0x1F0000FF0000FFull;
+  v = (v ^ (v >> 16)) & 0x1F00000000FFFFull;
+  v = (v ^ (v >> 32)) & 0x1FFFFFull;
+  return v;
+}
+AHF_HD uint64_t ahf_morton(uint64_t key)
+{
+  return ahf_spread3((uint64_t)ahf_kx(key)) | ahf_spread3((uint64_t)ahf_ky(key)) << 1 |
+         ahf_spread3((uint64_t)ahf_kz(key)) << 2;
 }
 
 /* number of contributions a target node receives (sum of the bucket sizes of

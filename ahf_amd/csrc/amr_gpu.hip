@@ -121,43 +121,68 @@ __global__ void k_pmask(level_view v, uint32_t *__restrict__ pmask)
   pmask[n] = ahf_presence_mask(&v, n >> 3, (int)(n & 7));
 }
 
-/* separations of every bucket entry from its node (entries [0, *n)) */
-__global__ void k_prep(level_view v, const int64_t *__restrict__ n_ptr, int64_t n_max,
-                       const uint32_t *__restrict__ pnode, const uint32_t *__restrict__ plist,
+/* position record of every bucket entry (entries [0, *n)) */
+__global__ void k_prep(const int64_t *__restrict__ n_ptr, int64_t n_max, const uint32_t *__restrict__ plist,
                        const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
-                       double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz)
+                       ahf_q4 *__restrict__ qp)
 {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_max || q >= *n_ptr) return;
   const uint32_t p = plist[q];
-  double sep[3];
-  ahf_particle_sep(&v, (int64_t)pnode[q], px[p], py[p], pz[p], sep);
-  sx[q] = sep[0];
-  sy[q] = sep[1];
-  sz[q] = sep[2];
+  ahf_q4 e;
+  e.x = px[p];
+  e.y = py[p];
+  e.z = pz[p];
+  e.w = 0.0f;
+  qp[q] = e;
+}
+
+/* Morton key of every octet, for the target order of the gather */
+__global__ void k_morton_keys(level_view v, uint64_t *__restrict__ mk, uint32_t *__restrict__ idx)
+{
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= v.n_oct) return;
+  mk[o] = ahf_morton(v.oct_key[o]);
+  idx[o] = (uint32_t)o;
 }
 
 #define AHF_GATHER_KMAX 384
+#define AHF_XCDS 8
+
+/*
+ * Target node of a gather thread.  Workgroups are dealt to the 8 XCDs in
+ * turn; the block index is remapped so that every XCD walks one contiguous
+ * stretch of the Morton order of the octets (tord), keeping its sources in
+ * its own L2.  The 8 children of an octet are 8 adjacent lanes.
+ */
+__device__ inline int64_t gather_target(const uint32_t *__restrict__ tord, int64_t n_oct, int64_t per_xcd)
+{
+  const int64_t b = blockIdx.x;
+  const int64_t lb = (b % AHF_XCDS) * per_xcd + b / AHF_XCDS;
+  const int64_t t = lb * blockDim.x + threadIdx.x;
+  if (t >= 8 * n_oct) return -1;
+  return 8 * (int64_t)tord[t >> 3] + (t & 7);
+}
 
 /* exact-order density, one lane per target node; targets with more than
  * AHF_GATHER_KMAX contributions are left to k_gather_heavy */
-__global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const uint32_t *__restrict__ pmask,
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const uint32_t *__restrict__ tord,
+                                                            int64_t per_xcd, const uint32_t *__restrict__ pmask,
                                                             const int64_t *__restrict__ p0,
-                                                            const double *__restrict__ sx,
-                                                            const double *__restrict__ sy,
-                                                            const double *__restrict__ sz, int descending, double m2pd,
-                                                            float *__restrict__ dens, uint32_t *__restrict__ heavy,
+                                                            const ahf_q4 *__restrict__ qp, int descending,
+                                                            double m2pd, float *__restrict__ dens,
+                                                            uint32_t *__restrict__ heavy,
                                                             uint32_t *__restrict__ n_heavy)
 {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= 8 * v.n_oct) return;
+  const int64_t n = gather_target(tord, v.n_oct, per_xcd);
+  if (n < 0) return;
   const int64_t o = n >> 3;
   const int c = (int)(n & 7);
   if (ahf_gather_count(&v, pmask, p0, o, c) > AHF_GATHER_KMAX) {
     heavy[atomicAdd(n_heavy, 1u)] = (uint32_t)n;
     return;
   }
-  dens[n] = ahf_gather_dens_sep(&v, pmask, p0, sx, sy, sz, descending, m2pd, o, c);
+  dens[n] = ahf_gather_dens_q(&v, pmask, p0, qp, descending, m2pd, o, c);
 }
 
 __device__ inline double readlane_d(double x, int l)
@@ -172,10 +197,8 @@ __device__ inline double readlane_d(double x, int l)
  * the FP32 node value in AHF's order */
 __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const uint32_t *__restrict__ pmask,
                                                             const int64_t *__restrict__ p0,
-                                                            const double *__restrict__ sx,
-                                                            const double *__restrict__ sy,
-                                                            const double *__restrict__ sz, int descending, double m2pd,
-                                                            float *__restrict__ dens,
+                                                            const ahf_q4 *__restrict__ qp, int descending,
+                                                            double m2pd, float *__restrict__ dens,
                                                             const uint32_t *__restrict__ heavy,
                                                             const uint32_t *__restrict__ n_heavy)
 {
@@ -193,6 +216,7 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const 
     ahf_axis_order(Z, v.L, oz);
     ahf_axis_order(Y, v.L, oy);
     ahf_axis_order(X, v.L, ox);
+    const double L = (double)v.L;
     float d = -1.0f;
     for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++)
@@ -202,12 +226,17 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const 
           if (s < 0) continue;
           if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
           const int64_t q0 = p0[s], cnt = p0[s + 1] - q0;
+          if (cnt == 0) continue;
+          const double cx = ahf_centre((X + ex + v.L) % v.L, L);
+          const double cy = ahf_centre((Y + ey + v.L) % v.L, L);
+          const double cz = ahf_centre((Z + ez + v.L) % v.L, L);
           for (int64_t base = 0; base < cnt; base += 64) {
             const int64_t k = base + lane;
             double cv = 0.0;
             if (k < cnt) {
-              const int64_t q = descending ? q0 + cnt - 1 - k : q0 + k;
-              cv = ahf_contrib(m2pd, sx[q], sy[q], sz[q], -ex, -ey, -ez);
+              const ahf_q4 e4 = qp[descending ? q0 + cnt - 1 - k : q0 + k];
+              cv = ahf_contrib(m2pd, ahf_pn_sep(e4.x, cx, L), ahf_pn_sep(e4.y, cy, L), ahf_pn_sep(e4.z, cz, L),
+                               -ex, -ey, -ez);
             }
             const int mm = (int)(cnt - base < 64 ? cnt - base : 64);
             for (int l = 0; l < mm; l++) d = (float)((double)d + readlane_d(cv, l));
@@ -417,40 +446,60 @@ static level_view make_view(const DevLevel &l)
   return v;
 }
 
-/* TSC deposit of one level in AHF's exact order: separations per bucket
- * entry, then one lane per light target and one wavefront per heavy one.
- * n_max bounds the number of bucket entries (the exact count is on the
- * device, node_p0[nodes]). */
+/*
+ * Exact-order gather of one level: position record per bucket entry, the
+ * octets sorted into Morton order (target order of the light gather), one
+ * lane per light target, one wavefront per heavy target.  n_max bounds the
+ * number of bucket entries (the exact count is on the device,
+ * node_p0[nodes]).  Needs pmask.
+ */
+static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_max, int descending, bool timed)
+{
+  const int64_t nn = 8 * l.n_oct, no = l.n_oct;
+  level_view v = make_view(l);
+  Arena &A = c->arena;
+  l.dens = A.get<float>(nn);
+  l.qp = A.get<ahf_q4>(n_max > 0 ? n_max : 1);
+  uint32_t *heavy = A.get<uint32_t>(nn);
+  uint64_t *mk0 = A.get<uint64_t>(no), *mk1 = A.get<uint64_t>(no);
+  uint32_t *ix0 = A.get<uint32_t>(no), *ix1 = A.get<uint32_t>(no);
+  uint32_t *rtmp = A.get<uint32_t>(radix_tmp_u32<uint64_t>(no));
+  if (!l.dens || !l.qp || !heavy || !mk0 || !mk1 || !ix0 || !ix1 || !rtmp) return AHFGPU_ENOMEM;
+  uint32_t *n_heavy = (uint32_t *)(c->d_anomaly + 1);
+  hipMemsetAsync(n_heavy, 0, sizeof(uint32_t), c->st);
+  hipEvent_t e0 = timed ? c->timer_begin() : nullptr;
+  LAUNCH(k_prep, n_max, l.node_p0 + nn, n_max, l.plist, c->px, c->py, c->pz, l.qp);
+  int axis_bits = 1;
+  while ((1ll << axis_bits) < l.L / 2) axis_bits++;
+  LAUNCH(k_morton_keys, no, v, mk0, ix0);
+  const int w = radix_sort<uint64_t, uint32_t>(mk0, ix0, mk1, ix1, no, 3 * axis_bits, rtmp, c->st);
+  const uint32_t *tord = w ? ix1 : ix0;
+  if (timed) c->timer_end(e0, TIMER_PREP, 16.0 * (double)n_max + 12.0 * (double)n_max + 24.0 * (double)no);
+  hipEvent_t e1 = timed ? c->timer_begin() : nullptr;
+  const int64_t per_xcd = (nblk(nn) + AHF_XCDS - 1) / AHF_XCDS;
+  hipLaunchKernelGGL(k_gather_light, dim3((unsigned)(AHF_XCDS * per_xcd)), dim3(AHF_BLOCK), 0, c->st, v, tord,
+                     per_xcd, l.pmask, l.node_p0, l.qp, descending, l.m2pd, l.dens, heavy, n_heavy);
+  if (timed) c->timer_end(e1, TIMER_GATHER, 12.0 * (double)n_max + 13.0 * (double)nn);
+  hipEvent_t e2 = timed ? c->timer_begin() : nullptr;
+  hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.qp,
+                     descending, l.m2pd, l.dens, heavy, n_heavy);
+  if (timed) c->timer_end(e2, TIMER_HEAVY, 0.0);
+  return 0;
+}
+
+/* TSC deposit of one level in AHF's exact order (even levels walk their
+ * lists from the tail of each bucket, odd levels from the head) */
 static int deposit_level(ahfgpu_ctx *c, DevLevel &l, int level_index, int64_t n_max)
 {
   const int64_t nn = 8 * l.n_oct;
-  level_view v = make_view(l);
-  Arena &A = c->arena;
   if (!l.pmask) {
-    l.pmask = A.get<uint32_t>(nn);
+    l.pmask = c->arena.get<uint32_t>(nn);
     if (!l.pmask) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (pmask, level %d)", level_index);
+    level_view v = make_view(l);
     LAUNCH(k_pmask, nn, v, l.pmask);
   }
-  l.dens = A.get<float>(nn);
-  l.sx = A.get<double>(n_max);
-  l.sy = A.get<double>(n_max);
-  l.sz = A.get<double>(n_max);
-  uint32_t *heavy = A.get<uint32_t>(nn);
-  if (!l.dens || !l.sx || !l.sy || !l.sz || !heavy)
+  if (gather_level(c, l, n_max, (level_index % 2) == 0, true))
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit, level %d)", level_index);
-  uint32_t *n_heavy = (uint32_t *)(c->d_anomaly + 1);
-  hipMemsetAsync(n_heavy, 0, sizeof(uint32_t), c->st);
-  const int desc = (level_index % 2) == 0;
-  hipEvent_t e0 = c->timer_begin();
-  LAUNCH(k_prep, n_max, v, l.node_p0 + nn, n_max, l.pnode, l.plist, c->px, c->py, c->pz, l.sx, l.sy, l.sz);
-  c->timer_end(e0, TIMER_PREP, 16.0 * (double)n_max + 24.0 * (double)n_max);
-  hipEvent_t e1 = c->timer_begin();
-  LAUNCH(k_gather_light, nn, v, l.pmask, l.node_p0, l.sx, l.sy, l.sz, desc, l.m2pd, l.dens, heavy, n_heavy);
-  c->timer_end(e1, TIMER_GATHER, 24.0 * (double)n_max + 13.0 * (double)nn);
-  hipEvent_t e2 = c->timer_begin();
-  hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.sx, l.sy,
-                     l.sz, desc, l.m2pd, l.dens, heavy, n_heavy);
-  c->timer_end(e2, TIMER_HEAVY, 0.0);
   return 0;
 }
 
@@ -613,8 +662,15 @@ extern "C" int ahfgpu_amr_build(ahfgpu_ctx *c, const ahfgpu_amr_params *p, ahfgp
 {
   if (!c || !p) return AHFGPU_EINVAL;
   if (c->N <= 0) return c->fail(AHFGPU_EINVAL, "no particles resident (call ahfgpu_set_particles first)");
-  if (p->L_dom < 4 || (p->L_dom & (p->L_dom - 1)) || p->L_max < p->L_dom || (p->L_max & (p->L_max - 1)))
-    return c->fail(AHFGPU_EINVAL, "L_dom and L_max must be powers of two with 4 <= L_dom <= L_max");
+  /* octets need an even domain grid; refinement doubles it up to L_max */
+  bool lmax_ok = p->L_dom >= 4 && (p->L_dom & 1) == 0 && p->L_max >= p->L_dom && p->L_max % p->L_dom == 0;
+  if (lmax_ok) {
+    const int64_t r = p->L_max / p->L_dom;
+    lmax_ok = (r & (r - 1)) == 0;
+  }
+  if (!lmax_ok)
+    return c->fail(AHFGPU_EINVAL, "need an even L_dom >= 4 and L_max = L_dom * 2^k (got %lld, %lld)",
+                   (long long)p->L_dom, (long long)p->L_max);
   if (p->L_max > (1 << 22)) return c->fail(AHFGPU_EINVAL, "L_max above 2^22 is not supported");
   if (c->N >= (int64_t)UINT32_MAX) return c->fail(AHFGPU_EINVAL, "at most 2^32-1 particles");
   if ((int64_t)p->L_dom * p->L_dom * p->L_dom >= ((int64_t)1 << 31))
@@ -865,23 +921,11 @@ extern "C" int ahfgpu_deposit_level(ahfgpu_ctx *c, int64_t L, const uint64_t *oc
   LAUNCH(k_nbr_search, 27 * n_oct, l.oct_key, n_oct, half, l.nbr);
   l.m2pd = m2pd;
   /* buckets hold the lists in list order: walk them forwards */
-  const int64_t nn_ = nn;
   level_view v = make_view(l);
   l.pmask = A.get<uint32_t>(nn);
-  l.dens = A.get<float>(nn);
-  l.sx = A.get<double>(ne);
-  l.sy = A.get<double>(ne);
-  l.sz = A.get<double>(ne);
-  uint32_t *heavy = A.get<uint32_t>(nn);
-  if (!l.pmask || !l.dens || !l.sx || !l.sy || !l.sz || !heavy)
-    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit)");
-  uint32_t *n_heavy = (uint32_t *)(c->d_anomaly + 1);
-  hipMemsetAsync(n_heavy, 0, sizeof(uint32_t), c->st);
+  if (!l.pmask) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit)");
   LAUNCH(k_pmask, nn, v, l.pmask);
-  LAUNCH(k_prep, ne, v, l.node_p0 + nn_, ne, l.pnode, l.plist, c->px, c->py, c->pz, l.sx, l.sy, l.sz);
-  LAUNCH(k_gather_light, nn, v, l.pmask, l.node_p0, l.sx, l.sy, l.sz, 0, l.m2pd, l.dens, heavy, n_heavy);
-  hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.sx, l.sy,
-                     l.sz, 0, l.m2pd, l.dens, heavy, n_heavy);
+  if (gather_level(c, l, ne, 0, false)) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit)");
   if (dens_out) hipMemcpyAsync(dens_out, l.dens, (size_t)nn * 4, hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "deposit: %s", hipGetErrorString(hipGetLastError()));

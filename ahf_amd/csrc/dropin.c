@@ -1,0 +1,530 @@
+/*
+ * dropin.c -- AHF's hot-path entry points (include/ahf_dropin.h) with AHF's
+ * prototypes, structs and globals, implemented on the device API of
+ * include/ahf_gpu.h.  Host C: this file only converts between AHF's
+ * pointer structures (linked quads, AoS particles, linked particle lists,
+ * HALO member arrays) and the library's flat device arrays; every deposit,
+ * refinement, relink, sort and unbinding runs on the GPU.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ahf_dropin.h"
+#include "ahf_gpu.h"
+#include "octet_raster.h"
+
+/* AHF defines these (src/common.c:23-34, src/libahf/ahf_halos.c:163); the
+ * weak copies only let the library link on its own */
+__attribute__((weak)) param_simu simu;
+__attribute__((weak)) info_global global;
+__attribute__((weak)) double r_fac, v_fac, phi_fac, Hubble;
+
+static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
+static ahfgpu_ctx *g_ctx;
+static int g_device;
+static const part *g_res_parts; /* particle set resident on the device */
+static uint64_t g_res_n;
+
+static void fatal(const char *what)
+{
+  fprintf(stderr, "ahf_amd: %s: %s\n", what, g_ctx ? ahfgpu_last_error(g_ctx) : "no device context");
+  fflush(stderr);
+  exit(1);
+}
+
+static void *xcalloc(size_t n, size_t sz)
+{
+  void *p = calloc(n ? n : 1, sz);
+  if (!p) fatal("out of host memory");
+  return p;
+}
+
+static ahfgpu_ctx *ctx_get(void)
+{
+  if (!g_ctx && ahfgpu_create(&g_ctx, g_device)) {
+    g_ctx = NULL;
+    fatal("cannot create a GPU context");
+  }
+  return g_ctx;
+}
+
+int ahf_gpu_init(int device)
+{
+  pthread_mutex_lock(&g_lock);
+  int rc = 0;
+  if (g_ctx && g_device != device) {
+    ahfgpu_destroy(g_ctx);
+    g_ctx = NULL;
+  }
+  g_device = device;
+  g_res_parts = NULL;
+  g_res_n = 0;
+  if (!g_ctx) rc = ahfgpu_create(&g_ctx, device);
+  if (rc) g_ctx = NULL;
+  pthread_mutex_unlock(&g_lock);
+  return rc;
+}
+
+void ahf_gpu_finalize(void)
+{
+  pthread_mutex_lock(&g_lock);
+  if (g_ctx) ahfgpu_destroy(g_ctx);
+  g_ctx = NULL;
+  g_res_parts = NULL;
+  g_res_n = 0;
+  pthread_mutex_unlock(&g_lock);
+}
+
+/* copy global.fst_part (AoS) to the device as SoA, unless already there */
+static void ensure_particles(int force)
+{
+  const uint64_t n = global.no_part;
+  if (!force && g_res_parts == global.fst_part && g_res_n == n) return;
+  if (!global.fst_part || n == 0) fatal("no particles (global.fst_part)");
+  float *pos = (float *)malloc(n * 3 * sizeof(float)), *mom = (float *)malloc(n * 3 * sizeof(float));
+  if (!pos || !mom) fatal("out of host memory");
+  for (uint64_t i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      pos[3 * i + d] = global.fst_part[i].pos[d];
+      mom[3 * i + d] = global.fst_part[i].mom[d];
+    }
+  int rc = ahfgpu_set_particles(ctx_get(), pos, mom, n);
+  free(pos);
+  free(mom);
+  if (rc) fatal("particle upload");
+  g_res_parts = global.fst_part;
+  g_res_n = n;
+}
+
+/* ------------------------------------------------------------ grids ---- */
+
+/* alloc_quads() layout of a full periodic grid (src/libutility/alloc_struct.c:476-510) */
+static void full_quads(gridls *g, long l1dim)
+{
+  pqptr pq = (pqptr)xcalloc(1, sizeof(pquad));
+  g->pquad = pq;
+  pq->length = (int)l1dim;
+  pq->loc = (cqptr)xcalloc((size_t)l1dim, sizeof(cquad));
+  for (long z = 0; z < l1dim; z++) {
+    cqptr cq = pq->loc + z;
+    cq->length = (int)l1dim;
+    cq->loc = (nqptr)xcalloc((size_t)l1dim, sizeof(nquad));
+    for (long y = 0; y < l1dim; y++) {
+      cq->loc[y].length = (int)l1dim;
+      cq->loc[y].loc = (nptr)xcalloc((size_t)l1dim, sizeof(node));
+    }
+  }
+  g->no_pquad = 1;
+  g->pquad_array = (pqptr *)xcalloc(1, sizeof(pqptr));
+  g->pquad_array[0] = g->pquad;
+}
+
+gridls *gen_domgrids(int *no_grids)
+{
+  double frac = (double)simu.NGRID_DOM / (double)simu.NGRID_MIN;
+  int e = (int)(float)(log(frac) / log(2.));
+  *no_grids = e + 1;
+  gridls *list = (gridls *)xcalloc((size_t)*no_grids, sizeof(gridls));
+  gridls *g = list;
+  for (long l1 = simu.NGRID_MIN; l1 <= simu.NGRID_DOM; l1 *= 2, g++) {
+    double dl = (double)l1;
+    g->timecounter = global.super_t;
+    g->l1dim = (long unsigned)l1;
+    g->spacing = 1.0 / dl;
+    g->spacing2 = g->spacing * g->spacing;
+    g->masstodens = (dl * dl * dl) / simu.no_vpart;
+    g->masstopartdens = (dl * dl * dl) / (double)simu.no_part;
+    g->critdens = simu.Nth_dom * g->masstodens;
+    full_quads(g, l1);
+  }
+  global.dom_grid = list + (*no_grids - 1);
+  global.domgrid_no = *no_grids - 1;
+  return list;
+}
+
+void ll(long unsigned npart, partptr fst_part, gridls *cur_grid)
+{
+  const long unsigned L = cur_grid->l1dim;
+  pqptr pq = cur_grid->pquad;
+  for (long unsigned j = 0; j < npart; j++) {
+    partptr p = fst_part + j;
+    unsigned long n[3];
+    for (int d = 0; d < 3; d++) {
+      n[d] = (unsigned long)((double)L * p->pos[d]);
+      if (n[d] > L - 1) n[d] = 0;
+    }
+    nptr nd = pq->loc[n[2]].loc[n[1]].loc + n[0];
+    p->ll = nd->ll;
+    nd->ll = p;
+  }
+}
+
+#define FOR_NODES(g, BODY)                                                   \
+  for (pqptr pq_ = (g)->pquad; pq_; pq_ = pq_->next)                          \
+    for (cqptr cq0_ = pq_->loc; cq0_ < pq_->loc + pq_->length; cq0_++)        \
+      for (cqptr cq_ = cq0_; cq_; cq_ = cq_->next)                            \
+        for (nqptr nq0_ = cq_->loc; nq0_ < cq_->loc + cq_->length; nq0_++)    \
+          for (nqptr nq_ = nq0_; nq_; nq_ = nq_->next)                        \
+            for (nptr nd = nq_->loc; nd < nq_->loc + nq_->length; nd++) { BODY }
+
+void zero_dens(gridls *cur_grid)
+{
+  FOR_NODES(cur_grid, nd->dens = (flouble)(-simu.mean_dens); nd->force.tempdens = 0.0f;)
+}
+
+/* a grid read into flat arrays: octets (sorted), links, the node of every
+ * octet-major index, and the per-node lists in list order */
+typedef struct {
+  int64_t n_oct, n_node, n_listed;
+  uint64_t *key;
+  uint8_t *link;
+  nptr *node_of;   /* octet-major index -> node */
+  int64_t *list0;
+  uint32_t *list;
+} flat_grid;
+
+static void flat_free(flat_grid *f)
+{
+  free(f->key); free(f->link); free(f->node_of); free(f->list0); free(f->list);
+  memset(f, 0, sizeof(*f));
+}
+
+static int cmp_u64(const void *a, const void *b)
+{
+  uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+static int64_t find_key(const uint64_t *k, int64_t n, uint64_t want)
+{
+  int64_t lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    int64_t m = (lo + hi) / 2;
+    if (k[m] == want) return m;
+    if (k[m] < want) lo = m + 1; else hi = m - 1;
+  }
+  return -1;
+}
+
+#define PK(x, y, z) ((uint64_t)(x) | ((uint64_t)(y) << 21) | ((uint64_t)(z) << 42))
+
+/* every AHF grid is a union of 2x2x2 node blocks (runs of even start and
+ * length, mirrored rows and planes); returns 0, or -1 if this one is not */
+static int flatten_grid(gridls *g, flat_grid *f)
+{
+  memset(f, 0, sizeof(*f));
+  int64_t n = 0;
+  FOR_NODES(g, (void)nd; n++;)
+  f->n_node = n;
+  if (n == 0 || n % 8) return -1;
+  f->n_oct = n / 8;
+  f->key = (uint64_t *)xcalloc((size_t)f->n_oct, 8);
+  f->link = (uint8_t *)xcalloc((size_t)f->n_oct, 1);
+  f->node_of = (nptr *)xcalloc((size_t)n, sizeof(nptr));
+  f->list0 = (int64_t *)xcalloc((size_t)n + 1, 8);
+  /* pass 1: octet keys from the (even, even, even) nodes, in traversal order
+   * (which is (z,y,x) order, so already sorted) */
+  int64_t no = 0;
+  for (pqptr pq = g->pquad; pq; pq = pq->next) {
+    long z = pq->z;
+    for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++, z++)
+      for (cqptr cq = cq0; cq; cq = cq->next) {
+        long y = cq->y;
+        for (nqptr nq0 = cq->loc; nq0 < cq->loc + cq->length; nq0++, y++)
+          for (nqptr nq = nq0; nq; nq = nq->next) {
+            if ((nq->x & 1) || (nq->length & 1)) return -1;
+            if ((z & 1) || (y & 1)) continue;
+            for (long x = nq->x; x < nq->x + nq->length; x += 2) {
+              if (no >= f->n_oct) return -1;
+              f->key[no++] = PK(x / 2, y / 2, z / 2);
+            }
+          }
+      }
+  }
+  if (no != f->n_oct) return -1;
+  for (int64_t i = 1; i < no; i++)
+    if (f->key[i] <= f->key[i - 1]) qsort(f->key, (size_t)no, 8, cmp_u64), i = no;
+  /* pass 2: node map, links, list lengths */
+  int64_t *cnt = f->list0 + 1;
+  for (pqptr pq = g->pquad; pq; pq = pq->next) {
+    long z = pq->z;
+    for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++, z++)
+      for (cqptr cq = cq0; cq; cq = cq->next) {
+        long y = cq->y;
+        for (nqptr nq0 = cq->loc; nq0 < cq->loc + cq->length; nq0++, y++)
+          for (nqptr nq = nq0; nq; nq = nq->next) {
+            long x = nq->x;
+            for (nptr nd = nq->loc; nd < nq->loc + nq->length; nd++, x++) {
+              int64_t o = find_key(f->key, no, PK(x / 2, y / 2, z / 2));
+              if (o < 0) return -1;
+              int64_t idx = 8 * o + (z & 1) * 4 + (y & 1) * 2 + (x & 1);
+              if (f->node_of[idx]) return -1;
+              f->node_of[idx] = nd;
+              if (!(x & 1) && !(y & 1) && !(z & 1) && nd + 2 < nq->loc + nq->length) f->link[o] = 1;
+              for (partptr p = nd->ll; p; p = p->ll) cnt[idx]++;
+            }
+          }
+      }
+  }
+  for (int64_t i = 0; i < n; i++) f->list0[i + 1] += f->list0[i];
+  f->n_listed = f->list0[n];
+  f->list = (uint32_t *)xcalloc((size_t)f->n_listed, 4);
+  for (int64_t i = 0; i < n; i++) {
+    int64_t w = f->list0[i];
+    for (partptr p = f->node_of[i]->ll; p; p = p->ll) f->list[w++] = (uint32_t)(p - global.fst_part);
+  }
+  return 0;
+}
+
+boolean assign_npart(gridls *cur_grid)
+{
+  pthread_mutex_lock(&g_lock);
+  flat_grid f;
+  if (flatten_grid(cur_grid, &f)) {
+    pthread_mutex_unlock(&g_lock);
+    fatal("assign_npart: grid is not made of 2x2x2 node blocks");
+  }
+  ensure_particles(0);
+  float *dens = (float *)xcalloc((size_t)f.n_node, 4);
+  if (ahfgpu_deposit_level(ctx_get(), (int64_t)cur_grid->l1dim, f.key, f.link, f.n_oct, f.list0, f.list,
+                           cur_grid->masstopartdens, dens)) {
+    pthread_mutex_unlock(&g_lock);
+    fatal("assign_npart");
+  }
+  for (int64_t i = 0; i < f.n_node; i++) f.node_of[i]->dens = dens[i];
+  cur_grid->size.no_part = (long unsigned)f.n_listed;
+  cur_grid->size.no_nodes = (long unsigned)f.n_node;
+  free(dens);
+  flat_free(&f);
+  pthread_mutex_unlock(&g_lock);
+  /* src/libamr_serial/density.c:420-437 */
+  if (cur_grid->size.no_nodes < AHF_MIN_NNODES) return 0;
+  if (simu.np_limit == 0) return 1;
+  if (cur_grid->l1dim == 2 * (long unsigned)simu.NGRID_DOM && simu.Nth_dom < 3.) return 1;
+  return ((double)cur_grid->size.no_nodes / (double)cur_grid->size.no_part) > 7.5 ? 0 : 1;
+}
+
+/* AHF quads of a level held as octets (calloc'd as c_pquad/c_cquad/c_nquad/
+ * c_node would, so free_grid() releases them); node_of receives the node of
+ * every octet-major index */
+static void build_quads(gridls *g, const uint64_t *key, const uint8_t *link, int64_t n_oct, nptr *node_of)
+{
+  octet_raster_t r;
+  if (octet_raster_build(key, link, n_oct, &r)) fatal("out of host memory");
+  int64_t iq = 0, ic = 0, nd = 0;
+  pqptr prev_pq = NULL;
+  g->no_pquad = 0;
+  for (int64_t k = 0; k < r.n_pq; k++) {
+    pqptr pq = (pqptr)xcalloc(1, sizeof(pquad));
+    if (prev_pq) prev_pq->next = pq; else g->pquad = pq;
+    prev_pq = pq;
+    g->no_pquad++;
+    pq->z = r.pq[2 * k];
+    pq->length = r.pq[2 * k + 1];
+    pq->loc = (cqptr)xcalloc((size_t)pq->length, sizeof(cquad));
+    for (int zz = 0; zz < pq->length; zz++) {
+      const int z = pq->z + zz;
+      cqptr prev_cq = NULL;
+      while (ic < r.n_cq && r.cq[3 * ic] == z) {
+        cqptr cq = prev_cq ? (cqptr)xcalloc(1, sizeof(cquad)) : pq->loc + zz;
+        if (prev_cq) prev_cq->next = cq;
+        prev_cq = cq;
+        cq->y = r.cq[3 * ic + 1];
+        cq->length = r.cq[3 * ic + 2];
+        cq->loc = (nqptr)xcalloc((size_t)cq->length, sizeof(nquad));
+        for (int yy = 0; yy < cq->length; yy++) {
+          const int y = cq->y + yy;
+          nqptr prev_nq = NULL;
+          while (iq < r.n_nq && r.nq[4 * iq] == z && r.nq[4 * iq + 1] == y) {
+            nqptr nq = prev_nq ? (nqptr)xcalloc(1, sizeof(nquad)) : cq->loc + yy;
+            if (prev_nq) prev_nq->next = nq;
+            prev_nq = nq;
+            nq->x = r.nq[4 * iq + 2];
+            nq->length = r.nq[4 * iq + 3];
+            nq->loc = (nptr)xcalloc((size_t)nq->length, sizeof(node));
+            for (int xx = 0; xx < nq->length; xx++) node_of[r.order[nd++]] = nq->loc + xx;
+            iq++;
+          }
+        }
+        ic++;
+      }
+    }
+  }
+  free(g->pquad_array);
+  g->pquad_array = (pqptr *)xcalloc((size_t)g->no_pquad, sizeof(pqptr));
+  int64_t i = 0;
+  for (pqptr pq = g->pquad; pq; pq = pq->next) g->pquad_array[i++] = pq;
+  octet_raster_free(&r);
+}
+
+/* densities and final particle lists of device level `lev` into grid g */
+static void fill_level(gridls *g, int lev, int build)
+{
+  ahfgpu_ctx *c = ctx_get();
+  int64_t L, n_oct, n_listed;
+  double crit, m2pd;
+  if (ahfgpu_amr_level_sizes(c, lev, &L, &n_oct, &n_listed, &crit, &m2pd)) fatal("level sizes");
+  const int64_t nn = 8 * n_oct;
+  uint64_t *key = (uint64_t *)xcalloc((size_t)n_oct, 8);
+  uint8_t *link = (uint8_t *)xcalloc((size_t)n_oct, 1);
+  float *dens = (float *)xcalloc((size_t)nn, 4);
+  int64_t *list0 = (int64_t *)xcalloc((size_t)nn + 1, 8);
+  uint32_t *list = (uint32_t *)xcalloc((size_t)n_listed + 1, 4);
+  if (ahfgpu_amr_level_copy(c, lev, key, link, dens, list0, list)) fatal("level copy");
+  nptr *node_of = (nptr *)xcalloc((size_t)nn, sizeof(nptr));
+  if (build) {
+    build_quads(g, key, link, n_oct, node_of);
+  } else {
+    flat_grid f;
+    if (flatten_grid(g, &f) || f.n_oct != n_oct) fatal("domain grid does not match the device level");
+    memcpy(node_of, f.node_of, (size_t)nn * sizeof(nptr));
+    flat_free(&f);
+  }
+  partptr P = global.fst_part;
+  for (int64_t i = 0; i < nn; i++) {
+    nptr nd = node_of[i];
+    nd->dens = dens[i];
+    nd->force.tempdens = 0.0f;
+    nd->ll = list0[i] < list0[i + 1] ? P + list[list0[i]] : NULL;
+    for (int64_t q = list0[i]; q < list0[i + 1]; q++)
+      P[list[q]].ll = q + 1 < list0[i + 1] ? P + list[q + 1] : NULL;
+  }
+  free(key); free(link); free(dens); free(list0); free(list); free(node_of);
+}
+
+boolean gen_AMRhierarchy(gridls **grid_list, int *no_grids)
+{
+  gridls *fin = *grid_list + *no_grids - 1;
+  /* src/libamr_serial/generate_grids.c:307-308 */
+  if ((!global.fst_cycle && (long)fin->l1dim == global.fin_l1dim) || (long)fin->l1dim == simu.NGRID_MAX) {
+    if (global.fst_cycle) {
+      global.fin_l1dim = (long)fin->l1dim;
+      global.fst_cycle = 0;
+    }
+    return 0;
+  }
+  if ((long)fin->l1dim != simu.NGRID_DOM) fatal("gen_AMRhierarchy: expects the domain grid as the finest grid");
+  pthread_mutex_lock(&g_lock);
+  ensure_particles(1);
+  ahfgpu_amr_params p = {simu.NGRID_DOM, simu.NGRID_MAX, simu.Nth_dom, simu.Nth_ref};
+  ahfgpu_amr_info info;
+  if (ahfgpu_amr_build(ctx_get(), &p, &info)) {
+    pthread_mutex_unlock(&g_lock);
+    fatal("gen_AMRhierarchy");
+  }
+  const int dom = *no_grids - 1, nl = info.n_levels;
+  /* AHF reallocates one extra gridls whenever it attempts a refinement that
+   * is not kept (src/libamr_serial/generate_grids.c:146-192) */
+  const int attempted_extra = ((long)(simu.NGRID_DOM << (nl - 1)) != simu.NGRID_MAX) ? 1 : 0;
+  gridls *gl = (gridls *)realloc(*grid_list, (size_t)(*no_grids + nl - 1 + attempted_extra) * sizeof(gridls));
+  if (!gl) fatal("gen_AMRhierarchy: realloc of the grid list");
+  memset(gl + *no_grids, 0, (size_t)(nl - 1 + attempted_extra) * sizeof(gridls));
+  *grid_list = gl;
+  global.dom_grid = gl + global.domgrid_no;
+  fill_level(gl + dom, 0, 0);
+  gl[dom].size.no_nodes = (long unsigned)info.n_nodes[0];
+  gl[dom].size.no_part = (long unsigned)info.n_part[0];
+  for (int k = 1; k < nl + attempted_extra; k++) {
+    gridls *coa = gl + dom + k - 1, *f = gl + dom + k;
+    f->masstodens = coa->masstodens * AHF_CRITMULTI;
+    f->masstopartdens = coa->masstopartdens * AHF_CRITMULTI;
+    f->critdens = simu.Nth_ref * f->masstodens;
+    f->timecounter = coa->timecounter;
+    f->timestep = coa->timestep / 2;
+    f->l1dim = coa->l1dim * 2;
+    f->spacing = 1.0 / (double)f->l1dim;
+    f->spacing2 = f->spacing * f->spacing;
+    f->next = 0;
+    coa->next = 1;
+    if (k < nl) {
+      fill_level(f, k, 1);
+      f->size.no_nodes = (long unsigned)info.n_nodes[k];
+      f->size.no_part = (long unsigned)info.n_part[k];
+    }
+  }
+  *no_grids += nl - 1;
+  global.fin_l1dim = (long)gl[*no_grids - 1].l1dim;
+  global.fst_cycle = 0;
+  pthread_mutex_unlock(&g_lock);
+  return 0;
+}
+
+/* ----------------------------------------------------------- haloes ---- */
+
+static int run_halos(HALO *h, long unsigned nh, int mode)
+{
+  ahfgpu_ctx *c = ctx_get();
+  ensure_particles(0);
+  uint64_t tot = 0;
+  for (long unsigned i = 0; i < nh; i++) tot += h[i].npart;
+  double *cen = (double *)xcalloc(3 * nh, 8);
+  uint64_t *np = (uint64_t *)xcalloc(nh, 8), *ip = (uint64_t *)xcalloc(tot, 8);
+  uint64_t w = 0;
+  for (long unsigned i = 0; i < nh; i++) {
+    cen[3 * i] = h[i].pos.x;
+    cen[3 * i + 1] = h[i].pos.y;
+    cen[3 * i + 2] = h[i].pos.z;
+    np[i] = h[i].npart;
+    for (uint64_t j = 0; j < h[i].npart; j++) ip[w++] = h[i].ipart[j];
+  }
+  ahfgpu_unbind_params up = {simu.AHF_MINPART, 0, simu.AHF_VTUNE, r_fac, v_fac, phi_fac, Hubble};
+  int rc = ahfgpu_set_halos(c, cen, np, ip, nh);
+  if (!rc) rc = ahfgpu_unbind(c, &up, mode, 1e-4, NULL);
+  double *props = (double *)xcalloc(5 * nh, 8);
+  if (!rc) rc = ahfgpu_unbind_results(c, np, props, ip);
+  if (!rc) {
+    w = 0;
+    for (long unsigned i = 0; i < nh; i++) {
+      const uint64_t n_in = h[i].npart;
+      if (n_in >= (uint64_t)simu.AHF_MINPART) {
+        if (mode & AHFGPU_UNBIND_SORT_ONLY) {
+          memcpy(h[i].ipart, ip + w, n_in * 8);
+        } else {
+          free(h[i].ipart);
+          h[i].ipart = (long unsigned *)xcalloc(np[i] ? np[i] : 1, 8);
+          memcpy(h[i].ipart, ip + w, np[i] * 8);
+          h[i].npart = np[i];
+          h[i].M_vir = props[5 * i];
+          if (props[5 * i + 4] > 0) h[i].R_vir = props[5 * i + 1];
+          h[i].Phi0 = props[5 * i + 2];
+        }
+      }
+      w += n_in;
+    }
+  }
+  ahfgpu_release_halos(c);
+  free(cen); free(np); free(ip); free(props);
+  return rc;
+}
+
+void sort_halo_particles(HALO *halo)
+{
+  if (halo->npart < (long unsigned)simu.AHF_MINPART) return;
+  pthread_mutex_lock(&g_lock);
+  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_SORT_ONLY);
+  pthread_mutex_unlock(&g_lock);
+  if (rc) fatal("sort_halo_particles");
+}
+
+void rem_unbound(HALO *halo)
+{
+  if (halo->npart < (long unsigned)simu.AHF_MINPART) return;
+  pthread_mutex_lock(&g_lock);
+  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_NO_SORT);
+  pthread_mutex_unlock(&g_lock);
+  if (rc) fatal("rem_unbound");
+}
+
+int ahf_unbind_halos(HALO *halos, long unsigned nhalos)
+{
+  if (nhalos == 0) return 0;
+  pthread_mutex_lock(&g_lock);
+  int rc = run_halos(halos, nhalos, AHFGPU_UNBIND_BATCHED);
+  pthread_mutex_unlock(&g_lock);
+  return rc;
+}

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/ahf_gpu.h"
+#include "amr_core.h"
 
 #define AHF_CRITMULTI_D 8.0 /* src/param.h:118 */
 
@@ -84,7 +85,7 @@ struct DevLevel {
   int64_t *node_p0 = nullptr;
   uint32_t *plist = nullptr;
   uint32_t *pnode = nullptr;   /* per bucket entry: its node */
-  double *sx = nullptr, *sy = nullptr, *sz = nullptr; /* per bucket entry: particle-node separation */
+  ahf_q4 *qp = nullptr;                             /* per bucket entry: particle position        */
   int8_t *child = nullptr;     /* per bucket entry: fine child it moved to, or -1 */
   int32_t *node2oct = nullptr; /* per node: octet of the next level, or -1 */
   double m2d = 0, m2pd = 0, crit = 0;

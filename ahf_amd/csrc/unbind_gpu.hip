@@ -490,6 +490,30 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_unbind_one(UnbindConst K, int64_t
   unbind_halo(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz, out + h);
 }
 
+/* sort-only calls: every halo keeps all its members */
+__global__ void k_sorted_out(const int64_t *__restrict__ off, int64_t nh, HaloOut *__restrict__ out)
+{
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  HaloOut o;
+  o.npart = off[h + 1] - off[h];
+  o.M_vir = 0.0;
+  o.R_vir = 0.0;
+  o.Phi0 = 0.0;
+  o.niter = 0;
+  o.replays = 0;
+  o.in_b = 0;
+  o.rvir_set = 0;
+  o.particle_iterations = 0;
+  out[h] = o;
+}
+
+__global__ void k_copy_u32(const uint32_t *__restrict__ in, int64_t n, uint32_t *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
 __global__ void k_u64_to_u32(const uint64_t *__restrict__ in, int64_t n, uint32_t *__restrict__ out)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -590,6 +614,8 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   HaloSet *s = (HaloSet *)c->halo_set;
   if (!s) return c->fail(AHFGPU_EINVAL, "no halos resident (call ahfgpu_set_halos first)");
   if (p->minpart < 2 || p->minpart / 2 > 64) return c->fail(AHFGPU_EINVAL, "NminPerHalo must be in [2,129]");
+  if ((mode & 3) > 1 || (mode & ~(3 | AHFGPU_UNBIND_SORT_ONLY | AHFGPU_UNBIND_NO_SORT)))
+    return c->fail(AHFGPU_EINVAL, "bad unbinding mode %d", mode);
   if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
   Arena &A = c->uarena;
   A.reset();
@@ -621,7 +647,12 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   c->ev_next = 0;
   hipEventRecord(c->ev_u0, c->st);
   hipEvent_t es = c->timer_begin();
-  if (ne > 0) {
+  const int run = mode & 3;
+  const bool sort_only = (mode & AHFGPU_UNBIND_SORT_ONLY) != 0;
+  const bool no_sort = (mode & AHFGPU_UNBIND_NO_SORT) != 0;
+  if (ne > 0 && no_sort) {
+    hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->ip_in, ne, s->buf_a);
+  } else if (ne > 0) {
     hipLaunchKernelGGL(k_dist_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, ne, s->ip_in, s->cen,
                        c->px, c->py, c->pz, p->minpart, k0, v0);
     int w = radix_sort<uint64_t, uint32_t>(k0, v0, k1, v1, ne, 64, (uint32_t *)rtmp, c->st);
@@ -637,7 +668,9 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   }
   c->timer_end(es, TIMER_USORT, 2.0 * 12.0 * (double)ne + 12.0 * (double)ne);
   hipEvent_t eu = c->timer_begin();
-  if (mode == 0) {
+  if (sort_only) {
+    if (nh) hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, s->out);
+  } else if (run == 0) {
     hipMemsetAsync(next, 0, sizeof(int32_t), c->st);
     int dev_cu = 256;
     hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);

@@ -125,9 +125,15 @@ int ahfgpu_deposit_level(ahfgpu_ctx *ctx, int64_t L, const uint64_t *oct_key, co
 int ahfgpu_set_halos(ahfgpu_ctx *ctx, const double *centres, const uint64_t *npart, const uint64_t *ipart,
                      uint64_t nh);
 
+#define AHFGPU_UNBIND_BATCHED 0   /* all haloes in one persistent launch   */
+#define AHFGPU_UNBIND_PLAIN 1     /* one launch per halo, synchronised     */
+#define AHFGPU_UNBIND_SORT_ONLY 4 /* radial sort only (sort_halo_particles) */
+#define AHFGPU_UNBIND_NO_SORT 8   /* unbind the lists in the given order   */
+
 /* radial sort + unbinding of every resident halo.  mode 0: all haloes in one
  * persistent launch (size-ordered work queue); mode 1: one launch per halo,
- * synchronised per halo (the per-call form).  band: members whose
+ * synchronised per halo (the per-call form); or-ed with SORT_ONLY or
+ * NO_SORT.  band: members whose
  * vel^2 lies within band*VescTune^2*v_esc^2 of the threshold are decided by
  * an exact sequential replay (1e-4 recommended).  The resident input lists
  * are not modified, so repeated calls redo the same work. */

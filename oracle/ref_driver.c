@@ -6,7 +6,7 @@
  *   - oracle/_ref/ahf_ref: compiled against the reference headers and linked
  *     with the reference's own objects, compiled from the sources where they
  *     lie under the reference tree (no copies, no stand-ins);
- *   - _build/ahf_dropin_drv (tests only): the same harness linked against
+ *   - oracle/_build/ahf_dropin_drv (tests only): the same harness linked against
  *     this project's drop-in library, to show the drop-in is call-compatible.
  *
  * It follows the call order of src/main.c:614-648 (gen_domgrids, ll,
@@ -282,6 +282,38 @@ static void cmd_unbind(const snap_data *s, const char *out, int threads)
   fprintf(stdout, "{\"unbind_seconds\": %.6f, \"threads\": %d}\n", t, threads);
 }
 
+#ifndef AHF_REF_BUILD
+/* the drop-in library's batched replacement of the per-halo loop */
+static void cmd_unbind_batch(const snap_data *s, const char *out)
+{
+  setup_globals(s);
+  partptr fst = make_particles(s);
+  global.fst_part = fst;
+  global.no_part = s->h.npart;
+  HALO *h = make_halos(s);
+  double t0 = now_s();
+  if (ahf_unbind_halos(h, s->h.nhalos)) die("ahf_unbind_halos failed");
+  double t = now_s() - t0;
+  FILE *f = fopen(out, "wb");
+  if (!f) die("cannot open output");
+  fwrite("AHFHALO1", 1, 8, f);
+  uint64_t nh = s->h.nhalos;
+  fwrite(&nh, 8, 1, f);
+  for (uint64_t i = 0; i < nh; i++) {
+    uint64_t n = h[i].npart;
+    double v[4] = {h[i].M_vir, h[i].R_vir, h[i].Phi0, 0.0};
+    fwrite(&n, 8, 1, f);
+    fwrite(v, 8, 4, f);
+    for (uint64_t j = 0; j < n; j++) {
+      uint64_t q = h[i].ipart[j];
+      fwrite(&q, 8, 1, f);
+    }
+  }
+  fclose(f);
+  fprintf(stdout, "{\"unbind_seconds\": %.6f}\n", t);
+}
+#endif
+
 /* -------------------------------------------------------------- time ---- */
 
 static void cmd_time(const snap_data *s, int threads, int reps)
@@ -372,6 +404,11 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "unbind")) {
     if (argc < 4) die("unbind needs an output path");
     cmd_unbind(&s, argv[3], threads);
+#ifndef AHF_REF_BUILD
+  } else if (!strcmp(argv[1], "unbind-batch")) {
+    if (argc < 4) die("unbind-batch needs an output path");
+    cmd_unbind_batch(&s, argv[3]);
+#endif
   } else if (!strcmp(argv[1], "time")) {
     cmd_time(&s, threads, reps < 1 ? 1 : reps);
   } else {

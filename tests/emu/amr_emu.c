@@ -57,28 +57,25 @@ static void set_view(lev_t *l, int64_t L, int64_t n_oct, int64_t n_row, int64_t 
   l->v.plane_z = l->plane_z; l->v.plane_row0 = l->plane_row0;
 }
 
-/* deposit as the device does it: per-entry separations (k_prep), then the
- * exact-order gather of every node from them (k_gather_light/_heavy) */
-static void deposit_sep(lev_t *l, const float *px, const float *py, const float *pz, int descending)
+/* deposit as the device does it: one position record per bucket entry
+ * (k_prep), then the exact-order gather of every node from them
+ * (k_gather_light/_heavy) */
+static void deposit_q(lev_t *l, const float *px, const float *py, const float *pz, int descending)
 {
   int64_t nn = 8 * l->v.n_oct, ne = l->node_p0[nn];
   if (!l->pmask) {
     l->pmask = (uint32_t *)xm((size_t)nn * 4);
     for (int64_t n = 0; n < nn; n++) l->pmask[n] = ahf_presence_mask(&l->v, n / 8, (int)(n % 8));
   }
-  double *sx = (double *)xm((size_t)ne * 8), *sy = (double *)xm((size_t)ne * 8), *sz = (double *)xm((size_t)ne * 8);
-  for (int64_t n = 0; n < nn; n++)
-    for (int64_t q = l->node_p0[n]; q < l->node_p0[n + 1]; q++) {
-      uint32_t p = l->plist[q];
-      double sep[3];
-      ahf_particle_sep(&l->v, n, px[p], py[p], pz[p], sep);
-      sx[q] = sep[0]; sy[q] = sep[1]; sz[q] = sep[2];
-    }
+  ahf_q4 *qp = (ahf_q4 *)xm((size_t)(ne > 0 ? ne : 1) * sizeof(ahf_q4));
+  for (int64_t q = 0; q < ne; q++) {
+    uint32_t p = l->plist[q];
+    qp[q].x = px[p]; qp[q].y = py[p]; qp[q].z = pz[p]; qp[q].w = 0.0f;
+  }
   l->dens = (float *)xm((size_t)nn * 4);
   for (int64_t n = 0; n < nn; n++)
-    l->dens[n] = ahf_gather_dens_sep(&l->v, l->pmask, l->node_p0, sx, sy, sz, descending, l->m2pd, n / 8,
-                                     (int)(n % 8));
-  free(sx); free(sy); free(sz);
+    l->dens[n] = ahf_gather_dens_q(&l->v, l->pmask, l->node_p0, qp, descending, l->m2pd, n / 8, (int)(n % 8));
+  free(qp);
 }
 
 int main(int argc, char **argv)
@@ -153,7 +150,7 @@ int main(int argc, char **argv)
     free(fill);
     free(key);
   }
-  deposit_sep(d, px, py, pz, 1);
+  deposit_q(d, px, py, pz, 1);
 
   int anomaly = 0;
   for (;;) {
@@ -274,7 +271,7 @@ int main(int argc, char **argv)
     free(fill);
     free(child);
     /* deposit on the fine level: list order flips each level */
-    deposit_sep(f, px, py, pz, (nlev % 2) == 0);
+    deposit_q(f, px, py, pz, (nlev % 2) == 0);
     if (nnf < AHF_MIN_NNODES) {
       /* relink_back: keep the fine lists to append them to the mothers */
       c->rolled_back_child = 1;

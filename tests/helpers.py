@@ -4,12 +4,14 @@ import numpy as np
 from ahf_amd import dumps
 
 
-def fuzz_snapshot(path, seed):
+def fuzz_snapshot(path, seed, non_pow2=False):
     """Random small snapshot: background + Gaussian clusters, some placed on
     the periodic faces, some particles exactly on cell faces; random
-    thresholds and depth.  Returns a dict of the parameters."""
+    thresholds and depth.  non_pow2 draws domain grids that are even but not
+    powers of two (AHF takes any LgridDomain).  Returns a dict of the
+    parameters."""
     rng = np.random.default_rng(seed)
-    L = int(rng.choice([4, 8, 8, 16, 16, 32]))
+    L = int(rng.choice([6, 10, 12, 20, 24] if non_pow2 else [4, 8, 8, 16, 16, 32]))
     n_bg = int(rng.integers(0, L ** 3 + 1))
     parts = [rng.random((n_bg, 3))]
     for _ in range(int(rng.integers(1, 6))):

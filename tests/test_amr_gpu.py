@@ -42,7 +42,7 @@ def _check(oracle, ctx, snap, tmp_path, tag):
     assert info.n_levels == len(b)
 
 
-@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 11), (1, 64, 1001), (0, 64, 1000)])
+@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 11), (1, 64, 1001), (0, 64, 1000), (1, 48, 7), (3, 48, 5)])
 def test_generated_configs(oracle, snapgen, ctx, tmp_path, cfg, side, seed):
     snap = str(tmp_path / f"cfg{cfg}_{side}.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
@@ -54,3 +54,10 @@ def test_fuzz(oracle, ctx, tmp_path, seed):
     snap = str(tmp_path / f"fuzz{seed}.snap")
     fuzz_snapshot(snap, seed)
     _check(oracle, ctx, snap, tmp_path, f"fuzz{seed}")
+
+
+@pytest.mark.parametrize("seed", list(range(40, 52)))
+def test_fuzz_non_pow2(oracle, ctx, tmp_path, seed):
+    snap = str(tmp_path / f"fuzz{seed}.snap")
+    fuzz_snapshot(snap, seed, non_pow2=True)
+    _check(oracle, ctx, snap, tmp_path, f"fuzz_np2_{seed}")

@@ -36,7 +36,14 @@ def test_emu_fuzz(oracle, emu, tmp_path, seed):
     _cmp(oracle, emu, snap, tmp_path)
 
 
-@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 21), (1, 64, 1001), (0, 32, 1000)])
+@pytest.mark.parametrize("seed", range(260, 272))
+def test_emu_fuzz_non_pow2(oracle, emu, tmp_path, seed):
+    snap = str(tmp_path / "f.snap")
+    fuzz_snapshot(snap, seed, non_pow2=True)
+    _cmp(oracle, emu, snap, tmp_path)
+
+
+@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 21), (1, 64, 1001), (0, 32, 1000), (1, 48, 7)])
 def test_emu_configs(oracle, emu, snapgen, tmp_path, cfg, side, seed):
     snap = str(tmp_path / "g.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0

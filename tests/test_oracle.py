@@ -94,6 +94,16 @@ def test_amr_matches_reference_fuzz(oracle, ref_bin, tmp_path, seed):
     assert not diffs, diffs[:5]
 
 
+@pytest.mark.parametrize("seed", range(140, 150))
+def test_amr_matches_reference_non_pow2(oracle, ref_bin, tmp_path, seed):
+    snap, o, r = str(tmp_path / "f.snap"), str(tmp_path / "o.grid"), str(tmp_path / "r.grid")
+    fuzz_snapshot(snap, seed, non_pow2=True)
+    assert oracle.oracle_amr_dump(snap.encode(), o.encode()) > 0
+    _ref_amr(ref_bin, snap, r)
+    diffs = dumps.compare_grids(dumps.read_grids(o), dumps.read_grids(r))
+    assert not diffs, diffs[:5]
+
+
 def _halo_eq(a, b):
     assert np.array_equal(a.npart, b.npart)
     for x, y in zip(a.ipart, b.ipart):
