@@ -159,6 +159,9 @@ AHF_HD void ahf_axis_order(int64_t X, int64_t L, int ord[3])
   else { ord[0] = -1; ord[1] = 0; ord[2] = 1; }
 }
 
+/* x in [-L, 2L) folded into [0, L) */
+AHF_HD int64_t ahf_wrap(int64_t x, int64_t L) { return x < 0 ? x + L : (x >= L ? x - L : x); }
+
 /* weight index k in AHF's weights[k][dim] for target-source = d */
 AHF_HD double ahf_tsc_w(double s, int d)
 {
@@ -183,6 +186,14 @@ AHF_HD double ahf_pn_sep(float p, double cen, double L)
 AHF_HD double ahf_centre(int64_t X, double L)
 {
   return ahf_f1mod((((double)X) / L) + 0.5 / L + 1.0);
+}
+
+/* centres of the nodes at X-1, X, X+1 (periodic) along one axis */
+AHF_HD void ahf_centres3(int64_t X, int64_t L, double c[3])
+{
+  c[0] = ahf_centre(ahf_wrap(X - 1, L), (double)L);
+  c[1] = ahf_centre(X, (double)L);
+  c[2] = ahf_centre(ahf_wrap(X + 1, L), (double)L);
 }
 
 /*
@@ -215,9 +226,9 @@ AHF_HD float ahf_gather_dens(const level_view *lv, const uint32_t *pmask, const 
         if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
         const int64_t p0 = node_p0[s], p1 = node_p0[s + 1];
         if (p0 == p1) continue;
-        const double cenx = ahf_centre((X + ex + lv->L) % lv->L, L);
-        const double ceny = ahf_centre((Y + ey + lv->L) % lv->L, L);
-        const double cenz = ahf_centre((Z + ez + lv->L) % lv->L, L);
+        const double cenx = ahf_centre(ahf_wrap(X + ex, lv->L), L);
+        const double ceny = ahf_centre(ahf_wrap(Y + ey, lv->L), L);
+        const double cenz = ahf_centre(ahf_wrap(Z + ez, lv->L), L);
         for (int64_t q = 0; q < p1 - p0; q++) {
           const uint32_t p = plist[descending ? p1 - 1 - q : p0 + q];
           const double wx = ahf_tsc_w(ahf_pn_sep(px[p], cenx, L), -ex);
@@ -283,9 +294,9 @@ AHF_HD float ahf_gather_dens_q(const level_view *lv, const uint32_t *pmask, cons
         if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
         const int64_t p0 = node_p0[s], p1 = node_p0[s + 1];
         if (p0 == p1) continue;
-        const double cx = ahf_centre((X + ex + lv->L) % lv->L, L);
-        const double cy = ahf_centre((Y + ey + lv->L) % lv->L, L);
-        const double cz = ahf_centre((Z + ez + lv->L) % lv->L, L);
+        const double cx = ahf_centre(ahf_wrap(X + ex, lv->L), L);
+        const double cy = ahf_centre(ahf_wrap(Y + ey, lv->L), L);
+        const double cz = ahf_centre(ahf_wrap(Z + ez, lv->L), L);
         for (int64_t k = 0; k < p1 - p0; k++) {
           const ahf_q4 e4 = qp[descending ? p1 - 1 - k : p0 + k];
           const double w = ahf_contrib(mass2dens, ahf_pn_sep(e4.x, cx, L), ahf_pn_sep(e4.y, cy, L),

@@ -53,7 +53,7 @@ __global__ void k_domain_octets(int64_t half, uint64_t *__restrict__ key, uint8_
   for (int ez = -1; ez <= 1; ez++)
     for (int ey = -1; ey <= 1; ey++)
       for (int ex = -1; ex <= 1; ex++) {
-        const int64_t nx = (ox + ex + half) % half, ny = (oy + ey + half) % half, nz = (oz + ez + half) % half;
+        const int64_t nx = ahf_wrap(ox + ex, half), ny = ahf_wrap(oy + ey, half), nz = ahf_wrap(oz + ez, half);
         nbr[27 * o + ahf_e27(ex, ey, ez)] = (int32_t)((nz * half + ny) * half + nx);
       }
 }
@@ -146,43 +146,176 @@ __global__ void k_morton_keys(level_view v, uint64_t *__restrict__ mk, uint32_t 
   idx[o] = (uint32_t)o;
 }
 
-#define AHF_GATHER_KMAX 384
 #define AHF_XCDS 8
+#define AHF_GL_NB 64 /* source nodes around one octet: 4x4x4 */
 
-/*
- * Target node of a gather thread.  Workgroups are dealt to the 8 XCDs in
- * turn; the block index is remapped so that every XCD walks one contiguous
- * stretch of the Morton order of the octets (tord), keeping its sources in
- * its own L2.  The 8 children of an octet are 8 adjacent lanes.
- */
-__device__ inline int64_t gather_target(const uint32_t *__restrict__ tord, int64_t n_oct, int64_t per_xcd)
+/* contribution of one entry record to a target at offset (-ex,-ey,-ez) from
+ * its source node, whose centre is (cx,cy,cz) */
+__device__ inline double entry_contrib(const ahf_q4 &e, double cx, double cy, double cz, double L, double m2pd,
+                                       int ex, int ey, int ez)
 {
-  const int64_t b = blockIdx.x;
-  const int64_t lb = (b % AHF_XCDS) * per_xcd + b / AHF_XCDS;
-  const int64_t t = lb * blockDim.x + threadIdx.x;
-  if (t >= 8 * n_oct) return -1;
-  return 8 * (int64_t)tord[t >> 3] + (t & 7);
+  return ahf_contrib(m2pd, ahf_pn_sep(e.x, cx, L), ahf_pn_sep(e.y, cy, L), ahf_pn_sep(e.z, cz, L), -ex, -ey, -ez);
 }
 
-/* exact-order density, one lane per target node; targets with more than
- * AHF_GATHER_KMAX contributions are left to k_gather_heavy */
+/* ahf_tsc_w with the offset known at compile time */
+template <int D>
+__device__ __forceinline__ double tsc_w_c(double s)
+{
+  if constexpr (D == 0) return 0.75 - s * s;
+  else if constexpr (D < 0) return ((0.5 - s) * (0.5 - s)) / 2;
+  else return ((0.5 + s) * (0.5 + s)) / 2;
+}
+
+template <int EX, int EY, int EZ>
+__device__ __forceinline__ double entry_contrib_c(const ahf_q4 &e, double cx, double cy, double cz, double L,
+                                                  double m2pd)
+{
+  return m2pd * tsc_w_c<-EZ>(ahf_pn_sep(e.z, cz, L)) * tsc_w_c<-EY>(ahf_pn_sep(e.y, cy, L)) *
+         tsc_w_c<-EX>(ahf_pn_sep(e.x, cx, L));
+}
+
+/*
+ * Source (EX,EY,EZ) of a target whose 27 neighbours need no periodic wrap:
+ * its entries in walk order (step +1, or -1 as a 32-bit wrap), loaded four
+ * at a time ahead of the sequential FP32 rounding.  rec points at the
+ * target's own node in the octet's 4x4x4 neighbourhood records.
+ */
+template <int EX, int EY, int EZ>
+__device__ __forceinline__ void gather_source_c(float &d, const uint4 *rec, const ahf_q4 *__restrict__ qp,
+                                                uint32_t step, double L, double m2pd, const double *cxv,
+                                                const double *cyv, const double *czv)
+{
+  constexpr int OFF = EZ * 16 + EY * 4 + EX;
+  constexpr int BIT = (1 - EZ) * 9 + (1 - EY) * 3 + (1 - EX);
+  const uint4 r = rec[OFF];
+  if (!((r.z >> BIT) & 1u) || r.y == 0) return;
+  const double cx = cxv[EX + 1], cy = cyv[EY + 1], cz = czv[EZ + 1];
+  const uint32_t cnt = r.y;
+  uint32_t q = step == 1u ? r.x : r.x + cnt - 1u;
+  uint32_t k = 0;
+  for (; k + 4 <= cnt; k += 4) {
+    const ahf_q4 e0 = qp[q], e1 = qp[q + step], e2 = qp[q + 2u * step], e3 = qp[q + 3u * step];
+    const double w0 = entry_contrib_c<EX, EY, EZ>(e0, cx, cy, cz, L, m2pd);
+    const double w1 = entry_contrib_c<EX, EY, EZ>(e1, cx, cy, cz, L, m2pd);
+    const double w2 = entry_contrib_c<EX, EY, EZ>(e2, cx, cy, cz, L, m2pd);
+    const double w3 = entry_contrib_c<EX, EY, EZ>(e3, cx, cy, cz, L, m2pd);
+    d = (float)((double)d + w0);
+    d = (float)((double)d + w1);
+    d = (float)((double)d + w2);
+    d = (float)((double)d + w3);
+    q += 4u * step;
+  }
+  for (; k < cnt; k++, q += step) d = (float)((double)d + entry_contrib_c<EX, EY, EZ>(qp[q], cx, cy, cz, L, m2pd));
+}
+
+#define AHF_GS_ROW(EY, EZ)                                                           \
+  gather_source_c<-1, EY, EZ>(d, rec, qp, step, L, m2pd, cxv, cyv, czv);            \
+  gather_source_c<0, EY, EZ>(d, rec, qp, step, L, m2pd, cxv, cyv, czv);             \
+  gather_source_c<1, EY, EZ>(d, rec, qp, step, L, m2pd, cxv, cyv, czv)
+#define AHF_GS_PLANE(EZ) \
+  AHF_GS_ROW(-1, EZ);    \
+  AHF_GS_ROW(0, EZ);     \
+  AHF_GS_ROW(1, EZ)
+
+/*
+ * Exact-order density, one lane per target node.
+ *
+ * Block b is remapped so that each XCD (blocks are dealt to the 8 XCDs in
+ * turn) walks one contiguous stretch of the Morton order of the octets
+ * (tord); the 8 children of an octet are 8 adjacent lanes.  The 8 lanes of
+ * an octet first load the records of the 64 source nodes of the octet's
+ * 4x4x4 neighbourhood (bucket start, entry count, presence mask) into LDS;
+ * every child then counts and gathers its 27 sources from there.  Targets
+ * with more than kmax contributions are left to k_gather_heavy.  Targets
+ * away from the periodic faces walk their sources in the fixed order
+ * z,y,x = -1,0,1 with compile-time offsets; the others follow AHF's rotated
+ * order at the faces (ahf_axis_order).
+ */
 __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const uint32_t *__restrict__ tord,
                                                             int64_t per_xcd, const uint32_t *__restrict__ pmask,
                                                             const int64_t *__restrict__ p0,
                                                             const ahf_q4 *__restrict__ qp, int descending,
-                                                            double m2pd, float *__restrict__ dens,
+                                                            double m2pd, uint32_t kmax, float *__restrict__ dens,
                                                             uint32_t *__restrict__ heavy,
                                                             uint32_t *__restrict__ n_heavy)
 {
-  const int64_t n = gather_target(tord, v.n_oct, per_xcd);
-  if (n < 0) return;
-  const int64_t o = n >> 3;
-  const int c = (int)(n & 7);
-  if (ahf_gather_count(&v, pmask, p0, o, c) > AHF_GATHER_KMAX) {
+  __shared__ uint4 s_rec[AHF_BLOCK / 8 * AHF_GL_NB];
+  const int64_t b = blockIdx.x;
+  const int64_t t = ((b % AHF_XCDS) * per_xcd + b / AHF_XCDS) * blockDim.x + threadIdx.x;
+  const bool active = t < 8 * v.n_oct;
+  const int c = threadIdx.x & 7;
+  const int nb0 = (threadIdx.x >> 3) * AHF_GL_NB;
+  const int64_t o = active ? (int64_t)tord[t >> 3] : 0;
+  if (active) {
+    const int32_t *nrow = v.nbr + 27 * o;
+    for (int m = 0; m < 8; m++) {
+      const int u = c * 8 + m;
+      const int i = u & 3, j = (u >> 2) & 3, k = u >> 4;
+      const int on = nrow[ahf_e27(i == 0 ? -1 : (i == 3), j == 0 ? -1 : (j == 3), k == 0 ? -1 : (k == 3))];
+      uint4 r = make_uint4(0u, 0u, 0u, 0u);
+      if (on >= 0) {
+        const int64_t sn = 8 * (int64_t)on + (((k + 1) & 1) << 2 | ((j + 1) & 1) << 1 | ((i + 1) & 1));
+        const int64_t a = p0[sn];
+        r.x = (uint32_t)a;
+        r.y = (uint32_t)(p0[sn + 1] - a);
+        r.z = pmask[sn];
+      }
+      s_rec[nb0 + u] = r;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+  /* the target's own node in the neighbourhood: offsets (dx,dy,dz) add
+   * dz*16 + dy*4 + dx */
+  const uint4 *rec = s_rec + nb0 + (cz + 1) * 16 + (cy + 1) * 4 + (cx + 1);
+  const int64_t n = 8 * o + c;
+  uint64_t total = 0;
+  for (int ez = -1; ez <= 1; ez++)
+    for (int ey = -1; ey <= 1; ey++)
+      for (int ex = -1; ex <= 1; ex++) {
+        const uint4 r = rec[ez * 16 + ey * 4 + ex];
+        if ((r.z >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u) total += r.y;
+      }
+  if (total > kmax) {
     heavy[atomicAdd(n_heavy, 1u)] = (uint32_t)n;
     return;
   }
-  dens[n] = ahf_gather_dens_q(&v, pmask, p0, qp, descending, m2pd, o, c);
+  const double L = (double)v.L;
+  int64_t X, Y, Z;
+  ahf_node_xyz(&v, o, c, &X, &Y, &Z);
+  double cxv[3], cyv[3], czv[3];
+  ahf_centres3(X, v.L, cxv);
+  ahf_centres3(Y, v.L, cyv);
+  ahf_centres3(Z, v.L, czv);
+  float d = -1.0f;
+  const bool inner = X > 0 && X < v.L - 1 && Y > 0 && Y < v.L - 1 && Z > 0 && Z < v.L - 1;
+  if (inner) {
+    const uint32_t step = descending ? 0xFFFFFFFFu : 1u;
+    AHF_GS_PLANE(-1);
+    AHF_GS_PLANE(0);
+    AHF_GS_PLANE(1);
+  } else {
+    int oz[3], oy[3], ox[3];
+    ahf_axis_order(Z, v.L, oz);
+    ahf_axis_order(Y, v.L, oy);
+    ahf_axis_order(X, v.L, ox);
+    for (int a3 = 0; a3 < 3; a3++)
+      for (int b3 = 0; b3 < 3; b3++)
+        for (int e3 = 0; e3 < 3; e3++) {
+          const int ez = oz[a3], ey = oy[b3], ex = ox[e3];
+          const uint4 r = rec[ez * 16 + ey * 4 + ex];
+          if (!((r.z >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
+          const int64_t cnt = r.y;
+          if (cnt == 0) continue;
+          const int64_t q0 = r.x;
+          const double scx = cxv[ex + 1], scy = cyv[ey + 1], scz = czv[ez + 1];
+          for (int64_t k = 0; k < cnt; k++)
+            d = (float)((double)d +
+                        entry_contrib(qp[descending ? q0 + cnt - 1 - k : q0 + k], scx, scy, scz, L, m2pd, ex, ey, ez));
+        }
+  }
+  dens[n] = d;
 }
 
 __device__ inline double readlane_d(double x, int l)
@@ -217,6 +350,10 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const 
     ahf_axis_order(Y, v.L, oy);
     ahf_axis_order(X, v.L, ox);
     const double L = (double)v.L;
+    double cxv[3], cyv[3], czv[3];
+    ahf_centres3(X, v.L, cxv);
+    ahf_centres3(Y, v.L, cyv);
+    ahf_centres3(Z, v.L, czv);
     float d = -1.0f;
     for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++)
@@ -227,9 +364,7 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const 
           if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
           const int64_t q0 = p0[s], cnt = p0[s + 1] - q0;
           if (cnt == 0) continue;
-          const double cx = ahf_centre((X + ex + v.L) % v.L, L);
-          const double cy = ahf_centre((Y + ey + v.L) % v.L, L);
-          const double cz = ahf_centre((Z + ez + v.L) % v.L, L);
+          const double cx = cxv[ex + 1], cy = cyv[ey + 1], cz = czv[ez + 1];
           for (int64_t base = 0; base < cnt; base += 64) {
             const int64_t k = base + lane;
             double cv = 0.0;
@@ -351,8 +486,8 @@ __global__ void k_nbr_next(const int32_t *__restrict__ nbr, const int64_t *__res
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 27 * n2) return;
-  const int64_t o2 = t / 27;
-  const int e = (int)(t % 27);
+  const int64_t o2 = t < (int64_t)UINT32_MAX ? (int64_t)((uint32_t)t / 27u) : t / 27;
+  const int e = (int)(t - 27 * o2);
   const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
   const int64_t pn = parent2[o2];
   const int64_t m = ahf_nbr_node(nbr, pn >> 3, (int)(pn & 7), ex, ey, ez);
@@ -403,9 +538,8 @@ __global__ void k_nbr_search(const uint64_t *__restrict__ key, int64_t n_oct, in
   const int e = (int)(t % 27);
   const int ex = e % 3 - 1, ey = (e / 3) % 3 - 1, ez = e / 9 - 1;
   const uint64_t k = key[o];
-  const uint64_t want = ahf_pack3((uint64_t)((ahf_kx(k) + ex + half) % half),
-                                  (uint64_t)((ahf_ky(k) + ey + half) % half),
-                                  (uint64_t)((ahf_kz(k) + ez + half) % half));
+  const uint64_t want = ahf_pack3((uint64_t)ahf_wrap(ahf_kx(k) + ex, half), (uint64_t)ahf_wrap(ahf_ky(k) + ey, half),
+                                  (uint64_t)ahf_wrap(ahf_kz(k) + ez, half));
   int64_t lo = 0, hi = n_oct - 1, found = -1;
   while (lo <= hi) {
     const int64_t mid = (lo + hi) >> 1;
@@ -477,8 +611,11 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_max, int descendin
   if (timed) c->timer_end(e0, TIMER_PREP, 16.0 * (double)n_max + 12.0 * (double)n_max + 24.0 * (double)no);
   hipEvent_t e1 = timed ? c->timer_begin() : nullptr;
   const int64_t per_xcd = (nblk(nn) + AHF_XCDS - 1) / AHF_XCDS;
+  /* long lanes in a sparse launch leave the chip idle: hand them to the
+   * wavefront-per-node kernel sooner */
+  const uint32_t kmax = nn >= ((int64_t)1 << 21) ? 384u : 96u;
   hipLaunchKernelGGL(k_gather_light, dim3((unsigned)(AHF_XCDS * per_xcd)), dim3(AHF_BLOCK), 0, c->st, v, tord,
-                     per_xcd, l.pmask, l.node_p0, l.qp, descending, l.m2pd, l.dens, heavy, n_heavy);
+                     per_xcd, l.pmask, l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
   if (timed) c->timer_end(e1, TIMER_GATHER, 12.0 * (double)n_max + 13.0 * (double)nn);
   hipEvent_t e2 = timed ? c->timer_begin() : nullptr;
   hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.qp,

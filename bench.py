@@ -159,6 +159,8 @@ def main():
         log(f"plain form: {1e3 * (tp1 - tp0) / args.plain_steps:.3f} ms/step")
 
     deposits = int(info.deposits)
+    log("levels (nodes, particles): " + ", ".join(f"({int(info.n_nodes[i])}, {int(info.n_part[i])})"
+                                                   for i in range(info.n_levels)))
     stage_ms = {k: v[0] / args.steps for k, v in stages.items()}
     stage_n = {k: v[1] / args.steps for k, v in stages.items()}
     log("stages (ms/step): " + ", ".join(f"{k} {v:.3f} ({stage_n[k]:.0f})" for k, v in stage_ms.items()))
