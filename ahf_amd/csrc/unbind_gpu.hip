@@ -63,6 +63,80 @@ __device__ inline int64_t halo_of(const int64_t *off, int64_t nh, int64_t e)
   return lo;
 }
 
+/* sort key of a member: dist^2 from the centre (signed periodic wrap,
+ * src/libahf/ahf_halos.c:5827-5840), as FP64 bits; non-negative, so the bit
+ * order is the value order */
+__device__ inline uint64_t dist_key(uint32_t p, const double *__restrict__ c, const float *__restrict__ px,
+                                    const float *__restrict__ py, const float *__restrict__ pz)
+{
+  const double dX = wrap_s((double)px[p] - c[0]);
+  const double dY = wrap_s((double)py[p] - c[1]);
+  const double dZ = wrap_s((double)pz[p] - c[2]);
+  const double d2 = dX * dX + dY * dY + dZ * dZ;
+  return (uint64_t)__double_as_longlong(d2);
+}
+
+#define AHF_SORT_SMALL 8192
+#define AHF_SORT_THREADS 1024
+
+/*
+ * Radial sort of haloes of at most AHF_SORT_SMALL members, one workgroup per
+ * halo: keys in LDS, bitonic sort on (key, position in the list), so equal
+ * distances keep their input order as in the radix path.  Haloes below
+ * minpart keep their order (AHF does not sort them).
+ */
+__global__ void __launch_bounds__(AHF_SORT_THREADS) k_sort_small(const int64_t *__restrict__ off, int64_t nh,
+                                                                 const uint32_t *__restrict__ ip,
+                                                                 const double *__restrict__ cen,
+                                                                 const float *__restrict__ px,
+                                                                 const float *__restrict__ py,
+                                                                 const float *__restrict__ pz, int32_t minpart,
+                                                                 uint32_t *__restrict__ out)
+{
+  __shared__ uint64_t s_key[AHF_SORT_SMALL];
+  __shared__ uint32_t s_idx[AHF_SORT_SMALL];
+  const int tid = threadIdx.x, bs = blockDim.x;
+  for (int64_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const int64_t o0 = off[h];
+    const int n = (int)(off[h + 1] - o0);
+    if (n < minpart || n < 2) {
+      for (int i = tid; i < n; i += bs) out[o0 + i] = ip[o0 + i];
+      continue;
+    }
+    int m = 2;
+    while (m < n) m <<= 1;
+    for (int i = tid; i < m; i += bs) {
+      if (i < n) {
+        s_key[i] = dist_key(ip[o0 + i], cen + 3 * h, px, py, pz);
+        s_idx[i] = (uint32_t)i;
+      } else {
+        s_key[i] = ~0ull; /* above every finite key: padding sorts last */
+        s_idx[i] = 0xFFFFFFFFu;
+      }
+    }
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < m; i += bs) {
+          const int l = i ^ j;
+          if (l <= i) continue;
+          const uint64_t ka = s_key[i], kb = s_key[l];
+          const uint32_t ia = s_idx[i], ib = s_idx[l];
+          const bool a_gt_b = ka > kb || (ka == kb && ia > ib);
+          if (a_gt_b == ((i & k) == 0)) {
+            s_key[i] = kb;
+            s_key[l] = ka;
+            s_idx[i] = ib;
+            s_idx[l] = ia;
+          }
+        }
+        __syncthreads();
+      }
+    for (int i = tid; i < n; i += bs) out[o0 + i] = ip[o0 + s_idx[i]];
+    __syncthreads();
+  }
+}
+
 __global__ void k_dist_keys(const int64_t *__restrict__ off, int64_t nh, int64_t ne, const uint32_t *__restrict__ ip,
                             const double *__restrict__ cen, const float *__restrict__ px,
                             const float *__restrict__ py, const float *__restrict__ pz, int32_t minpart,
@@ -76,12 +150,7 @@ __global__ void k_dist_keys(const int64_t *__restrict__ off, int64_t nh, int64_t
     key[e] = 0; /* not sorted by AHF: keep input order */
     return;
   }
-  const uint32_t p = ip[e];
-  const double dX = wrap_s((double)px[p] - cen[3 * h]);
-  const double dY = wrap_s((double)py[p] - cen[3 * h + 1]);
-  const double dZ = wrap_s((double)pz[p] - cen[3 * h + 2]);
-  const double d2 = dX * dX + dY * dY + dZ * dZ;
-  key[e] = (uint64_t)__double_as_longlong(d2); /* non-negative: bit order = value order */
+  key[e] = dist_key(ip[e], cen + 3 * h, px, py, pz);
 }
 
 __global__ void k_halo_keys(const int64_t *__restrict__ off, int64_t nh, int64_t ne, const uint32_t *__restrict__ v,
@@ -514,11 +583,6 @@ __global__ void k_copy_u32(const uint32_t *__restrict__ in, int64_t n, uint32_t 
   if (i < n) out[i] = in[i];
 }
 
-__global__ void k_u64_to_u32(const uint64_t *__restrict__ in, int64_t n, uint32_t *__restrict__ out)
-{
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (uint32_t)in[i];
-}
 
 /* ============================================================ host side */
 
@@ -532,6 +596,11 @@ struct HaloSet {
   HaloOut *out = nullptr;
   uint32_t *buf_a = nullptr, *buf_b = nullptr;
   int done = 0;
+  /* one device block and one pinned host block, laid out alike (off, cen,
+   * order, ip_in) and kept across calls: a caller that unbinds one halo
+   * at a time pays one copy and no allocation per call */
+  char *dev = nullptr, *host = nullptr;
+  size_t cap = 0;
 };
 
 static inline unsigned nb_of(int64_t n)
@@ -542,64 +611,77 @@ static inline unsigned nb_of(int64_t n)
 static void free_halos(HaloSet *s)
 {
   if (!s) return;
-  void *ps[] = {s->off, s->ip_in, s->cen, s->order};
-  for (auto p : ps)
-    if (p) (void)hipFree(p);
+  if (s->dev) (void)hipFree(s->dev);
+  if (s->host) (void)hipHostFree(s->host);
   delete s;
 }
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 extern "C" int ahfgpu_set_halos(ahfgpu_ctx *c, const double *centres, const uint64_t *npart, const uint64_t *ipart,
                                 uint64_t nh)
 {
   if (!c || (nh && (!centres || !npart))) return AHFGPU_EINVAL;
   if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
-  free_halos((HaloSet *)c->halo_set);
+  HaloSet *s = (HaloSet *)c->halo_set;
+  if (!s) s = new HaloSet();
   c->halo_set = nullptr;
-  HaloSet *s = new HaloSet();
+  s->done = 0;
+  s->out = nullptr;
+  s->buf_a = s->buf_b = nullptr;
   s->nh = (int64_t)nh;
   s->h_off.assign(nh + 1, 0);
   for (uint64_t h = 0; h < nh; h++) s->h_off[h + 1] = s->h_off[h] + (int64_t)npart[h];
   s->ne = s->h_off[nh];
   if (s->ne >= (int64_t)UINT32_MAX) {
-    delete s;
+    free_halos(s);
     return c->fail(AHFGPU_EINVAL, "at most 2^32-1 halo members in total");
+  }
+  if (s->ne && !ipart) {
+    free_halos(s);
+    return AHFGPU_EINVAL;
   }
   for (int64_t e = 0; e < s->ne; e++)
     if (ipart[e] >= (uint64_t)c->N) {
-      delete s;
+      free_halos(s);
       return c->fail(AHFGPU_EINVAL, "member %lld refers to particle %llu beyond the resident set", (long long)e,
                      (unsigned long long)ipart[e]);
     }
-  std::vector<int32_t> order(nh);
-  for (uint64_t h = 0; h < nh; h++) order[h] = (int32_t)h;
-  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return npart[x] > npart[y]; });
-  bool ok = hipMalloc((void **)&s->off, (nh + 1) * sizeof(int64_t)) == hipSuccess &&
-            hipMalloc((void **)&s->ip_in, (s->ne + 1) * sizeof(uint32_t)) == hipSuccess &&
-            hipMalloc((void **)&s->cen, (3 * nh + 1) * sizeof(double)) == hipSuccess &&
-            hipMalloc((void **)&s->order, (nh + 1) * sizeof(int32_t)) == hipSuccess;
-  if (!ok) {
-    (void)hipGetLastError();
-    free_halos(s);
-    return c->fail(AHFGPU_ENOMEM, "halo arrays");
-  }
-  hipMemcpyAsync(s->off, s->h_off.data(), (nh + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->st);
-  if (nh) {
-    hipMemcpyAsync(s->cen, centres, 3 * nh * sizeof(double), hipMemcpyHostToDevice, c->st);
-    hipMemcpyAsync(s->order, order.data(), nh * sizeof(int32_t), hipMemcpyHostToDevice, c->st);
-  }
-  if (s->ne) {
-    uint64_t *stage = nullptr;
-    if (hipMalloc((void **)&stage, s->ne * sizeof(uint64_t)) != hipSuccess) {
+  const size_t o_off = 0, o_cen = align256((nh + 1) * sizeof(int64_t));
+  const size_t o_ord = o_cen + align256((3 * nh + 1) * sizeof(double));
+  const size_t o_ip = o_ord + align256((nh + 1) * sizeof(int32_t));
+  const size_t need = o_ip + align256(((size_t)s->ne + 1) * sizeof(uint32_t));
+  if (need > s->cap) {
+    /* the previous call's copy may still read the pinned block */
+    (void)hipStreamSynchronize(c->st);
+    if (s->dev) (void)hipFree(s->dev);
+    if (s->host) (void)hipHostFree(s->host);
+    s->dev = s->host = nullptr;
+    s->cap = 0;
+    const size_t cap = need + need / 2;
+    if (hipMalloc((void **)&s->dev, cap) != hipSuccess || hipHostMalloc((void **)&s->host, cap) != hipSuccess) {
       (void)hipGetLastError();
       free_halos(s);
-      return c->fail(AHFGPU_ENOMEM, "halo staging");
+      return c->fail(AHFGPU_ENOMEM, "halo arrays");
     }
-    hipMemcpyAsync(stage, ipart, s->ne * sizeof(uint64_t), hipMemcpyHostToDevice, c->st);
-    hipLaunchKernelGGL(k_u64_to_u32, dim3(nb_of(s->ne)), dim3(AHF_BLOCK), 0, c->st, stage, s->ne, s->ip_in);
-    hipStreamSynchronize(c->st);
-    (void)hipFree(stage);
+    s->cap = cap;
+  } else {
+    (void)hipStreamSynchronize(c->st);
   }
-  if (hipStreamSynchronize(c->st) != hipSuccess) {
+  int64_t *h_off = (int64_t *)(s->host + o_off);
+  double *h_cen = (double *)(s->host + o_cen);
+  int32_t *h_ord = (int32_t *)(s->host + o_ord);
+  uint32_t *h_ip = (uint32_t *)(s->host + o_ip);
+  memcpy(h_off, s->h_off.data(), (nh + 1) * sizeof(int64_t));
+  if (nh) memcpy(h_cen, centres, 3 * nh * sizeof(double));
+  for (uint64_t h = 0; h < nh; h++) h_ord[h] = (int32_t)h;
+  std::stable_sort(h_ord, h_ord + nh, [&](int32_t x, int32_t y) { return npart[x] > npart[y]; });
+  for (int64_t e = 0; e < s->ne; e++) h_ip[e] = (uint32_t)ipart[e];
+  s->off = (int64_t *)(s->dev + o_off);
+  s->cen = (double *)(s->dev + o_cen);
+  s->order = (int32_t *)(s->dev + o_ord);
+  s->ip_in = (uint32_t *)(s->dev + o_ip);
+  if (hipMemcpyAsync(s->dev, s->host, need, hipMemcpyHostToDevice, c->st) != hipSuccess) {
     free_halos(s);
     return c->fail(AHFGPU_EHIP, "halo upload");
   }
@@ -648,10 +730,24 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   hipEventRecord(c->ev_u0, c->st);
   hipEvent_t es = c->timer_begin();
   const int run = mode & 3;
+  int dev_cu = 256;
+  hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+  /* the largest halo that is sorted decides the sort: LDS per halo, or the
+   * radix sort of all members */
+  int64_t max_sorted = 0;
+  for (int64_t h = 0; h < nh; h++) {
+    const int64_t n = s->h_off[h + 1] - s->h_off[h];
+    if (n >= p->minpart && n > max_sorted) max_sorted = n;
+  }
+  const bool small_sort = max_sorted <= AHF_SORT_SMALL;
   const bool sort_only = (mode & AHFGPU_UNBIND_SORT_ONLY) != 0;
   const bool no_sort = (mode & AHFGPU_UNBIND_NO_SORT) != 0;
   if (ne > 0 && no_sort) {
     hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->ip_in, ne, s->buf_a);
+  } else if (ne > 0 && small_sort) {
+    const int64_t grid = std::min<int64_t>(nh, (int64_t)dev_cu * 2);
+    hipLaunchKernelGGL(k_sort_small, dim3((unsigned)grid), dim3(AHF_SORT_THREADS), 0, c->st, s->off, nh, s->ip_in,
+                       s->cen, c->px, c->py, c->pz, p->minpart, s->buf_a);
   } else if (ne > 0) {
     hipLaunchKernelGGL(k_dist_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, ne, s->ip_in, s->cen,
                        c->px, c->py, c->pz, p->minpart, k0, v0);
@@ -672,8 +768,6 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     if (nh) hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, s->out);
   } else if (run == 0) {
     hipMemsetAsync(next, 0, sizeof(int32_t), c->st);
-    int dev_cu = 256;
-    hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
     const int64_t grid = std::min<int64_t>(nh > 0 ? nh : 1, (int64_t)dev_cu * 8);
     hipLaunchKernelGGL(k_unbind_batched, dim3((unsigned)grid), dim3(AHF_BLOCK), 0, c->st, K, s->off, s->order, nh,
                        next, s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz,

@@ -77,6 +77,51 @@ def cpu_baseline(snap, threads):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def copy_probe(nbytes=1 << 30, reps=5):
+    """Device-to-device copies of nbytes through the HIP runtime: the copy
+    ceiling of this GPU (read + write bytes / time) and, under
+    rocprofv3 --pmc, the reference dispatches that calibrate FETCH_SIZE and
+    WRITE_SIZE (each copy is one runtime copy kernel moving nbytes each way)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    P = ctypes.c_void_p
+    a, b = P(), P()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        return None
+    try:
+        hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+        e0, e1 = P(), P()
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        return {"bytes_per_copy": nbytes, "copies": reps + 1,
+                "GBs": 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9}
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def load_traffic():
+    """HBM bytes per k_gather_light launch of this workload, measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the copy-kernel
+    calibration (profiles/gather_light_traffic.json, written by
+    python -m ahf_amd.rocprof_summary traffic)."""
+    path = os.path.join(ROOT, "profiles", "gather_light_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -117,6 +162,9 @@ def main():
     log(f"snapshot: {s.npart} particles, {s.nhalos} haloes, {s.n_ipart} members, L_dom {s.L_dom}")
 
     ctx = ahf_amd.Context(0)
+    copy = copy_probe()
+    if copy:
+        log(f"device copy ceiling: {copy['GBs']:.0f} GB/s (read + write)")
     ctx.set_particles(s.pos, s.mom)
     ctx.set_halos(s.centre, s.halo_npart, s.ipart)
     uargs = (s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble)
@@ -158,6 +206,13 @@ def main():
         plain = s.npart * args.plain_steps / (tp1 - tp0)
         log(f"plain form: {1e3 * (tp1 - tp0) / args.plain_steps:.3f} ms/step")
 
+    # the drop-in boundary hands over host arrays: time their upload (AoS
+    # pageable host memory -> SoA device arrays) separately; not part of value
+    tu0 = time.perf_counter()
+    ctx.set_particles(s.pos, s.mom)
+    upload_ms = 1e3 * (time.perf_counter() - tu0)
+    log(f"particle upload (host -> HBM, pos + mom): {upload_ms:.2f} ms")
+
     deposits = int(info.deposits)
     log("levels (nodes, particles): " + ", ".join(f"({int(info.n_nodes[i])}, {int(info.n_part[i])})"
                                                    for i in range(info.n_levels)))
@@ -173,6 +228,10 @@ def main():
     dep_bytes = sum(12 * p + 13 * n for p, n in levels)
     gather_avg_ms = stage_ms["gather_light"] / max(1.0, stage_n["gather_light"])
     achieved = (dep_bytes / max(1.0, stage_n["gather_light"])) / (gather_avg_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+    cfg_tag = f"cfg{args.config}_{args.side}_{args.seed}"
+    if traffic and traffic.get("workload") != cfg_tag:
+        traffic = None
 
     out = {
         "metric": METRIC,
@@ -214,10 +273,15 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "traffic_source": traffic["source"] if traffic else None,
             "avg_launch_ms": gather_avg_ms,
             "launches_per_step": stage_n["gather_light"],
+            "algorithmic_bytes_per_launch": dep_bytes / max(1.0, stage_n["gather_light"]),
         },
+        "copy_ceiling_GBs": copy["GBs"] if copy else None,
+        "upload_ms": upload_ms,
+        "pcie_inclusive_value": s.npart / ((ms_per_step + upload_ms) * 1e-3),
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -44,6 +44,14 @@
 
 #include "snapfile.h"
 
+#ifndef AHF_REF_BUILD
+/* The program owns AHF's globals, as AHF does (src/common.c and
+ * src/libahf/ahf_halos.c:163); the drop-in library reads these copies. */
+param_simu simu;
+info_global global;
+double r_fac, v_fac, phi_fac, Hubble;
+#endif
+
 static double now_s(void)
 {
   struct timespec ts;

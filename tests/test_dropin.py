@@ -1,9 +1,13 @@
 """The drop-in: AHF's own entry points (gen_domgrids, ll, zero_dens,
 assign_npart, gen_AMRhierarchy, sort_halo_particles, rem_unbound) served by
-libahf_amd.so, driven by the same harness that runs the reference's functions
-(oracle/ref_driver.c built against include/ahf_dropin.h).  The grids it
-returns through AHF's structs must equal the oracle's bit for bit; the haloes
-must have the oracle's membership in the oracle's order."""
+libahf_amd.so, driven by the harness that runs the reference's functions
+(oracle/ref_driver.c), linked two ways:
+  - oracle/_build/ahf_dropin_drv: compiled against include/ahf_dropin.h;
+  - oracle/_ref/ahf_ref_linked (where the reference tree was present at build
+    time): the reference's own objects, with the replaced functions made
+    local by objcopy, linked with the library -- the integration recipe.
+The grids returned through AHF's structs must equal the oracle's bit for bit;
+the haloes must have the oracle's membership in the oracle's order."""
 import os
 import subprocess
 
@@ -18,13 +22,18 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRV = os.path.join(ROOT, "oracle", "_build", "ahf_dropin_drv")
+LINKED = os.path.join(ROOT, "oracle", "_ref", "ahf_ref_linked")
 
 
-@pytest.fixture(scope="module")
-def drv():
-    if not os.path.exists(DRV):
-        pytest.fail("oracle/_build/ahf_dropin_drv missing: run `make -C oracle dropin` after building the library")
-    return DRV
+@pytest.fixture(scope="module", params=["dropin", "linked"])
+def drv(request):
+    if request.param == "dropin":
+        if not os.path.exists(DRV):
+            pytest.fail("oracle/_build/ahf_dropin_drv missing: run `make -C oracle dropin` after building the library")
+        return DRV
+    if not os.path.exists(LINKED):
+        pytest.skip("oracle/_ref/ahf_ref_linked not built (reference tree absent at build time)")
+    return LINKED
 
 
 def _run(cmd):
@@ -72,6 +81,8 @@ def _halo_match(ref, got, tag):
 @pytest.mark.parametrize("cmd", ["unbind", "unbind-batch"])
 @pytest.mark.parametrize("cfg,side,seed", [(1, 64, 1001), (2, 400, 1002)])
 def test_dropin_unbind(oracle, drv, snapgen, tmp_path, cmd, cfg, side, seed):
+    if cmd == "unbind-batch" and drv == LINKED:
+        pytest.skip("the reference's driver has no batched command")
     snap = str(tmp_path / "h.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
     o_out, d_out = str(tmp_path / "o.halo"), str(tmp_path / "d.halo")
