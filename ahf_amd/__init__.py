@@ -76,6 +76,7 @@ def load():
     lib.ahfgpu_set_particles.argtypes = [P, P, P, ctypes.c_uint64]
     lib.ahfgpu_amr_build.argtypes = [P, ctypes.POINTER(AmrParams), ctypes.POINTER(AmrInfo)]
     lib.ahfgpu_amr_dump.argtypes = [P, ctypes.c_char_p]
+    lib.ahfgpu_set_gather_mode.argtypes = [P, ctypes.c_int]
     lib.ahfgpu_amr_level_sizes.argtypes = [P, ctypes.c_int] + [P] * 5
     lib.ahfgpu_amr_level_copy.argtypes = [P, ctypes.c_int] + [P] * 5
     lib.ahfgpu_last_timing.argtypes = [P, P, P, P, P]
@@ -128,6 +129,12 @@ class Context:
         info = AmrInfo()
         self._check(self.lib.ahfgpu_amr_build(self.h, ctypes.byref(p), ctypes.byref(info)))
         return info
+
+    GATHER_AUTO, GATHER_SOURCE, GATHER_FLAT = 0, 1, 2
+
+    def set_gather_mode(self, mode):
+        """deposit kernel choice (same results in every mode)"""
+        self._check(self.lib.ahfgpu_set_gather_mode(self.h, int(mode)))
 
     def amr_dump(self, path):
         self._check(self.lib.ahfgpu_amr_dump(self.h, path.encode()))

@@ -68,10 +68,20 @@ typedef struct {
   const int32_t *plane_row0;/* [n_plane+1] first row of each plane         */
 } level_view;
 
-/* bucket entry record: FP32 position of the entry's particle (w unused) */
+/* bucket entry record of dense levels: FP32 position of the entry's
+ * particle (w unused) */
 typedef struct __attribute__((aligned(16))) {
   float x, y, z, w;
 } ahf_q4;
+
+/* TSC weights of one bucket entry against its own node, for the offsets
+ * d = -1, 0, +1 (index d+1) of the node it contributes to:
+ *   a[k]  = mass2dens * w(sz, k-1)   (the first product of AHF's
+ *           mass2dens * wz * wy * wx, density.c:393-400)
+ *   wy[k] = w(sy, k-1), wx[k] = w(sx, k-1)                               */
+typedef struct __attribute__((aligned(16))) {
+  double a[3], wy[3], wx[3], pad;
+} ahf_w9;
 
 AHF_HD uint64_t ahf_pack3(uint64_t x, uint64_t y, uint64_t z)
 {
@@ -196,48 +206,12 @@ AHF_HD void ahf_centres3(int64_t X, int64_t L, double c[3])
   c[2] = ahf_centre(ahf_wrap(X + 1, L), (double)L);
 }
 
-/*
- * Density of one target node in AHF's exact accumulation order
- * (assign_npart, src/libamr_serial/density.c:293-409): -mean_dens, then the
- * contributions of the source nodes in traversal order, each source's
- * particles in list order, each contribution mass2dens*wz*wy*wx rounded
- * into the FP32 node as float += double.
- */
-AHF_HD float ahf_gather_dens(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0,
-                             const uint32_t *plist, int descending, const float *px, const float *py,
-                             const float *pz, double mass2dens, int64_t o, int c)
+/* contribution mass2dens*wz*wy*wx of a particle with separations (sx,sy,sz)
+ * from its node to the node at weight offsets (dx,dy,dz)
+ * (src/libamr_serial/density.c:393-400) */
+AHF_HD double ahf_contrib(double m, double sx, double sy, double sz, int dx, int dy, int dz)
 {
-  const double L = (double)lv->L;
-  const uint64_t key = lv->oct_key[o];
-  const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
-  const int64_t X = 2 * ahf_kx(key) + cx, Y = 2 * ahf_ky(key) + cy, Z = 2 * ahf_kz(key) + cz;
-  int oz[3], oy[3], ox[3];
-  ahf_axis_order(Z, lv->L, oz);
-  ahf_axis_order(Y, lv->L, oy);
-  ahf_axis_order(X, lv->L, ox);
-  float dens = -1.0f;
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++)
-      for (int e = 0; e < 3; e++) {
-        const int ez = oz[a], ey = oy[b], ex = ox[e];
-        const int64_t s = ahf_nbr_node(lv->nbr, o, c, ex, ey, ez);
-        if (s < 0) continue;
-        /* target seen from the source: d = -e */
-        if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
-        const int64_t p0 = node_p0[s], p1 = node_p0[s + 1];
-        if (p0 == p1) continue;
-        const double cenx = ahf_centre(ahf_wrap(X + ex, lv->L), L);
-        const double ceny = ahf_centre(ahf_wrap(Y + ey, lv->L), L);
-        const double cenz = ahf_centre(ahf_wrap(Z + ez, lv->L), L);
-        for (int64_t q = 0; q < p1 - p0; q++) {
-          const uint32_t p = plist[descending ? p1 - 1 - q : p0 + q];
-          const double wx = ahf_tsc_w(ahf_pn_sep(px[p], cenx, L), -ex);
-          const double wy = ahf_tsc_w(ahf_pn_sep(py[p], ceny, L), -ey);
-          const double wz = ahf_tsc_w(ahf_pn_sep(pz[p], cenz, L), -ez);
-          dens = (float)((double)dens + mass2dens * wz * wy * wx);
-        }
-      }
-  return dens;
+  return m * ahf_tsc_w(sz, dz) * ahf_tsc_w(sy, dy) * ahf_tsc_w(sx, dx);
 }
 
 /* coordinates of node (o,c) of a level */
@@ -261,49 +235,66 @@ AHF_HD void ahf_particle_sep(const level_view *lv, int64_t node, float x, float 
   sep[2] = ahf_pn_sep(z, ahf_centre(Z, L), L);
 }
 
-/* one contribution mass2dens*wz*wy*wx for target-source offset (dx,dy,dz),
- * multiplied in AHF's order (src/libamr_serial/density.c:393-400) */
-AHF_HD double ahf_contrib(double m, double sx, double sy, double sz, int dx, int dy, int dz)
+/* the weight record of an entry whose particle is at (x,y,z) and is linked
+ * to node `node` (separations as in src/libamr_serial/density.c:340-360) */
+AHF_HD void ahf_entry_weights(const level_view *lv, int64_t node, float x, float y, float z, double mass2dens,
+                              ahf_w9 *w)
 {
-  return m * ahf_tsc_w(sz, dz) * ahf_tsc_w(sy, dy) * ahf_tsc_w(sx, dx);
+  double sep[3];
+  ahf_particle_sep(lv, node, x, y, z, sep);
+  for (int k = 0; k < 3; k++) {
+    w->a[k] = mass2dens * ahf_tsc_w(sep[2], k - 1);
+    w->wy[k] = ahf_tsc_w(sep[1], k - 1);
+    w->wx[k] = ahf_tsc_w(sep[0], k - 1);
+  }
+  w->pad = 0.0;
+}
+
+/* contribution of an entry to the node at offset (-ex,-ey,-ez) from the
+ * entry's own node: ((mass2dens*wz)*wy)*wx, AHF's association
+ * (src/libamr_serial/density.c:393-400) */
+AHF_HD double ahf_w9_contrib(const ahf_w9 *w, int ex, int ey, int ez)
+{
+  return w->a[1 - ez] * w->wy[1 - ey] * w->wx[1 - ex];
 }
 
 /*
- * Exact-order density of one target node from the particle positions stored
- * per bucket entry (qp[q] = FP32 position of the particle of entry q).  Same
- * sequence of roundings as ahf_gather_dens: the separation of a particle from
- * its own node's centre is recomputed at every visit.
+ * Exact-order density of one target node from the weight records of the
+ * bucket entries, walked as one flat sequence: the sources that contribute
+ * (present and non-empty) form a 27-bit mask in traversal order; the walk
+ * takes them lowest bit first and each one's entries in list order.  Same
+ * roundings as ahf_gather_dens.
  */
-AHF_HD float ahf_gather_dens_q(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0,
-                               const ahf_q4 *qp, int descending, double mass2dens, int64_t o, int c)
+AHF_HD float ahf_gather_dens_w(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0,
+                               const ahf_w9 *W, int descending, int64_t o, int c)
 {
-  const double L = (double)lv->L;
   int64_t X, Y, Z;
   ahf_node_xyz(lv, o, c, &X, &Y, &Z);
-  int oz[3], oy[3], ox[3];
-  ahf_axis_order(Z, lv->L, oz);
-  ahf_axis_order(Y, lv->L, oy);
-  ahf_axis_order(X, lv->L, ox);
+  const int rx = X == 0 ? 1 : (X == lv->L - 1 ? 2 : 0);
+  const int ry = Y == 0 ? 1 : (Y == lv->L - 1 ? 2 : 0);
+  const int rz = Z == 0 ? 1 : (Z == lv->L - 1 ? 2 : 0);
+  uint32_t mask = 0;
+  int64_t src[27];
+  for (int t = 0; t < 27; t++) {
+    const int ez = (t / 9 + rz) % 3 - 1, ey = ((t / 3) % 3 + ry) % 3 - 1, ex = (t % 3 + rx) % 3 - 1;
+    const int64_t s = ahf_nbr_node(lv->nbr, o, c, ex, ey, ez);
+    src[t] = s;
+    if (s < 0) continue;
+    if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
+    if (node_p0[s + 1] > node_p0[s]) mask |= 1u << t;
+  }
   float dens = -1.0f;
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++)
-      for (int e = 0; e < 3; e++) {
-        const int ez = oz[a], ey = oy[b], ex = ox[e];
-        const int64_t s = ahf_nbr_node(lv->nbr, o, c, ex, ey, ez);
-        if (s < 0) continue;
-        if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
-        const int64_t p0 = node_p0[s], p1 = node_p0[s + 1];
-        if (p0 == p1) continue;
-        const double cx = ahf_centre(ahf_wrap(X + ex, lv->L), L);
-        const double cy = ahf_centre(ahf_wrap(Y + ey, lv->L), L);
-        const double cz = ahf_centre(ahf_wrap(Z + ez, lv->L), L);
-        for (int64_t k = 0; k < p1 - p0; k++) {
-          const ahf_q4 e4 = qp[descending ? p1 - 1 - k : p0 + k];
-          const double w = ahf_contrib(mass2dens, ahf_pn_sep(e4.x, cx, L), ahf_pn_sep(e4.y, cy, L),
-                                       ahf_pn_sep(e4.z, cz, L), -ex, -ey, -ez);
-          dens = (float)((double)dens + w);
-        }
-      }
+  while (mask) {
+    int t = 0;
+    while (!((mask >> t) & 1u)) t++;
+    mask &= mask - 1u;
+    const int ez = (t / 9 + rz) % 3 - 1, ey = ((t / 3) % 3 + ry) % 3 - 1, ex = (t % 3 + rx) % 3 - 1;
+    const int64_t s = src[t], p0 = node_p0[s], p1 = node_p0[s + 1];
+    for (int64_t k = 0; k < p1 - p0; k++) {
+      const int64_t q = descending ? p1 - 1 - k : p0 + k;
+      dens = (float)((double)dens + ahf_w9_contrib(&W[q], ex, ey, ez));
+    }
+  }
   return dens;
 }
 
@@ -332,23 +323,6 @@ AHF_HD uint64_t ahf_morton(uint64_t key)
 {
   return ahf_spread3((uint64_t)ahf_kx(key)) | ahf_spread3((uint64_t)ahf_ky(key)) << 1 |
          ahf_spread3((uint64_t)ahf_kz(key)) << 2;
-}
-
-/* number of contributions a target node receives (sum of the bucket sizes of
- * the sources that see it) */
-AHF_HD int64_t ahf_gather_count(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0, int64_t o,
-                                int c)
-{
-  int64_t k = 0;
-  for (int ez = -1; ez <= 1; ez++)
-    for (int ey = -1; ey <= 1; ey++)
-      for (int ex = -1; ex <= 1; ex++) {
-        const int64_t s = ahf_nbr_node(lv->nbr, o, c, ex, ey, ez);
-        if (s < 0) continue;
-        if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
-        k += node_p0[s + 1] - node_p0[s];
-      }
-  return k;
 }
 
 /* test_node, src/libamr_serial/refine_grid.c:113-138: returns NF_* bits */

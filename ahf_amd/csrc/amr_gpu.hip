@@ -121,8 +121,24 @@ __global__ void k_pmask(level_view v, uint32_t *__restrict__ pmask)
   pmask[n] = ahf_presence_mask(&v, n >> 3, (int)(n & 7));
 }
 
-/* position record of every bucket entry (entries [0, *n)) */
-__global__ void k_prep(const int64_t *__restrict__ n_ptr, int64_t n_max, const uint32_t *__restrict__ plist,
+/* weight record of every bucket entry (entries [0, *n)): its particle's
+ * TSC weights against the entry's own node */
+__global__ void k_prep(level_view v, const int64_t *__restrict__ n_ptr, int64_t n_max,
+                       const uint32_t *__restrict__ pnode, const uint32_t *__restrict__ plist,
+                       const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
+                       double m2pd, ahf_w9 *__restrict__ W)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_max || q >= *n_ptr) return;
+  const uint32_t p = plist[q];
+  ahf_w9 w;
+  ahf_entry_weights(&v, (int64_t)pnode[q], px[p], py[p], pz[p], m2pd, &w);
+  W[q] = w;
+}
+
+/* position record of every bucket entry (entries [0, *n)), for the
+ * per-source walk of dense levels */
+__global__ void k_prep_q(const int64_t *__restrict__ n_ptr, int64_t n_max, const uint32_t *__restrict__ plist,
                        const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
                        ahf_q4 *__restrict__ qp)
 {
@@ -137,6 +153,40 @@ __global__ void k_prep(const int64_t *__restrict__ n_ptr, int64_t n_max, const u
   qp[q] = e;
 }
 
+
+/* Morton order of a full grid of half^3 octets, half a power of two: the
+ * i-th octet in Morton order is the one whose interleaved coordinates are i */
+__global__ void k_morton_full(int64_t half, uint32_t *__restrict__ tord)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= half * half * half) return;
+  const int64_t x = (int64_t)ahf_compact3((uint64_t)i), y = (int64_t)ahf_compact3((uint64_t)i >> 1),
+                z = (int64_t)ahf_compact3((uint64_t)i >> 2);
+  tord[i] = (uint32_t)((z * half + y) * half + x);
+}
+
+/* Morton order of the next level from this one's: a fine octet is a
+ * refined node, its Morton code is the coarse octet's followed by the child
+ * index cz*4+cy*2+cx, so the fine octets in Morton order are the refined
+ * nodes taken octet by octet in Morton order, children in index order.
+ * Position k = 8*(rank of the octet in Morton order) + child. */
+__global__ void k_morton_next_flags(const uint32_t *__restrict__ tord, const int32_t *__restrict__ node2oct,
+                                    int64_t nn, int32_t *__restrict__ flag)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  flag[k] = node2oct[8 * (int64_t)tord[k >> 3] + (k & 7)] >= 0 ? 1 : 0;
+}
+
+__global__ void k_morton_next_emit(const uint32_t *__restrict__ tord, const int32_t *__restrict__ node2oct,
+                                   int64_t nn, const int32_t *__restrict__ pos, uint32_t *__restrict__ ftord)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  const int32_t o2 = node2oct[8 * (int64_t)tord[k >> 3] + (k & 7)];
+  if (o2 >= 0) ftord[pos[k]] = (uint32_t)o2;
+}
+
 /* Morton key of every octet, for the target order of the gather */
 __global__ void k_morton_keys(level_view v, uint64_t *__restrict__ mk, uint32_t *__restrict__ idx)
 {
@@ -148,6 +198,125 @@ __global__ void k_morton_keys(level_view v, uint64_t *__restrict__ mk, uint32_t 
 
 #define AHF_XCDS 8
 #define AHF_GL_NB 64 /* source nodes around one octet: 4x4x4 */
+
+/* traversal position t of a target's sources -> offset along one axis, for
+ * the axis rotation r (0 inside, 1 at X = 0, 2 at X = L-1; ahf_axis_order) */
+__device__ __forceinline__ int rot_offset(int i, int r)
+{
+  const int v = i + r;
+  return (v >= 3 ? v - 3 : v) - 1;
+}
+
+/* the three weights of record q for a source at offset (ex,ey,ez) */
+__device__ __forceinline__ double w9_contrib(const ahf_w9 *__restrict__ W, uint32_t q, int iz, int iy, int ix)
+{
+  const double *w = reinterpret_cast<const double *>(W + q);
+  return w[iz] * w[3 + iy] * w[6 + ix];
+}
+
+/*
+ * Exact-order density, one lane per target node.
+ *
+ * Block b is remapped so that each XCD (blocks are dealt to the 8 XCDs in
+ * turn) walks one contiguous stretch of the Morton order of the octets
+ * (tord); the 8 children of an octet are 8 adjacent lanes.  The 8 lanes of
+ * an octet first load the records of the 64 source nodes of the octet's
+ * 4x4x4 neighbourhood (bucket start, entry count, presence mask) into LDS.
+ * Each lane then marks, in a 27-bit mask in AHF's traversal order, the
+ * sources that contribute (present as get_TSCnodes finds them, non-empty),
+ * and walks them as one flat sequence of entries: the wavefront runs as long
+ * as its longest lane, not as the sum over sources of the longest bucket.
+ * Every contribution is two products of the entry's precomputed weights
+ * (k_prep) and the FP32 rounding.  Targets with more than kmax
+ * contributions are left to k_gather_heavy.
+ */
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const uint32_t *__restrict__ tord,
+                                                            int64_t per_xcd, const uint32_t *__restrict__ pmask,
+                                                            const int64_t *__restrict__ p0,
+                                                            const ahf_w9 *__restrict__ W, int descending,
+                                                            uint32_t kmax, float *__restrict__ dens,
+                                                            uint32_t *__restrict__ heavy,
+                                                            uint32_t *__restrict__ n_heavy)
+{
+  __shared__ uint4 s_rec[AHF_BLOCK / 8 * AHF_GL_NB];
+  const int64_t b = blockIdx.x;
+  const int64_t t = ((b % AHF_XCDS) * per_xcd + b / AHF_XCDS) * blockDim.x + threadIdx.x;
+  const bool active = t < 8 * v.n_oct;
+  const int c = threadIdx.x & 7;
+  const int nb0 = (threadIdx.x >> 3) * AHF_GL_NB;
+  const int64_t o = active ? (int64_t)tord[t >> 3] : 0;
+  if (active) {
+    const int32_t *nrow = v.nbr + 27 * o;
+    for (int m = 0; m < 8; m++) {
+      const int u = c * 8 + m;
+      const int i = u & 3, j = (u >> 2) & 3, k = u >> 4;
+      const int on = nrow[ahf_e27(i == 0 ? -1 : (i == 3), j == 0 ? -1 : (j == 3), k == 0 ? -1 : (k == 3))];
+      uint4 r = make_uint4(0u, 0u, 0u, 0u);
+      if (on >= 0) {
+        const int64_t sn = 8 * (int64_t)on + (((k + 1) & 1) << 2 | ((j + 1) & 1) << 1 | ((i + 1) & 1));
+        const int64_t a = p0[sn];
+        r.x = (uint32_t)a;
+        r.y = (uint32_t)(p0[sn + 1] - a);
+        r.z = pmask[sn];
+      }
+      s_rec[nb0 + u] = r;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+  /* the target's own node in the neighbourhood: offsets (dx,dy,dz) add
+   * dz*16 + dy*4 + dx */
+  const uint4 *rec = s_rec + nb0 + (cz + 1) * 16 + (cy + 1) * 4 + (cx + 1);
+  const int64_t n = 8 * o + c;
+  int64_t X, Y, Z;
+  ahf_node_xyz(&v, o, c, &X, &Y, &Z);
+  const int rx = X == 0 ? 1 : (X == v.L - 1 ? 2 : 0);
+  const int ry = Y == 0 ? 1 : (Y == v.L - 1 ? 2 : 0);
+  const int rz = Z == 0 ? 1 : (Z == v.L - 1 ? 2 : 0);
+  uint32_t mask = 0;
+  uint64_t total = 0;
+#pragma unroll 1
+  for (int tz = 0; tz < 3; tz++)
+#pragma unroll
+    for (int ty = 0; ty < 3; ty++)
+#pragma unroll
+      for (int tx = 0; tx < 3; tx++) {
+        const int ez = rot_offset(tz, rz), ey = rot_offset(ty, ry), ex = rot_offset(tx, rx);
+        const uint32_t *rw = reinterpret_cast<const uint32_t *>(rec + ez * 16 + ey * 4 + ex);
+        const uint32_t cnt = rw[1], pm = rw[2];
+        if (((pm >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u) && cnt) {
+          mask |= 1u << (tz * 9 + ty * 3 + tx);
+          total += cnt;
+        }
+      }
+  if (total > kmax) {
+    heavy[atomicAdd(n_heavy, 1u)] = (uint32_t)n;
+    return;
+  }
+  const uint32_t step = descending ? 0xFFFFFFFFu : 1u;
+  float d = -1.0f;
+  uint32_t rem = 0, q = 0;
+  int iz = 0, iy = 0, ix = 0;
+  for (uint32_t it = 0; it < (uint32_t)total; it++) {
+    if (rem == 0) {
+      const int tt = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      const int tz = tt / 9, ty = (tt / 3) % 3, tx = tt % 3;
+      const int ez = rot_offset(tz, rz), ey = rot_offset(ty, ry), ex = rot_offset(tx, rx);
+      const uint4 r = rec[ez * 16 + ey * 4 + ex];
+      rem = r.y;
+      q = step == 1u ? r.x : r.x + r.y - 1u;
+      iz = 1 - ez;
+      iy = 1 - ey;
+      ix = 1 - ex;
+    }
+    d = (float)((double)d + w9_contrib(W, q, iz, iy, ix));
+    q += step;
+    rem--;
+  }
+  dens[n] = d;
+}
 
 /* contribution of one entry record to a target at offset (-ex,-ey,-ez) from
  * its source node, whose centre is (cx,cy,cz) */
@@ -218,7 +387,8 @@ __device__ __forceinline__ void gather_source_c(float &d, const uint4 *rec, cons
   AHF_GS_ROW(1, EZ)
 
 /*
- * Exact-order density, one lane per target node.
+ * Exact-order density for dense levels, one lane per target node, walking
+ * the 27 sources one by one over the entries' position records.
  *
  * Block b is remapped so that each XCD (blocks are dealt to the 8 XCDs in
  * turn) walks one contiguous stretch of the Morton order of the octets
@@ -231,7 +401,7 @@ __device__ __forceinline__ void gather_source_c(float &d, const uint4 *rec, cons
  * z,y,x = -1,0,1 with compile-time offsets; the others follow AHF's rotated
  * order at the faces (ahf_axis_order).
  */
-__global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const uint32_t *__restrict__ tord,
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather_src(level_view v, const uint32_t *__restrict__ tord,
                                                             int64_t per_xcd, const uint32_t *__restrict__ pmask,
                                                             const int64_t *__restrict__ p0,
                                                             const ahf_q4 *__restrict__ qp, int descending,
@@ -329,6 +499,50 @@ __device__ inline double readlane_d(double x, int l)
  * lanes compute 64 contributions at a time, the wave then folds them into
  * the FP32 node value in AHF's order */
 __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const uint32_t *__restrict__ pmask,
+                                                            const int64_t *__restrict__ p0,
+                                                            const ahf_w9 *__restrict__ W, int descending,
+                                                            float *__restrict__ dens,
+                                                            const uint32_t *__restrict__ heavy,
+                                                            const uint32_t *__restrict__ n_heavy)
+{
+  const int lane = threadIdx.x & 63;
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t nh = (int64_t)*n_heavy;
+  for (int64_t i = gw; i < nh; i += nw) {
+    const int64_t n = heavy[i];
+    const int64_t o = n >> 3;
+    const int c = (int)(n & 7);
+    int64_t X, Y, Z;
+    ahf_node_xyz(&v, o, c, &X, &Y, &Z);
+    int oz[3], oy[3], ox[3];
+    ahf_axis_order(Z, v.L, oz);
+    ahf_axis_order(Y, v.L, oy);
+    ahf_axis_order(X, v.L, ox);
+    float d = -1.0f;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        for (int e = 0; e < 3; e++) {
+          const int ez = oz[a], ey = oy[b], ex = ox[e];
+          const int64_t s = ahf_nbr_node(v.nbr, o, c, ex, ey, ez);
+          if (s < 0) continue;
+          if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
+          const int64_t q0 = p0[s], cnt = p0[s + 1] - q0;
+          for (int64_t base = 0; base < cnt; base += 64) {
+            const int64_t k = base + lane;
+            double cv = 0.0;
+            if (k < cnt) cv = w9_contrib(W, (uint32_t)(descending ? q0 + cnt - 1 - k : q0 + k), 1 - ez, 1 - ey, 1 - ex);
+            const int mm = (int)(cnt - base < 64 ? cnt - base : 64);
+            for (int l = 0; l < mm; l++) d = (float)((double)d + readlane_d(cv, l));
+          }
+        }
+    if (lane == 0) dens[n] = d;
+  }
+}
+
+
+/* k_gather_heavy over position records (dense levels) */
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy_q(level_view v, const uint32_t *__restrict__ pmask,
                                                             const int64_t *__restrict__ p0,
                                                             const ahf_q4 *__restrict__ qp, int descending,
                                                             double m2pd, float *__restrict__ dens,
@@ -580,53 +794,84 @@ static level_view make_view(const DevLevel &l)
   return v;
 }
 
+/* bucket entries per node from which a level counts as dense */
+#define AHF_DENSE_ENTRIES_PER_NODE 3
+
 /*
- * Exact-order gather of one level: position record per bucket entry, the
- * octets sorted into Morton order (target order of the light gather), one
- * lane per light target, one wavefront per heavy target.  n_max bounds the
- * number of bucket entries (the exact count is on the device,
- * node_p0[nodes]).  Needs pmask.
+ * Exact-order gather of one level over its n_ent bucket entries: the octets
+ * sorted into Morton order (the target order), then either
+ *  - dense levels (many entries per node): a position record per entry and
+ *    the per-source walk (k_gather_src), whose records stay in L2; or
+ *  - sparse levels: a TSC weight record per entry and the flat walk
+ *    (k_gather_light), which keeps a wavefront's lanes busy when buckets
+ *    hold a particle or two;
+ * and one wavefront per node with more than kmax contributions.  Every
+ * choice gives the same bits.  Needs pmask.
  */
-static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_max, int descending, bool timed)
+static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descending, bool timed)
 {
   const int64_t nn = 8 * l.n_oct, no = l.n_oct;
   level_view v = make_view(l);
   Arena &A = c->arena;
+  const bool dense = c->gather_mode == AHFGPU_GATHER_SOURCE ||
+                     (c->gather_mode == AHFGPU_GATHER_AUTO && n_ent >= AHF_DENSE_ENTRIES_PER_NODE * nn);
   l.dens = A.get<float>(nn);
-  l.qp = A.get<ahf_q4>(n_max > 0 ? n_max : 1);
+  if (dense)
+    l.qp = A.get<ahf_q4>(n_ent > 0 ? n_ent : 1);
+  else
+    l.W = A.get<ahf_w9>(n_ent > 0 ? n_ent : 1);
   uint32_t *heavy = A.get<uint32_t>(nn);
-  uint64_t *mk0 = A.get<uint64_t>(no), *mk1 = A.get<uint64_t>(no);
-  uint32_t *ix0 = A.get<uint32_t>(no), *ix1 = A.get<uint32_t>(no);
-  uint32_t *rtmp = A.get<uint32_t>(radix_tmp_u32<uint64_t>(no));
-  if (!l.dens || !l.qp || !heavy || !mk0 || !mk1 || !ix0 || !ix1 || !rtmp) return AHFGPU_ENOMEM;
+  if (!l.dens || !(dense ? (void *)l.qp : (void *)l.W) || !heavy) return AHFGPU_ENOMEM;
   uint32_t *n_heavy = (uint32_t *)(c->d_anomaly + 1);
   hipMemsetAsync(n_heavy, 0, sizeof(uint32_t), c->st);
   hipEvent_t e0 = timed ? c->timer_begin() : nullptr;
-  LAUNCH(k_prep, n_max, l.node_p0 + nn, n_max, l.plist, c->px, c->py, c->pz, l.qp);
-  int axis_bits = 1;
-  while ((1ll << axis_bits) < l.L / 2) axis_bits++;
-  LAUNCH(k_morton_keys, no, v, mk0, ix0);
-  const int w = radix_sort<uint64_t, uint32_t>(mk0, ix0, mk1, ix1, no, 3 * axis_bits, rtmp, c->st);
-  const uint32_t *tord = w ? ix1 : ix0;
-  if (timed) c->timer_end(e0, TIMER_PREP, 16.0 * (double)n_max + 12.0 * (double)n_max + 24.0 * (double)no);
+  if (dense)
+    LAUNCH(k_prep_q, n_ent, l.node_p0 + nn, n_ent, l.plist, c->px, c->py, c->pz, l.qp);
+  else
+    LAUNCH(k_prep, n_ent, v, l.node_p0 + nn, n_ent, l.pnode, l.plist, c->px, c->py, c->pz, l.m2pd, l.W);
+  if (!l.tord) {
+    /* no order inherited from a coarser level: sort the octets' Morton keys */
+    uint64_t *mk0 = A.get<uint64_t>(no), *mk1 = A.get<uint64_t>(no);
+    uint32_t *ix0 = A.get<uint32_t>(no), *ix1 = A.get<uint32_t>(no);
+    uint32_t *rtmp = A.get<uint32_t>(radix_tmp_u32<uint64_t>(no));
+    if (!mk0 || !mk1 || !ix0 || !ix1 || !rtmp) return AHFGPU_ENOMEM;
+    int axis_bits = 1;
+    while ((1ll << axis_bits) < l.L / 2) axis_bits++;
+    LAUNCH(k_morton_keys, no, v, mk0, ix0);
+    const int w = radix_sort<uint64_t, uint32_t>(mk0, ix0, mk1, ix1, no, 3 * axis_bits, rtmp, c->st);
+    l.tord = w ? ix1 : ix0;
+  }
+  const uint32_t *tord = l.tord;
+  if (timed)
+    c->timer_end(e0, TIMER_PREP, (12.0 + 8.0 + (dense ? 16.0 : 80.0)) * (double)n_ent + 24.0 * (double)no);
   hipEvent_t e1 = timed ? c->timer_begin() : nullptr;
   const int64_t per_xcd = (nblk(nn) + AHF_XCDS - 1) / AHF_XCDS;
-  /* long lanes in a sparse launch leave the chip idle: hand them to the
+  const dim3 grid((unsigned)(AHF_XCDS * per_xcd));
+  /* long lanes in a small launch leave the chip idle: hand them to the
    * wavefront-per-node kernel sooner */
   const uint32_t kmax = nn >= ((int64_t)1 << 21) ? 384u : 96u;
-  hipLaunchKernelGGL(k_gather_light, dim3((unsigned)(AHF_XCDS * per_xcd)), dim3(AHF_BLOCK), 0, c->st, v, tord,
-                     per_xcd, l.pmask, l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
-  if (timed) c->timer_end(e1, TIMER_GATHER, 12.0 * (double)n_max + 13.0 * (double)nn);
+  if (dense)
+    hipLaunchKernelGGL(k_gather_src, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0, l.qp,
+                       descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
+  else
+    hipLaunchKernelGGL(k_gather_light, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0, l.W,
+                       descending, kmax, l.dens, heavy, n_heavy);
+  if (timed) c->timer_end(e1, TIMER_GATHER, 12.0 * (double)n_ent + 13.0 * (double)nn);
   hipEvent_t e2 = timed ? c->timer_begin() : nullptr;
-  hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.qp,
-                     descending, l.m2pd, l.dens, heavy, n_heavy);
+  if (dense)
+    hipLaunchKernelGGL(k_gather_heavy_q, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.qp,
+                       descending, l.m2pd, l.dens, heavy, n_heavy);
+  else
+    hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.W,
+                       descending, l.dens, heavy, n_heavy);
   if (timed) c->timer_end(e2, TIMER_HEAVY, 0.0);
   return 0;
 }
 
-/* TSC deposit of one level in AHF's exact order (even levels walk their
- * lists from the tail of each bucket, odd levels from the head) */
-static int deposit_level(ahfgpu_ctx *c, DevLevel &l, int level_index, int64_t n_max)
+/* TSC deposit of one level over its n_ent bucket entries in AHF's exact
+ * order (even levels walk their lists from the tail of each bucket, odd
+ * levels from the head) */
+static int deposit_level(ahfgpu_ctx *c, DevLevel &l, int level_index, int64_t n_ent)
 {
   const int64_t nn = 8 * l.n_oct;
   if (!l.pmask) {
@@ -635,7 +880,7 @@ static int deposit_level(ahfgpu_ctx *c, DevLevel &l, int level_index, int64_t n_
     level_view v = make_view(l);
     LAUNCH(k_pmask, nn, v, l.pmask);
   }
-  if (gather_level(c, l, n_max, (level_index % 2) == 0, true))
+  if (gather_level(c, l, n_ent, (level_index % 2) == 0, true))
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit, level %d)", level_index);
   return 0;
 }
@@ -663,6 +908,11 @@ static int build_domain(ahfgpu_ctx *c, DevLevel &d)
       !d.plane_row0 || !d.node_p0 || !k0 || !v0 || !k1 || !v1 || !rtmp)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (domain)");
   LAUNCH(k_domain_octets, no, half, d.oct_key, d.oct_link, d.nbr);
+  if ((half & (half - 1)) == 0) {
+    d.tord = A.get<uint32_t>(no);
+    if (!d.tord) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (domain)");
+    LAUNCH(k_morton_full, no, half, d.tord);
+  }
   LAUNCH(k_domain_rows, half * half + 1, half, d.row_key, d.row_oct0, d.row_plane, d.plane_z, d.plane_row0);
   hipEvent_t e0 = c->timer_begin();
   LAUNCH(k_bin_domain, N, c->px, c->py, c->pz, N, L, k0, v0);
@@ -759,6 +1009,15 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   LAUNCH(k_compact, nrank, v, l.flags, cnt, oct_off, row_off, plane_off, f.oct_key, f.oct_link, f.oct_parent,
          f.row_key, f.row_oct0, f.row_plane, l.node2oct);
   LAUNCH(k_plane_fill, nplr, v, phas, plane_off, row_off, f.plane_z, f.plane_row0);
+  if (l.tord) {
+    int32_t *mflag = A.get<int32_t>(nn), *mpos = A.get<int32_t>(nn + 1);
+    int32_t *mtmp = A.get<int32_t>(scan_tmp_elems(nn));
+    f.tord = A.get<uint32_t>(n2);
+    if (!mflag || !mpos || !mtmp || !f.tord) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", ci + 1);
+    LAUNCH(k_morton_next_flags, nn, l.tord, l.node2oct, nn, mflag);
+    exclusive_scan<int32_t, int32_t>(mflag, mpos, nn, mtmp, c->st);
+    LAUNCH(k_morton_next_emit, nn, l.tord, l.node2oct, nn, mpos, f.tord);
+  }
   hipLaunchKernelGGL(k_set2, dim3(1), dim3(1), 0, c->st, f.row_oct0, f.n_row, (int32_t)f.n_oct, f.plane_row0,
                      f.n_plane, (int32_t)f.n_row);
   LAUNCH(k_nbr_next, 27 * n2, l.nbr, f.oct_parent, l.node2oct, n2, f.nbr);
@@ -776,11 +1035,17 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   f.plist = w ? v1 : v0;
   LAUNCH(k_csr_lower_bound, nn2 + 1, f.pnode, nent, nn2, f.node_p0);
   c->timer_end(e2, TIMER_RELINK, 28.0 * (double)nent);
-  f.n_part = -1; /* on the device until the next synchronisation */
+  /* the particles the fine level keeps: its deposit kernel depends on how
+   * many entries each node holds */
+  int64_t nent_f = 0;
+  hipMemcpyAsync(&nent_f, f.node_p0 + nn2, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "level %d relink: %s", ci + 1, hipGetErrorString(hipGetLastError()));
+  f.n_part = nent_f;
   f.m2d = l.m2d * AHF_CRITMULTI_D;
   f.m2pd = l.m2pd * AHF_CRITMULTI_D;
   f.crit = c->params.Nth_ref * f.m2d;
-  int rc = deposit_level(c, f, ci + 1, nent);
+  int rc = deposit_level(c, f, ci + 1, nent_f);
   if (rc) return rc;
   if (nn2 < AHF_MIN_NNODES) {
     /* relink_back (src/libamr_serial/generate_grids.c:231-238): the fine

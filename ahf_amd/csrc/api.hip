@@ -118,6 +118,15 @@ extern "C" int ahfgpu_set_particles(ahfgpu_ctx *c, const float *pos, const float
   return 0;
 }
 
+extern "C" int ahfgpu_set_gather_mode(ahfgpu_ctx *c, int mode)
+{
+  if (!c) return AHFGPU_EINVAL;
+  if (mode != AHFGPU_GATHER_AUTO && mode != AHFGPU_GATHER_SOURCE && mode != AHFGPU_GATHER_FLAT)
+    return c->fail(AHFGPU_EINVAL, "unknown gather mode %d", mode);
+  c->gather_mode = mode;
+  return 0;
+}
+
 extern "C" int ahfgpu_last_timing(ahfgpu_ctx *c, double *amr_ms, double *unbind_ms, double *gather_ms,
                                   int64_t *gather_launches)
 {

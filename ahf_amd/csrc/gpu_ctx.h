@@ -85,7 +85,9 @@ struct DevLevel {
   int64_t *node_p0 = nullptr;
   uint32_t *plist = nullptr;
   uint32_t *pnode = nullptr;   /* per bucket entry: its node */
-  ahf_q4 *qp = nullptr;                             /* per bucket entry: particle position        */
+  ahf_w9 *W = nullptr;                              /* per bucket entry: TSC weights (sparse)     */
+  ahf_q4 *qp = nullptr;                             /* per bucket entry: position (dense levels)  */
+  uint32_t *tord = nullptr;                         /* octets in Morton order                      */
   int8_t *child = nullptr;     /* per bucket entry: fine child it moved to, or -1 */
   int32_t *node2oct = nullptr; /* per node: octet of the next level, or -1 */
   double m2d = 0, m2pd = 0, crit = 0;
@@ -130,6 +132,7 @@ struct ahfgpu_ctx {
   int built = 0;
   int32_t *d_anomaly = nullptr;
   void *halo_set = nullptr; /* resident halo lists (unbind_gpu.hip) */
+  int gather_mode = 0;      /* AHFGPU_GATHER_*: deposit kernel choice */
   hipEvent_t ev_a0 = nullptr, ev_a1 = nullptr, ev_u0 = nullptr, ev_u1 = nullptr;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_next = 0;

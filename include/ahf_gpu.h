@@ -119,6 +119,15 @@ int ahfgpu_deposit_level(ahfgpu_ctx *ctx, int64_t L, const uint64_t *oct_key, co
                          int64_t n_oct, const int64_t *list0, const uint32_t *list, double masstopartdens,
                          float *dens_out);
 
+#define AHFGPU_GATHER_AUTO 0   /* per level: SOURCE when the level holds at least 3 entries per node */
+#define AHFGPU_GATHER_SOURCE 1 /* walk the 27 sources over per-entry positions */
+#define AHFGPU_GATHER_FLAT 2   /* one flat walk per node over per-entry TSC weights */
+
+/* No counterpart in the reference: choose the deposit kernel of the
+ * following builds and deposits.  Every mode gives the same bits; AUTO
+ * (the default) picks the faster one per level. */
+int ahfgpu_set_gather_mode(ahfgpu_ctx *ctx, int mode);
+
 /* copy nh haloes to the device: centres (box units, nh x 3), member counts
  * and member particle indices (CSR, indices into the resident particles) --
  * the HALO.pos / npart / ipart that sort_halo_particles() receives */

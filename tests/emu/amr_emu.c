@@ -57,8 +57,8 @@ static void set_view(lev_t *l, int64_t L, int64_t n_oct, int64_t n_row, int64_t 
   l->v.plane_z = l->plane_z; l->v.plane_row0 = l->plane_row0;
 }
 
-/* deposit as the device does it: one position record per bucket entry
- * (k_prep), then the exact-order gather of every node from them
+/* deposit as the device does it: one weight record per bucket entry
+ * (k_prep), then the exact-order flat walk of every node over them
  * (k_gather_light/_heavy) */
 static void deposit_q(lev_t *l, const float *px, const float *py, const float *pz, int descending)
 {
@@ -67,15 +67,16 @@ static void deposit_q(lev_t *l, const float *px, const float *py, const float *p
     l->pmask = (uint32_t *)xm((size_t)nn * 4);
     for (int64_t n = 0; n < nn; n++) l->pmask[n] = ahf_presence_mask(&l->v, n / 8, (int)(n % 8));
   }
-  ahf_q4 *qp = (ahf_q4 *)xm((size_t)(ne > 0 ? ne : 1) * sizeof(ahf_q4));
-  for (int64_t q = 0; q < ne; q++) {
-    uint32_t p = l->plist[q];
-    qp[q].x = px[p]; qp[q].y = py[p]; qp[q].z = pz[p]; qp[q].w = 0.0f;
-  }
+  ahf_w9 *W = (ahf_w9 *)xm((size_t)(ne > 0 ? ne : 1) * sizeof(ahf_w9));
+  for (int64_t n = 0; n < nn; n++)
+    for (int64_t q = l->node_p0[n]; q < l->node_p0[n + 1]; q++) {
+      uint32_t p = l->plist[q];
+      ahf_entry_weights(&l->v, n, px[p], py[p], pz[p], l->m2pd, &W[q]);
+    }
   l->dens = (float *)xm((size_t)nn * 4);
   for (int64_t n = 0; n < nn; n++)
-    l->dens[n] = ahf_gather_dens_q(&l->v, l->pmask, l->node_p0, qp, descending, l->m2pd, n / 8, (int)(n % 8));
-  free(qp);
+    l->dens[n] = ahf_gather_dens_w(&l->v, l->pmask, l->node_p0, W, descending, n / 8, (int)(n % 8));
+  free(W);
 }
 
 int main(int argc, char **argv)

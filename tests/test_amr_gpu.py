@@ -21,43 +21,52 @@ def ctx():
     c.close()
 
 
-def _gpu_dump(ctx, snap_path, out_path):
+def _gpu_dump(ctx, snap_path, out_path, mode=0):
     s = dumps.read_snapshot(snap_path)
+    ctx.set_gather_mode(mode)
     ctx.set_particles(s.pos, s.mom)
     info = ctx.amr_build(s.L_dom, s.L_max, s.Nth_dom, s.Nth_ref)
     ctx.amr_dump(out_path)
     return info
 
 
-def _check(oracle, ctx, snap, tmp_path, tag):
+# every deposit kernel (per-source walk, flat walk, and the per-level choice)
+# must give the oracle's bits
+MODES = [0, 1, 2]
+
+
+def _check(oracle, ctx, snap, tmp_path, tag, mode=0):
     o_out = str(tmp_path / f"{tag}_oracle.grid")
     g_out = str(tmp_path / f"{tag}_gpu.grid")
     n = oracle.oracle_amr_dump(snap.encode(), o_out.encode())
     assert n > 0, f"oracle failed ({n})"
-    print(f"[amr-gpu] case {tag}: {os.path.getsize(snap)} B snapshot", flush=True)
-    info = _gpu_dump(ctx, snap, g_out)
+    print(f"[amr-gpu] case {tag} gather mode {mode}: {os.path.getsize(snap)} B snapshot", flush=True)
+    info = _gpu_dump(ctx, snap, g_out, mode)
     a, b = dumps.read_grids(g_out), dumps.read_grids(o_out)
     diffs = dumps.compare_grids(a, b)
     assert not diffs, f"{tag}: " + "; ".join(diffs[:6])
     assert info.n_levels == len(b)
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("cfg,side,seed", [(1, 32, 11), (1, 64, 1001), (0, 64, 1000), (1, 48, 7), (3, 48, 5)])
-def test_generated_configs(oracle, snapgen, ctx, tmp_path, cfg, side, seed):
+def test_generated_configs(oracle, snapgen, ctx, tmp_path, cfg, side, seed, mode):
     snap = str(tmp_path / f"cfg{cfg}_{side}.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
-    _check(oracle, ctx, snap, tmp_path, f"cfg{cfg}_{side}_{seed}")
+    _check(oracle, ctx, snap, tmp_path, f"cfg{cfg}_{side}_{seed}", mode)
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("seed", list(range(0, 40)))
-def test_fuzz(oracle, ctx, tmp_path, seed):
+def test_fuzz(oracle, ctx, tmp_path, seed, mode):
     snap = str(tmp_path / f"fuzz{seed}.snap")
     fuzz_snapshot(snap, seed)
-    _check(oracle, ctx, snap, tmp_path, f"fuzz{seed}")
+    _check(oracle, ctx, snap, tmp_path, f"fuzz{seed}", mode)
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("seed", list(range(40, 52)))
-def test_fuzz_non_pow2(oracle, ctx, tmp_path, seed):
+def test_fuzz_non_pow2(oracle, ctx, tmp_path, seed, mode):
     snap = str(tmp_path / f"fuzz{seed}.snap")
     fuzz_snapshot(snap, seed, non_pow2=True)
-    _check(oracle, ctx, snap, tmp_path, f"fuzz_np2_{seed}")
+    _check(oracle, ctx, snap, tmp_path, f"fuzz_np2_{seed}", mode)
