@@ -16,8 +16,9 @@
  *   scans +
  *   k_compact      octets of the next level, sorted (z,y,x)
  *   k_nbr_next     27-octet neighbour table of the next level
- *   k_relink_child particles to the first interior child containing them,
- *                  then a stable radix sort builds the fine buckets
+ *   k_relink_child particles to the first interior child containing them
+ *   k_fine_count,  the fine buckets, each filled from its mother's bucket
+ *   k_fine_fill    in bucket order
  *
  * The per-thread rules live in amr_core.h.  The domain level is binned by a
  * stable LSD radix sort (devprims.h) so every node's bucket is in particle
@@ -716,13 +717,11 @@ __global__ void k_relink_child(int64_t n_ent, const uint32_t *__restrict__ pnode
                                const uint32_t *__restrict__ plist, const int32_t *__restrict__ node2oct,
                                const uint64_t *__restrict__ key, const uint32_t *__restrict__ pmask2,
                                const float *__restrict__ px, const float *__restrict__ py,
-                               const float *__restrict__ pz, double Lf, int8_t *__restrict__ child,
-                               uint32_t *__restrict__ kout, uint32_t *__restrict__ vout)
+                               const float *__restrict__ pz, double Lf, int8_t *__restrict__ child)
 {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_ent) return;
   const uint32_t m = pnode[q], p = plist[q];
-  vout[q] = p;
   const int32_t o2 = node2oct[m];
   int ch = -1;
   if (o2 >= 0) {
@@ -738,7 +737,107 @@ __global__ void k_relink_child(int64_t n_ent, const uint32_t *__restrict__ pnode
     }
   }
   child[q] = (int8_t)ch;
-  kout[q] = ch >= 0 ? (uint32_t)(8 * (int64_t)o2 + ch) : 0xFFFFFFFFu;
+}
+
+/*
+ * Fine buckets without a sort: a particle only moves to a child of its own
+ * node, so fine node t = 8*o2 + c holds the entries of its mother's bucket
+ * whose child is c, in bucket order (AHF's head insertion during relink
+ * leaves them in this order relative to the walk).  One thread per fine
+ * node walks the mother's bucket: first to count, then, after a scan, to
+ * write; the 8 children of an octet read the same bucket.  Buckets of more
+ * than AHF_BIG_BUCKET entries are listed for k_fine_big, which walks them
+ * one wavefront at a time.
+ */
+#define AHF_BIG_BUCKET 256
+
+__global__ void k_fine_count(const int64_t *__restrict__ oct_parent, int64_t nn2, const int64_t *__restrict__ p0c,
+                             const int8_t *__restrict__ child, int64_t *__restrict__ cnt,
+                             uint32_t *__restrict__ big, uint32_t *__restrict__ n_big)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nn2) return;
+  const int64_t m = oct_parent[t >> 3];
+  const int8_t c = (int8_t)(t & 7);
+  const int64_t a = p0c[m], b = p0c[m + 1];
+  if (b - a > AHF_BIG_BUCKET) {
+    if (c == 0) big[atomicAdd(n_big, 1u)] = (uint32_t)(t >> 3);
+    return;
+  }
+  int64_t k = 0;
+  for (int64_t q = a; q < b; q++) k += child[q] == c;
+  cnt[t] = k;
+}
+
+__global__ void k_fine_fill(const int64_t *__restrict__ oct_parent, int64_t nn2, const int64_t *__restrict__ p0c,
+                            const int8_t *__restrict__ child, const uint32_t *__restrict__ plistc,
+                            const int64_t *__restrict__ p0f, uint32_t *__restrict__ plistf,
+                            uint32_t *__restrict__ pnodef)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nn2) return;
+  const int64_t m = oct_parent[t >> 3];
+  const int8_t c = (int8_t)(t & 7);
+  const int64_t a = p0c[m], b = p0c[m + 1];
+  if (b - a > AHF_BIG_BUCKET) return;
+  int64_t w = p0f[t];
+  for (int64_t q = a; q < b; q++)
+    if (child[q] == c) {
+      plistf[w] = plistc[q];
+      pnodef[w] = (uint32_t)t;
+      w++;
+    }
+}
+
+/* the listed big buckets, one wavefront per bucket: 64 entries per step,
+ * each ranked among the earlier entries of its child by ballots.  FILL =
+ * false writes the 8 counts, FILL = true the entries. */
+template <bool FILL>
+__global__ void __launch_bounds__(AHF_BLOCK) k_fine_big(const int64_t *__restrict__ oct_parent,
+                                                        const int64_t *__restrict__ p0c,
+                                                        const int8_t *__restrict__ child,
+                                                        const uint32_t *__restrict__ plistc,
+                                                        const uint32_t *__restrict__ big,
+                                                        const uint32_t *__restrict__ n_big,
+                                                        int64_t *__restrict__ cnt, const int64_t *__restrict__ p0f,
+                                                        uint32_t *__restrict__ plistf,
+                                                        uint32_t *__restrict__ pnodef)
+{
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t nb = (int64_t)*n_big;
+  for (int64_t i = gw; i < nb; i += nw) {
+    const int64_t o2 = big[i];
+    const int64_t m = oct_parent[o2];
+    const int64_t a = p0c[m], b = p0c[m + 1];
+    /* lane c < 8 keeps the running count of child c */
+    int64_t run = 0, base = 0;
+    if (FILL && lane < 8) base = p0f[8 * o2 + lane];
+    for (int64_t q0 = a; q0 < b; q0 += 64) {
+      const int64_t q = q0 + lane;
+      const int ch = q < b ? (int)child[q] : -1;
+      uint64_t bal[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) bal[k] = __ballot(ch == k);
+      /* every lane takes part in the shuffle: the first free slot of its child */
+      const int64_t slot = __shfl(base + run, ch >= 0 ? ch : 0, 64);
+      if (FILL && ch >= 0) {
+        uint64_t mine = bal[0];
+#pragma unroll
+        for (int k = 1; k < 8; k++) mine = ch == k ? bal[k] : mine;
+        const int64_t pos = slot + (int64_t)__popcll(mine & lt);
+        plistf[pos] = plistc[q];
+        pnodef[pos] = (uint32_t)(8 * o2 + ch);
+      }
+      uint64_t my = bal[0];
+#pragma unroll
+      for (int k = 1; k < 8; k++) my = lane == k ? bal[k] : my;
+      if (lane < 8) run += (int64_t)__popcll(my);
+    }
+    if (!FILL && lane < 8) cnt[8 * o2 + lane] = run;
+  }
 }
 
 /* neighbour table of a standalone level: binary search of the periodic
@@ -997,12 +1096,11 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   f.node_p0 = A.get<int64_t>(nn2 + 1);
   l.node2oct = A.get<int32_t>(nn);
   l.child = A.get<int8_t>(nent);
-  uint32_t *k0 = A.get<uint32_t>(nent), *v0 = A.get<uint32_t>(nent), *k1 = A.get<uint32_t>(nent),
-           *v1 = A.get<uint32_t>(nent);
-  uint32_t *rtmp = A.get<uint32_t>(radix_tmp_u32<uint32_t>(nent));
+  int64_t *fcnt = A.get<int64_t>(nn2);
+  uint32_t *fbig = A.get<uint32_t>(n2);
+  int64_t *ftmp = A.get<int64_t>(scan_tmp_elems(nn2));
   if (!f.oct_key || !f.oct_link || !f.oct_parent || !f.nbr || !f.row_key || !f.row_oct0 || !f.row_plane ||
-      !f.plane_z || !f.plane_row0 || !f.pmask || !f.node_p0 || !l.node2oct || !l.child || !k0 || !v0 || !k1 ||
-      !v1 || !rtmp)
+      !f.plane_z || !f.plane_row0 || !f.pmask || !f.node_p0 || !l.node2oct || !l.child || !fcnt || !fbig || !ftmp)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", ci + 1);
   hipEvent_t e1 = c->timer_begin();
   hipMemsetAsync(l.node2oct, 0xFF, (size_t)nn * sizeof(int32_t), c->st);
@@ -1024,17 +1122,24 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   level_view fv = make_view(f);
   LAUNCH(k_pmask, nn2, fv, f.pmask);
   c->timer_end(e1, TIMER_BUILD, 150.0 * (double)n2);
-  /* relink: child of every entry, then a stable sort by fine node */
+  /* relink: child of every entry, then the fine buckets */
   hipEvent_t e2 = c->timer_begin();
   LAUNCH(k_relink_child, nent, nent, l.pnode, l.plist, l.node2oct, l.oct_key, f.pmask, c->px, c->py, c->pz,
-         (double)f.L, l.child, k0, v0);
-  int bits = 1;
-  while ((1ll << bits) < nn2 + 1) bits++;
-  const int w = radix_sort<uint32_t, uint32_t>(k0, v0, k1, v1, nent, bits, rtmp, c->st);
-  f.pnode = w ? k1 : k0;
-  f.plist = w ? v1 : v0;
-  LAUNCH(k_csr_lower_bound, nn2 + 1, f.pnode, nent, nn2, f.node_p0);
-  c->timer_end(e2, TIMER_RELINK, 28.0 * (double)nent);
+         (double)f.L, l.child);
+  uint32_t *n_big = (uint32_t *)(c->d_anomaly + 2);
+  hipMemsetAsync(n_big, 0, sizeof(uint32_t), c->st);
+  LAUNCH(k_fine_count, nn2, f.oct_parent, nn2, l.node_p0, l.child, fcnt, fbig, n_big);
+  hipLaunchKernelGGL(k_fine_big<false>, dim3(512), dim3(AHF_BLOCK), 0, c->st, f.oct_parent, l.node_p0, l.child,
+                     l.plist, fbig, n_big, fcnt, nullptr, nullptr, nullptr);
+  exclusive_scan<int64_t, int64_t>(fcnt, f.node_p0, nn2, ftmp, c->st);
+  /* the fine level keeps at most the coarse level's entries */
+  f.plist = A.get<uint32_t>(nent);
+  f.pnode = A.get<uint32_t>(nent);
+  if (!f.plist || !f.pnode) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", ci + 1);
+  LAUNCH(k_fine_fill, nn2, f.oct_parent, nn2, l.node_p0, l.child, l.plist, f.node_p0, f.plist, f.pnode);
+  hipLaunchKernelGGL(k_fine_big<true>, dim3(512), dim3(AHF_BLOCK), 0, c->st, f.oct_parent, l.node_p0, l.child,
+                     l.plist, fbig, n_big, nullptr, f.node_p0, f.plist, f.pnode);
+  c->timer_end(e2, TIMER_RELINK, 29.0 * (double)nent + 24.0 * (double)nn2);
   /* the particles the fine level keeps: its deposit kernel depends on how
    * many entries each node holds */
   int64_t nent_f = 0;
