@@ -278,18 +278,81 @@ struct HaloOut {
   int64_t particle_iterations;
 };
 
+/* exclusive scans over a workgroup of BS threads (exclusive value taken
+ * from the neighbouring lane's inclusive value, block total in *tot) */
+template <int BS, typename T>
+__device__ inline T blk_excl_scan(T v, T *tot, T *sh /* [BS / 64] */)
+{
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T u = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += u;
+  }
+  T ex = __shfl_up(inc, 1, 64);
+  if (lane == 0) ex = 0;
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  T base = 0, t = 0;
+#pragma unroll
+  for (int k = 0; k < BS / 64; k++) {
+    const T s = sh[k];
+    if (k < w) base += s;
+    t += s;
+  }
+  __syncthreads();
+  *tot = t;
+  return base + ex;
+}
+
+template <int BS>
+__device__ inline D4 blk_excl_scan4(D4 v, D4 *tot, D4 *sh /* [BS / 64] */)
+{
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  D4 inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const D4 u = shfl_up4(inc, d);
+    if (lane >= d) {
+      inc.x += u.x; inc.y += u.y; inc.z += u.z; inc.m += u.m;
+    }
+  }
+  D4 ex = shfl_up4(inc, 1);
+  if (lane == 0) ex = D4{0, 0, 0, 0};
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  D4 base = {0, 0, 0, 0}, t = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < BS / 64; k++) {
+    const D4 s = sh[k];
+    if (k < w) { base.x += s.x; base.y += s.y; base.z += s.z; base.m += s.m; }
+    t.x += s.x; t.y += s.y; t.z += s.z; t.m += s.m;
+  }
+  __syncthreads();
+  *tot = t;
+  return D4{base.x + ex.x, base.y + ex.y, base.z + ex.z, base.m + ex.m};
+}
+
 /*
- * The unbinding of one halo by one workgroup.  a/b: two member buffers at
- * the halo's offset (sorted list in a); dist/phi: FP64 scratch; st: status.
+ * The unbinding of one halo by one workgroup of BS threads.  a/b: two member
+ * buffers at the halo's offset (sorted list in a); dist/phi: FP64 scratch;
+ * st: status.  Passes over the members go in chunks of BS*IPT: thread t
+ * takes IPT consecutive members, sums them sequentially, and the block scans
+ * the thread totals.
  */
+template <int BS, int IPT>
 __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint32_t *b, double *dist,
                             double *phi, uint8_t *st, const double cen[3], const float *__restrict__ px,
                             const float *__restrict__ py, const float *__restrict__ pz,
                             const float *__restrict__ vx, const float *__restrict__ vy,
                             const float *__restrict__ vz, HaloOut *out)
 {
-  __shared__ D4 sh4[AHF_WAVES];
-  __shared__ double shd[AHF_WAVES];
+  constexpr int NW = BS / 64;
+  constexpr int64_t CH = (int64_t)BS * IPT;
+  __shared__ D4 sh4[NW];
+  __shared__ double shd[NW];
+  __shared__ int64_t shi[NW];
   __shared__ int sflag[4];
   __shared__ double sPhi0;
   __shared__ int64_t s_seed;
@@ -313,7 +376,7 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
     niter++;
     pit += n;
     /* pass 1: distances */
-    for (int64_t j = t; j < n; j += AHF_BLOCK) {
+    for (int64_t j = t; j < n; j += BS) {
       const uint32_t p = cur[j];
       const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
                    dZ = wrap_s((double)pz[p] - cen[2]);
@@ -322,25 +385,35 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
     __syncthreads();
     /* pass 2: mid-point terms and their prefix sum Phi_j */
     double carry = 0.0;
-    for (int64_t base = 0; base < n; base += AHF_BLOCK) {
-      const int64_t j = base + t;
-      double term = 0.0;
-      if (j < n) {
-        const double dj = dist[j];
-        if (dj > MZERO) {
-          const double I_now = (double)(j + 1) / (dj * dj);
-          double I_prev = 0.0, d_prev = 0.0;
-          if (j > 0) {
-            d_prev = dist[j - 1];
-            if (d_prev > MZERO) I_prev = (double)j / (d_prev * d_prev);
+    for (int64_t base = 0; base < n; base += CH) {
+      const int64_t j0 = base + (int64_t)t * IPT;
+      double loc[IPT];
+      double run = 0.0;
+#pragma unroll
+      for (int i = 0; i < IPT; i++) {
+        const int64_t j = j0 + i;
+        double term = 0.0;
+        if (j < n) {
+          const double dj = dist[j];
+          if (dj > MZERO) {
+            const double I_now = (double)(j + 1) / (dj * dj);
+            double I_prev = 0.0, d_prev = 0.0;
+            if (j > 0) {
+              d_prev = dist[j - 1];
+              if (d_prev > MZERO) I_prev = (double)j / (d_prev * d_prev);
+            }
+            const double I_mid = (I_now + I_prev) / 2.;
+            term = I_mid * (dj - d_prev);
           }
-          const double I_mid = (I_now + I_prev) / 2.;
-          term = I_mid * (dj - d_prev);
         }
+        run += term;
+        loc[i] = run;
       }
       double tot;
-      const double ex = block_excl_scan<double>(term, &tot);
-      if (j < n) phi[j] = carry + ex + term;
+      const double ex = blk_excl_scan<BS, double>(run, &tot, shd);
+#pragma unroll
+      for (int i = 0; i < IPT; i++)
+        if (j0 + i < n) phi[j0 + i] = carry + ex + loc[i];
       carry += tot;
     }
     __syncthreads();
@@ -362,51 +435,67 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
         s_seed = 0;
       }
     }
-    for (int64_t j = t; j < n; j += AHF_BLOCK) st[j] = 1;
+    for (int64_t j = t; j < n; j += BS) st[j] = 1;
     __syncthreads();
     Phi0 = sPhi0;
     const uint32_t ps = cur[s_seed];
     const double sx = (double)vx[ps], sy = (double)vy[ps], sz = (double)vz[ps];
-    /* sweeps to the fixed point of the sequential bound-set recurrence */
+    /* sweeps to the fixed point of the sequential bound-set recurrence: a
+     * member's test uses the statuses of the previous sweep for the earlier
+     * members of its chunk and this sweep's for earlier chunks; at the fixed
+     * point every test sees the final statuses, the sequential answer */
     int uncertain = 0;
     for (int sweep = 0;; sweep++) {
       if (t == 0) { sflag[0] = 0; sflag[1] = 0; }
       __syncthreads();
       D4 cv = {sx, sy, sz, 1.0};
-      for (int64_t base = 0; base < n; base += AHF_BLOCK) {
-        const int64_t j = base + t;
-        const int bj = j < n ? st[j] : 0;
-        uint32_t p = 0;
-        double fx = 0, fy = 0, fz = 0;
-        if (j < n) {
-          p = cur[j];
-          fx = (double)vx[p]; fy = (double)vy[p]; fz = (double)vz[p];
+      for (int64_t base = 0; base < n; base += CH) {
+        const int64_t j0 = base + (int64_t)t * IPT;
+        double fx[IPT], fy[IPT], fz[IPT];
+        int bj[IPT];
+        D4 run = {0, 0, 0, 0};
+        D4 lex[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+          const int64_t j = j0 + i;
+          bj[i] = j < n ? st[j] : 0;
+          fx[i] = fy[i] = fz[i] = 0.0;
+          if (j < n) {
+            const uint32_t p = cur[j];
+            fx[i] = (double)vx[p]; fy[i] = (double)vy[p]; fz[i] = (double)vz[p];
+          }
+          lex[i] = run;
+          if (bj[i]) { run.x += fx[i]; run.y += fy[i]; run.z += fz[i]; run.m += 1.0; }
         }
-        D4 c = {bj ? fx : 0.0, bj ? fy : 0.0, bj ? fz : 0.0, bj ? 1.0 : 0.0};
         D4 tot;
-        D4 ex = block_excl_scan4(c, &tot, sh4);
-        int nb = bj;
-        if (j < n) {
-          const double VX = cv.x + ex.x, VY = cv.y + ex.y, VZ = cv.z + ex.z, M = cv.m + ex.m;
+        const D4 ex = blk_excl_scan4<BS>(run, &tot, sh4);
+        D4 nrun = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+          const int64_t j = j0 + i;
+          if (j >= n) continue;
+          const uint32_t p = cur[j];
+          const double VX = cv.x + ex.x + lex[i].x, VY = cv.y + ex.y + lex[i].y, VZ = cv.z + ex.z + lex[i].z,
+                       M = cv.m + ex.m + lex[i].m;
           const double dj = dist[j];
           double v_esc2;
           if (dj > MZERO) v_esc2 = (2 * fabs(phi[j] - Phi0) * K.phi_fac);
           else v_esc2 = 1e30;
           const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
                        dZ = wrap_s((double)pz[p] - cen[2]);
-          const double dVX = (fx - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
-          const double dVY = (fy - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
-          const double dVZ = (fz - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
+          const double dVX = (fx[i] - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
+          const double dVY = (fy[i] - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
+          const double dVZ = (fz[i] - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
           const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
           const double lim = K.v2_tune * v_esc2;
-          nb = !(vel2 > lim);
+          const int nb = !(vel2 > lim);
           if (v_esc2 < 1e30 && fabs(vel2 - lim) <= K.band * lim) sflag[1] = 1;
-          if (nb != bj) { st[j] = (uint8_t)nb; sflag[0] = 1; }
+          if (nb != bj[i]) { st[j] = (uint8_t)nb; sflag[0] = 1; }
+          if (nb) { nrun.x += fx[i]; nrun.y += fy[i]; nrun.z += fz[i]; nrun.m += 1.0; }
         }
         /* later chunks see the updated statuses of this chunk */
-        D4 nv = {nb && j < n ? fx : 0.0, nb && j < n ? fy : 0.0, nb && j < n ? fz : 0.0, nb && j < n ? 1.0 : 0.0};
         D4 tot2;
-        (void)block_excl_scan4(nv, &tot2, sh4);
+        (void)blk_excl_scan4<BS>(nrun, &tot2, sh4);
         cv.x += tot2.x; cv.y += tot2.y; cv.z += tot2.z; cv.m += tot2.m;
       }
       __syncthreads();
@@ -476,17 +565,27 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
     if (t == 0) s_last = 0ull;
     __syncthreads();
     int64_t w0 = 0;
-    for (int64_t base = 0; base < n; base += AHF_BLOCK) {
-      const int64_t j = base + t;
-      const int bj = j < n ? st[j] : 0;
+    unsigned long long last = 0ull;
+    for (int64_t base = 0; base < n; base += CH) {
+      const int64_t j0 = base + (int64_t)t * IPT;
+      int64_t cnt = 0;
+#pragma unroll
+      for (int i = 0; i < IPT; i++)
+        if (j0 + i < n && st[j0 + i]) cnt++;
       int64_t tot;
-      const int64_t ex = block_excl_scan<int64_t>((int64_t)bj, &tot);
-      if (bj) {
-        nxt[w0 + ex] = cur[j];
-        atomicMax(&s_last, (unsigned long long)(j + 1));
+      const int64_t ex = blk_excl_scan<BS, int64_t>(cnt, &tot, shi);
+      int64_t w = w0 + ex;
+#pragma unroll
+      for (int i = 0; i < IPT; i++) {
+        const int64_t j = j0 + i;
+        if (j < n && st[j]) {
+          nxt[w++] = cur[j];
+          last = (unsigned long long)(j + 1);
+        }
       }
       w0 += tot;
     }
+    if (last) atomicMax(&s_last, last);
     __syncthreads();
     if (t == 0 && s_last > 0) s_rvir = dist[s_last - 1];
     __syncthreads();
@@ -517,22 +616,28 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
   }
 }
 
-/* persistent form: workgroups take haloes from the queue (largest first) */
-__global__ void __launch_bounds__(AHF_BLOCK) k_unbind_batched(UnbindConst K, const int64_t *__restrict__ off,
-                                                              const int32_t *__restrict__ order, int64_t nh,
-                                                              int32_t *__restrict__ next, uint32_t *a, uint32_t *b,
-                                                              double *dist, double *phi, uint8_t *st,
-                                                              const double *__restrict__ cen,
-                                                              const float *__restrict__ px, const float *__restrict__ py,
-                                                              const float *__restrict__ pz, const float *__restrict__ vx,
-                                                              const float *__restrict__ vy, const float *__restrict__ vz,
-                                                              HaloOut *__restrict__ out)
+/* haloes of at least this many members go to 1024-thread workgroups */
+#define AHF_BIG_HALO 2048
+#define AHF_BIG_BLOCK 1024
+#define AHF_UNBIND_IPT 4
+
+/* persistent form: workgroups take haloes order[q0..q1) from the queue
+ * (largest first) */
+template <int BS>
+__global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int64_t *__restrict__ off,
+                                                       const int32_t *__restrict__ order, int64_t q0, int64_t q1,
+                                                       int32_t *__restrict__ next, uint32_t *a, uint32_t *b,
+                                                       double *dist, double *phi, uint8_t *st,
+                                                       const double *__restrict__ cen, const float *__restrict__ px,
+                                                       const float *__restrict__ py, const float *__restrict__ pz,
+                                                       const float *__restrict__ vx, const float *__restrict__ vy,
+                                                       const float *__restrict__ vz, HaloOut *__restrict__ out)
 {
   __shared__ int64_t sh_h;
   for (;;) {
     if (threadIdx.x == 0) {
-      const int32_t q = atomicAdd(next, 1);
-      sh_h = q < nh ? order[q] : -1;
+      const int64_t q = q0 + atomicAdd(next, 1);
+      sh_h = q < q1 ? order[q] : -1;
     }
     __syncthreads();
     const int64_t h = sh_h;
@@ -540,26 +645,28 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_unbind_batched(UnbindConst K, con
     if (h < 0) return;
     const int64_t o = off[h], n = off[h + 1] - o;
     const double c3[3] = {cen[3 * h], cen[3 * h + 1], cen[3 * h + 2]};
-    unbind_halo(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz, out + h);
+    unbind_halo<BS, AHF_UNBIND_IPT>(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz,
+                                    out + h);
     __syncthreads();
   }
 }
 
 /* plain form: one halo per launch */
-__global__ void __launch_bounds__(AHF_BLOCK) k_unbind_one(UnbindConst K, int64_t h, const int64_t *__restrict__ off,
-                                                          uint32_t *a, uint32_t *b, double *dist, double *phi,
-                                                          uint8_t *st, const double *__restrict__ cen,
-                                                          const float *__restrict__ px, const float *__restrict__ py,
-                                                          const float *__restrict__ pz, const float *__restrict__ vx,
-                                                          const float *__restrict__ vy, const float *__restrict__ vz,
-                                                          HaloOut *__restrict__ out)
+template <int BS>
+__global__ void __launch_bounds__(BS) k_unbind_one(UnbindConst K, int64_t h, const int64_t *__restrict__ off,
+                                                   uint32_t *a, uint32_t *b, double *dist, double *phi,
+                                                   uint8_t *st, const double *__restrict__ cen,
+                                                   const float *__restrict__ px, const float *__restrict__ py,
+                                                   const float *__restrict__ pz, const float *__restrict__ vx,
+                                                   const float *__restrict__ vy, const float *__restrict__ vz,
+                                                   HaloOut *__restrict__ out)
 {
   const int64_t o = off[h], n = off[h + 1] - o;
   const double c3[3] = {cen[3 * h], cen[3 * h + 1], cen[3 * h + 2]};
-  unbind_halo(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz, out + h);
+  unbind_halo<BS, AHF_UNBIND_IPT>(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz,
+                                  out + h);
 }
 
-/* sort-only calls: every halo keeps all its members */
 __global__ void k_sorted_out(const int64_t *__restrict__ off, int64_t nh, HaloOut *__restrict__ out)
 {
   const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -720,7 +827,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   double *dist = A.get<double>(ne), *phi = A.get<double>(ne);
   uint8_t *st = A.get<uint8_t>(ne);
   s->out = A.get<HaloOut>(nh);
-  int32_t *next = A.get<int32_t>(1);
+  int32_t *next = A.get<int32_t>(2);
   if (!k0 || !k1 || !v0 || !v1 || !hk0 || !hk1 || !rtmp || !s->buf_a || !s->buf_b || !dist || !phi || !st ||
       !s->out || !next)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (unbinding)");
@@ -767,17 +874,38 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   if (sort_only) {
     if (nh) hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, s->out);
   } else if (run == 0) {
-    hipMemsetAsync(next, 0, sizeof(int32_t), c->st);
-    const int64_t grid = std::min<int64_t>(nh > 0 ? nh : 1, (int64_t)dev_cu * 8);
-    hipLaunchKernelGGL(k_unbind_batched, dim3((unsigned)grid), dim3(AHF_BLOCK), 0, c->st, K, s->off, s->order, nh,
-                       next, s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
-                       s->out);
+    /* the order is largest first: haloes of AHF_BIG_HALO members or more
+     * get 1024-thread workgroups, the rest 256 */
+    int64_t nbig = 0;
+    {
+      std::vector<int32_t> ord((size_t)nh);
+      if (nh) hipMemcpy(ord.data(), s->order, (size_t)nh * sizeof(int32_t), hipMemcpyDeviceToHost);
+      while (nbig < nh && s->h_off[ord[nbig] + 1] - s->h_off[ord[nbig]] >= AHF_BIG_HALO) nbig++;
+    }
+    hipMemsetAsync(next, 0, 2 * sizeof(int32_t), c->st);
+    if (nbig > 0) {
+      const int64_t grid = std::min<int64_t>(nbig, (int64_t)dev_cu * 2);
+      hipLaunchKernelGGL(k_unbind_batched<AHF_BIG_BLOCK>, dim3((unsigned)grid), dim3(AHF_BIG_BLOCK), 0, c->st, K,
+                         s->off, s->order, (int64_t)0, nbig, next, s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px,
+                         c->py, c->pz, c->vx, c->vy, c->vz, s->out);
+    }
+    if (nh > nbig) {
+      const int64_t grid = std::min<int64_t>(nh - nbig, (int64_t)dev_cu * 8);
+      hipLaunchKernelGGL(k_unbind_batched<AHF_BLOCK>, dim3((unsigned)grid), dim3(AHF_BLOCK), 0, c->st, K, s->off,
+                         s->order, nbig, nh, next + 1, s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px, c->py,
+                         c->pz, c->vx, c->vy, c->vz, s->out);
+    }
   } else {
     /* plain form: one halo per launch, results synchronised per halo as a
      * per-call rem_unbound() would need them */
     for (int64_t h = 0; h < nh; h++) {
-      hipLaunchKernelGGL(k_unbind_one, dim3(1), dim3(AHF_BLOCK), 0, c->st, K, h, s->off, s->buf_a, s->buf_b, dist,
-                         phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz, s->out);
+      if (s->h_off[h + 1] - s->h_off[h] >= AHF_BIG_HALO)
+        hipLaunchKernelGGL(k_unbind_one<AHF_BIG_BLOCK>, dim3(1), dim3(AHF_BIG_BLOCK), 0, c->st, K, h, s->off,
+                           s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
+                           s->out);
+      else
+        hipLaunchKernelGGL(k_unbind_one<AHF_BLOCK>, dim3(1), dim3(AHF_BLOCK), 0, c->st, K, h, s->off, s->buf_a,
+                           s->buf_b, dist, phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz, s->out);
       hipStreamSynchronize(c->st);
     }
   }
