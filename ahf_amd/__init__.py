@@ -154,7 +154,7 @@ class Context:
         self._check(self.lib.ahfgpu_set_halos(self.h, self._cen.ctypes.data, self._hnp.ctypes.data,
                                               ip.ctypes.data, self._hnp.shape[0]))
 
-    def unbind(self, minpart, vtune, r_fac, v_fac, phi_fac, hubble, mode=0, band=1e-4):
+    def unbind(self, minpart, vtune, r_fac, v_fac, phi_fac, hubble, mode=0, band=0.0):
         p = UnbindParams(minpart, 0, vtune, r_fac, v_fac, phi_fac, hubble)
         info = UnbindInfo()
         self._check(self.lib.ahfgpu_unbind(self.h, ctypes.byref(p), mode, band, ctypes.byref(info)))

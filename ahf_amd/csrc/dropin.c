@@ -474,7 +474,7 @@ static int run_halos(HALO *h, long unsigned nh, int mode)
   }
   ahfgpu_unbind_params up = {simu.AHF_MINPART, 0, simu.AHF_VTUNE, r_fac, v_fac, phi_fac, Hubble};
   int rc = ahfgpu_set_halos(c, cen, np, ip, nh);
-  if (!rc) rc = ahfgpu_unbind(c, &up, mode, 1e-4, NULL);
+  if (!rc) rc = ahfgpu_unbind(c, &up, mode, 0.0, NULL);
   double *props = (double *)xcalloc(5 * nh, 8);
   if (!rc) rc = ahfgpu_unbind_results(c, np, props, ip);
   if (!rc) {

@@ -18,10 +18,14 @@
  *        solved by sweeps that assume a bound set, prefix-sum the velocities
  *        of that set and re-test; the sweeps stop at the fixed point, which
  *        is the sequential answer;
- *      - a member whose vel^2 lies within `band` (relative) of
- *        VescTune^2 v_esc^2 is decided by an exact sequential replay of the
- *        iteration with AHF's own operation order (rounding of the parallel
- *        sums can only matter there);
+ *      - the parallel sums round differently from AHF's running sums, by at
+ *        most gam_n * sum|terms| (any order of summation); carried through
+ *        the escape test this bounds, per member, how far the sequential
+ *        vel^2 and limit can lie from the parallel ones.  A member whose
+ *        margin is inside that bound (or inside `band`, relative, if the
+ *        caller asks for more) sends the halo to an exact sequential replay
+ *        of the iteration in AHF's own operation order; every other
+ *        decision is the sequential one;
  *      - stable compaction of the bound members.
  */
 #include <hip/hip_runtime.h>
@@ -375,13 +379,25 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
   while (nremove > 3) {
     niter++;
     pit += n;
-    /* pass 1: distances */
+    /* pass 1: distances, and the sums of |v| that bound the rounding of the
+     * bulk-velocity sums */
+    D4 av = {0, 0, 0, 0};
     for (int64_t j = t; j < n; j += BS) {
       const uint32_t p = cur[j];
       const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
                    dZ = wrap_s((double)pz[p] - cen[2]);
       dist[j] = sqrt(dX * dX + dY * dY + dZ * dZ);
+      av.x += fabs((double)vx[p]);
+      av.y += fabs((double)vy[p]);
+      av.z += fabs((double)vz[p]);
     }
+    D4 asum;
+    (void)blk_excl_scan4<BS>(av, &asum, sh4);
+    /* any order of summing m terms is within gam*sum|terms| of the exact sum
+     * (gam = m u / (1 - m u), u = 2^-53); a few terms more cover the adds
+     * around the scans and the seed counted twice */
+    const double Nd = (double)n + 8.0, U = 1.1102230246251565e-16;
+    const double gam = Nd * U / (1.0 - Nd * U);
     __syncthreads();
     /* pass 2: mid-point terms and their prefix sum Phi_j */
     double carry = 0.0;
@@ -489,7 +505,23 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
           const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
           const double lim = K.v2_tune * v_esc2;
           const int nb = !(vel2 > lim);
-          if (v_esc2 < 1e30 && fabs(vel2 - lim) <= K.band * lim) sflag[1] = 1;
+          if (v_esc2 < 1e30) {
+            /* certified bound on how far the sequential vel^2 and limit can
+             * lie from these: Phi_j and Phi0 are sums of non-negative terms
+             * (error <= gam*Phi0 each), the velocity sums are within
+             * gam*2*sum|v| (the seed is counted twice); propagated through
+             * the test, plus a few ulps per operation */
+            const double e_phi = 2.5 * gam * fabs(Phi0);
+            const double e_lim = K.v2_tune * 4.0 * K.phi_fac * e_phi;
+            const double e_vx = fabs(K.v_fac) * 5.0 * gam * asum.x / M;
+            const double e_vy = fabs(K.v_fac) * 5.0 * gam * asum.y / M;
+            const double e_vz = fabs(K.v_fac) * 5.0 * gam * asum.z / M;
+            const double e_vel2 = 2.0 * (fabs(dVX) * e_vx + fabs(dVY) * e_vy + fabs(dVZ) * e_vz) + e_vx * e_vx +
+                                  e_vy * e_vy + e_vz * e_vz;
+            const double tol = 4.0 * (e_vel2 + e_lim) + 1e-14 * (vel2 + lim);
+            const double gap = fabs(vel2 - lim);
+            if (gap <= tol || gap <= K.band * lim) sflag[1] = 1;
+          }
           if (nb != bj[i]) { st[j] = (uint8_t)nb; sflag[0] = 1; }
           if (nb) { nrun.x += fx[i]; nrun.y += fy[i]; nrun.z += fz[i]; nrun.m += 1.0; }
         }

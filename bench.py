@@ -143,7 +143,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the one-halo-per-launch form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU timing")
-    ap.add_argument("--band", type=float, default=1e-4)
+    ap.add_argument("--band", type=float, default=0.0,
+                    help="extra relative band sent to the exact replay (0: the certified bound only)")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -142,10 +142,12 @@ int ahfgpu_set_halos(ahfgpu_ctx *ctx, const double *centres, const uint64_t *npa
 /* radial sort + unbinding of every resident halo.  mode 0: all haloes in one
  * persistent launch (size-ordered work queue); mode 1: one launch per halo,
  * synchronised per halo (the per-call form); or-ed with SORT_ONLY or
- * NO_SORT.  band: members whose
- * vel^2 lies within band*VescTune^2*v_esc^2 of the threshold are decided by
- * an exact sequential replay (1e-4 recommended).  The resident input lists
- * are not modified, so repeated calls redo the same work. */
+ * NO_SORT.  A member whose bound test could go the other way under AHF's
+ * sequential rounding (certified error bound of the parallel sums) is
+ * decided by an exact sequential replay of the iteration; band > 0 widens
+ * that to every member within band*VescTune^2*v_esc^2 of the threshold (0:
+ * the certified bound only).  The resident input lists are not modified, so
+ * repeated calls redo the same work. */
 int ahfgpu_unbind(ahfgpu_ctx *ctx, const ahfgpu_unbind_params *p, int mode, double band,
                   ahfgpu_unbind_info *info);
 

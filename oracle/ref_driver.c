@@ -18,7 +18,7 @@
  * Usage:
  *   ahf_ref amr    <snap> <grids.out> [--reps R]   dump the AMR hierarchy
  *   ahf_ref unbind <snap> <halos.out> [--threads T] [--reps R]
- *   ahf_ref time   <snap> [--threads T] [--max-halo-part N]
+ *   ahf_ref time   <snap> [--threads T] [--reps R] [--skip-amr] [--skip-1thread]
  *   ahf_ref layout                                  struct sizes/offsets (JSON)
  */
 #define _POSIX_C_SOURCE 200809L
@@ -324,12 +324,12 @@ static void cmd_unbind_batch(const snap_data *s, const char *out)
 
 /* -------------------------------------------------------------- time ---- */
 
-static void cmd_time(const snap_data *s, int threads, int reps)
+static void cmd_time(const snap_data *s, int threads, int reps, int skip_amr, int skip_1thread)
 {
   setup_globals(s);
-  double best_amr = 1e30, best_unb = 1e30, best_unb1 = 1e30;
+  double best_amr = skip_amr ? 0.0 : 1e30, best_unb = 1e30, best_unb1 = skip_1thread ? 0.0 : 1e30;
   int no_grids = 0;
-  for (int r = 0; r < reps; r++) {
+  for (int r = 0; r < reps && !skip_amr; r++) {
     setup_globals(s);
     partptr fst = make_particles(s);
     gridls *gl;
@@ -344,6 +344,7 @@ static void cmd_time(const snap_data *s, int threads, int reps)
     HALO *h = make_halos(s);
     double t = run_unbind(h, s->h.nhalos, threads);
     if (t < best_unb) best_unb = t;
+    if (skip_1thread) continue;
     HALO *h1 = make_halos(s);
     double t1 = run_unbind(h1, s->h.nhalos, 1);
     if (t1 < best_unb1) best_unb1 = t1;
@@ -395,10 +396,12 @@ int main(int argc, char **argv)
     return 0;
   }
   if (argc < 3) die("usage: ahf_ref amr|unbind|time|layout <snap> [out] [options]");
-  int threads = 0, reps = 1;
+  int threads = 0, reps = 1, skip_amr = 0, skip_1thread = 0;
   for (int i = 3; i < argc; i++) {
     if (!strcmp(argv[i], "--threads") && i + 1 < argc) threads = atoi(argv[++i]);
     else if (!strcmp(argv[i], "--reps") && i + 1 < argc) reps = atoi(argv[++i]);
+    else if (!strcmp(argv[i], "--skip-amr")) skip_amr = 1;
+    else if (!strcmp(argv[i], "--skip-1thread")) skip_1thread = 1;
   }
   snap_data s;
   int rc = snap_read(argv[2], &s);
@@ -418,7 +421,7 @@ int main(int argc, char **argv)
     cmd_unbind_batch(&s, argv[3]);
 #endif
   } else if (!strcmp(argv[1], "time")) {
-    cmd_time(&s, threads, reps < 1 ? 1 : reps);
+    cmd_time(&s, threads, reps < 1 ? 1 : reps, skip_amr, skip_1thread);
   } else {
     die("unknown command");
   }

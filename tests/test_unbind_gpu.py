@@ -17,10 +17,10 @@ def ctx():
     c.close()
 
 
-def _run_gpu(ctx, s, mode):
+def _run_gpu(ctx, s, mode, band=0.0):
     ctx.set_particles(s.pos, s.mom)
     ctx.set_halos(s.centre, s.halo_npart, s.ipart)
-    info = ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=mode)
+    info = ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=mode, band=band)
     return info, ctx.unbind_results()
 
 
@@ -51,6 +51,21 @@ def test_unbind_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, 
     info, (npart, props, lists) = _run_gpu(ctx, s, mode)
     _compare(ref, npart, props, lists, f"cfg{cfg}/{side}/mode{mode}")
     assert info.n_member_out == int(ref.npart.sum())
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_unbind_replay_band(oracle, snapgen, ctx, tmp_path, mode):
+    """a wide band sends haloes with members near the threshold to the exact
+    sequential replay: the results must still be the oracle's"""
+    snap = str(tmp_path / "b.snap")
+    assert snapgen.ahf_snapgen_write(2, 2000, 1002, snap.encode()) == 0
+    out = str(tmp_path / "o.halo")
+    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 1) == 0
+    ref = dumps.read_halos(out)
+    s = dumps.read_snapshot(snap)
+    info, (npart, props, lists) = _run_gpu(ctx, s, mode, band=1e-2)
+    assert info.n_exact_replays > 0
+    _compare(ref, npart, props, lists, f"band/mode{mode}")
 
 
 def test_unbind_repeatable(snapgen, ctx, tmp_path):
