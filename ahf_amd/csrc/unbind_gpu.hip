@@ -280,6 +280,7 @@ struct HaloOut {
   int32_t in_b; /* final list is in buffer b (0 = a) */
   int32_t rvir_set;
   int64_t particle_iterations;
+  int32_t sweeps, pad;
 };
 
 /* exclusive scans over a workgroup of BS threads (exclusive value taken
@@ -345,9 +346,17 @@ __device__ inline D4 blk_excl_scan4(D4 v, D4 *tot, D4 *sh /* [BS / 64] */)
  * takes IPT consecutive members, sums them sequentially, and the block scans
  * the thread totals.
  */
+/* member data of one iteration, gathered once into the halo's scratch in
+ * list order: periodic offsets from the centre (FP64) and velocities */
+struct Staged {
+  double *dx, *dy, *dz;
+  float *vx, *vy, *vz;
+};
+
 template <int BS, int IPT>
-__device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint32_t *b, double *dist,
-                            double *phi, uint8_t *st, const double cen[3], const float *__restrict__ px,
+__device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uint32_t *a, uint32_t *b,
+                            double *dist, double *phi, uint8_t *st, Staged g, const double cen[3],
+                            const float *__restrict__ px,
                             const float *__restrict__ py, const float *__restrict__ pz,
                             const float *__restrict__ vx, const float *__restrict__ vy,
                             const float *__restrict__ vz, HaloOut *out)
@@ -365,13 +374,14 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
   const int t = threadIdx.x;
   int64_t n = n0;
   uint32_t *cur = a, *nxt = b;
-  int niter = 0, replays = 0, in_b = 0, rvir_set = 0;
+  int niter = 0, replays = 0, in_b = 0, rvir_set = 0, sweeps = 0;
   double R_vir = 0.0, Phi0 = 0.0, M_vir = (double)n;
   int64_t pit = 0;
   if (n < K.minpart) {
     if (t == 0) {
       out->npart = n; out->M_vir = 0.0; out->R_vir = 0.0; out->Phi0 = 0.0;
       out->niter = 0; out->replays = 0; out->in_b = 0; out->rvir_set = 0; out->particle_iterations = 0;
+      out->sweeps = 0;
     }
     return;
   }
@@ -382,14 +392,34 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
     /* pass 1: distances, and the sums of |v| that bound the rounding of the
      * bulk-velocity sums */
     D4 av = {0, 0, 0, 0};
-    for (int64_t j = t; j < n; j += BS) {
-      const uint32_t p = cur[j];
-      const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
-                   dZ = wrap_s((double)pz[p] - cen[2]);
-      dist[j] = sqrt(dX * dX + dY * dY + dZ * dZ);
-      av.x += fabs((double)vx[p]);
-      av.y += fabs((double)vy[p]);
-      av.z += fabs((double)vz[p]);
+    /* eight members per thread at a time: their particle gathers are
+     * independent and overlap */
+    for (int64_t base = 0; base < n; base += 8 * (int64_t)BS) {
+      uint32_t pp[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int64_t j = base + (int64_t)i * BS + t;
+        pp[i] = j < n ? cur[j] : 0u;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int64_t j = base + (int64_t)i * BS + t;
+        if (j >= n) continue;
+        const uint32_t p = pp[i];
+        const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
+                     dZ = wrap_s((double)pz[p] - cen[2]);
+        const float ux = vx[p], uy = vy[p], uz = vz[p];
+        dist[j] = sqrt(dX * dX + dY * dY + dZ * dZ);
+        g.dx[j] = dX;
+        g.dy[j] = dY;
+        g.dz[j] = dZ;
+        g.vx[j] = ux;
+        g.vy[j] = uy;
+        g.vz[j] = uz;
+        av.x += fabs((double)ux);
+        av.y += fabs((double)uy);
+        av.z += fabs((double)uz);
+      }
     }
     D4 asum;
     (void)blk_excl_scan4<BS>(av, &asum, sh4);
@@ -437,16 +467,7 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
     /* seed of the bulk velocity (:3441-3470) */
     if (t == 0) {
       if (niter == 1) {
-        double m2[65];
-        int ix[65];
-        const int nv = K.no_vbulk;
-        for (int j = 0; j < nv; j++) {
-          const uint32_t p = cur[j];
-          const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
-          m2[j + 1] = (double)v2;
-        }
-        dev_indexx(nv, m2, ix);
-        s_seed = ix[nv / 2] - 1;
+        s_seed = seed0;
       } else {
         s_seed = 0;
       }
@@ -454,51 +475,54 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
     for (int64_t j = t; j < n; j += BS) st[j] = 1;
     __syncthreads();
     Phi0 = sPhi0;
-    const uint32_t ps = cur[s_seed];
-    const double sx = (double)vx[ps], sy = (double)vy[ps], sz = (double)vz[ps];
-    /* sweeps to the fixed point of the sequential bound-set recurrence: a
-     * member's test uses the statuses of the previous sweep for the earlier
-     * members of its chunk and this sweep's for earlier chunks; at the fixed
-     * point every test sees the final statuses, the sequential answer */
+    const double sx = (double)g.vx[s_seed], sy = (double)g.vy[s_seed], sz = (double)g.vz[s_seed];
+    /* the sequential bound-set recurrence, chunk by chunk: a chunk is
+     * evaluated with the final statuses of every earlier chunk (carried in
+     * cv) and its own statuses of the previous evaluation, and evaluated
+     * again until they no longer change; then it holds the sequential answer
+     * and its totals carry to the next chunk.  Only the final evaluation's
+     * margins count for the exact replay. */
     int uncertain = 0;
-    for (int sweep = 0;; sweep++) {
-      if (t == 0) { sflag[0] = 0; sflag[1] = 0; }
-      __syncthreads();
-      D4 cv = {sx, sy, sz, 1.0};
-      for (int64_t base = 0; base < n; base += CH) {
-        const int64_t j0 = base + (int64_t)t * IPT;
-        double fx[IPT], fy[IPT], fz[IPT];
+    D4 cv = {sx, sy, sz, 1.0};
+    for (int64_t base = 0; base < n; base += CH) {
+      const int64_t j0 = base + (int64_t)t * IPT;
+      double fx[IPT], fy[IPT], fz[IPT];
+#pragma unroll
+      for (int i = 0; i < IPT; i++) {
+        const int64_t j = j0 + i;
+        fx[i] = fy[i] = fz[i] = 0.0;
+        if (j < n) {
+          fx[i] = (double)g.vx[j]; fy[i] = (double)g.vy[j]; fz[i] = (double)g.vz[j];
+        }
+      }
+      for (;;) {
+        sweeps++;
+        if (t == 0) { sflag[0] = 0; sflag[1] = 0; }
+        __syncthreads();
         int bj[IPT];
         D4 run = {0, 0, 0, 0};
-        D4 lex[IPT];
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
           const int64_t j = j0 + i;
           bj[i] = j < n ? st[j] : 0;
-          fx[i] = fy[i] = fz[i] = 0.0;
-          if (j < n) {
-            const uint32_t p = cur[j];
-            fx[i] = (double)vx[p]; fy[i] = (double)vy[p]; fz[i] = (double)vz[p];
-          }
-          lex[i] = run;
           if (bj[i]) { run.x += fx[i]; run.y += fy[i]; run.z += fz[i]; run.m += 1.0; }
         }
         D4 tot;
         const D4 ex = blk_excl_scan4<BS>(run, &tot, sh4);
-        D4 nrun = {0, 0, 0, 0};
+        /* the thread's own earlier members, summed again in the same order */
+        D4 lex = {0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
           const int64_t j = j0 + i;
           if (j >= n) continue;
-          const uint32_t p = cur[j];
-          const double VX = cv.x + ex.x + lex[i].x, VY = cv.y + ex.y + lex[i].y, VZ = cv.z + ex.z + lex[i].z,
-                       M = cv.m + ex.m + lex[i].m;
+          const double VX = cv.x + ex.x + lex.x, VY = cv.y + ex.y + lex.y, VZ = cv.z + ex.z + lex.z,
+                       M = cv.m + ex.m + lex.m;
+          if (bj[i]) { lex.x += fx[i]; lex.y += fy[i]; lex.z += fz[i]; lex.m += 1.0; }
           const double dj = dist[j];
           double v_esc2;
           if (dj > MZERO) v_esc2 = (2 * fabs(phi[j] - Phi0) * K.phi_fac);
           else v_esc2 = 1e30;
-          const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
-                       dZ = wrap_s((double)pz[p] - cen[2]);
+          const double dX = g.dx[j], dY = g.dy[j], dZ = g.dz[j];
           const double dVX = (fx[i] - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
           const double dVY = (fy[i] - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
           const double dVZ = (fz[i] - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
@@ -523,18 +547,17 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
             if (gap <= tol || gap <= K.band * lim) sflag[1] = 1;
           }
           if (nb != bj[i]) { st[j] = (uint8_t)nb; sflag[0] = 1; }
-          if (nb) { nrun.x += fx[i]; nrun.y += fy[i]; nrun.z += fz[i]; nrun.m += 1.0; }
         }
-        /* later chunks see the updated statuses of this chunk */
-        D4 tot2;
-        (void)blk_excl_scan4<BS>(nrun, &tot2, sh4);
-        cv.x += tot2.x; cv.y += tot2.y; cv.z += tot2.z; cv.m += tot2.m;
+        __syncthreads();
+        const int changed = sflag[0], unc = sflag[1];
+        __syncthreads();
+        if (!changed) {
+          /* the statuses used were the final ones: so are the totals */
+          uncertain |= unc;
+          cv.x += tot.x; cv.y += tot.y; cv.z += tot.z; cv.m += tot.m;
+          break;
+        }
       }
-      __syncthreads();
-      const int changed = sflag[0];
-      uncertain = sflag[1];
-      __syncthreads();
-      if (!changed) break;
     }
     /* exact replay of this iteration where rounding could decide (thread 0) */
     if (uncertain) {
@@ -557,10 +580,8 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
         double VX = sx, VY = sy, VZ = sz, M = 1.0, Phi = 0;
         I_now = 0; I_prev = 0; d_prev = 0; M_r = 0;
         for (int64_t j = 0; j < n; j++) {
-          const uint32_t p = cur[j];
           M_r += 1.0;
-          const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
-                       dZ = wrap_s((double)pz[p] - cen[2]);
+          const double dX = g.dx[j], dY = g.dy[j], dZ = g.dz[j];
           dd = sqrt(dX * dX + dY * dY + dZ * dZ);
           double v_esc2;
           if (dd > MZERO) {
@@ -571,7 +592,7 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
           } else {
             v_esc2 = 1e30;
           }
-          const double fx = (double)vx[p], fy = (double)vy[p], fz = (double)vz[p];
+          const double fx = (double)g.vx[j], fy = (double)g.vy[j], fz = (double)g.vz[j];
           const double dVX = (fx - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
           const double dVY = (fy - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
           const double dVZ = (fz - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
@@ -581,9 +602,9 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
           } else {
             st[j] = 1;
             M += 1.0;
-            VX += 1.0 * vx[p];
-            VY += 1.0 * vy[p];
-            VZ += 1.0 * vz[p];
+            VX += 1.0 * g.vx[j];
+            VY += 1.0 * g.vy[j];
+            VZ += 1.0 * g.vz[j];
           }
           I_prev = I_now;
           d_prev = dd;
@@ -645,12 +666,37 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
     out->in_b = in_b;
     out->rvir_set = rvir_set;
     out->particle_iterations = pit;
+    out->sweeps = sweeps;
   }
 }
 
-/* haloes of at least this many members go to 1024-thread workgroups */
+/* the first iteration's seed of every halo (:3441-3470): the median |v|^2
+ * (FP32) of the first no_vbulk members of the sorted list, by AHF's indexx;
+ * one thread per halo */
+__global__ void k_seed(UnbindConst K, const int64_t *__restrict__ off, int64_t nh, const uint32_t *__restrict__ a,
+                       const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
+                       int64_t *__restrict__ seed)
+{
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  const int64_t o = off[h], n = off[h + 1] - o;
+  seed[h] = 0;
+  if (n < K.minpart || K.no_vbulk < 1) return;
+  double m2[65];
+  int ix[65];
+  const int nv = K.no_vbulk;
+  for (int j = 0; j < nv; j++) {
+    const uint32_t p = a[o + j];
+    const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+    m2[j + 1] = (double)v2;
+  }
+  dev_indexx(nv, m2, ix);
+  seed[h] = ix[nv / 2] - 1;
+}
+
+/* haloes of at least this many members go to 512-thread workgroups */
 #define AHF_BIG_HALO 2048
-#define AHF_BIG_BLOCK 1024
+#define AHF_BIG_BLOCK 512
 #define AHF_UNBIND_IPT 4
 
 /* persistent form: workgroups take haloes order[q0..q1) from the queue
@@ -658,8 +704,9 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, uint32_t *a, uint3
 template <int BS>
 __global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int64_t *__restrict__ off,
                                                        const int32_t *__restrict__ order, int64_t q0, int64_t q1,
-                                                       int32_t *__restrict__ next, uint32_t *a, uint32_t *b,
-                                                       double *dist, double *phi, uint8_t *st,
+                                                       int32_t *__restrict__ next, const int64_t *__restrict__ seed,
+                                                       uint32_t *a, uint32_t *b, double *dist, double *phi,
+                                                       uint8_t *st, Staged g,
                                                        const double *__restrict__ cen, const float *__restrict__ px,
                                                        const float *__restrict__ py, const float *__restrict__ pz,
                                                        const float *__restrict__ vx, const float *__restrict__ vy,
@@ -677,8 +724,9 @@ __global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int6
     if (h < 0) return;
     const int64_t o = off[h], n = off[h + 1] - o;
     const double c3[3] = {cen[3 * h], cen[3 * h + 1], cen[3 * h + 2]};
-    unbind_halo<BS, AHF_UNBIND_IPT>(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz,
-                                    out + h);
+    const Staged go = {g.dx + o, g.dy + o, g.dz + o, g.vx + o, g.vy + o, g.vz + o};
+    unbind_halo<BS, AHF_UNBIND_IPT>(K, n, seed[h], a + o, b + o, dist + o, phi + o, st + o, go, c3, px, py, pz, vx,
+                                    vy, vz, out + h);
     __syncthreads();
   }
 }
@@ -686,8 +734,9 @@ __global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int6
 /* plain form: one halo per launch */
 template <int BS>
 __global__ void __launch_bounds__(BS) k_unbind_one(UnbindConst K, int64_t h, const int64_t *__restrict__ off,
-                                                   uint32_t *a, uint32_t *b, double *dist, double *phi,
-                                                   uint8_t *st, const double *__restrict__ cen,
+                                                   const int64_t *__restrict__ seed, uint32_t *a, uint32_t *b,
+                                                   double *dist, double *phi,
+                                                   uint8_t *st, Staged g, const double *__restrict__ cen,
                                                    const float *__restrict__ px, const float *__restrict__ py,
                                                    const float *__restrict__ pz, const float *__restrict__ vx,
                                                    const float *__restrict__ vy, const float *__restrict__ vz,
@@ -695,8 +744,9 @@ __global__ void __launch_bounds__(BS) k_unbind_one(UnbindConst K, int64_t h, con
 {
   const int64_t o = off[h], n = off[h + 1] - o;
   const double c3[3] = {cen[3 * h], cen[3 * h + 1], cen[3 * h + 2]};
-  unbind_halo<BS, AHF_UNBIND_IPT>(K, n, a + o, b + o, dist + o, phi + o, st + o, c3, px, py, pz, vx, vy, vz,
-                                  out + h);
+  const Staged go = {g.dx + o, g.dy + o, g.dz + o, g.vx + o, g.vy + o, g.vz + o};
+  unbind_halo<BS, AHF_UNBIND_IPT>(K, n, seed[h], a + o, b + o, dist + o, phi + o, st + o, go, c3, px, py, pz, vx,
+                                  vy, vz, out + h);
 }
 
 __global__ void k_sorted_out(const int64_t *__restrict__ off, int64_t nh, HaloOut *__restrict__ out)
@@ -858,10 +908,13 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   s->buf_b = A.get<uint32_t>(ne);
   double *dist = A.get<double>(ne), *phi = A.get<double>(ne);
   uint8_t *st = A.get<uint8_t>(ne);
+  Staged g = {A.get<double>(ne), A.get<double>(ne), A.get<double>(ne),
+              A.get<float>(ne), A.get<float>(ne), A.get<float>(ne)};
   s->out = A.get<HaloOut>(nh);
   int32_t *next = A.get<int32_t>(2);
+  int64_t *seed = A.get<int64_t>(nh);
   if (!k0 || !k1 || !v0 || !v1 || !hk0 || !hk1 || !rtmp || !s->buf_a || !s->buf_b || !dist || !phi || !st ||
-      !s->out || !next)
+      !s->out || !next || !seed || !g.dx || !g.dy || !g.dz || !g.vx || !g.vy || !g.vz)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (unbinding)");
   c->timers_reset(TIMER_USORT, TIMER_UNBIND);
   c->timed.clear();
@@ -905,9 +958,15 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   hipEvent_t eu = c->timer_begin();
   if (sort_only) {
     if (nh) hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, s->out);
+  } else if (nh) {
+    hipLaunchKernelGGL(k_seed, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, K, s->off, nh, s->buf_a, c->vx, c->vy,
+                       c->vz, seed);
+  }
+  if (sort_only) {
+    /* nothing more */
   } else if (run == 0) {
     /* the order is largest first: haloes of AHF_BIG_HALO members or more
-     * get 1024-thread workgroups, the rest 256 */
+     * get AHF_BIG_BLOCK-thread workgroups, the rest AHF_BLOCK */
     int64_t nbig = 0;
     {
       std::vector<int32_t> ord((size_t)nh);
@@ -918,13 +977,14 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     if (nbig > 0) {
       const int64_t grid = std::min<int64_t>(nbig, (int64_t)dev_cu * 2);
       hipLaunchKernelGGL(k_unbind_batched<AHF_BIG_BLOCK>, dim3((unsigned)grid), dim3(AHF_BIG_BLOCK), 0, c->st, K,
-                         s->off, s->order, (int64_t)0, nbig, next, s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px,
+                         s->off, s->order, (int64_t)0, nbig, next, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px,
                          c->py, c->pz, c->vx, c->vy, c->vz, s->out);
     }
     if (nh > nbig) {
       const int64_t grid = std::min<int64_t>(nh - nbig, (int64_t)dev_cu * 8);
       hipLaunchKernelGGL(k_unbind_batched<AHF_BLOCK>, dim3((unsigned)grid), dim3(AHF_BLOCK), 0, c->st, K, s->off,
-                         s->order, nbig, nh, next + 1, s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px, c->py,
+                         s->order, nbig, nh, next + 1, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px,
+                         c->py,
                          c->pz, c->vx, c->vy, c->vz, s->out);
     }
   } else {
@@ -933,11 +993,11 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     for (int64_t h = 0; h < nh; h++) {
       if (s->h_off[h + 1] - s->h_off[h] >= AHF_BIG_HALO)
         hipLaunchKernelGGL(k_unbind_one<AHF_BIG_BLOCK>, dim3(1), dim3(AHF_BIG_BLOCK), 0, c->st, K, h, s->off,
-                           s->buf_a, s->buf_b, dist, phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
+                           seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
                            s->out);
       else
-        hipLaunchKernelGGL(k_unbind_one<AHF_BLOCK>, dim3(1), dim3(AHF_BLOCK), 0, c->st, K, h, s->off, s->buf_a,
-                           s->buf_b, dist, phi, st, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz, s->out);
+        hipLaunchKernelGGL(k_unbind_one<AHF_BLOCK>, dim3(1), dim3(AHF_BLOCK), 0, c->st, K, h, s->off, seed,
+                           s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz, s->out);
       hipStreamSynchronize(c->st);
     }
   }
@@ -960,6 +1020,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
       info->n_member_out += o.npart;
       info->particle_iterations += o.particle_iterations;
       info->n_exact_replays += o.replays;
+      if (o.sweeps > info->max_sweeps) info->max_sweeps = o.sweeps;
       if (o.niter > info->max_iterations) info->max_iterations = o.niter;
     }
   }

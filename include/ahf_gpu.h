@@ -80,7 +80,7 @@ typedef struct {
   int64_t particle_iterations;         /* sum over halos and iterations        */
   int64_t n_exact_replays;             /* halo iterations re-run sequentially  */
   int32_t max_iterations;
-  int32_t pad;
+  int32_t max_sweeps;                  /* most fixed-point sweeps of one halo  */
 } ahfgpu_unbind_info;
 
 int ahfgpu_create(ahfgpu_ctx **ctx, int device);
