@@ -5,9 +5,9 @@
       per-kernel dispatch count, total / average / min / max duration
   python -m ahf_amd.rocprof_summary counters DB --kernel PREFIX
       counter values per dispatch of the matching kernels
-  python -m ahf_amd.rocprof_summary traffic --fetch DB1 --write DB2 --kernel PREFIX
+  python -m ahf_amd.rocprof_summary traffic --fetch DB1 --write DB2 --kernel NAME [--kernel NAME ...]
                                             --copy-bytes N --workload TAG [--out JSON]
-      HBM bytes per launch of the matching kernel from a FETCH_SIZE run and a
+      HBM bytes per launch of the named kernels from a FETCH_SIZE run and a
       WRITE_SIZE run.  Both counters read KiB per dispatch; each is scaled by
       what it reads for the runtime's device-to-device copy kernel
       (__amd_rocclr_copyBuffer) moving a known N bytes in the same run.
@@ -43,23 +43,30 @@ def kernel_stats(db):
     return out
 
 
-def dispatch_counters(db, prefix):
+def _matches(name, kernels):
+    """kernels: one name prefix, or a list of exact kernel names"""
+    if isinstance(kernels, str):
+        return name.startswith(kernels)
+    return name in kernels
+
+
+def dispatch_counters(db, kernels):
     """{dispatch_id: (kernel, grid_size, duration_ns, {counter: value})} for
-    kernels whose name starts with prefix."""
+    the matching kernels (see _matches)."""
     con = sqlite3.connect(db)
     rows = con.execute("select dispatch_id, kernel_name, grid_size, counter_name, sum(value), duration "
                        "from counters_collection group by dispatch_id, counter_name").fetchall()
     out = {}
     for did, kn, gs, cn, v, dur in rows:
-        if not _short(kn).startswith(prefix):
+        if not _matches(_short(kn), kernels):
             continue
         rec = out.setdefault(did, (_short(kn), gs, dur, {}))
         rec[3][cn] = v
     return dict(sorted(out.items()))
 
 
-def _per_dispatch(db, prefix, counter):
-    return [c[counter] for _, (_, _, _, c) in dispatch_counters(db, prefix).items() if counter in c]
+def _per_dispatch(db, kernels, counter):
+    return [c[counter] for _, (_, _, _, c) in dispatch_counters(db, kernels).items() if counter in c]
 
 
 def _probe_copies(db, counter):
@@ -73,7 +80,7 @@ def _probe_copies(db, counter):
     return [v for gs, v in recs if gs == g]
 
 
-def traffic(fetch_db, write_db, prefix, copy_bytes):
+def traffic(fetch_db, write_db, kernels, copy_bytes):
     f_copy = _probe_copies(fetch_db, "FETCH_SIZE")
     w_copy = _probe_copies(write_db, "WRITE_SIZE")
     if not f_copy or not w_copy:
@@ -81,13 +88,13 @@ def traffic(fetch_db, write_db, prefix, copy_bytes):
     # KiB read for the copy kernel vs the bytes it moves
     f_scale = copy_bytes / (1024.0 * statistics.median(f_copy))
     w_scale = copy_bytes / (1024.0 * statistics.median(w_copy))
-    f_k = _per_dispatch(fetch_db, prefix, "FETCH_SIZE")
-    w_k = _per_dispatch(write_db, prefix, "WRITE_SIZE")
+    f_k = _per_dispatch(fetch_db, kernels, "FETCH_SIZE")
+    w_k = _per_dispatch(write_db, kernels, "WRITE_SIZE")
     if not f_k or not w_k:
-        raise SystemExit(f"no dispatches of {prefix} with FETCH_SIZE / WRITE_SIZE")
+        raise SystemExit(f"no dispatches of {kernels} with FETCH_SIZE / WRITE_SIZE")
     fetch = 1024.0 * f_scale * sum(f_k) / len(f_k)
     write = 1024.0 * w_scale * sum(w_k) / len(w_k)
-    return {"kernel": prefix, "dispatches": len(f_k), "fetch_bytes_per_launch": fetch,
+    return {"kernel": kernels if isinstance(kernels, str) else " + ".join(kernels), "dispatches": len(f_k), "fetch_bytes_per_launch": fetch,
             "write_bytes_per_launch": write, "bytes_per_launch": fetch + write,
             "fetch_scale": f_scale, "write_scale": w_scale,
             "copy_fetch_KiB": statistics.median(f_copy), "copy_write_KiB": statistics.median(w_copy),
@@ -106,7 +113,8 @@ def main(argv=None):
     c = sub.add_parser("traffic")
     c.add_argument("--fetch", required=True)
     c.add_argument("--write", required=True)
-    c.add_argument("--kernel", required=True)
+    c.add_argument("--kernel", required=True, action="append",
+                   help="exact kernel name; repeat for a stage made of several kernels")
     c.add_argument("--copy-bytes", type=int, required=True)
     c.add_argument("--workload", required=True)
     c.add_argument("--source", default="")

@@ -122,6 +122,46 @@ def load_traffic():
         return json.load(f)
 
 
+def unbinding_workload(seed=1002, reps=3, cpu=True, threads=16):
+    """The unbinding-throughput workload: 100,000 isolated NFW
+    haloes, N ~ N^-2 on [20, 2e6] members, 15% of members above escape speed
+    and 5% within 1e-4 of it; sort + unbinding of all haloes, batched, with
+    the reference's own sort_halo_particles + rem_unbound (OpenMP over
+    haloes) timed on the same file."""
+    import numpy as np
+
+    import ahf_amd
+    from ahf_amd import dumps
+    snap = make_snapshot(2, 0, seed)
+    s = dumps.read_snapshot(snap)
+    ctx = ahf_amd.Context(0)
+    ctx.set_particles(s.pos, s.mom)
+    ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+    args = (s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble)
+    ctx.unbind(*args)
+    ms = []
+    for _ in range(reps):
+        info = ctx.unbind(*args)
+        ms.append(ctx.last_timing()["unbind_ms"])
+    ctx.close()
+    t = float(np.median(ms))
+    out = {"workload": f"{s.nhalos} isolated NFW haloes, {s.n_ipart} members (largest {int(s.halo_npart.max())}), "
+                       f"seed {seed}",
+           "gpu_ms": t, "members_per_s": s.n_ipart / (t * 1e-3),
+           "particle_iterations_per_s": info.particle_iterations / (t * 1e-3),
+           "halos_per_s": s.nhalos / (t * 1e-3), "exact_replays": int(info.n_exact_replays),
+           "max_iterations": int(info.max_iterations)}
+    if cpu and os.path.exists(REF_BIN):
+        r = subprocess.run([REF_BIN, "time", snap, "--threads", str(threads), "--skip-amr", "--skip-1thread"],
+                           capture_output=True, text=True, timeout=900)
+        if r.returncode == 0:
+            rb = json.loads(r.stdout.strip().splitlines()[-1])
+            out["cpu_reference_ms"] = 1e3 * rb["unbind_seconds"]
+            out["cpu_threads"] = threads
+            out["vs_cpu"] = out["cpu_reference_ms"] / t
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -143,6 +183,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the one-halo-per-launch form")
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU timing")
+    ap.add_argument("--no-unbind-workload", action="store_true",
+                    help="skip the isolated-halo unbinding workload (100k haloes)")
     ap.add_argument("--band", type=float, default=0.0,
                     help="extra relative band sent to the exact replay (0: the certified bound only)")
     args = ap.parse_args()
@@ -268,7 +310,8 @@ def main():
         "stage_ms": stage_ms,
         "dominant_stage": dom,
         "roofline": {
-            "kernel": "k_gather_light (exact-order TSC deposit, one launch per level)",
+            "kernel": ("exact-order TSC deposit, one launch per level: k_gather_src on dense levels "
+                       "(domain), k_gather_light on sparse ones"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -302,6 +345,11 @@ def main():
             }
             out["vs_cpu"] = value / out["cpu_baseline"]["value"]
     ctx.close()
+    if not args.no_unbind_workload and args.side == 0:
+        u = unbinding_workload(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
+        log(f"unbinding workload: {u['gpu_ms']:.2f} ms on the GPU"
+            + (f", reference CPU {u['cpu_reference_ms']:.1f} ms" if "cpu_reference_ms" in u else ""))
+        out["unbinding_workload"] = u
     print(json.dumps(out))
 
 
