@@ -40,7 +40,8 @@ class AmrInfo(ctypes.Structure):
 class UnbindParams(ctypes.Structure):
     _fields_ = [("minpart", ctypes.c_int32), ("pad", ctypes.c_int32), ("vtune", ctypes.c_double),
                 ("r_fac", ctypes.c_double), ("v_fac", ctypes.c_double), ("phi_fac", ctypes.c_double),
-                ("hubble", ctypes.c_double)]
+                ("hubble", ctypes.c_double), ("ovlim", ctypes.c_double), ("rho_vir", ctypes.c_double),
+                ("rho_fac", ctypes.c_double)]
 
 
 class UnbindInfo(ctypes.Structure):
@@ -154,18 +155,27 @@ class Context:
         self._check(self.lib.ahfgpu_set_halos(self.h, self._cen.ctypes.data, self._hnp.ctypes.data,
                                               ip.ctypes.data, self._hnp.shape[0]))
 
-    def unbind(self, minpart, vtune, r_fac, v_fac, phi_fac, hubble, mode=0, band=0.0):
-        p = UnbindParams(minpart, 0, vtune, r_fac, v_fac, phi_fac, hubble)
+    # ahfgpu_unbind modes (or-ed: BATCHED/PLAIN with one step selection)
+    UNBIND_BATCHED, UNBIND_PLAIN = 0, 1
+    UNBIND_SORT_ONLY, UNBIND_NO_SORT, UNBIND_WITH_RVIR, UNBIND_RVIR_ONLY = 4, 8, 16, 32
+    # bits of props[:, 5]: the fields some step wrote
+    SET_MVIR, SET_RVIR, SET_PHI0, SET_OVDENS = 1, 2, 4, 8
+    PROPS = 6
+
+    def unbind(self, minpart, vtune, r_fac, v_fac, phi_fac, hubble, mode=0, band=0.0,
+               ovlim=0.0, rho_vir=1.0, rho_fac=0.0):
+        """ovlim, rho_vir, rho_fac: the rem_outsideRvir cut (modes WITH_RVIR, RVIR_ONLY)"""
+        p = UnbindParams(minpart, 0, vtune, r_fac, v_fac, phi_fac, hubble, ovlim, rho_vir, rho_fac)
         info = UnbindInfo()
         self._check(self.lib.ahfgpu_unbind(self.h, ctypes.byref(p), mode, band, ctypes.byref(info)))
         return info
 
     def unbind_results(self):
-        """(npart, props[nh,5] = M_vir, R_vir, Phi0, iterations, R_vir set, list of member arrays)"""
+        """(npart, props[nh,6] = M_vir, R_vir, Phi0, ovdens, iterations, SET_* mask, list of member arrays)"""
         import numpy as np
         nh = self._hnp.shape[0]
         npart = np.zeros(nh, np.uint64)
-        props = np.zeros((nh, 5), np.float64)
+        props = np.zeros((nh, self.PROPS), np.float64)
         ip = np.zeros(int(self._hnp.sum()) + 1, np.uint64)
         self._check(self.lib.ahfgpu_unbind_results(self.h, npart.ctypes.data, props.ctypes.data, ip.ctypes.data))
         off = np.concatenate([[0], np.cumsum(self._hnp.astype(np.int64))])

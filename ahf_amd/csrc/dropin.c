@@ -22,7 +22,7 @@
  * weak copies only let the library link on its own */
 __attribute__((weak)) param_simu simu;
 __attribute__((weak)) info_global global;
-__attribute__((weak)) double r_fac, v_fac, phi_fac, Hubble;
+__attribute__((weak)) double r_fac, v_fac, phi_fac, rho_fac, Hubble;
 
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static ahfgpu_ctx *g_ctx;
@@ -456,7 +456,7 @@ boolean gen_AMRhierarchy(gridls **grid_list, int *no_grids)
 
 /* ----------------------------------------------------------- haloes ---- */
 
-static int run_halos(HALO *h, long unsigned nh, int mode)
+static int run_halos(HALO *h, long unsigned nh, int mode, int clear_phi0)
 {
   ahfgpu_ctx *c = ctx_get();
   ensure_particles(0);
@@ -472,27 +472,35 @@ static int run_halos(HALO *h, long unsigned nh, int mode)
     np[i] = h[i].npart;
     for (uint64_t j = 0; j < h[i].npart; j++) ip[w++] = h[i].ipart[j];
   }
-  ahfgpu_unbind_params up = {simu.AHF_MINPART, 0, simu.AHF_VTUNE, r_fac, v_fac, phi_fac, Hubble};
+  ahfgpu_unbind_params up = {simu.AHF_MINPART, 0, simu.AHF_VTUNE, r_fac, v_fac, phi_fac, Hubble,
+                             global.ovlim, global.rho_vir, rho_fac};
   int rc = ahfgpu_set_halos(c, cen, np, ip, nh);
   if (!rc) rc = ahfgpu_unbind(c, &up, mode, 0.0, NULL);
-  double *props = (double *)xcalloc(5 * nh, 8);
+  double *props = (double *)xcalloc(AHFGPU_PROPS * nh, 8);
   if (!rc) rc = ahfgpu_unbind_results(c, np, props, ip);
   if (!rc) {
     w = 0;
     for (long unsigned i = 0; i < nh; i++) {
       const uint64_t n_in = h[i].npart;
-      if (n_in >= (uint64_t)simu.AHF_MINPART) {
-        if (mode & AHFGPU_UNBIND_SORT_ONLY) {
-          memcpy(h[i].ipart, ip + w, n_in * 8);
-        } else {
-          free(h[i].ipart);
-          h[i].ipart = (long unsigned *)xcalloc(np[i] ? np[i] : 1, 8);
-          memcpy(h[i].ipart, ip + w, np[i] * 8);
-          h[i].npart = np[i];
-          h[i].M_vir = props[5 * i];
-          if (props[5 * i + 4] > 0) h[i].R_vir = props[5 * i + 1];
-          h[i].Phi0 = props[5 * i + 2];
-        }
+      const double *q = props + AHFGPU_PROPS * i;
+      int set = (int)q[5];
+      if (!clear_phi0) set &= ~AHFGPU_SET_PHI0;
+      if (mode & AHFGPU_UNBIND_SORT_ONLY) {
+        /* sort_halo_particles reorders the array in place (below NminPerHalo
+         * the order is kept, so this copy changes nothing) */
+        memcpy(h[i].ipart, ip + w, n_in * 8);
+      } else if (set) {
+        /* rem_unbound installs a new array, rem_outsideRvir realloc()s the
+         * old one: either way the caller owns a malloc'd array of npart */
+        long unsigned *lst = (long unsigned *)realloc(h[i].ipart, (np[i] ? np[i] : 1) * 8);
+        if (!lst) fatal("out of host memory");
+        memcpy(lst, ip + w, np[i] * 8);
+        h[i].ipart = lst;
+        h[i].npart = np[i];
+        if (set & AHFGPU_SET_MVIR) h[i].M_vir = q[0];
+        if (set & AHFGPU_SET_RVIR) h[i].R_vir = q[1];
+        if (set & AHFGPU_SET_PHI0) h[i].Phi0 = q[2];
+        if (set & AHFGPU_SET_OVDENS) h[i].ovdens = q[3];
       }
       w += n_in;
     }
@@ -506,7 +514,7 @@ void sort_halo_particles(HALO *halo)
 {
   if (halo->npart < (long unsigned)simu.AHF_MINPART) return;
   pthread_mutex_lock(&g_lock);
-  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_SORT_ONLY);
+  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_SORT_ONLY, 0);
   pthread_mutex_unlock(&g_lock);
   if (rc) fatal("sort_halo_particles");
 }
@@ -515,7 +523,7 @@ void rem_unbound(HALO *halo)
 {
   if (halo->npart < (long unsigned)simu.AHF_MINPART) return;
   pthread_mutex_lock(&g_lock);
-  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_NO_SORT);
+  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_NO_SORT, 1);
   pthread_mutex_unlock(&g_lock);
   if (rc) fatal("rem_unbound");
 }
@@ -524,7 +532,25 @@ int ahf_unbind_halos(HALO *halos, long unsigned nhalos)
 {
   if (nhalos == 0) return 0;
   pthread_mutex_lock(&g_lock);
-  int rc = run_halos(halos, nhalos, AHFGPU_UNBIND_BATCHED);
+  int rc = run_halos(halos, nhalos, AHFGPU_UNBIND_BATCHED, 1);
+  pthread_mutex_unlock(&g_lock);
+  return rc;
+}
+
+void rem_outsideRvir(HALO *halo, int icall)
+{
+  if (halo->npart < (long unsigned)simu.AHF_MINPART) return;
+  pthread_mutex_lock(&g_lock);
+  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_RVIR_ONLY, icall == 0);
+  pthread_mutex_unlock(&g_lock);
+  if (rc) fatal("rem_outsideRvir");
+}
+
+int ahf_construct_halos(HALO *halos, long unsigned nhalos)
+{
+  if (nhalos == 0) return 0;
+  pthread_mutex_lock(&g_lock);
+  int rc = run_halos(halos, nhalos, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_WITH_RVIR, 1);
   pthread_mutex_unlock(&g_lock);
   return rc;
 }

@@ -345,6 +345,11 @@ static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo
   hd.v_fac = v_fac;
   hd.phi_fac = GRAV * g->pmass / (g->box * a);
   hd.hubble = 100.0 * sqrt(0.7 * (1.0 - 1.0 / (a * a)) + 0.3 * (1.0 / (a * a * a) - 1.0 / (a * a)) + 1.0 / (a * a));
+  /* haloes are sized at 200 times the critical density, so rem_outsideRvir
+   * cuts near the generated virial radius: Delta = 200 relative to
+   * a^3 rho_crit(a), the comoving form of src/libahf/ahf_halos.c:204-221 */
+  hd.ovlim = 200.0;
+  hd.rho_vir = RHOC0 * a * a * a * (hd.hubble / 100.0) * (hd.hubble / 100.0);
   hd.nhalos = (uint64_t)nh;
   hd.n_ipart = (uint64_t)n_ip;
   hd.seed = seed;

@@ -27,6 +27,10 @@
  *        of the iteration in AHF's own operation order; every other
  *        decision is the sequential one;
  *      - stable compaction of the bound members.
+ * 3. rem_outsideRvir (:3687-3889), on request before and after the
+ *    unbinding as ahf_halos_sfc.c:143,153 call it: the overdensity of the
+ *    sphere through member j depends on j and its distance only, so every
+ *    member is tested at once and the cut is the first failure.
  */
 #include <hip/hip_runtime.h>
 
@@ -42,10 +46,13 @@
 using namespace ahf;
 
 #define MZERO 5e-16 /* MACHINE_ZERO, src/param.h:125 */
+#define PI_AHF 3.14159265358979323846264338 /* src/param.h:143 */
 
 struct UnbindConst {
   double v2_tune, v_fac, r_fac, phi_fac, hubble, band;
+  double ovlim, rho_vir, rho_fac;
   int32_t minpart, no_vbulk;
+  int32_t cut_before, unbind, cut_after, pad;
 };
 
 __device__ inline double wrap_s(double d)
@@ -275,13 +282,78 @@ __device__ void dev_indexx(int n, const double *arr1, int *indx1)
 /* per-halo output record */
 struct HaloOut {
   int64_t npart;
-  double M_vir, R_vir, Phi0;
+  double M_vir, R_vir, Phi0, ovdens;
   int32_t niter, replays;
   int32_t in_b; /* final list is in buffer b (0 = a) */
-  int32_t rvir_set;
+  int32_t set;  /* AHFGPU_SET_* fields written */
   int64_t particle_iterations;
   int32_t sweeps, pad;
 };
+
+/* overdensity of the sphere through member j at particle p, as
+ * src/libahf/ahf_halos.c:3832-3856 evaluates it (M = j+1 unit masses) */
+__device__ inline double sphere_ovdens(const UnbindConst &K, int64_t j, uint32_t p, const double cen[3],
+                                       const float *__restrict__ px, const float *__restrict__ py,
+                                       const float *__restrict__ pz, double *R_out)
+{
+  const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
+               dZ = wrap_s((double)pz[p] - cen[2]);
+  const double R = sqrt(dX * dX + dY * dY + dZ * dZ);
+  const double V = 4. * PI_AHF / 3. * (R * R * R);
+  *R_out = R;
+  return (double)(j + 1) / V * K.rho_fac / K.rho_vir;
+}
+
+/*
+ * rem_outsideRvir on the first n >= 1 members of list l, one workgroup of BS
+ * threads: the members kept are those up to and including the first whose
+ * sphere is below ovlim (a NaN overdensity stops the walk too).  Chunks of
+ * 4*BS members are tested until one holds a failure.  Returns the count
+ * kept; thread 0 receives M_vir, R_vir and ovdens of the last member kept.
+ */
+template <int BS>
+__device__ int64_t rvir_cut(const UnbindConst &K, int64_t n, const uint32_t *l, const double cen[3],
+                            const float *__restrict__ px, const float *__restrict__ py,
+                            const float *__restrict__ pz, double *M, double *R, double *od)
+{
+  constexpr int PER = 4;
+  constexpr unsigned long long NONE = ~0ull;
+  __shared__ unsigned long long s_first;
+  const int t = threadIdx.x;
+  if (t == 0) s_first = NONE;
+  __syncthreads();
+  unsigned long long first = NONE;
+  for (int64_t base = 0; base < n; base += (int64_t)PER * BS) {
+    uint32_t pp[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int64_t j = base + (int64_t)i * BS + t;
+      pp[i] = j < n ? l[j] : 0u;
+    }
+    unsigned long long f = NONE;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int64_t j = base + (int64_t)i * BS + t;
+      if (j >= n) continue;
+      double r;
+      const double o = sphere_ovdens(K, j, pp[i], cen, px, py, pz, &r);
+      if (!(o >= K.ovlim) && (unsigned long long)j < f) f = (unsigned long long)j;
+    }
+    if (f != NONE) atomicMin(&s_first, f);
+    __syncthreads();
+    first = s_first;
+    __syncthreads();
+    if (first != NONE) break;
+  }
+  const int64_t kept = first != NONE ? (int64_t)first + 1 : n;
+  if (t == 0) {
+    double r;
+    *od = sphere_ovdens(K, kept - 1, l[kept - 1], cen, px, py, pz, &r);
+    *R = r;
+    *M = (double)kept;
+  }
+  return kept;
+}
 
 /* exclusive scans over a workgroup of BS threads (exclusive value taken
  * from the neighbouring lane's inclusive value, block total in *tot) */
@@ -374,18 +446,24 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
   const int t = threadIdx.x;
   int64_t n = n0;
   uint32_t *cur = a, *nxt = b;
-  int niter = 0, replays = 0, in_b = 0, rvir_set = 0, sweeps = 0;
-  double R_vir = 0.0, Phi0 = 0.0, M_vir = (double)n;
+  int niter = 0, replays = 0, in_b = 0, rvir_set = 0, sweeps = 0, set = 0;
+  double R_vir = 0.0, Phi0 = 0.0, M_vir = 0.0, ovdens = 0.0;
   int64_t pit = 0;
-  if (n < K.minpart) {
-    if (t == 0) {
-      out->npart = n; out->M_vir = 0.0; out->R_vir = 0.0; out->Phi0 = 0.0;
-      out->niter = 0; out->replays = 0; out->in_b = 0; out->rvir_set = 0; out->particle_iterations = 0;
-      out->sweeps = 0;
-    }
-    return;
+  /* rem_outsideRvir(halo, 0): returns below minpart, else cuts and clears
+   * Phi0 (:3696, :3864-3871) */
+  if (K.cut_before && n >= K.minpart) {
+    n = rvir_cut<BS>(K, n, cur, cen, px, py, pz, &M_vir, &R_vir, &ovdens);
+    Phi0 = 0.0;
+    set = AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_PHI0 | AHFGPU_SET_OVDENS;
   }
-  int64_t nremove = 4;
+  /* rem_unbound returns below minpart (:3312) */
+  const bool unbind = K.unbind && n >= K.minpart;
+  if (unbind) {
+    /* its R_vir is a local assigned by the first bound member */
+    R_vir = 0.0;
+    set = (set | AHFGPU_SET_MVIR | AHFGPU_SET_PHI0) & ~AHFGPU_SET_RVIR;
+  }
+  int64_t nremove = unbind ? 4 : 0;
   while (nremove > 3) {
     niter++;
     pit += n;
@@ -656,15 +734,22 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
     __syncthreads();
     if ((double)n < (double)K.minpart) break;
   }
+  if (rvir_set) set |= AHFGPU_SET_RVIR;
+  /* rem_outsideRvir(halo, 1) on the bound list */
+  if (K.cut_after && n >= K.minpart) {
+    n = rvir_cut<BS>(K, n, cur, cen, px, py, pz, &M_vir, &R_vir, &ovdens);
+    set |= AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_OVDENS;
+  }
   if (t == 0) {
     out->npart = n;
     out->M_vir = M_vir;
     out->R_vir = R_vir;
     out->Phi0 = Phi0;
+    out->ovdens = ovdens;
     out->niter = niter;
     out->replays = replays;
     out->in_b = in_b;
-    out->rvir_set = rvir_set;
+    out->set = set;
     out->particle_iterations = pit;
     out->sweeps = sweeps;
   }
@@ -758,11 +843,14 @@ __global__ void k_sorted_out(const int64_t *__restrict__ off, int64_t nh, HaloOu
   o.M_vir = 0.0;
   o.R_vir = 0.0;
   o.Phi0 = 0.0;
+  o.ovdens = 0.0;
   o.niter = 0;
   o.replays = 0;
   o.in_b = 0;
-  o.rvir_set = 0;
+  o.set = 0;
   o.particle_iterations = 0;
+  o.sweeps = 0;
+  o.pad = 0;
   out[h] = o;
 }
 
@@ -885,8 +973,14 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   HaloSet *s = (HaloSet *)c->halo_set;
   if (!s) return c->fail(AHFGPU_EINVAL, "no halos resident (call ahfgpu_set_halos first)");
   if (p->minpart < 2 || p->minpart / 2 > 64) return c->fail(AHFGPU_EINVAL, "NminPerHalo must be in [2,129]");
-  if ((mode & 3) > 1 || (mode & ~(3 | AHFGPU_UNBIND_SORT_ONLY | AHFGPU_UNBIND_NO_SORT)))
-    return c->fail(AHFGPU_EINVAL, "bad unbinding mode %d", mode);
+  {
+    const int steps = mode & ~3;
+    const bool ok = (mode & 3) <= 1 &&
+                    (steps == 0 || steps == AHFGPU_UNBIND_SORT_ONLY || steps == AHFGPU_UNBIND_NO_SORT ||
+                     steps == AHFGPU_UNBIND_WITH_RVIR || steps == (AHFGPU_UNBIND_WITH_RVIR | AHFGPU_UNBIND_NO_SORT) ||
+                     steps == AHFGPU_UNBIND_RVIR_ONLY);
+    if (!ok) return c->fail(AHFGPU_EINVAL, "bad unbinding mode %d", mode);
+  }
   if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
   Arena &A = c->uarena;
   A.reset();
@@ -898,8 +992,15 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   K.phi_fac = p->phi_fac;
   K.hubble = p->hubble;
   K.band = band;
+  K.ovlim = p->ovlim;
+  K.rho_vir = p->rho_vir;
+  K.rho_fac = p->rho_fac;
   K.minpart = p->minpart;
   K.no_vbulk = p->minpart / 2;
+  K.cut_before = (mode & (AHFGPU_UNBIND_WITH_RVIR | AHFGPU_UNBIND_RVIR_ONLY)) != 0;
+  K.unbind = (mode & AHFGPU_UNBIND_RVIR_ONLY) == 0;
+  K.cut_after = (mode & AHFGPU_UNBIND_WITH_RVIR) != 0;
+  K.pad = 0;
   uint64_t *k0 = A.get<uint64_t>(ne), *k1 = A.get<uint64_t>(ne);
   uint32_t *v0 = A.get<uint32_t>(ne), *v1 = A.get<uint32_t>(ne);
   uint32_t *hk0 = A.get<uint32_t>(ne), *hk1 = A.get<uint32_t>(ne);
@@ -933,7 +1034,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   }
   const bool small_sort = max_sorted <= AHF_SORT_SMALL;
   const bool sort_only = (mode & AHFGPU_UNBIND_SORT_ONLY) != 0;
-  const bool no_sort = (mode & AHFGPU_UNBIND_NO_SORT) != 0;
+  const bool no_sort = (mode & (AHFGPU_UNBIND_NO_SORT | AHFGPU_UNBIND_RVIR_ONLY)) != 0;
   if (ne > 0 && no_sort) {
     hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->ip_in, ne, s->buf_a);
   } else if (ne > 0 && small_sort) {
@@ -958,7 +1059,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   hipEvent_t eu = c->timer_begin();
   if (sort_only) {
     if (nh) hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, s->out);
-  } else if (nh) {
+  } else if (nh && K.unbind) {
     hipLaunchKernelGGL(k_seed, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, K, s->off, nh, s->buf_a, c->vx, c->vy,
                        c->vz, seed);
   }
@@ -1044,11 +1145,13 @@ extern "C" int ahfgpu_unbind_results(ahfgpu_ctx *c, uint64_t *npart, double *pro
     const HaloOut &o = ho[(size_t)h];
     if (npart) npart[h] = (uint64_t)o.npart;
     if (props) {
-      props[5 * h + 0] = o.M_vir;
-      props[5 * h + 1] = o.R_vir;
-      props[5 * h + 2] = o.Phi0;
-      props[5 * h + 3] = (double)o.niter;
-      props[5 * h + 4] = (double)o.rvir_set;
+      double *q = props + AHFGPU_PROPS * h;
+      q[0] = o.M_vir;
+      q[1] = o.R_vir;
+      q[2] = o.Phi0;
+      q[3] = o.ovdens;
+      q[4] = (double)o.niter;
+      q[5] = (double)o.set;
     }
     if (ipart) {
       const std::vector<uint32_t> &src = o.in_b ? b : a;

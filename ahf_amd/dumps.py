@@ -1,5 +1,5 @@
 """Readers for the grid-hierarchy and halo dumps (AHFGRID1 / AHFHALO1) and the
-snapshot container (AHFSNAP1).
+snapshot container (AHFSNAP2).
 
 The same dump layout is written by the reference harness
 (oracle/ref_driver.c), the CPU restatement (oracle/ahf_oracle.c) and this
@@ -12,7 +12,7 @@ import struct
 
 import numpy as np
 
-SNAP_HEADER = struct.Struct("<8sQddddd ii dd ii ddddd QQQ ii")
+SNAP_HEADER = struct.Struct("<8sQddddd ii dd ii ddddd QQQ ii dd")
 
 
 @dataclasses.dataclass
@@ -37,6 +37,8 @@ class Snapshot:
     n_ipart: int
     seed: int
     config: int
+    ovlim: float             # virial overdensity of rem_outsideRvir
+    rho_vir: float           # the density it is relative to
     pos: np.ndarray          # (npart, 3) float32, box units
     mom: np.ndarray          # (npart, 3) float32
     ids: np.ndarray          # (npart,) uint64
@@ -50,15 +52,21 @@ class Snapshot:
         np.cumsum(self.halo_npart.astype(np.int64), out=off[1:])
         return off
 
+    @property
+    def rho_fac(self) -> float:
+        """pmass / boxsize^3 (src/libahf/ahf_halos.c:204), evaluated as AHF does."""
+        b = self.boxsize
+        return self.pmass / (b * b * b)
+
 
 def read_snapshot(path: str) -> Snapshot:
     with open(path, "rb") as f:
         raw = f.read(SNAP_HEADER.size)
         v = SNAP_HEADER.unpack(raw)
-        if v[0] != b"AHFSNAP1":
-            raise ValueError(f"{path}: not an AHFSNAP1 file")
+        if v[0] != b"AHFSNAP2":
+            raise ValueError(f"{path}: not an AHFSNAP2 file")
         (npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, _p0, vt, rf, vf, pf, hb,
-         nh, ni, seed, cfg, _p1) = v[1:]
+         nh, ni, seed, cfg, _p1, ovl, rhv) = v[1:]
         pos = np.fromfile(f, dtype=np.float32, count=3 * npart).reshape(npart, 3)
         mom = np.fromfile(f, dtype=np.float32, count=3 * npart).reshape(npart, 3)
         ids = np.fromfile(f, dtype=np.uint64, count=npart)
@@ -66,13 +74,14 @@ def read_snapshot(path: str) -> Snapshot:
         hnp = np.fromfile(f, dtype=np.uint64, count=nh)
         ip = np.fromfile(f, dtype=np.uint64, count=ni)
     return Snapshot(npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, vt, rf, vf, pf, hb,
-                    nh, ni, seed, cfg, pos, mom, ids, cen, hnp, ip)
+                    nh, ni, seed, cfg, ovl, rhv, pos, mom, ids, cen, hnp, ip)
 
 
 def write_snapshot(path: str, pos, mom=None, ids=None, *, boxsize=100.0, pmass=1e10,
                    L_dom=16, L_max=None, Nth_dom=5.0, Nth_ref=5.0, minpart=20, vtune=1.5,
                    r_fac=None, v_fac=None, phi_fac=None, hubble=100.0, centres=None,
-                   halo_lists=None, seed=0, config=-1, omega0=0.3, lambda0=0.7, a=1.0):
+                   halo_lists=None, seed=0, config=-1, omega0=0.3, lambda0=0.7, a=1.0,
+                   ovlim=200.0, rho_vir=2.7755397e11):
     """Write a hand-made snapshot (tests).  pos in box units."""
     pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
     n = pos.shape[0]
@@ -88,9 +97,9 @@ def write_snapshot(path: str, pos, mom=None, ids=None, *, boxsize=100.0, pmass=1
     hnp = np.array([len(h) for h in halo_lists], dtype=np.uint64)
     ip = (np.concatenate([np.asarray(h, np.uint64) for h in halo_lists])
           if halo_lists else np.zeros(0, np.uint64))
-    hdr = SNAP_HEADER.pack(b"AHFSNAP1", n, boxsize, omega0, lambda0, a, pmass, L_dom, L_max,
+    hdr = SNAP_HEADER.pack(b"AHFSNAP2", n, boxsize, omega0, lambda0, a, pmass, L_dom, L_max,
                            Nth_dom, Nth_ref, minpart, 0, vtune, r_fac, v_fac, phi_fac, hubble,
-                           len(halo_lists), ip.size, seed, config, 0)
+                           len(halo_lists), ip.size, seed, config, 0, ovlim, rho_vir)
     with open(path, "wb") as f:
         f.write(hdr)
         pos.tofile(f)
@@ -159,6 +168,7 @@ class HaloResult:
     R_vir: np.ndarray
     Phi0: np.ndarray
     ipart: list           # per-halo uint64 arrays (bound members, in order)
+    ovdens: np.ndarray = None   # overdensity at R_vir (0 where never set)
 
 
 def read_halos(path: str) -> HaloResult:
@@ -170,14 +180,15 @@ def read_halos(path: str) -> HaloResult:
         mv = np.zeros(nh)
         rv = np.zeros(nh)
         p0 = np.zeros(nh)
+        od = np.zeros(nh)
         lists = []
         for i in range(nh):
             (n,) = struct.unpack("<Q", f.read(8))
             vals = struct.unpack("<dddd", f.read(32))
             npart[i] = n
-            mv[i], rv[i], p0[i] = vals[0], vals[1], vals[2]
+            mv[i], rv[i], p0[i], od[i] = vals
             lists.append(np.fromfile(f, np.uint64, n))
-    return HaloResult(npart, mv, rv, p0, lists)
+    return HaloResult(npart, mv, rv, p0, lists, od)
 
 
 def compare_grids(a: list[GridLevel], b: list[GridLevel], *, dens_exact=True, rtol=0.0):

@@ -10,8 +10,9 @@
  * carries weak definitions only so that it also links on its own):
  *   simu   NGRID_DOM, NGRID_MAX, Nth_dom, Nth_ref, AHF_MINPART, AHF_VTUNE,
  *          mean_dens, no_part, no_vpart
- *   global fst_part, no_part, dom_grid, domgrid_no, fst_cycle, fin_l1dim
- *   r_fac, v_fac, phi_fac, Hubble
+ *   global fst_part, no_part, dom_grid, domgrid_no, fst_cycle, fin_l1dim,
+ *          ovlim, rho_vir
+ *   r_fac, v_fac, phi_fac, rho_fac, Hubble
  *
  * Errors: like the reference, the grid functions terminate the process with
  * a message on stderr when they cannot continue (device missing, allocation
@@ -28,7 +29,7 @@ extern "C" {
 
 extern param_simu simu;
 extern info_global global;
-extern double r_fac, v_fac, phi_fac, Hubble;
+extern double r_fac, v_fac, phi_fac, rho_fac, Hubble;
 
 /* src/libamr_serial/generate_grids.h:7 -- the domain grid, with AHF's quads */
 gridls *gen_domgrids(int *no_grids);
@@ -60,6 +61,20 @@ void sort_halo_particles(HALO *halo);
  * halo->ipart is replaced by a new malloc'd array, as in the reference.
  * Safe to call from several threads (calls are serialised on the GPU). */
 void rem_unbound(HALO *halo);
+
+/* src/libahf/ahf_halos.h:24 -- cut the radially sorted members at the
+ * first sphere below global.ovlim (overdensity relative to global.rho_vir);
+ * sets npart, M_vir, R_vir, ovdens, and Phi0 = 0 when icall == 0;
+ * halo->ipart is realloc()ed as in the reference.  Thread-safe like
+ * rem_unbound. */
+void rem_outsideRvir(HALO *halo, int icall);
+
+/* No counterpart in the reference: the construction steps of
+ * src/libahf/ahf_halos_sfc.c:140-153 after the gather -- sort_halo_particles,
+ * rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1) -- for all nhalos
+ * haloes in one batched GPU pass.  Returns 0, or a negative AHFGPU_E*
+ * code. */
+int ahf_construct_halos(HALO *halos, long unsigned nhalos);
 
 /* No counterpart in the reference: sort_halo_particles + rem_unbound for
  * all nhalos haloes in one batched GPU pass, replacing the per-halo calls

@@ -14,7 +14,9 @@
  *          lltools.c:32-92, density.c:238-492, refine_grid.c:113-985,
  *          relink.c:31-423)
  *       the per-halo sort_halo_particles + rem_unbound loop
- *         (src/libahf/ahf_halos.c:504-510, 3292-3607, 5786-5894).
+ *         (src/libahf/ahf_halos.c:504-510, 3292-3607, 5786-5894), with
+ *         rem_outsideRvir before and after rem_unbound on request
+ *         (ahf_halos.c:3687-3889, called at ahf_halos_sfc.c:143,153).
  *
  *  2. Drop-in versions of AHF's own entry points with AHF's prototypes and
  *     data structures (ahf_dropin.h), implemented on top of layer 1.
@@ -62,7 +64,9 @@ typedef struct {
 } ahfgpu_amr_info;
 
 /* unbinding parameters: NminPerHalo, VescTune and the conversion factors of
- * src/libahf/ahf_halos.c:199-213 */
+ * src/libahf/ahf_halos.c:199-213; for the rem_outsideRvir cut also
+ * global.ovlim, global.rho_vir and rho_fac (= pmass/boxsize^3) of
+ * ahf_halos.c:204-221 */
 typedef struct {
   int32_t minpart;
   int32_t pad;
@@ -71,6 +75,9 @@ typedef struct {
   double v_fac;
   double phi_fac;
   double hubble;
+  double ovlim;
+  double rho_vir;
+  double rho_fac;
 } ahfgpu_unbind_params;
 
 typedef struct {
@@ -138,11 +145,16 @@ int ahfgpu_set_halos(ahfgpu_ctx *ctx, const double *centres, const uint64_t *npa
 #define AHFGPU_UNBIND_PLAIN 1     /* one launch per halo, synchronised     */
 #define AHFGPU_UNBIND_SORT_ONLY 4 /* radial sort only (sort_halo_particles) */
 #define AHFGPU_UNBIND_NO_SORT 8   /* unbind the lists in the given order   */
+#define AHFGPU_UNBIND_WITH_RVIR 16 /* rem_outsideRvir before and after     */
+#define AHFGPU_UNBIND_RVIR_ONLY 32 /* one rem_outsideRvir cut, no sort     */
 
 /* radial sort + unbinding of every resident halo.  mode 0: all haloes in one
  * persistent launch (size-ordered work queue); mode 1: one launch per halo,
- * synchronised per halo (the per-call form); or-ed with SORT_ONLY or
- * NO_SORT.  A member whose bound test could go the other way under AHF's
+ * synchronised per halo (the per-call form); or-ed with one of SORT_ONLY,
+ * NO_SORT, WITH_RVIR (the steps of ahf_halos_sfc.c:140-153: sort,
+ * rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1)) or RVIR_ONLY (the
+ * cut alone on the lists as given, as one rem_outsideRvir call; the caller
+ * clears Phi0 for icall 0).  A member whose bound test could go the other way under AHF's
  * sequential rounding (certified error bound of the parallel sums) is
  * decided by an exact sequential replay of the iteration; band > 0 widens
  * that to every member within band*VescTune^2*v_esc^2 of the threshold (0:
@@ -151,9 +163,17 @@ int ahfgpu_set_halos(ahfgpu_ctx *ctx, const double *centres, const uint64_t *npa
 int ahfgpu_unbind(ahfgpu_ctx *ctx, const ahfgpu_unbind_params *p, int mode, double band,
                   ahfgpu_unbind_info *info);
 
-/* results of the last ahfgpu_unbind: per halo npart and 5 numbers (M_vir,
- * R_vir, Phi0, iterations, R_vir-was-set), and the bound members of halo h
- * at ipart[offset_h .. offset_h + npart_h) (offsets of the input CSR) */
+/* results of the last ahfgpu_unbind: per halo npart and AHFGPU_PROPS numbers
+ * (M_vir, R_vir, Phi0, ovdens, unbinding iterations, and a mask of the
+ * AHFGPU_SET_* fields that some step wrote -- the others keep the caller's
+ * values; R_vir is not set when rem_unbound bound no member, where AHF
+ * stores an unassigned local), and the members kept of halo h at
+ * ipart[offset_h .. offset_h + npart_h) (offsets of the input CSR) */
+#define AHFGPU_PROPS 6
+#define AHFGPU_SET_MVIR 1
+#define AHFGPU_SET_RVIR 2
+#define AHFGPU_SET_PHI0 4
+#define AHFGPU_SET_OVDENS 8
 int ahfgpu_unbind_results(ahfgpu_ctx *ctx, uint64_t *npart, double *props, uint64_t *ipart);
 void ahfgpu_release_halos(ahfgpu_ctx *ctx);
 

@@ -30,6 +30,7 @@
 #define MIN_NNODES 125     /* src/param.h:54 */
 #define CRITMULTI 8.0      /* src/param.h:118 */
 #define MACHINE_ZERO 5e-16 /* src/param.h:125 */
+#define PI_AHF 3.14159265358979323846264338 /* src/param.h:143 */
 
 /* ------------------------------------------------------ small vectors ---- */
 
@@ -1047,6 +1048,127 @@ void oracle_rem_unbound(const float *pos, const float *mom, uint64_t *ipart, uin
     res->niter = niter;
     if ((double)n < (double)P->minpart) break;
   }
+}
+
+/*
+ * rem_outsideRvir, src/libahf/ahf_halos.c:3687-3889, with AHFprofilerise
+ * off (src/define.h:74) and equal masses; both calls (icall 0 and 1) cut
+ * the same way, icall 0 also clears Phi0 (done by the caller).  Members are
+ * taken in list order; after each one the overdensity of the sphere through
+ * it is M/V * rho_fac / rho_vir, and the walk stops after the first member
+ * whose sphere is below ovlim (that member is kept).  Below minpart the
+ * function returns before touching anything: the result is n and nothing is
+ * written.
+ */
+uint64_t oracle_rem_outsideRvir(const float *pos, const uint64_t *ipart, uint64_t n, const double c[3],
+                                const oracle_rvir_params *P, double *M_vir, double *R_vir, double *ovdens)
+{
+  if (n < (uint64_t)P->minpart) return n;
+  uint64_t kept = 0;
+  double M = 0.0, R = -1.0, od = 2 * P->ovlim;
+  while (kept < n && od >= P->ovlim) {
+    const float *p = pos + 3 * ipart[kept];
+    M += 1.0;
+    double dX = fabs((double)p[0] - c[0]), dY = fabs((double)p[1] - c[1]), dZ = fabs((double)p[2] - c[2]);
+    if (dX > 0.5) dX -= 1.0;
+    if (dY > 0.5) dY -= 1.0;
+    if (dZ > 0.5) dZ -= 1.0;
+    R = sqrt(dX * dX + dY * dY + dZ * dZ);
+    const double V = 4. * PI_AHF / 3. * (R * R * R);
+    od = M / V * P->rho_fac / P->rho_vir;
+    kept++;
+  }
+  *M_vir = M;
+  *R_vir = R;
+  *ovdens = od;
+  return kept;
+}
+
+/*
+ * The part of ahf_halos_sfc_constructHalo (src/libahf/ahf_halos_sfc.c:140-153)
+ * between the gather and the profiles, on one halo: sort_halo_particles,
+ * rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1).  st starts as a
+ * zeroed HALO would; each step writes only the fields the reference step
+ * writes.  ipart is reordered and truncated in place.
+ */
+void oracle_construct_halo(const float *pos, const float *mom, uint64_t *ipart, uint64_t n, const double c[3],
+                           const oracle_unbind_params *U, const oracle_rvir_params *R, int stable_ties,
+                           oracle_halo_state *st)
+{
+  st->npart = n;
+  oracle_sort_halo(pos, n, ipart, c, U->minpart, stable_ties);
+  if (n >= (uint64_t)R->minpart) {
+    st->npart = oracle_rem_outsideRvir(pos, ipart, n, c, R, &st->M_vir, &st->R_vir, &st->ovdens);
+    st->Phi0 = 0.0;
+    st->set |= ORACLE_SET_MVIR | ORACLE_SET_RVIR | ORACLE_SET_OVDENS | ORACLE_SET_PHI0;
+  }
+  if (st->npart >= (uint64_t)U->minpart) {
+    oracle_unbind_result r;
+    oracle_rem_unbound(pos, mom, ipart, st->npart, c, U, &r);
+    st->npart = r.npart;
+    st->M_vir = r.M_vir;
+    st->Phi0 = r.Phi0;
+    st->niter = r.niter;
+    st->set |= ORACLE_SET_MVIR | ORACLE_SET_PHI0;
+    /* AHF stores its R_vir local, which stays unassigned when no member was
+     * ever bound */
+    if (r.rvir_set) {
+      st->R_vir = r.R_vir;
+      st->set |= ORACLE_SET_RVIR;
+    } else {
+      st->R_vir = 0.0;
+      st->set &= ~ORACLE_SET_RVIR;
+    }
+  }
+  if (st->npart >= (uint64_t)R->minpart) {
+    st->npart = oracle_rem_outsideRvir(pos, ipart, st->npart, c, R, &st->M_vir, &st->R_vir, &st->ovdens);
+    st->set |= ORACLE_SET_MVIR | ORACLE_SET_RVIR | ORACLE_SET_OVDENS;
+  }
+}
+
+static void dump_states(FILE *f, const snap_data *s, const uint64_t *off, const oracle_halo_state *st)
+{
+  const uint64_t nh = s->h.nhalos;
+  put(f, "AHFHALO1", 1, 8);
+  put(f, &nh, 8, 1);
+  for (uint64_t h = 0; h < nh; h++) {
+    uint64_t n = st[h].npart;
+    double v[4] = {st[h].M_vir, st[h].R_vir, st[h].Phi0, st[h].ovdens};
+    put(f, &n, 8, 1);
+    put(f, v, 8, 4);
+    put(f, s->ipart + off[h], 8, (size_t)n);
+  }
+}
+
+int oracle_construct_dump(const char *snap_path, const char *out_path, int threads, int stable_ties)
+{
+  snap_data s;
+  if (snap_read(snap_path, &s)) return -1;
+  const uint64_t nh = s.h.nhalos;
+  uint64_t *off = (uint64_t *)malloc((nh + 1) * sizeof(uint64_t));
+  off[0] = 0;
+  for (uint64_t i = 0; i < nh; i++) off[i + 1] = off[i] + s.halo_npart[i];
+  oracle_halo_state *st = (oracle_halo_state *)calloc(nh ? nh : 1, sizeof(oracle_halo_state));
+  const oracle_unbind_params U = {s.h.minpart, s.h.vtune, s.h.r_fac, s.h.v_fac, s.h.phi_fac, s.h.hubble};
+  const double b = s.h.boxsize;
+  const oracle_rvir_params R = {s.h.minpart, s.h.ovlim, s.h.rho_vir, s.h.pmass / (b * b * b)};
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  long i;
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++)
+    oracle_construct_halo(s.pos, s.mom, s.ipart + off[i], s.halo_npart[i], s.centre + 3 * i, &U, &R, stable_ties,
+                          &st[i]);
+  FILE *f = fopen(out_path, "wb");
+  if (!f) return -2;
+  dump_states(f, &s, off, st);
+  fclose(f);
+  free(off); free(st);
+  snap_free(&s);
+  return 0;
 }
 
 int oracle_unbind_dump(const char *snap_path, const char *out_path, int threads, int stable_ties)

@@ -47,6 +47,37 @@ typedef struct {
   int32_t rvir_set;   /* 0: no particle was ever bound (R_vir undefined in AHF) */
 } oracle_unbind_result;
 
+/* rem_outsideRvir on one radially sorted member list: returns the members
+ * kept; M_vir, R_vir, ovdens are written unless n < minpart */
+typedef struct {
+  int32_t minpart;
+  double ovlim, rho_vir, rho_fac;
+} oracle_rvir_params;
+
+uint64_t oracle_rem_outsideRvir(const float *pos, const uint64_t *ipart, uint64_t n, const double centre[3],
+                                const oracle_rvir_params *p, double *M_vir, double *R_vir, double *ovdens);
+
+/* a halo's fields after the construction steps, as a zeroed HALO would hold
+ * them; `set` says which fields some step wrote */
+#define ORACLE_SET_MVIR 1
+#define ORACLE_SET_RVIR 2
+#define ORACLE_SET_PHI0 4
+#define ORACLE_SET_OVDENS 8
+typedef struct {
+  uint64_t npart;
+  double M_vir, R_vir, Phi0, ovdens;
+  int32_t niter;
+  int32_t set;
+} oracle_halo_state;
+
+void oracle_construct_halo(const float *pos, const float *mom, uint64_t *ipart, uint64_t n, const double centre[3],
+                           const oracle_unbind_params *u, const oracle_rvir_params *r, int stable_ties,
+                           oracle_halo_state *st);
+
+/* sort + rem_outsideRvir(0) + rem_unbound + rem_outsideRvir(1) for every halo
+ * of the snapshot; AHFHALO1 dump with ovdens as the fourth value */
+int oracle_construct_dump(const char *snap_path, const char *out_path, int threads, int stable_ties);
+
 void oracle_sort_halo(const float *pos, uint64_t n, uint64_t *ipart, const double centre[3],
                       int32_t minpart, int stable_ties);
 void oracle_rem_unbound(const float *pos, const float *mom, uint64_t *ipart, uint64_t n,

@@ -11,13 +11,15 @@
  *
  * It follows the call order of src/main.c:614-648 (gen_domgrids, ll,
  * zero_dens + assign_npart, gen_AMRhierarchy) and of the per-halo part of
- * src/libahf/ahf_halos.c:504-510 + ahf_halos_sfc.c:140-149
- * (sort_halo_particles, rem_unbound; the gather and rem_outsideRvir steps
- * are replaced by the halo lists stored in the snapshot).
+ * src/libahf/ahf_halos.c:504-510 + ahf_halos_sfc.c:140-153: `unbind` runs
+ * sort_halo_particles and rem_unbound, `construct` runs sort_halo_particles,
+ * rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1); the gather step is
+ * replaced by the halo lists stored in the snapshot.
  *
  * Usage:
  *   ahf_ref amr    <snap> <grids.out> [--reps R]   dump the AMR hierarchy
  *   ahf_ref unbind <snap> <halos.out> [--threads T] [--reps R]
+ *   ahf_ref construct <snap> <halos.out> [--threads T]
  *   ahf_ref time   <snap> [--threads T] [--reps R] [--skip-amr] [--skip-1thread]
  *   ahf_ref layout                                  struct sizes/offsets (JSON)
  */
@@ -49,7 +51,7 @@
  * src/libahf/ahf_halos.c:163); the drop-in library reads these copies. */
 param_simu simu;
 info_global global;
-double r_fac, v_fac, phi_fac, Hubble;
+double r_fac, v_fac, phi_fac, rho_fac, Hubble;
 #endif
 
 static double now_s(void)
@@ -92,6 +94,10 @@ static void setup_globals(const snap_data *s)
   v_fac = s->h.v_fac;
   phi_fac = s->h.phi_fac;
   Hubble = s->h.hubble;
+  /* src/libahf/ahf_halos.c:204-221 */
+  rho_fac = simu.pmass / (simu.boxsize * simu.boxsize * simu.boxsize);
+  global.ovlim = s->h.ovlim;
+  global.rho_vir = s->h.rho_vir;
 }
 
 static partptr make_particles(const snap_data *s)
@@ -263,22 +269,15 @@ static double run_unbind(HALO *h, uint64_t nh, int threads)
   return now_s() - t0;
 }
 
-static void cmd_unbind(const snap_data *s, const char *out, int threads)
+static void dump_halos(const char *out, const HALO *h, uint64_t nh)
 {
-  setup_globals(s);
-  partptr fst = make_particles(s);
-  global.fst_part = fst;
-  global.no_part = s->h.npart;
-  HALO *h = make_halos(s);
-  double t = run_unbind(h, s->h.nhalos, threads);
   FILE *f = fopen(out, "wb");
   if (!f) die("cannot open output");
   fwrite("AHFHALO1", 1, 8, f);
-  uint64_t nh = s->h.nhalos;
   fwrite(&nh, 8, 1, f);
   for (uint64_t i = 0; i < nh; i++) {
     uint64_t n = h[i].npart;
-    double v[4] = {h[i].M_vir, h[i].R_vir, h[i].Phi0, 0.0};
+    double v[4] = {h[i].M_vir, h[i].R_vir, h[i].Phi0, h[i].ovdens};
     fwrite(&n, 8, 1, f);
     fwrite(v, 8, 4, f);
     for (uint64_t j = 0; j < n; j++) {
@@ -287,38 +286,61 @@ static void cmd_unbind(const snap_data *s, const char *out, int threads)
     }
   }
   fclose(f);
+}
+
+static HALO *halo_setup(const snap_data *s)
+{
+  setup_globals(s);
+  partptr fst = make_particles(s);
+  global.fst_part = fst;
+  global.no_part = s->h.npart;
+  return make_halos(s);
+}
+
+static void cmd_unbind(const snap_data *s, const char *out, int threads)
+{
+  HALO *h = halo_setup(s);
+  double t = run_unbind(h, s->h.nhalos, threads);
+  dump_halos(out, h, s->h.nhalos);
   fprintf(stdout, "{\"unbind_seconds\": %.6f, \"threads\": %d}\n", t, threads);
 }
 
-#ifndef AHF_REF_BUILD
-/* the drop-in library's batched replacement of the per-halo loop */
-static void cmd_unbind_batch(const snap_data *s, const char *out)
+/* the construction steps of src/libahf/ahf_halos_sfc.c:140-153 after the
+ * gather, one halo per thread as in src/libahf/ahf_halos.c:504-510 */
+static void cmd_construct(const snap_data *s, const char *out, int threads)
 {
-  setup_globals(s);
-  partptr fst = make_particles(s);
-  global.fst_part = fst;
-  global.no_part = s->h.npart;
-  HALO *h = make_halos(s);
+  HALO *h = halo_setup(s);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  const uint64_t nh = s->h.nhalos;
   double t0 = now_s();
-  if (ahf_unbind_halos(h, s->h.nhalos)) die("ahf_unbind_halos failed");
-  double t = now_s() - t0;
-  FILE *f = fopen(out, "wb");
-  if (!f) die("cannot open output");
-  fwrite("AHFHALO1", 1, 8, f);
-  uint64_t nh = s->h.nhalos;
-  fwrite(&nh, 8, 1, f);
-  for (uint64_t i = 0; i < nh; i++) {
-    uint64_t n = h[i].npart;
-    double v[4] = {h[i].M_vir, h[i].R_vir, h[i].Phi0, 0.0};
-    fwrite(&n, 8, 1, f);
-    fwrite(v, 8, 4, f);
-    for (uint64_t j = 0; j < n; j++) {
-      uint64_t q = h[i].ipart[j];
-      fwrite(&q, 8, 1, f);
-    }
+  long i;
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++) {
+    sort_halo_particles(h + i);
+    rem_outsideRvir(h + i, 0);
+    rem_unbound(h + i);
+    rem_outsideRvir(h + i, 1);
   }
-  fclose(f);
-  fprintf(stdout, "{\"unbind_seconds\": %.6f}\n", t);
+  double t = now_s() - t0;
+  dump_halos(out, h, nh);
+  fprintf(stdout, "{\"construct_seconds\": %.6f, \"threads\": %d}\n", t, threads);
+}
+
+#ifndef AHF_REF_BUILD
+/* the drop-in library's batched replacements of the per-halo loop */
+static void cmd_batch(const snap_data *s, const char *out, int construct)
+{
+  HALO *h = halo_setup(s);
+  double t0 = now_s();
+  if (construct ? ahf_construct_halos(h, s->h.nhalos) : ahf_unbind_halos(h, s->h.nhalos))
+    die("batched halo pass failed");
+  double t = now_s() - t0;
+  dump_halos(out, h, s->h.nhalos);
+  fprintf(stdout, "{\"batch_seconds\": %.6f}\n", t);
 }
 #endif
 
@@ -415,10 +437,13 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "unbind")) {
     if (argc < 4) die("unbind needs an output path");
     cmd_unbind(&s, argv[3], threads);
+  } else if (!strcmp(argv[1], "construct")) {
+    if (argc < 4) die("construct needs an output path");
+    cmd_construct(&s, argv[3], threads);
 #ifndef AHF_REF_BUILD
-  } else if (!strcmp(argv[1], "unbind-batch")) {
-    if (argc < 4) die("unbind-batch needs an output path");
-    cmd_unbind_batch(&s, argv[3]);
+  } else if (!strcmp(argv[1], "unbind-batch") || !strcmp(argv[1], "construct-batch")) {
+    if (argc < 4) die("a batched command needs an output path");
+    cmd_batch(&s, argv[3], !strcmp(argv[1], "construct-batch"));
 #endif
   } else if (!strcmp(argv[1], "time")) {
     cmd_time(&s, threads, reps < 1 ? 1 : reps, skip_amr, skip_1thread);

@@ -11,6 +11,9 @@
  *     (src/startrun.c:499-517);
  *   - unbinding parameters: NminPerHalo, VescTune and the conversion factors
  *     r_fac, v_fac, phi_fac, Hubble of src/libahf/ahf_halos.c:199-213;
+ *   - the virial overdensity of rem_outsideRvir(): global.ovlim and
+ *     global.rho_vir (src/libahf/ahf_halos.c:204-221; rho_fac there is
+ *     pmass/boxsize^3 and is not stored);
  *   - a halo list: centre (box units, FP64) and member indices (CSR),
  *     the input rem_unbound() sees after sort_halo_particles().
  *
@@ -31,7 +34,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define SNAP_MAGIC "AHFSNAP1"
+#define SNAP_MAGIC "AHFSNAP2"
 
 typedef struct {
   char     magic[8];
@@ -52,6 +55,8 @@ typedef struct {
   uint64_t seed;
   int32_t  config;      /* generator configuration, -1 if hand-made */
   int32_t  pad1;
+  double   ovlim;       /* virial overdensity (global.ovlim) */
+  double   rho_vir;     /* reference density it is relative to (global.rho_vir) */
 } snap_header;
 
 typedef struct {

@@ -26,6 +26,13 @@ def _load_oracle():
     lib = ctypes.CDLL(ORACLE_SO)
     lib.oracle_amr_dump.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
     lib.oracle_unbind_dump.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    lib.oracle_construct_dump.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    lib.oracle_rem_outsideRvir.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_rem_outsideRvir.restype = ctypes.c_uint64
+    lib.oracle_sort_halo.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_int32, ctypes.c_int]
+    lib.oracle_sort_halo.restype = None
     lib.oracle_tsc_weights.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     lib.oracle_tsc_weights.restype = None
     lib.oracle_indexx.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]

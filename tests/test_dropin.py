@@ -1,5 +1,6 @@
 """The drop-in: AHF's own entry points (gen_domgrids, ll, zero_dens,
-assign_npart, gen_AMRhierarchy, sort_halo_particles, rem_unbound) served by
+assign_npart, gen_AMRhierarchy, sort_halo_particles, rem_outsideRvir,
+rem_unbound) served by
 libahf_amd.so, driven by the harness that runs the reference's functions
 (oracle/ref_driver.c), linked two ways:
   - oracle/_build/ahf_dropin_drv: compiled against include/ahf_dropin.h;
@@ -76,6 +77,7 @@ def _halo_match(ref, got, tag):
     # Phi0 comes from a parallel prefix sum on the GPU: FP64 rounding only
     rel = np.abs(ref.Phi0[big] - got.Phi0[big]) / np.maximum(np.abs(ref.Phi0[big]), 1e-300)
     assert rel.size == 0 or rel.max() < 1e-9, f"{tag}: Phi0 rel err {rel.max():.2e}"
+    assert np.array_equal(ref.ovdens, got.ovdens)
 
 
 @pytest.mark.parametrize("cmd", ["unbind", "unbind-batch"])
@@ -89,5 +91,21 @@ def test_dropin_unbind(oracle, drv, snapgen, tmp_path, cmd, cfg, side, seed):
     # the GPU radial sort is stable on equal distances; so is this oracle run
     assert oracle.oracle_unbind_dump(snap.encode(), o_out.encode(), 4, 1) == 0
     extra = ["--threads", "4"] if cmd == "unbind" else []
+    _run([drv, cmd, snap, d_out, *extra])
+    _halo_match(dumps.read_halos(o_out), dumps.read_halos(d_out), f"{cmd} cfg{cfg}/{side}")
+
+
+@pytest.mark.parametrize("cmd", ["construct", "construct-batch"])
+@pytest.mark.parametrize("cfg,side,seed", [(1, 64, 1001), (2, 400, 1002), (3, 48, 5)])
+def test_dropin_construct(oracle, drv, snapgen, tmp_path, cmd, cfg, side, seed):
+    """sort_halo_particles, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1)
+    per halo through AHF's prototypes, or batched"""
+    if cmd == "construct-batch" and drv == LINKED:
+        pytest.skip("the reference's driver has no batched command")
+    snap = str(tmp_path / "c.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    o_out, d_out = str(tmp_path / "o.halo"), str(tmp_path / "d.halo")
+    assert oracle.oracle_construct_dump(snap.encode(), o_out.encode(), 4, 1) == 0
+    extra = ["--threads", "4"] if cmd == "construct" else []
     _run([drv, cmd, snap, d_out, *extra])
     _halo_match(dumps.read_halos(o_out), dumps.read_halos(d_out), f"{cmd} cfg{cfg}/{side}")

@@ -175,3 +175,125 @@ def test_indexx_is_a_sort(oracle):
         perm = idx[1:].astype(np.int64) - 1
         assert sorted(perm.tolist()) == list(range(n))
         assert np.all(np.diff(a[perm]) >= 0)
+
+
+# ---------------------------------------------------------------- rem_outsideRvir
+
+
+class RvirParams(ctypes.Structure):
+    _fields_ = [("minpart", ctypes.c_int32), ("ovlim", ctypes.c_double), ("rho_vir", ctypes.c_double),
+                ("rho_fac", ctypes.c_double)]
+
+
+SCALE = 1e-6   # rho_fac / rho_vir of these tests: haloes of radius ~0.01 box
+
+
+def _rvir(oracle, pos, centre, ipart=None, minpart=20, ovlim=200.0, rho_vir=1.0, rho_fac=SCALE):
+    pos = np.ascontiguousarray(pos, np.float32)
+    ip = np.arange(len(pos), dtype=np.uint64) if ipart is None else np.ascontiguousarray(ipart, np.uint64)
+    c = np.ascontiguousarray(centre, np.float64)
+    out = [ctypes.c_double(-7.0) for _ in range(3)]
+    p = RvirParams(minpart, ovlim, rho_vir, rho_fac)
+    k = oracle.oracle_rem_outsideRvir(pos.ctypes.data, ip.ctypes.data, len(ip), c.ctypes.data, ctypes.byref(p),
+                                      *[ctypes.byref(o) for o in out])
+    return int(k), out[0].value, out[1].value, out[2].value
+
+
+def _shells(rng, ovd, centre):
+    """one member per radius: member j on a random direction at the radius
+    where the sphere through it has overdensity ovd[j] (rho_fac/rho_vir =
+    SCALE)"""
+    n = len(ovd)
+    r = ((np.arange(1, n + 1) * SCALE / (4.0 / 3.0 * np.pi * np.asarray(ovd, float))) ** (1 / 3))
+    u = rng.standard_normal((n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    return (np.asarray(centre) + r[:, None] * u) % 1.0, r
+
+
+def test_rvir_cut_known_answer(oracle):
+    """the first member whose sphere is below ovlim is the last one kept;
+    M_vir counts members, R_vir is its distance, ovdens its sphere's"""
+    rng = np.random.default_rng(11)
+    c = np.array([0.5, 0.5, 0.5])
+    for k in (20, 37, 150):
+        ovd = np.full(200, 400.0)
+        ovd[k:] = 100.0                      # falls below 200 at member k
+        pos, _ = _shells(rng, ovd, c)
+        # radii must keep the list sorted: from k on the shells lie further out
+        d = np.linalg.norm(pos.astype(np.float32).astype(np.float64) - c, axis=1)
+        assert np.all(np.diff(d[k - 1:]) > 0)
+        kept, M, R, od = _rvir(oracle, pos, c)
+        assert kept == k + 1 and M == float(k + 1)
+        assert abs(R - d[k]) <= 1e-15 * d[k]
+        # positions are FP32 near 0.5 (spacing 6e-8) at radii ~0.01
+        assert abs(od / 100.0 - 1.0) < 1e-4
+        assert abs(od - (k + 1) / (4.0 / 3.0 * np.pi * R ** 3) * SCALE) <= 1e-13 * od
+
+
+def test_rvir_cut_no_failure_keeps_all(oracle):
+    rng = np.random.default_rng(12)
+    c = np.array([0.3, 0.6, 0.2])
+    pos, r = _shells(rng, np.full(64, 1000.0), c)
+    kept, M, R, od = _rvir(oracle, pos, c)
+    assert kept == 64 and M == 64.0
+    assert abs(R / r[-1] - 1.0) < 3e-5 and abs(od / 1000.0 - 1.0) < 1e-4
+
+
+def test_rvir_cut_density_units(oracle):
+    """ovdens scales as rho_fac / rho_vir: a factor 4 on rho_fac moves the
+    cut to where the sphere's M/V falls below ovlim/4"""
+    rng = np.random.default_rng(13)
+    c = np.array([0.5, 0.5, 0.5])
+    ovd = np.linspace(1005.0, 15.0, 100)    # M/V decreasing, never within 2% of a limit
+    pos, _ = _shells(rng, ovd, c)
+    k1 = _rvir(oracle, pos, c, ovlim=200.0)[0]
+    k4 = _rvir(oracle, pos, c, ovlim=200.0, rho_fac=4.0 * SCALE)[0]
+    first = lambda lim: int(np.argmax(ovd < lim)) + 1
+    assert k1 == first(200.0) and k4 == first(50.0)
+
+
+def test_rvir_cut_periodic(oracle):
+    """a halo across the box corner cuts as its translate in the middle"""
+    rng = np.random.default_rng(14)
+    ovd = np.concatenate([np.full(30, 500.0), np.full(30, 50.0)])
+    mid = np.array([0.5, 0.5, 0.5])
+    pos, _ = _shells(rng, ovd, mid)
+    edge = np.array([0.999, 0.0005, 0.9999])
+    shifted = (pos - mid + edge) % 1.0
+    assert np.any(np.abs(shifted - edge) > 0.5)   # some members wrap
+    a, b = _rvir(oracle, pos, mid), _rvir(oracle, shifted, edge)
+    assert a[0] == b[0] == 31 and a[1] == b[1]
+    assert abs(a[2] / b[2] - 1.0) < 3e-5
+
+
+def test_rvir_cut_small_and_centre(oracle):
+    rng = np.random.default_rng(15)
+    c = np.array([0.5, 0.5, 0.5])
+    pos, _ = _shells(rng, np.full(30, 10.0), c)
+    # below NminPerHalo: nothing is written
+    assert _rvir(oracle, pos[:19], c) == (19, -7.0, -7.0, -7.0)
+    # a member on the centre has an infinite overdensity and is kept; the
+    # next one is below ovlim and is the last kept
+    pos[0] = c
+    kept, M, R, od = _rvir(oracle, pos, c)
+    assert kept == 2 and M == 2.0
+
+
+@pytest.mark.parametrize("cfg,side,seed", [(0, 64, 1000), (1, 64, 1001), (2, 800, 1002), (3, 48, 5)])
+def test_construct_matches_reference(oracle, ref_bin, snapgen, tmp_path, cfg, side, seed):
+    """sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1) per halo:
+    the oracle against the reference's own functions"""
+    snap = str(tmp_path / "h.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    o, r = str(tmp_path / "o.halo"), str(tmp_path / "r.halo")
+    assert oracle.oracle_construct_dump(snap.encode(), o.encode(), 4, 0) == 0
+    res = subprocess.run([ref_bin, "construct", snap, r, "--threads", "4"], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-2000:]
+    a, b = dumps.read_halos(o), dumps.read_halos(r)
+    _halo_eq(a, b)
+    assert np.array_equal(a.ovdens, b.ovdens)
+    if cfg in (2, 3):
+        # the cuts take members away that unbinding alone keeps
+        u = str(tmp_path / "u.halo")
+        assert oracle.oracle_unbind_dump(snap.encode(), u.encode(), 4, 0) == 0
+        assert int(dumps.read_halos(u).npart.sum()) > int(a.npart.sum())

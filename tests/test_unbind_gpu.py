@@ -1,6 +1,9 @@
-"""GPU parity of sort + unbinding against the CPU oracle: identical bound
-membership and order for every halo, and identical M_vir, R_vir; Phi0 within
-the FP64 rounding of a parallel prefix sum."""
+"""GPU parity of sort + unbinding (and of the rem_outsideRvir cuts around
+it) against the CPU oracle: identical membership and order for every halo,
+identical M_vir, R_vir, ovdens; Phi0 within the FP64 rounding of a parallel
+prefix sum."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -31,7 +34,7 @@ def _compare(ref, npart, props, lists, tag):
             bad.append(h)
     assert not bad, f"{tag}: {len(bad)} haloes differ, first {bad[:5]}"
     assert np.array_equal(ref.M_vir, props[:, 0])
-    set_ = props[:, 4] > 0
+    set_ = (props[:, 5].astype(np.int64) & 2) != 0
     assert np.array_equal(ref.R_vir[set_], props[set_, 1])
     big = ref.npart > 0
     rel = np.abs(ref.Phi0[big] - props[big, 2]) / np.maximum(np.abs(ref.Phi0[big]), 1e-300)
@@ -78,3 +81,72 @@ def test_unbind_repeatable(snapgen, ctx, tmp_path):
     assert np.array_equal(r1[0], r2[0])
     for a, b in zip(r1[2], r2[2]):
         assert np.array_equal(a, b)
+
+
+WITH_RVIR, RVIR_ONLY = 16, 32
+
+
+def _construct_gpu(ctx, s, mode, centre=None, npart=None, ipart=None):
+    ctx.set_particles(s.pos, s.mom)
+    ctx.set_halos(s.centre if centre is None else centre, s.halo_npart if npart is None else npart,
+                  s.ipart if ipart is None else ipart)
+    info = ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=mode,
+                      ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+    return info, ctx.unbind_results()
+
+
+@pytest.mark.parametrize("cfg,side,seed,plain", [(1, 64, 1001, 0), (1, 64, 1001, 1), (0, 64, 1000, 0),
+                                                 (2, 2000, 1002, 0), (2, 300, 7, 1), (3, 48, 5, 0)])
+def test_construct_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, plain):
+    """sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1) in one pass"""
+    snap = str(tmp_path / "c.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    out = str(tmp_path / "o.halo")
+    assert oracle.oracle_construct_dump(snap.encode(), out.encode(), 8, 1) == 0
+    ref = dumps.read_halos(out)
+    s = dumps.read_snapshot(snap)
+    print(f"[construct-gpu] cfg{cfg} side {side} plain {plain}: {s.nhalos} haloes {s.n_ipart} members", flush=True)
+    info, (npart, props, lists) = _construct_gpu(ctx, s, plain | WITH_RVIR)
+    _compare(ref, npart, props, lists, f"construct cfg{cfg}/{side}/{plain}")
+    assert np.array_equal(ref.R_vir, props[:, 1])
+    assert np.array_equal(ref.ovdens, props[:, 3])
+    small = s.halo_npart < s.minpart
+    assert np.all(props[small, 5] == 0) and np.all(props[~small, 5] != 0)
+    assert info.n_member_out == int(ref.npart.sum())
+
+
+class _RvirParams(ctypes.Structure):
+    _fields_ = [("minpart", ctypes.c_int32), ("ovlim", ctypes.c_double), ("rho_vir", ctypes.c_double),
+                ("rho_fac", ctypes.c_double)]
+
+
+@pytest.mark.parametrize("plain", [0, 1])
+def test_rvir_only_matches_oracle(oracle, snapgen, ctx, tmp_path, plain):
+    """one rem_outsideRvir call on radially sorted lists"""
+    snap = str(tmp_path / "r.snap")
+    assert snapgen.ahf_snapgen_write(2, 1000, 31, snap.encode()) == 0
+    s = dumps.read_snapshot(snap)
+    off = s.halo_offsets
+    ip = s.ipart.copy()
+    for h in range(s.nhalos):
+        seg = ip[off[h]:off[h + 1]]
+        c = np.ascontiguousarray(s.centre[h])
+        oracle.oracle_sort_halo(s.pos.ctypes.data, len(seg), seg.ctypes.data, c.ctypes.data, s.minpart, 1)
+    info, (npart, props, lists) = _construct_gpu(ctx, s, plain | RVIR_ONLY, ipart=ip)
+    P = _RvirParams(s.minpart, s.ovlim, s.rho_vir, s.rho_fac)
+    cut = 0
+    for h in range(s.nhalos):
+        seg = np.ascontiguousarray(ip[off[h]:off[h + 1]])
+        c = np.ascontiguousarray(s.centre[h])
+        o = [ctypes.c_double(0.0) for _ in range(3)]
+        k = oracle.oracle_rem_outsideRvir(s.pos.ctypes.data, seg.ctypes.data, len(seg), c.ctypes.data,
+                                          ctypes.byref(P), *[ctypes.byref(x) for x in o])
+        assert int(npart[h]) == k and np.array_equal(lists[h], seg[:k]), f"halo {h}"
+        if len(seg) >= s.minpart:
+            assert (props[h, 0], props[h, 1], props[h, 3]) == (o[0].value, o[1].value, o[2].value), f"halo {h}"
+            assert props[h, 2] == 0.0 and int(props[h, 5]) == 15
+        else:
+            assert int(props[h, 5]) == 0
+        cut += len(seg) - k
+    assert cut > 0
+    assert info.max_iterations == 0
