@@ -87,6 +87,10 @@ def load():
     lib.ahfgpu_unbind_results.argtypes = [P, P, P, P]
     lib.ahfgpu_release_halos.argtypes = [P]
     lib.ahfgpu_release_halos.restype = None
+    lib.ahfgpu_set_sfc_keys.argtypes = [P, P, ctypes.c_uint64]
+    lib.ahfgpu_sfc_keys.argtypes = [P, P]
+    lib.ahfgpu_gather_halos.argtypes = [P, P, P, P, ctypes.c_uint64]
+    lib.ahfgpu_halo_members.argtypes = [P, P, P]
     lib.ahfgpu_timer_count.argtypes = []
     lib.ahfgpu_timer_get.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), P, P, P]
     _lib = lib
@@ -124,6 +128,7 @@ class Context:
             mom = np.ascontiguousarray(mom, dtype=np.float32)
             momp = mom.ctypes.data
         self._check(self.lib.ahfgpu_set_particles(self.h, pos.ctypes.data, momp, pos.shape[0]))
+        self._n = pos.shape[0]
 
     def amr_build(self, L_dom, L_max, Nth_dom, Nth_ref):
         p = AmrParams(L_dom, L_max, Nth_dom, Nth_ref)
@@ -154,6 +159,42 @@ class Context:
         ip = np.ascontiguousarray(ipart, dtype=np.uint64)
         self._check(self.lib.ahfgpu_set_halos(self.h, self._cen.ctypes.data, self._hnp.ctypes.data,
                                               ip.ctypes.data, self._hnp.shape[0]))
+
+    def sfc_keys(self, copy=False):
+        """SFC keys of the resident particles, computed on the device (they
+        must come out in order); returned when copy is set"""
+        import numpy as np
+        out = np.zeros(self._n, np.uint64) if copy else None
+        self._check(self.lib.ahfgpu_sfc_keys(self.h, out.ctypes.data if copy else None))
+        return out
+
+    def set_sfc_keys(self, keys):
+        import numpy as np
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        self._check(self.lib.ahfgpu_set_sfc_keys(self.h, k.ctypes.data, k.shape[0]))
+
+    def gather_halos(self, centres, gather_rad, skip=None):
+        """member lists of every halo from its centre and gather radius (box units)"""
+        import numpy as np
+        self._cen = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1, 3)
+        rad = np.ascontiguousarray(gather_rad, dtype=np.float64)
+        nh = self._cen.shape[0]
+        sk = None if skip is None else np.ascontiguousarray(skip, dtype=np.uint8)
+        self._check(self.lib.ahfgpu_gather_halos(self.h, self._cen.ctypes.data, rad.ctypes.data,
+                                                 None if sk is None else sk.ctypes.data, nh))
+        hnp = np.zeros(nh, np.uint64)
+        self._check(self.lib.ahfgpu_halo_members(self.h, hnp.ctypes.data, None))
+        self._hnp = hnp
+
+    def halo_members(self):
+        """(npart, list of member arrays) of the resident halo lists"""
+        import numpy as np
+        nh = self._hnp.shape[0]
+        npart = np.zeros(nh, np.uint64)
+        ip = np.zeros(int(self._hnp.sum()) + 1, np.uint64)
+        self._check(self.lib.ahfgpu_halo_members(self.h, npart.ctypes.data, ip.ctypes.data))
+        off = np.concatenate([[0], np.cumsum(npart.astype(np.int64))])
+        return npart, [ip[off[h]:off[h + 1]].copy() for h in range(nh)]
 
     # ahfgpu_unbind modes (or-ed: BATCHED/PLAIN with one step selection)
     UNBIND_BATCHED, UNBIND_PLAIN = 0, 1

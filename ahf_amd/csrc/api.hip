@@ -51,6 +51,10 @@ static void free_particles(ahfgpu_ctx *c)
     *p = nullptr;
   }
   c->cap = 0;
+  if (c->sfckey) (void)hipFree(c->sfckey);
+  c->sfckey = nullptr;
+  c->key_cap = 0;
+  c->keys_ok = 0;
 }
 
 extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
@@ -63,6 +67,7 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   c->uarena.release();
   free_particles(c);
   if (c->d_anomaly) (void)hipFree(c->d_anomaly);
+  if (c->d_flag) (void)hipFree(c->d_flag);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   hipEvent_t evs[] = {c->ev_a0, c->ev_a1, c->ev_u0, c->ev_u1};
   for (auto e : evs)
@@ -115,6 +120,7 @@ extern "C" int ahfgpu_set_particles(ahfgpu_ctx *c, const float *pos, const float
   if (e != hipSuccess) return c->fail(AHFGPU_EHIP, "particle upload: %s", hipGetErrorString(e));
   c->N = (int64_t)n;
   c->built = 0;
+  c->keys_ok = 0;
   return 0;
 }
 

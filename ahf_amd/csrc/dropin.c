@@ -22,6 +22,7 @@
  * weak copies only let the library link on its own */
 __attribute__((weak)) param_simu simu;
 __attribute__((weak)) info_global global;
+__attribute__((weak)) global_info_t global_info;
 __attribute__((weak)) double r_fac, v_fac, phi_fac, rho_fac, Hubble;
 
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
@@ -29,6 +30,8 @@ static ahfgpu_ctx *g_ctx;
 static int g_device;
 static const part *g_res_parts; /* particle set resident on the device */
 static uint64_t g_res_n;
+static uint64_t g_upload;        /* uploads so far; keys belong to one */
+static uint64_t g_keys_upload;   /* upload whose keys are resident, 0: none */
 
 static void fatal(const char *what)
 {
@@ -99,6 +102,25 @@ static void ensure_particles(int force)
   if (rc) fatal("particle upload");
   g_res_parts = global.fst_part;
   g_res_n = n;
+  g_upload++;
+}
+
+/* AHF's SFC keys of the resident particles (part.sfckey, set and sorted by
+ * src/main.c:341-356), for the halo gather */
+static void ensure_keys(void)
+{
+  if (global_info.fst_part != global.fst_part || global_info.no_part != global.no_part)
+    fatal("global_info and global hold different particle arrays");
+  ensure_particles(0);
+  if (g_keys_upload == g_upload) return;
+  const uint64_t n = global_info.no_part;
+  uint64_t *k = (uint64_t *)malloc(n * sizeof(uint64_t));
+  if (!k) fatal("out of host memory");
+  for (uint64_t i = 0; i < n; i++) k[i] = global_info.fst_part[i].sfckey;
+  int rc = ahfgpu_set_sfc_keys(ctx_get(), k, n);
+  free(k);
+  if (rc) fatal("SFC keys");
+  g_keys_upload = g_upload;
 }
 
 /* ------------------------------------------------------------ grids ---- */
@@ -551,6 +573,98 @@ int ahf_construct_halos(HALO *halos, long unsigned nhalos)
   if (nhalos == 0) return 0;
   pthread_mutex_lock(&g_lock);
   int rc = run_halos(halos, nhalos, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_WITH_RVIR, 1);
+  pthread_mutex_unlock(&g_lock);
+  return rc;
+}
+
+/* the halo gather for nh haloes; with construct, the steps after it too.
+ * Haloes whose npart is 0 are skipped (src/libahf/ahf_halos_sfc.c:122-123
+ * for the construction; the gather alone appends nothing for them) */
+static int gather_halos(HALO *h, long unsigned nh, int construct)
+{
+  ahfgpu_ctx *c = ctx_get();
+  ensure_keys();
+  double *cen = (double *)xcalloc(4 * nh, 8), *rad = cen + 3 * nh;
+  uint8_t *skip = (uint8_t *)xcalloc(nh, 1);
+  for (long unsigned i = 0; i < nh; i++) {
+    cen[3 * i] = h[i].pos.x;
+    cen[3 * i + 1] = h[i].pos.y;
+    cen[3 * i + 2] = h[i].pos.z;
+    rad[i] = h[i].gatherRad;
+    skip[i] = construct && h[i].npart == 0;
+  }
+  uint64_t *np = (uint64_t *)xcalloc(nh, 8);
+  int rc = ahfgpu_gather_halos(c, cen, rad, skip, nh);
+  if (!rc) rc = ahfgpu_halo_members(c, np, NULL);
+  uint64_t tot = 0;
+  for (long unsigned i = 0; i < nh; i++) tot += np[i];
+  uint64_t *ip = (uint64_t *)xcalloc(tot, 8);
+  double *props = (double *)xcalloc(AHFGPU_PROPS * nh, 8);
+  if (!rc && construct) {
+    ahfgpu_unbind_params up = {simu.AHF_MINPART, 0, simu.AHF_VTUNE, r_fac, v_fac, phi_fac, Hubble,
+                               global.ovlim, global.rho_vir, rho_fac};
+    rc = ahfgpu_unbind(c, &up, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_WITH_RVIR, 0.0, NULL);
+    uint64_t *nin = (uint64_t *)xcalloc(nh, 8);
+    memcpy(nin, np, nh * 8);
+    if (!rc) rc = ahfgpu_unbind_results(c, np, props, ip);
+    /* results sit at the offsets of the gathered lists */
+    uint64_t w = 0, r = 0;
+    for (long unsigned i = 0; !rc && i < nh; i++) {
+      memmove(ip + w, ip + r, np[i] * 8);
+      w += np[i];
+      r += nin[i];
+    }
+    free(nin);
+  } else if (!rc) {
+    rc = ahfgpu_halo_members(c, NULL, ip);
+  }
+  if (!rc) {
+    uint64_t w = 0;
+    for (long unsigned i = 0; i < nh; i++) {
+      if (skip[i]) continue;
+      if (construct) {
+        /* ahf_halos_sfc_constructHalo starts from an empty list */
+        h[i].nll = 0;
+        h[i].ll = NULL;
+        h[i].npart = 0;
+        h[i].ipart = NULL;
+      }
+      if (np[i]) {
+        long unsigned *lst = (long unsigned *)realloc(h[i].ipart, (h[i].npart + np[i]) * 8);
+        if (!lst) fatal("out of host memory");
+        memcpy(lst + h[i].npart, ip + w, np[i] * 8);
+        h[i].ipart = lst;
+        h[i].npart += np[i];
+      }
+      w += np[i];
+      if (construct) {
+        const double *q = props + AHFGPU_PROPS * i;
+        const int set = (int)q[5];
+        if (set & AHFGPU_SET_MVIR) h[i].M_vir = q[0];
+        if (set & AHFGPU_SET_RVIR) h[i].R_vir = q[1];
+        if (set & AHFGPU_SET_PHI0) h[i].Phi0 = q[2];
+        if (set & AHFGPU_SET_OVDENS) h[i].ovdens = q[3];
+      }
+    }
+  }
+  ahfgpu_release_halos(c);
+  free(cen); free(skip); free(np); free(ip); free(props);
+  return rc;
+}
+
+void ahf_halos_sfc_gatherParts(HALO *halo)
+{
+  pthread_mutex_lock(&g_lock);
+  int rc = gather_halos(halo, 1, 0);
+  pthread_mutex_unlock(&g_lock);
+  if (rc) fatal("ahf_halos_sfc_gatherParts");
+}
+
+int ahf_gather_construct_halos(HALO *halos, long unsigned nhalos)
+{
+  if (nhalos == 0) return 0;
+  pthread_mutex_lock(&g_lock);
+  int rc = gather_halos(halos, nhalos, 1);
   pthread_mutex_unlock(&g_lock);
   return rc;
 }

@@ -104,10 +104,12 @@ enum {
   TIMER_RELINK,      /* k_relink_child + radix sort + CSR               */
   TIMER_USORT,       /* unbinding: radial sort                          */
   TIMER_UNBIND,      /* unbinding: iterations                           */
+  TIMER_HGATHER,     /* halo gather: cells, tests, member lists         */
   TIMER_COUNT
 };
-static const char *const k_timer_names[TIMER_COUNT] = {"bin", "prep", "gather_light", "gather_heavy", "refine",
-                                                       "build_level", "relink", "unbind_sort", "unbind"};
+static const char *const k_timer_names[TIMER_COUNT] = {"bin",    "prep",        "gather_light", "gather_heavy",
+                                                       "refine", "build_level", "relink",       "unbind_sort",
+                                                       "unbind", "halo_gather"};
 
 struct TimedLaunch {
   hipEvent_t a, b;
@@ -131,7 +133,12 @@ struct ahfgpu_ctx {
   int rolled_back = 0;
   int built = 0;
   int32_t *d_anomaly = nullptr;
-  void *halo_set = nullptr; /* resident halo lists (unbind_gpu.hip) */
+  void *halo_set = nullptr; /* resident halo lists (halo_set.h) */
+  uint64_t *sfckey = nullptr; /* SFC keys of the resident particles */
+  int64_t key_cap = 0;
+  int keys_ok = 0;            /* keys present and in order */
+  int32_t *d_flag = nullptr;  /* one device word for checks */
+  int64_t gather_candidates = 0;
   int gather_mode = 0;      /* AHFGPU_GATHER_*: deposit kernel choice */
   hipEvent_t ev_a0 = nullptr, ev_a1 = nullptr, ev_u0 = nullptr, ev_u1 = nullptr;
   std::vector<hipEvent_t> ev_pool;

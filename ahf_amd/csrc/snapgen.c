@@ -26,9 +26,11 @@
  *
  * Randomness is a counter-based splitmix64 stream per (seed, object), so the
  * file is byte-identical for a given (config, seed, side) whatever the
- * thread count.  Particles are finally ordered along a 21-bit Morton curve
- * (AHF orders them along a Hilbert curve, src/main.c:341-356; any fixed
- * order is a valid input to the path and the checkers replay the same one).
+ * thread count.  Particles are finally ordered by their 21-bit Hilbert key,
+ * as AHF orders them (src/main.c:341-356); equal keys keep the generation
+ * order.  Each halo gets AHF's gather radius (src/libahf/ahf_halos.c:
+ * 2987-3050): half the distance to the nearest halo with more particles,
+ * at least its virial radius, at most MaxGatherRad = 3 Mpc/h (and 1/4 box).
  */
 #include <math.h>
 #include <stdint.h>
@@ -40,6 +42,7 @@
 #endif
 
 #include "../../oracle/snapfile.h"
+#include "sfc_core.h"
 
 #define RHOC0 2.7755397e11   /* h^2 Msun / Mpc^3, src/param.h:38 */
 #define GRAV 4.3006485e-9    /* Mpc km^2 / (Msun s^2), src/param.h:39 */
@@ -194,16 +197,6 @@ static int cmp_u64pair(const void *a, const void *b)
   return x[1] < y[1] ? -1 : (x[1] > y[1]);
 }
 
-static uint64_t spread21(uint64_t v)
-{
-  v &= 0x1FFFFF;
-  v = (v | v << 32) & 0x1F00000000FFFFull;
-  v = (v | v << 16) & 0x1F0000FF0000FFull;
-  v = (v | v << 8) & 0x100F00F00F00F00Full;
-  v = (v | v << 4) & 0x10C30C30C30C30C3ull;
-  v = (v | v << 2) & 0x1249249249249249ull;
-  return v;
-}
 
 static inline float to_unit(double x_mpc, double box, int periodic)
 {
@@ -213,6 +206,69 @@ static inline float to_unit(double x_mpc, double box, int periodic)
   if (f >= 1.0f) f = 0.0f;
   if (f < 0.0f) f = 0.0f;
   return f;
+}
+
+/*
+ * AHF's gather radius per halo (src/libahf/ahf_halos.c:2987-3050) in box
+ * units: half the periodic distance to the nearest halo with more
+ * particles, or maxGathRad when there is none; then at least the virial
+ * radius and at most maxGathRad = min(MaxGatherRad/box, 1/4) with
+ * MaxGatherRad = 3 Mpc/h.  Only haloes within 2 maxGathRad can set a
+ * radius below the cap, so a cell grid of that width finds them.
+ */
+static double *gather_radii(const gen_params *g, const halo_spec *hs, const int64_t *np, const double *cen, int64_t nh)
+{
+  double *grad = (double *)malloc((size_t)(nh ? nh : 1) * sizeof(double));
+  if (!grad) return NULL;
+  const double gmax = fmin(3.0 / g->box, 0.25);
+  int nc = (int)floor(1.0 / (2.0 * gmax));
+  if (nc < 3) nc = 1; /* fewer than 3 cells a side: one cell, the whole box */
+  const int64_t ncell = (int64_t)nc * nc * nc;
+  int64_t *head = (int64_t *)malloc((size_t)ncell * sizeof(int64_t));
+  int64_t *next = (int64_t *)malloc((size_t)(nh ? nh : 1) * sizeof(int64_t));
+  if (!head || !next) { free(grad); free(head); free(next); return NULL; }
+  for (int64_t c = 0; c < ncell; c++) head[c] = -1;
+  int *cc = (int *)malloc((size_t)(nh ? nh : 1) * 3 * sizeof(int));
+  if (!cc) { free(grad); free(head); free(next); return NULL; }
+  for (int64_t i = 0; i < nh; i++) {
+    for (int d = 0; d < 3; d++) {
+      int c = (int)floor(cen[3 * i + d] * nc);
+      cc[3 * i + d] = c < 0 ? 0 : c >= nc ? nc - 1 : c;
+    }
+    const int64_t c = ((int64_t)cc[3 * i + 2] * nc + cc[3 * i + 1]) * nc + cc[3 * i];
+    next[i] = head[c];
+    head[c] = i;
+  }
+  const int reach = nc >= 3 ? 1 : 0;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < nh; i++) {
+    double best = 1e300;
+    int found = 0;
+    for (int a = -reach; a <= reach; a++)
+      for (int b = -reach; b <= reach; b++)
+        for (int e = -reach; e <= reach; e++) {
+          const int x = (cc[3 * i] + e + nc) % nc, y = (cc[3 * i + 1] + b + nc) % nc,
+                    z = (cc[3 * i + 2] + a + nc) % nc;
+          for (int64_t j = head[((int64_t)z * nc + y) * nc + x]; j >= 0; j = next[j]) {
+            if (j == i || np[j] <= np[i]) continue;
+            double d2 = 0.0;
+            for (int d = 0; d < 3; d++) {
+              double dd = fabs(cen[3 * i + d] - cen[3 * j + d]);
+              if (dd > 0.5) dd = 1.0 - dd;
+              d2 += dd * dd;
+            }
+            if (d2 < best) best = d2;
+            found = 1;
+          }
+        }
+    double r = found ? 0.5 * sqrt(best) : gmax;
+    const double rv = r_vir_of(hs[i].M) / g->box;
+    if (r < rv) r = rv;
+    if (r > gmax) r = gmax;
+    grad[i] = r;
+  }
+  free(head); free(next); free(cc);
+  return grad;
 }
 
 static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo_spec *hs,
@@ -268,15 +324,12 @@ static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo
     }
   }
 
-  /* Morton order */
+  /* Hilbert order */
   uint64_t *kv = (uint64_t *)malloc((size_t)N * 2 * sizeof(uint64_t));
   if (!kv) return -1;
 #pragma omp parallel for schedule(static)
   for (int64_t p = 0; p < N; p++) {
-    uint64_t ix = (uint64_t)(pos[3 * p + 0] * 2097152.0f);
-    uint64_t iy = (uint64_t)(pos[3 * p + 1] * 2097152.0f);
-    uint64_t iz = (uint64_t)(pos[3 * p + 2] * 2097152.0f);
-    kv[2 * p] = spread21(ix) | spread21(iy) << 1 | spread21(iz) << 2;
+    kv[2 * p] = ahf_sfc_key((double)pos[3 * p + 0], (double)pos[3 * p + 1], (double)pos[3 * p + 2], AHF_SFC_BITS);
     kv[2 * p + 1] = (uint64_t)p;
   }
   qsort(kv, (size_t)N, 2 * sizeof(uint64_t), cmp_u64pair);
@@ -325,6 +378,8 @@ static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo
       cen[3 * i + d] = u;
     }
   }
+  double *grad = gather_radii(g, hs, hostn, cen, nh);
+  if (!grad) return -1;
 
   snap_header hd;
   memset(&hd, 0, sizeof(hd));
@@ -363,11 +418,12 @@ static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo
     bad |= fwrite(mom2, sizeof(float), (size_t)N * 3, f) != (size_t)N * 3;
     bad |= fwrite(id, sizeof(uint64_t), (size_t)N, f) != (size_t)N;
     bad |= fwrite(cen, sizeof(double), (size_t)nh * 3, f) != (size_t)nh * 3;
+    bad |= fwrite(grad, sizeof(double), (size_t)nh, f) != (size_t)nh;
     bad |= fwrite(hnp, sizeof(uint64_t), (size_t)nh, f) != (size_t)nh;
     bad |= fwrite(ipart, sizeof(uint64_t), (size_t)n_ip, f) != (size_t)n_ip;
     bad |= fclose(f) != 0;
   }
-  free(newidx); free(pos2); free(mom2); free(id); free(hostn);
+  free(newidx); free(pos2); free(mom2); free(id); free(hostn); free(grad);
   free(ipart); free(hnp); free(cen); free(fill);
   return bad ? -2 : 0;
 }

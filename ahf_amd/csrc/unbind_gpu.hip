@@ -42,6 +42,7 @@
 #include "../../include/ahf_gpu.h"
 #include "devprims.h"
 #include "gpu_ctx.h"
+#include "halo_set.h"
 
 using namespace ahf;
 
@@ -863,102 +864,32 @@ __global__ void k_copy_u32(const uint32_t *__restrict__ in, int64_t n, uint32_t 
 
 /* ============================================================ host side */
 
-struct HaloSet {
-  int64_t nh = 0, ne = 0;
-  int64_t *off = nullptr;     /* nh+1 */
-  uint32_t *ip_in = nullptr;  /* ne: members as given */
-  double *cen = nullptr;      /* nh*3 */
-  int32_t *order = nullptr;   /* haloes largest first */
-  std::vector<int64_t> h_off;
-  HaloOut *out = nullptr;
-  uint32_t *buf_a = nullptr, *buf_b = nullptr;
-  int done = 0;
-  /* one device block and one pinned host block, laid out alike (off, cen,
-   * order, ip_in) and kept across calls: a caller that unbinds one halo
-   * at a time pays one copy and no allocation per call */
-  char *dev = nullptr, *host = nullptr;
-  size_t cap = 0;
-};
-
 static inline unsigned nb_of(int64_t n)
 {
   return (unsigned)((n + AHF_BLOCK - 1) / AHF_BLOCK);
 }
-
-static void free_halos(HaloSet *s)
-{
-  if (!s) return;
-  if (s->dev) (void)hipFree(s->dev);
-  if (s->host) (void)hipHostFree(s->host);
-  delete s;
-}
-
-static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 extern "C" int ahfgpu_set_halos(ahfgpu_ctx *c, const double *centres, const uint64_t *npart, const uint64_t *ipart,
                                 uint64_t nh)
 {
   if (!c || (nh && (!centres || !npart))) return AHFGPU_EINVAL;
   if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
-  HaloSet *s = (HaloSet *)c->halo_set;
-  if (!s) s = new HaloSet();
-  c->halo_set = nullptr;
-  s->done = 0;
-  s->out = nullptr;
-  s->buf_a = s->buf_b = nullptr;
-  s->nh = (int64_t)nh;
-  s->h_off.assign(nh + 1, 0);
-  for (uint64_t h = 0; h < nh; h++) s->h_off[h + 1] = s->h_off[h] + (int64_t)npart[h];
-  s->ne = s->h_off[nh];
-  if (s->ne >= (int64_t)UINT32_MAX) {
-    free_halos(s);
-    return c->fail(AHFGPU_EINVAL, "at most 2^32-1 halo members in total");
-  }
-  if (s->ne && !ipart) {
-    free_halos(s);
-    return AHFGPU_EINVAL;
-  }
-  for (int64_t e = 0; e < s->ne; e++)
-    if (ipart[e] >= (uint64_t)c->N) {
-      free_halos(s);
+  std::vector<int64_t> off(nh + 1, 0);
+  for (uint64_t h = 0; h < nh; h++) off[h + 1] = off[h] + (int64_t)npart[h];
+  const int64_t ne = off[nh];
+  if (ne >= (int64_t)UINT32_MAX) return c->fail(AHFGPU_EINVAL, "at most 2^32-1 halo members in total");
+  if (ne && !ipart) return AHFGPU_EINVAL;
+  for (int64_t e = 0; e < ne; e++)
+    if (ipart[e] >= (uint64_t)c->N)
       return c->fail(AHFGPU_EINVAL, "member %lld refers to particle %llu beyond the resident set", (long long)e,
                      (unsigned long long)ipart[e]);
-    }
-  const size_t o_off = 0, o_cen = align256((nh + 1) * sizeof(int64_t));
-  const size_t o_ord = o_cen + align256((3 * nh + 1) * sizeof(double));
-  const size_t o_ip = o_ord + align256((nh + 1) * sizeof(int32_t));
-  const size_t need = o_ip + align256(((size_t)s->ne + 1) * sizeof(uint32_t));
-  if (need > s->cap) {
-    /* the previous call's copy may still read the pinned block */
-    (void)hipStreamSynchronize(c->st);
-    if (s->dev) (void)hipFree(s->dev);
-    if (s->host) (void)hipHostFree(s->host);
-    s->dev = s->host = nullptr;
-    s->cap = 0;
-    const size_t cap = need + need / 2;
-    if (hipMalloc((void **)&s->dev, cap) != hipSuccess || hipHostMalloc((void **)&s->host, cap) != hipSuccess) {
-      (void)hipGetLastError();
-      free_halos(s);
-      return c->fail(AHFGPU_ENOMEM, "halo arrays");
-    }
-    s->cap = cap;
-  } else {
-    (void)hipStreamSynchronize(c->st);
-  }
-  int64_t *h_off = (int64_t *)(s->host + o_off);
-  double *h_cen = (double *)(s->host + o_cen);
-  int32_t *h_ord = (int32_t *)(s->host + o_ord);
-  uint32_t *h_ip = (uint32_t *)(s->host + o_ip);
-  memcpy(h_off, s->h_off.data(), (nh + 1) * sizeof(int64_t));
-  if (nh) memcpy(h_cen, centres, 3 * nh * sizeof(double));
-  for (uint64_t h = 0; h < nh; h++) h_ord[h] = (int32_t)h;
-  std::stable_sort(h_ord, h_ord + nh, [&](int32_t x, int32_t y) { return npart[x] > npart[y]; });
-  for (int64_t e = 0; e < s->ne; e++) h_ip[e] = (uint32_t)ipart[e];
-  s->off = (int64_t *)(s->dev + o_off);
-  s->cen = (double *)(s->dev + o_cen);
-  s->order = (int32_t *)(s->dev + o_ord);
-  s->ip_in = (uint32_t *)(s->dev + o_ip);
-  if (hipMemcpyAsync(s->dev, s->host, need, hipMemcpyHostToDevice, c->st) != hipSuccess) {
+  HaloSet *s = nullptr;
+  int rc = halo_set_reserve(c, (int64_t)nh, ne, &s);
+  if (rc) return rc;
+  halo_set_host_meta(s, off.data(), centres);
+  uint32_t *h_ip = (uint32_t *)(s->host + s->o_ip);
+  for (int64_t e = 0; e < ne; e++) h_ip[e] = (uint32_t)ipart[e];
+  if (hipMemcpyAsync(s->dev, s->host, s->need, hipMemcpyHostToDevice, c->st) != hipSuccess) {
     free_halos(s);
     return c->fail(AHFGPU_EHIP, "halo upload");
   }

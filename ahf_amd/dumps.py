@@ -1,5 +1,5 @@
 """Readers for the grid-hierarchy and halo dumps (AHFGRID1 / AHFHALO1) and the
-snapshot container (AHFSNAP2).
+snapshot container (AHFSNAP3).
 
 The same dump layout is written by the reference harness
 (oracle/ref_driver.c), the CPU restatement (oracle/ahf_oracle.c) and this
@@ -43,6 +43,7 @@ class Snapshot:
     mom: np.ndarray          # (npart, 3) float32
     ids: np.ndarray          # (npart,) uint64
     centre: np.ndarray       # (nhalos, 3) float64
+    gather_rad: np.ndarray   # (nhalos,) float64, box units
     halo_npart: np.ndarray   # (nhalos,) uint64
     ipart: np.ndarray        # (n_ipart,) uint64
 
@@ -63,26 +64,28 @@ def read_snapshot(path: str) -> Snapshot:
     with open(path, "rb") as f:
         raw = f.read(SNAP_HEADER.size)
         v = SNAP_HEADER.unpack(raw)
-        if v[0] != b"AHFSNAP2":
-            raise ValueError(f"{path}: not an AHFSNAP2 file")
+        if v[0] != b"AHFSNAP3":
+            raise ValueError(f"{path}: not an AHFSNAP3 file")
         (npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, _p0, vt, rf, vf, pf, hb,
          nh, ni, seed, cfg, _p1, ovl, rhv) = v[1:]
         pos = np.fromfile(f, dtype=np.float32, count=3 * npart).reshape(npart, 3)
         mom = np.fromfile(f, dtype=np.float32, count=3 * npart).reshape(npart, 3)
         ids = np.fromfile(f, dtype=np.uint64, count=npart)
         cen = np.fromfile(f, dtype=np.float64, count=3 * nh).reshape(nh, 3)
+        grad = np.fromfile(f, dtype=np.float64, count=nh)
         hnp = np.fromfile(f, dtype=np.uint64, count=nh)
         ip = np.fromfile(f, dtype=np.uint64, count=ni)
     return Snapshot(npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, vt, rf, vf, pf, hb,
-                    nh, ni, seed, cfg, ovl, rhv, pos, mom, ids, cen, hnp, ip)
+                    nh, ni, seed, cfg, ovl, rhv, pos, mom, ids, cen, grad, hnp, ip)
 
 
 def write_snapshot(path: str, pos, mom=None, ids=None, *, boxsize=100.0, pmass=1e10,
                    L_dom=16, L_max=None, Nth_dom=5.0, Nth_ref=5.0, minpart=20, vtune=1.5,
                    r_fac=None, v_fac=None, phi_fac=None, hubble=100.0, centres=None,
                    halo_lists=None, seed=0, config=-1, omega0=0.3, lambda0=0.7, a=1.0,
-                   ovlim=200.0, rho_vir=2.7755397e11):
-    """Write a hand-made snapshot (tests).  pos in box units."""
+                   ovlim=200.0, rho_vir=2.7755397e11, gather_rad=None):
+    """Write a hand-made snapshot (tests).  pos in box units; gather_rad per
+    halo (box units, default 0.05)."""
     pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
     n = pos.shape[0]
     mom = np.zeros((n, 3), np.float32) if mom is None else np.ascontiguousarray(mom, np.float32)
@@ -95,9 +98,11 @@ def write_snapshot(path: str, pos, mom=None, ids=None, *, boxsize=100.0, pmass=1
     centres = np.zeros((0, 3)) if centres is None else np.asarray(centres, np.float64).reshape(-1, 3)
     halo_lists = [] if halo_lists is None else halo_lists
     hnp = np.array([len(h) for h in halo_lists], dtype=np.uint64)
+    grad = (np.full(len(halo_lists), 0.05) if gather_rad is None
+            else np.asarray(gather_rad, np.float64).reshape(len(halo_lists)))
     ip = (np.concatenate([np.asarray(h, np.uint64) for h in halo_lists])
           if halo_lists else np.zeros(0, np.uint64))
-    hdr = SNAP_HEADER.pack(b"AHFSNAP2", n, boxsize, omega0, lambda0, a, pmass, L_dom, L_max,
+    hdr = SNAP_HEADER.pack(b"AHFSNAP3", n, boxsize, omega0, lambda0, a, pmass, L_dom, L_max,
                            Nth_dom, Nth_ref, minpart, 0, vtune, r_fac, v_fac, phi_fac, hubble,
                            len(halo_lists), ip.size, seed, config, 0, ovlim, rho_vir)
     with open(path, "wb") as f:
@@ -106,6 +111,7 @@ def write_snapshot(path: str, pos, mom=None, ids=None, *, boxsize=100.0, pmass=1
         mom.tofile(f)
         ids.tofile(f)
         centres.astype(np.float64).tofile(f)
+        grad.tofile(f)
         hnp.tofile(f)
         ip.tofile(f)
 

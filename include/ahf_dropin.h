@@ -12,6 +12,8 @@
  *          mean_dens, no_part, no_vpart
  *   global fst_part, no_part, dom_grid, domgrid_no, fst_cycle, fin_l1dim,
  *          ovlim, rho_vir
+ *   global_info  fst_part, no_part (the same array as global's, ordered by
+ *          part.sfckey, src/main.c:341-356, 624-625)
  *   r_fac, v_fac, phi_fac, rho_fac, Hubble
  *
  * Errors: like the reference, the grid functions terminate the process with
@@ -29,6 +31,7 @@ extern "C" {
 
 extern param_simu simu;
 extern info_global global;
+extern global_info_t global_info;
 extern double r_fac, v_fac, phi_fac, rho_fac, Hubble;
 
 /* src/libamr_serial/generate_grids.h:7 -- the domain grid, with AHF's quads */
@@ -68,6 +71,21 @@ void rem_unbound(HALO *halo);
  * halo->ipart is realloc()ed as in the reference.  Thread-safe like
  * rem_unbound. */
 void rem_outsideRvir(HALO *halo, int icall);
+
+/* src/libahf/ahf_halos_sfc.h:28 -- append to halo->ipart the particles
+ * within halo->gatherRad of halo->pos, in the reference's order (search
+ * cells in block order, particles in key order); halo->ipart is realloc()ed
+ * and halo->npart grows.  Reads part.sfckey of global_info's particles,
+ * which must be in key order.  Thread-safe like rem_unbound. */
+void ahf_halos_sfc_gatherParts(HALO *halo);
+
+/* No counterpart in the reference: the whole of
+ * ahf_halos_sfc_constructHalo (src/libahf/ahf_halos_sfc.c:119-157) but
+ * HaloProfiles, for all nhalos haloes in one batched GPU pass: haloes with
+ * npart == 0 are left alone; the others start from an empty list and get
+ * gatherParts, sort_halo_particles, rem_outsideRvir(0), rem_unbound and
+ * rem_outsideRvir(1).  Returns 0, or a negative AHFGPU_E* code. */
+int ahf_gather_construct_halos(HALO *halos, long unsigned nhalos);
 
 /* No counterpart in the reference: the construction steps of
  * src/libahf/ahf_halos_sfc.c:140-153 after the gather -- sort_halo_particles,

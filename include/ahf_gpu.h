@@ -141,6 +141,26 @@ int ahfgpu_set_gather_mode(ahfgpu_ctx *ctx, int mode);
 int ahfgpu_set_halos(ahfgpu_ctx *ctx, const double *centres, const uint64_t *npart, const uint64_t *ipart,
                      uint64_t nh);
 
+/* SFC keys of the resident particles (AHF's part.sfckey, 21 bits per axis
+ * on AHF's Hilbert curve, src/main.c:341-356): uploaded from the caller, or
+ * computed on the device.  The particles must be in key order, as AHF
+ * sorts them; both calls check it and fail with AHFGPU_EINVAL otherwise.
+ * ahfgpu_set_particles drops the keys.  keys_out may be NULL. */
+int ahfgpu_set_sfc_keys(ahfgpu_ctx *ctx, const uint64_t *keys, uint64_t n);
+int ahfgpu_sfc_keys(ahfgpu_ctx *ctx, uint64_t *keys_out);
+
+/* ahf_halos_sfc_gatherParts (src/libahf/ahf_halos_sfc.c:171-367) for nh
+ * haloes at once: the members of halo h are the resident particles within
+ * gather_rad[h] (box units) of centres[h] (periodic), in AHF's order (the
+ * cells of the 3x3x3 search block in block order, each cell in key order).
+ * skip[h] != 0 gives halo h no members (AHF skips a halo whose grid count
+ * is 0, ahf_halos_sfc.c:122-123); skip may be NULL.  The lists replace the
+ * resident halo lists, as ahfgpu_set_halos would, ready for ahfgpu_unbind;
+ * ahfgpu_halo_members reads them back. */
+int ahfgpu_gather_halos(ahfgpu_ctx *ctx, const double *centres, const double *gather_rad, const uint8_t *skip,
+                        uint64_t nh);
+int ahfgpu_halo_members(ahfgpu_ctx *ctx, uint64_t *npart, uint64_t *ipart);
+
 #define AHFGPU_UNBIND_BATCHED 0   /* all haloes in one persistent launch   */
 #define AHFGPU_UNBIND_PLAIN 1     /* one launch per halo, synchronised     */
 #define AHFGPU_UNBIND_SORT_ONLY 4 /* radial sort only (sort_halo_particles) */

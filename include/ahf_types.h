@@ -140,6 +140,20 @@ typedef struct info_global {
   int           ioflag;
 } info_global;
 
+/* src/common.h:77-92 without WITH_MPI: the particle array as the SFC halo
+ * gather sees it (the same array as info_global.fst_part, src/main.c:624),
+ * ordered by sfckey */
+typedef struct global_info_struct {
+  partptr   fst_part;
+  uint64_t  no_part;
+  sfc_key_t minkey;
+  sfc_key_t maxkey;
+  int       ctype;                /* sfc_curve_t: SFC_CURVE_HILBERT = 0 */
+  int       level;
+  int       rank;
+  int       size;
+} global_info_t;
+
 typedef struct tline {
   double age[AHF_MAXTIME], hubble[AHF_MAXTIME], omega[AHF_MAXTIME],
          lambda[AHF_MAXTIME], virial[AHF_MAXTIME], growth[AHF_MAXTIME],

@@ -1209,3 +1209,213 @@ int oracle_unbind_dump(const char *snap_path, const char *out_path, int threads,
   snap_free(&s);
   return 0;
 }
+
+/* ===================================================== SFC keys, gather */
+
+#define SFC_BITS 21          /* BITS_PER_DIMENSION, src/param.h:92 */
+#define GATHER_FAC 1.001     /* GATHERRAD_FAC, src/libahf/ahf_halos_sfc.c:46 */
+
+/*
+ * Hilbert index of grid cell c[0..2] on a 2^bits grid: the curve of
+ * src/libsfc/hilbert.c:197-241 (Butz's algorithm as D. Moore implements
+ * it), restated digit by digit.  The per-level digits hold one bit of each
+ * coordinate (x in bit 0); each is XORed with the digit of the level above,
+ * then, from the top, with a flip bit and rotated right by a rotation that
+ * both depend on the previous output digit.  The output digits below the
+ * top one get bit 2 flipped, and the whole index is Gray-decoded.
+ */
+uint64_t oracle_hilbert_c2i(const uint64_t c[3], unsigned bits)
+{
+  unsigned dig[64];
+  for (unsigned l = 0; l < bits; l++) {
+    const unsigned b = bits - 1 - l;
+    dig[l] = (unsigned)(((c[0] >> b) & 1) | (((c[1] >> b) & 1) << 1) | (((c[2] >> b) & 1) << 2));
+  }
+  for (unsigned l = bits; l-- > 1;) dig[l] ^= dig[l - 1];
+  uint64_t h = 0;
+  unsigned rot = 0, flip = 0;
+  for (unsigned l = 0; l < bits; l++) {
+    const unsigned v = (dig[l] ^ flip) & 7u;
+    const unsigned t = rot ? ((v >> rot) | (v << (3 - rot))) & 7u : v;
+    h = (h << 3) | t;
+    flip = 1u << rot;
+    rot = (rot + 1 + ((t & 1u) ? 1 : (t & 2u) ? 2 : 0)) % 3;
+  }
+  for (unsigned l = 1; l < bits; l++) h ^= (uint64_t)4 << (3 * (bits - 1 - l));
+  uint64_t r = 0, acc = 0;
+  for (int i = 3 * (int)bits - 1; i >= 0; i--) {
+    acc ^= (h >> i) & 1;
+    r |= acc << i;
+  }
+  return r;
+}
+
+/* sfc_curve_calcKey (src/libsfc/hilbert_util.c:68-92): truncate to the
+ * 2^bits grid, the last cell taking the upper face */
+uint64_t oracle_sfc_key(double x, double y, double z, unsigned bits)
+{
+  const uint64_t max = (uint64_t)1 << bits;
+  const double p[3] = {x, y, z};
+  uint64_t c[3];
+  for (int d = 0; d < 3; d++) {
+    c[d] = (uint64_t)trunc(p[d] * (double)max);
+    if (c[d] == max) c[d] = max - 1;
+  }
+  return oracle_hilbert_c2i(c, bits);
+}
+
+/* first index of the sorted keys at or above k (what util_bsearch followed
+ * by the step back over equal keys returns in local_findMinOffset,
+ * src/libahf/ahf_halos_sfc.c:370-414), n if none */
+static uint64_t lower_key(const uint64_t *key, uint64_t n, uint64_t k)
+{
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (key[mid] < k) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+static double sep1(double a, double b)
+{
+  double d = fabs(a - b);
+  if (d > 0.5) d = 1.0 - d;
+  return d;
+}
+
+/*
+ * ahf_halos_sfc_gatherParts, src/libahf/ahf_halos_sfc.c:171-367, for one
+ * halo: the members (indices into the key-sorted particles) are appended
+ * to out (capacity grown with realloc) in the order the reference adds them.
+ * Returns the member count; *n_cand receives the particles tested.
+ */
+uint64_t oracle_gather_halo(const float *pos, const uint64_t *key, uint64_t n, const double c[3], double rad,
+                            uint64_t **out, uint64_t *n_cand)
+{
+  unsigned bits = 1;
+  while ((GATHER_FAC * rad < 1. / (1 << (bits + 1))) && (bits + 1) <= SFC_BITS) bits++;
+  const double rad2 = rad * rad;
+  uint64_t ncell = 0, cells[27];
+  if (bits == 1) {
+    for (uint64_t k = 0; k < 8; k++) cells[ncell++] = k;
+  } else {
+    const uint64_t m = (uint64_t)1 << bits;
+    uint64_t b[3];
+    for (int d = 0; d < 3; d++) {
+      b[d] = (uint64_t)trunc(c[d] * (double)m);
+      if (b[d] == m) b[d] = m - 1;
+    }
+    const double g = 1. / (double)m;
+    const double corner = 0.5 * sqrt(3.) * g;
+    for (int i = -1; i <= 1; i++)
+      for (int j = -1; j <= 1; j++)
+        for (int k = -1; k <= 1; k++) {
+          const uint64_t q[3] = {(b[0] + m + i) % m, (b[1] + m + j) % m, (b[2] + m + k) % m};
+          double d2 = 0.0;
+          const double dx = sep1(g * q[0] + 0.5 * g, c[0]), dy = sep1(g * q[1] + 0.5 * g, c[1]),
+                       dz = sep1(g * q[2] + 0.5 * g, c[2]);
+          d2 = dx * dx + dy * dy + dz * dz;
+          if (sqrt(d2) > corner + GATHER_FAC * rad) continue;
+          cells[ncell++] = oracle_hilbert_c2i(q, bits);
+        }
+  }
+  const unsigned sh = 3 * (SFC_BITS - bits);
+  uint64_t cnt = 0, cap = 0, tested = 0;
+  for (uint64_t i = 0; i < ncell; i++) {
+    const uint64_t kmin = cells[i] << sh, kmax = kmin + (((uint64_t)1 << sh) - 1);
+    for (uint64_t o = lower_key(key, n, kmin); o < n && key[o] <= kmax; o++) {
+      tested++;
+      const float *p = pos + 3 * o;
+      const double dx = sep1((double)p[0], c[0]), dy = sep1((double)p[1], c[1]), dz = sep1((double)p[2], c[2]);
+      if (dx * dx + dy * dy + dz * dz <= rad2) {
+        if (cnt == cap) {
+          cap = cap ? 2 * cap : 64;
+          *out = (uint64_t *)realloc(*out, cap * sizeof(uint64_t));
+        }
+        (*out)[cnt++] = o;
+      }
+    }
+  }
+  if (n_cand) *n_cand = tested;
+  return cnt;
+}
+
+/* keys of every particle of a snapshot (main.c:343-350); 1 if they are in
+ * non-decreasing order, as AHF's sort leaves them */
+static int snap_keys(const snap_data *s, uint64_t *key)
+{
+  const uint64_t n = s->h.npart;
+  long i;
+#pragma omp parallel for schedule(static) private(i)
+  for (i = 0; i < (long)n; i++)
+    key[i] = oracle_sfc_key((double)s->pos[3 * i], (double)s->pos[3 * i + 1], (double)s->pos[3 * i + 2], SFC_BITS);
+  for (uint64_t j = 1; j < n; j++)
+    if (key[j] < key[j - 1]) return 0;
+  return 1;
+}
+
+/*
+ * The whole of ahf_halos_sfc_constructHalo (src/libahf/ahf_halos_sfc.c:
+ * 119-157) but the profiles, for every halo of the snapshot: haloes whose
+ * snapshot list is empty are skipped (:122-123); the others get npart = 0,
+ * then gatherParts from the centre and gather radius, and, when construct
+ * is set, sort_halo_particles, rem_outsideRvir(0), rem_unbound,
+ * rem_outsideRvir(1).  Writes the AHFHALO1 dump; *n_cand receives the
+ * particles the gather tested.  Returns 0, -1 (bad snapshot) or -3
+ * (particles not in key order).
+ */
+int oracle_gather_dump(const char *snap_path, const char *out_path, int threads, int stable_ties, int construct,
+                       uint64_t *n_cand)
+{
+  snap_data s;
+  if (snap_read(snap_path, &s)) return -1;
+  const uint64_t nh = s.h.nhalos, n = s.h.npart;
+  uint64_t *key = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+  if (!snap_keys(&s, key)) {
+    free(key);
+    snap_free(&s);
+    return -3;
+  }
+  uint64_t **lists = (uint64_t **)calloc(nh ? nh : 1, sizeof(uint64_t *));
+  uint64_t *cand = (uint64_t *)calloc(nh ? nh : 1, sizeof(uint64_t));
+  oracle_halo_state *st = (oracle_halo_state *)calloc(nh ? nh : 1, sizeof(oracle_halo_state));
+  const oracle_unbind_params U = {s.h.minpart, s.h.vtune, s.h.r_fac, s.h.v_fac, s.h.phi_fac, s.h.hubble};
+  const double b = s.h.boxsize;
+  const oracle_rvir_params R = {s.h.minpart, s.h.ovlim, s.h.rho_vir, s.h.pmass / (b * b * b)};
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  long i;
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++) {
+    if (s.halo_npart[i] == 0) continue;
+    const uint64_t m = oracle_gather_halo(s.pos, key, n, s.centre + 3 * i, s.gather_rad[i], &lists[i], &cand[i]);
+    st[i].npart = m;
+    if (construct)
+      oracle_construct_halo(s.pos, s.mom, lists[i], m, s.centre + 3 * i, &U, &R, stable_ties, &st[i]);
+  }
+  uint64_t tot = 0;
+  for (uint64_t h = 0; h < nh; h++) tot += cand[h];
+  if (n_cand) *n_cand = tot;
+  FILE *f = fopen(out_path, "wb");
+  int rc = f ? 0 : -2;
+  if (f) {
+    put(f, "AHFHALO1", 1, 8);
+    put(f, &nh, 8, 1);
+    for (uint64_t h = 0; h < nh; h++) {
+      const uint64_t m = s.halo_npart[h] == 0 ? 0 : st[h].npart;
+      double v[4] = {st[h].M_vir, st[h].R_vir, st[h].Phi0, st[h].ovdens};
+      put(f, &m, 8, 1);
+      put(f, v, 8, 4);
+      if (m) put(f, lists[h], 8, (size_t)m);
+    }
+    fclose(f);
+  }
+  for (uint64_t h = 0; h < nh; h++) free(lists[h]);
+  free(lists); free(cand); free(st); free(key);
+  snap_free(&s);
+  return rc;
+}

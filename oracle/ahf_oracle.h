@@ -78,6 +78,23 @@ void oracle_construct_halo(const float *pos, const float *mom, uint64_t *ipart, 
  * of the snapshot; AHFHALO1 dump with ovdens as the fourth value */
 int oracle_construct_dump(const char *snap_path, const char *out_path, int threads, int stable_ties);
 
+/* Hilbert index of a grid cell (src/libsfc/hilbert.c:197) and the SFC key
+ * of a position (src/libsfc/hilbert_util.c:68) */
+uint64_t oracle_hilbert_c2i(const uint64_t c[3], unsigned bits);
+uint64_t oracle_sfc_key(double x, double y, double z, unsigned bits);
+
+/* ahf_halos_sfc_gatherParts for one halo over key-sorted particles: the
+ * members are written to *out (realloc'ed), their count returned */
+uint64_t oracle_gather_halo(const float *pos, const uint64_t *key, uint64_t n, const double centre[3],
+                            double gather_rad, uint64_t **out, uint64_t *n_cand);
+
+/* gatherParts (construct = 0), or gatherParts + sort + rem_outsideRvir(0) +
+ * rem_unbound + rem_outsideRvir(1) (construct = 1), for every halo of the
+ * snapshot from its centre and gather radius; AHFHALO1 dump.  -3 when the
+ * particles are not in key order. */
+int oracle_gather_dump(const char *snap_path, const char *out_path, int threads, int stable_ties, int construct,
+                       uint64_t *n_cand);
+
 void oracle_sort_halo(const float *pos, uint64_t n, uint64_t *ipart, const double centre[3],
                       int32_t minpart, int stable_ties);
 void oracle_rem_unbound(const float *pos, const float *mom, uint64_t *ipart, uint64_t n,

@@ -20,6 +20,10 @@
  *   ahf_ref amr    <snap> <grids.out> [--reps R]   dump the AMR hierarchy
  *   ahf_ref unbind <snap> <halos.out> [--threads T] [--reps R]
  *   ahf_ref construct <snap> <halos.out> [--threads T]
+ *   ahf_ref gather <snap> <halos.out> [--threads T]  ahf_halos_sfc_gatherParts
+ *   ahf_ref construct-sfc <snap> <halos.out> [--threads T]
+ *                       gatherParts + the construct steps, per halo
+ *   ahf_ref keys <snap> <keys.out> <bits>          SFC key of every particle
  *   ahf_ref time   <snap> [--threads T] [--reps R] [--skip-amr] [--skip-1thread]
  *   ahf_ref layout                                  struct sizes/offsets (JSON)
  */
@@ -40,8 +44,18 @@
 #include "tdef.h"
 #include "libamr_serial/amr_serial.h"
 #include "libahf/ahf_halos.h"
+#include "libahf/ahf_halos_sfc.h"
+#include "libsfc/sfc_curve.h"
+#define DRV_SFC_KEY(x, y, z, bits) sfc_curve_calcKey(SFC_CURVE_HILBERT, (x), (y), (z), (bits))
+#define DRV_SFC_BITS BITS_PER_DIMENSION
 #else
 #include "ahf_dropin.h"
+/* AHF computes the keys itself before the path (src/main.c:341-356); this
+ * build has no reference library, so the harness uses the header the
+ * library shares with its kernels */
+#include "../ahf_amd/csrc/sfc_core.h"
+#define DRV_SFC_KEY(x, y, z, bits) ahf_sfc_key((x), (y), (z), (bits))
+#define DRV_SFC_BITS AHF_SFC_BITS
 #endif
 
 #include "snapfile.h"
@@ -51,6 +65,7 @@
  * src/libahf/ahf_halos.c:163); the drop-in library reads these copies. */
 param_simu simu;
 info_global global;
+global_info_t global_info;
 double r_fac, v_fac, phi_fac, rho_fac, Hubble;
 #endif
 
@@ -330,14 +345,85 @@ static void cmd_construct(const snap_data *s, const char *out, int threads)
   fprintf(stdout, "{\"construct_seconds\": %.6f, \"threads\": %d}\n", t, threads);
 }
 
+/* SFC keys of every particle and the gather's view of them
+ * (src/main.c:341-356); the snapshot already holds the particles in key
+ * order, which is checked, not re-sorted */
+static void setup_sfc(const snap_data *s)
+{
+  partptr p = global.fst_part;
+  const uint64_t n = s->h.npart;
+  for (uint64_t i = 0; i < n; i++)
+    p[i].sfckey = DRV_SFC_KEY((double)p[i].pos[0], (double)p[i].pos[1], (double)p[i].pos[2], DRV_SFC_BITS);
+  for (uint64_t i = 1; i < n; i++)
+    if (p[i].sfckey < p[i - 1].sfckey) die("particles are not in SFC key order");
+  global_info.fst_part = p;
+  global_info.no_part = n;
+  global_info.ctype = 0; /* SFC_CURVE_HILBERT */
+}
+
+static void cmd_keys(const snap_data *s, const char *out, unsigned bits)
+{
+  FILE *f = fopen(out, "wb");
+  if (!f) die("cannot open output");
+  for (uint64_t i = 0; i < s->h.npart; i++) {
+    uint64_t k = DRV_SFC_KEY((double)s->pos[3 * i], (double)s->pos[3 * i + 1], (double)s->pos[3 * i + 2], bits);
+    fwrite(&k, 8, 1, f);
+  }
+  fclose(f);
+}
+
+/* ahf_halos_sfc_constructHalo (src/libahf/ahf_halos_sfc.c:119-157) without
+ * HaloProfiles: haloes with an empty list are skipped, the others start
+ * from npart = 0 and gather from their centre and gatherRad */
+static void cmd_gather(const snap_data *s, const char *out, int threads, int construct)
+{
+  HALO *h = halo_setup(s);
+  setup_sfc(s);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  const uint64_t nh = s->h.nhalos;
+  double t0 = now_s();
+  long i;
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++) {
+    HALO *x = h + i;
+    x->gatherRad = s->gather_rad[i];
+    if (x->npart == 0) continue;
+    x->nll = 0;
+    x->ll = NULL;
+    x->npart = 0;
+    free(x->ipart);
+    x->ipart = NULL;
+    ahf_halos_sfc_gatherParts(x);
+    if (construct) {
+      sort_halo_particles(x);
+      rem_outsideRvir(x, 0);
+      rem_unbound(x);
+      rem_outsideRvir(x, 1);
+    }
+  }
+  double t = now_s() - t0;
+  dump_halos(out, h, nh);
+  fprintf(stdout, "{\"gather_seconds\": %.6f, \"threads\": %d}\n", t, threads);
+}
+
 #ifndef AHF_REF_BUILD
 /* the drop-in library's batched replacements of the per-halo loop */
 static void cmd_batch(const snap_data *s, const char *out, int construct)
 {
   HALO *h = halo_setup(s);
   double t0 = now_s();
-  if (construct ? ahf_construct_halos(h, s->h.nhalos) : ahf_unbind_halos(h, s->h.nhalos))
-    die("batched halo pass failed");
+  if (construct == 2) {
+    setup_sfc(s);
+    for (uint64_t i = 0; i < s->h.nhalos; i++) h[i].gatherRad = s->gather_rad[i];
+  }
+  const int rc = construct == 2 ? ahf_gather_construct_halos(h, s->h.nhalos)
+                 : construct    ? ahf_construct_halos(h, s->h.nhalos)
+                                : ahf_unbind_halos(h, s->h.nhalos);
+  if (rc) die("batched halo pass failed");
   double t = now_s() - t0;
   dump_halos(out, h, s->h.nhalos);
   fprintf(stdout, "{\"batch_seconds\": %.6f}\n", t);
@@ -388,8 +474,8 @@ static void cmd_layout(void)
   printf("\"sizeof.cquad\": %zu, \"sizeof.pquad\": %zu, \"sizeof.gridls\": %zu, ", sizeof(cquad),
          sizeof(pquad), sizeof(gridls));
   printf("\"sizeof.HALO\": %zu, \"sizeof.HALOPROFILE\": %zu, ", sizeof(HALO), sizeof(HALOPROFILE));
-  printf("\"sizeof.param_simu\": %zu, \"sizeof.info_global\": %zu, ", sizeof(param_simu),
-         sizeof(info_global));
+  printf("\"sizeof.param_simu\": %zu, \"sizeof.info_global\": %zu, \"sizeof.global_info_t\": %zu, ", sizeof(param_simu),
+         sizeof(info_global), sizeof(global_info_t));
   OFF(part, pos); OFF(part, mom); OFF(part, sfckey); OFF(part, id);
   OFF(node, dens); OFF(node, ll); OFF(node, force);
   OFF(nquad, loc); OFF(nquad, x); OFF(nquad, length); OFF(nquad, next);
@@ -403,6 +489,8 @@ static void cmd_layout(void)
   OFF(info_global, dom_grid); OFF(info_global, domgrid_no); OFF(info_global, fst_part);
   OFF(info_global, no_part); OFF(info_global, a); OFF(info_global, fin_l1dim);
   OFF(info_global, fst_cycle); OFF(info_global, ioflag);
+  OFF(global_info_t, fst_part); OFF(global_info_t, no_part); OFF(global_info_t, ctype);
+  OFF(global_info_t, size);
   OFF(param_simu, NGRID_DOM); OFF(param_simu, NGRID_MAX); OFF(param_simu, Nth_dom);
   OFF(param_simu, Nth_ref); OFF(param_simu, AHF_VTUNE); OFF(param_simu, AHF_MINPART);
   OFF(param_simu, timeline); OFF(param_simu, mean_dens); OFF(param_simu, boxsize);
@@ -440,10 +528,17 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "construct")) {
     if (argc < 4) die("construct needs an output path");
     cmd_construct(&s, argv[3], threads);
+  } else if (!strcmp(argv[1], "gather") || !strcmp(argv[1], "construct-sfc")) {
+    if (argc < 4) die("gather needs an output path");
+    cmd_gather(&s, argv[3], threads, !strcmp(argv[1], "construct-sfc"));
+  } else if (!strcmp(argv[1], "keys")) {
+    if (argc < 5) die("keys needs an output path and a bit count");
+    cmd_keys(&s, argv[3], (unsigned)atoi(argv[4]));
 #ifndef AHF_REF_BUILD
-  } else if (!strcmp(argv[1], "unbind-batch") || !strcmp(argv[1], "construct-batch")) {
+  } else if (!strcmp(argv[1], "unbind-batch") || !strcmp(argv[1], "construct-batch") ||
+             !strcmp(argv[1], "construct-sfc-batch")) {
     if (argc < 4) die("a batched command needs an output path");
-    cmd_batch(&s, argv[3], !strcmp(argv[1], "construct-batch"));
+    cmd_batch(&s, argv[3], !strcmp(argv[1], "construct-sfc-batch") ? 2 : !strcmp(argv[1], "construct-batch"));
 #endif
   } else if (!strcmp(argv[1], "time")) {
     cmd_time(&s, threads, reps < 1 ? 1 : reps, skip_amr, skip_1thread);

@@ -14,8 +14,13 @@
  *   - the virial overdensity of rem_outsideRvir(): global.ovlim and
  *     global.rho_vir (src/libahf/ahf_halos.c:204-221; rho_fac there is
  *     pmass/boxsize^3 and is not stored);
- *   - a halo list: centre (box units, FP64) and member indices (CSR),
- *     the input rem_unbound() sees after sort_halo_particles().
+ *   - a halo list: centre (box units, FP64), gather radius (box units, the
+ *     HALO.gatherRad that ahf_halos_sfc_gatherParts() receives) and member
+ *     indices (CSR), the input rem_unbound() sees after
+ *     sort_halo_particles().
+ *
+ * Generated snapshots hold their particles in Hilbert-key order, as AHF
+ * sorts them (src/main.c:341-356).
  *
  * Layout (little endian, no padding between sections):
  *   snap_header
@@ -23,6 +28,7 @@
  *   float    mom[npart][3]
  *   uint64_t id[npart]
  *   double   centre[nhalos][3]
+ *   double   gather_rad[nhalos]
  *   uint64_t halo_npart[nhalos]
  *   uint64_t ipart[n_ipart]
  */
@@ -34,7 +40,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define SNAP_MAGIC "AHFSNAP2"
+#define SNAP_MAGIC "AHFSNAP3"
 
 typedef struct {
   char     magic[8];
@@ -65,6 +71,7 @@ typedef struct {
   float    *mom;        /* npart*3 */
   uint64_t *id;         /* npart */
   double   *centre;     /* nhalos*3 */
+  double   *gather_rad; /* nhalos */
   uint64_t *halo_npart; /* nhalos */
   uint64_t *ipart;      /* n_ipart */
 } snap_data;
@@ -83,13 +90,15 @@ static inline int snap_read(const char *path, snap_data *s)
   s->mom = (float *)malloc(n * 3 * sizeof(float) + 1);
   s->id = (uint64_t *)malloc(n * sizeof(uint64_t) + 1);
   s->centre = (double *)malloc(nh * 3 * sizeof(double) + 1);
+  s->gather_rad = (double *)malloc(nh * sizeof(double) + 1);
   s->halo_npart = (uint64_t *)malloc(nh * sizeof(uint64_t) + 1);
   s->ipart = (uint64_t *)malloc(ni * sizeof(uint64_t) + 1);
-  int bad = !s->pos || !s->mom || !s->id || !s->centre || !s->halo_npart || !s->ipart;
+  int bad = !s->pos || !s->mom || !s->id || !s->centre || !s->gather_rad || !s->halo_npart || !s->ipart;
   bad = bad || fread(s->pos, sizeof(float), n * 3, f) != n * 3;
   bad = bad || fread(s->mom, sizeof(float), n * 3, f) != n * 3;
   bad = bad || fread(s->id, sizeof(uint64_t), n, f) != n;
   bad = bad || fread(s->centre, sizeof(double), nh * 3, f) != nh * 3;
+  bad = bad || fread(s->gather_rad, sizeof(double), nh, f) != nh;
   bad = bad || fread(s->halo_npart, sizeof(uint64_t), nh, f) != nh;
   bad = bad || fread(s->ipart, sizeof(uint64_t), ni, f) != ni;
   fclose(f);
@@ -99,7 +108,7 @@ static inline int snap_read(const char *path, snap_data *s)
 static inline void snap_free(snap_data *s)
 {
   free(s->pos); free(s->mom); free(s->id);
-  free(s->centre); free(s->halo_npart); free(s->ipart);
+  free(s->centre); free(s->gather_rad); free(s->halo_npart); free(s->ipart);
   memset(s, 0, sizeof(*s));
 }
 

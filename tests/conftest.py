@@ -33,6 +33,12 @@ def _load_oracle():
     lib.oracle_sort_halo.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_int32, ctypes.c_int]
     lib.oracle_sort_halo.restype = None
+    lib.oracle_hilbert_c2i.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+    lib.oracle_hilbert_c2i.restype = ctypes.c_uint64
+    lib.oracle_sfc_key.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint]
+    lib.oracle_sfc_key.restype = ctypes.c_uint64
+    lib.oracle_gather_dump.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p]
     lib.oracle_tsc_weights.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     lib.oracle_tsc_weights.restype = None
     lib.oracle_indexx.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]

@@ -14,7 +14,8 @@ int main(void)
   printf("\"sizeof.cquad\": %zu, \"sizeof.pquad\": %zu, \"sizeof.gridls\": %zu, ", sizeof(cquad), sizeof(pquad),
          sizeof(gridls));
   printf("\"sizeof.HALO\": %zu, \"sizeof.HALOPROFILE\": %zu, ", sizeof(HALO), sizeof(HALOPROFILE));
-  printf("\"sizeof.param_simu\": %zu, \"sizeof.info_global\": %zu, ", sizeof(param_simu), sizeof(info_global));
+  printf("\"sizeof.param_simu\": %zu, \"sizeof.info_global\": %zu, \"sizeof.global_info_t\": %zu, ", sizeof(param_simu), sizeof(info_global),
+         sizeof(global_info_t));
   OFF(part, pos); OFF(part, mom); OFF(part, sfckey); OFF(part, id);
   OFF(node, dens); OFF(node, ll); OFF(node, force);
   OFF(nquad, loc); OFF(nquad, x); OFF(nquad, length); OFF(nquad, next);
@@ -28,6 +29,8 @@ int main(void)
   OFF(info_global, dom_grid); OFF(info_global, domgrid_no); OFF(info_global, fst_part);
   OFF(info_global, no_part); OFF(info_global, a); OFF(info_global, fin_l1dim);
   OFF(info_global, fst_cycle); OFF(info_global, ioflag);
+  OFF(global_info_t, fst_part); OFF(global_info_t, no_part); OFF(global_info_t, ctype);
+  OFF(global_info_t, size);
   OFF(param_simu, NGRID_DOM); OFF(param_simu, NGRID_MAX); OFF(param_simu, Nth_dom);
   OFF(param_simu, Nth_ref); OFF(param_simu, AHF_VTUNE); OFF(param_simu, AHF_MINPART);
   OFF(param_simu, timeline); OFF(param_simu, mean_dens); OFF(param_simu, boxsize);

@@ -109,3 +109,20 @@ def test_dropin_construct(oracle, drv, snapgen, tmp_path, cmd, cfg, side, seed):
     extra = ["--threads", "4"] if cmd == "construct" else []
     _run([drv, cmd, snap, d_out, *extra])
     _halo_match(dumps.read_halos(o_out), dumps.read_halos(d_out), f"{cmd} cfg{cfg}/{side}")
+
+
+@pytest.mark.parametrize("cmd", ["gather", "construct-sfc", "construct-sfc-batch"])
+@pytest.mark.parametrize("cfg,side,seed", [(1, 48, 1001), (3, 48, 5), (2, 400, 1002)])
+def test_dropin_gather(oracle, drv, snapgen, tmp_path, cmd, cfg, side, seed):
+    """ahf_halos_sfc_gatherParts per halo through AHF's prototype (alone, and
+    followed by the construction steps), or the whole construction batched"""
+    if cmd == "construct-sfc-batch" and drv == LINKED:
+        pytest.skip("the reference's driver has no batched command")
+    snap = str(tmp_path / "g.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    o_out, d_out = str(tmp_path / "o.halo"), str(tmp_path / "d.halo")
+    construct = 0 if cmd == "gather" else 1
+    assert oracle.oracle_gather_dump(snap.encode(), o_out.encode(), 4, 1, construct, None) == 0
+    extra = [] if cmd.endswith("batch") else ["--threads", "4"]
+    _run([drv, cmd, snap, d_out, *extra])
+    _halo_match(dumps.read_halos(o_out), dumps.read_halos(d_out), f"{cmd} cfg{cfg}/{side}")

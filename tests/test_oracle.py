@@ -297,3 +297,111 @@ def test_construct_matches_reference(oracle, ref_bin, snapgen, tmp_path, cfg, si
         u = str(tmp_path / "u.halo")
         assert oracle.oracle_unbind_dump(snap.encode(), u.encode(), 4, 0) == 0
         assert int(dumps.read_halos(u).npart.sum()) > int(a.npart.sum())
+
+
+# ------------------------------------------------------- SFC keys and gather
+
+
+def _c2i(oracle, x, y, z, bits):
+    c = np.array([x, y, z], np.uint64)
+    return int(oracle.oracle_hilbert_c2i(c.ctypes.data, bits))
+
+
+@pytest.mark.parametrize("bits", [1, 2, 3, 4])
+def test_hilbert_is_a_hilbert_curve(oracle, bits):
+    """every cell once, consecutive keys in face-adjacent cells, and the
+    key of a cell's parent is the key with its last 3 bits dropped"""
+    m = 1 << bits
+    cells = {}
+    for x in range(m):
+        for y in range(m):
+            for z in range(m):
+                cells[_c2i(oracle, x, y, z, bits)] = (x, y, z)
+    assert sorted(cells) == list(range(m ** 3))
+    for k in range(m ** 3 - 1):
+        a, b = np.array(cells[k]), np.array(cells[k + 1])
+        assert np.abs(a - b).sum() == 1, (k, cells[k], cells[k + 1])
+    if bits > 1:
+        for k, (x, y, z) in cells.items():
+            assert _c2i(oracle, x >> 1, y >> 1, z >> 1, bits - 1) == k >> 3
+
+
+def test_sfc_keys_match_reference(oracle, ref_bin, snapgen, tmp_path):
+    snap = str(tmp_path / "k.snap")
+    assert snapgen.ahf_snapgen_write(1, 32, 3, snap.encode()) == 0
+    s = dumps.read_snapshot(snap)
+    # the positions AHF can hold at the edges of the unit box
+    edge = np.array([[0, 0, 0], [0.5, 0.5, 0.5], [np.nextafter(np.float32(1), np.float32(0))] * 3,
+                     [0.25, 0.75, 1e-7]], np.float32)
+    snap2 = str(tmp_path / "e.snap")
+    dumps.write_snapshot(snap2, np.concatenate([edge, s.pos[:4000]]))
+    pos = dumps.read_snapshot(snap2).pos.astype(np.float64)
+    for bits in (1, 2, 5, 13, 21):
+        out = str(tmp_path / "k.bin")
+        subprocess.run([ref_bin, "keys", snap2, out, str(bits)], check=True, capture_output=True)
+        ref = np.fromfile(out, np.uint64)
+        ours = np.array([oracle.oracle_sfc_key(p[0], p[1], p[2], bits) for p in pos], np.uint64)
+        assert np.array_equal(ref, ours), bits
+
+
+def _brute_gather(s, key, h):
+    """every particle within the gather radius, periodic, ordered by search
+    cell (block order) and key: the set and order the gather must give"""
+    c, R = s.centre[h], s.gather_rad[h]
+    d = np.abs(s.pos.astype(np.float64) - c)
+    d = np.where(d > 0.5, 1.0 - d, d)
+    inside = np.nonzero((d * d).sum(axis=1) <= R * R)[0]
+    bits = 1
+    while 1.001 * R < 1.0 / (1 << (bits + 1)) and bits + 1 <= 21:
+        bits += 1
+    cell = (key[inside] >> np.uint64(3 * (21 - bits))).astype(np.int64)
+    if bits == 1:
+        rank = cell
+    else:
+        m = 1 << bits
+        b = np.minimum(np.trunc(c * m).astype(np.int64), m - 1)
+        g = np.stack([(s.pos[inside, i].astype(np.float64) * m).astype(np.int64) for i in range(3)], 1)
+        g = np.minimum(g, m - 1)
+        off = (g - b + 1) % m          # 0, 1, 2 inside the block
+        assert np.all(off <= 2)
+        rank = off[:, 0] * 9 + off[:, 1] * 3 + off[:, 2]
+    return inside[np.lexsort((inside, rank))]
+
+
+def test_gather_brute_force(oracle, snapgen, tmp_path):
+    snap = str(tmp_path / "g.snap")
+    assert snapgen.ahf_snapgen_write(1, 40, 17, snap.encode()) == 0
+    s = dumps.read_snapshot(snap)
+    key = np.array([oracle.oracle_sfc_key(*p, 21) for p in s.pos.astype(np.float64)], np.uint64)
+    assert np.all(np.diff(key.astype(np.float64)) >= 0)
+    out = str(tmp_path / "g.halo")
+    nc = ctypes.c_uint64()
+    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 4, 0, 0, ctypes.byref(nc)) == 0
+    g = dumps.read_halos(out)
+    assert s.nhalos > 5
+    for h in range(s.nhalos):
+        assert np.array_equal(g.ipart[h], _brute_gather(s, key, h)), h
+    assert nc.value >= int(g.npart.sum())
+
+
+@pytest.mark.parametrize("cfg,side,seed", [(1, 48, 1001), (0, 40, 1000), (3, 48, 5), (2, 400, 1002)])
+def test_gather_matches_reference(oracle, ref_bin, snapgen, tmp_path, cfg, side, seed):
+    """gatherParts alone, and followed by the construction steps, against
+    the reference's own functions"""
+    snap = str(tmp_path / "g.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    for cmd, construct in (("gather", 0), ("construct-sfc", 1)):
+        o, r = str(tmp_path / "o.halo"), str(tmp_path / "r.halo")
+        assert oracle.oracle_gather_dump(snap.encode(), o.encode(), 4, 0, construct, None) == 0
+        res = subprocess.run([ref_bin, cmd, snap, r, "--threads", "4"], capture_output=True, text=True)
+        assert res.returncode == 0, res.stderr[-2000:]
+        a, b = dumps.read_halos(o), dumps.read_halos(r)
+        _halo_eq(a, b)
+        assert np.array_equal(a.ovdens, b.ovdens)
+
+
+def test_gather_rejects_unsorted(oracle, tmp_path):
+    rng = np.random.default_rng(3)
+    snap = str(tmp_path / "u.snap")
+    dumps.write_snapshot(snap, rng.random((500, 3)), centres=[[0.5, 0.5, 0.5]], halo_lists=[np.arange(30)])
+    assert oracle.oracle_gather_dump(snap.encode(), str(tmp_path / "u.halo").encode(), 1, 0, 0, None) == -3
