@@ -6,13 +6,14 @@
  *
  * The particles are ordered by key, so each cell of a halo's 3x3x3 search
  * block is a contiguous range of them.  All haloes at once:
- *   1. one thread per halo finds its (up to) 27 cells and their particle
- *      ranges by binary search over the keys;
+ *   1. two threads per cell of every halo's block find the cell's particle
+ *      range by binary search over the keys;
  *   2. the ranges of all haloes, in halo order and block order, are laid
  *      end to end by a scan: candidate q is one particle of one cell of one
  *      halo, and q's order is the order in which AHF adds members;
- *   3. one thread per candidate tests the distance; a scan of the results
- *      gives every member its place in the halo's list (CSR over haloes).
+ *   3. one thread per candidate tests the distance (a table gives the cell
+ *      of every run of 256 candidates); a scan of the results gives every
+ *      member its place in the halo's list (CSR over haloes).
  * The lists land in the context's resident halo set, ready for the
  * unbinding pass.
  */
@@ -58,54 +59,66 @@ __device__ inline int64_t lower_key(const uint64_t *__restrict__ key, int64_t n,
   return lo;
 }
 
-/* the particle ranges of every halo's search cells, AHF_SHELL slots per
- * halo in block order; skipped haloes (skip[h] != 0) get empty slots */
+/* the particle range of one search cell: thread t handles slot t % 27 of
+ * halo t / 27 and one of its two bounds (the first index at or above kmin,
+ * or above kmax); skipped haloes (skip[h] != 0) and far cells get empty
+ * ranges */
 __global__ void k_gather_cells(const double *__restrict__ cen, const double *__restrict__ rad,
-                               const uint8_t *__restrict__ skip, int64_t nh, const uint64_t *__restrict__ key,
-                               int64_t n, int64_t *__restrict__ cnt, int64_t *__restrict__ lo)
+                               const uint8_t *__restrict__ skip, int64_t nslot, const uint64_t *__restrict__ key,
+                               int64_t n, int64_t *__restrict__ bound /* 2 per slot */)
 {
-  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= nh) return;
-  uint64_t kmin[AHF_SHELL], kmax[AHF_SHELL];
-  if (skip[h]) {
-    for (int s = 0; s < AHF_SHELL; s++) { cnt[AHF_SHELL * h + s] = 0; lo[AHF_SHELL * h + s] = 0; }
-    return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * nslot) return;
+  const int64_t slot = t >> 1, h = slot / AHF_SHELL;
+  const int s = (int)(slot % AHF_SHELL);
+  int64_t b = 0;
+  if (!skip[h]) {
+    uint64_t kmin, kmax;
+    ahf_gather_cell(cen[3 * h], cen[3 * h + 1], cen[3 * h + 2], rad[h], s, &kmin, &kmax);
+    if (kmin <= kmax) b = lower_key(key, n, (t & 1) ? kmax + 1 : kmin);
   }
-  ahf_gather_cells(cen[3 * h], cen[3 * h + 1], cen[3 * h + 2], rad[h], kmin, kmax);
-  for (int s = 0; s < AHF_SHELL; s++) {
-    int64_t a = 0, b = 0;
-    if (kmin[s] <= kmax[s]) {
-      a = lower_key(key, n, kmin[s]);
-      b = lower_key(key, n, kmax[s] + 1);
-    }
-    lo[AHF_SHELL * h + s] = a;
-    cnt[AHF_SHELL * h + s] = b - a;
-  }
+  bound[t] = b;
 }
 
-/* the slot holding candidate q: the last slot starting at or before q (an
- * empty slot shares its start with the next one, so the last is the one
- * that holds q) */
-__device__ inline int64_t slot_of(const int64_t *__restrict__ start, int64_t nslot, int64_t q)
+__global__ void k_gather_counts(const int64_t *__restrict__ bound, int64_t nslot, int64_t *__restrict__ cnt)
 {
-  int64_t lo = 0, hi = nslot;
-  while (lo < hi) {
-    const int64_t mid = lo + ((hi - lo) >> 1);
-    if (start[mid] <= q) lo = mid + 1; else hi = mid;
-  }
-  return lo - 1;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nslot) cnt[s] = bound[2 * s + 1] - bound[2 * s];
 }
 
-__global__ void k_gather_test(const int64_t *__restrict__ start, const int64_t *__restrict__ lo, int64_t nslot,
-                              int64_t nc, const double *__restrict__ cen, const double *__restrict__ rad,
-                              const float *__restrict__ px, const float *__restrict__ py,
-                              const float *__restrict__ pz, uint8_t *__restrict__ flag)
+#define AHF_CAND_BLOCK 256 /* candidates per entry of the block -> slot table */
+
+/* for every run of AHF_CAND_BLOCK candidates, the slot holding its first
+ * candidate: each non-empty slot claims the runs that start inside it */
+__global__ void k_gather_blockslot(const int64_t *__restrict__ start, int64_t nslot, int32_t *__restrict__ tab)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslot) return;
+  const int64_t a = start[s], e = start[s + 1];
+  if (a == e) return;
+  for (int64_t b = (a + AHF_CAND_BLOCK - 1) / AHF_CAND_BLOCK; b * AHF_CAND_BLOCK < e; b++) tab[b] = (int32_t)s;
+}
+
+/* the slot holding candidate q: from the slot of q's run, forward past the
+ * slots that end at or before q (empty slots share their start with the
+ * next one) */
+__device__ inline int64_t slot_of(const int64_t *__restrict__ start, const int32_t *__restrict__ tab, int64_t q)
+{
+  int64_t s = tab[q / AHF_CAND_BLOCK];
+  while (start[s + 1] <= q) s++;
+  return s;
+}
+
+__global__ void k_gather_test(const int64_t *__restrict__ start, const int64_t *__restrict__ bound,
+                              const int32_t *__restrict__ tab, int64_t nc, const double *__restrict__ cen,
+                              const double *__restrict__ rad, const float *__restrict__ px,
+                              const float *__restrict__ py, const float *__restrict__ pz, uint8_t *__restrict__ flag)
 {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nc) return;
-  const int64_t s = slot_of(start, nslot, q);
+  const int64_t s = slot_of(start, tab, q);
   const int64_t h = s / AHF_SHELL;
-  const int64_t p = lo[s] + (q - start[s]);
+  const int64_t p = bound[2 * s] + (q - start[s]);
   const double r = rad[h];
   flag[q] = (uint8_t)ahf_gather_member(px[p], py[p], pz[p], cen[3 * h], cen[3 * h + 1], cen[3 * h + 2], r * r);
 }
@@ -120,14 +133,14 @@ __global__ void k_gather_off(const int64_t *__restrict__ start, int64_t nh, int6
   off[h] = (int64_t)pos[q < nc ? q : nc];
 }
 
-__global__ void k_gather_write(const int64_t *__restrict__ start, const int64_t *__restrict__ lo, int64_t nslot,
-                               int64_t nc, const uint8_t *__restrict__ flag, const uint32_t *__restrict__ pos,
-                               uint32_t *__restrict__ ip)
+__global__ void k_gather_write(const int64_t *__restrict__ start, const int64_t *__restrict__ bound,
+                               const int32_t *__restrict__ tab, int64_t nc, const uint8_t *__restrict__ flag,
+                               const uint32_t *__restrict__ pos, uint32_t *__restrict__ ip)
 {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nc || !flag[q]) return;
-  const int64_t s = slot_of(start, nslot, q);
-  ip[pos[q]] = (uint32_t)(lo[s] + (q - start[s]));
+  const int64_t s = slot_of(start, tab, q);
+  ip[pos[q]] = (uint32_t)(bound[2 * s] + (q - start[s]));
 }
 
 static int ensure_key_array(ahfgpu_ctx *c)
@@ -203,9 +216,9 @@ extern "C" int ahfgpu_gather_halos(ahfgpu_ctx *c, const double *centres, const d
   const int64_t nslot = AHF_SHELL * (int64_t)nh;
   double *d_in = A.get<double>(4 * (int64_t)nh);
   uint8_t *d_skip = A.get<uint8_t>((int64_t)nh);
-  int64_t *cnt = A.get<int64_t>(nslot), *lo = A.get<int64_t>(nslot), *start = A.get<int64_t>(nslot + 1);
+  int64_t *cnt = A.get<int64_t>(nslot), *bound = A.get<int64_t>(2 * nslot), *start = A.get<int64_t>(nslot + 1);
   int64_t *stmp = A.get<int64_t>(scan_tmp_elems(nslot));
-  if (!d_in || !d_skip || !cnt || !lo || !start || !stmp)
+  if (!d_in || !d_skip || !cnt || !bound || !start || !stmp)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (gather)");
   double *d_cen = d_in, *d_rad = d_in + 3 * nh;
   c->timers_reset(TIMER_HGATHER, TIMER_HGATHER);
@@ -217,8 +230,9 @@ extern "C" int ahfgpu_gather_halos(ahfgpu_ctx *c, const double *centres, const d
     hipMemcpyAsync(d_rad, gather_rad, nh * sizeof(double), hipMemcpyHostToDevice, c->st);
     if (skip) hipMemcpyAsync(d_skip, skip, nh, hipMemcpyHostToDevice, c->st);
     else hipMemsetAsync(d_skip, 0, nh, c->st);
-    hipLaunchKernelGGL(k_gather_cells, dim3(nblk((int64_t)nh)), dim3(AHF_BLOCK), 0, c->st, d_cen, d_rad, d_skip,
-                       (int64_t)nh, c->sfckey, c->N, cnt, lo);
+    hipLaunchKernelGGL(k_gather_cells, dim3(nblk(2 * nslot)), dim3(AHF_BLOCK), 0, c->st, d_cen, d_rad, d_skip,
+                       nslot, c->sfckey, c->N, bound);
+    hipLaunchKernelGGL(k_gather_counts, dim3(nblk(nslot)), dim3(AHF_BLOCK), 0, c->st, bound, nslot, cnt);
   }
   exclusive_scan<int64_t, int64_t>(cnt, start, nslot, stmp, c->st);
   int64_t nc = 0;
@@ -229,9 +243,13 @@ extern "C" int ahfgpu_gather_halos(ahfgpu_ctx *c, const double *centres, const d
   uint32_t *pos = A.get<uint32_t>(nc + 1);
   uint32_t *ftmp = A.get<uint32_t>(scan_tmp_elems(nc));
   int64_t *d_off = A.get<int64_t>((int64_t)nh + 1);
-  if (!flag || !pos || !ftmp || !d_off) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (gather)");
-  if (nc) hipLaunchKernelGGL(k_gather_test, dim3(nblk(nc)), dim3(AHF_BLOCK), 0, c->st, start, lo, nslot, nc, d_cen,
-                             d_rad, c->px, c->py, c->pz, flag);
+  int32_t *tab = A.get<int32_t>(nc / AHF_CAND_BLOCK + 1);
+  if (!flag || !pos || !ftmp || !d_off || !tab) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (gather)");
+  if (nc) {
+    hipLaunchKernelGGL(k_gather_blockslot, dim3(nblk(nslot)), dim3(AHF_BLOCK), 0, c->st, start, nslot, tab);
+    hipLaunchKernelGGL(k_gather_test, dim3(nblk(nc)), dim3(AHF_BLOCK), 0, c->st, start, bound, tab, nc, d_cen, d_rad,
+                       c->px, c->py, c->pz, flag);
+  }
   exclusive_scan<uint8_t, uint32_t>(flag, pos, nc, ftmp, c->st);
   hipLaunchKernelGGL(k_gather_off, dim3(nblk((int64_t)nh + 1)), dim3(AHF_BLOCK), 0, c->st, start, (int64_t)nh, nc, pos,
                      d_off);
@@ -244,7 +262,7 @@ extern "C" int ahfgpu_gather_halos(ahfgpu_ctx *c, const double *centres, const d
   if (rc) return rc;
   halo_set_host_meta(s, off.data(), centres);
   hipMemcpyAsync(s->dev, s->host, s->o_ip, hipMemcpyHostToDevice, c->st);
-  if (nc) hipLaunchKernelGGL(k_gather_write, dim3(nblk(nc)), dim3(AHF_BLOCK), 0, c->st, start, lo, nslot, nc, flag,
+  if (nc) hipLaunchKernelGGL(k_gather_write, dim3(nblk(nc)), dim3(AHF_BLOCK), 0, c->st, start, bound, tab, nc, flag,
                              pos, s->ip_in);
   /* algorithmic bytes: per candidate a 12-byte position read, per member a
    * 4-byte index written */

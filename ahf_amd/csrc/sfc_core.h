@@ -100,52 +100,40 @@ AHF_HD double ahf_gather_sep(double a, double b)
 }
 
 /*
- * The key ranges the gather walks for a halo: slot s of the 3x3x3 block
+ * The key range the gather walks for slot s of a halo's 3x3x3 search block
  * (x offset outer, z inner, each -1, 0, +1 around the centre's cell,
- * periodic) holds [kmin, kmax] at full resolution, or kmin > kmax when the
+ * periodic): [*kmin, *kmax] at full resolution, or *kmin > *kmax when the
  * cell is too far to hold a member (ahf_halos_sfc.c:254-296).  On the
- * 2-cell grid (bits = 1) the 8 cells are walked in key order 0..7 with no
- * distance test, and slots 8..26 are empty.  Returns bits.
+ * 2-cell grid (bits = 1) slots 0..7 are the 8 cells in key order, with no
+ * distance test, and slots 8..26 are empty.
  */
-AHF_HD unsigned ahf_gather_cells(double cx, double cy, double cz, double gatherRad, uint64_t kmin[AHF_SHELL],
-                                 uint64_t kmax[AHF_SHELL])
+AHF_HD void ahf_gather_cell(double cx, double cy, double cz, double gatherRad, int s, uint64_t *kmin, uint64_t *kmax)
 {
   const unsigned bits = ahf_gather_bits(gatherRad);
   const unsigned sh = 3u * (AHF_SFC_BITS - bits);
   const uint64_t span = ((uint64_t)1 << sh) - 1;
-  for (int s = 0; s < AHF_SHELL; s++) {
-    kmin[s] = 1;
-    kmax[s] = 0;
-  }
+  *kmin = 1;
+  *kmax = 0;
   if (bits == 1) {
-    for (uint64_t k = 0; k < 8; k++) {
-      kmin[k] = k << sh;
-      kmax[k] = (k << sh) + span;
+    if (s < 8) {
+      *kmin = (uint64_t)s << sh;
+      *kmax = ((uint64_t)s << sh) + span;
     }
-    return bits;
+    return;
   }
   const uint64_t n1 = (uint64_t)1 << bits;
-  const uint64_t bx = ahf_grid_coord(cx, bits), by = ahf_grid_coord(cy, bits), bz = ahf_grid_coord(cz, bits);
   const double fac = 1. / (double)n1;
   const double reach = 0.5 * sqrt(3.) * fac + AHF_GATHERRAD_FAC * gatherRad;
-  int s = 0;
-  for (uint64_t i = n1 - 1; i <= n1 + 1; i++) {
-    const uint64_t x = (bx + i) % n1;
-    for (uint64_t j = n1 - 1; j <= n1 + 1; j++) {
-      const uint64_t y = (by + j) % n1;
-      for (uint64_t k = n1 - 1; k <= n1 + 1; k++, s++) {
-        const uint64_t z = (bz + k) % n1;
-        const double dx = ahf_gather_sep((fac * (double)x) + 0.5 * fac, cx);
-        const double dy = ahf_gather_sep((fac * (double)y) + 0.5 * fac, cy);
-        const double dz = ahf_gather_sep((fac * (double)z) + 0.5 * fac, cz);
-        if (sqrt(dx * dx + dy * dy + dz * dz) > reach) continue;
-        const uint64_t key = ahf_hilbert_key(x, y, z, bits);
-        kmin[s] = key << sh;
-        kmax[s] = (key << sh) + span;
-      }
-    }
-  }
-  return bits;
+  const uint64_t x = (ahf_grid_coord(cx, bits) + n1 - 1 + (uint64_t)(s / 9)) % n1;
+  const uint64_t y = (ahf_grid_coord(cy, bits) + n1 - 1 + (uint64_t)((s / 3) % 3)) % n1;
+  const uint64_t z = (ahf_grid_coord(cz, bits) + n1 - 1 + (uint64_t)(s % 3)) % n1;
+  const double dx = ahf_gather_sep((fac * (double)x) + 0.5 * fac, cx);
+  const double dy = ahf_gather_sep((fac * (double)y) + 0.5 * fac, cy);
+  const double dz = ahf_gather_sep((fac * (double)z) + 0.5 * fac, cz);
+  if (sqrt(dx * dx + dy * dy + dz * dz) > reach) return;
+  const uint64_t key = ahf_hilbert_key(x, y, z, bits);
+  *kmin = key << sh;
+  *kmax = (key << sh) + span;
 }
 
 /* the member test: squared periodic distance of an FP32 position from the

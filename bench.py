@@ -21,6 +21,7 @@ import json
 import os
 import platform
 import subprocess
+import tempfile
 import sys
 import time
 
@@ -39,7 +40,7 @@ def log(msg):
 
 def snapshot_path(cfg, side, seed):
     d = os.environ.get("AHF_BENCH_DIR", "/tmp")
-    return os.path.join(d, f"ahf_bench_cfg{cfg}_{side}_{seed}.snap")
+    return os.path.join(d, f"ahf_bench_s3_cfg{cfg}_{side}_{seed}.snap")
 
 
 def make_snapshot(cfg, side, seed):
@@ -162,6 +163,59 @@ def unbinding_workload(seed=1002, reps=3, cpu=True, threads=16):
     return out
 
 
+def construction_workload(snap, reps=3, cpu=True, threads=16):
+    """The halo construction of ahf_halos_sfc_constructHalo but the profiles,
+    for every halo of the box workload: gatherParts from each centre and
+    gather radius, sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1),
+    with the member lists resident on the device throughout.  The reference's
+    own five functions per halo (OpenMP over haloes) are timed on the same
+    file."""
+    import numpy as np
+
+    import ahf_amd
+    from ahf_amd import dumps
+    s = dumps.read_snapshot(snap)
+    ctx = ahf_amd.Context(0)
+    ctx.set_particles(s.pos, s.mom)
+    tk = time.perf_counter()
+    ctx.sfc_keys()
+    keys_ms = 1e3 * (time.perf_counter() - tk)
+    skip = (s.halo_npart == 0).astype(np.uint8)
+    kw = dict(mode=ctx.UNBIND_WITH_RVIR, ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+    args = (s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble)
+
+    def run():
+        ctx.gather_halos(s.centre, s.gather_rad, skip)
+        gathered = int(ctx._hnp.sum())
+        info = ctx.unbind(*args, **kw)
+        return gathered, info
+
+    run()
+    g_ms, u_ms, wall = [], [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        gathered, info = run()
+        wall.append(1e3 * (time.perf_counter() - t0))
+        g_ms.append(ctx.timers()["halo_gather"][0])
+        u_ms.append(ctx.last_timing()["unbind_ms"])
+    ctx.close()
+    g, u = float(np.median(g_ms)), float(np.median(u_ms))
+    out = {"workload": f"{s.nhalos} haloes of the box workload gathered from their centres and gather radii",
+           "gathered_members": gathered, "final_members": int(info.n_member_out),
+           "gather_ms": g, "construct_ms": u, "gpu_ms": g + u, "wall_ms": float(np.median(wall)),
+           "keys_ms": keys_ms, "halos_per_s": s.nhalos / ((g + u) * 1e-3)}
+    if cpu and os.path.exists(REF_BIN):
+        with tempfile.TemporaryDirectory() as d:
+            r = subprocess.run([REF_BIN, "construct-sfc", snap, os.path.join(d, "h.halo"), "--threads", str(threads)],
+                               capture_output=True, text=True, timeout=900)
+        if r.returncode == 0:
+            rb = json.loads(r.stdout.strip().splitlines()[-1])
+            out["cpu_reference_ms"] = 1e3 * rb["gather_seconds"]
+            out["cpu_threads"] = threads
+            out["vs_cpu"] = out["cpu_reference_ms"] / (g + u)
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -185,6 +239,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the reference CPU timing")
     ap.add_argument("--no-unbind-workload", action="store_true",
                     help="skip the isolated-halo unbinding workload (100k haloes)")
+    ap.add_argument("--no-construct-workload", action="store_true",
+                    help="skip the halo construction workload (gather + cuts + unbinding)")
     ap.add_argument("--band", type=float, default=0.0,
                     help="extra relative band sent to the exact replay (0: the certified bound only)")
     args = ap.parse_args()
@@ -345,6 +401,12 @@ def main():
             }
             out["vs_cpu"] = value / out["cpu_baseline"]["value"]
     ctx.close()
+    if not args.no_construct_workload:
+        cw = construction_workload(snap, cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
+        log(f"construction workload: gather {cw['gather_ms']:.2f} ms + construct {cw['construct_ms']:.2f} ms "
+            f"on the GPU ({cw['gathered_members']} gathered, {cw['final_members']} kept)"
+            + (f", reference CPU {cw['cpu_reference_ms']:.1f} ms" if "cpu_reference_ms" in cw else ""))
+        out["construction_workload"] = cw
     if not args.no_unbind_workload and args.side == 0:
         u = unbinding_workload(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
         log(f"unbinding workload: {u['gpu_ms']:.2f} ms on the GPU"
