@@ -1,0 +1,118 @@
+"""The committed golden fixtures (tests/golden/*.npz, written by
+tests/golden/make_golden.py from the reference's own compiled functions):
+the oracle must reproduce every stored reference output bit for bit, and on a
+GPU so must the library.  These pin the oracle where the reference tree is
+absent."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from ahf_amd import dumps
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+HEADER = ("boxsize", "omega0", "lambda0", "a", "pmass", "L_dom", "L_max", "Nth_dom", "Nth_ref", "minpart", "vtune",
+          "r_fac", "v_fac", "phi_fac", "hubble", "seed", "config", "ovlim", "rho_vir")
+
+
+def _load(path, tmp_path):
+    z = np.load(path, allow_pickle=False)
+    off = np.concatenate([[0], np.cumsum(z["halo_npart"].astype(np.int64))])
+    lists = [z["ipart"][off[h]:off[h + 1]] for h in range(len(z["halo_npart"]))]
+    kw = {k: z[k].item() for k in HEADER}
+    snap = str(tmp_path / "in.snap")
+    dumps.write_snapshot(snap, z["pos"], z["mom"] if "mom" in z else None, centres=z["centre"],
+                         halo_lists=lists, gather_rad=z["gather_rad"], **kw)
+    return z, snap
+
+
+def _ref(z, cmd, tmp_path):
+    out = str(tmp_path / f"ref_{cmd}.dump")
+    z["ref_" + cmd].tofile(out)
+    return out
+
+
+def _halos_equal(a, b, phi_rtol=0.0):
+    assert np.array_equal(a.npart, b.npart)
+    bad = [h for h in range(len(a.ipart)) if not np.array_equal(a.ipart[h], b.ipart[h])]
+    assert not bad, f"{len(bad)} member lists differ, first {bad[:5]}"
+    ok = a.npart >= 1
+    assert np.array_equal(a.M_vir, b.M_vir)
+    assert np.array_equal(a.R_vir[ok], b.R_vir[ok])
+    assert np.array_equal(a.ovdens, b.ovdens)
+    if phi_rtol == 0.0:
+        assert np.array_equal(a.Phi0, b.Phi0)
+    else:
+        big = a.npart > 0
+        rel = np.abs(a.Phi0[big] - b.Phi0[big]) / np.maximum(np.abs(a.Phi0[big]), 1e-300)
+        assert rel.size == 0 or rel.max() < phi_rtol
+
+
+def test_fixtures_present():
+    assert len(FIXTURES) >= 4
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[:-4] for p in FIXTURES])
+def test_oracle_reproduces_golden(oracle, tmp_path, path):
+    z, snap = _load(path, tmp_path)
+    for cmd in z["commands"].tolist():
+        out = str(tmp_path / f"oracle_{cmd}.dump")
+        ref = _ref(z, cmd, tmp_path)
+        if cmd == "amr":
+            assert oracle.oracle_amr_dump(snap.encode(), out.encode()) > 0
+            diffs = dumps.compare_grids(dumps.read_grids(out), dumps.read_grids(ref))
+            assert not diffs, diffs[:5]
+            continue
+        if cmd == "unbind":
+            rc = oracle.oracle_unbind_dump(snap.encode(), out.encode(), 1, 0)
+        elif cmd == "construct":
+            rc = oracle.oracle_construct_dump(snap.encode(), out.encode(), 1, 0)
+        else:
+            rc = oracle.oracle_gather_dump(snap.encode(), out.encode(), 1, 0, int(cmd == "construct-sfc"), None)
+        assert rc == 0, cmd
+        _halos_equal(dumps.read_halos(out), dumps.read_halos(ref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[:-4] for p in FIXTURES])
+def test_gpu_reproduces_golden(tmp_path, path):
+    """the library's device path on the fixture inputs against the stored
+    reference outputs (Phi0 to the rounding of the parallel prefix sum)"""
+    import ahf_amd
+    z, snap = _load(path, tmp_path)
+    s = dumps.read_snapshot(snap)
+    ctx = ahf_amd.Context(0)
+    try:
+        ctx.set_particles(s.pos, s.mom)
+        uargs = (s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble)
+        rv = dict(ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+        for cmd in z["commands"].tolist():
+            print(f"[golden-gpu] {os.path.basename(path)} {cmd}", flush=True)
+            ref = _ref(z, cmd, tmp_path)
+            if cmd == "amr":
+                ctx.amr_build(s.L_dom, s.L_max, s.Nth_dom, s.Nth_ref)
+                out = str(tmp_path / "gpu.grid")
+                ctx.amr_dump(out)
+                diffs = dumps.compare_grids(dumps.read_grids(out), dumps.read_grids(ref))
+                assert not diffs, diffs[:5]
+                continue
+            r = dumps.read_halos(ref)
+            if cmd in ("gather", "construct-sfc"):
+                ctx.sfc_keys()
+                ctx.gather_halos(s.centre, s.gather_rad, skip=(s.halo_npart == 0))
+                if cmd == "gather":
+                    npart, lists = ctx.halo_members()
+                    assert np.array_equal(npart, r.npart)
+                    assert all(np.array_equal(a, b) for a, b in zip(lists, r.ipart))
+                    continue
+                ctx.unbind(*uargs, mode=ctx.UNBIND_WITH_RVIR, **rv)
+            else:
+                ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+                ctx.unbind(*uargs, mode=ctx.UNBIND_WITH_RVIR if cmd == "construct" else 0, **rv)
+            npart, props, lists = ctx.unbind_results()
+            got = dumps.HaloResult(npart, props[:, 0], props[:, 1], props[:, 2], lists, props[:, 3])
+            _halos_equal(r, got, phi_rtol=1e-9)
+    finally:
+        ctx.close()
