@@ -136,7 +136,7 @@ class Context:
         self._check(self.lib.ahfgpu_amr_build(self.h, ctypes.byref(p), ctypes.byref(info)))
         return info
 
-    GATHER_AUTO, GATHER_SOURCE, GATHER_FLAT = 0, 1, 2
+    GATHER_AUTO, GATHER_SOURCE, GATHER_FLAT, GATHER_WAVE = 0, 1, 2, 3
 
     def set_gather_mode(self, mode):
         """deposit kernel choice (same results in every mode)"""

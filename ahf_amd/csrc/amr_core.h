@@ -510,4 +510,65 @@ AHF_HD int64_t ahf_row_rank(const level_view *lv, int64_t r, int cy, int cz)
   return 4 * q0 + cz * 2 * nr + 2 * (r - q0) + cy;
 }
 
+/*
+ * Exact parallel form of AHF's density accumulation d = (float)((double)d + c)
+ * (src/libamr_serial/density.c:362-383) over a run of contributions.
+ *
+ * While |d| stays in one binade [2^e, 2^(e+1)), d = K*u with u = 2^(e-23)
+ * and K an integer, 2^23 <= |K| < 2^24.  If the double sum d + c also lies
+ * in that binade, its rounding step is v = u*2^-29, and since K*u is an
+ * even multiple of v, fl64(K*u + c) = K*u + v*R with R = rint(c/v), which
+ * depends on c alone.  The float rounding then gives K' = K + r, where r is
+ * R*2^-29 rounded to nearest, and on an exact tie the r that makes K' even.
+ * So a run of steps inside one binade is a prefix sum of integers; only a
+ * tie needs the parity of the running K.  A step that leaves the binade (or
+ * starts from zero or a subnormal) is taken as the plain FP32 step.
+ *
+ * ahf_fold_exponent: e of a normal float (0 for zero, subnormal, inf, NaN,
+ * with *ok = 0).  ahf_fold_lane: for one contribution c in binade e, R as an
+ * integer (ok = 0 when c is not finite or R would not fit), the rounded
+ * increment r for the non-tie case, and whether it is a tie (then r is the
+ * lower candidate).
+ */
+AHF_HD int ahf_fold_exponent(float d, int *ok)
+{
+  union { float f; uint32_t u; } b;
+  b.f = d;
+  const int ex = (int)((b.u >> 23) & 0xFFu);
+  *ok = ex != 0 && ex != 0xFF;
+  return ex - 127;
+}
+
+AHF_HD int ahf_fold_lane(double c, int e, int64_t *R, int64_t *r, int *tie)
+{
+  *R = 0;
+  *r = 0;
+  *tie = 0;
+  if (!(c == c) || c > 1e300 || c < -1e300) return 0;
+  const double q = rint(ldexp(c, 52 - e)); /* c / v, exact scaling, then rounded to nearest even */
+  if (q >= 4611686018427387904.0 || q <= -4611686018427387904.0) return 0; /* |R| < 2^62 */
+  const int64_t Ri = (int64_t)q;
+  const int64_t fl = Ri >= 0 ? Ri / ((int64_t)1 << 29) : -((-Ri + ((int64_t)1 << 29) - 1) / ((int64_t)1 << 29));
+  const int64_t rem = Ri - fl * ((int64_t)1 << 29); /* floor division: rem in [0, 2^29) */
+  *R = Ri;
+  *r = fl + (rem > ((int64_t)1 << 28) ? 1 : 0);
+  *tie = rem == ((int64_t)1 << 28);
+  return 1;
+}
+
+/* K (the integer of d in its binade) and the binade tests of one step */
+AHF_HD int64_t ahf_fold_K(float d, int e) { return (int64_t)ldexp((double)d, 23 - e); }
+AHF_HD int ahf_fold_K_in(int64_t K, int neg)
+{
+  return neg ? (K > -((int64_t)1 << 24) && K <= -((int64_t)1 << 23)) : (K >= ((int64_t)1 << 23) && K < ((int64_t)1 << 24));
+}
+/* the double sum T*v = K*u + c stays inside the binade with a margin of one
+ * rounding step, so its rounding step is v */
+AHF_HD int ahf_fold_T_in(int64_t K, int64_t R, int neg)
+{
+  const int64_t T = K * ((int64_t)1 << 29) + R;
+  const int64_t lo = ((int64_t)1 << 52) + 1, hi = ((int64_t)1 << 53) - 1;
+  return neg ? (T <= -lo && T >= -hi) : (T >= lo && T <= hi);
+}
+
 #endif /* AHF_AMR_CORE_H */

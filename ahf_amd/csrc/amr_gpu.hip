@@ -496,9 +496,136 @@ __device__ inline double readlane_d(double x, int l)
   return __hiloint2double(hi, lo);
 }
 
+__device__ inline int64_t readlane_i64(int64_t x, int l)
+{
+  const int lo = __builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)x, l);
+  const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), l);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+/*
+ * AHF's d = (float)((double)d + c) over the contributions cv of lanes
+ * [0, mm), in lane order, for the whole wavefront (d and mm uniform).
+ * Inside one binade of d the steps are integer increments of d/ulp that
+ * depend on c alone (ahf_fold_lane), so a run of them is one prefix sum over
+ * the lanes; exact ties take the parity of the running value, lane by lane.
+ * The first lane whose step leaves the binade (or whose start is zero or
+ * subnormal) takes the plain FP32 step, and the fold resumes after it.
+ * Every round settles at least one lane.
+ */
+__device__ float fold_wave(float d, double cv, int mm)
+{
+  const int lane = threadIdx.x & 63;
+  int start = 0;
+  while (start < mm) {
+    int ok;
+    const int e = ahf_fold_exponent(d, &ok);
+    if (!ok) {
+      d = (float)((double)d + readlane_d(cv, start));
+      start++;
+      continue;
+    }
+    const int neg = d < 0.0f;
+    const int64_t K0 = ahf_fold_K(d, e);
+    const bool act = lane >= start && lane < mm;
+    int64_t R = 0, r = 0;
+    int tie = 0, lok = 0;
+    if (act) lok = ahf_fold_lane(cv, e, &R, &r, &tie);
+    uint64_t tmask = __ballot(act && lok && tie);
+    while (tmask) {
+      const int l = __ffsll((unsigned long long)tmask) - 1;
+      const uint64_t below = __ballot(act && (r & 1)) & ((1ull << l) - 1);
+      const int par = (int)((K0 & 1) ^ (int64_t)(__popcll(below) & 1));
+      const int64_t rl = readlane_i64(r, l);
+      if (lane == l && ((par + (int)(rl & 1)) & 1)) r += 1;
+      tmask &= tmask - 1;
+    }
+    int64_t P = act ? r : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t t = __shfl_up(P, off, 64);
+      if (lane >= off) P += t;
+    }
+    const int64_t Kin = K0 + P - (act ? r : 0);
+    const int valid = act && lok && ahf_fold_K_in(Kin, neg) && ahf_fold_T_in(Kin, R, neg);
+    const uint64_t badm = __ballot(act && !valid);
+    const int bad = badm ? __ffsll((unsigned long long)badm) - 1 : mm;
+    if (bad > start) d = (float)ldexp((double)readlane_i64(K0 + P, bad - 1), e - 23);
+    if (bad < mm) {
+      d = (float)((double)d + readlane_d(cv, bad));
+      start = bad + 1;
+    } else {
+      start = mm;
+    }
+  }
+  return d;
+}
+
+/*
+ * The contributing sources of heavy target n, flattened: lane j < 27 holds
+ * source j of AHF's visiting order (ahf_axis_order on each axis, z outer),
+ * its first entry q0, its entry count (0 when absent or masked out) and the
+ * exclusive prefix of the counts; the return value is the total.
+ */
+struct HeavySrc {
+  int64_t q0, cnt, off;
+  int ex, ey, ez;
+};
+
+__device__ inline int64_t heavy_sources(const level_view &v, const uint32_t *__restrict__ pmask,
+                                        const int64_t *__restrict__ p0, int64_t n, HeavySrc *hs)
+{
+  const int lane = threadIdx.x & 63;
+  const int64_t o = n >> 3;
+  const int c = (int)(n & 7);
+  hs->q0 = 0;
+  hs->cnt = 0;
+  hs->ex = hs->ey = hs->ez = 0;
+  if (lane < 27) {
+    int64_t X, Y, Z;
+    ahf_node_xyz(&v, o, c, &X, &Y, &Z);
+    int oz[3], oy[3], ox[3];
+    ahf_axis_order(Z, v.L, oz);
+    ahf_axis_order(Y, v.L, oy);
+    ahf_axis_order(X, v.L, ox);
+    hs->ez = oz[lane / 9];
+    hs->ey = oy[(lane / 3) % 3];
+    hs->ex = ox[lane % 3];
+    const int64_t s = ahf_nbr_node(v.nbr, o, c, hs->ex, hs->ey, hs->ez);
+    if (s >= 0 && ((pmask[s] >> ((1 - hs->ez) * 9 + (1 - hs->ey) * 3 + (1 - hs->ex))) & 1u)) {
+      hs->q0 = p0[s];
+      hs->cnt = p0[s + 1] - hs->q0;
+    }
+  }
+  int64_t inc = hs->cnt;
+#pragma unroll
+  for (int d = 1; d < 32; d <<= 1) {
+    const int64_t t = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += t;
+  }
+  hs->off = inc - hs->cnt;
+  return readlane_i64(inc, 26);
+}
+
+/* the source (lane of heavy_sources) holding flat contribution k: the last
+ * source whose prefix is at most k (empty sources share their prefix with
+ * the next one, so the last is the one that holds k) */
+__device__ inline int heavy_source_of(int64_t off, int64_t k)
+{
+  int idx = 0;
+#pragma unroll
+  for (int step = 16; step >= 1; step >>= 1) {
+    const int cand = idx + step;
+    const int64_t oc = __shfl(off, cand < 27 ? cand : 26, 64);
+    if (cand < 27 && oc <= k) idx = cand;
+  }
+  return idx;
+}
+
 /* exact-order density of the heavy targets, one wavefront per target: the
- * lanes compute 64 contributions at a time, the wave then folds them into
- * the FP32 node value in AHF's order */
+ * node's contributions, source by source in AHF's order and each source's
+ * entries in list order, are taken 64 at a time (one per lane) and folded
+ * into the FP32 node value (fold_wave) */
 __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const uint32_t *__restrict__ pmask,
                                                             const int64_t *__restrict__ p0,
                                                             const ahf_w9 *__restrict__ W, int descending,
@@ -512,35 +639,21 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const 
   const int64_t nh = (int64_t)*n_heavy;
   for (int64_t i = gw; i < nh; i += nw) {
     const int64_t n = heavy[i];
-    const int64_t o = n >> 3;
-    const int c = (int)(n & 7);
-    int64_t X, Y, Z;
-    ahf_node_xyz(&v, o, c, &X, &Y, &Z);
-    int oz[3], oy[3], ox[3];
-    ahf_axis_order(Z, v.L, oz);
-    ahf_axis_order(Y, v.L, oy);
-    ahf_axis_order(X, v.L, ox);
+    HeavySrc hs;
+    const int64_t total = heavy_sources(v, pmask, p0, n, &hs);
     float d = -1.0f;
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++)
-        for (int e = 0; e < 3; e++) {
-          const int ez = oz[a], ey = oy[b], ex = ox[e];
-          const int64_t s = ahf_nbr_node(v.nbr, o, c, ex, ey, ez);
-          if (s < 0) continue;
-          if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
-          const int64_t q0 = p0[s], cnt = p0[s + 1] - q0;
-          for (int64_t base = 0; base < cnt; base += 64) {
-            const int64_t k = base + lane;
-            double cv = 0.0;
-            if (k < cnt) cv = w9_contrib(W, (uint32_t)(descending ? q0 + cnt - 1 - k : q0 + k), 1 - ez, 1 - ey, 1 - ex);
-            const int mm = (int)(cnt - base < 64 ? cnt - base : 64);
-            for (int l = 0; l < mm; l++) d = (float)((double)d + readlane_d(cv, l));
-          }
-        }
+    for (int64_t base = 0; base < total; base += 64) {
+      const int64_t k = base + lane;
+      const int j = heavy_source_of(hs.off, k);
+      const int64_t q0 = __shfl(hs.q0, j, 64), cnt = __shfl(hs.cnt, j, 64), kk = k - __shfl(hs.off, j, 64);
+      const int ex = __shfl(hs.ex, j, 64), ey = __shfl(hs.ey, j, 64), ez = __shfl(hs.ez, j, 64);
+      double cv = 0.0;
+      if (k < total) cv = w9_contrib(W, (uint32_t)(descending ? q0 + cnt - 1 - kk : q0 + kk), 1 - ez, 1 - ey, 1 - ex);
+      d = fold_wave(d, cv, (int)(total - base < 64 ? total - base : 64));
+    }
     if (lane == 0) dens[n] = d;
   }
 }
-
 
 /* k_gather_heavy over position records (dense levels) */
 __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy_q(level_view v, const uint32_t *__restrict__ pmask,
@@ -554,44 +667,31 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy_q(level_view v, cons
   const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
   const int64_t nh = (int64_t)*n_heavy;
+  const double L = (double)v.L;
   for (int64_t i = gw; i < nh; i += nw) {
     const int64_t n = heavy[i];
-    const int64_t o = n >> 3;
-    const int c = (int)(n & 7);
     int64_t X, Y, Z;
-    ahf_node_xyz(&v, o, c, &X, &Y, &Z);
-    int oz[3], oy[3], ox[3];
-    ahf_axis_order(Z, v.L, oz);
-    ahf_axis_order(Y, v.L, oy);
-    ahf_axis_order(X, v.L, ox);
-    const double L = (double)v.L;
+    ahf_node_xyz(&v, n >> 3, (int)(n & 7), &X, &Y, &Z);
     double cxv[3], cyv[3], czv[3];
     ahf_centres3(X, v.L, cxv);
     ahf_centres3(Y, v.L, cyv);
     ahf_centres3(Z, v.L, czv);
+    HeavySrc hs;
+    const int64_t total = heavy_sources(v, pmask, p0, n, &hs);
     float d = -1.0f;
-    for (int a = 0; a < 3; a++)
-      for (int b = 0; b < 3; b++)
-        for (int e = 0; e < 3; e++) {
-          const int ez = oz[a], ey = oy[b], ex = ox[e];
-          const int64_t s = ahf_nbr_node(v.nbr, o, c, ex, ey, ez);
-          if (s < 0) continue;
-          if (!((pmask[s] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
-          const int64_t q0 = p0[s], cnt = p0[s + 1] - q0;
-          if (cnt == 0) continue;
-          const double cx = cxv[ex + 1], cy = cyv[ey + 1], cz = czv[ez + 1];
-          for (int64_t base = 0; base < cnt; base += 64) {
-            const int64_t k = base + lane;
-            double cv = 0.0;
-            if (k < cnt) {
-              const ahf_q4 e4 = qp[descending ? q0 + cnt - 1 - k : q0 + k];
-              cv = ahf_contrib(m2pd, ahf_pn_sep(e4.x, cx, L), ahf_pn_sep(e4.y, cy, L), ahf_pn_sep(e4.z, cz, L),
-                               -ex, -ey, -ez);
-            }
-            const int mm = (int)(cnt - base < 64 ? cnt - base : 64);
-            for (int l = 0; l < mm; l++) d = (float)((double)d + readlane_d(cv, l));
-          }
-        }
+    for (int64_t base = 0; base < total; base += 64) {
+      const int64_t k = base + lane;
+      const int j = heavy_source_of(hs.off, k);
+      const int64_t q0 = __shfl(hs.q0, j, 64), cnt = __shfl(hs.cnt, j, 64), kk = k - __shfl(hs.off, j, 64);
+      const int ex = __shfl(hs.ex, j, 64), ey = __shfl(hs.ey, j, 64), ez = __shfl(hs.ez, j, 64);
+      double cv = 0.0;
+      if (k < total) {
+        const ahf_q4 e4 = qp[descending ? q0 + cnt - 1 - kk : q0 + kk];
+        cv = ahf_contrib(m2pd, ahf_pn_sep(e4.x, cxv[ex + 1], L), ahf_pn_sep(e4.y, cyv[ey + 1], L),
+                         ahf_pn_sep(e4.z, czv[ez + 1], L), -ex, -ey, -ez);
+      }
+      d = fold_wave(d, cv, (int)(total - base < 64 ? total - base : 64));
+    }
     if (lane == 0) dens[n] = d;
   }
 }
@@ -913,7 +1013,8 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   level_view v = make_view(l);
   Arena &A = c->arena;
   const bool dense = c->gather_mode == AHFGPU_GATHER_SOURCE ||
-                     (c->gather_mode == AHFGPU_GATHER_AUTO && n_ent >= AHF_DENSE_ENTRIES_PER_NODE * nn);
+                     ((c->gather_mode == AHFGPU_GATHER_AUTO || c->gather_mode == AHFGPU_GATHER_WAVE) &&
+                      n_ent >= AHF_DENSE_ENTRIES_PER_NODE * nn);
   l.dens = A.get<float>(nn);
   if (dense)
     l.qp = A.get<ahf_q4>(n_ent > 0 ? n_ent : 1);
@@ -948,7 +1049,7 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   const dim3 grid((unsigned)(AHF_XCDS * per_xcd));
   /* long lanes in a small launch leave the chip idle: hand them to the
    * wavefront-per-node kernel sooner */
-  const uint32_t kmax = nn >= ((int64_t)1 << 21) ? 384u : 96u;
+  const uint32_t kmax = c->gather_mode == AHFGPU_GATHER_WAVE ? 0u : nn >= ((int64_t)1 << 21) ? 384u : 96u;
   if (dense)
     hipLaunchKernelGGL(k_gather_src, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0, l.qp,
                        descending, l.m2pd, kmax, l.dens, heavy, n_heavy);

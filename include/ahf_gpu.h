@@ -129,6 +129,7 @@ int ahfgpu_deposit_level(ahfgpu_ctx *ctx, int64_t L, const uint64_t *oct_key, co
 #define AHFGPU_GATHER_AUTO 0   /* per level: SOURCE when the level holds at least 3 entries per node */
 #define AHFGPU_GATHER_SOURCE 1 /* walk the 27 sources over per-entry positions */
 #define AHFGPU_GATHER_FLAT 2   /* one flat walk per node over per-entry TSC weights */
+#define AHFGPU_GATHER_WAVE 3   /* AUTO's walks, but every node with a contribution folded by a wavefront */
 
 /* No counterpart in the reference: choose the deposit kernel of the
  * following builds and deposits.  Every mode gives the same bits; AUTO

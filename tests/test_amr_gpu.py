@@ -32,7 +32,7 @@ def _gpu_dump(ctx, snap_path, out_path, mode=0):
 
 # every deposit kernel (per-source walk, flat walk, and the per-level choice)
 # must give the oracle's bits
-MODES = [0, 1, 2]
+MODES = [0, 1, 2, 3]  # AUTO, SOURCE, FLAT, WAVE (every node folded by a wavefront)
 
 
 def _check(oracle, ctx, snap, tmp_path, tag, mode=0):

@@ -48,3 +48,17 @@ def test_emu_configs(oracle, emu, snapgen, tmp_path, cfg, side, seed):
     snap = str(tmp_path / "g.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
     _cmp(oracle, emu, snap, tmp_path)
+
+
+def test_wavefront_fold_replay(tmp_path):
+    """the heavy kernels' parallel fold (integer prefix sums inside a binade,
+    exact ties by parity, plain steps across binades) replayed on the host
+    against AHF's sequential FP32 accumulation, on random and adversarial
+    runs; any differing bit fails"""
+    exe = str(tmp_path / "fold_check")
+    subprocess.run(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-Wno-unused-function", "-o", exe,
+                    os.path.join(HERE, "emu", "fold_check.c"), "-lm"], check=True)
+    for seed in (1, 2):
+        r = subprocess.run([exe, "300000", str(seed)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:] + r.stdout
+        assert '"mismatches": 0' in r.stdout
