@@ -87,6 +87,7 @@ def load():
     lib.ahfgpu_unbind_results.argtypes = [P, P, P, P]
     lib.ahfgpu_release_halos.argtypes = [P]
     lib.ahfgpu_release_halos.restype = None
+    lib.ahfgpu_set_unbind_split.argtypes = [P, ctypes.c_int64]
     lib.ahfgpu_set_sfc_keys.argtypes = [P, P, ctypes.c_uint64]
     lib.ahfgpu_sfc_keys.argtypes = [P, P]
     lib.ahfgpu_gather_halos.argtypes = [P, P, P, P, ctypes.c_uint64]
@@ -137,6 +138,11 @@ class Context:
         return info
 
     GATHER_AUTO, GATHER_SOURCE, GATHER_FLAT, GATHER_WAVE = 0, 1, 2, 3
+
+    def set_unbind_split(self, min_members):
+        """haloes of at least min_members go through the split unbinding path
+        (0: default, negative: never); same results either way"""
+        self._check(self.lib.ahfgpu_set_unbind_split(self.h, int(min_members)))
 
     def set_gather_mode(self, mode):
         """deposit kernel choice (same results in every mode)"""

@@ -231,6 +231,7 @@ __device__ __forceinline__ double w9_contrib(const ahf_w9 *__restrict__ W, uint3
  * (k_prep) and the FP32 rounding.  Targets with more than kmax
  * contributions are left to k_gather_heavy.
  */
+template <bool OFF32>
 __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const uint32_t *__restrict__ tord,
                                                             int64_t per_xcd, const uint32_t *__restrict__ pmask,
                                                             const int64_t *__restrict__ p0,
@@ -266,16 +267,25 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const 
   __syncthreads();
   if (!active) return;
   const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
-  /* the target's own node in the neighbourhood: offsets (dx,dy,dz) add
-   * dz*16 + dy*4 + dx */
-  const uint4 *rec = s_rec + nb0 + (cz + 1) * 16 + (cy + 1) * 4 + (cx + 1);
+  /* neighbourhood record of the source at offset (ex,ey,ez): index
+   * own + (ez+1)*16 + (ey+1)*4 + (ex+1) with own the target's corner */
+  const uint4 *rec0 = s_rec + nb0 + cz * 16 + cy * 4 + cx;
   const int64_t n = 8 * o + c;
   int64_t X, Y, Z;
   ahf_node_xyz(&v, o, c, &X, &Y, &Z);
   const int rx = X == 0 ? 1 : (X == v.L - 1 ? 2 : 0);
   const int ry = Y == 0 ? 1 : (Y == v.L - 1 ? 2 : 0);
   const int rz = Z == 0 ? 1 : (Z == v.L - 1 ? 2 : 0);
-  uint32_t mask = 0;
+  /* per axis, offset+1 of the source at traversal rank 0, 1, 2 as 2-bit
+   * fields (identity 0|1<<2|2<<4 inside the box) */
+  const uint32_t px = (uint32_t)(rot_offset(0, rx) + 1) | (uint32_t)(rot_offset(1, rx) + 1) << 2 |
+                      (uint32_t)(rot_offset(2, rx) + 1) << 4;
+  const uint32_t py = (uint32_t)(rot_offset(0, ry) + 1) | (uint32_t)(rot_offset(1, ry) + 1) << 2 |
+                      (uint32_t)(rot_offset(2, ry) + 1) << 4;
+  const uint32_t pz = (uint32_t)(rot_offset(0, rz) + 1) | (uint32_t)(rot_offset(1, rz) + 1) << 2 |
+                      (uint32_t)(rot_offset(2, rz) + 1) << 4;
+  /* contributing sources as bits tz*16 + ty*4 + tx of their traversal ranks */
+  uint64_t mask = 0;
   uint64_t total = 0;
 #pragma unroll 1
   for (int tz = 0; tz < 3; tz++)
@@ -284,10 +294,10 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const 
 #pragma unroll
       for (int tx = 0; tx < 3; tx++) {
         const int ez = rot_offset(tz, rz), ey = rot_offset(ty, ry), ex = rot_offset(tx, rx);
-        const uint32_t *rw = reinterpret_cast<const uint32_t *>(rec + ez * 16 + ey * 4 + ex);
+        const uint32_t *rw = reinterpret_cast<const uint32_t *>(rec0 + (ez + 1) * 16 + (ey + 1) * 4 + (ex + 1));
         const uint32_t cnt = rw[1], pm = rw[2];
         if (((pm >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u) && cnt) {
-          mask |= 1u << (tz * 9 + ty * 3 + tx);
+          mask |= 1ull << (tz * 16 + ty * 4 + tx);
           total += cnt;
         }
       }
@@ -296,23 +306,34 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const 
     return;
   }
   const uint32_t step = descending ? 0xFFFFFFFFu : 1u;
+  const char *Wb = reinterpret_cast<const char *>(W);
   float d = -1.0f;
-  uint32_t rem = 0, q = 0;
-  int iz = 0, iy = 0, ix = 0;
+  uint32_t rem = 0, q = 0, oz = 0, oy = 0, ox = 0;
   for (uint32_t it = 0; it < (uint32_t)total; it++) {
     if (rem == 0) {
-      const int tt = __builtin_ctz(mask);
-      mask &= mask - 1u;
-      const int tz = tt / 9, ty = (tt / 3) % 3, tx = tt % 3;
-      const int ez = rot_offset(tz, rz), ey = rot_offset(ty, ry), ex = rot_offset(tx, rx);
-      const uint4 r = rec[ez * 16 + ey * 4 + ex];
+      const int pos = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      const uint32_t ez1 = (pz >> ((pos >> 4) << 1)) & 3u, ey1 = (py >> (((pos >> 2) & 3) << 1)) & 3u,
+                     ex1 = (px >> ((pos & 3) << 1)) & 3u;
+      const uint4 r = rec0[ez1 * 16 + ey1 * 4 + ex1];
       rem = r.y;
       q = step == 1u ? r.x : r.x + r.y - 1u;
-      iz = 1 - ez;
-      iy = 1 - ey;
-      ix = 1 - ex;
+      /* byte offsets of a[1-ez], wy[1-ey], wx[1-ex] inside a record */
+      oz = (2u - ez1) * 8u;
+      oy = 24u + (2u - ey1) * 8u;
+      ox = 48u + (2u - ex1) * 8u;
     }
-    d = (float)((double)d + w9_contrib(W, q, iz, iy, ix));
+    double w;
+    if constexpr (OFF32) {
+      const uint32_t base = q * (uint32_t)sizeof(ahf_w9);
+      w = *reinterpret_cast<const double *>(Wb + (base + oz)) * *reinterpret_cast<const double *>(Wb + (base + oy)) *
+          *reinterpret_cast<const double *>(Wb + (base + ox));
+    } else {
+      const char *rb = Wb + (uint64_t)q * sizeof(ahf_w9);
+      w = *reinterpret_cast<const double *>(rb + oz) * *reinterpret_cast<const double *>(rb + oy) *
+          *reinterpret_cast<const double *>(rb + ox);
+    }
+    d = (float)((double)d + w);
     q += step;
     rem--;
   }
@@ -1053,9 +1074,12 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   if (dense)
     hipLaunchKernelGGL(k_gather_src, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0, l.qp,
                        descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
+  else if ((uint64_t)(n_ent > 0 ? n_ent : 1) * sizeof(ahf_w9) < ((uint64_t)1 << 32))
+    hipLaunchKernelGGL(k_gather_light<true>, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0,
+                       l.W, descending, kmax, l.dens, heavy, n_heavy);
   else
-    hipLaunchKernelGGL(k_gather_light, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0, l.W,
-                       descending, kmax, l.dens, heavy, n_heavy);
+    hipLaunchKernelGGL(k_gather_light<false>, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0,
+                       l.W, descending, kmax, l.dens, heavy, n_heavy);
   if (timed) c->timer_end(e1, TIMER_GATHER, 12.0 * (double)n_ent + 13.0 * (double)nn);
   hipEvent_t e2 = timed ? c->timer_begin() : nullptr;
   if (dense)

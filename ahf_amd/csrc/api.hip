@@ -33,7 +33,12 @@ extern "C" int ahfgpu_create(ahfgpu_ctx **out, int device)
   if (hipSetDevice(device) != hipSuccess) return AHFGPU_EHIP;
   ahfgpu_ctx *c = new ahfgpu_ctx();
   c->device = device;
-  if (hipStreamCreate(&c->st) != hipSuccess || hipMalloc((void **)&c->d_anomaly, 64) != hipSuccess ||
+  if (hipStreamCreate(&c->st) != hipSuccess || hipStreamCreate(&c->st2) != hipSuccess ||
+      hipStreamCreate(&c->st3) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_s0, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_s1, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_s2, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc((void **)&c->d_anomaly, 64) != hipSuccess ||
       hipEventCreate(&c->ev_a0) != hipSuccess || hipEventCreate(&c->ev_a1) != hipSuccess ||
       hipEventCreate(&c->ev_u0) != hipSuccess || hipEventCreate(&c->ev_u1) != hipSuccess) {
     delete c;
@@ -69,10 +74,14 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   if (c->d_anomaly) (void)hipFree(c->d_anomaly);
   if (c->d_flag) (void)hipFree(c->d_flag);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
-  hipEvent_t evs[] = {c->ev_a0, c->ev_a1, c->ev_u0, c->ev_u1};
+  (void)hipStreamSynchronize(c->st2);
+  (void)hipStreamSynchronize(c->st3);
+  hipEvent_t evs[] = {c->ev_a0, c->ev_a1, c->ev_u0, c->ev_u1, c->ev_s0, c->ev_s1, c->ev_s2};
   for (auto e : evs)
     if (e) (void)hipEventDestroy(e);
   if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
+  if (c->st3) (void)hipStreamDestroy(c->st3);
   delete c;
 }
 

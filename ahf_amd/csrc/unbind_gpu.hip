@@ -89,35 +89,65 @@ __device__ inline uint64_t dist_key(uint32_t p, const double *__restrict__ c, co
 }
 
 #define AHF_SORT_SMALL 8192
-#define AHF_SORT_THREADS 1024
+#define AHF_SORT_TIERS 5
+
+/* which radial sort a halo of n members takes: 0 keeps the input order (AHF
+ * does not sort haloes below minpart), 1..4 sort in LDS with capacities 64,
+ * 512, 2048 and AHF_SORT_SMALL, -1 goes to the radix sort */
+__host__ __device__ inline int sort_tier(int64_t n, int32_t minpart)
+{
+  if (n < minpart || n < 2) return 0;
+  if (n <= 64) return 1;
+  if (n <= 512) return 2;
+  if (n <= 2048) return 3;
+  if (n <= AHF_SORT_SMALL) return 4;
+  return -1;
+}
+
+struct SortBase {
+  int64_t base[AHF_SORT_TIERS];
+};
+
+/* the halo list of each tier, from per-tier counters that start at zero
+ * (order within a tier is arbitrary: each halo is sorted on its own) */
+__global__ void k_sort_classify(const int64_t *__restrict__ off, int64_t nh, int32_t minpart, SortBase b,
+                                int32_t *__restrict__ cnt, int32_t *__restrict__ list)
+{
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  const int t = sort_tier(off[h + 1] - off[h], minpart);
+  if (t < 0) return;
+  list[b.base[t] + atomicAdd(&cnt[t], 1)] = (int32_t)h;
+}
 
 /*
- * Radial sort of haloes of at most AHF_SORT_SMALL members, one workgroup per
- * halo: keys in LDS, bitonic sort on (key, position in the list), so equal
- * distances keep their input order as in the radix path.  Haloes below
- * minpart keep their order (AHF does not sort them).
+ * Radial sort of the haloes list[0..cnt), at most CAP members each, one
+ * workgroup of BS threads per halo: keys in LDS, bitonic sort on (key,
+ * position in the list), so equal distances keep their input order as in the
+ * radix path.  With copy_only the lists are copied unsorted.
  */
-__global__ void __launch_bounds__(AHF_SORT_THREADS) k_sort_small(const int64_t *__restrict__ off, int64_t nh,
-                                                                 const uint32_t *__restrict__ ip,
-                                                                 const double *__restrict__ cen,
-                                                                 const float *__restrict__ px,
-                                                                 const float *__restrict__ py,
-                                                                 const float *__restrict__ pz, int32_t minpart,
-                                                                 uint32_t *__restrict__ out)
+template <int CAP, int BS>
+__global__ void __launch_bounds__(BS) k_sort_tier(const int32_t *__restrict__ list, int64_t cnt,
+                                                  const int64_t *__restrict__ off, const uint32_t *__restrict__ ip,
+                                                  const double *__restrict__ cen, const float *__restrict__ px,
+                                                  const float *__restrict__ py, const float *__restrict__ pz,
+                                                  int copy_only, uint32_t *__restrict__ out)
 {
-  __shared__ uint64_t s_key[AHF_SORT_SMALL];
-  __shared__ uint32_t s_idx[AHF_SORT_SMALL];
-  const int tid = threadIdx.x, bs = blockDim.x;
-  for (int64_t h = blockIdx.x; h < nh; h += gridDim.x) {
+  __shared__ uint64_t s_key[CAP];
+  __shared__ uint32_t s_idx[CAP];
+  const int tid = threadIdx.x;
+  for (int64_t t = blockIdx.x; t < cnt; t += gridDim.x) {
+    const int64_t h = list[t];
     const int64_t o0 = off[h];
-    const int n = (int)(off[h + 1] - o0);
-    if (n < minpart || n < 2) {
-      for (int i = tid; i < n; i += bs) out[o0 + i] = ip[o0 + i];
+    const int64_t nl = off[h + 1] - o0;
+    if (copy_only) {
+      for (int64_t i = tid; i < nl; i += BS) out[o0 + i] = ip[o0 + i];
       continue;
     }
+    const int n = (int)nl;
     int m = 2;
     while (m < n) m <<= 1;
-    for (int i = tid; i < m; i += bs) {
+    for (int i = tid; i < m; i += BS) {
       if (i < n) {
         s_key[i] = dist_key(ip[o0 + i], cen + 3 * h, px, py, pz);
         s_idx[i] = (uint32_t)i;
@@ -129,9 +159,10 @@ __global__ void __launch_bounds__(AHF_SORT_THREADS) k_sort_small(const int64_t *
     __syncthreads();
     for (int k = 2; k <= m; k <<= 1)
       for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = tid; i < m; i += bs) {
-          const int l = i ^ j;
-          if (l <= i) continue;
+        /* pair q compares element i (bit j clear) with i + j */
+        for (int q = tid; q < (m >> 1); q += BS) {
+          const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+          const int l = i + j;
           const uint64_t ka = s_key[i], kb = s_key[l];
           const uint32_t ia = s_idx[i], ib = s_idx[l];
           const bool a_gt_b = ka > kb || (ka == kb && ia > ib);
@@ -144,25 +175,9 @@ __global__ void __launch_bounds__(AHF_SORT_THREADS) k_sort_small(const int64_t *
         }
         __syncthreads();
       }
-    for (int i = tid; i < n; i += bs) out[o0 + i] = ip[o0 + s_idx[i]];
+    for (int i = tid; i < n; i += BS) out[o0 + i] = ip[o0 + s_idx[i]];
     __syncthreads();
   }
-}
-
-__global__ void k_dist_keys(const int64_t *__restrict__ off, int64_t nh, int64_t ne, const uint32_t *__restrict__ ip,
-                            const double *__restrict__ cen, const float *__restrict__ px,
-                            const float *__restrict__ py, const float *__restrict__ pz, int32_t minpart,
-                            uint64_t *__restrict__ key, uint32_t *__restrict__ val)
-{
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const int64_t h = halo_of(off, nh, e);
-  val[e] = (uint32_t)e;
-  if (off[h + 1] - off[h] < minpart) {
-    key[e] = 0; /* not sorted by AHF: keep input order */
-    return;
-  }
-  key[e] = dist_key(ip[e], cen + 3 * h, px, py, pz);
 }
 
 __global__ void k_halo_keys(const int64_t *__restrict__ off, int64_t nh, int64_t ne, const uint32_t *__restrict__ v,
@@ -172,11 +187,32 @@ __global__ void k_halo_keys(const int64_t *__restrict__ off, int64_t nh, int64_t
   if (i < ne) hk[i] = (uint32_t)halo_of(off, nh, (int64_t)v[i]);
 }
 
-__global__ void k_gather_members(const uint32_t *__restrict__ v, const uint32_t *__restrict__ ip, int64_t ne,
-                                  uint32_t *__restrict__ out)
+/* the radix path over the members of the haloes bh[0..nb) only (those too
+ * big for the LDS tiers): flat index e in [0, bo[nb]) is member e - bo[k] of
+ * halo bh[k] */
+__global__ void k_dist_keys_sel(const int64_t *__restrict__ bo, const int32_t *__restrict__ bh, int64_t nb,
+                                int64_t ne, const int64_t *__restrict__ off, const uint32_t *__restrict__ ip,
+                                const double *__restrict__ cen, const float *__restrict__ px,
+                                const float *__restrict__ py, const float *__restrict__ pz,
+                                uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int64_t k = halo_of(bo, nb, e);
+  const int64_t h = bh[k];
+  val[e] = (uint32_t)e;
+  key[e] = dist_key(ip[off[h] + (e - bo[k])], cen + 3 * h, px, py, pz);
+}
+
+__global__ void k_gather_members_sel(const int64_t *__restrict__ bo, const int32_t *__restrict__ bh, int64_t nb,
+                                     int64_t ne, const int64_t *__restrict__ off, const uint32_t *__restrict__ v,
+                                     const uint32_t *__restrict__ ip, uint32_t *__restrict__ out)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ne) out[i] = ip[v[i]];
+  if (i >= ne) return;
+  const int64_t k = halo_of(bo, nb, i);
+  const int64_t base = off[bh[k]] - bo[k];
+  out[base + i] = ip[base + (int64_t)v[i]];
 }
 
 /* ------------------------------------------------------ block helpers */
@@ -426,6 +462,192 @@ struct Staged {
   float *vx, *vy, *vz;
 };
 
+/*
+ * One iteration's bound-set recurrence of a halo of n members (one
+ * workgroup of BS threads), from its staged data (dist, Phi_j in phi, the
+ * offsets and velocities in g), Phi0, the sums of |v| (asum) and the
+ * rounding factor gam of those parallel sums: st[j] = 1 for the members that
+ * stay bound, as AHF's sequential loop decides them.  seed is the list
+ * position of the bulk-velocity seed.  A member whose test the certified
+ * rounding bound cannot settle sends the iteration to the exact sequential
+ * replay (thread 0, in AHF's operation order), which also recomputes Phi0.
+ * Returns 1 when it replayed.
+ */
+template <int BS, int IPT>
+__device__ int rec_eval(const UnbindConst &K, int64_t n, int64_t seed, const double *dist, const double *phi,
+                        uint8_t *st, Staged g, D4 asum, double gam, double *Phi0_io, int *sweeps)
+{
+  constexpr int NW = BS / 64;
+  constexpr int64_t CH = (int64_t)BS * IPT;
+  __shared__ D4 r_sh4[NW];
+  __shared__ int sflag[2]; /* double-buffered: bit 0 a status changed, bit 1 a test was uncertain */
+  __shared__ double s_phi0;
+  D4 *sh4 = r_sh4;
+  const int t = threadIdx.x;
+  if (t == 0) sflag[0] = sflag[1] = 0;
+  __syncthreads();
+  int ev = 0;
+  double Phi0 = *Phi0_io;
+  const double sx = (double)g.vx[seed], sy = (double)g.vy[seed], sz = (double)g.vz[seed];
+  /* the sequential bound-set recurrence, chunk by chunk: a chunk is
+   * evaluated with the final statuses of every earlier chunk (carried in
+   * cv) and its own statuses of the previous evaluation, and evaluated
+   * again until they no longer change; then it holds the sequential answer
+   * and its totals carry to the next chunk.  Only the final evaluation's
+   * margins count for the exact replay. */
+  int uncertain = 0;
+  D4 cv = {sx, sy, sz, 1.0};
+  /* per-halo factors of the certified bound */
+  const double e_phi = 2.5 * gam * fabs(Phi0);
+  const double e_lim = K.v2_tune * 4.0 * K.phi_fac * e_phi;
+  const double cvx = fabs(K.v_fac) * 5.0 * gam * asum.x, cvy = fabs(K.v_fac) * 5.0 * gam * asum.y,
+               cvz = fabs(K.v_fac) * 5.0 * gam * asum.z;
+  for (int64_t base = 0; base < n; base += CH) {
+    const int64_t j0 = base + (int64_t)t * IPT;
+    /* what a member's test needs besides the running bulk velocity, once per
+     * chunk: its velocity, its Hubble term and its limit v2_tune*v_esc2
+     * (no limit, lim < 0, inside MACHINE_ZERO) */
+    double fx[IPT], fy[IPT], fz[IPT], hx[IPT], hy[IPT], hz[IPT], lm[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+      const int64_t j = j0 + i;
+      fx[i] = fy[i] = fz[i] = hx[i] = hy[i] = hz[i] = 0.0;
+      lm[i] = -1.0;
+      if (j < n) {
+        fx[i] = (double)g.vx[j]; fy[i] = (double)g.vy[j]; fz[i] = (double)g.vz[j];
+        hx[i] = K.hubble * (g.dx[j]) * K.r_fac;
+        hy[i] = K.hubble * (g.dy[j]) * K.r_fac;
+        hz[i] = K.hubble * (g.dz[j]) * K.r_fac;
+        if (dist[j] > MZERO) lm[i] = K.v2_tune * (2 * fabs(phi[j] - Phi0) * K.phi_fac);
+      }
+    }
+    /* statuses of the thread's members, kept in registers while the chunk
+     * settles; the first evaluation assumes all bound */
+    int bj[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; i++) bj[i] = j0 + i < n ? 1 : 0;
+    for (;;) {
+      (*sweeps)++;
+      /* the flag word of this evaluation; the other one was read by every
+       * thread before the previous evaluation's barrier, and is cleared for
+       * the next evaluation once this one's scan has passed its barriers */
+      int *flag = sflag + (ev & 1);
+      D4 run = {0, 0, 0, 0};
+#pragma unroll
+      for (int i = 0; i < IPT; i++)
+        if (bj[i]) { run.x += fx[i]; run.y += fy[i]; run.z += fz[i]; run.m += 1.0; }
+      D4 tot;
+      const D4 ex = blk_excl_scan4<BS>(run, &tot, sh4);
+      /* the thread's own earlier members, summed again in the same order */
+      D4 lex = {0, 0, 0, 0};
+      int chg = 0, unc = 0;
+#pragma unroll
+      for (int i = 0; i < IPT; i++) {
+        const int64_t j = j0 + i;
+        if (j >= n) continue;
+        const double VX = cv.x + ex.x + lex.x, VY = cv.y + ex.y + lex.y, VZ = cv.z + ex.z + lex.z,
+                     M = cv.m + ex.m + lex.m;
+        if (bj[i]) { lex.x += fx[i]; lex.y += fy[i]; lex.z += fz[i]; lex.m += 1.0; }
+        const double dVX = (fx[i] - VX / M) * K.v_fac + hx[i];
+        const double dVY = (fy[i] - VY / M) * K.v_fac + hy[i];
+        const double dVZ = (fz[i] - VZ / M) * K.v_fac + hz[i];
+        const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
+        /* v_esc2 = 1e30 inside MACHINE_ZERO: the limit is never exceeded */
+        const double lim = lm[i] >= 0.0 ? lm[i] : K.v2_tune * 1e30;
+        const int nb = !(vel2 > lim);
+        if (lm[i] >= 0.0) {
+          /* certified bound on how far the sequential vel^2 and limit can
+           * lie from these: Phi_j and Phi0 are sums of non-negative terms
+           * (error <= gam*Phi0 each), the velocity sums are within
+           * gam*2*sum|v| (the seed is counted twice); propagated through
+           * the test, plus a few ulps per operation */
+          const double rM = 1.0 / M;
+          const double e_vx = cvx * rM, e_vy = cvy * rM, e_vz = cvz * rM;
+          const double e_vel2 = 2.0 * (fabs(dVX) * e_vx + fabs(dVY) * e_vy + fabs(dVZ) * e_vz) + e_vx * e_vx +
+                                e_vy * e_vy + e_vz * e_vz;
+          const double tol = 4.0 * (e_vel2 + e_lim) + 1e-14 * (vel2 + lim);
+          const double gap = fabs(vel2 - lim);
+          if (gap <= tol || gap <= K.band * lim) unc = 1;
+        }
+        if (nb != bj[i]) { bj[i] = nb; chg = 1; }
+      }
+      if (t == 0) sflag[(ev + 1) & 1] = 0;
+      if (chg | unc) atomicOr(flag, chg | (unc << 1));
+      __syncthreads();
+      const int f = *flag;
+      ev++;
+      if (!(f & 1)) {
+        /* the statuses used were the final ones: so are the totals */
+        uncertain |= f >> 1;
+        cv.x += tot.x; cv.y += tot.y; cv.z += tot.z; cv.m += tot.m;
+#pragma unroll
+        for (int i = 0; i < IPT; i++)
+          if (j0 + i < n) st[j0 + i] = (uint8_t)bj[i];
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  /* exact replay of this iteration where rounding could decide (thread 0) */
+  int replayed = 0;
+  if (uncertain) {
+    replayed = 1;
+    if (t == 0) {
+      double I_now = 0, I_prev = 0, d_prev = 0, M_r = 0, P0 = 0, dd = 0;
+      for (int64_t j = 0; j < n; j++) {
+        M_r += 1.0;
+        dd = dist[j];
+        if (dd > MZERO) {
+          I_now = M_r / (dd * dd);
+          const double I_mid = (I_now + I_prev) / 2.;
+          P0 += I_mid * (dd - d_prev);
+        }
+        d_prev = dd;
+        I_prev = I_now;
+      }
+      P0 += M_r / dd;
+      s_phi0 = P0;
+      double VX = sx, VY = sy, VZ = sz, M = 1.0, Phi = 0;
+      I_now = 0; I_prev = 0; d_prev = 0; M_r = 0;
+      for (int64_t j = 0; j < n; j++) {
+        M_r += 1.0;
+        const double dX = g.dx[j], dY = g.dy[j], dZ = g.dz[j];
+        dd = sqrt(dX * dX + dY * dY + dZ * dZ);
+        double v_esc2;
+        if (dd > MZERO) {
+          I_now = M_r / (dd * dd);
+          const double I_mid = (I_now + I_prev) / 2.;
+          Phi += I_mid * (dd - d_prev);
+          v_esc2 = (2 * fabs(Phi - P0) * K.phi_fac);
+        } else {
+          v_esc2 = 1e30;
+        }
+        const double fx = (double)g.vx[j], fy = (double)g.vy[j], fz = (double)g.vz[j];
+        const double dVX = (fx - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
+        const double dVY = (fy - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
+        const double dVZ = (fz - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
+        const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
+        if (vel2 > K.v2_tune * v_esc2) {
+          st[j] = 0;
+        } else {
+          st[j] = 1;
+          M += 1.0;
+          VX += 1.0 * g.vx[j];
+          VY += 1.0 * g.vy[j];
+          VZ += 1.0 * g.vz[j];
+        }
+        I_prev = I_now;
+        d_prev = dd;
+      }
+    }
+    __syncthreads();
+    Phi0 = s_phi0;
+  }
+  *Phi0_io = Phi0;
+  __syncthreads();
+  return replayed;
+}
+
 template <int BS, int IPT>
 __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uint32_t *a, uint32_t *b,
                             double *dist, double *phi, uint8_t *st, Staged g, const double cen[3],
@@ -439,9 +661,7 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
   __shared__ D4 sh4[NW];
   __shared__ double shd[NW];
   __shared__ int64_t shi[NW];
-  __shared__ int sflag[4];
   __shared__ double sPhi0;
-  __shared__ int64_t s_seed;
   __shared__ unsigned long long s_last; /* 1 + index of the last bound member, 0 = none */
   __shared__ double s_rvir;
   const int t = threadIdx.x;
@@ -543,155 +763,9 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
     }
     __syncthreads();
     if (t == 0) sPhi0 = phi[n - 1] + (double)n / dist[n - 1];
-    /* seed of the bulk velocity (:3441-3470) */
-    if (t == 0) {
-      if (niter == 1) {
-        s_seed = seed0;
-      } else {
-        s_seed = 0;
-      }
-    }
-    for (int64_t j = t; j < n; j += BS) st[j] = 1;
     __syncthreads();
     Phi0 = sPhi0;
-    const double sx = (double)g.vx[s_seed], sy = (double)g.vy[s_seed], sz = (double)g.vz[s_seed];
-    /* the sequential bound-set recurrence, chunk by chunk: a chunk is
-     * evaluated with the final statuses of every earlier chunk (carried in
-     * cv) and its own statuses of the previous evaluation, and evaluated
-     * again until they no longer change; then it holds the sequential answer
-     * and its totals carry to the next chunk.  Only the final evaluation's
-     * margins count for the exact replay. */
-    int uncertain = 0;
-    D4 cv = {sx, sy, sz, 1.0};
-    for (int64_t base = 0; base < n; base += CH) {
-      const int64_t j0 = base + (int64_t)t * IPT;
-      double fx[IPT], fy[IPT], fz[IPT];
-#pragma unroll
-      for (int i = 0; i < IPT; i++) {
-        const int64_t j = j0 + i;
-        fx[i] = fy[i] = fz[i] = 0.0;
-        if (j < n) {
-          fx[i] = (double)g.vx[j]; fy[i] = (double)g.vy[j]; fz[i] = (double)g.vz[j];
-        }
-      }
-      for (;;) {
-        sweeps++;
-        if (t == 0) { sflag[0] = 0; sflag[1] = 0; }
-        __syncthreads();
-        int bj[IPT];
-        D4 run = {0, 0, 0, 0};
-#pragma unroll
-        for (int i = 0; i < IPT; i++) {
-          const int64_t j = j0 + i;
-          bj[i] = j < n ? st[j] : 0;
-          if (bj[i]) { run.x += fx[i]; run.y += fy[i]; run.z += fz[i]; run.m += 1.0; }
-        }
-        D4 tot;
-        const D4 ex = blk_excl_scan4<BS>(run, &tot, sh4);
-        /* the thread's own earlier members, summed again in the same order */
-        D4 lex = {0, 0, 0, 0};
-#pragma unroll
-        for (int i = 0; i < IPT; i++) {
-          const int64_t j = j0 + i;
-          if (j >= n) continue;
-          const double VX = cv.x + ex.x + lex.x, VY = cv.y + ex.y + lex.y, VZ = cv.z + ex.z + lex.z,
-                       M = cv.m + ex.m + lex.m;
-          if (bj[i]) { lex.x += fx[i]; lex.y += fy[i]; lex.z += fz[i]; lex.m += 1.0; }
-          const double dj = dist[j];
-          double v_esc2;
-          if (dj > MZERO) v_esc2 = (2 * fabs(phi[j] - Phi0) * K.phi_fac);
-          else v_esc2 = 1e30;
-          const double dX = g.dx[j], dY = g.dy[j], dZ = g.dz[j];
-          const double dVX = (fx[i] - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
-          const double dVY = (fy[i] - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
-          const double dVZ = (fz[i] - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
-          const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
-          const double lim = K.v2_tune * v_esc2;
-          const int nb = !(vel2 > lim);
-          if (v_esc2 < 1e30) {
-            /* certified bound on how far the sequential vel^2 and limit can
-             * lie from these: Phi_j and Phi0 are sums of non-negative terms
-             * (error <= gam*Phi0 each), the velocity sums are within
-             * gam*2*sum|v| (the seed is counted twice); propagated through
-             * the test, plus a few ulps per operation */
-            const double e_phi = 2.5 * gam * fabs(Phi0);
-            const double e_lim = K.v2_tune * 4.0 * K.phi_fac * e_phi;
-            const double e_vx = fabs(K.v_fac) * 5.0 * gam * asum.x / M;
-            const double e_vy = fabs(K.v_fac) * 5.0 * gam * asum.y / M;
-            const double e_vz = fabs(K.v_fac) * 5.0 * gam * asum.z / M;
-            const double e_vel2 = 2.0 * (fabs(dVX) * e_vx + fabs(dVY) * e_vy + fabs(dVZ) * e_vz) + e_vx * e_vx +
-                                  e_vy * e_vy + e_vz * e_vz;
-            const double tol = 4.0 * (e_vel2 + e_lim) + 1e-14 * (vel2 + lim);
-            const double gap = fabs(vel2 - lim);
-            if (gap <= tol || gap <= K.band * lim) sflag[1] = 1;
-          }
-          if (nb != bj[i]) { st[j] = (uint8_t)nb; sflag[0] = 1; }
-        }
-        __syncthreads();
-        const int changed = sflag[0], unc = sflag[1];
-        __syncthreads();
-        if (!changed) {
-          /* the statuses used were the final ones: so are the totals */
-          uncertain |= unc;
-          cv.x += tot.x; cv.y += tot.y; cv.z += tot.z; cv.m += tot.m;
-          break;
-        }
-      }
-    }
-    /* exact replay of this iteration where rounding could decide (thread 0) */
-    if (uncertain) {
-      replays++;
-      if (t == 0) {
-        double I_now = 0, I_prev = 0, d_prev = 0, M_r = 0, P0 = 0, dd = 0;
-        for (int64_t j = 0; j < n; j++) {
-          M_r += 1.0;
-          dd = dist[j];
-          if (dd > MZERO) {
-            I_now = M_r / (dd * dd);
-            const double I_mid = (I_now + I_prev) / 2.;
-            P0 += I_mid * (dd - d_prev);
-          }
-          d_prev = dd;
-          I_prev = I_now;
-        }
-        P0 += M_r / dd;
-        sPhi0 = P0;
-        double VX = sx, VY = sy, VZ = sz, M = 1.0, Phi = 0;
-        I_now = 0; I_prev = 0; d_prev = 0; M_r = 0;
-        for (int64_t j = 0; j < n; j++) {
-          M_r += 1.0;
-          const double dX = g.dx[j], dY = g.dy[j], dZ = g.dz[j];
-          dd = sqrt(dX * dX + dY * dY + dZ * dZ);
-          double v_esc2;
-          if (dd > MZERO) {
-            I_now = M_r / (dd * dd);
-            const double I_mid = (I_now + I_prev) / 2.;
-            Phi += I_mid * (dd - d_prev);
-            v_esc2 = (2 * fabs(Phi - P0) * K.phi_fac);
-          } else {
-            v_esc2 = 1e30;
-          }
-          const double fx = (double)g.vx[j], fy = (double)g.vy[j], fz = (double)g.vz[j];
-          const double dVX = (fx - VX / M) * K.v_fac + K.hubble * (dX) * K.r_fac;
-          const double dVY = (fy - VY / M) * K.v_fac + K.hubble * (dY) * K.r_fac;
-          const double dVZ = (fz - VZ / M) * K.v_fac + K.hubble * (dZ) * K.r_fac;
-          const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
-          if (vel2 > K.v2_tune * v_esc2) {
-            st[j] = 0;
-          } else {
-            st[j] = 1;
-            M += 1.0;
-            VX += 1.0 * g.vx[j];
-            VY += 1.0 * g.vy[j];
-            VZ += 1.0 * g.vz[j];
-          }
-          I_prev = I_now;
-          d_prev = dd;
-        }
-      }
-      __syncthreads();
-      Phi0 = sPhi0;
-    }
+    replays += rec_eval<BS, IPT>(K, n, niter == 1 ? seed0 : 0, dist, phi, st, g, asum, gam, &Phi0, &sweeps);
     /* stable compaction into the other buffer; R_vir = distance of the last
      * bound member */
     if (t == 0) s_last = 0ull;
@@ -782,6 +856,7 @@ __global__ void k_seed(UnbindConst K, const int64_t *__restrict__ off, int64_t n
 
 /* haloes of at least this many members go to 512-thread workgroups */
 #define AHF_BIG_HALO 2048
+#define AHF_WAVE_HALO 512 /* haloes below this many members: one wavefront each */
 #define AHF_BIG_BLOCK 512
 #define AHF_UNBIND_IPT 4
 
@@ -862,6 +937,324 @@ __global__ void k_copy_u32(const uint32_t *__restrict__ in, int64_t n, uint32_t 
 }
 
 
+/* ------------------------------------------------------- split path */
+
+/*
+ * Haloes of at least split_min members.  On one workgroup their passes over
+ * the members (staging, the Phi prefix, compaction) are bound by what one CU
+ * can fetch; here they run as flat kernels over tiles of all such haloes at
+ * once, and only the recurrence (rec_eval) keeps one workgroup per halo.
+ * The Phi sums come out in another association than the persistent
+ * kernel's; the certified bound of rec_eval holds for any order.
+ */
+#define AHF_SPLIT_BLOCK 512
+#define AHF_SPLIT_DEFAULT 16384 /* default split_min */
+#define AHF_TILE 1024 /* members per tile: 256 threads x 4 */
+
+struct SplitHalo {
+  int64_t h, o, n, pit, last, w, seed;
+  double M_vir, R_vir, Phi0, ovdens;
+  double asum[3], phitot;
+  int32_t niter, replays, in_b, set, rvir_set, sweeps, done, active;
+};
+
+/* tile t covers members [j0, j0 + AHF_TILE) of split halo `seg` */
+struct SplitTile {
+  int32_t seg, first;  /* first: index of the halo's first tile */
+  int64_t j0;
+};
+
+__device__ inline const uint32_t *split_cur(const SplitHalo &H, const uint32_t *a, const uint32_t *b)
+{
+  return H.in_b ? b : a;
+}
+
+/* pass 1 of an iteration: offsets, velocities, distance of every member */
+__global__ void __launch_bounds__(256) k_sp_stage(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
+                                                  const uint32_t *a, const uint32_t *b,
+                                                  const double *__restrict__ cen, const float *__restrict__ px,
+                                                  const float *__restrict__ py, const float *__restrict__ pz,
+                                                  const float *__restrict__ vx, const float *__restrict__ vy,
+                                                  const float *__restrict__ vz, double *__restrict__ dist, Staged g)
+{
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  const uint32_t *cur = split_cur(H, a, b);
+  const double *c = cen + 3 * H.h;
+  for (int i = 0; i < AHF_TILE / 256; i++) {
+    const int64_t j = T.j0 + (int64_t)i * 256 + threadIdx.x;
+    if (j >= H.n) break;
+    const int64_t e = H.o + j;
+    const uint32_t p = cur[e];
+    const double dX = wrap_s((double)px[p] - c[0]), dY = wrap_s((double)py[p] - c[1]), dZ = wrap_s((double)pz[p] - c[2]);
+    dist[e] = sqrt(dX * dX + dY * dY + dZ * dZ);
+    g.dx[e] = dX;
+    g.dy[e] = dY;
+    g.dz[e] = dZ;
+    g.vx[e] = vx[p];
+    g.vy[e] = vy[p];
+    g.vz[e] = vz[p];
+  }
+}
+
+/* the mid-point term of member j (pass 2 of unbind_halo) and its |v| */
+__device__ inline D4 split_term(const double *dist, const Staged &g, int64_t e, int64_t j)
+{
+  D4 r = {0.0, fabs((double)g.vx[e]), fabs((double)g.vy[e]), fabs((double)g.vz[e])};
+  const double dj = dist[e];
+  if (dj > MZERO) {
+    const double I_now = (double)(j + 1) / (dj * dj);
+    double I_prev = 0.0, d_prev = 0.0;
+    if (j > 0) {
+      d_prev = dist[e - 1];
+      if (d_prev > MZERO) I_prev = (double)j / (d_prev * d_prev);
+    }
+    const double I_mid = (I_now + I_prev) / 2.;
+    r.x = I_mid * (dj - d_prev);
+  }
+  return r;
+}
+
+/* the tile's inclusive prefix of (term, |vx|, |vy|, |vz|) into phi (term
+ * part) and its total into ttot */
+__global__ void __launch_bounds__(256) k_sp_phi_tiles(const SplitHalo *__restrict__ sh,
+                                                      const SplitTile *__restrict__ tiles,
+                                                      const double *__restrict__ dist, Staged g,
+                                                      double *__restrict__ phi, D4 *__restrict__ ttot)
+{
+  __shared__ D4 sh4[4];
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  const int64_t j0 = T.j0 + (int64_t)threadIdx.x * 4;
+  D4 run = {0, 0, 0, 0};
+  double loc[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int64_t j = j0 + i;
+    if (j < H.n) {
+      const D4 r = split_term(dist, g, H.o + j, j);
+      run.x += r.x; run.y += r.y; run.z += r.z; run.m += r.m;
+    }
+    loc[i] = run.x;
+  }
+  D4 tot;
+  const D4 ex = blk_excl_scan4<256>(run, &tot, sh4);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (j0 + i < H.n) phi[H.o + j0 + i] = ex.x + loc[i];
+  if (threadIdx.x == 0) ttot[blockIdx.x] = tot;
+}
+
+/* per split halo, in tile order: the carry into each tile (exclusive sum of
+ * the tiles before it) and the halo's totals */
+__global__ void k_sp_phi_carry(SplitHalo *sh, int64_t nseg, const int32_t *__restrict__ seg_tile0,
+                               const D4 *__restrict__ ttot, D4 *__restrict__ tcarry)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg || !sh[i].active) return;
+  D4 acc = {0, 0, 0, 0};
+  for (int32_t t = seg_tile0[i]; t < seg_tile0[i + 1]; t++) {
+    tcarry[t] = acc;
+    acc.x += ttot[t].x; acc.y += ttot[t].y; acc.z += ttot[t].z; acc.m += ttot[t].m;
+  }
+  sh[i].phitot = acc.x;
+  sh[i].asum[0] = acc.y;
+  sh[i].asum[1] = acc.z;
+  sh[i].asum[2] = acc.m;
+}
+
+__global__ void __launch_bounds__(256) k_sp_phi_add(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
+                                                    const D4 *__restrict__ tcarry, double *__restrict__ phi)
+{
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  const double cx = tcarry[blockIdx.x].x;
+  for (int i = 0; i < AHF_TILE / 256; i++) {
+    const int64_t j = T.j0 + (int64_t)i * 256 + threadIdx.x;
+    if (j < H.n) phi[H.o + j] = cx + phi[H.o + j];
+  }
+}
+
+/* the recurrence of one iteration, one workgroup per active split halo */
+__global__ void __launch_bounds__(AHF_SPLIT_BLOCK) k_sp_rec(UnbindConst K, SplitHalo *sh, const double *dist,
+                                                            const double *phi, uint8_t *st, Staged g)
+{
+  SplitHalo &H = sh[blockIdx.x];
+  if (!H.active) return;
+  const int64_t o = H.o, n = H.n;
+  const Staged go = {g.dx + o, g.dy + o, g.dz + o, g.vx + o, g.vy + o, g.vz + o};
+  double Phi0 = phi[o + n - 1] + (double)n / dist[o + n - 1];
+  const double Nd = (double)n + 8.0, U = 1.1102230246251565e-16;
+  const double gam = Nd * U / (1.0 - Nd * U);
+  const D4 asum = {H.asum[0], H.asum[1], H.asum[2], 0.0};
+  int sweeps = 0;
+  const int rep = rec_eval<AHF_SPLIT_BLOCK, AHF_UNBIND_IPT>(K, n, H.niter == 1 ? H.seed : 0, dist + o, phi + o,
+                                                            st + o, go, asum, gam, &Phi0, &sweeps);
+  if (threadIdx.x == 0) {
+    H.Phi0 = Phi0;
+    H.replays += rep;
+    H.sweeps += sweeps;
+  }
+}
+
+/* compaction: the tile's exclusive count of bound members, its total, and
+ * 1 + the index of its last bound member */
+__global__ void __launch_bounds__(256) k_sp_cnt_tiles(const SplitHalo *__restrict__ sh,
+                                                      const SplitTile *__restrict__ tiles,
+                                                      const uint8_t *__restrict__ st, int32_t *__restrict__ wloc,
+                                                      int64_t *__restrict__ tcnt, int64_t *__restrict__ tlast)
+{
+  __shared__ int64_t shi[4];
+  __shared__ unsigned long long s_last;
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  if (threadIdx.x == 0) s_last = 0ull;
+  const int64_t j0 = T.j0 + (int64_t)threadIdx.x * 4;
+  int64_t cnt = 0;
+  unsigned long long last = 0ull;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (j0 + i < H.n && st[H.o + j0 + i]) {
+      cnt++;
+      last = (unsigned long long)(j0 + i + 1);
+    }
+  int64_t tot;
+  int64_t w = blk_excl_scan<256, int64_t>(cnt, &tot, shi);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (j0 + i < H.n && st[H.o + j0 + i]) wloc[H.o + j0 + i] = (int32_t)(w++);
+  if (last) atomicMax(&s_last, last);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    tcnt[blockIdx.x] = tot;
+    tlast[blockIdx.x] = (int64_t)s_last;
+  }
+}
+
+__global__ void k_sp_cnt_carry(SplitHalo *sh, int64_t nseg, const int32_t *__restrict__ seg_tile0,
+                               const int64_t *__restrict__ tcnt, const int64_t *__restrict__ tlast,
+                               int64_t *__restrict__ tcarry)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg || !sh[i].active) return;
+  int64_t acc = 0, last = 0;
+  for (int32_t t = seg_tile0[i]; t < seg_tile0[i + 1]; t++) {
+    tcarry[t] = acc;
+    acc += tcnt[t];
+    if (tlast[t] > last) last = tlast[t];
+  }
+  sh[i].w = acc;
+  sh[i].last = last;
+}
+
+__global__ void __launch_bounds__(256) k_sp_scatter(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
+                                                    const int64_t *__restrict__ tcarry, const uint8_t *__restrict__ st,
+                                                    const int32_t *__restrict__ wloc, uint32_t *a, uint32_t *b)
+{
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  const uint32_t *cur = H.in_b ? b : a;
+  uint32_t *nxt = H.in_b ? a : b;
+  const int64_t base = tcarry[blockIdx.x];
+  for (int i = 0; i < AHF_TILE / 256; i++) {
+    const int64_t j = T.j0 + (int64_t)i * 256 + threadIdx.x;
+    if (j < H.n && st[H.o + j]) nxt[H.o + base + wloc[H.o + j]] = cur[H.o + j];
+  }
+}
+
+/* the end of an iteration: rem_unbound's bookkeeping (:3590-3600) */
+__global__ void k_sp_update(UnbindConst K, SplitHalo *sh, int64_t nseg, const double *__restrict__ dist)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg || !sh[i].active) return;
+  SplitHalo &H = sh[i];
+  if (H.last > 0) {
+    H.R_vir = dist[H.o + H.last - 1];
+    H.rvir_set = 1;
+  }
+  const int64_t nremove = H.n - H.w;
+  H.n = H.w;
+  H.M_vir = (double)H.w;
+  H.in_b ^= 1;
+  if (!(nremove > 3) || (double)H.n < (double)K.minpart) H.done = 1;
+}
+
+/* the start of an iteration (pit, niter), and the active flags */
+__global__ void k_sp_begin(SplitHalo *sh, int64_t nseg)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  SplitHalo &H = sh[i];
+  H.active = !H.done;
+  if (H.active) {
+    H.niter++;
+    H.pit += H.n;
+  }
+}
+
+/* rem_outsideRvir on a split halo's current list (call 0 before the
+ * unbinding, call 1 after), with the field bookkeeping of unbind_halo */
+__global__ void __launch_bounds__(AHF_SPLIT_BLOCK) k_sp_cut(UnbindConst K, SplitHalo *sh, int call, const uint32_t *a,
+                                                            const uint32_t *b, const double *__restrict__ cen,
+                                                            const float *__restrict__ px, const float *__restrict__ py,
+                                                            const float *__restrict__ pz)
+{
+  SplitHalo &H = sh[blockIdx.x];
+  if (H.n < K.minpart) return;
+  double M, R, od;
+  const double c3[3] = {cen[3 * H.h], cen[3 * H.h + 1], cen[3 * H.h + 2]};
+  const int64_t kept = rvir_cut<AHF_SPLIT_BLOCK>(K, H.n, split_cur(H, a, b) + H.o, c3, px, py, pz, &M, &R, &od);
+  if (threadIdx.x == 0) {
+    H.n = kept;
+    H.M_vir = M;
+    H.R_vir = R;
+    H.ovdens = od;
+    if (call == 0) {
+      H.Phi0 = 0.0;
+      H.set = AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_PHI0 | AHFGPU_SET_OVDENS;
+    } else {
+      H.set |= AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_OVDENS;
+    }
+  }
+}
+
+/* entry into the unbinding proper (:3312): done below minpart, else R_vir
+ * becomes the local of rem_unbound */
+__global__ void k_sp_enter(UnbindConst K, SplitHalo *sh, int64_t nseg)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  SplitHalo &H = sh[i];
+  if (K.unbind && H.n >= K.minpart) {
+    H.R_vir = 0.0;
+    H.set = (H.set | AHFGPU_SET_MVIR | AHFGPU_SET_PHI0) & ~AHFGPU_SET_RVIR;
+    H.done = 0;
+  } else {
+    H.done = 1;
+  }
+}
+
+__global__ void k_sp_out(const SplitHalo *__restrict__ sh, int64_t nseg, HaloOut *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  const SplitHalo &H = sh[i];
+  HaloOut o;
+  o.npart = H.n;
+  o.M_vir = H.M_vir;
+  o.R_vir = H.R_vir;
+  o.Phi0 = H.Phi0;
+  o.ovdens = H.ovdens;
+  o.niter = H.niter;
+  o.replays = H.replays;
+  o.in_b = H.in_b;
+  o.set = H.set | (H.rvir_set ? AHFGPU_SET_RVIR : 0);
+  o.particle_iterations = H.pit;
+  o.sweeps = H.sweeps;
+  o.pad = 0;
+  out[H.h] = o;
+}
+
 /* ============================================================ host side */
 
 static inline unsigned nb_of(int64_t n)
@@ -894,6 +1287,98 @@ extern "C" int ahfgpu_set_halos(ahfgpu_ctx *c, const double *centres, const uint
     return c->fail(AHFGPU_EHIP, "halo upload");
   }
   c->halo_set = s;
+  return 0;
+}
+
+/*
+ * The split path for the haloes ord[0..nsplit) (the largest), on the second
+ * stream, after the sort and the seeds on the first: rem_outsideRvir(0),
+ * then iterations of staging / Phi prefix / recurrence / compaction over
+ * tiles of all unfinished haloes until each one stops as rem_unbound does,
+ * then rem_outsideRvir(1).  The host reads the haloes' state between
+ * iterations to lay out the next iteration's tiles.  The first stream waits
+ * for the result before the timing ends.
+ */
+__global__ void k_sp_init(SplitHalo *sh, int64_t nseg, const int32_t *__restrict__ hid, const int64_t *__restrict__ off,
+                          const int64_t *__restrict__ seed)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  SplitHalo H;
+  memset(&H, 0, sizeof(H));
+  H.h = hid[i];
+  H.o = off[H.h];
+  H.n = off[H.h + 1] - H.o;
+  H.seed = seed[H.h];
+  sh[i] = H;
+}
+
+static int split_unbind(ahfgpu_ctx *c, const UnbindConst &K, HaloSet *s, const int32_t *ord, int64_t nsplit,
+                        const int64_t *seed, double *dist, double *phi, uint8_t *st, Staged g)
+{
+  Arena &A = c->uarena;
+  SplitHalo *sh = A.get<SplitHalo>(nsplit);
+  int32_t *hid = A.get<int32_t>(nsplit);
+  int32_t *wloc = A.get<int32_t>(s->ne);
+  int32_t *seg_tile0 = A.get<int32_t>(nsplit + 1);
+  int64_t max_tiles = 0;
+  for (int64_t i = 0; i < nsplit; i++) max_tiles += (s->h_off[ord[i] + 1] - s->h_off[ord[i]] + AHF_TILE - 1) / AHF_TILE;
+  SplitTile *tiles = A.get<SplitTile>(max_tiles);
+  D4 *ttot = A.get<D4>(max_tiles), *tcar = A.get<D4>(max_tiles);
+  int64_t *tcnt = A.get<int64_t>(max_tiles), *tlast = A.get<int64_t>(max_tiles), *tcarry = A.get<int64_t>(max_tiles);
+  if (!sh || !hid || !wloc || !seg_tile0 || !tiles || !ttot || !tcar || !tcnt || !tlast || !tcarry)
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (split unbinding)");
+  hipStream_t s2 = c->st2;
+  hipStreamWaitEvent(s2, c->ev_s0, 0);
+  hipMemcpyAsync(hid, ord, (size_t)nsplit * sizeof(int32_t), hipMemcpyHostToDevice, s2);
+  const unsigned gs = nb_of(nsplit);
+  hipLaunchKernelGGL(k_sp_init, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, hid, s->off, seed);
+  if (K.cut_before)
+    hipLaunchKernelGGL(k_sp_cut, dim3((unsigned)nsplit), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, 0, s->buf_a, s->buf_b,
+                       s->cen, c->px, c->py, c->pz);
+  hipLaunchKernelGGL(k_sp_enter, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit);
+  std::vector<SplitHalo> hs((size_t)nsplit);
+  std::vector<SplitTile> ht;
+  std::vector<int32_t> t0((size_t)nsplit + 1);
+  for (;;) {
+    hipMemcpyAsync(hs.data(), sh, (size_t)nsplit * sizeof(SplitHalo), hipMemcpyDeviceToHost, s2);
+    if (hipStreamSynchronize(s2) != hipSuccess) return c->fail(AHFGPU_EHIP, "split unbinding");
+    ht.clear();
+    for (int64_t i = 0; i < nsplit; i++) {
+      t0[(size_t)i] = (int32_t)ht.size();
+      if (hs[(size_t)i].done) continue;
+      for (int64_t j0 = 0; j0 < hs[(size_t)i].n; j0 += AHF_TILE) ht.push_back(SplitTile{(int32_t)i, t0[(size_t)i], j0});
+    }
+    t0[(size_t)nsplit] = (int32_t)ht.size();
+    if (ht.empty()) break;
+    const unsigned nt = (unsigned)ht.size();
+    hipMemcpyAsync(tiles, ht.data(), ht.size() * sizeof(SplitTile), hipMemcpyHostToDevice, s2);
+    hipMemcpyAsync(seg_tile0, t0.data(), t0.size() * sizeof(int32_t), hipMemcpyHostToDevice, s2);
+    hipLaunchKernelGGL(k_sp_begin, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit);
+    hipLaunchKernelGGL(k_sp_stage, dim3(nt), dim3(256), 0, s2, sh, tiles, s->buf_a, s->buf_b, s->cen, c->px, c->py,
+                       c->pz, c->vx, c->vy, c->vz, dist, g);
+    hipLaunchKernelGGL(k_sp_phi_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, dist, g, phi, ttot);
+    hipLaunchKernelGGL(k_sp_phi_carry, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0, ttot, tcar);
+    hipLaunchKernelGGL(k_sp_phi_add, dim3(nt), dim3(256), 0, s2, sh, tiles, tcar, phi);
+    hipLaunchKernelGGL(k_sp_rec, dim3((unsigned)nsplit), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, dist, phi, st, g);
+    hipLaunchKernelGGL(k_sp_cnt_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, st, wloc, tcnt, tlast);
+    hipLaunchKernelGGL(k_sp_cnt_carry, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0, tcnt, tlast, tcarry);
+    hipLaunchKernelGGL(k_sp_scatter, dim3(nt), dim3(256), 0, s2, sh, tiles, tcarry, st, wloc, s->buf_a, s->buf_b);
+    hipLaunchKernelGGL(k_sp_update, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit, dist);
+  }
+  if (K.cut_after)
+    hipLaunchKernelGGL(k_sp_cut, dim3((unsigned)nsplit), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, 1, s->buf_a, s->buf_b,
+                       s->cen, c->px, c->py, c->pz);
+  hipLaunchKernelGGL(k_sp_out, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, s->out);
+  hipEventRecord(c->ev_s1, s2);
+  hipStreamWaitEvent(c->st, c->ev_s1, 0);
+  return 0;
+}
+
+extern "C" int ahfgpu_set_unbind_split(ahfgpu_ctx *c, int64_t min_members)
+{
+  if (!c) return AHFGPU_EINVAL;
+  c->split_min = min_members;
   return 0;
 }
 
@@ -943,7 +1428,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   Staged g = {A.get<double>(ne), A.get<double>(ne), A.get<double>(ne),
               A.get<float>(ne), A.get<float>(ne), A.get<float>(ne)};
   s->out = A.get<HaloOut>(nh);
-  int32_t *next = A.get<int32_t>(2);
+  int32_t *next = A.get<int32_t>(3);
   int64_t *seed = A.get<int64_t>(nh);
   if (!k0 || !k1 || !v0 || !v1 || !hk0 || !hk1 || !rtmp || !s->buf_a || !s->buf_b || !dist || !phi || !st ||
       !s->out || !next || !seed || !g.dx || !g.dy || !g.dz || !g.vx || !g.vy || !g.vz)
@@ -956,35 +1441,72 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   const int run = mode & 3;
   int dev_cu = 256;
   hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
-  /* the largest halo that is sorted decides the sort: LDS per halo, or the
-   * radix sort of all members */
-  int64_t max_sorted = 0;
+  /* haloes of at most AHF_SORT_SMALL members are sorted in LDS, one
+   * workgroup each, in tiers by size; the bigger ones by the radix sort of
+   * their members */
+  std::vector<int64_t> bo(1, 0);
+  std::vector<int32_t> bh;
+  int64_t tcount[AHF_SORT_TIERS] = {0, 0, 0, 0, 0};
   for (int64_t h = 0; h < nh; h++) {
     const int64_t n = s->h_off[h + 1] - s->h_off[h];
-    if (n >= p->minpart && n > max_sorted) max_sorted = n;
+    const int t = sort_tier(n, p->minpart);
+    if (t >= 0) {
+      tcount[t]++;
+    } else {
+      bh.push_back((int32_t)h);
+      bo.push_back(bo.back() + n);
+    }
   }
-  const bool small_sort = max_sorted <= AHF_SORT_SMALL;
+  const int64_t nsel = (int64_t)bh.size(), nes = bo.back();
   const bool sort_only = (mode & AHFGPU_UNBIND_SORT_ONLY) != 0;
   const bool no_sort = (mode & (AHFGPU_UNBIND_NO_SORT | AHFGPU_UNBIND_RVIR_ONLY)) != 0;
   if (ne > 0 && no_sort) {
     hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->ip_in, ne, s->buf_a);
-  } else if (ne > 0 && small_sort) {
-    const int64_t grid = std::min<int64_t>(nh, (int64_t)dev_cu * 2);
-    hipLaunchKernelGGL(k_sort_small, dim3((unsigned)grid), dim3(AHF_SORT_THREADS), 0, c->st, s->off, nh, s->ip_in,
-                       s->cen, c->px, c->py, c->pz, p->minpart, s->buf_a);
   } else if (ne > 0) {
-    hipLaunchKernelGGL(k_dist_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, ne, s->ip_in, s->cen,
-                       c->px, c->py, c->pz, p->minpart, k0, v0);
-    int w = radix_sort<uint64_t, uint32_t>(k0, v0, k1, v1, ne, 64, (uint32_t *)rtmp, c->st);
-    uint32_t *vs = w ? v1 : v0;
-    uint32_t *vo = w ? v0 : v1;
-    hipLaunchKernelGGL(k_halo_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, ne, vs, hk0);
-    int hb = 1;
-    while ((1ll << hb) < nh) hb++;
-    int w2 = radix_sort<uint32_t, uint32_t>(hk0, vs, hk1, vo, ne, hb, (uint32_t *)rtmp, c->st);
-    uint32_t *vsorted = w2 ? vo : vs;
-    hipLaunchKernelGGL(k_gather_members, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, vsorted, s->ip_in, ne,
-                       s->buf_a);
+    if (nh > nsel) {
+      SortBase sb;
+      int64_t acc = 0;
+      for (int t = 0; t < AHF_SORT_TIERS; t++) {
+        sb.base[t] = acc;
+        acc += tcount[t];
+      }
+      int32_t *tcnt = A.get<int32_t>(AHF_SORT_TIERS);
+      int32_t *tlist = A.get<int32_t>(acc);
+      if (!tcnt || !tlist) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (sort)");
+      hipMemsetAsync(tcnt, 0, AHF_SORT_TIERS * sizeof(int32_t), c->st);
+      hipLaunchKernelGGL(k_sort_classify, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, p->minpart, sb, tcnt,
+                         tlist);
+      auto grid_of = [](int64_t n) { return dim3((unsigned)std::min<int64_t>(n, (int64_t)1 << 20)); };
+#define AHF_SORT_LAUNCH(T, CAP, BS, COPY)                                                                          \
+  if (tcount[T] > 0)                                                                                               \
+    hipLaunchKernelGGL((k_sort_tier<CAP, BS>), grid_of(tcount[T]), dim3(BS), 0, c->st, tlist + sb.base[T],         \
+                       tcount[T], s->off, s->ip_in, s->cen, c->px, c->py, c->pz, COPY, s->buf_a);
+      AHF_SORT_LAUNCH(4, AHF_SORT_SMALL, 1024, 0)
+      AHF_SORT_LAUNCH(3, 2048, 512, 0)
+      AHF_SORT_LAUNCH(2, 512, 256, 0)
+      AHF_SORT_LAUNCH(1, 64, 64, 0)
+      AHF_SORT_LAUNCH(0, 64, 64, 1)
+#undef AHF_SORT_LAUNCH
+    }
+    if (nsel > 0) {
+      int64_t *d_bo = A.get<int64_t>(nsel + 1);
+      int32_t *d_bh = A.get<int32_t>(nsel);
+      if (!d_bo || !d_bh) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (sort)");
+      hipMemcpyAsync(d_bo, bo.data(), (size_t)(nsel + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->st);
+      hipMemcpyAsync(d_bh, bh.data(), (size_t)nsel * sizeof(int32_t), hipMemcpyHostToDevice, c->st);
+      hipLaunchKernelGGL(k_dist_keys_sel, dim3(nb_of(nes)), dim3(AHF_BLOCK), 0, c->st, d_bo, d_bh, nsel, nes, s->off,
+                         s->ip_in, s->cen, c->px, c->py, c->pz, k0, v0);
+      int w = radix_sort<uint64_t, uint32_t>(k0, v0, k1, v1, nes, 64, (uint32_t *)rtmp, c->st);
+      uint32_t *vs = w ? v1 : v0;
+      uint32_t *vo = w ? v0 : v1;
+      hipLaunchKernelGGL(k_halo_keys, dim3(nb_of(nes)), dim3(AHF_BLOCK), 0, c->st, d_bo, nsel, nes, vs, hk0);
+      int hb = 1;
+      while ((1ll << hb) < nsel) hb++;
+      int w2 = radix_sort<uint32_t, uint32_t>(hk0, vs, hk1, vo, nes, hb, (uint32_t *)rtmp, c->st);
+      uint32_t *vsorted = w2 ? vo : vs;
+      hipLaunchKernelGGL(k_gather_members_sel, dim3(nb_of(nes)), dim3(AHF_BLOCK), 0, c->st, d_bo, d_bh, nsel, nes,
+                         s->off, vsorted, s->ip_in, s->buf_a);
+    }
   }
   c->timer_end(es, TIMER_USORT, 2.0 * 12.0 * (double)ne + 12.0 * (double)ne);
   hipEvent_t eu = c->timer_begin();
@@ -997,27 +1519,51 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   if (sort_only) {
     /* nothing more */
   } else if (run == 0) {
-    /* the order is largest first: haloes of AHF_BIG_HALO members or more
-     * get AHF_BIG_BLOCK-thread workgroups, the rest AHF_BLOCK */
-    int64_t nbig = 0;
-    {
-      std::vector<int32_t> ord((size_t)nh);
-      if (nh) hipMemcpy(ord.data(), s->order, (size_t)nh * sizeof(int32_t), hipMemcpyDeviceToHost);
-      while (nbig < nh && s->h_off[ord[nbig] + 1] - s->h_off[ord[nbig]] >= AHF_BIG_HALO) nbig++;
-    }
-    hipMemsetAsync(next, 0, 2 * sizeof(int32_t), c->st);
-    if (nbig > 0) {
-      const int64_t grid = std::min<int64_t>(nbig, (int64_t)dev_cu * 2);
+    /* the order is largest first: haloes of split_min members or more take
+     * the split path on the second stream; of the rest, haloes of
+     * AHF_BIG_HALO members or more get AHF_BIG_BLOCK-thread workgroups, the
+     * others AHF_BLOCK */
+    int64_t nbig = 0, nsplit = 0;
+    std::vector<int32_t> ord((size_t)nh);
+    if (nh) hipMemcpy(ord.data(), s->order, (size_t)nh * sizeof(int32_t), hipMemcpyDeviceToHost);
+    const int64_t split_min = c->split_min > 0 ? c->split_min : AHF_SPLIT_DEFAULT;
+    auto size_of = [&](int64_t q) { return s->h_off[ord[q] + 1] - s->h_off[ord[q]]; };
+    if (c->split_min >= 0)
+      while (nsplit < nh && size_of(nsplit) >= split_min && size_of(nsplit) >= p->minpart) nsplit++;
+    while (nbig < nh && size_of(nbig) >= AHF_BIG_HALO) nbig++;
+    if (nbig < nsplit) nbig = nsplit;
+    /* haloes below AHF_WAVE_HALO members: one wavefront each */
+    int64_t nmid = nbig;
+    while (nmid < nh && size_of(nmid) >= AHF_WAVE_HALO) nmid++;
+    hipMemsetAsync(next, 0, 3 * sizeof(int32_t), c->st);
+    /* the second and third streams start after the sort, the seeds and the
+     * queue counters */
+    hipEventRecord(c->ev_s0, c->st);
+    if (nbig > nsplit) {
+      const int64_t grid = std::min<int64_t>(nbig - nsplit, (int64_t)dev_cu * 2);
       hipLaunchKernelGGL(k_unbind_batched<AHF_BIG_BLOCK>, dim3((unsigned)grid), dim3(AHF_BIG_BLOCK), 0, c->st, K,
-                         s->off, s->order, (int64_t)0, nbig, next, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px,
+                         s->off, s->order, nsplit, nbig, next, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px,
                          c->py, c->pz, c->vx, c->vy, c->vz, s->out);
     }
-    if (nh > nbig) {
-      const int64_t grid = std::min<int64_t>(nh - nbig, (int64_t)dev_cu * 8);
+    if (nmid > nbig) {
+      const int64_t grid = std::min<int64_t>(nmid - nbig, (int64_t)dev_cu * 8);
       hipLaunchKernelGGL(k_unbind_batched<AHF_BLOCK>, dim3((unsigned)grid), dim3(AHF_BLOCK), 0, c->st, K, s->off,
-                         s->order, nbig, nh, next + 1, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px,
+                         s->order, nbig, nmid, next + 1, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px,
                          c->py,
                          c->pz, c->vx, c->vy, c->vz, s->out);
+    }
+    if (nh > nmid) {
+      const int64_t grid = std::min<int64_t>(nh - nmid, (int64_t)dev_cu * 32);
+      hipStreamWaitEvent(c->st3, c->ev_s0, 0);
+      hipLaunchKernelGGL(k_unbind_batched<64>, dim3((unsigned)grid), dim3(64), 0, c->st3, K, s->off, s->order, nmid,
+                         nh, next + 2, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx,
+                         c->vy, c->vz, s->out);
+      hipEventRecord(c->ev_s2, c->st3);
+      hipStreamWaitEvent(c->st, c->ev_s2, 0);
+    }
+    if (nsplit > 0) {
+      int rc = split_unbind(c, K, s, ord.data(), nsplit, seed, dist, phi, st, g);
+      if (rc) return rc;
     }
   } else {
     /* plain form: one halo per launch, results synchronised per halo as a

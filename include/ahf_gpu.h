@@ -184,6 +184,15 @@ int ahfgpu_halo_members(ahfgpu_ctx *ctx, uint64_t *npart, uint64_t *ipart);
 int ahfgpu_unbind(ahfgpu_ctx *ctx, const ahfgpu_unbind_params *p, int mode, double band,
                   ahfgpu_unbind_info *info);
 
+/* No counterpart in the reference: in the batched mode, haloes of at least
+ * min_members members (and NminPerHalo) are unbound on a second path that
+ * runs their passes over the members as flat kernels over all such haloes
+ * and keeps one workgroup per halo only for the recurrence; the others stay
+ * in the persistent kernel.  0 restores the default (16384), a negative
+ * value sends every halo to the persistent kernel.  Both paths give the
+ * same members and fields. */
+int ahfgpu_set_unbind_split(ahfgpu_ctx *ctx, int64_t min_members);
+
 /* results of the last ahfgpu_unbind: per halo npart and AHFGPU_PROPS numbers
  * (M_vir, R_vir, Phi0, ovdens, unbinding iterations, and a mask of the
  * AHFGPU_SET_* fields that some step wrote -- the others keep the caller's

@@ -20,7 +20,8 @@ def ctx():
     c.close()
 
 
-def _run_gpu(ctx, s, mode, band=0.0):
+def _run_gpu(ctx, s, mode, band=0.0, split=0):
+    ctx.set_unbind_split(split)
     ctx.set_particles(s.pos, s.mom)
     ctx.set_halos(s.centre, s.halo_npart, s.ipart)
     info = ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=mode, band=band)
@@ -41,23 +42,27 @@ def _compare(ref, npart, props, lists, tag):
     assert rel.size == 0 or rel.max() < 1e-9, f"{tag}: Phi0 rel err {rel.max():.2e}"
 
 
+# split: 0 the default threshold, 32 every halo of 32 members or more through
+# the split path, -1 none
+@pytest.mark.parametrize("split", [0, 32, -1])
 @pytest.mark.parametrize("cfg,side,seed,mode", [(1, 64, 1001, 0), (1, 64, 1001, 1), (0, 64, 1000, 0),
                                                 (2, 2000, 1002, 0), (2, 300, 7, 1)])
-def test_unbind_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, mode):
+def test_unbind_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, mode, split):
     snap = str(tmp_path / "h.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
     out = str(tmp_path / "o.halo")
     assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 1) == 0
     ref = dumps.read_halos(out)
     s = dumps.read_snapshot(snap)
-    print(f"[unbind-gpu] cfg{cfg} side {side} mode {mode}: {s.nhalos} haloes {s.n_ipart} members", flush=True)
-    info, (npart, props, lists) = _run_gpu(ctx, s, mode)
+    print(f"[unbind-gpu] cfg{cfg} side {side} mode {mode} split {split}: {s.nhalos} haloes {s.n_ipart} members",
+          flush=True)
+    info, (npart, props, lists) = _run_gpu(ctx, s, mode, split=split)
     _compare(ref, npart, props, lists, f"cfg{cfg}/{side}/mode{mode}")
     assert info.n_member_out == int(ref.npart.sum())
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_unbind_replay_band(oracle, snapgen, ctx, tmp_path, mode):
+@pytest.mark.parametrize("mode,split", [(0, 0), (1, 0), (0, 32)])
+def test_unbind_replay_band(oracle, snapgen, ctx, tmp_path, mode, split):
     """a wide band sends haloes with members near the threshold to the exact
     sequential replay: the results must still be the oracle's"""
     snap = str(tmp_path / "b.snap")
@@ -66,7 +71,7 @@ def test_unbind_replay_band(oracle, snapgen, ctx, tmp_path, mode):
     assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 1) == 0
     ref = dumps.read_halos(out)
     s = dumps.read_snapshot(snap)
-    info, (npart, props, lists) = _run_gpu(ctx, s, mode, band=1e-2)
+    info, (npart, props, lists) = _run_gpu(ctx, s, mode, band=1e-2, split=split)
     assert info.n_exact_replays > 0
     _compare(ref, npart, props, lists, f"band/mode{mode}")
 
@@ -86,7 +91,8 @@ def test_unbind_repeatable(snapgen, ctx, tmp_path):
 WITH_RVIR, RVIR_ONLY = 16, 32
 
 
-def _construct_gpu(ctx, s, mode, centre=None, npart=None, ipart=None):
+def _construct_gpu(ctx, s, mode, centre=None, npart=None, ipart=None, split=0):
+    ctx.set_unbind_split(split)
     ctx.set_particles(s.pos, s.mom)
     ctx.set_halos(s.centre if centre is None else centre, s.halo_npart if npart is None else npart,
                   s.ipart if ipart is None else ipart)
@@ -95,9 +101,10 @@ def _construct_gpu(ctx, s, mode, centre=None, npart=None, ipart=None):
     return info, ctx.unbind_results()
 
 
+@pytest.mark.parametrize("split", [0, 32])
 @pytest.mark.parametrize("cfg,side,seed,plain", [(1, 64, 1001, 0), (1, 64, 1001, 1), (0, 64, 1000, 0),
                                                  (2, 2000, 1002, 0), (2, 300, 7, 1), (3, 48, 5, 0)])
-def test_construct_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, plain):
+def test_construct_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, plain, split):
     """sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1) in one pass"""
     snap = str(tmp_path / "c.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
@@ -106,7 +113,7 @@ def test_construct_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, see
     ref = dumps.read_halos(out)
     s = dumps.read_snapshot(snap)
     print(f"[construct-gpu] cfg{cfg} side {side} plain {plain}: {s.nhalos} haloes {s.n_ipart} members", flush=True)
-    info, (npart, props, lists) = _construct_gpu(ctx, s, plain | WITH_RVIR)
+    info, (npart, props, lists) = _construct_gpu(ctx, s, plain | WITH_RVIR, split=split)
     _compare(ref, npart, props, lists, f"construct cfg{cfg}/{side}/{plain}")
     assert np.array_equal(ref.R_vir, props[:, 1])
     assert np.array_equal(ref.ovdens, props[:, 3])
@@ -132,7 +139,7 @@ def test_rvir_only_matches_oracle(oracle, snapgen, ctx, tmp_path, plain):
         seg = ip[off[h]:off[h + 1]]
         c = np.ascontiguousarray(s.centre[h])
         oracle.oracle_sort_halo(s.pos.ctypes.data, len(seg), seg.ctypes.data, c.ctypes.data, s.minpart, 1)
-    info, (npart, props, lists) = _construct_gpu(ctx, s, plain | RVIR_ONLY, ipart=ip)
+    info, (npart, props, lists) = _construct_gpu(ctx, s, plain | RVIR_ONLY, ipart=ip, split=32)
     P = _RvirParams(s.minpart, s.ovlim, s.rho_vir, s.rho_fac)
     cut = 0
     for h in range(s.nhalos):
