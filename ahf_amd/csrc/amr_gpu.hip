@@ -510,6 +510,208 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_src(level_view v, const ui
   dens[n] = d;
 }
 
+#define AHF_SLAB 16                          /* target tile: 16x16 nodes of one plane */
+#define AHF_SLAB_W (AHF_SLAB + 2)            /* its sources in one plane: 18x18 nodes */
+#define AHF_SLAB_N (AHF_SLAB_W * AHF_SLAB_W)
+#define AHF_SLAB_CAP_BIG 2752                /* staged entries per plane, dense levels */
+#define AHF_SLAB_CAP_SMALL 512               /* the same for about one entry per node */
+
+/* node (X,Y,Z) of a level that holds every octet: its octets are all
+ * (L/2)^3 keys in (z,y,x) order */
+__device__ __forceinline__ int64_t full_node(int64_t X, int64_t Y, int64_t Z, int64_t H)
+{
+  const int64_t o = ((Z >> 1) * H + (Y >> 1)) * H + (X >> 1);
+  return 8 * o + ((Z & 1) << 2 | (Y & 1) << 1 | (X & 1));
+}
+
+/* ahf_tsc_w(s, d) for an offset d known only at run time, as
+ * A + B*(C + D*s)^2 with
+ *   d =  0: C = 0,   D = 1,  A = 0.75, B = -1   -> 0.75 - s*s
+ *   d = +1: C = 0.5, D = 1,  A = 0,    B = 0.5  -> ((0.5 + s)^2)/2
+ *   d = -1: C = 0.5, D = -1, A = 0,    B = 0.5  -> ((0.5 - s)^2)/2
+ * Every step is exact or rounds exactly as AHF's formula (products with
+ * +-1 and 0.5 and sums with 0 are exact, x + (-y) is x - y), so the bits
+ * are those of ahf_tsc_w. */
+struct TscRt {
+  double A, B, C, D;
+  __device__ __forceinline__ explicit TscRt(int d)
+      : A(d == 0 ? 0.75 : 0.0), B(d == 0 ? -1.0 : 0.5), C(d == 0 ? 0.0 : 0.5), D(d < 0 ? -1.0 : 1.0)
+  {
+  }
+  __device__ __forceinline__ double operator()(double s) const
+  {
+    const double q = C + D * s;
+    return A + B * (q * q);
+  }
+};
+
+/*
+ * Exact-order density of a level that holds every octet (the domain grid,
+ * and any level refined everywhere), one workgroup per 16x16 tile of target
+ * nodes in one z-plane, one lane per target.
+ *
+ * All targets of the tile share Z, so they visit the three source planes in
+ * the same order (ahf_axis_order of Z).  The workgroup stages one source
+ * plane at a time: the entries of its 18x18 source nodes are copied into
+ * LDS, each as m2pd*wz (wz the TSC weight along z for this plane's offset)
+ * and either its separations sy, sx (W9 false) or its three weights along
+ * y and along x (W9 true).  Every lane then walks its (up to) nine sources
+ * of the plane in AHF's order, each source's entries in list order, and
+ * rounds ((m2pd*wz)*wy)*wx into its FP32 node value as AHF does.  Entries of
+ * nodes that do not fit in the CAP staged slots are read from the position
+ * records instead, with the same arithmetic.  Targets with more than kmax
+ * contributions are left to k_gather_heavy_q.  Tiles are dealt so that each
+ * XCD walks a contiguous run of them (consecutive planes share two source
+ * planes in its L2).
+ */
+template <int CAP, bool W9>
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather_slab(level_view v, int64_t per_xcd,
+                                                           const uint32_t *__restrict__ pmask,
+                                                           const int64_t *__restrict__ p0,
+                                                           const ahf_q4 *__restrict__ qp, int descending,
+                                                           double m2pd, uint32_t kmax, float *__restrict__ dens,
+                                                           uint32_t *__restrict__ heavy,
+                                                           uint32_t *__restrict__ n_heavy)
+{
+  constexpr int NV = W9 ? 7 : 3; /* doubles per staged entry */
+  __shared__ uint32_t s_start[3 * AHF_SLAB_N], s_cnt[3 * AHF_SLAB_N], s_pm[3 * AHF_SLAB_N];
+  __shared__ uint32_t s_off[AHF_SLAB_N + 1];
+  __shared__ double s_cx[AHF_SLAB_W], s_cy[AHF_SLAB_W];
+  __shared__ double s_e[NV * CAP];
+  const int64_t L = v.L, H = L >> 1, T = L / AHF_SLAB;
+  const int64_t b = blockIdx.x;
+  const int64_t tile = (b % AHF_XCDS) * per_xcd + b / AHF_XCDS;
+  if (tile >= T * T * L) return; /* the whole workgroup */
+  const int64_t Z = tile / (T * T), ty = (tile / T) % T, tx = tile % T;
+  const int tid = threadIdx.x, lx = tid & (AHF_SLAB - 1), ly = tid / AHF_SLAB;
+  const int64_t X = tx * AHF_SLAB + lx, Y = ty * AHF_SLAB + ly;
+  const double Ld = (double)L;
+  int oz[3], oy[3], ox[3];
+  ahf_axis_order(Z, L, oz);
+  ahf_axis_order(Y, L, oy);
+  ahf_axis_order(X, L, ox);
+  /* bucket start, entry count and presence mask of the 3 x 18 x 18 source
+   * nodes, planes in traversal order; centres of the 18 source columns and
+   * rows */
+  for (int i = tid; i < 3 * AHF_SLAB_N; i += AHF_BLOCK) {
+    const int k = i / AHF_SLAB_N, u = i - k * AHF_SLAB_N;
+    const int64_t Xs = ahf_wrap(tx * AHF_SLAB - 1 + u % AHF_SLAB_W, L);
+    const int64_t Ys = ahf_wrap(ty * AHF_SLAB - 1 + u / AHF_SLAB_W, L);
+    const int64_t sn = full_node(Xs, Ys, ahf_wrap(Z + oz[k], L), H);
+    const int64_t a = p0[sn];
+    s_start[i] = (uint32_t)a;
+    s_cnt[i] = (uint32_t)(p0[sn + 1] - a);
+    s_pm[i] = pmask[sn];
+  }
+  if (tid < AHF_SLAB_W) s_cx[tid] = ahf_centre(ahf_wrap(tx * AHF_SLAB - 1 + tid, L), Ld);
+  else if (tid < 2 * AHF_SLAB_W)
+    s_cy[tid - AHF_SLAB_W] = ahf_centre(ahf_wrap(ty * AHF_SLAB - 1 + (tid - AHF_SLAB_W), L), Ld);
+  __syncthreads();
+  /* contributing sources as bits k*9 + ty*3 + tx of their traversal ranks */
+  uint32_t mask = 0;
+  uint64_t total = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        const int ez = oz[k], ey = oy[bb], ex = ox[e];
+        const int u = k * AHF_SLAB_N + (ly + 1 + ey) * AHF_SLAB_W + (lx + 1 + ex);
+        if (!((s_pm[u] >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u)) continue;
+        total += s_cnt[u];
+        if (s_cnt[u]) mask |= 1u << (k * 9 + bb * 3 + e);
+      }
+  const int64_t n = full_node(X, Y, Z, H);
+  const bool active = total <= kmax;
+  if (!active) heavy[atomicAdd(n_heavy, 1u)] = (uint32_t)n;
+  const uint32_t step = descending ? 0xFFFFFFFFu : 1u;
+  float d = -1.0f;
+  for (int k = 0; k < 3; k++) {
+    const int ez = oz[k];
+    const uint32_t *cnt_k = s_cnt + k * AHF_SLAB_N;
+    const uint32_t *start_k = s_start + k * AHF_SLAB_N;
+    /* LDS slot of each node's first entry: node pairs per thread */
+    uint32_t c0 = 0, c1 = 0;
+    if (tid < AHF_SLAB_N / 2) {
+      c0 = cnt_k[2 * tid];
+      c1 = cnt_k[2 * tid + 1];
+    }
+    uint32_t tot = 0;
+    const uint32_t ex0 = ahf::block_excl_scan<uint32_t>(c0 + c1, &tot);
+    if (tid < AHF_SLAB_N / 2) {
+      s_off[2 * tid] = ex0;
+      s_off[2 * tid + 1] = ex0 + c0;
+    }
+    if (tid == 0) s_off[AHF_SLAB_N] = tot;
+    __syncthreads();
+    const double czs = ahf_centre(ahf_wrap(Z + ez, L), Ld);
+    const TscRt tz(-ez);
+    const uint32_t lim = tot < (uint32_t)CAP ? tot : (uint32_t)CAP;
+    for (uint32_t e = tid; e < lim; e += AHF_BLOCK) {
+      /* the node holding entry e: s_off[lo] <= e < s_off[lo + 1] */
+      int lo = 0, hi = AHF_SLAB_N;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_off[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      if (s_off[lo] + cnt_k[lo] > (uint32_t)CAP) continue; /* read from the records */
+      const ahf_q4 q = qp[start_k[lo] + (e - s_off[lo])];
+      const int row = lo / AHF_SLAB_W, col = lo - row * AHF_SLAB_W;
+      const double sx = ahf_pn_sep(q.x, s_cx[col], Ld);
+      const double sy = ahf_pn_sep(q.y, s_cy[row], Ld);
+      s_e[e] = m2pd * tz(ahf_pn_sep(q.z, czs, Ld));
+      if constexpr (W9) {
+        s_e[1 * CAP + e] = tsc_w_c<-1>(sy);
+        s_e[2 * CAP + e] = tsc_w_c<0>(sy);
+        s_e[3 * CAP + e] = tsc_w_c<1>(sy);
+        s_e[4 * CAP + e] = tsc_w_c<-1>(sx);
+        s_e[5 * CAP + e] = tsc_w_c<0>(sx);
+        s_e[6 * CAP + e] = tsc_w_c<1>(sx);
+      } else {
+        s_e[CAP + e] = sy;
+        s_e[2 * CAP + e] = sx;
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const uint32_t m9 = (mask >> (9 * k)) & 0x1FFu;
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        if (!((m9 >> t) & 1u)) continue;
+        const int ey = oy[t / 3], ex = ox[t % 3];
+        const int u = (ly + 1 + ey) * AHF_SLAB_W + (lx + 1 + ex);
+        const uint32_t cnt = cnt_k[u], o = s_off[u];
+        if (o + cnt <= (uint32_t)CAP) {
+          uint32_t q = descending ? o + cnt - 1u : o;
+          if constexpr (W9) {
+            /* weight index 1 - e for a source at offset e */
+            const double *wy = s_e + (2 - ey) * CAP;
+            const double *wx = s_e + (5 - ex) * CAP;
+            for (uint32_t j = 0; j < cnt; j++, q += step) d = (float)((double)d + (s_e[q] * wy[q]) * wx[q]);
+          } else {
+            const TscRt fy(-ey), fx(-ex);
+            for (uint32_t j = 0; j < cnt; j++, q += step)
+              d = (float)((double)d + (s_e[q] * fy(s_e[CAP + q])) * fx(s_e[2 * CAP + q]));
+          }
+        } else {
+          const TscRt fy(-ey), fx(-ex);
+          const double cxs = s_cx[lx + 1 + ex], cys = s_cy[ly + 1 + ey];
+          uint32_t q = descending ? start_k[u] + cnt - 1u : start_k[u];
+          for (uint32_t j = 0; j < cnt; j++, q += step) {
+            const ahf_q4 r = qp[q];
+            const double az = m2pd * tz(ahf_pn_sep(r.z, czs, Ld));
+            d = (float)((double)d + (az * fy(ahf_pn_sep(r.y, cys, Ld))) * fx(ahf_pn_sep(r.x, cxs, Ld)));
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (active) dens[n] = d;
+}
+
 __device__ inline double readlane_d(double x, int l)
 {
   const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
@@ -1033,7 +1235,10 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   const int64_t nn = 8 * l.n_oct, no = l.n_oct;
   level_view v = make_view(l);
   Arena &A = c->arena;
-  const bool dense = c->gather_mode == AHFGPU_GATHER_SOURCE ||
+  /* a level that holds every octet takes the staged plane walk */
+  const bool slab = c->gather_mode == AHFGPU_GATHER_AUTO && l.L % AHF_SLAB == 0 &&
+                    no == (l.L / 2) * (l.L / 2) * (l.L / 2);
+  const bool dense = slab || c->gather_mode == AHFGPU_GATHER_SOURCE ||
                      ((c->gather_mode == AHFGPU_GATHER_AUTO || c->gather_mode == AHFGPU_GATHER_WAVE) &&
                       n_ent >= AHF_DENSE_ENTRIES_PER_NODE * nn);
   l.dens = A.get<float>(nn);
@@ -1071,7 +1276,16 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   /* long lanes in a small launch leave the chip idle: hand them to the
    * wavefront-per-node kernel sooner */
   const uint32_t kmax = c->gather_mode == AHFGPU_GATHER_WAVE ? 0u : nn >= ((int64_t)1 << 21) ? 384u : 96u;
-  if (dense)
+  if (slab) {
+    const int64_t side = l.L / AHF_SLAB, per = (side * side * l.L + AHF_XCDS - 1) / AHF_XCDS;
+    const dim3 sgrid((unsigned)(AHF_XCDS * per));
+    if (n_ent >= 2 * nn)
+      hipLaunchKernelGGL((k_gather_slab<AHF_SLAB_CAP_BIG, false>), sgrid, dim3(AHF_BLOCK), 0, c->st, v, per, l.pmask,
+                         l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
+    else
+      hipLaunchKernelGGL((k_gather_slab<AHF_SLAB_CAP_SMALL, true>), sgrid, dim3(AHF_BLOCK), 0, c->st, v, per, l.pmask,
+                         l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
+  } else if (dense)
     hipLaunchKernelGGL(k_gather_src, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0, l.qp,
                        descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
   else if ((uint64_t)(n_ent > 0 ? n_ent : 1) * sizeof(ahf_w9) < ((uint64_t)1 << 32))

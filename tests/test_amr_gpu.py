@@ -4,6 +4,7 @@ per-node particle list in AHF's order)."""
 import os
 import sys
 
+import numpy as np
 import pytest
 
 from ahf_amd import dumps
@@ -70,3 +71,20 @@ def test_fuzz_non_pow2(oracle, ctx, tmp_path, seed, mode):
     snap = str(tmp_path / f"fuzz{seed}.snap")
     fuzz_snapshot(snap, seed, non_pow2=True)
     _check(oracle, ctx, snap, tmp_path, f"fuzz_np2_{seed}", mode)
+
+
+@pytest.mark.parametrize("n_bg,clump", [(2000, 2500), (14000, 4000), (40000, 9000)])
+def test_staged_plane_overflow(oracle, ctx, tmp_path, n_bg, clump):
+    """levels that hold every octet are deposited plane by plane from LDS;
+    a tight clump fills a plane's staged slots, so the nodes after it in the
+    plane are walked from the position records instead (both slot counts:
+    under and over two entries per node)"""
+    rng = np.random.default_rng(n_bg + clump)
+    bg = rng.random((n_bg, 3))
+    c = np.array([0.02, 0.5, 0.97])
+    cl = (c + 0.004 * rng.standard_normal((clump, 3))) % 1.0
+    pos = np.concatenate([bg, cl]).astype(np.float32)
+    pos[pos >= 1.0] = 0.0
+    snap = str(tmp_path / f"clump{n_bg}.snap")
+    dumps.write_snapshot(snap, pos, L_dom=16, L_max=128, Nth_dom=2.0, Nth_ref=3.0)
+    _check(oracle, ctx, snap, tmp_path, f"clump{n_bg}_{clump}", 0)
