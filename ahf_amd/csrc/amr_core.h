@@ -342,6 +342,33 @@ AHF_HD uint8_t ahf_node_flags(const level_view *lv, const uint32_t *pmask, const
 }
 
 /*
+ * ahf_presence_mask from values read once per octet: its 27 neighbour
+ * octets nb[] (index ahf_e27), and for each (ey,ez) the x-run link bit of
+ * the neighbour octets at (0,ey,ez) and (-1,ey,ez) (lk0, lkm, index
+ * (ez+1)*3 + (ey+1); read only where those octets exist).  Same rules.
+ */
+AHF_HD uint32_t ahf_presence_mask_oct(const int32_t *nb, const uint8_t *lk0, const uint8_t *lkm, int64_t ox,
+                                      int64_t half, int c)
+{
+  const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+  uint32_t m = 0;
+  for (int dz = -1; dz <= 1; dz++) {
+    const int ez = ahf_carry(cz, dz);
+    if (nb[ahf_e27(0, 0, ez)] < 0) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int ey = ahf_carry(cy, dy);
+      if (nb[ahf_e27(0, ey, ez)] < 0) continue;
+      const int base = (dz + 1) * 9 + (dy + 1) * 3;
+      const int li = (ez + 1) * 3 + (ey + 1);
+      m |= 1u << (base + 1);
+      if (cx == 0 || (nb[ahf_e27(1, ey, ez)] >= 0 && (ox == half - 1 || (lk0[li] & 1)))) m |= 1u << (base + 2);
+      if (cx == 1 || (nb[ahf_e27(-1, ey, ez)] >= 0 && (ox == 0 || (lkm[li] & 1)))) m |= 1u << (base + 0);
+    }
+  }
+  return m;
+}
+
+/*
  * Block test of ref_pquad / ref_cquad (src/libamr_serial/refine_grid.c:
  * 357-400 and 641-686 choose low/up offsets, :403-455 / :689-751 loop over
  * the block but its last member(s), :491-540 / :777-826 CASE 2 tests one

@@ -115,11 +115,46 @@ __global__ void k_csr_lower_bound(const uint32_t *__restrict__ skey, int64_t n, 
   p0[m] = lo;
 }
 
-__global__ void k_pmask(level_view v, uint32_t *__restrict__ pmask)
+/* one thread per octet: its neighbour row and the run links it needs are
+ * read once for the 8 children (ahf_presence_mask_oct); imask, when given,
+ * receives the octet's interior children as bits (all 27 present) */
+__global__ void k_pmask(level_view v, uint32_t *__restrict__ pmask, uint8_t *__restrict__ imask)
 {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= 8 * v.n_oct) return;
-  pmask[n] = ahf_presence_mask(&v, n >> 3, (int)(n & 7));
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= v.n_oct) return;
+  int32_t nb[27];
+#pragma unroll
+  for (int i = 0; i < 27; i++) nb[i] = v.nbr[27 * o + i];
+  uint8_t lk0[9], lkm[9];
+#pragma unroll
+  for (int ez = -1; ez <= 1; ez++)
+#pragma unroll
+    for (int ey = -1; ey <= 1; ey++) {
+      const int li = (ez + 1) * 3 + (ey + 1);
+      const int32_t a = nb[ahf_e27(0, ey, ez)], b = nb[ahf_e27(-1, ey, ez)];
+      lk0[li] = a >= 0 ? v.oct_link[a] : 0;
+      lkm[li] = b >= 0 ? v.oct_link[b] : 0;
+    }
+  const int64_t ox = ahf_kx(v.oct_key[o]), half = v.L / 2;
+  uint4 r0, r1;
+  r0.x = ahf_presence_mask_oct(nb, lk0, lkm, ox, half, 0);
+  r0.y = ahf_presence_mask_oct(nb, lk0, lkm, ox, half, 1);
+  r0.z = ahf_presence_mask_oct(nb, lk0, lkm, ox, half, 2);
+  r0.w = ahf_presence_mask_oct(nb, lk0, lkm, ox, half, 3);
+  r1.x = ahf_presence_mask_oct(nb, lk0, lkm, ox, half, 4);
+  r1.y = ahf_presence_mask_oct(nb, lk0, lkm, ox, half, 5);
+  r1.z = ahf_presence_mask_oct(nb, lk0, lkm, ox, half, 6);
+  r1.w = ahf_presence_mask_oct(nb, lk0, lkm, ox, half, 7);
+  uint4 *out = reinterpret_cast<uint4 *>(pmask + 8 * o);
+  out[0] = r0;
+  out[1] = r1;
+  if (imask) {
+    const uint32_t pm[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    uint32_t im = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) im |= (pm[c] == AHF_FULL27) ? 1u << c : 0u;
+    imask[o] = (uint8_t)im;
+  }
 }
 
 /* weight record of every bucket entry (entries [0, *n)): its particle's
@@ -515,6 +550,7 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_src(level_view v, const ui
 #define AHF_SLAB_N (AHF_SLAB_W * AHF_SLAB_W)
 #define AHF_SLAB_CAP_BIG 2752                /* staged entries per plane, dense levels */
 #define AHF_SLAB_CAP_SMALL 512               /* the same for about one entry per node */
+#define AHF_MARCH_CAP_SMALL 448              /* the marching walk at about one entry per node */
 
 /* node (X,Y,Z) of a level that holds every octet: its octets are all
  * (L/2)^3 keys in (z,y,x) order */
@@ -548,7 +584,8 @@ struct TscRt {
 /*
  * Exact-order density of a level that holds every octet (the domain grid,
  * and any level refined everywhere), one workgroup per 16x16 tile of target
- * nodes in one z-plane, one lane per target.
+ * nodes in one z-plane, one lane per target; with faces set only the planes
+ * Z = 0 and Z = L-1, whose z order wraps (the others march, k_gather_march).
  *
  * All targets of the tile share Z, so they visit the three source planes in
  * the same order (ahf_axis_order of Z).  The workgroup stages one source
@@ -571,7 +608,7 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_slab(level_view v, int64_t
                                                            const ahf_q4 *__restrict__ qp, int descending,
                                                            double m2pd, uint32_t kmax, float *__restrict__ dens,
                                                            uint32_t *__restrict__ heavy,
-                                                           uint32_t *__restrict__ n_heavy)
+                                                           uint32_t *__restrict__ n_heavy, int faces)
 {
   constexpr int NV = W9 ? 7 : 3; /* doubles per staged entry */
   __shared__ uint32_t s_start[3 * AHF_SLAB_N], s_cnt[3 * AHF_SLAB_N], s_pm[3 * AHF_SLAB_N];
@@ -581,8 +618,9 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_slab(level_view v, int64_t
   const int64_t L = v.L, H = L >> 1, T = L / AHF_SLAB;
   const int64_t b = blockIdx.x;
   const int64_t tile = (b % AHF_XCDS) * per_xcd + b / AHF_XCDS;
-  if (tile >= T * T * L) return; /* the whole workgroup */
-  const int64_t Z = tile / (T * T), ty = (tile / T) % T, tx = tile % T;
+  if (tile >= T * T * (faces ? 2 : L)) return; /* the whole workgroup */
+  const int64_t zi = tile / (T * T), ty = (tile / T) % T, tx = tile % T;
+  const int64_t Z = faces ? (zi == 0 ? 0 : L - 1) : zi;
   const int tid = threadIdx.x, lx = tid & (AHF_SLAB - 1), ly = tid / AHF_SLAB;
   const int64_t X = tx * AHF_SLAB + lx, Y = ty * AHF_SLAB + ly;
   const double Ld = (double)L;
@@ -710,6 +748,381 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_slab(level_view v, int64_t
     __syncthreads();
   }
   if (active) dens[n] = d;
+}
+
+/* shared state of one marching workgroup (k_gather_march) */
+struct MarchView {
+  const uint32_t *cnt, *start, *off; /* the walked plane's slot */
+  const double *e;                   /* its staged entries */
+  const double *cx, *cy;             /* source column and row centres */
+  const ahf_q4 *qp;
+  double m2pd, L, czs;
+  uint32_t step;
+  int lx, ly;
+};
+
+/* the contribution of a staged or recorded entry to the three targets above,
+ * at and below the walked plane, in the FP32 values d0 (plane P+1, which sees
+ * the source at z offset -1), d1 (plane P) and d2 (plane P-1), each only
+ * where its use flag is set */
+__device__ __forceinline__ void fold3(float &d0, float &d1, float &d2, bool u0, bool u1, bool u2, double a0, double a1,
+                                      double a2, double wy, double wx)
+{
+  const float n0 = (float)((double)d0 + (a0 * wy) * wx);
+  const float n1 = (float)((double)d1 + (a1 * wy) * wx);
+  const float n2 = (float)((double)d2 + (a2 * wy) * wx);
+  d0 = u0 ? n0 : d0;
+  d1 = u1 ? n1 : d1;
+  d2 = u2 ? n2 : d2;
+}
+
+/*
+ * Source (B,E) (traversal ranks along y and x) of the walked plane for the
+ * lane's three targets.  Inside the box (FACE false) the offsets are B-1,
+ * E-1 and the TSC weights take compile-time forms; at the periodic faces
+ * they follow the lane's rotated order.  The z weights of the three targets
+ * are those of offsets -1, 0, +1 (TSC index d = 1, 0, -1).
+ */
+template <int CAP, bool FULL, bool FACE, int B, int E>
+__device__ __forceinline__ void march_source(const MarchView &m, const int *oy, const int *ox, uint32_t m0, uint32_t m1,
+                                             uint32_t m2, float &d0, float &d1, float &d2)
+{
+  constexpr int BIT = B * 3 + E;
+  const bool u0 = (m0 >> BIT) & 1u, u1 = (m1 >> BIT) & 1u, u2 = (m2 >> BIT) & 1u;
+  if (!(u0 || u1 || u2)) return;
+  const int ey = FACE ? oy[B] : B - 1, ex = FACE ? ox[E] : E - 1;
+  const int u = (m.ly + 1 + ey) * AHF_SLAB_W + (m.lx + 1 + ex);
+  const uint32_t cnt = m.cnt[u], o = m.off[u];
+  const TscRt fy(-ey), fx(-ex);
+  if (o + cnt <= (uint32_t)CAP) {
+    uint32_t q = m.step == 1u ? o : o + cnt - 1u;
+    for (uint32_t j = 0; j < cnt; j++, q += m.step) {
+      double a0, a1, a2, wy, wx;
+      if constexpr (FULL) {
+        a0 = m.e[q];
+        a1 = m.e[CAP + q];
+        a2 = m.e[2 * CAP + q];
+        wy = m.e[(4 - ey) * CAP + q];
+        wx = m.e[(7 - ex) * CAP + q];
+      } else {
+        const double sz = m.e[q], sy = m.e[CAP + q], sx = m.e[2 * CAP + q];
+        a0 = m.m2pd * tsc_w_c<1>(sz);
+        a1 = m.m2pd * tsc_w_c<0>(sz);
+        a2 = m.m2pd * tsc_w_c<-1>(sz);
+        if constexpr (FACE) {
+          wy = fy(sy);
+          wx = fx(sx);
+        } else {
+          wy = tsc_w_c<1 - B>(sy);
+          wx = tsc_w_c<1 - E>(sx);
+        }
+      }
+      fold3(d0, d1, d2, u0, u1, u2, a0, a1, a2, wy, wx);
+    }
+  } else {
+    const double cxs = m.cx[m.lx + 1 + ex], cys = m.cy[m.ly + 1 + ey];
+    uint32_t q = m.step == 1u ? m.start[u] : m.start[u] + cnt - 1u;
+    for (uint32_t j = 0; j < cnt; j++, q += m.step) {
+      const ahf_q4 r = m.qp[q];
+      const double sz = ahf_pn_sep(r.z, m.czs, m.L);
+      fold3(d0, d1, d2, u0, u1, u2, m.m2pd * tsc_w_c<1>(sz), m.m2pd * tsc_w_c<0>(sz), m.m2pd * tsc_w_c<-1>(sz),
+            fy(ahf_pn_sep(r.y, cys, m.L)), fx(ahf_pn_sep(r.x, cxs, m.L)));
+    }
+  }
+}
+
+template <int CAP, bool FULL, bool FACE>
+__device__ __forceinline__ void march_plane(const MarchView &m, const int *oy, const int *ox, uint32_t m0, uint32_t m1,
+                                            uint32_t m2, float &d0, float &d1, float &d2)
+{
+  march_source<CAP, FULL, FACE, 0, 0>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+  march_source<CAP, FULL, FACE, 0, 1>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+  march_source<CAP, FULL, FACE, 0, 2>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+  march_source<CAP, FULL, FACE, 1, 0>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+  march_source<CAP, FULL, FACE, 1, 1>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+  march_source<CAP, FULL, FACE, 1, 2>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+  march_source<CAP, FULL, FACE, 2, 0>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+  march_source<CAP, FULL, FACE, 2, 1>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+  march_source<CAP, FULL, FACE, 2, 2>(m, oy, ox, m0, m1, m2, d0, d1, d2);
+}
+
+/* presence and entry counts of a lane's nine sources in one staged source
+ * plane S, for the three targets that visit it: role 0 is the target S+1
+ * (S is its z offset -1), role 1 the target S, role 2 the target S-1.
+ * Bit r*9 + B*3 + E of m is set when the source at traversal ranks (B,E)
+ * is present for role r (get_TSCnodes) and holds entries; c[r] sums them. */
+struct PlaneSum {
+  uint32_t m, c0, c1, c2;
+};
+
+__device__ __forceinline__ PlaneSum plane_sum(const uint32_t *cnt, const uint32_t *pm, const int *oy, const int *ox,
+                                              int lx, int ly)
+{
+  PlaneSum r = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int bb = 0; bb < 3; bb++)
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+      const int ey = oy[bb], ex = ox[e];
+      const int u = (ly + 1 + ey) * AHF_SLAB_W + (lx + 1 + ex);
+      const uint32_t c = cnt[u];
+      if (!c) continue;
+      const uint32_t w = pm[u] >> ((1 - ey) * 3 + (1 - ex));
+      const int t = bb * 3 + e;
+      /* role r sees the plane at z offset r-1: presence bit (2-r)*9 + ... */
+      if ((w >> 18) & 1u) {
+        r.m |= 1u << t;
+        r.c0 += c;
+      }
+      if ((w >> 9) & 1u) {
+        r.m |= 1u << (9 + t);
+        r.c1 += c;
+      }
+      if (w & 1u) {
+        r.m |= 1u << (18 + t);
+        r.c2 += c;
+      }
+    }
+  return r;
+}
+
+/*
+ * Exact-order density of the planes 1 <= Z <= L-2 of a level that holds
+ * every octet, whose sources lie in the planes Z-1, Z, Z+1 in that order.
+ * One workgroup per 16x16 column tile and run of zn target planes, one lane
+ * per (x,y) column.  The workgroup marches up through the source planes,
+ * staging each one in LDS once (as in k_gather_slab), and every lane folds
+ * the staged plane P into its three open targets: plane P+1 (its first
+ * source plane), P (its second) and P-1 (its third, after which the value
+ * is final).  All three visit the plane's sources in the same order, so one
+ * walk serves them and the TSC weights along y and x are shared.  Bucket
+ * records of the planes P, P+1, P+2 are kept in a ring of three slots, and
+ * each lane summarises a plane's sources once when it arrives, so that
+ * target P+1's contribution count (for the kmax test) is known when it
+ * opens.  FULL stages the nine weights of each entry and walks the plane's
+ * sources as one flat sequence from a per-lane queue in LDS (few entries
+ * per source: the wavefront runs as long as its longest lane over the
+ * plane, not over each source); otherwise the separations are staged and
+ * the sources are walked one by one with compile-time weights.
+ */
+template <int CAP, bool FULL>
+__global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_t per_xcd, int64_t zn,
+                                                            const uint32_t *__restrict__ pmask,
+                                                            const int64_t *__restrict__ p0,
+                                                            const ahf_q4 *__restrict__ qp, int descending,
+                                                            double m2pd, uint32_t kmax, float *__restrict__ dens,
+                                                            uint32_t *__restrict__ heavy,
+                                                            uint32_t *__restrict__ n_heavy)
+{
+  constexpr int NV = FULL ? 9 : 3; /* doubles per staged entry */
+  __shared__ uint32_t s_start[3 * AHF_SLAB_N], s_cnt[3 * AHF_SLAB_N], s_pm[AHF_SLAB_N];
+  __shared__ uint32_t s_off[AHF_SLAB_N + 1];
+  __shared__ double s_cx[AHF_SLAB_W], s_cy[AHF_SLAB_W];
+  __shared__ double s_e[NV * CAP];
+  __shared__ uint32_t s_q[FULL ? 9 * AHF_BLOCK : 1];
+  const int64_t L = v.L, H = L >> 1, T = L / AHF_SLAB;
+  const int64_t nch = (L - 2 + zn - 1) / zn;
+  const int64_t b = blockIdx.x;
+  const int64_t w = (b % AHF_XCDS) * per_xcd + b / AHF_XCDS;
+  if (w >= T * T * nch) return; /* the whole workgroup */
+  const int64_t ch = w / (T * T), ty = (w / T) % T, tx = w % T;
+  const int64_t Z0 = 1 + ch * zn, Z1 = Z0 + zn < L - 1 ? Z0 + zn : L - 1;
+  const int tid = threadIdx.x, lx = tid & (AHF_SLAB - 1), ly = tid / AHF_SLAB;
+  const int64_t X = tx * AHF_SLAB + lx, Y = ty * AHF_SLAB + ly;
+  const double Ld = (double)L;
+  const bool face = tx == 0 || tx == T - 1 || ty == 0 || ty == T - 1;
+  int oy[3], ox[3];
+  ahf_axis_order(Y, L, oy);
+  ahf_axis_order(X, L, ox);
+  if (tid < AHF_SLAB_W) s_cx[tid] = ahf_centre(ahf_wrap(tx * AHF_SLAB - 1 + tid, L), Ld);
+  else if (tid < 2 * AHF_SLAB_W)
+    s_cy[tid - AHF_SLAB_W] = ahf_centre(ahf_wrap(ty * AHF_SLAB - 1 + (tid - AHF_SLAB_W), L), Ld);
+  /* bucket records of source plane Zs into ring slot Zs mod 3 (presence
+   * masks into the one slot that plane_sum reads) */
+  auto load_plane = [&](int64_t Zs) {
+    const int slot = (int)(Zs % 3);
+    const int64_t Zw = ahf_wrap(Zs, L);
+    for (int i = tid; i < AHF_SLAB_N; i += AHF_BLOCK) {
+      const int64_t Xs = ahf_wrap(tx * AHF_SLAB - 1 + i % AHF_SLAB_W, L);
+      const int64_t Ys = ahf_wrap(ty * AHF_SLAB - 1 + i / AHF_SLAB_W, L);
+      const int64_t sn = full_node(Xs, Ys, Zw, H);
+      const int64_t a = p0[sn];
+      s_start[slot * AHF_SLAB_N + i] = (uint32_t)a;
+      s_cnt[slot * AHF_SLAB_N + i] = (uint32_t)(p0[sn + 1] - a);
+      s_pm[i] = pmask[sn]; /* summarised before the next load */
+    }
+  };
+  auto sum_of = [&](int64_t Zs) {
+    const int slot = (int)(Zs % 3);
+    return plane_sum(s_cnt + slot * AHF_SLAB_N, s_pm, oy, ox, lx, ly);
+  };
+  load_plane(Z0 - 1);
+  __syncthreads();
+  PlaneSum sP = sum_of(Z0 - 1);
+  __syncthreads();
+  load_plane(Z0);
+  __syncthreads();
+  PlaneSum sP1 = sum_of(Z0);
+  __syncthreads();
+  MarchView m;
+  m.off = s_off;
+  m.e = s_e;
+  m.cx = s_cx;
+  m.cy = s_cy;
+  m.qp = qp;
+  m.m2pd = m2pd;
+  m.L = Ld;
+  m.step = descending ? 0xFFFFFFFFu : 1u;
+  m.lx = lx;
+  m.ly = ly;
+  /* the open targets: P+1 (n), P (c), P-1 (p); plane summaries of P, P+1 */
+  float dn = -1.0f, dc = -1.0f, dp = -1.0f;
+  uint32_t mn = 0, mc = 0, mp = 0;
+  bool an = false, ac = false, ap = false;
+  for (int64_t P = Z0 - 1; P <= Z1; P++) {
+    load_plane(P + 2);
+    __syncthreads();
+    const PlaneSum sP2 = sum_of(P + 2);
+    const int sp = (int)(P % 3);
+    const uint32_t *cnt_p = s_cnt + sp * AHF_SLAB_N, *start_p = s_start + sp * AHF_SLAB_N;
+    uint32_t c0 = 0, c1 = 0;
+    if (tid < AHF_SLAB_N / 2) {
+      c0 = cnt_p[2 * tid];
+      c1 = cnt_p[2 * tid + 1];
+    }
+    uint32_t tot = 0;
+    const uint32_t ex0 = ahf::block_excl_scan<uint32_t>(c0 + c1, &tot);
+    if (tid < AHF_SLAB_N / 2) {
+      s_off[2 * tid] = ex0;
+      s_off[2 * tid + 1] = ex0 + c0;
+    }
+    if (tid == 0) s_off[AHF_SLAB_N] = tot;
+    __syncthreads();
+    const double czs = ahf_centre(ahf_wrap(P, L), Ld);
+    if constexpr (FULL) {
+      /* one thread per source node: few entries each */
+      for (int u = tid; u < AHF_SLAB_N; u += AHF_BLOCK) {
+        const uint32_t c = cnt_p[u], o = s_off[u];
+        if (o + c > (uint32_t)CAP) continue; /* read from the records */
+        const int row = u / AHF_SLAB_W, col = u - row * AHF_SLAB_W;
+        const double cxs = s_cx[col], cys = s_cy[row];
+        for (uint32_t j = 0; j < c; j++) {
+          const ahf_q4 q = qp[start_p[u] + j];
+          const double sz = ahf_pn_sep(q.z, czs, Ld);
+          const double sy = ahf_pn_sep(q.y, cys, Ld);
+          const double sx = ahf_pn_sep(q.x, cxs, Ld);
+          const uint32_t e = o + j;
+          s_e[e] = m2pd * tsc_w_c<1>(sz);
+          s_e[CAP + e] = m2pd * tsc_w_c<0>(sz);
+          s_e[2 * CAP + e] = m2pd * tsc_w_c<-1>(sz);
+          s_e[3 * CAP + e] = tsc_w_c<-1>(sy);
+          s_e[4 * CAP + e] = tsc_w_c<0>(sy);
+          s_e[5 * CAP + e] = tsc_w_c<1>(sy);
+          s_e[6 * CAP + e] = tsc_w_c<-1>(sx);
+          s_e[7 * CAP + e] = tsc_w_c<0>(sx);
+          s_e[8 * CAP + e] = tsc_w_c<1>(sx);
+        }
+      }
+    } else {
+      const uint32_t lim = tot < (uint32_t)CAP ? tot : (uint32_t)CAP;
+      for (uint32_t e = tid; e < lim; e += AHF_BLOCK) {
+        /* the node holding entry e: s_off[lo] <= e < s_off[lo + 1] */
+        int lo = 0, hi = AHF_SLAB_N;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_off[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        if (s_off[lo] + cnt_p[lo] > (uint32_t)CAP) continue; /* read from the records */
+        const ahf_q4 q = qp[start_p[lo] + (e - s_off[lo])];
+        const int row = lo / AHF_SLAB_W, col = lo - row * AHF_SLAB_W;
+        s_e[e] = ahf_pn_sep(q.z, czs, Ld);
+        s_e[CAP + e] = ahf_pn_sep(q.y, s_cy[row], Ld);
+        s_e[2 * CAP + e] = ahf_pn_sep(q.x, s_cx[col], Ld);
+      }
+    }
+    /* target P+1 opens: planes P (its z offset -1), P+1 (0), P+2 (+1) */
+    mn = 0;
+    an = false;
+    dn = -1.0f;
+    if (P + 1 >= Z0 && P + 1 < Z1) {
+      const uint64_t total = (uint64_t)sP.c0 + sP1.c1 + sP2.c2;
+      an = total <= kmax;
+      if (an) mn = (sP.m & 0x1FFu) | (sP1.m & (0x1FFu << 9)) | (sP2.m & (0x1FFu << 18));
+      else heavy[atomicAdd(n_heavy, 1u)] = (uint32_t)full_node(X, Y, P + 1, H);
+    }
+    __syncthreads();
+    m.cnt = cnt_p;
+    m.start = start_p;
+    m.czs = czs;
+    /* plane P is the first source plane of P+1, the second of P, the third of P-1 */
+    const uint32_t m0 = mn & 0x1FFu, m1 = ac ? (mc >> 9) & 0x1FFu : 0u, m2 = ap ? (mp >> 18) & 0x1FFu : 0u;
+    if constexpr (FULL) {
+      /* the lane's sources of this plane in order, as one flat sequence */
+      uint32_t *qq = s_q + tid * 9;
+      uint32_t nq = 0, ntot = 0;
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const uint32_t roles = ((m0 >> t) & 1u) | ((m1 >> t) & 1u) << 1 | ((m2 >> t) & 1u) << 2;
+        if (!roles) continue;
+        const int ey = oy[t / 3], ex = ox[t % 3];
+        const uint32_t u = (uint32_t)((ly + 1 + ey) * AHF_SLAB_W + (lx + 1 + ex));
+        const uint32_t c = cnt_p[u], o = s_off[u];
+        const uint32_t staged = o + c <= (uint32_t)CAP;
+        /* LDS slot or node (10 bits), count (9), roles (3), weight indices
+         * 1-ey and 1-ex (2 + 2), staged (1) */
+        qq[nq++] = (staged ? o : u) | c << 10 | roles << 19 | (uint32_t)(1 - ey) << 22 | (uint32_t)(1 - ex) << 24 |
+                   staged << 26;
+        ntot += c;
+      }
+      uint32_t k = 0, rem = 0, q = 0;
+      int iy = 0, ix = 0;
+      bool r0 = false, r1 = false, r2 = false, stg = true;
+      double cys = 0.0, cxs = 0.0;
+      for (uint32_t it = 0; it < ntot; it++) {
+        if (rem == 0) {
+          const uint32_t wq = qq[k++];
+          rem = (wq >> 10) & 0x1FFu;
+          r0 = (wq >> 19) & 1u;
+          r1 = (wq >> 20) & 1u;
+          r2 = (wq >> 21) & 1u;
+          iy = (int)((wq >> 22) & 3u);
+          ix = (int)((wq >> 24) & 3u);
+          stg = (wq >> 26) & 1u;
+          const uint32_t first = stg ? (wq & 0x3FFu) : start_p[wq & 0x3FFu];
+          q = m.step == 1u ? first : first + rem - 1u;
+          if (!stg) {
+            cys = s_cy[ly + 2 - iy];
+            cxs = s_cx[lx + 2 - ix];
+          }
+        }
+        if (stg) {
+          fold3(dn, dc, dp, r0, r1, r2, s_e[q], s_e[CAP + q], s_e[2 * CAP + q], s_e[(3 + iy) * CAP + q],
+                s_e[(6 + ix) * CAP + q]);
+        } else {
+          const ahf_q4 r = qp[q];
+          const double sz = ahf_pn_sep(r.z, czs, Ld);
+          fold3(dn, dc, dp, r0, r1, r2, m2pd * tsc_w_c<1>(sz), m2pd * tsc_w_c<0>(sz), m2pd * tsc_w_c<-1>(sz),
+                TscRt(iy - 1)(ahf_pn_sep(r.y, cys, Ld)), TscRt(ix - 1)(ahf_pn_sep(r.x, cxs, Ld)));
+        }
+        q += m.step;
+        rem--;
+      }
+    } else {
+      if (face) march_plane<CAP, FULL, true>(m, oy, ox, m0, m1, m2, dn, dc, dp);
+      else march_plane<CAP, FULL, false>(m, oy, ox, m0, m1, m2, dn, dc, dp);
+    }
+    if (ap) dens[full_node(X, Y, P - 1, H)] = dp;
+    dp = dc;
+    dc = dn;
+    mp = mc;
+    mc = mn;
+    ap = ac;
+    ac = an;
+    sP = sP1;
+    sP1 = sP2;
+    __syncthreads();
+  }
 }
 
 __device__ inline double readlane_d(double x, int l)
@@ -919,12 +1332,73 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy_q(level_view v, cons
   }
 }
 
+/*
+ * ahf_node_flags for the 8 children of one octet per thread: a child with
+ * all 27 neighbours present is interior, and tested when one of its 18
+ * neighbours at x offsets 0, +1 reaches the density threshold.  The
+ * children's neighbours lie in the 3x4x4 nodes from (X0, Y0-1, Z0) (X0,
+ * Y0, Z0 the octet's first node), each density read once.  Missing
+ * neighbour octets only occur for children that are not interior.
+ */
 __global__ void k_flags(level_view v, const uint32_t *__restrict__ pmask, const float *__restrict__ dens,
                         double thr, uint8_t *__restrict__ flags)
 {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= 8 * v.n_oct) return;
-  flags[n] = ahf_node_flags(&v, pmask, dens, thr, n >> 3, (int)(n & 7));
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= v.n_oct) return;
+  const uint4 *pm4 = reinterpret_cast<const uint4 *>(pmask + 8 * o);
+  const uint4 pa = pm4[0], pb = pm4[1];
+  const uint32_t pm[8] = {pa.x, pa.y, pa.z, pa.w, pb.x, pb.y, pb.z, pb.w};
+  uint32_t any = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) any |= (pm[c] == AHF_FULL27) ? 1u << c : 0u;
+  uint64_t hit = 0; /* bit xi + 3*(yi + 4*zi) of the 3x4x4 neighbourhood */
+  if (any) {
+    int32_t nb[27];
+#pragma unroll
+    for (int i = 0; i < 27; i++) nb[i] = v.nbr[27 * o + i];
+    /* every load issued at once: a missing octet reads this octet's node
+     * and its bit is dropped */
+    float dv[48];
+#pragma unroll
+    for (int zi = 0; zi < 4; zi++)
+#pragma unroll
+      for (int yi = 0; yi < 4; yi++)
+#pragma unroll
+        for (int xi = 0; xi < 3; xi++) {
+          const int zr = zi - 1, yr = yi - 1, xr = xi;
+          const int oz = zr < 0 ? -1 : (zr > 1 ? 1 : 0), oy = yr < 0 ? -1 : (yr > 1 ? 1 : 0), oxo = xr > 1 ? 1 : 0;
+          const int32_t on = nb[ahf_e27(oxo, oy, oz)];
+          const int64_t m = 8 * (int64_t)(on >= 0 ? on : (int32_t)o) + ((zr & 1) << 2 | (yr & 1) << 1 | (xr & 1));
+          dv[xi + 3 * (yi + 4 * zi)] = dens[m];
+        }
+#pragma unroll
+    for (int zi = 0; zi < 4; zi++)
+#pragma unroll
+      for (int yi = 0; yi < 4; yi++)
+#pragma unroll
+        for (int xi = 0; xi < 3; xi++) {
+          const int zr = zi - 1, yr = yi - 1, xr = xi;
+          const int oz = zr < 0 ? -1 : (zr > 1 ? 1 : 0), oy = yr < 0 ? -1 : (yr > 1 ? 1 : 0), oxo = xr > 1 ? 1 : 0;
+          const int bit = xi + 3 * (yi + 4 * zi);
+          if (nb[ahf_e27(oxo, oy, oz)] >= 0 && (double)dv[bit] >= thr) hit |= 1ull << bit;
+        }
+  }
+  uint8_t f[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+    /* xi in {cx, cx+1}, yi in {cy..cy+2}, zi in {cz..cz+2} */
+    uint64_t box = 0;
+#pragma unroll
+    for (int zi = 0; zi < 3; zi++)
+#pragma unroll
+      for (int yi = 0; yi < 3; yi++) box |= 3ull << (cx + 3 * (cy + yi + 4 * (cz + zi)));
+    f[c] = !((any >> c) & 1u) ? (uint8_t)0 : (uint8_t)(NF_INTERIOR | ((hit & box) ? NF_TEST : 0u));
+  }
+  uint2 w;
+  w.x = (uint32_t)f[0] | (uint32_t)f[1] << 8 | (uint32_t)f[2] << 16 | (uint32_t)f[3] << 24;
+  w.y = (uint32_t)f[4] | (uint32_t)f[5] << 8 | (uint32_t)f[6] << 16 | (uint32_t)f[7] << 24;
+  *reinterpret_cast<uint2 *>(flags + 8 * o) = w;
 }
 
 /* one thread per level row (octet row r, cy, cz) */
@@ -1036,27 +1510,41 @@ __global__ void k_nbr_next(const int32_t *__restrict__ nbr, const int64_t *__res
  * child its particle moves to (first interior child, in fine traversal
  * order, whose closed cell holds it); the key for the stable sort that
  * builds the fine buckets is the fine node, or all-ones if it stays */
+/* imask2: interior children of each fine octet (k_pmask); qp: the
+ * entries' position records when the level has them, else the particle
+ * arrays are read */
 __global__ void k_relink_child(int64_t n_ent, const uint32_t *__restrict__ pnode,
                                const uint32_t *__restrict__ plist, const int32_t *__restrict__ node2oct,
-                               const uint64_t *__restrict__ key, const uint32_t *__restrict__ pmask2,
-                               const float *__restrict__ px, const float *__restrict__ py,
-                               const float *__restrict__ pz, double Lf, int8_t *__restrict__ child)
+                               const uint64_t *__restrict__ key, const uint8_t *__restrict__ imask2,
+                               const ahf_q4 *__restrict__ qp, const float *__restrict__ px,
+                               const float *__restrict__ py, const float *__restrict__ pz, double Lf,
+                               int8_t *__restrict__ child)
 {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_ent) return;
-  const uint32_t m = pnode[q], p = plist[q];
+  const uint32_t m = pnode[q];
   const int32_t o2 = node2oct[m];
   int ch = -1;
   if (o2 >= 0) {
-    uint8_t imask = 0;
-    for (int k = 0; k < 8; k++)
-      if (pmask2[8 * (int64_t)o2 + k] == AHF_FULL27) imask |= (uint8_t)(1u << k);
+    const uint8_t imask = imask2[o2];
     if (imask) {
       const uint64_t mk = key[m >> 3];
       const int cm = (int)(m & 7);
       const int64_t MX = 2 * ahf_kx(mk) + (cm & 1), MY = 2 * ahf_ky(mk) + ((cm >> 1) & 1),
                     MZ = 2 * ahf_kz(mk) + (cm >> 2);
-      ch = ahf_relink_child(px[p], py[p], pz[p], MX, MY, MZ, Lf, imask);
+      float x, y, z;
+      if (qp) {
+        const ahf_q4 r = qp[q];
+        x = r.x;
+        y = r.y;
+        z = r.z;
+      } else {
+        const uint32_t p = plist[q];
+        x = px[p];
+        y = py[p];
+        z = pz[p];
+      }
+      ch = ahf_relink_child(x, y, z, MX, MY, MZ, Lf, imask);
     }
   }
   child[q] = (int8_t)ch;
@@ -1067,49 +1555,68 @@ __global__ void k_relink_child(int64_t n_ent, const uint32_t *__restrict__ pnode
  * node, so fine node t = 8*o2 + c holds the entries of its mother's bucket
  * whose child is c, in bucket order (AHF's head insertion during relink
  * leaves them in this order relative to the walk).  One thread per fine
- * node walks the mother's bucket: first to count, then, after a scan, to
- * write; the 8 children of an octet read the same bucket.  Buckets of more
+ * octet walks the mother's bucket once: first to count its 8 children, then,
+ * after a scan, to write them.  Buckets of more
  * than AHF_BIG_BUCKET entries are listed for k_fine_big, which walks them
  * one wavefront at a time.
  */
-#define AHF_BIG_BUCKET 256
+#define AHF_BIG_BUCKET 32
 
-__global__ void k_fine_count(const int64_t *__restrict__ oct_parent, int64_t nn2, const int64_t *__restrict__ p0c,
+__global__ void k_fine_count(const int64_t *__restrict__ oct_parent, int64_t n2, const int64_t *__restrict__ p0c,
                              const int8_t *__restrict__ child, int64_t *__restrict__ cnt,
                              uint32_t *__restrict__ big, uint32_t *__restrict__ n_big)
 {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nn2) return;
-  const int64_t m = oct_parent[t >> 3];
-  const int8_t c = (int8_t)(t & 7);
+  const int64_t t8 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t8 >= n2) return;
+  const int64_t m = oct_parent[t8];
   const int64_t a = p0c[m], b = p0c[m + 1];
   if (b - a > AHF_BIG_BUCKET) {
-    if (c == 0) big[atomicAdd(n_big, 1u)] = (uint32_t)(t >> 3);
+    big[atomicAdd(n_big, 1u)] = (uint32_t)t8;
     return;
   }
-  int64_t k = 0;
-  for (int64_t q = a; q < b; q++) k += child[q] == c;
-  cnt[t] = k;
+  /* the 8 children's counts as 16-bit fields (at most AHF_BIG_BUCKET) */
+  uint64_t lo = 0, hi = 0;
+  for (int64_t q = a; q < b; q++) {
+    const int c = child[q];
+    if (c < 0) continue;
+    if (c < 4) lo += 1ull << (16 * c);
+    else hi += 1ull << (16 * (c - 4));
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    cnt[8 * t8 + c] = (int64_t)((lo >> (16 * c)) & 0xFFFFu);
+    cnt[8 * t8 + 4 + c] = (int64_t)((hi >> (16 * c)) & 0xFFFFu);
+  }
 }
 
-__global__ void k_fine_fill(const int64_t *__restrict__ oct_parent, int64_t nn2, const int64_t *__restrict__ p0c,
+__global__ void k_fine_fill(const int64_t *__restrict__ oct_parent, int64_t n2, const int64_t *__restrict__ p0c,
                             const int8_t *__restrict__ child, const uint32_t *__restrict__ plistc,
                             const int64_t *__restrict__ p0f, uint32_t *__restrict__ plistf,
                             uint32_t *__restrict__ pnodef)
 {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nn2) return;
-  const int64_t m = oct_parent[t >> 3];
-  const int8_t c = (int8_t)(t & 7);
+  const int64_t t8 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t8 >= n2) return;
+  const int64_t m = oct_parent[t8];
   const int64_t a = p0c[m], b = p0c[m + 1];
   if (b - a > AHF_BIG_BUCKET) return;
-  int64_t w = p0f[t];
-  for (int64_t q = a; q < b; q++)
-    if (child[q] == c) {
-      plistf[w] = plistc[q];
-      pnodef[w] = (uint32_t)t;
-      w++;
-    }
+  uint32_t w0 = (uint32_t)p0f[8 * t8 + 0], w1 = (uint32_t)p0f[8 * t8 + 1], w2 = (uint32_t)p0f[8 * t8 + 2],
+           w3 = (uint32_t)p0f[8 * t8 + 3], w4 = (uint32_t)p0f[8 * t8 + 4], w5 = (uint32_t)p0f[8 * t8 + 5],
+           w6 = (uint32_t)p0f[8 * t8 + 6], w7 = (uint32_t)p0f[8 * t8 + 7];
+  for (int64_t q = a; q < b; q++) {
+    const int c = child[q];
+    if (c < 0) continue;
+    const uint32_t w = c == 0 ? w0 : c == 1 ? w1 : c == 2 ? w2 : c == 3 ? w3 : c == 4 ? w4 : c == 5 ? w5 : c == 6 ? w6 : w7;
+    plistf[w] = plistc[q];
+    pnodef[w] = (uint32_t)(8 * t8 + c);
+    w0 += c == 0;
+    w1 += c == 1;
+    w2 += c == 2;
+    w3 += c == 3;
+    w4 += c == 4;
+    w5 += c == 5;
+    w6 += c == 6;
+    w7 += c == 7;
+  }
 }
 
 /* the listed big buckets, one wavefront per bucket: 64 entries per step,
@@ -1242,6 +1749,8 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
                      ((c->gather_mode == AHFGPU_GATHER_AUTO || c->gather_mode == AHFGPU_GATHER_WAVE) &&
                       n_ent >= AHF_DENSE_ENTRIES_PER_NODE * nn);
   l.dens = A.get<float>(nn);
+  l.qp = nullptr;
+  l.W = nullptr;
   if (dense)
     l.qp = A.get<ahf_q4>(n_ent > 0 ? n_ent : 1);
   else
@@ -1277,14 +1786,23 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
    * wavefront-per-node kernel sooner */
   const uint32_t kmax = c->gather_mode == AHFGPU_GATHER_WAVE ? 0u : nn >= ((int64_t)1 << 21) ? 384u : 96u;
   if (slab) {
-    const int64_t side = l.L / AHF_SLAB, per = (side * side * l.L + AHF_XCDS - 1) / AHF_XCDS;
-    const dim3 sgrid((unsigned)(AHF_XCDS * per));
-    if (n_ent >= 2 * nn)
-      hipLaunchKernelGGL((k_gather_slab<AHF_SLAB_CAP_BIG, false>), sgrid, dim3(AHF_BLOCK), 0, c->st, v, per, l.pmask,
-                         l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
-    else
-      hipLaunchKernelGGL((k_gather_slab<AHF_SLAB_CAP_SMALL, true>), sgrid, dim3(AHF_BLOCK), 0, c->st, v, per, l.pmask,
-                         l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
+    /* the planes 1..L-2 march, the two face planes take the per-plane walk */
+    const int64_t side = l.L / AHF_SLAB;
+    const int64_t zn = l.L <= 128 ? 8 : 16, nch = (l.L - 2 + zn - 1) / zn;
+    const int64_t per_m = (side * side * nch + AHF_XCDS - 1) / AHF_XCDS;
+    const int64_t per_f = (side * side * 2 + AHF_XCDS - 1) / AHF_XCDS;
+    const dim3 mgrid((unsigned)(AHF_XCDS * per_m)), fgrid((unsigned)(AHF_XCDS * per_f));
+    if (n_ent >= 2 * nn) {
+      hipLaunchKernelGGL((k_gather_march<AHF_SLAB_CAP_BIG, false>), mgrid, dim3(AHF_BLOCK), 0, c->st, v, per_m, zn,
+                         l.pmask, l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
+      hipLaunchKernelGGL((k_gather_slab<AHF_SLAB_CAP_BIG, false>), fgrid, dim3(AHF_BLOCK), 0, c->st, v, per_f, l.pmask,
+                         l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy, 1);
+    } else {
+      hipLaunchKernelGGL((k_gather_march<AHF_MARCH_CAP_SMALL, true>), mgrid, dim3(AHF_BLOCK), 0, c->st, v, per_m, zn,
+                         l.pmask, l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
+      hipLaunchKernelGGL((k_gather_slab<AHF_SLAB_CAP_SMALL, true>), fgrid, dim3(AHF_BLOCK), 0, c->st, v, per_f, l.pmask,
+                         l.node_p0, l.qp, descending, l.m2pd, kmax, l.dens, heavy, n_heavy, 1);
+    }
   } else if (dense)
     hipLaunchKernelGGL(k_gather_src, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0, l.qp,
                        descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
@@ -1316,7 +1834,7 @@ static int deposit_level(ahfgpu_ctx *c, DevLevel &l, int level_index, int64_t n_
     l.pmask = c->arena.get<uint32_t>(nn);
     if (!l.pmask) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (pmask, level %d)", level_index);
     level_view v = make_view(l);
-    LAUNCH(k_pmask, nn, v, l.pmask);
+    LAUNCH(k_pmask, l.n_oct, v, l.pmask, (uint8_t *)nullptr);
   }
   if (gather_level(c, l, n_ent, (level_index % 2) == 0, true))
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit, level %d)", level_index);
@@ -1389,7 +1907,7 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   if (!l.flags || !cnt || !oct_off || !rhas || !row_off || !phas || !plane_off || !stmp64 || !stmp32)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (flags, level %d)", ci);
   hipEvent_t e0 = c->timer_begin();
-  LAUNCH(k_flags, nn, v, l.pmask, l.dens, l.crit - 1.0, l.flags);
+  LAUNCH(k_flags, l.n_oct, v, l.pmask, l.dens, l.crit - 1.0, l.flags);
   hipMemsetAsync(c->d_anomaly, 0, sizeof(int32_t), c->st);
   LAUNCH(k_refine_rows, nrank, v, l.flags, cnt, c->d_anomaly);
   exclusive_scan<int64_t, int64_t>(cnt, oct_off, nrank, stmp64, c->st);
@@ -1459,15 +1977,17 @@ static int refine_level(ahfgpu_ctx *c, int ci)
                      f.n_plane, (int32_t)f.n_row);
   LAUNCH(k_nbr_next, 27 * n2, l.nbr, f.oct_parent, l.node2oct, n2, f.nbr);
   level_view fv = make_view(f);
-  LAUNCH(k_pmask, nn2, fv, f.pmask);
+  uint8_t *imask2 = A.get<uint8_t>(n2);
+  if (!imask2) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", ci + 1);
+  LAUNCH(k_pmask, fv.n_oct, fv, f.pmask, imask2);
   c->timer_end(e1, TIMER_BUILD, 150.0 * (double)n2);
   /* relink: child of every entry, then the fine buckets */
   hipEvent_t e2 = c->timer_begin();
-  LAUNCH(k_relink_child, nent, nent, l.pnode, l.plist, l.node2oct, l.oct_key, f.pmask, c->px, c->py, c->pz,
+  LAUNCH(k_relink_child, nent, nent, l.pnode, l.plist, l.node2oct, l.oct_key, imask2, l.qp, c->px, c->py, c->pz,
          (double)f.L, l.child);
   uint32_t *n_big = (uint32_t *)(c->d_anomaly + 2);
   hipMemsetAsync(n_big, 0, sizeof(uint32_t), c->st);
-  LAUNCH(k_fine_count, nn2, f.oct_parent, nn2, l.node_p0, l.child, fcnt, fbig, n_big);
+  LAUNCH(k_fine_count, n2, f.oct_parent, n2, l.node_p0, l.child, fcnt, fbig, n_big);
   hipLaunchKernelGGL(k_fine_big<false>, dim3(512), dim3(AHF_BLOCK), 0, c->st, f.oct_parent, l.node_p0, l.child,
                      l.plist, fbig, n_big, fcnt, nullptr, nullptr, nullptr);
   exclusive_scan<int64_t, int64_t>(fcnt, f.node_p0, nn2, ftmp, c->st);
@@ -1475,7 +1995,7 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   f.plist = A.get<uint32_t>(nent);
   f.pnode = A.get<uint32_t>(nent);
   if (!f.plist || !f.pnode) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", ci + 1);
-  LAUNCH(k_fine_fill, nn2, f.oct_parent, nn2, l.node_p0, l.child, l.plist, f.node_p0, f.plist, f.pnode);
+  LAUNCH(k_fine_fill, n2, f.oct_parent, n2, l.node_p0, l.child, l.plist, f.node_p0, f.plist, f.pnode);
   hipLaunchKernelGGL(k_fine_big<true>, dim3(512), dim3(AHF_BLOCK), 0, c->st, f.oct_parent, l.node_p0, l.child,
                      l.plist, fbig, n_big, nullptr, f.node_p0, f.plist, f.pnode);
   c->timer_end(e2, TIMER_RELINK, 29.0 * (double)nent + 24.0 * (double)nn2);
@@ -1770,7 +2290,7 @@ extern "C" int ahfgpu_deposit_level(ahfgpu_ctx *c, int64_t L, const uint64_t *oc
   level_view v = make_view(l);
   l.pmask = A.get<uint32_t>(nn);
   if (!l.pmask) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit)");
-  LAUNCH(k_pmask, nn, v, l.pmask);
+  LAUNCH(k_pmask, l.n_oct, v, l.pmask, (uint8_t *)nullptr);
   if (gather_level(c, l, ne, 0, false)) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (deposit)");
   if (dens_out) hipMemcpyAsync(dens_out, l.dens, (size_t)nn * 4, hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
