@@ -1486,6 +1486,18 @@ __global__ void k_plane_fill(level_view v, const int32_t *__restrict__ has, cons
   prow2[np] = row_off[4 * q0 + cz * 2 * nr];
 }
 
+/* the refine step's sizes gathered for one read-back */
+__global__ void k_sizes(const int64_t *__restrict__ n_oct, const int64_t *__restrict__ n_part,
+                        const int32_t *__restrict__ n_row, const int32_t *__restrict__ n_plane,
+                        const int32_t *__restrict__ anomaly, int64_t *__restrict__ out)
+{
+  out[0] = *n_oct;
+  out[1] = *n_part;
+  out[2] = *n_row;
+  out[3] = *n_plane;
+  out[4] = *anomaly;
+}
+
 __global__ void k_set2(int32_t *__restrict__ a, int64_t ia, int32_t va, int32_t *__restrict__ b, int64_t ib,
                        int32_t vb)
 {
@@ -1916,18 +1928,17 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   LAUNCH(k_plane_has, nplr, v, cnt, phas);
   exclusive_scan<int32_t, int32_t>(phas, plane_off, nplr, stmp32, c->st);
   c->timer_end(e0, TIMER_REFINE, 4.0 * (double)nn);
-  /* sizes of the new level, and the particle count of this level */
+  /* sizes of the new level, and the particle count of this level, in one
+   * read-back */
   struct {
-    int64_t n_oct, n_part;
-    int32_t n_row, n_plane, anomaly, pad;
+    int64_t n_oct, n_part, n_row, n_plane, anomaly;
   } sz;
-  hipMemcpyAsync(&sz.n_oct, oct_off + nrank, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
-  hipMemcpyAsync(&sz.n_part, l.node_p0 + nn, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
-  hipMemcpyAsync(&sz.n_row, row_off + nrank, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
-  hipMemcpyAsync(&sz.n_plane, plane_off + nplr, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
-  hipMemcpyAsync(&sz.anomaly, c->d_anomaly, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
+  hipLaunchKernelGGL(k_sizes, dim3(1), dim3(1), 0, c->st, oct_off + nrank, l.node_p0 + nn, row_off + nrank,
+                     plane_off + nplr, c->d_anomaly, c->d_sizes);
+  hipMemcpyAsync(c->h_sizes, c->d_sizes, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "refine sizes: %s", hipGetErrorString(hipGetLastError()));
+  memcpy(&sz, c->h_sizes, sizeof(sz));
   l.n_part = sz.n_part;
   if (sz.anomaly)
     return c->fail(AHFGPU_EUNSUP, "level %d: AHF would create an empty x-run here (undefined behaviour)", ci);
@@ -2001,10 +2012,10 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   c->timer_end(e2, TIMER_RELINK, 29.0 * (double)nent + 24.0 * (double)nn2);
   /* the particles the fine level keeps: its deposit kernel depends on how
    * many entries each node holds */
-  int64_t nent_f = 0;
-  hipMemcpyAsync(&nent_f, f.node_p0 + nn2, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
+  hipMemcpyAsync(c->h_sizes + 7, f.node_p0 + nn2, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "level %d relink: %s", ci + 1, hipGetErrorString(hipGetLastError()));
+  const int64_t nent_f = c->h_sizes[7];
   f.n_part = nent_f;
   f.m2d = l.m2d * AHF_CRITMULTI_D;
   f.m2pd = l.m2pd * AHF_CRITMULTI_D;

@@ -38,7 +38,8 @@ extern "C" int ahfgpu_create(ahfgpu_ctx **out, int device)
       hipEventCreateWithFlags(&c->ev_s0, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_s1, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_s2, hipEventDisableTiming) != hipSuccess ||
-      hipMalloc((void **)&c->d_anomaly, 64) != hipSuccess ||
+      hipMalloc((void **)&c->d_anomaly, 64) != hipSuccess || hipMalloc((void **)&c->d_sizes, 64) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_sizes, 64, hipHostMallocDefault) != hipSuccess ||
       hipEventCreate(&c->ev_a0) != hipSuccess || hipEventCreate(&c->ev_a1) != hipSuccess ||
       hipEventCreate(&c->ev_u0) != hipSuccess || hipEventCreate(&c->ev_u1) != hipSuccess) {
     delete c;
@@ -72,6 +73,8 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   c->uarena.release();
   free_particles(c);
   if (c->d_anomaly) (void)hipFree(c->d_anomaly);
+  if (c->d_sizes) (void)hipFree(c->d_sizes);
+  if (c->h_sizes) (void)hipHostFree(c->h_sizes);
   if (c->d_flag) (void)hipFree(c->d_flag);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(c->st2);

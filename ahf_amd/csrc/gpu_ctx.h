@@ -137,6 +137,8 @@ struct ahfgpu_ctx {
   int rolled_back = 0;
   int built = 0;
   int32_t *d_anomaly = nullptr;
+  int64_t *d_sizes = nullptr;   /* level sizes gathered for one read-back  */
+  int64_t *h_sizes = nullptr;   /* pinned host copy of d_sizes             */
   void *halo_set = nullptr; /* resident halo lists (halo_set.h) */
   uint64_t *sfckey = nullptr; /* SFC keys of the resident particles */
   int64_t key_cap = 0;
