@@ -80,7 +80,10 @@ def _probe_copies(db, counter):
     return [v for gs, v in recs if gs == g]
 
 
-def traffic(fetch_db, write_db, kernels, copy_bytes):
+def traffic(fetch_db, write_db, kernels, copy_bytes, units=None):
+    """HBM bytes per launch of the stage made of `kernels`; with units, per
+    unit of work instead of per dispatch (e.g. per grid level when a level's
+    stage is one or two dispatches)"""
     f_copy = _probe_copies(fetch_db, "FETCH_SIZE")
     w_copy = _probe_copies(write_db, "WRITE_SIZE")
     if not f_copy or not w_copy:
@@ -92,8 +95,8 @@ def traffic(fetch_db, write_db, kernels, copy_bytes):
     w_k = _per_dispatch(write_db, kernels, "WRITE_SIZE")
     if not f_k or not w_k:
         raise SystemExit(f"no dispatches of {kernels} with FETCH_SIZE / WRITE_SIZE")
-    fetch = 1024.0 * f_scale * sum(f_k) / len(f_k)
-    write = 1024.0 * w_scale * sum(w_k) / len(w_k)
+    fetch = 1024.0 * f_scale * sum(f_k) / (units or len(f_k))
+    write = 1024.0 * w_scale * sum(w_k) / (units or len(w_k))
     return {"kernel": kernels if isinstance(kernels, str) else " + ".join(kernels), "dispatches": len(f_k), "fetch_bytes_per_launch": fetch,
             "write_bytes_per_launch": write, "bytes_per_launch": fetch + write,
             "fetch_scale": f_scale, "write_scale": w_scale,
@@ -116,6 +119,8 @@ def main(argv=None):
     c.add_argument("--kernel", required=True, action="append",
                    help="exact kernel name; repeat for a stage made of several kernels")
     c.add_argument("--copy-bytes", type=int, required=True)
+    c.add_argument("--units", type=int, default=None,
+                   help="divide by this many launches (the stage's timed regions) instead of the dispatch count")
     c.add_argument("--workload", required=True)
     c.add_argument("--source", default="")
     c.add_argument("--out")
@@ -130,7 +135,7 @@ def main(argv=None):
             vals = " ".join(f"{n}={v:.0f}" for n, v in sorted(cs.items()))
             print(f"{did} {k} grid={gs} dur_us={dur / 1e3 if dur else 0:.1f} {vals}")
     else:
-        t = traffic(args.fetch, args.write, args.kernel, args.copy_bytes)
+        t = traffic(args.fetch, args.write, args.kernel, args.copy_bytes, args.units)
         t["workload"] = args.workload
         t["source"] = args.source
         text = json.dumps(t, indent=1)

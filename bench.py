@@ -112,11 +112,12 @@ def copy_probe(nbytes=1 << 30, reps=5):
 
 
 def load_traffic():
-    """HBM bytes per k_gather_light launch of this workload, measured with
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by the copy-kernel
-    calibration (profiles/gather_light_traffic.json, written by
+    """HBM bytes per deposit launch (one grid level) of this workload,
+    measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over the deposit
+    kernels and scaled by the copy-kernel calibration
+    (profiles/deposit_traffic.json, written by
     python -m ahf_amd.rocprof_summary traffic)."""
-    path = os.path.join(ROOT, "profiles", "gather_light_traffic.json")
+    path = os.path.join(ROOT, "profiles", "deposit_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -153,11 +154,12 @@ def unbinding_workload(seed=1002, reps=3, cpu=True, threads=16):
            "halos_per_s": s.nhalos / (t * 1e-3), "exact_replays": int(info.n_exact_replays),
            "max_iterations": int(info.max_iterations)}
     if cpu and os.path.exists(REF_BIN):
-        r = subprocess.run([REF_BIN, "time", snap, "--threads", str(threads), "--skip-amr", "--skip-1thread"],
+        r = subprocess.run([REF_BIN, "time", snap, "--threads", str(threads), "--skip-amr"],
                            capture_output=True, text=True, timeout=900)
         if r.returncode == 0:
             rb = json.loads(r.stdout.strip().splitlines()[-1])
             out["cpu_reference_ms"] = 1e3 * rb["unbind_seconds"]
+            out["cpu_reference_1thread_ms"] = 1e3 * rb.get("unbind_seconds_1thread", 0.0)
             out["cpu_threads"] = threads
             out["vs_cpu"] = out["cpu_reference_ms"] / t
     return out
@@ -366,8 +368,8 @@ def main():
         "stage_ms": stage_ms,
         "dominant_stage": dom,
         "roofline": {
-            "kernel": ("exact-order TSC deposit, one launch per level: k_gather_src on dense levels "
-                       "(domain), k_gather_light on sparse ones"),
+            "kernel": ("exact-order TSC deposit, one timed launch per grid level: k_gather_march + "
+                       "k_gather_slab (face planes) on levels holding every octet, k_gather_light on the others"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -397,7 +399,8 @@ def main():
                 "kind": "reference",
                 "sample": (f"whole workload once: AMR build {rb['amr_seconds']:.2f}s on 1 thread "
                            f"(reference default: deposit is serial), unbinding {rb['unbind_seconds']:.3f}s "
-                           f"with OpenMP over haloes on {threads} threads; host {cpu_model()}"),
+                           f"with OpenMP over haloes on {threads} threads "
+                           f"({rb.get('unbind_seconds_1thread', 0.0):.3f}s on 1 thread); host {cpu_model()}"),
             }
             out["vs_cpu"] = value / out["cpu_baseline"]["value"]
     ctx.close()
