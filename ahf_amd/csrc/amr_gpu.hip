@@ -1125,6 +1125,20 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
   }
 }
 
+/* inclusive prefix sum over the 64 lanes of a wavefront by DPP row shifts
+ * (within each row of 16) and row broadcasts of lanes 15 and 31 into the
+ * rows above (the GFX9 scan; register-only) */
+__device__ __forceinline__ int32_t wave_incl_scan_dpp(int32_t v)
+{
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); /* row_shr:1 */
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); /* row_shr:2 */
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); /* row_shr:4 */
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); /* row_shr:8 */
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); /* row_bcast:15 into rows 1, 3 */
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); /* row_bcast:31 into rows 2, 3 */
+  return v;
+}
+
 __device__ inline double readlane_d(double x, int l)
 {
   const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
@@ -1176,17 +1190,16 @@ __device__ float fold_wave(float d, double cv, int mm)
       if (lane == l && ((par + (int)(rl & 1)) & 1)) r += 1;
       tmask &= tmask - 1;
     }
-    int64_t P = act ? r : 0;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int64_t t = __shfl_up(P, off, 64);
-      if (lane >= off) P += t;
-    }
-    const int64_t Kin = K0 + P - (act ? r : 0);
+    /* a step that stays in the binade has |r| < 2^23, so r clamped to
+     * +-2^24 keeps every prefix up to the first step that leaves it exact,
+     * and 64 clamped steps fit in 32 bits */
+    const int32_t rc = (int32_t)(r > ((int64_t)1 << 24) ? ((int64_t)1 << 24) : (r < -((int64_t)1 << 24) ? -((int64_t)1 << 24) : r));
+    const int32_t Pi = wave_incl_scan_dpp(act ? rc : 0);
+    const int64_t Kin = K0 + (int64_t)(Pi - (act ? rc : 0));
     const int valid = act && lok && ahf_fold_K_in(Kin, neg) && ahf_fold_T_in(Kin, R, neg);
     const uint64_t badm = __ballot(act && !valid);
     const int bad = badm ? __ffsll((unsigned long long)badm) - 1 : mm;
-    if (bad > start) d = (float)ldexp((double)readlane_i64(K0 + P, bad - 1), e - 23);
+    if (bad > start) d = (float)ldexp((double)(K0 + (int64_t)__builtin_amdgcn_readlane(Pi, bad - 1)), e - 23);
     if (bad < mm) {
       d = (float)((double)d + readlane_d(cv, bad));
       start = bad + 1;
@@ -1243,19 +1256,48 @@ __device__ inline int64_t heavy_sources(const level_view &v, const uint32_t *__r
   return readlane_i64(inc, 26);
 }
 
-/* the source (lane of heavy_sources) holding flat contribution k: the last
- * source whose prefix is at most k (empty sources share their prefix with
- * the next one, so the last is the one that holds k) */
-__device__ inline int heavy_source_of(int64_t off, int64_t k)
+/*
+ * The source (lane of heavy_sources) and entry of flat contribution k =
+ * base + lane, for one round of 64.  j0, uniform, is the source holding
+ * contribution base and is advanced round by round; the few sources that
+ * start inside the round are taken from their lanes with readlane (scalar
+ * reads, no cross-lane shuffles).  A source holds k when it is the last
+ * whose prefix is at most k (empty sources share their prefix with the next).
+ */
+struct HeavyLane {
+  int64_t q;
+  int ex, ey, ez;
+  bool in;
+};
+
+__device__ inline HeavyLane heavy_lane(const HeavySrc &hs, int64_t total, int64_t base, int &j0, int descending)
 {
-  int idx = 0;
-#pragma unroll
-  for (int step = 16; step >= 1; step >>= 1) {
-    const int cand = idx + step;
-    const int64_t oc = __shfl(off, cand < 27 ? cand : 26, 64);
-    if (cand < 27 && oc <= k) idx = cand;
+  const int lane = threadIdx.x & 63;
+  const int64_t k = base + lane;
+  while (j0 < 26 && readlane_i64(hs.off, j0 + 1) <= base) j0++;
+  const int64_t kend = (base + 63 < total - 1) ? base + 63 : total - 1;
+  const int code = hs.ex + 1 + 3 * (hs.ey + 1) + 9 * (hs.ez + 1);
+  int64_t q0 = readlane_i64(hs.q0, j0), cnt = readlane_i64(hs.cnt, j0), off = readlane_i64(hs.off, j0);
+  int cd = __builtin_amdgcn_readlane(code, j0);
+  for (int j = j0 + 1; j < 27; j++) {
+    const int64_t oj = readlane_i64(hs.off, j);
+    if (oj > kend) break;
+    const int64_t qj = readlane_i64(hs.q0, j), cj = readlane_i64(hs.cnt, j);
+    const int dj = __builtin_amdgcn_readlane(code, j);
+    const bool mine = k >= oj;
+    q0 = mine ? qj : q0;
+    cnt = mine ? cj : cnt;
+    off = mine ? oj : off;
+    cd = mine ? dj : cd;
   }
-  return idx;
+  HeavyLane r;
+  const int64_t kk = k - off;
+  r.in = k < total;
+  r.q = descending ? q0 + cnt - 1 - kk : q0 + kk;
+  r.ex = cd % 3 - 1;
+  r.ey = (cd / 3) % 3 - 1;
+  r.ez = cd / 9 - 1;
+  return r;
 }
 
 /* exact-order density of the heavy targets, one wavefront per target: the
@@ -1278,14 +1320,44 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const 
     HeavySrc hs;
     const int64_t total = heavy_sources(v, pmask, p0, n, &hs);
     float d = -1.0f;
+    /* the next round's weights are loaded before this round is folded */
+    const double *w0 = nullptr, *w1 = nullptr, *w2 = nullptr;
+    int j0 = 0;
+    auto fetch = [&](int64_t base) {
+      const HeavyLane hl = heavy_lane(hs, total, base, j0, descending);
+      if (hl.in) {
+        const double *w = reinterpret_cast<const double *>(W + hl.q);
+        w0 = w + (1 - hl.ez);
+        w1 = w + 3 + (1 - hl.ey);
+        w2 = w + 6 + (1 - hl.ex);
+      } else {
+        w0 = w1 = w2 = nullptr;
+      }
+    };
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (total > 0) {
+      fetch(0);
+      if (w0) {
+        a0 = *w0;
+        a1 = *w1;
+        a2 = *w2;
+      }
+    }
     for (int64_t base = 0; base < total; base += 64) {
-      const int64_t k = base + lane;
-      const int j = heavy_source_of(hs.off, k);
-      const int64_t q0 = __shfl(hs.q0, j, 64), cnt = __shfl(hs.cnt, j, 64), kk = k - __shfl(hs.off, j, 64);
-      const int ex = __shfl(hs.ex, j, 64), ey = __shfl(hs.ey, j, 64), ez = __shfl(hs.ez, j, 64);
-      double cv = 0.0;
-      if (k < total) cv = w9_contrib(W, (uint32_t)(descending ? q0 + cnt - 1 - kk : q0 + kk), 1 - ez, 1 - ey, 1 - ex);
+      const double cv = w0 ? (a0 * a1) * a2 : 0.0;
+      double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+      if (base + 64 < total) {
+        fetch(base + 64);
+        if (w0) {
+          b0 = *w0;
+          b1 = *w1;
+          b2 = *w2;
+        }
+      }
       d = fold_wave(d, cv, (int)(total - base < 64 ? total - base : 64));
+      a0 = b0;
+      a1 = b1;
+      a2 = b2;
     }
     if (lane == 0) dens[n] = d;
   }
@@ -1315,18 +1387,34 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy_q(level_view v, cons
     HeavySrc hs;
     const int64_t total = heavy_sources(v, pmask, p0, n, &hs);
     float d = -1.0f;
+    /* the next round's records are loaded before this round is folded */
+    int ex = 0, ey = 0, ez = 0;
+    bool in = false;
+    ahf_q4 e4 = {0.0f, 0.0f, 0.0f, 0.0f};
+    int j0 = 0;
+    auto fetch = [&](int64_t base, ahf_q4 &rec, int &fx, int &fy, int &fz, bool &ok) {
+      const HeavyLane hl = heavy_lane(hs, total, base, j0, descending);
+      fx = hl.ex;
+      fy = hl.ey;
+      fz = hl.ez;
+      ok = hl.in;
+      if (ok) rec = qp[hl.q];
+    };
+    if (total > 0) fetch(0, e4, ex, ey, ez, in);
     for (int64_t base = 0; base < total; base += 64) {
-      const int64_t k = base + lane;
-      const int j = heavy_source_of(hs.off, k);
-      const int64_t q0 = __shfl(hs.q0, j, 64), cnt = __shfl(hs.cnt, j, 64), kk = k - __shfl(hs.off, j, 64);
-      const int ex = __shfl(hs.ex, j, 64), ey = __shfl(hs.ey, j, 64), ez = __shfl(hs.ez, j, 64);
-      double cv = 0.0;
-      if (k < total) {
-        const ahf_q4 e4 = qp[descending ? q0 + cnt - 1 - kk : q0 + kk];
-        cv = ahf_contrib(m2pd, ahf_pn_sep(e4.x, cxv[ex + 1], L), ahf_pn_sep(e4.y, cyv[ey + 1], L),
-                         ahf_pn_sep(e4.z, czv[ez + 1], L), -ex, -ey, -ez);
-      }
+      const double cv = in ? ahf_contrib(m2pd, ahf_pn_sep(e4.x, cxv[ex + 1], L), ahf_pn_sep(e4.y, cyv[ey + 1], L),
+                                         ahf_pn_sep(e4.z, czv[ez + 1], L), -ex, -ey, -ez)
+                           : 0.0;
+      ahf_q4 n4 = {0.0f, 0.0f, 0.0f, 0.0f};
+      int nx = 0, ny = 0, nz = 0;
+      bool nin = false;
+      if (base + 64 < total) fetch(base + 64, n4, nx, ny, nz, nin);
       d = fold_wave(d, cv, (int)(total - base < 64 ? total - base : 64));
+      e4 = n4;
+      ex = nx;
+      ey = ny;
+      ez = nz;
+      in = nin;
     }
     if (lane == 0) dens[n] = d;
   }

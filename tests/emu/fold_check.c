@@ -48,15 +48,17 @@ static float fold_wave(float d, const double *cv, int mm, long *stats)
         for (int j = start; j < l; j++) k += r[j];
         if ((k + r[l]) & 1) r[l] += 1;
       }
-    int64_t P = 0;
+    /* steps clamped to +-2^24 and summed in 32 bits, as the kernel does */
+    int32_t P = 0;
     int bad = mm;
     for (int l = start; l < mm; l++) {
-      const int64_t Kin = K0 + P;
+      const int64_t Kin = K0 + (int64_t)P;
       if (!(lok[l] && ahf_fold_K_in(Kin, neg) && ahf_fold_T_in(Kin, R[l], neg))) {
         bad = l;
         break;
       }
-      P += r[l];
+      const int64_t lim = (int64_t)1 << 24;
+      P += (int32_t)(r[l] > lim ? lim : (r[l] < -lim ? -lim : r[l]));
     }
     if (bad > start) d = (float)ldexp((double)(K0 + P), e - 23);
     stats[1]++;
