@@ -529,6 +529,25 @@ AHF_HD int ahf_relink_child(float px, float py, float pz, int64_t MX, int64_t MY
   return -1;
 }
 
+/* ahf_relink_child with the six child-centre coordinates computed once
+ * (the same expressions, so the same bits) */
+AHF_HD int ahf_relink_child_h(float px, float py, float pz, int64_t MX, int64_t MY, int64_t MZ, double Lf,
+                              uint8_t interior_mask)
+{
+  const double h = 0.5 / Lf;
+  const double xc[2] = {((double)(2 * MX) / Lf) + h, ((double)(2 * MX + 1) / Lf) + h};
+  const double yc[2] = {((double)(2 * MY) / Lf) + h, ((double)(2 * MY + 1) / Lf) + h};
+  const double zc[2] = {((double)(2 * MZ) / Lf) + h, ((double)(2 * MZ + 1) / Lf) + h};
+  const int inx[2] = {fabs((double)px - xc[0]) <= h, fabs((double)px - xc[1]) <= h};
+  const int iny[2] = {fabs((double)py - yc[0]) <= h, fabs((double)py - yc[1]) <= h};
+  const int inz[2] = {fabs((double)pz - zc[0]) <= h, fabs((double)pz - zc[1]) <= h};
+  for (int c = 0; c < 8; c++) {
+    if (!((interior_mask >> c) & 1u)) continue;
+    if (inx[c & 1] && iny[(c >> 1) & 1] && inz[c >> 2]) return c;
+  }
+  return -1;
+}
+
 /* position of level-row (r,cy,cz) in (Z,Y) raster order of the level's rows */
 AHF_HD int64_t ahf_row_rank(const level_view *lv, int64_t r, int cy, int cz)
 {

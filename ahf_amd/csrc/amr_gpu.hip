@@ -1060,6 +1060,42 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
     if constexpr (FULL) {
       /* the lane's sources of this plane in order, as one flat sequence */
       uint32_t *qq = s_q + tid * 9;
+      if (tot <= (uint32_t)CAP) {
+        /* every source of the plane is staged (the common case): LDS slot
+         * (10 bits), count (9), roles (3), weight indices 1-ey and 1-ex;
+         * the next source's word is read one switch ahead */
+        uint32_t nq = 0, ntot = 0;
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          const uint32_t roles = ((m0 >> t) & 1u) | ((m1 >> t) & 1u) << 1 | ((m2 >> t) & 1u) << 2;
+          if (!roles) continue;
+          const int ey = oy[t / 3], ex = ox[t % 3];
+          const int u = (ly + 1 + ey) * AHF_SLAB_W + (lx + 1 + ex);
+          const uint32_t c = cnt_p[u];
+          qq[nq++] = s_off[u] | c << 10 | roles << 19 | (uint32_t)(1 - ey) << 22 | (uint32_t)(1 - ex) << 24;
+          ntot += c;
+        }
+        uint32_t k = 0, rem = 0, q = 0, nxt = qq[0];
+        uint32_t oy3 = 0, ox3 = 0;
+        bool r0 = false, r1 = false, r2 = false;
+        for (uint32_t it = 0; it < ntot; it++) {
+          if (rem == 0) {
+            const uint32_t wq = nxt;
+            k++;
+            nxt = qq[k < 9u ? k : 8u];
+            rem = (wq >> 10) & 0x1FFu;
+            q = m.step == 1u ? (wq & 0x3FFu) : (wq & 0x3FFu) + rem - 1u;
+            r0 = (wq >> 19) & 1u;
+            r1 = (wq >> 20) & 1u;
+            r2 = (wq >> 21) & 1u;
+            oy3 = (3u + ((wq >> 22) & 3u)) * (uint32_t)CAP;
+            ox3 = (6u + ((wq >> 24) & 3u)) * (uint32_t)CAP;
+          }
+          fold3(dn, dc, dp, r0, r1, r2, s_e[q], s_e[CAP + q], s_e[2 * CAP + q], s_e[oy3 + q], s_e[ox3 + q]);
+          q += m.step;
+          rem--;
+        }
+      } else {
       uint32_t nq = 0, ntot = 0;
 #pragma unroll
       for (int t = 0; t < 9; t++) {
@@ -1107,6 +1143,7 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
         }
         q += m.step;
         rem--;
+      }
       }
     } else {
       if (face) march_plane<CAP, FULL, true>(m, oy, ox, m0, m1, m2, dn, dc, dp);
@@ -1644,7 +1681,7 @@ __global__ void k_relink_child(int64_t n_ent, const uint32_t *__restrict__ pnode
         y = py[p];
         z = pz[p];
       }
-      ch = ahf_relink_child(x, y, z, MX, MY, MZ, Lf, imask);
+      ch = ahf_relink_child_h(x, y, z, MX, MY, MZ, Lf, imask);
     }
   }
   child[q] = (int8_t)ch;
@@ -1655,8 +1692,8 @@ __global__ void k_relink_child(int64_t n_ent, const uint32_t *__restrict__ pnode
  * node, so fine node t = 8*o2 + c holds the entries of its mother's bucket
  * whose child is c, in bucket order (AHF's head insertion during relink
  * leaves them in this order relative to the walk).  One thread per fine
- * octet walks the mother's bucket once: first to count its 8 children, then,
- * after a scan, to write them.  Buckets of more
+ * octet walks the mother's bucket once to count its 8 children; after a scan,
+ * each entry finds its place from the earlier entries of its bucket.  Buckets of more
  * than AHF_BIG_BUCKET entries are listed for k_fine_big, which walks them
  * one wavefront at a time.
  */
@@ -1689,34 +1726,28 @@ __global__ void k_fine_count(const int64_t *__restrict__ oct_parent, int64_t n2,
   }
 }
 
-__global__ void k_fine_fill(const int64_t *__restrict__ oct_parent, int64_t n2, const int64_t *__restrict__ p0c,
-                            const int8_t *__restrict__ child, const uint32_t *__restrict__ plistc,
-                            const int64_t *__restrict__ p0f, uint32_t *__restrict__ plistf,
-                            uint32_t *__restrict__ pnodef)
+/* one thread per coarse bucket entry: its place in the fine bucket of its
+ * child is the fine bucket's start plus the number of earlier entries of
+ * the mother's bucket with the same child (buckets over AHF_BIG_BUCKET
+ * entries are k_fine_big's) */
+__global__ void k_fine_fill(int64_t n_ent, const uint32_t *__restrict__ pnode, const int32_t *__restrict__ node2oct,
+                            const int64_t *__restrict__ p0c, const int8_t *__restrict__ child,
+                            const uint32_t *__restrict__ plistc, const int64_t *__restrict__ p0f,
+                            uint32_t *__restrict__ plistf, uint32_t *__restrict__ pnodef)
 {
-  const int64_t t8 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t8 >= n2) return;
-  const int64_t m = oct_parent[t8];
-  const int64_t a = p0c[m], b = p0c[m + 1];
-  if (b - a > AHF_BIG_BUCKET) return;
-  uint32_t w0 = (uint32_t)p0f[8 * t8 + 0], w1 = (uint32_t)p0f[8 * t8 + 1], w2 = (uint32_t)p0f[8 * t8 + 2],
-           w3 = (uint32_t)p0f[8 * t8 + 3], w4 = (uint32_t)p0f[8 * t8 + 4], w5 = (uint32_t)p0f[8 * t8 + 5],
-           w6 = (uint32_t)p0f[8 * t8 + 6], w7 = (uint32_t)p0f[8 * t8 + 7];
-  for (int64_t q = a; q < b; q++) {
-    const int c = child[q];
-    if (c < 0) continue;
-    const uint32_t w = c == 0 ? w0 : c == 1 ? w1 : c == 2 ? w2 : c == 3 ? w3 : c == 4 ? w4 : c == 5 ? w5 : c == 6 ? w6 : w7;
-    plistf[w] = plistc[q];
-    pnodef[w] = (uint32_t)(8 * t8 + c);
-    w0 += c == 0;
-    w1 += c == 1;
-    w2 += c == 2;
-    w3 += c == 3;
-    w4 += c == 4;
-    w5 += c == 5;
-    w6 += c == 6;
-    w7 += c == 7;
-  }
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_ent) return;
+  const int c = child[q];
+  if (c < 0) return;
+  const uint32_t m = pnode[q];
+  const int64_t a = p0c[m];
+  if (p0c[m + 1] - a > AHF_BIG_BUCKET) return;
+  const int64_t t = 8 * (int64_t)node2oct[m] + c;
+  int64_t rank = 0;
+  for (int64_t k = a; k < q; k++) rank += child[k] == c;
+  const int64_t w = p0f[t] + rank;
+  plistf[w] = plistc[q];
+  pnodef[w] = (uint32_t)t;
 }
 
 /* the listed big buckets, one wavefront per bucket: 64 entries per step,
@@ -2094,7 +2125,7 @@ static int refine_level(ahfgpu_ctx *c, int ci)
   f.plist = A.get<uint32_t>(nent);
   f.pnode = A.get<uint32_t>(nent);
   if (!f.plist || !f.pnode) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (level %d)", ci + 1);
-  LAUNCH(k_fine_fill, n2, f.oct_parent, n2, l.node_p0, l.child, l.plist, f.node_p0, f.plist, f.pnode);
+  LAUNCH(k_fine_fill, nent, nent, l.pnode, l.node2oct, l.node_p0, l.child, l.plist, f.node_p0, f.plist, f.pnode);
   hipLaunchKernelGGL(k_fine_big<true>, dim3(512), dim3(AHF_BLOCK), 0, c->st, f.oct_parent, l.node_p0, l.child,
                      l.plist, fbig, n_big, nullptr, f.node_p0, f.plist, f.pnode);
   c->timer_end(e2, TIMER_RELINK, 29.0 * (double)nent + 24.0 * (double)nn2);
