@@ -2130,12 +2130,23 @@ static int refine_level(ahfgpu_ctx *c, int ci)
                      l.plist, fbig, n_big, nullptr, f.node_p0, f.plist, f.pnode);
   c->timer_end(e2, TIMER_RELINK, 29.0 * (double)nent + 24.0 * (double)nn2);
   /* the particles the fine level keeps: its deposit kernel depends on how
-   * many entries each node holds */
-  hipMemcpyAsync(c->h_sizes + 7, f.node_p0 + nn2, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
-  if (hipStreamSynchronize(c->st) != hipSuccess)
-    return c->fail(AHFGPU_EHIP, "level %d relink: %s", ci + 1, hipGetErrorString(hipGetLastError()));
-  const int64_t nent_f = c->h_sizes[7];
-  f.n_part = nent_f;
+   * many entries each node holds, which is read back only where the coarse
+   * level's count (an upper bound) leaves the choice open; the bound then
+   * sizes the deposit's records and launches, and the count is read with
+   * the next level's sizes */
+  const int gm = c->gather_mode;
+  const bool full_f = gm == AHFGPU_GATHER_AUTO && f.L % AHF_SLAB == 0 && f.n_oct == (f.L / 2) * (f.L / 2) * (f.L / 2);
+  const bool decided = gm == AHFGPU_GATHER_FLAT || gm == AHFGPU_GATHER_SOURCE || full_f ||
+                       nent < AHF_DENSE_ENTRIES_PER_NODE * nn2;
+  int64_t nent_f = nent;
+  f.n_part = -1;
+  if (!decided) {
+    hipMemcpyAsync(c->h_sizes + 7, f.node_p0 + nn2, sizeof(int64_t), hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess)
+      return c->fail(AHFGPU_EHIP, "level %d relink: %s", ci + 1, hipGetErrorString(hipGetLastError()));
+    nent_f = c->h_sizes[7];
+    f.n_part = nent_f;
+  }
   f.m2d = l.m2d * AHF_CRITMULTI_D;
   f.m2pd = l.m2pd * AHF_CRITMULTI_D;
   f.crit = c->params.Nth_ref * f.m2d;

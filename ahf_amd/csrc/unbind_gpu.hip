@@ -420,20 +420,49 @@ __device__ inline T blk_excl_scan(T v, T *tot, T *sh /* [BS / 64] */)
   return base + ex;
 }
 
+/* a double moved across lanes by one DPP pattern (both halves); lanes
+ * without a source, or outside the row mask, read 0.0 */
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double v)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_add4(D4 &a)
+{
+  a.x += dpp_d<CTRL, ROWS>(a.x);
+  a.y += dpp_d<CTRL, ROWS>(a.y);
+  a.z += dpp_d<CTRL, ROWS>(a.z);
+  a.m += dpp_d<CTRL, ROWS>(a.m);
+}
+
+/* inclusive and exclusive wavefront scans of D4 by DPP row shifts and row
+ * broadcasts (the GFX9 scan; register-only): any summation order serves
+ * the unbinding, whose decisions carry a certified rounding bound */
+__device__ __forceinline__ void wave_scan4_dpp(D4 v, D4 *inc, D4 *exc)
+{
+  D4 a = v;
+  dpp_add4<0x111, 0xf>(a); /* row_shr:1 */
+  dpp_add4<0x112, 0xf>(a); /* row_shr:2 */
+  dpp_add4<0x114, 0xf>(a); /* row_shr:4 */
+  dpp_add4<0x118, 0xf>(a); /* row_shr:8 */
+  dpp_add4<0x142, 0xa>(a); /* row_bcast:15 into rows 1, 3 */
+  dpp_add4<0x143, 0xc>(a); /* row_bcast:31 into rows 2, 3 */
+  *inc = a;
+  /* wave_shr:1: the inclusive value of the lane below, 0 on lane 0 */
+  *exc = D4{dpp_d<0x138, 0xf>(a.x), dpp_d<0x138, 0xf>(a.y), dpp_d<0x138, 0xf>(a.z), dpp_d<0x138, 0xf>(a.m)};
+}
+
 template <int BS>
 __device__ inline D4 blk_excl_scan4(D4 v, D4 *tot, D4 *sh /* [BS / 64] */)
 {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  D4 inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const D4 u = shfl_up4(inc, d);
-    if (lane >= d) {
-      inc.x += u.x; inc.y += u.y; inc.z += u.z; inc.m += u.m;
-    }
-  }
-  D4 ex = shfl_up4(inc, 1);
-  if (lane == 0) ex = D4{0, 0, 0, 0};
+  D4 inc, ex;
+  wave_scan4_dpp(v, &inc, &ex);
+  (void)lane;
   if (lane == 63) sh[w] = inc;
   __syncthreads();
   D4 base = {0, 0, 0, 0}, t = {0, 0, 0, 0};
