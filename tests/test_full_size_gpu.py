@@ -1,0 +1,88 @@
+"""GPU parity at the full sizes of the generated workloads: the 128^3 and
+256^3 boxes (AMR hierarchy bit for bit, unbinding, and gather + the
+construction steps with identical haloes) and the 100,000-halo unbinding
+workload (22.4 M members), each against the CPU oracle on the same file."""
+import os
+
+import numpy as np
+import pytest
+
+from ahf_amd import dumps
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import ahf_amd
+    c = ahf_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _snap(snapgen, tmp_path, cfg, seed):
+    path = str(tmp_path / f"full_cfg{cfg}.snap")
+    assert snapgen.ahf_snapgen_write(cfg, 0, seed, path.encode()) == 0
+    return path
+
+
+def _same_haloes(ref, npart, lists, props, tag):
+    assert np.array_equal(ref.npart, npart.astype(ref.npart.dtype)), tag
+    bad = [h for h in range(len(lists)) if not np.array_equal(ref.ipart[h], lists[h])]
+    assert not bad, f"{tag}: {len(bad)} member lists differ, first {bad[:5]}"
+    assert np.array_equal(ref.M_vir, props[:, 0]), tag
+    big = ref.npart > 0
+    rel = np.abs(ref.Phi0[big] - props[big, 2]) / np.maximum(np.abs(ref.Phi0[big]), 1e-300)
+    assert rel.size == 0 or rel.max() < 1e-9, f"{tag}: Phi0 rel err {rel.max():.2e}"
+
+
+@pytest.mark.parametrize("cfg,seed", [(0, 1000), (1, 1001)])
+def test_amr_full_size(oracle, snapgen, ctx, tmp_path, cfg, seed):
+    snap = _snap(snapgen, tmp_path, cfg, seed)
+    s = dumps.read_snapshot(snap)
+    print(f"[full-gpu] amr cfg{cfg}: {s.npart} particles, L_dom {s.L_dom}", flush=True)
+    o_out, g_out = str(tmp_path / "oracle.grid"), str(tmp_path / "gpu.grid")
+    assert oracle.oracle_amr_dump(snap.encode(), o_out.encode()) > 0
+    ctx.set_gather_mode(0)
+    ctx.set_particles(s.pos, s.mom)
+    info = ctx.amr_build(s.L_dom, s.L_max, s.Nth_dom, s.Nth_ref)
+    ctx.amr_dump(g_out)
+    a, b = dumps.read_grids(g_out), dumps.read_grids(o_out)
+    diffs = dumps.compare_grids(a, b)
+    assert not diffs, "; ".join(diffs[:6])
+    assert info.n_levels == len(b)
+    os.remove(o_out)
+    os.remove(g_out)
+
+
+@pytest.mark.parametrize("cfg,seed", [(1, 1001), (2, 1002)])
+def test_unbind_full_size(oracle, ctx, snapgen, tmp_path, cfg, seed):
+    snap = _snap(snapgen, tmp_path, cfg, seed)
+    s = dumps.read_snapshot(snap)
+    print(f"[full-gpu] unbind cfg{cfg}: {s.nhalos} haloes, {s.n_ipart} members", flush=True)
+    out = str(tmp_path / "o.halo")
+    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 1) == 0
+    ref = dumps.read_halos(out)
+    ctx.set_unbind_split(0)
+    ctx.set_particles(s.pos, s.mom)
+    ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+    ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble)
+    npart, props, lists = ctx.unbind_results()
+    _same_haloes(ref, npart, lists, props, f"cfg{cfg}")
+
+
+def test_gather_construct_full_size(oracle, ctx, snapgen, tmp_path):
+    snap = _snap(snapgen, tmp_path, 1, 1001)
+    s = dumps.read_snapshot(snap)
+    print(f"[full-gpu] gather + construct cfg1: {s.nhalos} haloes", flush=True)
+    out = str(tmp_path / "o.halo")
+    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 8, 1, 1, None) == 0
+    ref = dumps.read_halos(out)
+    ctx.set_particles(s.pos, s.mom)
+    ctx.sfc_keys()
+    ctx.gather_halos(s.centre, s.gather_rad, skip=(s.halo_npart == 0))
+    ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=ctx.UNBIND_WITH_RVIR,
+               ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+    npart, props, lists = ctx.unbind_results()
+    _same_haloes(ref, npart, lists, props, "gather+construct cfg1")
+    assert np.array_equal(ref.ovdens, props[:, 3])
