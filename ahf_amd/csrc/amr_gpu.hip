@@ -1358,39 +1358,24 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_heavy(level_view v, const 
     const int64_t total = heavy_sources(v, pmask, p0, n, &hs);
     float d = -1.0f;
     /* the next round's weights are loaded before this round is folded */
-    const double *w0 = nullptr, *w1 = nullptr, *w2 = nullptr;
     int j0 = 0;
-    auto fetch = [&](int64_t base) {
+    auto fetch = [&](int64_t base, double &b0, double &b1, double &b2) {
       const HeavyLane hl = heavy_lane(hs, total, base, j0, descending);
+      b0 = b1 = b2 = 0.0;
       if (hl.in) {
         const double *w = reinterpret_cast<const double *>(W + hl.q);
-        w0 = w + (1 - hl.ez);
-        w1 = w + 3 + (1 - hl.ey);
-        w2 = w + 6 + (1 - hl.ex);
-      } else {
-        w0 = w1 = w2 = nullptr;
+        b0 = w[1 - hl.ez];
+        b1 = w[3 + (1 - hl.ey)];
+        b2 = w[6 + (1 - hl.ex)];
       }
     };
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    if (total > 0) {
-      fetch(0);
-      if (w0) {
-        a0 = *w0;
-        a1 = *w1;
-        a2 = *w2;
-      }
-    }
+    if (total > 0) fetch(0, a0, a1, a2);
     for (int64_t base = 0; base < total; base += 64) {
-      const double cv = w0 ? (a0 * a1) * a2 : 0.0;
+      /* lanes past the end fold nothing (mm below) */
+      const double cv = (a0 * a1) * a2;
       double b0 = 0.0, b1 = 0.0, b2 = 0.0;
-      if (base + 64 < total) {
-        fetch(base + 64);
-        if (w0) {
-          b0 = *w0;
-          b1 = *w1;
-          b2 = *w2;
-        }
-      }
+      if (base + 64 < total) fetch(base + 64, b0, b1, b2);
       d = fold_wave(d, cv, (int)(total - base < 64 ? total - base : 64));
       a0 = b0;
       a1 = b1;
@@ -1856,6 +1841,9 @@ static level_view make_view(const DevLevel &l)
 
 /* bucket entries per node from which a level counts as dense */
 #define AHF_DENSE_ENTRIES_PER_NODE 3
+/* heavy-node workgroups: 7 wavefronts per SIMD on 256 CUs, each wavefront
+ * folding one node at a time */
+#define AHF_HEAVY_BLOCKS 1792
 
 /*
  * Exact-order gather of one level over its n_ent bucket entries: the octets
@@ -1946,10 +1934,10 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   if (timed) c->timer_end(e1, TIMER_GATHER, 12.0 * (double)n_ent + 13.0 * (double)nn);
   hipEvent_t e2 = timed ? c->timer_begin() : nullptr;
   if (dense)
-    hipLaunchKernelGGL(k_gather_heavy_q, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.qp,
+    hipLaunchKernelGGL(k_gather_heavy_q, dim3(AHF_HEAVY_BLOCKS), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.qp,
                        descending, l.m2pd, l.dens, heavy, n_heavy);
   else
-    hipLaunchKernelGGL(k_gather_heavy, dim3(1024), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.W,
+    hipLaunchKernelGGL(k_gather_heavy, dim3(AHF_HEAVY_BLOCKS), dim3(AHF_BLOCK), 0, c->st, v, l.pmask, l.node_p0, l.W,
                        descending, l.dens, heavy, n_heavy);
   if (timed) c->timer_end(e2, TIMER_HEAVY, 0.0);
   return 0;
