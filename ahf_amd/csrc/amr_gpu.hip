@@ -1062,7 +1062,8 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
       uint32_t *qq = s_q + tid * 9;
       if (tot <= (uint32_t)CAP) {
         /* every source of the plane is staged (the common case): LDS slot
-         * (10 bits), count (9), roles (3), weight indices 1-ey and 1-ex;
+         * (10 bits), count (12 bits; a light target's sources hold at most
+         * kmax < 4096 entries), roles (3), weight indices 1-ey and 1-ex;
          * the next source's word is read one switch ahead */
         uint32_t nq = 0, ntot = 0;
 #pragma unroll
@@ -1072,7 +1073,7 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
           const int ey = oy[t / 3], ex = ox[t % 3];
           const int u = (ly + 1 + ey) * AHF_SLAB_W + (lx + 1 + ex);
           const uint32_t c = cnt_p[u];
-          qq[nq++] = s_off[u] | c << 10 | roles << 19 | (uint32_t)(1 - ey) << 22 | (uint32_t)(1 - ex) << 24;
+          qq[nq++] = s_off[u] | c << 10 | roles << 22 | (uint32_t)(1 - ey) << 25 | (uint32_t)(1 - ex) << 27;
           ntot += c;
         }
         uint32_t k = 0, rem = 0, q = 0, nxt = qq[0];
@@ -1083,13 +1084,13 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
             const uint32_t wq = nxt;
             k++;
             nxt = qq[k < 9u ? k : 8u];
-            rem = (wq >> 10) & 0x1FFu;
+            rem = (wq >> 10) & 0xFFFu;
             q = m.step == 1u ? (wq & 0x3FFu) : (wq & 0x3FFu) + rem - 1u;
-            r0 = (wq >> 19) & 1u;
-            r1 = (wq >> 20) & 1u;
-            r2 = (wq >> 21) & 1u;
-            oy3 = (3u + ((wq >> 22) & 3u)) * (uint32_t)CAP;
-            ox3 = (6u + ((wq >> 24) & 3u)) * (uint32_t)CAP;
+            r0 = (wq >> 22) & 1u;
+            r1 = (wq >> 23) & 1u;
+            r2 = (wq >> 24) & 1u;
+            oy3 = (3u + ((wq >> 25) & 3u)) * (uint32_t)CAP;
+            ox3 = (6u + ((wq >> 27) & 3u)) * (uint32_t)CAP;
           }
           fold3(dn, dc, dp, r0, r1, r2, s_e[q], s_e[CAP + q], s_e[2 * CAP + q], s_e[oy3 + q], s_e[ox3 + q]);
           q += m.step;
@@ -1105,10 +1106,10 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
         const uint32_t u = (uint32_t)((ly + 1 + ey) * AHF_SLAB_W + (lx + 1 + ex));
         const uint32_t c = cnt_p[u], o = s_off[u];
         const uint32_t staged = o + c <= (uint32_t)CAP;
-        /* LDS slot or node (10 bits), count (9), roles (3), weight indices
+        /* LDS slot or node (10 bits), count (12), roles (3), weight indices
          * 1-ey and 1-ex (2 + 2), staged (1) */
-        qq[nq++] = (staged ? o : u) | c << 10 | roles << 19 | (uint32_t)(1 - ey) << 22 | (uint32_t)(1 - ex) << 24 |
-                   staged << 26;
+        qq[nq++] = (staged ? o : u) | c << 10 | roles << 22 | (uint32_t)(1 - ey) << 25 | (uint32_t)(1 - ex) << 27 |
+                   staged << 29;
         ntot += c;
       }
       uint32_t k = 0, rem = 0, q = 0;
@@ -1118,13 +1119,13 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
       for (uint32_t it = 0; it < ntot; it++) {
         if (rem == 0) {
           const uint32_t wq = qq[k++];
-          rem = (wq >> 10) & 0x1FFu;
-          r0 = (wq >> 19) & 1u;
-          r1 = (wq >> 20) & 1u;
-          r2 = (wq >> 21) & 1u;
-          iy = (int)((wq >> 22) & 3u);
-          ix = (int)((wq >> 24) & 3u);
-          stg = (wq >> 26) & 1u;
+          rem = (wq >> 10) & 0xFFFu;
+          r0 = (wq >> 22) & 1u;
+          r1 = (wq >> 23) & 1u;
+          r2 = (wq >> 24) & 1u;
+          iy = (int)((wq >> 25) & 3u);
+          ix = (int)((wq >> 27) & 3u);
+          stg = (wq >> 29) & 1u;
           const uint32_t first = stg ? (wq & 0x3FFu) : start_p[wq & 0x3FFu];
           q = m.step == 1u ? first : first + rem - 1u;
           if (!stg) {
@@ -1844,6 +1845,9 @@ static level_view make_view(const DevLevel &l)
 /* heavy-node workgroups: 7 wavefronts per SIMD on 256 CUs, each wavefront
  * folding one node at a time */
 #define AHF_HEAVY_BLOCKS 1792
+/* nodes with more contributions than this go to the heavy kernels */
+#define AHF_KMAX_BIG 384u   /* levels of 2^21 nodes or more */
+#define AHF_KMAX_SMALL 96u
 
 /*
  * Exact-order gather of one level over its n_ent bucket entries: the octets
@@ -1903,7 +1907,8 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   const dim3 grid((unsigned)(AHF_XCDS * per_xcd));
   /* long lanes in a small launch leave the chip idle: hand them to the
    * wavefront-per-node kernel sooner */
-  const uint32_t kmax = c->gather_mode == AHFGPU_GATHER_WAVE ? 0u : nn >= ((int64_t)1 << 21) ? 384u : 96u;
+  const uint32_t kmax = c->gather_mode == AHFGPU_GATHER_WAVE ? 0u : nn >= ((int64_t)1 << 21) ? AHF_KMAX_BIG : AHF_KMAX_SMALL;
+  static_assert(AHF_KMAX_BIG < 4096 && AHF_KMAX_SMALL < 4096, "the march's queue words hold counts below 4096");
   if (slab) {
     /* the planes 1..L-2 march, the two face planes take the per-plane walk */
     const int64_t side = l.L / AHF_SLAB;
