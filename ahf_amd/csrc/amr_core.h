@@ -262,8 +262,9 @@ AHF_HD double ahf_w9_contrib(const ahf_w9 *w, int ex, int ey, int ez)
  * Exact-order density of one target node from the weight records of the
  * bucket entries, walked as one flat sequence: the sources that contribute
  * (present and non-empty) form a 27-bit mask in traversal order; the walk
- * takes them lowest bit first and each one's entries in list order.  Same
- * roundings as ahf_gather_dens.
+ * takes them lowest bit first and each one's entries in list order, and
+ * rounds each contribution into the FP32 value as AHF does
+ * (src/libamr_serial/density.c:393-400).
  */
 AHF_HD float ahf_gather_dens_w(const level_view *lv, const uint32_t *pmask, const int64_t *node_p0,
                                const ahf_w9 *W, int descending, int64_t o, int c)
