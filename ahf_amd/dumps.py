@@ -197,6 +197,56 @@ def read_halos(path: str) -> HaloResult:
     return HaloResult(npart, mv, rv, p0, lists, od)
 
 
+# HaloProfiles output: the integral properties and the per-bin profile
+# values in the order the AHFPROF1 dump writes them
+PROF_PROPS = ("nbins", "M_vir", "vel_x", "vel_y", "vel_z", "R_max", "V2_max", "r2", "sigV", "v_esc2", "Ekin",
+              "Epot", "axis_x", "axis_y", "axis_z", "E1_x", "E1_y", "E1_z", "E2_x", "E2_y", "E2_z", "E3_x",
+              "E3_y", "E3_z", "AngMom_x", "AngMom_y", "AngMom_z", "lambda", "lambdaE", "cNFW", "R1", "cR1",
+              "SurfP", "fMhires", "mbp_offset", "pos_mbp_x", "pos_mbp_y", "pos_mbp_z", "vel_mbp_x",
+              "vel_mbp_y", "vel_mbp_z", "com_offset", "pos_com_x", "pos_com_y", "pos_com_z")
+PROF_BINS = ("r", "npart", "nvpart", "ovdens", "dens", "v2_circ", "v_esc2", "sig_v", "Ekin", "Epot", "Lx", "Ly",
+             "Lz", "E1x", "E1y", "E1z", "E2x", "E2y", "E2z", "E3x", "E3y", "E3z", "axis1", "axis2", "axis3")
+
+
+@dataclasses.dataclass
+class ProfileResult:
+    npart: np.ndarray       # (nh,) members after the construction
+    nbins: np.ndarray       # (nh,) 0 where HaloProfiles skipped the halo
+    props: np.ndarray       # (nh, len(PROF_PROPS)), zero rows where nbins == 0
+    prof: list              # per halo (nbins, len(PROF_BINS))
+
+
+def read_profiles(path: str) -> ProfileResult:
+    with open(path, "rb") as f:
+        if f.read(8) != b"AHFPROF1":
+            raise ValueError(f"{path}: not an AHFPROF1 dump")
+        (nh,) = struct.unpack("<Q", f.read(8))
+        npart = np.zeros(nh, np.uint64)
+        nbins = np.zeros(nh, np.int64)
+        props = np.zeros((nh, len(PROF_PROPS)))
+        prof = []
+        for i in range(nh):
+            n, nb = struct.unpack("<QQ", f.read(16))
+            npart[i], nbins[i] = n, nb
+            if nb:
+                props[i] = np.fromfile(f, np.float64, len(PROF_PROPS))
+                prof.append(np.fromfile(f, np.float64, nb * len(PROF_BINS)).reshape(nb, len(PROF_BINS)))
+            else:
+                prof.append(np.zeros((0, len(PROF_BINS))))
+    return ProfileResult(npart, nbins, props, prof)
+
+
+def write_profiles(path: str, npart, nbins, props, prof) -> None:
+    with open(path, "wb") as f:
+        f.write(b"AHFPROF1")
+        f.write(struct.pack("<Q", len(npart)))
+        for i in range(len(npart)):
+            f.write(struct.pack("<QQ", int(npart[i]), int(nbins[i])))
+            if nbins[i]:
+                f.write(np.ascontiguousarray(props[i], np.float64).tobytes())
+                f.write(np.ascontiguousarray(prof[i], np.float64).tobytes())
+
+
 def compare_grids(a: list[GridLevel], b: list[GridLevel], *, dens_exact=True, rtol=0.0):
     """Return a list of human-readable differences (empty when identical)."""
     diffs = []

@@ -26,6 +26,9 @@
 #endif
 
 #include "snapfile.h"
+#include "oracle_profiles.h"
+
+_Static_assert(ORACLE_PROF_NQ == AHF_PROF_NQ && ORACLE_PROF_NP == AHF_PROF_NP, "profile value counts");
 
 #define MIN_NNODES 125     /* src/param.h:54 */
 #define CRITMULTI 8.0      /* src/param.h:118 */
@@ -1416,6 +1419,120 @@ int oracle_gather_dump(const char *snap_path, const char *out_path, int threads,
   }
   for (uint64_t h = 0; h < nh; h++) free(lists[h]);
   free(lists); free(cand); free(st); free(key);
+  snap_free(&s);
+  return rc;
+}
+
+/* ======================================================== HaloProfiles */
+
+/* HaloProfiles (src/libahf/ahf_halos.c:3961-5060) of one constructed halo:
+ * the members ip[0..n) in their radial order, Phi0 and R_vir as the
+ * construction left them.  Returns nbins (0 below minpart). */
+int oracle_halo_profiles(const float *pos, const float *mom, const uint64_t *ip, uint64_t n, const double c[3],
+                         double Phi0, double R_vir, const oracle_prof_params *p, double *props, double *prof)
+{
+  const ahf_prof_consts K = {p->minpart, p->r_fac, p->v_fac, p->m_fac, p->phi_fac, p->hubble};
+  if (n < (uint64_t)p->minpart || n == 0) return 0;
+  double *scratch = (double *)malloc(6 * n * sizeof(double));
+  if (!scratch) return -1;
+  const int nb = ahf_halo_profiles((int64_t)n, ip, pos, pos + 1, pos + 2, mom, mom + 1, mom + 2, 3, c, Phi0, R_vir,
+                                   &K, scratch, props, prof);
+  free(scratch);
+  return nb;
+}
+
+int oracle_prof_nbins(uint64_t n)
+{
+  return ahf_prof_nbins((int64_t)n);
+}
+
+void oracle_prof_get_axes(double it[9], double axes[3])
+{
+  double t[3][3];
+  for (int i = 0; i < 9; i++) t[i / 3][i % 3] = it[i];
+  ahf_prof_get_axes(t, &axes[0], &axes[1], &axes[2]);
+  for (int i = 0; i < 9; i++) it[i] = t[i / 3][i % 3];
+}
+
+double oracle_prof_cnfw(double V2_max, double V2_vir) { return ahf_prof_cnfw(V2_max, V2_vir); }
+double oracle_prof_cr1(double R1) { return ahf_prof_cr1(R1); }
+
+void oracle_prof_find_max(const double *x, const double *y, int64_t n, int ismooth, double *x_max, double *y_max)
+{
+  double *w = (double *)malloc(2 * (size_t)(n > 0 ? n : 1) * sizeof(double));
+  ahf_prof_find_max(x, y, n, ismooth, w, w + n, x_max, y_max);
+  free(w);
+}
+
+/* the halo construction of every halo of the snapshot (from its lists, or
+ * gathered from its centre and radius with gather = 1), then HaloProfiles;
+ * AHFPROF1 dump: nhalos, then per halo npart, nbins, and for nbins > 0 the
+ * ORACLE_PROF_NP properties and nbins x ORACLE_PROF_NQ profile values */
+int oracle_profiles_dump(const char *snap_path, const char *out_path, int threads, int gather)
+{
+  snap_data s;
+  if (snap_read(snap_path, &s)) return -1;
+  const uint64_t nh = s.h.nhalos, n = s.h.npart;
+  uint64_t *key = NULL;
+  if (gather) {
+    key = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    if (!snap_keys(&s, key)) {
+      free(key);
+      snap_free(&s);
+      return -3;
+    }
+  }
+  uint64_t *off = (uint64_t *)malloc((nh + 1) * sizeof(uint64_t));
+  off[0] = 0;
+  for (uint64_t i = 0; i < nh; i++) off[i + 1] = off[i] + s.halo_npart[i];
+  uint64_t **lists = (uint64_t **)calloc(nh ? nh : 1, sizeof(uint64_t *));
+  oracle_halo_state *st = (oracle_halo_state *)calloc(nh ? nh : 1, sizeof(oracle_halo_state));
+  int *nbins = (int *)calloc(nh ? nh : 1, sizeof(int));
+  double *props = (double *)calloc((nh ? nh : 1) * AHF_PROF_NP, sizeof(double));
+  double *prof = (double *)calloc((nh ? nh : 1) * AHF_PROF_MAXBINS * AHF_PROF_NQ, sizeof(double));
+  const oracle_unbind_params U = {s.h.minpart, s.h.vtune, s.h.r_fac, s.h.v_fac, s.h.phi_fac, s.h.hubble};
+  const double b = s.h.boxsize;
+  const oracle_rvir_params R = {s.h.minpart, s.h.ovlim, s.h.rho_vir, s.h.pmass / (b * b * b)};
+  const oracle_prof_params P = {s.h.minpart, s.h.r_fac, s.h.v_fac, s.h.pmass, s.h.phi_fac, s.h.hubble};
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  long i;
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++) {
+    uint64_t m = s.halo_npart[i];
+    if (gather) {
+      if (m == 0) continue;
+      uint64_t cand;
+      m = oracle_gather_halo(s.pos, key, n, s.centre + 3 * i, s.gather_rad[i], &lists[i], &cand);
+    } else {
+      lists[i] = (uint64_t *)malloc((m ? m : 1) * sizeof(uint64_t));
+      memcpy(lists[i], s.ipart + off[i], m * sizeof(uint64_t));
+    }
+    st[i].npart = m;
+    oracle_construct_halo(s.pos, s.mom, lists[i], m, s.centre + 3 * i, &U, &R, 0, &st[i]);
+    nbins[i] = oracle_halo_profiles(s.pos, s.mom, lists[i], st[i].npart, s.centre + 3 * i, st[i].Phi0, st[i].R_vir,
+                                    &P, props + AHF_PROF_NP * i, prof + (size_t)AHF_PROF_MAXBINS * AHF_PROF_NQ * i);
+  }
+  FILE *f = fopen(out_path, "wb");
+  int rc = f ? 0 : -2;
+  if (f) {
+    put(f, "AHFPROF1", 1, 8);
+    put(f, &nh, 8, 1);
+    for (uint64_t h = 0; h < nh; h++) {
+      const uint64_t v[2] = {st[h].npart, (uint64_t)(nbins[h] > 0 ? nbins[h] : 0)};
+      put(f, v, 8, 2);
+      if (v[1]) {
+        put(f, props + AHF_PROF_NP * h, 8, AHF_PROF_NP);
+        put(f, prof + (size_t)AHF_PROF_MAXBINS * AHF_PROF_NQ * h, 8, (size_t)v[1] * AHF_PROF_NQ);
+      }
+    }
+    fclose(f);
+  }
+  for (uint64_t h = 0; h < nh; h++) free(lists[h]);
+  free(lists); free(st); free(nbins); free(props); free(prof); free(off); free(key);
   snap_free(&s);
   return rc;
 }

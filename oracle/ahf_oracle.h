@@ -101,6 +101,26 @@ void oracle_rem_unbound(const float *pos, const float *mom, uint64_t *ipart, uin
                         const double centre[3], const oracle_unbind_params *p,
                         oracle_unbind_result *res);
 
+/* HaloProfiles (src/libahf/ahf_halos.c:3961-5060) for AHF's default flags:
+ * props[ORACLE_PROF_NP] and prof[nbins * ORACLE_PROF_NQ] of one constructed
+ * halo (members in radial order); returns nbins, 0 below minpart.  The
+ * order of the values is the enum of oracle_profiles.h. */
+#define ORACLE_PROF_NQ 25
+#define ORACLE_PROF_NP 45
+typedef struct {
+  int64_t minpart;
+  double r_fac, v_fac, m_fac, phi_fac, hubble;
+} oracle_prof_params;
+int oracle_halo_profiles(const float *pos, const float *mom, const uint64_t *ipart, uint64_t n,
+                         const double centre[3], double Phi0, double R_vir, const oracle_prof_params *p,
+                         double *props, double *prof);
+int oracle_profiles_dump(const char *snap_path, const char *out_path, int threads, int gather);
+int oracle_prof_nbins(uint64_t n);
+void oracle_prof_get_axes(double it[9], double axes[3]);
+double oracle_prof_cnfw(double V2_max, double V2_vir);
+double oracle_prof_cr1(double R1);
+void oracle_prof_find_max(const double *x, const double *y, int64_t n, int ismooth, double *x_max, double *y_max);
+
 #ifdef __cplusplus
 }
 #endif

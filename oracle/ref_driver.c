@@ -23,6 +23,10 @@
  *   ahf_ref gather <snap> <halos.out> [--threads T]  ahf_halos_sfc_gatherParts
  *   ahf_ref construct-sfc <snap> <halos.out> [--threads T]
  *                       gatherParts + the construct steps, per halo
+ *   ahf_ref profiles <snap> <prof.out> [--threads T]
+ *                       the construct steps + HaloProfiles, per halo
+ *   ahf_ref profiles-sfc <snap> <prof.out> [--threads T]
+ *                       gatherParts + the construct steps + HaloProfiles
  *   ahf_ref keys <snap> <keys.out> <bits>          SFC key of every particle
  *   ahf_ref time   <snap> [--threads T] [--reps R] [--skip-amr] [--skip-1thread]
  *   ahf_ref layout                                  struct sizes/offsets (JSON)
@@ -66,7 +70,7 @@
 param_simu simu;
 info_global global;
 global_info_t global_info;
-double r_fac, v_fac, phi_fac, rho_fac, Hubble;
+double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac;
 #endif
 
 static double now_s(void)
@@ -109,8 +113,9 @@ static void setup_globals(const snap_data *s)
   v_fac = s->h.v_fac;
   phi_fac = s->h.phi_fac;
   Hubble = s->h.hubble;
-  /* src/libahf/ahf_halos.c:204-221 */
+  /* src/libahf/ahf_halos.c:199-206 */
   rho_fac = simu.pmass / (simu.boxsize * simu.boxsize * simu.boxsize);
+  m_fac = simu.pmass;
   global.ovlim = s->h.ovlim;
   global.rho_vir = s->h.rho_vir;
 }
@@ -410,6 +415,83 @@ static void cmd_gather(const snap_data *s, const char *out, int threads, int con
   fprintf(stdout, "{\"gather_seconds\": %.6f, \"threads\": %d}\n", t, threads);
 }
 
+/* the per-halo construction of cmd_construct (gather = 0) or cmd_gather
+ * (gather = 1) followed by HaloProfiles, as ahf_halos_sfc_constructHalo
+ * ends (src/libahf/ahf_halos_sfc.c:156); AHFPROF1 dump of what
+ * HaloProfiles sets: per halo npart, nbins, and for nbins > 0 the integral
+ * properties and the nbins x 25 profile values */
+static void dump_profiles(const char *out, const HALO *h, uint64_t nh)
+{
+  FILE *f = fopen(out, "wb");
+  if (!f) die("cannot open output");
+  fwrite("AHFPROF1", 1, 8, f);
+  fwrite(&nh, 8, 1, f);
+  for (uint64_t i = 0; i < nh; i++) {
+    const HALO *x = h + i;
+    const uint64_t nb = (x->prof.nbins > 0 && x->prof.r) ? (uint64_t)x->prof.nbins : 0;
+    const uint64_t v[2] = {x->npart, nb};
+    fwrite(v, 8, 2, f);
+    if (!nb) continue;
+    const double p[45] = {(double)nb, x->M_vir, x->vel.x, x->vel.y, x->vel.z, x->R_max, x->V2_max, x->r2, x->sigV,
+                          x->v_esc2, x->Ekin, x->Epot, x->axis.x, x->axis.y, x->axis.z, x->E1.x, x->E1.y, x->E1.z,
+                          x->E2.x, x->E2.y, x->E2.z, x->E3.x, x->E3.y, x->E3.z, x->AngMom.x, x->AngMom.y,
+                          x->AngMom.z, x->lambda, x->lambdaE, x->cNFW, x->R1, x->cR1, x->SurfP, x->fMhires,
+                          x->mbp_offset, x->pos_mbp.x, x->pos_mbp.y, x->pos_mbp.z, x->vel_mbp.x, x->vel_mbp.y,
+                          x->vel_mbp.z, x->com_offset, x->pos_com.x, x->pos_com.y, x->pos_com.z};
+    fwrite(p, 8, 45, f);
+    const HALOPROFILE *q = &x->prof;
+    for (uint64_t b = 0; b < nb; b++) {
+      const double r[25] = {q->r[b], (double)q->npart[b], q->nvpart[b], q->ovdens[b], q->dens[b], q->v2_circ[b],
+                            q->v_esc2[b], q->sig_v[b], q->Ekin[b], q->Epot[b], q->Lx[b], q->Ly[b], q->Lz[b],
+                            q->E1x[b], q->E1y[b], q->E1z[b], q->E2x[b], q->E2y[b], q->E2z[b], q->E3x[b],
+                            q->E3y[b], q->E3z[b], q->axis1[b], q->axis2[b], q->axis3[b]};
+      fwrite(r, 8, 25, f);
+    }
+  }
+  fclose(f);
+}
+
+static void cmd_profiles(const snap_data *s, const char *out, int threads, int gather)
+{
+  HALO *h = halo_setup(s);
+  if (gather) setup_sfc(s);
+#ifdef AHF_REF_BUILD
+  /* HaloProfiles logs each halo to io.logfile */
+  io.logfile = fopen("/dev/null", "w");
+  if (!io.logfile) die("cannot open /dev/null");
+#endif
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  const uint64_t nh = s->h.nhalos;
+  double t0 = now_s();
+  long i;
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++) {
+    HALO *x = h + i;
+    if (gather) {
+      x->gatherRad = s->gather_rad[i];
+      if (x->npart == 0) continue;
+      x->nll = 0;
+      x->ll = NULL;
+      x->npart = 0;
+      free(x->ipart);
+      x->ipart = NULL;
+      ahf_halos_sfc_gatherParts(x);
+    }
+    sort_halo_particles(x);
+    rem_outsideRvir(x, 0);
+    rem_unbound(x);
+    rem_outsideRvir(x, 1);
+    HaloProfiles(x);
+  }
+  double t = now_s() - t0;
+  dump_profiles(out, h, nh);
+  fprintf(stdout, "{\"profiles_seconds\": %.6f, \"threads\": %d}\n", t, threads);
+}
+
 #ifndef AHF_REF_BUILD
 /* the drop-in library's batched replacements of the per-halo loop */
 static void cmd_batch(const snap_data *s, const char *out, int construct)
@@ -531,6 +613,9 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "gather") || !strcmp(argv[1], "construct-sfc")) {
     if (argc < 4) die("gather needs an output path");
     cmd_gather(&s, argv[3], threads, !strcmp(argv[1], "construct-sfc"));
+  } else if (!strcmp(argv[1], "profiles") || !strcmp(argv[1], "profiles-sfc")) {
+    if (argc < 4) die("profiles needs an output path");
+    cmd_profiles(&s, argv[3], threads, !strcmp(argv[1], "profiles-sfc"));
   } else if (!strcmp(argv[1], "keys")) {
     if (argc < 5) die("keys needs an output path and a bit count");
     cmd_keys(&s, argv[3], (unsigned)atoi(argv[4]));

@@ -43,6 +43,17 @@ def _load_oracle():
     lib.oracle_tsc_weights.restype = None
     lib.oracle_indexx.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]
     lib.oracle_indexx.restype = None
+    lib.oracle_profiles_dump.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    lib.oracle_prof_nbins.argtypes = [ctypes.c_uint64]
+    lib.oracle_prof_get_axes.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_prof_get_axes.restype = None
+    lib.oracle_prof_cnfw.argtypes = [ctypes.c_double, ctypes.c_double]
+    lib.oracle_prof_cnfw.restype = ctypes.c_double
+    lib.oracle_prof_cr1.argtypes = [ctypes.c_double]
+    lib.oracle_prof_cr1.restype = ctypes.c_double
+    lib.oracle_prof_find_max.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.oracle_prof_find_max.restype = None
     return lib
 
 
