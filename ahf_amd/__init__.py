@@ -51,6 +51,12 @@ class UnbindInfo(ctypes.Structure):
                 ("max_sweeps", ctypes.c_int32)]
 
 
+class ProfileParams(ctypes.Structure):
+    _fields_ = [("minpart", ctypes.c_int32), ("pad", ctypes.c_int32), ("r_fac", ctypes.c_double),
+                ("v_fac", ctypes.c_double), ("m_fac", ctypes.c_double), ("phi_fac", ctypes.c_double),
+                ("hubble", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -94,6 +100,8 @@ def load():
     lib.ahfgpu_halo_members.argtypes = [P, P, P]
     lib.ahfgpu_timer_count.argtypes = []
     lib.ahfgpu_timer_get.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), P, P, P]
+    lib.ahfgpu_halo_profiles.argtypes = [P, ctypes.POINTER(ProfileParams), P, P]
+    lib.ahfgpu_profile_results.argtypes = [P, P, P]
     _lib = lib
     return lib
 
@@ -228,6 +236,25 @@ class Context:
         off = np.concatenate([[0], np.cumsum(self._hnp.astype(np.int64))])
         lists = [ip[off[h]:off[h] + int(npart[h])].copy() for h in range(nh)]
         return npart, props, lists
+
+    PROF_NQ, PROF_NP = 25, 45
+
+    def halo_profiles(self, minpart, r_fac, v_fac, m_fac, phi_fac, hubble, phi0_rvir=None):
+        """HaloProfiles of every resident halo: of the last unbinding result,
+        or (phi0_rvir given, shape (nh, 2)) of the lists as given.  Returns
+        (nbins[nh], props[nh, PROF_NP], list of per-halo (nbins, PROF_NQ))"""
+        import numpy as np
+        nh = self._hnp.shape[0]
+        p = ProfileParams(minpart, 0, r_fac, v_fac, m_fac, phi_fac, hubble)
+        nb = np.zeros(nh, np.int32)
+        pr = None if phi0_rvir is None else np.ascontiguousarray(phi0_rvir, np.float64).reshape(nh, 2)
+        self._check(self.lib.ahfgpu_halo_profiles(self.h, ctypes.byref(p), None if pr is None else pr.ctypes.data,
+                                                  nb.ctypes.data))
+        props = np.zeros((nh, self.PROF_NP))
+        prof = np.zeros((int(nb.sum()) + 1, self.PROF_NQ))
+        self._check(self.lib.ahfgpu_profile_results(self.h, props.ctypes.data, prof.ctypes.data))
+        off = np.concatenate([[0], np.cumsum(nb.astype(np.int64))])
+        return nb, props, [prof[off[h]:off[h + 1]].copy() for h in range(nh)]
 
     def timers(self):
         """{stage: (ms, launches, algorithmic bytes)} of the last build + unbinding"""

@@ -69,6 +69,7 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
   ahfgpu_release_halos(c);
+  ahfgpu_free_profiles(c);
   c->arena.release();
   c->uarena.release();
   free_particles(c);

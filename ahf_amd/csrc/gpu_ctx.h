@@ -105,11 +105,12 @@ enum {
   TIMER_USORT,       /* unbinding: radial sort                          */
   TIMER_UNBIND,      /* unbinding: iterations                           */
   TIMER_HGATHER,     /* halo gather: cells, tests, member lists         */
+  TIMER_PROFILES,    /* HaloProfiles: k_profiles                        */
   TIMER_COUNT
 };
 static const char *const k_timer_names[TIMER_COUNT] = {"bin",    "prep",        "gather_light", "gather_heavy",
                                                        "refine", "build_level", "relink",       "unbind_sort",
-                                                       "unbind", "halo_gather"};
+                                                       "unbind", "halo_gather", "halo_profiles"};
 
 struct TimedLaunch {
   hipEvent_t a, b;
@@ -127,6 +128,8 @@ struct ahfgpu_ctx {
   char err[512] = {0};
   Arena arena;        /* AMR build buffers */
   Arena uarena;       /* unbinding buffers */
+  Arena parena;       /* profile buffers */
+  void *prof_result = nullptr; /* last ahfgpu_halo_profiles result (profiles_gpu.hip) */
   int64_t N = 0;
   float *px = nullptr, *py = nullptr, *pz = nullptr;
   float *vx = nullptr, *vy = nullptr, *vz = nullptr;
@@ -209,5 +212,8 @@ struct ahfgpu_ctx {
     gather_launches = t_n[TIMER_GATHER];
   }
 };
+
+/* profiles_gpu.hip: drop the last profile result and its buffers */
+void ahfgpu_free_profiles(ahfgpu_ctx *c);
 
 #endif

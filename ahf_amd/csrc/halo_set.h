@@ -18,7 +18,16 @@
 
 #include "gpu_ctx.h"
 
-struct HaloOut;
+/* per-halo result of the unbinding steps (device) */
+struct HaloOut {
+  int64_t npart;
+  double M_vir, R_vir, Phi0, ovdens;
+  int32_t niter, replays;
+  int32_t in_b; /* final list is in buffer b (0 = a) */
+  int32_t set;  /* AHFGPU_SET_* fields written */
+  int64_t particle_iterations;
+  int32_t sweeps, pad;
+};
 
 struct HaloSet {
   int64_t nh = 0, ne = 0;

@@ -1,0 +1,788 @@
+/*
+ * profiles_gpu.hip -- HaloProfiles (src/libahf/ahf_halos.c:3961-5060) of
+ * every resident halo for AHF's default flags (unit masses, R_max and r2
+ * from the sorted particle list, Hubble drag in the kinetic energy).
+ *
+ * One wavefront per halo walks the members in their radial order, 64 at a
+ * time.  AHF's walk is one running sum after another (centre of mass,
+ * inertia tensor, bulk velocity, angular momentum, kinetic energy, the
+ * potential integral, potential energy), each rounded in member order.  The
+ * wavefront computes every per-member term in parallel (a lane per member),
+ * stages the terms in LDS, and then lets each running sum advance over the
+ * 64 staged terms on a lane of its own (a lane per sum): the additions are
+ * AHF's, in AHF's order, so the sums carry AHF's roundings, while the
+ * divisions, square roots and products around them run 64 wide.  A sum
+ * whose running value feeds later terms (the bulk velocity into the
+ * angular momentum and kinetic energy, the potential into the energies)
+ * leaves its running value in place of the terms.
+ *
+ * Radial bins: member j is taken into the current bin and closes every bin
+ * whose radius it reaches (ahf_halos.c:4209-4620: the loop condition tests
+ * the distance of the member taken last).  With bin radii that do not
+ * decrease (the sorted lists always give these), the bins closed after
+ * member j are a running maximum over the members, a wavefront scan; other
+ * radii take the same automaton sequentially.  A bin's values are the
+ * running sums at the member that closes it.
+ *
+ * After the walk the lanes turn to the bins (one lane per bin, at most 64
+ * bins below 2^32 members): profile values and the inertia eigen
+ * decomposition per bin; then R_max and r2 from the smoothed per-member
+ * curves (find_max, src/libutility/general.c:590-665) and the integral
+ * properties.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/ahf_gpu.h"
+#include "devprims.h"
+#include "gpu_ctx.h"
+#include "halo_set.h"
+
+#define PRF_PI 3.14159265358979323846264338 /* src/param.h:39 */
+#define PRF_GRAV 4.3006485e-9              /* src/param.h:45 */
+#define PRF_ZERO 1e-6                       /* src/param.h:124 (no -DDOUBLE) */
+#define PRF_MZERO 5e-16                     /* src/param.h:125 */
+#define PRF_MINSHELL 10                     /* AHF_MINPART_SHELL, src/param.h:16 */
+#define PRF_NIGNORE 5                       /* AHF_Rmax_r2_NIGNORE, src/param.h:25 */
+#define PRF_ROW 65                          /* LDS row of 64 terms + 1 (bank spread) */
+
+/* per-bin record columns: the running values at the member that closes a bin */
+enum { BT_A11 = 0, BT_A22, BT_A33, BT_A12, BT_A13, BT_A23, BT_LX, BT_LY, BT_LZ, BT_T, BT_EP, BT_VESC, BT_N };
+
+/* profile values per bin and integral properties, in the order of
+ * include/ahf_gpu.h */
+enum {
+  PQ_R, PQ_NPART, PQ_NVPART, PQ_OVDENS, PQ_DENS, PQ_V2CIRC, PQ_VESC2, PQ_SIGV, PQ_EKIN, PQ_EPOT, PQ_LX, PQ_LY, PQ_LZ,
+  PQ_E1X, PQ_E1Y, PQ_E1Z, PQ_E2X, PQ_E2Y, PQ_E2Z, PQ_E3X, PQ_E3Y, PQ_E3Z, PQ_AX1, PQ_AX2, PQ_AX3, PQ_N
+};
+enum {
+  PP_NBINS, PP_MVIR, PP_VX, PP_VY, PP_VZ, PP_RMAX, PP_V2MAX, PP_R2, PP_SIGV, PP_VESC2, PP_EKIN, PP_EPOT,
+  PP_AX1, PP_AX2, PP_AX3, PP_E1X, PP_E1Y, PP_E1Z, PP_E2X, PP_E2Y, PP_E2Z, PP_E3X, PP_E3Y, PP_E3Z,
+  PP_LX, PP_LY, PP_LZ, PP_LAMBDA, PP_LAMBDAE, PP_CNFW, PP_R1, PP_CR1, PP_SURFP, PP_FMHIRES,
+  PP_MBP_OFF, PP_MBP_X, PP_MBP_Y, PP_MBP_Z, PP_MBP_VX, PP_MBP_VY, PP_MBP_VZ,
+  PP_COM_OFF, PP_COM_X, PP_COM_Y, PP_COM_Z, PP_N
+};
+static_assert(PQ_N == AHFGPU_PROF_NQ && PP_N == AHFGPU_PROF_NP, "profile value counts");
+
+struct ProfConst {
+  double r_fac, v_fac, m_fac, phi_fac, hubble;
+  int64_t minpart;
+};
+
+struct ProfShared {
+  double ta[12][PRF_ROW]; /* centre of mass x3, inertia x6, bulk velocity x3 */
+  double tc[5][PRF_ROW];  /* angular momentum x3, kinetic energy, potential */
+  double te[PRF_ROW];     /* potential energy */
+  double rad[64];         /* bin radii */
+  double bt[64][BT_N];    /* per-bin record */
+  int64_t bclose[64];     /* member index that closed each bin */
+  int32_t bprev[64], bcur[64];
+  double fin[16];
+  int64_t ifin[4];
+};
+
+/* ------------------------------------------------------- per-bin maths */
+
+/* a(i,j) -= s*(a(k,l) + a(i,j)*tau), a(k,l) += s*(a(i,j) - a(k,l)*tau),
+ * both from the values before the step */
+__device__ inline void prf_rotate(double a[4][4], int i, int j, int k, int l, double s, double tau)
+{
+  const double g = a[i][j], h = a[k][l];
+  a[i][j] = g - s * (h + g * tau);
+  a[k][l] = h + s * (g - h * tau);
+}
+
+/*
+ * Eigen decomposition of a symmetric 3x3 matrix by cyclic Jacobi sweeps
+ * (Rutishauser's method in the form Numerical Recipes gives it, as
+ * src/libutility/general.c:1163-1240 uses it): rotations above a threshold
+ * of 0.2*S/9 in the first three sweeps, off-diagonal elements negligible
+ * against both diagonal ones dropped after the fourth, at most 50 sweeps.
+ * 1-based indices as there, so the rotations and the signs of the
+ * eigenvectors come out the same.
+ */
+__device__ void prf_jacobi3(double a[4][4], double d[4], double v[4][4])
+{
+  double b[4], z[4];
+  for (int p = 1; p <= 3; p++) {
+    for (int q = 1; q <= 3; q++) v[p][q] = 0.0;
+    v[p][p] = 1.0;
+    b[p] = d[p] = a[p][p];
+    z[p] = 0.0;
+  }
+  for (int sweep = 1; sweep <= 50; sweep++) {
+    const double sm = (fabs(a[1][2]) + fabs(a[1][3])) + fabs(a[2][3]);
+    if (sm == 0.0) return;
+    const double tresh = sweep < 4 ? 0.2 * sm / 9 : 0.0;
+    for (int p = 1; p <= 2; p++) {
+      for (int q = p + 1; q <= 3; q++) {
+        const double g = 100.0 * fabs(a[p][q]);
+        if (sweep > 4 && fabs(d[p]) + g == fabs(d[p]) && fabs(d[q]) + g == fabs(d[q])) {
+          a[p][q] = 0.0;
+        } else if (fabs(a[p][q]) > tresh) {
+          double h = d[q] - d[p], t;
+          if (fabs(h) + g == fabs(h)) {
+            t = a[p][q] / h;
+          } else {
+            const double theta = 0.5 * h / a[p][q];
+            t = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+            if (theta < 0.0) t = -t;
+          }
+          const double c = 1.0 / sqrt(1 + t * t), s = t * c, tau = s / (1.0 + c);
+          h = t * a[p][q];
+          z[p] -= h;
+          z[q] += h;
+          d[p] -= h;
+          d[q] += h;
+          a[p][q] = 0.0;
+          for (int j = 1; j <= p - 1; j++) prf_rotate(a, j, p, j, q, s, tau);
+          for (int j = p + 1; j <= q - 1; j++) prf_rotate(a, p, j, j, q, s, tau);
+          for (int j = q + 1; j <= 3; j++) prf_rotate(a, p, j, q, j, s, tau);
+          for (int j = 1; j <= 3; j++) prf_rotate(v, j, p, j, q, s, tau);
+        }
+      }
+    }
+    for (int p = 1; p <= 3; p++) {
+      b[p] += z[p];
+      d[p] = b[p];
+      z[p] = 0.0;
+    }
+  }
+}
+
+/* get_axes (src/libutility/specific.c:135-170): eigenvalues largest first,
+ * eigenvectors in the columns of e (ties keep their index order, as the
+ * insertion pass of indexx does for three values) */
+__device__ void prf_axes(const double t[6], double ax[3], double e[3][3])
+{
+  double a[4][4] = {}, d[4], v[4][4];
+  a[1][1] = t[0];
+  a[2][2] = t[1];
+  a[3][3] = t[2];
+  a[1][2] = a[2][1] = t[3];
+  a[1][3] = a[3][1] = t[4];
+  a[2][3] = a[3][2] = t[5];
+  prf_jacobi3(a, d, v);
+  int idx[4] = {0, 1, 2, 3};
+  for (int j = 2; j <= 3; j++) {
+    const int ix = idx[j];
+    int i = j - 1;
+    for (; i >= 1 && !(d[idx[i]] <= d[ix]); i--) idx[i + 1] = idx[i];
+    idx[i + 1] = ix;
+  }
+  for (int k = 0; k < 3; k++) {
+    ax[k] = d[idx[3 - k]];
+    for (int r = 0; r < 3; r++) e[r][k] = v[r + 1][idx[3 - k]];
+  }
+}
+
+/* calc_cNFW (specific.c:2024-2060): bisection on Prada et al. (2012) Eq. 9 */
+__device__ double prf_cnfw(double V2_max, double V2_vir)
+{
+  const double ratio = V2_max / V2_vir;
+  if (ratio <= 1 || ratio > 5.9) return -1;
+  double a = 2.2, b = 100;
+  while (b - a > 1e-3) {
+    const double c = (a + b) / 2;
+    const double fa = 0.216 * a / (log(1 + a) - a / (1 + a)) - ratio;
+    const double fc = 0.216 * c / (log(1 + c) - c / (1 + c)) - ratio;
+    if (fa * fc > 0) a = c;
+    else b = c;
+  }
+  return (a + b) / 2.0;
+}
+
+/* calc_cR1 (specific.c:1990-2016): bisection on Wang et al.'s R1(c) */
+__device__ double prf_cr1(double R1)
+{
+  if (R1 <= 0.19 || 0.585 <= R1) return -1.0;
+  double a = 1.0, b = 500.0;
+  while (b - a > 1e-3) {
+    const double c = (a + b) / 2;
+    const double fa = (a - 2.0 * log(1.0 + a) + a / (1.0 + a)) / (a * (log(1.0 + a) - a / (1.0 + a))) - R1;
+    const double fc = (c - 2.0 * log(1.0 + c) + c / (1.0 + c)) / (c * (log(1.0 + c) - c / (1.0 + c))) - R1;
+    if (fa * fc > 0) a = c;
+    else b = c;
+  }
+  return (a + b) / 2.0;
+}
+
+/* ---------------------------------------------------- wavefront helpers */
+
+/* one running sum per lane (lanes < nch) over the cnt staged terms of row
+ * T[lane]: each term is replaced by the running value; at every member of
+ * the mask cm the value is recorded for the bins it closes, in column col
+ * (col < 0: not recorded) */
+__device__ inline void chain_pass(double (*T)[PRF_ROW], int nch, int cnt, uint64_t cm, ProfShared &S, double &acc,
+                                  int col)
+{
+  const int lane = threadIdx.x;
+  if (lane < nch) {
+    double a = acc;
+    double *row = T[lane];
+    int i = 0;
+    uint64_t m = cm;
+    while (m) {
+      const int e = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      for (; i <= e; i++) {
+        a += row[i];
+        row[i] = a;
+      }
+      if (col >= 0)
+        for (int b = S.bprev[e]; b < S.bcur[e]; b++) S.bt[b][col] = a;
+    }
+    for (; i < cnt; i++) {
+      a += row[i];
+      row[i] = a;
+    }
+    acc = a;
+  }
+  __syncthreads();
+}
+
+__device__ inline int wave_prefix_max(int v)
+{
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v = max(v, u);
+  }
+  return v;
+}
+
+/* the first maximum of y[0..m-2] above -10 as find_max's left pass takes
+ * it (general.c:628-637), m-1 when there is none; its right pass cannot
+ * move it (the range it scans lies inside the first) */
+__device__ int64_t wave_first_max(const double *y, int64_t m)
+{
+  const int lane = threadIdx.x;
+  double bv = -10.0;
+  int64_t bi = -1;
+  for (int64_t i = lane; i < m - 1; i += 64) {
+    const double v = y[i];
+    if (v > bv) {
+      bv = v;
+      bi = i;
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const double ov = __shfl_xor(bv, d, 64);
+    const int64_t oi = __shfl_xor(bi, d, 64);
+    if (oi >= 0 && (bi < 0 || ov > bv || (!(bv > ov) && oi < bi))) {
+      bv = ov;
+      bi = oi;
+    }
+  }
+  return bi >= 0 ? bi : m - 1;
+}
+
+/* smooth3 (general.c:548-566), one pass from src into dst (m >= 3) */
+__device__ void smooth_pass(const double *src, double *dst, int64_t m)
+{
+  for (int64_t i = threadIdx.x; i < m; i += 64) {
+    double v;
+    if (i == 0) v = (src[0] + src[1]) / 2.;
+    else if (i == m - 1) v = (src[m - 1] + src[m - 2]) / 2.;
+    else v = (src[i - 1] + src[i] + src[i + 1]) / 3.;
+    dst[i] = v;
+  }
+  __syncthreads();
+}
+
+/* ------------------------------------------------------------- kernel */
+
+__global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__restrict__ off,
+                                                 const int32_t *__restrict__ order, const HaloOut *__restrict__ out,
+                                                 const double *__restrict__ phi_rvir,
+                                                 const uint32_t *__restrict__ ip_in, const uint32_t *__restrict__ buf_a,
+                                                 const uint32_t *__restrict__ buf_b, const double *__restrict__ cen,
+                                                 const int32_t *__restrict__ nbins_h, const int64_t *__restrict__ pofs,
+                                                 const float *__restrict__ px, const float *__restrict__ py,
+                                                 const float *__restrict__ pz, const float *__restrict__ vx,
+                                                 const float *__restrict__ vy, const float *__restrict__ vz,
+                                                 double *__restrict__ s_rad, double *__restrict__ s_vc2,
+                                                 double *__restrict__ s_dr2, double *__restrict__ s_tmp,
+                                                 double *__restrict__ props, double *__restrict__ prof)
+{
+  __shared__ ProfShared S;
+  const int lane = threadIdx.x;
+  const int64_t h = order ? order[blockIdx.x] : blockIdx.x;
+  const int nbins = nbins_h[h];
+  if (nbins <= 0) return;
+  const int64_t base = off[h];
+  int64_t n;
+  const uint32_t *ip;
+  double Phi0, R_vir;
+  if (out) {
+    n = out[h].npart;
+    ip = (out[h].in_b ? buf_b : buf_a) + base;
+    Phi0 = out[h].Phi0;
+    R_vir = out[h].R_vir;
+  } else {
+    n = off[h + 1] - base;
+    ip = ip_in + base;
+    Phi0 = phi_rvir[2 * h];
+    R_vir = phi_rvir[2 * h + 1];
+  }
+  const double Xc = cen[3 * h], Yc = cen[3 * h + 1], Zc = cen[3 * h + 2];
+  double *rad_m = s_rad + base, *vc2 = s_vc2 + base, *dr2 = s_dr2 + base, *tmp = s_tmp + base;
+  const double FourPiThird = 4. * PRF_PI / 3.;
+
+  /* binning_parameter (specific.c:259-319): the first member from
+   * MINPART/10 on (below the last) whose distance is not below MACHINE_ZERO,
+   * else half the last member's distance; distances with |d| folded to
+   * 1-|d| beyond half the box */
+  auto dist_fold = [&](int64_t j) {
+    const uint32_t p = ip[j];
+    double dX = fabs((double)px[p] - Xc), dY = fabs((double)py[p] - Yc), dZ = fabs((double)pz[p] - Zc);
+    if (dX > 0.5) dX = 1.0 - dX;
+    if (dY > 0.5) dY = 1.0 - dY;
+    if (dZ > 0.5) dZ = 1.0 - dZ;
+    return sqrt(dX * dX + dY * dY + dZ * dZ);
+  };
+  const double dist_max = dist_fold(n - 1);
+  double dist_min = dist_max / 2.;
+  for (int64_t j0 = (int64_t)floor(((double)K.minpart / 10.) + 0.5); j0 < n - 1; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const double d = j < n - 1 ? dist_fold(j) : 0.0;
+    const uint64_t m = __ballot(j < n - 1 && !(d < PRF_MZERO));
+    if (m) {
+      dist_min = __shfl(d, __ffsll((unsigned long long)m) - 1, 64);
+      break;
+    }
+  }
+  const double ldmin = log10(dist_min), ldmax = log10(dist_max);
+  const double ldr = (ldmax - ldmin) / (double)nbins;
+  if (lane < nbins) S.rad[lane] = lane == nbins - 1 ? dist_max + PRF_ZERO : pow(10., ldmin + ((double)lane + 1) * ldr);
+  __syncthreads();
+  const bool mono = __ballot(lane < nbins - 1 && !(S.rad[lane] <= S.rad[lane + 1])) == 0;
+
+  /* the walk */
+  double accA = 0.0, accC = 0.0, accE = 0.0;
+  double I_car = 0.0, d_car = 0.0, Emin = 1e30;
+  int64_t mb = -1, P = 0;
+  int bcar = 0;
+  const double hub = K.hubble, rf = K.r_fac, vf = K.v_fac;
+  for (int64_t j0 = 0; j0 < n && bcar < nbins; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const bool inb = j < n;
+    double dX = 0.0, dY = 0.0, dZ = 0.0, d = 0.0, vxp = 0.0, vyp = 0.0, vzp = 0.0;
+    uint32_t p = 0;
+    if (inb) {
+      p = ip[j];
+      dX = (double)px[p] - Xc;
+      dY = (double)py[p] - Yc;
+      dZ = (double)pz[p] - Zc;
+      if (dX > 0.5) dX -= 1.0;
+      if (dY > 0.5) dY -= 1.0;
+      if (dZ > 0.5) dZ -= 1.0;
+      if (dX < -0.5) dX += 1.0;
+      if (dY < -0.5) dY += 1.0;
+      if (dZ < -0.5) dZ += 1.0;
+      d = sqrt(dX * dX + dY * dY + dZ * dZ);
+      vxp = (double)vx[p];
+      vyp = (double)vy[p];
+      vzp = (double)vz[p];
+    }
+    /* bins closed after each member */
+    int bp, bi;
+    if (mono) {
+      int c = 0;
+      if (inb) {
+        int lo = 0, hi = nbins; /* count of radii <= d */
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (S.rad[mid] <= d) lo = mid + 1;
+          else hi = mid;
+        }
+        c = j == n - 1 ? nbins : lo;
+      }
+      bi = max(wave_prefix_max(c), bcar);
+      bp = __shfl_up(bi, 1, 64);
+      if (lane == 0) bp = bcar;
+    } else {
+      S.ta[0][lane] = d;
+      __syncthreads();
+      if (lane == 0) {
+        int b = bcar;
+        for (int i = 0; i < 64; i++) {
+          S.bprev[i] = b;
+          if (b < nbins && j0 + i < n)
+            while (b < nbins && (S.ta[0][i] >= S.rad[b] || j0 + i == n - 1)) b++;
+          S.bcur[i] = b;
+        }
+      }
+      __syncthreads();
+      bp = S.bprev[lane];
+      bi = S.bcur[lane];
+      __syncthreads();
+    }
+    const bool proc = inb && bp < nbins;
+    const int cnt = __popcll(__ballot(proc));
+    const uint64_t cm = __ballot(proc && bi > bp);
+    if (proc) {
+      S.ta[0][lane] = Xc + dX;
+      S.ta[1][lane] = Yc + dY;
+      S.ta[2][lane] = Zc + dZ;
+      S.ta[3][lane] = dX * dX;
+      S.ta[4][lane] = dY * dY;
+      S.ta[5][lane] = dZ * dZ;
+      S.ta[6][lane] = dX * dY;
+      S.ta[7][lane] = dX * dZ;
+      S.ta[8][lane] = dY * dZ;
+      S.ta[9][lane] = vxp;
+      S.ta[10][lane] = vyp;
+      S.ta[11][lane] = vzp;
+      S.bprev[lane] = bp;
+      S.bcur[lane] = bi;
+      for (int b = bp; b < bi; b++) S.bclose[b] = j;
+      rad_m[j] = d;
+    }
+    __syncthreads();
+    /* centre of mass, inertia tensor, bulk velocity */
+    chain_pass(S.ta, 12, cnt, cm, S, accA, lane >= 3 && lane < 9 ? BT_A11 + lane - 3 : -1);
+
+    /* angular momentum and kinetic energy terms about the running bulk
+     * velocity; the potential integral's term (ahf_halos.c:4333-4395) */
+    const double M = (double)(j + 1);
+    double dVX = vxp - S.ta[9][lane] / M, dVY = vyp - S.ta[10][lane] / M, dVZ = vzp - S.ta[11][lane] / M;
+    const double lx = dY * dVZ - dZ * dVY, ly = dZ * dVX - dX * dVZ, lz = dX * dVY - dY * dVX;
+    dVX += hub * dX * rf / vf;
+    dVY += hub * dY * rf / vf;
+    dVZ += hub * dZ * rf / vf;
+    const double T = dVX * dVX + dVY * dVY + dVZ * dVZ;
+    const bool fl = proc && d > PRF_MZERO;
+    const double inow_own = M / (d * d);
+    const uint64_t fm = __ballot(fl);
+    const uint64_t le = fm & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+    const double got = __shfl(inow_own, le ? 63 - __clzll((long long)le) : 0, 64);
+    const double I_now = le ? got : I_car;
+    double I_prev = __shfl_up(I_now, 1, 64), pre = __shfl_up(d, 1, 64);
+    if (lane == 0) {
+      I_prev = I_car;
+      pre = d_car;
+    }
+    if (proc) {
+      S.tc[0][lane] = lx;
+      S.tc[1][lane] = ly;
+      S.tc[2][lane] = lz;
+      S.tc[3][lane] = T;
+      S.tc[4][lane] = fl ? ((I_now + I_prev) / 2.) * (d - pre) : -0.0; /* -0.0: an exact no-op */
+      vc2[j] = M / d;
+      const double dV = FourPiThird * ((d * d * d) - (pre * pre * pre));
+      const double hm = (d + pre) / 2.;
+      dr2[j] = 1.0 / dV * (hm * hm);
+    }
+    I_car = __shfl(I_now, cnt - 1, 64);
+    d_car = __shfl(d, cnt - 1, 64);
+    __syncthreads();
+    /* angular momentum, kinetic energy, potential */
+    chain_pass(S.tc, 5, cnt, cm, S, accC, lane < 4 ? BT_LX + lane : -1);
+
+    const double U = S.tc[4][lane] - Phi0;
+    const double Ep = 0.5 * T + U;
+    if (proc) {
+      S.te[lane] = U;
+      for (int b = bp; b < bi; b++) S.bt[b][BT_VESC] = 2 * fabs(U);
+    }
+    /* most bound member: the first at the lowest energy below the one so far */
+    {
+      double bv = proc && Ep < Emin ? Ep : 0.0;
+      int bl = proc && Ep < Emin ? lane : 64;
+#pragma unroll
+      for (int dd = 32; dd >= 1; dd >>= 1) {
+        const double ov = __shfl_xor(bv, dd, 64);
+        const int ol = __shfl_xor(bl, dd, 64);
+        if (ol < 64 && (bl == 64 || ov < bv || (!(bv < ov) && ol < bl))) {
+          bv = ov;
+          bl = ol;
+        }
+      }
+      if (bl < 64) {
+        Emin = bv;
+        mb = j0 + bl;
+      }
+    }
+    __syncthreads();
+    /* potential energy */
+    chain_pass(reinterpret_cast<double(*)[PRF_ROW]>(S.te), 1, cnt, cm, S, accE, BT_EP);
+    bcar = __shfl(bi, cnt - 1, 64);
+    P += cnt;
+    if (cnt < 64) break;
+  }
+  /* members the walk did not reach read as AHF's zeroed arrays */
+  for (int64_t j = P + lane; j < n; j += 64) rad_m[j] = vc2[j] = dr2[j] = 0.0;
+  if (lane < 12) S.fin[lane] = accA;
+  __syncthreads();
+
+  /* the bins */
+  double *pr = prof + pofs[h] * PQ_N;
+  if (lane < nbins) {
+    const int b = lane;
+    const double M = (double)(S.bclose[b] + 1), Mp = b ? (double)(S.bclose[b - 1] + 1) : 0.0;
+    const double r = S.rad[b], rp = b ? S.rad[b - 1] : 0.0;
+    const double Vol = FourPiThird * (r * r * r), Vp = b ? FourPiThird * (rp * rp * rp) : 0.0;
+    const double dM = M - Mp, dV = Vol - Vp;
+    double ax[3] = {1.0, 0.0, 0.0}, e[3][3] = {};
+    if (S.bclose[b] + 1 > PRF_MINSHELL) prf_axes(S.bt[b], ax, e);
+    double *q = pr + b * PQ_N;
+    q[PQ_R] = r;
+    q[PQ_NPART] = M;
+    q[PQ_NVPART] = M;
+    q[PQ_OVDENS] = M / Vol;
+    q[PQ_DENS] = dV > 0 ? dM / dV : 0.0;
+    q[PQ_V2CIRC] = M / r;
+    q[PQ_VESC2] = S.bt[b][BT_VESC];
+    q[PQ_SIGV] = sqrt(S.bt[b][BT_T] / M);
+    q[PQ_EKIN] = 0.5 * S.bt[b][BT_T];
+    q[PQ_EPOT] = 0.5 * S.bt[b][BT_EP];
+    q[PQ_LX] = S.bt[b][BT_LX];
+    q[PQ_LY] = S.bt[b][BT_LY];
+    q[PQ_LZ] = S.bt[b][BT_LZ];
+    for (int k = 0; k < 3; k++)
+      for (int r3 = 0; r3 < 3; r3++) q[PQ_E1X + 3 * k + r3] = e[r3][k];
+    q[PQ_AX1] = 1.0;
+    q[PQ_AX2] = ax[0] > 0. ? sqrt(ax[1] / ax[0]) : 0.0;
+    q[PQ_AX3] = ax[0] > 0. ? sqrt(ax[2] / ax[0]) : 0.0;
+  }
+  __syncthreads();
+
+  /* r2 and R_max: find_max over members NIGNORE.. of rho r^2 (three
+   * smoothing passes) and of V_c^2 (one pass) */
+  const int64_t m = n - PRF_NIGNORE;
+  const double *x = rad_m + PRF_NIGNORE;
+  const double *y = dr2 + PRF_NIGNORE;
+  if (m >= 3) {
+    smooth_pass(dr2 + PRF_NIGNORE, tmp, m);
+    smooth_pass(tmp, dr2 + PRF_NIGNORE, m);
+    smooth_pass(dr2 + PRF_NIGNORE, tmp, m);
+    y = tmp;
+  }
+  int64_t i0 = wave_first_max(y, m);
+  const double r2 = (x[i0] + x[i0]) / 2;
+  __syncthreads();
+  y = vc2 + PRF_NIGNORE;
+  if (m >= 3) {
+    smooth_pass(vc2 + PRF_NIGNORE, tmp, m);
+    y = tmp;
+  }
+  i0 = wave_first_max(y, m);
+  const double R_max = (x[i0] + x[i0]) / 2;
+  /* the first member at or beyond R_max (ahf_halos.c:4892-4894) */
+  int64_t ib = n - 1;
+  for (int64_t j0 = 0; j0 < n - 1; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const uint64_t mk = __ballot(j < n - 1 && !(rad_m[j] < R_max));
+    if (mk) {
+      ib = j0 + __ffsll((unsigned long long)mk) - 1;
+      break;
+    }
+  }
+  if (lane != 0) return;
+
+  const double rib = rad_m[ib];
+  const double ovd = ib < P ? (double)(ib + 1) / (FourPiThird * (rib * rib * rib)) : 0.0;
+  const double M_max = ovd * FourPiThird * (rib * rib * rib);
+  const double V_max = M_max / R_max;
+  const double Ms = (double)P;
+  const double *last = pr + (nbins - 1) * PQ_N, *prev = pr + (nbins - 2) * PQ_N;
+  double *Pp = props + h * PP_N;
+  const double Lx = last[PQ_LX], Ly = last[PQ_LY], Lz = last[PQ_LZ];
+  const double absL = sqrt(Lx * Lx + Ly * Ly + Lz * Lz);
+  Pp[PP_NBINS] = (double)nbins;
+  Pp[PP_MVIR] = Ms;
+  Pp[PP_VX] = S.fin[9] / Ms;
+  Pp[PP_VY] = S.fin[10] / Ms;
+  Pp[PP_VZ] = S.fin[11] / Ms;
+  Pp[PP_RMAX] = R_max;
+  Pp[PP_V2MAX] = V_max;
+  Pp[PP_R2] = r2;
+  Pp[PP_SIGV] = last[PQ_SIGV];
+  Pp[PP_VESC2] = last[PQ_VESC2];
+  Pp[PP_EKIN] = last[PQ_EKIN];
+  Pp[PP_EPOT] = last[PQ_EPOT];
+  Pp[PP_AX1] = last[PQ_AX1];
+  Pp[PP_AX2] = last[PQ_AX2];
+  Pp[PP_AX3] = last[PQ_AX3];
+  for (int k = 0; k < 9; k++) Pp[PP_E1X + k] = last[PQ_E1X + k];
+  if (absL > 0) {
+    Pp[PP_LX] = Lx / absL;
+    Pp[PP_LY] = Ly / absL;
+    Pp[PP_LZ] = Lz / absL;
+    double lam = absL / Ms / sqrt(2. * Ms * R_vir);
+    lam *= K.v_fac * sqrt(K.r_fac / (PRF_GRAV * K.m_fac));
+    Pp[PP_LAMBDA] = lam;
+    /* calc_lambdaE (ahf_halos.c:3936-3951) */
+    double t1 = sqrt(K.m_fac * Ms);
+    t1 = t1 * t1 * t1;
+    double t2 = Pp[PP_EKIN] * K.m_fac * (K.v_fac * K.v_fac);
+    double t3 = Pp[PP_EPOT] * K.m_fac * K.phi_fac;
+    t2 = sqrt(fabs(t2 + t3));
+    t1 = t2 / t1;
+    t2 = K.m_fac * K.r_fac * K.v_fac * absL;
+    t2 = t2 / (K.m_fac * Ms);
+    Pp[PP_LAMBDAE] = t1 * t2 / PRF_GRAV;
+  } else {
+    Pp[PP_LX] = Pp[PP_LY] = Pp[PP_LZ] = Pp[PP_LAMBDA] = Pp[PP_LAMBDAE] = 0.0;
+  }
+  Pp[PP_CNFW] = prf_cnfw(V_max, Ms / R_vir);
+  /* calc_R1 (specific.c:1977-1988) */
+  {
+    const double r0 = pr[PQ_R];
+    double R1 = ((r0 / 2.0) * (r0 / 2.0) * (r0 / 2.0)) * pr[PQ_DENS] * r0;
+    for (int b = 1; b < nbins; b++) {
+      const double rb = pr[b * PQ_N + PQ_R], rq = pr[(b - 1) * PQ_N + PQ_R];
+      const double rmid = (rb + rq) / 2.0, drr = rb - rq;
+      R1 += (rmid * rmid * rmid) * pr[b * PQ_N + PQ_DENS] * drr;
+    }
+    Pp[PP_R1] = 4 * PRF_PI * R1 / Ms / R_vir;
+  }
+  Pp[PP_CR1] = prf_cr1(Pp[PP_R1]);
+  {
+    const double Ts = 2.0 * (last[PQ_EKIN] - prev[PQ_EKIN]);
+    const double frad = fabs(prev[PQ_R] / last[PQ_R]);
+    Pp[PP_SURFP] = -0.125 * ((1. + frad) * (1. + frad) * (1. + frad)) / (1. - (frad * frad * frad)) * Ts;
+  }
+  Pp[PP_FMHIRES] = Ms > 0 ? Ms / (Ms + 0.0) : 0.0;
+  if (mb >= 0) {
+    const uint32_t q = ip[mb];
+    double dX = fabs(px[q] - Xc), dY = fabs(py[q] - Yc), dZ = fabs(pz[q] - Zc);
+    if (dX > 0.5) dX -= 1.0;
+    if (dY > 0.5) dY -= 1.0;
+    if (dZ > 0.5) dZ -= 1.0;
+    Pp[PP_MBP_OFF] = sqrt(dX * dX + dY * dY + dZ * dZ);
+    Pp[PP_MBP_X] = px[q];
+    Pp[PP_MBP_Y] = py[q];
+    Pp[PP_MBP_Z] = pz[q];
+    Pp[PP_MBP_VX] = vx[q];
+    Pp[PP_MBP_VY] = vy[q];
+    Pp[PP_MBP_VZ] = vz[q];
+  } else {
+    Pp[PP_MBP_OFF] = -1.0;
+    Pp[PP_MBP_X] = Pp[PP_MBP_Y] = Pp[PP_MBP_Z] = Pp[PP_MBP_VX] = Pp[PP_MBP_VY] = Pp[PP_MBP_VZ] = 0.0;
+  }
+  {
+    const double cx = fmod(S.fin[0] / Ms + 1., 1.), cy = fmod(S.fin[1] / Ms + 1., 1.),
+                 cz = fmod(S.fin[2] / Ms + 1., 1.);
+    double dX = fabs(cx - Xc), dY = fabs(cy - Yc), dZ = fabs(cz - Zc);
+    if (dX > 0.5) dX -= 1.0;
+    if (dY > 0.5) dY -= 1.0;
+    if (dZ > 0.5) dZ -= 1.0;
+    Pp[PP_COM_OFF] = sqrt(dX * dX + dY * dY + dZ * dZ);
+    Pp[PP_COM_X] = cx;
+    Pp[PP_COM_Y] = cy;
+    Pp[PP_COM_Z] = cz;
+  }
+}
+
+/* member count of every halo after the construction steps (or as given) */
+__global__ void k_prof_count(const int64_t *__restrict__ off, const HaloOut *__restrict__ out, int64_t nh,
+                             int64_t *__restrict__ n)
+{
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  n[h] = out ? out[h].npart : off[h + 1] - off[h];
+}
+
+/* --------------------------------------------------------------- host */
+
+struct ProfResult {
+  int64_t nh = 0, nbins_tot = 0;
+  double *props = nullptr, *prof = nullptr;
+};
+
+extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *p, const double *phi0_rvir,
+                                    int32_t *nbins_out)
+{
+  if (!c || !p) return AHFGPU_EINVAL;
+  HaloSet *s = (HaloSet *)c->halo_set;
+  if (!s) return c->fail(AHFGPU_EINVAL, "no halos resident (call ahfgpu_set_halos first)");
+  if (!phi0_rvir && !s->done) return c->fail(AHFGPU_EINVAL, "no construction result (run ahfgpu_unbind or pass Phi0/R_vir)");
+  /* find_max reads members NIGNORE.. of each list, so a halo needs more */
+  if (p->minpart <= PRF_NIGNORE) return c->fail(AHFGPU_EINVAL, "NminPerHalo must exceed %d", PRF_NIGNORE);
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  const int64_t nh = s->nh, ne = s->ne;
+  const HaloOut *out = phi0_rvir ? nullptr : s->out;
+  Arena &A = c->parena;
+  A.reset();
+  int64_t *d_n = A.get<int64_t>(nh);
+  int32_t *d_nb = A.get<int32_t>(nh);
+  int64_t *d_pofs = A.get<int64_t>(nh);
+  double *d_pr = phi0_rvir ? A.get<double>(2 * nh) : nullptr;
+  double *sc = A.get<double>(4 * ne);
+  if (!d_n || !d_nb || !d_pofs || !sc || (phi0_rvir && !d_pr))
+    return c->fail(AHFGPU_ENOMEM, "device arena exhausted (profiles)");
+  std::vector<int64_t> hn((size_t)nh), pofs((size_t)nh);
+  std::vector<int32_t> nb((size_t)nh);
+  if (nh) {
+    hipLaunchKernelGGL(k_prof_count, dim3((unsigned)((nh + AHF_BLOCK - 1) / AHF_BLOCK)), dim3(AHF_BLOCK), 0, c->st,
+                       s->off, out, nh, d_n);
+    hipMemcpyAsync(hn.data(), d_n, (size_t)nh * 8, hipMemcpyDeviceToHost, c->st);
+    if (hipStreamSynchronize(c->st) != hipSuccess)
+      return c->fail(AHFGPU_EHIP, "profiles: %s", hipGetErrorString(hipGetLastError()));
+  }
+  /* binning_parameter's bin count (specific.c:266-270), on the host's libm
+   * as AHF computes it */
+  int64_t tot = 0;
+  for (int64_t h = 0; h < nh; h++) {
+    const int64_t n = hn[(size_t)h];
+    int k = 0;
+    if (n >= p->minpart) {
+      k = (int)(6.2 * (log10((double)n)) - 3.5);
+      k = k > 2 ? k : 2;
+    }
+    nb[(size_t)h] = k;
+    pofs[(size_t)h] = tot;
+    tot += k;
+  }
+  ProfResult *r = (ProfResult *)c->prof_result;
+  if (!r) c->prof_result = r = new ProfResult();
+  r->nh = nh;
+  r->nbins_tot = tot;
+  r->props = A.get<double>(nh * PP_N);
+  r->prof = A.get<double>(tot * PQ_N);
+  if (!r->props || !r->prof) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (profiles)");
+  if (nh) {
+    hipMemcpyAsync(d_nb, nb.data(), (size_t)nh * 4, hipMemcpyHostToDevice, c->st);
+    hipMemcpyAsync(d_pofs, pofs.data(), (size_t)nh * 8, hipMemcpyHostToDevice, c->st);
+    if (phi0_rvir) hipMemcpyAsync(d_pr, phi0_rvir, (size_t)nh * 16, hipMemcpyHostToDevice, c->st);
+    hipMemsetAsync(r->props, 0, (size_t)nh * PP_N * 8, c->st);
+    ProfConst K{p->r_fac, p->v_fac, p->m_fac, p->phi_fac, p->hubble, p->minpart};
+    hipEvent_t e0 = c->timer_begin();
+    hipLaunchKernelGGL(k_profiles, dim3((unsigned)nh), dim3(64), 0, c->st, K, s->off, s->order, out, d_pr, s->ip_in,
+                       s->buf_a, s->buf_b, s->cen, d_nb, d_pofs, c->px, c->py, c->pz, c->vx, c->vy, c->vz, sc,
+                       sc + ne, sc + 2 * ne, sc + 3 * ne, r->props, r->prof);
+    c->timer_end(e0, TIMER_PROFILES, 0.0);
+  }
+  c->timers_reset(TIMER_PROFILES, TIMER_PROFILES);
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "profiles: %s", hipGetErrorString(hipGetLastError()));
+  c->timers_collect();
+  if (nbins_out && nh) memcpy(nbins_out, nb.data(), (size_t)nh * 4);
+  return 0;
+}
+
+extern "C" int ahfgpu_profile_results(ahfgpu_ctx *c, double *props, double *prof)
+{
+  if (!c) return AHFGPU_EINVAL;
+  ProfResult *r = (ProfResult *)c->prof_result;
+  if (!r) return c->fail(AHFGPU_EINVAL, "no profile result (call ahfgpu_halo_profiles first)");
+  if (props && r->nh) hipMemcpy(props, r->props, (size_t)r->nh * PP_N * 8, hipMemcpyDeviceToHost);
+  if (prof && r->nbins_tot) hipMemcpy(prof, r->prof, (size_t)r->nbins_tot * PQ_N * 8, hipMemcpyDeviceToHost);
+  if (hipDeviceSynchronize() != hipSuccess) return c->fail(AHFGPU_EHIP, "profile results");
+  return 0;
+}
+
+void ahfgpu_free_profiles(ahfgpu_ctx *c)
+{
+  delete (ProfResult *)c->prof_result;
+  c->prof_result = nullptr;
+  c->parena.release();
+}

@@ -316,17 +316,6 @@ __device__ void dev_indexx(int n, const double *arr1, int *indx1)
   }
 }
 
-/* per-halo output record */
-struct HaloOut {
-  int64_t npart;
-  double M_vir, R_vir, Phi0, ovdens;
-  int32_t niter, replays;
-  int32_t in_b; /* final list is in buffer b (0 = a) */
-  int32_t set;  /* AHFGPU_SET_* fields written */
-  int64_t particle_iterations;
-  int32_t sweeps, pad;
-};
-
 /* overdensity of the sphere through member j at particle p, as
  * src/libahf/ahf_halos.c:3832-3856 evaluates it (M = j+1 unit masses) */
 __device__ inline double sphere_ovdens(const UnbindConst &K, int64_t j, uint32_t p, const double cen[3],

@@ -207,6 +207,29 @@ int ahfgpu_set_unbind_split(ahfgpu_ctx *ctx, int64_t min_members);
 int ahfgpu_unbind_results(ahfgpu_ctx *ctx, uint64_t *npart, double *props, uint64_t *ipart);
 void ahfgpu_release_halos(ahfgpu_ctx *ctx);
 
+/* HaloProfiles (src/libahf/ahf_halos.c:3961-5060, AHF's default flags:
+ * unit masses, R_max and r2 from the sorted particle list, Hubble drag) of
+ * every resident halo, one wavefront per halo.  With phi0_rvir NULL the
+ * haloes are the result of the last ahfgpu_unbind (members, Phi0, R_vir);
+ * otherwise the resident lists as given, in their order, with Phi0 and
+ * R_vir from phi0_rvir[2*h], [2*h+1].  nbins[h] (may be NULL) receives
+ * each halo's bin count, 0 where HaloProfiles skips the halo (fewer than
+ * minpart members); minpart must exceed 5.  ahfgpu_profile_results then
+ * copies AHFGPU_PROF_NP properties per halo (zeros where nbins is 0) and
+ * AHFGPU_PROF_NQ values per bin, the bins of all haloes back to back in
+ * halo order.  The value orders are listed below. */
+#define AHFGPU_PROF_NQ 25 /* r npart nvpart ovdens dens v2_circ v_esc2 sig_v Ekin Epot Lx Ly Lz
+                             E1xyz E2xyz E3xyz axis1 axis2 axis3 */
+#define AHFGPU_PROF_NP 45 /* nbins M_vir vel.xyz R_max V2_max r2 sigV v_esc2 Ekin Epot axis.xyz
+                             E1.xyz E2.xyz E3.xyz AngMom.xyz lambda lambdaE cNFW R1 cR1 SurfP
+                             fMhires mbp_offset pos_mbp.xyz vel_mbp.xyz com_offset pos_com.xyz */
+typedef struct {
+  int32_t minpart, pad;                    /* simu.AHF_MINPART */
+  double r_fac, v_fac, m_fac, phi_fac, hubble; /* ahf_halos.c:199-206 */
+} ahfgpu_profile_params;
+int ahfgpu_halo_profiles(ahfgpu_ctx *ctx, const ahfgpu_profile_params *p, const double *phi0_rvir, int32_t *nbins);
+int ahfgpu_profile_results(ahfgpu_ctx *ctx, double *props, double *prof);
+
 /* device milliseconds of the last build / unbind (HIP events on the
  * library's stream), and of the dominant kernel of the last build */
 int ahfgpu_last_timing(ahfgpu_ctx *ctx, double *amr_ms, double *unbind_ms, double *gather_ms,
