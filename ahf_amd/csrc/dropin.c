@@ -23,7 +23,7 @@
 __attribute__((weak)) param_simu simu;
 __attribute__((weak)) info_global global;
 __attribute__((weak)) global_info_t global_info;
-__attribute__((weak)) double r_fac, v_fac, phi_fac, rho_fac, Hubble;
+__attribute__((weak)) double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac;
 
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static ahfgpu_ctx *g_ctx;
@@ -577,7 +577,177 @@ int ahf_construct_halos(HALO *halos, long unsigned nhalos)
   return rc;
 }
 
-/* the halo gather for nh haloes; with construct, the steps after it too.
+/* --------------------------------------------------------- profiles ---- */
+
+/* c_profile (src/libutility/alloc_struct.c:614-640): the profile arrays of
+ * nbins entries, each calloc()ed on its own as AHF frees them */
+static void alloc_profile(HALO *h, int nbins)
+{
+  HALOPROFILE *q = &h->prof;
+  q->nbins = nbins;
+  double **arr[] = {&q->r, &q->nvpart, &q->ovdens, &q->dens, &q->v2_circ, &q->v_esc2, &q->sig_v, &q->Ekin,
+                    &q->Epot, &q->Lx, &q->Ly, &q->Lz, &q->axis1, &q->E1x, &q->E1y, &q->E1z, &q->axis2,
+                    &q->E2x, &q->E2y, &q->E2z, &q->axis3, &q->E3x, &q->E3y, &q->E3z};
+  for (size_t k = 0; k < sizeof(arr) / sizeof(arr[0]); k++) *arr[k] = (double *)xcalloc((size_t)nbins, 8);
+  q->npart = (long unsigned *)xcalloc((size_t)nbins, sizeof(long unsigned));
+}
+
+/* the fields HaloProfiles sets (src/libahf/ahf_halos.c:4069, 4674-4710,
+ * 4922-5018) from one halo's AHFGPU_PROF_NP properties and nbins x
+ * AHFGPU_PROF_NQ profile values */
+static void store_profile(HALO *h, int nbins, const double *P, const double *Q)
+{
+  alloc_profile(h, nbins);
+  HALOPROFILE *q = &h->prof;
+  for (int b = 0; b < nbins; b++) {
+    const double *v = Q + (size_t)b * AHFGPU_PROF_NQ;
+    q->r[b] = v[0];
+    q->npart[b] = (long unsigned)v[1];
+    q->nvpart[b] = v[2];
+    q->ovdens[b] = v[3];
+    q->dens[b] = v[4];
+    q->v2_circ[b] = v[5];
+    q->v_esc2[b] = v[6];
+    q->sig_v[b] = v[7];
+    q->Ekin[b] = v[8];
+    q->Epot[b] = v[9];
+    q->Lx[b] = v[10];
+    q->Ly[b] = v[11];
+    q->Lz[b] = v[12];
+    q->E1x[b] = v[13];
+    q->E1y[b] = v[14];
+    q->E1z[b] = v[15];
+    q->E2x[b] = v[16];
+    q->E2y[b] = v[17];
+    q->E2z[b] = v[18];
+    q->E3x[b] = v[19];
+    q->E3y[b] = v[20];
+    q->E3z[b] = v[21];
+    q->axis1[b] = v[22];
+    q->axis2[b] = v[23];
+    q->axis3[b] = v[24];
+  }
+  h->M_vir = P[1];
+  h->vel.x = P[2];
+  h->vel.y = P[3];
+  h->vel.z = P[4];
+  h->R_max = P[5];
+  h->V2_max = P[6];
+  h->r2 = P[7];
+  h->sigV = P[8];
+  h->v_esc2 = P[9];
+  h->Ekin = P[10];
+  h->Epot = P[11];
+  h->axis.x = P[12];
+  h->axis.y = P[13];
+  h->axis.z = P[14];
+  h->E1.x = P[15];
+  h->E1.y = P[16];
+  h->E1.z = P[17];
+  h->E2.x = P[18];
+  h->E2.y = P[19];
+  h->E2.z = P[20];
+  h->E3.x = P[21];
+  h->E3.y = P[22];
+  h->E3.z = P[23];
+  h->AngMom.x = P[24];
+  h->AngMom.y = P[25];
+  h->AngMom.z = P[26];
+  h->lambda = P[27];
+  h->lambdaE = P[28];
+  h->cNFW = P[29];
+  h->R1 = P[30];
+  h->cR1 = P[31];
+  h->SurfP = P[32];
+  h->fMhires = P[33];
+  h->mbp_offset = P[34];
+  h->pos_mbp.x = P[35];
+  h->pos_mbp.y = P[36];
+  h->pos_mbp.z = P[37];
+  h->vel_mbp.x = P[38];
+  h->vel_mbp.y = P[39];
+  h->vel_mbp.z = P[40];
+  h->com_offset = P[41];
+  h->pos_com.x = P[42];
+  h->pos_com.y = P[43];
+  h->pos_com.z = P[44];
+}
+
+static ahfgpu_profile_params profile_params(void)
+{
+  ahfgpu_profile_params pp = {simu.AHF_MINPART, 0, r_fac, v_fac, m_fac, phi_fac, Hubble};
+  return pp;
+}
+
+/* HaloProfiles of the resident haloes (last unbinding result, or the lists
+ * as given with phi_rvir) into h[0..nh) */
+static int profiles_into(ahfgpu_ctx *c, HALO *h, long unsigned nh, const double *phi_rvir)
+{
+  const ahfgpu_profile_params pp = profile_params();
+  int32_t *nb = (int32_t *)xcalloc(nh, 4);
+  int rc = ahfgpu_halo_profiles(c, &pp, phi_rvir, nb);
+  uint64_t tot = 0;
+  for (long unsigned i = 0; i < nh; i++) tot += (uint64_t)nb[i];
+  double *P = (double *)xcalloc(nh * AHFGPU_PROF_NP, 8), *Q = (double *)xcalloc(tot * AHFGPU_PROF_NQ + 1, 8);
+  if (!rc) rc = ahfgpu_profile_results(c, P, Q);
+  if (!rc) {
+    uint64_t w = 0;
+    for (long unsigned i = 0; i < nh; i++) {
+      if (nb[i] <= 0) continue;
+      store_profile(h + i, nb[i], P + (size_t)i * AHFGPU_PROF_NP, Q + w * AHFGPU_PROF_NQ);
+      w += (uint64_t)nb[i];
+    }
+  }
+  free(nb); free(P); free(Q);
+  return rc;
+}
+
+static int run_profiles(HALO *h, long unsigned nh)
+{
+  ahfgpu_ctx *c = ctx_get();
+  ensure_particles(0);
+  uint64_t tot = 0;
+  for (long unsigned i = 0; i < nh; i++) tot += h[i].npart;
+  double *cen = (double *)xcalloc(5 * nh, 8), *pr = cen + 3 * nh;
+  uint64_t *np = (uint64_t *)xcalloc(nh, 8), *ip = (uint64_t *)xcalloc(tot + 1, 8);
+  uint64_t w = 0;
+  for (long unsigned i = 0; i < nh; i++) {
+    cen[3 * i] = h[i].pos.x;
+    cen[3 * i + 1] = h[i].pos.y;
+    cen[3 * i + 2] = h[i].pos.z;
+    pr[2 * i] = h[i].Phi0;
+    pr[2 * i + 1] = h[i].R_vir;
+    np[i] = h[i].npart;
+    for (uint64_t j = 0; j < h[i].npart; j++) ip[w++] = h[i].ipart[j];
+  }
+  int rc = ahfgpu_set_halos(c, cen, np, ip, nh);
+  if (!rc) rc = profiles_into(c, h, nh, pr);
+  ahfgpu_release_halos(c);
+  free(cen); free(np); free(ip);
+  return rc;
+}
+
+int HaloProfiles(HALO *halo)
+{
+  if (halo->npart < (long unsigned)simu.AHF_MINPART) return 1;
+  pthread_mutex_lock(&g_lock);
+  int rc = run_profiles(halo, 1);
+  pthread_mutex_unlock(&g_lock);
+  if (rc) fatal("HaloProfiles");
+  return 1;
+}
+
+int ahf_halo_profiles(HALO *halos, long unsigned nhalos)
+{
+  if (nhalos == 0) return 0;
+  pthread_mutex_lock(&g_lock);
+  int rc = run_profiles(halos, nhalos);
+  pthread_mutex_unlock(&g_lock);
+  return rc;
+}
+
+/* the halo gather for nh haloes; with construct, the steps after it too
+ * (construct == 2: and HaloProfiles, on the lists the device holds).
  * Haloes whose npart is 0 are skipped (src/libahf/ahf_halos_sfc.c:122-123
  * for the construction; the gather alone appends nothing for them) */
 static int gather_halos(HALO *h, long unsigned nh, int construct)
@@ -647,6 +817,11 @@ static int gather_halos(HALO *h, long unsigned nh, int construct)
       }
     }
   }
+  if (!rc && construct == 2) {
+    /* skipped haloes have no result: their npart of 0 keeps them below
+     * minpart, as AHF never profiles them either */
+    rc = profiles_into(c, h, nh, NULL);
+  }
   ahfgpu_release_halos(c);
   free(cen); free(skip); free(np); free(ip); free(props);
   return rc;
@@ -665,6 +840,15 @@ int ahf_gather_construct_halos(HALO *halos, long unsigned nhalos)
   if (nhalos == 0) return 0;
   pthread_mutex_lock(&g_lock);
   int rc = gather_halos(halos, nhalos, 1);
+  pthread_mutex_unlock(&g_lock);
+  return rc;
+}
+
+int ahf_construct_profile_halos(HALO *halos, long unsigned nhalos)
+{
+  if (nhalos == 0) return 0;
+  pthread_mutex_lock(&g_lock);
+  int rc = gather_halos(halos, nhalos, 2);
   pthread_mutex_unlock(&g_lock);
   return rc;
 }

@@ -76,7 +76,6 @@ struct ProfConst {
 struct ProfShared {
   double ta[12][PRF_ROW]; /* centre of mass x3, inertia x6, bulk velocity x3 */
   double tc[5][PRF_ROW];  /* angular momentum x3, kinetic energy, potential */
-  double te[PRF_ROW];     /* potential energy */
   double rad[64];         /* bin radii */
   double bt[64][BT_N];    /* per-bin record */
   int64_t bclose[64];     /* member index that closed each bin */
@@ -213,34 +212,61 @@ __device__ double prf_cr1(double R1)
 
 /* ---------------------------------------------------- wavefront helpers */
 
-/* one running sum per lane (lanes < nch) over the cnt staged terms of row
- * T[lane]: each term is replaced by the running value; at every member of
- * the mask cm the value is recorded for the bins it closes, in column col
- * (col < 0: not recorded) */
-__device__ inline void chain_pass(double (*T)[PRF_ROW], int nch, int cnt, uint64_t cm, ProfShared &S, double &acc,
-                                  int col)
+/* a running sum over row[i0..i1): each term replaced by the running value.
+ * Sixteen terms are read at a time so that one LDS round trip serves
+ * sixteen dependent additions.  WITH_E: a second sum over the running
+ * values less phi0 rides along (the potential energy over the potential,
+ * ahf_halos.c:4383-4395: Upart = Phi - Phi0 with unit weights). */
+template <bool WITH_E>
+__device__ inline void chain_run(double *row, int i0, int i1, double &acc, double &acc_e, double phi0)
 {
-  const int lane = threadIdx.x;
-  if (lane < nch) {
-    double a = acc;
-    double *row = T[lane];
+  double a = acc, e = acc_e;
+  int i = i0;
+  for (; i + 16 <= i1; i += 16) {
+    double t[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) t[k] = row[i + k];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      a += t[k];
+      t[k] = a;
+      if (WITH_E) e += a - phi0;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) row[i + k] = t[k];
+  }
+  for (; i < i1; i++) {
+    a += row[i];
+    row[i] = a;
+    if (WITH_E) e += a - phi0;
+  }
+  acc = a;
+  acc_e = e;
+}
+
+/* one running sum per lane that has a row (row != nullptr) over the cnt
+ * staged terms: each term is replaced by the running value; at every member
+ * of the mask cm the value is recorded for the bins it closes, in column
+ * col (col < 0: not recorded), and with WITH_E the potential energy in
+ * column ecol */
+template <bool WITH_E>
+__device__ inline void chain_pass(double *row, int cnt, uint64_t cm, ProfShared &S, double &acc, double &acc_e,
+                                  double phi0, int col, int ecol)
+{
+  if (row) {
     int i = 0;
     uint64_t m = cm;
     while (m) {
       const int e = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
-      for (; i <= e; i++) {
-        a += row[i];
-        row[i] = a;
+      chain_run<WITH_E>(row, i, e + 1, acc, acc_e, phi0);
+      i = e + 1;
+      for (int b = S.bprev[e]; b < S.bcur[e]; b++) {
+        if (col >= 0) S.bt[b][col] = acc;
+        if (WITH_E && ecol >= 0) S.bt[b][ecol] = acc_e;
       }
-      if (col >= 0)
-        for (int b = S.bprev[e]; b < S.bcur[e]; b++) S.bt[b][col] = a;
     }
-    for (; i < cnt; i++) {
-      a += row[i];
-      row[i] = a;
-    }
-    acc = a;
+    chain_run<WITH_E>(row, i, cnt, acc, acc_e, phi0);
   }
   __syncthreads();
 }
@@ -256,44 +282,99 @@ __device__ inline int wave_prefix_max(int v)
   return v;
 }
 
-/* the first maximum of y[0..m-2] above -10 as find_max's left pass takes
- * it (general.c:628-637), m-1 when there is none; its right pass cannot
- * move it (the range it scans lies inside the first) */
-__device__ int64_t wave_first_max(const double *y, int64_t m)
+/* y[t] after PASSES passes of smooth3 (general.c:548-566) over y[0..m),
+ * m >= 3, from the raw values y[t-PASSES..t+PASSES]: each pass averages
+ * three neighbours, two at either end, with smooth3's operand order */
+template <int PASSES>
+__device__ inline double smoothed_at(const double *__restrict__ y, int64_t m, int64_t t)
 {
-  const int lane = threadIdx.x;
-  double bv = -10.0;
-  int64_t bi = -1;
-  for (int64_t i = lane; i < m - 1; i += 64) {
-    const double v = y[i];
-    if (v > bv) {
-      bv = v;
-      bi = i;
-    }
+  double w[2 * PASSES + 1];
+#pragma unroll
+  for (int k = -PASSES; k <= PASSES; k++) {
+    const int64_t i = t + k;
+    w[k + PASSES] = (i >= 0 && i < m) ? y[i] : 0.0;
   }
+#pragma unroll
+  for (int p = 1; p <= PASSES; p++) {
+    double nw[2 * PASSES + 1];
+#pragma unroll
+    for (int k = -(PASSES - p); k <= PASSES - p; k++) {
+      const int64_t i = t + k;
+      const int c = k + PASSES;
+      double v = 0.0;
+      if (i == 0) v = (w[c] + w[c + 1]) / 2.;
+      else if (i == m - 1) v = (w[c] + w[c - 1]) / 2.;
+      else if (i > 0 && i < m - 1) v = (w[c - 1] + w[c] + w[c + 1]) / 3.;
+      nw[c] = v;
+    }
+#pragma unroll
+    for (int k = -(PASSES - p); k <= PASSES - p; k++) w[k + PASSES] = nw[k + PASSES];
+  }
+  return w[PASSES];
+}
+
+/* (value, index) of a lane against another: find_max's left pass keeps the
+ * first maximum above -10 (general.c:628-637) */
+__device__ inline void first_max_merge(double &bv, int64_t &bi, double ov, int64_t oi)
+{
+  if (oi >= 0 && (bi < 0 || ov > bv || (!(bv > ov) && oi < bi))) {
+    bv = ov;
+    bi = oi;
+  }
+}
+
+__device__ inline int64_t wave_first_max_reduce(double bv, int64_t bi, int64_t m)
+{
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     const double ov = __shfl_xor(bv, d, 64);
     const int64_t oi = __shfl_xor(bi, d, 64);
-    if (oi >= 0 && (bi < 0 || ov > bv || (!(bv > ov) && oi < bi))) {
-      bv = ov;
-      bi = oi;
-    }
+    first_max_merge(bv, bi, ov, oi);
   }
   return bi >= 0 ? bi : m - 1;
 }
 
-/* smooth3 (general.c:548-566), one pass from src into dst (m >= 3) */
-__device__ void smooth_pass(const double *src, double *dst, int64_t m)
+/* r2's and R_max's peaks in one pass over the members from NIGNORE on:
+ * the first maxima over [0, m-2] of rho r^2 after three smoothing passes
+ * and of V_c^2 after one (find_max's right pass cannot move a maximum of
+ * its left pass: the range it scans lies inside the left pass's) */
+__device__ void find_peaks(const double *__restrict__ dr2, const double *__restrict__ vc2, int64_t m, int64_t *i_r2,
+                           int64_t *i_rmax)
 {
-  for (int64_t i = threadIdx.x; i < m; i += 64) {
-    double v;
-    if (i == 0) v = (src[0] + src[1]) / 2.;
-    else if (i == m - 1) v = (src[m - 1] + src[m - 2]) / 2.;
-    else v = (src[i - 1] + src[i] + src[i + 1]) / 3.;
-    dst[i] = v;
+  const int lane = threadIdx.x;
+  double bv3 = -10.0, bv1 = -10.0;
+  int64_t bi3 = -1, bi1 = -1;
+  for (int64_t t0 = 0; t0 < m - 1; t0 += 4 * 64) {
+    double v3[4], v1[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t t = t0 + u * 64 + lane;
+      const bool ok = t < m - 1;
+      const int64_t tt = ok ? t : 0;
+      if (m >= 3) {
+        v3[u] = smoothed_at<3>(dr2, m, tt);
+        v1[u] = smoothed_at<1>(vc2, m, tt);
+      } else {
+        v3[u] = dr2[tt];
+        v1[u] = vc2[tt];
+      }
+      if (!ok) v3[u] = v1[u] = -10.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t t = t0 + u * 64 + lane;
+      if (v3[u] > bv3) {
+        bv3 = v3[u];
+        bi3 = t;
+      }
+      if (v1[u] > bv1) {
+        bv1 = v1[u];
+        bi1 = t;
+      }
+    }
   }
-  __syncthreads();
+  *i_r2 = wave_first_max_reduce(bv3, bi3, m);
+  *i_rmax = wave_first_max_reduce(bv1, bi1, m);
 }
 
 /* ------------------------------------------------------------- kernel */
@@ -308,7 +389,7 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
                                                  const float *__restrict__ pz, const float *__restrict__ vx,
                                                  const float *__restrict__ vy, const float *__restrict__ vz,
                                                  double *__restrict__ s_rad, double *__restrict__ s_vc2,
-                                                 double *__restrict__ s_dr2, double *__restrict__ s_tmp,
+                                                 double *__restrict__ s_dr2,
                                                  double *__restrict__ props, double *__restrict__ prof)
 {
   __shared__ ProfShared S;
@@ -332,7 +413,7 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
     R_vir = phi_rvir[2 * h + 1];
   }
   const double Xc = cen[3 * h], Yc = cen[3 * h + 1], Zc = cen[3 * h + 2];
-  double *rad_m = s_rad + base, *vc2 = s_vc2 + base, *dr2 = s_dr2 + base, *tmp = s_tmp + base;
+  double *rad_m = s_rad + base, *vc2 = s_vc2 + base, *dr2 = s_dr2 + base;
   const double FourPiThird = 4. * PRF_PI / 3.;
 
   /* binning_parameter (specific.c:259-319): the first member from
@@ -364,22 +445,51 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
   __syncthreads();
   const bool mono = __ballot(lane < nbins - 1 && !(S.rad[lane] <= S.rad[lane + 1])) == 0;
 
-  /* the walk */
-  double accA = 0.0, accC = 0.0, accE = 0.0;
+  /* the walk: per chunk, the bulk velocity's running sums (lanes 0-2),
+   * then the terms about it, then every other running sum in one pass:
+   * angular momentum (lanes 0-2), kinetic energy (3), the potential and
+   * with it the potential energy (4), centre of mass (5-7), inertia tensor
+   * (8-13) */
+  double accV = 0.0, accC = 0.0, accE = 0.0, dummy = 0.0;
   double I_car = 0.0, d_car = 0.0, Emin = 1e30;
   int64_t mb = -1, P = 0;
   int bcar = 0;
   const double hub = K.hubble, rf = K.r_fac, vf = K.v_fac;
+  double *const rowV = lane < 3 ? S.ta[9 + lane] : nullptr;
+  double *const rowC = lane < 5 ? S.tc[lane] : lane < 14 ? S.ta[lane - 5] : nullptr;
+  const int colC = lane < 4 ? BT_LX + lane : (lane >= 8 && lane < 14) ? BT_A11 + lane - 8 : -1;
+  const int ecolC = lane == 4 ? BT_EP : -1;
+  /* the members of the next chunk are read while the current one is
+   * summed, their indices one chunk earlier still */
+  float nx = 0.f, ny = 0.f, nz = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
+  if (lane < n) {
+    const uint32_t q = ip[lane];
+    nx = px[q];
+    ny = py[q];
+    nz = pz[q];
+    nvx = vx[q];
+    nvy = vy[q];
+    nvz = vz[q];
+  }
+  uint32_t qn = 64 + lane < n ? ip[64 + lane] : 0;
   for (int64_t j0 = 0; j0 < n && bcar < nbins; j0 += 64) {
     const int64_t j = j0 + lane;
     const bool inb = j < n;
     double dX = 0.0, dY = 0.0, dZ = 0.0, d = 0.0, vxp = 0.0, vyp = 0.0, vzp = 0.0;
-    uint32_t p = 0;
+    const float cx = nx, cy = ny, cz = nz, cvx = nvx, cvy = nvy, cvz = nvz;
+    if (j + 64 < n) {
+      nx = px[qn];
+      ny = py[qn];
+      nz = pz[qn];
+      nvx = vx[qn];
+      nvy = vy[qn];
+      nvz = vz[qn];
+    }
+    qn = j + 128 < n ? ip[j + 128] : 0;
     if (inb) {
-      p = ip[j];
-      dX = (double)px[p] - Xc;
-      dY = (double)py[p] - Yc;
-      dZ = (double)pz[p] - Zc;
+      dX = (double)cx - Xc;
+      dY = (double)cy - Yc;
+      dZ = (double)cz - Zc;
       if (dX > 0.5) dX -= 1.0;
       if (dY > 0.5) dY -= 1.0;
       if (dZ > 0.5) dZ -= 1.0;
@@ -387,24 +497,21 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
       if (dY < -0.5) dY += 1.0;
       if (dZ < -0.5) dZ += 1.0;
       d = sqrt(dX * dX + dY * dY + dZ * dZ);
-      vxp = (double)vx[p];
-      vyp = (double)vy[p];
-      vzp = (double)vz[p];
+      vxp = (double)cvx;
+      vyp = (double)cvy;
+      vzp = (double)cvz;
     }
     /* bins closed after each member */
     int bp, bi;
     if (mono) {
-      int c = 0;
+      /* the bins from the current one on whose radius d reaches (with
+       * radii that do not decrease, those below it are closed already) */
+      int c = bcar;
       if (inb) {
-        int lo = 0, hi = nbins; /* count of radii <= d */
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (S.rad[mid] <= d) lo = mid + 1;
-          else hi = mid;
-        }
-        c = j == n - 1 ? nbins : lo;
+        while (c < nbins && S.rad[c] <= d) c++;
+        if (j == n - 1) c = nbins;
       }
-      bi = max(wave_prefix_max(c), bcar);
+      bi = __ballot(c > bcar) ? wave_prefix_max(c) : bcar;
       bp = __shfl_up(bi, 1, 64);
       if (lane == 0) bp = bcar;
     } else {
@@ -446,8 +553,8 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
       rad_m[j] = d;
     }
     __syncthreads();
-    /* centre of mass, inertia tensor, bulk velocity */
-    chain_pass(S.ta, 12, cnt, cm, S, accA, lane >= 3 && lane < 9 ? BT_A11 + lane - 3 : -1);
+    /* bulk velocity */
+    chain_pass<false>(rowV, cnt, cm, S, accV, dummy, 0.0, -1, -1);
 
     /* angular momentum and kinetic energy terms about the running bulk
      * velocity; the potential integral's term (ahf_halos.c:4333-4395) */
@@ -483,17 +590,16 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
     I_car = __shfl(I_now, cnt - 1, 64);
     d_car = __shfl(d, cnt - 1, 64);
     __syncthreads();
-    /* angular momentum, kinetic energy, potential */
-    chain_pass(S.tc, 5, cnt, cm, S, accC, lane < 4 ? BT_LX + lane : -1);
+    /* angular momentum, kinetic energy, potential and potential energy,
+     * centre of mass, inertia tensor */
+    chain_pass<true>(rowC, cnt, cm, S, accC, accE, Phi0, colC, ecolC);
 
     const double U = S.tc[4][lane] - Phi0;
     const double Ep = 0.5 * T + U;
-    if (proc) {
-      S.te[lane] = U;
+    if (proc)
       for (int b = bp; b < bi; b++) S.bt[b][BT_VESC] = 2 * fabs(U);
-    }
     /* most bound member: the first at the lowest energy below the one so far */
-    {
+    if (__ballot(proc && Ep < Emin)) {
       double bv = proc && Ep < Emin ? Ep : 0.0;
       int bl = proc && Ep < Emin ? lane : 64;
 #pragma unroll
@@ -505,21 +611,18 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
           bl = ol;
         }
       }
-      if (bl < 64) {
-        Emin = bv;
-        mb = j0 + bl;
-      }
+      Emin = bv;
+      mb = j0 + bl;
     }
     __syncthreads();
-    /* potential energy */
-    chain_pass(reinterpret_cast<double(*)[PRF_ROW]>(S.te), 1, cnt, cm, S, accE, BT_EP);
     bcar = __shfl(bi, cnt - 1, 64);
     P += cnt;
     if (cnt < 64) break;
   }
   /* members the walk did not reach read as AHF's zeroed arrays */
   for (int64_t j = P + lane; j < n; j += 64) rad_m[j] = vc2[j] = dr2[j] = 0.0;
-  if (lane < 12) S.fin[lane] = accA;
+  if (lane < 3) S.fin[9 + lane] = accV;
+  if (lane >= 5 && lane < 8) S.fin[lane - 5] = accC;
   __syncthreads();
 
   /* the bins */
@@ -558,30 +661,27 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
    * smoothing passes) and of V_c^2 (one pass) */
   const int64_t m = n - PRF_NIGNORE;
   const double *x = rad_m + PRF_NIGNORE;
-  const double *y = dr2 + PRF_NIGNORE;
-  if (m >= 3) {
-    smooth_pass(dr2 + PRF_NIGNORE, tmp, m);
-    smooth_pass(tmp, dr2 + PRF_NIGNORE, m);
-    smooth_pass(dr2 + PRF_NIGNORE, tmp, m);
-    y = tmp;
-  }
-  int64_t i0 = wave_first_max(y, m);
-  const double r2 = (x[i0] + x[i0]) / 2;
-  __syncthreads();
-  y = vc2 + PRF_NIGNORE;
-  if (m >= 3) {
-    smooth_pass(vc2 + PRF_NIGNORE, tmp, m);
-    y = tmp;
-  }
-  i0 = wave_first_max(y, m);
-  const double R_max = (x[i0] + x[i0]) / 2;
+  int64_t i_r2, i_rm;
+  find_peaks(dr2 + PRF_NIGNORE, vc2 + PRF_NIGNORE, m, &i_r2, &i_rm);
+  const double r2 = (x[i_r2] + x[i_r2]) / 2;
+  const double R_max = (x[i_rm] + x[i_rm]) / 2;
   /* the first member at or beyond R_max (ahf_halos.c:4892-4894) */
   int64_t ib = n - 1;
-  for (int64_t j0 = 0; j0 < n - 1; j0 += 64) {
-    const int64_t j = j0 + lane;
-    const uint64_t mk = __ballot(j < n - 1 && !(rad_m[j] < R_max));
-    if (mk) {
-      ib = j0 + __ffsll((unsigned long long)mk) - 1;
+  for (int64_t j0 = 0; j0 < n - 1; j0 += 4 * 64) {
+    bool hit[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t j = j0 + u * 64 + lane;
+      hit[u] = j < n - 1 && !(rad_m[j] < R_max);
+    }
+    int64_t found = -1;
+#pragma unroll
+    for (int u = 3; u >= 0; u--) {
+      const uint64_t mk = __ballot(hit[u]);
+      if (mk) found = j0 + u * 64 + __ffsll((unsigned long long)mk) - 1;
+    }
+    if (found >= 0) {
+      ib = found;
       break;
     }
   }
@@ -716,7 +816,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
   int32_t *d_nb = A.get<int32_t>(nh);
   int64_t *d_pofs = A.get<int64_t>(nh);
   double *d_pr = phi0_rvir ? A.get<double>(2 * nh) : nullptr;
-  double *sc = A.get<double>(4 * ne);
+  double *sc = A.get<double>(3 * ne);
   if (!d_n || !d_nb || !d_pofs || !sc || (phi0_rvir && !d_pr))
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (profiles)");
   std::vector<int64_t> hn((size_t)nh), pofs((size_t)nh);
@@ -758,7 +858,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     hipEvent_t e0 = c->timer_begin();
     hipLaunchKernelGGL(k_profiles, dim3((unsigned)nh), dim3(64), 0, c->st, K, s->off, s->order, out, d_pr, s->ip_in,
                        s->buf_a, s->buf_b, s->cen, d_nb, d_pofs, c->px, c->py, c->pz, c->vx, c->vy, c->vz, sc,
-                       sc + ne, sc + 2 * ne, sc + 3 * ne, r->props, r->prof);
+                       sc + ne, sc + 2 * ne, r->props, r->prof);
     c->timer_end(e0, TIMER_PROFILES, 0.0);
   }
   c->timers_reset(TIMER_PROFILES, TIMER_PROFILES);

@@ -145,6 +145,16 @@ def unbinding_workload(seed=1002, reps=3, cpu=True, threads=16):
     for _ in range(reps):
         info = ctx.unbind(*args)
         ms.append(ctx.last_timing()["unbind_ms"])
+    # the construction steps of ahf_halos_sfc_constructHalo on the same lists
+    # (sort, rem_outsideRvir, rem_unbound, rem_outsideRvir), then HaloProfiles
+    pargs = (s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble)
+    kw = dict(mode=ctx.UNBIND_WITH_RVIR, ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+    cms, pms = [], []
+    for _ in range(reps):
+        ctx.unbind(*args, **kw)
+        cms.append(ctx.last_timing()["unbind_ms"])
+        nb, _, _ = ctx.halo_profiles(*pargs)
+        pms.append(ctx.timers()["halo_profiles"][0])
     ctx.close()
     t = float(np.median(ms))
     out = {"workload": f"{s.nhalos} isolated NFW haloes, {s.n_ipart} members (largest {int(s.halo_npart.max())}), "
@@ -152,7 +162,9 @@ def unbinding_workload(seed=1002, reps=3, cpu=True, threads=16):
            "gpu_ms": t, "members_per_s": s.n_ipart / (t * 1e-3),
            "particle_iterations_per_s": info.particle_iterations / (t * 1e-3),
            "halos_per_s": s.nhalos / (t * 1e-3), "exact_replays": int(info.n_exact_replays),
-           "max_iterations": int(info.max_iterations)}
+           "max_iterations": int(info.max_iterations),
+           "construct_ms": float(np.median(cms)), "profiles_ms": float(np.median(pms)),
+           "profiled_halos": int((nb > 0).sum())}
     if cpu and os.path.exists(REF_BIN):
         r = subprocess.run([REF_BIN, "time", snap, "--threads", str(threads), "--skip-amr"],
                            capture_output=True, text=True, timeout=900)
@@ -162,16 +174,23 @@ def unbinding_workload(seed=1002, reps=3, cpu=True, threads=16):
             out["cpu_reference_1thread_ms"] = 1e3 * rb.get("unbind_seconds_1thread", 0.0)
             out["cpu_threads"] = threads
             out["vs_cpu"] = out["cpu_reference_ms"] / t
+        with tempfile.TemporaryDirectory() as d:
+            r = subprocess.run([REF_BIN, "profiles", snap, os.path.join(d, "h.prof"), "--threads", str(threads)],
+                               capture_output=True, text=True, timeout=900)
+        if r.returncode == 0:
+            rb = json.loads(r.stdout.strip().splitlines()[-1])
+            out["cpu_reference_construct_ms"] = 1e3 * rb["construct_seconds"]
+            out["cpu_reference_profiles_ms"] = 1e3 * rb["profiles_seconds"]
     return out
 
 
 def construction_workload(snap, reps=3, cpu=True, threads=16):
-    """The halo construction of ahf_halos_sfc_constructHalo but the profiles,
-    for every halo of the box workload: gatherParts from each centre and
-    gather radius, sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1),
+    """The halo construction of ahf_halos_sfc_constructHalo for every halo of
+    the box workload: gatherParts from each centre and gather radius, sort,
+    rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1), then HaloProfiles,
     with the member lists resident on the device throughout.  The reference's
-    own five functions per halo (OpenMP over haloes) are timed on the same
-    file."""
+    own six functions per halo (OpenMP over haloes) are timed on the same
+    file, the profiles apart."""
     import numpy as np
 
     import ahf_amd
@@ -186,35 +205,43 @@ def construction_workload(snap, reps=3, cpu=True, threads=16):
     kw = dict(mode=ctx.UNBIND_WITH_RVIR, ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
     args = (s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble)
 
+    pargs = (s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble)
+
     def run():
         ctx.gather_halos(s.centre, s.gather_rad, skip)
         gathered = int(ctx._hnp.sum())
         info = ctx.unbind(*args, **kw)
-        return gathered, info
+        t_u = ctx.last_timing()["unbind_ms"]
+        nb, _, _ = ctx.halo_profiles(*pargs)
+        return gathered, info, t_u, nb
 
     run()
-    g_ms, u_ms, wall = [], [], []
+    g_ms, u_ms, p_ms, wall = [], [], [], []
     for _ in range(reps):
         t0 = time.perf_counter()
-        gathered, info = run()
+        gathered, info, t_u, nb = run()
         wall.append(1e3 * (time.perf_counter() - t0))
         g_ms.append(ctx.timers()["halo_gather"][0])
-        u_ms.append(ctx.last_timing()["unbind_ms"])
+        u_ms.append(t_u)
+        p_ms.append(ctx.timers()["halo_profiles"][0])
     ctx.close()
-    g, u = float(np.median(g_ms)), float(np.median(u_ms))
+    g, u, pm = float(np.median(g_ms)), float(np.median(u_ms)), float(np.median(p_ms))
     out = {"workload": f"{s.nhalos} haloes of the box workload gathered from their centres and gather radii",
            "gathered_members": gathered, "final_members": int(info.n_member_out),
-           "gather_ms": g, "construct_ms": u, "gpu_ms": g + u, "wall_ms": float(np.median(wall)),
-           "keys_ms": keys_ms, "halos_per_s": s.nhalos / ((g + u) * 1e-3)}
+           "profiled_halos": int((nb > 0).sum()), "profile_bins": int(nb.sum()),
+           "gather_ms": g, "construct_ms": u, "profiles_ms": pm, "gpu_ms": g + u + pm,
+           "wall_ms": float(np.median(wall)), "keys_ms": keys_ms, "halos_per_s": s.nhalos / ((g + u + pm) * 1e-3)}
     if cpu and os.path.exists(REF_BIN):
         with tempfile.TemporaryDirectory() as d:
-            r = subprocess.run([REF_BIN, "construct-sfc", snap, os.path.join(d, "h.halo"), "--threads", str(threads)],
+            r = subprocess.run([REF_BIN, "profiles-sfc", snap, os.path.join(d, "h.prof"), "--threads", str(threads)],
                                capture_output=True, text=True, timeout=900)
         if r.returncode == 0:
             rb = json.loads(r.stdout.strip().splitlines()[-1])
-            out["cpu_reference_ms"] = 1e3 * rb["gather_seconds"]
+            out["cpu_reference_construct_ms"] = 1e3 * rb["construct_seconds"]
+            out["cpu_reference_profiles_ms"] = 1e3 * rb["profiles_seconds"]
+            out["cpu_reference_ms"] = out["cpu_reference_construct_ms"] + out["cpu_reference_profiles_ms"]
             out["cpu_threads"] = threads
-            out["vs_cpu"] = out["cpu_reference_ms"] / (g + u)
+            out["vs_cpu"] = out["cpu_reference_ms"] / (g + u + pm)
     return out
 
 
@@ -407,6 +434,7 @@ def main():
     if not args.no_construct_workload:
         cw = construction_workload(snap, cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
         log(f"construction workload: gather {cw['gather_ms']:.2f} ms + construct {cw['construct_ms']:.2f} ms "
+            f"+ profiles {cw['profiles_ms']:.2f} ms "
             f"on the GPU ({cw['gathered_members']} gathered, {cw['final_members']} kept)"
             + (f", reference CPU {cw['cpu_reference_ms']:.1f} ms" if "cpu_reference_ms" in cw else ""))
         out["construction_workload"] = cw
@@ -414,6 +442,9 @@ def main():
         u = unbinding_workload(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
         log(f"unbinding workload: {u['gpu_ms']:.2f} ms on the GPU"
             + (f", reference CPU {u['cpu_reference_ms']:.1f} ms" if "cpu_reference_ms" in u else ""))
+        log(f"  constructed (cuts + unbinding) {u['construct_ms']:.2f} ms, profiles {u['profiles_ms']:.2f} ms on the GPU"
+            + (f"; reference CPU {u['cpu_reference_construct_ms']:.1f} ms + {u['cpu_reference_profiles_ms']:.1f} ms"
+               if "cpu_reference_profiles_ms" in u else ""))
         out["unbinding_workload"] = u
     print(json.dumps(out))
 

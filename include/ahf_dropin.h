@@ -14,7 +14,7 @@
  *          ovlim, rho_vir
  *   global_info  fst_part, no_part (the same array as global's, ordered by
  *          part.sfckey, src/main.c:341-356, 624-625)
- *   r_fac, v_fac, phi_fac, rho_fac, Hubble
+ *   r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac
  *
  * Errors: like the reference, the grid functions terminate the process with
  * a message on stderr when they cannot continue (device missing, allocation
@@ -32,7 +32,7 @@ extern "C" {
 extern param_simu simu;
 extern info_global global;
 extern global_info_t global_info;
-extern double r_fac, v_fac, phi_fac, rho_fac, Hubble;
+extern double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac;
 
 /* src/libamr_serial/generate_grids.h:7 -- the domain grid, with AHF's quads */
 gridls *gen_domgrids(int *no_grids);
@@ -79,6 +79,21 @@ void rem_outsideRvir(HALO *halo, int icall);
  * which must be in key order.  Thread-safe like rem_unbound. */
 void ahf_halos_sfc_gatherParts(HALO *halo);
 
+/* src/libahf/ahf_halos.h:15 -- radial profiles and integral properties of
+ * a constructed halo (members in radial order, Phi0 and R_vir set), for
+ * AHF's default flags; halo->prof's arrays are calloc()ed as c_profile
+ * does.  Does nothing below simu.AHF_MINPART members; returns TRUE.
+ * Thread-safe like rem_unbound. */
+int HaloProfiles(HALO *halo);
+/* No counterpart in the reference: HaloProfiles for all nhalos haloes in one
+ * GPU pass.  Returns 0, or a negative AHFGPU_E* code. */
+int ahf_halo_profiles(HALO *halos, long unsigned nhalos);
+/* No counterpart in the reference: the whole of
+ * ahf_halos_sfc_constructHalo (src/libahf/ahf_halos_sfc.c:119-157) for all
+ * nhalos haloes in one batched GPU pass, HaloProfiles included: what
+ * ahf_gather_construct_halos does, then the profiles of the lists the
+ * device holds.  Returns 0, or a negative AHFGPU_E* code. */
+int ahf_construct_profile_halos(HALO *halos, long unsigned nhalos);
 /* No counterpart in the reference: the whole of
  * ahf_halos_sfc_constructHalo (src/libahf/ahf_halos_sfc.c:119-157) but
  * HaloProfiles, for all nhalos haloes in one batched GPU pass: haloes with

@@ -27,6 +27,9 @@
  *                       the construct steps + HaloProfiles, per halo
  *   ahf_ref profiles-sfc <snap> <prof.out> [--threads T]
  *                       gatherParts + the construct steps + HaloProfiles
+ *   (drop-in build only) unbind-batch, construct-batch, construct-sfc-batch,
+ *   profiles-batch, profiles-sfc-batch <snap> <out>: the library's batched
+ *                       replacements of the per-halo loop
  *   ahf_ref keys <snap> <keys.out> <bits>          SFC key of every particle
  *   ahf_ref time   <snap> [--threads T] [--reps R] [--skip-amr] [--skip-1thread]
  *   ahf_ref layout                                  struct sizes/offsets (JSON)
@@ -468,6 +471,8 @@ static void cmd_profiles(const snap_data *s, const char *out, int threads, int g
   const uint64_t nh = s->h.nhalos;
   double t0 = now_s();
   long i;
+  /* the construction of every halo, then the profiles of every halo: per
+   * halo the same calls in the same order as one loop, timed apart */
 #pragma omp parallel for schedule(dynamic) private(i)
   for (i = 0; i < (long)nh; i++) {
     HALO *x = h + i;
@@ -485,11 +490,17 @@ static void cmd_profiles(const snap_data *s, const char *out, int threads, int g
     rem_outsideRvir(x, 0);
     rem_unbound(x);
     rem_outsideRvir(x, 1);
-    HaloProfiles(x);
   }
-  double t = now_s() - t0;
+  double t1 = now_s();
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++) {
+    if (gather && s->halo_npart[i] == 0) continue;
+    HaloProfiles(h + i);
+  }
+  double t2 = now_s();
   dump_profiles(out, h, nh);
-  fprintf(stdout, "{\"profiles_seconds\": %.6f, \"threads\": %d}\n", t, threads);
+  fprintf(stdout, "{\"construct_seconds\": %.6f, \"profiles_seconds\": %.6f, \"threads\": %d}\n", t1 - t0,
+          t2 - t1, threads);
 }
 
 #ifndef AHF_REF_BUILD
@@ -498,16 +509,24 @@ static void cmd_batch(const snap_data *s, const char *out, int construct)
 {
   HALO *h = halo_setup(s);
   double t0 = now_s();
-  if (construct == 2) {
+  if (construct == 2 || construct == 3) {
     setup_sfc(s);
     for (uint64_t i = 0; i < s->h.nhalos; i++) h[i].gatherRad = s->gather_rad[i];
   }
-  const int rc = construct == 2 ? ahf_gather_construct_halos(h, s->h.nhalos)
-                 : construct    ? ahf_construct_halos(h, s->h.nhalos)
-                                : ahf_unbind_halos(h, s->h.nhalos);
+  int rc;
+  if (construct == 4) {
+    rc = ahf_construct_halos(h, s->h.nhalos);
+    if (!rc) rc = ahf_halo_profiles(h, s->h.nhalos);
+  } else {
+    rc = construct == 3 ? ahf_construct_profile_halos(h, s->h.nhalos)
+         : construct == 2 ? ahf_gather_construct_halos(h, s->h.nhalos)
+         : construct    ? ahf_construct_halos(h, s->h.nhalos)
+                        : ahf_unbind_halos(h, s->h.nhalos);
+  }
   if (rc) die("batched halo pass failed");
   double t = now_s() - t0;
-  dump_halos(out, h, s->h.nhalos);
+  if (construct >= 3) dump_profiles(out, h, s->h.nhalos);
+  else dump_halos(out, h, s->h.nhalos);
   fprintf(stdout, "{\"batch_seconds\": %.6f}\n", t);
 }
 #endif
@@ -621,9 +640,13 @@ int main(int argc, char **argv)
     cmd_keys(&s, argv[3], (unsigned)atoi(argv[4]));
 #ifndef AHF_REF_BUILD
   } else if (!strcmp(argv[1], "unbind-batch") || !strcmp(argv[1], "construct-batch") ||
-             !strcmp(argv[1], "construct-sfc-batch")) {
+             !strcmp(argv[1], "construct-sfc-batch") || !strcmp(argv[1], "profiles-sfc-batch") ||
+             !strcmp(argv[1], "profiles-batch")) {
     if (argc < 4) die("a batched command needs an output path");
-    cmd_batch(&s, argv[3], !strcmp(argv[1], "construct-sfc-batch") ? 2 : !strcmp(argv[1], "construct-batch"));
+    cmd_batch(&s, argv[3], !strcmp(argv[1], "profiles-batch")       ? 4
+                           : !strcmp(argv[1], "profiles-sfc-batch")  ? 3
+                           : !strcmp(argv[1], "construct-sfc-batch") ? 2
+                                                                     : !strcmp(argv[1], "construct-batch"));
 #endif
   } else if (!strcmp(argv[1], "time")) {
     cmd_time(&s, threads, reps < 1 ? 1 : reps, skip_amr, skip_1thread);

@@ -17,7 +17,7 @@ import pytest
 
 from ahf_amd import dumps
 
-from helpers import fuzz_snapshot
+from helpers import fuzz_snapshot, profiles_match
 
 pytestmark = pytest.mark.gpu
 
@@ -126,3 +126,22 @@ def test_dropin_gather(oracle, drv, snapgen, tmp_path, cmd, cfg, side, seed):
     extra = [] if cmd.endswith("batch") else ["--threads", "4"]
     _run([drv, cmd, snap, d_out, *extra])
     _halo_match(dumps.read_halos(o_out), dumps.read_halos(d_out), f"{cmd} cfg{cfg}/{side}")
+
+
+@pytest.mark.parametrize("cmd", ["profiles", "profiles-sfc", "profiles-batch", "profiles-sfc-batch"])
+@pytest.mark.parametrize("cfg,side,seed", [(2, 400, 1002), (3, 48, 5)])
+def test_dropin_profiles(oracle, drv, snapgen, tmp_path, cmd, cfg, side, seed):
+    """HaloProfiles per halo through AHF's prototype after the construction
+    steps (from the given lists, or gathered), or all of it batched"""
+    if cmd.endswith("batch") and drv == LINKED:
+        pytest.skip("the reference's driver has no batched command")
+    snap = str(tmp_path / "p.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    o_out, d_out = str(tmp_path / "o.prof"), str(tmp_path / "d.prof")
+    gather = 1 if "sfc" in cmd else 0
+    assert oracle.oracle_profiles_dump(snap.encode(), o_out.encode(), 4, gather) == 0
+    extra = [] if cmd.endswith("batch") else ["--threads", "4"]
+    _run([drv, cmd, snap, d_out, *extra])
+    ref, got = dumps.read_profiles(o_out), dumps.read_profiles(d_out)
+    assert np.array_equal(ref.npart, got.npart)
+    profiles_match(ref, got.nbins, got.props, got.prof, f"{cmd} cfg{cfg}/{side}", end_to_end=True)

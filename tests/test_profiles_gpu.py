@@ -16,16 +16,9 @@ import pytest
 
 from ahf_amd import dumps
 
+from helpers import profiles_match as _compare
+
 pytestmark = pytest.mark.gpu
-
-# values computed from the bin radii or through log(): device libm
-LIBM_BINS = {"r", "ovdens", "dens", "v2_circ"}
-LIBM_PROPS = {"R1", "SurfP"}
-BISECT_PROPS = {"cNFW", "cR1"}
-# values built on Phi0 (end to end only)
-PHI_BINS = {"v_esc2", "Epot"}
-PHI_PROPS = {"v_esc2", "Epot", "lambdaE"}
-
 
 @pytest.fixture(scope="module")
 def ctx():
@@ -63,47 +56,6 @@ def _oracle_profile(oracle, s, lists, phi_rvir, minpart=None):
         nb[h] = k
         prof.append(pr[:k].copy())
     return dumps.ProfileResult(np.array([len(x) for x in lists], np.uint64), nb, props, prof)
-
-
-def _close(a, b, rtol):
-    a, b = np.asarray(a), np.asarray(b)
-    same = (a == b) | (np.isnan(a) & np.isnan(b))
-    with np.errstate(invalid="ignore"):
-        near = np.abs(a - b) <= rtol * np.maximum(np.abs(a), np.abs(b))
-    return bool(np.all(same | near))
-
-
-def _compare(ref, nb, props, prof, tag, end_to_end=False):
-    """exact where the inputs fix the bits; the tolerances above elsewhere"""
-    assert np.array_equal(ref.nbins, nb), tag
-    assert int((nb > 0).sum()) > 0, tag
-    exact_bits, total = 0, 0
-    for j, name in enumerate(dumps.PROF_PROPS):
-        a, b = ref.props[:, j], props[:, j]
-        total += a.size
-        exact_bits += int(np.sum((a == b) | (np.isnan(a) & np.isnan(b))))
-        if name in BISECT_PROPS:
-            assert np.all((a == b) | (np.abs(a - b) <= 1e-3)), f"{tag}: {name}"
-        elif name in LIBM_PROPS:
-            assert _close(a, b, 1e-12), f"{tag}: {name}"
-        elif end_to_end and (name in PHI_PROPS or name.startswith(("pos_mbp", "vel_mbp", "mbp"))):
-            if name in PHI_PROPS:
-                assert _close(a, b, 1e-8), f"{tag}: {name}"
-        else:
-            assert _close(a, b, 0.0), f"{tag}: {name} differs in {int(np.sum(a != b))} haloes"
-    for h in range(len(nb)):
-        x, y = ref.prof[h], prof[h]
-        assert x.shape == y.shape, (tag, h)
-        for j, name in enumerate(dumps.PROF_BINS):
-            total += x.shape[0]
-            exact_bits += int(np.sum((x[:, j] == y[:, j]) | (np.isnan(x[:, j]) & np.isnan(y[:, j]))))
-            if name in LIBM_BINS:
-                assert _close(x[:, j], y[:, j], 1e-12), f"{tag}: halo {h} {name}"
-            elif end_to_end and name in PHI_BINS:
-                assert _close(x[:, j], y[:, j], 1e-8), f"{tag}: halo {h} {name}"
-            else:
-                assert _close(x[:, j], y[:, j], 0.0), f"{tag}: halo {h} bin value {name}"
-    print(f"[profiles-gpu] {tag}: {exact_bits}/{total} values bit-exact", flush=True)
 
 
 def _constructed(oracle, snap, tmp_path, gather):
