@@ -45,11 +45,13 @@ def build_gpu(verbose=False):
         obj = os.path.join(LIB, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
         objs.append(obj)
-    dropin = os.path.join(CSRC, "dropin.c")
-    if os.path.exists(dropin):
-        obj = os.path.join(LIB, "dropin.o")
+    for csrc in ("dropin.c", "halo_io.c"):
+        path = os.path.join(CSRC, csrc)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(LIB, csrc.replace(".c", ".o"))
         _run(["gcc", "-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
-              "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c", dropin, "-o", obj])
+              "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c", path, "-o", obj])
         objs.append(obj)
     out = os.path.join(LIB, "libahf_amd.so")
     tmp = out + ".tmp"

@@ -23,7 +23,7 @@
 __attribute__((weak)) param_simu simu;
 __attribute__((weak)) info_global global;
 __attribute__((weak)) global_info_t global_info;
-__attribute__((weak)) double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac;
+__attribute__((weak)) double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac, x_fac;
 
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static ahfgpu_ctx *g_ctx;

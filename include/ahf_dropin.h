@@ -14,7 +14,7 @@
  *          ovlim, rho_vir
  *   global_info  fst_part, no_part (the same array as global's, ordered by
  *          part.sfckey, src/main.c:341-356, 624-625)
- *   r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac
+ *   r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac, x_fac (the writers)
  *
  * Errors: like the reference, the grid functions terminate the process with
  * a message on stderr when they cannot continue (device missing, allocation
@@ -32,7 +32,7 @@ extern "C" {
 extern param_simu simu;
 extern info_global global;
 extern global_info_t global_info;
-extern double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac;
+extern double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac, x_fac;
 
 /* src/libamr_serial/generate_grids.h:7 -- the domain grid, with AHF's quads */
 gridls *gen_domgrids(int *no_grids);
@@ -115,6 +115,15 @@ int ahf_construct_halos(HALO *halos, long unsigned nhalos);
  * negative AHFGPU_E* code. */
 int ahf_unbind_halos(HALO *halos, long unsigned nhalos);
 
+/* src/libahf/ahf_io.h -- the ASCII catalogues of AHF's default build:
+ * <fprefix>.AHF_halos (one line per halo of at least simu.AHF_MINPART
+ * members, in the order idx[0..numHalos) gives), .AHF_profiles (its
+ * bins, radii inside the convergence radius written negative) and
+ * .AHF_particles (member ids), byte for byte as AHF writes them.  Host
+ * code; the process exits when a file cannot be opened, as AHF's does. */
+void ahf_io_WriteHalos(const char *fprefix, HALO *halos, unsigned long *idx, int numHalos);
+void ahf_io_WriteProfiles(const char *fprefix, HALO *halos, unsigned long *idx, int numHalos);
+void ahf_io_WriteParticles(const char *fprefix, HALO *halos, unsigned long *idx, int numHalos);
 /* No counterpart in the reference: choose the device (default 0, first use)
  * and release the library's device state. */
 int ahf_gpu_init(int device);

@@ -27,6 +27,10 @@
  *                       the construct steps + HaloProfiles, per halo
  *   ahf_ref profiles-sfc <snap> <prof.out> [--threads T]
  *                       gatherParts + the construct steps + HaloProfiles
+ *   ahf_ref write-from <snap> <halos.dump> <prof.dump> <prefix>
+ *                       AHF's catalogue writers (.AHF_halos, .AHF_profiles,
+ *                       .AHF_particles) on haloes read from an AHFHALO1
+ *                       and an AHFPROF1 dump, in decreasing npart order
  *   (drop-in build only) unbind-batch, construct-batch, construct-sfc-batch,
  *   profiles-batch, profiles-sfc-batch <snap> <out>: the library's batched
  *                       replacements of the per-halo loop
@@ -52,6 +56,7 @@
 #include "libamr_serial/amr_serial.h"
 #include "libahf/ahf_halos.h"
 #include "libahf/ahf_halos_sfc.h"
+#include "libahf/ahf_io.h"
 #include "libsfc/sfc_curve.h"
 #define DRV_SFC_KEY(x, y, z, bits) sfc_curve_calcKey(SFC_CURVE_HILBERT, (x), (y), (z), (bits))
 #define DRV_SFC_BITS BITS_PER_DIMENSION
@@ -73,7 +78,7 @@
 param_simu simu;
 info_global global;
 global_info_t global_info;
-double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac;
+double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac, x_fac;
 #endif
 
 static double now_s(void)
@@ -119,6 +124,7 @@ static void setup_globals(const snap_data *s)
   /* src/libahf/ahf_halos.c:199-206 */
   rho_fac = simu.pmass / (simu.boxsize * simu.boxsize * simu.boxsize);
   m_fac = simu.pmass;
+  x_fac = simu.boxsize;
   global.ovlim = s->h.ovlim;
   global.rho_vir = s->h.rho_vir;
 }
@@ -503,6 +509,106 @@ static void cmd_profiles(const snap_data *s, const char *out, int threads, int g
           t2 - t1, threads);
 }
 
+/* haloes as the construction and HaloProfiles leave them, read back from
+ * an AHFHALO1 dump (members, M_vir, R_vir, Phi0, ovdens) and an AHFPROF1
+ * dump (profiles and integral properties), then written by AHF's writers
+ * (src/libahf/ahf_halos.c:824-858) */
+static uint64_t rd64(FILE *f)
+{
+  uint64_t v;
+  if (fread(&v, 8, 1, f) != 1) die("short dump");
+  return v;
+}
+
+static void rdd(FILE *f, double *v, size_t n)
+{
+  if (n && fread(v, 8, n, f) != n) die("short dump");
+}
+
+static void cmd_write_from(const snap_data *s, const char *hpath, const char *ppath, const char *prefix)
+{
+  setup_globals(s);
+  partptr fst = make_particles(s);
+  global.fst_part = fst;
+  global.no_part = s->h.npart;
+  FILE *fh = fopen(hpath, "rb"), *fp = fopen(ppath, "rb");
+  char magic[8];
+  if (!fh || !fp) die("cannot open dumps");
+  if (fread(magic, 1, 8, fh) != 8 || memcmp(magic, "AHFHALO1", 8)) die("not an AHFHALO1 dump");
+  if (fread(magic, 1, 8, fp) != 8 || memcmp(magic, "AHFPROF1", 8)) die("not an AHFPROF1 dump");
+  const uint64_t nh = rd64(fh);
+  if (rd64(fp) != nh) die("dumps disagree");
+  HALO *h = (HALO *)calloc(nh ? nh : 1, sizeof(HALO));
+  for (uint64_t i = 0; i < nh; i++) {
+    HALO *x = h + i;
+    double v[4];
+    x->npart = rd64(fh);
+    rdd(fh, v, 4);
+    x->M_vir = v[0];
+    x->R_vir = v[1];
+    x->Phi0 = v[2];
+    x->ovdens = v[3];
+    x->ipart = (long unsigned *)malloc((x->npart ? x->npart : 1) * sizeof(long unsigned));
+    for (uint64_t j = 0; j < x->npart; j++) x->ipart[j] = rd64(fh);
+    x->pos.x = s->centre[3 * i];
+    x->pos.y = s->centre[3 * i + 1];
+    x->pos.z = s->centre[3 * i + 2];
+    x->hostHalo = -1;
+    if (rd64(fp) != x->npart) die("dumps disagree on npart");
+    const uint64_t nb = rd64(fp);
+    if (!nb) continue;
+    double P[45];
+    rdd(fp, P, 45);
+    x->M_vir = P[1];
+    x->vel.x = P[2]; x->vel.y = P[3]; x->vel.z = P[4];
+    x->R_max = P[5]; x->V2_max = P[6]; x->r2 = P[7]; x->sigV = P[8]; x->v_esc2 = P[9];
+    x->Ekin = P[10]; x->Epot = P[11];
+    x->axis.x = P[12]; x->axis.y = P[13]; x->axis.z = P[14];
+    x->E1.x = P[15]; x->E1.y = P[16]; x->E1.z = P[17];
+    x->E2.x = P[18]; x->E2.y = P[19]; x->E2.z = P[20];
+    x->E3.x = P[21]; x->E3.y = P[22]; x->E3.z = P[23];
+    x->AngMom.x = P[24]; x->AngMom.y = P[25]; x->AngMom.z = P[26];
+    x->lambda = P[27]; x->lambdaE = P[28]; x->cNFW = P[29]; x->R1 = P[30]; x->cR1 = P[31];
+    x->SurfP = P[32]; x->fMhires = P[33]; x->mbp_offset = P[34];
+    x->pos_mbp.x = P[35]; x->pos_mbp.y = P[36]; x->pos_mbp.z = P[37];
+    x->vel_mbp.x = P[38]; x->vel_mbp.y = P[39]; x->vel_mbp.z = P[40];
+    x->com_offset = P[41];
+    x->pos_com.x = P[42]; x->pos_com.y = P[43]; x->pos_com.z = P[44];
+    HALOPROFILE *q = &x->prof;
+    q->nbins = (int)nb;
+    double **arr[] = {&q->r, &q->nvpart, &q->ovdens, &q->dens, &q->v2_circ, &q->v_esc2, &q->sig_v, &q->Ekin,
+                      &q->Epot, &q->Lx, &q->Ly, &q->Lz, &q->E1x, &q->E1y, &q->E1z, &q->E2x, &q->E2y, &q->E2z,
+                      &q->E3x, &q->E3y, &q->E3z, &q->axis1, &q->axis2, &q->axis3};
+    for (size_t k = 0; k < sizeof(arr) / sizeof(arr[0]); k++) *arr[k] = (double *)calloc(nb, 8);
+    q->npart = (long unsigned *)calloc(nb, sizeof(long unsigned));
+    for (uint64_t b = 0; b < nb; b++) {
+      double r[25];
+      rdd(fp, r, 25);
+      q->r[b] = r[0];
+      q->npart[b] = (long unsigned)r[1];
+      for (int k = 1; k < 24; k++) (*arr[k])[b] = r[k + 1];
+    }
+  }
+  fclose(fh);
+  fclose(fp);
+  /* decreasing npart, ties by index */
+  unsigned long *idx = (unsigned long *)malloc((nh ? nh : 1) * sizeof(unsigned long));
+  for (uint64_t i = 0; i < nh; i++) idx[i] = i;
+  for (uint64_t i = 1; i < nh; i++) {
+    const unsigned long t = idx[i];
+    uint64_t k = i;
+    while (k > 0 && h[idx[k - 1]].npart < h[t].npart) {
+      idx[k] = idx[k - 1];
+      k--;
+    }
+    idx[k] = t;
+  }
+  ahf_io_WriteHalos(prefix, h, idx, (int)nh);
+  ahf_io_WriteProfiles(prefix, h, idx, (int)nh);
+  ahf_io_WriteParticles(prefix, h, idx, (int)nh);
+  fprintf(stdout, "{\"written\": %llu}\n", (unsigned long long)nh);
+}
+
 #ifndef AHF_REF_BUILD
 /* the drop-in library's batched replacements of the per-halo loop */
 static void cmd_batch(const snap_data *s, const char *out, int construct)
@@ -635,6 +741,9 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "profiles") || !strcmp(argv[1], "profiles-sfc")) {
     if (argc < 4) die("profiles needs an output path");
     cmd_profiles(&s, argv[3], threads, !strcmp(argv[1], "profiles-sfc"));
+  } else if (!strcmp(argv[1], "write-from")) {
+    if (argc < 6) die("write-from needs two dumps and a prefix");
+    cmd_write_from(&s, argv[3], argv[4], argv[5]);
   } else if (!strcmp(argv[1], "keys")) {
     if (argc < 5) die("keys needs an output path and a bit count");
     cmd_keys(&s, argv[3], (unsigned)atoi(argv[4]));
