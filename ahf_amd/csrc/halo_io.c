@@ -173,6 +173,8 @@ static FILE *open_out(const char *fprefix, const char *suffix)
   char *name = (char *)malloc(n);
   if (!name) return NULL;
   snprintf(name, n, "%s%s", fprefix, suffix);
+  /* AHF's default build names each file on stderr (VERBOSE, src/define.h:20) */
+  fprintf(stderr, "%s\n", name);
   FILE *f = fopen(name, "w");
   if (!f) {
     fprintf(stderr, "could not open %s\n", name);

@@ -27,6 +27,10 @@
  *                       the construct steps + HaloProfiles, per halo
  *   ahf_ref profiles-sfc <snap> <prof.out> [--threads T]
  *                       gatherParts + the construct steps + HaloProfiles
+ *   ahf_ref catalogue-sfc <snap> <prefix> [--threads T]
+ *                       ahf_halos_sfc_constructHalo per halo, then the
+ *                       three catalogues (catalogue-sfc-batch in the drop-in
+ *                       build: the library's batched pass)
  *   ahf_ref write-from <snap> <halos.dump> <prof.dump> <prefix>
  *                       AHF's catalogue writers (.AHF_halos, .AHF_profiles,
  *                       .AHF_particles) on haloes read from an AHFHALO1
@@ -513,6 +517,27 @@ static void cmd_profiles(const snap_data *s, const char *out, int threads, int g
  * an AHFHALO1 dump (members, M_vir, R_vir, Phi0, ovdens) and an AHFPROF1
  * dump (profiles and integral properties), then written by AHF's writers
  * (src/libahf/ahf_halos.c:824-858) */
+/* AHF's three catalogues of h[0..nh), haloes in decreasing npart order
+ * (ties by index) */
+static void write_catalogue(HALO *h, uint64_t nh, const char *prefix)
+{
+  unsigned long *idx = (unsigned long *)malloc((nh ? nh : 1) * sizeof(unsigned long));
+  for (uint64_t i = 0; i < nh; i++) idx[i] = i;
+  for (uint64_t i = 1; i < nh; i++) {
+    const unsigned long t = idx[i];
+    uint64_t k = i;
+    while (k > 0 && h[idx[k - 1]].npart < h[t].npart) {
+      idx[k] = idx[k - 1];
+      k--;
+    }
+    idx[k] = t;
+  }
+  ahf_io_WriteHalos(prefix, h, idx, (int)nh);
+  ahf_io_WriteProfiles(prefix, h, idx, (int)nh);
+  ahf_io_WriteParticles(prefix, h, idx, (int)nh);
+  free(idx);
+}
+
 static uint64_t rd64(FILE *f)
 {
   uint64_t v;
@@ -591,22 +616,61 @@ static void cmd_write_from(const snap_data *s, const char *hpath, const char *pp
   }
   fclose(fh);
   fclose(fp);
-  /* decreasing npart, ties by index */
-  unsigned long *idx = (unsigned long *)malloc((nh ? nh : 1) * sizeof(unsigned long));
-  for (uint64_t i = 0; i < nh; i++) idx[i] = i;
-  for (uint64_t i = 1; i < nh; i++) {
-    const unsigned long t = idx[i];
-    uint64_t k = i;
-    while (k > 0 && h[idx[k - 1]].npart < h[t].npart) {
-      idx[k] = idx[k - 1];
-      k--;
-    }
-    idx[k] = t;
-  }
-  ahf_io_WriteHalos(prefix, h, idx, (int)nh);
-  ahf_io_WriteProfiles(prefix, h, idx, (int)nh);
-  ahf_io_WriteParticles(prefix, h, idx, (int)nh);
+  write_catalogue(h, nh, prefix);
   fprintf(stdout, "{\"written\": %llu}\n", (unsigned long long)nh);
+}
+
+/* the whole of ahf_halos_sfc_constructHalo per halo (gather, construction
+ * steps, HaloProfiles), or (drop-in build, batch) the library's batched
+ * pass over all haloes; then the catalogues */
+static void cmd_catalogue(const snap_data *s, const char *prefix, int threads, int batch)
+{
+  HALO *h = halo_setup(s);
+  setup_sfc(s);
+  const uint64_t nh = s->h.nhalos;
+  for (uint64_t i = 0; i < nh; i++) {
+    h[i].gatherRad = s->gather_rad[i];
+    h[i].hostHalo = -1;
+  }
+#ifdef AHF_REF_BUILD
+  io.logfile = fopen("/dev/null", "w");
+  if (!io.logfile) die("cannot open /dev/null");
+#endif
+  double t0 = now_s();
+#ifndef AHF_REF_BUILD
+  if (batch) {
+    if (ahf_construct_profile_halos(h, nh)) die("batched construction failed");
+  } else
+#else
+  (void)batch;
+#endif
+  {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    long i;
+#pragma omp parallel for schedule(dynamic) private(i)
+    for (i = 0; i < (long)nh; i++) {
+      HALO *x = h + i;
+      if (x->npart == 0) continue;
+      x->nll = 0;
+      x->ll = NULL;
+      x->npart = 0;
+      free(x->ipart);
+      x->ipart = NULL;
+      ahf_halos_sfc_gatherParts(x);
+      sort_halo_particles(x);
+      rem_outsideRvir(x, 0);
+      rem_unbound(x);
+      rem_outsideRvir(x, 1);
+      HaloProfiles(x);
+    }
+  }
+  double t = now_s() - t0;
+  write_catalogue(h, nh, prefix);
+  fprintf(stdout, "{\"catalogue_seconds\": %.6f, \"threads\": %d}\n", t, threads);
 }
 
 #ifndef AHF_REF_BUILD
@@ -741,6 +805,9 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "profiles") || !strcmp(argv[1], "profiles-sfc")) {
     if (argc < 4) die("profiles needs an output path");
     cmd_profiles(&s, argv[3], threads, !strcmp(argv[1], "profiles-sfc"));
+  } else if (!strcmp(argv[1], "catalogue-sfc") || !strcmp(argv[1], "catalogue-sfc-batch")) {
+    if (argc < 4) die("catalogue needs an output prefix");
+    cmd_catalogue(&s, argv[3], threads, !strcmp(argv[1], "catalogue-sfc-batch"));
   } else if (!strcmp(argv[1], "write-from")) {
     if (argc < 6) die("write-from needs two dumps and a prefix");
     cmd_write_from(&s, argv[3], argv[4], argv[5]);
