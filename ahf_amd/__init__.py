@@ -102,6 +102,7 @@ def load():
     lib.ahfgpu_timer_get.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), P, P, P]
     lib.ahfgpu_halo_profiles.argtypes = [P, ctypes.POINTER(ProfileParams), P, P]
     lib.ahfgpu_profile_results.argtypes = [P, P, P]
+    lib.ahfgpu_set_profile_split.argtypes = [P, ctypes.c_int64]
     _lib = lib
     return lib
 
@@ -238,6 +239,11 @@ class Context:
         return npart, props, lists
 
     PROF_NQ, PROF_NP = 25, 45
+
+    def set_profile_split(self, min_members):
+        """haloes of at least min_members are profiled by four wavefronts
+        (0: default, negative: never); same values either way"""
+        self._check(self.lib.ahfgpu_set_profile_split(self.h, int(min_members)))
 
     def halo_profiles(self, minpart, r_fac, v_fac, m_fac, phi_fac, hubble, phi0_rvir=None):
         """HaloProfiles of every resident halo: of the last unbinding result,

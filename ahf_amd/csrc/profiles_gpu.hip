@@ -35,6 +35,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "../../include/ahf_gpu.h"
@@ -48,7 +50,8 @@
 #define PRF_MZERO 5e-16                     /* src/param.h:125 */
 #define PRF_MINSHELL 10                     /* AHF_MINPART_SHELL, src/param.h:16 */
 #define PRF_NIGNORE 5                       /* AHF_Rmax_r2_NIGNORE, src/param.h:25 */
-#define PRF_ROW 65                          /* LDS row of 64 terms + 1 (bank spread) */
+#define AHF_PROF_BIG 16384                  /* default: haloes this large take a workgroup of */
+#define AHF_PROF_BIG_NW 8                   /* ... this many wavefronts */
 
 /* per-bin record columns: the running values at the member that closes a bin */
 enum { BT_A11 = 0, BT_A22, BT_A33, BT_A12, BT_A13, BT_A23, BT_LX, BT_LY, BT_LZ, BT_T, BT_EP, BT_VESC, BT_N };
@@ -73,15 +76,24 @@ struct ProfConst {
   int64_t minpart;
 };
 
+/* LDS of one halo's workgroup of NW wavefronts: a chunk is 64*NW members */
+template <int NW>
 struct ProfShared {
-  double ta[12][PRF_ROW]; /* centre of mass x3, inertia x6, bulk velocity x3 */
-  double tc[5][PRF_ROW];  /* angular momentum x3, kinetic energy, potential */
-  double rad[64];         /* bin radii */
-  double bt[64][BT_N];    /* per-bin record */
-  int64_t bclose[64];     /* member index that closed each bin */
-  int32_t bprev[64], bcur[64];
+  static constexpr int C = 64 * NW;
+  double ta[12][C + 1]; /* centre of mass x3, inertia x6, bulk velocity x3 (rows padded: bank spread) */
+  double tc[5][C + 1];  /* angular momentum x3, kinetic energy, potential */
+  double d_s[C];        /* each member's distance */
+  double i_s[C];        /* I_now after each member; first the own M/d^2 */
+  double rad[64];       /* bin radii */
+  double bt[64][BT_N];  /* per-bin record */
+  int64_t bclose[64];   /* member index that closed each bin */
+  int32_t bprev[C], bcur[C];
+  uint64_t cm[NW], fm[NW], hm[NW]; /* per-wavefront masks: closing members, I_now steps, hits */
+  int32_t wv[NW];                  /* per-wavefront maxima / counts */
+  double rv[NW];                   /* per-wavefront reduction values */
+  int64_t ri[NW];
   double fin[16];
-  int64_t ifin[4];
+  int32_t flag;
 };
 
 /* ------------------------------------------------------- per-bin maths */
@@ -246,24 +258,26 @@ __device__ inline void chain_run(double *row, int i0, int i1, double &acc, doubl
 
 /* one running sum per lane that has a row (row != nullptr) over the cnt
  * staged terms: each term is replaced by the running value; at every member
- * of the mask cm the value is recorded for the bins it closes, in column
- * col (col < 0: not recorded), and with WITH_E the potential energy in
- * column ecol */
-template <bool WITH_E>
-__device__ inline void chain_pass(double *row, int cnt, uint64_t cm, ProfShared &S, double &acc, double &acc_e,
-                                  double phi0, int col, int ecol)
+ * of the masks cm (one per wavefront) the value is recorded for the bins it
+ * closes, in column col (col < 0: not recorded), and with WITH_E the
+ * potential energy in column ecol */
+template <bool WITH_E, int NW>
+__device__ inline void chain_pass(double *row, int cnt, ProfShared<NW> &S, double &acc, double &acc_e, double phi0,
+                                  int col, int ecol)
 {
   if (row) {
     int i = 0;
-    uint64_t m = cm;
-    while (m) {
-      const int e = __ffsll((unsigned long long)m) - 1;
-      m &= m - 1;
-      chain_run<WITH_E>(row, i, e + 1, acc, acc_e, phi0);
-      i = e + 1;
-      for (int b = S.bprev[e]; b < S.bcur[e]; b++) {
-        if (col >= 0) S.bt[b][col] = acc;
-        if (WITH_E && ecol >= 0) S.bt[b][ecol] = acc_e;
+    for (int w = 0; w < NW; w++) {
+      uint64_t m = S.cm[w];
+      while (m) {
+        const int e = 64 * w + __ffsll((unsigned long long)m) - 1;
+        m &= m - 1;
+        chain_run<WITH_E>(row, i, e + 1, acc, acc_e, phi0);
+        i = e + 1;
+        for (int b = S.bprev[e]; b < S.bcur[e]; b++) {
+          if (col >= 0) S.bt[b][col] = acc;
+          if (WITH_E && ecol >= 0) S.bt[b][ecol] = acc_e;
+        }
       }
     }
     chain_run<WITH_E>(row, i, cnt, acc, acc_e, phi0);
@@ -323,32 +337,23 @@ __device__ inline void first_max_merge(double &bv, int64_t &bi, double ov, int64
   }
 }
 
-__device__ inline int64_t wave_first_max_reduce(double bv, int64_t bi, int64_t m)
-{
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    const double ov = __shfl_xor(bv, d, 64);
-    const int64_t oi = __shfl_xor(bi, d, 64);
-    first_max_merge(bv, bi, ov, oi);
-  }
-  return bi >= 0 ? bi : m - 1;
-}
-
 /* r2's and R_max's peaks in one pass over the members from NIGNORE on:
  * the first maxima over [0, m-2] of rho r^2 after three smoothing passes
  * and of V_c^2 after one (find_max's right pass cannot move a maximum of
  * its left pass: the range it scans lies inside the left pass's) */
-__device__ void find_peaks(const double *__restrict__ dr2, const double *__restrict__ vc2, int64_t m, int64_t *i_r2,
-                           int64_t *i_rmax)
+template <int NW>
+__device__ void find_peaks(const double *__restrict__ dr2, const double *__restrict__ vc2, int64_t m,
+                           ProfShared<NW> &S, int64_t *i_r2, int64_t *i_rmax)
 {
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  constexpr int C = 64 * NW;
   double bv3 = -10.0, bv1 = -10.0;
   int64_t bi3 = -1, bi1 = -1;
-  for (int64_t t0 = 0; t0 < m - 1; t0 += 4 * 64) {
+  for (int64_t t0 = 0; t0 < m - 1; t0 += 4 * C) {
     double v3[4], v1[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int64_t t = t0 + u * 64 + lane;
+      const int64_t t = t0 + u * C + tid;
       const bool ok = t < m - 1;
       const int64_t tt = ok ? t : 0;
       if (m >= 3) {
@@ -362,7 +367,7 @@ __device__ void find_peaks(const double *__restrict__ dr2, const double *__restr
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int64_t t = t0 + u * 64 + lane;
+      const int64_t t = t0 + u * C + tid;
       if (v3[u] > bv3) {
         bv3 = v3[u];
         bi3 = t;
@@ -373,28 +378,53 @@ __device__ void find_peaks(const double *__restrict__ dr2, const double *__restr
       }
     }
   }
-  *i_r2 = wave_first_max_reduce(bv3, bi3, m);
-  *i_rmax = wave_first_max_reduce(bv1, bi1, m);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    first_max_merge(bv3, bi3, __shfl_xor(bv3, d, 64), __shfl_xor(bi3, d, 64));
+    first_max_merge(bv1, bi1, __shfl_xor(bv1, d, 64), __shfl_xor(bi1, d, 64));
+  }
+  __syncthreads();
+  if (lane == 0) {
+    S.rv[wid] = bv3;
+    S.ri[wid] = bi3;
+  }
+  __syncthreads();
+  double v = S.rv[0];
+  int64_t i = S.ri[0];
+  for (int w = 1; w < NW; w++) first_max_merge(v, i, S.rv[w], S.ri[w]);
+  *i_r2 = i >= 0 ? i : m - 1;
+  __syncthreads();
+  if (lane == 0) {
+    S.rv[wid] = bv1;
+    S.ri[wid] = bi1;
+  }
+  __syncthreads();
+  v = S.rv[0];
+  i = S.ri[0];
+  for (int w = 1; w < NW; w++) first_max_merge(v, i, S.rv[w], S.ri[w]);
+  *i_rmax = i >= 0 ? i : m - 1;
+  __syncthreads();
 }
 
 /* ------------------------------------------------------------- kernel */
 
-__global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__restrict__ off,
-                                                 const int32_t *__restrict__ order, const HaloOut *__restrict__ out,
-                                                 const double *__restrict__ phi_rvir,
-                                                 const uint32_t *__restrict__ ip_in, const uint32_t *__restrict__ buf_a,
-                                                 const uint32_t *__restrict__ buf_b, const double *__restrict__ cen,
-                                                 const int32_t *__restrict__ nbins_h, const int64_t *__restrict__ pofs,
-                                                 const float *__restrict__ px, const float *__restrict__ py,
-                                                 const float *__restrict__ pz, const float *__restrict__ vx,
-                                                 const float *__restrict__ vy, const float *__restrict__ vz,
-                                                 double *__restrict__ s_rad, double *__restrict__ s_vc2,
-                                                 double *__restrict__ s_dr2,
-                                                 double *__restrict__ props, double *__restrict__ prof)
+/* one workgroup of NW wavefronts per halo: haloes list[blockIdx.x] (list:
+ * the haloes this launch takes; NW = 1 skips haloes of big or more members,
+ * which a launch with more wavefronts takes) */
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_profiles(
+    ProfConst K, const int64_t *__restrict__ off, const int32_t *__restrict__ list, int64_t big,
+    const HaloOut *__restrict__ out, const double *__restrict__ phi_rvir, const uint32_t *__restrict__ ip_in,
+    const uint32_t *__restrict__ buf_a, const uint32_t *__restrict__ buf_b, const double *__restrict__ cen,
+    const int32_t *__restrict__ nbins_h, const int64_t *__restrict__ pofs, const float *__restrict__ px,
+    const float *__restrict__ py, const float *__restrict__ pz, const float *__restrict__ vx,
+    const float *__restrict__ vy, const float *__restrict__ vz, double *__restrict__ s_rad,
+    double *__restrict__ s_vc2, double *__restrict__ s_dr2, double *__restrict__ props, double *__restrict__ prof)
 {
-  __shared__ ProfShared S;
-  const int lane = threadIdx.x;
-  const int64_t h = order ? order[blockIdx.x] : blockIdx.x;
+  constexpr int C = 64 * NW;
+  __shared__ ProfShared<NW> S;
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int64_t h = list[blockIdx.x];
   const int nbins = nbins_h[h];
   if (nbins <= 0) return;
   const int64_t base = off[h];
@@ -412,6 +442,7 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
     Phi0 = phi_rvir[2 * h];
     R_vir = phi_rvir[2 * h + 1];
   }
+  if (NW == 1 && n >= big) return;
   const double Xc = cen[3 * h], Yc = cen[3 * h + 1], Zc = cen[3 * h + 2];
   double *rad_m = s_rad + base, *vc2 = s_vc2 + base, *dr2 = s_dr2 + base;
   const double FourPiThird = 4. * PRF_PI / 3.;
@@ -419,7 +450,7 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
   /* binning_parameter (specific.c:259-319): the first member from
    * MINPART/10 on (below the last) whose distance is not below MACHINE_ZERO,
    * else half the last member's distance; distances with |d| folded to
-   * 1-|d| beyond half the box */
+   * 1-|d| beyond half the box (the first wavefront; LDS to the others) */
   auto dist_fold = [&](int64_t j) {
     const uint32_t p = ip[j];
     double dX = fabs((double)px[p] - Xc), dY = fabs((double)py[p] - Yc), dZ = fabs((double)pz[p] - Zc);
@@ -428,20 +459,23 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
     if (dZ > 0.5) dZ = 1.0 - dZ;
     return sqrt(dX * dX + dY * dY + dZ * dZ);
   };
-  const double dist_max = dist_fold(n - 1);
-  double dist_min = dist_max / 2.;
-  for (int64_t j0 = (int64_t)floor(((double)K.minpart / 10.) + 0.5); j0 < n - 1; j0 += 64) {
-    const int64_t j = j0 + lane;
-    const double d = j < n - 1 ? dist_fold(j) : 0.0;
-    const uint64_t m = __ballot(j < n - 1 && !(d < PRF_MZERO));
-    if (m) {
-      dist_min = __shfl(d, __ffsll((unsigned long long)m) - 1, 64);
-      break;
+  if (wid == 0) {
+    const double dist_max = dist_fold(n - 1);
+    double dist_min = dist_max / 2.;
+    for (int64_t j0 = (int64_t)floor(((double)K.minpart / 10.) + 0.5); j0 < n - 1; j0 += 64) {
+      const int64_t j = j0 + lane;
+      const double d = j < n - 1 ? dist_fold(j) : 0.0;
+      const uint64_t m = __ballot(j < n - 1 && !(d < PRF_MZERO));
+      if (m) {
+        dist_min = __shfl(d, __ffsll((unsigned long long)m) - 1, 64);
+        break;
+      }
     }
+    const double ldmin = log10(dist_min), ldmax = log10(dist_max);
+    const double ldr = (ldmax - ldmin) / (double)nbins;
+    if (lane < nbins)
+      S.rad[lane] = lane == nbins - 1 ? dist_max + PRF_ZERO : pow(10., ldmin + ((double)lane + 1) * ldr);
   }
-  const double ldmin = log10(dist_min), ldmax = log10(dist_max);
-  const double ldr = (ldmax - ldmin) / (double)nbins;
-  if (lane < nbins) S.rad[lane] = lane == nbins - 1 ? dist_max + PRF_ZERO : pow(10., ldmin + ((double)lane + 1) * ldr);
   __syncthreads();
   const bool mono = __ballot(lane < nbins - 1 && !(S.rad[lane] <= S.rad[lane + 1])) == 0;
 
@@ -449,21 +483,21 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
    * then the terms about it, then every other running sum in one pass:
    * angular momentum (lanes 0-2), kinetic energy (3), the potential and
    * with it the potential energy (4), centre of mass (5-7), inertia tensor
-   * (8-13) */
+   * (8-13); the sums run on the first wavefront */
   double accV = 0.0, accC = 0.0, accE = 0.0, dummy = 0.0;
   double I_car = 0.0, d_car = 0.0, Emin = 1e30;
   int64_t mb = -1, P = 0;
   int bcar = 0;
   const double hub = K.hubble, rf = K.r_fac, vf = K.v_fac;
-  double *const rowV = lane < 3 ? S.ta[9 + lane] : nullptr;
-  double *const rowC = lane < 5 ? S.tc[lane] : lane < 14 ? S.ta[lane - 5] : nullptr;
-  const int colC = lane < 4 ? BT_LX + lane : (lane >= 8 && lane < 14) ? BT_A11 + lane - 8 : -1;
-  const int ecolC = lane == 4 ? BT_EP : -1;
+  double *const rowV = tid < 3 ? S.ta[9 + tid] : nullptr;
+  double *const rowC = tid < 5 ? S.tc[tid] : tid < 14 ? S.ta[tid - 5] : nullptr;
+  const int colC = tid < 4 ? BT_LX + tid : (tid >= 8 && tid < 14) ? BT_A11 + tid - 8 : -1;
+  const int ecolC = tid == 4 ? BT_EP : -1;
   /* the members of the next chunk are read while the current one is
    * summed, their indices one chunk earlier still */
   float nx = 0.f, ny = 0.f, nz = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
-  if (lane < n) {
-    const uint32_t q = ip[lane];
+  if (tid < n) {
+    const uint32_t q = ip[tid];
     nx = px[q];
     ny = py[q];
     nz = pz[q];
@@ -471,13 +505,13 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
     nvy = vy[q];
     nvz = vz[q];
   }
-  uint32_t qn = 64 + lane < n ? ip[64 + lane] : 0;
-  for (int64_t j0 = 0; j0 < n && bcar < nbins; j0 += 64) {
-    const int64_t j = j0 + lane;
+  uint32_t qn = C + tid < n ? ip[C + tid] : 0;
+  for (int64_t j0 = 0; j0 < n && bcar < nbins; j0 += C) {
+    const int64_t j = j0 + tid;
     const bool inb = j < n;
     double dX = 0.0, dY = 0.0, dZ = 0.0, d = 0.0, vxp = 0.0, vyp = 0.0, vzp = 0.0;
     const float cx = nx, cy = ny, cz = nz, cvx = nvx, cvy = nvy, cvz = nvz;
-    if (j + 64 < n) {
+    if (j + C < n) {
       nx = px[qn];
       ny = py[qn];
       nz = pz[qn];
@@ -485,7 +519,7 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
       nvy = vy[qn];
       nvz = vz[qn];
     }
-    qn = j + 128 < n ? ip[j + 128] : 0;
+    qn = j + 2 * C < n ? ip[j + 2 * C] : 0;
     if (inb) {
       dX = (double)cx - Xc;
       dY = (double)cy - Yc;
@@ -501,6 +535,7 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
       vyp = (double)cvy;
       vzp = (double)cvz;
     }
+    S.d_s[tid] = d;
     /* bins closed after each member */
     int bp, bi;
     if (mono) {
@@ -511,124 +546,175 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
         while (c < nbins && S.rad[c] <= d) c++;
         if (j == n - 1) c = nbins;
       }
-      bi = __ballot(c > bcar) ? wave_prefix_max(c) : bcar;
-      bp = __shfl_up(bi, 1, 64);
-      if (lane == 0) bp = bcar;
-    } else {
-      S.ta[0][lane] = d;
+      const uint64_t any = __ballot(c > bcar);
+      if (lane == 0) S.hm[wid] = any;
       __syncthreads();
-      if (lane == 0) {
+      bool some = false;
+      for (int w = 0; w < NW; w++) some |= S.hm[w] != 0;
+      bi = bcar;
+      if (some) {
+        bi = wave_prefix_max(c);
+        if (lane == 63) S.wv[wid] = bi;
+        __syncthreads();
+        for (int w = 0; w < wid; w++) bi = max(bi, S.wv[w]);
+        bi = max(bi, bcar);
+      }
+      S.bcur[tid] = bi;
+      __syncthreads();
+      bp = tid ? S.bcur[tid - 1] : bcar;
+    } else {
+      __syncthreads();
+      if (tid == 0) {
         int b = bcar;
-        for (int i = 0; i < 64; i++) {
+        for (int i = 0; i < C; i++) {
           S.bprev[i] = b;
           if (b < nbins && j0 + i < n)
-            while (b < nbins && (S.ta[0][i] >= S.rad[b] || j0 + i == n - 1)) b++;
+            while (b < nbins && (S.d_s[i] >= S.rad[b] || j0 + i == n - 1)) b++;
           S.bcur[i] = b;
         }
       }
       __syncthreads();
-      bp = S.bprev[lane];
-      bi = S.bcur[lane];
-      __syncthreads();
+      bp = S.bprev[tid];
+      bi = S.bcur[tid];
     }
     const bool proc = inb && bp < nbins;
-    const int cnt = __popcll(__ballot(proc));
-    const uint64_t cm = __ballot(proc && bi > bp);
+    {
+      const uint64_t pm = __ballot(proc), cmw = __ballot(proc && bi > bp);
+      if (lane == 0) {
+        S.wv[wid] = __popcll(pm);
+        S.cm[wid] = cmw;
+      }
+    }
+    __syncthreads();
+    int cnt = 0;
+    for (int w = 0; w < NW; w++) cnt += S.wv[w];
     if (proc) {
-      S.ta[0][lane] = Xc + dX;
-      S.ta[1][lane] = Yc + dY;
-      S.ta[2][lane] = Zc + dZ;
-      S.ta[3][lane] = dX * dX;
-      S.ta[4][lane] = dY * dY;
-      S.ta[5][lane] = dZ * dZ;
-      S.ta[6][lane] = dX * dY;
-      S.ta[7][lane] = dX * dZ;
-      S.ta[8][lane] = dY * dZ;
-      S.ta[9][lane] = vxp;
-      S.ta[10][lane] = vyp;
-      S.ta[11][lane] = vzp;
-      S.bprev[lane] = bp;
-      S.bcur[lane] = bi;
+      S.ta[0][tid] = Xc + dX;
+      S.ta[1][tid] = Yc + dY;
+      S.ta[2][tid] = Zc + dZ;
+      S.ta[3][tid] = dX * dX;
+      S.ta[4][tid] = dY * dY;
+      S.ta[5][tid] = dZ * dZ;
+      S.ta[6][tid] = dX * dY;
+      S.ta[7][tid] = dX * dZ;
+      S.ta[8][tid] = dY * dZ;
+      S.ta[9][tid] = vxp;
+      S.ta[10][tid] = vyp;
+      S.ta[11][tid] = vzp;
+      S.bprev[tid] = bp;
+      S.bcur[tid] = bi;
       for (int b = bp; b < bi; b++) S.bclose[b] = j;
       rad_m[j] = d;
     }
     __syncthreads();
     /* bulk velocity */
-    chain_pass<false>(rowV, cnt, cm, S, accV, dummy, 0.0, -1, -1);
+    chain_pass<false, NW>(rowV, cnt, S, accV, dummy, 0.0, -1, -1);
 
     /* angular momentum and kinetic energy terms about the running bulk
      * velocity; the potential integral's term (ahf_halos.c:4333-4395) */
     const double M = (double)(j + 1);
-    double dVX = vxp - S.ta[9][lane] / M, dVY = vyp - S.ta[10][lane] / M, dVZ = vzp - S.ta[11][lane] / M;
+    double dVX = vxp - S.ta[9][tid] / M, dVY = vyp - S.ta[10][tid] / M, dVZ = vzp - S.ta[11][tid] / M;
     const double lx = dY * dVZ - dZ * dVY, ly = dZ * dVX - dX * dVZ, lz = dX * dVY - dY * dVX;
     dVX += hub * dX * rf / vf;
     dVY += hub * dY * rf / vf;
     dVZ += hub * dZ * rf / vf;
     const double T = dVX * dVX + dVY * dVY + dVZ * dVZ;
+    /* I_now: M/d^2 of the last member so far with d > MACHINE_ZERO */
     const bool fl = proc && d > PRF_MZERO;
-    const double inow_own = M / (d * d);
-    const uint64_t fm = __ballot(fl);
-    const uint64_t le = fm & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-    const double got = __shfl(inow_own, le ? 63 - __clzll((long long)le) : 0, 64);
-    const double I_now = le ? got : I_car;
-    double I_prev = __shfl_up(I_now, 1, 64), pre = __shfl_up(d, 1, 64);
-    if (lane == 0) {
-      I_prev = I_car;
-      pre = d_car;
+    S.i_s[tid] = M / (d * d);
+    {
+      const uint64_t fmw = __ballot(fl);
+      if (lane == 0) S.fm[wid] = fmw;
     }
+    __syncthreads();
+    int64_t src = -1;
+    {
+      const uint64_t le = S.fm[wid] & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+      if (le) src = 64 * wid + 63 - __clzll((long long)le);
+      for (int w = wid - 1; w >= 0 && src < 0; w--)
+        if (S.fm[w]) src = 64 * w + 63 - __clzll((long long)S.fm[w]);
+    }
+    const double I_now = src >= 0 ? S.i_s[src] : I_car;
+    __syncthreads();
+    S.i_s[tid] = I_now;
+    __syncthreads();
+    const double I_prev = tid ? S.i_s[tid - 1] : I_car, pre = tid ? S.d_s[tid - 1] : d_car;
     if (proc) {
-      S.tc[0][lane] = lx;
-      S.tc[1][lane] = ly;
-      S.tc[2][lane] = lz;
-      S.tc[3][lane] = T;
-      S.tc[4][lane] = fl ? ((I_now + I_prev) / 2.) * (d - pre) : -0.0; /* -0.0: an exact no-op */
+      S.tc[0][tid] = lx;
+      S.tc[1][tid] = ly;
+      S.tc[2][tid] = lz;
+      S.tc[3][tid] = T;
+      S.tc[4][tid] = fl ? ((I_now + I_prev) / 2.) * (d - pre) : -0.0; /* -0.0: an exact no-op */
       vc2[j] = M / d;
       const double dV = FourPiThird * ((d * d * d) - (pre * pre * pre));
       const double hm = (d + pre) / 2.;
       dr2[j] = 1.0 / dV * (hm * hm);
     }
-    I_car = __shfl(I_now, cnt - 1, 64);
-    d_car = __shfl(d, cnt - 1, 64);
+    I_car = S.i_s[cnt - 1];
+    d_car = S.d_s[cnt - 1];
     __syncthreads();
     /* angular momentum, kinetic energy, potential and potential energy,
      * centre of mass, inertia tensor */
-    chain_pass<true>(rowC, cnt, cm, S, accC, accE, Phi0, colC, ecolC);
+    chain_pass<true, NW>(rowC, cnt, S, accC, accE, Phi0, colC, ecolC);
 
-    const double U = S.tc[4][lane] - Phi0;
+    const double U = S.tc[4][tid] - Phi0;
     const double Ep = 0.5 * T + U;
     if (proc)
       for (int b = bp; b < bi; b++) S.bt[b][BT_VESC] = 2 * fabs(U);
     /* most bound member: the first at the lowest energy below the one so far */
-    if (__ballot(proc && Ep < Emin)) {
-      double bv = proc && Ep < Emin ? Ep : 0.0;
-      int bl = proc && Ep < Emin ? lane : 64;
+    {
+      const bool cand = proc && Ep < Emin;
+      const uint64_t cw = __ballot(cand);
+      if (lane == 0) S.hm[wid] = cw;
+      __syncthreads();
+      bool some = false;
+      for (int w = 0; w < NW; w++) some |= S.hm[w] != 0;
+      if (some) {
+        double bv = cand ? Ep : 0.0;
+        int64_t bl = cand ? tid : -1;
 #pragma unroll
-      for (int dd = 32; dd >= 1; dd >>= 1) {
-        const double ov = __shfl_xor(bv, dd, 64);
-        const int ol = __shfl_xor(bl, dd, 64);
-        if (ol < 64 && (bl == 64 || ov < bv || (!(bv < ov) && ol < bl))) {
-          bv = ov;
-          bl = ol;
+        for (int dd = 32; dd >= 1; dd >>= 1) {
+          const double ov = __shfl_xor(bv, dd, 64);
+          const int64_t ol = __shfl_xor(bl, dd, 64);
+          if (ol >= 0 && (bl < 0 || ov < bv || (!(bv < ov) && ol < bl))) {
+            bv = ov;
+            bl = ol;
+          }
         }
+        if (lane == 0) {
+          S.rv[wid] = bv;
+          S.ri[wid] = bl;
+        }
+        __syncthreads();
+        bv = S.rv[0];
+        bl = S.ri[0];
+        for (int w = 1; w < NW; w++) {
+          const double ov = S.rv[w];
+          const int64_t ol = S.ri[w];
+          if (ol >= 0 && (bl < 0 || ov < bv || (!(bv < ov) && ol < bl))) {
+            bv = ov;
+            bl = ol;
+          }
+        }
+        Emin = bv;
+        mb = j0 + bl;
       }
-      Emin = bv;
-      mb = j0 + bl;
     }
+    bcar = S.bcur[cnt - 1];
     __syncthreads();
-    bcar = __shfl(bi, cnt - 1, 64);
     P += cnt;
-    if (cnt < 64) break;
+    if (cnt < C) break;
   }
   /* members the walk did not reach read as AHF's zeroed arrays */
-  for (int64_t j = P + lane; j < n; j += 64) rad_m[j] = vc2[j] = dr2[j] = 0.0;
-  if (lane < 3) S.fin[9 + lane] = accV;
-  if (lane >= 5 && lane < 8) S.fin[lane - 5] = accC;
+  for (int64_t j = P + tid; j < n; j += C) rad_m[j] = vc2[j] = dr2[j] = 0.0;
+  if (tid < 3) S.fin[9 + tid] = accV;
+  if (tid >= 5 && tid < 8) S.fin[tid - 5] = accC;
   __syncthreads();
 
   /* the bins */
   double *pr = prof + pofs[h] * PQ_N;
-  if (lane < nbins) {
-    const int b = lane;
+  if (tid < nbins) {
+    const int b = tid;
     const double M = (double)(S.bclose[b] + 1), Mp = b ? (double)(S.bclose[b - 1] + 1) : 0.0;
     const double r = S.rad[b], rp = b ? S.rad[b - 1] : 0.0;
     const double Vol = FourPiThird * (r * r * r), Vp = b ? FourPiThird * (rp * rp * rp) : 0.0;
@@ -662,30 +748,26 @@ __global__ __launch_bounds__(64) void k_profiles(ProfConst K, const int64_t *__r
   const int64_t m = n - PRF_NIGNORE;
   const double *x = rad_m + PRF_NIGNORE;
   int64_t i_r2, i_rm;
-  find_peaks(dr2 + PRF_NIGNORE, vc2 + PRF_NIGNORE, m, &i_r2, &i_rm);
+  find_peaks<NW>(dr2 + PRF_NIGNORE, vc2 + PRF_NIGNORE, m, S, &i_r2, &i_rm);
   const double r2 = (x[i_r2] + x[i_r2]) / 2;
   const double R_max = (x[i_rm] + x[i_rm]) / 2;
   /* the first member at or beyond R_max (ahf_halos.c:4892-4894) */
   int64_t ib = n - 1;
-  for (int64_t j0 = 0; j0 < n - 1; j0 += 4 * 64) {
-    bool hit[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int64_t j = j0 + u * 64 + lane;
-      hit[u] = j < n - 1 && !(rad_m[j] < R_max);
-    }
+  for (int64_t j0 = 0; j0 < n - 1; j0 += C) {
+    const int64_t j = j0 + tid;
+    const uint64_t mk = __ballot(j < n - 1 && !(rad_m[j] < R_max));
+    if (lane == 0) S.hm[wid] = mk;
+    __syncthreads();
     int64_t found = -1;
-#pragma unroll
-    for (int u = 3; u >= 0; u--) {
-      const uint64_t mk = __ballot(hit[u]);
-      if (mk) found = j0 + u * 64 + __ffsll((unsigned long long)mk) - 1;
-    }
+    for (int w = NW - 1; w >= 0; w--)
+      if (S.hm[w]) found = j0 + 64 * w + __ffsll((unsigned long long)S.hm[w]) - 1;
+    __syncthreads();
     if (found >= 0) {
       ib = found;
       break;
     }
   }
-  if (lane != 0) return;
+  if (tid != 0) return;
 
   const double rib = rad_m[ib];
   const double ovd = ib < P ? (double)(ib + 1) / (FourPiThird * (rib * rib * rib)) : 0.0;
@@ -842,23 +924,44 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     pofs[(size_t)h] = tot;
     tot += k;
   }
+  /* haloes of big members or more go to workgroups of AHF_PROF_BIG_NW
+   * wavefronts on the second stream, largest first; the rest to one
+   * wavefront each */
+  const int64_t big = c->prof_split == 0 ? AHF_PROF_BIG : c->prof_split < 0 ? INT64_MAX : c->prof_split;
+  std::vector<int32_t> bigl;
+  for (int64_t h = 0; h < nh; h++)
+    if (nb[(size_t)h] > 0 && hn[(size_t)h] >= big) bigl.push_back((int32_t)h);
+  std::stable_sort(bigl.begin(), bigl.end(), [&](int32_t x, int32_t y) { return hn[(size_t)x] > hn[(size_t)y]; });
+  const int64_t nbig = (int64_t)bigl.size();
+  int32_t *d_big = nbig ? A.get<int32_t>(nbig) : nullptr;
   ProfResult *r = (ProfResult *)c->prof_result;
   if (!r) c->prof_result = r = new ProfResult();
   r->nh = nh;
   r->nbins_tot = tot;
   r->props = A.get<double>(nh * PP_N);
   r->prof = A.get<double>(tot * PQ_N);
-  if (!r->props || !r->prof) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (profiles)");
+  if (!r->props || !r->prof || (nbig && !d_big)) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (profiles)");
   if (nh) {
     hipMemcpyAsync(d_nb, nb.data(), (size_t)nh * 4, hipMemcpyHostToDevice, c->st);
     hipMemcpyAsync(d_pofs, pofs.data(), (size_t)nh * 8, hipMemcpyHostToDevice, c->st);
+    if (nbig) hipMemcpyAsync(d_big, bigl.data(), (size_t)nbig * 4, hipMemcpyHostToDevice, c->st);
     if (phi0_rvir) hipMemcpyAsync(d_pr, phi0_rvir, (size_t)nh * 16, hipMemcpyHostToDevice, c->st);
     hipMemsetAsync(r->props, 0, (size_t)nh * PP_N * 8, c->st);
     ProfConst K{p->r_fac, p->v_fac, p->m_fac, p->phi_fac, p->hubble, p->minpart};
     hipEvent_t e0 = c->timer_begin();
-    hipLaunchKernelGGL(k_profiles, dim3((unsigned)nh), dim3(64), 0, c->st, K, s->off, s->order, out, d_pr, s->ip_in,
-                       s->buf_a, s->buf_b, s->cen, d_nb, d_pofs, c->px, c->py, c->pz, c->vx, c->vy, c->vz, sc,
-                       sc + ne, sc + 2 * ne, r->props, r->prof);
+    if (nbig) {
+      hipEventRecord(c->ev_s0, c->st);
+      hipStreamWaitEvent(c->st2, c->ev_s0, 0);
+      hipLaunchKernelGGL(k_profiles<AHF_PROF_BIG_NW>, dim3((unsigned)nbig), dim3(64 * AHF_PROF_BIG_NW), 0, c->st2, K,
+                         s->off, d_big, big, out, d_pr,
+                         s->ip_in, s->buf_a, s->buf_b, s->cen, d_nb, d_pofs, c->px, c->py, c->pz, c->vx, c->vy,
+                         c->vz, sc, sc + ne, sc + 2 * ne, r->props, r->prof);
+      hipEventRecord(c->ev_s1, c->st2);
+    }
+    hipLaunchKernelGGL(k_profiles<1>, dim3((unsigned)nh), dim3(64), 0, c->st, K, s->off, s->order, big, out, d_pr,
+                       s->ip_in, s->buf_a, s->buf_b, s->cen, d_nb, d_pofs, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
+                       sc, sc + ne, sc + 2 * ne, r->props, r->prof);
+    if (nbig) hipStreamWaitEvent(c->st, c->ev_s1, 0);
     c->timer_end(e0, TIMER_PROFILES, 0.0);
   }
   c->timers_reset(TIMER_PROFILES, TIMER_PROFILES);
@@ -885,4 +988,11 @@ void ahfgpu_free_profiles(ahfgpu_ctx *c)
   delete (ProfResult *)c->prof_result;
   c->prof_result = nullptr;
   c->parena.release();
+}
+
+extern "C" int ahfgpu_set_profile_split(ahfgpu_ctx *c, int64_t min_members)
+{
+  if (!c) return AHFGPU_EINVAL;
+  c->prof_split = min_members;
+  return 0;
 }

@@ -72,8 +72,11 @@ CASES = [(2, 800, 1002, 0), (0, 64, 1000, 0), (3, 48, 5, 0), (2, 2000, 1002, 0),
          (1, 64, 1001, 0)]
 
 
+# split: 0 the default (haloes of 16384 members or more on four wavefronts),
+# 64 nearly every halo on four wavefronts, -1 every halo on one
+@pytest.mark.parametrize("split", [0, 64, -1])
 @pytest.mark.parametrize("cfg,side,seed,gather", CASES)
-def test_profiles_of_given_haloes(oracle, snapgen, ctx, tmp_path, cfg, side, seed, gather):
+def test_profiles_of_given_haloes(oracle, snapgen, ctx, tmp_path, cfg, side, seed, gather, split):
     """the kernel on the oracle's constructed haloes against the oracle's
     HaloProfiles dump of the same snapshot"""
     snap = str(tmp_path / "p.snap")
@@ -83,13 +86,15 @@ def test_profiles_of_given_haloes(oracle, snapgen, ctx, tmp_path, cfg, side, see
     o = str(tmp_path / "o.prof")
     assert oracle.oracle_profiles_dump(snap.encode(), o.encode(), 8, gather) == 0
     ref = dumps.read_profiles(o)
-    print(f"[profiles-gpu] cfg{cfg} side {side} gather {gather}: {s.nhalos} haloes", flush=True)
+    print(f"[profiles-gpu] cfg{cfg} side {side} gather {gather} split {split}: {s.nhalos} haloes", flush=True)
+    ctx.set_profile_split(split)
     ctx.set_particles(s.pos, s.mom)
     ip = np.concatenate([x for x in c.ipart] + [np.zeros(0, np.uint64)])
     ctx.set_halos(s.centre, c.npart, ip)
     nb, props, prof = ctx.halo_profiles(s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble,
                                         phi0_rvir=np.stack([c.Phi0, c.R_vir], axis=1))
-    _compare(ref, nb, props, prof, f"cfg{cfg}/{side}/g{gather}")
+    ctx.set_profile_split(0)
+    _compare(ref, nb, props, prof, f"cfg{cfg}/{side}/g{gather}/split{split}")
 
 
 @pytest.mark.parametrize("cfg,side,seed,gather", [(2, 800, 1002, 0), (3, 48, 5, 0), (2, 400, 1002, 1)])
@@ -126,7 +131,8 @@ def _sorted_halo(oracle, s, h):
     return ip
 
 
-def test_profiles_edge_cases(oracle, ctx, tmp_path):
+@pytest.mark.parametrize("split", [0, 6])
+def test_profiles_edge_cases(oracle, ctx, tmp_path, split):
     """lists in reverse order (bin radii that decrease: the sequential bin
     automaton), a member on the centre (no potential step), members at
     equal distances, the smallest halo find_max accepts, and a halo below
@@ -157,10 +163,13 @@ def test_profiles_edge_cases(oracle, ctx, tmp_path):
     phi_rvir = np.array([[-3.0e-3, 0.03], [-1e-3, 0.02], [0.0, 0.01], [0.0, 0.01], [-2e-3, 0.02]])
     ref = _oracle_profile(oracle, s, given, phi_rvir, minpart=6)
     assert list(ref.nbins[2:4]) == [2, 0]
+    print(f"[profiles-gpu] edge cases, split {split}", flush=True)
+    ctx.set_profile_split(split)
     ctx.set_particles(s.pos, s.mom)
     ctx.set_halos(s.centre, np.array(sizes, np.uint64), np.concatenate(given))
     nb, props, prof = ctx.halo_profiles(6, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble, phi0_rvir=phi_rvir)
-    _compare(ref, nb, props, prof, "edge cases")
+    ctx.set_profile_split(0)
+    _compare(ref, nb, props, prof, f"edge cases, split {split}")
 
 
 def test_profiles_rejects_tiny_minpart(ctx):
