@@ -3,12 +3,14 @@ functions (oracle/_ref/ahf_ref, built from the reference sources by
 `make -C oracle ref`).
 
 Each fixture is data only: the input snapshot's arrays and parameters, and
-the reference's output dump (AHFGRID1 or AHFHALO1 bytes) for one command:
+the reference's output dump (AHFGRID1, AHFHALO1 or AHFPROF1 bytes) for one command:
   amr            gen_domgrids, ll, zero_dens + assign_npart, gen_AMRhierarchy
   unbind         sort_halo_particles + rem_unbound per halo
   construct      sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1)
   gather         ahf_halos_sfc_gatherParts per halo
   construct-sfc  gather followed by the four construction steps
+  profiles       the four construction steps, then HaloProfiles (AHFPROF1)
+  profiles-sfc   gather, the four construction steps, then HaloProfiles
 tests/test_golden.py replays the inputs through the oracle (and, on a GPU,
 through the library) and compares with the stored outputs.
 
@@ -38,7 +40,7 @@ HEADER = ("boxsize", "omega0", "lambda0", "a", "pmass", "L_dom", "L_max", "Nth_d
 CASES = [
     ("amr_fuzz142", ("fuzz", 142, True), ["amr"]),
     ("amr_fuzz305", ("fuzz", 305, False), ["amr"]),
-    ("halo_cfg2_h40", ("gen", 2, 40, 12), ["unbind", "construct"]),
+    ("halo_cfg2_h40", ("gen", 2, 40, 12), ["unbind", "construct", "profiles"]),
     ("sfc_cfg1_side24", ("gen+centres", 1, 24, 11), ["gather", "construct-sfc"]),
 ]
 

@@ -52,8 +52,9 @@ def _close(a, b, rtol):
     return bool(np.all(same | near))
 
 
-def profiles_match(ref, nb, props, prof, tag, end_to_end=False):
-    """exact where the inputs fix the bits; the tolerances above elsewhere"""
+def profiles_match(ref, nb, props, prof, tag, end_to_end=False, exact=False):
+    """exact where the inputs fix the bits; the tolerances above elsewhere
+    (exact: every value bit for bit, for host results against the reference)"""
     assert np.array_equal(ref.nbins, nb), tag
     assert int((nb > 0).sum()) > 0, tag
     exact_bits, total = 0, 0
@@ -61,7 +62,9 @@ def profiles_match(ref, nb, props, prof, tag, end_to_end=False):
         a, b = ref.props[:, j], props[:, j]
         total += a.size
         exact_bits += int(np.sum((a == b) | (np.isnan(a) & np.isnan(b))))
-        if name in BISECT_PROPS:
+        if exact:
+            assert _close(a, b, 0.0), f"{tag}: {name}"
+        elif name in BISECT_PROPS:
             assert np.all((a == b) | (np.abs(a - b) <= 1e-3)), f"{tag}: {name}"
         elif name in LIBM_PROPS:
             assert _close(a, b, 1e-12), f"{tag}: {name}"
@@ -76,7 +79,9 @@ def profiles_match(ref, nb, props, prof, tag, end_to_end=False):
         for j, name in enumerate(dumps.PROF_BINS):
             total += x.shape[0]
             exact_bits += int(np.sum((x[:, j] == y[:, j]) | (np.isnan(x[:, j]) & np.isnan(y[:, j]))))
-            if name in LIBM_BINS:
+            if exact:
+                assert _close(x[:, j], y[:, j], 0.0), f"{tag}: halo {h} {name}"
+            elif name in LIBM_BINS:
                 assert _close(x[:, j], y[:, j], 1e-12), f"{tag}: halo {h} {name}"
             elif end_to_end and name in PHI_BINS:
                 assert _close(x[:, j], y[:, j], 1e-8), f"{tag}: halo {h} {name}"

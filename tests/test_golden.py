@@ -11,6 +11,8 @@ import pytest
 
 from ahf_amd import dumps
 
+from helpers import profiles_match
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
 HEADER = ("boxsize", "omega0", "lambda0", "a", "pmass", "L_dom", "L_max", "Nth_dom", "Nth_ref", "minpart", "vtune",
@@ -65,6 +67,12 @@ def test_oracle_reproduces_golden(oracle, tmp_path, path):
             diffs = dumps.compare_grids(dumps.read_grids(out), dumps.read_grids(ref))
             assert not diffs, diffs[:5]
             continue
+        if cmd in ("profiles", "profiles-sfc"):
+            assert oracle.oracle_profiles_dump(snap.encode(), out.encode(), 1, int(cmd == "profiles-sfc")) == 0
+            a, b = dumps.read_profiles(out), dumps.read_profiles(ref)
+            assert np.array_equal(a.npart, b.npart) and int((a.nbins > 0).sum()) > 0
+            profiles_match(b, a.nbins, a.props, a.prof, f"golden {cmd}", exact=True)
+            continue
         if cmd == "unbind":
             rc = oracle.oracle_unbind_dump(snap.encode(), out.encode(), 1, 0)
         elif cmd == "construct":
@@ -97,6 +105,17 @@ def test_gpu_reproduces_golden(tmp_path, path):
                 ctx.amr_dump(out)
                 diffs = dumps.compare_grids(dumps.read_grids(out), dumps.read_grids(ref))
                 assert not diffs, diffs[:5]
+                continue
+            if cmd in ("profiles", "profiles-sfc"):
+                rp = dumps.read_profiles(ref)
+                if cmd == "profiles-sfc":
+                    ctx.sfc_keys()
+                    ctx.gather_halos(s.centre, s.gather_rad, skip=(s.halo_npart == 0))
+                else:
+                    ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+                ctx.unbind(*uargs, mode=ctx.UNBIND_WITH_RVIR, **rv)
+                nb, props, prof = ctx.halo_profiles(s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble)
+                profiles_match(rp, nb, props, prof, f"golden gpu {cmd}", end_to_end=True)
                 continue
             r = dumps.read_halos(ref)
             if cmd in ("gather", "construct-sfc"):
