@@ -50,7 +50,7 @@
 #define PRF_MZERO 5e-16                     /* src/param.h:125 */
 #define PRF_MINSHELL 10                     /* AHF_MINPART_SHELL, src/param.h:16 */
 #define PRF_NIGNORE 5                       /* AHF_Rmax_r2_NIGNORE, src/param.h:25 */
-#define AHF_PROF_BIG 16384                  /* default: haloes this large take a workgroup of */
+#define AHF_PROF_BIG 2048                   /* default: haloes this large take a workgroup of */
 #define AHF_PROF_BIG_NW 8                   /* ... this many wavefronts */
 
 /* per-bin record columns: the running values at the member that closes a bin */
