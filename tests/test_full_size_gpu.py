@@ -86,3 +86,46 @@ def test_gather_construct_full_size(oracle, ctx, snapgen, tmp_path):
     npart, props, lists = ctx.unbind_results()
     _same_haloes(ref, npart, lists, props, "gather+construct cfg1")
     assert np.array_equal(ref.ovdens, props[:, 3])
+
+
+@pytest.mark.parametrize("cfg,seed,gather", [(2, 1002, 0), (1, 1001, 1)])
+def test_profiles_full_size(oracle, ctx, snapgen, tmp_path, cfg, seed, gather):
+    """HaloProfiles at full size: the 100,000-halo workload (its largest halo
+    of 809,536 members walked by eight wavefronts) and the 256^3 box's
+    gathered haloes.  Given the oracle's constructed haloes, the profiles
+    match the oracle's bit for bit but for the values through log10/pow/log;
+    with the construction on the device too, the energies built on Phi0
+    agree to 1e-8."""
+    from helpers import profiles_match
+    snap = _snap(snapgen, tmp_path, cfg, seed)
+    s = dumps.read_snapshot(snap)
+    print(f"[full-gpu] profiles cfg{cfg}: {s.nhalos} haloes", flush=True)
+    c_out, p_out = str(tmp_path / "c.halo"), str(tmp_path / "p.prof")
+    if gather:
+        assert oracle.oracle_gather_dump(snap.encode(), c_out.encode(), 16, 0, 1, None) == 0
+    else:
+        assert oracle.oracle_construct_dump(snap.encode(), c_out.encode(), 16, 0) == 0
+    assert oracle.oracle_profiles_dump(snap.encode(), p_out.encode(), 16, gather) == 0
+    c, ref = dumps.read_halos(c_out), dumps.read_profiles(p_out)
+    ctx.set_profile_split(0)
+    ctx.set_particles(s.pos, s.mom)
+    # the oracle's haloes
+    ip = np.concatenate([x for x in c.ipart] + [np.zeros(0, np.uint64)])
+    ctx.set_halos(s.centre, c.npart, ip)
+    nb, props, prof = ctx.halo_profiles(s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble,
+                                        phi0_rvir=np.stack([c.Phi0, c.R_vir], axis=1))
+    profiles_match(ref, nb, props, prof, f"full cfg{cfg} given")
+    # constructed on the device
+    if gather:
+        ctx.sfc_keys()
+        ctx.gather_halos(s.centre, s.gather_rad, skip=(s.halo_npart == 0).astype(np.uint8))
+    else:
+        ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+    ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=ctx.UNBIND_WITH_RVIR,
+               ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+    nb, props, prof = ctx.halo_profiles(s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble)
+    npart, _, _ = ctx.unbind_results()
+    assert np.array_equal(npart, ref.npart)
+    profiles_match(ref, nb, props, prof, f"full cfg{cfg} end to end", end_to_end=True)
+    for f in (c_out, p_out, snap):
+        os.remove(f)
