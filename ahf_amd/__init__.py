@@ -241,7 +241,7 @@ class Context:
     PROF_NQ, PROF_NP = 25, 45
 
     def set_profile_split(self, min_members):
-        """haloes of at least min_members are profiled by eight wavefronts
+        """haloes of at least min_members are profiled by four wavefronts
         (0: default, negative: never); same values either way"""
         self._check(self.lib.ahfgpu_set_profile_split(self.h, int(min_members)))
 

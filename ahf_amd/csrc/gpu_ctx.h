@@ -125,7 +125,7 @@ struct ahfgpu_ctx {
   hipStream_t st3 = nullptr; /* the one-wavefront unbinding tier beside st (default priority) */
   hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr, ev_s2 = nullptr;
   int64_t split_min = 0;     /* haloes of at least this many members take the split path; 0: default */
-  int64_t prof_split = 0;    /* haloes of at least this many members are profiled by eight wavefronts; 0: default */
+  int64_t prof_split = 0;    /* haloes of at least this many members are profiled by four wavefronts; 0: default */
   char err[512] = {0};
   Arena arena;        /* AMR build buffers */
   Arena uarena;       /* unbinding buffers */

@@ -51,7 +51,7 @@
 #define PRF_MINSHELL 10                     /* AHF_MINPART_SHELL, src/param.h:16 */
 #define PRF_NIGNORE 5                       /* AHF_Rmax_r2_NIGNORE, src/param.h:25 */
 #define AHF_PROF_BIG 2048                   /* default: haloes this large take a workgroup of */
-#define AHF_PROF_BIG_NW 8                   /* ... this many wavefronts */
+#define AHF_PROF_BIG_NW 4                   /* ... this many wavefronts */
 
 /* per-bin record columns: the running values at the member that closes a bin */
 enum { BT_A11 = 0, BT_A22, BT_A33, BT_A12, BT_A13, BT_A23, BT_LX, BT_LY, BT_LZ, BT_T, BT_EP, BT_VESC, BT_N };
@@ -94,6 +94,11 @@ struct ProfShared {
   int64_t ri[NW];
   double fin[16];
   int32_t flag;
+  /* the pipelined walk of NW > 1: two chunks in flight */
+  static constexpr int C2 = NW > 1 ? C : 1;
+  double tg[NW > 1 ? 2 : 1][9][C2 + 1]; /* centre of mass x3, inertia x6 of each chunk */
+  int32_t bp2[NW > 1 ? 2 : 1][C2], bc2[NW > 1 ? 2 : 1][C2];
+  uint64_t cm2[2][NW];
 };
 
 /* ------------------------------------------------------- per-bin maths */
@@ -283,6 +288,31 @@ __device__ inline void chain_pass(double *row, int cnt, ProfShared<NW> &S, doubl
     chain_run<WITH_E>(row, i, cnt, acc, acc_e, phi0);
   }
   __syncthreads();
+}
+
+/* chain_pass over explicit event arrays and without the barrier: two
+ * passes of different chunks run on different wavefronts at once */
+template <bool WITH_E, int NW>
+__device__ inline void chain_events(double *row, int cnt, const uint64_t *cm, const int32_t *bprev,
+                                    const int32_t *bcur, ProfShared<NW> &S, double &acc, double &acc_e, double phi0,
+                                    int col, int ecol)
+{
+  if (!row) return;
+  int i = 0;
+  for (int w = 0; w < NW; w++) {
+    uint64_t m = cm[w];
+    while (m) {
+      const int e = 64 * w + __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      chain_run<WITH_E>(row, i, e + 1, acc, acc_e, phi0);
+      i = e + 1;
+      for (int b = bprev[e]; b < bcur[e]; b++) {
+        if (col >= 0) S.bt[b][col] = acc;
+        if (WITH_E && ecol >= 0) S.bt[b][ecol] = acc_e;
+      }
+    }
+  }
+  chain_run<WITH_E>(row, i, cnt, acc, acc_e, phi0);
 }
 
 __device__ inline int wave_prefix_max(int v)
@@ -506,6 +536,236 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
     nvz = vz[q];
   }
   uint32_t qn = C + tid < n ? ip[C + tid] : 0;
+  if constexpr (NW > 1) {
+    /* two chunks in flight: the bulk velocity's sums of chunk k+1 run on the
+     * second wavefront while the first runs chunk k's other sums */
+    struct Chunk {
+      int64_t j0;
+      int cnt, bp, bi;
+      bool proc;
+      double dX, dY, dZ, d, vxp, vyp, vzp, T;
+    };
+    double *const rowV2 = (tid >= 64 && tid < 67) ? S.ta[9 + tid - 64] : nullptr;
+    /* stage chunk j0 into buffer par: geometry, bins, terms */
+    auto stage = [&](int64_t j0, int par, Chunk &r) {
+      const int64_t j = j0 + tid;
+      const bool inb = j < n;
+      r.j0 = j0;
+      r.dX = r.dY = r.dZ = r.d = r.vxp = r.vyp = r.vzp = r.T = 0.0;
+      const float cx = nx, cy = ny, cz = nz, cvx = nvx, cvy = nvy, cvz = nvz;
+      if (j + C < n) {
+        nx = px[qn];
+        ny = py[qn];
+        nz = pz[qn];
+        nvx = vx[qn];
+        nvy = vy[qn];
+        nvz = vz[qn];
+      }
+      qn = j + 2 * C < n ? ip[j + 2 * C] : 0;
+      if (inb) {
+        double dX = (double)cx - Xc, dY = (double)cy - Yc, dZ = (double)cz - Zc;
+        if (dX > 0.5) dX -= 1.0;
+        if (dY > 0.5) dY -= 1.0;
+        if (dZ > 0.5) dZ -= 1.0;
+        if (dX < -0.5) dX += 1.0;
+        if (dY < -0.5) dY += 1.0;
+        if (dZ < -0.5) dZ += 1.0;
+        r.dX = dX;
+        r.dY = dY;
+        r.dZ = dZ;
+        r.d = sqrt(dX * dX + dY * dY + dZ * dZ);
+        r.vxp = (double)cvx;
+        r.vyp = (double)cvy;
+        r.vzp = (double)cvz;
+      }
+      const double d = r.d;
+      S.d_s[tid] = d;
+      int bp, bi;
+      if (mono) {
+        int c = bcar;
+        if (inb) {
+          while (c < nbins && S.rad[c] <= d) c++;
+          if (j == n - 1) c = nbins;
+        }
+        const uint64_t any = __ballot(c > bcar);
+        if (lane == 0) S.hm[wid] = any;
+        __syncthreads();
+        bool some = false;
+        for (int w = 0; w < NW; w++) some |= S.hm[w] != 0;
+        bi = bcar;
+        if (some) {
+          bi = wave_prefix_max(c);
+          if (lane == 63) S.wv[wid] = bi;
+          __syncthreads();
+          for (int w = 0; w < wid; w++) bi = max(bi, S.wv[w]);
+          bi = max(bi, bcar);
+        }
+        S.bc2[par][tid] = bi;
+        __syncthreads();
+        bp = tid ? S.bc2[par][tid - 1] : bcar;
+      } else {
+        __syncthreads();
+        if (tid == 0) {
+          int b = bcar;
+          for (int i = 0; i < C; i++) {
+            S.bp2[par][i] = b;
+            if (b < nbins && j0 + i < n)
+              while (b < nbins && (S.d_s[i] >= S.rad[b] || j0 + i == n - 1)) b++;
+            S.bc2[par][i] = b;
+          }
+        }
+        __syncthreads();
+        bp = S.bp2[par][tid];
+        bi = S.bc2[par][tid];
+      }
+      const bool proc = inb && bp < nbins;
+      {
+        const uint64_t pm = __ballot(proc), cmw = __ballot(proc && bi > bp);
+        if (lane == 0) {
+          S.wv[wid] = __popcll(pm);
+          S.cm2[par][wid] = cmw;
+        }
+      }
+      __syncthreads();
+      int cnt = 0;
+      for (int w = 0; w < NW; w++) cnt += S.wv[w];
+      if (proc) {
+        S.tg[par][0][tid] = Xc + r.dX;
+        S.tg[par][1][tid] = Yc + r.dY;
+        S.tg[par][2][tid] = Zc + r.dZ;
+        S.tg[par][3][tid] = r.dX * r.dX;
+        S.tg[par][4][tid] = r.dY * r.dY;
+        S.tg[par][5][tid] = r.dZ * r.dZ;
+        S.tg[par][6][tid] = r.dX * r.dY;
+        S.tg[par][7][tid] = r.dX * r.dZ;
+        S.tg[par][8][tid] = r.dY * r.dZ;
+        S.ta[9][tid] = r.vxp;
+        S.ta[10][tid] = r.vyp;
+        S.ta[11][tid] = r.vzp;
+        S.bp2[par][tid] = bp;
+        S.bc2[par][tid] = bi;
+        for (int b = bp; b < bi; b++) S.bclose[b] = j;
+        rad_m[j] = d;
+      }
+      __syncthreads();
+      bcar = S.bc2[par][cnt - 1];
+      r.cnt = cnt;
+      r.bp = bp;
+      r.bi = bi;
+      r.proc = proc;
+    };
+    /* the terms about the running bulk velocity (ahf_halos.c:4333-4395) */
+    auto terms = [&](Chunk &r) {
+      const int64_t j = r.j0 + tid;
+      const double M = (double)(j + 1), d = r.d;
+      double dVX = r.vxp - S.ta[9][tid] / M, dVY = r.vyp - S.ta[10][tid] / M, dVZ = r.vzp - S.ta[11][tid] / M;
+      const double lx = r.dY * dVZ - r.dZ * dVY, ly = r.dZ * dVX - r.dX * dVZ, lz = r.dX * dVY - r.dY * dVX;
+      dVX += hub * r.dX * rf / vf;
+      dVY += hub * r.dY * rf / vf;
+      dVZ += hub * r.dZ * rf / vf;
+      r.T = dVX * dVX + dVY * dVY + dVZ * dVZ;
+      const bool fl = r.proc && d > PRF_MZERO;
+      S.i_s[tid] = M / (d * d);
+      {
+        const uint64_t fmw = __ballot(fl);
+        if (lane == 0) S.fm[wid] = fmw;
+      }
+      __syncthreads();
+      int64_t src = -1;
+      {
+        const uint64_t le = S.fm[wid] & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+        if (le) src = 64 * wid + 63 - __clzll((long long)le);
+        for (int w = wid - 1; w >= 0 && src < 0; w--)
+          if (S.fm[w]) src = 64 * w + 63 - __clzll((long long)S.fm[w]);
+      }
+      const double I_now = src >= 0 ? S.i_s[src] : I_car;
+      __syncthreads();
+      S.i_s[tid] = I_now;
+      __syncthreads();
+      const double I_prev = tid ? S.i_s[tid - 1] : I_car, pre = tid ? S.d_s[tid - 1] : d_car;
+      if (r.proc) {
+        S.tc[0][tid] = lx;
+        S.tc[1][tid] = ly;
+        S.tc[2][tid] = lz;
+        S.tc[3][tid] = r.T;
+        S.tc[4][tid] = fl ? ((I_now + I_prev) / 2.) * (d - pre) : -0.0; /* -0.0: an exact no-op */
+        vc2[j] = M / d;
+        const double dV = FourPiThird * ((d * d * d) - (pre * pre * pre));
+        const double hm = (d + pre) / 2.;
+        dr2[j] = 1.0 / dV * (hm * hm);
+      }
+      I_car = S.i_s[r.cnt - 1];
+      d_car = S.d_s[r.cnt - 1];
+      __syncthreads();
+    };
+    /* energies, escape speeds, the most bound member */
+    auto finish = [&](const Chunk &r) {
+      const double U = S.tc[4][tid] - Phi0;
+      const double Ep = 0.5 * r.T + U;
+      if (r.proc)
+        for (int b = r.bp; b < r.bi; b++) S.bt[b][BT_VESC] = 2 * fabs(U);
+      const bool cand = r.proc && Ep < Emin;
+      const uint64_t cw = __ballot(cand);
+      if (lane == 0) S.hm[wid] = cw;
+      __syncthreads();
+      bool some = false;
+      for (int w = 0; w < NW; w++) some |= S.hm[w] != 0;
+      if (some) {
+        double bv = cand ? Ep : 0.0;
+        int64_t bl = cand ? tid : -1;
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) {
+          const double ov = __shfl_xor(bv, dd, 64);
+          const int64_t ol = __shfl_xor(bl, dd, 64);
+          if (ol >= 0 && (bl < 0 || ov < bv || (!(bv < ov) && ol < bl))) {
+            bv = ov;
+            bl = ol;
+          }
+        }
+        if (lane == 0) {
+          S.rv[wid] = bv;
+          S.ri[wid] = bl;
+        }
+        __syncthreads();
+        bv = S.rv[0];
+        bl = S.ri[0];
+        for (int w = 1; w < NW; w++) {
+          const double ov = S.rv[w];
+          const int64_t ol = S.ri[w];
+          if (ol >= 0 && (bl < 0 || ov < bv || (!(bv < ov) && ol < bl))) {
+            bv = ov;
+            bl = ol;
+          }
+        }
+        Emin = bv;
+        mb = r.j0 + bl;
+      }
+      __syncthreads();
+    };
+    Chunk cur, nxt;
+    int par = 0;
+    stage(0, 0, cur);
+    chain_events<false, NW>(rowV2, cur.cnt, S.cm2[0], S.bp2[0], S.bc2[0], S, accV, dummy, 0.0, -1, -1);
+    __syncthreads();
+    terms(cur);
+    for (;;) {
+      const bool next = cur.cnt == C && cur.j0 + C < n && bcar < nbins;
+      if (next) stage(cur.j0 + C, par ^ 1, nxt);
+      /* chunk k+1's bulk velocity beside chunk k's other sums */
+      if (next) chain_events<false, NW>(rowV2, nxt.cnt, S.cm2[par ^ 1], S.bp2[par ^ 1], S.bc2[par ^ 1], S, accV,
+                                        dummy, 0.0, -1, -1);
+      double *const rowCk = tid < 5 ? S.tc[tid] : tid < 14 ? S.tg[par][tid - 5] : nullptr;
+      chain_events<true, NW>(rowCk, cur.cnt, S.cm2[par], S.bp2[par], S.bc2[par], S, accC, accE, Phi0, colC, ecolC);
+      __syncthreads();
+      finish(cur);
+      P += cur.cnt;
+      if (!next) break;
+      terms(nxt);
+      cur = nxt;
+      par ^= 1;
+    }
+    if (tid >= 64 && tid < 67) S.fin[9 + tid - 64] = accV;
+  } else {
   for (int64_t j0 = 0; j0 < n && bcar < nbins; j0 += C) {
     const int64_t j = j0 + tid;
     const bool inb = j < n;
@@ -705,9 +965,10 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
     P += cnt;
     if (cnt < C) break;
   }
+  if (tid < 3) S.fin[9 + tid] = accV;
+  }
   /* members the walk did not reach read as AHF's zeroed arrays */
   for (int64_t j = P + tid; j < n; j += C) rad_m[j] = vc2[j] = dr2[j] = 0.0;
-  if (tid < 3) S.fin[9 + tid] = accV;
   if (tid >= 5 && tid < 8) S.fin[tid - 5] = accC;
   __syncthreads();
 

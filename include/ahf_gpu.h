@@ -230,10 +230,10 @@ typedef struct {
 int ahfgpu_halo_profiles(ahfgpu_ctx *ctx, const ahfgpu_profile_params *p, const double *phi0_rvir, int32_t *nbins);
 int ahfgpu_profile_results(ahfgpu_ctx *ctx, double *props, double *prof);
 /* No counterpart in the reference: haloes of at least min_members members
- * are profiled by workgroups of eight wavefronts on a second stream (the
- * per-member work spread over 512 lanes), the others by one wavefront
- * each.  0 restores the default (2048), a negative value keeps every halo
- * on one wavefront.  Both give the same values. */
+ * are profiled by workgroups of four wavefronts on a second stream (the
+ * per-member work spread over 256 lanes, two chunks in flight), the
+ * others by one wavefront each.  0 restores the default (2048), a negative
+ * value keeps every halo on one wavefront.  Both give the same values. */
 int ahfgpu_set_profile_split(ahfgpu_ctx *ctx, int64_t min_members);
 
 /* device milliseconds of the last build / unbind (HIP events on the

@@ -91,7 +91,7 @@ def test_gather_construct_full_size(oracle, ctx, snapgen, tmp_path):
 @pytest.mark.parametrize("cfg,seed,gather", [(2, 1002, 0), (1, 1001, 1)])
 def test_profiles_full_size(oracle, ctx, snapgen, tmp_path, cfg, seed, gather):
     """HaloProfiles at full size: the 100,000-halo workload (its largest halo
-    of 809,536 members walked by eight wavefronts) and the 256^3 box's
+    of 809,536 members walked by four wavefronts) and the 256^3 box's
     gathered haloes.  Given the oracle's constructed haloes, the profiles
     match the oracle's bit for bit but for the values through log10/pow/log;
     with the construction on the device too, the energies built on Phi0

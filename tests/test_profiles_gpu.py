@@ -72,8 +72,8 @@ CASES = [(2, 800, 1002, 0), (0, 64, 1000, 0), (3, 48, 5, 0), (2, 2000, 1002, 0),
          (1, 64, 1001, 0)]
 
 
-# split: 0 the default (haloes of 2048 members or more on eight wavefronts),
-# 64 nearly every halo on eight wavefronts, -1 every halo on one
+# split: 0 the default (haloes of 2048 members or more on four wavefronts),
+# 64 nearly every halo on four wavefronts, -1 every halo on one
 @pytest.mark.parametrize("split", [0, 64, -1])
 @pytest.mark.parametrize("cfg,side,seed,gather", CASES)
 def test_profiles_of_given_haloes(oracle, snapgen, ctx, tmp_path, cfg, side, seed, gather, split):
