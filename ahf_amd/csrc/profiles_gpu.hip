@@ -24,6 +24,11 @@
  * radii take the same automaton sequentially.  A bin's values are the
  * running sums at the member that closes it.
  *
+ * Haloes of 2048 members or more take a workgroup of four wavefronts (256
+ * members per chunk) on a second stream, with two chunks in flight: the
+ * second wavefront runs the next chunk's bulk-velocity sums while the first
+ * runs the current chunk's other sums.
+ *
  * After the walk the lanes turn to the bins (one lane per bin, at most 64
  * bins below 2^32 members): profile values and the inertia eigen
  * decomposition per bin; then R_max and r2 from the smoothed per-member
