@@ -48,7 +48,7 @@ class UnbindInfo(ctypes.Structure):
     _fields_ = [("n_halos", ctypes.c_int64), ("n_member_in", ctypes.c_int64),
                 ("n_member_out", ctypes.c_int64), ("particle_iterations", ctypes.c_int64),
                 ("n_exact_replays", ctypes.c_int64), ("max_iterations", ctypes.c_int32),
-                ("max_sweeps", ctypes.c_int32)]
+                ("max_sweeps", ctypes.c_int32), ("n_tie_halos", ctypes.c_int64)]
 
 
 class ProfileParams(ctypes.Structure):

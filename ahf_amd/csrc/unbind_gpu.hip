@@ -43,6 +43,7 @@
 #include "devprims.h"
 #include "gpu_ctx.h"
 #include "halo_set.h"
+#include "nr_order.h"
 
 using namespace ahf;
 
@@ -881,15 +882,13 @@ __global__ void k_seed(UnbindConst K, const int64_t *__restrict__ off, int64_t n
 /* persistent form: workgroups take haloes order[q0..q1) from the queue
  * (largest first) */
 template <int BS>
-__global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int64_t *__restrict__ off,
-                                                       const int32_t *__restrict__ order, int64_t q0, int64_t q1,
-                                                       int32_t *__restrict__ next, const int64_t *__restrict__ seed,
-                                                       uint32_t *a, uint32_t *b, double *dist, double *phi,
-                                                       uint8_t *st, Staged g,
-                                                       const double *__restrict__ cen, const float *__restrict__ px,
-                                                       const float *__restrict__ py, const float *__restrict__ pz,
-                                                       const float *__restrict__ vx, const float *__restrict__ vy,
-                                                       const float *__restrict__ vz, HaloOut *__restrict__ out)
+__device__ void unbind_queue(const UnbindConst &K, const int64_t *__restrict__ off, const int32_t *__restrict__ order,
+                             int64_t q0, int64_t q1, int32_t *__restrict__ next, const int64_t *__restrict__ seed,
+                             uint32_t *a, uint32_t *b, double *dist, double *phi, uint8_t *st, Staged g,
+                             const double *__restrict__ cen, const float *__restrict__ px,
+                             const float *__restrict__ py, const float *__restrict__ pz,
+                             const float *__restrict__ vx, const float *__restrict__ vy,
+                             const float *__restrict__ vz, HaloOut *__restrict__ out)
 {
   __shared__ int64_t sh_h;
   for (;;) {
@@ -908,6 +907,20 @@ __global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int6
                                     vy, vz, out + h);
     __syncthreads();
   }
+}
+
+template <int BS>
+__global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int64_t *__restrict__ off,
+                                                       const int32_t *__restrict__ order, int64_t q0, int64_t q1,
+                                                       int32_t *__restrict__ next, const int64_t *__restrict__ seed,
+                                                       uint32_t *a, uint32_t *b, double *dist, double *phi,
+                                                       uint8_t *st, Staged g,
+                                                       const double *__restrict__ cen, const float *__restrict__ px,
+                                                       const float *__restrict__ py, const float *__restrict__ pz,
+                                                       const float *__restrict__ vx, const float *__restrict__ vy,
+                                                       const float *__restrict__ vz, HaloOut *__restrict__ out)
+{
+  unbind_queue<BS>(K, off, order, q0, q1, next, seed, a, b, dist, phi, st, g, cen, px, py, pz, vx, vy, vz, out);
 }
 
 /* plain form: one halo per launch */
@@ -1273,6 +1286,362 @@ __global__ void k_sp_out(const SplitHalo *__restrict__ sh, int64_t nseg, HaloOut
   out[H.h] = o;
 }
 
+/* ------------------------------------------ indexx order of equal keys */
+
+/*
+ * AHF sorts a halo's members with Numerical Recipes' indexx
+ * (src/libutility/general.c:1093-1153, called at ahf_halos.c:5846).  For
+ * distinct distances any sort gives its order; on equal distances indexx
+ * leaves a permutation that depends on the whole array, while the sorts
+ * above keep list order.  Haloes whose sorted keys hold a tie are therefore
+ * sorted again in indexx's own order: all their members laid end to end,
+ * every segment of every such halo partitioned at once, one level per
+ * round of kernels (the per-segment rules are in nr_order.h, where the
+ * partition is derived).  A halo without a tie costs one comparison per
+ * member.
+ */
+
+/* per member of the haloes with ties, and per segment of the replay */
+struct TieWork {
+  int32_t *th;            /* T: the haloes with ties                      */
+  int64_t *tsz, *toff;    /* T, T+1: their sizes and offsets in the replay */
+  int64_t *tmp64;
+  double *key;            /* m: dist^2 in list order                      */
+  uint32_t *idx;          /* m: indexx's indx (1-based, per halo)         */
+  int32_t *seg;           /* m: segment of each position, -1 when settled  */
+  uint8_t *fge, *fle;     /* m: membership in P and Q                     */
+  int32_t *sGE, *sLE;     /* m+1: their exclusive prefix counts           */
+  int32_t *tmp32;
+  uint32_t *P, *Q;        /* m: the lists, at their segment's first position */
+  int32_t *l[2], *ir[2], *b[2]; /* segment bounds and halo start (two levels) */
+  double *a;              /* per segment: pivot key, pivot, element for l  */
+  uint32_t *pv, *cur;
+  int32_t *K, *I, *J, *cnt, *coff;
+  uint8_t *act;
+  int32_t *err;
+};
+
+__global__ void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t v)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+/* tie[h] = 1 where two neighbours of halo h's sorted list have equal keys
+ * (haloes that AHF sorts only) */
+__global__ void k_tie_detect(const int64_t *__restrict__ off, int64_t nh, int64_t ne, int32_t minpart,
+                             const uint32_t *__restrict__ sorted, const double *__restrict__ cen,
+                             const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
+                             uint8_t *__restrict__ tie)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int64_t h = halo_of(off, nh, e);
+  const int64_t o = off[h], n = off[h + 1] - o;
+  if (sort_tier(n, minpart) == 0 || e + 1 >= o + n) return;
+  const double *c = cen + 3 * h;
+  if (dist_key(sorted[e], c, px, py, pz) == dist_key(sorted[e + 1], c, px, py, pz)) tie[h] = 1;
+}
+
+__global__ void k_tie_list(const uint8_t *__restrict__ tie, const int32_t *__restrict__ tpos, int64_t nh,
+                           const int64_t *__restrict__ off, int32_t *__restrict__ th, int64_t *__restrict__ tsz)
+{
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh || !tie[h]) return;
+  th[tpos[h]] = (int32_t)h;
+  tsz[tpos[h]] = off[h + 1] - off[h];
+}
+
+/* the keys in list order (indexx's arr[1..n]), indx = 1..n, and one
+ * segment per halo */
+__global__ void k_tie_init(const int64_t *__restrict__ toff, int64_t T, int64_t m, const int32_t *__restrict__ th,
+                           const int64_t *__restrict__ off, const uint32_t *__restrict__ ip_in,
+                           const double *__restrict__ cen, const float *__restrict__ px, const float *__restrict__ py,
+                           const float *__restrict__ pz, TieWork W)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m) return;
+  const int64_t k = halo_of(toff, T, g);
+  const int64_t h = th[k];
+  const int64_t loc = g - toff[k];
+  W.key[g] = __longlong_as_double((long long)dist_key(ip_in[off[h] + loc], cen + 3 * h, px, py, pz));
+  W.idx[g] = (uint32_t)(loc + 1);
+  W.seg[g] = (int32_t)k;
+  if (loc == 0) {
+    W.l[0][k] = (int32_t)toff[k];
+    W.ir[0][k] = (int32_t)(toff[k + 1] - 1);
+    W.b[0][k] = (int32_t)toff[k];
+  }
+  if (g == 0) *W.err = 0;
+}
+
+/* per segment: straight insertion below 8 elements (then settled), else
+ * the median of three */
+__global__ void k_nr_prep(int64_t S, const int32_t *__restrict__ l, const int32_t *__restrict__ ir,
+                          const int32_t *__restrict__ b, TieWork W)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int64_t L = l[s], R = ir[s], B = b[s];
+  if (R - L < NR_SMALL) {
+    nr_insertion(W.key, W.idx, B, L, R);
+    W.act[s] = 0;
+    return;
+  }
+  W.act[s] = 1;
+  uint32_t pv;
+  W.a[s] = nr_median3(W.key, W.idx, B, L, R, &pv);
+  W.pv[s] = pv;
+}
+
+__global__ void k_nr_flags(int64_t m, const int32_t *__restrict__ l, const int32_t *__restrict__ ir,
+                           const int32_t *__restrict__ b, TieWork W)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m) return;
+  const int32_t s = W.seg[g];
+  unsigned f = 0;
+  if (s >= 0 && W.act[s]) f = nr_flags(nr_key(W.key, W.idx, b[s], g), W.a[s], g, l[s], ir[s]);
+  W.fge[g] = (uint8_t)(f & 1u);
+  W.fle[g] = (uint8_t)(f >> 1);
+}
+
+/* P and Q of every segment, at the segment's first position */
+__global__ void k_nr_lists(int64_t m, const int32_t *__restrict__ l, const int32_t *__restrict__ ir, TieWork W)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m) return;
+  const int32_t s = W.seg[g];
+  if (s < 0 || !W.act[s]) return;
+  const int64_t L = l[s], R = ir[s];
+  if (W.fge[g]) W.P[L + (W.sGE[g] - W.sGE[L + 2])] = (uint32_t)g;
+  if (W.fle[g]) W.Q[L + (W.sLE[R] - W.sLE[g + 1])] = (uint32_t)g;
+}
+
+/* per segment: where indexx's scans stop, and the element that goes to l
+ * (read before the swaps) */
+__global__ void k_nr_meet(int64_t S, const int32_t *__restrict__ l, const int32_t *__restrict__ ir, TieWork W)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S || !W.act[s]) return;
+  const int64_t L = l[s], R = ir[s];
+  const int64_t nP = W.sGE[R + 1] - W.sGE[L + 2], nQ = W.sLE[R] - W.sLE[L];
+  int64_t K, i, j;
+  int js;
+  if (nr_meet(W.P + L, W.Q + L, nP, nQ, &K, &i, &j, &js)) {
+    atomicOr(W.err, 1);
+    K = 0;
+    i = R + 1;
+    j = L;
+    js = 0;
+  }
+  W.K[s] = (int32_t)K;
+  W.I[s] = (int32_t)i;
+  W.J[s] = (int32_t)j;
+  W.cur[s] = js ? W.idx[W.Q[L + K - 1]] : W.idx[j];
+}
+
+/* the K swaps of every segment: position L + k swaps P[k] with Q[k] (the
+ * pairs are disjoint) */
+__global__ void k_nr_swap(int64_t m, const int32_t *__restrict__ l, TieWork W)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m) return;
+  const int32_t s = W.seg[g];
+  if (s < 0 || !W.act[s] || g - l[s] >= W.K[s]) return;
+  const uint32_t p = W.P[g], q = W.Q[g];
+  const uint32_t t = W.idx[p];
+  W.idx[p] = W.idx[q];
+  W.idx[q] = t;
+}
+
+/* per segment: the pivot to j, and how many segments follow */
+__global__ void k_nr_place(int64_t S, const int32_t *__restrict__ l, const int32_t *__restrict__ ir, TieWork W)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  if (!W.act[s]) {
+    W.cnt[s] = 0;
+    return;
+  }
+  const int64_t L = l[s], R = ir[s];
+  W.idx[L] = W.cur[s];
+  W.idx[W.J[s]] = W.pv[s];
+  W.cnt[s] = (W.J[s] - 1 - L >= 1) + (R - W.I[s] >= 1);
+}
+
+__global__ void k_nr_children(int64_t S, const int32_t *__restrict__ l, const int32_t *__restrict__ ir,
+                              const int32_t *__restrict__ b, int32_t *__restrict__ l2, int32_t *__restrict__ ir2,
+                              int32_t *__restrict__ b2, TieWork W)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S || !W.act[s]) return;
+  int64_t c = W.coff[s];
+  const int32_t L = l[s], R = ir[s], B = b[s], I = W.I[s], J = W.J[s];
+  if (J - 1 - L >= 1) {
+    l2[c] = L;
+    ir2[c] = J - 1;
+    b2[c] = B;
+    c++;
+  }
+  if (R - I >= 1) {
+    l2[c] = I;
+    ir2[c] = R;
+    b2[c] = B;
+  }
+}
+
+__global__ void k_nr_reseg(int64_t m, const int32_t *__restrict__ l, const int32_t *__restrict__ ir, TieWork W)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m) return;
+  const int32_t s = W.seg[g];
+  if (s < 0) return;
+  if (!W.act[s]) {
+    W.seg[g] = -1;
+    return;
+  }
+  const int64_t I = W.I[s], J = W.J[s];
+  const int left = J - 1 - l[s] >= 1, right = ir[s] - I >= 1;
+  if (g < J) W.seg[g] = left ? W.coff[s] : -1;
+  else if (g >= I) W.seg[g] = right ? W.coff[s] + left : -1;
+  else W.seg[g] = -1;
+}
+
+/* the lists in indexx's order replace the sorted lists of those haloes */
+__global__ void k_tie_out(int64_t m, const int64_t *__restrict__ toff, int64_t T, const int32_t *__restrict__ th,
+                          const int64_t *__restrict__ off, const uint32_t *__restrict__ ip_in,
+                          const uint32_t *__restrict__ idx, uint32_t *__restrict__ out)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m) return;
+  const int64_t k = halo_of(toff, T, g);
+  const int64_t o = off[th[k]];
+  out[o + (g - toff[k])] = ip_in[o + (int64_t)idx[g] - 1];
+}
+
+static inline unsigned nb_of(int64_t n);
+
+/* the bytes the replay takes for T haloes and m members (taken in this
+ * order by nr_tie_order) */
+static size_t tie_work_bytes(int64_t T, int64_t m)
+{
+  const size_t cs = (size_t)(m / 2 + T + 1), M = (size_t)m + 1, t = (size_t)T + 1;
+  size_t z = 0;
+  auto add = [&](size_t b) { z += (b + 255) & ~(size_t)255; };
+  add(t * 4); add(t * 8); add(t * 8); add((size_t)scan_tmp_elems(T) * 8 + 64);
+  add(M * 8); add(M * 4); add(M * 4); add(M); add(M); add(M * 4); add(M * 4);
+  add((size_t)scan_tmp_elems(m > (int64_t)cs ? m : (int64_t)cs) * 4 + 64); add(M * 4); add(M * 4);
+  for (int k = 0; k < 6; k++) add(cs * 4);
+  add(cs * 8); add(cs * 4); add(cs * 4);
+  for (int k = 0; k < 4; k++) add(cs * 4);
+  add((cs + 1) * 4); add(cs); add(64);
+  return z;
+}
+
+/*
+ * Re-sort in indexx's order the haloes whose sorted list (in `sorted`, the
+ * members as given in ip_in) holds equal keys.  tie/tpos/tmp32h: nh, nh+1
+ * and scan_tmp_elems(nh) words of scratch; take(bytes) hands out device
+ * memory for the replay (nullptr when out).  *n_tie: the haloes re-sorted.
+ * Reads back the halo count, the member count and one segment count per
+ * level.
+ */
+template <class Take>
+static int nr_tie_order(ahfgpu_ctx *c, hipStream_t st, const int64_t *off, int64_t nh, int64_t ne, int32_t minpart,
+                        const uint32_t *ip_in, const double *cen, uint32_t *sorted, uint8_t *tie, int32_t *tpos,
+                        int32_t *tmp32h, Take take, int64_t *n_tie)
+{
+  *n_tie = 0;
+  if (ne < 2 || nh < 1) return 0;
+  hipLaunchKernelGGL(k_fill_u8, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, tie, nh, (uint8_t)0);
+  hipLaunchKernelGGL(k_tie_detect, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, minpart, sorted, cen, c->px,
+                     c->py, c->pz, tie);
+  exclusive_scan<uint8_t, int32_t>(tie, tpos, nh, tmp32h, st);
+  int32_t T = 0;
+  if (hipMemcpyAsync(&T, tpos + nh, sizeof(T), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "radial sort: tie count: %s", hipGetErrorString(hipGetLastError()));
+  if (T == 0) return 0;
+  TieWork W;
+  W.th = (int32_t *)take((size_t)(T + 1) * 4);
+  W.tsz = (int64_t *)take((size_t)(T + 1) * 8);
+  W.toff = (int64_t *)take((size_t)(T + 1) * 8);
+  W.tmp64 = (int64_t *)take((size_t)scan_tmp_elems(T) * 8 + 64);
+  if (!W.th || !W.tsz || !W.toff || !W.tmp64) return c->fail(AHFGPU_ENOMEM, "radial sort: tie buffers");
+  hipLaunchKernelGGL(k_tie_list, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, tie, tpos, nh, off, W.th, W.tsz);
+  exclusive_scan<int64_t, int64_t>(W.tsz, W.toff, T, W.tmp64, st);
+  int64_t m = 0;
+  if (hipMemcpyAsync(&m, W.toff + T, sizeof(m), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "radial sort: tie members: %s", hipGetErrorString(hipGetLastError()));
+  if (m >= ((int64_t)1 << 31) - 64) return c->fail(AHFGPU_EUNSUP, "radial sort: over 2^31 members in haloes with ties");
+  const int64_t cs = m / 2 + T + 1, M = m + 1;
+  W.key = (double *)take((size_t)M * 8);
+  W.idx = (uint32_t *)take((size_t)M * 4);
+  W.seg = (int32_t *)take((size_t)M * 4);
+  W.fge = (uint8_t *)take((size_t)M);
+  W.fle = (uint8_t *)take((size_t)M);
+  W.sGE = (int32_t *)take((size_t)M * 4);
+  W.sLE = (int32_t *)take((size_t)M * 4);
+  W.tmp32 = (int32_t *)take((size_t)scan_tmp_elems(std::max(m, cs)) * 4 + 64);
+  W.P = (uint32_t *)take((size_t)M * 4);
+  W.Q = (uint32_t *)take((size_t)M * 4);
+  for (int k = 0; k < 2; k++) {
+    W.l[k] = (int32_t *)take((size_t)cs * 4);
+    W.ir[k] = (int32_t *)take((size_t)cs * 4);
+    W.b[k] = (int32_t *)take((size_t)cs * 4);
+  }
+  W.a = (double *)take((size_t)cs * 8);
+  W.pv = (uint32_t *)take((size_t)cs * 4);
+  W.cur = (uint32_t *)take((size_t)cs * 4);
+  W.K = (int32_t *)take((size_t)cs * 4);
+  W.I = (int32_t *)take((size_t)cs * 4);
+  W.J = (int32_t *)take((size_t)cs * 4);
+  W.cnt = (int32_t *)take((size_t)cs * 4);
+  W.coff = (int32_t *)take((size_t)(cs + 1) * 4);
+  W.act = (uint8_t *)take((size_t)cs);
+  W.err = (int32_t *)take(64);
+  if (!W.key || !W.idx || !W.seg || !W.fge || !W.fle || !W.sGE || !W.sLE || !W.tmp32 || !W.P || !W.Q || !W.l[0] ||
+      !W.ir[0] || !W.b[0] || !W.l[1] || !W.ir[1] || !W.b[1] || !W.a || !W.pv || !W.cur || !W.K || !W.I || !W.J ||
+      !W.cnt || !W.coff || !W.act || !W.err)
+    return c->fail(AHFGPU_ENOMEM, "radial sort: tie buffers");
+  hipLaunchKernelGGL(k_tie_init, dim3(nb_of(m)), dim3(AHF_BLOCK), 0, st, W.toff, (int64_t)T, m, W.th, off, ip_in, cen,
+                     c->px, c->py, c->pz, W);
+  int64_t S = T;
+  int cur = 0;
+  while (S > 0) {
+    const int32_t *l = W.l[cur], *ir = W.ir[cur], *b = W.b[cur];
+    hipLaunchKernelGGL(k_nr_prep, dim3(nb_of(S)), dim3(AHF_BLOCK), 0, st, S, l, ir, b, W);
+    hipLaunchKernelGGL(k_nr_flags, dim3(nb_of(m)), dim3(AHF_BLOCK), 0, st, m, l, ir, b, W);
+    exclusive_scan<uint8_t, int32_t>(W.fge, W.sGE, m, W.tmp32, st);
+    exclusive_scan<uint8_t, int32_t>(W.fle, W.sLE, m, W.tmp32, st);
+    hipLaunchKernelGGL(k_nr_lists, dim3(nb_of(m)), dim3(AHF_BLOCK), 0, st, m, l, ir, W);
+    hipLaunchKernelGGL(k_nr_meet, dim3(nb_of(S)), dim3(AHF_BLOCK), 0, st, S, l, ir, W);
+    hipLaunchKernelGGL(k_nr_swap, dim3(nb_of(m)), dim3(AHF_BLOCK), 0, st, m, l, W);
+    hipLaunchKernelGGL(k_nr_place, dim3(nb_of(S)), dim3(AHF_BLOCK), 0, st, S, l, ir, W);
+    exclusive_scan<int32_t, int32_t>(W.cnt, W.coff, S, W.tmp32, st);
+    hipLaunchKernelGGL(k_nr_children, dim3(nb_of(S)), dim3(AHF_BLOCK), 0, st, S, l, ir, b, W.l[cur ^ 1],
+                       W.ir[cur ^ 1], W.b[cur ^ 1], W);
+    hipLaunchKernelGGL(k_nr_reseg, dim3(nb_of(m)), dim3(AHF_BLOCK), 0, st, m, l, ir, W);
+    int32_t S2 = 0;
+    if (hipMemcpyAsync(&S2, W.coff + S, sizeof(S2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return c->fail(AHFGPU_EHIP, "radial sort: tie replay: %s", hipGetErrorString(hipGetLastError()));
+    S = S2;
+    cur ^= 1;
+  }
+  int32_t err = 0;
+  if (hipMemcpyAsync(&err, W.err, sizeof(err), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "radial sort: tie replay: %s", hipGetErrorString(hipGetLastError()));
+  if (err) return c->fail(AHFGPU_EUNSUP, "radial sort: indexx replay found an inconsistent partition");
+  hipLaunchKernelGGL(k_tie_out, dim3(nb_of(m)), dim3(AHF_BLOCK), 0, st, m, W.toff, (int64_t)T, W.th, off, ip_in, W.idx,
+                     sorted);
+  *n_tie = T;
+  return 0;
+}
+
 /* ============================================================ host side */
 
 static inline unsigned nb_of(int64_t n)
@@ -1400,25 +1769,22 @@ extern "C" int ahfgpu_set_unbind_split(ahfgpu_ctx *c, int64_t min_members)
   return 0;
 }
 
-extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int mode, double band,
-                             ahfgpu_unbind_info *info)
+/* NminPerHalo within what the seed median supports, and a mode of the
+ * documented combinations (max_run: the highest run form allowed) */
+static int check_unbind_args(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int mode, int max_run)
 {
-  if (!c || !p) return AHFGPU_EINVAL;
-  HaloSet *s = (HaloSet *)c->halo_set;
-  if (!s) return c->fail(AHFGPU_EINVAL, "no halos resident (call ahfgpu_set_halos first)");
   if (p->minpart < 2 || p->minpart / 2 > 64) return c->fail(AHFGPU_EINVAL, "NminPerHalo must be in [2,129]");
-  {
-    const int steps = mode & ~3;
-    const bool ok = (mode & 3) <= 1 &&
-                    (steps == 0 || steps == AHFGPU_UNBIND_SORT_ONLY || steps == AHFGPU_UNBIND_NO_SORT ||
-                     steps == AHFGPU_UNBIND_WITH_RVIR || steps == (AHFGPU_UNBIND_WITH_RVIR | AHFGPU_UNBIND_NO_SORT) ||
-                     steps == AHFGPU_UNBIND_RVIR_ONLY);
-    if (!ok) return c->fail(AHFGPU_EINVAL, "bad unbinding mode %d", mode);
-  }
-  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
-  Arena &A = c->uarena;
-  A.reset();
-  const int64_t ne = s->ne, nh = s->nh;
+  const int steps = mode & ~3;
+  const bool ok = (mode & 3) <= max_run &&
+                  (steps == 0 || steps == AHFGPU_UNBIND_SORT_ONLY || steps == AHFGPU_UNBIND_NO_SORT ||
+                   steps == AHFGPU_UNBIND_WITH_RVIR || steps == (AHFGPU_UNBIND_WITH_RVIR | AHFGPU_UNBIND_NO_SORT) ||
+                   steps == AHFGPU_UNBIND_RVIR_ONLY);
+  if (!ok) return c->fail(AHFGPU_EINVAL, "bad unbinding mode %d", mode);
+  return 0;
+}
+
+static UnbindConst unbind_const(const ahfgpu_unbind_params *p, int mode, double band)
+{
   UnbindConst K;
   K.v2_tune = p->vtune * p->vtune;
   K.v_fac = p->v_fac;
@@ -1435,6 +1801,24 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   K.unbind = (mode & AHFGPU_UNBIND_RVIR_ONLY) == 0;
   K.cut_after = (mode & AHFGPU_UNBIND_WITH_RVIR) != 0;
   K.pad = 0;
+  return K;
+}
+
+extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int mode, double band,
+                             ahfgpu_unbind_info *info)
+{
+  if (!c || !p) return AHFGPU_EINVAL;
+  HaloSet *s = (HaloSet *)c->halo_set;
+  if (!s) return c->fail(AHFGPU_EINVAL, "no halos resident (call ahfgpu_set_halos first)");
+  {
+    const int rc = check_unbind_args(c, p, mode, 1);
+    if (rc) return rc;
+  }
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  Arena &A = c->uarena;
+  A.reset();
+  const int64_t ne = s->ne, nh = s->nh;
+  const UnbindConst K = unbind_const(p, mode, band);
   uint64_t *k0 = A.get<uint64_t>(ne), *k1 = A.get<uint64_t>(ne);
   uint32_t *v0 = A.get<uint32_t>(ne), *v1 = A.get<uint32_t>(ne);
   uint32_t *hk0 = A.get<uint32_t>(ne), *hk1 = A.get<uint32_t>(ne);
@@ -1478,6 +1862,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   const int64_t nsel = (int64_t)bh.size(), nes = bo.back();
   const bool sort_only = (mode & AHFGPU_UNBIND_SORT_ONLY) != 0;
   const bool no_sort = (mode & (AHFGPU_UNBIND_NO_SORT | AHFGPU_UNBIND_RVIR_ONLY)) != 0;
+  int64_t n_tie = 0;
   if (ne > 0 && no_sort) {
     hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->ip_in, ne, s->buf_a);
   } else if (ne > 0) {
@@ -1525,6 +1910,13 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
       hipLaunchKernelGGL(k_gather_members_sel, dim3(nb_of(nes)), dim3(AHF_BLOCK), 0, c->st, d_bo, d_bh, nsel, nes,
                          s->off, vsorted, s->ip_in, s->buf_a);
     }
+    /* haloes with equal distances: indexx's order of them */
+    uint8_t *tie = A.get<uint8_t>(nh);
+    int32_t *tpos = A.get<int32_t>(nh + 1), *tmp32h = A.get<int32_t>(scan_tmp_elems(nh) + 16);
+    if (!tie || !tpos || !tmp32h) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (sort)");
+    const int rc = nr_tie_order(c, c->st, s->off, nh, ne, p->minpart, s->ip_in, s->cen, s->buf_a, tie, tpos, tmp32h,
+                                [&](size_t bytes) { return A.raw(bytes); }, &n_tie);
+    if (rc) return rc;
   }
   c->timer_end(es, TIMER_USORT, 2.0 * 12.0 * (double)ne + 12.0 * (double)ne);
   hipEvent_t eu = c->timer_begin();
@@ -1612,6 +2004,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     memset(info, 0, sizeof(*info));
     info->n_halos = nh;
     info->n_member_in = ne;
+    info->n_tie_halos = n_tie;
     for (auto &o : ho) {
       info->n_member_out += o.npart;
       info->particle_iterations += o.particle_iterations;

@@ -88,6 +88,8 @@ typedef struct {
   int64_t n_exact_replays;             /* halo iterations re-run sequentially  */
   int32_t max_iterations;
   int32_t max_sweeps;                  /* most fixed-point sweeps of one halo  */
+  int64_t n_tie_halos;                 /* haloes with equal distances, sorted
+                                          again in indexx's order of them      */
 } ahfgpu_unbind_info;
 
 int ahfgpu_create(ahfgpu_ctx **ctx, int device);
@@ -206,6 +208,7 @@ int ahfgpu_set_unbind_split(ahfgpu_ctx *ctx, int64_t min_members);
 #define AHFGPU_SET_OVDENS 8
 int ahfgpu_unbind_results(ahfgpu_ctx *ctx, uint64_t *npart, double *props, uint64_t *ipart);
 void ahfgpu_release_halos(ahfgpu_ctx *ctx);
+
 
 /* HaloProfiles (src/libahf/ahf_halos.c:3961-5060, AHF's default flags:
  * unit masses, R_max and r2 from the sorted particle list, Hubble drag) of

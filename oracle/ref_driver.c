@@ -19,6 +19,7 @@
  * Usage:
  *   ahf_ref amr    <snap> <grids.out> [--reps R]   dump the AMR hierarchy
  *   ahf_ref unbind <snap> <halos.out> [--threads T] [--reps R]
+ *   ahf_ref sort   <snap> <halos.out> [--threads T]   sort_halo_particles
  *   ahf_ref construct <snap> <halos.out> [--threads T]
  *   ahf_ref gather <snap> <halos.out> [--threads T]  ahf_halos_sfc_gatherParts
  *   ahf_ref construct-sfc <snap> <halos.out> [--threads T]
@@ -336,6 +337,24 @@ static void cmd_unbind(const snap_data *s, const char *out, int threads)
   double t = run_unbind(h, s->h.nhalos, threads);
   dump_halos(out, h, s->h.nhalos);
   fprintf(stdout, "{\"unbind_seconds\": %.6f, \"threads\": %d}\n", t, threads);
+}
+
+/* sort_halo_particles alone per halo (src/libahf/ahf_halos.c:5786-5894):
+ * the radial order, equal distances in indexx's order */
+static void cmd_sort(const snap_data *s, const char *out, int threads)
+{
+  HALO *h = halo_setup(s);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+  const uint64_t nh = s->h.nhalos;
+  long i;
+#pragma omp parallel for schedule(dynamic) private(i)
+  for (i = 0; i < (long)nh; i++) sort_halo_particles(h + i);
+  dump_halos(out, h, nh);
+  fprintf(stdout, "{\"sorted\": %llu}\n", (unsigned long long)nh);
 }
 
 /* the construction steps of src/libahf/ahf_halos_sfc.c:140-153 after the
@@ -796,6 +815,9 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "unbind")) {
     if (argc < 4) die("unbind needs an output path");
     cmd_unbind(&s, argv[3], threads);
+  } else if (!strcmp(argv[1], "sort")) {
+    if (argc < 4) die("sort needs an output path");
+    cmd_sort(&s, argv[3], threads);
   } else if (!strcmp(argv[1], "construct")) {
     if (argc < 4) die("construct needs an output path");
     cmd_construct(&s, argv[3], threads);

@@ -5,6 +5,7 @@ functions (oracle/_ref/ahf_ref, built from the reference sources by
 Each fixture is data only: the input snapshot's arrays and parameters, and
 the reference's output dump (AHFGRID1, AHFHALO1 or AHFPROF1 bytes) for one command:
   amr            gen_domgrids, ll, zero_dens + assign_npart, gen_AMRhierarchy
+  sort           sort_halo_particles per halo
   unbind         sort_halo_particles + rem_unbound per halo
   construct      sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1)
   gather         ahf_halos_sfc_gatherParts per halo
@@ -30,7 +31,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
 from ahf_amd import dumps  # noqa: E402
-from helpers import fuzz_snapshot  # noqa: E402
+from helpers import fuzz_snapshot, tie_snapshot  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "ahf_ref")
 HEADER = ("boxsize", "omega0", "lambda0", "a", "pmass", "L_dom", "L_max", "Nth_dom", "Nth_ref", "minpart", "vtune",
@@ -42,12 +43,16 @@ CASES = [
     ("amr_fuzz305", ("fuzz", 305, False), ["amr"]),
     ("halo_cfg2_h40", ("gen", 2, 40, 12), ["unbind", "construct", "profiles"]),
     ("sfc_cfg1_side24", ("gen+centres", 1, 24, 11), ["gather", "construct-sfc"]),
+    ("halo_ties_s5", ("ties", 5), ["sort", "unbind", "construct"]),
 ]
 
 
 def make_input(spec, path):
     if spec[0] == "fuzz":
         fuzz_snapshot(path, spec[1], non_pow2=spec[2])
+        return
+    if spec[0] == "ties":
+        tie_snapshot(path, spec[1])
         return
     lib = ctypes.CDLL(os.path.join(ROOT, "ahf_amd", "lib", "libahf_snapgen.so"))
     lib.ahf_snapgen_write.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_char_p]

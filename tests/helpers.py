@@ -90,3 +90,80 @@ def profiles_match(ref, nb, props, prof, tag, end_to_end=False, exact=False):
     print(f"[profiles-gpu] {tag}: {exact_bits}/{total} values bit-exact", flush=True)
 
 
+
+
+def tie_snapshot(path, seed, big=False):
+    """Haloes whose members lie at exactly equal distances from the centre, in
+    shuffled list order, so the radial sort's permutation of equal keys is
+    indexx's to decide (src/libutility/general.c:1093-1153):
+      - cubic lattices around a centre on a lattice point (spacing 2^-10, all
+        coordinates dyadic, so shells of equal dist^2 are exact);
+      - the same across the periodic corner (centre at 0);
+      - random members with a third of them duplicated at the position of
+        another member;
+      - a lattice shell mixed with random members;
+      - a single position repeated (every key equal);
+      - with big, also a 23^3 lattice (12,167 members: the radial sort's
+        path for haloes over 8192 members).
+    Velocities are Gaussian around a halo bulk velocity, with a tail above
+    the escape speed, so unbinding removes some members.  Returns the number
+    of haloes."""
+    rng = np.random.default_rng(seed)
+    dx = 2.0 ** -10
+    pos_parts, lists, centres, n0 = [], [], [], 0
+
+    def add(p, c):
+        nonlocal n0
+        p = np.asarray(p, np.float64) % 1.0
+        pos_parts.append(p)
+        order = rng.permutation(len(p)) + n0
+        lists.append(order.astype(np.uint64))
+        centres.append(c)
+        n0 += len(p)
+
+    for k in range(int(rng.integers(3, 6))):
+        r = int(rng.integers(2, 6))
+        g = np.arange(-r, r + 1) * dx
+        X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+        c = np.round(rng.uniform(0.1, 0.9, 3) / dx) * dx
+        add(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + c, c)
+    r = 3
+    g = np.arange(-r, r + 1) * dx
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    add(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1), np.zeros(3))
+    for k in range(3):
+        c = np.round(rng.uniform(0.1, 0.9, 3) / dx) * dx
+        n = int(rng.integers(60, 600))
+        p = c + np.round(rng.normal(0.0, 8.0, (n, 3))) * dx
+        dup = rng.integers(0, n, n // 3)
+        p[rng.integers(0, n, n // 3)] = p[dup]
+        add(p, c)
+    c = np.round(rng.uniform(0.1, 0.9, 3) / dx) * dx
+    shell = np.array([[i, j, k] for i in (-2, 0, 2) for j in (-2, 0, 2) for k in (-2, 0, 2)], np.float64) * dx
+    add(np.concatenate([c + shell, c + rng.normal(0.0, 4 * dx, (150, 3))]), c)
+    c = np.round(rng.uniform(0.1, 0.9, 3) / dx) * dx
+    add(np.repeat((c + 3 * dx)[None, :], 64, 0), c)
+    if big:
+        g = np.arange(-11, 12) * dx
+        X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+        c = np.round(rng.uniform(0.1, 0.9, 3) / dx) * dx
+        add(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + c, c)
+    pos = np.concatenate(pos_parts).astype(np.float32)
+    pos[pos >= 1.0] = 0.0
+    mom = np.zeros_like(pos)
+    off = 0
+    for lst in lists:
+        n = len(lst)
+        v = rng.normal(0.0, 150.0, (n, 3)) + rng.normal(0.0, 60.0, 3)
+        fast = rng.random(n) < 0.2
+        v[fast] *= 4.0
+        mom[lst.astype(np.int64)] = v
+        off += n
+    bg = rng.random((2000, 3)).astype(np.float32)
+    pos = np.concatenate([pos, bg])
+    mom = np.concatenate([mom, rng.normal(0.0, 0.1, (2000, 3)).astype(np.float32)])
+    # a small box and particle mass so that these haloes bind most members
+    dumps.write_snapshot(path, pos, mom, boxsize=1.0, pmass=1e8, L_dom=16, centres=np.array(centres),
+                         halo_lists=lists, r_fac=1.0, v_fac=1.0, phi_fac=4.3e-9 * 1e8, hubble=0.0,
+                         gather_rad=np.full(len(lists), 0.02), seed=seed)
+    return len(lists)

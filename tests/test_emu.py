@@ -62,3 +62,19 @@ def test_wavefront_fold_replay(tmp_path):
         r = subprocess.run([exe, "300000", str(seed)], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr[-2000:] + r.stdout
         assert '"mismatches": 0' in r.stdout
+
+
+def test_indexx_level_replay(oracle, tmp_path):
+    """the radial sort's level-synchronous indexx (every segment of every
+    halo with ties partitioned at once, ahf_amd/csrc/nr_order.h) replayed on
+    the host against the oracle's sequential indexx on tie-heavy arrays (a
+    few values, all equal, runs, reversed runs, organ pipes, lattice shells,
+    duplicates); any permutation that differs fails"""
+    exe = str(tmp_path / "nr_check")
+    lib = os.path.join(os.path.dirname(HERE), "oracle", "liboracle.so")
+    subprocess.run(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-Wno-unused-function", "-o", exe,
+                    os.path.join(HERE, "emu", "nr_check.c"), lib, "-Wl,-rpath," + os.path.dirname(lib)], check=True)
+    for seed in (1, 2):
+        r = subprocess.run([exe, "2000", str(seed)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:] + r.stdout
+        assert '"mismatches": 0' in r.stdout

@@ -61,7 +61,7 @@ def test_unbind_full_size(oracle, ctx, snapgen, tmp_path, cfg, seed):
     s = dumps.read_snapshot(snap)
     print(f"[full-gpu] unbind cfg{cfg}: {s.nhalos} haloes, {s.n_ipart} members", flush=True)
     out = str(tmp_path / "o.halo")
-    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 1) == 0
+    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 0) == 0
     ref = dumps.read_halos(out)
     ctx.set_unbind_split(0)
     ctx.set_particles(s.pos, s.mom)
@@ -76,7 +76,7 @@ def test_gather_construct_full_size(oracle, ctx, snapgen, tmp_path):
     s = dumps.read_snapshot(snap)
     print(f"[full-gpu] gather + construct cfg1: {s.nhalos} haloes", flush=True)
     out = str(tmp_path / "o.halo")
-    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 8, 1, 1, None) == 0
+    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 8, 0, 1, None) == 0
     ref = dumps.read_halos(out)
     ctx.set_particles(s.pos, s.mom)
     ctx.sfc_keys()

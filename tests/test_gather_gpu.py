@@ -58,7 +58,7 @@ CASES = [(1, 48, 1001), (1, 64, 7), (0, 40, 1000), (3, 48, 5), (2, 400, 1002), (
 def test_gather_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed):
     snap, s = _snap(snapgen, tmp_path, cfg, side, seed)
     out = str(tmp_path / "o.halo")
-    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 8, 1, 0, None) == 0
+    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 8, 0, 0, None) == 0
     ref = dumps.read_halos(out)
     print(f"[gather-gpu] cfg{cfg} side {side}: {s.nhalos} haloes, {int(ref.npart.sum())} members", flush=True)
     ctx.set_particles(s.pos, s.mom)
@@ -76,7 +76,7 @@ def test_gather_construct_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, si
     the device with the lists never leaving it"""
     snap, s = _snap(snapgen, tmp_path, cfg, side, seed)
     out = str(tmp_path / "o.halo")
-    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 8, 1, 1, None) == 0
+    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 8, 0, 1, None) == 0
     ref = dumps.read_halos(out)
     print(f"[gather-gpu] construct cfg{cfg} side {side}", flush=True)
     ctx.set_particles(s.pos, s.mom)
@@ -111,7 +111,7 @@ def test_gather_edge_cases(oracle, ctx, tmp_path):
     snap = str(tmp_path / "e.snap")
     dumps.write_snapshot(snap, pos, centres=cen, halo_lists=[[0]] * 5 + [[]], gather_rad=rad)
     out = str(tmp_path / "e.halo")
-    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 1, 1, 0, None) == 0
+    assert oracle.oracle_gather_dump(snap.encode(), out.encode(), 1, 0, 0, None) == 0
     ref = dumps.read_halos(out)
     ctx.set_particles(pos)
     ctx.sfc_keys()

@@ -53,7 +53,7 @@ def _halos_equal(a, b, phi_rtol=0.0):
 
 
 def test_fixtures_present():
-    assert len(FIXTURES) >= 4
+    assert len(FIXTURES) >= 5
 
 
 @pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p)[:-4] for p in FIXTURES])
@@ -72,6 +72,16 @@ def test_oracle_reproduces_golden(oracle, tmp_path, path):
             a, b = dumps.read_profiles(out), dumps.read_profiles(ref)
             assert np.array_equal(a.npart, b.npart) and int((a.nbins > 0).sum()) > 0
             profiles_match(b, a.nbins, a.props, a.prof, f"golden {cmd}", exact=True)
+            continue
+        if cmd == "sort":
+            r = dumps.read_halos(ref)
+            s = dumps.read_snapshot(snap)
+            off = s.halo_offsets
+            for h in range(s.nhalos):
+                ip = s.ipart[off[h]:off[h + 1]].copy()
+                oracle.oracle_sort_halo(s.pos.ctypes.data, len(ip), ip.ctypes.data, s.centre[h].ctypes.data,
+                                        s.minpart, 0)
+                assert np.array_equal(ip, r.ipart[h]), f"sort: halo {h}"
             continue
         if cmd == "unbind":
             rc = oracle.oracle_unbind_dump(snap.encode(), out.encode(), 1, 0)
@@ -118,6 +128,14 @@ def test_gpu_reproduces_golden(tmp_path, path):
                 profiles_match(rp, nb, props, prof, f"golden gpu {cmd}", end_to_end=True)
                 continue
             r = dumps.read_halos(ref)
+            if cmd == "sort":
+                ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+                ctx.unbind(*uargs, mode=ctx.UNBIND_SORT_ONLY)
+                npart, _, lists = ctx.unbind_results()
+                assert np.array_equal(npart, r.npart)
+                bad = [h for h in range(len(lists)) if not np.array_equal(lists[h], r.ipart[h])]
+                assert not bad, f"sort: {len(bad)} haloes differ, first {bad[:5]}"
+                continue
             if cmd in ("gather", "construct-sfc"):
                 ctx.sfc_keys()
                 ctx.gather_halos(s.centre, s.gather_rad, skip=(s.halo_npart == 0))

@@ -125,6 +125,41 @@ def test_unbind_matches_reference(oracle, ref_bin, snapgen, tmp_path, cfg, side,
     _halo_eq(dumps.read_halos(o), dumps.read_halos(r))
 
 
+@pytest.mark.parametrize("seed,big", [(1, False), (2, True), (3, False), (4, True)])
+def test_ties_match_reference(oracle, ref_bin, tmp_path, seed, big):
+    """haloes at exactly equal distances (lattices, duplicated positions,
+    the periodic corner): the oracle's indexx mode gives the reference's
+    permutation of equal keys in the sort and the same haloes after
+    unbinding and construction; a stable order of equal keys differs from
+    it in every halo, so the comparison decides"""
+    from helpers import tie_snapshot
+    snap = str(tmp_path / "t.snap")
+    nh = tie_snapshot(snap, seed, big=big)
+    s = dumps.read_snapshot(snap)
+    r = str(tmp_path / "r.halo")
+    res = subprocess.run([ref_bin, "sort", snap, r, "--threads", "2"], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-2000:]
+    ref = dumps.read_halos(r)
+    off = s.halo_offsets
+    stable_differs = 0
+    for h in range(nh):
+        ip = s.ipart[off[h]:off[h + 1]].copy()
+        ip2 = ip.copy()
+        oracle.oracle_sort_halo(s.pos.ctypes.data, len(ip), ip.ctypes.data, s.centre[h].ctypes.data, s.minpart, 0)
+        oracle.oracle_sort_halo(s.pos.ctypes.data, len(ip2), ip2.ctypes.data, s.centre[h].ctypes.data, s.minpart, 1)
+        assert np.array_equal(ip, ref.ipart[h]), f"halo {h}"
+        stable_differs += not np.array_equal(ip2, ref.ipart[h])
+    assert stable_differs == nh
+    for cmd, fn in (("unbind", oracle.oracle_unbind_dump), ("construct", oracle.oracle_construct_dump)):
+        o = str(tmp_path / f"o_{cmd}.halo")
+        assert fn(snap.encode(), o.encode(), 2, 0) == 0
+        res = subprocess.run([ref_bin, cmd, snap, r, "--threads", "2"], capture_output=True, text=True)
+        assert res.returncode == 0, res.stderr[-2000:]
+        a, b = dumps.read_halos(o), dumps.read_halos(r)
+        assert 0 < int(a.npart.sum()) < int(s.halo_npart.sum())
+        _halo_eq(a, b)
+
+
 def _unbind_one(oracle, tmp_path, pos, mom, centre, **kw):
     snap = str(tmp_path / "one.snap")
     dumps.write_snapshot(snap, pos, mom, centres=[centre], halo_lists=[np.arange(len(pos))], **kw)

@@ -51,7 +51,7 @@ def test_unbind_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, 
     snap = str(tmp_path / "h.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
     out = str(tmp_path / "o.halo")
-    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 1) == 0
+    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 0) == 0
     ref = dumps.read_halos(out)
     s = dumps.read_snapshot(snap)
     print(f"[unbind-gpu] cfg{cfg} side {side} mode {mode} split {split}: {s.nhalos} haloes {s.n_ipart} members",
@@ -68,7 +68,7 @@ def test_unbind_replay_band(oracle, snapgen, ctx, tmp_path, mode, split):
     snap = str(tmp_path / "b.snap")
     assert snapgen.ahf_snapgen_write(2, 2000, 1002, snap.encode()) == 0
     out = str(tmp_path / "o.halo")
-    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 1) == 0
+    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 8, 0) == 0
     ref = dumps.read_halos(out)
     s = dumps.read_snapshot(snap)
     info, (npart, props, lists) = _run_gpu(ctx, s, mode, band=1e-2, split=split)
@@ -109,7 +109,7 @@ def test_construct_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, see
     snap = str(tmp_path / "c.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
     out = str(tmp_path / "o.halo")
-    assert oracle.oracle_construct_dump(snap.encode(), out.encode(), 8, 1) == 0
+    assert oracle.oracle_construct_dump(snap.encode(), out.encode(), 8, 0) == 0
     ref = dumps.read_halos(out)
     s = dumps.read_snapshot(snap)
     print(f"[construct-gpu] cfg{cfg} side {side} plain {plain}: {s.nhalos} haloes {s.n_ipart} members", flush=True)
@@ -157,3 +157,44 @@ def test_rvir_only_matches_oracle(oracle, snapgen, ctx, tmp_path, plain):
         cut += len(seg) - k
     assert cut > 0
     assert info.max_iterations == 0
+
+
+@pytest.mark.parametrize("seed,big", [(1, False), (2, True), (3, False), (4, True)])
+def test_ties_match_oracle(oracle, ctx, tmp_path, seed, big):
+    """haloes with members at exactly equal distances (lattices, duplicated
+    positions, a halo across the periodic corner, one repeated position,
+    with big a lattice over 8192 members): the device sorts them in
+    indexx's order of equal keys, so the sort alone, the unbinding (batched,
+    per halo, split path) and the construction give the oracle's haloes in
+    its reference-pinned indexx mode"""
+    from helpers import tie_snapshot
+    snap = str(tmp_path / "t.snap")
+    nh = tie_snapshot(snap, seed, big=big)
+    s = dumps.read_snapshot(snap)
+    print(f"[unbind-gpu] ties seed {seed}: {nh} haloes, {s.n_ipart} members", flush=True)
+    ctx.set_particles(s.pos, s.mom)
+    ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+    info = ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=ctx.UNBIND_SORT_ONLY)
+    assert info.n_tie_halos == nh
+    _, _, lists = ctx.unbind_results()
+    off = s.halo_offsets
+    for h in range(nh):
+        ip = s.ipart[off[h]:off[h + 1]].copy()
+        oracle.oracle_sort_halo(s.pos.ctypes.data, len(ip), ip.ctypes.data, s.centre[h].ctypes.data, s.minpart, 0)
+        assert np.array_equal(ip, lists[h]), f"sort: halo {h}"
+    out = str(tmp_path / "o.halo")
+    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 4, 0) == 0
+    ref = dumps.read_halos(out)
+    for mode, split in ((0, 0), (1, 0), (0, 32)):
+        info, (npart, props, lists) = _run_gpu(ctx, s, mode, split=split)
+        assert info.n_tie_halos == nh
+        _compare(ref, npart, props, lists, f"ties {seed} mode {mode} split {split}")
+    ctx.set_unbind_split(0)
+    assert oracle.oracle_construct_dump(snap.encode(), out.encode(), 4, 0) == 0
+    ref = dumps.read_halos(out)
+    ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+    ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=ctx.UNBIND_WITH_RVIR,
+               ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+    npart, props, lists = ctx.unbind_results()
+    _compare(ref, npart, props, lists, f"ties {seed} construct")
+    assert np.array_equal(ref.ovdens, props[:, 3])
