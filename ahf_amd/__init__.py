@@ -14,6 +14,9 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
 MAX_LEVELS = 40
+# limits of ahfgpu_unbind_noalloc (include/ahf_gpu.h)
+NOALLOC_MAX_HALOS = 65536
+NOALLOC_MAX_MEMBERS = 1048576
 
 
 class NativeLibraryMissing(RuntimeError):
@@ -94,6 +97,8 @@ def load():
     lib.ahfgpu_release_halos.argtypes = [P]
     lib.ahfgpu_release_halos.restype = None
     lib.ahfgpu_set_unbind_split.argtypes = [P, ctypes.c_int64]
+    lib.ahfgpu_unbind_noalloc.argtypes = [P, ctypes.POINTER(UnbindParams), ctypes.c_int, ctypes.c_double,
+                                          P, P, P, ctypes.c_int64, P, P, P]
     lib.ahfgpu_set_sfc_keys.argtypes = [P, P, ctypes.c_uint64]
     lib.ahfgpu_sfc_keys.argtypes = [P, P]
     lib.ahfgpu_gather_halos.argtypes = [P, P, P, P, ctypes.c_uint64]

@@ -45,6 +45,9 @@ extern "C" int ahfgpu_create(ahfgpu_ctx **out, int device)
     delete c;
     return AHFGPU_EHIP;
   }
+  /* the scratch of ahfgpu_unbind_noalloc on this device; without it only
+   * that entry point fails (it says so), the rest of the library works */
+  (void)ahf_noalloc_init(device);
   *out = c;
   return 0;
 }

@@ -217,4 +217,8 @@ struct ahfgpu_ctx {
 /* profiles_gpu.hip: drop the last profile result and its buffers */
 void ahfgpu_free_profiles(ahfgpu_ctx *c);
 
+/* unbind_gpu.hip: take the device address of the static scratch of
+ * ahfgpu_unbind_noalloc on `device` (current), once per device */
+int ahf_noalloc_init(int device);
+
 #endif

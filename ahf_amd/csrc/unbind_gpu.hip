@@ -37,6 +37,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ahf_gpu.h"
@@ -921,6 +922,24 @@ __global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int6
                                                        const float *__restrict__ vz, HaloOut *__restrict__ out)
 {
   unbind_queue<BS>(K, off, order, q0, q1, next, seed, a, b, dist, phi, st, g, cen, px, py, pz, vx, vy, vz, out);
+}
+
+/* the same with the queue range read from the device: order[qb[tier] ..
+ * qb[tier + 1]) */
+template <int BS>
+__global__ void __launch_bounds__(BS) k_unbind_batched_dq(UnbindConst K, const int64_t *__restrict__ off,
+                                                          const int32_t *__restrict__ order,
+                                                          const int64_t *__restrict__ qb, int tier,
+                                                          int32_t *__restrict__ next, const int64_t *__restrict__ seed,
+                                                          uint32_t *a, uint32_t *b, double *dist, double *phi,
+                                                          uint8_t *st, Staged g, const double *__restrict__ cen,
+                                                          const float *__restrict__ px, const float *__restrict__ py,
+                                                          const float *__restrict__ pz, const float *__restrict__ vx,
+                                                          const float *__restrict__ vy, const float *__restrict__ vz,
+                                                          HaloOut *__restrict__ out)
+{
+  unbind_queue<BS>(K, off, order, qb[tier], qb[tier + 1], next, seed, a, b, dist, phi, st, g, cen, px, py, pz, vx, vy,
+                   vz, out);
 }
 
 /* plain form: one halo per launch */
@@ -2055,4 +2074,350 @@ extern "C" void ahfgpu_release_halos(ahfgpu_ctx *c)
   if (!c) return;
   free_halos((HaloSet *)c->halo_set);
   c->halo_set = nullptr;
+}
+
+/* ================================== unbinding without device allocation */
+
+/*
+ * ahfgpu_unbind_noalloc: the batched unbinding of haloes handed over in
+ * device memory, for callers that cannot allocate.  Every intermediate
+ * lives in `scratch`, one static array per GPU that the library runs on,
+ * held until the process ends; its address on each device is taken once,
+ * when a context is created there (ahf_noalloc_init).  Nothing here calls
+ * hipMalloc or hipFree.  The steps are those of ahfgpu_unbind in the
+ * batched mode with every halo in the persistent kernel, all laid out on
+ * the device: the largest-first queue order (a stable radix sort of the
+ * sizes), its tier bounds, the radial sort (stable radix sorts on the
+ * distance key and then the halo), the seeds, the three persistent tiers,
+ * and the copy of the results into the caller's arrays.
+ */
+#define AHF_NA_SCRATCH_BYTES ((size_t)192 << 20)
+#define AHF_NA_MAX_DEVICES 64
+
+static __device__ __align__(256) unsigned char scratch[AHF_NA_SCRATCH_BYTES];
+
+static void *g_na_scratch[AHF_NA_MAX_DEVICES]; /* scratch's device address, per device */
+static std::mutex g_na_mu;
+
+/* byte offsets of the arrays of one call inside scratch */
+struct NaLayout {
+  size_t k0, k1, v0, v1, hk0, hk1, rtmp, a, b, dist, phi, st, dx, dy, dz, vx, vy, vz;
+  size_t ok0, ok1, ov0, ov1, ortmp, seed, out, qb, words;
+  size_t tie, tpos, tmp32h, tiew, tiew_bytes; /* the indexx replay of haloes with equal distances */
+  size_t total;
+};
+
+static NaLayout na_layout(int64_t nh, int64_t ne)
+{
+  NaLayout L;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t r = o;
+    o += align256(bytes > 0 ? bytes : 1);
+    return r;
+  };
+  const size_t m = (size_t)(ne > 0 ? ne : 1), h = (size_t)(nh > 0 ? nh : 1);
+  L.k0 = take(m * 8);
+  L.k1 = take(m * 8);
+  L.v0 = take(m * 4);
+  L.v1 = take(m * 4);
+  L.hk0 = take(m * 4);
+  L.hk1 = take(m * 4);
+  L.rtmp = take((size_t)radix_tmp_u32<uint64_t>((int64_t)m) * 4);
+  L.a = take(m * 4);
+  L.b = take(m * 4);
+  L.dist = take(m * 8);
+  L.phi = take(m * 8);
+  L.st = take(m);
+  L.dx = take(m * 8);
+  L.dy = take(m * 8);
+  L.dz = take(m * 8);
+  L.vx = take(m * 4);
+  L.vy = take(m * 4);
+  L.vz = take(m * 4);
+  L.ok0 = take(h * 4);
+  L.ok1 = take(h * 4);
+  L.ov0 = take(h * 4);
+  L.ov1 = take(h * 4);
+  L.ortmp = take((size_t)radix_tmp_u32<uint32_t>((int64_t)h) * 4);
+  L.seed = take(h * 8);
+  L.out = take(h * sizeof(HaloOut));
+  L.qb = take(4 * sizeof(int64_t));
+  L.words = take(8 * sizeof(int32_t));
+  L.tie = take(h);
+  L.tpos = take((h + 1) * 4);
+  L.tmp32h = take((size_t)scan_tmp_elems((int64_t)h) * 4 + 64);
+  L.tiew_bytes = tie_work_bytes((int64_t)h, (int64_t)m);
+  L.tiew = take(L.tiew_bytes);
+  L.total = o;
+  return L;
+}
+
+/* once per device, with that device current: the address of scratch */
+int ahf_noalloc_init(int device)
+{
+  if (device < 0 || device >= AHF_NA_MAX_DEVICES) return AHFGPU_EINVAL;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_scratch[device]) return 0;
+  if (na_layout(AHFGPU_NOALLOC_MAX_HALOS, AHFGPU_NOALLOC_MAX_MEMBERS).total > AHF_NA_SCRATCH_BYTES)
+    return AHFGPU_EINVAL;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return AHFGPU_EHIP;
+  }
+  g_na_scratch[device] = ptr;
+  return 0;
+}
+
+static void *na_scratch_of(int device)
+{
+  if (device < 0 || device >= AHF_NA_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  return g_na_scratch[device];
+}
+
+__global__ void k_na_zero(int32_t *w, int n)
+{
+  if ((int)threadIdx.x < n) w[threadIdx.x] = 0;
+}
+
+/* the caller's device inputs, before any kernel relies on them: offsets
+ * that start at 0, do not decrease and end at ne; members below N.  Bit 0
+ * of *bad: offsets, bit 1: members */
+__global__ void k_na_check(const int64_t *__restrict__ off, int64_t nh, int64_t ne, const uint32_t *__restrict__ ip,
+                           int64_t N, int32_t *__restrict__ bad)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= nh) {
+    const int64_t o = off[i];
+    const bool ok = o >= 0 && o <= ne && (i > 0 || o == 0) && (i < nh || o == ne) && (i == 0 || off[i - 1] <= o);
+    if (!ok) atomicOr(bad, 1);
+  }
+  if (i < ne && (int64_t)ip[i] >= N) atomicOr(bad, 2);
+}
+
+/* queue order: sizes as descending keys, a stable sort keeps equal sizes in
+ * index order (as the host's stable sort does) */
+__global__ void k_na_order_keys(const int64_t *__restrict__ off, int64_t nh, uint32_t *__restrict__ key,
+                                uint32_t *__restrict__ val)
+{
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  key[h] = 0xFFFFFFFFu - (uint32_t)(off[h + 1] - off[h]);
+  val[h] = (uint32_t)h;
+}
+
+/* the first queue position whose halo has fewer than `lim` members (sizes do
+ * not increase along the queue) */
+__device__ inline int64_t na_first_below(const int64_t *off, const int32_t *order, int64_t nh, int64_t lim)
+{
+  int64_t lo = 0, hi = nh;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const int32_t h = order[mid];
+    if (off[h + 1] - off[h] >= lim) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+/* tier bounds of the persistent kernels and their queue counters */
+__global__ void k_na_bounds(const int64_t *__restrict__ off, const int32_t *__restrict__ order, int64_t nh,
+                            int64_t *__restrict__ qb, int32_t *__restrict__ next)
+{
+  if (threadIdx.x != 0) return;
+  const int64_t nbig = na_first_below(off, order, nh, AHF_BIG_HALO);
+  int64_t nmid = na_first_below(off, order, nh, AHF_WAVE_HALO);
+  if (nmid < nbig) nmid = nbig;
+  qb[0] = 0;
+  qb[1] = nbig;
+  qb[2] = nmid;
+  qb[3] = nh;
+  next[0] = next[1] = next[2] = 0;
+}
+
+/* radial sort keys of all members: dist^2 where the halo is sorted, 0 where
+ * AHF leaves the list as it is; the value is the list position */
+__global__ void k_na_member_keys(const int64_t *__restrict__ off, int64_t nh, int64_t ne, int32_t minpart,
+                                 const uint32_t *__restrict__ ip, const double *__restrict__ cen,
+                                 const float *__restrict__ px, const float *__restrict__ py,
+                                 const float *__restrict__ pz, uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int64_t h = halo_of(off, nh, e);
+  const int64_t n = off[h + 1] - off[h];
+  key[e] = sort_tier(n, minpart) != 0 ? dist_key(ip[e], cen + 3 * h, px, py, pz) : 0ull;
+  val[e] = (uint32_t)e;
+}
+
+__global__ void k_na_take(const uint32_t *__restrict__ pos, int64_t ne, const uint32_t *__restrict__ ip,
+                          uint32_t *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ne) out[i] = ip[pos[i]];
+}
+
+__global__ void k_na_out_halo(const HaloOut *__restrict__ out, int64_t nh, uint64_t *__restrict__ npart,
+                              double *__restrict__ props)
+{
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  const HaloOut o = out[h];
+  npart[h] = (uint64_t)o.npart;
+  double *q = props + AHFGPU_PROPS * h;
+  q[0] = o.M_vir;
+  q[1] = o.R_vir;
+  q[2] = o.Phi0;
+  q[3] = o.ovdens;
+  q[4] = (double)o.niter;
+  q[5] = (double)o.set;
+}
+
+__global__ void k_na_out_members(const int64_t *__restrict__ off, int64_t nh, int64_t ne,
+                                 const HaloOut *__restrict__ out, const uint32_t *__restrict__ a,
+                                 const uint32_t *__restrict__ b, uint32_t *__restrict__ dst)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int64_t h = halo_of(off, nh, e);
+  if (e - off[h] < out[h].npart) dst[e] = out[h].in_b ? b[e] : a[e];
+}
+
+extern "C" int ahfgpu_unbind_noalloc(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int mode, double band,
+                                     const double *centres, const int64_t *off, const uint32_t *members, int64_t nh,
+                                     uint64_t *npart_out, double *props_out, uint32_t *members_out)
+{
+  if (!c || !p) return AHFGPU_EINVAL;
+  {
+    const int rc = check_unbind_args(c, p, mode, 0);
+    if (rc) return rc;
+  }
+  if (nh < 0 || nh > AHFGPU_NOALLOC_MAX_HALOS)
+    return c->fail(AHFGPU_EINVAL, "ahfgpu_unbind_noalloc takes at most %d haloes (got %lld)",
+                   AHFGPU_NOALLOC_MAX_HALOS, (long long)nh);
+  if (nh == 0) return 0;
+  if (!centres || !off || !npart_out || !props_out || !members_out)
+    return c->fail(AHFGPU_EINVAL, "ahfgpu_unbind_noalloc: null device array");
+  if (c->N <= 0) return c->fail(AHFGPU_EINVAL, "no particles resident");
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  char *base = (char *)na_scratch_of(c->device);
+  if (!base) {
+    /* set up when the context was created; retried here, as that call may
+     * have failed */
+    if (ahf_noalloc_init(c->device) != 0 || !(base = (char *)na_scratch_of(c->device)))
+      return c->fail(AHFGPU_EHIP, "ahfgpu_unbind_noalloc: no device address for the scratch array of device %d",
+                     c->device);
+  }
+  hipStream_t st = c->st;
+  /* the member count: the last offset, after checking the offsets' ends */
+  int64_t ne = 0;
+  if (hipMemcpyAsync(&ne, off + nh, sizeof(ne), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "ahfgpu_unbind_noalloc: reading off[nh]: %s", hipGetErrorString(hipGetLastError()));
+  if (ne < 0 || ne > AHFGPU_NOALLOC_MAX_MEMBERS)
+    return c->fail(AHFGPU_EINVAL, "ahfgpu_unbind_noalloc takes at most %d members (off[nh] = %lld)",
+                   AHFGPU_NOALLOC_MAX_MEMBERS, (long long)ne);
+  if (ne > 0 && !members) return c->fail(AHFGPU_EINVAL, "ahfgpu_unbind_noalloc: null member list");
+  const NaLayout L = na_layout(nh, ne);
+  if (L.total > AHF_NA_SCRATCH_BYTES) return c->fail(AHFGPU_EINVAL, "ahfgpu_unbind_noalloc: scratch too small");
+  uint64_t *k0 = (uint64_t *)(base + L.k0), *k1 = (uint64_t *)(base + L.k1);
+  uint32_t *v0 = (uint32_t *)(base + L.v0), *v1 = (uint32_t *)(base + L.v1);
+  uint32_t *hk0 = (uint32_t *)(base + L.hk0), *hk1 = (uint32_t *)(base + L.hk1);
+  uint32_t *rtmp = (uint32_t *)(base + L.rtmp);
+  uint32_t *buf_a = (uint32_t *)(base + L.a), *buf_b = (uint32_t *)(base + L.b);
+  double *dist = (double *)(base + L.dist), *phi = (double *)(base + L.phi);
+  uint8_t *stt = (uint8_t *)(base + L.st);
+  const Staged g = {(double *)(base + L.dx), (double *)(base + L.dy), (double *)(base + L.dz),
+                    (float *)(base + L.vx),  (float *)(base + L.vy),  (float *)(base + L.vz)};
+  uint32_t *ok0 = (uint32_t *)(base + L.ok0), *ok1 = (uint32_t *)(base + L.ok1);
+  uint32_t *ov0 = (uint32_t *)(base + L.ov0), *ov1 = (uint32_t *)(base + L.ov1);
+  uint32_t *ortmp = (uint32_t *)(base + L.ortmp);
+  int64_t *seed = (int64_t *)(base + L.seed);
+  HaloOut *hout = (HaloOut *)(base + L.out);
+  int64_t *qb = (int64_t *)(base + L.qb);
+  int32_t *words = (int32_t *)(base + L.words); /* [0..2] queue counters, [3] input check */
+  /* the caller's offsets and members are checked before anything reads
+   * through them; scratch keeps the last call's contents, so the check
+   * word is cleared first */
+  hipLaunchKernelGGL(k_na_zero, dim3(1), dim3(64), 0, st, words, 8);
+  const int64_t nchk = std::max<int64_t>(nh + 1, ne);
+  hipLaunchKernelGGL(k_na_check, dim3(nb_of(nchk)), dim3(AHF_BLOCK), 0, st, off, nh, ne, members, c->N, words + 3);
+  int32_t bad = 0;
+  if (hipMemcpyAsync(&bad, words + 3, sizeof(bad), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "ahfgpu_unbind_noalloc: input check: %s", hipGetErrorString(hipGetLastError()));
+  if (bad & 1) return c->fail(AHFGPU_EINVAL, "ahfgpu_unbind_noalloc: offsets must start at 0 and not decrease");
+  if (bad & 2) return c->fail(AHFGPU_EINVAL, "ahfgpu_unbind_noalloc: a member refers to a particle beyond the resident set");
+  const UnbindConst K = unbind_const(p, mode, band);
+  /* largest first, and the tier bounds */
+  hipLaunchKernelGGL(k_na_order_keys, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, off, nh, ok0, ov0);
+  const int wo = radix_sort<uint32_t, uint32_t>(ok0, ov0, ok1, ov1, nh, 32, ortmp, st);
+  const int32_t *order = (const int32_t *)(wo ? ov1 : ov0);
+  hipLaunchKernelGGL(k_na_bounds, dim3(1), dim3(64), 0, st, off, order, nh, qb, words);
+  /* radial sort (or the lists as given) into buf_a */
+  const bool sort_only = (mode & AHFGPU_UNBIND_SORT_ONLY) != 0;
+  const bool no_sort = (mode & (AHFGPU_UNBIND_NO_SORT | AHFGPU_UNBIND_RVIR_ONLY)) != 0;
+  if (ne > 0 && no_sort) {
+    hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, members, ne, buf_a);
+  } else if (ne > 0) {
+    hipLaunchKernelGGL(k_na_member_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, p->minpart, members,
+                       centres, c->px, c->py, c->pz, k0, v0);
+    const int w = radix_sort<uint64_t, uint32_t>(k0, v0, k1, v1, ne, 64, rtmp, st);
+    uint32_t *vs = w ? v1 : v0, *vo = w ? v0 : v1;
+    hipLaunchKernelGGL(k_halo_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, vs, hk0);
+    int hb = 1;
+    while ((1ll << hb) < nh) hb++;
+    const int w2 = radix_sort<uint32_t, uint32_t>(hk0, vs, hk1, vo, ne, hb, rtmp, st);
+    hipLaunchKernelGGL(k_na_take, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, w2 ? vo : vs, ne, members, buf_a);
+    /* haloes with equal distances: indexx's order of them, in the region
+     * reserved for the largest replay these sizes allow */
+    char *tw = base + L.tiew;
+    size_t tw_left = L.tiew_bytes;
+    auto take_tw = [&](size_t bytes) -> void * {
+      bytes = align256(bytes > 0 ? bytes : 1);
+      if (bytes > tw_left) return nullptr;
+      void *r = tw;
+      tw += bytes;
+      tw_left -= bytes;
+      return r;
+    };
+    int64_t n_tie = 0;
+    const int rc = nr_tie_order(c, st, off, nh, ne, p->minpart, members, centres, buf_a, (uint8_t *)(base + L.tie),
+                                (int32_t *)(base + L.tpos), (int32_t *)(base + L.tmp32h), take_tw, &n_tie);
+    if (rc) return rc;
+  }
+  if (sort_only) {
+    hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, off, nh, hout);
+  } else {
+    if (K.unbind)
+      hipLaunchKernelGGL(k_seed, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, K, off, nh, buf_a, c->vx, c->vy, c->vz,
+                         seed);
+    int dev_cu = 256;
+    hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+    /* the one-wavefront tier on the third stream, as in the batched form */
+    hipEventRecord(c->ev_s0, st);
+    hipStreamWaitEvent(c->st3, c->ev_s0, 0);
+    hipLaunchKernelGGL(k_unbind_batched_dq<AHF_BIG_BLOCK>, dim3((unsigned)std::min<int64_t>(nh, 2 * dev_cu)),
+                       dim3(AHF_BIG_BLOCK), 0, st, K, off, order, qb, 0, words, seed, buf_a, buf_b, dist, phi, stt, g,
+                       centres, c->px, c->py, c->pz, c->vx, c->vy, c->vz, hout);
+    hipLaunchKernelGGL(k_unbind_batched_dq<AHF_BLOCK>, dim3((unsigned)std::min<int64_t>(nh, 8 * dev_cu)),
+                       dim3(AHF_BLOCK), 0, st, K, off, order, qb, 1, words + 1, seed, buf_a, buf_b, dist, phi, stt, g,
+                       centres, c->px, c->py, c->pz, c->vx, c->vy, c->vz, hout);
+    hipLaunchKernelGGL(k_unbind_batched_dq<64>, dim3((unsigned)std::min<int64_t>(nh, 32 * dev_cu)), dim3(64), 0,
+                       c->st3, K, off, order, qb, 2, words + 2, seed, buf_a, buf_b, dist, phi, stt, g, centres, c->px,
+                       c->py, c->pz, c->vx, c->vy, c->vz, hout);
+    hipEventRecord(c->ev_s2, c->st3);
+    hipStreamWaitEvent(st, c->ev_s2, 0);
+  }
+  hipLaunchKernelGGL(k_na_out_halo, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, hout, nh, npart_out, props_out);
+  if (ne > 0)
+    hipLaunchKernelGGL(k_na_out_members, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, hout, buf_a, buf_b,
+                       members_out);
+  const hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return c->fail(AHFGPU_EHIP, "ahfgpu_unbind_noalloc: %s", hipGetErrorString(e));
+  const hipError_t e2 = hipGetLastError();
+  if (e2 != hipSuccess) return c->fail(AHFGPU_EHIP, "ahfgpu_unbind_noalloc: %s", hipGetErrorString(e2));
+  return 0;
 }

@@ -209,6 +209,33 @@ int ahfgpu_set_unbind_split(ahfgpu_ctx *ctx, int64_t min_members);
 int ahfgpu_unbind_results(ahfgpu_ctx *ctx, uint64_t *npart, double *props, uint64_t *ipart);
 void ahfgpu_release_halos(ahfgpu_ctx *ctx);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ahfgpu_set_halos + ahfgpu_unbind in the batched mode
+ * (mode 0 or-ed with one step selection; every halo in the persistent
+ * kernel) + ahfgpu_unbind_results in one call, with the haloes and the
+ * results in the caller's DEVICE memory and every intermediate in one static
+ * device array per GPU of 192 MiB, held until the process ends.  The call
+ * allocates nothing.
+ *   centres: nh x 3 FP64 (box units); off: nh+1 int64 CSR offsets (off[0] =
+ *   0, not decreasing); members: off[nh] indices into the resident particles
+ *   of ctx; npart_out: nh; props_out: nh x AHFGPU_PROPS (as
+ *   ahfgpu_unbind_results); members_out: off[nh] entries, halo h's kept
+ *   members at members_out[off[h] .. off[h] + npart_out[h]) (the entries
+ *   after them are left as they were).
+ * At most AHFGPU_NOALLOC_MAX_HALOS haloes and AHFGPU_NOALLOC_MAX_MEMBERS
+ * members; larger inputs, bad offsets or members fail with AHFGPU_EINVAL
+ * before any result is written.  The work runs on the context's stream and
+ * the call returns when it is done; the inputs must be ready before the
+ * call.  Every call on one GPU shares that GPU's array and leaves its
+ * contents behind: calls that overlap on one GPU (two threads, two
+ * contexts) race. */
+#define AHFGPU_NOALLOC_MAX_HALOS 65536
+#define AHFGPU_NOALLOC_MAX_MEMBERS 1048576
+int ahfgpu_unbind_noalloc(ahfgpu_ctx *ctx, const ahfgpu_unbind_params *p, int mode, double band,
+                          const double *centres, const int64_t *off, const uint32_t *members, int64_t nh,
+                          uint64_t *npart_out, double *props_out, uint32_t *members_out);
+
 
 /* HaloProfiles (src/libahf/ahf_halos.c:3961-5060, AHF's default flags:
  * unit masses, R_max and r2 from the sorted particle list, Hubble drag) of
