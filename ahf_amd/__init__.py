@@ -151,7 +151,7 @@ class Context:
         self._check(self.lib.ahfgpu_amr_build(self.h, ctypes.byref(p), ctypes.byref(info)))
         return info
 
-    GATHER_AUTO, GATHER_SOURCE, GATHER_FLAT, GATHER_WAVE = 0, 1, 2, 3
+    GATHER_AUTO, GATHER_SOURCE, GATHER_FLAT, GATHER_WAVE, GATHER_FLAT_WIDE = 0, 1, 2, 3, 4
 
     def set_unbind_split(self, min_members):
         """haloes of at least min_members go through the split unbinding path
@@ -161,6 +161,33 @@ class Context:
     def set_gather_mode(self, mode):
         """deposit kernel choice (same results in every mode)"""
         self._check(self.lib.ahfgpu_set_gather_mode(self.h, int(mode)))
+
+    def amr_level(self, lev, lists=True):
+        """level `lev` of the last hierarchy: dict of L, critdens,
+        masstopartdens, oct_key (n_oct), oct_link, dens (8 n_oct, octet-major
+        nodes) and, with lists, list0 (8 n_oct + 1) and list (particle
+        indices in AHF's list order)"""
+        import numpy as np
+        L, no, nl = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        crit, m2pd = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.ahfgpu_amr_level_sizes(self.h, lev, ctypes.byref(L), ctypes.byref(no), ctypes.byref(nl),
+                                                    ctypes.byref(crit), ctypes.byref(m2pd)))
+        n_oct = no.value
+        key = np.zeros(max(n_oct, 1), np.uint64)
+        link = np.zeros(max(n_oct, 1), np.uint8)
+        dens = np.zeros(max(8 * n_oct, 1), np.float32)
+        out = {"L": L.value, "critdens": crit.value, "masstopartdens": m2pd.value}
+        if lists:
+            list0 = np.zeros(8 * n_oct + 1, np.int64)
+            lst = np.zeros(max(nl.value, 1), np.uint32)
+            self._check(self.lib.ahfgpu_amr_level_copy(self.h, lev, key.ctypes.data, link.ctypes.data,
+                                                       dens.ctypes.data, list0.ctypes.data, lst.ctypes.data))
+            out.update(list0=list0, list=lst[:int(list0[-1])])
+        else:
+            self._check(self.lib.ahfgpu_amr_level_copy(self.h, lev, key.ctypes.data, link.ctypes.data,
+                                                       dens.ctypes.data, None, None))
+        out.update(oct_key=key[:n_oct], oct_link=link[:n_oct], dens=dens[:8 * n_oct])
+        return out
 
     def amr_dump(self, path):
         self._check(self.lib.ahfgpu_amr_dump(self.h, path.encode()))

@@ -1930,7 +1930,8 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   } else if (dense)
     hipLaunchKernelGGL(k_gather_src, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0, l.qp,
                        descending, l.m2pd, kmax, l.dens, heavy, n_heavy);
-  else if ((uint64_t)(n_ent > 0 ? n_ent : 1) * sizeof(ahf_w9) < ((uint64_t)1 << 32))
+  else if ((uint64_t)(n_ent > 0 ? n_ent : 1) * sizeof(ahf_w9) < ((uint64_t)1 << 32) &&
+           c->gather_mode != AHFGPU_GATHER_FLAT_WIDE)
     hipLaunchKernelGGL(k_gather_light<true>, grid, dim3(AHF_BLOCK), 0, c->st, v, tord, per_xcd, l.pmask, l.node_p0,
                        l.W, descending, kmax, l.dens, heavy, n_heavy);
   else
@@ -2129,7 +2130,7 @@ static int refine_level(ahfgpu_ctx *c, int ci)
    * the next level's sizes */
   const int gm = c->gather_mode;
   const bool full_f = gm == AHFGPU_GATHER_AUTO && f.L % AHF_SLAB == 0 && f.n_oct == (f.L / 2) * (f.L / 2) * (f.L / 2);
-  const bool decided = gm == AHFGPU_GATHER_FLAT || gm == AHFGPU_GATHER_SOURCE || full_f ||
+  const bool decided = gm == AHFGPU_GATHER_FLAT || gm == AHFGPU_GATHER_FLAT_WIDE || gm == AHFGPU_GATHER_SOURCE || full_f ||
                        nent < AHF_DENSE_ENTRIES_PER_NODE * nn2;
   int64_t nent_f = nent;
   f.n_part = -1;

@@ -144,7 +144,7 @@ extern "C" int ahfgpu_set_gather_mode(ahfgpu_ctx *c, int mode)
 {
   if (!c) return AHFGPU_EINVAL;
   if (mode != AHFGPU_GATHER_AUTO && mode != AHFGPU_GATHER_SOURCE && mode != AHFGPU_GATHER_FLAT &&
-      mode != AHFGPU_GATHER_WAVE)
+      mode != AHFGPU_GATHER_WAVE && mode != AHFGPU_GATHER_FLAT_WIDE)
     return c->fail(AHFGPU_EINVAL, "unknown gather mode %d", mode);
   c->gather_mode = mode;
   return 0;

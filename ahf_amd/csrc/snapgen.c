@@ -1,7 +1,7 @@
 /*
  * snapgen.c -- seeded synthetic snapshots for the deposit + unbinding path.
  *
- * Writes the project's snapshot container (oracle/snapfile.h) holding
+ * Writes the project's snapshot container (snapfile.h) holding
  * particles in AHF's internal units (positions in box units, velocities in
  * units where v * v_fac = peculiar km/s, v_fac = boxsize/t_unit/a with
  * t_unit = 1/H0, src/startrun.c:547 and src/libahf/ahf_halos.c:201) and the
@@ -20,9 +20,19 @@
  *             c ~ U(4,20), 15% of particles above escape speed and 5% within
  *             1e-4 (relative) of VescTune * v_esc.  No background, no grid.
  *   config 3: 512^3, 300 Mpc/h, 70/30 background/haloes, 20000 hosts from the
- *             config-1 mass function.  LgridDomain 256.
+ *             config-1 mass function.  LgridDomain 256, LgridMax
+ *             LgridDomain * 2^10 (up to 10 refinement levels).
+ *   config 4: 896^3, 300 Mpc/h, 200000 hosts from the config-1 mass
+ *             function; hosts above 1e12 carry 5-20 subhaloes holding 10% of
+ *             the host mass, and every subhalo again carries subhaloes
+ *             holding 10% of its own mass, three nested levels deep (3-8 at
+ *             the second level, 2-4 at the third, where each would hold at
+ *             least NminPerHalo particles); 10% interlopers.  A halo's list
+ *             holds its own particles and those of all its subhaloes.
+ *             LgridDomain 448, LgridMax LgridDomain * 2^12 (12 levels).
  * All use Omega_m = 0.3, Omega_L = 0.7, a = 1, VescTune 1.5, NminPerHalo 20,
- * NperDomCell = NperRefCell = 5, LgridMax = LgridDomain * 2^8.
+ * NperDomCell = NperRefCell = 5, and unless stated LgridMax = LgridDomain *
+ * 2^8.
  *
  * Randomness is a counter-based splitmix64 stream per (seed, object), so the
  * file is byte-identical for a given (config, seed, side) whatever the
@@ -41,7 +51,7 @@
 #include <omp.h>
 #endif
 
-#include "../../oracle/snapfile.h"
+#include "snapfile.h"
 #include "sfc_core.h"
 
 #define RHOC0 2.7755397e11   /* h^2 Msun / Mpc^3, src/param.h:38 */
@@ -146,15 +156,77 @@ static void place_halo(halo_spec *h, const gen_params *g, rng_t *r)
   }
 }
 
-/* hosts (and for config 1 their subhaloes); returns count */
-static int64_t make_halos(const gen_params *g, uint64_t seed, int64_t nhost, int subs,
+/* room for one more halo spec */
+static halo_spec *halo_push(halo_spec **hs, int64_t *nh, int64_t *cap)
+{
+  if (*nh == *cap) {
+    const int64_t nc = *cap * 2 + 64;
+    halo_spec *p = (halo_spec *)realloc(*hs, (size_t)nc * sizeof(halo_spec));
+    if (!p) return NULL;
+    memset(p + *cap, 0, (size_t)(nc - *cap) * sizeof(halo_spec));
+    *hs = p;
+    *cap = nc;
+  }
+  return &(*hs)[(*nh)++];
+}
+
+/*
+ * ns subhaloes of halo `parent` (an index: the array may move) holding 10%
+ * of its mass, placed inside its NFW profile (mass fraction 0.1-0.9) with
+ * its bulk velocity plus its virial dispersion; the parent keeps 90%.
+ * depth > 1 gives each new subhalo subhaloes of its own (nested levels).
+ */
+static int add_subs(const gen_params *g, rng_t *r, halo_spec **hs, int64_t *nh, int64_t *cap, int64_t parent, int ns,
+                    int depth)
+{
+  static const int lo_n[4] = {0, 5, 3, 2}, span_n[4] = {0, 16, 6, 3};
+  double w[20], wsum = 0.0;
+  for (int k = 0; k < ns; k++) { w[k] = 0.2 + rng_uniform(r); wsum += w[k]; }
+  const double Mp = (*hs)[parent].M, cp = (*hs)[parent].c, rv = r_vir_of(Mp), rs = rv / cp;
+  const double sig = sqrt(GRAV * Mp / (2.0 * rv));
+  (*hs)[parent].M = 0.9 * Mp;
+  const int64_t first = *nh;
+  for (int k = 0; k < ns; k++) {
+    halo_spec *s = halo_push(hs, nh, cap);
+    if (!s) return -1;
+    const halo_spec *P = &(*hs)[parent];
+    s->M = 0.1 * Mp * w[k] / wsum;
+    s->c = 5.0 + 10.0 * rng_uniform(r);
+    s->frac_unb = 0.10;
+    s->host = (int)parent;
+    double rr = nfw_radius(0.1 + 0.8 * rng_uniform(r), P->c, rs), dir[3];
+    rng_dir(r, dir);
+    for (int d = 0; d < 3; d++) {
+      s->cen[d] = P->cen[d] + rr * dir[d];
+      s->vbulk[d] = P->vbulk[d] + sig * rng_gauss(r);
+    }
+  }
+  if (depth > 1) {
+    const int level = 4 - depth + 1; /* nesting level of the grandchildren: 2 or 3 */
+    for (int64_t i = first; i < first + ns; i++) {
+      int nc = lo_n[level] + (int)(rng_uniform(r) * (double)span_n[level]);
+      if (nc > lo_n[level] + span_n[level] - 1) nc = lo_n[level] + span_n[level] - 1;
+      /* only where every grandchild would hold NminPerHalo particles */
+      while (nc > 0 && 0.1 * (*hs)[i].M / (double)nc < 20.0 * g->pmass) nc--;
+      if (nc > 0 && add_subs(g, r, hs, nh, cap, i, nc, depth - 1)) return -1;
+    }
+  }
+  return 0;
+}
+
+/* hosts and their subhaloes (sub_depth nested levels below hosts above
+ * 1e12 Msun/h: 0 none, 1 config 1, 3 config 4); returns the count, -1 when
+ * out of memory */
+static int64_t make_halos(const gen_params *g, uint64_t seed, int64_t nhost, int sub_depth,
                           double mlo, double mhi, int logflat, halo_spec **out)
 {
   int64_t cap = nhost * 22 + 16, nh = 0;
   halo_spec *hs = (halo_spec *)calloc((size_t)cap, sizeof(halo_spec));
+  if (!hs) return -1;
   rng_t r = rng_make(seed, 1);
   for (int64_t i = 0; i < nhost; i++) {
-    halo_spec *h = &hs[nh++];
+    halo_spec *h = halo_push(&hs, &nh, &cap);
+    if (!h) { free(hs); return -1; }
     h->M = logflat ? exp(log(mlo) + rng_uniform(&r) * (log(mhi) - log(mlo)))
                    : powerlaw(&r, 1.9, mlo, mhi);
     h->c = 5.0 + 10.0 * rng_uniform(&r);
@@ -162,28 +234,10 @@ static int64_t make_halos(const gen_params *g, uint64_t seed, int64_t nhost, int
     h->frac_edge = 0.0;
     h->host = -1;
     place_halo(h, g, &r);
-    if (subs && h->M > 1e12) {
+    if (sub_depth > 0 && h->M > 1e12) {
       int ns = 5 + (int)(rng_uniform(&r) * 16.0);
       if (ns > 20) ns = 20;
-      double w[20], wsum = 0.0;
-      for (int k = 0; k < ns; k++) { w[k] = 0.2 + rng_uniform(&r); wsum += w[k]; }
-      double Mhost = h->M, rv = r_vir_of(Mhost), rs = rv / h->c;
-      double sig = sqrt(GRAV * Mhost / (2.0 * rv));
-      int64_t hi = nh - 1;
-      hs[hi].M = 0.9 * Mhost;
-      for (int k = 0; k < ns; k++) {
-        halo_spec *s = &hs[nh++];
-        s->M = 0.1 * Mhost * w[k] / wsum;
-        s->c = 5.0 + 10.0 * rng_uniform(&r);
-        s->frac_unb = 0.10;
-        s->host = (int)hi;
-        double rr = nfw_radius(0.1 + 0.8 * rng_uniform(&r), hs[hi].c, rs), dir[3];
-        rng_dir(&r, dir);
-        for (int d = 0; d < 3; d++) {
-          s->cen[d] = hs[hi].cen[d] + rr * dir[d];
-          s->vbulk[d] = hs[hi].vbulk[d] + sig * rng_gauss(&r);
-        }
-      }
+      if (add_subs(g, &r, &hs, &nh, &cap, nh - 1, ns, sub_depth)) { free(hs); return -1; }
     }
   }
   *out = hs;
@@ -195,6 +249,64 @@ static int cmp_u64pair(const void *a, const void *b)
   const uint64_t *x = (const uint64_t *)a, *y = (const uint64_t *)b;
   if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
   return x[1] < y[1] ? -1 : (x[1] > y[1]);
+}
+
+/*
+ * Sort n (key, index) pairs by key, then index (the same total order as one
+ * qsort with cmp_u64pair): a parallel scatter into 2^16 buckets by the top
+ * 16 of the 63 key bits, then each bucket sorted on its own.  Returns -1
+ * when out of memory.
+ */
+static int sort_pairs(uint64_t *kv, int64_t n)
+{
+  enum { BB = 16, NB = 1 << BB };
+  const int shift = 3 * AHF_SFC_BITS - BB;
+  uint64_t *tmp = (uint64_t *)malloc((size_t)(n ? n : 1) * 2 * sizeof(uint64_t));
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  int64_t *cnt = (int64_t *)calloc((size_t)nt * NB + NB + 1, sizeof(int64_t));
+  if (!tmp || !cnt) { free(tmp); free(cnt); return -1; }
+  int64_t *boff = cnt + (size_t)nt * NB;
+  const int64_t chunk = (n + nt - 1) / nt;
+#pragma omp parallel num_threads(nt)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    const int64_t a = t * chunk, b = a + chunk < n ? a + chunk : n;
+    int64_t *c = cnt + (size_t)t * NB;
+    for (int64_t i = a; i < b; i++) c[kv[2 * i] >> shift]++;
+#pragma omp barrier
+#pragma omp single
+    {
+      int64_t acc = 0;
+      for (int64_t k = 0; k < NB; k++) {
+        boff[k] = acc;
+        for (int u = 0; u < nt; u++) {
+          const int64_t v = cnt[(size_t)u * NB + k];
+          cnt[(size_t)u * NB + k] = acc;
+          acc += v;
+        }
+      }
+      boff[NB] = acc;
+    }
+    for (int64_t i = a; i < b; i++) {
+      const int64_t q = c[kv[2 * i] >> shift]++;
+      tmp[2 * q] = kv[2 * i];
+      tmp[2 * q + 1] = kv[2 * i + 1];
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t k = 0; k < NB; k++)
+    if (boff[k + 1] - boff[k] > 1)
+      qsort(tmp + 2 * boff[k], (size_t)(boff[k + 1] - boff[k]), 2 * sizeof(uint64_t), cmp_u64pair);
+  memcpy(kv, tmp, (size_t)n * 2 * sizeof(uint64_t));
+  free(tmp);
+  free(cnt);
+  return 0;
 }
 
 
@@ -271,8 +383,9 @@ static double *gather_radii(const gen_params *g, const halo_spec *hs, const int6
   return grad;
 }
 
-static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo_spec *hs,
-                      int64_t nh, int64_t n_bg)
+/* the snapshot of the haloes hs[0..nh) plus n_bg background particles, in
+ * memory (arrays malloc'd into *out; release with snap_free) */
+static int build_snap(const gen_params *g, uint64_t seed, halo_spec *hs, int64_t nh, int64_t n_bg, snap_data *out)
 {
   const double a = 1.0, v_fac = g->box / 0.01 / a;
   int64_t nhalo_part = 0;
@@ -332,7 +445,7 @@ static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo
     kv[2 * p] = ahf_sfc_key((double)pos[3 * p + 0], (double)pos[3 * p + 1], (double)pos[3 * p + 2], AHF_SFC_BITS);
     kv[2 * p + 1] = (uint64_t)p;
   }
-  qsort(kv, (size_t)N, 2 * sizeof(uint64_t), cmp_u64pair);
+  if (sort_pairs(kv, N)) return -1;
   uint64_t *newidx = (uint64_t *)malloc((size_t)N * sizeof(uint64_t));
   float *pos2 = (float *)malloc((size_t)N * 3 * sizeof(float));
   float *mom2 = (float *)malloc((size_t)N * 3 * sizeof(float));
@@ -348,14 +461,14 @@ static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo
   }
   free(kv); free(pos); free(mom);
 
-  /* halo lists: a host's list holds its own particles and its subhaloes' */
+  /* halo lists: a halo's list holds its own particles and those of every
+   * subhalo nested in it (a subhalo always comes after its parent) */
   int64_t n_ip = 0;
-  for (int64_t i = 0; i < nh; i++) n_ip += hs[i].n + (hs[i].host >= 0 ? hs[i].n : 0);
-  int64_t *hostn = (int64_t *)calloc((size_t)nh, sizeof(int64_t));
+  int64_t *hostn = (int64_t *)calloc((size_t)(nh ? nh : 1), sizeof(int64_t));
+  if (!hostn) return -1;
   for (int64_t i = 0; i < nh; i++) hostn[i] = hs[i].n;
-  for (int64_t i = 0; i < nh; i++)
-    if (hs[i].host >= 0) hostn[hs[i].host] += hs[i].n;
-  n_ip = 0;
+  for (int64_t i = nh - 1; i >= 0; i--)
+    if (hs[i].host >= 0) hostn[hs[i].host] += hostn[i];
   for (int64_t i = 0; i < nh; i++) n_ip += hostn[i];
   uint64_t *ipart = (uint64_t *)malloc((size_t)(n_ip ? n_ip : 1) * sizeof(uint64_t));
   uint64_t *hnp = (uint64_t *)malloc((size_t)(nh ? nh : 1) * sizeof(uint64_t));
@@ -363,13 +476,9 @@ static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo
   int64_t *fill = (int64_t *)calloc((size_t)(nh ? nh : 1), sizeof(int64_t));
   int64_t off = 0;
   for (int64_t i = 0; i < nh; i++) { fill[i] = off; off += hostn[i]; }
-  for (int64_t i = 0; i < nh; i++) {
-    for (int64_t j = 0; j < hs[i].n; j++) ipart[fill[i]++] = newidx[hs[i].first + j];
-    if (hs[i].host >= 0) {
-      int64_t hh = hs[i].host;
-      for (int64_t j = 0; j < hs[i].n; j++) ipart[fill[hh]++] = newidx[hs[i].first + j];
-    }
-  }
+  for (int64_t i = 0; i < nh; i++)
+    for (int64_t a = i; a >= 0; a = hs[a].host)
+      for (int64_t j = 0; j < hs[i].n; j++) ipart[fill[a]++] = newidx[hs[i].first + j];
   for (int64_t i = 0; i < nh; i++) {
     hnp[i] = (uint64_t)hostn[i];
     for (int d = 0; d < 3; d++) {
@@ -410,31 +519,47 @@ static int write_snap(const char *path, const gen_params *g, uint64_t seed, halo
   hd.seed = seed;
   hd.config = g->config;
 
+  out->h = hd;
+  out->pos = pos2;
+  out->mom = mom2;
+  out->id = id;
+  out->centre = cen;
+  out->gather_rad = grad;
+  out->halo_npart = hnp;
+  out->ipart = ipart;
+  free(newidx); free(hostn); free(fill);
+  return 0;
+}
+
+static int write_snap(const char *path, const snap_data *d)
+{
+  const snap_header *hd = &d->h;
+  const size_t N = (size_t)hd->npart, nh = (size_t)hd->nhalos, n_ip = (size_t)hd->n_ipart;
   FILE *f = fopen(path, "wb");
   int bad = !f;
   if (!bad) {
-    bad |= fwrite(&hd, sizeof(hd), 1, f) != 1;
-    bad |= fwrite(pos2, sizeof(float), (size_t)N * 3, f) != (size_t)N * 3;
-    bad |= fwrite(mom2, sizeof(float), (size_t)N * 3, f) != (size_t)N * 3;
-    bad |= fwrite(id, sizeof(uint64_t), (size_t)N, f) != (size_t)N;
-    bad |= fwrite(cen, sizeof(double), (size_t)nh * 3, f) != (size_t)nh * 3;
-    bad |= fwrite(grad, sizeof(double), (size_t)nh, f) != (size_t)nh;
-    bad |= fwrite(hnp, sizeof(uint64_t), (size_t)nh, f) != (size_t)nh;
-    bad |= fwrite(ipart, sizeof(uint64_t), (size_t)n_ip, f) != (size_t)n_ip;
+    bad |= fwrite(hd, sizeof(*hd), 1, f) != 1;
+    bad |= fwrite(d->pos, sizeof(float), N * 3, f) != N * 3;
+    bad |= fwrite(d->mom, sizeof(float), N * 3, f) != N * 3;
+    bad |= fwrite(d->id, sizeof(uint64_t), N, f) != N;
+    bad |= fwrite(d->centre, sizeof(double), nh * 3, f) != nh * 3;
+    bad |= fwrite(d->gather_rad, sizeof(double), nh, f) != nh;
+    bad |= fwrite(d->halo_npart, sizeof(uint64_t), nh, f) != nh;
+    bad |= fwrite(d->ipart, sizeof(uint64_t), n_ip, f) != n_ip;
     bad |= fclose(f) != 0;
   }
-  free(newidx); free(pos2); free(mom2); free(id); free(hostn); free(grad);
-  free(ipart); free(hnp); free(cen); free(fill);
   return bad ? -2 : 0;
 }
 
 /*
- * ahf_snapgen_write: generate workload `config` at `side`^3 particles
+ * ahf_snapgen_make: generate workload `config` at `side`^3 particles
  * (side <= 0: the full size) with `seed`, scaling the halo count by
- * (side/side_full)^3; writes `path`.  Returns 0 on success.
+ * (side/side_full)^3, into *out (release with ahf_snapgen_release).
+ * Returns 0 on success.
  */
-int ahf_snapgen_write(int config, int side, uint64_t seed, const char *path)
+int ahf_snapgen_make(int config, int side, uint64_t seed, snap_data *out)
 {
+  memset(out, 0, sizeof(*out));
   gen_params g;
   memset(&g, 0, sizeof(g));
   g.config = config;
@@ -442,8 +567,8 @@ int ahf_snapgen_write(int config, int side, uint64_t seed, const char *path)
   g.Nth_dom = 5.0;
   g.Nth_ref = 5.0;
   g.sigma_bg = 50.0;
-  int full_side = config == 0 ? 128 : config == 1 ? 256 : config == 3 ? 512 : 0;
-  int dom_full = config == 0 ? 64 : config == 1 ? 128 : config == 3 ? 256 : 0;
+  int full_side = config == 0 ? 128 : config == 1 ? 256 : config == 3 ? 512 : config == 4 ? 896 : 0;
+  int dom_full = config == 0 ? 64 : config == 1 ? 128 : config == 3 ? 256 : config == 4 ? 448 : 0;
   halo_spec *hs = NULL;
   int64_t nh = 0, n_bg = 0;
   if (config == 2) {
@@ -470,7 +595,7 @@ int ahf_snapgen_write(int config, int side, uint64_t seed, const char *path)
     }
     nh = nhalo;
     n_bg = 0;
-  } else if (config == 0 || config == 1 || config == 3) {
+  } else if (config == 0 || config == 1 || config == 3 || config == 4) {
     if (side <= 0) side = full_side;
     double scale = (double)side / (double)full_side;
     g.npart = (int64_t)side * side * side;
@@ -478,11 +603,12 @@ int ahf_snapgen_write(int config, int side, uint64_t seed, const char *path)
     g.pmass = 0.3 * RHOC0 * g.box * g.box * g.box / (double)g.npart;
     g.L_dom = (int)(dom_full * scale);
     if (g.L_dom < 4) g.L_dom = 4;
-    g.L_max = g.L_dom << 8;
-    int64_t nhost = config == 0 ? 40 : config == 1 ? 2000 : 20000;
+    g.L_max = g.L_dom << (config == 3 ? 10 : config == 4 ? 12 : 8);
+    int64_t nhost = config == 0 ? 40 : config == 1 ? 2000 : config == 3 ? 20000 : 200000;
     nhost = (int64_t)ceil((double)nhost * scale * scale * scale);
     if (config == 0) nh = make_halos(&g, seed, nhost, 0, 1e11, 1e13, 1, &hs);
-    else nh = make_halos(&g, seed, nhost, config == 1, 1e10, 1e14, 0, &hs);
+    else nh = make_halos(&g, seed, nhost, config == 1 ? 1 : config == 4 ? 3 : 0, 1e10, 1e14, 0, &hs);
+    if (nh < 0) return -1;
     int64_t tot = 0;
     for (int64_t i = 0; i < nh; i++) { hs[i].n = n_of_mass(hs[i].M, g.pmass); tot += hs[i].n; }
     int64_t budget = config == 3 ? (int64_t)(0.3 * (double)g.npart) : g.npart;
@@ -497,7 +623,22 @@ int ahf_snapgen_write(int config, int side, uint64_t seed, const char *path)
   } else {
     return -10;
   }
-  int rc = write_snap(path, &g, seed, hs, nh, n_bg);
+  int rc = build_snap(&g, seed, hs, nh, n_bg, out);
   free(hs);
+  return rc;
+}
+
+void ahf_snapgen_release(snap_data *d)
+{
+  snap_free(d);
+}
+
+/* ahf_snapgen_make, written to `path` */
+int ahf_snapgen_write(int config, int side, uint64_t seed, const char *path)
+{
+  snap_data d;
+  int rc = ahf_snapgen_make(config, side, seed, &d);
+  if (rc == 0) rc = write_snap(path, &d);
+  snap_free(&d);
   return rc;
 }

@@ -7,6 +7,7 @@ package's GPU path, so any two of them can be compared field by field.
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import struct
 
@@ -77,6 +78,63 @@ def read_snapshot(path: str) -> Snapshot:
         ip = np.fromfile(f, dtype=np.uint64, count=ni)
     return Snapshot(npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, vt, rf, vf, pf, hb,
                     nh, ni, seed, cfg, ovl, rhv, pos, mom, ids, cen, grad, hnp, ip)
+
+
+class _SnapData(ctypes.Structure):
+    """snap_data of ahf_amd/csrc/snapfile.h: the header bytes, then the
+    seven array pointers"""
+    _fields_ = [("h", ctypes.c_uint8 * SNAP_HEADER.size)] + [
+        (n, ctypes.c_void_p) for n in ("pos", "mom", "id", "centre", "gather_rad", "halo_npart", "ipart")]
+
+
+class _GenOwner:
+    """keeps the generator's arrays alive while a numpy view of them exists"""
+
+    def __init__(self, lib, data):
+        self.lib, self.data = lib, data
+
+    def __del__(self):
+        self.lib.ahf_snapgen_release(ctypes.byref(self.data))
+
+
+def _view(owner, addr, ctype, count, dtype, shape):
+    buf = (ctype * max(int(count), 1)).from_address(addr)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=dtype)[:int(count)].reshape(shape)
+
+
+def generate(config: int, side: int = 0, seed: int = 0, lib_path: str | None = None) -> Snapshot:
+    """Workload `config` made in memory by the seeded generator
+    (ahf_amd/csrc/snapgen.c, the same arrays ahf_snapgen_write puts in a
+    file), for sizes where a file is too large to pass around (the 896^3
+    workload).  The arrays are views of the generator's memory, released
+    when no view is left."""
+    import os
+    if lib_path is None:
+        lib_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libahf_snapgen.so")
+    lib = ctypes.CDLL(lib_path)
+    lib.ahf_snapgen_make.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p]
+    lib.ahf_snapgen_release.argtypes = [ctypes.c_void_p]
+    lib.ahf_snapgen_release.restype = None
+    d = _SnapData()
+    rc = lib.ahf_snapgen_make(config, side, seed, ctypes.byref(d))
+    if rc:
+        lib.ahf_snapgen_release(ctypes.byref(d))
+        raise RuntimeError(f"snapshot generation failed ({rc})")
+    owner = _GenOwner(lib, d)
+    v = SNAP_HEADER.unpack(bytes(d.h))
+    (npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, _p0, vt, rf, vf, pf, hb,
+     nh, ni, sd, cfg, _p1, ovl, rhv) = v[1:]
+    f32, f64, u64 = ctypes.c_float, ctypes.c_double, ctypes.c_uint64
+    pos = _view(owner, d.pos, f32, 3 * npart, np.float32, (npart, 3))
+    mom = _view(owner, d.mom, f32, 3 * npart, np.float32, (npart, 3))
+    ids = _view(owner, d.id, u64, npart, np.uint64, (npart,))
+    cen = _view(owner, d.centre, f64, 3 * nh, np.float64, (nh, 3))
+    grad = _view(owner, d.gather_rad, f64, nh, np.float64, (nh,))
+    hnp = _view(owner, d.halo_npart, u64, nh, np.uint64, (nh,))
+    ip = _view(owner, d.ipart, u64, ni, np.uint64, (ni,))
+    return Snapshot(npart, box, om, ol, a, pm, Ld, Lm, nd, nr, mp, vt, rf, vf, pf, hb,
+                    nh, ni, sd, cfg, ovl, rhv, pos, mom, ids, cen, grad, hnp, ip)
 
 
 def write_snapshot(path: str, pos, mom=None, ids=None, *, boxsize=100.0, pmass=1e10,

@@ -40,7 +40,7 @@ def log(msg):
 
 def snapshot_path(cfg, side, seed):
     d = os.environ.get("AHF_BENCH_DIR", "/tmp")
-    return os.path.join(d, f"ahf_bench_s3_cfg{cfg}_{side}_{seed}.snap")
+    return os.path.join(d, f"ahf_bench_s4_cfg{cfg}_{side}_{seed}.snap")
 
 
 def make_snapshot(cfg, side, seed):
@@ -261,7 +261,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1, help="0, 1 (default) or 3: box workloads")
+    ap.add_argument("--config", type=int, default=1, help="0, 1 (default), 3 or 4: box workloads")
     ap.add_argument("--side", type=int, default=0, help="particles per dimension (0: full size)")
     ap.add_argument("--seed", type=int, default=1001)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the one-halo-per-launch form")
@@ -285,9 +285,15 @@ def main():
     import ahf_amd
     from ahf_amd import dumps
 
-    snap = make_snapshot(args.config, args.side, args.seed)
-    s = dumps.read_snapshot(snap)
-    log(f"snapshot: {s.npart} particles, {s.nhalos} haloes, {s.n_ipart} members, L_dom {s.L_dom}")
+    # the 512^3 and 896^3 boxes are made in memory (a file of the 896^3 one
+    # would hold 30 GB); the others go through a cached snapshot file, which
+    # the reference's timing reads as well
+    in_memory = args.config in (3, 4)
+    snap = None if in_memory else make_snapshot(args.config, args.side, args.seed)
+    tg = time.perf_counter()
+    s = dumps.generate(args.config, args.side, args.seed) if in_memory else dumps.read_snapshot(snap)
+    log(f"snapshot: {s.npart} particles, {s.nhalos} haloes, {s.n_ipart} members, L_dom {s.L_dom}, "
+        f"L_max {s.L_max} ({time.perf_counter() - tg:.1f}s)")
 
     ctx = ahf_amd.Context(0)
     copy = copy_probe()
@@ -356,6 +362,11 @@ def main():
     dep_bytes = sum(12 * p + 13 * n for p, n in levels)
     gather_avg_ms = stage_ms["gather_light"] / max(1.0, stage_n["gather_light"])
     achieved = (dep_bytes / max(1.0, stage_n["gather_light"])) / (gather_avg_ms * 1e-3) / 1e9
+    # the whole deposit per level: the position/weight records and the
+    # Morton order (prep), the lane-per-node walks (gather_light) and the
+    # wavefront-per-node folds of the heaviest nodes (gather_heavy)
+    dep_ms = stage_ms["prep"] + stage_ms["gather_light"] + stage_ms["gather_heavy"]
+    achieved_incl = dep_bytes / (dep_ms * 1e-3) / 1e9
     traffic = load_traffic()
     cfg_tag = f"cfg{args.config}_{args.side}_{args.seed}"
     if traffic and traffic.get("workload") != cfg_tag:
@@ -407,6 +418,9 @@ def main():
             "avg_launch_ms": gather_avg_ms,
             "launches_per_step": stage_n["gather_light"],
             "algorithmic_bytes_per_launch": dep_bytes / max(1.0, stage_n["gather_light"]),
+            "achieved_deposit_inclusive": achieved_incl,
+            "frac_deposit_inclusive": achieved_incl / HBM_PEAK_GBS,
+            "deposit_inclusive_ms_per_step": dep_ms,
         },
         "copy_ceiling_GBs": copy["GBs"] if copy else None,
         "upload_ms": upload_ms,
@@ -416,21 +430,31 @@ def main():
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         threads = min(threads, 16)
-        rb = cpu_baseline(snap, threads)
+        # the 512^3 and 896^3 boxes: the same generator at 1/64 of the
+        # particles (side / 4), a bounded sample of the same workload
+        cpu_snap, n_cpu, what = snap, s.npart, "whole workload once"
+        if in_memory:
+            side = (args.side or {3: 512, 4: 896}[args.config]) // 4
+            cpu_snap = make_snapshot(args.config, side, args.seed)
+            n_cpu = side ** 3
+            what = f"the same generator at {side}^3 ({n_cpu} particles, 1/64 of the workload)"
+        rb = cpu_baseline(cpu_snap, threads)
         if rb:
             cpu_s = rb["amr_seconds"] + rb["unbind_seconds"]
             out["cpu_baseline"] = {
-                "value": s.npart / cpu_s,
+                "value": n_cpu / cpu_s,
                 "unit": "particles/s",
                 "cores": threads,
                 "kind": "reference",
-                "sample": (f"whole workload once: AMR build {rb['amr_seconds']:.2f}s on 1 thread "
+                "sample": (f"{what}: AMR build {rb['amr_seconds']:.2f}s on 1 thread "
                            f"(reference default: deposit is serial), unbinding {rb['unbind_seconds']:.3f}s "
                            f"with OpenMP over haloes on {threads} threads "
                            f"({rb.get('unbind_seconds_1thread', 0.0):.3f}s on 1 thread); host {cpu_model()}"),
             }
             out["vs_cpu"] = value / out["cpu_baseline"]["value"]
     ctx.close()
+    if in_memory:
+        args.no_construct_workload = True
     if not args.no_construct_workload:
         cw = construction_workload(snap, cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
         log(f"construction workload: gather {cw['gather_ms']:.2f} ms + construct {cw['construct_ms']:.2f} ms "
@@ -438,7 +462,7 @@ def main():
             f"on the GPU ({cw['gathered_members']} gathered, {cw['final_members']} kept)"
             + (f", reference CPU {cw['cpu_reference_ms']:.1f} ms" if "cpu_reference_ms" in cw else ""))
         out["construction_workload"] = cw
-    if not args.no_unbind_workload and args.side == 0:
+    if not args.no_unbind_workload and args.side == 0 and args.config == 1:
         u = unbinding_workload(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
         log(f"unbinding workload: {u['gpu_ms']:.2f} ms on the GPU"
             + (f", reference CPU {u['cpu_reference_ms']:.1f} ms" if "cpu_reference_ms" in u else ""))

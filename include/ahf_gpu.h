@@ -132,6 +132,8 @@ int ahfgpu_deposit_level(ahfgpu_ctx *ctx, int64_t L, const uint64_t *oct_key, co
 #define AHFGPU_GATHER_SOURCE 1 /* walk the 27 sources over per-entry positions */
 #define AHFGPU_GATHER_FLAT 2   /* one flat walk per node over per-entry TSC weights */
 #define AHFGPU_GATHER_WAVE 3   /* AUTO's walks, but every node with a contribution folded by a wavefront */
+#define AHFGPU_GATHER_FLAT_WIDE 4 /* FLAT with 64-bit record offsets, the form a level takes when its
+                                     weight records pass 4 GiB */
 
 /* No counterpart in the reference: choose the deposit kernel of the
  * following builds and deposits.  Every mode gives the same bits; AUTO
@@ -235,7 +237,6 @@ void ahfgpu_release_halos(ahfgpu_ctx *ctx);
 int ahfgpu_unbind_noalloc(ahfgpu_ctx *ctx, const ahfgpu_unbind_params *p, int mode, double band,
                           const double *centres, const int64_t *off, const uint32_t *members, int64_t nh,
                           uint64_t *npart_out, double *props_out, uint32_t *members_out);
-
 
 /* HaloProfiles (src/libahf/ahf_halos.c:3961-5060, AHF's default flags:
  * unit masses, R_max and r2 from the sorted particle list, Hubble drag) of

@@ -25,7 +25,7 @@
 #include <omp.h>
 #endif
 
-#include "snapfile.h"
+#include "../ahf_amd/csrc/snapfile.h"
 #include "oracle_profiles.h"
 
 _Static_assert(ORACLE_PROF_NQ == AHF_PROF_NQ && ORACLE_PROF_NP == AHF_PROF_NP, "profile value counts");

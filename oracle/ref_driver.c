@@ -75,7 +75,7 @@
 #define DRV_SFC_BITS AHF_SFC_BITS
 #endif
 
-#include "snapfile.h"
+#include "../ahf_amd/csrc/snapfile.h"
 
 #ifndef AHF_REF_BUILD
 /* The program owns AHF's globals, as AHF does (src/common.c and

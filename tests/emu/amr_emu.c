@@ -18,7 +18,7 @@
 
 #include "../../ahf_amd/csrc/amr_core.h"
 #include "../../ahf_amd/csrc/level_host.h"
-#include "../../oracle/snapfile.h"
+#include "../../ahf_amd/csrc/snapfile.h"
 
 typedef struct {
   level_view v;

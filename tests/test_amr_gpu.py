@@ -33,7 +33,9 @@ def _gpu_dump(ctx, snap_path, out_path, mode=0):
 
 # every deposit kernel (per-source walk, flat walk, and the per-level choice)
 # must give the oracle's bits
-MODES = [0, 1, 2, 3]  # AUTO, SOURCE, FLAT, WAVE (every node folded by a wavefront)
+# AUTO, SOURCE, FLAT, WAVE (every node folded by a wavefront), FLAT_WIDE (the
+# flat walk with the 64-bit record offsets of levels over 4 GiB of records)
+MODES = [0, 1, 2, 3, 4]
 
 
 def _check(oracle, ctx, snap, tmp_path, tag, mode=0):
@@ -50,7 +52,8 @@ def _check(oracle, ctx, snap, tmp_path, tag, mode=0):
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 11), (1, 64, 1001), (0, 64, 1000), (1, 48, 7), (3, 48, 5)])
+@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 11), (1, 64, 1001), (0, 64, 1000), (1, 48, 7), (3, 48, 5),
+                                          (3, 64, 1003), (4, 64, 1004)])
 def test_generated_configs(oracle, snapgen, ctx, tmp_path, cfg, side, seed, mode):
     snap = str(tmp_path / f"cfg{cfg}_{side}.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0

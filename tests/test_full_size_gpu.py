@@ -1,7 +1,15 @@
 """GPU parity at the full sizes of the generated workloads: the 128^3 and
 256^3 boxes (AMR hierarchy bit for bit, unbinding, and gather + the
 construction steps with identical haloes) and the 100,000-halo unbinding
-workload (22.4 M members), each against the CPU oracle on the same file."""
+workload (22.4 M members), each against the CPU oracle on the same file.
+
+The 512^3 box (10 levels) and the 896^3 box (12 levels, 719 M particles)
+are made in memory: their hierarchies are checked through the properties
+of tests/amr_props.py (the CPU oracle takes minutes at 512^3 and would not
+fit this suite's time at 896^3), and the construction of all their haloes
+(sort, rem_outsideRvir, rem_unbound, rem_outsideRvir) against the oracle
+on the haloes' members.  With AHF_FULL_ORACLE=1 the 512^3 hierarchy is
+also compared with the oracle's bit for bit."""
 import os
 
 import numpy as np
@@ -129,3 +137,119 @@ def test_profiles_full_size(oracle, ctx, snapgen, tmp_path, cfg, seed, gather):
     profiles_match(ref, nb, props, prof, f"full cfg{cfg} end to end", end_to_end=True)
     for f in (c_out, p_out, snap):
         os.remove(f)
+
+
+def _members_snapshot(s, path, halos=None):
+    """a snapshot of only the members of the given haloes (all by default),
+    indices renumbered, for the oracle's construction steps"""
+    off = s.halo_offsets
+    hs = np.arange(s.nhalos) if halos is None else np.asarray(halos)
+    lists = [s.ipart[off[h]:off[h + 1]] for h in hs]
+    allm = np.unique(np.concatenate(lists)) if lists else np.zeros(0, np.uint64)
+    remap = [np.searchsorted(allm, x).astype(np.uint64) for x in lists]
+    dumps.write_snapshot(path, s.pos[allm.astype(np.int64)], s.mom[allm.astype(np.int64)], boxsize=s.boxsize,
+                         pmass=s.pmass, L_dom=s.L_dom, L_max=s.L_max, Nth_dom=s.Nth_dom, Nth_ref=s.Nth_ref,
+                         minpart=s.minpart, vtune=s.vtune, r_fac=s.r_fac, v_fac=s.v_fac, phi_fac=s.phi_fac,
+                         hubble=s.hubble, centres=s.centre[hs], halo_lists=remap, seed=s.seed, config=s.config,
+                         omega0=s.omega0, lambda0=s.lambda0, a=s.a, ovlim=s.ovlim, rho_vir=s.rho_vir,
+                         gather_rad=s.gather_rad[hs])
+    return hs, allm
+
+
+@pytest.fixture(scope="module")
+def big_snaps():
+    """the 512^3 and 896^3 workloads, made once for this module"""
+    cache = {}
+
+    def get(cfg, seed):
+        if (cfg, seed) not in cache:
+            cache.clear()
+            cache[(cfg, seed)] = dumps.generate(cfg, 0, seed)
+        return cache[(cfg, seed)]
+    yield get
+    cache.clear()
+
+
+@pytest.mark.parametrize("cfg,seed,levels", [(3, 1003, 11), (4, 1004, 13)])
+def test_amr_full_size_properties(ctx, big_snaps, cfg, seed, levels):
+    import amr_props
+    s = big_snaps(cfg, seed)
+    print(f"[full-gpu] amr cfg{cfg}: {s.npart} particles, L_dom {s.L_dom}, L_max {s.L_max}", flush=True)
+    ctx.set_gather_mode(0)
+    ctx.set_particles(s.pos, s.mom)
+    info = ctx.amr_build(s.L_dom, s.L_max, s.Nth_dom, s.Nth_ref)
+    print(f"[full-gpu] cfg{cfg}: {info.n_levels} levels, nodes "
+          f"{[int(info.n_nodes[i]) for i in range(info.n_levels)]}", flush=True)
+    assert 3 <= info.n_levels <= levels
+    assert int(info.n_part[0]) == s.npart
+    lv = [ctx.amr_level(i) for i in range(info.n_levels)]
+    for i in range(info.n_levels):
+        lv[i]["n_part"] = int(info.n_part[i])
+    fails = amr_props.check_hierarchy(lv, s.pos, np.random.default_rng(seed), sample=1500)
+    assert not fails, fails
+
+
+@pytest.mark.parametrize("cfg,seed", [(3, 1003), (4, 1004)])
+def test_construct_full_size_big(oracle, ctx, big_snaps, tmp_path, cfg, seed):
+    """sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1) of every
+    halo of the 512^3 / 896^3 box against the oracle on the haloes'
+    members; gathered from their centres on the device, the constructed
+    members are a subset of the gathered lists, which lie within the
+    gather radius"""
+    s = big_snaps(cfg, seed)
+    print(f"[full-gpu] construct cfg{cfg}: {s.nhalos} haloes, {s.n_ipart} members", flush=True)
+    snap = str(tmp_path / "m.snap")
+    hs, allm = _members_snapshot(s, snap)
+    out = str(tmp_path / "o.halo")
+    assert oracle.oracle_construct_dump(snap.encode(), out.encode(), 16, 0) == 0
+    ref = dumps.read_halos(out)
+    os.remove(snap)
+    ctx.set_unbind_split(0)
+    ctx.set_particles(s.pos, s.mom)
+    ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+    kw = dict(mode=ctx.UNBIND_WITH_RVIR, ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+    ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, **kw)
+    npart, props, lists = ctx.unbind_results()
+    ref.ipart = [allm[x.astype(np.int64)] for x in ref.ipart]
+    _same_haloes(ref, npart, lists, props, f"construct cfg{cfg}")
+    assert np.array_equal(ref.ovdens, props[:, 3])
+    # gathered on the device (a sample of 4000 haloes, the largest included)
+    rng = np.random.default_rng(seed)
+    pick = np.union1d(rng.choice(s.nhalos, min(s.nhalos, 4000), replace=False), np.argsort(s.halo_npart)[-20:])
+    skip = np.ones(s.nhalos, np.uint8)
+    skip[pick] = 0
+    skip[s.halo_npart == 0] = 1
+    ctx.sfc_keys()
+    ctx.gather_halos(s.centre, s.gather_rad, skip=skip)
+    gnp, glists = ctx.halo_members()
+    print(f"[full-gpu] gathered {int(gnp.sum())} members for {int((skip == 0).sum())} haloes", flush=True)
+    for h in pick[:300]:
+        g = glists[h].astype(np.int64)
+        d = np.abs(s.pos[g].astype(np.float64) - s.centre[h])
+        d = np.minimum(d, 1.0 - d)
+        assert np.all(np.sum(d * d, axis=1) <= s.gather_rad[h] ** 2 * (1 + 1e-12)), h
+    ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, **kw)
+    npart2, _, lists2 = ctx.unbind_results()
+    for h in pick:
+        if npart2[h]:
+            assert np.all(np.isin(lists2[h], glists[h])), h
+    assert int(npart2[pick].sum()) > 0
+
+
+@pytest.mark.skipif(os.environ.get("AHF_FULL_ORACLE") != "1",
+                    reason="opt-in (AHF_FULL_ORACLE=1): the oracle takes minutes to build the 512^3 hierarchy")
+def test_amr_full_size_cfg3_oracle(oracle, snapgen, ctx, tmp_path):
+    """the 512^3 box (10 levels) bit for bit against the oracle"""
+    snap = _snap(snapgen, tmp_path, 3, 1003)
+    s = dumps.read_snapshot(snap)
+    print(f"[full-gpu] amr cfg3 against the oracle: {s.npart} particles", flush=True)
+    o_out, g_out = str(tmp_path / "oracle.grid"), str(tmp_path / "gpu.grid")
+    assert oracle.oracle_amr_dump(snap.encode(), o_out.encode()) > 0
+    ctx.set_gather_mode(0)
+    ctx.set_particles(s.pos, s.mom)
+    info = ctx.amr_build(s.L_dom, s.L_max, s.Nth_dom, s.Nth_ref)
+    ctx.amr_dump(g_out)
+    del s
+    diffs = dumps.compare_grids(dumps.read_grids(g_out), dumps.read_grids(o_out))
+    assert not diffs, "; ".join(diffs[:6])
+    print(f"[full-gpu] cfg3: {info.n_levels} levels bit-identical to the oracle", flush=True)

@@ -51,7 +51,7 @@ def test_unsorted_keys_rejected(snapgen, ctx, tmp_path):
     ctx.gather_halos(s.centre, s.gather_rad)
 
 
-CASES = [(1, 48, 1001), (1, 64, 7), (0, 40, 1000), (3, 48, 5), (2, 400, 1002), (2, 2000, 77)]
+CASES = [(1, 48, 1001), (1, 64, 7), (0, 40, 1000), (3, 48, 5), (2, 400, 1002), (2, 2000, 77), (4, 64, 1004)]
 
 
 @pytest.mark.parametrize("cfg,side,seed", CASES)

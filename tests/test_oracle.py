@@ -72,7 +72,7 @@ def _ref_amr(ref_bin, snap, out):
     assert r.returncode == 0, r.stderr[-2000:]
 
 
-@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 11), (1, 64, 1001), (0, 48, 1000)])
+@pytest.mark.parametrize("cfg,side,seed", [(1, 32, 11), (1, 64, 1001), (0, 48, 1000), (3, 64, 1003), (4, 64, 1004)])
 def test_amr_matches_reference_configs(oracle, ref_bin, snapgen, tmp_path, cfg, side, seed):
     snap = str(tmp_path / "g.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
@@ -114,7 +114,7 @@ def _halo_eq(a, b):
     assert np.array_equal(a.R_vir[ok], b.R_vir[ok])
 
 
-@pytest.mark.parametrize("cfg,side,seed", [(0, 64, 1000), (1, 64, 1001), (2, 800, 1002)])
+@pytest.mark.parametrize("cfg,side,seed", [(0, 64, 1000), (1, 64, 1001), (2, 800, 1002), (4, 96, 1004)])
 def test_unbind_matches_reference(oracle, ref_bin, snapgen, tmp_path, cfg, side, seed):
     snap = str(tmp_path / "h.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
@@ -314,7 +314,7 @@ def test_rvir_cut_small_and_centre(oracle):
     assert kept == 2 and M == 2.0
 
 
-@pytest.mark.parametrize("cfg,side,seed", [(0, 64, 1000), (1, 64, 1001), (2, 800, 1002), (3, 48, 5)])
+@pytest.mark.parametrize("cfg,side,seed", [(0, 64, 1000), (1, 64, 1001), (2, 800, 1002), (3, 48, 5), (4, 96, 1004)])
 def test_construct_matches_reference(oracle, ref_bin, snapgen, tmp_path, cfg, side, seed):
     """sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1) per halo:
     the oracle against the reference's own functions"""
@@ -419,7 +419,7 @@ def test_gather_brute_force(oracle, snapgen, tmp_path):
     assert nc.value >= int(g.npart.sum())
 
 
-@pytest.mark.parametrize("cfg,side,seed", [(1, 48, 1001), (0, 40, 1000), (3, 48, 5), (2, 400, 1002)])
+@pytest.mark.parametrize("cfg,side,seed", [(1, 48, 1001), (0, 40, 1000), (3, 48, 5), (2, 400, 1002), (4, 64, 1004)])
 def test_gather_matches_reference(oracle, ref_bin, snapgen, tmp_path, cfg, side, seed):
     """gatherParts alone, and followed by the construction steps, against
     the reference's own functions"""

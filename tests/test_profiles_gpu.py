@@ -69,7 +69,7 @@ def _constructed(oracle, snap, tmp_path, gather):
 
 
 CASES = [(2, 800, 1002, 0), (0, 64, 1000, 0), (3, 48, 5, 0), (2, 2000, 1002, 0), (2, 400, 1002, 1),
-         (1, 64, 1001, 0)]
+         (1, 64, 1001, 0), (4, 96, 1004, 0)]
 
 
 # split: 0 the default (haloes of 2048 members or more on four wavefronts),

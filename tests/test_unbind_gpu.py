@@ -46,7 +46,7 @@ def _compare(ref, npart, props, lists, tag):
 # the split path, -1 none
 @pytest.mark.parametrize("split", [0, 32, -1])
 @pytest.mark.parametrize("cfg,side,seed,mode", [(1, 64, 1001, 0), (1, 64, 1001, 1), (0, 64, 1000, 0),
-                                                (2, 2000, 1002, 0), (2, 300, 7, 1)])
+                                                (2, 2000, 1002, 0), (2, 300, 7, 1), (4, 96, 1004, 0)])
 def test_unbind_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, mode, split):
     snap = str(tmp_path / "h.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
@@ -103,7 +103,7 @@ def _construct_gpu(ctx, s, mode, centre=None, npart=None, ipart=None, split=0):
 
 @pytest.mark.parametrize("split", [0, 32])
 @pytest.mark.parametrize("cfg,side,seed,plain", [(1, 64, 1001, 0), (1, 64, 1001, 1), (0, 64, 1000, 0),
-                                                 (2, 2000, 1002, 0), (2, 300, 7, 1), (3, 48, 5, 0)])
+                                                 (4, 96, 1004, 0), (2, 2000, 1002, 0), (2, 300, 7, 1), (3, 48, 5, 0)])
 def test_construct_matches_oracle(oracle, snapgen, ctx, tmp_path, cfg, side, seed, plain, split):
     """sort, rem_outsideRvir(0), rem_unbound, rem_outsideRvir(1) in one pass"""
     snap = str(tmp_path / "c.snap")
