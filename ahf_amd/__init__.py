@@ -60,6 +60,16 @@ class ProfileParams(ctypes.Structure):
                 ("hubble", ctypes.c_double)]
 
 
+class Schedule(ctypes.Structure):
+    """ahfgpu_schedule: thresholds between equivalent kernels (0: default)"""
+    _fields_ = [("split_min", ctypes.c_int64), ("prof_split", ctypes.c_int64), ("big_halo", ctypes.c_int64),
+                ("wave_halo", ctypes.c_int64), ("kmax_big", ctypes.c_int32), ("kmax_small", ctypes.c_int32),
+                ("march_zn", ctypes.c_int32), ("graph_batch", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -108,6 +118,8 @@ def load():
     lib.ahfgpu_halo_profiles.argtypes = [P, ctypes.POINTER(ProfileParams), P, P]
     lib.ahfgpu_profile_results.argtypes = [P, P, P]
     lib.ahfgpu_set_profile_split.argtypes = [P, ctypes.c_int64]
+    lib.ahfgpu_set_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
+    lib.ahfgpu_get_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
     _lib = lib
     return lib
 
@@ -157,6 +169,20 @@ class Context:
         """haloes of at least min_members go through the split unbinding path
         (0: default, negative: never); same results either way"""
         self._check(self.lib.ahfgpu_set_unbind_split(self.h, int(min_members)))
+
+    def schedule(self):
+        """the kernel schedule in force (ahfgpu_schedule fields, 0 = default)"""
+        sc = Schedule()
+        self._check(self.lib.ahfgpu_get_schedule(self.h, ctypes.byref(sc)))
+        return sc.as_dict()
+
+    def set_schedule(self, **fields):
+        """change some schedule fields (the others keep their value)"""
+        sc = Schedule()
+        self._check(self.lib.ahfgpu_get_schedule(self.h, ctypes.byref(sc)))
+        for k, v in fields.items():
+            setattr(sc, k, int(v))
+        self._check(self.lib.ahfgpu_set_schedule(self.h, ctypes.byref(sc)))
 
     def set_gather_mode(self, mode):
         """deposit kernel choice (same results in every mode)"""

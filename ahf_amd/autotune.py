@@ -1,0 +1,229 @@
+"""Autotuner of the kernel schedule (ahfgpu_schedule): the thresholds that
+choose between kernels that give the same results -- which haloes take the
+split unbinding path, the unbinding and HaloProfiles workgroup tiers, the
+contribution count above which a node's density is folded by a wavefront,
+the march's planes per workgroup and the split path's graph replays per done
+check.
+
+Run on the GPU, it times each candidate value on the workloads the values
+matter for (the 256^3 box: one AMR build and the unbinding of its haloes;
+the 100,000-halo workload: the construction and HaloProfiles of its
+haloes; the deposit fields again on the 512^3 box, kept as that workload's
+own schedule), one field at a time with the others at their best so far, and
+keeps a value only if every result is the default schedule's: every grid
+level's densities and node sets bit for bit, every halo's members in order,
+M_vir, R_vir and ovdens exactly, and Phi0 to the rounding of its parallel
+prefix sum.  It writes the choice to schedule.json beside the library's
+directory, which ahfgpu_create applies (or the file $AHF_SCHEDULE names).
+
+    python -m ahf_amd.autotune [--out PATH] [--reps N] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(HERE, "schedule.json")
+
+# candidate values per field, in the order they are tuned (the default first)
+CANDIDATES = {
+    "kmax_big": [384, 192, 256, 512, 768, 1024],
+    "kmax_small": [96, 48, 64, 128, 192],
+    "march_zn": [0, 8, 16],
+    "big_halo": [2048, 1024, 4096, 8192],
+    "wave_halo": [512, 256, 1024],
+    "split_min": [16384, 4096, 8192, 32768, 65536, -1],
+    "graph_batch": [4, 2, 8],
+    "prof_split": [2048, 512, 1024, 4096, 8192],
+}
+# which timing decides each field
+DECIDES = {"kmax_big": "amr", "kmax_small": "amr", "march_zn": "amr", "big_halo": "unbind", "wave_halo": "unbind",
+           "split_min": "unbind", "graph_batch": "unbind", "prof_split": "profiles"}
+
+
+def log(msg):
+    sys.stderr.write(f"[autotune] {msg}\n")
+    sys.stderr.flush()
+
+
+class Workloads:
+    def __init__(self, ctx, quick, box_cfg=1):
+        from ahf_amd import dumps
+        self.ctx = ctx
+        self.box = dumps.generate(box_cfg, 128 if quick else 0, 1000 + box_cfg)
+        self.iso = dumps.generate(2, 2000 if quick else 0, 1002)
+        log(f"box: {self.box.npart} particles; isolated haloes: {self.iso.nhalos} haloes, {self.iso.n_ipart} members")
+        self.resident = None
+
+    def _particles(self, s):
+        if self.resident is not s:
+            self.ctx.set_particles(s.pos, s.mom)
+            self.resident = s
+
+    def run_amr(self):
+        s, c = self.box, self.ctx
+        self._particles(s)
+        info = c.amr_build(s.L_dom, s.L_max, s.Nth_dom, s.Nth_ref)
+        return info, c.last_timing()["amr_ms"]
+
+    def amr_digest(self):
+        info, _ = self.run_amr()
+        h = hashlib.sha256()
+        for i in range(info.n_levels):
+            lv = self.ctx.amr_level(i, lists=False)
+            h.update(lv["oct_key"].tobytes())
+            h.update(lv["dens"].tobytes())
+        return h.hexdigest()
+
+    def run_unbind(self):
+        c = self.ctx
+        ms = 0.0
+        for s in (self.box, self.iso):
+            self._particles(s)
+            c.set_halos(s.centre, s.halo_npart, s.ipart)
+            c.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=c.UNBIND_WITH_RVIR,
+                     ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+            ms += c.last_timing()["unbind_ms"]
+        return ms
+
+    def unbind_result(self):
+        c, s = self.ctx, self.iso
+        self.run_unbind()
+        npart, props, lists = c.unbind_results()
+        h = hashlib.sha256()
+        h.update(npart.tobytes())
+        for x in lists:
+            h.update(x.tobytes())
+        for j in (0, 1, 3):
+            h.update(props[:, j].tobytes())
+        return h.hexdigest(), props[:, 2].copy()
+
+    def construct_once(self):
+        """the constructed haloes of the isolated-halo workload under the
+        schedule in force: the fixed input of the HaloProfiles timings"""
+        c, s = self.ctx, self.iso
+        self._particles(s)
+        c.set_halos(s.centre, s.halo_npart, s.ipart)
+        c.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=c.UNBIND_WITH_RVIR,
+                 ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+        npart, props, lists = c.unbind_results()
+        self.built = (npart, np.concatenate(lists + [np.zeros(0, np.uint64)]), np.stack([props[:, 2], props[:, 1]], 1))
+
+    def run_profiles(self):
+        c, s = self.ctx, self.iso
+        self._particles(s)
+        npart, ip, pr = self.built
+        c.set_halos(s.centre, npart, ip)
+        nb, props, prof = c.halo_profiles(s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble, phi0_rvir=pr)
+        return nb, props, prof, c.timers()["halo_profiles"][0]
+
+    def profiles_digest(self):
+        nb, props, prof, _ = self.run_profiles()
+        h = hashlib.sha256()
+        h.update(nb.tobytes())
+        h.update(props.tobytes())
+        for p in prof:
+            h.update(p.tobytes())
+        return h.hexdigest()
+
+
+def tune(reps=5, quick=False, box_cfg=1, fields=None):
+    """coordinate search over the fields (all by default) on the box
+    workload `box_cfg` (1: 256^3, 3: 512^3) and the isolated haloes"""
+    import ahf_amd
+    ctx = ahf_amd.Context(0)
+    # start from the defaults, whatever schedule file is in force
+    ctx.set_schedule(**{k: 0 for k in CANDIDATES})
+    w = Workloads(ctx, quick, box_cfg)
+    fields = list(CANDIDATES) if fields is None else fields
+    kinds = {DECIDES[f] for f in fields}
+    ref_amr = w.amr_digest() if "amr" in kinds else None
+    ref_unb, ref_phi = w.unbind_result() if "unbind" in kinds else (None, None)
+    if "profiles" in kinds:
+        w.construct_once()
+        ref_prof = w.profiles_digest()
+    timers = {"amr": lambda: w.run_amr()[1], "unbind": w.run_unbind, "profiles": lambda: w.run_profiles()[3]}
+
+    def timed(kind):
+        timers[kind]()
+        return float(np.median([timers[kind]() for _ in range(reps)]))
+
+    def same(kind):
+        """the results the field's kernels produce, against the default
+        schedule's: the hierarchy bit for bit; the haloes exactly but for
+        Phi0, which takes the rounding of a parallel prefix sum; the
+        profiles of the same haloes bit for bit"""
+        if kind == "amr" and w.amr_digest() != ref_amr:
+            return "AMR hierarchy differs"
+        if kind == "unbind":
+            d, phi = w.unbind_result()
+            if d != ref_unb:
+                return "unbinding results differ"
+            big = np.abs(ref_phi) > 0
+            if big.any() and np.max(np.abs(phi[big] - ref_phi[big]) / np.abs(ref_phi[big])) > 1e-9:
+                return "Phi0 beyond the rounding of a parallel sum"
+        if kind == "profiles" and w.profiles_digest() != ref_prof:
+            return "HaloProfiles differ"
+        return None
+
+    chosen = {k: 0 for k in fields}
+    report = {}
+    for field in fields:
+        values = CANDIDATES[field]
+        kind = DECIDES[field]
+        rows = []
+        for v in values:
+            ctx.set_schedule(**{field: v})
+            t = timed(kind)
+            bad = same(kind) if v != values[0] else None
+            rows.append((v, t, bad))
+            log(f"{field} = {v}: {kind} {t:.3f} ms" + (f"  REJECTED: {bad}" if bad else ""))
+        ok = [r for r in rows if r[2] is None]
+        best = min(ok, key=lambda r: r[1])
+        # keep the default unless another value is faster by more than 2 %
+        if best[0] != values[0] and best[1] > 0.98 * rows[0][1]:
+            best = rows[0]
+        chosen[field] = best[0]
+        ctx.set_schedule(**{field: best[0]})
+        report[field] = {"chosen": best[0], "timings_ms": {str(v): t for v, t, _ in rows},
+                         "rejected": {str(v): b for v, _, b in rows if b}, "decided_by": kind}
+    total = {k: timed(k) for k in kinds}
+    ctx.close()
+    return chosen, report, total
+
+
+AMR_FIELDS = [f for f in CANDIDATES if DECIDES[f] == "amr"]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--quick", action="store_true", help="reduced workloads (a check of the tuner itself)")
+    args = ap.parse_args()
+    t0 = time.time()
+    chosen, report, total = tune(args.reps, args.quick, 1)
+    out = dict(chosen)
+    # the deposit fields again on the 512^3 box, whose dense halo cores give
+    # the wavefront folds far more work: a schedule per workload that bench.py
+    # applies to that box (the top-level values are the 256^3 box's)
+    c3, r3, t3 = tune(max(3, args.reps // 2), args.quick, 3, AMR_FIELDS)
+    out["per_workload"] = {"cfg3": c3}
+    out["tuning"] = {"workloads": "256^3 box (seed 1001), 100,000 isolated haloes (seed 1002); deposit fields "
+                                  "also on the 512^3 box (seed 1003)" + (" at reduced size" if args.quick else ""),
+                     "fields": report, "fields_cfg3": r3, "final_ms": total, "final_ms_cfg3": t3,
+                     "seconds": time.time() - t0}
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    log(f"wrote {args.out}: " + ", ".join(f"{k}={v}" for k, v in chosen.items()))
+
+
+if __name__ == "__main__":
+    main()

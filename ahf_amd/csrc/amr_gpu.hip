@@ -1907,12 +1907,14 @@ static int gather_level(ahfgpu_ctx *c, DevLevel &l, int64_t n_ent, int descendin
   const dim3 grid((unsigned)(AHF_XCDS * per_xcd));
   /* long lanes in a small launch leave the chip idle: hand them to the
    * wavefront-per-node kernel sooner */
-  const uint32_t kmax = c->gather_mode == AHFGPU_GATHER_WAVE ? 0u : nn >= ((int64_t)1 << 21) ? AHF_KMAX_BIG : AHF_KMAX_SMALL;
+  const uint32_t kmax_big = c->kmax_big > 0 ? (uint32_t)c->kmax_big : AHF_KMAX_BIG;
+  const uint32_t kmax_small = c->kmax_small > 0 ? (uint32_t)c->kmax_small : AHF_KMAX_SMALL;
+  const uint32_t kmax = c->gather_mode == AHFGPU_GATHER_WAVE ? 0u : nn >= ((int64_t)1 << 21) ? kmax_big : kmax_small;
   static_assert(AHF_KMAX_BIG < 4096 && AHF_KMAX_SMALL < 4096, "the march's queue words hold counts below 4096");
   if (slab) {
     /* the planes 1..L-2 march, the two face planes take the per-plane walk */
     const int64_t side = l.L / AHF_SLAB;
-    const int64_t zn = l.L <= 128 ? 8 : 16, nch = (l.L - 2 + zn - 1) / zn;
+    const int64_t zn = c->march_zn > 0 ? c->march_zn : l.L <= 128 ? 8 : 16, nch = (l.L - 2 + zn - 1) / zn;
     const int64_t per_m = (side * side * nch + AHF_XCDS - 1) / AHF_XCDS;
     const int64_t per_f = (side * side * 2 + AHF_XCDS - 1) / AHF_XCDS;
     const dim3 mgrid((unsigned)(AHF_XCDS * per_m)), fgrid((unsigned)(AHF_XCDS * per_f));

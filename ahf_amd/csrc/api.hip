@@ -4,7 +4,12 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+
+#include <string>
 
 #include "../../include/ahf_gpu.h"
 #include "devprims.h"
@@ -18,6 +23,94 @@ __global__ void k_deinterleave(const float *__restrict__ xyz, int64_t n, float *
   x[i] = xyz[3 * i];
   y[i] = xyz[3 * i + 1];
   z[i] = xyz[3 * i + 2];
+}
+
+/* the integer value of "key" in a flat JSON object, if present */
+static bool json_int(const std::string &txt, const char *key, long long *v)
+{
+  const std::string k = std::string("\"") + key + "\"";
+  size_t p = txt.find(k);
+  if (p == std::string::npos) return false;
+  p = txt.find(':', p + k.size());
+  if (p == std::string::npos) return false;
+  char *end = nullptr;
+  const long long x = strtoll(txt.c_str() + p + 1, &end, 10);
+  if (end == txt.c_str() + p + 1) return false;
+  *v = x;
+  return true;
+}
+
+/* the schedule file: $AHF_SCHEDULE, else schedule.json beside the
+ * library's directory (the package directory); silently absent */
+static void load_schedule(ahfgpu_ctx *c)
+{
+  std::string path;
+  const char *env = getenv("AHF_SCHEDULE");
+  if (env && *env) {
+    path = env;
+  } else {
+    Dl_info info;
+    if (!dladdr((void *)&ahfgpu_create, &info) || !info.dli_fname) return;
+    path = info.dli_fname;
+    for (int up = 0; up < 2; up++) {
+      const size_t slash = path.find_last_of('/');
+      if (slash == std::string::npos) return;
+      path.resize(slash);
+    }
+    path += "/schedule.json";
+  }
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return;
+  std::string txt;
+  char buf[4096];
+  size_t n;
+  while ((n = fread(buf, 1, sizeof(buf), f)) > 0 && txt.size() < (1u << 20)) txt.append(buf, n);
+  fclose(f);
+  ahfgpu_schedule sc;
+  ahfgpu_get_schedule(c, &sc);
+  long long v;
+  if (json_int(txt, "split_min", &v)) sc.split_min = v;
+  if (json_int(txt, "prof_split", &v)) sc.prof_split = v;
+  if (json_int(txt, "big_halo", &v)) sc.big_halo = v;
+  if (json_int(txt, "wave_halo", &v)) sc.wave_halo = v;
+  if (json_int(txt, "kmax_big", &v)) sc.kmax_big = (int32_t)v;
+  if (json_int(txt, "kmax_small", &v)) sc.kmax_small = (int32_t)v;
+  if (json_int(txt, "march_zn", &v)) sc.march_zn = (int32_t)v;
+  if (json_int(txt, "graph_batch", &v)) sc.graph_batch = (int32_t)v;
+  if (ahfgpu_set_schedule(c, &sc) != 0) fprintf(stderr, "ahfgpu: schedule %s ignored: %s\n", path.c_str(), c->err);
+}
+
+extern "C" int ahfgpu_set_schedule(ahfgpu_ctx *c, const ahfgpu_schedule *s)
+{
+  if (!c || !s) return AHFGPU_EINVAL;
+  if (s->kmax_big < 0 || s->kmax_big >= 4096 || s->kmax_small < 0 || s->kmax_small >= 4096)
+    return c->fail(AHFGPU_EINVAL, "kmax must be in [0, 4096)");
+  if (s->march_zn != 0 && s->march_zn != 8 && s->march_zn != 16) return c->fail(AHFGPU_EINVAL, "march_zn: 0, 8 or 16");
+  if (s->big_halo < 0 || s->wave_halo < 0) return c->fail(AHFGPU_EINVAL, "tier thresholds must not be negative");
+  if (s->graph_batch < 0 || s->graph_batch > 64) return c->fail(AHFGPU_EINVAL, "graph_batch must be in [0, 64]");
+  c->split_min = s->split_min;
+  c->prof_split = s->prof_split;
+  c->big_halo = s->big_halo;
+  c->wave_halo = s->wave_halo;
+  c->kmax_big = s->kmax_big;
+  c->kmax_small = s->kmax_small;
+  c->march_zn = s->march_zn;
+  c->split_graph_batch = s->graph_batch;
+  return 0;
+}
+
+extern "C" int ahfgpu_get_schedule(ahfgpu_ctx *c, ahfgpu_schedule *s)
+{
+  if (!c || !s) return AHFGPU_EINVAL;
+  s->split_min = c->split_min;
+  s->prof_split = c->prof_split;
+  s->big_halo = c->big_halo;
+  s->wave_halo = c->wave_halo;
+  s->kmax_big = c->kmax_big;
+  s->kmax_small = c->kmax_small;
+  s->march_zn = c->march_zn;
+  s->graph_batch = c->split_graph_batch;
+  return 0;
 }
 
 extern "C" int ahfgpu_create(ahfgpu_ctx **out, int device)
@@ -40,6 +133,7 @@ extern "C" int ahfgpu_create(ahfgpu_ctx **out, int device)
       hipEventCreateWithFlags(&c->ev_s2, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void **)&c->d_anomaly, 64) != hipSuccess || hipMalloc((void **)&c->d_sizes, 64) != hipSuccess ||
       hipHostMalloc((void **)&c->h_sizes, 64, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_words, 64, hipHostMallocDefault) != hipSuccess ||
       hipEventCreate(&c->ev_a0) != hipSuccess || hipEventCreate(&c->ev_a1) != hipSuccess ||
       hipEventCreate(&c->ev_u0) != hipSuccess || hipEventCreate(&c->ev_u1) != hipSuccess) {
     delete c;
@@ -48,6 +142,7 @@ extern "C" int ahfgpu_create(ahfgpu_ctx **out, int device)
   /* the scratch of ahfgpu_unbind_noalloc on this device; without it only
    * that entry point fails (it says so), the rest of the library works */
   (void)ahf_noalloc_init(device);
+  load_schedule(c);
   *out = c;
   return 0;
 }
@@ -79,6 +174,7 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   if (c->d_anomaly) (void)hipFree(c->d_anomaly);
   if (c->d_sizes) (void)hipFree(c->d_sizes);
   if (c->h_sizes) (void)hipHostFree(c->h_sizes);
+  if (c->h_words) (void)hipHostFree(c->h_words);
   if (c->d_flag) (void)hipFree(c->d_flag);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(c->st2);

@@ -126,6 +126,10 @@ struct ahfgpu_ctx {
   hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr, ev_s2 = nullptr;
   int64_t split_min = 0;     /* haloes of at least this many members take the split path; 0: default */
   int64_t prof_split = 0;    /* haloes of at least this many members are profiled by four wavefronts; 0: default */
+  /* the rest of the schedule (ahfgpu_schedule); 0: the default */
+  int32_t kmax_big = 0, kmax_small = 0; /* contributions above which a node's density is folded by a wavefront */
+  int32_t march_zn = 0;                 /* target planes per march workgroup */
+  int64_t big_halo = 0, wave_halo = 0;  /* unbinding tiers: 512-thread workgroups from, one wavefront below */
   char err[512] = {0};
   Arena arena;        /* AMR build buffers */
   Arena uarena;       /* unbinding buffers */
@@ -143,6 +147,9 @@ struct ahfgpu_ctx {
   int32_t *d_anomaly = nullptr;
   int64_t *d_sizes = nullptr;   /* level sizes gathered for one read-back  */
   int64_t *h_sizes = nullptr;   /* pinned host copy of d_sizes             */
+  int32_t *h_words = nullptr;   /* pinned host words (the split path's done flag) */
+  int split_graph_batch = 0;    /* graph replays between done checks; 0: 4 */
+  int split_graph_launches = 0; /* replays of the last split unbinding */
   void *halo_set = nullptr; /* resident halo lists (halo_set.h) */
   uint64_t *sfckey = nullptr; /* SFC keys of the resident particles */
   int64_t key_cap = 0;

@@ -1021,12 +1021,14 @@ __device__ inline const uint32_t *split_cur(const SplitHalo &H, const uint32_t *
 
 /* pass 1 of an iteration: offsets, velocities, distance of every member */
 __global__ void __launch_bounds__(256) k_sp_stage(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
+                                                  const int32_t *__restrict__ ntp,
                                                   const uint32_t *a, const uint32_t *b,
                                                   const double *__restrict__ cen, const float *__restrict__ px,
                                                   const float *__restrict__ py, const float *__restrict__ pz,
                                                   const float *__restrict__ vx, const float *__restrict__ vy,
                                                   const float *__restrict__ vz, double *__restrict__ dist, Staged g)
 {
+  if ((int32_t)blockIdx.x >= *ntp) return;
   const SplitTile T = tiles[blockIdx.x];
   const SplitHalo &H = sh[T.seg];
   const uint32_t *cur = split_cur(H, a, b);
@@ -1069,9 +1071,11 @@ __device__ inline D4 split_term(const double *dist, const Staged &g, int64_t e, 
  * part) and its total into ttot */
 __global__ void __launch_bounds__(256) k_sp_phi_tiles(const SplitHalo *__restrict__ sh,
                                                       const SplitTile *__restrict__ tiles,
+                                                  const int32_t *__restrict__ ntp,
                                                       const double *__restrict__ dist, Staged g,
                                                       double *__restrict__ phi, D4 *__restrict__ ttot)
 {
+  if ((int32_t)blockIdx.x >= *ntp) return;
   __shared__ D4 sh4[4];
   const SplitTile T = tiles[blockIdx.x];
   const SplitHalo &H = sh[T.seg];
@@ -1114,8 +1118,10 @@ __global__ void k_sp_phi_carry(SplitHalo *sh, int64_t nseg, const int32_t *__res
 }
 
 __global__ void __launch_bounds__(256) k_sp_phi_add(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
+                                                  const int32_t *__restrict__ ntp,
                                                     const D4 *__restrict__ tcarry, double *__restrict__ phi)
 {
+  if ((int32_t)blockIdx.x >= *ntp) return;
   const SplitTile T = tiles[blockIdx.x];
   const SplitHalo &H = sh[T.seg];
   const double cx = tcarry[blockIdx.x].x;
@@ -1151,9 +1157,11 @@ __global__ void __launch_bounds__(AHF_SPLIT_BLOCK) k_sp_rec(UnbindConst K, Split
  * 1 + the index of its last bound member */
 __global__ void __launch_bounds__(256) k_sp_cnt_tiles(const SplitHalo *__restrict__ sh,
                                                       const SplitTile *__restrict__ tiles,
+                                                  const int32_t *__restrict__ ntp,
                                                       const uint8_t *__restrict__ st, int32_t *__restrict__ wloc,
                                                       int64_t *__restrict__ tcnt, int64_t *__restrict__ tlast)
 {
+  if ((int32_t)blockIdx.x >= *ntp) return;
   __shared__ int64_t shi[4];
   __shared__ unsigned long long s_last;
   const SplitTile T = tiles[blockIdx.x];
@@ -1198,9 +1206,11 @@ __global__ void k_sp_cnt_carry(SplitHalo *sh, int64_t nseg, const int32_t *__res
 }
 
 __global__ void __launch_bounds__(256) k_sp_scatter(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
+                                                  const int32_t *__restrict__ ntp,
                                                     const int64_t *__restrict__ tcarry, const uint8_t *__restrict__ st,
                                                     const int32_t *__restrict__ wloc, uint32_t *a, uint32_t *b)
 {
+  if ((int32_t)blockIdx.x >= *ntp) return;
   const SplitTile T = tiles[blockIdx.x];
   const SplitHalo &H = sh[T.seg];
   const uint32_t *cur = H.in_b ? b : a;
@@ -1242,30 +1252,125 @@ __global__ void k_sp_begin(SplitHalo *sh, int64_t nseg)
   }
 }
 
-/* rem_outsideRvir on a split halo's current list (call 0 before the
- * unbinding, call 1 after), with the field bookkeeping of unbind_halo */
-__global__ void __launch_bounds__(AHF_SPLIT_BLOCK) k_sp_cut(UnbindConst K, SplitHalo *sh, int call, const uint32_t *a,
-                                                            const uint32_t *b, const double *__restrict__ cen,
-                                                            const float *__restrict__ px, const float *__restrict__ py,
-                                                            const float *__restrict__ pz)
+/* tile layout of the split haloes: one tile per AHF_TILE members of every
+ * halo that takes part (mode 0: the haloes still iterating; mode 1: those
+ * of at least minpart members, for the rem_outsideRvir cut); seg_tile0
+ * receives each halo's first tile and *ntp the tile count.  One
+ * workgroup. */
+__global__ void __launch_bounds__(256) k_sp_layout(const SplitHalo *__restrict__ sh, int64_t nseg, int mode,
+                                                   int32_t minpart, int32_t *__restrict__ seg_tile0,
+                                                   int32_t *__restrict__ ntp)
 {
-  SplitHalo &H = sh[blockIdx.x];
-  if (H.n < K.minpart) return;
-  double M, R, od;
-  const double c3[3] = {cen[3 * H.h], cen[3 * H.h + 1], cen[3 * H.h + 2]};
-  const int64_t kept = rvir_cut<AHF_SPLIT_BLOCK>(K, H.n, split_cur(H, a, b) + H.o, c3, px, py, pz, &M, &R, &od);
-  if (threadIdx.x == 0) {
-    H.n = kept;
-    H.M_vir = M;
-    H.R_vir = R;
-    H.ovdens = od;
-    if (call == 0) {
-      H.Phi0 = 0.0;
-      H.set = AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_PHI0 | AHFGPU_SET_OVDENS;
-    } else {
-      H.set |= AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_OVDENS;
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < nseg; base += 256) {
+    const int64_t i = base + threadIdx.x;
+    int32_t t = 0;
+    if (i < nseg) {
+      const SplitHalo &H = sh[i];
+      const bool on = mode == 0 ? H.active != 0 : H.n >= minpart;
+      t = on ? (int32_t)((H.n + AHF_TILE - 1) / AHF_TILE) : 0;
     }
+    int32_t tot;
+    const int32_t ex = block_excl_scan<int32_t>(t, &tot);
+    if (i < nseg) seg_tile0[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
   }
+  if (threadIdx.x == 0) {
+    seg_tile0[nseg] = carry;
+    *ntp = carry;
+  }
+}
+
+/* the tiles of split halo blockIdx.x */
+__global__ void __launch_bounds__(256) k_sp_fill_tiles(const int32_t *__restrict__ seg_tile0,
+                                                       SplitTile *__restrict__ tiles)
+{
+  const int32_t t0 = seg_tile0[blockIdx.x], t1 = seg_tile0[blockIdx.x + 1];
+  for (int32_t t = t0 + (int32_t)threadIdx.x; t < t1; t += 256)
+    tiles[t] = SplitTile{(int32_t)blockIdx.x, t0, (int64_t)(t - t0) * AHF_TILE};
+}
+
+/* rem_outsideRvir over tiles: the first member of the tile whose sphere is
+ * below ovlim (a NaN overdensity counts as below), or none */
+__global__ void __launch_bounds__(256) k_sp_cut_tiles(UnbindConst K, const SplitHalo *__restrict__ sh,
+                                                      const SplitTile *__restrict__ tiles,
+                                                      const int32_t *__restrict__ ntp, const uint32_t *a,
+                                                      const uint32_t *b, const double *__restrict__ cen,
+                                                      const float *__restrict__ px, const float *__restrict__ py,
+                                                      const float *__restrict__ pz,
+                                                      unsigned long long *__restrict__ tfirst)
+{
+  if ((int32_t)blockIdx.x >= *ntp) return;
+  __shared__ unsigned long long s_first;
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  const uint32_t *cur = split_cur(H, a, b) + H.o;
+  const double c3[3] = {cen[3 * H.h], cen[3 * H.h + 1], cen[3 * H.h + 2]};
+  if (threadIdx.x == 0) s_first = ~0ull;
+  __syncthreads();
+  unsigned long long f = ~0ull;
+  for (int i = 0; i < AHF_TILE / 256; i++) {
+    const int64_t j = T.j0 + (int64_t)i * 256 + threadIdx.x;
+    if (j >= H.n) break;
+    double r;
+    const double o = sphere_ovdens(K, j, cur[j], c3, px, py, pz, &r);
+    if (!(o >= K.ovlim) && (unsigned long long)j < f) f = (unsigned long long)j;
+  }
+  if (f != ~0ull) atomicMin(&s_first, f);
+  __syncthreads();
+  if (threadIdx.x == 0) tfirst[blockIdx.x] = s_first;
+}
+
+/* per split halo: the cut is the first failure over its tiles in order;
+ * the members up to and including it are kept, with M_vir, R_vir and
+ * ovdens of the last one kept, and the field bookkeeping of unbind_halo
+ * (call 0 before the unbinding, call 1 after) */
+__global__ void k_sp_cut_fin(UnbindConst K, SplitHalo *sh, int64_t nseg, int call, const int32_t *__restrict__ seg_tile0,
+                             const unsigned long long *__restrict__ tfirst, const uint32_t *a, const uint32_t *b,
+                             const double *__restrict__ cen, const float *__restrict__ px,
+                             const float *__restrict__ py, const float *__restrict__ pz)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  SplitHalo &H = sh[i];
+  if (H.n < K.minpart) return;
+  unsigned long long first = ~0ull;
+  for (int32_t t = seg_tile0[i]; t < seg_tile0[i + 1]; t++)
+    if (tfirst[t] != ~0ull) {
+      first = tfirst[t];
+      break;
+    }
+  const int64_t kept = first != ~0ull ? (int64_t)first + 1 : H.n;
+  const uint32_t *cur = split_cur(H, a, b) + H.o;
+  const double c3[3] = {cen[3 * H.h], cen[3 * H.h + 1], cen[3 * H.h + 2]};
+  double R;
+  const double od = sphere_ovdens(K, kept - 1, cur[kept - 1], c3, px, py, pz, &R);
+  H.n = kept;
+  H.M_vir = (double)kept;
+  H.R_vir = R;
+  H.ovdens = od;
+  if (call == 0) {
+    H.Phi0 = 0.0;
+    H.set = AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_PHI0 | AHFGPU_SET_OVDENS;
+  } else {
+    H.set |= AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_OVDENS;
+  }
+}
+
+/* 1 in *flag while some split halo still iterates (one workgroup) */
+__global__ void __launch_bounds__(256) k_sp_any(const SplitHalo *__restrict__ sh, int64_t nseg, int32_t *flag)
+{
+  __shared__ int32_t any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < nseg; i += 256)
+    if (!sh[i].done) any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) *flag = any;
 }
 
 /* entry into the unbinding proper (:3312): done below minpart, else R_vir
@@ -1701,9 +1806,10 @@ extern "C" int ahfgpu_set_halos(ahfgpu_ctx *c, const double *centres, const uint
  * stream, after the sort and the seeds on the first: rem_outsideRvir(0),
  * then iterations of staging / Phi prefix / recurrence / compaction over
  * tiles of all unfinished haloes until each one stops as rem_unbound does,
- * then rem_outsideRvir(1).  The host reads the haloes' state between
- * iterations to lay out the next iteration's tiles.  The first stream waits
- * for the result before the timing ends.
+ * then rem_outsideRvir(1).  Each iteration lays out its own tiles on the
+ * device and is one replay of a captured graph; the host only reads a done
+ * flag between batches of replays.  The first stream waits for the result
+ * before the timing ends.
  */
 __global__ void k_sp_init(SplitHalo *sh, int64_t nseg, const int32_t *__restrict__ hid, const int64_t *__restrict__ off,
                           const int64_t *__restrict__ seed)
@@ -1727,54 +1833,82 @@ static int split_unbind(ahfgpu_ctx *c, const UnbindConst &K, HaloSet *s, const i
   int32_t *hid = A.get<int32_t>(nsplit);
   int32_t *wloc = A.get<int32_t>(s->ne);
   int32_t *seg_tile0 = A.get<int32_t>(nsplit + 1);
+  int32_t *words = A.get<int32_t>(8); /* [0] tile count, [1] some halo still iterates */
+  /* tiles only shrink: the first layout bounds every later one */
   int64_t max_tiles = 0;
   for (int64_t i = 0; i < nsplit; i++) max_tiles += (s->h_off[ord[i] + 1] - s->h_off[ord[i]] + AHF_TILE - 1) / AHF_TILE;
   SplitTile *tiles = A.get<SplitTile>(max_tiles);
   D4 *ttot = A.get<D4>(max_tiles), *tcar = A.get<D4>(max_tiles);
   int64_t *tcnt = A.get<int64_t>(max_tiles), *tlast = A.get<int64_t>(max_tiles), *tcarry = A.get<int64_t>(max_tiles);
-  if (!sh || !hid || !wloc || !seg_tile0 || !tiles || !ttot || !tcar || !tcnt || !tlast || !tcarry)
+  unsigned long long *tfirst = A.get<unsigned long long>(max_tiles);
+  if (!sh || !hid || !wloc || !seg_tile0 || !words || !tiles || !ttot || !tcar || !tcnt || !tlast || !tcarry || !tfirst)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (split unbinding)");
   hipStream_t s2 = c->st2;
+  int32_t *ntp = words, *any = words + 1;
+  const unsigned gs = nb_of(nsplit), nt = (unsigned)max_tiles, ns = (unsigned)nsplit;
   hipStreamWaitEvent(s2, c->ev_s0, 0);
   hipMemcpyAsync(hid, ord, (size_t)nsplit * sizeof(int32_t), hipMemcpyHostToDevice, s2);
-  const unsigned gs = nb_of(nsplit);
   hipLaunchKernelGGL(k_sp_init, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, hid, s->off, seed);
-  if (K.cut_before)
-    hipLaunchKernelGGL(k_sp_cut, dim3((unsigned)nsplit), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, 0, s->buf_a, s->buf_b,
-                       s->cen, c->px, c->py, c->pz);
+  auto cut = [&](int call) {
+    hipLaunchKernelGGL(k_sp_layout, dim3(1), dim3(256), 0, s2, sh, nsplit, 1, K.minpart, seg_tile0, ntp);
+    hipLaunchKernelGGL(k_sp_fill_tiles, dim3(ns), dim3(256), 0, s2, seg_tile0, tiles);
+    hipLaunchKernelGGL(k_sp_cut_tiles, dim3(nt), dim3(256), 0, s2, K, sh, tiles, ntp, s->buf_a, s->buf_b, s->cen, c->px,
+                       c->py, c->pz, tfirst);
+    hipLaunchKernelGGL(k_sp_cut_fin, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit, call, seg_tile0, tfirst, s->buf_a,
+                       s->buf_b, s->cen, c->px, c->py, c->pz);
+  };
+  if (K.cut_before) cut(0);
   hipLaunchKernelGGL(k_sp_enter, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit);
-  std::vector<SplitHalo> hs((size_t)nsplit);
-  std::vector<SplitTile> ht;
-  std::vector<int32_t> t0((size_t)nsplit + 1);
-  for (;;) {
-    hipMemcpyAsync(hs.data(), sh, (size_t)nsplit * sizeof(SplitHalo), hipMemcpyDeviceToHost, s2);
-    if (hipStreamSynchronize(s2) != hipSuccess) return c->fail(AHFGPU_EHIP, "split unbinding");
-    ht.clear();
-    for (int64_t i = 0; i < nsplit; i++) {
-      t0[(size_t)i] = (int32_t)ht.size();
-      if (hs[(size_t)i].done) continue;
-      for (int64_t j0 = 0; j0 < hs[(size_t)i].n; j0 += AHF_TILE) ht.push_back(SplitTile{(int32_t)i, t0[(size_t)i], j0});
-    }
-    t0[(size_t)nsplit] = (int32_t)ht.size();
-    if (ht.empty()) break;
-    const unsigned nt = (unsigned)ht.size();
-    hipMemcpyAsync(tiles, ht.data(), ht.size() * sizeof(SplitTile), hipMemcpyHostToDevice, s2);
-    hipMemcpyAsync(seg_tile0, t0.data(), t0.size() * sizeof(int32_t), hipMemcpyHostToDevice, s2);
-    hipLaunchKernelGGL(k_sp_begin, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit);
-    hipLaunchKernelGGL(k_sp_stage, dim3(nt), dim3(256), 0, s2, sh, tiles, s->buf_a, s->buf_b, s->cen, c->px, c->py,
-                       c->pz, c->vx, c->vy, c->vz, dist, g);
-    hipLaunchKernelGGL(k_sp_phi_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, dist, g, phi, ttot);
-    hipLaunchKernelGGL(k_sp_phi_carry, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0, ttot, tcar);
-    hipLaunchKernelGGL(k_sp_phi_add, dim3(nt), dim3(256), 0, s2, sh, tiles, tcar, phi);
-    hipLaunchKernelGGL(k_sp_rec, dim3((unsigned)nsplit), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, dist, phi, st, g);
-    hipLaunchKernelGGL(k_sp_cnt_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, st, wloc, tcnt, tlast);
-    hipLaunchKernelGGL(k_sp_cnt_carry, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0, tcnt, tlast, tcarry);
-    hipLaunchKernelGGL(k_sp_scatter, dim3(nt), dim3(256), 0, s2, sh, tiles, tcarry, st, wloc, s->buf_a, s->buf_b);
-    hipLaunchKernelGGL(k_sp_update, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit, dist);
+  /* one iteration of every unfinished halo, laid out on the device and
+   * captured once as a graph; an iteration of finished haloes does
+   * nothing, so the graph is replayed until the device says none is left */
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(s2, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "split unbinding: graph capture: %s", hipGetErrorString(hipGetLastError()));
+  hipLaunchKernelGGL(k_sp_begin, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit);
+  hipLaunchKernelGGL(k_sp_layout, dim3(1), dim3(256), 0, s2, sh, nsplit, 0, K.minpart, seg_tile0, ntp);
+  hipLaunchKernelGGL(k_sp_fill_tiles, dim3(ns), dim3(256), 0, s2, seg_tile0, tiles);
+  hipLaunchKernelGGL(k_sp_stage, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, s->buf_a, s->buf_b, s->cen, c->px, c->py,
+                     c->pz, c->vx, c->vy, c->vz, dist, g);
+  hipLaunchKernelGGL(k_sp_phi_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, dist, g, phi, ttot);
+  hipLaunchKernelGGL(k_sp_phi_carry, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0, ttot, tcar);
+  hipLaunchKernelGGL(k_sp_phi_add, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, tcar, phi);
+  hipLaunchKernelGGL(k_sp_rec, dim3(ns), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, dist, phi, st, g);
+  hipLaunchKernelGGL(k_sp_cnt_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, st, wloc, tcnt, tlast);
+  hipLaunchKernelGGL(k_sp_cnt_carry, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0, tcnt, tlast, tcarry);
+  hipLaunchKernelGGL(k_sp_scatter, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, tcarry, st, wloc, s->buf_a, s->buf_b);
+  hipLaunchKernelGGL(k_sp_update, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit, dist);
+  hipLaunchKernelGGL(k_sp_any, dim3(1), dim3(256), 0, s2, sh, nsplit, any);
+  const hipError_t ec = hipStreamEndCapture(s2, &graph);
+  if (ec != hipSuccess || !graph) return c->fail(AHFGPU_EHIP, "split unbinding: graph capture: %s", hipGetErrorString(ec));
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return c->fail(AHFGPU_EHIP, "split unbinding: graph instantiation: %s", hipGetErrorString(hipGetLastError()));
   }
-  if (K.cut_after)
-    hipLaunchKernelGGL(k_sp_cut, dim3((unsigned)nsplit), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, 1, s->buf_a, s->buf_b,
-                       s->cen, c->px, c->py, c->pz);
+  int rc = 0, launched = 0;
+  for (int batch = c->split_graph_batch > 0 ? c->split_graph_batch : 4;;) {
+    for (int k = 0; k < batch; k++) hipGraphLaunch(exec, s2);
+    launched += batch;
+    hipMemcpyAsync(c->h_words, any, sizeof(int32_t), hipMemcpyDeviceToHost, s2);
+    if (hipStreamSynchronize(s2) != hipSuccess) {
+      rc = c->fail(AHFGPU_EHIP, "split unbinding: %s", hipGetErrorString(hipGetLastError()));
+      break;
+    }
+    if (!c->h_words[0]) break;
+    /* rem_unbound stops when 3 or fewer members leave, so every iteration
+     * removes at least 4: a halo cannot iterate more than n/4 times */
+    if (launched > (int)(s->ne / 4) + 8) {
+      rc = c->fail(AHFGPU_EHIP, "split unbinding did not finish after %d iterations", launched);
+      break;
+    }
+    batch = 2;
+  }
+  c->split_graph_launches = launched;
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (rc) return rc;
+  if (K.cut_after) cut(1);
   hipLaunchKernelGGL(k_sp_out, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, s->out);
   hipEventRecord(c->ev_s1, s2);
   hipStreamWaitEvent(c->st, c->ev_s1, 0);
@@ -1959,11 +2093,13 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     auto size_of = [&](int64_t q) { return s->h_off[ord[q] + 1] - s->h_off[ord[q]]; };
     if (c->split_min >= 0)
       while (nsplit < nh && size_of(nsplit) >= split_min && size_of(nsplit) >= p->minpart) nsplit++;
-    while (nbig < nh && size_of(nbig) >= AHF_BIG_HALO) nbig++;
+    const int64_t big_halo = c->big_halo > 0 ? c->big_halo : AHF_BIG_HALO;
+    const int64_t wave_halo = c->wave_halo > 0 ? std::min<int64_t>(c->wave_halo, big_halo) : AHF_WAVE_HALO;
+    while (nbig < nh && size_of(nbig) >= big_halo) nbig++;
     if (nbig < nsplit) nbig = nsplit;
     /* haloes below AHF_WAVE_HALO members: one wavefront each */
     int64_t nmid = nbig;
-    while (nmid < nh && size_of(nmid) >= AHF_WAVE_HALO) nmid++;
+    while (nmid < nh && size_of(nmid) >= wave_halo) nmid++;
     hipMemsetAsync(next, 0, 3 * sizeof(int32_t), c->st);
     /* the second and third streams start after the sort, the seeds and the
      * queue counters */
@@ -1998,7 +2134,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     /* plain form: one halo per launch, results synchronised per halo as a
      * per-call rem_unbound() would need them */
     for (int64_t h = 0; h < nh; h++) {
-      if (s->h_off[h + 1] - s->h_off[h] >= AHF_BIG_HALO)
+      if (s->h_off[h + 1] - s->h_off[h] >= (c->big_halo > 0 ? c->big_halo : AHF_BIG_HALO))
         hipLaunchKernelGGL(k_unbind_one<AHF_BIG_BLOCK>, dim3(1), dim3(AHF_BIG_BLOCK), 0, c->st, K, h, s->off,
                            seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
                            s->out);
@@ -2224,11 +2360,11 @@ __device__ inline int64_t na_first_below(const int64_t *off, const int32_t *orde
 
 /* tier bounds of the persistent kernels and their queue counters */
 __global__ void k_na_bounds(const int64_t *__restrict__ off, const int32_t *__restrict__ order, int64_t nh,
-                            int64_t *__restrict__ qb, int32_t *__restrict__ next)
+                            int64_t big_halo, int64_t wave_halo, int64_t *__restrict__ qb, int32_t *__restrict__ next)
 {
   if (threadIdx.x != 0) return;
-  const int64_t nbig = na_first_below(off, order, nh, AHF_BIG_HALO);
-  int64_t nmid = na_first_below(off, order, nh, AHF_WAVE_HALO);
+  const int64_t nbig = na_first_below(off, order, nh, big_halo);
+  int64_t nmid = na_first_below(off, order, nh, wave_halo);
   if (nmid < nbig) nmid = nbig;
   qb[0] = 0;
   qb[1] = nbig;
@@ -2355,7 +2491,9 @@ extern "C" int ahfgpu_unbind_noalloc(ahfgpu_ctx *c, const ahfgpu_unbind_params *
   hipLaunchKernelGGL(k_na_order_keys, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, off, nh, ok0, ov0);
   const int wo = radix_sort<uint32_t, uint32_t>(ok0, ov0, ok1, ov1, nh, 32, ortmp, st);
   const int32_t *order = (const int32_t *)(wo ? ov1 : ov0);
-  hipLaunchKernelGGL(k_na_bounds, dim3(1), dim3(64), 0, st, off, order, nh, qb, words);
+  const int64_t big_halo = c->big_halo > 0 ? c->big_halo : AHF_BIG_HALO;
+  const int64_t wave_halo = c->wave_halo > 0 ? std::min<int64_t>(c->wave_halo, big_halo) : AHF_WAVE_HALO;
+  hipLaunchKernelGGL(k_na_bounds, dim3(1), dim3(64), 0, st, off, order, nh, big_halo, wave_halo, qb, words);
   /* radial sort (or the lists as given) into buf_a */
   const bool sort_only = (mode & AHFGPU_UNBIND_SORT_ONLY) != 0;
   const bool no_sort = (mode & (AHFGPU_UNBIND_NO_SORT | AHFGPU_UNBIND_RVIR_ONLY)) != 0;

@@ -296,6 +296,14 @@ def main():
         f"L_max {s.L_max} ({time.perf_counter() - tg:.1f}s)")
 
     ctx = ahf_amd.Context(0)
+    # the autotuner's schedule for this workload, where it tuned one apart
+    sched_path = os.environ.get("AHF_SCHEDULE") or os.path.join(ROOT, "ahf_amd", "schedule.json")
+    if os.path.exists(sched_path):
+        with open(sched_path) as f:
+            per = json.load(f).get("per_workload", {}).get(f"cfg{args.config}")
+        if per:
+            ctx.set_schedule(**per)
+            log(f"schedule for cfg{args.config}: {per}")
     copy = copy_probe()
     if copy:
         log(f"device copy ceiling: {copy['GBs']:.0f} GB/s (read + write)")
@@ -332,13 +340,16 @@ def main():
 
     # plain form: one halo per launch, synchronised per halo
     plain = None
+    plain_unb_ms = []
     if args.plain_steps > 0:
         tp0 = time.perf_counter()
         for i in range(args.plain_steps):
             step(mode=1)
+            plain_unb_ms.append(ctx.last_timing()["unbind_ms"])
         tp1 = time.perf_counter()
         plain = s.npart * args.plain_steps / (tp1 - tp0)
-        log(f"plain form: {1e3 * (tp1 - tp0) / args.plain_steps:.3f} ms/step")
+        log(f"plain form: {1e3 * (tp1 - tp0) / args.plain_steps:.3f} ms/step, unbinding "
+            f"{np.mean(plain_unb_ms):.3f} ms (batched {np.mean(unb_ms):.3f} ms)")
 
     # the drop-in boundary hands over host arrays: time their upload (AoS
     # pageable host memory -> SoA device arrays) separately; not part of value
@@ -403,6 +414,9 @@ def main():
         "exact_replays": int(uinfo.n_exact_replays),
         "plain_value": plain,
         "batched_over_plain": (value / plain) if plain else None,
+        "unbind_plain_ms": float(np.mean(plain_unb_ms)) if plain_unb_ms else None,
+        "unbind_batched_over_plain": (float(np.mean(plain_unb_ms)) / float(np.mean(unb_ms))) if plain_unb_ms else None,
+        "schedule": ctx.schedule(),
         "stage_ms": stage_ms,
         "dominant_stage": dom,
         "roofline": {

@@ -267,6 +267,25 @@ int ahfgpu_profile_results(ahfgpu_ctx *ctx, double *props, double *prof);
  * value keeps every halo on one wavefront.  Both give the same values. */
 int ahfgpu_set_profile_split(ahfgpu_ctx *ctx, int64_t min_members);
 
+/* No counterpart in the reference: the schedule of the kernels, the
+ * thresholds that choose between equivalent kernels (every choice gives the
+ * same results).  0 in a field keeps its default; ahfgpu_create applies the
+ * JSON file named by the environment variable AHF_SCHEDULE, or else
+ * schedule.json beside the library's directory (written by the autotuner,
+ * python -m ahf_amd.autotune), when it exists. */
+typedef struct {
+  int64_t split_min;   /* haloes of this many members or more take the split path (-1: none); 16384 */
+  int64_t prof_split;  /* HaloProfiles: haloes of this many members or more on four wavefronts (-1: none); 2048 */
+  int64_t big_halo;    /* unbinding: haloes of this many members or more on 512-thread workgroups; 2048 */
+  int64_t wave_halo;   /* unbinding: haloes below this many members on one wavefront; 512 */
+  int32_t kmax_big;    /* deposit: nodes with more contributions folded by a wavefront (levels of 2^21 nodes or more); 384 */
+  int32_t kmax_small;  /* ... on smaller levels; 96 */
+  int32_t march_zn;    /* deposit: target planes per march workgroup (8 or 16); by level size */
+  int32_t graph_batch; /* split unbinding: iteration graph replays between done checks; 4 */
+} ahfgpu_schedule;
+int ahfgpu_set_schedule(ahfgpu_ctx *ctx, const ahfgpu_schedule *s);
+int ahfgpu_get_schedule(ahfgpu_ctx *ctx, ahfgpu_schedule *s);
+
 /* device milliseconds of the last build / unbind (HIP events on the
  * library's stream), and of the dominant kernel of the last build */
 int ahfgpu_last_timing(ahfgpu_ctx *ctx, double *amr_ms, double *unbind_ms, double *gather_ms,

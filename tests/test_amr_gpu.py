@@ -91,3 +91,18 @@ def test_staged_plane_overflow(oracle, ctx, tmp_path, n_bg, clump):
     snap = str(tmp_path / f"clump{n_bg}.snap")
     dumps.write_snapshot(snap, pos, L_dom=16, L_max=128, Nth_dom=2.0, Nth_ref=3.0)
     _check(oracle, ctx, snap, tmp_path, f"clump{n_bg}_{clump}", 0)
+
+
+@pytest.mark.parametrize("sched", [dict(kmax_big=8, kmax_small=8, march_zn=8), dict(kmax_big=1000, kmax_small=300, march_zn=16)])
+@pytest.mark.parametrize("cfg,side,seed", [(1, 64, 1001), (3, 64, 1003)])
+def test_schedule_variants(oracle, snapgen, ctx, tmp_path, cfg, side, seed, sched):
+    """the deposit's fold threshold and the march's planes per workgroup
+    change nothing in the hierarchy"""
+    snap = str(tmp_path / f"cfg{cfg}_{side}.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    before = ctx.schedule()
+    try:
+        ctx.set_schedule(**sched)
+        _check(oracle, ctx, snap, tmp_path, f"cfg{cfg}_{side}_{seed}_sched", 0)
+    finally:
+        ctx.set_schedule(**before)

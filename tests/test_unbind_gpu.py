@@ -198,3 +198,46 @@ def test_ties_match_oracle(oracle, ctx, tmp_path, seed, big):
     npart, props, lists = ctx.unbind_results()
     _compare(ref, npart, props, lists, f"ties {seed} construct")
     assert np.array_equal(ref.ovdens, props[:, 3])
+
+
+@pytest.mark.parametrize("sched", [dict(split_min=4096, big_halo=1024, wave_halo=256, graph_batch=1),
+                                   dict(split_min=-1, big_halo=8192, wave_halo=1024),
+                                   dict(split_min=64, graph_batch=8)])
+def test_schedule_variants(oracle, snapgen, ctx, tmp_path, sched):
+    """every schedule (split path threshold, workgroup tiers, graph replays
+    between done checks) gives the oracle's haloes"""
+    snap = str(tmp_path / "s.snap")
+    assert snapgen.ahf_snapgen_write(2, 2000, 1002, snap.encode()) == 0
+    out = str(tmp_path / "o.halo")
+    assert oracle.oracle_construct_dump(snap.encode(), out.encode(), 8, 0) == 0
+    ref = dumps.read_halos(out)
+    s = dumps.read_snapshot(snap)
+    before = ctx.schedule()
+    try:
+        ctx.set_schedule(**sched)
+        ctx.set_particles(s.pos, s.mom)
+        ctx.set_halos(s.centre, s.halo_npart, s.ipart)
+        ctx.unbind(s.minpart, s.vtune, s.r_fac, s.v_fac, s.phi_fac, s.hubble, mode=ctx.UNBIND_WITH_RVIR,
+                   ovlim=s.ovlim, rho_vir=s.rho_vir, rho_fac=s.rho_fac)
+        npart, props, lists = ctx.unbind_results()
+        _compare(ref, npart, props, lists, f"schedule {sched}")
+        assert np.array_equal(ref.ovdens, props[:, 3])
+    finally:
+        ctx.set_schedule(**before)
+
+
+def test_schedule_file(tmp_path, monkeypatch):
+    """a context applies the schedule file named by AHF_SCHEDULE"""
+    import json
+
+    import ahf_amd
+    path = tmp_path / "sched.json"
+    path.write_text(json.dumps({"split_min": 12345, "kmax_big": 500, "march_zn": 8, "graph_batch": 3,
+                                "tuning": {"fields": {"split_min": {"chosen": 1}}}}))
+    monkeypatch.setenv("AHF_SCHEDULE", str(path))
+    c = ahf_amd.Context(0)
+    try:
+        sc = c.schedule()
+        assert sc["split_min"] == 12345 and sc["kmax_big"] == 500 and sc["march_zn"] == 8 and sc["graph_batch"] == 3
+    finally:
+        c.close()
