@@ -70,6 +70,17 @@ class Schedule(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class GridinfoParams(ctypes.Structure):
+    _fields_ = [("ovlim", ctypes.c_double), ("rho_vir", ctypes.c_double), ("refine_mass", ctypes.c_double),
+                ("boxsize", ctypes.c_double)]
+
+
+class GridinfoInfo(ctypes.Structure):
+    _fields_ = [("min_ref", ctypes.c_int32), ("no_grids", ctypes.c_int32), ("tot_iso", ctypes.c_int32),
+                ("n_colours", ctypes.c_int32), ("max_ovdens", ctypes.c_double),
+                ("num_iso", ctypes.c_int32 * MAX_LEVELS), ("device_ms", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -120,6 +131,8 @@ def load():
     lib.ahfgpu_set_profile_split.argtypes = [P, ctypes.c_int64]
     lib.ahfgpu_set_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
     lib.ahfgpu_get_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
+    lib.ahfgpu_gridinfo.argtypes = [P, ctypes.POINTER(GridinfoParams), ctypes.POINTER(GridinfoInfo)]
+    lib.ahfgpu_gridinfo_results.argtypes = [P, ctypes.c_int, P, P]
     _lib = lib
     return lib
 
@@ -319,6 +332,34 @@ class Context:
         self._check(self.lib.ahfgpu_profile_results(self.h, props.ctypes.data, prof.ctypes.data))
         off = np.concatenate([[0], np.cumsum(nb.astype(np.int64))])
         return nb, props, [prof[off[h]:off[h + 1]].copy() for h in range(nh)]
+
+    def gridinfo(self, ovlim, rho_vir, refine_mass, boxsize):
+        """ahf_gridinfo on the last hierarchy (src/libahf/ahf_gridinfo.c:59-860):
+        a dumps.GridInfo whose colours are per AHF grid in octet-major node
+        order (the order of amr_level(min_ref + g)), refs the surviving
+        refinements (colour, grid, isoRefIndex, periodic x, y, z) in AHF's
+        list order; the device milliseconds in .device_ms"""
+        import numpy as np
+        from . import dumps
+        info = GridinfoInfo()
+        self._check(self.lib.ahfgpu_gridinfo(self.h, ctypes.byref(GridinfoParams(ovlim, rho_vir, refine_mass, boxsize)),
+                                             ctypes.byref(info)))
+        cols, refs = [], []
+        for g in range(info.no_grids):
+            n_oct = ctypes.c_int64()
+            self._check(self.lib.ahfgpu_amr_level_sizes(self.h, info.min_ref + g, None, ctypes.byref(n_oct), None,
+                                                        None, None))
+            col = np.zeros(max(8 * n_oct.value, 1), np.int32)
+            ref = np.zeros((max(int(info.num_iso[g]), 1), 6), np.int32)
+            self._check(self.lib.ahfgpu_gridinfo_results(self.h, g, col.ctypes.data, ref.ctypes.data))
+            cols.append(col[:8 * n_oct.value])
+            refs.append(ref[:int(info.num_iso[g])])
+        out = dumps.GridInfo(info.min_ref, info.no_grids, info.tot_iso, info.max_ovdens,
+                             np.array(info.num_iso[:info.no_grids], np.int32), cols,
+                             np.concatenate(refs) if refs else np.zeros((0, 6), np.int32))
+        out.device_ms = info.device_ms
+        out.n_colours = info.n_colours
+        return out
 
     def timers(self):
         """{stage: (ms, launches, algorithmic bytes)} of the last build + unbinding"""

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 
-HIP_SOURCES = ["api.hip", "amr_gpu.hip", "unbind_gpu.hip", "gather_gpu.hip", "profiles_gpu.hip"]
+HIP_SOURCES = ["api.hip", "amr_gpu.hip", "unbind_gpu.hip", "gather_gpu.hip", "profiles_gpu.hip", "gridinfo_gpu.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # parity-critical FP64/FP32 arithmetic: no contraction into FMA; packed FP32
 # instructions disabled for every device function

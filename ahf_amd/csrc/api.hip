@@ -168,6 +168,7 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   (void)hipStreamSynchronize(c->st);
   ahfgpu_release_halos(c);
   ahfgpu_free_profiles(c);
+  ahfgpu_free_gridinfo(c);
   c->arena.release();
   c->uarena.release();
   free_particles(c);

@@ -135,6 +135,8 @@ struct ahfgpu_ctx {
   Arena uarena;       /* unbinding buffers */
   Arena parena;       /* profile buffers */
   void *prof_result = nullptr; /* last ahfgpu_halo_profiles result (profiles_gpu.hip) */
+  Arena garena;       /* gridinfo buffers */
+  void *gi_result = nullptr;   /* last ahfgpu_gridinfo result (gridinfo_gpu.hip) */
   int64_t N = 0;
   float *px = nullptr, *py = nullptr, *pz = nullptr;
   float *vx = nullptr, *vy = nullptr, *vz = nullptr;
@@ -227,5 +229,7 @@ void ahfgpu_free_profiles(ahfgpu_ctx *c);
 /* unbind_gpu.hip: take the device address of the static scratch of
  * ahfgpu_unbind_noalloc on `device` (current), once per device */
 int ahf_noalloc_init(int device);
+/* releases the last ahfgpu_gridinfo result (gridinfo_gpu.hip) */
+void ahfgpu_free_gridinfo(ahfgpu_ctx *c);
 
 #endif

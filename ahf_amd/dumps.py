@@ -341,3 +341,44 @@ def compare_grids(a: list[GridLevel], b: list[GridLevel], *, dens_exact=True, rt
         elif not np.array_equal(x.lists, y.lists):
             diffs.append(f"{tag}: per-node particle lists differ in content or order")
     return diffs
+
+
+@dataclasses.dataclass
+class GridInfo:
+    """ahf_gridinfo's result (AHFGINF1 dump): the first AHF grid, the number
+    of AHF grids, the spatially isolated refinements per grid and in
+    total, the largest refinement overdensity, per AHF grid the node colours
+    in traversal order, and per surviving colour (ascending) its
+    spatialRefIndex entry: name, grid, index among the grid's refinements,
+    periodic x, y, z"""
+    min_ref: int
+    no_grids: int
+    tot_iso: int
+    max_ovdens: float
+    num_iso: np.ndarray
+    colours: list
+    refs: np.ndarray
+
+
+def read_gridinfo(path: str) -> GridInfo:
+    with open(path, "rb") as f:
+        if f.read(8) != b"AHFGINF1":
+            raise ValueError(f"{path}: not an AHFGINF1 dump")
+        min_ref, ng, tot, nn = struct.unpack("<iiii", f.read(16))
+        (mo,) = struct.unpack("<d", f.read(8))
+        num_iso = np.fromfile(f, np.int32, max(ng, 0))
+        cols = []
+        for _ in range(max(ng, 0)):
+            (n,) = struct.unpack("<Q", f.read(8))
+            cols.append(np.fromfile(f, np.int32, n))
+        refs = np.fromfile(f, np.int32, 6 * nn).reshape(nn, 6)
+    return GridInfo(min_ref, ng, tot, mo, num_iso, cols, refs)
+
+
+def ahf_rho_vir(omega0: float, lambda0: float, a: float) -> float:
+    """a^3 calc_rho_vir(a) with the critical density (src/libutility/
+    cosmology.c:29-53, 97-112; H0 = 100, Grav = 4.3006485e-9), in AHF's
+    operation order"""
+    H = 100.0 * np.sqrt(lambda0 * (1. - 1. / (a * a)) + omega0 * (1. / (a * a * a) - 1. / (a * a)) + 1. / (a * a))
+    rho_crit = 3. * (H * H) / 8. / 3.14159265358979323846264338 / 4.3006485e-9
+    return float((a * a * a) * rho_crit)

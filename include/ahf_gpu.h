@@ -267,6 +267,39 @@ int ahfgpu_profile_results(ahfgpu_ctx *ctx, double *props, double *prof);
  * value keeps every halo on one wavefront.  Both give the same values. */
 int ahfgpu_set_profile_split(ahfgpu_ctx *ctx, int64_t min_members);
 
+/* ahf_gridinfo (src/libahf/ahf_gridinfo.c:59-860, called at src/main.c:657)
+ * on the last hierarchy: the spatially connected refinements of every grid
+ * from min_ref on.  min_ref is the last grid whose refinement overdensity
+ * (Nth_ref * refine_mass / (boxsize/L)^3) / rho_vir lies below ovlim
+ * (ahf_gridinfo.c:147-175; Nth_ref is the build's); max_ovdens is the
+ * finest grid's.  Per AHF grid g (hierarchy level min_ref + g) every node
+ * gets the colour of its connected component over the six face neighbours
+ * get_TSCnodes finds, colours numbered 1, 2, ... across the grids in the
+ * order AHF starts them (traversal order (z, y, x), grid by grid).
+ * ahfgpu_gridinfo_results copies grid g's node colours (8*n_oct, octet-major
+ * node order as ahfgpu_amr_level_copy) and its refinements in AHF's list
+ * order, 6 int32 each: colour, grid, index among the grid's refinements
+ * (spatialRefIndex.isoRefIndex), periodic x, y, z (:604-760).  A node that
+ * meets four or more distinct colours overflows AHF's three-colour holder
+ * (undefined behaviour there); the connected components are reported. */
+typedef struct {
+  double ovlim;       /* virial overdensity, calc_virial(a) */
+  double rho_vir;     /* a^3 calc_rho_vir(a), the density it is relative to */
+  double refine_mass; /* simu.pmass * simu.med_weight */
+  double boxsize;     /* simu.boxsize */
+} ahfgpu_gridinfo_params;
+typedef struct {
+  int32_t min_ref;    /* ahf.min_ref: the hierarchy level of AHF grid 0 */
+  int32_t no_grids;   /* ahf.no_grids */
+  int32_t tot_iso;    /* totnumIsoRef */
+  int32_t n_colours;  /* colours started over all grids */
+  double max_ovdens;  /* global.max_ovdens */
+  int32_t num_iso[AHFGPU_MAX_LEVELS]; /* numIsoRef per AHF grid */
+  double device_ms;   /* the pass on the device (HIP events on the library's stream) */
+} ahfgpu_gridinfo_info;
+int ahfgpu_gridinfo(ahfgpu_ctx *ctx, const ahfgpu_gridinfo_params *p, ahfgpu_gridinfo_info *info);
+int ahfgpu_gridinfo_results(ahfgpu_ctx *ctx, int grid, int32_t *colour, int32_t *refs);
+
 /* No counterpart in the reference: the schedule of the kernels, the
  * thresholds that choose between equivalent kernels (every choice gives the
  * same results).  0 in a field keeps its default; ahfgpu_create applies the

@@ -1536,3 +1536,244 @@ int oracle_profiles_dump(const char *snap_path, const char *out_path, int thread
   snap_free(&s);
   return rc;
 }
+
+/* ------------------------------------------------------------ gridinfo ---- */
+
+/*
+ * ahf_gridinfo, src/libahf/ahf_gridinfo.c:59-860: the spatially connected
+ * refinements of every grid from min_ref on.  Restated on the grids of
+ * build_hierarchy (nodes in traversal order):
+ *   - min_ref: the last grid i (counting the domain as 0) whose refinement
+ *     overdensity (Nth_ref * refine_mass / (boxsize/L)^3) / rho_vir lies
+ *     below ovlim (:147-175); max_ovdens is the last grid's;
+ *   - per grid, colours zeroed, then every node in traversal order looks at
+ *     its six face neighbours as get_TSCnodes finds them (colourInfo,
+ *     :1126-1310: -y, -x, -z, +y, +x, +z, distinct non-zero colours in
+ *     that order of discovery, sorted ascending into three holders): none
+ *     coloured -> a new colour and a new SPATIALREF at the list's tail; one
+ *     -> it; two -> the lower, the higher colour deleted and replaced; three
+ *     -> the lowest, the other two deleted and replaced.  A replacement
+ *     restarts the node loop at the deleted colour's first node (the one
+ *     deleted last when two are) and recolours until the first uncoloured
+ *     node, which the loop then takes as usual (:206-560);
+ *   - periodic flags: a node on a low face (x, y or z = 0) whose neighbour
+ *     across that face exists marks its colour's SPATIALREF (:604-690,
+ *     testBound :1090-1124);
+ *   - spatialRefIndex[name] = {grid, rank among that grid's refinements in
+ *     list order, periodic flags} (:700-760).
+ * More than three distinct colours around one node overflow AHF's holder
+ * (undefined behaviour): reported, as the AMR anomalies are.
+ */
+typedef struct {
+  int32_t name, level;
+  int64_t start; /* node of the grid where the colour began */
+  int32_t px, py, pz;
+  int alive;
+} gi_ref;
+
+typedef struct {
+  int32_t holder[3];
+  int num_replace;
+} gi_col;
+
+static int gi_colour_info(const int32_t *colour, const int64_t t[27], gi_col *c)
+{
+  /* r = -y, g = -x, b = -z, t = +y, h = +x, n = +z */
+  const int nb[6] = {1 * 9 + 0 * 3 + 1, 1 * 9 + 1 * 3 + 0, 0 * 9 + 1 * 3 + 1, 1 * 9 + 2 * 3 + 1, 1 * 9 + 1 * 3 + 2,
+                     2 * 9 + 1 * 3 + 1};
+  int32_t seen[6];
+  int ncol = 0;
+  for (int i = 0; i < 6; i++) {
+    const int32_t v = t[nb[i]] >= 0 ? colour[t[nb[i]]] : 0;
+    if (!v) continue;
+    int old = 0;
+    for (int j = 0; j < ncol; j++) old |= seen[j] == v;
+    if (!old) seen[ncol++] = v;
+  }
+  if (ncol > 3) return -1;
+  c->holder[0] = c->holder[1] = c->holder[2] = 0;
+  for (int i = 0; i < ncol; i++) c->holder[i] = seen[i];
+  /* sort the three holders ascending (zeros first) */
+  for (int i = 0; i < 3; i++)
+    for (int j = i + 1; j < 3; j++)
+      if (c->holder[j] < c->holder[i]) {
+        const int32_t x = c->holder[i];
+        c->holder[i] = c->holder[j];
+        c->holder[j] = x;
+      }
+  const int32_t r = c->holder[0], g = c->holder[1], b = c->holder[2];
+  const int nz = (r == 0) + (g == 0) + (b == 0);
+  if (nz == 3) c->num_replace = 0;
+  else if (nz == 2) c->num_replace = 1;
+  else if (nz == 1) c->num_replace = g == b ? 1 : 2;
+  else if (r == g && r == b) c->num_replace = 1;
+  else if ((r == g && r != b) || (r == b && r != g)) c->num_replace = 2;
+  else if (g == b && g != r) c->num_replace = 3;
+  else c->num_replace = 4;
+  return 0;
+}
+
+/* the replacement rule of a recolouring sweep at one node */
+static void gi_replace(int32_t *colour, int64_t n, const gi_col *c)
+{
+  const int32_t cur = colour[n];
+  if (c->num_replace == 2) {
+    if (cur == c->holder[2]) colour[n] = c->holder[1];
+  } else if (c->num_replace == 3) {
+    if (cur == c->holder[2]) colour[n] = c->holder[0];
+  } else if (c->num_replace == 4) {
+    if (cur == c->holder[2]) colour[n] = c->holder[0];
+    if (cur == c->holder[1]) colour[n] = c->holder[0];
+  }
+}
+
+/* remove the SPATIALREF of colour `name`, returning its first node */
+static int64_t gi_delete(gi_ref *refs, int64_t nref, int32_t name)
+{
+  for (int64_t i = 0; i < nref; i++)
+    if (refs[i].alive && refs[i].name == name) {
+      refs[i].alive = 0;
+      return refs[i].start;
+    }
+  return -1;
+}
+
+int oracle_gridinfo_dump(const char *snap_path, const char *out_path, double ovlim, double rho_vir,
+                         double refine_mass)
+{
+  snap_data s;
+  if (snap_read(snap_path, &s)) return -1;
+  int64_t *next = (int64_t *)malloc((size_t)(s.h.npart ? s.h.npart : 1) * sizeof(int64_t));
+  gridlist_t gl;
+  memset(&gl, 0, sizeof(gl));
+  int64_t anomalies = build_hierarchy(&s, &gl, next);
+  /* min_ref and max_ovdens (:147-175) */
+  const int nref_grids = gl.n - 1;
+  int start = 1;
+  double refine_ovdens = 0.0;
+  for (int i = 0; i <= nref_grids; i++) {
+    const double fl1dim = (double)gl.g[i].L;
+    const double len = s.h.boxsize / fl1dim;
+    const double vol = len * len * len;
+    refine_ovdens = (s.h.Nth_ref * refine_mass / vol) / rho_vir;
+    if (refine_ovdens < ovlim) start = i;
+  }
+  const int min_ref = start, no_grids = nref_grids - min_ref + 1;
+  int32_t **colour = (int32_t **)calloc((size_t)(no_grids > 0 ? no_grids : 1), sizeof(int32_t *));
+  int64_t cap = 1024, nref = 0;
+  gi_ref *refs = (gi_ref *)malloc((size_t)cap * sizeof(gi_ref));
+  int32_t col_counter = 0;
+  int replace = 0, ub = 0;
+  gi_col c;
+  memset(&c, 0, sizeof(c));
+  int64_t t[27];
+  for (int lv = 0; lv < no_grids; lv++) {
+    grid_t *g = &gl.g[min_ref + lv];
+    int32_t *cl = colour[lv] = (int32_t *)calloc((size_t)(g->nnode ? g->nnode : 1), sizeof(int32_t));
+    for (int64_t n = 0; n < g->nnode; n++) {
+      if (replace && cl[n] == 0) replace = 0;
+      if (replace) {
+        gi_replace(cl, n, &c);
+        continue;
+      }
+      tsc(g, n, t);
+      if (gi_colour_info(cl, t, &c)) {
+        ub++;
+        c.num_replace = 1;
+      }
+      if (c.num_replace == 0) {
+        col_counter++;
+        cl[n] = col_counter;
+        if (nref == cap) {
+          cap *= 2;
+          refs = (gi_ref *)realloc(refs, (size_t)cap * sizeof(gi_ref));
+        }
+        refs[nref++] = (gi_ref){col_counter, lv, n, 0, 0, 0, 1};
+      } else if (c.num_replace == 1) {
+        cl[n] = c.holder[2];
+      } else {
+        int64_t s0;
+        if (c.num_replace == 2) {
+          cl[n] = c.holder[1];
+          s0 = gi_delete(refs, nref, c.holder[2]);
+        } else if (c.num_replace == 3) {
+          cl[n] = c.holder[0];
+          s0 = gi_delete(refs, nref, c.holder[2]);
+        } else {
+          cl[n] = c.holder[0];
+          (void)gi_delete(refs, nref, c.holder[2]);
+          s0 = gi_delete(refs, nref, c.holder[1]);
+        }
+        if (s0 < 0) {
+          ub++;
+          continue;
+        }
+        /* back to the deleted colour's first node; the loop goes on from
+         * the node after it, recolouring until an uncoloured node */
+        replace = 1;
+        gi_replace(cl, s0, &c);
+        n = s0;
+      }
+    }
+  }
+  /* periodic flags (:604-690) */
+  for (int lv = 0; lv < no_grids; lv++) {
+    grid_t *g = &gl.g[min_ref + lv];
+    for (int64_t n = 0; n < g->nnode; n++) {
+      int64_t x, y, z;
+      node_xyz(g, n, &x, &y, &z);
+      if (x != 0 && y != 0 && z != 0) continue;
+      tsc(g, n, t);
+      const int py = y == 0 && t[1 * 9 + 0 * 3 + 1] >= 0, px = x == 0 && t[1 * 9 + 1 * 3 + 0] >= 0,
+                pz = z == 0 && t[0 * 9 + 1 * 3 + 1] >= 0;
+      if (!(px || py || pz)) continue;
+      for (int64_t i = 0; i < nref; i++)
+        if (refs[i].alive && refs[i].name == colour[lv][n]) {
+          refs[i].px |= px;
+          refs[i].py |= py;
+          refs[i].pz |= pz;
+          break;
+        }
+    }
+  }
+  /* spatialRefIndex and numIsoRef (:700-760) */
+  int32_t *num_iso = (int32_t *)calloc((size_t)(no_grids > 0 ? no_grids : 1), sizeof(int32_t));
+  int32_t *iso = (int32_t *)calloc((size_t)(nref ? nref : 1), sizeof(int32_t));
+  int32_t tot = 0, alive = 0;
+  for (int64_t i = 0; i < nref; i++)
+    if (refs[i].alive) {
+      iso[i] = num_iso[refs[i].level]++;
+      tot++;
+      alive++;
+    }
+  FILE *f = fopen(out_path, "wb");
+  if (!f) return -2;
+  const int32_t hdr[4] = {min_ref, no_grids, tot, alive};
+  put(f, "AHFGINF1", 1, 8);
+  put(f, hdr, 4, 4);
+  put(f, &refine_ovdens, 8, 1);
+  put(f, num_iso, 4, (size_t)(no_grids > 0 ? no_grids : 0));
+  for (int lv = 0; lv < no_grids; lv++) {
+    const uint64_t n = (uint64_t)gl.g[min_ref + lv].nnode;
+    put(f, &n, 8, 1);
+    put(f, colour[lv], 4, (size_t)n);
+  }
+  /* surviving colours, ascending (the list is in creation order) */
+  for (int64_t i = 0; i < nref; i++)
+    if (refs[i].alive) {
+      const int32_t e[6] = {refs[i].name, refs[i].level, iso[i], refs[i].px, refs[i].py, refs[i].pz};
+      put(f, e, 4, 6);
+    }
+  fclose(f);
+  for (int lv = 0; lv < no_grids; lv++) free(colour[lv]);
+  free(colour); free(refs); free(num_iso); free(iso);
+  for (int i = 0; i < gl.n; i++) grid_free(&gl.g[i]);
+  free(gl.g);
+  free(next);
+  snap_free(&s);
+  if (ub || anomalies) {
+    fprintf(stderr, "oracle: %lld undefined-behaviour cases of the reference met\n", (long long)(ub + anomalies));
+    return -100;
+  }
+  return no_grids;
+}

@@ -115,6 +115,14 @@ int oracle_halo_profiles(const float *pos, const float *mom, const uint64_t *ipa
                          const double centre[3], double Phi0, double R_vir, const oracle_prof_params *p,
                          double *props, double *prof);
 int oracle_profiles_dump(const char *snap_path, const char *out_path, int threads, int gather);
+
+/* ahf_gridinfo (src/libahf/ahf_gridinfo.c:59-860) after the AMR build:
+ * min_ref from ovlim and rho_vir (the comoving density AHF's overdensities
+ * are relative to) and refine_mass (pmass * med_weight); writes the
+ * AHFGINF1 dump of oracle/ref_driver.c's gridinfo command.  Returns the
+ * number of AHF grids, or < 0. */
+int oracle_gridinfo_dump(const char *snap_path, const char *out_path, double ovlim, double rho_vir,
+                         double refine_mass);
 int oracle_prof_nbins(uint64_t n);
 void oracle_prof_get_axes(double it[9], double axes[3]);
 double oracle_prof_cnfw(double V2_max, double V2_vir);

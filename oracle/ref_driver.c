@@ -20,6 +20,8 @@
  *   ahf_ref amr    <snap> <grids.out> [--reps R]   dump the AMR hierarchy
  *   ahf_ref unbind <snap> <halos.out> [--threads T] [--reps R]
  *   ahf_ref sort   <snap> <halos.out> [--threads T]   sort_halo_particles
+ *   ahf_ref gridinfo <snap> <ginfo.out>   the AMR hierarchy, then ahf_gridinfo
+ *                       (reference build only)
  *   ahf_ref construct <snap> <halos.out> [--threads T]
  *   ahf_ref gather <snap> <halos.out> [--threads T]  ahf_halos_sfc_gatherParts
  *   ahf_ref construct-sfc <snap> <halos.out> [--threads T]
@@ -63,6 +65,8 @@
 #include "libahf/ahf_halos_sfc.h"
 #include "libahf/ahf_io.h"
 #include "libsfc/sfc_curve.h"
+#include "libahf/ahf.h"
+#include "globals.h"
 #define DRV_SFC_KEY(x, y, z, bits) sfc_curve_calcKey(SFC_CURVE_HILBERT, (x), (y), (z), (bits))
 #define DRV_SFC_BITS BITS_PER_DIMENSION
 #else
@@ -338,6 +342,98 @@ static void cmd_unbind(const snap_data *s, const char *out, int threads)
   dump_halos(out, h, s->h.nhalos);
   fprintf(stdout, "{\"unbind_seconds\": %.6f, \"threads\": %d}\n", t, threads);
 }
+
+#ifdef AHF_REF_BUILD
+/* ------------------------------------------------------------ gridinfo --- */
+
+static int cmp_i32(const void *a, const void *b)
+{
+  const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+  return (x > y) - (x < y);
+}
+
+/* the AMR hierarchy, then AHF's ahf_gridinfo (src/main.c:657): the
+ * spatially connected refinements.  The virial overdensity it compares
+ * with is the snapshot's (simu.UserDvir), relative to the critical
+ * density.  Dump AHFGINF1: min_ref, no_grids, totnumIsoRef, the number of
+ * surviving colours, max_ovdens, numIsoRef per AHF grid, per AHF grid its
+ * node count and node colours in traversal order, then per surviving
+ * colour (ascending) its spatialRefIndex entry. */
+static void cmd_gridinfo(const snap_data *s, const char *out)
+{
+  setup_globals(s);
+  simu.UserDvir = s->h.ovlim;
+  simu.UseRhoBack = 0;
+  simu.med_weight = 1.0;
+  io.logfile = fopen("/dev/null", "w");
+  partptr fst = make_particles(s);
+  gridls *grid_list;
+  int no_grids;
+  (void)run_amr(s, fst, &grid_list, &no_grids);
+  const double t0 = now_s();
+  ahf_gridinfo(grid_list, no_grids - 1);
+  const double t = now_s() - t0;
+  FILE *f = fopen(out, "wb");
+  if (!f) die("cannot open output");
+  int32_t hdr[4] = {ahf.min_ref, ahf.no_grids, totnumIsoRef, 0};
+  /* the colours that survive: those the nodes carry */
+  int32_t *names = NULL;
+  size_t nn = 0, cap = 0;
+  for (int g = 0; g < ahf.no_grids; g++) {
+    gridls *cg = global.dom_grid + ahf.min_ref + g;
+    for (pqptr pq = cg->pquad; pq; pq = pq->next)
+      for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++)
+        for (cqptr cq = cq0; cq; cq = cq->next)
+          for (nqptr nq0 = cq->loc; nq0 < cq->loc + cq->length; nq0++)
+            for (nqptr nq = nq0; nq; nq = nq->next)
+              for (nptr nd = nq->loc; nd < nq->loc + nq->length; nd++) {
+                if (nn == cap) {
+                  cap = cap ? 2 * cap : 1024;
+                  names = (int32_t *)realloc(names, cap * sizeof(int32_t));
+                  if (!names) die("out of memory");
+                }
+                names[nn++] = nd->force.colour;
+              }
+  }
+  if (nn) qsort(names, nn, sizeof(int32_t), cmp_i32);
+  size_t nu = 0;
+  for (size_t i = 0; i < nn; i++)
+    if (i == 0 || names[i] != names[i - 1]) names[nu++] = names[i];
+  hdr[3] = (int32_t)nu;
+  fwrite("AHFGINF1", 1, 8, f);
+  fwrite(hdr, 4, 4, f);
+  fwrite(&global.max_ovdens, 8, 1, f);
+  for (int g = 0; g < ahf.no_grids; g++) fwrite(&numIsoRef[g], 4, 1, f);
+  for (int g = 0; g < ahf.no_grids; g++) {
+    gridls *cg = global.dom_grid + ahf.min_ref + g;
+    uint64_t n = 0;
+    for (pqptr pq = cg->pquad; pq; pq = pq->next)
+      for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++)
+        for (cqptr cq = cq0; cq; cq = cq->next)
+          for (nqptr nq0 = cq->loc; nq0 < cq->loc + cq->length; nq0++)
+            for (nqptr nq = nq0; nq; nq = nq->next) n += (uint64_t)nq->length;
+    fwrite(&n, 8, 1, f);
+    for (pqptr pq = cg->pquad; pq; pq = pq->next)
+      for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++)
+        for (cqptr cq = cq0; cq; cq = cq->next)
+          for (nqptr nq0 = cq->loc; nq0 < cq->loc + cq->length; nq0++)
+            for (nqptr nq = nq0; nq; nq = nq->next)
+              for (nptr nd = nq->loc; nd < nq->loc + nq->length; nd++) {
+                const int32_t c = nd->force.colour;
+                fwrite(&c, 4, 1, f);
+              }
+  }
+  for (size_t i = 0; i < nu; i++) {
+    const SRINDEX *r = &spatialRefIndex[names[i]];
+    const int32_t e[6] = {names[i], r->refLevel, r->isoRefIndex, r->periodic.x, r->periodic.y, r->periodic.z};
+    fwrite(e, 4, 6, f);
+  }
+  fclose(f);
+  free(names);
+  fprintf(stdout, "{\"gridinfo_seconds\": %.6f, \"min_ref\": %d, \"no_grids\": %d, \"totnumIsoRef\": %d}\n", t,
+          ahf.min_ref, ahf.no_grids, totnumIsoRef);
+}
+#endif
 
 /* sort_halo_particles alone per halo (src/libahf/ahf_halos.c:5786-5894):
  * the radial order, equal distances in indexx's order */
@@ -815,6 +911,11 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "unbind")) {
     if (argc < 4) die("unbind needs an output path");
     cmd_unbind(&s, argv[3], threads);
+#ifdef AHF_REF_BUILD
+  } else if (!strcmp(argv[1], "gridinfo")) {
+    if (argc < 4) die("gridinfo needs an output path");
+    cmd_gridinfo(&s, argv[3]);
+#endif
   } else if (!strcmp(argv[1], "sort")) {
     if (argc < 4) die("sort needs an output path");
     cmd_sort(&s, argv[3], threads);

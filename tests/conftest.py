@@ -54,6 +54,8 @@ def _load_oracle():
     lib.oracle_prof_find_max.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.oracle_prof_find_max.restype = None
+    lib.oracle_gridinfo_dump.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double]
     return lib
 
 

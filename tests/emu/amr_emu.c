@@ -8,7 +8,13 @@
  * be checked against the CPU oracle without a GPU; it is never linked into
  * the library.
  *
- *   amr_emu <snapshot> <grids.out>
+ *   amr_emu <snapshot> <grids.out> [<ginfo.out> <ovlim> <rho_vir> <refine_mass>]
+ *
+ * With the last four arguments it also replays the ahf_gridinfo pass
+ * (ahf_amd/csrc/gridinfo_gpu.hip: traversal ranks, start flags, union of
+ * each node with its found face neighbours before it under the lowest rank,
+ * colours from the starts, refinements at the roots) and writes the
+ * AHFGINF1 dump of the oracle.
  */
 #include <math.h>
 #include <stdint.h>
@@ -17,6 +23,7 @@
 #include <string.h>
 
 #include "../../ahf_amd/csrc/amr_core.h"
+#include "../../ahf_amd/csrc/gridinfo_core.h"
 #include "../../ahf_amd/csrc/level_host.h"
 #include "../../ahf_amd/csrc/snapfile.h"
 
@@ -77,6 +84,110 @@ static void deposit_q(lev_t *l, const float *px, const float *py, const float *p
   for (int64_t n = 0; n < nn; n++)
     l->dens[n] = ahf_gather_dens_w(&l->v, l->pmask, l->node_p0, W, descending, n / 8, (int)(n % 8));
   free(W);
+}
+
+static int64_t gi_find(int64_t *par, int64_t x)
+{
+  while (par[x] != x) x = par[x];
+  return x;
+}
+
+/* the gridinfo pass on levels [0, nlev) as the kernels run it */
+static int gridinfo_replay(lev_t *lev, int nlev, const snap_data *s, const char *out, double ovlim, double rho_vir,
+                           double refine_mass)
+{
+  int startg = 1;
+  double ovd = 0.0;
+  for (int i = 0; i < nlev; i++) {
+    const double len = s->h.boxsize / (double)lev[i].v.L;
+    const double vol = len * len * len;
+    ovd = (s->h.Nth_ref * refine_mass / vol) / rho_vir;
+    if (ovd < ovlim) startg = i;
+  }
+  const int min_ref = startg, no_grids = nlev - 1 - min_ref + 1;
+  FILE *f = fopen(out, "wb");
+  if (!f) return 1;
+  int32_t *num_iso = (int32_t *)xm(sizeof(int32_t) * (size_t)(no_grids > 0 ? no_grids : 1));
+  int32_t **colour = (int32_t **)xm(sizeof(int32_t *) * (size_t)(no_grids > 0 ? no_grids : 1));
+  int64_t *nnode = (int64_t *)xm(sizeof(int64_t) * (size_t)(no_grids > 0 ? no_grids : 1));
+  int32_t *refs = NULL;
+  int64_t nrefs = 0, base = 0;
+  int32_t tot = 0;
+  for (int g = 0; g < no_grids; g++) {
+    lev_t *l = &lev[min_ref + g];
+    const int64_t nn = 8 * l->v.n_oct;
+    int64_t *rank = (int64_t *)xm((size_t)nn * 8), *par = (int64_t *)xm((size_t)nn * 8);
+    uint8_t *start = (uint8_t *)xm((size_t)nn), *seen = (uint8_t *)xm((size_t)nn);
+    uint32_t *pf = (uint32_t *)xm((size_t)nn * 4);
+    int64_t *sname = (int64_t *)xm((size_t)nn * 8), *rpos = (int64_t *)xm((size_t)nn * 8);
+    for (int64_t n = 0; n < nn; n++) {
+      rank[n] = ahf_gi_rank(&l->v, n);
+      if (rank[n] < 0 || rank[n] >= nn || seen[rank[n]]) {
+        fprintf(stderr, "emu: traversal ranks of grid %d are not a permutation\n", g);
+        return 4;
+      }
+      seen[rank[n]] = 1;
+      par[n] = n;
+    }
+    for (int64_t n = 0; n < nn; n++) {
+      int64_t u[6];
+      ahf_gi_faces(&l->v, l->pmask[n], n, u);
+      int before = 0;
+      for (int d = 0; d < 6; d++)
+        if (u[d] >= 0 && rank[u[d]] < rank[n]) {
+          before = 1;
+          int64_t a = gi_find(par, rank[u[d]]), b = gi_find(par, rank[n]);
+          if (a != b) {
+            if (a < b) par[b] = a;
+            else par[a] = b;
+          }
+        }
+      start[rank[n]] = (uint8_t)!before;
+    }
+    memset(pf, 0, (size_t)nn * 4);
+    for (int64_t n = 0; n < nn; n++) pf[gi_find(par, rank[n])] |= ahf_gi_periodic(&l->v, l->pmask[n], n);
+    int64_t ns = 0, nr = 0;
+    for (int64_t r = 0; r < nn; r++) {
+      sname[r] = ns;
+      ns += start[r];
+      rpos[r] = nr;
+      nr += par[r] == r;
+    }
+    colour[g] = (int32_t *)xm((size_t)nn * 4);
+    nnode[g] = nn;
+    for (int64_t n = 0; n < nn; n++) colour[g][rank[n]] = (int32_t)(base + sname[gi_find(par, rank[n])] + 1);
+    refs = (int32_t *)realloc(refs, sizeof(int32_t) * 6 * (size_t)(nrefs + nr + 1));
+    for (int64_t r = 0; r < nn; r++)
+      if (par[r] == r) {
+        int32_t *e = refs + 6 * (nrefs + rpos[r]);
+        e[0] = (int32_t)(base + sname[r] + 1);
+        e[1] = g;
+        e[2] = (int32_t)rpos[r];
+        e[3] = (int32_t)(pf[r] & 1u);
+        e[4] = (int32_t)((pf[r] >> 1) & 1u);
+        e[5] = (int32_t)((pf[r] >> 2) & 1u);
+      }
+    nrefs += nr;
+    num_iso[g] = (int32_t)nr;
+    tot += (int32_t)nr;
+    base += ns;
+    free(rank); free(par); free(start); free(seen); free(pf); free(sname); free(rpos);
+  }
+  const int32_t hdr[4] = {min_ref, no_grids, tot, (int32_t)nrefs};
+  fwrite("AHFGINF1", 1, 8, f);
+  fwrite(hdr, 4, 4, f);
+  fwrite(&ovd, 8, 1, f);
+  if (no_grids > 0) fwrite(num_iso, 4, (size_t)no_grids, f);
+  for (int g = 0; g < no_grids; g++) {
+    const uint64_t n = (uint64_t)nnode[g];
+    fwrite(&n, 8, 1, f);
+    fwrite(colour[g], 4, (size_t)n, f);
+    free(colour[g]);
+  }
+  if (nrefs) fwrite(refs, 4, 6 * (size_t)nrefs, f);
+  fclose(f);
+  free(refs); free(colour); free(nnode); free(num_iso);
+  return 0;
 }
 
 int main(int argc, char **argv)
@@ -328,6 +439,10 @@ int main(int argc, char **argv)
     h->list0[nn] = w;
   }
   int rc = hierarchy_dump(argv[2], hl, nlev);
+  if (!rc && argc >= 7) {
+    rc = gridinfo_replay(lev, nlev, &s, argv[3], atof(argv[4]), atof(argv[5]), atof(argv[6]));
+    if (rc == 4) return 4;
+  }
   printf("{\"levels\": %d, \"anomaly\": %d}\n", nlev, anomaly);
   return rc ? 1 : (anomaly ? 3 : 0);
 }

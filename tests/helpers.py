@@ -167,3 +167,23 @@ def tie_snapshot(path, seed, big=False):
                          halo_lists=lists, r_fac=1.0, v_fac=1.0, phi_fac=4.3e-9 * 1e8, hubble=0.0,
                          gather_rad=np.full(len(lists), 0.02), seed=seed)
     return len(lists)
+
+
+def gridinfo_setup(path, how):
+    """ahf_gridinfo's parameters for a snapshot file (ovlim, rho_vir,
+    refine_mass, boxsize), the snapshot's virial overdensity rewritten so
+    that the AHF grids start where `how` says: "hdr" keeps the snapshot's
+    value, "dom" puts the first AHF grid on the domain grid (an ovlim
+    between the domain grid's refinement overdensity and the next grid's),
+    "one" on the first refinement (an ovlim below every grid's: AHF's
+    default start, ahf_gridinfo.c:158)"""
+    s = dumps.read_snapshot(path)
+    rho = dumps.ahf_rho_vir(s.omega0, s.lambda0, s.a)
+    ovd0 = (s.Nth_ref * s.pmass / (s.boxsize / float(s.L_dom)) ** 3) / rho
+    ovlim = {"hdr": s.ovlim, "dom": 2.0 * ovd0, "one": 0.5 * ovd0}[how]
+    if how != "hdr":
+        off = dumps.SNAP_HEADER.size - 16
+        with open(path, "r+b") as f:
+            f.seek(off)
+            f.write(np.float64(ovlim).tobytes())
+    return ovlim, rho, s.pmass, s.boxsize

@@ -3,6 +3,7 @@ replayed on the host by tests/emu/amr_emu.c, against the CPU oracle."""
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from ahf_amd import dumps
@@ -78,3 +79,39 @@ def test_indexx_level_replay(oracle, tmp_path):
         r = subprocess.run([exe, "2000", str(seed)], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr[-2000:] + r.stdout
         assert '"mismatches": 0' in r.stdout
+
+
+def _cmp_gridinfo(oracle, emu, snap, tmp_path, how):
+    from helpers import gridinfo_setup
+    ovlim, rho, pmass, _ = gridinfo_setup(snap, how)
+    o, e = str(tmp_path / "o.ginf"), str(tmp_path / "e.ginf")
+    rc = oracle.oracle_gridinfo_dump(snap.encode(), o.encode(), ovlim, rho, pmass)
+    if rc == -100:
+        pytest.skip("the reference meets undefined behaviour on this input")
+    assert rc >= 0
+    r = subprocess.run([emu, snap, str(tmp_path / "e.grid"), e, repr(ovlim), repr(rho), repr(pmass)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    a, b = dumps.read_gridinfo(e), dumps.read_gridinfo(o)
+    assert (a.min_ref, a.no_grids, a.tot_iso, a.max_ovdens) == (b.min_ref, b.no_grids, b.tot_iso, b.max_ovdens)
+    assert np.array_equal(a.num_iso, b.num_iso)
+    for g, (x, y) in enumerate(zip(a.colours, b.colours)):
+        assert np.array_equal(x, y), f"grid {g}: node colours differ"
+    assert np.array_equal(a.refs, b.refs)
+
+
+@pytest.mark.parametrize("seed", range(300, 340))
+def test_emu_gridinfo_fuzz(oracle, emu, tmp_path, seed):
+    """the gridinfo kernels' rules (traversal ranks from the octet rows and
+    planes, starts, unions under the lowest rank) replayed on the host
+    against the oracle's sequential colouring"""
+    snap = str(tmp_path / "f.snap")
+    fuzz_snapshot(snap, seed, non_pow2=seed % 4 == 3)
+    _cmp_gridinfo(oracle, emu, snap, tmp_path, ("dom", "one")[seed % 2])
+
+
+@pytest.mark.parametrize("cfg,side,seed,how", [(1, 64, 1001, "one"), (0, 48, 1000, "dom"), (3, 48, 5, "hdr")])
+def test_emu_gridinfo_configs(oracle, emu, snapgen, tmp_path, cfg, side, seed, how):
+    snap = str(tmp_path / "g.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    _cmp_gridinfo(oracle, emu, snap, tmp_path, how)

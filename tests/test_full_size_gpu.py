@@ -187,6 +187,32 @@ def test_amr_full_size_properties(ctx, big_snaps, cfg, seed, levels):
         lv[i]["n_part"] = int(info.n_part[i])
     fails = amr_props.check_hierarchy(lv, s.pos, np.random.default_rng(seed), sample=1500)
     assert not fails, fails
+    del lv
+    _gridinfo_properties(ctx, s, info, f"cfg{cfg}")
+
+
+def _gridinfo_properties(ctx, s, info, tag):
+    """ahf_gridinfo on the full hierarchy, from the first refinement on and
+    from where the snapshot's virial overdensity starts it: the eight nodes
+    of an octet are face neighbours of each other and share a colour; every
+    grid's colours are exactly its refinements' names, which ascend in list
+    order and carry the indices 0, 1, ... of the grid"""
+    rho = dumps.ahf_rho_vir(s.omega0, s.lambda0, s.a)
+    ovd0 = (s.Nth_ref * s.pmass / (s.boxsize / float(s.L_dom)) ** 3) / rho
+    for ovlim in (0.5 * ovd0, s.ovlim):
+        gi = ctx.gridinfo(ovlim, rho, s.pmass, s.boxsize)
+        print(f"[full-gpu] gridinfo {tag}: min_ref {gi.min_ref}, {gi.no_grids} grids, {gi.tot_iso} refinements, "
+              f"{gi.device_ms:.2f} ms", flush=True)
+        assert gi.min_ref + gi.no_grids == info.n_levels
+        assert gi.tot_iso == int(gi.num_iso.sum()) == len(gi.refs)
+        assert np.all(np.diff(gi.refs[:, 0]) > 0) and gi.refs[0, 0] >= 1
+        assert gi.n_colours >= gi.tot_iso
+        for g in range(gi.no_grids):
+            col = gi.colours[g].reshape(-1, 8)
+            assert np.all(col == col[:, :1]), f"{tag}: grid {g}: an octet's nodes differ in colour"
+            mine = gi.refs[gi.refs[:, 1] == g]
+            assert np.array_equal(mine[:, 2], np.arange(gi.num_iso[g]))
+            assert np.array_equal(np.unique(col[:, 0]), mine[:, 0]), f"{tag}: grid {g}: colours != refinements"
 
 
 @pytest.mark.parametrize("cfg,seed", [(3, 1003), (4, 1004)])

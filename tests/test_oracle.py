@@ -440,3 +440,75 @@ def test_gather_rejects_unsorted(oracle, tmp_path):
     snap = str(tmp_path / "u.snap")
     dumps.write_snapshot(snap, rng.random((500, 3)), centres=[[0.5, 0.5, 0.5]], halo_lists=[np.arange(30)])
     assert oracle.oracle_gather_dump(snap.encode(), str(tmp_path / "u.halo").encode(), 1, 0, 0, None) == -3
+
+
+# ahf_gridinfo (src/libahf/ahf_gridinfo.c:59-860): the oracle's restatement
+# against the reference's own function, on generated boxes and fuzz
+# snapshots, with the AHF grids starting on the domain grid, on the first
+# refinement and where the snapshot's virial overdensity puts them
+GI_CASES = [(1, 64, 1001, "hdr"), (0, 64, 1000, "dom"), (3, 64, 1003, "one"), (4, 96, 1004, "hdr"),
+            (1, 96, 5, "one"), (2, 1500, 1002, "dom")]
+
+
+def _gridinfo_pair(oracle, ref_bin, snap, tmp_path, how):
+    from helpers import gridinfo_setup
+    ovlim, rho, pmass, box = gridinfo_setup(snap, how)
+    o, r = str(tmp_path / "o.ginf"), str(tmp_path / "r.ginf")
+    rc = oracle.oracle_gridinfo_dump(snap.encode(), o.encode(), ovlim, rho, pmass)
+    p = subprocess.run([ref_bin, "gridinfo", snap, r], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr[-2000:]
+    return rc, dumps.read_gridinfo(o), dumps.read_gridinfo(r)
+
+
+def _same_gridinfo(a, b):
+    assert (a.min_ref, a.no_grids, a.tot_iso) == (b.min_ref, b.no_grids, b.tot_iso)
+    assert a.max_ovdens == b.max_ovdens
+    assert np.array_equal(a.num_iso, b.num_iso)
+    assert len(a.colours) == len(b.colours)
+    for g, (x, y) in enumerate(zip(a.colours, b.colours)):
+        assert np.array_equal(x, y), f"grid {g}: node colours differ"
+    assert np.array_equal(a.refs, b.refs)
+
+
+@pytest.mark.parametrize("cfg,side,seed,how", GI_CASES)
+def test_gridinfo_matches_reference_configs(oracle, ref_bin, snapgen, tmp_path, cfg, side, seed, how):
+    snap = str(tmp_path / "g.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    rc, o, r = _gridinfo_pair(oracle, ref_bin, snap, tmp_path, how)
+    assert rc >= 0
+    _same_gridinfo(o, r)
+    assert o.no_grids >= 1 and o.tot_iso >= 1
+    if how == "dom":
+        # the whole periodic domain is one refinement, periodic in x, y, z
+        assert o.min_ref == 0 and o.num_iso[0] == 1
+        assert list(o.refs[0]) == [1, 0, 0, 1, 1, 1]
+        assert np.all(o.colours[0] == 1)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_gridinfo_matches_reference_fuzz(oracle, ref_bin, tmp_path, seed):
+    snap = str(tmp_path / "f.snap")
+    fuzz_snapshot(snap, 7000 + seed, non_pow2=seed % 3 == 2)
+    rc, o, r = _gridinfo_pair(oracle, ref_bin, snap, tmp_path, ("dom", "one")[seed % 2])
+    if rc == -100:
+        pytest.skip("the reference meets undefined behaviour on this input")
+    _same_gridinfo(o, r)
+
+
+def test_gridinfo_known_answer(oracle, tmp_path):
+    # two dense clumps in a sparse box: the first refinement holds exactly
+    # two isolated refinements, in traversal (z, y, x) order of their first
+    # nodes; the clump on the x = 0 face is periodic in x only
+    rng = np.random.default_rng(4)
+    a = (np.array([0.0, 0.3, 0.3]) + 0.01 * rng.standard_normal((4000, 3))) % 1.0
+    b = np.array([0.6, 0.7, 0.7]) + 0.01 * rng.standard_normal((4000, 3))
+    snap = str(tmp_path / "k.snap")
+    dumps.write_snapshot(snap, np.concatenate([a, b]), L_dom=16, L_max=32, Nth_dom=1.0, Nth_ref=5.0)
+    from helpers import gridinfo_setup
+    ovlim, rho, pmass, _ = gridinfo_setup(snap, "one")
+    out = str(tmp_path / "k.ginf")
+    assert oracle.oracle_gridinfo_dump(snap.encode(), out.encode(), ovlim, rho, pmass) == 1
+    g = dumps.read_gridinfo(out)
+    assert g.min_ref == 1 and g.no_grids == 1 and g.tot_iso == 2
+    assert g.refs.tolist() == [[1, 0, 0, 1, 0, 0], [g.refs[1][0], 0, 1, 0, 0, 0]]
+    assert sorted(set(g.colours[0].tolist())) == [1, int(g.refs[1][0])]
