@@ -27,20 +27,30 @@
  * octet of the plane), of the octet rows before in the plane (4 each), of
  * the node row cy = 0 of this octet row when cy = 1 (2 per octet of the
  * row), of the octets before in the row (2 each), and cx */
-AHF_HD int64_t ahf_gi_rank(const level_view *v, int64_t n)
+AHF_HD int64_t ahf_gi_row_of(const level_view *v, int64_t o)
 {
-  const int64_t o = n >> 3;
-  const int c = (int)(n & 7), cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
   int64_t lo = 0, hi = v->n_row - 1;
   while (lo < hi) {
     const int64_t mid = (lo + hi + 1) >> 1;
     if (v->row_oct0[mid] <= o) lo = mid;
     else hi = mid - 1;
   }
-  const int64_t r = lo, p = v->row_plane[r];
+  return lo;
+}
+
+/* rank of node c of octet o in octet row r */
+AHF_HD int64_t ahf_gi_rank_in(const level_view *v, int64_t o, int64_t r, int c)
+{
+  const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+  const int64_t p = v->row_plane[r];
   const int64_t pb = v->row_oct0[v->plane_row0[p]], pn = v->row_oct0[v->plane_row0[p + 1]] - pb;
   const int64_t rb = v->row_oct0[r], rn = v->row_oct0[r + 1] - rb;
   return 8 * pb + cz * 4 * pn + 4 * (rb - pb) + cy * 2 * rn + 2 * (o - rb) + cx;
+}
+
+AHF_HD int64_t ahf_gi_rank(const level_view *v, int64_t n)
+{
+  return ahf_gi_rank_in(v, n >> 3, ahf_gi_row_of(v, n >> 3), (int)(n & 7));
 }
 
 /* the found face neighbours of node n (node index, or -1 where

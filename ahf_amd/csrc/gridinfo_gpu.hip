@@ -63,12 +63,13 @@ struct GiState {
   }
 };
 
-/* traversal rank of every node */
+/* traversal ranks of the eight nodes of every octet */
 __global__ void k_gi_rank(level_view v, int32_t *__restrict__ rank)
 {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= 8 * v.n_oct) return;
-  rank[n] = (int32_t)ahf_gi_rank(&v, n);
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= v.n_oct) return;
+  const int64_t r = ahf_gi_row_of(&v, o);
+  for (int c = 0; c < 8; c++) rank[8 * o + c] = (int32_t)ahf_gi_rank_in(&v, o, r, c);
 }
 
 /* every rank its own parent */
@@ -79,9 +80,11 @@ __global__ void k_gi_iota(int32_t *__restrict__ parent, int64_t nn)
 }
 
 /* start flags (no found face neighbour before the node) and periodic
- * flags, in rank order */
+ * flags, in rank order; each rank's parent is its lowest found face
+ * neighbour before it (itself if none), which leaves the unions only the
+ * other edges */
 __global__ void k_gi_init(level_view v, const uint32_t *__restrict__ pmask, const int32_t *__restrict__ rank,
-                          uint8_t *__restrict__ start, uint8_t *__restrict__ pflag)
+                          int32_t *__restrict__ parent, uint8_t *__restrict__ start, uint8_t *__restrict__ pflag)
 {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= 8 * v.n_oct) return;
@@ -89,10 +92,14 @@ __global__ void k_gi_init(level_view v, const uint32_t *__restrict__ pmask, cons
   const int32_t rn = rank[n];
   int64_t u[6];
   ahf_gi_faces(&v, m, n, u);
-  bool before = false;
+  int32_t lo = rn;
   for (int d = 0; d < 6; d++)
-    if (u[d] >= 0 && rank[u[d]] < rn) before = true;
-  start[rn] = before ? 0 : 1;
+    if (u[d] >= 0) {
+      const int32_t ru = rank[u[d]];
+      lo = ru < lo ? ru : lo;
+    }
+  parent[rn] = lo;
+  start[rn] = lo == rn ? 1 : 0;
   pflag[rn] = (uint8_t)ahf_gi_periodic(&v, m, n);
 }
 
@@ -136,16 +143,28 @@ __global__ void k_gi_union(level_view v, const uint32_t *__restrict__ pmask, con
   const int32_t rn = rank[n];
   int64_t u[6];
   ahf_gi_faces(&v, pmask[n], n, u);
+  int32_t ru[6], lo = rn;
+  for (int d = 0; d < 6; d++) {
+    ru[d] = u[d] >= 0 ? rank[u[d]] : INT32_MAX;
+    lo = ru[d] < lo ? ru[d] : lo;
+  }
+  /* the edge to the lowest is the initial parent */
   for (int d = 0; d < 6; d++)
-    if (u[d] >= 0) {
-      const int32_t ru = rank[u[d]];
-      if (ru < rn) gi_union(parent, ru, rn);
-    }
+    if (ru[d] < rn && ru[d] != lo) gi_union(parent, ru[d], rn);
 }
 
-/* final roots (the unions are complete: the parents are read only, so no
- * compression races with the roots written), root flags and the
- * component's periodic flags */
+/* path compression once the unions are complete: every parent becomes the
+ * root or, where another thread's halving lands after it, an ancestor */
+__global__ void k_gi_compress(int64_t nn, int32_t *parent)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nn) return;
+  const int32_t q = gi_find(parent, (int32_t)r);
+  if (q != (int32_t)r) parent[r] = q;
+}
+
+/* final roots (the parents are read only here, so every walk ends at the
+ * root), root flags and the component's periodic flags */
 __global__ void k_gi_roots(int64_t nn, const int32_t *__restrict__ parent, const uint8_t *__restrict__ pflag_in,
                            int32_t *__restrict__ rootof, uint8_t *__restrict__ root, uint32_t *__restrict__ pflag_root)
 {
@@ -268,10 +287,11 @@ extern "C" int ahfgpu_gridinfo(ahfgpu_ctx *c, const ahfgpu_gridinfo_params *p, a
       return c->fail(AHFGPU_ENOMEM, "device arena exhausted (gridinfo)");
     const level_view v = gi_view(l);
     const unsigned nb = gi_nb(nn);
-    hipLaunchKernelGGL(k_gi_rank, dim3(nb), dim3(AHF_BLOCK), 0, st, v, G.rank);
+    hipLaunchKernelGGL(k_gi_rank, dim3(gi_nb(l.n_oct)), dim3(AHF_BLOCK), 0, st, v, G.rank);
     hipLaunchKernelGGL(k_gi_iota, dim3(nb), dim3(AHF_BLOCK), 0, st, G.parent, nn);
-    hipLaunchKernelGGL(k_gi_init, dim3(nb), dim3(AHF_BLOCK), 0, st, v, l.pmask, G.rank, G.start, G.pflag);
+    hipLaunchKernelGGL(k_gi_init, dim3(nb), dim3(AHF_BLOCK), 0, st, v, l.pmask, G.rank, G.parent, G.start, G.pflag);
     hipLaunchKernelGGL(k_gi_union, dim3(nb), dim3(AHF_BLOCK), 0, st, v, l.pmask, G.rank, G.parent);
+    hipLaunchKernelGGL(k_gi_compress, dim3(nb), dim3(AHF_BLOCK), 0, st, nn, G.parent);
     hipLaunchKernelGGL(k_gi_zero_u32, dim3(nb), dim3(AHF_BLOCK), 0, st, pf_root, nn);
     hipLaunchKernelGGL(k_gi_roots, dim3(nb), dim3(AHF_BLOCK), 0, st, nn, G.parent, G.pflag, G.rootof, G.root, pf_root);
     exclusive_scan<uint8_t, int32_t>(G.start, G.sname, nn, tmp, st);

@@ -78,6 +78,44 @@ def cpu_baseline(snap, threads):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+def gridinfo_figures(ctx, s, reps=5):
+    """ahf_gridinfo (src/libahf/ahf_gridinfo.c:59-860) on the resident
+    hierarchy, with the snapshot's virial overdensity: device milliseconds
+    (median of reps) and nodes coloured per second"""
+    import numpy as np
+
+    from ahf_amd import dumps
+    rho = dumps.ahf_rho_vir(s.omega0, s.lambda0, s.a)
+    ms = []
+    for _ in range(reps):
+        gi = ctx.gridinfo(s.ovlim, rho, s.pmass, s.boxsize)
+        ms.append(gi.device_ms)
+    nodes = int(sum(len(c) for c in gi.colours))
+    out = {"device_ms": float(np.median(ms)), "nodes": nodes, "no_grids": gi.no_grids, "min_ref": gi.min_ref,
+           "refinements": gi.tot_iso, "nodes_per_s": nodes / (float(np.median(ms)) * 1e-3)}
+    return out
+
+
+def gridinfo_cpu(out, cpu_snap):
+    """the reference's own ahf_gridinfo on the CPU sample (after its AMR
+    build, which is not timed), added to the device figures"""
+    from ahf_amd import dumps
+    out = dict(out)
+    if cpu_snap and os.path.exists(REF_BIN):
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "g.ginf")
+            r = subprocess.run([REF_BIN, "gridinfo", cpu_snap, path], capture_output=True, text=True, timeout=600)
+            if r.returncode == 0:
+                t = json.loads(r.stdout.strip().splitlines()[-1])["gridinfo_seconds"]
+                g = dumps.read_gridinfo(path)
+                n = int(sum(len(c) for c in g.colours))
+                out.update(cpu_seconds=t, cpu_nodes=n, cpu_nodes_per_s=n / max(t, 1e-9),
+                           cpu_sample=f"{os.path.basename(cpu_snap)}, {n} nodes of {g.no_grids} grids, 1 thread")
+            else:
+                log(f"reference gridinfo failed: {r.stderr[-300:]}")
+    return out
+
+
 def copy_probe(nbytes=1 << 30, reps=5):
     """Device-to-device copies of nbytes through the HIP runtime: the copy
     ceiling of this GPU (read + write bytes / time) and, under
@@ -351,6 +389,9 @@ def main():
         log(f"plain form: {1e3 * (tp1 - tp0) / args.plain_steps:.3f} ms/step, unbinding "
             f"{np.mean(plain_unb_ms):.3f} ms (batched {np.mean(unb_ms):.3f} ms)")
 
+    # ahf_gridinfo on the hierarchy just built
+    gi_fig = gridinfo_figures(ctx, s)
+
     # the drop-in boundary hands over host arrays: time their upload (AoS
     # pageable host memory -> SoA device arrays) separately; not part of value
     tu0 = time.perf_counter()
@@ -441,6 +482,7 @@ def main():
         "pcie_inclusive_value": s.npart / ((ms_per_step + upload_ms) * 1e-3),
         "cpu_baseline": None,
     }
+    cpu_snap = snap
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         threads = min(threads, 16)
@@ -466,6 +508,10 @@ def main():
                            f"({rb.get('unbind_seconds_1thread', 0.0):.3f}s on 1 thread); host {cpu_model()}"),
             }
             out["vs_cpu"] = value / out["cpu_baseline"]["value"]
+    out["gridinfo"] = g = gridinfo_cpu(gi_fig, None if args.no_cpu else cpu_snap)
+    log(f"gridinfo: {g['device_ms']:.3f} ms on the GPU for {g['nodes']} nodes of {g['no_grids']} grids, "
+        f"{g['refinements']} refinements"
+        + (f"; reference CPU {g['cpu_nodes_per_s']:.3g} nodes/s ({g['cpu_sample']})" if g.get("cpu_nodes_per_s") else ""))
     ctx.close()
     if in_memory:
         args.no_construct_workload = True
