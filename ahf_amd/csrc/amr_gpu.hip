@@ -234,6 +234,13 @@ __global__ void k_morton_keys(level_view v, uint64_t *__restrict__ mk, uint32_t 
 
 #define AHF_XCDS 8
 #define AHF_GL_NB 64 /* source nodes around one octet: 4x4x4 */
+/* k_gather_light keeps the records of an octet's 4x4x4 neighbourhood as
+ * three word arrays (bucket start, entry count, presence mask) at index
+ * i + 4j + 18k, octets 72 words apart: the 8 children of 8 octets reading
+ * the same relative source then hit 64 distinct banks ({0,1,4,5,18,19,22,23}
+ * + 8 * octet modulo 64; the first 32 lanes also cover 32 banks) */
+#define AHF_GL_PS 18
+#define AHF_GL_OS 72
 
 /* traversal position t of a target's sources -> offset along one axis, for
  * the axis rotation r (0 inside, 1 at X = 0, 2 at X = L-1; ahf_axis_order) */
@@ -275,12 +282,13 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const 
                                                             uint32_t *__restrict__ heavy,
                                                             uint32_t *__restrict__ n_heavy)
 {
-  __shared__ uint4 s_rec[AHF_BLOCK / 8 * AHF_GL_NB];
+  __shared__ uint32_t s_st[AHF_BLOCK / 8 * AHF_GL_OS], s_ct[AHF_BLOCK / 8 * AHF_GL_OS],
+      s_pk[AHF_BLOCK / 8 * AHF_GL_OS];
   const int64_t b = blockIdx.x;
   const int64_t t = ((b % AHF_XCDS) * per_xcd + b / AHF_XCDS) * blockDim.x + threadIdx.x;
   const bool active = t < 8 * v.n_oct;
   const int c = threadIdx.x & 7;
-  const int nb0 = (threadIdx.x >> 3) * AHF_GL_NB;
+  const int nb0 = (threadIdx.x >> 3) * AHF_GL_OS;
   const int64_t o = active ? (int64_t)tord[t >> 3] : 0;
   if (active) {
     const int32_t *nrow = v.nbr + 27 * o;
@@ -288,23 +296,26 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const 
       const int u = c * 8 + m;
       const int i = u & 3, j = (u >> 2) & 3, k = u >> 4;
       const int on = nrow[ahf_e27(i == 0 ? -1 : (i == 3), j == 0 ? -1 : (j == 3), k == 0 ? -1 : (k == 3))];
-      uint4 r = make_uint4(0u, 0u, 0u, 0u);
+      uint32_t st = 0u, ct = 0u, pk = 0u;
       if (on >= 0) {
         const int64_t sn = 8 * (int64_t)on + (((k + 1) & 1) << 2 | ((j + 1) & 1) << 1 | ((i + 1) & 1));
         const int64_t a = p0[sn];
-        r.x = (uint32_t)a;
-        r.y = (uint32_t)(p0[sn + 1] - a);
-        r.z = pmask[sn];
+        st = (uint32_t)a;
+        ct = (uint32_t)(p0[sn + 1] - a);
+        pk = pmask[sn];
       }
-      s_rec[nb0 + u] = r;
+      const int x = nb0 + i + 4 * j + AHF_GL_PS * k;
+      s_st[x] = st;
+      s_ct[x] = ct;
+      s_pk[x] = pk;
     }
   }
   __syncthreads();
   if (!active) return;
   const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
   /* neighbourhood record of the source at offset (ex,ey,ez): index
-   * own + (ez+1)*16 + (ey+1)*4 + (ex+1) with own the target's corner */
-  const uint4 *rec0 = s_rec + nb0 + cz * 16 + cy * 4 + cx;
+   * own + (ez+1)*18 + (ey+1)*4 + (ex+1) with own the target's corner */
+  const int rec0 = nb0 + cz * AHF_GL_PS + cy * 4 + cx;
   const int64_t n = 8 * o + c;
   int64_t X, Y, Z;
   ahf_node_xyz(&v, o, c, &X, &Y, &Z);
@@ -329,8 +340,8 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const 
 #pragma unroll
       for (int tx = 0; tx < 3; tx++) {
         const int ez = rot_offset(tz, rz), ey = rot_offset(ty, ry), ex = rot_offset(tx, rx);
-        const uint32_t *rw = reinterpret_cast<const uint32_t *>(rec0 + (ez + 1) * 16 + (ey + 1) * 4 + (ex + 1));
-        const uint32_t cnt = rw[1], pm = rw[2];
+        const int x = rec0 + (ez + 1) * AHF_GL_PS + (ey + 1) * 4 + (ex + 1);
+        const uint32_t cnt = s_ct[x], pm = s_pk[x];
         if (((pm >> ((1 - ez) * 9 + (1 - ey) * 3 + (1 - ex))) & 1u) && cnt) {
           mask |= 1ull << (tz * 16 + ty * 4 + tx);
           total += cnt;
@@ -350,9 +361,10 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_light(level_view v, const 
       mask &= mask - 1ull;
       const uint32_t ez1 = (pz >> ((pos >> 4) << 1)) & 3u, ey1 = (py >> (((pos >> 2) & 3) << 1)) & 3u,
                      ex1 = (px >> ((pos & 3) << 1)) & 3u;
-      const uint4 r = rec0[ez1 * 16 + ey1 * 4 + ex1];
-      rem = r.y;
-      q = step == 1u ? r.x : r.x + r.y - 1u;
+      const int x = rec0 + (int)(ez1 * AHF_GL_PS + ey1 * 4 + ex1);
+      const uint32_t st = s_st[x];
+      rem = s_ct[x];
+      q = step == 1u ? st : st + rem - 1u;
       /* byte offsets of a[1-ez], wy[1-ey], wx[1-ex] inside a record */
       oz = (2u - ez1) * 8u;
       oy = 24u + (2u - ey1) * 8u;

@@ -8,18 +8,22 @@
  * finds them) that are already coloured, merging the others into it by
  * recolouring, or starts a new colour when none is.  The neighbours
  * already coloured are exactly those before it in traversal order, so the
- * result is the connected components of the graph whose edges join each
- * node to its found face neighbours before it; a component's colour is the
- * one its first node started (merges keep the lowest colour, and colours
- * are numbered in the order they start).  Here, per grid:
- *   - each node's rank in traversal order, from the octet rows and planes;
- *   - a start flag (no found face neighbour before it) and, by a scan of
- *     the flags in rank order, the colour each start begins;
- *   - union-find over the edges, every root the component's lowest rank
- *     (a larger root always hooks under a smaller one, by compare-and-swap);
- *   - each node's colour is its root's start colour; the roots, in rank
- *     order, are the grid's refinements in AHF's list order, with the
- *     periodic flags of their nodes on the low faces (testBound).
+ * result is the connected components of the face-neighbour graph, each
+ * carrying the colour its first node started (merges keep the lowest
+ * colour, and colours are numbered in the order they start).  The
+ * components are unions of whole octets and colours start only at the
+ * first nodes of octets, in octet order (gridinfo_core.h), so per grid:
+ *   - per octet, its edges to the octets before it in x, y and z, its start
+ *     flag and its periodic flags;
+ *   - the parents seeded with the first octet of each x-run segment (a
+ *     scan of the segment starts), then union-find over the y edges and,
+ *     after a compression, over the z edges and the x wraps, every root the
+ *     component's lowest octet (a larger root always hooks under a smaller
+ *     one, by compare-and-swap);
+ *   - a scan of the start flags names the colours; every node of an octet
+ *     takes its root's colour, and the roots, in octet order, are the
+ *     grid's refinements in AHF's list order, with the periodic flags of
+ *     their octets (testBound).
  * AHF's colour holder has room for three colours; a node that would see
  * four or more distinct ones overflows it, which AHF leaves undefined and
  * this pass does not reproduce.
@@ -32,24 +36,16 @@
 
 #include "../../include/ahf_gpu.h"
 #include "amr_core.h"
-#include "gridinfo_core.h"
 #include "devprims.h"
 #include "gpu_ctx.h"
+#include "gridinfo_core.h"
 
 using namespace ahf;
 
 struct GiLevel {
-  int32_t *rank;   /* per node: traversal rank */
-  int32_t *parent; /* per rank: union-find parent, always a lower rank */
-  int32_t *rootof; /* per rank: the component's root (its lowest rank) */
-  uint8_t *start;  /* per rank: a colour starts here */
-  int32_t *sname;  /* per rank (n+1): starts before it */
-  uint8_t *pflag;  /* per rank: periodic flags of the component (at its root) */
-  uint8_t *root;   /* per rank: a root */
-  int32_t *rpos;   /* per rank (n+1): roots before it */
-  int32_t *colour; /* per node: the colour */
-  int32_t *refs;   /* per root: name, grid, index, periodic x, y, z */
-  int64_t n_nodes = 0, n_roots = 0, n_starts = 0;
+  int64_t n_oct = 0, n_roots = 0, n_starts = 0;
+  int32_t *colour = nullptr; /* per node (8 per octet): the colour */
+  int32_t *refs = nullptr;   /* per root: name, grid, index, periodic x, y, z */
 };
 
 struct GiState {
@@ -63,47 +59,37 @@ struct GiState {
   }
 };
 
-/* traversal ranks of the eight nodes of every octet */
-__global__ void k_gi_rank(level_view v, int32_t *__restrict__ rank)
+/* per octet: x-run segment start, colour start and periodic flags */
+__global__ void k_gi_flags(level_view v, uint8_t *__restrict__ seg, uint8_t *__restrict__ start,
+                           uint8_t *__restrict__ pflag)
 {
   const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= v.n_oct) return;
-  const int64_t r = ahf_gi_row_of(&v, o);
-  for (int c = 0; c < 8; c++) rank[8 * o + c] = (int32_t)ahf_gi_rank_in(&v, o, r, c);
+  int64_t e[3];
+  ahf_gi_oct_edges(&v, o, e);
+  seg[o] = (uint8_t)ahf_gi_oct_seg_start(&v, o, e);
+  start[o] = (uint8_t)ahf_gi_oct_start(&v, o, e);
+  pflag[o] = (uint8_t)ahf_gi_oct_periodic(&v, o, e);
 }
 
-/* every rank its own parent */
-__global__ void k_gi_iota(int32_t *__restrict__ parent, int64_t nn)
+/* the first octet of every x-run segment, by segment (sid: exclusive scan
+ * of the segment starts) */
+__global__ void k_gi_seg_first(int64_t n, const uint8_t *__restrict__ seg, const int32_t *__restrict__ sid,
+                               int32_t *__restrict__ first)
 {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < nn) parent[r] = (int32_t)r;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < n && seg[o]) first[sid[o]] = (int32_t)o;
 }
 
-/* start flags (no found face neighbour before the node) and periodic
- * flags, in rank order; each rank's parent is its lowest found face
- * neighbour before it (itself if none), which leaves the unions only the
- * other edges */
-__global__ void k_gi_init(level_view v, const uint32_t *__restrict__ pmask, const int32_t *__restrict__ rank,
-                          int32_t *__restrict__ parent, uint8_t *__restrict__ start, uint8_t *__restrict__ pflag)
+/* every octet's parent: the first octet of its segment */
+__global__ void k_gi_seed(int64_t n, const uint8_t *__restrict__ seg, const int32_t *__restrict__ sid,
+                          const int32_t *__restrict__ first, int32_t *__restrict__ parent)
 {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= 8 * v.n_oct) return;
-  const uint32_t m = pmask[n];
-  const int32_t rn = rank[n];
-  int64_t u[6];
-  ahf_gi_faces(&v, m, n, u);
-  int32_t lo = rn;
-  for (int d = 0; d < 6; d++)
-    if (u[d] >= 0) {
-      const int32_t ru = rank[u[d]];
-      lo = ru < lo ? ru : lo;
-    }
-  parent[rn] = lo;
-  start[rn] = lo == rn ? 1 : 0;
-  pflag[rn] = (uint8_t)ahf_gi_periodic(&v, m, n);
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < n) parent[o] = first[sid[o] + seg[o] - 1];
 }
 
-/* the root of rank x: parents only ever point to lower ranks */
+/* the root of x: parents only ever point to lower octets */
 __device__ inline int32_t gi_find(int32_t *par, int32_t x)
 {
   volatile int32_t *p = par;
@@ -111,7 +97,7 @@ __device__ inline int32_t gi_find(int32_t *par, int32_t x)
   if (cur == x) return cur;
   int32_t prev = x, nxt;
   while (cur > (nxt = p[cur])) {
-    p[prev] = nxt; /* path halving: still a lower rank of the same component */
+    p[prev] = nxt; /* path halving: still a lower octet of the same component */
     prev = cur;
     cur = nxt;
   }
@@ -134,68 +120,67 @@ __device__ inline void gi_union(int32_t *par, int32_t a, int32_t b)
   }
 }
 
-/* the edges: every node to its found face neighbours before it */
-__global__ void k_gi_union(level_view v, const uint32_t *__restrict__ pmask, const int32_t *__restrict__ rank,
-                           int32_t *parent)
+/* phase 0: the y edges; phase 1: the z edges and the x wraps */
+__global__ void k_gi_union(level_view v, int phase, int32_t *parent)
 {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= 8 * v.n_oct) return;
-  const int32_t rn = rank[n];
-  int64_t u[6];
-  ahf_gi_faces(&v, pmask[n], n, u);
-  int32_t ru[6], lo = rn;
-  for (int d = 0; d < 6; d++) {
-    ru[d] = u[d] >= 0 ? rank[u[d]] : INT32_MAX;
-    lo = ru[d] < lo ? ru[d] : lo;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= v.n_oct) return;
+  int64_t e[3];
+  ahf_gi_oct_edges(&v, o, e);
+  if (phase == 0) {
+    if (e[1] >= 0) gi_union(parent, (int32_t)e[1], (int32_t)o);
+  } else {
+    if (e[2] >= 0) gi_union(parent, (int32_t)e[2], (int32_t)o);
+    if (e[0] >= 0 && ahf_kx(v.oct_key[o]) == 0) gi_union(parent, (int32_t)e[0], (int32_t)o);
   }
-  /* the edge to the lowest is the initial parent */
-  for (int d = 0; d < 6; d++)
-    if (ru[d] < rn && ru[d] != lo) gi_union(parent, ru[d], rn);
 }
 
-/* path compression once the unions are complete: every parent becomes the
- * root or, where another thread's halving lands after it, an ancestor */
-__global__ void k_gi_compress(int64_t nn, int32_t *parent)
+/* path compression between the phases: every parent becomes the root or,
+ * where another thread's halving lands after it, an ancestor */
+__global__ void k_gi_compress(int64_t n, int32_t *parent)
 {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nn) return;
-  const int32_t q = gi_find(parent, (int32_t)r);
-  if (q != (int32_t)r) parent[r] = q;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  const int32_t q = gi_find(parent, (int32_t)o);
+  if (q != (int32_t)o) parent[o] = q;
 }
 
 /* final roots (the parents are read only here, so every walk ends at the
  * root), root flags and the component's periodic flags */
-__global__ void k_gi_roots(int64_t nn, const int32_t *__restrict__ parent, const uint8_t *__restrict__ pflag_in,
+__global__ void k_gi_roots(int64_t n, const int32_t *__restrict__ parent, const uint8_t *__restrict__ pflag,
                            int32_t *__restrict__ rootof, uint8_t *__restrict__ root, uint32_t *__restrict__ pflag_root)
 {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nn) return;
-  int32_t q = (int32_t)r, nxt;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  int32_t q = (int32_t)o, nxt;
   while ((nxt = parent[q]) != q) q = nxt;
-  rootof[r] = q;
-  root[r] = q == (int32_t)r ? 1 : 0;
-  if (pflag_in[r]) atomicOr(&pflag_root[q], (uint32_t)pflag_in[r]);
+  rootof[o] = q;
+  root[o] = q == (int32_t)o ? 1 : 0;
+  if (pflag[o]) atomicOr(&pflag_root[q], (uint32_t)pflag[o]);
 }
 
-__global__ void k_gi_colour(int64_t nn, const int32_t *__restrict__ rank, const int32_t *__restrict__ rootof,
-                            const int32_t *__restrict__ sname, int32_t base, int32_t *__restrict__ colour)
+/* the colour of the eight nodes of every octet */
+__global__ void k_gi_colour(int64_t n, const int32_t *__restrict__ rootof, const int32_t *__restrict__ sname,
+                            int32_t base, int4 *__restrict__ colour)
 {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
-  colour[n] = base + sname[rootof[rank[n]]] + 1;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  const int32_t c = base + sname[rootof[o]] + 1;
+  colour[2 * o] = make_int4(c, c, c, c);
+  colour[2 * o + 1] = make_int4(c, c, c, c);
 }
 
-__global__ void k_gi_refs(int64_t nn, const uint8_t *__restrict__ root, const int32_t *__restrict__ rpos,
+__global__ void k_gi_refs(int64_t n, const uint8_t *__restrict__ root, const int32_t *__restrict__ rpos,
                           const int32_t *__restrict__ sname, const uint32_t *__restrict__ pflag_root, int32_t base,
                           int32_t grid, int32_t *__restrict__ refs)
 {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nn || !root[r]) return;
-  int32_t *e = refs + 6 * (int64_t)rpos[r];
-  const uint32_t f = pflag_root[r];
-  e[0] = base + sname[r] + 1;
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n || !root[o]) return;
+  int32_t *e = refs + 6 * (int64_t)rpos[o];
+  const uint32_t f = pflag_root[o];
+  e[0] = base + sname[o] + 1;
   e[1] = grid;
-  e[2] = rpos[r];
+  e[2] = rpos[o];
   e[3] = (int32_t)(f & 1u);
   e[4] = (int32_t)((f >> 1) & 1u);
   e[5] = (int32_t)((f >> 2) & 1u);
@@ -266,49 +251,47 @@ extern "C" int ahfgpu_gridinfo(ahfgpu_ctx *c, const ahfgpu_gridinfo_params *p, a
   hipEventRecord(S->e0, st);
   for (int g = 0; g < no_grids && g < AHFGPU_MAX_LEVELS; g++) {
     const DevLevel &l = c->lev[(size_t)(min_ref + g)];
-    const int64_t nn = 8 * l.n_oct;
-    if (nn >= ((int64_t)1 << 31)) return c->fail(AHFGPU_EUNSUP, "gridinfo: a grid of 2^31 nodes or more");
-    if (!l.pmask) return c->fail(AHFGPU_EINVAL, "gridinfo: level %d has no presence masks", min_ref + g);
+    const int64_t n = l.n_oct;
+    if (8 * n >= ((int64_t)1 << 31)) return c->fail(AHFGPU_EUNSUP, "gridinfo: a grid of 2^31 nodes or more");
     GiLevel G;
-    G.n_nodes = nn;
-    G.rank = A.get<int32_t>(nn);
-    G.parent = A.get<int32_t>(nn);
-    G.rootof = A.get<int32_t>(nn);
-    G.start = A.get<uint8_t>(nn);
-    G.sname = A.get<int32_t>(nn + 1);
-    G.pflag = A.get<uint8_t>(nn);
-    G.root = A.get<uint8_t>(nn);
-    G.rpos = A.get<int32_t>(nn + 1);
-    G.colour = A.get<int32_t>(nn);
-    uint32_t *pf_root = A.get<uint32_t>(nn);
-    int32_t *tmp = A.get<int32_t>(scan_tmp_elems(nn) + 16);
-    if (!G.rank || !G.parent || !G.rootof || !G.start || !G.sname || !G.pflag || !G.root || !G.rpos || !G.colour || !pf_root ||
-        !tmp)
+    G.n_oct = n;
+    uint8_t *seg = A.get<uint8_t>(n), *start = A.get<uint8_t>(n), *pflag = A.get<uint8_t>(n),
+            *root = A.get<uint8_t>(n);
+    int32_t *sid = A.get<int32_t>(n + 1), *first = A.get<int32_t>(n), *parent = A.get<int32_t>(n),
+            *rootof = A.get<int32_t>(n), *sname = A.get<int32_t>(n + 1), *rpos = A.get<int32_t>(n + 1);
+    uint32_t *pf_root = A.get<uint32_t>(n);
+    int32_t *tmp = A.get<int32_t>(scan_tmp_elems(n) + 16);
+    G.colour = A.get<int32_t>(8 * n);
+    if (!seg || !start || !pflag || !root || !sid || !first || !parent || !rootof || !sname || !rpos || !pf_root ||
+        !tmp || !G.colour)
       return c->fail(AHFGPU_ENOMEM, "device arena exhausted (gridinfo)");
     const level_view v = gi_view(l);
-    const unsigned nb = gi_nb(nn);
-    hipLaunchKernelGGL(k_gi_rank, dim3(gi_nb(l.n_oct)), dim3(AHF_BLOCK), 0, st, v, G.rank);
-    hipLaunchKernelGGL(k_gi_iota, dim3(nb), dim3(AHF_BLOCK), 0, st, G.parent, nn);
-    hipLaunchKernelGGL(k_gi_init, dim3(nb), dim3(AHF_BLOCK), 0, st, v, l.pmask, G.rank, G.parent, G.start, G.pflag);
-    hipLaunchKernelGGL(k_gi_union, dim3(nb), dim3(AHF_BLOCK), 0, st, v, l.pmask, G.rank, G.parent);
-    hipLaunchKernelGGL(k_gi_compress, dim3(nb), dim3(AHF_BLOCK), 0, st, nn, G.parent);
-    hipLaunchKernelGGL(k_gi_zero_u32, dim3(nb), dim3(AHF_BLOCK), 0, st, pf_root, nn);
-    hipLaunchKernelGGL(k_gi_roots, dim3(nb), dim3(AHF_BLOCK), 0, st, nn, G.parent, G.pflag, G.rootof, G.root, pf_root);
-    exclusive_scan<uint8_t, int32_t>(G.start, G.sname, nn, tmp, st);
-    exclusive_scan<uint8_t, int32_t>(G.root, G.rpos, nn, tmp, st);
-    hipLaunchKernelGGL(k_gi_colour, dim3(nb), dim3(AHF_BLOCK), 0, st, nn, G.rank, G.rootof, G.sname, (int32_t)base,
-                       G.colour);
+    const unsigned nb = gi_nb(n);
+    hipLaunchKernelGGL(k_gi_flags, dim3(nb), dim3(AHF_BLOCK), 0, st, v, seg, start, pflag);
+    exclusive_scan<uint8_t, int32_t>(seg, sid, n, tmp, st);
+    hipLaunchKernelGGL(k_gi_seg_first, dim3(nb), dim3(AHF_BLOCK), 0, st, n, seg, sid, first);
+    hipLaunchKernelGGL(k_gi_seed, dim3(nb), dim3(AHF_BLOCK), 0, st, n, seg, sid, first, parent);
+    hipLaunchKernelGGL(k_gi_union, dim3(nb), dim3(AHF_BLOCK), 0, st, v, 0, parent);
+    hipLaunchKernelGGL(k_gi_compress, dim3(nb), dim3(AHF_BLOCK), 0, st, n, parent);
+    hipLaunchKernelGGL(k_gi_union, dim3(nb), dim3(AHF_BLOCK), 0, st, v, 1, parent);
+    hipLaunchKernelGGL(k_gi_compress, dim3(nb), dim3(AHF_BLOCK), 0, st, n, parent);
+    hipLaunchKernelGGL(k_gi_zero_u32, dim3(nb), dim3(AHF_BLOCK), 0, st, pf_root, n);
+    hipLaunchKernelGGL(k_gi_roots, dim3(nb), dim3(AHF_BLOCK), 0, st, n, parent, pflag, rootof, root, pf_root);
+    exclusive_scan<uint8_t, int32_t>(start, sname, n, tmp, st);
+    exclusive_scan<uint8_t, int32_t>(root, rpos, n, tmp, st);
+    hipLaunchKernelGGL(k_gi_colour, dim3(nb), dim3(AHF_BLOCK), 0, st, n, rootof, sname, (int32_t)base,
+                       reinterpret_cast<int4 *>(G.colour));
     int32_t cnt[2] = {0, 0};
-    hipMemcpyAsync(&cnt[0], G.sname + nn, sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    hipMemcpyAsync(&cnt[1], G.rpos + nn, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(&cnt[0], sname + n, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(&cnt[1], rpos + n, sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess)
       return c->fail(AHFGPU_EHIP, "gridinfo: %s", hipGetErrorString(hipGetLastError()));
     G.n_starts = cnt[0];
     G.n_roots = cnt[1];
     G.refs = A.get<int32_t>(6 * (G.n_roots > 0 ? G.n_roots : 1));
     if (!G.refs) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (gridinfo)");
-    hipLaunchKernelGGL(k_gi_refs, dim3(nb), dim3(AHF_BLOCK), 0, st, nn, G.root, G.rpos, G.sname, pf_root,
-                       (int32_t)base, (int32_t)g, G.refs);
+    hipLaunchKernelGGL(k_gi_refs, dim3(nb), dim3(AHF_BLOCK), 0, st, n, root, rpos, sname, pf_root, (int32_t)base,
+                       (int32_t)g, G.refs);
     base += G.n_starts;
     if (base >= INT32_MAX) return c->fail(AHFGPU_EUNSUP, "gridinfo: colours beyond 2^31");
     S->info.num_iso[g] = (int32_t)G.n_roots;
@@ -332,7 +315,7 @@ extern "C" int ahfgpu_gridinfo_results(ahfgpu_ctx *c, int g, int32_t *colour_out
   GiState *S = (GiState *)c->gi_result;
   if (!S || g < 0 || g >= (int)S->lev.size()) return c->fail(AHFGPU_EINVAL, "no gridinfo result for grid %d", g);
   const GiLevel &G = S->lev[(size_t)g];
-  if (colour_out) hipMemcpyAsync(colour_out, G.colour, (size_t)G.n_nodes * 4, hipMemcpyDeviceToHost, c->st);
+  if (colour_out) hipMemcpyAsync(colour_out, G.colour, (size_t)G.n_oct * 32, hipMemcpyDeviceToHost, c->st);
   if (refs_out && G.n_roots)
     hipMemcpyAsync(refs_out, G.refs, (size_t)G.n_roots * 6 * 4, hipMemcpyDeviceToHost, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess) return c->fail(AHFGPU_EHIP, "gridinfo results");

@@ -10,9 +10,9 @@
  *
  *   amr_emu <snapshot> <grids.out> [<ginfo.out> <ovlim> <rho_vir> <refine_mass>]
  *
- * With the last four arguments it also replays the ahf_gridinfo pass
- * (ahf_amd/csrc/gridinfo_gpu.hip: traversal ranks, start flags, union of
- * each node with its found face neighbours before it under the lowest rank,
+ * With the last four arguments it also replays the ahf_gridinfo pass with
+ * the per-octet rules of the kernels (ahf_amd/csrc/gridinfo_core.h: octet
+ * edges, start and periodic flags; components under their lowest octet,
  * colours from the starts, refinements at the roots) and writes the
  * AHFGINF1 dump of the oracle.
  */
@@ -92,7 +92,34 @@ static int64_t gi_find(int64_t *par, int64_t x)
   return x;
 }
 
-/* the gridinfo pass on levels [0, nlev) as the kernels run it */
+static void gi_join(int64_t *par, int64_t a, int64_t b)
+{
+  a = gi_find(par, a);
+  b = gi_find(par, b);
+  if (a < b) par[b] = a;
+  else if (b < a) par[a] = b;
+}
+
+/* traversal rank (z, y, x) of node n = 8*o + c, from the octet rows and
+ * planes: the order in which the AHFGINF1 dump lists a grid's nodes */
+static int64_t node_rank(const level_view *v, int64_t n)
+{
+  const int64_t o = n >> 3;
+  const int c = (int)(n & 7), cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+  int64_t lo = 0, hi = v->n_row - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (v->row_oct0[mid] <= o) lo = mid;
+    else hi = mid - 1;
+  }
+  const int64_t r = lo, p = v->row_plane[r];
+  const int64_t pb = v->row_oct0[v->plane_row0[p]], pn = v->row_oct0[v->plane_row0[p + 1]] - pb;
+  const int64_t rb = v->row_oct0[r], rn = v->row_oct0[r + 1] - rb;
+  return 8 * pb + cz * 4 * pn + 4 * (rb - pb) + cy * 2 * rn + 2 * (o - rb) + cx;
+}
+
+/* the gridinfo pass on levels [0, nlev) with the kernels' per-octet rules
+ * (gridinfo_core.h), components joined sequentially */
 static int gridinfo_replay(lev_t *lev, int nlev, const snap_data *s, const char *out, double ovlim, double rho_vir,
                            double refine_mass)
 {
@@ -115,63 +142,56 @@ static int gridinfo_replay(lev_t *lev, int nlev, const snap_data *s, const char 
   int32_t tot = 0;
   for (int g = 0; g < no_grids; g++) {
     lev_t *l = &lev[min_ref + g];
-    const int64_t nn = 8 * l->v.n_oct;
-    int64_t *rank = (int64_t *)xm((size_t)nn * 8), *par = (int64_t *)xm((size_t)nn * 8);
-    uint8_t *start = (uint8_t *)xm((size_t)nn), *seen = (uint8_t *)xm((size_t)nn);
-    uint32_t *pf = (uint32_t *)xm((size_t)nn * 4);
-    int64_t *sname = (int64_t *)xm((size_t)nn * 8), *rpos = (int64_t *)xm((size_t)nn * 8);
-    for (int64_t n = 0; n < nn; n++) {
-      rank[n] = ahf_gi_rank(&l->v, n);
-      if (rank[n] < 0 || rank[n] >= nn || seen[rank[n]]) {
-        fprintf(stderr, "emu: traversal ranks of grid %d are not a permutation\n", g);
-        return 4;
-      }
-      seen[rank[n]] = 1;
-      par[n] = n;
+    const int64_t no = l->v.n_oct, nn = 8 * no;
+    int64_t *par = (int64_t *)xm((size_t)no * 8), *sname = (int64_t *)xm((size_t)no * 8);
+    int64_t *rpos = (int64_t *)xm((size_t)no * 8);
+    uint8_t *start = (uint8_t *)xm((size_t)no), *seen = (uint8_t *)xm((size_t)nn);
+    uint32_t *pf = (uint32_t *)xm((size_t)no * 4);
+    for (int64_t o = 0; o < no; o++) par[o] = o;
+    for (int64_t o = 0; o < no; o++) {
+      int64_t e[3];
+      ahf_gi_oct_edges(&l->v, o, e);
+      start[o] = (uint8_t)ahf_gi_oct_start(&l->v, o, e);
+      pf[o] = ahf_gi_oct_periodic(&l->v, o, e);
+      for (int d = 0; d < 3; d++)
+        if (e[d] >= 0) gi_join(par, e[d], o);
     }
-    for (int64_t n = 0; n < nn; n++) {
-      int64_t u[6];
-      ahf_gi_faces(&l->v, l->pmask[n], n, u);
-      int before = 0;
-      for (int d = 0; d < 6; d++)
-        if (u[d] >= 0 && rank[u[d]] < rank[n]) {
-          before = 1;
-          int64_t a = gi_find(par, rank[u[d]]), b = gi_find(par, rank[n]);
-          if (a != b) {
-            if (a < b) par[b] = a;
-            else par[a] = b;
-          }
-        }
-      start[rank[n]] = (uint8_t)!before;
-    }
-    memset(pf, 0, (size_t)nn * 4);
-    for (int64_t n = 0; n < nn; n++) pf[gi_find(par, rank[n])] |= ahf_gi_periodic(&l->v, l->pmask[n], n);
+    for (int64_t o = 0; o < no; o++)
+      if (par[o] != o) pf[gi_find(par, o)] |= pf[o];
     int64_t ns = 0, nr = 0;
-    for (int64_t r = 0; r < nn; r++) {
-      sname[r] = ns;
-      ns += start[r];
-      rpos[r] = nr;
-      nr += par[r] == r;
+    for (int64_t o = 0; o < no; o++) {
+      sname[o] = ns;
+      ns += start[o];
+      rpos[o] = nr;
+      nr += par[o] == o;
     }
     colour[g] = (int32_t *)xm((size_t)nn * 4);
     nnode[g] = nn;
-    for (int64_t n = 0; n < nn; n++) colour[g][rank[n]] = (int32_t)(base + sname[gi_find(par, rank[n])] + 1);
+    for (int64_t n = 0; n < nn; n++) {
+      const int64_t r = node_rank(&l->v, n);
+      if (r < 0 || r >= nn || seen[r]) {
+        fprintf(stderr, "emu: traversal ranks of grid %d are not a permutation\n", g);
+        return 4;
+      }
+      seen[r] = 1;
+      colour[g][r] = (int32_t)(base + sname[gi_find(par, n >> 3)] + 1);
+    }
     refs = (int32_t *)realloc(refs, sizeof(int32_t) * 6 * (size_t)(nrefs + nr + 1));
-    for (int64_t r = 0; r < nn; r++)
-      if (par[r] == r) {
-        int32_t *e = refs + 6 * (nrefs + rpos[r]);
-        e[0] = (int32_t)(base + sname[r] + 1);
+    for (int64_t o = 0; o < no; o++)
+      if (par[o] == o) {
+        int32_t *e = refs + 6 * (nrefs + rpos[o]);
+        e[0] = (int32_t)(base + sname[o] + 1);
         e[1] = g;
-        e[2] = (int32_t)rpos[r];
-        e[3] = (int32_t)(pf[r] & 1u);
-        e[4] = (int32_t)((pf[r] >> 1) & 1u);
-        e[5] = (int32_t)((pf[r] >> 2) & 1u);
+        e[2] = (int32_t)rpos[o];
+        e[3] = (int32_t)(pf[o] & 1u);
+        e[4] = (int32_t)((pf[o] >> 1) & 1u);
+        e[5] = (int32_t)((pf[o] >> 2) & 1u);
       }
     nrefs += nr;
     num_iso[g] = (int32_t)nr;
     tot += (int32_t)nr;
     base += ns;
-    free(rank); free(par); free(start); free(seen); free(pf); free(sname); free(rpos);
+    free(par); free(sname); free(rpos); free(start); free(seen); free(pf);
   }
   const int32_t hdr[4] = {min_ref, no_grids, tot, (int32_t)nrefs};
   fwrite("AHFGINF1", 1, 8, f);
