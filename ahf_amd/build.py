@@ -50,12 +50,14 @@ def build_gpu(verbose=False):
         if not os.path.exists(path):
             continue
         obj = os.path.join(LIB, csrc.replace(".c", ".o"))
-        _run(["gcc", "-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+        _run(["gcc", "-std=c11", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wextra",
               "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c", path, "-o", obj])
         objs.append(obj)
     out = os.path.join(LIB, "libahf_amd.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-lm"])
+    # the drop-in's host loops use GNU OpenMP, the runtime AHF itself links
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread", "-lm",
+          "-l:libgomp.so.1"])
     os.replace(tmp, out)
     for o in objs:
         os.remove(o)

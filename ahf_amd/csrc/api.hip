@@ -14,6 +14,7 @@
 #include "../../include/ahf_gpu.h"
 #include "devprims.h"
 #include "gpu_ctx.h"
+#include "halo_set.h"
 
 __global__ void k_deinterleave(const float *__restrict__ xyz, int64_t n, float *__restrict__ x,
                                float *__restrict__ y, float *__restrict__ z)
@@ -167,6 +168,8 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->st);
   ahfgpu_release_halos(c);
+  free_halos((HaloSet *)c->halo_spare);
+  c->halo_spare = nullptr;
   ahfgpu_free_profiles(c);
   ahfgpu_free_gridinfo(c);
   c->arena.release();

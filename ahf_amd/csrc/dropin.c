@@ -13,6 +13,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "ahf_dropin.h"
 #include "ahf_gpu.h"
@@ -91,10 +94,12 @@ static void ensure_particles(int force)
   if (!global.fst_part || n == 0) fatal("no particles (global.fst_part)");
   float *pos = (float *)malloc(n * 3 * sizeof(float)), *mom = (float *)malloc(n * 3 * sizeof(float));
   if (!pos || !mom) fatal("out of host memory");
+  const part *P = global.fst_part;
+#pragma omp parallel for schedule(static)
   for (uint64_t i = 0; i < n; i++)
     for (int d = 0; d < 3; d++) {
-      pos[3 * i + d] = global.fst_part[i].pos[d];
-      mom[3 * i + d] = global.fst_part[i].mom[d];
+      pos[3 * i + d] = P[i].pos[d];
+      mom[3 * i + d] = P[i].mom[d];
     }
   int rc = ahfgpu_set_particles(ctx_get(), pos, mom, n);
   free(pos);
@@ -169,21 +174,49 @@ gridls *gen_domgrids(int *no_grids)
   return list;
 }
 
+/* AHF's head insertion in particle order (src/libamr_serial/lltools.c:32):
+ * every node's list runs from its last particle to its first, then what the
+ * node held before.  The node of every particle is found in parallel; then
+ * each thread owns a range of planes and inserts, in particle order, the
+ * particles of its planes, so every list comes out as the sequential loop
+ * leaves it. */
 void ll(long unsigned npart, partptr fst_part, gridls *cur_grid)
 {
   const long unsigned L = cur_grid->l1dim;
   pqptr pq = cur_grid->pquad;
-  for (long unsigned j = 0; j < npart; j++) {
-    partptr p = fst_part + j;
-    unsigned long n[3];
-    for (int d = 0; d < 3; d++) {
-      n[d] = (unsigned long)((double)L * p->pos[d]);
-      if (n[d] > L - 1) n[d] = 0;
+  uint32_t *zs = (uint32_t *)malloc((npart ? npart : 1) * sizeof(uint32_t));
+  uint64_t *yx = (uint64_t *)malloc((npart ? npart : 1) * sizeof(uint64_t));
+  if (!zs || !yx) fatal("out of host memory");
+#pragma omp parallel
+  {
+#pragma omp for schedule(static)
+    for (long unsigned j = 0; j < npart; j++) {
+      const partptr p = fst_part + j;
+      unsigned long n[3];
+      for (int d = 0; d < 3; d++) {
+        n[d] = (unsigned long)((double)L * p->pos[d]);
+        if (n[d] > L - 1) n[d] = 0;
+      }
+      zs[j] = (uint32_t)n[2];
+      yx[j] = (uint64_t)n[1] * L + n[0];
     }
-    nptr nd = pq->loc[n[2]].loc[n[1]].loc + n[0];
-    p->ll = nd->ll;
-    nd->ll = p;
+#ifdef _OPENMP
+    const long unsigned T = (long unsigned)omp_get_num_threads(), t = (long unsigned)omp_get_thread_num();
+#else
+    const long unsigned T = 1, t = 0;
+#endif
+    const uint32_t z0 = (uint32_t)(L * t / T), z1 = (uint32_t)(L * (t + 1) / T);
+    for (long unsigned j = 0; z0 < z1 && j < npart; j++) {
+      const uint32_t z = zs[j];
+      if (z < z0 || z >= z1) continue;
+      const partptr p = fst_part + j;
+      nptr nd = pq->loc[z].loc[yx[j] / L].loc + yx[j] % L;
+      p->ll = nd->ll;
+      nd->ll = p;
+    }
   }
+  free(zs);
+  free(yx);
 }
 
 #define FOR_NODES(g, BODY)                                                   \
@@ -271,8 +304,11 @@ static int flatten_grid(gridls *g, flat_grid *f)
   if (no != f->n_oct) return -1;
   for (int64_t i = 1; i < no; i++)
     if (f->key[i] <= f->key[i - 1]) qsort(f->key, (size_t)no, 8, cmp_u64), i = no;
-  /* pass 2: node map, links, list lengths */
-  int64_t *cnt = f->list0 + 1;
+  /* pass 2: every node and its coordinates in traversal order, then (in
+   * parallel) the node map, the links and the list lengths */
+  nptr *nodes = (nptr *)xcalloc((size_t)n, sizeof(nptr));
+  uint64_t *nxyz = (uint64_t *)xcalloc((size_t)n, 8);
+  int64_t k = 0;
   for (pqptr pq = g->pquad; pq; pq = pq->next) {
     long z = pq->z;
     for (cqptr cq0 = pq->loc; cq0 < pq->loc + pq->length; cq0++, z++)
@@ -282,23 +318,45 @@ static int flatten_grid(gridls *g, flat_grid *f)
           for (nqptr nq = nq0; nq; nq = nq->next) {
             long x = nq->x;
             for (nptr nd = nq->loc; nd < nq->loc + nq->length; nd++, x++) {
-              int64_t o = find_key(f->key, no, PK(x / 2, y / 2, z / 2));
-              if (o < 0) return -1;
-              int64_t idx = 8 * o + (z & 1) * 4 + (y & 1) * 2 + (x & 1);
-              if (f->node_of[idx]) return -1;
-              f->node_of[idx] = nd;
-              if (!(x & 1) && !(y & 1) && !(z & 1) && nd + 2 < nq->loc + nq->length) f->link[o] = 1;
-              for (partptr p = nd->ll; p; p = p->ll) cnt[idx]++;
+              nodes[k] = nd;
+              /* bit 63: the x-run goes on past the next node */
+              nxyz[k++] = PK(x, y, z) | (nd + 2 < nq->loc + nq->length ? 1ull << 63 : 0ull);
             }
           }
       }
   }
+  int64_t *cnt = f->list0 + 1;
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int64_t i = 0; i < n; i++) {
+    const uint64_t w = nxyz[i];
+    const int64_t x = (int64_t)(w & 0x1FFFFF), y = (int64_t)((w >> 21) & 0x1FFFFF), z = (int64_t)((w >> 42) & 0x1FFFFF);
+    const int64_t o = find_key(f->key, no, PK(x / 2, y / 2, z / 2));
+    if (o < 0) {
+      bad = 1;
+      continue;
+    }
+    const int64_t idx = 8 * o + (z & 1) * 4 + (y & 1) * 2 + (x & 1);
+    if (__atomic_exchange_n(&f->node_of[idx], nodes[i], __ATOMIC_RELAXED)) {
+      bad = 1;
+      continue;
+    }
+    if (!(x & 1) && !(y & 1) && !(z & 1) && (w >> 63)) f->link[o] = 1;
+    int64_t c = 0;
+    for (partptr p = nodes[i]->ll; p; p = p->ll) c++;
+    cnt[idx] = c;
+  }
+  free(nodes);
+  free(nxyz);
+  if (bad) return -1;
   for (int64_t i = 0; i < n; i++) f->list0[i + 1] += f->list0[i];
   f->n_listed = f->list0[n];
   f->list = (uint32_t *)xcalloc((size_t)f->n_listed, 4);
+  const partptr P0 = global.fst_part;
+#pragma omp parallel for schedule(dynamic, 4096)
   for (int64_t i = 0; i < n; i++) {
     int64_t w = f->list0[i];
-    for (partptr p = f->node_of[i]->ll; p; p = p->ll) f->list[w++] = (uint32_t)(p - global.fst_part);
+    for (partptr p = f->node_of[i]->ll; p; p = p->ll) f->list[w++] = (uint32_t)(p - P0);
   }
   return 0;
 }
@@ -318,6 +376,7 @@ boolean assign_npart(gridls *cur_grid)
     pthread_mutex_unlock(&g_lock);
     fatal("assign_npart");
   }
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < f.n_node; i++) f.node_of[i]->dens = dens[i];
   cur_grid->size.no_part = (long unsigned)f.n_listed;
   cur_grid->size.no_nodes = (long unsigned)f.n_node;
@@ -338,9 +397,19 @@ static void build_quads(gridls *g, const uint64_t *key, const uint8_t *link, int
 {
   octet_raster_t r;
   if (octet_raster_build(key, link, n_oct, &r)) fatal("out of host memory");
-  int64_t iq = 0, ic = 0, nd = 0;
+  /* the pquad chain, and for every node plane its cquad slot and its first
+   * cquad, nquad and node (in traversal order) */
+  typedef struct {
+    cqptr slot;
+    int32_t z;
+    int64_t ic, iq, nd;
+  } plane_t;
+  int64_t np = 0;
+  for (int64_t k = 0; k < r.n_pq; k++) np += r.pq[2 * k + 1];
+  plane_t *pl = (plane_t *)xcalloc((size_t)np + 1, sizeof(plane_t));
   pqptr prev_pq = NULL;
   g->no_pquad = 0;
+  int64_t ic = 0, iq = 0, nd = 0, j = 0;
   for (int64_t k = 0; k < r.n_pq; k++) {
     pqptr pq = (pqptr)xcalloc(1, sizeof(pquad));
     if (prev_pq) prev_pq->next = pq; else g->pquad = pq;
@@ -349,34 +418,48 @@ static void build_quads(gridls *g, const uint64_t *key, const uint8_t *link, int
     pq->z = r.pq[2 * k];
     pq->length = r.pq[2 * k + 1];
     pq->loc = (cqptr)xcalloc((size_t)pq->length, sizeof(cquad));
-    for (int zz = 0; zz < pq->length; zz++) {
+    for (int zz = 0; zz < pq->length; zz++, j++) {
       const int z = pq->z + zz;
-      cqptr prev_cq = NULL;
-      while (ic < r.n_cq && r.cq[3 * ic] == z) {
-        cqptr cq = prev_cq ? (cqptr)xcalloc(1, sizeof(cquad)) : pq->loc + zz;
-        if (prev_cq) prev_cq->next = cq;
-        prev_cq = cq;
-        cq->y = r.cq[3 * ic + 1];
-        cq->length = r.cq[3 * ic + 2];
-        cq->loc = (nqptr)xcalloc((size_t)cq->length, sizeof(nquad));
-        for (int yy = 0; yy < cq->length; yy++) {
-          const int y = cq->y + yy;
-          nqptr prev_nq = NULL;
-          while (iq < r.n_nq && r.nq[4 * iq] == z && r.nq[4 * iq + 1] == y) {
-            nqptr nq = prev_nq ? (nqptr)xcalloc(1, sizeof(nquad)) : cq->loc + yy;
-            if (prev_nq) prev_nq->next = nq;
-            prev_nq = nq;
-            nq->x = r.nq[4 * iq + 2];
-            nq->length = r.nq[4 * iq + 3];
-            nq->loc = (nptr)xcalloc((size_t)nq->length, sizeof(node));
-            for (int xx = 0; xx < nq->length; xx++) node_of[r.order[nd++]] = nq->loc + xx;
-            iq++;
-          }
-        }
-        ic++;
-      }
+      pl[j].slot = pq->loc + zz;
+      pl[j].z = z;
+      pl[j].ic = ic;
+      pl[j].iq = iq;
+      pl[j].nd = nd;
+      while (ic < r.n_cq && r.cq[3 * ic] == z) ic++;
+      while (iq < r.n_nq && r.nq[4 * iq] == z) nd += r.nq[4 * iq++ + 3];
     }
   }
+  /* the cquads, nquads and nodes of every plane */
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t jp = 0; jp < np; jp++) {
+    const int z = pl[jp].z;
+    int64_t jc = pl[jp].ic, jq = pl[jp].iq, jn = pl[jp].nd;
+    cqptr prev_cq = NULL;
+    while (jc < r.n_cq && r.cq[3 * jc] == z) {
+      cqptr cq = prev_cq ? (cqptr)xcalloc(1, sizeof(cquad)) : pl[jp].slot;
+      if (prev_cq) prev_cq->next = cq;
+      prev_cq = cq;
+      cq->y = r.cq[3 * jc + 1];
+      cq->length = r.cq[3 * jc + 2];
+      cq->loc = (nqptr)xcalloc((size_t)cq->length, sizeof(nquad));
+      for (int yy = 0; yy < cq->length; yy++) {
+        const int y = cq->y + yy;
+        nqptr prev_nq = NULL;
+        while (jq < r.n_nq && r.nq[4 * jq] == z && r.nq[4 * jq + 1] == y) {
+          nqptr nq = prev_nq ? (nqptr)xcalloc(1, sizeof(nquad)) : cq->loc + yy;
+          if (prev_nq) prev_nq->next = nq;
+          prev_nq = nq;
+          nq->x = r.nq[4 * jq + 2];
+          nq->length = r.nq[4 * jq + 3];
+          nq->loc = (nptr)xcalloc((size_t)nq->length, sizeof(node));
+          for (int xx = 0; xx < nq->length; xx++) node_of[r.order[jn++]] = nq->loc + xx;
+          jq++;
+        }
+      }
+      jc++;
+    }
+  }
+  free(pl);
   free(g->pquad_array);
   g->pquad_array = (pqptr *)xcalloc((size_t)g->no_pquad, sizeof(pqptr));
   int64_t i = 0;
@@ -408,6 +491,7 @@ static void fill_level(gridls *g, int lev, int build)
     flat_free(&f);
   }
   partptr P = global.fst_part;
+#pragma omp parallel for schedule(dynamic, 4096)
   for (int64_t i = 0; i < nn; i++) {
     nptr nd = node_of[i];
     nd->dens = dens[i];
@@ -478,21 +562,21 @@ boolean gen_AMRhierarchy(gridls **grid_list, int *no_grids)
 
 /* ----------------------------------------------------------- haloes ---- */
 
-static int run_halos(HALO *h, long unsigned nh, int mode, int clear_phi0)
+static int run_halos(HALO **h, long unsigned nh, int mode, int clear_phi0)
 {
   ahfgpu_ctx *c = ctx_get();
   ensure_particles(0);
   uint64_t tot = 0;
-  for (long unsigned i = 0; i < nh; i++) tot += h[i].npart;
+  for (long unsigned i = 0; i < nh; i++) tot += h[i]->npart;
   double *cen = (double *)xcalloc(3 * nh, 8);
   uint64_t *np = (uint64_t *)xcalloc(nh, 8), *ip = (uint64_t *)xcalloc(tot, 8);
   uint64_t w = 0;
   for (long unsigned i = 0; i < nh; i++) {
-    cen[3 * i] = h[i].pos.x;
-    cen[3 * i + 1] = h[i].pos.y;
-    cen[3 * i + 2] = h[i].pos.z;
-    np[i] = h[i].npart;
-    for (uint64_t j = 0; j < h[i].npart; j++) ip[w++] = h[i].ipart[j];
+    cen[3 * i] = h[i]->pos.x;
+    cen[3 * i + 1] = h[i]->pos.y;
+    cen[3 * i + 2] = h[i]->pos.z;
+    np[i] = h[i]->npart;
+    for (uint64_t j = 0; j < h[i]->npart; j++) ip[w++] = h[i]->ipart[j];
   }
   ahfgpu_unbind_params up = {simu.AHF_MINPART, 0, simu.AHF_VTUNE, r_fac, v_fac, phi_fac, Hubble,
                              global.ovlim, global.rho_vir, rho_fac};
@@ -503,26 +587,26 @@ static int run_halos(HALO *h, long unsigned nh, int mode, int clear_phi0)
   if (!rc) {
     w = 0;
     for (long unsigned i = 0; i < nh; i++) {
-      const uint64_t n_in = h[i].npart;
+      const uint64_t n_in = h[i]->npart;
       const double *q = props + AHFGPU_PROPS * i;
       int set = (int)q[5];
       if (!clear_phi0) set &= ~AHFGPU_SET_PHI0;
       if (mode & AHFGPU_UNBIND_SORT_ONLY) {
         /* sort_halo_particles reorders the array in place (below NminPerHalo
          * the order is kept, so this copy changes nothing) */
-        memcpy(h[i].ipart, ip + w, n_in * 8);
+        memcpy(h[i]->ipart, ip + w, n_in * 8);
       } else if (set) {
         /* rem_unbound installs a new array, rem_outsideRvir realloc()s the
          * old one: either way the caller owns a malloc'd array of npart */
-        long unsigned *lst = (long unsigned *)realloc(h[i].ipart, (np[i] ? np[i] : 1) * 8);
+        long unsigned *lst = (long unsigned *)realloc(h[i]->ipart, (np[i] ? np[i] : 1) * 8);
         if (!lst) fatal("out of host memory");
         memcpy(lst, ip + w, np[i] * 8);
-        h[i].ipart = lst;
-        h[i].npart = np[i];
-        if (set & AHFGPU_SET_MVIR) h[i].M_vir = q[0];
-        if (set & AHFGPU_SET_RVIR) h[i].R_vir = q[1];
-        if (set & AHFGPU_SET_PHI0) h[i].Phi0 = q[2];
-        if (set & AHFGPU_SET_OVDENS) h[i].ovdens = q[3];
+        h[i]->ipart = lst;
+        h[i]->npart = np[i];
+        if (set & AHFGPU_SET_MVIR) h[i]->M_vir = q[0];
+        if (set & AHFGPU_SET_RVIR) h[i]->R_vir = q[1];
+        if (set & AHFGPU_SET_PHI0) h[i]->Phi0 = q[2];
+        if (set & AHFGPU_SET_OVDENS) h[i]->ovdens = q[3];
       }
       w += n_in;
     }
@@ -532,48 +616,151 @@ static int run_halos(HALO *h, long unsigned nh, int mode, int clear_phi0)
   return rc;
 }
 
+static int run_profiles(HALO **h, long unsigned nh);
+static int gather_halos(HALO **h, long unsigned nh, int construct);
+
+/* the batched functions take the haloes as an array of pointers */
+static HALO **halo_ptrs(HALO *h, long unsigned nh)
+{
+  HALO **hp = (HALO **)xcalloc(nh, sizeof(HALO *));
+  for (long unsigned i = 0; i < nh; i++) hp[i] = h + i;
+  return hp;
+}
+
+/*
+ * The per-halo functions are called from AHF's OpenMP halo loop
+ * (src/libahf/ahf_halos.c:504-510 around ahf_halos_sfc_constructHalo), one
+ * halo per call.  Calls made at the same time are combined: each call queues
+ * its request and waits; whichever caller finds no batch running takes
+ * every queued request and runs it as one batched pass per kind of request
+ * (all sort_halo_particles together, all rem_unbound together, ...), then
+ * wakes the callers whose requests it ran.  A thread's requests for one halo
+ * follow each other (each call returns before the next is made), so a batch
+ * holds each halo at most once and the haloes in it are independent.
+ */
+enum { REQ_SORT, REQ_UNBIND, REQ_RVIR0, REQ_RVIR1, REQ_PROFILES, REQ_GATHER, REQ_KINDS };
+
+typedef struct halo_req {
+  HALO *h;
+  int kind, rc, done;
+  struct halo_req *next;
+} halo_req;
+
+static pthread_mutex_t q_lock = PTHREAD_MUTEX_INITIALIZER;
+static pthread_cond_t q_cv = PTHREAD_COND_INITIALIZER;
+static halo_req *q_head = NULL, **q_tail = &q_head;
+static int q_running = 0;
+static uint64_t q_batches = 0, q_requests = 0;
+
+static int run_kind(int kind, HALO **hp, long unsigned n)
+{
+  switch (kind) {
+  case REQ_SORT: return run_halos(hp, n, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_SORT_ONLY, 0);
+  case REQ_UNBIND: return run_halos(hp, n, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_NO_SORT, 1);
+  case REQ_RVIR0: return run_halos(hp, n, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_RVIR_ONLY, 1);
+  case REQ_RVIR1: return run_halos(hp, n, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_RVIR_ONLY, 0);
+  case REQ_PROFILES: return run_profiles(hp, n);
+  default: return gather_halos(hp, n, 0);
+  }
+}
+
+/* every request of the batch, one batched pass per kind */
+static void run_batch(halo_req *batch)
+{
+  long unsigned n[REQ_KINDS] = {0}, tot = 0;
+  for (halo_req *r = batch; r; r = r->next) n[r->kind]++, tot++;
+  HALO **hp = (HALO **)xcalloc(tot, sizeof(HALO *));
+  pthread_mutex_lock(&g_lock);
+  for (int k = 0; k < REQ_KINDS; k++) {
+    if (!n[k]) continue;
+    long unsigned m = 0;
+    for (halo_req *r = batch; r; r = r->next)
+      if (r->kind == k) hp[m++] = r->h;
+    const int rc = run_kind(k, hp, m);
+    for (halo_req *r = batch; r; r = r->next)
+      if (r->kind == k) r->rc = rc;
+  }
+  q_batches++;
+  q_requests += tot;
+  pthread_mutex_unlock(&g_lock);
+  free(hp);
+}
+
+static int submit(HALO *h, int kind)
+{
+  halo_req req = {h, kind, 0, 0, NULL};
+  pthread_mutex_lock(&q_lock);
+  *q_tail = &req;
+  q_tail = &req.next;
+  while (!req.done) {
+    if (q_running) {
+      pthread_cond_wait(&q_cv, &q_lock);
+      continue;
+    }
+    /* no batch running: run every queued request, this one included */
+    halo_req *batch = q_head;
+    q_head = NULL;
+    q_tail = &q_head;
+    q_running = 1;
+    pthread_mutex_unlock(&q_lock);
+    run_batch(batch);
+    pthread_mutex_lock(&q_lock);
+    for (halo_req *r = batch, *nx; r; r = nx) {
+      nx = r->next; /* r lives on its caller's stack until done is seen */
+      r->done = 1;
+    }
+    q_running = 0;
+    pthread_cond_broadcast(&q_cv);
+  }
+  pthread_mutex_unlock(&q_lock);
+  return req.rc;
+}
+
+void ahf_gpu_halo_call_stats(uint64_t *batches, uint64_t *requests)
+{
+  pthread_mutex_lock(&g_lock);
+  if (batches) *batches = q_batches;
+  if (requests) *requests = q_requests;
+  pthread_mutex_unlock(&g_lock);
+}
+
 void sort_halo_particles(HALO *halo)
 {
   if (halo->npart < (long unsigned)simu.AHF_MINPART) return;
-  pthread_mutex_lock(&g_lock);
-  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_SORT_ONLY, 0);
-  pthread_mutex_unlock(&g_lock);
-  if (rc) fatal("sort_halo_particles");
+  if (submit(halo, REQ_SORT)) fatal("sort_halo_particles");
 }
 
 void rem_unbound(HALO *halo)
 {
   if (halo->npart < (long unsigned)simu.AHF_MINPART) return;
-  pthread_mutex_lock(&g_lock);
-  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_NO_SORT, 1);
-  pthread_mutex_unlock(&g_lock);
-  if (rc) fatal("rem_unbound");
+  if (submit(halo, REQ_UNBIND)) fatal("rem_unbound");
 }
 
 int ahf_unbind_halos(HALO *halos, long unsigned nhalos)
 {
   if (nhalos == 0) return 0;
+  HALO **hp = halo_ptrs(halos, nhalos);
   pthread_mutex_lock(&g_lock);
-  int rc = run_halos(halos, nhalos, AHFGPU_UNBIND_BATCHED, 1);
+  int rc = run_halos(hp, nhalos, AHFGPU_UNBIND_BATCHED, 1);
   pthread_mutex_unlock(&g_lock);
+  free(hp);
   return rc;
 }
 
 void rem_outsideRvir(HALO *halo, int icall)
 {
   if (halo->npart < (long unsigned)simu.AHF_MINPART) return;
-  pthread_mutex_lock(&g_lock);
-  int rc = run_halos(halo, 1, AHFGPU_UNBIND_PLAIN | AHFGPU_UNBIND_RVIR_ONLY, icall == 0);
-  pthread_mutex_unlock(&g_lock);
-  if (rc) fatal("rem_outsideRvir");
+  if (submit(halo, icall == 0 ? REQ_RVIR0 : REQ_RVIR1)) fatal("rem_outsideRvir");
 }
 
 int ahf_construct_halos(HALO *halos, long unsigned nhalos)
 {
   if (nhalos == 0) return 0;
+  HALO **hp = halo_ptrs(halos, nhalos);
   pthread_mutex_lock(&g_lock);
-  int rc = run_halos(halos, nhalos, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_WITH_RVIR, 1);
+  int rc = run_halos(hp, nhalos, AHFGPU_UNBIND_BATCHED | AHFGPU_UNBIND_WITH_RVIR, 1);
   pthread_mutex_unlock(&g_lock);
+  free(hp);
   return rc;
 }
 
@@ -681,7 +868,7 @@ static ahfgpu_profile_params profile_params(void)
 
 /* HaloProfiles of the resident haloes (last unbinding result, or the lists
  * as given with phi_rvir) into h[0..nh) */
-static int profiles_into(ahfgpu_ctx *c, HALO *h, long unsigned nh, const double *phi_rvir)
+static int profiles_into(ahfgpu_ctx *c, HALO **h, long unsigned nh, const double *phi_rvir)
 {
   const ahfgpu_profile_params pp = profile_params();
   int32_t *nb = (int32_t *)xcalloc(nh, 4);
@@ -694,7 +881,7 @@ static int profiles_into(ahfgpu_ctx *c, HALO *h, long unsigned nh, const double 
     uint64_t w = 0;
     for (long unsigned i = 0; i < nh; i++) {
       if (nb[i] <= 0) continue;
-      store_profile(h + i, nb[i], P + (size_t)i * AHFGPU_PROF_NP, Q + w * AHFGPU_PROF_NQ);
+      store_profile(h[i], nb[i], P + (size_t)i * AHFGPU_PROF_NP, Q + w * AHFGPU_PROF_NQ);
       w += (uint64_t)nb[i];
     }
   }
@@ -702,23 +889,23 @@ static int profiles_into(ahfgpu_ctx *c, HALO *h, long unsigned nh, const double 
   return rc;
 }
 
-static int run_profiles(HALO *h, long unsigned nh)
+static int run_profiles(HALO **h, long unsigned nh)
 {
   ahfgpu_ctx *c = ctx_get();
   ensure_particles(0);
   uint64_t tot = 0;
-  for (long unsigned i = 0; i < nh; i++) tot += h[i].npart;
+  for (long unsigned i = 0; i < nh; i++) tot += h[i]->npart;
   double *cen = (double *)xcalloc(5 * nh, 8), *pr = cen + 3 * nh;
   uint64_t *np = (uint64_t *)xcalloc(nh, 8), *ip = (uint64_t *)xcalloc(tot + 1, 8);
   uint64_t w = 0;
   for (long unsigned i = 0; i < nh; i++) {
-    cen[3 * i] = h[i].pos.x;
-    cen[3 * i + 1] = h[i].pos.y;
-    cen[3 * i + 2] = h[i].pos.z;
-    pr[2 * i] = h[i].Phi0;
-    pr[2 * i + 1] = h[i].R_vir;
-    np[i] = h[i].npart;
-    for (uint64_t j = 0; j < h[i].npart; j++) ip[w++] = h[i].ipart[j];
+    cen[3 * i] = h[i]->pos.x;
+    cen[3 * i + 1] = h[i]->pos.y;
+    cen[3 * i + 2] = h[i]->pos.z;
+    pr[2 * i] = h[i]->Phi0;
+    pr[2 * i + 1] = h[i]->R_vir;
+    np[i] = h[i]->npart;
+    for (uint64_t j = 0; j < h[i]->npart; j++) ip[w++] = h[i]->ipart[j];
   }
   int rc = ahfgpu_set_halos(c, cen, np, ip, nh);
   if (!rc) rc = profiles_into(c, h, nh, pr);
@@ -730,19 +917,18 @@ static int run_profiles(HALO *h, long unsigned nh)
 int HaloProfiles(HALO *halo)
 {
   if (halo->npart < (long unsigned)simu.AHF_MINPART) return 1;
-  pthread_mutex_lock(&g_lock);
-  int rc = run_profiles(halo, 1);
-  pthread_mutex_unlock(&g_lock);
-  if (rc) fatal("HaloProfiles");
+  if (submit(halo, REQ_PROFILES)) fatal("HaloProfiles");
   return 1;
 }
 
 int ahf_halo_profiles(HALO *halos, long unsigned nhalos)
 {
   if (nhalos == 0) return 0;
+  HALO **hp = halo_ptrs(halos, nhalos);
   pthread_mutex_lock(&g_lock);
-  int rc = run_profiles(halos, nhalos);
+  int rc = run_profiles(hp, nhalos);
   pthread_mutex_unlock(&g_lock);
+  free(hp);
   return rc;
 }
 
@@ -750,18 +936,18 @@ int ahf_halo_profiles(HALO *halos, long unsigned nhalos)
  * (construct == 2: and HaloProfiles, on the lists the device holds).
  * Haloes whose npart is 0 are skipped (src/libahf/ahf_halos_sfc.c:122-123
  * for the construction; the gather alone appends nothing for them) */
-static int gather_halos(HALO *h, long unsigned nh, int construct)
+static int gather_halos(HALO **h, long unsigned nh, int construct)
 {
   ahfgpu_ctx *c = ctx_get();
   ensure_keys();
   double *cen = (double *)xcalloc(4 * nh, 8), *rad = cen + 3 * nh;
   uint8_t *skip = (uint8_t *)xcalloc(nh, 1);
   for (long unsigned i = 0; i < nh; i++) {
-    cen[3 * i] = h[i].pos.x;
-    cen[3 * i + 1] = h[i].pos.y;
-    cen[3 * i + 2] = h[i].pos.z;
-    rad[i] = h[i].gatherRad;
-    skip[i] = construct && h[i].npart == 0;
+    cen[3 * i] = h[i]->pos.x;
+    cen[3 * i + 1] = h[i]->pos.y;
+    cen[3 * i + 2] = h[i]->pos.z;
+    rad[i] = h[i]->gatherRad;
+    skip[i] = construct && h[i]->npart == 0;
   }
   uint64_t *np = (uint64_t *)xcalloc(nh, 8);
   int rc = ahfgpu_gather_halos(c, cen, rad, skip, nh);
@@ -794,26 +980,26 @@ static int gather_halos(HALO *h, long unsigned nh, int construct)
       if (skip[i]) continue;
       if (construct) {
         /* ahf_halos_sfc_constructHalo starts from an empty list */
-        h[i].nll = 0;
-        h[i].ll = NULL;
-        h[i].npart = 0;
-        h[i].ipart = NULL;
+        h[i]->nll = 0;
+        h[i]->ll = NULL;
+        h[i]->npart = 0;
+        h[i]->ipart = NULL;
       }
       if (np[i]) {
-        long unsigned *lst = (long unsigned *)realloc(h[i].ipart, (h[i].npart + np[i]) * 8);
+        long unsigned *lst = (long unsigned *)realloc(h[i]->ipart, (h[i]->npart + np[i]) * 8);
         if (!lst) fatal("out of host memory");
-        memcpy(lst + h[i].npart, ip + w, np[i] * 8);
-        h[i].ipart = lst;
-        h[i].npart += np[i];
+        memcpy(lst + h[i]->npart, ip + w, np[i] * 8);
+        h[i]->ipart = lst;
+        h[i]->npart += np[i];
       }
       w += np[i];
       if (construct) {
         const double *q = props + AHFGPU_PROPS * i;
         const int set = (int)q[5];
-        if (set & AHFGPU_SET_MVIR) h[i].M_vir = q[0];
-        if (set & AHFGPU_SET_RVIR) h[i].R_vir = q[1];
-        if (set & AHFGPU_SET_PHI0) h[i].Phi0 = q[2];
-        if (set & AHFGPU_SET_OVDENS) h[i].ovdens = q[3];
+        if (set & AHFGPU_SET_MVIR) h[i]->M_vir = q[0];
+        if (set & AHFGPU_SET_RVIR) h[i]->R_vir = q[1];
+        if (set & AHFGPU_SET_PHI0) h[i]->Phi0 = q[2];
+        if (set & AHFGPU_SET_OVDENS) h[i]->ovdens = q[3];
       }
     }
   }
@@ -829,26 +1015,27 @@ static int gather_halos(HALO *h, long unsigned nh, int construct)
 
 void ahf_halos_sfc_gatherParts(HALO *halo)
 {
-  pthread_mutex_lock(&g_lock);
-  int rc = gather_halos(halo, 1, 0);
-  pthread_mutex_unlock(&g_lock);
-  if (rc) fatal("ahf_halos_sfc_gatherParts");
+  if (submit(halo, REQ_GATHER)) fatal("ahf_halos_sfc_gatherParts");
 }
 
 int ahf_gather_construct_halos(HALO *halos, long unsigned nhalos)
 {
   if (nhalos == 0) return 0;
+  HALO **hp = halo_ptrs(halos, nhalos);
   pthread_mutex_lock(&g_lock);
-  int rc = gather_halos(halos, nhalos, 1);
+  int rc = gather_halos(hp, nhalos, 1);
   pthread_mutex_unlock(&g_lock);
+  free(hp);
   return rc;
 }
 
 int ahf_construct_profile_halos(HALO *halos, long unsigned nhalos)
 {
   if (nhalos == 0) return 0;
+  HALO **hp = halo_ptrs(halos, nhalos);
   pthread_mutex_lock(&g_lock);
-  int rc = gather_halos(halos, nhalos, 2);
+  int rc = gather_halos(hp, nhalos, 2);
   pthread_mutex_unlock(&g_lock);
+  free(hp);
   return rc;
 }

@@ -153,6 +153,7 @@ struct ahfgpu_ctx {
   int split_graph_batch = 0;    /* graph replays between done checks; 0: 4 */
   int split_graph_launches = 0; /* replays of the last split unbinding */
   void *halo_set = nullptr; /* resident halo lists (halo_set.h) */
+  void *halo_spare = nullptr; /* a released set's buffers, kept for reuse */
   uint64_t *sfckey = nullptr; /* SFC keys of the resident particles */
   int64_t key_cap = 0;
   int keys_ok = 0;            /* keys present and in order */

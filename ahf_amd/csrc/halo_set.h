@@ -36,11 +36,16 @@ struct HaloSet {
   double *cen = nullptr;      /* nh*3 */
   int32_t *order = nullptr;   /* haloes largest first */
   std::vector<int64_t> h_off;
+  std::vector<int32_t> h_ord; /* the host copy of order */
   HaloOut *out = nullptr;
   uint32_t *buf_a = nullptr, *buf_b = nullptr;
   int done = 0;
   char *dev = nullptr, *host = nullptr;
   size_t cap = 0;
+  char *res = nullptr; /* pinned staging of the results */
+  size_t res_cap = 0;
+  char *up = nullptr; /* pinned staging of per-call uploads */
+  size_t up_cap = 0;
   size_t o_cen = 0, o_ord = 0, o_ip = 0, need = 0;
 };
 
@@ -49,6 +54,8 @@ static inline void free_halos(HaloSet *s)
   if (!s) return;
   if (s->dev) (void)hipFree(s->dev);
   if (s->host) (void)hipHostFree(s->host);
+  if (s->res) (void)hipHostFree(s->res);
+  if (s->up) (void)hipHostFree(s->up);
   delete s;
 }
 
@@ -60,6 +67,11 @@ static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static inline int halo_set_reserve(ahfgpu_ctx *c, int64_t nh, int64_t ne, HaloSet **out)
 {
   HaloSet *s = (HaloSet *)c->halo_set;
+  if (!s) {
+    /* the buffers of the last released set, kept for reuse */
+    s = (HaloSet *)c->halo_spare;
+    c->halo_spare = nullptr;
+  }
   if (!s) s = new HaloSet();
   c->halo_set = nullptr;
   s->done = 0;
@@ -106,6 +118,32 @@ static inline void halo_set_host_meta(HaloSet *s, const int64_t *off, const doub
   for (int64_t h = 0; h < nh; h++) ord[h] = (int32_t)h;
   std::stable_sort(ord, ord + nh,
                    [&](int32_t x, int32_t y) { return off[x + 1] - off[x] > off[y + 1] - off[y]; });
+  s->h_ord.assign(ord, ord + nh);
 }
+
+/* a pinned buffer of at least `bytes` (grown as needed), or nullptr */
+static inline char *pinned_grow(char **p, size_t *cap, size_t bytes)
+{
+  if (bytes > *cap) {
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const size_t want = bytes + bytes / 2 + 4096;
+    if (hipHostMalloc((void **)p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return nullptr;
+    }
+    *cap = want;
+  }
+  return *p;
+}
+
+/* pinned staging for the results (read back after the stream is idle) */
+static inline char *halo_set_res(HaloSet *s, size_t bytes) { return pinned_grow(&s->res, &s->res_cap, bytes); }
+
+/* pinned staging for the uploads of one unbinding call (every call ends
+ * with its stream synchronised, so the next one may overwrite it) */
+static inline char *halo_set_up(HaloSet *s, size_t bytes) { return pinned_grow(&s->up, &s->up_cap, bytes); }
 
 #endif

@@ -110,17 +110,8 @@ struct SortBase {
   int64_t base[AHF_SORT_TIERS];
 };
 
-/* the halo list of each tier, from per-tier counters that start at zero
- * (order within a tier is arbitrary: each halo is sorted on its own) */
-__global__ void k_sort_classify(const int64_t *__restrict__ off, int64_t nh, int32_t minpart, SortBase b,
-                                int32_t *__restrict__ cnt, int32_t *__restrict__ list)
-{
-  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= nh) return;
-  const int t = sort_tier(off[h + 1] - off[h], minpart);
-  if (t < 0) return;
-  list[b.base[t] + atomicAdd(&cnt[t], 1)] = (int32_t)h;
-}
+/* the haloes of each tier lie at base[t] of one list (built on the host;
+ * each halo is sorted on its own) */
 
 /*
  * Radial sort of the haloes list[0..cnt), at most CAP members each, one
@@ -1682,10 +1673,12 @@ static int nr_tie_order(ahfgpu_ctx *c, hipStream_t st, const int64_t *off, int64
   hipLaunchKernelGGL(k_tie_detect, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, minpart, sorted, cen, c->px,
                      c->py, c->pz, tie);
   exclusive_scan<uint8_t, int32_t>(tie, tpos, nh, tmp32h, st);
-  int32_t T = 0;
-  if (hipMemcpyAsync(&T, tpos + nh, sizeof(T), hipMemcpyDeviceToHost, st) != hipSuccess ||
+  /* the counts come back through the context's pinned words */
+  int32_t *hT = c->h_words + 2;
+  if (hipMemcpyAsync(hT, tpos + nh, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "radial sort: tie count: %s", hipGetErrorString(hipGetLastError()));
+  const int32_t T = *hT;
   if (T == 0) return 0;
   TieWork W;
   W.th = (int32_t *)take((size_t)(T + 1) * 4);
@@ -1695,10 +1688,11 @@ static int nr_tie_order(ahfgpu_ctx *c, hipStream_t st, const int64_t *off, int64
   if (!W.th || !W.tsz || !W.toff || !W.tmp64) return c->fail(AHFGPU_ENOMEM, "radial sort: tie buffers");
   hipLaunchKernelGGL(k_tie_list, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, tie, tpos, nh, off, W.th, W.tsz);
   exclusive_scan<int64_t, int64_t>(W.tsz, W.toff, T, W.tmp64, st);
-  int64_t m = 0;
-  if (hipMemcpyAsync(&m, W.toff + T, sizeof(m), hipMemcpyDeviceToHost, st) != hipSuccess ||
+  int64_t *hm = (int64_t *)(c->h_words + 4);
+  if (hipMemcpyAsync(hm, W.toff + T, sizeof(int64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "radial sort: tie members: %s", hipGetErrorString(hipGetLastError()));
+  const int64_t m = *hm;
   if (m >= ((int64_t)1 << 31) - 64) return c->fail(AHFGPU_EUNSUP, "radial sort: over 2^31 members in haloes with ties");
   const int64_t cs = m / 2 + T + 1, M = m + 1;
   W.key = (double *)take((size_t)M * 8);
@@ -2026,12 +2020,20 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
         sb.base[t] = acc;
         acc += tcount[t];
       }
-      int32_t *tcnt = A.get<int32_t>(AHF_SORT_TIERS);
+      /* the tier lists, built on the host from the sizes it holds, go up
+       * through the set's pinned staging */
       int32_t *tlist = A.get<int32_t>(acc);
-      if (!tcnt || !tlist) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (sort)");
-      hipMemsetAsync(tcnt, 0, AHF_SORT_TIERS * sizeof(int32_t), c->st);
-      hipLaunchKernelGGL(k_sort_classify, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, p->minpart, sb, tcnt,
-                         tlist);
+      int32_t *hl = (int32_t *)halo_set_up(s, (size_t)acc * sizeof(int32_t) + 16);
+      if (!tlist || !hl) return c->fail(AHFGPU_ENOMEM, "sort tier lists");
+      {
+        int64_t fill[AHF_SORT_TIERS];
+        for (int t = 0; t < AHF_SORT_TIERS; t++) fill[t] = sb.base[t];
+        for (int64_t h = 0; h < nh; h++) {
+          const int t = sort_tier(s->h_off[h + 1] - s->h_off[h], p->minpart);
+          if (t >= 0) hl[fill[t]++] = (int32_t)h;
+        }
+      }
+      hipMemcpyAsync(tlist, hl, (size_t)acc * sizeof(int32_t), hipMemcpyHostToDevice, c->st);
       auto grid_of = [](int64_t n) { return dim3((unsigned)std::min<int64_t>(n, (int64_t)1 << 20)); };
 #define AHF_SORT_LAUNCH(T, CAP, BS, COPY)                                                                          \
   if (tcount[T] > 0)                                                                                               \
@@ -2087,8 +2089,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
      * AHF_BIG_HALO members or more get AHF_BIG_BLOCK-thread workgroups, the
      * others AHF_BLOCK */
     int64_t nbig = 0, nsplit = 0;
-    std::vector<int32_t> ord((size_t)nh);
-    if (nh) hipMemcpy(ord.data(), s->order, (size_t)nh * sizeof(int32_t), hipMemcpyDeviceToHost);
+    const std::vector<int32_t> &ord = s->h_ord;
     const int64_t split_min = c->split_min > 0 ? c->split_min : AHF_SPLIT_DEFAULT;
     auto size_of = [&](int64_t q) { return s->h_off[ord[q] + 1] - s->h_off[ord[q]]; };
     if (c->split_min >= 0)
@@ -2117,7 +2118,13 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
                          c->py,
                          c->pz, c->vx, c->vy, c->vz, s->out);
     }
-    if (nh > nmid) {
+    if (nh > nmid && nmid == 0) {
+      /* only one-wavefront haloes: on the first stream, no hand-over */
+      const int64_t grid = std::min<int64_t>(nh, (int64_t)dev_cu * 32);
+      hipLaunchKernelGGL(k_unbind_batched<64>, dim3((unsigned)grid), dim3(64), 0, c->st, K, s->off, s->order, 0, nh,
+                         next + 2, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx,
+                         c->vy, c->vz, s->out);
+    } else if (nh > nmid) {
       const int64_t grid = std::min<int64_t>(nh - nmid, (int64_t)dev_cu * 32);
       hipStreamWaitEvent(c->st3, c->ev_s0, 0);
       hipLaunchKernelGGL(k_unbind_batched<64>, dim3((unsigned)grid), dim3(64), 0, c->st3, K, s->off, s->order, nmid,
@@ -2177,15 +2184,22 @@ extern "C" int ahfgpu_unbind_results(ahfgpu_ctx *c, uint64_t *npart, double *pro
   HaloSet *s = (HaloSet *)c->halo_set;
   if (!s || !s->done) return c->fail(AHFGPU_EINVAL, "no unbinding result");
   const int64_t nh = s->nh, ne = s->ne;
-  std::vector<HaloOut> ho((size_t)nh);
-  std::vector<uint32_t> a((size_t)ne), b((size_t)ne);
-  if (nh) hipMemcpy(ho.data(), s->out, (size_t)nh * sizeof(HaloOut), hipMemcpyDeviceToHost);
+  if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  /* one pinned block: the per-halo records, then both member buffers */
+  const size_t o_a = ((size_t)nh * sizeof(HaloOut) + 255) & ~(size_t)255, o_b = o_a + (((size_t)ne * 4 + 255) & ~(size_t)255);
+  char *R = halo_set_res(s, o_b + (size_t)ne * 4 + 16);
+  if (!R) return c->fail(AHFGPU_ENOMEM, "pinned staging of the results");
+  if (nh) hipMemcpyAsync(R, s->out, (size_t)nh * sizeof(HaloOut), hipMemcpyDeviceToHost, c->st);
   if (ne) {
-    hipMemcpy(a.data(), s->buf_a, (size_t)ne * 4, hipMemcpyDeviceToHost);
-    hipMemcpy(b.data(), s->buf_b, (size_t)ne * 4, hipMemcpyDeviceToHost);
+    hipMemcpyAsync(R + o_a, s->buf_a, (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
+    hipMemcpyAsync(R + o_b, s->buf_b, (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
   }
+  if (hipStreamSynchronize(c->st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "unbinding results: %s", hipGetErrorString(hipGetLastError()));
+  const HaloOut *ho = (const HaloOut *)R;
+  const uint32_t *a = (const uint32_t *)(R + o_a), *b = (const uint32_t *)(R + o_b);
   for (int64_t h = 0; h < nh; h++) {
-    const HaloOut &o = ho[(size_t)h];
+    const HaloOut &o = ho[h];
     if (npart) npart[h] = (uint64_t)o.npart;
     if (props) {
       double *q = props + AHFGPU_PROPS * h;
@@ -2197,9 +2211,9 @@ extern "C" int ahfgpu_unbind_results(ahfgpu_ctx *c, uint64_t *npart, double *pro
       q[5] = (double)o.set;
     }
     if (ipart) {
-      const std::vector<uint32_t> &src = o.in_b ? b : a;
+      const uint32_t *src = o.in_b ? b : a;
       const int64_t base = s->h_off[(size_t)h];
-      for (int64_t j = 0; j < o.npart; j++) ipart[base + j] = src[(size_t)(base + j)];
+      for (int64_t j = 0; j < o.npart; j++) ipart[base + j] = src[base + j];
     }
   }
   return 0;
@@ -2208,8 +2222,17 @@ extern "C" int ahfgpu_unbind_results(ahfgpu_ctx *c, uint64_t *npart, double *pro
 extern "C" void ahfgpu_release_halos(ahfgpu_ctx *c)
 {
   if (!c) return;
-  free_halos((HaloSet *)c->halo_set);
+  /* the set's buffers stay with the context for the next set (the larger
+   * of two when one is kept already); ahfgpu_destroy frees them */
+  HaloSet *s = (HaloSet *)c->halo_set, *k = (HaloSet *)c->halo_spare;
   c->halo_set = nullptr;
+  if (!s) return;
+  if (k && k->cap >= s->cap) {
+    free_halos(s);
+    return;
+  }
+  free_halos(k);
+  c->halo_spare = s;
 }
 
 /* ================================== unbinding without device allocation */

@@ -78,6 +78,28 @@ def cpu_baseline(snap, threads):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
+LINKED_BIN = os.path.join(ROOT, "oracle", "_ref", "ahf_ref_linked")
+
+
+def dropin_figures(snap, threads):
+    """AHF's own objects linked with the library by the integration recipe
+    (oracle/_ref/ahf_ref_linked) on the same snapshot file: gen_domgrids,
+    ll, zero_dens, assign_npart and gen_AMRhierarchy end to end from and
+    into AHF's structs, and the per-halo sort_halo_particles + rem_unbound
+    calls of AHF's OpenMP halo loop on `threads` threads (best of 2 runs;
+    the first run also creates the device context)"""
+    if not snap or not os.path.exists(LINKED_BIN):
+        return None
+    r = subprocess.run([LINKED_BIN, "time", snap, "--threads", str(threads), "--reps", "2", "--skip-1thread"],
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log(f"drop-in timing failed: {r.stderr[-500:]}")
+        return None
+    t = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"amr_seconds": t["amr_seconds"], "unbind_seconds": t["unbind_seconds"], "threads": threads,
+            "harness": "oracle/_ref/ahf_ref_linked time (the reference's objects + libahf_amd.so)"}
+
+
 def gridinfo_figures(ctx, s, reps=5):
     """ahf_gridinfo (src/libahf/ahf_gridinfo.c:59-860) on the resident
     hierarchy, with the snapshot's virial overdensity: device milliseconds
@@ -509,10 +531,21 @@ def main():
             }
             out["vs_cpu"] = value / out["cpu_baseline"]["value"]
     out["gridinfo"] = g = gridinfo_cpu(gi_fig, None if args.no_cpu else cpu_snap)
+    ctx.close()
+    if not args.no_cpu and not in_memory:
+        dr = dropin_figures(snap, min(os.cpu_count() or 1, 16))
+        if dr and out["cpu_baseline"]:
+            dr["reference_amr_seconds"] = rb["amr_seconds"]
+            dr["reference_unbind_seconds"] = rb["unbind_seconds"]
+        out["dropin"] = dr
+        if dr:
+            log(f"drop-in (AHF's objects + the library): AMR {dr['amr_seconds']:.3f} s, per-halo unbinding "
+                f"{1e3 * dr['unbind_seconds']:.2f} ms on {dr['threads']} threads"
+                + (f"; reference {dr['reference_amr_seconds']:.3f} s, {1e3 * dr['reference_unbind_seconds']:.2f} ms"
+                   if "reference_amr_seconds" in dr else ""))
     log(f"gridinfo: {g['device_ms']:.3f} ms on the GPU for {g['nodes']} nodes of {g['no_grids']} grids, "
         f"{g['refinements']} refinements"
         + (f"; reference CPU {g['cpu_nodes_per_s']:.3g} nodes/s ({g['cpu_sample']})" if g.get("cpu_nodes_per_s") else ""))
-    ctx.close()
     if in_memory:
         args.no_construct_workload = True
     if not args.no_construct_workload:

@@ -128,6 +128,11 @@ void ahf_io_WriteParticles(const char *fprefix, HALO *halos, unsigned long *idx,
  * and release the library's device state. */
 int ahf_gpu_init(int device);
 void ahf_gpu_finalize(void);
+/* No counterpart in the reference: per-halo calls (sort_halo_particles,
+ * rem_outsideRvir, rem_unbound, HaloProfiles, ahf_halos_sfc_gatherParts)
+ * made at the same time from AHF's OpenMP loop are combined into batched
+ * passes; the number of such passes and of the calls they served so far. */
+void ahf_gpu_halo_call_stats(uint64_t *batches, uint64_t *requests);
 
 #ifdef __cplusplus
 }

@@ -209,6 +209,9 @@ int ahfgpu_set_unbind_split(ahfgpu_ctx *ctx, int64_t min_members);
 #define AHFGPU_SET_PHI0 4
 #define AHFGPU_SET_OVDENS 8
 int ahfgpu_unbind_results(ahfgpu_ctx *ctx, uint64_t *npart, double *props, uint64_t *ipart);
+/* drop the resident haloes; their buffers stay with the context for the
+ * next ahfgpu_set_halos or gather (no allocation per call for a caller that
+ * handles few haloes at a time) until ahfgpu_destroy */
 void ahfgpu_release_halos(ahfgpu_ctx *ctx);
 
 /* No counterpart in the reference.  For callers that cannot allocate device

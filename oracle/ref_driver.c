@@ -843,6 +843,11 @@ static void cmd_time(const snap_data *s, int threads, int reps, int skip_amr, in
     double t1 = run_unbind(h1, s->h.nhalos, 1);
     if (t1 < best_unb1) best_unb1 = t1;
   }
+#ifndef AHF_REF_BUILD
+  uint64_t nb = 0, nr = 0;
+  ahf_gpu_halo_call_stats(&nb, &nr);
+  fprintf(stderr, "per-halo calls: %llu in %llu combined passes\n", (unsigned long long)nr, (unsigned long long)nb);
+#endif
   fprintf(stdout,
           "{\"amr_seconds\": %.6f, \"unbind_seconds\": %.6f, \"unbind_seconds_1thread\": %.6f, "
           "\"threads\": %d, \"no_grids\": %d}\n",
