@@ -169,6 +169,23 @@ AHF_HD void ahf_axis_order(int64_t X, int64_t L, int ord[3])
   else { ord[0] = -1; ord[1] = 0; ord[2] = 1; }
 }
 
+/*
+ * One source row of a lane in a staged plane (the march's rows walk): the
+ * row's three sources are adjacent nodes, so their entries form the range
+ * [lo, hi) of the staged order with inner boundaries b1, b2 (first entries
+ * of the second and third source).  AHF's x order of the three is a
+ * rotation of the row (ahf_axis_order: rot 0 inside the box, 1 at x = 0,
+ * 2 at x = L-1), so the walk starts at the rot-th source and wraps round.
+ */
+AHF_HD uint32_t ahf_row_start(uint32_t lo, uint32_t b1, uint32_t b2, uint32_t hi, int rot)
+{
+  const uint32_t q = rot == 0 ? lo : (rot == 1 ? b1 : b2);
+  return q == hi ? lo : q; /* the sources from the first to the row's end are empty */
+}
+AHF_HD uint32_t ahf_row_next(uint32_t q, uint32_t lo, uint32_t hi) { return q + 1u == hi ? lo : q + 1u; }
+/* the source of entry q along the row: 0, 1, 2 (ex = -1, 0, +1) */
+AHF_HD uint32_t ahf_row_src(uint32_t q, uint32_t b1, uint32_t b2) { return q >= b2 ? 2u : (q >= b1 ? 1u : 0u); }
+
 /* x in [-L, 2L) folded into [0, L) */
 AHF_HD int64_t ahf_wrap(int64_t x, int64_t L) { return x < 0 ? x + L : (x >= L ? x - L : x); }
 

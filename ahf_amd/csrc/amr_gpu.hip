@@ -1012,18 +1012,26 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
     __syncthreads();
     const double czs = ahf_centre(ahf_wrap(P, L), Ld);
     if constexpr (FULL) {
-      /* one thread per source node: few entries each */
-      for (int u = tid; u < AHF_SLAB_N; u += AHF_BLOCK) {
-        const uint32_t c = cnt_p[u], o = s_off[u];
-        if (o + c > (uint32_t)CAP) continue; /* read from the records */
+      /* one thread per staged entry (the node holding entry e found by a
+       * binary search of the offsets): the plane's work spread evenly,
+       * whatever the nodes' counts */
+      const uint32_t lim = tot < (uint32_t)CAP ? tot : (uint32_t)CAP;
+      for (uint32_t e = tid; e < lim; e += AHF_BLOCK) {
+        int lo = 0, hi = AHF_SLAB_N;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_off[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        const int u = lo;
+        if (s_off[u] + cnt_p[u] > (uint32_t)CAP) continue; /* read from the records */
         const int row = u / AHF_SLAB_W, col = u - row * AHF_SLAB_W;
         const double cxs = s_cx[col], cys = s_cy[row];
-        for (uint32_t j = 0; j < c; j++) {
-          const ahf_q4 q = qp[start_p[u] + j];
+        {
+          const ahf_q4 q = qp[start_p[u] + (e - s_off[u])];
           const double sz = ahf_pn_sep(q.z, czs, Ld);
           const double sy = ahf_pn_sep(q.y, cys, Ld);
           const double sx = ahf_pn_sep(q.x, cxs, Ld);
-          const uint32_t e = o + j;
           s_e[e] = m2pd * tsc_w_c<1>(sz);
           s_e[CAP + e] = m2pd * tsc_w_c<0>(sz);
           s_e[2 * CAP + e] = m2pd * tsc_w_c<-1>(sz);
@@ -1070,6 +1078,66 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
     /* plane P is the first source plane of P+1, the second of P, the third of P-1 */
     const uint32_t m0 = mn & 0x1FFu, m1 = ac ? (mc >> 9) & 0x1FFu : 0u, m2 = ap ? (mp >> 18) & 0x1FFu : 0u;
     if constexpr (FULL) {
+      /* Rows fast path (ascending lists, every source staged): the three
+       * sources of one source row are adjacent nodes of the staged plane, so
+       * their entries form one range of the staged order, walked from the
+       * first source in the lane's x order and wrapping round the range
+       * (the order at the periodic faces is a rotation); the source of an
+       * entry, and so its x weight, follows from the range's inner
+       * boundaries.  A lane qualifies when each of its three targets takes
+       * either none or all of the plane's non-empty sources, and the
+       * wavefront takes the path only if all its lanes qualify. */
+      bool rows = false;
+      const int rot = ox[0] + 1; /* first source in x order: 0, 1 or 2 along a row */
+      if (tot <= (uint32_t)CAP && m.step == 1u) {
+        uint32_t ne9 = 0;
+#pragma unroll
+        for (int bb = 0; bb < 3; bb++) {
+          const int u = (ly + 1 + oy[bb]) * AHF_SLAB_W + lx;
+          const uint32_t o0 = s_off[u], o1 = s_off[u + 1], o2 = s_off[u + 2], o3 = s_off[u + 3];
+#pragma unroll
+          for (int e = 0; e < 3; e++) {
+            const int ex = ox[e];
+            const uint32_t ce = ex < 0 ? o1 - o0 : (ex == 0 ? o2 - o1 : o3 - o2);
+            if (ce) ne9 |= 1u << (bb * 3 + e);
+          }
+        }
+        const bool ok = (m0 == 0u || m0 == ne9) && (m1 == 0u || m1 == ne9) && (m2 == 0u || m2 == ne9);
+        rows = __ballot(!ok) == 0ull;
+      }
+      if (rows) {
+        const bool r0 = m0 != 0u, r1 = m1 != 0u, r2 = m2 != 0u;
+        const int oy0 = oy[0], oy1 = oy[1], oy2 = oy[2];
+        uint32_t ntot = 0;
+#pragma unroll
+        for (int bb = 0; bb < 3; bb++) {
+          const int u = (ly + 1 + oy[bb]) * AHF_SLAB_W + lx;
+          ntot += s_off[u + 3] - s_off[u];
+        }
+        /* the current row: next entry q, entries left, range [lo, hi) with
+         * inner boundaries b1, b2, and the offset of its y weight */
+        uint32_t q = 0, rem = 0, lo = 0, hi = 0, b1 = 0, b2 = 0, y3 = 0;
+        int br = -1;
+        for (uint32_t it = 0; it < ntot; it++) {
+          while (rem == 0u) {
+            br++;
+            const int ey = br == 0 ? oy0 : (br == 1 ? oy1 : oy2);
+            const int u = (ly + 1 + ey) * AHF_SLAB_W + lx;
+            lo = s_off[u];
+            b1 = s_off[u + 1];
+            b2 = s_off[u + 2];
+            hi = s_off[u + 3];
+            rem = hi - lo;
+            q = ahf_row_start(lo, b1, b2, hi, rot);
+            y3 = (3u + (uint32_t)(1 - ey)) * (uint32_t)CAP;
+          }
+          /* x weight index 1 - ex of the entry's source: 8, 7, 6 along the row */
+          const uint32_t wx = (8u - ahf_row_src(q, b1, b2)) * (uint32_t)CAP;
+          fold3(dn, dc, dp, r0, r1, r2, s_e[q], s_e[CAP + q], s_e[2 * CAP + q], s_e[y3 + q], s_e[wx + q]);
+          q = ahf_row_next(q, lo, hi);
+          rem--;
+        }
+      } else {
       /* the lane's sources of this plane in order, as one flat sequence */
       uint32_t *qq = s_q + tid * 9;
       if (tot <= (uint32_t)CAP) {
@@ -1156,6 +1224,7 @@ __global__ void __launch_bounds__(AHF_BLOCK) k_gather_march(level_view v, int64_
         }
         q += m.step;
         rem--;
+      }
       }
       }
     } else {

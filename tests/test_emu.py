@@ -115,3 +115,14 @@ def test_emu_gridinfo_configs(oracle, emu, snapgen, tmp_path, cfg, side, seed, h
     snap = str(tmp_path / "g.snap")
     assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
     _cmp_gridinfo(oracle, emu, snap, tmp_path, how)
+
+
+def test_march_rows_walk(tmp_path):
+    """the march's rows walk (the three sources of a source row as one
+    wrapped range of the staged plane) visits each lane's sources in AHF's
+    traversal order, at the periodic faces too, with each entry's x offset"""
+    exe = str(tmp_path / "march_rows_check")
+    subprocess.run(["gcc", "-std=c11", "-O2", "-Wno-unused-function", "-o", exe,
+                    os.path.join(HERE, "emu", "march_rows_check.c"), "-lm"], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr
