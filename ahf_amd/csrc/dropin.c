@@ -269,8 +269,10 @@ static int64_t find_key(const uint64_t *k, int64_t n, uint64_t want)
 #define PK(x, y, z) ((uint64_t)(x) | ((uint64_t)(y) << 21) | ((uint64_t)(z) << 42))
 
 /* every AHF grid is a union of 2x2x2 node blocks (runs of even start and
- * length, mirrored rows and planes); returns 0, or -1 if this one is not */
-static int flatten_grid(gridls *g, flat_grid *f)
+ * length, mirrored rows and planes); returns 0, or -1 if this one is not.
+ * With lists == 0 the particle lists are not read (list0 stays zero and
+ * list NULL). */
+static int flatten_grid(gridls *g, flat_grid *f, int lists)
 {
   memset(f, 0, sizeof(*f));
   int64_t n = 0;
@@ -342,6 +344,7 @@ static int flatten_grid(gridls *g, flat_grid *f)
       continue;
     }
     if (!(x & 1) && !(y & 1) && !(z & 1) && (w >> 63)) f->link[o] = 1;
+    if (!lists) continue;
     int64_t c = 0;
     for (partptr p = nodes[i]->ll; p; p = p->ll) c++;
     cnt[idx] = c;
@@ -349,6 +352,7 @@ static int flatten_grid(gridls *g, flat_grid *f)
   free(nodes);
   free(nxyz);
   if (bad) return -1;
+  if (!lists) return 0;
   for (int64_t i = 0; i < n; i++) f->list0[i + 1] += f->list0[i];
   f->n_listed = f->list0[n];
   f->list = (uint32_t *)xcalloc((size_t)f->n_listed, 4);
@@ -365,7 +369,7 @@ boolean assign_npart(gridls *cur_grid)
 {
   pthread_mutex_lock(&g_lock);
   flat_grid f;
-  if (flatten_grid(cur_grid, &f)) {
+  if (flatten_grid(cur_grid, &f, 1)) {
     pthread_mutex_unlock(&g_lock);
     fatal("assign_npart: grid is not made of 2x2x2 node blocks");
   }
@@ -486,7 +490,7 @@ static void fill_level(gridls *g, int lev, int build)
     build_quads(g, key, link, n_oct, node_of);
   } else {
     flat_grid f;
-    if (flatten_grid(g, &f) || f.n_oct != n_oct) fatal("domain grid does not match the device level");
+    if (flatten_grid(g, &f, 0) || f.n_oct != n_oct) fatal("domain grid does not match the device level");
     memcpy(node_of, f.node_of, (size_t)nn * sizeof(nptr));
     flat_free(&f);
   }
@@ -558,6 +562,98 @@ boolean gen_AMRhierarchy(gridls **grid_list, int *no_grids)
   global.fst_cycle = 0;
   pthread_mutex_unlock(&g_lock);
   return 0;
+}
+
+/* --------------------------------------------------------- gridinfo ---- */
+
+/* AHF defines these three in the object ahf_gridinfo replaces
+ * (src/libahf/ahf_gridinfo.c:43-45); `ahf` is src/common.c:26's */
+SRINDEX *spatialRefIndex = NULL;
+int *numIsoRef = NULL;
+int totnumIsoRef = 0;
+__attribute__((weak)) info_ahf ahf;
+
+/* AHF's cosmology (src/libutility/cosmology.c:47-53, 117-175), called as
+ * ahf_gridinfo calls it; weak references, so that the library also links
+ * without it */
+extern double calc_virial(double a) __attribute__((weak));
+extern double calc_rho_vir(double a) __attribute__((weak));
+
+void ahf_gridinfo(gridls *grid_list, int curgrid_no)
+{
+  (void)grid_list;
+  if (!calc_virial || !calc_rho_vir) fatal("ahf_gridinfo: AHF's calc_virial and calc_rho_vir are not linked");
+  /* src/libahf/ahf_gridinfo.c:126-166 */
+  const double a = global.a;
+  ahfgpu_gridinfo_params p;
+  p.ovlim = calc_virial(a);
+  p.rho_vir = (a * a * a) * calc_rho_vir(a);
+  p.refine_mass = simu.pmass * simu.med_weight;
+  p.boxsize = simu.boxsize;
+  pthread_mutex_lock(&g_lock);
+  ahfgpu_ctx *c = ctx_get();
+  int64_t L, n_oct, n_listed;
+  double crit, m2pd;
+  const int nref = curgrid_no - global.domgrid_no;
+  /* the grids must be the library's last hierarchy: as many refinements,
+   * and the finest one of the expected size */
+  if (nref < 0 || ahfgpu_amr_level_sizes(c, nref, &L, &n_oct, &n_listed, &crit, &m2pd) ||
+      (uint64_t)L != (uint64_t)global.dom_grid[nref].l1dim ||
+      ahfgpu_amr_level_sizes(c, nref + 1, &L, &n_oct, &n_listed, &crit, &m2pd) == 0) {
+    pthread_mutex_unlock(&g_lock);
+    fatal("ahf_gridinfo: the grids are not the hierarchy gen_AMRhierarchy built last");
+  }
+  ahfgpu_gridinfo_info info;
+  if (ahfgpu_gridinfo(c, &p, &info)) {
+    pthread_mutex_unlock(&g_lock);
+    fatal("ahf_gridinfo");
+  }
+  ahf.min_ref = info.min_ref;
+  ahf.no_grids = info.no_grids;
+  global.max_ovdens = info.max_ovdens;
+  /* src/libahf/ahf_gridinfo.c:720-763: one entry per colour started (names
+   * 1..n_colours; entry 0 and the merged colours keep -1, -1 and no
+   * periodicity; the last entry stays zero, as the reference leaves it) */
+  const int ncol = info.n_colours;
+  spatialRefIndex = (SRINDEX *)xcalloc((size_t)ncol + 1, sizeof(SRINDEX));
+  for (int i = 0; i < ncol; i++) spatialRefIndex[i].refLevel = spatialRefIndex[i].isoRefIndex = -1;
+  numIsoRef = (int *)xcalloc(info.no_grids > 0 ? (size_t)info.no_grids : 1, sizeof(int));
+  totnumIsoRef = info.tot_iso;
+  for (int g = 0; g < info.no_grids; g++) {
+    gridls *cg = global.dom_grid + info.min_ref + g;
+    flat_grid f;
+    if (flatten_grid(cg, &f, 0)) {
+      pthread_mutex_unlock(&g_lock);
+      fatal("ahf_gridinfo: a grid is not made of octets");
+    }
+    numIsoRef[g] = info.num_iso[g];
+    int32_t *col = (int32_t *)xcalloc((size_t)f.n_node, 4);
+    int32_t *refs = (int32_t *)xcalloc(6 * (size_t)(info.num_iso[g] > 0 ? info.num_iso[g] : 1), 4);
+    if (ahfgpu_amr_level_sizes(c, info.min_ref + g, &L, &n_oct, &n_listed, &crit, &m2pd) || n_oct != f.n_oct ||
+        ahfgpu_gridinfo_results(c, g, col, refs)) {
+      pthread_mutex_unlock(&g_lock);
+      fatal("ahf_gridinfo: grid results");
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < f.n_node; i++) f.node_of[i]->force.colour = col[i];
+    for (int k = 0; k < info.num_iso[g]; k++) {
+      const int32_t *e = refs + 6 * k;
+      if (e[0] < 1 || e[0] > ncol) {
+        pthread_mutex_unlock(&g_lock);
+        fatal("ahf_gridinfo: a refinement's colour is out of range");
+      }
+      SRINDEX *r = spatialRefIndex + e[0];
+      r->refLevel = e[1];
+      r->isoRefIndex = e[2];
+      r->periodic.x = e[3];
+      r->periodic.y = e[4];
+      r->periodic.z = e[5];
+    }
+    free(col);
+    free(refs);
+    flat_free(&f);
+  }
+  pthread_mutex_unlock(&g_lock);
 }
 
 /* ----------------------------------------------------------- haloes ---- */

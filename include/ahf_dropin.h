@@ -11,7 +11,7 @@
  *   simu   NGRID_DOM, NGRID_MAX, Nth_dom, Nth_ref, AHF_MINPART, AHF_VTUNE,
  *          mean_dens, no_part, no_vpart
  *   global fst_part, no_part, dom_grid, domgrid_no, fst_cycle, fin_l1dim,
- *          ovlim, rho_vir
+ *          ovlim, rho_vir, a (ahf_gridinfo)
  *   global_info  fst_part, no_part (the same array as global's, ordered by
  *          part.sfckey, src/main.c:341-356, 624-625)
  *   r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac, x_fac (the writers)
@@ -57,12 +57,28 @@ boolean assign_npart(gridls *cur_grid);
  * global.fst_part is linked to the domain grid (ll() was called). */
 boolean gen_AMRhierarchy(gridls **grid_list, int *no_grids);
 
+/* src/libahf/ahf.h:12 -- the spatially isolated refinements of the grids
+ * gen_AMRhierarchy built last (src/libahf/ahf_gridinfo.c:59-860), on the
+ * GPU: sets ahf.min_ref, ahf.no_grids and global.max_ovdens, colours every
+ * node of the AHF grids (node.force.colour), and allocates spatialRefIndex
+ * (one entry per colour started, plus one), numIsoRef and totnumIsoRef as
+ * the reference does.  Calls AHF's own calc_virial and calc_rho_vir
+ * (src/libutility/cosmology.c:47, 117; `double f(double a)`, which the
+ * program links) with global.a; reads simu.pmass, simu.med_weight,
+ * simu.boxsize, simu.Nth_ref.  AHF's log lines are not written. */
+extern info_ahf ahf;
+extern SRINDEX *spatialRefIndex;
+extern int *numIsoRef;
+extern int totnumIsoRef;
+void ahf_gridinfo(gridls *grid_list, int curgrid_no);
+
 /* src/libahf/ahf_halos.h:30 -- members ordered by distance from halo->pos */
 void sort_halo_particles(HALO *halo);
 
 /* src/libahf/ahf_halos.h:21 -- iterative removal of unbound members;
  * halo->ipart is replaced by a new malloc'd array, as in the reference.
- * Safe to call from several threads (calls are serialised on the GPU). */
+ * Safe to call from several threads: calls made together are combined into
+ * one batched GPU pass, and each returns when its own halo is done. */
 void rem_unbound(HALO *halo);
 
 /* src/libahf/ahf_halos.h:24 -- cut the radially sorted members at the

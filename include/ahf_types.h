@@ -9,7 +9,8 @@
  * These are the layouts AHF's own structs have under those flags
  * (src/tdef.h:33-77 particle, :107-137 node, :144-183 quads, :189-234
  * gridlist, :240-292 info_global, :296-309 tline, :315-411 param_simu,
- * :547-549 XYZ, :586-720 HALOPROFILE, :723-828 HALO).  A library built
+ * :547-549 XYZ, :586-720 HALOPROFILE, :723-828 HALO, :796-818 intXYZ, SRINDEX,
+ * info_ahf).  A library built
  * against this header can be linked into an AHF built with the same flags;
  * tests/test_layout.py compares every size and offset below with a probe
  * compiled against the reference headers.
@@ -245,6 +246,20 @@ typedef struct {
   int    refLev;
   int    numNodes;
 } HALO;
+
+/* ahf_gridinfo's outputs (src/tdef.h:796-818): the isolated refinement a
+ * colour names, and AHF's grid range */
+typedef struct { int x, y, z; } intXYZ;
+typedef struct {
+  int    refLevel;
+  int    isoRefIndex;
+  intXYZ periodic;
+} SRINDEX;
+typedef struct info_ahf {
+  int    no_grids;
+  int    min_ref;
+  time_t time;
+} info_ahf;
 
 #ifdef __cplusplus
 #define AHF_STATIC_ASSERT static_assert

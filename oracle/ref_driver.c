@@ -21,7 +21,6 @@
  *   ahf_ref unbind <snap> <halos.out> [--threads T] [--reps R]
  *   ahf_ref sort   <snap> <halos.out> [--threads T]   sort_halo_particles
  *   ahf_ref gridinfo <snap> <ginfo.out>   the AMR hierarchy, then ahf_gridinfo
- *                       (reference build only)
  *   ahf_ref construct <snap> <halos.out> [--threads T]
  *   ahf_ref gather <snap> <halos.out> [--threads T]  ahf_halos_sfc_gatherParts
  *   ahf_ref construct-sfc <snap> <halos.out> [--threads T]
@@ -46,6 +45,7 @@
  *   ahf_ref layout                                  struct sizes/offsets (JSON)
  */
 #define _POSIX_C_SOURCE 200809L
+#include <math.h>
 #include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -87,6 +87,7 @@
 param_simu simu;
 info_global global;
 global_info_t global_info;
+info_ahf ahf;
 double r_fac, v_fac, phi_fac, rho_fac, Hubble, m_fac, x_fac;
 #endif
 
@@ -102,6 +103,27 @@ static void die(const char *msg)
   fprintf(stderr, "ahf_ref: %s\n", msg);
   exit(2);
 }
+
+#ifndef AHF_REF_BUILD
+/* AHF's cosmology, which ahf_gridinfo calls, as AHF computes it for a run
+ * that names its virial overdensity (simu.UserDvir > 0) and normalises to
+ * the critical density: src/libutility/cosmology.c:29-53, 97-112, 117-123,
+ * with H0 = 100 and Grav, PI from src/param.h:45, 143 */
+double calc_virial(double a)
+{
+  (void)a;
+  if (!(simu.UserDvir > 0)) die("calc_virial: the harness needs simu.UserDvir > 0");
+  return simu.UserDvir;
+}
+
+double calc_rho_vir(double a)
+{
+  if (simu.UseRhoBack) die("calc_rho_vir: the harness normalises to the critical density only");
+  const double H = 100. * sqrt(simu.lambda0 * (1. - 1. / (a * a)) + simu.omega0 * (1. / (a * a * a) - 1. / (a * a)) +
+                               1. / (a * a));
+  return 3. * (H * H) / 8. / 3.14159265358979323846264338 / 4.3006485e-9;
+}
+#endif
 
 /* the parameter derivation of src/startrun.c:499-547 for what the path reads */
 static void setup_globals(const snap_data *s)
@@ -343,7 +365,6 @@ static void cmd_unbind(const snap_data *s, const char *out, int threads)
   fprintf(stdout, "{\"unbind_seconds\": %.6f, \"threads\": %d}\n", t, threads);
 }
 
-#ifdef AHF_REF_BUILD
 /* ------------------------------------------------------------ gridinfo --- */
 
 static int cmp_i32(const void *a, const void *b)
@@ -365,7 +386,9 @@ static void cmd_gridinfo(const snap_data *s, const char *out)
   simu.UserDvir = s->h.ovlim;
   simu.UseRhoBack = 0;
   simu.med_weight = 1.0;
+#ifdef AHF_REF_BUILD
   io.logfile = fopen("/dev/null", "w");
+#endif
   partptr fst = make_particles(s);
   gridls *grid_list;
   int no_grids;
@@ -433,7 +456,6 @@ static void cmd_gridinfo(const snap_data *s, const char *out)
   fprintf(stdout, "{\"gridinfo_seconds\": %.6f, \"min_ref\": %d, \"no_grids\": %d, \"totnumIsoRef\": %d}\n", t,
           ahf.min_ref, ahf.no_grids, totnumIsoRef);
 }
-#endif
 
 /* sort_halo_particles alone per halo (src/libahf/ahf_halos.c:5786-5894):
  * the radial order, equal distances in indexx's order */
@@ -887,6 +909,11 @@ static void cmd_layout(void)
   OFF(param_simu, timeline); OFF(param_simu, mean_dens); OFF(param_simu, boxsize);
   OFF(param_simu, pmass); OFF(param_simu, no_vpart); OFF(param_simu, no_part);
   OFF(param_simu, NGRID_MIN); OFF(param_simu, np_limit);
+  OFF(param_simu, UseRhoBack); OFF(param_simu, UserDvir); OFF(param_simu, omega0); OFF(param_simu, lambda0);
+  OFF(param_simu, med_weight); OFF(info_global, max_ovdens);
+  printf("\"sizeof.SRINDEX\": %zu, \"sizeof.info_ahf\": %zu, ", sizeof(SRINDEX), sizeof(info_ahf));
+  OFF(SRINDEX, refLevel); OFF(SRINDEX, isoRefIndex); OFF(SRINDEX, periodic);
+  OFF(info_ahf, no_grids); OFF(info_ahf, min_ref); OFF(info_ahf, time);
   printf("\"end\": 0}\n");
 }
 
@@ -916,11 +943,9 @@ int main(int argc, char **argv)
   } else if (!strcmp(argv[1], "unbind")) {
     if (argc < 4) die("unbind needs an output path");
     cmd_unbind(&s, argv[3], threads);
-#ifdef AHF_REF_BUILD
   } else if (!strcmp(argv[1], "gridinfo")) {
     if (argc < 4) die("gridinfo needs an output path");
     cmd_gridinfo(&s, argv[3]);
-#endif
   } else if (!strcmp(argv[1], "sort")) {
     if (argc < 4) die("sort needs an output path");
     cmd_sort(&s, argv[3], threads);

@@ -36,6 +36,11 @@ int main(void)
   OFF(param_simu, timeline); OFF(param_simu, mean_dens); OFF(param_simu, boxsize);
   OFF(param_simu, pmass); OFF(param_simu, no_vpart); OFF(param_simu, no_part);
   OFF(param_simu, NGRID_MIN); OFF(param_simu, np_limit);
+  OFF(param_simu, UseRhoBack); OFF(param_simu, UserDvir); OFF(param_simu, omega0); OFF(param_simu, lambda0);
+  OFF(param_simu, med_weight); OFF(info_global, max_ovdens);
+  printf("\"sizeof.SRINDEX\": %zu, \"sizeof.info_ahf\": %zu, ", sizeof(SRINDEX), sizeof(info_ahf));
+  OFF(SRINDEX, refLevel); OFF(SRINDEX, isoRefIndex); OFF(SRINDEX, periodic);
+  OFF(info_ahf, no_grids); OFF(info_ahf, min_ref); OFF(info_ahf, time);
   printf("\"end\": 0}\n");
   return 0;
 }

@@ -66,6 +66,48 @@ def test_dropin_amr_fuzz(oracle, drv, tmp_path, seed):
     _amr_case(oracle, drv, snap, tmp_path, f"fuzz{seed}")
 
 
+@pytest.mark.parametrize("cfg,side,seed,how", [(1, 64, 1001, "one"), (0, 48, 1000, "dom"), (3, 48, 5, "hdr"),
+                                              (4, 48, 1004, "one")])
+def test_dropin_gridinfo(oracle, drv, snapgen, tmp_path, cfg, side, seed, how):
+    """ahf_gridinfo through AHF's prototype after gen_AMRhierarchy: min_ref,
+    no_grids, max_ovdens, numIsoRef, totnumIsoRef, every node's colour in
+    AHF's traversal order and the spatialRefIndex entry of every colour the
+    nodes carry, as the oracle (pinned to the reference) gives them"""
+    from helpers import gridinfo_setup
+    snap = str(tmp_path / "gi.snap")
+    assert snapgen.ahf_snapgen_write(cfg, side, seed, snap.encode()) == 0
+    ovlim, rho, pmass, _ = gridinfo_setup(snap, how)
+    o_out, d_out = str(tmp_path / "o.ginf"), str(tmp_path / "d.ginf")
+    assert oracle.oracle_gridinfo_dump(snap.encode(), o_out.encode(), ovlim, rho, pmass) >= 0
+    _run([drv, "gridinfo", snap, d_out])
+    a, b = dumps.read_gridinfo(d_out), dumps.read_gridinfo(o_out)
+    assert (a.min_ref, a.no_grids, a.tot_iso, a.max_ovdens) == (b.min_ref, b.no_grids, b.tot_iso, b.max_ovdens)
+    assert a.no_grids >= 1
+    assert np.array_equal(a.num_iso, b.num_iso)
+    for g, (x, y) in enumerate(zip(a.colours, b.colours)):
+        assert np.array_equal(x, y), f"grid {g}: node colours differ"
+    assert np.array_equal(a.refs, b.refs)
+
+
+@pytest.mark.parametrize("seed", range(310, 314))
+def test_dropin_gridinfo_fuzz(oracle, drv, tmp_path, seed):
+    from helpers import gridinfo_setup
+    snap = str(tmp_path / "f.snap")
+    fuzz_snapshot(snap, seed)
+    ovlim, rho, pmass, _ = gridinfo_setup(snap, ("dom", "one")[seed % 2])
+    o_out, d_out = str(tmp_path / "o.ginf"), str(tmp_path / "d.ginf")
+    rc = oracle.oracle_gridinfo_dump(snap.encode(), o_out.encode(), ovlim, rho, pmass)
+    if rc == -100:
+        pytest.skip("the reference meets undefined behaviour on this input")
+    assert rc >= 0
+    _run([drv, "gridinfo", snap, d_out])
+    a, b = dumps.read_gridinfo(d_out), dumps.read_gridinfo(o_out)
+    assert (a.min_ref, a.no_grids, a.tot_iso, a.max_ovdens) == (b.min_ref, b.no_grids, b.tot_iso, b.max_ovdens)
+    assert np.array_equal(a.num_iso, b.num_iso)
+    assert all(np.array_equal(x, y) for x, y in zip(a.colours, b.colours))
+    assert np.array_equal(a.refs, b.refs)
+
+
 def _halo_match(ref, got, tag):
     assert np.array_equal(ref.npart, got.npart), f"{tag}: member counts differ"
     bad = [h for h in range(len(ref.ipart)) if not np.array_equal(ref.ipart[h], got.ipart[h])]
