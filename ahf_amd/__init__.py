@@ -131,6 +131,7 @@ def load():
     lib.ahfgpu_set_profile_split.argtypes = [P, ctypes.c_int64]
     lib.ahfgpu_set_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
     lib.ahfgpu_get_schedule.argtypes = [P, ctypes.POINTER(Schedule)]
+    lib.ahfgpu_set_timing.argtypes = [P, ctypes.c_int]
     lib.ahfgpu_gridinfo.argtypes = [P, ctypes.POINTER(GridinfoParams), ctypes.POINTER(GridinfoInfo)]
     lib.ahfgpu_gridinfo_results.argtypes = [P, ctypes.c_int, P, P]
     _lib = lib
@@ -230,6 +231,10 @@ class Context:
 
     def amr_dump(self, path):
         self._check(self.lib.ahfgpu_amr_dump(self.h, path.encode()))
+
+    def set_timing(self, on):
+        """stage timers on (the default) or off (the timings then read 0)"""
+        self._check(self.lib.ahfgpu_set_timing(self.h, 1 if on else 0))
 
     def last_timing(self):
         a, u, g = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -250,6 +250,13 @@ extern "C" int ahfgpu_set_gather_mode(ahfgpu_ctx *c, int mode)
   return 0;
 }
 
+extern "C" int ahfgpu_set_timing(ahfgpu_ctx *c, int on)
+{
+  if (!c) return AHFGPU_EINVAL;
+  c->timing = on != 0;
+  return 0;
+}
+
 extern "C" int ahfgpu_last_timing(ahfgpu_ctx *c, double *amr_ms, double *unbind_ms, double *gather_ms,
                                   int64_t *gather_launches)
 {

@@ -55,9 +55,11 @@ __device__ inline T block_excl_scan(T v, T *total)
 #define AHF_SCAN_ITEMS 8
 #define AHF_SCAN_TILE (AHF_BLOCK * AHF_SCAN_ITEMS)
 
+/* per tile: the exclusive scan and the tile's sum (into tile_sum, or into
+ * *total when the whole input is one tile) */
 template <typename Tin, typename Tout>
 __global__ void k_scan_tiles(const Tin *__restrict__ in, Tout *__restrict__ out, Tout *__restrict__ tile_sum,
-                             int64_t n)
+                             int64_t n, Tout *__restrict__ total)
 {
   const int64_t base = (int64_t)blockIdx.x * AHF_SCAN_TILE + (int64_t)threadIdx.x * AHF_SCAN_ITEMS;
   Tout v[AHF_SCAN_ITEMS];
@@ -76,7 +78,12 @@ __global__ void k_scan_tiles(const Tin *__restrict__ in, Tout *__restrict__ out,
     if (k < n) out[k] = ex;
     ex += v[i];
   }
-  if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+  if (threadIdx.x == 0) {
+    if (total)
+      *total = tot;
+    else
+      tile_sum[blockIdx.x] = tot;
+  }
 }
 
 template <typename T>
@@ -129,7 +136,12 @@ inline void exclusive_scan(const Tin *in, Tout *out, int64_t n, Tout *tmp, hipSt
     return;
   }
   const int64_t tiles = (n + AHF_SCAN_TILE - 1) / AHF_SCAN_TILE;
-  hipLaunchKernelGGL((k_scan_tiles<Tin, Tout>), dim3((unsigned)tiles), dim3(AHF_BLOCK), 0, st, in, out, tmp, n);
+  if (tiles == 1) {
+    hipLaunchKernelGGL((k_scan_tiles<Tin, Tout>), dim3(1), dim3(AHF_BLOCK), 0, st, in, out, tmp, n, out + n);
+    return;
+  }
+  hipLaunchKernelGGL((k_scan_tiles<Tin, Tout>), dim3((unsigned)tiles), dim3(AHF_BLOCK), 0, st, in, out, tmp, n,
+                     (Tout *)nullptr);
   hipLaunchKernelGGL((k_scan_single<Tout>), dim3(1), dim3(AHF_BLOCK), 0, st, tmp, tiles, out + n);
   if (tiles > 1) hipLaunchKernelGGL((k_scan_add<Tout>), dim3((unsigned)tiles), dim3(AHF_BLOCK), 0, st, out, tmp, n);
 }

@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -50,11 +51,16 @@ static void *xcalloc(size_t n, size_t sz)
   return p;
 }
 
+/* AHF's functions read no stage timings: the drop-in's context runs
+ * without them */
 static ahfgpu_ctx *ctx_get(void)
 {
-  if (!g_ctx && ahfgpu_create(&g_ctx, g_device)) {
-    g_ctx = NULL;
-    fatal("cannot create a GPU context");
+  if (!g_ctx) {
+    if (ahfgpu_create(&g_ctx, g_device)) {
+      g_ctx = NULL;
+      fatal("cannot create a GPU context");
+    }
+    ahfgpu_set_timing(g_ctx, 0);
   }
   return g_ctx;
 }
@@ -71,7 +77,10 @@ int ahf_gpu_init(int device)
   g_res_parts = NULL;
   g_res_n = 0;
   if (!g_ctx) rc = ahfgpu_create(&g_ctx, device);
-  if (rc) g_ctx = NULL;
+  if (rc)
+    g_ctx = NULL;
+  else
+    ahfgpu_set_timing(g_ctx, 0);
   pthread_mutex_unlock(&g_lock);
   return rc;
 }
@@ -744,9 +753,16 @@ typedef struct halo_req {
 
 static pthread_mutex_t q_lock = PTHREAD_MUTEX_INITIALIZER;
 static pthread_cond_t q_cv = PTHREAD_COND_INITIALIZER;
+static pthread_cond_t q_arrive = PTHREAD_COND_INITIALIZER;
 static halo_req *q_head = NULL, **q_tail = &q_head;
-static int q_running = 0;
+static int q_running = 0, q_len = 0;
 static uint64_t q_batches = 0, q_requests = 0;
+/* how long a new batch waits for the rest of the caller's OpenMP team to
+ * queue its next request (each thread has at most one request queued, so
+ * the team size bounds a batch); every arrival restarts the wait, and the
+ * whole wait is capped */
+#define GATHER_IDLE_US 40
+#define GATHER_MAX_US 200
 
 static int run_kind(int kind, HALO **hp, long unsigned n)
 {
@@ -782,22 +798,60 @@ static void run_batch(halo_req *batch)
   free(hp);
 }
 
+static void deadline_after(struct timespec *t, const struct timespec *from, long us)
+{
+  *t = *from;
+  t->tv_nsec += us * 1000L;
+  while (t->tv_nsec >= 1000000000L) {
+    t->tv_nsec -= 1000000000L;
+    t->tv_sec++;
+  }
+}
+
+static int ts_before(const struct timespec *a, const struct timespec *b)
+{
+  return a->tv_sec < b->tv_sec || (a->tv_sec == b->tv_sec && a->tv_nsec < b->tv_nsec);
+}
+
 static int submit(HALO *h, int kind)
 {
   halo_req req = {h, kind, 0, 0, NULL};
+#ifdef _OPENMP
+  const int team = omp_in_parallel() ? omp_get_num_threads() : 1;
+#else
+  const int team = 1;
+#endif
   pthread_mutex_lock(&q_lock);
   *q_tail = &req;
   q_tail = &req.next;
+  q_len++;
+  pthread_cond_signal(&q_arrive);
   while (!req.done) {
     if (q_running) {
       pthread_cond_wait(&q_cv, &q_lock);
       continue;
     }
-    /* no batch running: run every queued request, this one included */
+    /* no batch running: this thread runs the next one.  It first lets the
+     * rest of its team queue their requests, then takes every queued
+     * request, this one included */
+    q_running = 1;
+    if (team > 1 && q_len < team) {
+      struct timespec now, idle, cap;
+      clock_gettime(CLOCK_REALTIME, &now);
+      deadline_after(&cap, &now, GATHER_MAX_US);
+      while (q_len < team) {
+        deadline_after(&idle, &now, GATHER_IDLE_US);
+        const int seen = q_len;
+        if (pthread_cond_timedwait(&q_arrive, &q_lock, ts_before(&idle, &cap) ? &idle : &cap) != 0 && q_len == seen)
+          break;
+        clock_gettime(CLOCK_REALTIME, &now);
+        if (!ts_before(&now, &cap)) break;
+      }
+    }
     halo_req *batch = q_head;
     q_head = NULL;
     q_tail = &q_head;
-    q_running = 1;
+    q_len = 0;
     pthread_mutex_unlock(&q_lock);
     run_batch(batch);
     pthread_mutex_lock(&q_lock);

@@ -164,6 +164,7 @@ struct ahfgpu_ctx {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_next = 0;
   std::vector<TimedLaunch> timed;
+  bool timing = true; /* stage timers and the build/unbind event pairs */
   double amr_ms = 0, unbind_ms = 0, gather_ms = 0, gather_bytes = 0;
   int64_t gather_launches = 0;
   double t_ms[TIMER_COUNT] = {0}, t_bytes[TIMER_COUNT] = {0};
@@ -196,12 +197,14 @@ struct ahfgpu_ctx {
   }
   hipEvent_t timer_begin()
   {
+    if (!timing) return nullptr;
     hipEvent_t e = take_event();
     hipEventRecord(e, st);
     return e;
   }
   void timer_end(hipEvent_t a, int cat, double bytes)
   {
+    if (!a) return;
     hipEvent_t b = take_event();
     hipEventRecord(b, st);
     timed.push_back(TimedLaunch{a, b, cat, bytes});

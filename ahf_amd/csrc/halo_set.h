@@ -139,6 +139,11 @@ static inline char *pinned_grow(char **p, size_t *cap, size_t bytes)
   return *p;
 }
 
+/* layout of the results (device block and pinned staging alike): the
+ * per-halo records, then member buffers a and b, each 256-byte aligned */
+static inline size_t res_off_a(int64_t nh) { return align256((size_t)nh * sizeof(HaloOut)); }
+static inline size_t res_off_b(int64_t nh, int64_t ne) { return res_off_a(nh) + align256((size_t)ne * 4); }
+
 /* pinned staging for the results (read back after the stream is idle) */
 static inline char *halo_set_res(HaloSet *s, size_t bytes) { return pinned_grow(&s->res, &s->res_cap, bytes); }
 

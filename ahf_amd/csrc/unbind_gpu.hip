@@ -253,10 +253,9 @@ __device__ inline D4 block_excl_scan4(D4 v, D4 *tot, D4 *sh /* [AHF_WAVES] */)
   return ex;
 }
 
-/* exact replay helper: AHF's indexx on n <= 64 values (1-based) */
-__device__ void dev_indexx(int n, const double *arr1, int *indx1)
+/* AHF's indexx on n <= 64 values (1-based), with a 24-entry stack */
+__device__ void dev_indexx(int n, const double *arr1, int *indx1, int *stack)
 {
-  int stack[24];
   int top = 0, lo = 1, hi = n;
   for (int j = 1; j <= n; j++) indx1[j] = j;
   if (n == 0) return;
@@ -844,25 +843,36 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
 /* the first iteration's seed of every halo (:3441-3470): the median |v|^2
  * (FP32) of the first no_vbulk members of the sorted list, by AHF's indexx;
  * one thread per halo */
-__global__ void k_seed(UnbindConst K, const int64_t *__restrict__ off, int64_t nh, const uint32_t *__restrict__ a,
-                       const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
-                       int64_t *__restrict__ seed)
+/* one thread per halo, its indexx work arrays in LDS (as private arrays,
+ * indexed at run time, they would live in scratch memory) */
+#define AHF_SEED_BS 64
+__global__ void __launch_bounds__(AHF_SEED_BS) k_seed(UnbindConst K, const int64_t *__restrict__ off, int64_t nh,
+                                                      const uint32_t *__restrict__ a, const float *__restrict__ vx,
+                                                      const float *__restrict__ vy, const float *__restrict__ vz,
+                                                      int64_t *__restrict__ seed)
 {
-  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double s_m2[AHF_SEED_BS * 65];
+  __shared__ int s_ix[AHF_SEED_BS * 65];
+  __shared__ int s_stack[AHF_SEED_BS * 24];
+  const int64_t h = (int64_t)blockIdx.x * AHF_SEED_BS + threadIdx.x;
   if (h >= nh) return;
   const int64_t o = off[h], n = off[h + 1] - o;
   seed[h] = 0;
   if (n < K.minpart || K.no_vbulk < 1) return;
-  double m2[65];
-  int ix[65];
+  double *m2 = s_m2 + 65 * threadIdx.x;
+  int *ix = s_ix + 65 * threadIdx.x;
   const int nv = K.no_vbulk;
   for (int j = 0; j < nv; j++) {
     const uint32_t p = a[o + j];
     const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
     m2[j + 1] = (double)v2;
   }
-  dev_indexx(nv, m2, ix);
+  dev_indexx(nv, m2, ix, s_stack + 24 * threadIdx.x);
   seed[h] = ix[nv / 2] - 1;
+}
+static inline unsigned seed_blocks(int64_t nh)
+{
+  return (unsigned)((nh + AHF_SEED_BS - 1) / AHF_SEED_BS);
 }
 
 /* haloes of at least this many members go to 512-thread workgroups */
@@ -1660,12 +1670,14 @@ static size_t tie_work_bytes(int64_t T, int64_t m)
  * and scan_tmp_elems(nh) words of scratch; take(bytes) hands out device
  * memory for the replay (nullptr when out).  *n_tie: the haloes re-sorted.
  * Reads back the halo count, the member count and one segment count per
- * level.
+ * level.  With defer, only the detection runs: the count of haloes with
+ * ties goes to c->h_words[2] once the stream reaches it, *n_tie is -1, and
+ * the caller runs this again without defer if that count is not zero.
  */
 template <class Take>
 static int nr_tie_order(ahfgpu_ctx *c, hipStream_t st, const int64_t *off, int64_t nh, int64_t ne, int32_t minpart,
                         const uint32_t *ip_in, const double *cen, uint32_t *sorted, uint8_t *tie, int32_t *tpos,
-                        int32_t *tmp32h, Take take, int64_t *n_tie)
+                        int32_t *tmp32h, Take take, int64_t *n_tie, bool defer = false)
 {
   *n_tie = 0;
   if (ne < 2 || nh < 1) return 0;
@@ -1675,8 +1687,13 @@ static int nr_tie_order(ahfgpu_ctx *c, hipStream_t st, const int64_t *off, int64
   exclusive_scan<uint8_t, int32_t>(tie, tpos, nh, tmp32h, st);
   /* the counts come back through the context's pinned words */
   int32_t *hT = c->h_words + 2;
-  if (hipMemcpyAsync(hT, tpos + nh, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
+  if (hipMemcpyAsync(hT, tpos + nh, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess)
+    return c->fail(AHFGPU_EHIP, "radial sort: tie count: %s", hipGetErrorString(hipGetLastError()));
+  if (defer) {
+    *n_tie = -1;
+    return 0;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "radial sort: tie count: %s", hipGetErrorString(hipGetLastError()));
   const int32_t T = *hT;
   if (T == 0) return 0;
@@ -1970,13 +1987,19 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   uint32_t *v0 = A.get<uint32_t>(ne), *v1 = A.get<uint32_t>(ne);
   uint32_t *hk0 = A.get<uint32_t>(ne), *hk1 = A.get<uint32_t>(ne);
   uint32_t *rtmp = A.get<uint32_t>(radix_tmp_u32<uint64_t>(ne));
-  s->buf_a = A.get<uint32_t>(ne);
-  s->buf_b = A.get<uint32_t>(ne);
+  /* the per-halo records and both member buffers in one block, laid out as
+   * the results' pinned staging is, so that one copy reads them back */
+  {
+    const size_t o_a = res_off_a(nh), o_b = res_off_b(nh, ne);
+    char *blk = (char *)A.raw(o_b + (size_t)ne * 4 + 16);
+    s->out = blk ? (HaloOut *)blk : nullptr;
+    s->buf_a = blk ? (uint32_t *)(blk + o_a) : nullptr;
+    s->buf_b = blk ? (uint32_t *)(blk + o_b) : nullptr;
+  }
   double *dist = A.get<double>(ne), *phi = A.get<double>(ne);
   uint8_t *st = A.get<uint8_t>(ne);
   Staged g = {A.get<double>(ne), A.get<double>(ne), A.get<double>(ne),
               A.get<float>(ne), A.get<float>(ne), A.get<float>(ne)};
-  s->out = A.get<HaloOut>(nh);
   int32_t *next = A.get<int32_t>(3);
   int64_t *seed = A.get<int64_t>(nh);
   if (!k0 || !k1 || !v0 || !v1 || !hk0 || !hk1 || !rtmp || !s->buf_a || !s->buf_b || !dist || !phi || !st ||
@@ -1985,7 +2008,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   c->timers_reset(TIMER_USORT, TIMER_UNBIND);
   c->timed.clear();
   c->ev_next = 0;
-  hipEventRecord(c->ev_u0, c->st);
+  if (c->timing) hipEventRecord(c->ev_u0, c->st);
   hipEvent_t es = c->timer_begin();
   const int run = mode & 3;
   int dev_cu = 256;
@@ -2010,6 +2033,8 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   const bool sort_only = (mode & AHFGPU_UNBIND_SORT_ONLY) != 0;
   const bool no_sort = (mode & (AHFGPU_UNBIND_NO_SORT | AHFGPU_UNBIND_RVIR_ONLY)) != 0;
   int64_t n_tie = 0;
+  uint8_t *tie = nullptr;
+  int32_t *tpos = nullptr, *tmp32h = nullptr;
   if (ne > 0 && no_sort) {
     hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->ip_in, ne, s->buf_a);
   } else if (ne > 0) {
@@ -2065,12 +2090,15 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
       hipLaunchKernelGGL(k_gather_members_sel, dim3(nb_of(nes)), dim3(AHF_BLOCK), 0, c->st, d_bo, d_bh, nsel, nes,
                          s->off, vsorted, s->ip_in, s->buf_a);
     }
-    /* haloes with equal distances: indexx's order of them */
-    uint8_t *tie = A.get<uint8_t>(nh);
-    int32_t *tpos = A.get<int32_t>(nh + 1), *tmp32h = A.get<int32_t>(scan_tmp_elems(nh) + 16);
+    /* haloes with equal distances: indexx's order of them (the sort alone
+     * reads the count with its results and re-sorts only when it is not
+     * zero) */
+    tie = A.get<uint8_t>(nh);
+    tpos = A.get<int32_t>(nh + 1);
+    tmp32h = A.get<int32_t>(scan_tmp_elems(nh) + 16);
     if (!tie || !tpos || !tmp32h) return c->fail(AHFGPU_ENOMEM, "device arena exhausted (sort)");
     const int rc = nr_tie_order(c, c->st, s->off, nh, ne, p->minpart, s->ip_in, s->cen, s->buf_a, tie, tpos, tmp32h,
-                                [&](size_t bytes) { return A.raw(bytes); }, &n_tie);
+                                [&](size_t bytes) { return A.raw(bytes); }, &n_tie, sort_only);
     if (rc) return rc;
   }
   c->timer_end(es, TIMER_USORT, 2.0 * 12.0 * (double)ne + 12.0 * (double)ne);
@@ -2078,7 +2106,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   if (sort_only) {
     if (nh) hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, s->out);
   } else if (nh && K.unbind) {
-    hipLaunchKernelGGL(k_seed, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, K, s->off, nh, s->buf_a, c->vx, c->vy,
+    hipLaunchKernelGGL(k_seed, dim3(seed_blocks(nh)), dim3(AHF_SEED_BS), 0, c->st, K, s->off, nh, s->buf_a, c->vx, c->vy,
                        c->vz, seed);
   }
   if (sort_only) {
@@ -2152,12 +2180,23 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     }
   }
   c->timer_end(eu, TIMER_UNBIND, 0.0);
-  hipEventRecord(c->ev_u1, c->st);
+  if (c->timing) hipEventRecord(c->ev_u1, c->st);
   if (hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "unbinding: %s", hipGetErrorString(hipGetLastError()));
+  if (n_tie < 0) {
+    n_tie = 0;
+    if (c->h_words[2] > 0) {
+      /* the records k_sorted_out wrote do not depend on the order */
+      const int rc = nr_tie_order(c, c->st, s->off, nh, ne, p->minpart, s->ip_in, s->cen, s->buf_a, tie, tpos, tmp32h,
+                                  [&](size_t bytes) { return A.raw(bytes); }, &n_tie);
+      if (rc) return rc;
+      if (hipStreamSynchronize(c->st) != hipSuccess)
+        return c->fail(AHFGPU_EHIP, "radial sort: %s", hipGetErrorString(hipGetLastError()));
+    }
+  }
   c->timers_collect();
   float ms = 0.f;
-  hipEventElapsedTime(&ms, c->ev_u0, c->ev_u1);
+  if (c->timing) hipEventElapsedTime(&ms, c->ev_u0, c->ev_u1);
   c->unbind_ms = ms;
   s->done = 1;
   if (info) {
@@ -2185,14 +2224,19 @@ extern "C" int ahfgpu_unbind_results(ahfgpu_ctx *c, uint64_t *npart, double *pro
   if (!s || !s->done) return c->fail(AHFGPU_EINVAL, "no unbinding result");
   const int64_t nh = s->nh, ne = s->ne;
   if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
-  /* one pinned block: the per-halo records, then both member buffers */
-  const size_t o_a = ((size_t)nh * sizeof(HaloOut) + 255) & ~(size_t)255, o_b = o_a + (((size_t)ne * 4 + 255) & ~(size_t)255);
+  /* one pinned block: the per-halo records, then both member buffers,
+   * read back in one copy when the device holds them the same way */
+  const size_t o_a = res_off_a(nh), o_b = res_off_b(nh, ne);
   char *R = halo_set_res(s, o_b + (size_t)ne * 4 + 16);
   if (!R) return c->fail(AHFGPU_ENOMEM, "pinned staging of the results");
-  if (nh) hipMemcpyAsync(R, s->out, (size_t)nh * sizeof(HaloOut), hipMemcpyDeviceToHost, c->st);
-  if (ne) {
-    hipMemcpyAsync(R + o_a, s->buf_a, (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
-    hipMemcpyAsync(R + o_b, s->buf_b, (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
+  if ((char *)s->buf_a == (char *)s->out + o_a && (char *)s->buf_b == (char *)s->out + o_b) {
+    hipMemcpyAsync(R, s->out, o_b + (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
+  } else {
+    if (nh) hipMemcpyAsync(R, s->out, (size_t)nh * sizeof(HaloOut), hipMemcpyDeviceToHost, c->st);
+    if (ne) {
+      hipMemcpyAsync(R + o_a, s->buf_a, (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
+      hipMemcpyAsync(R + o_b, s->buf_b, (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
+    }
   }
   if (hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "unbinding results: %s", hipGetErrorString(hipGetLastError()));
@@ -2553,7 +2597,7 @@ extern "C" int ahfgpu_unbind_noalloc(ahfgpu_ctx *c, const ahfgpu_unbind_params *
     hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, off, nh, hout);
   } else {
     if (K.unbind)
-      hipLaunchKernelGGL(k_seed, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, K, off, nh, buf_a, c->vx, c->vy, c->vz,
+      hipLaunchKernelGGL(k_seed, dim3(seed_blocks(nh)), dim3(AHF_SEED_BS), 0, st, K, off, nh, buf_a, c->vx, c->vy, c->vz,
                          seed);
     int dev_cu = 256;
     hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);

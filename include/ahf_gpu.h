@@ -322,6 +322,12 @@ typedef struct {
 int ahfgpu_set_schedule(ahfgpu_ctx *ctx, const ahfgpu_schedule *s);
 int ahfgpu_get_schedule(ahfgpu_ctx *ctx, ahfgpu_schedule *s);
 
+/* No counterpart in the reference: the stage timers (HIP events around
+ * each stage, read by ahfgpu_last_timing and ahfgpu_timer_get) on (1, the
+ * default) or off (0: the timings read 0), for callers that make many
+ * small calls and do not read them */
+int ahfgpu_set_timing(ahfgpu_ctx *ctx, int on);
+
 /* device milliseconds of the last build / unbind (HIP events on the
  * library's stream), and of the dominant kernel of the last build */
 int ahfgpu_last_timing(ahfgpu_ctx *ctx, double *amr_ms, double *unbind_ms, double *gather_ms,

@@ -241,3 +241,24 @@ def test_schedule_file(tmp_path, monkeypatch):
         assert sc["split_min"] == 12345 and sc["kmax_big"] == 500 and sc["march_zn"] == 8 and sc["graph_batch"] == 3
     finally:
         c.close()
+
+
+def test_timing_off(oracle, snapgen, ctx, tmp_path):
+    """with the stage timers off the haloes are the same and the timings
+    read 0; on again, they are timed"""
+    snap = str(tmp_path / "t.snap")
+    assert snapgen.ahf_snapgen_write(1, 64, 1001, snap.encode()) == 0
+    out = str(tmp_path / "o.halo")
+    assert oracle.oracle_unbind_dump(snap.encode(), out.encode(), 4, 0) == 0
+    ref = dumps.read_halos(out)
+    s = dumps.read_snapshot(snap)
+    try:
+        ctx.set_timing(False)
+        _, (npart, props, lists) = _run_gpu(ctx, s, 0)
+        _compare(ref, npart, props, lists, "timing off")
+        assert ctx.last_timing()["unbind_ms"] == 0.0
+    finally:
+        ctx.set_timing(True)
+    _, (npart, props, lists) = _run_gpu(ctx, s, 0)
+    _compare(ref, npart, props, lists, "timing on")
+    assert ctx.last_timing()["unbind_ms"] > 0.0
