@@ -753,16 +753,21 @@ typedef struct halo_req {
 
 static pthread_mutex_t q_lock = PTHREAD_MUTEX_INITIALIZER;
 static pthread_cond_t q_cv = PTHREAD_COND_INITIALIZER;
-static pthread_cond_t q_arrive = PTHREAD_COND_INITIALIZER;
 static halo_req *q_head = NULL, **q_tail = &q_head;
+/* q_running and q_len change under q_lock and are also read without it
+ * (atomically) by threads that spin */
 static int q_running = 0, q_len = 0;
 static uint64_t q_batches = 0, q_requests = 0;
 /* how long a new batch waits for the rest of the caller's OpenMP team to
  * queue its next request (each thread has at most one request queued, so
- * the team size bounds a batch); every arrival restarts the wait, and the
- * whole wait is capped */
+ * the team size bounds a batch): until no request has arrived for
+ * GATHER_IDLE_US, at most GATHER_MAX_US.  A thread whose request is in a
+ * running batch spins on its own done flag for up to WAIT_SPIN_US (a pass
+ * takes a few hundred microseconds; a woken sleeper takes tens to run
+ * again), then sleeps until the batch ends. */
 #define GATHER_IDLE_US 40
 #define GATHER_MAX_US 200
+#define WAIT_SPIN_US 4000
 
 static int run_kind(int kind, HALO **hp, long unsigned n)
 {
@@ -798,19 +803,18 @@ static void run_batch(halo_req *batch)
   free(hp);
 }
 
-static void deadline_after(struct timespec *t, const struct timespec *from, long us)
+static inline void cpu_relax(void)
 {
-  *t = *from;
-  t->tv_nsec += us * 1000L;
-  while (t->tv_nsec >= 1000000000L) {
-    t->tv_nsec -= 1000000000L;
-    t->tv_sec++;
-  }
+#if defined(__x86_64__) || defined(__i386__)
+  __builtin_ia32_pause();
+#endif
 }
 
-static int ts_before(const struct timespec *a, const struct timespec *b)
+static int64_t mono_us(void)
 {
-  return a->tv_sec < b->tv_sec || (a->tv_sec == b->tv_sec && a->tv_nsec < b->tv_nsec);
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (int64_t)t.tv_sec * 1000000 + t.tv_nsec / 1000;
 }
 
 static int submit(HALO *h, int kind)
@@ -824,42 +828,63 @@ static int submit(HALO *h, int kind)
   pthread_mutex_lock(&q_lock);
   *q_tail = &req;
   q_tail = &req.next;
-  q_len++;
-  pthread_cond_signal(&q_arrive);
-  while (!req.done) {
+  __atomic_store_n(&q_len, q_len + 1, __ATOMIC_RELEASE);
+  while (!__atomic_load_n(&req.done, __ATOMIC_ACQUIRE)) {
     if (q_running) {
-      pthread_cond_wait(&q_cv, &q_lock);
+      /* a batch runs: spin on this request without the lock until it is
+       * done or the batch ends, then sleep if the spin ran out */
+      pthread_mutex_unlock(&q_lock);
+      const int64_t t0 = mono_us();
+      int spun_out = 0;
+      for (unsigned k = 1; !__atomic_load_n(&req.done, __ATOMIC_ACQUIRE) &&
+                           __atomic_load_n(&q_running, __ATOMIC_ACQUIRE);
+           k++) {
+        cpu_relax();
+        if ((k & 255) == 0 && mono_us() - t0 > WAIT_SPIN_US) {
+          spun_out = 1;
+          break;
+        }
+      }
+      pthread_mutex_lock(&q_lock);
+      while (spun_out && q_running && !__atomic_load_n(&req.done, __ATOMIC_ACQUIRE))
+        pthread_cond_wait(&q_cv, &q_lock);
       continue;
     }
     /* no batch running: this thread runs the next one.  It first lets the
      * rest of its team queue their requests, then takes every queued
      * request, this one included */
-    q_running = 1;
+    __atomic_store_n(&q_running, 1, __ATOMIC_RELEASE);
     if (team > 1 && q_len < team) {
-      struct timespec now, idle, cap;
-      clock_gettime(CLOCK_REALTIME, &now);
-      deadline_after(&cap, &now, GATHER_MAX_US);
-      while (q_len < team) {
-        deadline_after(&idle, &now, GATHER_IDLE_US);
-        const int seen = q_len;
-        if (pthread_cond_timedwait(&q_arrive, &q_lock, ts_before(&idle, &cap) ? &idle : &cap) != 0 && q_len == seen)
+      pthread_mutex_unlock(&q_lock);
+      const int64_t t0 = mono_us();
+      int64_t last = t0;
+      int seen = __atomic_load_n(&q_len, __ATOMIC_ACQUIRE);
+      for (;;) {
+        const int len = __atomic_load_n(&q_len, __ATOMIC_ACQUIRE);
+        if (len >= team) break;
+        const int64_t now = mono_us();
+        if (len != seen) {
+          seen = len;
+          last = now;
+        } else if (now - last > GATHER_IDLE_US || now - t0 > GATHER_MAX_US) {
           break;
-        clock_gettime(CLOCK_REALTIME, &now);
-        if (!ts_before(&now, &cap)) break;
+        }
+        cpu_relax();
       }
+      pthread_mutex_lock(&q_lock);
     }
     halo_req *batch = q_head;
     q_head = NULL;
     q_tail = &q_head;
-    q_len = 0;
+    __atomic_store_n(&q_len, 0, __ATOMIC_RELEASE);
     pthread_mutex_unlock(&q_lock);
     run_batch(batch);
     pthread_mutex_lock(&q_lock);
     for (halo_req *r = batch, *nx; r; r = nx) {
       nx = r->next; /* r lives on its caller's stack until done is seen */
-      r->done = 1;
+      __atomic_store_n(&r->done, 1, __ATOMIC_RELEASE);
     }
-    q_running = 0;
+    __atomic_store_n(&q_running, 0, __ATOMIC_RELEASE);
     pthread_cond_broadcast(&q_cv);
   }
   pthread_mutex_unlock(&q_lock);

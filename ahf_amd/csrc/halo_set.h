@@ -40,6 +40,7 @@ struct HaloSet {
   HaloOut *out = nullptr;
   uint32_t *buf_a = nullptr, *buf_b = nullptr;
   int done = 0;
+  int res_staged = 0; /* the results are in `res` already (read back with the unbinding's own synchronisation) */
   char *dev = nullptr, *host = nullptr;
   size_t cap = 0;
   char *res = nullptr; /* pinned staging of the results */
@@ -75,6 +76,7 @@ static inline int halo_set_reserve(ahfgpu_ctx *c, int64_t nh, int64_t ne, HaloSe
   if (!s) s = new HaloSet();
   c->halo_set = nullptr;
   s->done = 0;
+  s->res_staged = 0;
   s->out = nullptr;
   s->buf_a = s->buf_b = nullptr;
   s->nh = nh;

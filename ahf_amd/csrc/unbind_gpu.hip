@@ -843,36 +843,40 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
 /* the first iteration's seed of every halo (:3441-3470): the median |v|^2
  * (FP32) of the first no_vbulk members of the sorted list, by AHF's indexx;
  * one thread per halo */
-/* one thread per halo, its indexx work arrays in LDS (as private arrays,
- * indexed at run time, they would live in scratch memory) */
+/* one wavefront per halo (workgroups stride over the haloes): lane j
+ * loads member j's |v|^2, lane 0 runs indexx on them in LDS (no_vbulk <= 64
+ * values) */
 #define AHF_SEED_BS 64
 __global__ void __launch_bounds__(AHF_SEED_BS) k_seed(UnbindConst K, const int64_t *__restrict__ off, int64_t nh,
                                                       const uint32_t *__restrict__ a, const float *__restrict__ vx,
                                                       const float *__restrict__ vy, const float *__restrict__ vz,
                                                       int64_t *__restrict__ seed)
 {
-  __shared__ double s_m2[AHF_SEED_BS * 65];
-  __shared__ int s_ix[AHF_SEED_BS * 65];
-  __shared__ int s_stack[AHF_SEED_BS * 24];
-  const int64_t h = (int64_t)blockIdx.x * AHF_SEED_BS + threadIdx.x;
-  if (h >= nh) return;
-  const int64_t o = off[h], n = off[h + 1] - o;
-  seed[h] = 0;
-  if (n < K.minpart || K.no_vbulk < 1) return;
-  double *m2 = s_m2 + 65 * threadIdx.x;
-  int *ix = s_ix + 65 * threadIdx.x;
+  __shared__ double s_m2[65];
+  __shared__ int s_ix[65], s_stack[24];
   const int nv = K.no_vbulk;
-  for (int j = 0; j < nv; j++) {
-    const uint32_t p = a[o + j];
-    const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
-    m2[j + 1] = (double)v2;
+  for (int64_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const int64_t o = off[h], n = off[h + 1] - o;
+    if (n < K.minpart || nv < 1) {
+      if (threadIdx.x == 0) seed[h] = 0;
+      continue;
+    }
+    if ((int)threadIdx.x < nv) {
+      const uint32_t p = a[o + threadIdx.x];
+      const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+      s_m2[threadIdx.x + 1] = (double)v2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      dev_indexx(nv, s_m2, s_ix, s_stack);
+      seed[h] = s_ix[nv / 2] - 1;
+    }
+    __syncthreads();
   }
-  dev_indexx(nv, m2, ix, s_stack + 24 * threadIdx.x);
-  seed[h] = ix[nv / 2] - 1;
 }
 static inline unsigned seed_blocks(int64_t nh)
 {
-  return (unsigned)((nh + AHF_SEED_BS - 1) / AHF_SEED_BS);
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nh, (int64_t)1 << 20));
 }
 
 /* haloes of at least this many members go to 512-thread workgroups */
@@ -1979,6 +1983,8 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     if (rc) return rc;
   }
   if (hipSetDevice(c->device) != hipSuccess) return c->fail(AHFGPU_EHIP, "hipSetDevice");
+  s->done = 0;
+  s->res_staged = 0;
   Arena &A = c->uarena;
   A.reset();
   const int64_t ne = s->ne, nh = s->nh;
@@ -2181,11 +2187,19 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   }
   c->timer_end(eu, TIMER_UNBIND, 0.0);
   if (c->timing) hipEventRecord(c->ev_u1, c->st);
+  /* the results go to their pinned staging before the same
+   * synchronisation */
+  {
+    char *R = halo_set_res(s, res_off_b(nh, ne) + (size_t)ne * 4 + 16);
+    if (R && hipMemcpyAsync(R, s->out, res_off_b(nh, ne) + (size_t)ne * 4, hipMemcpyDeviceToHost, c->st) == hipSuccess)
+      s->res_staged = 1;
+  }
   if (hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "unbinding: %s", hipGetErrorString(hipGetLastError()));
   if (n_tie < 0) {
     n_tie = 0;
     if (c->h_words[2] > 0) {
+      s->res_staged = 0; /* the lists are re-sorted below */
       /* the records k_sorted_out wrote do not depend on the order */
       const int rc = nr_tie_order(c, c->st, s->off, nh, ne, p->minpart, s->ip_in, s->cen, s->buf_a, tie, tpos, tmp32h,
                                   [&](size_t bytes) { return A.raw(bytes); }, &n_tie);
@@ -2229,7 +2243,9 @@ extern "C" int ahfgpu_unbind_results(ahfgpu_ctx *c, uint64_t *npart, double *pro
   const size_t o_a = res_off_a(nh), o_b = res_off_b(nh, ne);
   char *R = halo_set_res(s, o_b + (size_t)ne * 4 + 16);
   if (!R) return c->fail(AHFGPU_ENOMEM, "pinned staging of the results");
-  if ((char *)s->buf_a == (char *)s->out + o_a && (char *)s->buf_b == (char *)s->out + o_b) {
+  if (s->res_staged) {
+    /* read back already */
+  } else if ((char *)s->buf_a == (char *)s->out + o_a && (char *)s->buf_b == (char *)s->out + o_b) {
     hipMemcpyAsync(R, s->out, o_b + (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
   } else {
     if (nh) hipMemcpyAsync(R, s->out, (size_t)nh * sizeof(HaloOut), hipMemcpyDeviceToHost, c->st);
@@ -2238,7 +2254,7 @@ extern "C" int ahfgpu_unbind_results(ahfgpu_ctx *c, uint64_t *npart, double *pro
       hipMemcpyAsync(R + o_b, s->buf_b, (size_t)ne * 4, hipMemcpyDeviceToHost, c->st);
     }
   }
-  if (hipStreamSynchronize(c->st) != hipSuccess)
+  if (!s->res_staged && hipStreamSynchronize(c->st) != hipSuccess)
     return c->fail(AHFGPU_EHIP, "unbinding results: %s", hipGetErrorString(hipGetLastError()));
   const HaloOut *ho = (const HaloOut *)R;
   const uint32_t *a = (const uint32_t *)(R + o_a), *b = (const uint32_t *)(R + o_b);
