@@ -210,17 +210,20 @@ static int gridinfo_replay(lev_t *lev, int nlev, const snap_data *s, const char 
   return 0;
 }
 
-int main(int argc, char **argv)
+/* the device pipeline replayed on snapshot s (only its header's grid
+ * parameters and its positions are read): the levels in lev[0..*nlev_out)
+ * (lev[*nlev_out] holds a level built and rolled back, if any) and their
+ * final host levels in *hl_out, whose key, link and density arrays are the
+ * levels' own.  lev must hold EMU_MAX_LEVELS zeroed entries.  Returns the
+ * anomaly flag of the refinement rules. */
+#define EMU_MAX_LEVELS 64
+static int emu_build(const snap_data *sp, lev_t *lev, int *nlev_out, host_level **hl_out)
 {
-  if (argc < 3) { fprintf(stderr, "usage: amr_emu snap out\n"); return 2; }
-  snap_data s;
-  if (snap_read(argv[1], &s)) { fprintf(stderr, "emu: bad snapshot\n"); return 2; }
+  const snap_data s = *sp;
   const int64_t N = (int64_t)s.h.npart, L0 = s.h.L_dom, half = L0 / 2;
   float *px = (float *)xm((size_t)N * 4), *py = (float *)xm((size_t)N * 4), *pz = (float *)xm((size_t)N * 4);
   for (int64_t p = 0; p < N; p++) { px[p] = s.pos[3 * p]; py[p] = s.pos[3 * p + 1]; pz[p] = s.pos[3 * p + 2]; }
 
-  lev_t lev[64];
-  memset(lev, 0, sizeof(lev));
   int nlev = 1;
   /* ---- domain level as octets of a virtual L/2 grid ---- */
   lev_t *d = &lev[0];
@@ -458,6 +461,41 @@ int main(int argc, char **argv)
     }
     h->list0[nn] = w;
   }
+  free(px);
+  free(py);
+  free(pz);
+  *nlev_out = nlev;
+  *hl_out = hl;
+  return anomaly;
+}
+
+/* everything emu_build allocated */
+static void emu_free(lev_t *lev, int nlev, host_level *hl)
+{
+  for (int i = 0; i < nlev; i++) {
+    free(hl[i].list0);
+    free(hl[i].list);
+  }
+  free(hl);
+  for (int i = 0; i < EMU_MAX_LEVELS; i++) {
+    lev_t *l = &lev[i];
+    free(l->oct_key); free(l->oct_link); free(l->nbr); free(l->row_key); free(l->row_oct0); free(l->row_plane);
+    free(l->plane_z); free(l->plane_row0); free(l->oct_parent); free(l->pmask); free(l->dens); free(l->flags);
+    free(l->node_p0); free(l->plist); free(l->moved); free(l->node2oct);
+    memset(l, 0, sizeof(*l)); /* fine_p0, fine_list: the next level's node_p0, plist */
+  }
+}
+
+#ifndef AMR_EMU_NO_MAIN
+int main(int argc, char **argv)
+{
+  if (argc < 3) { fprintf(stderr, "usage: amr_emu snap out\n"); return 2; }
+  snap_data s;
+  if (snap_read(argv[1], &s)) { fprintf(stderr, "emu: bad snapshot\n"); return 2; }
+  static lev_t lev[EMU_MAX_LEVELS];
+  int nlev = 0;
+  host_level *hl = NULL;
+  const int anomaly = emu_build(&s, lev, &nlev, &hl);
   int rc = hierarchy_dump(argv[2], hl, nlev);
   if (!rc && argc >= 7) {
     rc = gridinfo_replay(lev, nlev, &s, argv[3], atof(argv[4]), atof(argv[5]), atof(argv[6]));
@@ -466,3 +504,4 @@ int main(int argc, char **argv)
   printf("{\"levels\": %d, \"anomaly\": %d}\n", nlev, anomaly);
   return rc ? 1 : (anomaly ? 3 : 0);
 }
+#endif
