@@ -64,7 +64,8 @@ class Schedule(ctypes.Structure):
     """ahfgpu_schedule: thresholds between equivalent kernels (0: default)"""
     _fields_ = [("split_min", ctypes.c_int64), ("prof_split", ctypes.c_int64), ("big_halo", ctypes.c_int64),
                 ("wave_halo", ctypes.c_int64), ("kmax_big", ctypes.c_int32), ("kmax_small", ctypes.c_int32),
-                ("march_zn", ctypes.c_int32), ("graph_batch", ctypes.c_int32)]
+                ("march_zn", ctypes.c_int32), ("graph_batch", ctypes.c_int32), ("split_spec", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -41,11 +41,12 @@ CANDIDATES = {
     "wave_halo": [512, 256, 1024],
     "split_min": [16384, 4096, 8192, 32768, 65536, -1],
     "graph_batch": [4, 2, 8],
+    "split_spec": [3, 1, 2, 4, 6, -1],
     "prof_split": [2048, 512, 1024, 4096, 8192],
 }
 # which timing decides each field
 DECIDES = {"kmax_big": "amr", "kmax_small": "amr", "march_zn": "amr", "big_halo": "unbind", "wave_halo": "unbind",
-           "split_min": "unbind", "graph_batch": "unbind", "prof_split": "profiles"}
+           "split_min": "unbind", "graph_batch": "unbind", "split_spec": "unbind", "prof_split": "profiles"}
 
 
 def log(msg):

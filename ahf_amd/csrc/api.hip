@@ -78,6 +78,7 @@ static void load_schedule(ahfgpu_ctx *c)
   if (json_int(txt, "kmax_small", &v)) sc.kmax_small = (int32_t)v;
   if (json_int(txt, "march_zn", &v)) sc.march_zn = (int32_t)v;
   if (json_int(txt, "graph_batch", &v)) sc.graph_batch = (int32_t)v;
+  if (json_int(txt, "split_spec", &v)) sc.split_spec = (int32_t)v;
   if (ahfgpu_set_schedule(c, &sc) != 0) fprintf(stderr, "ahfgpu: schedule %s ignored: %s\n", path.c_str(), c->err);
 }
 
@@ -89,6 +90,7 @@ extern "C" int ahfgpu_set_schedule(ahfgpu_ctx *c, const ahfgpu_schedule *s)
   if (s->march_zn != 0 && s->march_zn != 8 && s->march_zn != 16) return c->fail(AHFGPU_EINVAL, "march_zn: 0, 8 or 16");
   if (s->big_halo < 0 || s->wave_halo < 0) return c->fail(AHFGPU_EINVAL, "tier thresholds must not be negative");
   if (s->graph_batch < 0 || s->graph_batch > 64) return c->fail(AHFGPU_EINVAL, "graph_batch must be in [0, 64]");
+  if (s->split_spec < -1 || s->split_spec > 16) return c->fail(AHFGPU_EINVAL, "split_spec must be in [-1, 16]");
   c->split_min = s->split_min;
   c->prof_split = s->prof_split;
   c->big_halo = s->big_halo;
@@ -97,6 +99,7 @@ extern "C" int ahfgpu_set_schedule(ahfgpu_ctx *c, const ahfgpu_schedule *s)
   c->kmax_small = s->kmax_small;
   c->march_zn = s->march_zn;
   c->split_graph_batch = s->graph_batch;
+  c->split_spec = s->split_spec;
   return 0;
 }
 
@@ -111,6 +114,8 @@ extern "C" int ahfgpu_get_schedule(ahfgpu_ctx *c, ahfgpu_schedule *s)
   s->kmax_small = c->kmax_small;
   s->march_zn = c->march_zn;
   s->graph_batch = c->split_graph_batch;
+  s->split_spec = c->split_spec;
+  s->pad = 0;
   return 0;
 }
 

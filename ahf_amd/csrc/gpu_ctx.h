@@ -150,6 +150,7 @@ struct ahfgpu_ctx {
   int64_t *d_sizes = nullptr;   /* level sizes gathered for one read-back  */
   int64_t *h_sizes = nullptr;   /* pinned host copy of d_sizes             */
   int32_t *h_words = nullptr;   /* pinned host words (the split path's done flag) */
+  int split_spec = 0;           /* speculative passes per split iteration; 0: 3, -1: none */
   int split_graph_batch = 0;    /* graph replays between done checks; 0: 4 */
   int split_graph_launches = 0; /* replays of the last split unbinding */
   void *halo_set = nullptr; /* resident halo lists (halo_set.h) */

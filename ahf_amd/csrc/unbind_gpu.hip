@@ -458,6 +458,25 @@ __device__ inline D4 blk_excl_scan4(D4 v, D4 *tot, D4 *sh /* [BS / 64] */)
   return D4{base.x + ex.x, base.y + ex.y, base.z + ex.z, base.m + ex.m};
 }
 
+/* exclusive prefix sums of one halo's tile values v(t), t in [t0, t1), by
+ * one workgroup of 256 threads in chunks of 256 with a running carry
+ * (start: the value before the first tile); put(t, prefix) receives them;
+ * returns start plus the total */
+template <class Get, class Put>
+__device__ inline D4 blk_scan_tiles4(int32_t t0, int32_t t1, D4 start, Get v, Put put, D4 *sh4)
+{
+  D4 carry = start;
+  for (int32_t b = t0; b < t1; b += 256) {
+    const int32_t t = b + (int32_t)threadIdx.x;
+    const D4 x = t < t1 ? v(t) : D4{0, 0, 0, 0};
+    D4 tot;
+    const D4 ex = blk_excl_scan4<256>(x, &tot, sh4);
+    if (t < t1) put(t, D4{carry.x + ex.x, carry.y + ex.y, carry.z + ex.z, carry.m + ex.m});
+    carry.x += tot.x; carry.y += tot.y; carry.z += tot.z; carry.m += tot.m;
+  }
+  return carry;
+}
+
 /*
  * The unbinding of one halo by one workgroup of BS threads.  a/b: two member
  * buffers at the halo's offset (sorted list in a); dist/phi: FP64 scratch;
@@ -472,6 +491,112 @@ struct Staged {
   float *vx, *vy, *vz;
 };
 
+/* per-halo factors of the certified bound of the bound test (see rec_eval) */
+struct RecBound {
+  double e_lim, cvx, cvy, cvz;
+};
+
+__device__ inline RecBound rec_bound(const UnbindConst &K, double Phi0, D4 asum, double gam)
+{
+  const double e_phi = 2.5 * gam * fabs(Phi0);
+  return RecBound{K.v2_tune * 4.0 * K.phi_fac * e_phi, fabs(K.v_fac) * 5.0 * gam * asum.x,
+                  fabs(K.v_fac) * 5.0 * gam * asum.y, fabs(K.v_fac) * 5.0 * gam * asum.z};
+}
+
+/*
+ * One chunk [base, base + BS*IPT) of the bound-set recurrence, settled: the
+ * chunk is evaluated with cv, the bulk-velocity totals of every member
+ * before it, and its own statuses of the previous evaluation (bj: the first
+ * guess on entry), and evaluated again until they no longer change; then
+ * bj holds the answer the sequential loop gives with that carry (the
+ * dependence is strictly on earlier members, so any first guess settles).
+ * Out: the chunk's bound totals (the same on every thread), whether a test
+ * of the final evaluation was uncertain.  flag: two words of LDS, ev the
+ * evaluation counter that alternates them.
+ */
+template <int BS, int IPT>
+__device__ void settle_chunk(const UnbindConst &K, int64_t n, int64_t base, const double *dist, const double *phi,
+                             const Staged &g, double Phi0, D4 cv, const RecBound &B, int bj[IPT], D4 *tot_out,
+                             int *unc_out, int *sweeps, D4 *sh4, int *sflag, int &ev)
+{
+  const int t = threadIdx.x;
+  const int64_t j0 = base + (int64_t)t * IPT;
+  /* what a member's test needs besides the running bulk velocity, once per
+   * chunk: its velocity, its Hubble term and its limit v2_tune*v_esc2
+   * (no limit, lim < 0, inside MACHINE_ZERO) */
+  double fx[IPT], fy[IPT], fz[IPT], hx[IPT], hy[IPT], hz[IPT], lm[IPT];
+#pragma unroll
+  for (int i = 0; i < IPT; i++) {
+    const int64_t j = j0 + i;
+    fx[i] = fy[i] = fz[i] = hx[i] = hy[i] = hz[i] = 0.0;
+    lm[i] = -1.0;
+    if (j < n) {
+      fx[i] = (double)g.vx[j]; fy[i] = (double)g.vy[j]; fz[i] = (double)g.vz[j];
+      hx[i] = K.hubble * (g.dx[j]) * K.r_fac;
+      hy[i] = K.hubble * (g.dy[j]) * K.r_fac;
+      hz[i] = K.hubble * (g.dz[j]) * K.r_fac;
+      if (dist[j] > MZERO) lm[i] = K.v2_tune * (2 * fabs(phi[j] - Phi0) * K.phi_fac);
+    }
+  }
+  for (;;) {
+    (*sweeps)++;
+    /* the flag word of this evaluation; the other one was read by every
+     * thread before the previous evaluation's barrier, and is cleared for
+     * the next evaluation once this one's scan has passed its barriers */
+    int *flag = sflag + (ev & 1);
+    D4 run = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < IPT; i++)
+      if (bj[i]) { run.x += fx[i]; run.y += fy[i]; run.z += fz[i]; run.m += 1.0; }
+    D4 tot;
+    const D4 ex = blk_excl_scan4<BS>(run, &tot, sh4);
+    /* the thread's own earlier members, summed again in the same order */
+    D4 lex = {0, 0, 0, 0};
+    int chg = 0, unc = 0;
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+      const int64_t j = j0 + i;
+      if (j >= n) continue;
+      const double VX = cv.x + ex.x + lex.x, VY = cv.y + ex.y + lex.y, VZ = cv.z + ex.z + lex.z,
+                   M = cv.m + ex.m + lex.m;
+      if (bj[i]) { lex.x += fx[i]; lex.y += fy[i]; lex.z += fz[i]; lex.m += 1.0; }
+      const double dVX = (fx[i] - VX / M) * K.v_fac + hx[i];
+      const double dVY = (fy[i] - VY / M) * K.v_fac + hy[i];
+      const double dVZ = (fz[i] - VZ / M) * K.v_fac + hz[i];
+      const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
+      /* v_esc2 = 1e30 inside MACHINE_ZERO: the limit is never exceeded */
+      const double lim = lm[i] >= 0.0 ? lm[i] : K.v2_tune * 1e30;
+      const int nb = !(vel2 > lim);
+      if (lm[i] >= 0.0) {
+        /* certified bound on how far the sequential vel^2 and limit can
+         * lie from these: Phi_j and Phi0 are sums of non-negative terms
+         * (error <= gam*Phi0 each), the velocity sums are within
+         * gam*2*sum|v| (the seed is counted twice); propagated through
+         * the test, plus a few ulps per operation */
+        const double rM = 1.0 / M;
+        const double e_vx = B.cvx * rM, e_vy = B.cvy * rM, e_vz = B.cvz * rM;
+        const double e_vel2 = 2.0 * (fabs(dVX) * e_vx + fabs(dVY) * e_vy + fabs(dVZ) * e_vz) + e_vx * e_vx +
+                              e_vy * e_vy + e_vz * e_vz;
+        const double tol = 4.0 * (e_vel2 + B.e_lim) + 1e-14 * (vel2 + lim);
+        const double gap = fabs(vel2 - lim);
+        if (gap <= tol || gap <= K.band * lim) unc = 1;
+      }
+      if (nb != bj[i]) { bj[i] = nb; chg = 1; }
+    }
+    if (t == 0) sflag[(ev + 1) & 1] = 0;
+    if (chg | unc) atomicOr(flag, chg | (unc << 1));
+    __syncthreads();
+    const int f = *flag;
+    ev++;
+    if (!(f & 1)) {
+      /* the statuses used were the final ones: so are the totals */
+      *unc_out = f >> 1;
+      *tot_out = tot;
+      return;
+    }
+  }
+}
+
 /*
  * One iteration's bound-set recurrence of a halo of n members (one
  * workgroup of BS threads), from its staged data (dist, Phi_j in phi, the
@@ -483,9 +608,19 @@ struct Staged {
  * replay (thread 0, in AHF's operation order), which also recomputes Phi0.
  * Returns 1 when it replayed.
  */
+/* where rec_eval starts when earlier members are settled already: member
+ * j (a chunk boundary of the caller's), the totals before it, and whether
+ * a settled test was uncertain */
+struct RecResume {
+  int64_t j;
+  D4 cv;
+  int unc;
+};
+
 template <int BS, int IPT>
 __device__ int rec_eval(const UnbindConst &K, int64_t n, int64_t seed, const double *dist, const double *phi,
-                        uint8_t *st, Staged g, D4 asum, double gam, double *Phi0_io, int *sweeps)
+                        uint8_t *st, Staged g, D4 asum, double gam, double *Phi0_io, int *sweeps,
+                        const RecResume *resume = nullptr)
 {
   constexpr int NW = BS / 64;
   constexpr int64_t CH = (int64_t)BS * IPT;
@@ -505,97 +640,24 @@ __device__ int rec_eval(const UnbindConst &K, int64_t n, int64_t seed, const dou
    * again until they no longer change; then it holds the sequential answer
    * and its totals carry to the next chunk.  Only the final evaluation's
    * margins count for the exact replay. */
-  int uncertain = 0;
-  D4 cv = {sx, sy, sz, 1.0};
-  /* per-halo factors of the certified bound */
-  const double e_phi = 2.5 * gam * fabs(Phi0);
-  const double e_lim = K.v2_tune * 4.0 * K.phi_fac * e_phi;
-  const double cvx = fabs(K.v_fac) * 5.0 * gam * asum.x, cvy = fabs(K.v_fac) * 5.0 * gam * asum.y,
-               cvz = fabs(K.v_fac) * 5.0 * gam * asum.z;
-  for (int64_t base = 0; base < n; base += CH) {
+  int uncertain = resume ? resume->unc : 0;
+  D4 cv = resume ? resume->cv : D4{sx, sy, sz, 1.0};
+  const RecBound B = rec_bound(K, Phi0, asum, gam);
+  for (int64_t base = resume ? resume->j : 0; base < n; base += CH) {
     const int64_t j0 = base + (int64_t)t * IPT;
-    /* what a member's test needs besides the running bulk velocity, once per
-     * chunk: its velocity, its Hubble term and its limit v2_tune*v_esc2
-     * (no limit, lim < 0, inside MACHINE_ZERO) */
-    double fx[IPT], fy[IPT], fz[IPT], hx[IPT], hy[IPT], hz[IPT], lm[IPT];
-#pragma unroll
-    for (int i = 0; i < IPT; i++) {
-      const int64_t j = j0 + i;
-      fx[i] = fy[i] = fz[i] = hx[i] = hy[i] = hz[i] = 0.0;
-      lm[i] = -1.0;
-      if (j < n) {
-        fx[i] = (double)g.vx[j]; fy[i] = (double)g.vy[j]; fz[i] = (double)g.vz[j];
-        hx[i] = K.hubble * (g.dx[j]) * K.r_fac;
-        hy[i] = K.hubble * (g.dy[j]) * K.r_fac;
-        hz[i] = K.hubble * (g.dz[j]) * K.r_fac;
-        if (dist[j] > MZERO) lm[i] = K.v2_tune * (2 * fabs(phi[j] - Phi0) * K.phi_fac);
-      }
-    }
     /* statuses of the thread's members, kept in registers while the chunk
      * settles; the first evaluation assumes all bound */
     int bj[IPT];
 #pragma unroll
     for (int i = 0; i < IPT; i++) bj[i] = j0 + i < n ? 1 : 0;
-    for (;;) {
-      (*sweeps)++;
-      /* the flag word of this evaluation; the other one was read by every
-       * thread before the previous evaluation's barrier, and is cleared for
-       * the next evaluation once this one's scan has passed its barriers */
-      int *flag = sflag + (ev & 1);
-      D4 run = {0, 0, 0, 0};
+    D4 tot;
+    int unc = 0;
+    settle_chunk<BS, IPT>(K, n, base, dist, phi, g, Phi0, cv, B, bj, &tot, &unc, sweeps, sh4, sflag, ev);
+    uncertain |= unc;
+    cv.x += tot.x; cv.y += tot.y; cv.z += tot.z; cv.m += tot.m;
 #pragma unroll
-      for (int i = 0; i < IPT; i++)
-        if (bj[i]) { run.x += fx[i]; run.y += fy[i]; run.z += fz[i]; run.m += 1.0; }
-      D4 tot;
-      const D4 ex = blk_excl_scan4<BS>(run, &tot, sh4);
-      /* the thread's own earlier members, summed again in the same order */
-      D4 lex = {0, 0, 0, 0};
-      int chg = 0, unc = 0;
-#pragma unroll
-      for (int i = 0; i < IPT; i++) {
-        const int64_t j = j0 + i;
-        if (j >= n) continue;
-        const double VX = cv.x + ex.x + lex.x, VY = cv.y + ex.y + lex.y, VZ = cv.z + ex.z + lex.z,
-                     M = cv.m + ex.m + lex.m;
-        if (bj[i]) { lex.x += fx[i]; lex.y += fy[i]; lex.z += fz[i]; lex.m += 1.0; }
-        const double dVX = (fx[i] - VX / M) * K.v_fac + hx[i];
-        const double dVY = (fy[i] - VY / M) * K.v_fac + hy[i];
-        const double dVZ = (fz[i] - VZ / M) * K.v_fac + hz[i];
-        const double vel2 = dVX * dVX + dVY * dVY + dVZ * dVZ;
-        /* v_esc2 = 1e30 inside MACHINE_ZERO: the limit is never exceeded */
-        const double lim = lm[i] >= 0.0 ? lm[i] : K.v2_tune * 1e30;
-        const int nb = !(vel2 > lim);
-        if (lm[i] >= 0.0) {
-          /* certified bound on how far the sequential vel^2 and limit can
-           * lie from these: Phi_j and Phi0 are sums of non-negative terms
-           * (error <= gam*Phi0 each), the velocity sums are within
-           * gam*2*sum|v| (the seed is counted twice); propagated through
-           * the test, plus a few ulps per operation */
-          const double rM = 1.0 / M;
-          const double e_vx = cvx * rM, e_vy = cvy * rM, e_vz = cvz * rM;
-          const double e_vel2 = 2.0 * (fabs(dVX) * e_vx + fabs(dVY) * e_vy + fabs(dVZ) * e_vz) + e_vx * e_vx +
-                                e_vy * e_vy + e_vz * e_vz;
-          const double tol = 4.0 * (e_vel2 + e_lim) + 1e-14 * (vel2 + lim);
-          const double gap = fabs(vel2 - lim);
-          if (gap <= tol || gap <= K.band * lim) unc = 1;
-        }
-        if (nb != bj[i]) { bj[i] = nb; chg = 1; }
-      }
-      if (t == 0) sflag[(ev + 1) & 1] = 0;
-      if (chg | unc) atomicOr(flag, chg | (unc << 1));
-      __syncthreads();
-      const int f = *flag;
-      ev++;
-      if (!(f & 1)) {
-        /* the statuses used were the final ones: so are the totals */
-        uncertain |= f >> 1;
-        cv.x += tot.x; cv.y += tot.y; cv.z += tot.z; cv.m += tot.m;
-#pragma unroll
-        for (int i = 0; i < IPT; i++)
-          if (j0 + i < n) st[j0 + i] = (uint8_t)bj[i];
-        break;
-      }
-    }
+    for (int i = 0; i < IPT; i++)
+      if (j0 + i < n) st[j0 + i] = (uint8_t)bj[i];
   }
   __syncthreads();
   /* exact replay of this iteration where rounding could decide (thread 0) */
@@ -1011,6 +1073,12 @@ struct SplitHalo {
   double M_vir, R_vir, Phi0, ovdens;
   double asum[3], phitot;
   int32_t niter, replays, in_b, set, rvir_set, sweeps, done, active;
+  /* the speculative passes: tiles [first tile, f_tile) are settled, the
+   * halo's tiles end at e_tile; where the sequential walk resumes */
+  int32_t f_tile, e_tile;
+  int64_t rs_j;
+  double rs_cv[4];
+  int32_t rs_unc, rs_pad;
 };
 
 /* tile t covers members [j0, j0 + AHF_TILE) of split halo `seg` */
@@ -1031,7 +1099,8 @@ __global__ void __launch_bounds__(256) k_sp_stage(const SplitHalo *__restrict__ 
                                                   const double *__restrict__ cen, const float *__restrict__ px,
                                                   const float *__restrict__ py, const float *__restrict__ pz,
                                                   const float *__restrict__ vx, const float *__restrict__ vy,
-                                                  const float *__restrict__ vz, double *__restrict__ dist, Staged g)
+                                                  const float *__restrict__ vz, double *__restrict__ dist, Staged g,
+                                                  uint8_t *__restrict__ st)
 {
   if ((int32_t)blockIdx.x >= *ntp) return;
   const SplitTile T = tiles[blockIdx.x];
@@ -1051,6 +1120,7 @@ __global__ void __launch_bounds__(256) k_sp_stage(const SplitHalo *__restrict__ 
     g.vx[e] = vx[p];
     g.vy[e] = vy[p];
     g.vz[e] = vz[p];
+    st[e] = 1;
   }
 }
 
@@ -1104,22 +1174,22 @@ __global__ void __launch_bounds__(256) k_sp_phi_tiles(const SplitHalo *__restric
   if (threadIdx.x == 0) ttot[blockIdx.x] = tot;
 }
 
-/* per split halo, in tile order: the carry into each tile (exclusive sum of
- * the tiles before it) and the halo's totals */
-__global__ void k_sp_phi_carry(SplitHalo *sh, int64_t nseg, const int32_t *__restrict__ seg_tile0,
-                               const D4 *__restrict__ ttot, D4 *__restrict__ tcarry)
+/* per split halo (one workgroup each): the carry into each tile (exclusive
+ * sum of the tiles before it) and the halo's totals */
+__global__ void __launch_bounds__(256) k_sp_phi_carry(SplitHalo *sh, const int32_t *__restrict__ seg_tile0,
+                                                      const D4 *__restrict__ ttot, D4 *__restrict__ tcarry)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nseg || !sh[i].active) return;
-  D4 acc = {0, 0, 0, 0};
-  for (int32_t t = seg_tile0[i]; t < seg_tile0[i + 1]; t++) {
-    tcarry[t] = acc;
-    acc.x += ttot[t].x; acc.y += ttot[t].y; acc.z += ttot[t].z; acc.m += ttot[t].m;
+  __shared__ D4 sh4[4];
+  const int64_t i = blockIdx.x;
+  if (!sh[i].active) return;
+  const D4 acc = blk_scan_tiles4(seg_tile0[i], seg_tile0[i + 1], D4{0, 0, 0, 0}, [&](int32_t t) { return ttot[t]; },
+                                 [&](int32_t t, D4 x) { tcarry[t] = x; }, sh4);
+  if (threadIdx.x == 0) {
+    sh[i].phitot = acc.x;
+    sh[i].asum[0] = acc.y;
+    sh[i].asum[1] = acc.z;
+    sh[i].asum[2] = acc.m;
   }
-  sh[i].phitot = acc.x;
-  sh[i].asum[0] = acc.y;
-  sh[i].asum[1] = acc.z;
-  sh[i].asum[2] = acc.m;
 }
 
 __global__ void __launch_bounds__(256) k_sp_phi_add(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
@@ -1136,9 +1206,180 @@ __global__ void __launch_bounds__(256) k_sp_phi_add(const SplitHalo *__restrict_
   }
 }
 
-/* the recurrence of one iteration, one workgroup per active split halo */
+/*
+ * The speculation of the recurrence.  Every unsettled tile of an iteration
+ * is settled at once (settle_chunk), each with the carry of the tiles
+ * before it under the statuses of the previous pass (all bound in the
+ * first), and the pass is repeated with the statuses it produced.  Per
+ * halo, the tiles before f_tile are settled: they hold the sequential
+ * answer.  In a pass, tile f_tile's carry comes from settled tiles only, so
+ * it settles to the sequential answer; so does every later tile up to the
+ * first one whose statuses changed in the pass, since the tiles between
+ * kept the statuses its carry assumed.  f_tile moves past that tile, or to
+ * the end when no tile changed.  The sequential walk (k_sp_rec) resumes at
+ * f_tile with the carry of the settled tiles.  The carries are sums in
+ * another association, which the certified bound of the test covers.
+ */
+__global__ void k_sp_spec_init(SplitHalo *sh, int64_t nseg, const int32_t *__restrict__ seg_tile0)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg || !sh[i].active) return;
+  sh[i].f_tile = seg_tile0[i];
+  sh[i].e_tile = seg_tile0[i + 1];
+  sh[i].rs_unc = 0;
+}
+
+__device__ inline bool split_settled(const SplitHalo &H) { return H.f_tile >= H.e_tile; }
+
+/* each tile's bound totals under the current statuses */
+__global__ void __launch_bounds__(256) k_sp_spec_tot(const SplitHalo *__restrict__ sh,
+                                                     const SplitTile *__restrict__ tiles,
+                                                     const int32_t *__restrict__ ntp, const uint8_t *__restrict__ st,
+                                                     Staged g, D4 *__restrict__ ttot)
+{
+  if ((int32_t)blockIdx.x >= *ntp) return;
+  __shared__ D4 sh4[4];
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  if (split_settled(H)) return;
+  const int64_t j0 = T.j0 + (int64_t)threadIdx.x * 4;
+  D4 run = {0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int64_t e = H.o + j0 + i;
+    if (j0 + i < H.n && st[e]) {
+      run.x += (double)g.vx[e]; run.y += (double)g.vy[e]; run.z += (double)g.vz[e]; run.m += 1.0;
+    }
+  }
+  D4 tot;
+  (void)blk_excl_scan4<256>(run, &tot, sh4);
+  if (threadIdx.x == 0) ttot[blockIdx.x] = tot;
+}
+
+/* the seed's velocity, counted once, starts every halo's bulk-velocity sums */
+__device__ inline D4 split_seed_cv(const SplitHalo &H, const Staged &g)
+{
+  const int64_t e = H.o + (H.niter == 1 ? H.seed : 0);
+  return D4{(double)g.vx[e], (double)g.vy[e], (double)g.vz[e], 1.0};
+}
+
+/* per split halo (one workgroup each): the carry into each tile */
+__global__ void __launch_bounds__(256) k_sp_spec_carry(const SplitHalo *__restrict__ sh,
+                                                       const int32_t *__restrict__ seg_tile0,
+                                                       const D4 *__restrict__ ttot, Staged g, D4 *__restrict__ tcv)
+{
+  __shared__ D4 sh4[4];
+  const SplitHalo &H = sh[blockIdx.x];
+  if (!H.active || split_settled(H)) return;
+  (void)blk_scan_tiles4(seg_tile0[blockIdx.x], H.e_tile, split_seed_cv(H, g), [&](int32_t t) { return ttot[t]; },
+                        [&](int32_t t, D4 x) { tcv[t] = x; }, sh4);
+}
+
+/* one pass: every unsettled tile settled with its carry; tchg: its
+ * statuses changed, tunc: a test of its final evaluation was uncertain */
+__global__ void __launch_bounds__(256) k_sp_spec_eval(UnbindConst K, const SplitHalo *__restrict__ sh,
+                                                      const SplitTile *__restrict__ tiles,
+                                                      const int32_t *__restrict__ ntp, const double *__restrict__ dist,
+                                                      const double *__restrict__ phi, uint8_t *__restrict__ st,
+                                                      Staged g, const D4 *__restrict__ tcv,
+                                                      uint8_t *__restrict__ tchg, uint8_t *__restrict__ tunc)
+{
+  if ((int32_t)blockIdx.x >= *ntp) return;
+  __shared__ D4 sh4[4];
+  __shared__ int sflag[2];
+  __shared__ int s_chg;
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  if ((int32_t)blockIdx.x < H.f_tile || split_settled(H)) return;
+  const int64_t o = H.o, n = H.n;
+  const Staged go = {g.dx + o, g.dy + o, g.dz + o, g.vx + o, g.vy + o, g.vz + o};
+  const double Phi0 = phi[o + n - 1] + (double)n / dist[o + n - 1];
+  const double Nd = (double)n + 8.0, U = 1.1102230246251565e-16;
+  const double gam = Nd * U / (1.0 - Nd * U);
+  const RecBound B = rec_bound(K, Phi0, D4{H.asum[0], H.asum[1], H.asum[2], 0.0}, gam);
+  if (threadIdx.x == 0) {
+    sflag[0] = sflag[1] = 0;
+    s_chg = 0;
+  }
+  __syncthreads();
+  const int64_t j0 = T.j0 + (int64_t)threadIdx.x * 4;
+  int bj[4], b0[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) bj[i] = b0[i] = j0 + i < n ? (int)st[o + j0 + i] : 0;
+  D4 tot;
+  int unc = 0, sweeps = 0, ev = 0;
+  settle_chunk<256, 4>(K, n, T.j0, dist + o, phi + o, go, Phi0, tcv[blockIdx.x], B, bj, &tot, &unc, &sweeps, sh4,
+                       sflag, ev);
+  int chg = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (j0 + i < n) {
+      chg |= bj[i] != b0[i];
+      st[o + j0 + i] = (uint8_t)bj[i];
+    }
+  if (chg) atomicOr(&s_chg, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    tchg[blockIdx.x] = (uint8_t)s_chg;
+    tunc[blockIdx.x] = (uint8_t)unc;
+  }
+}
+
+/* after a pass, per split halo (one workgroup each): the tiles up to the
+ * first one that changed are settled (their final evaluations' uncertain
+ * flags are kept) */
+__global__ void __launch_bounds__(256) k_sp_spec_advance(SplitHalo *sh, const uint8_t *__restrict__ tchg,
+                                                         const uint8_t *__restrict__ tunc)
+{
+  __shared__ int s_first, s_unc;
+  SplitHalo &H = sh[blockIdx.x];
+  if (!H.active || split_settled(H)) return;
+  const int32_t f = H.f_tile, e = H.e_tile;
+  if (threadIdx.x == 0) {
+    s_first = e;
+    s_unc = 0;
+  }
+  __syncthreads();
+  for (int32_t t = f + (int32_t)threadIdx.x; t < e; t += 256)
+    if (tchg[t]) atomicMin(&s_first, t);
+  __syncthreads();
+  const int32_t last = s_first < e ? s_first : e - 1; /* the settled tiles are [f, last] */
+  int u = 0;
+  for (int32_t t = f + (int32_t)threadIdx.x; t <= last; t += 256) u |= tunc[t];
+  if (u) atomicOr(&s_unc, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    H.f_tile = last + 1;
+    H.rs_unc |= s_unc;
+  }
+}
+
+/* after the passes, per split halo (one workgroup each): where the
+ * sequential walk resumes and the carry of the settled tiles (ttot: their
+ * final totals) */
+__global__ void __launch_bounds__(256) k_sp_spec_resume(SplitHalo *sh, const int32_t *__restrict__ seg_tile0,
+                                                        const SplitTile *__restrict__ tiles,
+                                                        const D4 *__restrict__ ttot, Staged g)
+{
+  __shared__ D4 sh4[4];
+  SplitHalo &H = sh[blockIdx.x];
+  if (!H.active) return;
+  const bool settled = split_settled(H);
+  const D4 acc = blk_scan_tiles4(seg_tile0[blockIdx.x], settled ? seg_tile0[blockIdx.x] : H.f_tile,
+                                 split_seed_cv(H, g), [&](int32_t t) { return ttot[t]; }, [&](int32_t, D4) {}, sh4);
+  if (threadIdx.x == 0) {
+    H.rs_j = settled ? H.n : tiles[H.f_tile].j0;
+    H.rs_cv[0] = acc.x;
+    H.rs_cv[1] = acc.y;
+    H.rs_cv[2] = acc.z;
+    H.rs_cv[3] = acc.m;
+  }
+}
+
+/* the recurrence of one iteration, one workgroup per active split halo;
+ * with spec, from where the speculative passes leave it */
 __global__ void __launch_bounds__(AHF_SPLIT_BLOCK) k_sp_rec(UnbindConst K, SplitHalo *sh, const double *dist,
-                                                            const double *phi, uint8_t *st, Staged g)
+                                                            const double *phi, uint8_t *st, Staged g, int spec)
 {
   SplitHalo &H = sh[blockIdx.x];
   if (!H.active) return;
@@ -1149,8 +1390,10 @@ __global__ void __launch_bounds__(AHF_SPLIT_BLOCK) k_sp_rec(UnbindConst K, Split
   const double gam = Nd * U / (1.0 - Nd * U);
   const D4 asum = {H.asum[0], H.asum[1], H.asum[2], 0.0};
   int sweeps = 0;
+  const RecResume R = {H.rs_j, D4{H.rs_cv[0], H.rs_cv[1], H.rs_cv[2], H.rs_cv[3]}, H.rs_unc};
   const int rep = rec_eval<AHF_SPLIT_BLOCK, AHF_UNBIND_IPT>(K, n, H.niter == 1 ? H.seed : 0, dist + o, phi + o,
-                                                            st + o, go, asum, gam, &Phi0, &sweeps);
+                                                            st + o, go, asum, gam, &Phi0, &sweeps,
+                                                            spec ? &R : nullptr);
   if (threadIdx.x == 0) {
     H.Phi0 = Phi0;
     H.replays += rep;
@@ -1194,20 +1437,35 @@ __global__ void __launch_bounds__(256) k_sp_cnt_tiles(const SplitHalo *__restric
   }
 }
 
-__global__ void k_sp_cnt_carry(SplitHalo *sh, int64_t nseg, const int32_t *__restrict__ seg_tile0,
-                               const int64_t *__restrict__ tcnt, const int64_t *__restrict__ tlast,
-                               int64_t *__restrict__ tcarry)
+/* per split halo (one workgroup each): each tile's first output slot, the
+ * bound count and 1 + the index of the last bound member */
+__global__ void __launch_bounds__(256) k_sp_cnt_carry(SplitHalo *sh, const int32_t *__restrict__ seg_tile0,
+                                                      const int64_t *__restrict__ tcnt,
+                                                      const int64_t *__restrict__ tlast, int64_t *__restrict__ tcarry)
 {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nseg || !sh[i].active) return;
-  int64_t acc = 0, last = 0;
-  for (int32_t t = seg_tile0[i]; t < seg_tile0[i + 1]; t++) {
-    tcarry[t] = acc;
-    acc += tcnt[t];
-    if (tlast[t] > last) last = tlast[t];
+  __shared__ int64_t shi[4];
+  __shared__ unsigned long long s_last;
+  const int64_t i = blockIdx.x;
+  if (!sh[i].active) return;
+  if (threadIdx.x == 0) s_last = 0ull;
+  int64_t acc = 0;
+  unsigned long long last = 0ull;
+  const int32_t t0 = seg_tile0[i], t1 = seg_tile0[i + 1];
+  for (int32_t b = t0; b < t1; b += 256) {
+    const int32_t t = b + (int32_t)threadIdx.x;
+    const int64_t c = t < t1 ? tcnt[t] : 0;
+    if (t < t1 && (unsigned long long)tlast[t] > last) last = (unsigned long long)tlast[t];
+    int64_t tot;
+    const int64_t ex = blk_excl_scan<256, int64_t>(c, &tot, shi);
+    if (t < t1) tcarry[t] = acc + ex;
+    acc += tot;
   }
-  sh[i].w = acc;
-  sh[i].last = last;
+  if (last) atomicMax(&s_last, last);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sh[i].w = acc;
+    sh[i].last = (int64_t)s_last;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_sp_scatter(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
@@ -1856,8 +2114,13 @@ static int split_unbind(ahfgpu_ctx *c, const UnbindConst &K, HaloSet *s, const i
   D4 *ttot = A.get<D4>(max_tiles), *tcar = A.get<D4>(max_tiles);
   int64_t *tcnt = A.get<int64_t>(max_tiles), *tlast = A.get<int64_t>(max_tiles), *tcarry = A.get<int64_t>(max_tiles);
   unsigned long long *tfirst = A.get<unsigned long long>(max_tiles);
-  if (!sh || !hid || !wloc || !seg_tile0 || !words || !tiles || !ttot || !tcar || !tcnt || !tlast || !tcarry || !tfirst)
+  uint8_t *tchg = A.get<uint8_t>(max_tiles), *tunc = A.get<uint8_t>(max_tiles);
+  if (!sh || !hid || !wloc || !seg_tile0 || !words || !tiles || !ttot || !tcar || !tcnt || !tlast || !tcarry || !tfirst ||
+      !tchg || !tunc)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (split unbinding)");
+  /* speculative passes before the sequential walk (ttot and tcar serve
+   * them once the potential's prefix sums are done) */
+  const int spec = c->split_spec < 0 ? 0 : c->split_spec > 0 ? c->split_spec : 3;
   hipStream_t s2 = c->st2;
   int32_t *ntp = words, *any = words + 1;
   const unsigned gs = nb_of(nsplit), nt = (unsigned)max_tiles, ns = (unsigned)nsplit;
@@ -1885,13 +2148,25 @@ static int split_unbind(ahfgpu_ctx *c, const UnbindConst &K, HaloSet *s, const i
   hipLaunchKernelGGL(k_sp_layout, dim3(1), dim3(256), 0, s2, sh, nsplit, 0, K.minpart, seg_tile0, ntp);
   hipLaunchKernelGGL(k_sp_fill_tiles, dim3(ns), dim3(256), 0, s2, seg_tile0, tiles);
   hipLaunchKernelGGL(k_sp_stage, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, s->buf_a, s->buf_b, s->cen, c->px, c->py,
-                     c->pz, c->vx, c->vy, c->vz, dist, g);
+                     c->pz, c->vx, c->vy, c->vz, dist, g, st);
   hipLaunchKernelGGL(k_sp_phi_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, dist, g, phi, ttot);
-  hipLaunchKernelGGL(k_sp_phi_carry, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0, ttot, tcar);
+  hipLaunchKernelGGL(k_sp_phi_carry, dim3(ns), dim3(256), 0, s2, sh, seg_tile0, ttot, tcar);
   hipLaunchKernelGGL(k_sp_phi_add, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, tcar, phi);
-  hipLaunchKernelGGL(k_sp_rec, dim3(ns), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, dist, phi, st, g);
+  if (spec) hipLaunchKernelGGL(k_sp_spec_init, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0);
+  for (int pass = 0; pass < spec; pass++) {
+    hipLaunchKernelGGL(k_sp_spec_tot, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, st, g, ttot);
+    hipLaunchKernelGGL(k_sp_spec_carry, dim3(ns), dim3(256), 0, s2, sh, seg_tile0, ttot, g, tcar);
+    hipLaunchKernelGGL(k_sp_spec_eval, dim3(nt), dim3(256), 0, s2, K, sh, tiles, ntp, dist, phi, st, g, tcar, tchg,
+                       tunc);
+    hipLaunchKernelGGL(k_sp_spec_advance, dim3(ns), dim3(256), 0, s2, sh, tchg, tunc);
+  }
+  if (spec) {
+    hipLaunchKernelGGL(k_sp_spec_tot, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, st, g, ttot);
+    hipLaunchKernelGGL(k_sp_spec_resume, dim3(ns), dim3(256), 0, s2, sh, seg_tile0, tiles, ttot, g);
+  }
+  hipLaunchKernelGGL(k_sp_rec, dim3(ns), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, dist, phi, st, g, spec > 0 ? 1 : 0);
   hipLaunchKernelGGL(k_sp_cnt_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, st, wloc, tcnt, tlast);
-  hipLaunchKernelGGL(k_sp_cnt_carry, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit, seg_tile0, tcnt, tlast, tcarry);
+  hipLaunchKernelGGL(k_sp_cnt_carry, dim3(ns), dim3(256), 0, s2, sh, seg_tile0, tcnt, tlast, tcarry);
   hipLaunchKernelGGL(k_sp_scatter, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, tcarry, st, wloc, s->buf_a, s->buf_b);
   hipLaunchKernelGGL(k_sp_update, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit, dist);
   hipLaunchKernelGGL(k_sp_any, dim3(1), dim3(256), 0, s2, sh, nsplit, any);

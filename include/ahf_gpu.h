@@ -318,6 +318,9 @@ typedef struct {
   int32_t kmax_small;  /* ... on smaller levels; 96 */
   int32_t march_zn;    /* deposit: target planes per march workgroup (8 or 16); by level size */
   int32_t graph_batch; /* split unbinding: iteration graph replays between done checks; 4 */
+  int32_t split_spec;  /* split unbinding: passes settling all tiles at once before the sequential walk
+                          resumes (-1: none); 3 */
+  int32_t pad;
 } ahfgpu_schedule;
 int ahfgpu_set_schedule(ahfgpu_ctx *ctx, const ahfgpu_schedule *s);
 int ahfgpu_get_schedule(ahfgpu_ctx *ctx, ahfgpu_schedule *s);

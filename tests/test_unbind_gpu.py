@@ -202,10 +202,13 @@ def test_ties_match_oracle(oracle, ctx, tmp_path, seed, big):
 
 @pytest.mark.parametrize("sched", [dict(split_min=4096, big_halo=1024, wave_halo=256, graph_batch=1),
                                    dict(split_min=-1, big_halo=8192, wave_halo=1024),
-                                   dict(split_min=64, graph_batch=8)])
+                                   dict(split_min=64, graph_batch=8),
+                                   dict(split_min=64, split_spec=-1), dict(split_min=64, split_spec=1),
+                                   dict(split_min=32, split_spec=16)])
 def test_schedule_variants(oracle, snapgen, ctx, tmp_path, sched):
     """every schedule (split path threshold, workgroup tiers, graph replays
-    between done checks) gives the oracle's haloes"""
+    between done checks, speculative passes of the split recurrence: none,
+    one, sixteen) gives the oracle's haloes"""
     snap = str(tmp_path / "s.snap")
     assert snapgen.ahf_snapgen_write(2, 2000, 1002, snap.encode()) == 0
     out = str(tmp_path / "o.halo")
