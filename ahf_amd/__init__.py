@@ -65,7 +65,7 @@ class Schedule(ctypes.Structure):
     _fields_ = [("split_min", ctypes.c_int64), ("prof_split", ctypes.c_int64), ("big_halo", ctypes.c_int64),
                 ("wave_halo", ctypes.c_int64), ("kmax_big", ctypes.c_int32), ("kmax_small", ctypes.c_int32),
                 ("march_zn", ctypes.c_int32), ("graph_batch", ctypes.c_int32), ("split_spec", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32), ("prof_tiled", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -43,10 +43,11 @@ CANDIDATES = {
     "graph_batch": [4, 2, 8],
     "split_spec": [3, 1, 2, 4, 6, -1],
     "prof_split": [2048, 512, 1024, 4096, 8192],
+    "prof_tiled": [32768, 8192, 16384, 65536, 131072, -1],
 }
 # which timing decides each field
 DECIDES = {"kmax_big": "amr", "kmax_small": "amr", "march_zn": "amr", "big_halo": "unbind", "wave_halo": "unbind",
-           "split_min": "unbind", "graph_batch": "unbind", "split_spec": "unbind", "prof_split": "profiles"}
+           "split_min": "unbind", "graph_batch": "unbind", "split_spec": "unbind", "prof_split": "profiles", "prof_tiled": "profiles"}
 
 
 def log(msg):

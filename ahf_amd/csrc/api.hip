@@ -79,6 +79,7 @@ static void load_schedule(ahfgpu_ctx *c)
   if (json_int(txt, "march_zn", &v)) sc.march_zn = (int32_t)v;
   if (json_int(txt, "graph_batch", &v)) sc.graph_batch = (int32_t)v;
   if (json_int(txt, "split_spec", &v)) sc.split_spec = (int32_t)v;
+  if (json_int(txt, "prof_tiled", &v)) sc.prof_tiled = v;
   if (ahfgpu_set_schedule(c, &sc) != 0) fprintf(stderr, "ahfgpu: schedule %s ignored: %s\n", path.c_str(), c->err);
 }
 
@@ -100,6 +101,7 @@ extern "C" int ahfgpu_set_schedule(ahfgpu_ctx *c, const ahfgpu_schedule *s)
   c->march_zn = s->march_zn;
   c->split_graph_batch = s->graph_batch;
   c->split_spec = s->split_spec;
+  c->prof_tiled = s->prof_tiled;
   return 0;
 }
 
@@ -116,6 +118,7 @@ extern "C" int ahfgpu_get_schedule(ahfgpu_ctx *c, ahfgpu_schedule *s)
   s->graph_batch = c->split_graph_batch;
   s->split_spec = c->split_spec;
   s->pad = 0;
+  s->prof_tiled = c->prof_tiled;
   return 0;
 }
 
@@ -137,6 +140,7 @@ extern "C" int ahfgpu_create(ahfgpu_ctx **out, int device)
       hipEventCreateWithFlags(&c->ev_s0, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_s1, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_s2, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_s3, hipEventDisableTiming) != hipSuccess ||
       hipMalloc((void **)&c->d_anomaly, 64) != hipSuccess || hipMalloc((void **)&c->d_sizes, 64) != hipSuccess ||
       hipHostMalloc((void **)&c->h_sizes, 64, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&c->h_words, 64, hipHostMallocDefault) != hipSuccess ||
@@ -188,7 +192,7 @@ extern "C" void ahfgpu_destroy(ahfgpu_ctx *c)
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   (void)hipStreamSynchronize(c->st2);
   (void)hipStreamSynchronize(c->st3);
-  hipEvent_t evs[] = {c->ev_a0, c->ev_a1, c->ev_u0, c->ev_u1, c->ev_s0, c->ev_s1, c->ev_s2};
+  hipEvent_t evs[] = {c->ev_a0, c->ev_a1, c->ev_u0, c->ev_u1, c->ev_s0, c->ev_s1, c->ev_s2, c->ev_s3};
   for (auto e : evs)
     if (e) (void)hipEventDestroy(e);
   if (c->st) (void)hipStreamDestroy(c->st);

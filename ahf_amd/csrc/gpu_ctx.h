@@ -123,7 +123,7 @@ struct ahfgpu_ctx {
   hipStream_t st = nullptr;
   hipStream_t st2 = nullptr; /* the split unbinding path beside st (default priority) */
   hipStream_t st3 = nullptr; /* the one-wavefront unbinding tier beside st (default priority) */
-  hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr, ev_s2 = nullptr;
+  hipEvent_t ev_s0 = nullptr, ev_s1 = nullptr, ev_s2 = nullptr, ev_s3 = nullptr;
   int64_t split_min = 0;     /* haloes of at least this many members take the split path; 0: default */
   int64_t prof_split = 0;    /* haloes of at least this many members are profiled by four wavefronts; 0: default */
   /* the rest of the schedule (ahfgpu_schedule); 0: the default */
@@ -151,6 +151,7 @@ struct ahfgpu_ctx {
   int64_t *h_sizes = nullptr;   /* pinned host copy of d_sizes             */
   int32_t *h_words = nullptr;   /* pinned host words (the split path's done flag) */
   int split_spec = 0;           /* speculative passes per split iteration; 0: 3, -1: none */
+  int64_t prof_tiled = 0;       /* HaloProfiles over tiles from this many members; 0: default, -1: none */
   int split_graph_batch = 0;    /* graph replays between done checks; 0: 4 */
   int split_graph_launches = 0; /* replays of the last split unbinding */
   void *halo_set = nullptr; /* resident halo lists (halo_set.h) */

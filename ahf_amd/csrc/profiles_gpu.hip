@@ -441,6 +441,700 @@ __device__ void find_peaks(const double *__restrict__ dr2, const double *__restr
   __syncthreads();
 }
 
+/* binning_parameter (specific.c:259-319) by one wavefront: the first
+ * member from MINPART/10 on (below the last) whose distance is not below
+ * MACHINE_ZERO, else half the last member's distance; distances with |d|
+ * folded to 1-|d| beyond half the box; rad[0..nbins) the bin radii */
+__device__ void prf_radii(const ProfConst &K, int64_t n, const uint32_t *__restrict__ ip, double Xc, double Yc,
+                          double Zc, int nbins, const float *__restrict__ px, const float *__restrict__ py,
+                          const float *__restrict__ pz, double *rad)
+{
+  const int lane = threadIdx.x & 63;
+  auto dist_fold = [&](int64_t j) {
+    const uint32_t p = ip[j];
+    double dX = fabs((double)px[p] - Xc), dY = fabs((double)py[p] - Yc), dZ = fabs((double)pz[p] - Zc);
+    if (dX > 0.5) dX = 1.0 - dX;
+    if (dY > 0.5) dY = 1.0 - dY;
+    if (dZ > 0.5) dZ = 1.0 - dZ;
+    return sqrt(dX * dX + dY * dY + dZ * dZ);
+  };
+  const double dist_max = dist_fold(n - 1);
+  double dist_min = dist_max / 2.;
+  for (int64_t j0 = (int64_t)floor(((double)K.minpart / 10.) + 0.5); j0 < n - 1; j0 += 64) {
+    const int64_t j = j0 + lane;
+    const double d = j < n - 1 ? dist_fold(j) : 0.0;
+    const uint64_t m = __ballot(j < n - 1 && !(d < PRF_MZERO));
+    if (m) {
+      dist_min = __shfl(d, __ffsll((unsigned long long)m) - 1, 64);
+      break;
+    }
+  }
+  const double ldmin = log10(dist_min), ldmax = log10(dist_max);
+  const double ldr = (ldmax - ldmin) / (double)nbins;
+  if (lane < nbins) rad[lane] = lane == nbins - 1 ? dist_max + PRF_ZERO : pow(10., ldmin + ((double)lane + 1) * ldr);
+}
+
+/* ------------------------------------------- the walk over the device */
+
+/*
+ * Haloes of prof_tiled members or more are walked by kernels over tiles of
+ * 1024 members spread over the device, not by one workgroup.  Everything
+ * of the walk but its running sums is a function of one member or a
+ * running maximum over the members before it (the bins a member closes,
+ * the last member beyond MACHINE_ZERO), so it runs per member after
+ * prefix maxima over the tiles.  The running sums keep AHF's order and
+ * roundings: each is one lane of a wavefront that advances all of them at
+ * once, their terms staged in LDS by other wavefronts (k_prf_lanes).  The bulk velocity's sums come first,
+ * the terms that depend on them next, then the other sums, the potential
+ * energy's riding on the potential's.  k_profiles then takes the bins,
+ * totals and most bound member from here and does the rest.  Bin radii
+ * that decrease keep the sequential walk.
+ */
+#define PRF_TILE 1024
+#define AHF_PROF_TILED 32768
+
+struct PrfTiled {
+  int64_t h, n, P, to; /* halo, members, members the walk reaches, offset in the tiled scratch */
+  int64_t base;        /* the halo's offset in the member arrays */
+  int64_t mb;          /* most bound member, -1: none */
+  int64_t i_r2, i_rm;  /* r2's and R_max's members after NIGNORE (find_peaks) */
+  int64_t ib;          /* the first member at or beyond R_max */
+  double Xc, Yc, Zc, Phi0, Emin, Rmax;
+  int32_t nbins, ok, t0, t1; /* ok: radii that do not decrease; tiles [t0, t1) */
+  const uint32_t *ip;
+  double rad[64];
+  int64_t bclose[64];
+  double bt[64][BT_N];
+  double fin[16];
+};
+
+struct PrfTile {
+  int32_t seg, pad;
+  int64_t j0;
+};
+
+/* the per-member rows of the tiled walk (each Nt doubles or ints, Nt the
+ * tiled haloes' members end to end) */
+struct PrfRows {
+  double *dd, *gx, *gy, *gz, *v[3], *vr[3], *l[3], *tt, *phi, *phir, *com[3], *ine[6];
+  int32_t *c, *bp, *bi;
+};
+
+/* a running sum of the tiled walk: terms in (minus sub when sub_phi0),
+ * running values to out (or none), recorded at bin closings in column col
+ * (or none), the total to fin[fin] (or none) */
+struct PrfChain {
+  const double *in;
+  double *out;
+  int32_t col, fin, sub_phi0, pad;
+};
+
+/* per tiled halo (one wavefront): its members, centre, Phi0, bin radii */
+__global__ void __launch_bounds__(64) k_prf_bin(ProfConst K, PrfTiled *T, const int64_t *__restrict__ off,
+                                                const HaloOut *__restrict__ out, const double *__restrict__ phi_rvir,
+                                                const uint32_t *__restrict__ ip_in, const uint32_t *__restrict__ buf_a,
+                                                const uint32_t *__restrict__ buf_b, const double *__restrict__ cen,
+                                                const int32_t *__restrict__ nbins_h, const float *__restrict__ px,
+                                                const float *__restrict__ py, const float *__restrict__ pz)
+{
+  PrfTiled &H = T[blockIdx.x];
+  const int64_t h = H.h, base = off[h];
+  const uint32_t *ip = out ? (out[h].in_b ? buf_b : buf_a) + base : ip_in + base;
+  const double Phi0 = out ? out[h].Phi0 : phi_rvir[2 * h];
+  const int nbins = nbins_h[h];
+  const double Xc = cen[3 * h], Yc = cen[3 * h + 1], Zc = cen[3 * h + 2];
+  __shared__ double rad[64];
+  prf_radii(K, H.n, ip, Xc, Yc, Zc, nbins, px, py, pz, rad);
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const bool mono = __ballot(lane < nbins - 1 && !(rad[lane] <= rad[lane + 1])) == 0;
+  if (lane < nbins) H.rad[lane] = rad[lane];
+  if (lane == 0) {
+    H.ip = ip;
+    H.base = base;
+    H.Xc = Xc;
+    H.Yc = Yc;
+    H.Zc = Zc;
+    H.Phi0 = Phi0;
+    H.nbins = nbins;
+    H.ok = mono ? 1 : 0;
+    H.P = 0;
+    H.mb = -1;
+    H.Emin = 1e30;
+  }
+}
+
+/* per member: offsets, distance, velocity, the bins its distance reaches;
+ * per tile the largest of those and the last member beyond MACHINE_ZERO */
+__global__ void __launch_bounds__(256) k_prf_geom(const PrfTiled *__restrict__ T, const PrfTile *__restrict__ tiles,
+                                                  PrfRows R, const float *__restrict__ px,
+                                                  const float *__restrict__ py, const float *__restrict__ pz,
+                                                  const float *__restrict__ vx, const float *__restrict__ vy,
+                                                  const float *__restrict__ vz, int32_t *__restrict__ tc_max,
+                                                  int64_t *__restrict__ tfl_max)
+{
+  __shared__ int32_t s_c;
+  __shared__ unsigned long long s_fl;
+  const PrfTile Tl = tiles[blockIdx.x];
+  const PrfTiled &H = T[Tl.seg];
+  if (!H.ok) return;
+  if (threadIdx.x == 0) {
+    s_c = 0;
+    s_fl = 0ull;
+  }
+  __syncthreads();
+  int cmax = 0;
+  int64_t flmax = -1;
+  for (int i = 0; i < PRF_TILE / 256; i++) {
+    const int64_t j = Tl.j0 + (int64_t)i * 256 + threadIdx.x;
+    if (j >= H.n) break;
+    const int64_t e = H.to + j;
+    const uint32_t p = H.ip[j];
+    double dX = (double)px[p] - H.Xc, dY = (double)py[p] - H.Yc, dZ = (double)pz[p] - H.Zc;
+    if (dX > 0.5) dX -= 1.0;
+    if (dY > 0.5) dY -= 1.0;
+    if (dZ > 0.5) dZ -= 1.0;
+    if (dX < -0.5) dX += 1.0;
+    if (dY < -0.5) dY += 1.0;
+    if (dZ < -0.5) dZ += 1.0;
+    const double d = sqrt(dX * dX + dY * dY + dZ * dZ);
+    R.dd[e] = d;
+    R.gx[e] = dX;
+    R.gy[e] = dY;
+    R.gz[e] = dZ;
+    R.v[0][e] = (double)vx[p];
+    R.v[1][e] = (double)vy[p];
+    R.v[2][e] = (double)vz[p];
+    int c = 0;
+    while (c < H.nbins && H.rad[c] <= d) c++;
+    if (j == H.n - 1) c = H.nbins;
+    R.c[e] = c;
+    cmax = max(cmax, c);
+    if (d > PRF_MZERO) flmax = j;
+  }
+  if (cmax) atomicMax(&s_c, cmax);
+  if (flmax >= 0) atomicMax(&s_fl, (unsigned long long)(flmax + 1));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    tc_max[blockIdx.x] = s_c;
+    tfl_max[blockIdx.x] = (int64_t)s_fl - 1;
+  }
+}
+
+/* per tiled halo: the largest bin count and the last member beyond
+ * MACHINE_ZERO before each tile (exclusive prefix maxima, in place) */
+__global__ void k_prf_tscan(const PrfTiled *__restrict__ T, int64_t ntl, int32_t *__restrict__ tc_max,
+                            int64_t *__restrict__ tfl_max)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ntl || !T[s].ok) return;
+  int32_t c = 0;
+  int64_t fl = -1;
+  for (int32_t t = T[s].t0; t < T[s].t1; t++) {
+    const int32_t ct = tc_max[t];
+    const int64_t ft = tfl_max[t];
+    tc_max[t] = c;
+    tfl_max[t] = fl;
+    c = max(c, ct);
+    fl = max(fl, ft);
+  }
+}
+
+/* per member: the bins it closes, whether the walk reaches it, and its
+ * terms that do not depend on the bulk velocity (the potential's term,
+ * centre of mass, inertia tensor, V_c^2, rho r^2, ahf_halos.c:4333-4395);
+ * each thread takes four consecutive members */
+__global__ void __launch_bounds__(256) k_prf_terms1(PrfTiled *T, const PrfTile *__restrict__ tiles, PrfRows R,
+                                                    const int32_t *__restrict__ tc_before,
+                                                    const int64_t *__restrict__ tfl_before,
+                                                    double *__restrict__ rad_m, double *__restrict__ vc2,
+                                                    double *__restrict__ dr2)
+{
+  __shared__ int32_t s_wc[4];
+  __shared__ long long s_wf[4];
+  __shared__ unsigned long long s_P;
+  const PrfTile Tl = tiles[blockIdx.x];
+  PrfTiled &H = T[Tl.seg];
+  if (!H.ok) return;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0) s_P = 0ull;
+  const int64_t j0 = Tl.j0 + (int64_t)tid * 4;
+  /* the thread's own maxima, then exclusive maxima over the tile's
+   * threads (wavefront scans, then the wavefronts before) */
+  int cm = 0;
+  long long fm = -1;
+  for (int i = 0; i < 4; i++) {
+    const int64_t j = j0 + i;
+    if (j < H.n) {
+      cm = max(cm, R.c[H.to + j]);
+      if (R.dd[H.to + j] > PRF_MZERO) fm = j;
+    }
+  }
+  int ci = cm;
+  long long fi = fm;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int cu = __shfl_up(ci, d, 64);
+    const long long fu = __shfl_up(fi, d, 64);
+    if (lane >= d) {
+      ci = max(ci, cu);
+      fi = max(fi, fu);
+    }
+  }
+  if (lane == 63) {
+    s_wc[wid] = ci;
+    s_wf[wid] = fi;
+  }
+  __syncthreads();
+  int cx = tc_before[blockIdx.x];
+  long long fx = tfl_before[blockIdx.x];
+  for (int w = 0; w < wid; w++) {
+    cx = max(cx, s_wc[w]);
+    fx = max(fx, s_wf[w]);
+  }
+  {
+    const int cu = __shfl_up(ci, 1, 64);
+    const long long fu = __shfl_up(fi, 1, 64);
+    if (lane > 0) {
+      cx = max(cx, cu);
+      fx = max(fx, fu);
+    }
+  }
+  const double FourPiThird = 4. * PRF_PI / 3.;
+  /* bins closed before the thread's first member, the last member beyond
+   * MACHINE_ZERO before it, and that member's I */
+  int bcar = cx;
+  int64_t lf = fx;
+  auto I_of = [&](int64_t k) { return k >= 0 ? (double)(k + 1) / (R.dd[H.to + k] * R.dd[H.to + k]) : 0.0; };
+  double I_prev = I_of(lf);
+  int64_t last_proc = -1;
+  for (int i = 0; i < 4; i++) {
+    const int64_t j = j0 + i;
+    if (j >= H.n) break;
+    const int64_t e = H.to + j;
+    const int bp = bcar, bi = max(bcar, R.c[e]);
+    bcar = bi;
+    const double d = R.dd[e];
+    if (d > PRF_MZERO) lf = j;
+    const double I_now = I_of(lf);
+    const bool proc = bp < H.nbins;
+    R.bp[e] = bp;
+    R.bi[e] = bi;
+    if (proc) {
+      last_proc = j;
+      const double dX = R.gx[e], dY = R.gy[e], dZ = R.gz[e];
+      const double pre = j > 0 ? R.dd[e - 1] : 0.0;
+      const double M = (double)(j + 1);
+      R.com[0][e] = H.Xc + dX;
+      R.com[1][e] = H.Yc + dY;
+      R.com[2][e] = H.Zc + dZ;
+      R.ine[0][e] = dX * dX;
+      R.ine[1][e] = dY * dY;
+      R.ine[2][e] = dZ * dZ;
+      R.ine[3][e] = dX * dY;
+      R.ine[4][e] = dX * dZ;
+      R.ine[5][e] = dY * dZ;
+      R.phi[e] = d > PRF_MZERO ? ((I_now + I_prev) / 2.) * (d - pre) : -0.0; /* -0.0: an exact no-op */
+      rad_m[H.base + j] = d;
+      vc2[H.base + j] = M / d;
+      const double dV = FourPiThird * ((d * d * d) - (pre * pre * pre));
+      const double hm = (d + pre) / 2.;
+      dr2[H.base + j] = 1.0 / dV * (hm * hm);
+      for (int b = bp; b < bi; b++) H.bclose[b] = j;
+    } else {
+      /* members the walk does not reach read as AHF's zeroed arrays */
+      rad_m[H.base + j] = vc2[H.base + j] = dr2[H.base + j] = 0.0;
+    }
+    I_prev = I_now;
+  }
+  if (last_proc >= 0) atomicMax(&s_P, (unsigned long long)(last_proc + 1));
+  __syncthreads();
+  if (tid == 0 && s_P) atomicMax((unsigned long long *)&H.P, s_P);
+}
+
+/* per member the walk reaches: the angular momentum and kinetic energy
+ * terms about the running bulk velocity */
+__global__ void __launch_bounds__(256) k_prf_terms2(const PrfTiled *__restrict__ T, const PrfTile *__restrict__ tiles,
+                                                    PrfRows R, ProfConst K)
+{
+  const PrfTile Tl = tiles[blockIdx.x];
+  const PrfTiled &H = T[Tl.seg];
+  if (!H.ok) return;
+  const double hub = K.hubble, rf = K.r_fac, vf = K.v_fac;
+  for (int i = 0; i < PRF_TILE / 256; i++) {
+    const int64_t j = Tl.j0 + (int64_t)i * 256 + threadIdx.x;
+    if (j >= H.P) break;
+    const int64_t e = H.to + j;
+    const double M = (double)(j + 1);
+    const double dX = R.gx[e], dY = R.gy[e], dZ = R.gz[e];
+    double dVX = R.v[0][e] - R.vr[0][e] / M, dVY = R.v[1][e] - R.vr[1][e] / M, dVZ = R.v[2][e] - R.vr[2][e] / M;
+    R.l[0][e] = dY * dVZ - dZ * dVY;
+    R.l[1][e] = dZ * dVX - dX * dVZ;
+    R.l[2][e] = dX * dVY - dY * dVX;
+    dVX += hub * dX * rf / vf;
+    dVY += hub * dY * rf / vf;
+    dVZ += hub * dZ * rf / vf;
+    R.tt[e] = dVX * dVX + dVY * dVY + dVZ * dVZ;
+  }
+}
+
+/* running sums over the members the walk reaches, AHF's additions in AHF's
+ * order: one lane of one wavefront per sum, so that one FP64 addition
+ * advances all of them.  The wavefront's lanes read their terms, two
+ * members at a time and up to 24 members ahead, from columns in LDS
+ * (column c: sum c, PRF_LB members of a batch) and write the running
+ * values back over them.  Four more wavefronts stage the next batch in the
+ * other of two buffers while a batch is summed, and take the running
+ * values of the previous one: to the sums' rows (bulk velocity,
+ * potential), at the members that close bins to the bin records, the
+ * totals at the end.  Members past the last take -0.0, which leaves every
+ * sum as it is, so every batch has PRF_LB steps.  With E, one more lane
+ * sums the potential energy, Phi_j - Phi0 over the potential's running
+ * values Phi_j (ahf_halos.c:4383-4395), two batches behind: its terms are
+ * made from the potential's running values of the batch summed before. */
+#define PRF_LB 512       /* members per batch */
+#define PRF_LROWS 546    /* a column: the batch, 34 rows read ahead; 2 * 546 = 4 mod 64 dwords between columns */
+#define PRF_LHELP 256    /* lanes that stage and record */
+
+/* one batch of one lane's sum: PRF_LB members of its column, in place */
+__device__ __forceinline__ void prf_lane_batch(double *col, double &a)
+{
+  double2 x0[4], x1[4], x2[4], x3[4];
+  auto ld = [&](double2 *x, int r) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = *(const double2 *)(col + r + 2 * k);
+  };
+  auto run = [&](double2 *x, int r) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a += x[k].x;
+      x[k].x = a;
+      a += x[k].y;
+      x[k].y = a;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) *(double2 *)(col + r + 2 * k) = x[k];
+  };
+  ld(x0, 0);
+  ld(x1, 8);
+  ld(x2, 16);
+  ld(x3, 24);
+  for (int i = 0; i < PRF_LB; i += 32) {
+    /* rows i+32 .. i+63 are read ahead: at most PRF_LB + 31 < PRF_LROWS */
+    run(x0, i);
+    ld(x0, i + 32);
+    run(x1, i + 8);
+    ld(x1, i + 40);
+    run(x2, i + 16);
+    ld(x2, i + 48);
+    run(x3, i + 24);
+    ld(x3, i + 56);
+  }
+}
+
+template <int NC, bool E>
+__global__ void __launch_bounds__(64 + PRF_LHELP) k_prf_lanes(PrfTiled *T, const PrfChain *__restrict__ chains,
+                                                              const int32_t *__restrict__ bp,
+                                                              const int32_t *__restrict__ bi, int phi_c)
+{
+  constexpr int NL = NC + (E ? 1 : 0); /* columns: the sums, then the potential energy's */
+  constexpr int RPL = PRF_LB / PRF_LHELP;
+  __shared__ __align__(16) double s_v[2][NL][PRF_LROWS];
+  PrfTiled &H = T[blockIdx.x];
+  if (!H.ok) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t P = H.P, to = H.to;
+  const double Phi0 = H.Phi0;
+  const int64_t nb = (P + PRF_LB - 1) / PRF_LB;
+  /* batch b is summed at step b; its running values recorded at b + 1; the
+   * potential energy's terms of batch b made at b + 1, summed at b + 2 and
+   * recorded at b + 3 */
+  const int64_t nit = nb + (E ? 3 : 1);
+  if (w == 0) {
+    double a = 0.0;
+    __syncthreads(); /* batch 0 staged */
+    for (int64_t b = 0; b < nit; b++) {
+      if (lane < NL) prf_lane_batch(&s_v[b & 1][lane][0], a);
+      __syncthreads();
+    }
+    if (lane < NC && chains[lane].fin >= 0) H.fin[chains[lane].fin] = a;
+    return;
+  }
+  const int hl = threadIdx.x - 64;
+  /* global, not generic, accesses: generic ones count against the LDS
+   * counter too, and the barrier's wait on that would stall on the loads
+   * of the batch after next */
+  typedef const __attribute__((address_space(1))) double gcdouble;
+  typedef __attribute__((address_space(1))) double gdouble;
+  gcdouble *cin[NC];
+  gdouble *cout[NC];
+  int ccol[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    cin[c] = (gcdouble *)chains[c].in;
+    cout[c] = (gdouble *)chains[c].out;
+    ccol[c] = chains[c].col;
+  }
+  double nt[NC][RPL];
+  int2 bn[RPL], b0[RPL], bm1[RPL], bm2[RPL], bm3[RPL];
+  auto load = [&](int64_t B) {
+    if (B >= nb) return;
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+      const int64_t j = B * PRF_LB + k * PRF_LHELP + hl;
+      const int64_t ix = to + (j < P ? j : P - 1);
+#pragma unroll
+      for (int c = 0; c < NC; c++) nt[c][k] = cin[c][ix];
+      bn[k] = make_int2(bp[ix], bi[ix]);
+    }
+  };
+  auto stage = [&](int slot, int64_t B) {
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+      const int r = k * PRF_LHELP + hl;
+      const bool in = B < nb && B * PRF_LB + r < P;
+#pragma unroll
+      for (int c = 0; c < NC; c++) s_v[slot][c][r] = in ? nt[c][k] : -0.0;
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < RPL; k++) {
+    bm1[k] = bm2[k] = bm3[k] = make_int2(0, 0);
+    if (E) s_v[0][NC][k * PRF_LHELP + hl] = -0.0;
+  }
+  load(0);
+  stage(0, 0);
+#pragma unroll
+  for (int k = 0; k < RPL; k++) b0[k] = bn[k];
+  load(1);
+  __syncthreads();
+  for (int64_t b = 0; b < nit; b++) {
+    const int slot = (int)((b + 1) & 1);
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+      const int r = k * PRF_LHELP + hl;
+      /* the potential energy's running values of batch b - 3 at the bins
+       * their members close */
+      if (E && b >= 3 && (b - 3) * PRF_LB + r < P)
+        for (int x = bm3[k].x; x < bm3[k].y; x++) H.bt[x][BT_EP] = s_v[slot][NC][r];
+      /* the running values of batch b - 1 */
+      const bool prev = b >= 1 && b - 1 < nb && (b - 1) * PRF_LB + r < P;
+      if (prev) {
+        const int64_t e = to + (b - 1) * PRF_LB + r;
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+          if (cout[c]) cout[c][e] = s_v[slot][c][r];
+        for (int x = bm1[k].x; x < bm1[k].y; x++) {
+#pragma unroll
+          for (int c = 0; c < NC; c++)
+            if (ccol[c] >= 0) H.bt[x][ccol[c]] = s_v[slot][c][r];
+        }
+      }
+      if (E) s_v[slot][NC][r] = prev ? s_v[slot][phi_c][r] - Phi0 : -0.0;
+    }
+    stage(slot, b + 1);
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+      bm3[k] = bm2[k];
+      bm2[k] = bm1[k];
+      bm1[k] = b0[k];
+      b0[k] = bn[k];
+    }
+    load(b + 2);
+    __syncthreads();
+  }
+}
+
+/* per member the walk reaches: escape speed at the bins it closes, and
+ * per tile the first member at the lowest energy below 1e30 */
+__global__ void __launch_bounds__(256) k_prf_finish(PrfTiled *T, const PrfTile *__restrict__ tiles, PrfRows R,
+                                                    double *__restrict__ tEmin, int64_t *__restrict__ tmb)
+{
+  __shared__ double s_v[4];
+  __shared__ int64_t s_i[4];
+  const PrfTile Tl = tiles[blockIdx.x];
+  PrfTiled &H = T[Tl.seg];
+  if (!H.ok) return;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double bv = 1e30;
+  int64_t bl = -1;
+  for (int i = 0; i < PRF_TILE / 256; i++) {
+    const int64_t j = Tl.j0 + (int64_t)i * 256 + tid;
+    if (j >= H.P) break;
+    const int64_t e = H.to + j;
+    const double U = R.phir[e] - H.Phi0;
+    const double Ep = 0.5 * R.tt[e] + U;
+    for (int b = R.bp[e]; b < R.bi[e]; b++) H.bt[b][BT_VESC] = 2 * fabs(U);
+    if (Ep < bv) {
+      bv = Ep;
+      bl = j;
+    }
+  }
+  /* the first at the lowest: a lower index wins a tie */
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const double ov = __shfl_xor(bv, d, 64);
+    const int64_t ol = __shfl_xor(bl, d, 64);
+    if (ol >= 0 && (bl < 0 || ov < bv || (!(bv < ov) && ol < bl))) {
+      bv = ov;
+      bl = ol;
+    }
+  }
+  if (lane == 0) {
+    s_v[wid] = bv;
+    s_i[wid] = bl;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < 4; w++) {
+      const double ov = s_v[w];
+      const int64_t ol = s_i[w];
+      if (ol >= 0 && (bl < 0 || ov < bv || (!(bv < ov) && ol < bl))) {
+        bv = ov;
+        bl = ol;
+      }
+    }
+    tEmin[blockIdx.x] = bv;
+    tmb[blockIdx.x] = bl;
+  }
+}
+
+/* per tiled halo: the most bound member over its tiles */
+__global__ void k_prf_done(PrfTiled *T, int64_t ntl, const double *__restrict__ tEmin, const int64_t *__restrict__ tmb)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ntl || !T[s].ok) return;
+  PrfTiled &H = T[s];
+  double bv = 1e30;
+  int64_t bl = -1;
+  for (int32_t t = H.t0; t < H.t1; t++) {
+    const double ov = tEmin[t];
+    const int64_t ol = tmb[t];
+    if (ol >= 0 && (bl < 0 || ov < bv || (!(bv < ov) && ol < bl))) {
+      bv = ov;
+      bl = ol;
+    }
+  }
+  H.Emin = bv;
+  H.mb = bl;
+}
+
+/* r2's and R_max's peaks (find_peaks) over one tile of a halo walked over
+ * the device: the first maxima of the tile's t = member - NIGNORE in
+ * [j0, j0 + PRF_TILE), t < m - 1; tv/ti[2 tile + 0]: rho r^2 after three
+ * smoothing passes, [2 tile + 1]: V_c^2 after one */
+__global__ void __launch_bounds__(256) k_prf_peaks(const PrfTiled *__restrict__ T, const PrfTile *__restrict__ tiles,
+                                                   const double *__restrict__ vc2, const double *__restrict__ dr2,
+                                                   double *__restrict__ tv, int64_t *__restrict__ ti)
+{
+  __shared__ double s_v[2][4];
+  __shared__ int64_t s_i[2][4];
+  const PrfTile Tl = tiles[blockIdx.x];
+  const PrfTiled &H = T[Tl.seg];
+  if (!H.ok) return;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t m = H.n - PRF_NIGNORE;
+  const double *y3 = dr2 + H.base + PRF_NIGNORE, *y1 = vc2 + H.base + PRF_NIGNORE;
+  double bv3 = -10.0, bv1 = -10.0;
+  int64_t bi3 = -1, bi1 = -1;
+  for (int u = 0; u < PRF_TILE / 256; u++) {
+    const int64_t t = Tl.j0 + u * 256 + tid;
+    if (t >= m - 1) break;
+    const double v3 = m >= 3 ? smoothed_at<3>(y3, m, t) : y3[t];
+    const double v1 = m >= 3 ? smoothed_at<1>(y1, m, t) : y1[t];
+    if (v3 > bv3) {
+      bv3 = v3;
+      bi3 = t;
+    }
+    if (v1 > bv1) {
+      bv1 = v1;
+      bi1 = t;
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    first_max_merge(bv3, bi3, __shfl_xor(bv3, d, 64), __shfl_xor(bi3, d, 64));
+    first_max_merge(bv1, bi1, __shfl_xor(bv1, d, 64), __shfl_xor(bi1, d, 64));
+  }
+  if (lane == 0) {
+    s_v[0][wid] = bv3;
+    s_i[0][wid] = bi3;
+    s_v[1][wid] = bv1;
+    s_i[1][wid] = bi1;
+  }
+  __syncthreads();
+  if (tid < 2) {
+    double v = s_v[tid][0];
+    int64_t i = s_i[tid][0];
+    for (int w = 1; w < 4; w++) first_max_merge(v, i, s_v[tid][w], s_i[tid][w]);
+    tv[2 * blockIdx.x + tid] = v;
+    ti[2 * blockIdx.x + tid] = i;
+  }
+}
+
+/* per tiled halo (one wavefront): the peaks over its tiles, R_max */
+__global__ void __launch_bounds__(64) k_prf_peaks_done(PrfTiled *T, const double *__restrict__ rad_m,
+                                                       const double *__restrict__ tv, const int64_t *__restrict__ ti)
+{
+  PrfTiled &H = T[blockIdx.x];
+  if (!H.ok) return;
+  const int lane = threadIdx.x;
+  double bv3 = -10.0, bv1 = -10.0;
+  int64_t bi3 = -1, bi1 = -1;
+  for (int32_t t = H.t0 + lane; t < H.t1; t += 64) {
+    first_max_merge(bv3, bi3, tv[2 * t], ti[2 * t]);
+    first_max_merge(bv1, bi1, tv[2 * t + 1], ti[2 * t + 1]);
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    first_max_merge(bv3, bi3, __shfl_xor(bv3, d, 64), __shfl_xor(bi3, d, 64));
+    first_max_merge(bv1, bi1, __shfl_xor(bv1, d, 64), __shfl_xor(bi1, d, 64));
+  }
+  if (lane == 0) {
+    const int64_t m = H.n - PRF_NIGNORE;
+    H.i_r2 = bi3 >= 0 ? bi3 : m - 1;
+    H.i_rm = bi1 >= 0 ? bi1 : m - 1;
+    const double *x = rad_m + H.base + PRF_NIGNORE;
+    H.Rmax = (x[H.i_rm] + x[H.i_rm]) / 2;
+  }
+}
+
+/* per tile: the first member below the last at or beyond R_max
+ * (ahf_halos.c:4892-4894), INT64_MAX: none */
+__global__ void __launch_bounds__(256) k_prf_rmax(const PrfTiled *__restrict__ T, const PrfTile *__restrict__ tiles,
+                                                  const double *__restrict__ rad_m, int64_t *__restrict__ tib)
+{
+  __shared__ int64_t s_i[4];
+  const PrfTile Tl = tiles[blockIdx.x];
+  const PrfTiled &H = T[Tl.seg];
+  if (!H.ok) return;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int64_t f = INT64_MAX;
+  for (int u = PRF_TILE / 256 - 1; u >= 0; u--) {
+    const int64_t j = Tl.j0 + u * 256 + tid;
+    if (j < H.n - 1 && !(rad_m[H.base + j] < H.Rmax)) f = j;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) f = min(f, (int64_t)__shfl_xor(f, d, 64));
+  if (lane == 0) s_i[wid] = f;
+  __syncthreads();
+  if (tid == 0) tib[blockIdx.x] = min(min(s_i[0], s_i[1]), min(s_i[2], s_i[3]));
+}
+
+/* per tiled halo (one wavefront): the first member at or beyond R_max over
+ * its tiles, else the last */
+__global__ void __launch_bounds__(64) k_prf_rmax_done(PrfTiled *T, const int64_t *__restrict__ tib)
+{
+  PrfTiled &H = T[blockIdx.x];
+  if (!H.ok) return;
+  const int lane = threadIdx.x;
+  int64_t f = INT64_MAX;
+  for (int32_t t = H.t0 + lane; t < H.t1; t += 64) f = min(f, tib[t]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) f = min(f, (int64_t)__shfl_xor(f, d, 64));
+  if (lane == 0) H.ib = f < INT64_MAX ? f : H.n - 1;
+}
+
 /* ------------------------------------------------------------- kernel */
 
 /* one workgroup of NW wavefronts per halo: haloes list[blockIdx.x] (list:
@@ -454,7 +1148,8 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
     const int32_t *__restrict__ nbins_h, const int64_t *__restrict__ pofs, const float *__restrict__ px,
     const float *__restrict__ py, const float *__restrict__ pz, const float *__restrict__ vx,
     const float *__restrict__ vy, const float *__restrict__ vz, double *__restrict__ s_rad,
-    double *__restrict__ s_vc2, double *__restrict__ s_dr2, double *__restrict__ props, double *__restrict__ prof)
+    double *__restrict__ s_vc2, double *__restrict__ s_dr2, double *__restrict__ props, double *__restrict__ prof,
+    const PrfTiled *__restrict__ tl, const int32_t *__restrict__ tmap)
 {
   constexpr int C = 64 * NW;
   __shared__ ProfShared<NW> S;
@@ -486,30 +1181,13 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
    * MINPART/10 on (below the last) whose distance is not below MACHINE_ZERO,
    * else half the last member's distance; distances with |d| folded to
    * 1-|d| beyond half the box (the first wavefront; LDS to the others) */
-  auto dist_fold = [&](int64_t j) {
-    const uint32_t p = ip[j];
-    double dX = fabs((double)px[p] - Xc), dY = fabs((double)py[p] - Yc), dZ = fabs((double)pz[p] - Zc);
-    if (dX > 0.5) dX = 1.0 - dX;
-    if (dY > 0.5) dY = 1.0 - dY;
-    if (dZ > 0.5) dZ = 1.0 - dZ;
-    return sqrt(dX * dX + dY * dY + dZ * dZ);
-  };
-  if (wid == 0) {
-    const double dist_max = dist_fold(n - 1);
-    double dist_min = dist_max / 2.;
-    for (int64_t j0 = (int64_t)floor(((double)K.minpart / 10.) + 0.5); j0 < n - 1; j0 += 64) {
-      const int64_t j = j0 + lane;
-      const double d = j < n - 1 ? dist_fold(j) : 0.0;
-      const uint64_t m = __ballot(j < n - 1 && !(d < PRF_MZERO));
-      if (m) {
-        dist_min = __shfl(d, __ffsll((unsigned long long)m) - 1, 64);
-        break;
-      }
-    }
-    const double ldmin = log10(dist_min), ldmax = log10(dist_max);
-    const double ldr = (ldmax - ldmin) / (double)nbins;
-    if (lane < nbins)
-      S.rad[lane] = lane == nbins - 1 ? dist_max + PRF_ZERO : pow(10., ldmin + ((double)lane + 1) * ldr);
+  /* a halo walked over the device (k_prf_*) brings its radii, bins,
+   * totals and most bound member */
+  const PrfTiled *TP = (tmap && tmap[h] >= 0 && tl[tmap[h]].ok) ? &tl[tmap[h]] : nullptr;
+  if (TP) {
+    if (tid < nbins) S.rad[tid] = TP->rad[tid];
+  } else if (wid == 0) {
+    prf_radii(K, n, ip, Xc, Yc, Zc, nbins, px, py, pz, S.rad);
   }
   __syncthreads();
   const bool mono = __ballot(lane < nbins - 1 && !(S.rad[lane] <= S.rad[lane + 1])) == 0;
@@ -541,7 +1219,13 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
     nvz = vz[q];
   }
   uint32_t qn = C + tid < n ? ip[C + tid] : 0;
-  if constexpr (NW > 1) {
+  if (TP) {
+    P = TP->P;
+    mb = TP->mb;
+    for (int k = tid; k < 64 * BT_N; k += C) S.bt[k / BT_N][k % BT_N] = TP->bt[k / BT_N][k % BT_N];
+    if (tid < 64) S.bclose[tid] = TP->bclose[tid];
+    if (tid < 16) S.fin[tid] = TP->fin[tid];
+  } else if constexpr (NW > 1) {
     /* two chunks in flight: the bulk velocity's sums of chunk k+1 run on the
      * second wavefront while the first runs chunk k's other sums */
     struct Chunk {
@@ -973,8 +1657,9 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
   if (tid < 3) S.fin[9 + tid] = accV;
   }
   /* members the walk did not reach read as AHF's zeroed arrays */
-  for (int64_t j = P + tid; j < n; j += C) rad_m[j] = vc2[j] = dr2[j] = 0.0;
-  if (tid >= 5 && tid < 8) S.fin[tid - 5] = accC;
+  if (!TP)
+    for (int64_t j = P + tid; j < n; j += C) rad_m[j] = vc2[j] = dr2[j] = 0.0;
+  if (!TP && tid >= 5 && tid < 8) S.fin[tid - 5] = accC;
   __syncthreads();
 
   /* the bins */
@@ -1014,12 +1699,17 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
   const int64_t m = n - PRF_NIGNORE;
   const double *x = rad_m + PRF_NIGNORE;
   int64_t i_r2, i_rm;
-  find_peaks<NW>(dr2 + PRF_NIGNORE, vc2 + PRF_NIGNORE, m, S, &i_r2, &i_rm);
+  if (TP) {
+    i_r2 = TP->i_r2;
+    i_rm = TP->i_rm;
+  } else {
+    find_peaks<NW>(dr2 + PRF_NIGNORE, vc2 + PRF_NIGNORE, m, S, &i_r2, &i_rm);
+  }
   const double r2 = (x[i_r2] + x[i_r2]) / 2;
   const double R_max = (x[i_rm] + x[i_rm]) / 2;
   /* the first member at or beyond R_max (ahf_halos.c:4892-4894) */
-  int64_t ib = n - 1;
-  for (int64_t j0 = 0; j0 < n - 1; j0 += C) {
+  int64_t ib = TP ? TP->ib : n - 1;
+  for (int64_t j0 = 0; !TP && j0 < n - 1; j0 += C) {
     const int64_t j = j0 + tid;
     const uint64_t mk = __ballot(j < n - 1 && !(rad_m[j] < R_max));
     if (lane == 0) S.hm[wid] = mk;
@@ -1200,6 +1890,64 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
   std::stable_sort(bigl.begin(), bigl.end(), [&](int32_t x, int32_t y) { return hn[(size_t)x] > hn[(size_t)y]; });
   const int64_t nbig = (int64_t)bigl.size();
   int32_t *d_big = nbig ? A.get<int32_t>(nbig) : nullptr;
+  /* of those, haloes of prof_tiled members or more are walked over the
+   * device (k_prf_*); tiles of PRF_TILE members, rows end to end */
+  const int64_t tiled = c->prof_tiled == 0 ? AHF_PROF_TILED : c->prof_tiled < 0 ? INT64_MAX : c->prof_tiled;
+  std::vector<PrfTiled> htl;
+  std::vector<PrfTile> htile;
+  std::vector<int32_t> tmap_h;
+  int64_t Nt = 0;
+  for (int32_t h : bigl) {
+    const int64_t n = hn[(size_t)h];
+    if (n < tiled) continue;
+    if (tmap_h.empty()) tmap_h.assign((size_t)nh, -1);
+    PrfTiled T;
+    memset(&T, 0, sizeof(T));
+    T.h = h;
+    T.n = n;
+    T.to = Nt;
+    T.t0 = (int32_t)htile.size();
+    for (int64_t j0 = 0; j0 < n; j0 += PRF_TILE) htile.push_back(PrfTile{(int32_t)htl.size(), 0, j0});
+    T.t1 = (int32_t)htile.size();
+    tmap_h[(size_t)h] = (int32_t)htl.size();
+    htl.push_back(T);
+    Nt += n;
+  }
+  const int64_t ntl = (int64_t)htl.size(), ntile = (int64_t)htile.size();
+  PrfTiled *d_tl = nullptr;
+  PrfTile *d_tile = nullptr;
+  int32_t *d_tmap = nullptr, *tcmax = nullptr;
+  int64_t *tflmax = nullptr, *tmb = nullptr;
+  double *tEmin = nullptr, *tpv = nullptr;
+  int64_t *tpi = nullptr, *tib = nullptr;
+  PrfChain *d_chains = nullptr;
+  PrfRows R;
+  memset(&R, 0, sizeof(R));
+  if (ntl) {
+    d_tl = A.get<PrfTiled>(ntl);
+    d_tile = A.get<PrfTile>(ntile);
+    d_tmap = A.get<int32_t>(nh);
+    tcmax = A.get<int32_t>(ntile);
+    tflmax = A.get<int64_t>(ntile);
+    tmb = A.get<int64_t>(ntile);
+    tEmin = A.get<double>(ntile);
+    tpv = A.get<double>(2 * ntile);
+    tpi = A.get<int64_t>(2 * ntile);
+    tib = A.get<int64_t>(ntile);
+    d_chains = A.get<PrfChain>(17);
+    double *rows = A.get<double>(29 * Nt);
+    int32_t *irows = A.get<int32_t>(3 * Nt);
+    if (!d_tl || !d_tile || !d_tmap || !tcmax || !tflmax || !tmb || !tEmin || !tpv || !tpi || !tib || !d_chains ||
+        !rows || !irows)
+      return c->fail(AHFGPU_ENOMEM, "device arena exhausted (profiles over the device)");
+    double *w = rows;
+    auto take = [&]() { double *r = w; w += Nt; return r; };
+    R.dd = take(); R.gx = take(); R.gy = take(); R.gz = take();
+    for (int k = 0; k < 3; k++) { R.v[k] = take(); R.vr[k] = take(); R.l[k] = take(); R.com[k] = take(); }
+    R.tt = take(); R.phi = take(); R.phir = take();
+    for (int k = 0; k < 6; k++) R.ine[k] = take();
+    R.c = irows; R.bp = irows + Nt; R.bi = irows + 2 * Nt;
+  }
   ProfResult *r = (ProfResult *)c->prof_result;
   if (!r) c->prof_result = r = new ProfResult();
   r->nh = nh;
@@ -1211,6 +1959,25 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     hipMemcpyAsync(d_nb, nb.data(), (size_t)nh * 4, hipMemcpyHostToDevice, c->st);
     hipMemcpyAsync(d_pofs, pofs.data(), (size_t)nh * 8, hipMemcpyHostToDevice, c->st);
     if (nbig) hipMemcpyAsync(d_big, bigl.data(), (size_t)nbig * 4, hipMemcpyHostToDevice, c->st);
+    if (ntl) {
+      /* the running sums: bulk velocity (3), then angular momentum (3),
+       * kinetic energy, potential (with the potential energy over its
+       * running values), centre of mass (3), inertia tensor (6) */
+      std::vector<PrfChain> ch;
+      for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.v[k], R.vr[k], -1, 9 + k, 0, 0});
+      for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.l[k], nullptr, BT_LX + k, -1, 0, 0});
+      ch.push_back(PrfChain{R.tt, nullptr, BT_T, -1, 0, 0});
+      ch.push_back(PrfChain{R.phi, R.phir, -1, -1, 0, 0});
+      for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.com[k], nullptr, -1, k, 0, 0});
+      for (int k = 0; k < 6; k++) ch.push_back(PrfChain{R.ine[k], nullptr, BT_A11 + k, -1, 0, 0});
+      hipMemcpyAsync(d_chains, ch.data(), ch.size() * sizeof(PrfChain), hipMemcpyHostToDevice, c->st);
+      hipMemcpyAsync(d_tl, htl.data(), (size_t)ntl * sizeof(PrfTiled), hipMemcpyHostToDevice, c->st);
+      hipMemcpyAsync(d_tile, htile.data(), (size_t)ntile * sizeof(PrfTile), hipMemcpyHostToDevice, c->st);
+      hipMemcpyAsync(d_tmap, tmap_h.data(), (size_t)nh * 4, hipMemcpyHostToDevice, c->st);
+      /* the staging buffers are read before the host vectors go */
+      if (hipStreamSynchronize(c->st) != hipSuccess)
+        return c->fail(AHFGPU_EHIP, "profiles: %s", hipGetErrorString(hipGetLastError()));
+    }
     if (phi0_rvir) hipMemcpyAsync(d_pr, phi0_rvir, (size_t)nh * 16, hipMemcpyHostToDevice, c->st);
     hipMemsetAsync(r->props, 0, (size_t)nh * PP_N * 8, c->st);
     ProfConst K{p->r_fac, p->v_fac, p->m_fac, p->phi_fac, p->hubble, p->minpart};
@@ -1218,15 +1985,52 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     if (nbig) {
       hipEventRecord(c->ev_s0, c->st);
       hipStreamWaitEvent(c->st2, c->ev_s0, 0);
-      hipLaunchKernelGGL(k_profiles<AHF_PROF_BIG_NW>, dim3((unsigned)nbig), dim3(64 * AHF_PROF_BIG_NW), 0, c->st2, K,
-                         s->off, d_big, big, out, d_pr,
-                         s->ip_in, s->buf_a, s->buf_b, s->cen, d_nb, d_pofs, c->px, c->py, c->pz, c->vx, c->vy,
-                         c->vz, sc, sc + ne, sc + 2 * ne, r->props, r->prof);
+      if (ntl) {
+        hipStream_t s2 = c->st2;
+        const dim3 gt((unsigned)ntile), gh((unsigned)((ntl + 255) / 256));
+        hipLaunchKernelGGL(k_prf_bin, dim3((unsigned)ntl), dim3(64), 0, s2, K, d_tl, s->off, out, d_pr, s->ip_in,
+                           s->buf_a, s->buf_b, s->cen, d_nb, c->px, c->py, c->pz);
+        hipLaunchKernelGGL(k_prf_geom, gt, dim3(256), 0, s2, d_tl, d_tile, R, c->px, c->py, c->pz, c->vx, c->vy,
+                           c->vz, tcmax, tflmax);
+        hipLaunchKernelGGL(k_prf_tscan, gh, dim3(256), 0, s2, d_tl, ntl, tcmax, tflmax);
+        hipLaunchKernelGGL(k_prf_terms1, gt, dim3(256), 0, s2, d_tl, d_tile, R, tcmax, tflmax, sc, sc + ne,
+                           sc + 2 * ne);
+        /* r2, R_max and the first member beyond it on the third stream,
+         * beside the running sums */
+        hipEventRecord(c->ev_s2, s2);
+        hipStreamWaitEvent(c->st3, c->ev_s2, 0);
+        hipLaunchKernelGGL(k_prf_peaks, gt, dim3(256), 0, c->st3, d_tl, d_tile, sc + ne, sc + 2 * ne, tpv, tpi);
+        hipLaunchKernelGGL(k_prf_peaks_done, dim3((unsigned)ntl), dim3(64), 0, c->st3, d_tl, sc, tpv, tpi);
+        hipLaunchKernelGGL(k_prf_rmax, gt, dim3(256), 0, c->st3, d_tl, d_tile, sc, tib);
+        hipLaunchKernelGGL(k_prf_rmax_done, dim3((unsigned)ntl), dim3(64), 0, c->st3, d_tl, tib);
+        hipEventRecord(c->ev_s3, c->st3);
+        hipLaunchKernelGGL((k_prf_lanes<3, false>), dim3((unsigned)ntl), dim3(64 + PRF_LHELP), 0, s2, d_tl, d_chains,
+                           R.bp, R.bi, 0);
+        hipLaunchKernelGGL(k_prf_terms2, gt, dim3(256), 0, s2, d_tl, d_tile, R, K);
+        /* the potential energy's sum over the potential's (the fifth of
+         * these) */
+        hipLaunchKernelGGL((k_prf_lanes<14, true>), dim3((unsigned)ntl), dim3(64 + PRF_LHELP), 0, s2, d_tl,
+                           d_chains + 3, R.bp, R.bi, 4);
+        hipLaunchKernelGGL(k_prf_finish, gt, dim3(256), 0, s2, d_tl, d_tile, R, tEmin, tmb);
+        hipLaunchKernelGGL(k_prf_done, gh, dim3(256), 0, s2, d_tl, ntl, tEmin, tmb);
+        hipStreamWaitEvent(s2, c->ev_s3, 0);
+      }
+      /* the haloes walked over the device (the first ntl of the list) after
+       * their walk; the others on the first stream meanwhile */
+      auto big_launch = [&](hipStream_t st, int64_t i0, int64_t cnt) {
+        hipLaunchKernelGGL(k_profiles<AHF_PROF_BIG_NW>, dim3((unsigned)cnt), dim3(64 * AHF_PROF_BIG_NW), 0, st, K,
+                           s->off, d_big + i0, big, out, d_pr, s->ip_in, s->buf_a, s->buf_b, s->cen, d_nb, d_pofs,
+                           c->px, c->py, c->pz, c->vx, c->vy, c->vz, sc, sc + ne, sc + 2 * ne, r->props, r->prof,
+                           d_tl, d_tmap);
+      };
+      if (ntl) big_launch(c->st2, 0, ntl);
+      if (nbig > ntl) big_launch(ntl ? c->st : c->st2, ntl, nbig - ntl);
       hipEventRecord(c->ev_s1, c->st2);
     }
     hipLaunchKernelGGL(k_profiles<1>, dim3((unsigned)nh), dim3(64), 0, c->st, K, s->off, s->order, big, out, d_pr,
                        s->ip_in, s->buf_a, s->buf_b, s->cen, d_nb, d_pofs, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
-                       sc, sc + ne, sc + 2 * ne, r->props, r->prof);
+                       sc, sc + ne, sc + 2 * ne, r->props, r->prof, (const PrfTiled *)nullptr,
+                       (const int32_t *)nullptr);
     if (nbig) hipStreamWaitEvent(c->st, c->ev_s1, 0);
     c->timer_end(e0, TIMER_PROFILES, 0.0);
   }

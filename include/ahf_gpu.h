@@ -321,6 +321,8 @@ typedef struct {
   int32_t split_spec;  /* split unbinding: passes settling all tiles at once before the sequential walk
                           resumes (-1: none); 3 */
   int32_t pad;
+  int64_t prof_tiled;  /* HaloProfiles: haloes of this many members or more walked by kernels over the
+                          whole device, only their running sums sequential (-1: none); 32768 */
 } ahfgpu_schedule;
 int ahfgpu_set_schedule(ahfgpu_ctx *ctx, const ahfgpu_schedule *s);
 int ahfgpu_get_schedule(ahfgpu_ctx *ctx, ahfgpu_schedule *s);

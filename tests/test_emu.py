@@ -65,6 +65,21 @@ def test_wavefront_fold_replay(tmp_path):
         assert '"mismatches": 0' in r.stdout
 
 
+def test_fold64_replay(tmp_path):
+    """an FP64 running-sum fold (integer steps on a binade's grid,
+    exact ties by parity, plain steps across binades and from zero or
+    subnormals) replayed on the host against sequential FP64 additions, on
+    random and adversarial runs, terms aimed at binade boundaries among
+    them; any differing bit fails"""
+    exe = str(tmp_path / "fold64_check")
+    subprocess.run(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-Wno-unused-function", "-o", exe,
+                    os.path.join(HERE, "emu", "fold64_check.c"), "-lm"], check=True)
+    for seed in (1, 2):
+        r = subprocess.run([exe, "300000", str(seed)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:] + r.stdout
+        assert '"mismatches": 0' in r.stdout
+
+
 def test_indexx_level_replay(oracle, tmp_path):
     """the radial sort's level-synchronous indexx (every segment of every
     halo with ties partitioned at once, ahf_amd/csrc/nr_order.h) replayed on

@@ -68,13 +68,30 @@ def _constructed(oracle, snap, tmp_path, gather):
     return dumps.read_halos(out)
 
 
+def _set_split(ctx, split):
+    """split: an int, the four-wavefront threshold (0 default, -1 none);
+    "tN": haloes of N members or more walked over the device (k_prf_*)"""
+    if isinstance(split, str):
+        n = int(split[1:])
+        ctx.set_profile_split(n)
+        ctx.set_schedule(prof_tiled=n)
+    else:
+        ctx.set_profile_split(split)
+
+
+def _reset_split(ctx):
+    ctx.set_profile_split(0)
+    ctx.set_schedule(prof_tiled=0)
+
+
 CASES = [(2, 800, 1002, 0), (0, 64, 1000, 0), (3, 48, 5, 0), (2, 2000, 1002, 0), (2, 400, 1002, 1),
          (1, 64, 1001, 0), (4, 96, 1004, 0)]
 
 
 # split: 0 the default (haloes of 2048 members or more on four wavefronts),
-# 64 nearly every halo on four wavefronts, -1 every halo on one
-@pytest.mark.parametrize("split", [0, 64, -1])
+# 64 nearly every halo on four wavefronts, -1 every halo on one, t64 nearly
+# every halo walked over the device
+@pytest.mark.parametrize("split", [0, 64, -1, "t64"])
 @pytest.mark.parametrize("cfg,side,seed,gather", CASES)
 def test_profiles_of_given_haloes(oracle, snapgen, ctx, tmp_path, cfg, side, seed, gather, split):
     """the kernel on the oracle's constructed haloes against the oracle's
@@ -87,13 +104,15 @@ def test_profiles_of_given_haloes(oracle, snapgen, ctx, tmp_path, cfg, side, see
     assert oracle.oracle_profiles_dump(snap.encode(), o.encode(), 8, gather) == 0
     ref = dumps.read_profiles(o)
     print(f"[profiles-gpu] cfg{cfg} side {side} gather {gather} split {split}: {s.nhalos} haloes", flush=True)
-    ctx.set_profile_split(split)
-    ctx.set_particles(s.pos, s.mom)
-    ip = np.concatenate([x for x in c.ipart] + [np.zeros(0, np.uint64)])
-    ctx.set_halos(s.centre, c.npart, ip)
-    nb, props, prof = ctx.halo_profiles(s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble,
-                                        phi0_rvir=np.stack([c.Phi0, c.R_vir], axis=1))
-    ctx.set_profile_split(0)
+    _set_split(ctx, split)
+    try:
+        ctx.set_particles(s.pos, s.mom)
+        ip = np.concatenate([x for x in c.ipart] + [np.zeros(0, np.uint64)])
+        ctx.set_halos(s.centre, c.npart, ip)
+        nb, props, prof = ctx.halo_profiles(s.minpart, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble,
+                                            phi0_rvir=np.stack([c.Phi0, c.R_vir], axis=1))
+    finally:
+        _reset_split(ctx)
     _compare(ref, nb, props, prof, f"cfg{cfg}/{side}/g{gather}/split{split}")
 
 
@@ -131,7 +150,7 @@ def _sorted_halo(oracle, s, h):
     return ip
 
 
-@pytest.mark.parametrize("split", [0, 6])
+@pytest.mark.parametrize("split", [0, 6, "t6"])
 def test_profiles_edge_cases(oracle, ctx, tmp_path, split):
     """lists in reverse order (bin radii that decrease: the sequential bin
     automaton), a member on the centre (no potential step), members at
@@ -164,11 +183,13 @@ def test_profiles_edge_cases(oracle, ctx, tmp_path, split):
     ref = _oracle_profile(oracle, s, given, phi_rvir, minpart=6)
     assert list(ref.nbins[2:4]) == [2, 0]
     print(f"[profiles-gpu] edge cases, split {split}", flush=True)
-    ctx.set_profile_split(split)
-    ctx.set_particles(s.pos, s.mom)
-    ctx.set_halos(s.centre, np.array(sizes, np.uint64), np.concatenate(given))
-    nb, props, prof = ctx.halo_profiles(6, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble, phi0_rvir=phi_rvir)
-    ctx.set_profile_split(0)
+    _set_split(ctx, split)
+    try:
+        ctx.set_particles(s.pos, s.mom)
+        ctx.set_halos(s.centre, np.array(sizes, np.uint64), np.concatenate(given))
+        nb, props, prof = ctx.halo_profiles(6, s.r_fac, s.v_fac, s.pmass, s.phi_fac, s.hubble, phi0_rvir=phi_rvir)
+    finally:
+        _reset_split(ctx)
     _compare(ref, nb, props, prof, f"edge cases, split {split}")
 
 
