@@ -484,9 +484,10 @@ __device__ void prf_radii(const ProfConst &K, int64_t n, const uint32_t *__restr
  * the last member beyond MACHINE_ZERO), so it runs per member after
  * prefix maxima over the tiles.  The running sums keep AHF's order and
  * roundings: each is one lane of a wavefront that advances all of them at
- * once, their terms staged in LDS by other wavefronts (k_prf_lanes).  The bulk velocity's sums come first,
- * the terms that depend on them next, then the other sums, the potential
- * energy's riding on the potential's.  k_profiles then takes the bins,
+ * once, their terms staged in LDS by other wavefronts (k_prf_lanes).  The
+ * bulk velocity's and the potential's sums come first, the terms that
+ * depend on them next, then the other sums, the potential energy's over
+ * the potential's running values among them.  k_profiles then takes the bins,
  * totals and most bound member from here and does the rest.  Bin radii
  * that decrease keep the sequential walk.
  */
@@ -520,7 +521,7 @@ struct PrfRows {
   int32_t *c, *bp, *bi;
 };
 
-/* a running sum of the tiled walk: terms in (minus sub when sub_phi0),
+/* a running sum of the tiled walk: terms in (minus Phi0 when sub_phi0),
  * running values to out (or none), recorded at bin closings in column col
  * (or none), the total to fin[fin] (or none) */
 struct PrfChain {
@@ -780,169 +781,236 @@ __global__ void __launch_bounds__(256) k_prf_terms2(const PrfTiled *__restrict__
 
 /* running sums over the members the walk reaches, AHF's additions in AHF's
  * order: one lane of one wavefront per sum, so that one FP64 addition
- * advances all of them.  The wavefront's lanes read their terms, two
- * members at a time and up to 24 members ahead, from columns in LDS
- * (column c: sum c, PRF_LB members of a batch) and write the running
- * values back over them.  Four more wavefronts stage the next batch in the
- * other of two buffers while a batch is summed, and take the running
- * values of the previous one: to the sums' rows (bulk velocity,
- * potential), at the members that close bins to the bin records, the
- * totals at the end.  Members past the last take -0.0, which leaves every
- * sum as it is, so every batch has PRF_LB steps.  With E, one more lane
- * sums the potential energy, Phi_j - Phi0 over the potential's running
- * values Phi_j (ahf_halos.c:4383-4395), two batches behind: its terms are
- * made from the potential's running values of the batch summed before. */
+ * advances all of them.  The wavefront's lanes only read: their terms, up
+ * to 32 members ahead, from columns in LDS (column c: sum c, PRF_LB
+ * members of a batch), and write the running value before every
+ * LSUB-th member (a checkpoint).  Four more wavefronts, each owning a
+ * quarter of the batch's rows, stage the next batch in the other of two
+ * buffers while a batch is summed, and finish the previous one from its
+ * checkpoints: with OUT, by replaying their quarter in full (the same
+ * additions from the same value) to the sums' rows; else only up to the
+ * members that close bins, for the bin records.  Members past the last
+ * take -0.0, which leaves every sum as it is.  A sum with sub_phi0 adds
+ * its term minus Phi0 (the potential energy over the potential's running
+ * values, ahf_halos.c:4383-4395). */
 #define PRF_LB 512       /* members per batch */
+#define PRF_LQ 128       /* rows a helper wavefront owns */
 #define PRF_LROWS 546    /* a column: the batch, 34 rows read ahead; 2 * 546 = 4 mod 64 dwords between columns */
-#define PRF_LHELP 256    /* lanes that stage and record */
+#define PRF_LHELP 256    /* lanes that stage and finish */
 
-/* one batch of one lane's sum: PRF_LB members of its column, in place */
-__device__ __forceinline__ void prf_lane_batch(double *col, double &a)
+/* one batch of one lane's sum over its column; ck[s * NL]: the value before
+ * member s * LSUB of the batch */
+template <int NL, int LSUB>
+__device__ __forceinline__ void prf_lane_batch(const double *col, double &a, double *ck)
 {
-  double2 x0[4], x1[4], x2[4], x3[4];
-  auto ld = [&](double2 *x, int r) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) x[k] = *(const double2 *)(col + r + 2 * k);
-  };
-  auto run = [&](double2 *x, int r) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      a += x[k].x;
-      x[k].x = a;
-      a += x[k].y;
-      x[k].y = a;
+  /* short spans: the checkpoints wait in registers until the batch ends (a
+   * write to LDS between the reads would hold the reads behind it); long
+   * ones are written as they come */
+  constexpr bool REG = LSUB < 128;
+  double ckr[REG ? PRF_LB / LSUB : 1];
+  double x[16], y[16];
+  auto step = [&](int i) {
+    /* rows i+16 .. i+47 are read ahead: at most PRF_LB + 15 < PRF_LROWS */
+    if ((i & (LSUB - 1)) == 0) {
+      if constexpr (REG)
+        ckr[i / LSUB] = a;
+      else
+        ck[(i / LSUB) * NL] = a;
     }
+    /* the reads of the next sixteen issue before these sixteen additions
+     * (the scheduler would otherwise gather them behind the additions) */
 #pragma unroll
-    for (int k = 0; k < 4; k++) *(double2 *)(col + r + 2 * k) = x[k];
+    for (int k = 0; k < 16; k++) y[k] = col[i + 16 + k];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) a += x[k];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = col[i + 32 + k];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) a += y[k];
+    __builtin_amdgcn_sched_barrier(0);
   };
-  ld(x0, 0);
-  ld(x1, 8);
-  ld(x2, 16);
-  ld(x3, 24);
-  for (int i = 0; i < PRF_LB; i += 32) {
-    /* rows i+32 .. i+63 are read ahead: at most PRF_LB + 31 < PRF_LROWS */
-    run(x0, i);
-    ld(x0, i + 32);
-    run(x1, i + 8);
-    ld(x1, i + 40);
-    run(x2, i + 16);
-    ld(x2, i + 48);
-    run(x3, i + 24);
-    ld(x3, i + 56);
+#pragma unroll
+  for (int k = 0; k < 16; k++) x[k] = col[k];
+  if constexpr (REG) {
+#pragma unroll
+    for (int i = 0; i < PRF_LB; i += 32) step(i);
+#pragma unroll
+    for (int u = 0; u < PRF_LB / LSUB; u++) ck[u * NL] = ckr[u];
+  } else {
+    for (int i = 0; i < PRF_LB; i += 32) step(i);
   }
 }
 
-template <int NC, bool E>
+template <int NC, bool OUT, int LSUB>
 __global__ void __launch_bounds__(64 + PRF_LHELP) k_prf_lanes(PrfTiled *T, const PrfChain *__restrict__ chains,
                                                               const int32_t *__restrict__ bp,
-                                                              const int32_t *__restrict__ bi, int phi_c)
+                                                              const int32_t *__restrict__ bi)
 {
-  constexpr int NL = NC + (E ? 1 : 0); /* columns: the sums, then the potential energy's */
-  constexpr int RPL = PRF_LB / PRF_LHELP;
-  __shared__ __align__(16) double s_v[2][NL][PRF_LROWS];
+  constexpr int NBUF = OUT ? 1 : 2;       /* bins staged only for the bin records */
+  __shared__ __align__(16) double s_v[2][NC][PRF_LROWS];
+  __shared__ double s_ck[2][PRF_LB / LSUB][NC];
+  constexpr int NU = PRF_LQ / LSUB; /* checkpoint spans in a helper's rows */
+  static_assert(NU * NC <= 64, "one lane per sum and span");
+  __shared__ __align__(16) int2 s_bins[NBUF][OUT ? 1 : PRF_LB];
   PrfTiled &H = T[blockIdx.x];
   if (!H.ok) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t P = H.P, to = H.to;
   const double Phi0 = H.Phi0;
   const int64_t nb = (P + PRF_LB - 1) / PRF_LB;
-  /* batch b is summed at step b; its running values recorded at b + 1; the
-   * potential energy's terms of batch b made at b + 1, summed at b + 2 and
-   * recorded at b + 3 */
-  const int64_t nit = nb + (E ? 3 : 1);
+  /* batch b is summed at step b and finished at b + 1 */
+  const int64_t nit = nb + 1;
   if (w == 0) {
     double a = 0.0;
     __syncthreads(); /* batch 0 staged */
     for (int64_t b = 0; b < nit; b++) {
-      if (lane < NL) prf_lane_batch(&s_v[b & 1][lane][0], a);
+      if (lane < NC && b < nb) prf_lane_batch<NC, LSUB>(&s_v[b & 1][lane][0], a, &s_ck[b & 1][0][lane]);
       __syncthreads();
     }
     if (lane < NC && chains[lane].fin >= 0) H.fin[chains[lane].fin] = a;
     return;
   }
-  const int hl = threadIdx.x - 64;
+  /* helper wavefront hw owns rows [hw PRF_LQ, (hw + 1) PRF_LQ) of every
+   * batch, so that no other wavefront touches them between barriers */
+  const int hw = w - 1;
   /* global, not generic, accesses: generic ones count against the LDS
    * counter too, and the barrier's wait on that would stall on the loads
    * of the batch after next */
   typedef const __attribute__((address_space(1))) double gcdouble;
   typedef __attribute__((address_space(1))) double gdouble;
   gcdouble *cin[NC];
-  gdouble *cout[NC];
-  int ccol[NC];
+  bool csub[NC];
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     cin[c] = (gcdouble *)chains[c].in;
-    cout[c] = (gdouble *)chains[c].out;
-    ccol[c] = chains[c].col;
+    csub[c] = chains[c].sub_phi0 != 0;
   }
-  double nt[NC][RPL];
-  int2 bn[RPL], b0[RPL], bm1[RPL], bm2[RPL], bm3[RPL];
-  auto load = [&](int64_t B) {
+  /* this lane's sum when it replays: sum lc over span lu of the
+   * wavefront's rows (the full replay); sum lc from the span of a member
+   * that closes bins (the bin records) */
+  const bool lact = lane < NU * NC;
+  const int lc = lact ? lane % NC : 0, lu = lact ? lane / NC : 0;
+  gdouble *const lout = (gdouble *)chains[lc].out;
+  const int lcol = chains[lc].col;
+  /* the terms of two batches in flight: set B & 1 holds batch B from
+   * its load two steps before it is staged.  A lane takes two adjacent
+   * members (rows start at even offsets, so the pair is one 16-byte load
+   * and one 16-byte LDS write) */
+  typedef double vd2 __attribute__((ext_vector_type(2)));
+  typedef int vi2 __attribute__((ext_vector_type(2)));
+  typedef const __attribute__((address_space(1))) vd2 gcvd2;
+  typedef const __attribute__((address_space(1))) vi2 gcvi2;
+  struct Regs {
+    vd2 t[NC];
+    vi2 p, q;
+  };
+  Regs R0, R1;
+  const int r2 = hw * PRF_LQ + 2 * lane; /* this lane's rows r2, r2 + 1 */
+  const int64_t jlast = (H.n - 1) & ~(int64_t)1;
+  auto load = [&](Regs &Rg, int64_t B) {
     if (B >= nb) return;
+    const int64_t j = B * PRF_LB + r2;
+    const int64_t ix = to + (j < H.n ? j : jlast);
 #pragma unroll
-    for (int k = 0; k < RPL; k++) {
-      const int64_t j = B * PRF_LB + k * PRF_LHELP + hl;
-      const int64_t ix = to + (j < P ? j : P - 1);
-#pragma unroll
-      for (int c = 0; c < NC; c++) nt[c][k] = cin[c][ix];
-      bn[k] = make_int2(bp[ix], bi[ix]);
+    for (int c = 0; c < NC; c++) Rg.t[c] = *(gcvd2 *)(cin[c] + ix);
+    if constexpr (!OUT) {
+      Rg.p = *(gcvi2 *)(bp + ix);
+      Rg.q = *(gcvi2 *)(bi + ix);
     }
   };
-  auto stage = [&](int slot, int64_t B) {
+  auto stage = [&](const Regs &Rg, int slot, int64_t B) {
+    const bool in0 = B < nb && B * PRF_LB + r2 < P, in1 = B < nb && B * PRF_LB + r2 + 1 < P;
 #pragma unroll
-    for (int k = 0; k < RPL; k++) {
-      const int r = k * PRF_LHELP + hl;
-      const bool in = B < nb && B * PRF_LB + r < P;
-#pragma unroll
-      for (int c = 0; c < NC; c++) s_v[slot][c][r] = in ? nt[c][k] : -0.0;
+    for (int c = 0; c < NC; c++) {
+      const double t0 = csub[c] ? Rg.t[c].x - Phi0 : Rg.t[c].x;
+      const double t1 = csub[c] ? Rg.t[c].y - Phi0 : Rg.t[c].y;
+      *(double2 *)&s_v[slot][c][r2] = make_double2(in0 ? t0 : -0.0, in1 ? t1 : -0.0);
     }
+    if constexpr (!OUT)
+      *(int4 *)&s_bins[slot][r2] = make_int4(in0 ? Rg.p.x : 0, in0 ? Rg.q.x : 0, in1 ? Rg.p.y : 0, in1 ? Rg.q.y : 0);
   };
-#pragma unroll
-  for (int k = 0; k < RPL; k++) {
-    bm1[k] = bm2[k] = bm3[k] = make_int2(0, 0);
-    if (E) s_v[0][NC][k * PRF_LHELP + hl] = -0.0;
-  }
-  load(0);
-  stage(0, 0);
-#pragma unroll
-  for (int k = 0; k < RPL; k++) b0[k] = bn[k];
-  load(1);
-  __syncthreads();
-  for (int64_t b = 0; b < nit; b++) {
+  /* step b: finish batch b - 1 (in slot (b + 1) & 1), stage batch b + 1
+   * there, load batch b + 3 into the set batch b + 1 came in */
+  auto step = [&](int64_t b, Regs &Rn) {
     const int slot = (int)((b + 1) & 1);
+    const int64_t B = b - 1; /* the batch to finish, in slot */
+    if (B >= 0 && B < nb) {
+      if constexpr (OUT) {
+        /* every span in full, each sum to its row */
+        const int64_t r0 = B * PRF_LB + hw * PRF_LQ + lu * LSUB;
+        const int64_t mq = P - r0 < LSUB ? P - r0 : LSUB;
+        if (lact && mq > 0) {
+          const double *col = &s_v[slot][lc][hw * PRF_LQ + lu * LSUB];
+          double a = s_ck[slot][hw * NU + lu][lc];
+          double t[LSUB];
 #pragma unroll
-    for (int k = 0; k < RPL; k++) {
-      const int r = k * PRF_LHELP + hl;
-      /* the potential energy's running values of batch b - 3 at the bins
-       * their members close */
-      if (E && b >= 3 && (b - 3) * PRF_LB + r < P)
-        for (int x = bm3[k].x; x < bm3[k].y; x++) H.bt[x][BT_EP] = s_v[slot][NC][r];
-      /* the running values of batch b - 1 */
-      const bool prev = b >= 1 && b - 1 < nb && (b - 1) * PRF_LB + r < P;
-      if (prev) {
-        const int64_t e = to + (b - 1) * PRF_LB + r;
+          for (int k = 0; k < LSUB; k++) t[k] = col[k];
+          gdouble *o = lout + to + r0;
+          if (mq == LSUB) {
 #pragma unroll
-        for (int c = 0; c < NC; c++)
-          if (cout[c]) cout[c][e] = s_v[slot][c][r];
-        for (int x = bm1[k].x; x < bm1[k].y; x++) {
+            for (int k = 0; k < LSUB; k++) {
+              a += t[k];
+              o[k] = a;
+            }
+          } else {
+            for (int k = 0; k < mq; k++) {
+              a += col[k];
+              o[k] = a;
+            }
+          }
+        }
+      } else {
+        /* up to each member of the wavefront's rows that closes bins, in
+         * order, from the checkpoint of its span */
+        /* rows 2l (mk[0]) and 2l + 1 (mk[1]) of lane l */
+        uint64_t mk[2];
 #pragma unroll
-          for (int c = 0; c < NC; c++)
-            if (ccol[c] >= 0) H.bt[x][ccol[c]] = s_v[slot][c][r];
+        for (int k = 0; k < 2; k++) {
+          const int2 q = s_bins[slot][r2 + k];
+          mk[k] = __ballot(q.x < q.y);
+        }
+        const double *col = &s_v[slot][lc][hw * PRF_LQ];
+        double a = 0.0;
+        int pos = 0, span = -1;
+        uint64_t m = mk[0] | mk[1];
+        while (m) {
+          const int l = __ffsll((unsigned long long)m) - 1;
+          m &= m - 1;
+#pragma unroll
+          for (int k = 0; k < 2; k++) {
+            if (!((mk[k] >> l) & 1)) continue;
+            const int rr = 2 * l + k;
+            if (rr / LSUB != span) {
+              span = rr / LSUB;
+              pos = span * LSUB;
+              a = s_ck[slot][hw * NU + span][lc];
+            }
+            for (; pos <= rr; pos++) a += col[pos];
+            const int2 q = s_bins[slot][hw * PRF_LQ + rr];
+            if (lane < NC && lcol >= 0)
+              for (int x = q.x; x < q.y; x++) H.bt[x][lcol] = a;
+          }
         }
       }
-      if (E) s_v[slot][NC][r] = prev ? s_v[slot][phi_c][r] - Phi0 : -0.0;
     }
-    stage(slot, b + 1);
-#pragma unroll
-    for (int k = 0; k < RPL; k++) {
-      bm3[k] = bm2[k];
-      bm2[k] = bm1[k];
-      bm1[k] = b0[k];
-      b0[k] = bn[k];
-    }
-    load(b + 2);
+    stage(Rn, slot, b + 1);
+    load(Rn, b + 3);
     __syncthreads();
+  };
+  load(R0, 0);
+  stage(R0, 0, 0);
+  load(R1, 1);
+  load(R0, 2);
+  __syncthreads();
+  int64_t b = 0;
+  for (; b + 2 <= nit; b += 2) {
+    step(b, R1);
+    step(b + 1, R0);
   }
+  if (b < nit) step(b, R1);
 }
 
 /* per member the walk reaches: escape speed at the bins it closes, and
@@ -1911,7 +1979,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     T.t1 = (int32_t)htile.size();
     tmap_h[(size_t)h] = (int32_t)htl.size();
     htl.push_back(T);
-    Nt += n;
+    Nt += n + (n & 1); /* rows at even offsets: member pairs are 16-byte aligned */
   }
   const int64_t ntl = (int64_t)htl.size(), ntile = (int64_t)htile.size();
   PrfTiled *d_tl = nullptr;
@@ -1934,7 +2002,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     tpv = A.get<double>(2 * ntile);
     tpi = A.get<int64_t>(2 * ntile);
     tib = A.get<int64_t>(ntile);
-    d_chains = A.get<PrfChain>(17);
+    d_chains = A.get<PrfChain>(18);
     double *rows = A.get<double>(29 * Nt);
     int32_t *irows = A.get<int32_t>(3 * Nt);
     if (!d_tl || !d_tile || !d_tmap || !tcmax || !tflmax || !tmb || !tEmin || !tpv || !tpi || !tib || !d_chains ||
@@ -1960,16 +2028,18 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     hipMemcpyAsync(d_pofs, pofs.data(), (size_t)nh * 8, hipMemcpyHostToDevice, c->st);
     if (nbig) hipMemcpyAsync(d_big, bigl.data(), (size_t)nbig * 4, hipMemcpyHostToDevice, c->st);
     if (ntl) {
-      /* the running sums: bulk velocity (3), then angular momentum (3),
-       * kinetic energy, potential (with the potential energy over its
-       * running values), centre of mass (3), inertia tensor (6) */
+      /* the running sums: bulk velocity (3) and potential, each to its
+       * row; then angular momentum (3), kinetic energy, centre of mass
+       * (3), inertia tensor (6) and potential energy (over the potential's
+       * running values less Phi0) */
       std::vector<PrfChain> ch;
       for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.v[k], R.vr[k], -1, 9 + k, 0, 0});
+      ch.push_back(PrfChain{R.phi, R.phir, -1, -1, 0, 0});
       for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.l[k], nullptr, BT_LX + k, -1, 0, 0});
       ch.push_back(PrfChain{R.tt, nullptr, BT_T, -1, 0, 0});
-      ch.push_back(PrfChain{R.phi, R.phir, -1, -1, 0, 0});
       for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.com[k], nullptr, -1, k, 0, 0});
       for (int k = 0; k < 6; k++) ch.push_back(PrfChain{R.ine[k], nullptr, BT_A11 + k, -1, 0, 0});
+      ch.push_back(PrfChain{R.phir, nullptr, BT_EP, -1, 1, 0});
       hipMemcpyAsync(d_chains, ch.data(), ch.size() * sizeof(PrfChain), hipMemcpyHostToDevice, c->st);
       hipMemcpyAsync(d_tl, htl.data(), (size_t)ntl * sizeof(PrfTiled), hipMemcpyHostToDevice, c->st);
       hipMemcpyAsync(d_tile, htile.data(), (size_t)ntile * sizeof(PrfTile), hipMemcpyHostToDevice, c->st);
@@ -2004,13 +2074,11 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
         hipLaunchKernelGGL(k_prf_rmax, gt, dim3(256), 0, c->st3, d_tl, d_tile, sc, tib);
         hipLaunchKernelGGL(k_prf_rmax_done, dim3((unsigned)ntl), dim3(64), 0, c->st3, d_tl, tib);
         hipEventRecord(c->ev_s3, c->st3);
-        hipLaunchKernelGGL((k_prf_lanes<3, false>), dim3((unsigned)ntl), dim3(64 + PRF_LHELP), 0, s2, d_tl, d_chains,
-                           R.bp, R.bi, 0);
+        hipLaunchKernelGGL((k_prf_lanes<4, true, 32>), dim3((unsigned)ntl), dim3(64 + PRF_LHELP), 0, s2, d_tl, d_chains,
+                           R.bp, R.bi);
         hipLaunchKernelGGL(k_prf_terms2, gt, dim3(256), 0, s2, d_tl, d_tile, R, K);
-        /* the potential energy's sum over the potential's (the fifth of
-         * these) */
-        hipLaunchKernelGGL((k_prf_lanes<14, true>), dim3((unsigned)ntl), dim3(64 + PRF_LHELP), 0, s2, d_tl,
-                           d_chains + 3, R.bp, R.bi, 4);
+        hipLaunchKernelGGL((k_prf_lanes<14, false, 128>), dim3((unsigned)ntl), dim3(64 + PRF_LHELP), 0, s2, d_tl,
+                           d_chains + 4, R.bp, R.bi);
         hipLaunchKernelGGL(k_prf_finish, gt, dim3(256), 0, s2, d_tl, d_tile, R, tEmin, tmb);
         hipLaunchKernelGGL(k_prf_done, gh, dim3(256), 0, s2, d_tl, ntl, tEmin, tmb);
         hipStreamWaitEvent(s2, c->ev_s3, 0);
