@@ -484,10 +484,9 @@ __device__ void prf_radii(const ProfConst &K, int64_t n, const uint32_t *__restr
  * the last member beyond MACHINE_ZERO), so it runs per member after
  * prefix maxima over the tiles.  The running sums keep AHF's order and
  * roundings: each is one lane of a wavefront that advances all of them at
- * once, their terms staged in LDS by other wavefronts (k_prf_lanes).  The
- * bulk velocity's and the potential's sums come first, the terms that
- * depend on them next, then the other sums, the potential energy's over
- * the potential's running values among them.  k_profiles then takes the bins,
+ * once, their terms staged in LDS by other wavefronts (k_prf_walk); the
+ * sums whose terms depend on the bulk velocity's and the potential's
+ * running values run two batches behind those.  k_profiles then takes the bins,
  * totals and most bound member from here and does the rest.  Bin radii
  * that decrease keep the sequential walk.
  */
@@ -622,22 +621,43 @@ __global__ void __launch_bounds__(256) k_prf_geom(const PrfTiled *__restrict__ T
   }
 }
 
-/* per tiled halo: the largest bin count and the last member beyond
- * MACHINE_ZERO before each tile (exclusive prefix maxima, in place) */
-__global__ void k_prf_tscan(const PrfTiled *__restrict__ T, int64_t ntl, int32_t *__restrict__ tc_max,
-                            int64_t *__restrict__ tfl_max)
+/* per tiled halo (one wavefront): the largest bin count and the last
+ * member beyond MACHINE_ZERO before each tile (exclusive prefix maxima over
+ * the halo's tiles, in place) */
+__global__ void __launch_bounds__(64) k_prf_tscan(const PrfTiled *__restrict__ T, int32_t *__restrict__ tc_max,
+                                                  int64_t *__restrict__ tfl_max)
 {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= ntl || !T[s].ok) return;
-  int32_t c = 0;
+  const PrfTiled &H = T[blockIdx.x];
+  if (!H.ok) return;
+  const int lane = threadIdx.x;
+  int32_t c = 0;   /* over the tiles of earlier rounds */
   int64_t fl = -1;
-  for (int32_t t = T[s].t0; t < T[s].t1; t++) {
-    const int32_t ct = tc_max[t];
-    const int64_t ft = tfl_max[t];
-    tc_max[t] = c;
-    tfl_max[t] = fl;
-    c = max(c, ct);
-    fl = max(fl, ft);
+  for (int32_t t0 = H.t0; t0 < H.t1; t0 += 64) {
+    const int32_t t = t0 + lane;
+    const bool in = t < H.t1;
+    int32_t ci = in ? tc_max[t] : 0;
+    int64_t fi = in ? tfl_max[t] : -1;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int32_t cu = __shfl_up(ci, d, 64);
+      const int64_t fu = __shfl_up(fi, d, 64);
+      if (lane >= d) {
+        ci = max(ci, cu);
+        fi = max(fi, fu);
+      }
+    }
+    int32_t ce = __shfl_up(ci, 1, 64);
+    int64_t fe = __shfl_up(fi, 1, 64);
+    if (lane == 0) {
+      ce = 0;
+      fe = -1;
+    }
+    if (in) {
+      tc_max[t] = max(ce, c);
+      tfl_max[t] = max(fe, fl);
+    }
+    c = max(c, __shfl(ci, 63, 64));
+    fl = max(fl, __shfl(fi, 63, 64));
   }
 }
 
@@ -753,70 +773,54 @@ __global__ void __launch_bounds__(256) k_prf_terms1(PrfTiled *T, const PrfTile *
   if (tid == 0 && s_P) atomicMax((unsigned long long *)&H.P, s_P);
 }
 
-/* per member the walk reaches: the angular momentum and kinetic energy
- * terms about the running bulk velocity */
-__global__ void __launch_bounds__(256) k_prf_terms2(const PrfTiled *__restrict__ T, const PrfTile *__restrict__ tiles,
-                                                    PrfRows R, ProfConst K)
-{
-  const PrfTile Tl = tiles[blockIdx.x];
-  const PrfTiled &H = T[Tl.seg];
-  if (!H.ok) return;
-  const double hub = K.hubble, rf = K.r_fac, vf = K.v_fac;
-  for (int i = 0; i < PRF_TILE / 256; i++) {
-    const int64_t j = Tl.j0 + (int64_t)i * 256 + threadIdx.x;
-    if (j >= H.P) break;
-    const int64_t e = H.to + j;
-    const double M = (double)(j + 1);
-    const double dX = R.gx[e], dY = R.gy[e], dZ = R.gz[e];
-    double dVX = R.v[0][e] - R.vr[0][e] / M, dVY = R.v[1][e] - R.vr[1][e] / M, dVZ = R.v[2][e] - R.vr[2][e] / M;
-    R.l[0][e] = dY * dVZ - dZ * dVY;
-    R.l[1][e] = dZ * dVX - dX * dVZ;
-    R.l[2][e] = dX * dVY - dY * dVX;
-    dVX += hub * dX * rf / vf;
-    dVY += hub * dY * rf / vf;
-    dVZ += hub * dZ * rf / vf;
-    R.tt[e] = dVX * dVX + dVY * dVY + dVZ * dVZ;
-  }
-}
+/* the running sums over the members the walk reaches, in AHF's order
+ * (ahf_halos.c:4333-4395): one lane of one wavefront per sum, so that one
+ * FP64 addition advances all of them.
+ *
+ * The wavefront's lanes only read.  They take their terms, up to 32
+ * members ahead, from columns in LDS (column c: sum c, PRF_FB members of a
+ * batch), and keep the running value before every PRF_FS-th member as a
+ * checkpoint.  Sums 0..3 (bulk velocity, potential) sum batch b while sums
+ * 4..17 sum batch b - 2: their terms (angular momentum and kinetic energy
+ * over the bulk velocity's running values, the potential energy over the
+ * potential's) need the first four's running values of the batch.
+ *
+ * PRF_FH more wavefronts, each owning PRF_FQ rows of every batch (a row a
+ * lane), finish and stage while the batches are summed.  They replay the
+ * first four sums over batch b - 1 from their checkpoints (the same
+ * additions from the same values) into the rows (bulk velocity,
+ * potential).  They compute batch b - 1's terms of the other sums and
+ * stage them, with the centre of mass, inertia and potential energy terms.
+ * They stage the first four's terms of batch b + 1.  And they replay the
+ * other sums over batch b - 3 up to the members that close bins, for the
+ * bin records.
+ *
+ * Members past the last take -0.0, which leaves every sum as it is.  Sum c
+ * of chains[c]; sum 17 (sub_phi0) adds the potential's running value less
+ * Phi0. */
+#define PRF_FB 384       /* members per batch */
+#define PRF_FQ 64        /* rows a helper wavefront owns: a row a lane */
+#define PRF_FH 6         /* helper wavefronts */
+#define PRF_FROWS 418    /* a column: the batch, 34 rows read ahead; 2 * 418 = 4 mod 64 dwords between columns */
+#define PRF_FS 16        /* members per checkpoint */
+#define PRF_NV 4         /* sums to rows: bulk velocity, potential */
+#define PRF_NL 18        /* sums */
+static_assert(PRF_FB == PRF_FQ * PRF_FH && PRF_FQ == 64, "a helper lane owns one row");
 
-/* running sums over the members the walk reaches, AHF's additions in AHF's
- * order: one lane of one wavefront per sum, so that one FP64 addition
- * advances all of them.  The wavefront's lanes only read: their terms, up
- * to 32 members ahead, from columns in LDS (column c: sum c, PRF_LB
- * members of a batch), and write the running value before every
- * LSUB-th member (a checkpoint).  Four more wavefronts, each owning a
- * quarter of the batch's rows, stage the next batch in the other of two
- * buffers while a batch is summed, and finish the previous one from its
- * checkpoints: with OUT, by replaying their quarter in full (the same
- * additions from the same value) to the sums' rows; else only up to the
- * members that close bins, for the bin records.  Members past the last
- * take -0.0, which leaves every sum as it is.  A sum with sub_phi0 adds
- * its term minus Phi0 (the potential energy over the potential's running
- * values, ahf_halos.c:4383-4395). */
-#define PRF_LB 512       /* members per batch */
-#define PRF_LQ 128       /* rows a helper wavefront owns */
-#define PRF_LROWS 546    /* a column: the batch, 34 rows read ahead; 2 * 546 = 4 mod 64 dwords between columns */
-#define PRF_LHELP 256    /* lanes that stage and finish */
-
-/* one batch of one lane's sum over its column; ck[s * NL]: the value before
- * member s * LSUB of the batch */
-template <int NL, int LSUB>
+/* one batch of one lane's sum over its column; ck[s * PRF_NL]: the value
+ * before member s * PRF_FS of the batch, kept in registers until the batch
+ * ends (a write to LDS between the reads would hold the reads behind it) */
 __device__ __forceinline__ void prf_lane_batch(const double *col, double &a, double *ck)
 {
-  /* short spans: the checkpoints wait in registers until the batch ends (a
-   * write to LDS between the reads would hold the reads behind it); long
-   * ones are written as they come */
-  constexpr bool REG = LSUB < 128;
-  double ckr[REG ? PRF_LB / LSUB : 1];
+  static_assert(PRF_FS == 16, "two checkpoints per 32 members");
+  double ckr[PRF_FB / PRF_FS];
   double x[16], y[16];
-  auto step = [&](int i) {
-    /* rows i+16 .. i+47 are read ahead: at most PRF_LB + 15 < PRF_LROWS */
-    if ((i & (LSUB - 1)) == 0) {
-      if constexpr (REG)
-        ckr[i / LSUB] = a;
-      else
-        ck[(i / LSUB) * NL] = a;
-    }
+#pragma unroll
+  for (int k = 0; k < 16; k++) x[k] = col[k];
+#pragma unroll
+  for (int i = 0; i < PRF_FB; i += 32) {
+    /* rows i+16 .. i+47 are read ahead: at most PRF_FB + 15 < PRF_FROWS */
+    ckr[i / PRF_FS] = a;
     /* the reads of the next sixteen issue before these sixteen additions
      * (the scheduler would otherwise gather them behind the additions) */
 #pragma unroll
@@ -825,185 +829,204 @@ __device__ __forceinline__ void prf_lane_batch(const double *col, double &a, dou
 #pragma unroll
     for (int k = 0; k < 16; k++) a += x[k];
     __builtin_amdgcn_sched_barrier(0);
+    ckr[i / PRF_FS + 1] = a;
 #pragma unroll
     for (int k = 0; k < 16; k++) x[k] = col[i + 32 + k];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int k = 0; k < 16; k++) a += y[k];
     __builtin_amdgcn_sched_barrier(0);
-  };
-#pragma unroll
-  for (int k = 0; k < 16; k++) x[k] = col[k];
-  if constexpr (REG) {
-#pragma unroll
-    for (int i = 0; i < PRF_LB; i += 32) step(i);
-#pragma unroll
-    for (int u = 0; u < PRF_LB / LSUB; u++) ck[u * NL] = ckr[u];
-  } else {
-    for (int i = 0; i < PRF_LB; i += 32) step(i);
   }
+#pragma unroll
+  for (int u = 0; u < PRF_FB / PRF_FS; u++) ck[u * PRF_NL] = ckr[u];
 }
 
-template <int NC, bool OUT, int LSUB>
-__global__ void __launch_bounds__(64 + PRF_LHELP) k_prf_lanes(PrfTiled *T, const PrfChain *__restrict__ chains,
-                                                              const int32_t *__restrict__ bp,
-                                                              const int32_t *__restrict__ bi)
+/* wavefronts of a workgroup go to the compute unit's four SIMDs in turn:
+ * wavefront 4, which would share the summing wavefront's SIMD, only keeps
+ * the barriers, so that the helpers' arithmetic does not take the SIMD's
+ * issue from the additions */
+#define PRF_FW (2 + PRF_FH)
+__global__ void __launch_bounds__(64 * PRF_FW) k_prf_walk(PrfTiled *T, const PrfChain *__restrict__ chains,
+                                                          PrfRows R, ProfConst K)
 {
-  constexpr int NBUF = OUT ? 1 : 2;       /* bins staged only for the bin records */
-  __shared__ __align__(16) double s_v[2][NC][PRF_LROWS];
-  __shared__ double s_ck[2][PRF_LB / LSUB][NC];
-  constexpr int NU = PRF_LQ / LSUB; /* checkpoint spans in a helper's rows */
-  static_assert(NU * NC <= 64, "one lane per sum and span");
-  __shared__ __align__(16) int2 s_bins[NBUF][OUT ? 1 : PRF_LB];
+  __shared__ __align__(16) double s_v[2][PRF_NL][PRF_FROWS];
+  __shared__ double s_ck[2][PRF_FB / PRF_FS][PRF_NL];
+  __shared__ int2 s_bins[2][PRF_FB];
   PrfTiled &H = T[blockIdx.x];
   if (!H.ok) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t P = H.P, to = H.to;
   const double Phi0 = H.Phi0;
-  const int64_t nb = (P + PRF_LB - 1) / PRF_LB;
-  /* batch b is summed at step b and finished at b + 1 */
-  const int64_t nit = nb + 1;
+  const int64_t nb = (P + PRF_FB - 1) / PRF_FB;
+  /* step b sums slot b & 1: the first four over batch b, the others over
+   * batch b - 2 (-0.0 where there is none) */
+  const int64_t nit = nb + 3;
   if (w == 0) {
     double a = 0.0;
-    __syncthreads(); /* batch 0 staged */
+    __syncthreads(); /* the first batch staged */
     for (int64_t b = 0; b < nit; b++) {
-      if (lane < NC && b < nb) prf_lane_batch<NC, LSUB>(&s_v[b & 1][lane][0], a, &s_ck[b & 1][0][lane]);
+      if (lane < PRF_NL) prf_lane_batch(&s_v[b & 1][lane][0], a, &s_ck[b & 1][0][lane]);
       __syncthreads();
     }
-    if (lane < NC && chains[lane].fin >= 0) H.fin[chains[lane].fin] = a;
+    if (lane < PRF_NL && chains[lane].fin >= 0) H.fin[chains[lane].fin] = a;
     return;
   }
-  /* helper wavefront hw owns rows [hw PRF_LQ, (hw + 1) PRF_LQ) of every
-   * batch, so that no other wavefront touches them between barriers */
-  const int hw = w - 1;
+  if (w == 4) {
+    for (int64_t b = 0; b <= nit; b++) __syncthreads();
+    return;
+  }
+  /* helper wavefront hw owns rows [hw PRF_FQ, (hw + 1) PRF_FQ) of every
+   * batch, so that no other wavefront touches them between barriers; lane
+   * l owns row r1 */
+  const int hw = w < 4 ? w - 1 : w - 2;
+  const int r1 = hw * PRF_FQ + lane;
   /* global, not generic, accesses: generic ones count against the LDS
    * counter too, and the barrier's wait on that would stall on the loads
-   * of the batch after next */
-  typedef const __attribute__((address_space(1))) double gcdouble;
+   * of later batches */
   typedef __attribute__((address_space(1))) double gdouble;
-  gcdouble *cin[NC];
-  bool csub[NC];
-#pragma unroll
-  for (int c = 0; c < NC; c++) {
-    cin[c] = (gcdouble *)chains[c].in;
-    csub[c] = chains[c].sub_phi0 != 0;
-  }
-  /* this lane's sum when it replays: sum lc over span lu of the
-   * wavefront's rows (the full replay); sum lc from the span of a member
-   * that closes bins (the bin records) */
-  const bool lact = lane < NU * NC;
-  const int lc = lact ? lane % NC : 0, lu = lact ? lane / NC : 0;
-  gdouble *const lout = (gdouble *)chains[lc].out;
-  const int lcol = chains[lc].col;
-  /* the terms of two batches in flight: set B & 1 holds batch B from
-   * its load two steps before it is staged.  A lane takes two adjacent
-   * members (rows start at even offsets, so the pair is one 16-byte load
-   * and one 16-byte LDS write) */
-  typedef double vd2 __attribute__((ext_vector_type(2)));
-  typedef int vi2 __attribute__((ext_vector_type(2)));
-  typedef const __attribute__((address_space(1))) vd2 gcvd2;
-  typedef const __attribute__((address_space(1))) vi2 gcvi2;
+  typedef const __attribute__((address_space(1))) double gcdouble;
+  typedef const __attribute__((address_space(1))) int32_t gcint;
+  auto ld = [](const double *p, int64_t ix) { return *(gcdouble *)(p + ix); };
+  const double hub = K.hubble, rf = K.r_fac, vf = K.v_fac;
+  /* the first four's terms of one batch and the member data of another,
+   * two sets in flight */
   struct Regs {
-    vd2 t[NC];
-    vi2 p, q;
+    double vt[PRF_NV]; /* terms of sums 0..3 */
+    double g[3], v[3], com[3], ine[6];
+    int32_t p, q; /* bins closed: [p, q) */
   };
   Regs R0, R1;
-  const int r2 = hw * PRF_LQ + 2 * lane; /* this lane's rows r2, r2 + 1 */
-  const int64_t jlast = (H.n - 1) & ~(int64_t)1;
-  auto load = [&](Regs &Rg, int64_t B) {
-    if (B >= nb) return;
-    const int64_t j = B * PRF_LB + r2;
-    const int64_t ix = to + (j < H.n ? j : jlast);
+  /* set of step b: the first four's terms of batch b + 1 and the member
+   * data of batch b - 1; load(Rg, bq) fills it for step bq + 1 */
+  auto load = [&](Regs &Rg, int64_t bq) {
+    if (bq + 2 >= 0 && bq + 2 < nb) {
+      const int64_t j = (bq + 2) * PRF_FB + r1;
+      const int64_t ix = to + (j < P ? j : P - 1);
 #pragma unroll
-    for (int c = 0; c < NC; c++) Rg.t[c] = *(gcvd2 *)(cin[c] + ix);
-    if constexpr (!OUT) {
-      Rg.p = *(gcvi2 *)(bp + ix);
-      Rg.q = *(gcvi2 *)(bi + ix);
+      for (int c = 0; c < PRF_NV; c++) Rg.vt[c] = ld(chains[c].in, ix);
+    }
+    if (bq >= 0 && bq < nb) {
+      const int64_t j = bq * PRF_FB + r1;
+      const int64_t ix = to + (j < P ? j : P - 1);
+      Rg.g[0] = ld(R.gx, ix);
+      Rg.g[1] = ld(R.gy, ix);
+      Rg.g[2] = ld(R.gz, ix);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        Rg.v[k] = ld(R.v[k], ix);
+        Rg.com[k] = ld(R.com[k], ix);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; k++) Rg.ine[k] = ld(R.ine[k], ix);
+      Rg.p = *(gcint *)(R.bp + ix);
+      Rg.q = *(gcint *)(R.bi + ix);
     }
   };
-  auto stage = [&](const Regs &Rg, int slot, int64_t B) {
-    const bool in0 = B < nb && B * PRF_LB + r2 < P, in1 = B < nb && B * PRF_LB + r2 + 1 < P;
+  /* the first four's terms of batch B into slot s */
+  auto stage_v = [&](const Regs &Rg, int s, int64_t B) {
+    const bool in = B < nb && B * PRF_FB + r1 < P;
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-      const double t0 = csub[c] ? Rg.t[c].x - Phi0 : Rg.t[c].x;
-      const double t1 = csub[c] ? Rg.t[c].y - Phi0 : Rg.t[c].y;
-      *(double2 *)&s_v[slot][c][r2] = make_double2(in0 ? t0 : -0.0, in1 ? t1 : -0.0);
-    }
-    if constexpr (!OUT)
-      *(int4 *)&s_bins[slot][r2] = make_int4(in0 ? Rg.p.x : 0, in0 ? Rg.q.x : 0, in1 ? Rg.p.y : 0, in1 ? Rg.q.y : 0);
+    for (int c = 0; c < PRF_NV; c++) s_v[s][c][r1] = in ? Rg.vt[c] : -0.0;
   };
-  /* step b: finish batch b - 1 (in slot (b + 1) & 1), stage batch b + 1
-   * there, load batch b + 3 into the set batch b + 1 came in */
-  auto step = [&](int64_t b, Regs &Rn) {
-    const int slot = (int)((b + 1) & 1);
-    const int64_t B = b - 1; /* the batch to finish, in slot */
-    if (B >= 0 && B < nb) {
-      if constexpr (OUT) {
-        /* every span in full, each sum to its row */
-        const int64_t r0 = B * PRF_LB + hw * PRF_LQ + lu * LSUB;
-        const int64_t mq = P - r0 < LSUB ? P - r0 : LSUB;
-        if (lact && mq > 0) {
-          const double *col = &s_v[slot][lc][hw * PRF_LQ + lu * LSUB];
-          double a = s_ck[slot][hw * NU + lu][lc];
-          double t[LSUB];
+  /* the other sums' terms of batch B into slot s, from the member data and
+   * the first four's running values there (ahf_halos.c:4333-4395) */
+  auto stage_c = [&](const Regs &Rg, int s, int64_t B) {
+    const int64_t j = B * PRF_FB + r1;
+    const bool in = B >= 0 && B < nb && j < P;
+    double col[PRF_NL - PRF_NV];
+    const double M = (double)(j + 1);
+    const double dX = Rg.g[0], dY = Rg.g[1], dZ = Rg.g[2];
+    double dVX = Rg.v[0] - s_v[s][0][r1] / M, dVY = Rg.v[1] - s_v[s][1][r1] / M, dVZ = Rg.v[2] - s_v[s][2][r1] / M;
+    col[0] = dY * dVZ - dZ * dVY;
+    col[1] = dZ * dVX - dX * dVZ;
+    col[2] = dX * dVY - dY * dVX;
+    dVX += hub * dX * rf / vf;
+    dVY += hub * dY * rf / vf;
+    dVZ += hub * dZ * rf / vf;
+    col[3] = dVX * dVX + dVY * dVY + dVZ * dVZ;
+    if (in) R.tt[to + j] = col[3];
 #pragma unroll
-          for (int k = 0; k < LSUB; k++) t[k] = col[k];
-          gdouble *o = lout + to + r0;
-          if (mq == LSUB) {
+    for (int c = 0; c < 3; c++) col[4 + c] = Rg.com[c];
 #pragma unroll
-            for (int k = 0; k < LSUB; k++) {
-              a += t[k];
-              o[k] = a;
-            }
-          } else {
-            for (int k = 0; k < mq; k++) {
-              a += col[k];
-              o[k] = a;
-            }
-          }
-        }
-      } else {
-        /* up to each member of the wavefront's rows that closes bins, in
-         * order, from the checkpoint of its span */
-        /* rows 2l (mk[0]) and 2l + 1 (mk[1]) of lane l */
-        uint64_t mk[2];
+    for (int c = 0; c < 6; c++) col[7 + c] = Rg.ine[c];
+    col[13] = s_v[s][3][r1] - Phi0;
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-          const int2 q = s_bins[slot][r2 + k];
-          mk[k] = __ballot(q.x < q.y);
-        }
-        const double *col = &s_v[slot][lc][hw * PRF_LQ];
-        double a = 0.0;
-        int pos = 0, span = -1;
-        uint64_t m = mk[0] | mk[1];
-        while (m) {
-          const int l = __ffsll((unsigned long long)m) - 1;
-          m &= m - 1;
+    for (int c = 0; c < PRF_NL - PRF_NV; c++) s_v[s][PRF_NV + c][r1] = in ? col[c] : -0.0;
+    s_bins[s][r1] = in ? make_int2(Rg.p, Rg.q) : make_int2(0, 0);
+  };
+  /* the first four over batch B (slot s) into their rows and in place:
+   * lane (u, c) replays sum c over checkpoint span u of the wavefront's rows */
+  constexpr int NU = PRF_FQ / PRF_FS;
+  static_assert(NU * PRF_NV <= 64, "one lane per sum and span");
+  const bool vact = lane < NU * PRF_NV;
+  const int vc = vact ? lane % PRF_NV : 0, vu = vact ? lane / PRF_NV : 0;
+  gdouble *const vout = (gdouble *)chains[vc].out;
+  auto replay_v = [&](int s, int64_t B) {
+    const int64_t r0 = B * PRF_FB + hw * PRF_FQ + vu * PRF_FS;
+    const int64_t mq = P - r0 < PRF_FS ? P - r0 : PRF_FS;
+    if (!vact || mq <= 0) return;
+    double *colp = &s_v[s][vc][hw * PRF_FQ + vu * PRF_FS];
+    double a = s_ck[s][hw * NU + vu][vc];
+    gdouble *o = vout + to + r0;
+    if (mq == PRF_FS) {
+      double t[PRF_FS];
 #pragma unroll
-          for (int k = 0; k < 2; k++) {
-            if (!((mk[k] >> l) & 1)) continue;
-            const int rr = 2 * l + k;
-            if (rr / LSUB != span) {
-              span = rr / LSUB;
-              pos = span * LSUB;
-              a = s_ck[slot][hw * NU + span][lc];
-            }
-            for (; pos <= rr; pos++) a += col[pos];
-            const int2 q = s_bins[slot][hw * PRF_LQ + rr];
-            if (lane < NC && lcol >= 0)
-              for (int x = q.x; x < q.y; x++) H.bt[x][lcol] = a;
-          }
-        }
+      for (int k = 0; k < PRF_FS; k++) t[k] = colp[k];
+#pragma unroll
+      for (int k = 0; k < PRF_FS; k++) {
+        a += t[k];
+        colp[k] = a;
+        o[k] = a;
+      }
+    } else {
+      for (int k = 0; k < mq; k++) {
+        a += colp[k];
+        colp[k] = a;
+        o[k] = a;
       }
     }
-    stage(Rn, slot, b + 1);
-    load(Rn, b + 3);
+  };
+  /* the other sums over batch B (slot s) up to each of the wavefront's
+   * members that closes bins, in order, from the checkpoint of its span */
+  const int cc = lane < PRF_NL - PRF_NV ? PRF_NV + lane : PRF_NV;
+  const int ccol = chains[cc].col;
+  auto replay_bins = [&](int s) {
+    const int2 q1 = s_bins[s][r1];
+    uint64_t m = __ballot(q1.x < q1.y);
+    const double *colp = &s_v[s][cc][hw * PRF_FQ];
+    double a = 0.0;
+    int pos = 0, span = -1;
+    while (m) {
+      const int rr = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      if (rr / PRF_FS != span) {
+        span = rr / PRF_FS;
+        pos = span * PRF_FS;
+        a = s_ck[s][hw * NU + span][cc];
+      }
+      for (; pos <= rr; pos++) a += colp[pos];
+      const int2 q = s_bins[s][hw * PRF_FQ + rr];
+      if (lane < PRF_NL - PRF_NV && ccol >= 0)
+        for (int x = q.x; x < q.y; x++) H.bt[x][ccol] = a;
+    }
+  };
+  /* step b: slot (b + 1) & 1 holds the first four over batch b - 1 and
+   * the others over batch b - 3, summed at step b - 1 */
+  auto step = [&](int64_t b, Regs &Rg) {
+    const int s = (int)((b + 1) & 1);
+    if (b - 3 >= 0 && b - 3 < nb) replay_bins(s);
+    if (b - 1 >= 0 && b - 1 < nb) replay_v(s, b - 1);
+    stage_c(Rg, s, b - 1);
+    stage_v(Rg, s, b + 1);
+    load(Rg, b + 1);
     __syncthreads();
   };
+  /* slot 0 for step 0: the first four over batch 0, the others -0.0 */
+  load(R0, -2);
+  stage_v(R0, 0, 0);
+  stage_c(R0, 0, -1);
+  load(R1, -1);
   load(R0, 0);
-  stage(R0, 0, 0);
-  load(R1, 1);
-  load(R0, 2);
   __syncthreads();
   int64_t b = 0;
   for (; b + 2 <= nit; b += 2) {
@@ -1067,24 +1090,29 @@ __global__ void __launch_bounds__(256) k_prf_finish(PrfTiled *T, const PrfTile *
   }
 }
 
-/* per tiled halo: the most bound member over its tiles */
-__global__ void k_prf_done(PrfTiled *T, int64_t ntl, const double *__restrict__ tEmin, const int64_t *__restrict__ tmb)
+/* per tiled halo (one wavefront): the most bound member over its tiles,
+ * the lower index on equal energies */
+__global__ void __launch_bounds__(64) k_prf_done(PrfTiled *T, const double *__restrict__ tEmin,
+                                                 const int64_t *__restrict__ tmb)
 {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= ntl || !T[s].ok) return;
-  PrfTiled &H = T[s];
+  PrfTiled &H = T[blockIdx.x];
+  if (!H.ok) return;
+  const int lane = threadIdx.x;
   double bv = 1e30;
   int64_t bl = -1;
-  for (int32_t t = H.t0; t < H.t1; t++) {
-    const double ov = tEmin[t];
-    const int64_t ol = tmb[t];
+  auto merge = [&](double ov, int64_t ol) {
     if (ol >= 0 && (bl < 0 || ov < bv || (!(bv < ov) && ol < bl))) {
       bv = ov;
       bl = ol;
     }
+  };
+  for (int32_t t = H.t0 + lane; t < H.t1; t += 64) merge(tEmin[t], tmb[t]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) merge(__shfl_xor(bv, d, 64), __shfl_xor(bl, d, 64));
+  if (lane == 0) {
+    H.Emin = bv;
+    H.mb = bl;
   }
-  H.Emin = bv;
-  H.mb = bl;
 }
 
 /* r2's and R_max's peaks (find_peaks) over one tile of a halo walked over
@@ -2057,12 +2085,12 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
       hipStreamWaitEvent(c->st2, c->ev_s0, 0);
       if (ntl) {
         hipStream_t s2 = c->st2;
-        const dim3 gt((unsigned)ntile), gh((unsigned)((ntl + 255) / 256));
+        const dim3 gt((unsigned)ntile);
         hipLaunchKernelGGL(k_prf_bin, dim3((unsigned)ntl), dim3(64), 0, s2, K, d_tl, s->off, out, d_pr, s->ip_in,
                            s->buf_a, s->buf_b, s->cen, d_nb, c->px, c->py, c->pz);
         hipLaunchKernelGGL(k_prf_geom, gt, dim3(256), 0, s2, d_tl, d_tile, R, c->px, c->py, c->pz, c->vx, c->vy,
                            c->vz, tcmax, tflmax);
-        hipLaunchKernelGGL(k_prf_tscan, gh, dim3(256), 0, s2, d_tl, ntl, tcmax, tflmax);
+        hipLaunchKernelGGL(k_prf_tscan, dim3((unsigned)ntl), dim3(64), 0, s2, d_tl, tcmax, tflmax);
         hipLaunchKernelGGL(k_prf_terms1, gt, dim3(256), 0, s2, d_tl, d_tile, R, tcmax, tflmax, sc, sc + ne,
                            sc + 2 * ne);
         /* r2, R_max and the first member beyond it on the third stream,
@@ -2074,13 +2102,9 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
         hipLaunchKernelGGL(k_prf_rmax, gt, dim3(256), 0, c->st3, d_tl, d_tile, sc, tib);
         hipLaunchKernelGGL(k_prf_rmax_done, dim3((unsigned)ntl), dim3(64), 0, c->st3, d_tl, tib);
         hipEventRecord(c->ev_s3, c->st3);
-        hipLaunchKernelGGL((k_prf_lanes<4, true, 32>), dim3((unsigned)ntl), dim3(64 + PRF_LHELP), 0, s2, d_tl, d_chains,
-                           R.bp, R.bi);
-        hipLaunchKernelGGL(k_prf_terms2, gt, dim3(256), 0, s2, d_tl, d_tile, R, K);
-        hipLaunchKernelGGL((k_prf_lanes<14, false, 128>), dim3((unsigned)ntl), dim3(64 + PRF_LHELP), 0, s2, d_tl,
-                           d_chains + 4, R.bp, R.bi);
+        hipLaunchKernelGGL(k_prf_walk, dim3((unsigned)ntl), dim3(64 * PRF_FW), 0, s2, d_tl, d_chains, R, K);
         hipLaunchKernelGGL(k_prf_finish, gt, dim3(256), 0, s2, d_tl, d_tile, R, tEmin, tmb);
-        hipLaunchKernelGGL(k_prf_done, gh, dim3(256), 0, s2, d_tl, ntl, tEmin, tmb);
+        hipLaunchKernelGGL(k_prf_done, dim3((unsigned)ntl), dim3(64), 0, s2, d_tl, tEmin, tmb);
         hipStreamWaitEvent(s2, c->ev_s3, 0);
       }
       /* the haloes walked over the device (the first ntl of the list) after
