@@ -575,6 +575,7 @@ __global__ void __launch_bounds__(256) k_prf_geom(const PrfTiled *__restrict__ T
 {
   __shared__ int32_t s_c;
   __shared__ unsigned long long s_fl;
+  __shared__ double s_rad[64];
   const PrfTile Tl = tiles[blockIdx.x];
   const PrfTiled &H = T[Tl.seg];
   if (!H.ok) return;
@@ -582,6 +583,8 @@ __global__ void __launch_bounds__(256) k_prf_geom(const PrfTiled *__restrict__ T
     s_c = 0;
     s_fl = 0ull;
   }
+  const int nbins = H.nbins;
+  if (threadIdx.x < nbins) s_rad[threadIdx.x] = H.rad[threadIdx.x];
   __syncthreads();
   int cmax = 0;
   int64_t flmax = -1;
@@ -605,9 +608,18 @@ __global__ void __launch_bounds__(256) k_prf_geom(const PrfTiled *__restrict__ T
     R.v[0][e] = (double)vx[p];
     R.v[1][e] = (double)vy[p];
     R.v[2][e] = (double)vz[p];
-    int c = 0;
-    while (c < H.nbins && H.rad[c] <= d) c++;
-    if (j == H.n - 1) c = H.nbins;
+    /* bins whose radius is not above d: the radii of a tiled halo do not
+     * decrease, so AHF's walk up the bins stops where a binary search does */
+    int lo = 0, hi = nbins;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_rad[mid] <= d)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    int c = lo;
+    if (j == H.n - 1) c = nbins;
     R.c[e] = c;
     cmax = max(cmax, c);
     if (d > PRF_MZERO) flmax = j;
