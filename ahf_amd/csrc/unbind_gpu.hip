@@ -1107,10 +1107,15 @@ __global__ void __launch_bounds__(256) k_sp_stage(const SplitHalo *__restrict__ 
   const SplitHalo &H = sh[T.seg];
   const uint32_t *cur = split_cur(H, a, b);
   const double *c = cen + 3 * H.h;
+  /* a member's staged values do not change between iterations (the centre
+   * is fixed): after the first, the compaction has carried them along */
+  const bool carried = H.niter > 1;
   for (int i = 0; i < AHF_TILE / 256; i++) {
     const int64_t j = T.j0 + (int64_t)i * 256 + threadIdx.x;
     if (j >= H.n) break;
     const int64_t e = H.o + j;
+    st[e] = 1;
+    if (carried) continue;
     const uint32_t p = cur[e];
     const double dX = wrap_s((double)px[p] - c[0]), dY = wrap_s((double)py[p] - c[1]), dZ = wrap_s((double)pz[p] - c[2]);
     dist[e] = sqrt(dX * dX + dY * dY + dZ * dZ);
@@ -1120,7 +1125,6 @@ __global__ void __launch_bounds__(256) k_sp_stage(const SplitHalo *__restrict__ 
     g.vx[e] = vx[p];
     g.vy[e] = vy[p];
     g.vz[e] = vz[p];
-    st[e] = 1;
   }
 }
 
@@ -1471,7 +1475,9 @@ __global__ void __launch_bounds__(256) k_sp_cnt_carry(SplitHalo *sh, const int32
 __global__ void __launch_bounds__(256) k_sp_scatter(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
                                                   const int32_t *__restrict__ ntp,
                                                     const int64_t *__restrict__ tcarry, const uint8_t *__restrict__ st,
-                                                    const int32_t *__restrict__ wloc, uint32_t *a, uint32_t *b)
+                                                    const int32_t *__restrict__ wloc, uint32_t *a, uint32_t *b,
+                                                    const double *__restrict__ dist, Staged g,
+                                                    double *__restrict__ dist_t, Staged gt)
 {
   if ((int32_t)blockIdx.x >= *ntp) return;
   const SplitTile T = tiles[blockIdx.x];
@@ -1481,7 +1487,45 @@ __global__ void __launch_bounds__(256) k_sp_scatter(const SplitHalo *__restrict_
   const int64_t base = tcarry[blockIdx.x];
   for (int i = 0; i < AHF_TILE / 256; i++) {
     const int64_t j = T.j0 + (int64_t)i * 256 + threadIdx.x;
-    if (j < H.n && st[H.o + j]) nxt[H.o + base + wloc[H.o + j]] = cur[H.o + j];
+    const int64_t e = H.o + j;
+    if (j < H.n && st[e]) {
+      /* the member and its staged values, to their place in the next list */
+      const int64_t d = H.o + base + wloc[e];
+      nxt[d] = cur[e];
+      dist_t[d] = dist[e];
+      gt.dx[d] = g.dx[e];
+      gt.dy[d] = g.dy[e];
+      gt.dz[d] = g.dz[e];
+      gt.vx[d] = g.vx[e];
+      gt.vy[d] = g.vy[e];
+      gt.vz[d] = g.vz[e];
+    }
+  }
+}
+
+/* after the bookkeeping: the staged values of the members that stay, back
+ * in place for the next iteration (unfinished haloes only) */
+__global__ void __launch_bounds__(256) k_sp_restage(const SplitHalo *__restrict__ sh,
+                                                    const SplitTile *__restrict__ tiles,
+                                                    const int32_t *__restrict__ ntp,
+                                                    const double *__restrict__ dist_t, Staged gt,
+                                                    double *__restrict__ dist, Staged g)
+{
+  if ((int32_t)blockIdx.x >= *ntp) return;
+  const SplitTile T = tiles[blockIdx.x];
+  const SplitHalo &H = sh[T.seg];
+  if (H.done) return;
+  for (int i = 0; i < AHF_TILE / 256; i++) {
+    const int64_t j = T.j0 + (int64_t)i * 256 + threadIdx.x;
+    if (j >= H.n) break;
+    const int64_t e = H.o + j;
+    dist[e] = dist_t[e];
+    g.dx[e] = gt.dx[e];
+    g.dy[e] = gt.dy[e];
+    g.dz[e] = gt.dz[e];
+    g.vx[e] = gt.vx[e];
+    g.vy[e] = gt.vy[e];
+    g.vz[e] = gt.vz[e];
   }
 }
 
@@ -2115,8 +2159,12 @@ static int split_unbind(ahfgpu_ctx *c, const UnbindConst &K, HaloSet *s, const i
   int64_t *tcnt = A.get<int64_t>(max_tiles), *tlast = A.get<int64_t>(max_tiles), *tcarry = A.get<int64_t>(max_tiles);
   unsigned long long *tfirst = A.get<unsigned long long>(max_tiles);
   uint8_t *tchg = A.get<uint8_t>(max_tiles), *tunc = A.get<uint8_t>(max_tiles);
+  /* the staged values of the next list, written by the compaction */
+  double *dist_t = A.get<double>(s->ne);
+  const Staged gt = {A.get<double>(s->ne), A.get<double>(s->ne), A.get<double>(s->ne),
+                     A.get<float>(s->ne),  A.get<float>(s->ne),  A.get<float>(s->ne)};
   if (!sh || !hid || !wloc || !seg_tile0 || !words || !tiles || !ttot || !tcar || !tcnt || !tlast || !tcarry || !tfirst ||
-      !tchg || !tunc)
+      !tchg || !tunc || !dist_t || !gt.dx || !gt.dy || !gt.dz || !gt.vx || !gt.vy || !gt.vz)
     return c->fail(AHFGPU_ENOMEM, "device arena exhausted (split unbinding)");
   /* speculative passes before the sequential walk (ttot and tcar serve
    * them once the potential's prefix sums are done) */
@@ -2167,8 +2215,10 @@ static int split_unbind(ahfgpu_ctx *c, const UnbindConst &K, HaloSet *s, const i
   hipLaunchKernelGGL(k_sp_rec, dim3(ns), dim3(AHF_SPLIT_BLOCK), 0, s2, K, sh, dist, phi, st, g, spec > 0 ? 1 : 0);
   hipLaunchKernelGGL(k_sp_cnt_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, st, wloc, tcnt, tlast);
   hipLaunchKernelGGL(k_sp_cnt_carry, dim3(ns), dim3(256), 0, s2, sh, seg_tile0, tcnt, tlast, tcarry);
-  hipLaunchKernelGGL(k_sp_scatter, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, tcarry, st, wloc, s->buf_a, s->buf_b);
+  hipLaunchKernelGGL(k_sp_scatter, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, tcarry, st, wloc, s->buf_a, s->buf_b,
+                     dist, g, dist_t, gt);
   hipLaunchKernelGGL(k_sp_update, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit, dist);
+  hipLaunchKernelGGL(k_sp_restage, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, dist_t, gt, dist, g);
   hipLaunchKernelGGL(k_sp_any, dim3(1), dim3(256), 0, s2, sh, nsplit, any);
   const hipError_t ec = hipStreamEndCapture(s2, &graph);
   if (ec != hipSuccess || !graph) return c->fail(AHFGPU_EHIP, "split unbinding: graph capture: %s", hipGetErrorString(ec));
