@@ -90,7 +90,8 @@ struct ProfShared {
   double d_s[C];        /* each member's distance */
   double i_s[C];        /* I_now after each member; first the own M/d^2 */
   double rad[64];       /* bin radii */
-  double bt[64][BT_N];  /* per-bin record */
+  double *bt;           /* per-bin records, bin b at bt[b * PQ_N]: the halo's own profile rows, written
+                           over at the end (out of LDS, for occupancy) */
   int64_t bclose[64];   /* member index that closed each bin */
   int32_t bprev[C], bcur[C];
   uint64_t cm[NW], fm[NW], hm[NW]; /* per-wavefront masks: closing members, I_now steps, hits */
@@ -285,8 +286,8 @@ __device__ inline void chain_pass(double *row, int cnt, ProfShared<NW> &S, doubl
         chain_run<WITH_E>(row, i, e + 1, acc, acc_e, phi0);
         i = e + 1;
         for (int b = S.bprev[e]; b < S.bcur[e]; b++) {
-          if (col >= 0) S.bt[b][col] = acc;
-          if (WITH_E && ecol >= 0) S.bt[b][ecol] = acc_e;
+          if (col >= 0) S.bt[b * PQ_N + col] = acc;
+          if (WITH_E && ecol >= 0) S.bt[b * PQ_N + ecol] = acc_e;
         }
       }
     }
@@ -312,8 +313,8 @@ __device__ inline void chain_events(double *row, int cnt, const uint64_t *cm, co
       chain_run<WITH_E>(row, i, e + 1, acc, acc_e, phi0);
       i = e + 1;
       for (int b = bprev[e]; b < bcur[e]; b++) {
-        if (col >= 0) S.bt[b][col] = acc;
-        if (WITH_E && ecol >= 0) S.bt[b][ecol] = acc_e;
+        if (col >= 0) S.bt[b * PQ_N + col] = acc;
+        if (WITH_E && ecol >= 0) S.bt[b * PQ_N + ecol] = acc_e;
       }
     }
   }
@@ -1292,6 +1293,7 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
   /* a halo walked over the device (k_prf_*) brings its radii, bins,
    * totals and most bound member */
   const PrfTiled *TP = (tmap && tmap[h] >= 0 && tl[tmap[h]].ok) ? &tl[tmap[h]] : nullptr;
+  if (tid == 0) S.bt = prof + pofs[h] * PQ_N;
   if (TP) {
     if (tid < nbins) S.rad[tid] = TP->rad[tid];
   } else if (wid == 0) {
@@ -1330,7 +1332,7 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
   if (TP) {
     P = TP->P;
     mb = TP->mb;
-    for (int k = tid; k < 64 * BT_N; k += C) S.bt[k / BT_N][k % BT_N] = TP->bt[k / BT_N][k % BT_N];
+    for (int k = tid; k < nbins * BT_N; k += C) S.bt[(k / BT_N) * PQ_N + k % BT_N] = TP->bt[k / BT_N][k % BT_N];
     if (tid < 64) S.bclose[tid] = TP->bclose[tid];
     if (tid < 16) S.fin[tid] = TP->fin[tid];
   } else if constexpr (NW > 1) {
@@ -1500,7 +1502,7 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
       const double U = S.tc[4][tid] - Phi0;
       const double Ep = 0.5 * r.T + U;
       if (r.proc)
-        for (int b = r.bp; b < r.bi; b++) S.bt[b][BT_VESC] = 2 * fabs(U);
+        for (int b = r.bp; b < r.bi; b++) S.bt[b * PQ_N + BT_VESC] = 2 * fabs(U);
       const bool cand = r.proc && Ep < Emin;
       const uint64_t cw = __ballot(cand);
       if (lane == 0) S.hm[wid] = cw;
@@ -1717,7 +1719,7 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
     const double U = S.tc[4][tid] - Phi0;
     const double Ep = 0.5 * T + U;
     if (proc)
-      for (int b = bp; b < bi; b++) S.bt[b][BT_VESC] = 2 * fabs(U);
+      for (int b = bp; b < bi; b++) S.bt[b * PQ_N + BT_VESC] = 2 * fabs(U);
     /* most bound member: the first at the lowest energy below the one so far */
     {
       const bool cand = proc && Ep < Emin;
@@ -1778,22 +1780,25 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
     const double r = S.rad[b], rp = b ? S.rad[b - 1] : 0.0;
     const double Vol = FourPiThird * (r * r * r), Vp = b ? FourPiThird * (rp * rp * rp) : 0.0;
     const double dM = M - Mp, dV = Vol - Vp;
-    double ax[3] = {1.0, 0.0, 0.0}, e[3][3] = {};
-    if (S.bclose[b] + 1 > PRF_MINSHELL) prf_axes(S.bt[b], ax, e);
     double *q = pr + b * PQ_N;
+    /* the bin's record, read before the row is written over it */
+    double bt[BT_N];
+    for (int k = 0; k < BT_N; k++) bt[k] = q[k];
+    double ax[3] = {1.0, 0.0, 0.0}, e[3][3] = {};
+    if (S.bclose[b] + 1 > PRF_MINSHELL) prf_axes(bt, ax, e);
     q[PQ_R] = r;
     q[PQ_NPART] = M;
     q[PQ_NVPART] = M;
     q[PQ_OVDENS] = M / Vol;
     q[PQ_DENS] = dV > 0 ? dM / dV : 0.0;
     q[PQ_V2CIRC] = M / r;
-    q[PQ_VESC2] = S.bt[b][BT_VESC];
-    q[PQ_SIGV] = sqrt(S.bt[b][BT_T] / M);
-    q[PQ_EKIN] = 0.5 * S.bt[b][BT_T];
-    q[PQ_EPOT] = 0.5 * S.bt[b][BT_EP];
-    q[PQ_LX] = S.bt[b][BT_LX];
-    q[PQ_LY] = S.bt[b][BT_LY];
-    q[PQ_LZ] = S.bt[b][BT_LZ];
+    q[PQ_VESC2] = bt[BT_VESC];
+    q[PQ_SIGV] = sqrt(bt[BT_T] / M);
+    q[PQ_EKIN] = 0.5 * bt[BT_T];
+    q[PQ_EPOT] = 0.5 * bt[BT_EP];
+    q[PQ_LX] = bt[BT_LX];
+    q[PQ_LY] = bt[BT_LY];
+    q[PQ_LZ] = bt[BT_LZ];
     for (int k = 0; k < 3; k++)
       for (int r3 = 0; r3 < 3; r3++) q[PQ_E1X + 3 * k + r3] = e[r3][k];
     q[PQ_AX1] = 1.0;
