@@ -677,7 +677,8 @@ __global__ void __launch_bounds__(64) k_prf_tscan(const PrfTiled *__restrict__ T
 /* per member: the bins it closes, whether the walk reaches it, and its
  * terms that do not depend on the bulk velocity (the potential's term,
  * centre of mass, inertia tensor, V_c^2, rho r^2, ahf_halos.c:4333-4395);
- * each thread takes four consecutive members */
+ * rounds of 256 consecutive members, the carries (bins closed and the last
+ * member beyond MACHINE_ZERO before each member) as workgroup prefix maxima */
 __global__ void __launch_bounds__(256) k_prf_terms1(PrfTiled *T, const PrfTile *__restrict__ tiles, PrfRows R,
                                                     const int32_t *__restrict__ tc_before,
                                                     const int64_t *__restrict__ tfl_before,
@@ -692,65 +693,59 @@ __global__ void __launch_bounds__(256) k_prf_terms1(PrfTiled *T, const PrfTile *
   if (!H.ok) return;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   if (tid == 0) s_P = 0ull;
-  const int64_t j0 = Tl.j0 + (int64_t)tid * 4;
-  /* the thread's own maxima, then exclusive maxima over the tile's
-   * threads (wavefront scans, then the wavefronts before) */
-  int cm = 0;
-  long long fm = -1;
-  for (int i = 0; i < 4; i++) {
-    const int64_t j = j0 + i;
-    if (j < H.n) {
-      cm = max(cm, R.c[H.to + j]);
-      if (R.dd[H.to + j] > PRF_MZERO) fm = j;
-    }
-  }
-  int ci = cm;
-  long long fi = fm;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int cu = __shfl_up(ci, d, 64);
-    const long long fu = __shfl_up(fi, d, 64);
-    if (lane >= d) {
-      ci = max(ci, cu);
-      fi = max(fi, fu);
-    }
-  }
-  if (lane == 63) {
-    s_wc[wid] = ci;
-    s_wf[wid] = fi;
-  }
-  __syncthreads();
+  const double FourPiThird = 4. * PRF_PI / 3.;
+  auto I_of = [&](int64_t k) { return k >= 0 ? (double)(k + 1) / (R.dd[H.to + k] * R.dd[H.to + k]) : 0.0; };
+  /* over the rounds before: bins closed, last member beyond MACHINE_ZERO */
   int cx = tc_before[blockIdx.x];
   long long fx = tfl_before[blockIdx.x];
-  for (int w = 0; w < wid; w++) {
-    cx = max(cx, s_wc[w]);
-    fx = max(fx, s_wf[w]);
-  }
-  {
-    const int cu = __shfl_up(ci, 1, 64);
-    const long long fu = __shfl_up(fi, 1, 64);
-    if (lane > 0) {
-      cx = max(cx, cu);
-      fx = max(fx, fu);
-    }
-  }
-  const double FourPiThird = 4. * PRF_PI / 3.;
-  /* bins closed before the thread's first member, the last member beyond
-   * MACHINE_ZERO before it, and that member's I */
-  int bcar = cx;
-  int64_t lf = fx;
-  auto I_of = [&](int64_t k) { return k >= 0 ? (double)(k + 1) / (R.dd[H.to + k] * R.dd[H.to + k]) : 0.0; };
-  double I_prev = I_of(lf);
   int64_t last_proc = -1;
-  for (int i = 0; i < 4; i++) {
-    const int64_t j = j0 + i;
-    if (j >= H.n) break;
-    const int64_t e = H.to + j;
-    const int bp = bcar, bi = max(bcar, R.c[e]);
-    bcar = bi;
-    const double d = R.dd[e];
-    if (d > PRF_MZERO) lf = j;
-    const double I_now = I_of(lf);
+  for (int i = 0; i < PRF_TILE / 256; i++) {
+    const int64_t j = Tl.j0 + (int64_t)i * 256 + tid;
+    const bool inn = j < H.n;
+    const int64_t e = H.to + (inn ? j : 0);
+    const int c = inn ? R.c[e] : 0;
+    const double d = inn ? R.dd[e] : 0.0;
+    const long long f = inn && d > PRF_MZERO ? (long long)j : -1;
+    /* inclusive prefix maxima over the workgroup's members of this round */
+    int ci = c;
+    long long fi = f;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+      const int cu = __shfl_up(ci, dd, 64);
+      const long long fu = __shfl_up(fi, dd, 64);
+      if (lane >= dd) {
+        ci = max(ci, cu);
+        fi = max(fi, fu);
+      }
+    }
+    __syncthreads(); /* s_wc / s_wf of the round before are read */
+    if (lane == 63) {
+      s_wc[wid] = ci;
+      s_wf[wid] = fi;
+    }
+    __syncthreads();
+    int cw = cx;
+    long long fw = fx;
+    for (int w = 0; w < wid; w++) {
+      cw = max(cw, s_wc[w]);
+      fw = max(fw, s_wf[w]);
+    }
+    /* before this member (exclusive) and with it (inclusive) */
+    int ce = __shfl_up(ci, 1, 64);
+    long long fe = __shfl_up(fi, 1, 64);
+    if (lane == 0) {
+      ce = 0;
+      fe = -1;
+    }
+    const int bp = max(cw, ce), bi = max(max(cw, ci), bp);
+    const long long lf_prev = max(fw, fe), lf = max(fw, fi);
+    for (int w = 0; w < 4; w++) {
+      cx = max(cx, s_wc[w]);
+      fx = max(fx, s_wf[w]);
+    }
+    if (!inn) continue;
+    const double I_now = lf == j ? (double)(j + 1) / (d * d) : I_of(lf);
+    const double I_prev = I_of(lf_prev);
     const bool proc = bp < H.nbins;
     R.bp[e] = bp;
     R.bi[e] = bi;
@@ -779,7 +774,6 @@ __global__ void __launch_bounds__(256) k_prf_terms1(PrfTiled *T, const PrfTile *
       /* members the walk does not reach read as AHF's zeroed arrays */
       rad_m[H.base + j] = vc2[H.base + j] = dr2[H.base + j] = 0.0;
     }
-    I_prev = I_now;
   }
   if (last_proc >= 0) atomicMax(&s_P, (unsigned long long)(last_proc + 1));
   __syncthreads();
