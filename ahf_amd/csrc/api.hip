@@ -26,6 +26,17 @@ __global__ void k_deinterleave(const float *__restrict__ xyz, int64_t n, float *
   z[i] = xyz[3 * i + 2];
 }
 
+/* particle q's position and velocity into pv[2q], pv[2q + 1] */
+__global__ void k_pack_pv(int64_t n, const float *__restrict__ x, const float *__restrict__ y,
+                          const float *__restrict__ z, const float *__restrict__ vx, const float *__restrict__ vy,
+                          const float *__restrict__ vz, float4 *__restrict__ pv)
+{
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  pv[2 * q] = make_float4(x[q], y[q], z[q], vx[q]);
+  pv[2 * q + 1] = make_float4(vy[q], vz[q], 0.f, 0.f);
+}
+
 /* the integer value of "key" in a flat JSON object, if present */
 static bool json_int(const std::string &txt, const char *key, long long *v)
 {
@@ -164,6 +175,8 @@ static void free_particles(ahfgpu_ctx *c)
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
+  if (c->pv) (void)hipFree(c->pv);
+  c->pv = nullptr;
   c->cap = 0;
   if (c->sfckey) (void)hipFree(c->sfckey);
   c->sfckey = nullptr;
@@ -220,6 +233,11 @@ extern "C" int ahfgpu_set_particles(ahfgpu_ctx *c, const float *pos, const float
         free_particles(c);
         return c->fail(AHFGPU_ENOMEM, "particle arrays (%llu)", (unsigned long long)n);
       }
+    if (hipMalloc((void **)&c->pv, n * 2 * sizeof(float4)) != hipSuccess) {
+      (void)hipGetLastError();
+      free_particles(c);
+      return c->fail(AHFGPU_ENOMEM, "interleaved particle array (%llu)", (unsigned long long)n);
+    }
     c->cap = (int64_t)n;
   }
   float *stage = nullptr;
@@ -240,6 +258,8 @@ extern "C" int ahfgpu_set_particles(ahfgpu_ctx *c, const float *pos, const float
     hipMemsetAsync(c->vy, 0, n * sizeof(float), c->st);
     hipMemsetAsync(c->vz, 0, n * sizeof(float), c->st);
   }
+  hipLaunchKernelGGL(k_pack_pv, dim3(nb), dim3(AHF_BLOCK), 0, c->st, (int64_t)n, c->px, c->py, c->pz, c->vx, c->vy,
+                     c->vz, c->pv);
   hipError_t e = hipStreamSynchronize(c->st);
   (void)hipFree(stage);
   if (e != hipSuccess) return c->fail(AHFGPU_EHIP, "particle upload: %s", hipGetErrorString(e));

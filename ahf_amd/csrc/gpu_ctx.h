@@ -140,6 +140,9 @@ struct ahfgpu_ctx {
   int64_t N = 0;
   float *px = nullptr, *py = nullptr, *pz = nullptr;
   float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  /* the same, interleaved: particle q's x, y, z, vx in pv[2q] and vy, vz in
+   * pv[2q + 1] (one 32-byte sector for kernels that gather members) */
+  float4 *pv = nullptr;
   int64_t cap = 0;
   ahfgpu_amr_params params{};
   std::vector<DevLevel> lev;

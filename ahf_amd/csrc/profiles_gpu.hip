@@ -82,6 +82,20 @@ struct ProfConst {
 };
 
 /* LDS of one halo's workgroup of NW wavefronts: a chunk is 64*NW members */
+/* member q's position and velocity from the interleaved particle copy (one
+ * 32-byte sector instead of six scattered words) */
+__device__ __forceinline__ void pv_member(const float4 *__restrict__ pv, uint32_t q, float &x, float &y, float &z,
+                                          float &vx, float &vy, float &vz)
+{
+  const float4 a = pv[2 * (size_t)q], b = pv[2 * (size_t)q + 1];
+  x = a.x;
+  y = a.y;
+  z = a.z;
+  vx = a.w;
+  vy = b.x;
+  vz = b.y;
+}
+
 template <int NW>
 struct ProfShared {
   static constexpr int C = 64 * NW;
@@ -568,11 +582,8 @@ __global__ void __launch_bounds__(64) k_prf_bin(ProfConst K, PrfTiled *T, const 
 /* per member: offsets, distance, velocity, the bins its distance reaches;
  * per tile the largest of those and the last member beyond MACHINE_ZERO */
 __global__ void __launch_bounds__(256) k_prf_geom(const PrfTiled *__restrict__ T, const PrfTile *__restrict__ tiles,
-                                                  PrfRows R, const float *__restrict__ px,
-                                                  const float *__restrict__ py, const float *__restrict__ pz,
-                                                  const float *__restrict__ vx, const float *__restrict__ vy,
-                                                  const float *__restrict__ vz, int32_t *__restrict__ tc_max,
-                                                  int64_t *__restrict__ tfl_max)
+                                                  PrfRows R, const float4 *__restrict__ pv,
+                                                  int32_t *__restrict__ tc_max, int64_t *__restrict__ tfl_max)
 {
   __shared__ int32_t s_c;
   __shared__ unsigned long long s_fl;
@@ -594,7 +605,9 @@ __global__ void __launch_bounds__(256) k_prf_geom(const PrfTiled *__restrict__ T
     if (j >= H.n) break;
     const int64_t e = H.to + j;
     const uint32_t p = H.ip[j];
-    double dX = (double)px[p] - H.Xc, dY = (double)py[p] - H.Yc, dZ = (double)pz[p] - H.Zc;
+    float fx, fy, fz, fvx, fvy, fvz;
+    pv_member(pv, p, fx, fy, fz, fvx, fvy, fvz);
+    double dX = (double)fx - H.Xc, dY = (double)fy - H.Yc, dZ = (double)fz - H.Zc;
     if (dX > 0.5) dX -= 1.0;
     if (dY > 0.5) dY -= 1.0;
     if (dZ > 0.5) dZ -= 1.0;
@@ -606,9 +619,9 @@ __global__ void __launch_bounds__(256) k_prf_geom(const PrfTiled *__restrict__ T
     R.gx[e] = dX;
     R.gy[e] = dY;
     R.gz[e] = dZ;
-    R.v[0][e] = (double)vx[p];
-    R.v[1][e] = (double)vy[p];
-    R.v[2][e] = (double)vz[p];
+    R.v[0][e] = (double)fvx;
+    R.v[1][e] = (double)fvy;
+    R.v[2][e] = (double)fvz;
     /* bins whose radius is not above d: the radii of a tiled halo do not
      * decrease, so AHF's walk up the bins stops where a binary search does */
     int lo = 0, hi = nbins;
@@ -1252,7 +1265,7 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
     const float *__restrict__ py, const float *__restrict__ pz, const float *__restrict__ vx,
     const float *__restrict__ vy, const float *__restrict__ vz, double *__restrict__ s_rad,
     double *__restrict__ s_vc2, double *__restrict__ s_dr2, double *__restrict__ props, double *__restrict__ prof,
-    const PrfTiled *__restrict__ tl, const int32_t *__restrict__ tmap)
+    const PrfTiled *__restrict__ tl, const int32_t *__restrict__ tmap, const float4 *__restrict__ pv)
 {
   constexpr int C = 64 * NW;
   __shared__ ProfShared<NW> S;
@@ -1313,15 +1326,7 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
   /* the members of the next chunk are read while the current one is
    * summed, their indices one chunk earlier still */
   float nx = 0.f, ny = 0.f, nz = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
-  if (tid < n) {
-    const uint32_t q = ip[tid];
-    nx = px[q];
-    ny = py[q];
-    nz = pz[q];
-    nvx = vx[q];
-    nvy = vy[q];
-    nvz = vz[q];
-  }
+  if (tid < n) pv_member(pv, ip[tid], nx, ny, nz, nvx, nvy, nvz);
   uint32_t qn = C + tid < n ? ip[C + tid] : 0;
   if (TP) {
     P = TP->P;
@@ -1346,14 +1351,7 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
       r.j0 = j0;
       r.dX = r.dY = r.dZ = r.d = r.vxp = r.vyp = r.vzp = r.T = 0.0;
       const float cx = nx, cy = ny, cz = nz, cvx = nvx, cvy = nvy, cvz = nvz;
-      if (j + C < n) {
-        nx = px[qn];
-        ny = py[qn];
-        nz = pz[qn];
-        nvx = vx[qn];
-        nvy = vy[qn];
-        nvz = vz[qn];
-      }
+      if (j + C < n) pv_member(pv, qn, nx, ny, nz, nvx, nvy, nvz);
       qn = j + 2 * C < n ? ip[j + 2 * C] : 0;
       if (inb) {
         double dX = (double)cx - Xc, dY = (double)cy - Yc, dZ = (double)cz - Zc;
@@ -1564,14 +1562,7 @@ __global__ __launch_bounds__(64 * NW) void k_profiles(
     const bool inb = j < n;
     double dX = 0.0, dY = 0.0, dZ = 0.0, d = 0.0, vxp = 0.0, vyp = 0.0, vzp = 0.0;
     const float cx = nx, cy = ny, cz = nz, cvx = nvx, cvy = nvy, cvz = nvz;
-    if (j + C < n) {
-      nx = px[qn];
-      ny = py[qn];
-      nz = pz[qn];
-      nvx = vx[qn];
-      nvy = vy[qn];
-      nvz = vz[qn];
-    }
+    if (j + C < n) pv_member(pv, qn, nx, ny, nz, nvx, nvy, nvz);
     qn = j + 2 * C < n ? ip[j + 2 * C] : 0;
     if (inb) {
       dX = (double)cx - Xc;
@@ -2099,8 +2090,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
         const dim3 gt((unsigned)ntile);
         hipLaunchKernelGGL(k_prf_bin, dim3((unsigned)ntl), dim3(64), 0, s2, K, d_tl, s->off, out, d_pr, s->ip_in,
                            s->buf_a, s->buf_b, s->cen, d_nb, c->px, c->py, c->pz);
-        hipLaunchKernelGGL(k_prf_geom, gt, dim3(256), 0, s2, d_tl, d_tile, R, c->px, c->py, c->pz, c->vx, c->vy,
-                           c->vz, tcmax, tflmax);
+        hipLaunchKernelGGL(k_prf_geom, gt, dim3(256), 0, s2, d_tl, d_tile, R, c->pv, tcmax, tflmax);
         hipLaunchKernelGGL(k_prf_tscan, dim3((unsigned)ntl), dim3(64), 0, s2, d_tl, tcmax, tflmax);
         hipLaunchKernelGGL(k_prf_terms1, gt, dim3(256), 0, s2, d_tl, d_tile, R, tcmax, tflmax, sc, sc + ne,
                            sc + 2 * ne);
@@ -2124,7 +2114,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
         hipLaunchKernelGGL(k_profiles<AHF_PROF_BIG_NW>, dim3((unsigned)cnt), dim3(64 * AHF_PROF_BIG_NW), 0, st, K,
                            s->off, d_big + i0, big, out, d_pr, s->ip_in, s->buf_a, s->buf_b, s->cen, d_nb, d_pofs,
                            c->px, c->py, c->pz, c->vx, c->vy, c->vz, sc, sc + ne, sc + 2 * ne, r->props, r->prof,
-                           d_tl, d_tmap);
+                           d_tl, d_tmap, c->pv);
       };
       if (ntl) big_launch(c->st2, 0, ntl);
       if (nbig > ntl) big_launch(ntl ? c->st : c->st2, ntl, nbig - ntl);
@@ -2133,7 +2123,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     hipLaunchKernelGGL(k_profiles<1>, dim3((unsigned)nh), dim3(64), 0, c->st, K, s->off, s->order, big, out, d_pr,
                        s->ip_in, s->buf_a, s->buf_b, s->cen, d_nb, d_pofs, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
                        sc, sc + ne, sc + 2 * ne, r->props, r->prof, (const PrfTiled *)nullptr,
-                       (const int32_t *)nullptr);
+                       (const int32_t *)nullptr, c->pv);
     if (nbig) hipStreamWaitEvent(c->st, c->ev_s1, 0);
     c->timer_end(e0, TIMER_PROFILES, 0.0);
   }
