@@ -90,6 +90,17 @@ __device__ inline uint64_t dist_key(uint32_t p, const double *__restrict__ c, co
   return (uint64_t)__double_as_longlong(d2);
 }
 
+/* the same from the interleaved particle copy (x, y, z in pv[2p]) */
+__device__ inline uint64_t dist_key(uint32_t p, const double *__restrict__ c, const float4 *__restrict__ pv)
+{
+  const float4 a = pv[2 * (size_t)p];
+  const double dX = wrap_s((double)a.x - c[0]);
+  const double dY = wrap_s((double)a.y - c[1]);
+  const double dZ = wrap_s((double)a.z - c[2]);
+  const double d2 = dX * dX + dY * dY + dZ * dZ;
+  return (uint64_t)__double_as_longlong(d2);
+}
+
 #define AHF_SORT_SMALL 8192
 #define AHF_SORT_TIERS 5
 
@@ -122,8 +133,7 @@ struct SortBase {
 template <int CAP, int BS>
 __global__ void __launch_bounds__(BS) k_sort_tier(const int32_t *__restrict__ list, int64_t cnt,
                                                   const int64_t *__restrict__ off, const uint32_t *__restrict__ ip,
-                                                  const double *__restrict__ cen, const float *__restrict__ px,
-                                                  const float *__restrict__ py, const float *__restrict__ pz,
+                                                  const double *__restrict__ cen, const float4 *__restrict__ pv,
                                                   int copy_only, uint32_t *__restrict__ out)
 {
   __shared__ uint64_t s_key[CAP];
@@ -142,7 +152,7 @@ __global__ void __launch_bounds__(BS) k_sort_tier(const int32_t *__restrict__ li
     while (m < n) m <<= 1;
     for (int i = tid; i < m; i += BS) {
       if (i < n) {
-        s_key[i] = dist_key(ip[o0 + i], cen + 3 * h, px, py, pz);
+        s_key[i] = dist_key(ip[o0 + i], cen + 3 * h, pv);
         s_idx[i] = (uint32_t)i;
       } else {
         s_key[i] = ~0ull; /* above every finite key: padding sorts last */
@@ -185,8 +195,7 @@ __global__ void k_halo_keys(const int64_t *__restrict__ off, int64_t nh, int64_t
  * halo bh[k] */
 __global__ void k_dist_keys_sel(const int64_t *__restrict__ bo, const int32_t *__restrict__ bh, int64_t nb,
                                 int64_t ne, const int64_t *__restrict__ off, const uint32_t *__restrict__ ip,
-                                const double *__restrict__ cen, const float *__restrict__ px,
-                                const float *__restrict__ py, const float *__restrict__ pz,
+                                const double *__restrict__ cen, const float4 *__restrict__ pv,
                                 uint64_t *__restrict__ key, uint32_t *__restrict__ val)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -194,7 +203,7 @@ __global__ void k_dist_keys_sel(const int64_t *__restrict__ bo, const int32_t *_
   const int64_t k = halo_of(bo, nb, e);
   const int64_t h = bh[k];
   val[e] = (uint32_t)e;
-  key[e] = dist_key(ip[off[h] + (e - bo[k])], cen + 3 * h, px, py, pz);
+  key[e] = dist_key(ip[off[h] + (e - bo[k])], cen + 3 * h, pv);
 }
 
 __global__ void k_gather_members_sel(const int64_t *__restrict__ bo, const int32_t *__restrict__ bh, int64_t nb,
@@ -1096,11 +1105,8 @@ __device__ inline const uint32_t *split_cur(const SplitHalo &H, const uint32_t *
 __global__ void __launch_bounds__(256) k_sp_stage(const SplitHalo *__restrict__ sh, const SplitTile *__restrict__ tiles,
                                                   const int32_t *__restrict__ ntp,
                                                   const uint32_t *a, const uint32_t *b,
-                                                  const double *__restrict__ cen, const float *__restrict__ px,
-                                                  const float *__restrict__ py, const float *__restrict__ pz,
-                                                  const float *__restrict__ vx, const float *__restrict__ vy,
-                                                  const float *__restrict__ vz, double *__restrict__ dist, Staged g,
-                                                  uint8_t *__restrict__ st)
+                                                  const double *__restrict__ cen, const float4 *__restrict__ pv,
+                                                  double *__restrict__ dist, Staged g, uint8_t *__restrict__ st)
 {
   if ((int32_t)blockIdx.x >= *ntp) return;
   const SplitTile T = tiles[blockIdx.x];
@@ -1117,14 +1123,15 @@ __global__ void __launch_bounds__(256) k_sp_stage(const SplitHalo *__restrict__ 
     st[e] = 1;
     if (carried) continue;
     const uint32_t p = cur[e];
-    const double dX = wrap_s((double)px[p] - c[0]), dY = wrap_s((double)py[p] - c[1]), dZ = wrap_s((double)pz[p] - c[2]);
+    const float4 q0 = pv[2 * (size_t)p], q1 = pv[2 * (size_t)p + 1];
+    const double dX = wrap_s((double)q0.x - c[0]), dY = wrap_s((double)q0.y - c[1]), dZ = wrap_s((double)q0.z - c[2]);
     dist[e] = sqrt(dX * dX + dY * dY + dZ * dZ);
     g.dx[e] = dX;
     g.dy[e] = dY;
     g.dz[e] = dZ;
-    g.vx[e] = vx[p];
-    g.vy[e] = vy[p];
-    g.vz[e] = vz[p];
+    g.vx[e] = q0.w;
+    g.vy[e] = q1.x;
+    g.vz[e] = q1.y;
   }
 }
 
@@ -1762,8 +1769,7 @@ __global__ void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t v)
  * (haloes that AHF sorts only) */
 __global__ void k_tie_detect(const int64_t *__restrict__ off, int64_t nh, int64_t ne, int32_t minpart,
                              const uint32_t *__restrict__ sorted, const double *__restrict__ cen,
-                             const float *__restrict__ px, const float *__restrict__ py, const float *__restrict__ pz,
-                             uint8_t *__restrict__ tie)
+                             const float4 *__restrict__ pv, uint8_t *__restrict__ tie)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
@@ -1771,7 +1777,7 @@ __global__ void k_tie_detect(const int64_t *__restrict__ off, int64_t nh, int64_
   const int64_t o = off[h], n = off[h + 1] - o;
   if (sort_tier(n, minpart) == 0 || e + 1 >= o + n) return;
   const double *c = cen + 3 * h;
-  if (dist_key(sorted[e], c, px, py, pz) == dist_key(sorted[e + 1], c, px, py, pz)) tie[h] = 1;
+  if (dist_key(sorted[e], c, pv) == dist_key(sorted[e + 1], c, pv)) tie[h] = 1;
 }
 
 __global__ void k_tie_list(const uint8_t *__restrict__ tie, const int32_t *__restrict__ tpos, int64_t nh,
@@ -1988,8 +1994,8 @@ static int nr_tie_order(ahfgpu_ctx *c, hipStream_t st, const int64_t *off, int64
   *n_tie = 0;
   if (ne < 2 || nh < 1) return 0;
   hipLaunchKernelGGL(k_fill_u8, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, tie, nh, (uint8_t)0);
-  hipLaunchKernelGGL(k_tie_detect, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, minpart, sorted, cen, c->px,
-                     c->py, c->pz, tie);
+  hipLaunchKernelGGL(k_tie_detect, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, minpart, sorted, cen, c->pv,
+                     tie);
   exclusive_scan<uint8_t, int32_t>(tie, tpos, nh, tmp32h, st);
   /* the counts come back through the context's pinned words */
   int32_t *hT = c->h_words + 2;
@@ -2195,8 +2201,8 @@ static int split_unbind(ahfgpu_ctx *c, const UnbindConst &K, HaloSet *s, const i
   hipLaunchKernelGGL(k_sp_begin, dim3(gs), dim3(AHF_BLOCK), 0, s2, sh, nsplit);
   hipLaunchKernelGGL(k_sp_layout, dim3(1), dim3(256), 0, s2, sh, nsplit, 0, K.minpart, seg_tile0, ntp);
   hipLaunchKernelGGL(k_sp_fill_tiles, dim3(ns), dim3(256), 0, s2, seg_tile0, tiles);
-  hipLaunchKernelGGL(k_sp_stage, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, s->buf_a, s->buf_b, s->cen, c->px, c->py,
-                     c->pz, c->vx, c->vy, c->vz, dist, g, st);
+  hipLaunchKernelGGL(k_sp_stage, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, s->buf_a, s->buf_b, s->cen, c->pv, dist, g,
+                     st);
   hipLaunchKernelGGL(k_sp_phi_tiles, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, dist, g, phi, ttot);
   hipLaunchKernelGGL(k_sp_phi_carry, dim3(ns), dim3(256), 0, s2, sh, seg_tile0, ttot, tcar);
   hipLaunchKernelGGL(k_sp_phi_add, dim3(nt), dim3(256), 0, s2, sh, tiles, ntp, tcar, phi);
@@ -2394,7 +2400,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
 #define AHF_SORT_LAUNCH(T, CAP, BS, COPY)                                                                          \
   if (tcount[T] > 0)                                                                                               \
     hipLaunchKernelGGL((k_sort_tier<CAP, BS>), grid_of(tcount[T]), dim3(BS), 0, c->st, tlist + sb.base[T],         \
-                       tcount[T], s->off, s->ip_in, s->cen, c->px, c->py, c->pz, COPY, s->buf_a);
+                       tcount[T], s->off, s->ip_in, s->cen, c->pv, COPY, s->buf_a);
       AHF_SORT_LAUNCH(4, AHF_SORT_SMALL, 1024, 0)
       AHF_SORT_LAUNCH(3, 2048, 512, 0)
       AHF_SORT_LAUNCH(2, 512, 256, 0)
@@ -2409,7 +2415,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
       hipMemcpyAsync(d_bo, bo.data(), (size_t)(nsel + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->st);
       hipMemcpyAsync(d_bh, bh.data(), (size_t)nsel * sizeof(int32_t), hipMemcpyHostToDevice, c->st);
       hipLaunchKernelGGL(k_dist_keys_sel, dim3(nb_of(nes)), dim3(AHF_BLOCK), 0, c->st, d_bo, d_bh, nsel, nes, s->off,
-                         s->ip_in, s->cen, c->px, c->py, c->pz, k0, v0);
+                         s->ip_in, s->cen, c->pv, k0, v0);
       int w = radix_sort<uint64_t, uint32_t>(k0, v0, k1, v1, nes, 64, (uint32_t *)rtmp, c->st);
       uint32_t *vs = w ? v1 : v0;
       uint32_t *vo = w ? v0 : v1;
