@@ -79,18 +79,8 @@ __device__ inline int64_t halo_of(const int64_t *off, int64_t nh, int64_t e)
 
 /* sort key of a member: dist^2 from the centre (signed periodic wrap,
  * src/libahf/ahf_halos.c:5827-5840), as FP64 bits; non-negative, so the bit
- * order is the value order */
-__device__ inline uint64_t dist_key(uint32_t p, const double *__restrict__ c, const float *__restrict__ px,
-                                    const float *__restrict__ py, const float *__restrict__ pz)
-{
-  const double dX = wrap_s((double)px[p] - c[0]);
-  const double dY = wrap_s((double)py[p] - c[1]);
-  const double dZ = wrap_s((double)pz[p] - c[2]);
-  const double d2 = dX * dX + dY * dY + dZ * dZ;
-  return (uint64_t)__double_as_longlong(d2);
-}
-
-/* the same from the interleaved particle copy (x, y, z in pv[2p]) */
+ * order is the value order.  Positions from the interleaved particle copy
+ * (x, y, z in pv[2p]) */
 __device__ inline uint64_t dist_key(uint32_t p, const double *__restrict__ c, const float4 *__restrict__ pv)
 {
   const float4 a = pv[2 * (size_t)p];
@@ -318,13 +308,13 @@ __device__ void dev_indexx(int n, const double *arr1, int *indx1, int *stack)
 }
 
 /* overdensity of the sphere through member j at particle p, as
- * src/libahf/ahf_halos.c:3832-3856 evaluates it (M = j+1 unit masses) */
+ * src/libahf/ahf_halos.c:3832-3856 evaluates it (M = j+1 unit masses);
+ * the position from the interleaved particle copy */
 __device__ inline double sphere_ovdens(const UnbindConst &K, int64_t j, uint32_t p, const double cen[3],
-                                       const float *__restrict__ px, const float *__restrict__ py,
-                                       const float *__restrict__ pz, double *R_out)
+                                       const float4 *__restrict__ pv, double *R_out)
 {
-  const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
-               dZ = wrap_s((double)pz[p] - cen[2]);
+  const float4 a = pv[2 * (size_t)p];
+  const double dX = wrap_s((double)a.x - cen[0]), dY = wrap_s((double)a.y - cen[1]), dZ = wrap_s((double)a.z - cen[2]);
   const double R = sqrt(dX * dX + dY * dY + dZ * dZ);
   const double V = 4. * PI_AHF / 3. * (R * R * R);
   *R_out = R;
@@ -340,8 +330,7 @@ __device__ inline double sphere_ovdens(const UnbindConst &K, int64_t j, uint32_t
  */
 template <int BS>
 __device__ int64_t rvir_cut(const UnbindConst &K, int64_t n, const uint32_t *l, const double cen[3],
-                            const float *__restrict__ px, const float *__restrict__ py,
-                            const float *__restrict__ pz, double *M, double *R, double *od)
+                            const float4 *__restrict__ pv, double *M, double *R, double *od)
 {
   constexpr int PER = 4;
   constexpr unsigned long long NONE = ~0ull;
@@ -363,7 +352,7 @@ __device__ int64_t rvir_cut(const UnbindConst &K, int64_t n, const uint32_t *l, 
       const int64_t j = base + (int64_t)i * BS + t;
       if (j >= n) continue;
       double r;
-      const double o = sphere_ovdens(K, j, pp[i], cen, px, py, pz, &r);
+      const double o = sphere_ovdens(K, j, pp[i], cen, pv, &r);
       if (!(o >= K.ovlim) && (unsigned long long)j < f) f = (unsigned long long)j;
     }
     if (f != NONE) atomicMin(&s_first, f);
@@ -375,7 +364,7 @@ __device__ int64_t rvir_cut(const UnbindConst &K, int64_t n, const uint32_t *l, 
   const int64_t kept = first != NONE ? (int64_t)first + 1 : n;
   if (t == 0) {
     double r;
-    *od = sphere_ovdens(K, kept - 1, l[kept - 1], cen, px, py, pz, &r);
+    *od = sphere_ovdens(K, kept - 1, l[kept - 1], cen, pv, &r);
     *R = r;
     *M = (double)kept;
   }
@@ -732,10 +721,7 @@ __device__ int rec_eval(const UnbindConst &K, int64_t n, int64_t seed, const dou
 template <int BS, int IPT>
 __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uint32_t *a, uint32_t *b,
                             double *dist, double *phi, uint8_t *st, Staged g, const double cen[3],
-                            const float *__restrict__ px,
-                            const float *__restrict__ py, const float *__restrict__ pz,
-                            const float *__restrict__ vx, const float *__restrict__ vy,
-                            const float *__restrict__ vz, HaloOut *out)
+                            const float4 *__restrict__ pv, HaloOut *out)
 {
   constexpr int NW = BS / 64;
   constexpr int64_t CH = (int64_t)BS * IPT;
@@ -754,7 +740,7 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
   /* rem_outsideRvir(halo, 0): returns below minpart, else cuts and clears
    * Phi0 (:3696, :3864-3871) */
   if (K.cut_before && n >= K.minpart) {
-    n = rvir_cut<BS>(K, n, cur, cen, px, py, pz, &M_vir, &R_vir, &ovdens);
+    n = rvir_cut<BS>(K, n, cur, cen, pv, &M_vir, &R_vir, &ovdens);
     Phi0 = 0.0;
     set = AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_PHI0 | AHFGPU_SET_OVDENS;
   }
@@ -786,9 +772,10 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
         const int64_t j = base + (int64_t)i * BS + t;
         if (j >= n) continue;
         const uint32_t p = pp[i];
-        const double dX = wrap_s((double)px[p] - cen[0]), dY = wrap_s((double)py[p] - cen[1]),
-                     dZ = wrap_s((double)pz[p] - cen[2]);
-        const float ux = vx[p], uy = vy[p], uz = vz[p];
+        const float4 q0 = pv[2 * (size_t)p], q1 = pv[2 * (size_t)p + 1];
+        const double dX = wrap_s((double)q0.x - cen[0]), dY = wrap_s((double)q0.y - cen[1]),
+                     dZ = wrap_s((double)q0.z - cen[2]);
+        const float ux = q0.w, uy = q1.x, uz = q1.y;
         dist[j] = sqrt(dX * dX + dY * dY + dZ * dZ);
         g.dx[j] = dX;
         g.dy[j] = dY;
@@ -893,7 +880,7 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
   if (rvir_set) set |= AHFGPU_SET_RVIR;
   /* rem_outsideRvir(halo, 1) on the bound list */
   if (K.cut_after && n >= K.minpart) {
-    n = rvir_cut<BS>(K, n, cur, cen, px, py, pz, &M_vir, &R_vir, &ovdens);
+    n = rvir_cut<BS>(K, n, cur, cen, pv, &M_vir, &R_vir, &ovdens);
     set |= AHFGPU_SET_MVIR | AHFGPU_SET_RVIR | AHFGPU_SET_OVDENS;
   }
   if (t == 0) {
@@ -919,8 +906,7 @@ __device__ void unbind_halo(const UnbindConst &K, int64_t n0, int64_t seed0, uin
  * values) */
 #define AHF_SEED_BS 64
 __global__ void __launch_bounds__(AHF_SEED_BS) k_seed(UnbindConst K, const int64_t *__restrict__ off, int64_t nh,
-                                                      const uint32_t *__restrict__ a, const float *__restrict__ vx,
-                                                      const float *__restrict__ vy, const float *__restrict__ vz,
+                                                      const uint32_t *__restrict__ a, const float4 *__restrict__ pv,
                                                       int64_t *__restrict__ seed)
 {
   __shared__ double s_m2[65];
@@ -934,7 +920,9 @@ __global__ void __launch_bounds__(AHF_SEED_BS) k_seed(UnbindConst K, const int64
     }
     if ((int)threadIdx.x < nv) {
       const uint32_t p = a[o + threadIdx.x];
-      const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+      const float4 q0 = pv[2 * (size_t)p], q1 = pv[2 * (size_t)p + 1];
+      const float ux = q0.w, uy = q1.x, uz = q1.y;
+      const float v2 = ux * ux + uy * uy + uz * uz;
       s_m2[threadIdx.x + 1] = (double)v2;
     }
     __syncthreads();
@@ -962,10 +950,7 @@ template <int BS>
 __device__ void unbind_queue(const UnbindConst &K, const int64_t *__restrict__ off, const int32_t *__restrict__ order,
                              int64_t q0, int64_t q1, int32_t *__restrict__ next, const int64_t *__restrict__ seed,
                              uint32_t *a, uint32_t *b, double *dist, double *phi, uint8_t *st, Staged g,
-                             const double *__restrict__ cen, const float *__restrict__ px,
-                             const float *__restrict__ py, const float *__restrict__ pz,
-                             const float *__restrict__ vx, const float *__restrict__ vy,
-                             const float *__restrict__ vz, HaloOut *__restrict__ out)
+                             const double *__restrict__ cen, const float4 *__restrict__ pv, HaloOut *__restrict__ out)
 {
   __shared__ int64_t sh_h;
   for (;;) {
@@ -980,8 +965,7 @@ __device__ void unbind_queue(const UnbindConst &K, const int64_t *__restrict__ o
     const int64_t o = off[h], n = off[h + 1] - o;
     const double c3[3] = {cen[3 * h], cen[3 * h + 1], cen[3 * h + 2]};
     const Staged go = {g.dx + o, g.dy + o, g.dz + o, g.vx + o, g.vy + o, g.vz + o};
-    unbind_halo<BS, AHF_UNBIND_IPT>(K, n, seed[h], a + o, b + o, dist + o, phi + o, st + o, go, c3, px, py, pz, vx,
-                                    vy, vz, out + h);
+    unbind_halo<BS, AHF_UNBIND_IPT>(K, n, seed[h], a + o, b + o, dist + o, phi + o, st + o, go, c3, pv, out + h);
     __syncthreads();
   }
 }
@@ -992,12 +976,9 @@ __global__ void __launch_bounds__(BS) k_unbind_batched(UnbindConst K, const int6
                                                        int32_t *__restrict__ next, const int64_t *__restrict__ seed,
                                                        uint32_t *a, uint32_t *b, double *dist, double *phi,
                                                        uint8_t *st, Staged g,
-                                                       const double *__restrict__ cen, const float *__restrict__ px,
-                                                       const float *__restrict__ py, const float *__restrict__ pz,
-                                                       const float *__restrict__ vx, const float *__restrict__ vy,
-                                                       const float *__restrict__ vz, HaloOut *__restrict__ out)
+                                                       const double *__restrict__ cen, const float4 *__restrict__ pv, HaloOut *__restrict__ out)
 {
-  unbind_queue<BS>(K, off, order, q0, q1, next, seed, a, b, dist, phi, st, g, cen, px, py, pz, vx, vy, vz, out);
+  unbind_queue<BS>(K, off, order, q0, q1, next, seed, a, b, dist, phi, st, g, cen, pv, out);
 }
 
 /* the same with the queue range read from the device: order[qb[tier] ..
@@ -1009,13 +990,10 @@ __global__ void __launch_bounds__(BS) k_unbind_batched_dq(UnbindConst K, const i
                                                           int32_t *__restrict__ next, const int64_t *__restrict__ seed,
                                                           uint32_t *a, uint32_t *b, double *dist, double *phi,
                                                           uint8_t *st, Staged g, const double *__restrict__ cen,
-                                                          const float *__restrict__ px, const float *__restrict__ py,
-                                                          const float *__restrict__ pz, const float *__restrict__ vx,
-                                                          const float *__restrict__ vy, const float *__restrict__ vz,
+                                                          const float4 *__restrict__ pv,
                                                           HaloOut *__restrict__ out)
 {
-  unbind_queue<BS>(K, off, order, qb[tier], qb[tier + 1], next, seed, a, b, dist, phi, st, g, cen, px, py, pz, vx, vy,
-                   vz, out);
+  unbind_queue<BS>(K, off, order, qb[tier], qb[tier + 1], next, seed, a, b, dist, phi, st, g, cen, pv, out);
 }
 
 /* plain form: one halo per launch */
@@ -1024,16 +1002,13 @@ __global__ void __launch_bounds__(BS) k_unbind_one(UnbindConst K, int64_t h, con
                                                    const int64_t *__restrict__ seed, uint32_t *a, uint32_t *b,
                                                    double *dist, double *phi,
                                                    uint8_t *st, Staged g, const double *__restrict__ cen,
-                                                   const float *__restrict__ px, const float *__restrict__ py,
-                                                   const float *__restrict__ pz, const float *__restrict__ vx,
-                                                   const float *__restrict__ vy, const float *__restrict__ vz,
+                                                   const float4 *__restrict__ pv,
                                                    HaloOut *__restrict__ out)
 {
   const int64_t o = off[h], n = off[h + 1] - o;
   const double c3[3] = {cen[3 * h], cen[3 * h + 1], cen[3 * h + 2]};
   const Staged go = {g.dx + o, g.dy + o, g.dz + o, g.vx + o, g.vy + o, g.vz + o};
-  unbind_halo<BS, AHF_UNBIND_IPT>(K, n, seed[h], a + o, b + o, dist + o, phi + o, st + o, go, c3, px, py, pz, vx,
-                                  vy, vz, out + h);
+  unbind_halo<BS, AHF_UNBIND_IPT>(K, n, seed[h], a + o, b + o, dist + o, phi + o, st + o, go, c3, pv, out + h);
 }
 
 __global__ void k_sorted_out(const int64_t *__restrict__ off, int64_t nh, HaloOut *__restrict__ out)
@@ -1614,8 +1589,7 @@ __global__ void __launch_bounds__(256) k_sp_cut_tiles(UnbindConst K, const Split
                                                       const SplitTile *__restrict__ tiles,
                                                       const int32_t *__restrict__ ntp, const uint32_t *a,
                                                       const uint32_t *b, const double *__restrict__ cen,
-                                                      const float *__restrict__ px, const float *__restrict__ py,
-                                                      const float *__restrict__ pz,
+                                                      const float4 *__restrict__ pv,
                                                       unsigned long long *__restrict__ tfirst)
 {
   if ((int32_t)blockIdx.x >= *ntp) return;
@@ -1631,7 +1605,7 @@ __global__ void __launch_bounds__(256) k_sp_cut_tiles(UnbindConst K, const Split
     const int64_t j = T.j0 + (int64_t)i * 256 + threadIdx.x;
     if (j >= H.n) break;
     double r;
-    const double o = sphere_ovdens(K, j, cur[j], c3, px, py, pz, &r);
+    const double o = sphere_ovdens(K, j, cur[j], c3, pv, &r);
     if (!(o >= K.ovlim) && (unsigned long long)j < f) f = (unsigned long long)j;
   }
   if (f != ~0ull) atomicMin(&s_first, f);
@@ -1645,8 +1619,7 @@ __global__ void __launch_bounds__(256) k_sp_cut_tiles(UnbindConst K, const Split
  * (call 0 before the unbinding, call 1 after) */
 __global__ void k_sp_cut_fin(UnbindConst K, SplitHalo *sh, int64_t nseg, int call, const int32_t *__restrict__ seg_tile0,
                              const unsigned long long *__restrict__ tfirst, const uint32_t *a, const uint32_t *b,
-                             const double *__restrict__ cen, const float *__restrict__ px,
-                             const float *__restrict__ py, const float *__restrict__ pz)
+                             const double *__restrict__ cen, const float4 *__restrict__ pv)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nseg) return;
@@ -1662,7 +1635,7 @@ __global__ void k_sp_cut_fin(UnbindConst K, SplitHalo *sh, int64_t nseg, int cal
   const uint32_t *cur = split_cur(H, a, b) + H.o;
   const double c3[3] = {cen[3 * H.h], cen[3 * H.h + 1], cen[3 * H.h + 2]};
   double R;
-  const double od = sphere_ovdens(K, kept - 1, cur[kept - 1], c3, px, py, pz, &R);
+  const double od = sphere_ovdens(K, kept - 1, cur[kept - 1], c3, pv, &R);
   H.n = kept;
   H.M_vir = (double)kept;
   H.R_vir = R;
@@ -1793,15 +1766,14 @@ __global__ void k_tie_list(const uint8_t *__restrict__ tie, const int32_t *__res
  * segment per halo */
 __global__ void k_tie_init(const int64_t *__restrict__ toff, int64_t T, int64_t m, const int32_t *__restrict__ th,
                            const int64_t *__restrict__ off, const uint32_t *__restrict__ ip_in,
-                           const double *__restrict__ cen, const float *__restrict__ px, const float *__restrict__ py,
-                           const float *__restrict__ pz, TieWork W)
+                           const double *__restrict__ cen, const float4 *__restrict__ pv, TieWork W)
 {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= m) return;
   const int64_t k = halo_of(toff, T, g);
   const int64_t h = th[k];
   const int64_t loc = g - toff[k];
-  W.key[g] = __longlong_as_double((long long)dist_key(ip_in[off[h] + loc], cen + 3 * h, px, py, pz));
+  W.key[g] = __longlong_as_double((long long)dist_key(ip_in[off[h] + loc], cen + 3 * h, pv));
   W.idx[g] = (uint32_t)(loc + 1);
   W.seg[g] = (int32_t)k;
   if (loc == 0) {
@@ -2054,7 +2026,7 @@ static int nr_tie_order(ahfgpu_ctx *c, hipStream_t st, const int64_t *off, int64
       !W.cnt || !W.coff || !W.act || !W.err)
     return c->fail(AHFGPU_ENOMEM, "radial sort: tie buffers");
   hipLaunchKernelGGL(k_tie_init, dim3(nb_of(m)), dim3(AHF_BLOCK), 0, st, W.toff, (int64_t)T, m, W.th, off, ip_in, cen,
-                     c->px, c->py, c->pz, W);
+                     c->pv, W);
   int64_t S = T;
   int cur = 0;
   while (S > 0) {
@@ -2184,10 +2156,10 @@ static int split_unbind(ahfgpu_ctx *c, const UnbindConst &K, HaloSet *s, const i
   auto cut = [&](int call) {
     hipLaunchKernelGGL(k_sp_layout, dim3(1), dim3(256), 0, s2, sh, nsplit, 1, K.minpart, seg_tile0, ntp);
     hipLaunchKernelGGL(k_sp_fill_tiles, dim3(ns), dim3(256), 0, s2, seg_tile0, tiles);
-    hipLaunchKernelGGL(k_sp_cut_tiles, dim3(nt), dim3(256), 0, s2, K, sh, tiles, ntp, s->buf_a, s->buf_b, s->cen, c->px,
-                       c->py, c->pz, tfirst);
+    hipLaunchKernelGGL(k_sp_cut_tiles, dim3(nt), dim3(256), 0, s2, K, sh, tiles, ntp, s->buf_a, s->buf_b, s->cen, c->pv,
+                       tfirst);
     hipLaunchKernelGGL(k_sp_cut_fin, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit, call, seg_tile0, tfirst, s->buf_a,
-                       s->buf_b, s->cen, c->px, c->py, c->pz);
+                       s->buf_b, s->cen, c->pv);
   };
   if (K.cut_before) cut(0);
   hipLaunchKernelGGL(k_sp_enter, dim3(gs), dim3(AHF_BLOCK), 0, s2, K, sh, nsplit);
@@ -2443,8 +2415,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   if (sort_only) {
     if (nh) hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, c->st, s->off, nh, s->out);
   } else if (nh && K.unbind) {
-    hipLaunchKernelGGL(k_seed, dim3(seed_blocks(nh)), dim3(AHF_SEED_BS), 0, c->st, K, s->off, nh, s->buf_a, c->vx, c->vy,
-                       c->vz, seed);
+    hipLaunchKernelGGL(k_seed, dim3(seed_blocks(nh)), dim3(AHF_SEED_BS), 0, c->st, K, s->off, nh, s->buf_a, c->pv, seed);
   }
   if (sort_only) {
     /* nothing more */
@@ -2473,28 +2444,23 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     if (nbig > nsplit) {
       const int64_t grid = std::min<int64_t>(nbig - nsplit, (int64_t)dev_cu * 2);
       hipLaunchKernelGGL(k_unbind_batched<AHF_BIG_BLOCK>, dim3((unsigned)grid), dim3(AHF_BIG_BLOCK), 0, c->st, K,
-                         s->off, s->order, nsplit, nbig, next, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px,
-                         c->py, c->pz, c->vx, c->vy, c->vz, s->out);
+                         s->off, s->order, nsplit, nbig, next, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->pv, s->out);
     }
     if (nmid > nbig) {
       const int64_t grid = std::min<int64_t>(nmid - nbig, (int64_t)dev_cu * 8);
       hipLaunchKernelGGL(k_unbind_batched<AHF_BLOCK>, dim3((unsigned)grid), dim3(AHF_BLOCK), 0, c->st, K, s->off,
-                         s->order, nbig, nmid, next + 1, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px,
-                         c->py,
-                         c->pz, c->vx, c->vy, c->vz, s->out);
+                         s->order, nbig, nmid, next + 1, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->pv, s->out);
     }
     if (nh > nmid && nmid == 0) {
       /* only one-wavefront haloes: on the first stream, no hand-over */
       const int64_t grid = std::min<int64_t>(nh, (int64_t)dev_cu * 32);
       hipLaunchKernelGGL(k_unbind_batched<64>, dim3((unsigned)grid), dim3(64), 0, c->st, K, s->off, s->order, 0, nh,
-                         next + 2, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx,
-                         c->vy, c->vz, s->out);
+                         next + 2, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->pv, s->out);
     } else if (nh > nmid) {
       const int64_t grid = std::min<int64_t>(nh - nmid, (int64_t)dev_cu * 32);
       hipStreamWaitEvent(c->st3, c->ev_s0, 0);
       hipLaunchKernelGGL(k_unbind_batched<64>, dim3((unsigned)grid), dim3(64), 0, c->st3, K, s->off, s->order, nmid,
-                         nh, next + 2, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx,
-                         c->vy, c->vz, s->out);
+                         nh, next + 2, seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->pv, s->out);
       hipEventRecord(c->ev_s2, c->st3);
       hipStreamWaitEvent(c->st, c->ev_s2, 0);
     }
@@ -2508,11 +2474,11 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
     for (int64_t h = 0; h < nh; h++) {
       if (s->h_off[h + 1] - s->h_off[h] >= (c->big_halo > 0 ? c->big_halo : AHF_BIG_HALO))
         hipLaunchKernelGGL(k_unbind_one<AHF_BIG_BLOCK>, dim3(1), dim3(AHF_BIG_BLOCK), 0, c->st, K, h, s->off,
-                           seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz,
+                           seed, s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->pv,
                            s->out);
       else
         hipLaunchKernelGGL(k_unbind_one<AHF_BLOCK>, dim3(1), dim3(AHF_BLOCK), 0, c->st, K, h, s->off, seed,
-                           s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->px, c->py, c->pz, c->vx, c->vy, c->vz, s->out);
+                           s->buf_a, s->buf_b, dist, phi, st, g, s->cen, c->pv, s->out);
       hipStreamSynchronize(c->st);
     }
   }
@@ -2791,14 +2757,13 @@ __global__ void k_na_bounds(const int64_t *__restrict__ off, const int32_t *__re
  * AHF leaves the list as it is; the value is the list position */
 __global__ void k_na_member_keys(const int64_t *__restrict__ off, int64_t nh, int64_t ne, int32_t minpart,
                                  const uint32_t *__restrict__ ip, const double *__restrict__ cen,
-                                 const float *__restrict__ px, const float *__restrict__ py,
-                                 const float *__restrict__ pz, uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+                                 const float4 *__restrict__ pv, uint64_t *__restrict__ key, uint32_t *__restrict__ val)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const int64_t h = halo_of(off, nh, e);
   const int64_t n = off[h + 1] - off[h];
-  key[e] = sort_tier(n, minpart) != 0 ? dist_key(ip[e], cen + 3 * h, px, py, pz) : 0ull;
+  key[e] = sort_tier(n, minpart) != 0 ? dist_key(ip[e], cen + 3 * h, pv) : 0ull;
   val[e] = (uint32_t)e;
 }
 
@@ -2915,7 +2880,7 @@ extern "C" int ahfgpu_unbind_noalloc(ahfgpu_ctx *c, const ahfgpu_unbind_params *
     hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, members, ne, buf_a);
   } else if (ne > 0) {
     hipLaunchKernelGGL(k_na_member_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, p->minpart, members,
-                       centres, c->px, c->py, c->pz, k0, v0);
+                       centres, c->pv, k0, v0);
     const int w = radix_sort<uint64_t, uint32_t>(k0, v0, k1, v1, ne, 64, rtmp, st);
     uint32_t *vs = w ? v1 : v0, *vo = w ? v0 : v1;
     hipLaunchKernelGGL(k_halo_keys, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, st, off, nh, ne, vs, hk0);
@@ -2944,7 +2909,7 @@ extern "C" int ahfgpu_unbind_noalloc(ahfgpu_ctx *c, const ahfgpu_unbind_params *
     hipLaunchKernelGGL(k_sorted_out, dim3(nb_of(nh)), dim3(AHF_BLOCK), 0, st, off, nh, hout);
   } else {
     if (K.unbind)
-      hipLaunchKernelGGL(k_seed, dim3(seed_blocks(nh)), dim3(AHF_SEED_BS), 0, st, K, off, nh, buf_a, c->vx, c->vy, c->vz,
+      hipLaunchKernelGGL(k_seed, dim3(seed_blocks(nh)), dim3(AHF_SEED_BS), 0, st, K, off, nh, buf_a, c->pv,
                          seed);
     int dev_cu = 256;
     hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
@@ -2953,13 +2918,12 @@ extern "C" int ahfgpu_unbind_noalloc(ahfgpu_ctx *c, const ahfgpu_unbind_params *
     hipStreamWaitEvent(c->st3, c->ev_s0, 0);
     hipLaunchKernelGGL(k_unbind_batched_dq<AHF_BIG_BLOCK>, dim3((unsigned)std::min<int64_t>(nh, 2 * dev_cu)),
                        dim3(AHF_BIG_BLOCK), 0, st, K, off, order, qb, 0, words, seed, buf_a, buf_b, dist, phi, stt, g,
-                       centres, c->px, c->py, c->pz, c->vx, c->vy, c->vz, hout);
+                       centres, c->pv, hout);
     hipLaunchKernelGGL(k_unbind_batched_dq<AHF_BLOCK>, dim3((unsigned)std::min<int64_t>(nh, 8 * dev_cu)),
                        dim3(AHF_BLOCK), 0, st, K, off, order, qb, 1, words + 1, seed, buf_a, buf_b, dist, phi, stt, g,
-                       centres, c->px, c->py, c->pz, c->vx, c->vy, c->vz, hout);
+                       centres, c->pv, hout);
     hipLaunchKernelGGL(k_unbind_batched_dq<64>, dim3((unsigned)std::min<int64_t>(nh, 32 * dev_cu)), dim3(64), 0,
-                       c->st3, K, off, order, qb, 2, words + 2, seed, buf_a, buf_b, dist, phi, stt, g, centres, c->px,
-                       c->py, c->pz, c->vx, c->vy, c->vz, hout);
+                       c->st3, K, off, order, qb, 2, words + 2, seed, buf_a, buf_b, dist, phi, stt, g, centres, c->pv, hout);
     hipEventRecord(c->ev_s2, c->st3);
     hipStreamWaitEvent(st, c->ev_s2, 0);
   }
