@@ -861,11 +861,7 @@ __device__ __forceinline__ void prf_lane_batch(const double *col, double &a, dou
   for (int u = 0; u < PRF_FB / PRF_FS; u++) ck[u * PRF_NL] = ckr[u];
 }
 
-/* wavefronts of a workgroup go to the compute unit's four SIMDs in turn:
- * wavefront 4, which would share the summing wavefront's SIMD, only keeps
- * the barriers, so that the helpers' arithmetic does not take the SIMD's
- * issue from the additions */
-#define PRF_FW (2 + PRF_FH)
+#define PRF_FW (1 + PRF_FH) /* wavefronts: the sums, then the helpers */
 __global__ void __launch_bounds__(64 * PRF_FW) k_prf_walk(PrfTiled *T, const PrfChain *__restrict__ chains,
                                                           PrfRows R, ProfConst K)
 {
@@ -891,14 +887,10 @@ __global__ void __launch_bounds__(64 * PRF_FW) k_prf_walk(PrfTiled *T, const Prf
     if (lane < PRF_NL && chains[lane].fin >= 0) H.fin[chains[lane].fin] = a;
     return;
   }
-  if (w == 4) {
-    for (int64_t b = 0; b <= nit; b++) __syncthreads();
-    return;
-  }
   /* helper wavefront hw owns rows [hw PRF_FQ, (hw + 1) PRF_FQ) of every
    * batch, so that no other wavefront touches them between barriers; lane
    * l owns row r1 */
-  const int hw = w < 4 ? w - 1 : w - 2;
+  const int hw = w - 1;
   const int r1 = hw * PRF_FQ + lane;
   /* global, not generic, accesses: generic ones count against the LDS
    * counter too, and the barrier's wait on that would stall on the loads
