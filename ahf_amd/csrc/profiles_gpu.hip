@@ -531,7 +531,7 @@ struct PrfTile {
 /* the per-member rows of the tiled walk (each Nt doubles or ints, Nt the
  * tiled haloes' members end to end) */
 struct PrfRows {
-  double *dd, *gx, *gy, *gz, *v[3], *vr[3], *l[3], *tt, *phi, *phir, *com[3], *ine[6];
+  double *dd, *gx, *gy, *gz, *v[3], *tt, *phi, *phir, *com[3], *ine[6];
   int32_t *c, *bp, *bi;
 };
 
@@ -808,8 +808,8 @@ __global__ void __launch_bounds__(256) k_prf_terms1(PrfTiled *T, const PrfTile *
  * PRF_FH more wavefronts, each owning PRF_FQ rows of every batch (a row a
  * lane), finish and stage while the batches are summed.  They replay the
  * first four sums over batch b - 1 from their checkpoints (the same
- * additions from the same values) into the rows (bulk velocity,
- * potential).  They compute batch b - 1's terms of the other sums and
+ * additions from the same values) in place, the potential's also into its
+ * rows.  They compute batch b - 1's terms of the other sums and
  * stage them, with the centre of mass, inertia and potential energy terms.
  * They stage the first four's terms of batch b + 1.  And they replay the
  * other sums over batch b - 3 up to the members that close bins, for the
@@ -966,20 +966,22 @@ __global__ void __launch_bounds__(64 * PRF_FW) k_prf_walk(PrfTiled *T, const Prf
     for (int c = 0; c < PRF_NL - PRF_NV; c++) s_v[s][PRF_NV + c][r1] = in ? col[c] : -0.0;
     s_bins[s][r1] = in ? make_int2(Rg.p, Rg.q) : make_int2(0, 0);
   };
-  /* the first four over batch B (slot s) into their rows and in place:
-   * lane (u, c) replays sum c over checkpoint span u of the wavefront's rows */
+  /* the first four over batch B (slot s) in place, the potential's also
+   * into its rows: lane (u, c) replays sum c over checkpoint span u of the
+   * wavefront's rows */
   constexpr int NU = PRF_FQ / PRF_FS;
   static_assert(NU * PRF_NV <= 64, "one lane per sum and span");
   const bool vact = lane < NU * PRF_NV;
   const int vc = vact ? lane % PRF_NV : 0, vu = vact ? lane / PRF_NV : 0;
-  gdouble *const vout = (gdouble *)chains[vc].out;
+  gdouble *const vout = (gdouble *)chains[vc].out; /* the potential's rows; the others in LDS only */
+  const bool vst = vout != nullptr;
   auto replay_v = [&](int s, int64_t B) {
     const int64_t r0 = B * PRF_FB + hw * PRF_FQ + vu * PRF_FS;
     const int64_t mq = P - r0 < PRF_FS ? P - r0 : PRF_FS;
     if (!vact || mq <= 0) return;
     double *colp = &s_v[s][vc][hw * PRF_FQ + vu * PRF_FS];
     double a = s_ck[s][hw * NU + vu][vc];
-    gdouble *o = vout + to + r0;
+    gdouble *o = vst ? vout + to + r0 : nullptr;
     if (mq == PRF_FS) {
       double t[PRF_FS];
 #pragma unroll
@@ -988,13 +990,13 @@ __global__ void __launch_bounds__(64 * PRF_FW) k_prf_walk(PrfTiled *T, const Prf
       for (int k = 0; k < PRF_FS; k++) {
         a += t[k];
         colp[k] = a;
-        o[k] = a;
+        if (vst) o[k] = a;
       }
     } else {
       for (int k = 0; k < mq; k++) {
         a += colp[k];
         colp[k] = a;
-        o[k] = a;
+        if (vst) o[k] = a;
       }
     }
   };
@@ -2025,7 +2027,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     tpi = A.get<int64_t>(2 * ntile);
     tib = A.get<int64_t>(ntile);
     d_chains = A.get<PrfChain>(18);
-    double *rows = A.get<double>(29 * Nt);
+    double *rows = A.get<double>(19 * Nt);
     int32_t *irows = A.get<int32_t>(3 * Nt);
     if (!d_tl || !d_tile || !d_tmap || !tcmax || !tflmax || !tmb || !tEmin || !tpv || !tpi || !tib || !d_chains ||
         !rows || !irows)
@@ -2033,7 +2035,7 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     double *w = rows;
     auto take = [&]() { double *r = w; w += Nt; return r; };
     R.dd = take(); R.gx = take(); R.gy = take(); R.gz = take();
-    for (int k = 0; k < 3; k++) { R.v[k] = take(); R.vr[k] = take(); R.l[k] = take(); R.com[k] = take(); }
+    for (int k = 0; k < 3; k++) { R.v[k] = take(); R.com[k] = take(); }
     R.tt = take(); R.phi = take(); R.phir = take();
     for (int k = 0; k < 6; k++) R.ine[k] = take();
     R.c = irows; R.bp = irows + Nt; R.bi = irows + 2 * Nt;
@@ -2050,15 +2052,15 @@ extern "C" int ahfgpu_halo_profiles(ahfgpu_ctx *c, const ahfgpu_profile_params *
     hipMemcpyAsync(d_pofs, pofs.data(), (size_t)nh * 8, hipMemcpyHostToDevice, c->st);
     if (nbig) hipMemcpyAsync(d_big, bigl.data(), (size_t)nbig * 4, hipMemcpyHostToDevice, c->st);
     if (ntl) {
-      /* the running sums: bulk velocity (3) and potential, each to its
-       * row; then angular momentum (3), kinetic energy, centre of mass
-       * (3), inertia tensor (6) and potential energy (over the potential's
-       * running values less Phi0) */
+      /* the running sums: bulk velocity (3) and potential (to its row);
+       * then angular momentum (3) and kinetic energy (their terms made in
+       * the walk), centre of mass (3), inertia tensor (6) and potential
+       * energy (over the potential's running values less Phi0) */
       std::vector<PrfChain> ch;
-      for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.v[k], R.vr[k], -1, 9 + k, 0, 0});
+      for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.v[k], nullptr, -1, 9 + k, 0, 0});
       ch.push_back(PrfChain{R.phi, R.phir, -1, -1, 0, 0});
-      for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.l[k], nullptr, BT_LX + k, -1, 0, 0});
-      ch.push_back(PrfChain{R.tt, nullptr, BT_T, -1, 0, 0});
+      for (int k = 0; k < 3; k++) ch.push_back(PrfChain{nullptr, nullptr, BT_LX + k, -1, 0, 0});
+      ch.push_back(PrfChain{nullptr, nullptr, BT_T, -1, 0, 0});
       for (int k = 0; k < 3; k++) ch.push_back(PrfChain{R.com[k], nullptr, -1, k, 0, 0});
       for (int k = 0; k < 6; k++) ch.push_back(PrfChain{R.ine[k], nullptr, BT_A11 + k, -1, 0, 0});
       ch.push_back(PrfChain{R.phir, nullptr, BT_EP, -1, 1, 0});
