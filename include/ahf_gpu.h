@@ -97,7 +97,9 @@ void ahfgpu_destroy(ahfgpu_ctx *ctx);
 const char *ahfgpu_last_error(const ahfgpu_ctx *ctx);
 
 /* copy n particles (positions in box units and AHF velocities, both as
- * interleaved xyz FP32) from host memory to the device */
+ * interleaved xyz FP32) from host memory to the device; the device keeps
+ * them as arrays per component and once more interleaved per particle
+ * (56 bytes a particle in all) */
 int ahfgpu_set_particles(ahfgpu_ctx *ctx, const float *pos_xyz, const float *mom_xyz, uint64_t n);
 
 /* build the AMR hierarchy of the resident particles (device only; the last
