@@ -2099,8 +2099,9 @@ extern "C" int ahfgpu_set_halos(ahfgpu_ctx *c, const double *centres, const uint
 /*
  * The split path for the haloes ord[0..nsplit) (the largest), on the second
  * stream, after the sort and the seeds on the first: rem_outsideRvir(0),
- * then iterations of staging / Phi prefix / recurrence / compaction over
- * tiles of all unfinished haloes until each one stops as rem_unbound does,
+ * then iterations of staging (gathered once, then carried by the
+ * compaction) / Phi prefix / recurrence / compaction over tiles of all
+ * unfinished haloes until each one stops as rem_unbound does,
  * then rem_outsideRvir(1).  Each iteration lays out its own tiles on the
  * device and is one replay of a captured graph; the host only reads a done
  * flag between batches of replays.  The first stream waits for the result
