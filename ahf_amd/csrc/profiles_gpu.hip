@@ -818,10 +818,10 @@ __global__ void __launch_bounds__(256) k_prf_terms1(PrfTiled *T, const PrfTile *
  * Members past the last take -0.0, which leaves every sum as it is.  Sum c
  * of chains[c]; sum 17 (sub_phi0) adds the potential's running value less
  * Phi0. */
-#define PRF_FB 384       /* members per batch */
+#define PRF_FB 448       /* members per batch */
 #define PRF_FQ 64        /* rows a helper wavefront owns: a row a lane */
-#define PRF_FH 6         /* helper wavefronts */
-#define PRF_FROWS 418    /* a column: the batch, 34 rows read ahead; 2 * 418 = 4 mod 64 dwords between columns */
+#define PRF_FH 7         /* helper wavefronts */
+#define PRF_FROWS 482    /* a column: the batch, 34 rows read ahead; 2 * 482 = 4 mod 64 dwords between columns */
 #define PRF_FS 16        /* members per checkpoint */
 #define PRF_NV 4         /* sums to rows: bulk velocity, potential */
 #define PRF_NL 18        /* sums */
