@@ -1739,14 +1739,21 @@ __global__ void k_fill_u8(uint8_t *__restrict__ p, int64_t n, uint8_t v)
 }
 
 /* tie[h] = 1 where two neighbours of halo h's sorted list have equal keys
- * (haloes that AHF sorts only) */
+ * (haloes that AHF sorts only).  The halo of the workgroup's first member
+ * comes from one binary search, each thread's from a walk forward from it
+ * (a workgroup's consecutive members span few haloes) */
 __global__ void k_tie_detect(const int64_t *__restrict__ off, int64_t nh, int64_t ne, int32_t minpart,
                              const uint32_t *__restrict__ sorted, const double *__restrict__ cen,
                              const float4 *__restrict__ pv, uint8_t *__restrict__ tie)
 {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int64_t s_h0;
+  const int64_t e0 = (int64_t)blockIdx.x * blockDim.x;
+  if (threadIdx.x == 0) s_h0 = halo_of(off, nh, e0);
+  __syncthreads();
+  const int64_t e = e0 + threadIdx.x;
   if (e >= ne) return;
-  const int64_t h = halo_of(off, nh, e);
+  int64_t h = s_h0;
+  while (off[h + 1] <= e) h++;
   const int64_t o = off[h], n = off[h + 1] - o;
   if (sort_tier(n, minpart) == 0 || e + 1 >= o + n) return;
   const double *c = cen + 3 * h;
