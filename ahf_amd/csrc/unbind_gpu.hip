@@ -2352,6 +2352,7 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
   int64_t n_tie = 0;
   uint8_t *tie = nullptr;
   int32_t *tpos = nullptr, *tmp32h = nullptr;
+  bool tiers_forked = false;
   if (ne > 0 && no_sort) {
     hipLaunchKernelGGL(k_copy_u32, dim3(nb_of(ne)), dim3(AHF_BLOCK), 0, c->st, s->ip_in, ne, s->buf_a);
   } else if (ne > 0) {
@@ -2376,17 +2377,28 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
         }
       }
       hipMemcpyAsync(tlist, hl, (size_t)acc * sizeof(int32_t), hipMemcpyHostToDevice, c->st);
+      /* the tiers sort disjoint haloes into disjoint ranges of buf_a, as
+       * the radix sort below does: they run on the second and third
+       * streams beside it, and the first stream waits for both before the
+       * tie pass reads buf_a.  (The events are recorded again, for other
+       * hand-overs, after these waits are issued.) */
+      hipEventRecord(c->ev_s0, c->st);
+      hipStreamWaitEvent(c->st2, c->ev_s0, 0);
+      hipStreamWaitEvent(c->st3, c->ev_s0, 0);
       auto grid_of = [](int64_t n) { return dim3((unsigned)std::min<int64_t>(n, (int64_t)1 << 20)); };
-#define AHF_SORT_LAUNCH(T, CAP, BS, COPY)                                                                          \
+#define AHF_SORT_LAUNCH(T, CAP, BS, COPY, STREAM)                                                                  \
   if (tcount[T] > 0)                                                                                               \
-    hipLaunchKernelGGL((k_sort_tier<CAP, BS>), grid_of(tcount[T]), dim3(BS), 0, c->st, tlist + sb.base[T],         \
+    hipLaunchKernelGGL((k_sort_tier<CAP, BS>), grid_of(tcount[T]), dim3(BS), 0, STREAM, tlist + sb.base[T],        \
                        tcount[T], s->off, s->ip_in, s->cen, c->pv, COPY, s->buf_a);
-      AHF_SORT_LAUNCH(4, AHF_SORT_SMALL, 1024, 0)
-      AHF_SORT_LAUNCH(3, 2048, 512, 0)
-      AHF_SORT_LAUNCH(2, 512, 256, 0)
-      AHF_SORT_LAUNCH(1, 64, 64, 0)
-      AHF_SORT_LAUNCH(0, 64, 64, 1)
+      AHF_SORT_LAUNCH(4, AHF_SORT_SMALL, 1024, 0, c->st2)
+      AHF_SORT_LAUNCH(3, 2048, 512, 0, c->st3)
+      AHF_SORT_LAUNCH(2, 512, 256, 0, c->st3)
+      AHF_SORT_LAUNCH(1, 64, 64, 0, c->st2)
+      AHF_SORT_LAUNCH(0, 64, 64, 1, c->st2)
 #undef AHF_SORT_LAUNCH
+      hipEventRecord(c->ev_s1, c->st2);
+      hipEventRecord(c->ev_s2, c->st3);
+      tiers_forked = true;
     }
     if (nsel > 0) {
       int64_t *d_bo = A.get<int64_t>(nsel + 1);
@@ -2406,6 +2418,10 @@ extern "C" int ahfgpu_unbind(ahfgpu_ctx *c, const ahfgpu_unbind_params *p, int m
       uint32_t *vsorted = w2 ? vo : vs;
       hipLaunchKernelGGL(k_gather_members_sel, dim3(nb_of(nes)), dim3(AHF_BLOCK), 0, c->st, d_bo, d_bh, nsel, nes,
                          s->off, vsorted, s->ip_in, s->buf_a);
+    }
+    if (tiers_forked) {
+      hipStreamWaitEvent(c->st, c->ev_s1, 0);
+      hipStreamWaitEvent(c->st, c->ev_s2, 0);
     }
     /* haloes with equal distances: indexx's order of them (the sort alone
      * reads the count with its results and re-sorts only when it is not
