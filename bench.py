@@ -185,7 +185,7 @@ def load_traffic():
 
 
 def unbinding_workload(seed=1002, reps=3, cpu=True, threads=16):
-    """The unbinding-throughput workload (configs[2]): 100,000 isolated NFW
+    """The unbinding-throughput workload: 100,000 isolated NFW
     haloes, N ~ N^-2 on [20, 2e6] members, 15% of members above escape speed
     and 5% within 1e-4 of it; sort + unbinding of all haloes, batched, with
     the reference's own sort_halo_particles + rem_unbound (OpenMP over
@@ -335,9 +335,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              f"the path is sharded by halo and by slab as described in DESIGN.md, "
-              f"but multi-GPU runs are not part of it.", file=sys.stderr)
+        print(f"bench.py: this benchmark runs on one GPU only (asked for {args.gpus}); "
+              f"the unbinding would shard by halo and the deposit by slab of the domain grid, "
+              f"but multi-GPU runs are not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
